@@ -1,0 +1,171 @@
+/*
+ * ugmg.h -- C ABI of the MI355X multigrid solve path (libugmg.so).
+ *
+ * Replaces, behind UG's numproc interface, the CPU solve path
+ *   lmgc      Lmgc / LmgcPreProcess / LmgcPostProcess   np/procs/iter.cc:7707-7984
+ *   jac, gs   Smoother + JacobiStep / GSStep            np/procs/iter.cc:817-1087
+ *   transfer  RestrictDefect / InterpolateCorrection     np/procs/transfer.cc:724-773
+ *   ex        EXPreProcess / EXSmoother (band LU)        np/procs/iter.cc:8685-9071
+ *   ls        LinearDefect / LinearResiduum / LinearSolver np/procs/ls.cc:562-749
+ *   ugblas    dmatmul_minus / dmatmul / dnrm2x / ddot    np/algebra/ugblas.cc:2291-4042
+ * A UG numproc class flattens the MATRIX lists of each
+ * level in ConvertMatrix order (np/udm/disctools.cc:2061-2128) and hands the
+ * arrays over here; everything after that runs on the GPU.
+ *
+ * Conventions
+ *  - Every function returns 0 (UGMG_OK) or a negative UGMG_E_* code; the
+ *    message is available from ugmg_last_error().  No C++ exception crosses
+ *    this boundary.
+ *  - Vectors are `n*b` doubles, component-fastest, in the level's vector
+ *    list order.  Functions whose name ends in _host take host pointers and
+ *    copy; all others take DEVICE pointers and are stream ordered on the
+ *    context's stream (ugmg_set_stream), returning before the GPU finishes
+ *    unless they report a norm.
+ *  - Arrays passed in are caller owned and copied; device memory belongs to
+ *    the context.
+ *  - One context per host thread; calls on one context are not re-entrant.
+ */
+#ifndef UGMG_H
+#define UGMG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UGMG_MAXB 3
+
+enum {
+  UGMG_OK = 0,
+  UGMG_E_ARG = -1,       /* invalid argument                               */
+  UGMG_E_HIP = -2,       /* HIP runtime error (message has the HIP string) */
+  UGMG_E_STATE = -3,     /* call order (e.g. solve before preprocess)      */
+  UGMG_E_NOMEM = -4,     /* device or host allocation failed               */
+  UGMG_E_SINGULAR = -5,  /* zero pivot in the coarse LU / singular block   */
+  UGMG_E_BUDGET = -6,    /* request does not fit the HBM budget            */
+  UGMG_E_NODEV = -7      /* no GPU                                          */
+};
+
+enum { UGMG_SMOOTH_JAC = 0, UGMG_SMOOTH_GS = 1 };   /* GS = multicolour GS  */
+enum { UGMG_FP64 = 0, UGMG_MIXED = 1 };
+
+/* lmgc / smoother / base-solver settings, named after the numproc options. */
+typedef struct {
+  int nu1, nu2, gamma, baselevel;   /* lmgc $n1 $n2 $g $b                    */
+  int smoother;                     /* UGMG_SMOOTH_JAC / UGMG_SMOOTH_GS      */
+  double damp[UGMG_MAXB];           /* smoother $damp                         */
+  double lmgc_damp[UGMG_MAXB];      /* lmgc $damp (interpolation damping)     */
+  double base_damp[UGMG_MAXB];      /* ex $damp                               */
+  double base_abslimit[UGMG_MAXB];  /* base ls $abslimit                      */
+  double base_red[UGMG_MAXB];       /* base ls $red                           */
+  int base_maxiter;                 /* base ls $m                             */
+  int ex_reorder;                   /* ex $o (BFS band reorder)               */
+  int precision;                    /* UGMG_FP64 or UGMG_MIXED                */
+  int inner_cycles;                 /* MIXED: FP32 V-cycles per FP64 step     */
+  int use_graph;                    /* replay the cycle from a hipGraph       */
+} ugmg_params;
+
+/* LRESULT (np/procs/ls.h:71-77) */
+typedef struct {
+  int error_code;
+  int converged;
+  double first_defect[UGMG_MAXB];
+  double last_defect[UGMG_MAXB];
+  int number_of_linear_iterations;
+} ugmg_lresult;
+
+/* called after every iteration with the per-component defect (DoPCR hook) */
+typedef void (*ugmg_defect_cb)(void *user, int iteration, int ncomp,
+                               const double *defect);
+
+typedef struct ugmg_ctx ugmg_ctx;
+
+/* defaults of the numprocs: n1=n2=g=1, b=0, damp 1, base ls $m 1 $red 1e-12
+ * $abslimit 1e-10 (ABS_LIMIT, ls.cc:66), ex $o 1, FP64, no graph. */
+void ugmg_params_default(ugmg_params *p);
+
+int ugmg_create(int device, ugmg_ctx **out);
+int ugmg_destroy(ugmg_ctx *ctx);
+const char *ugmg_last_error(const ugmg_ctx *ctx);
+int ugmg_set_stream(ugmg_ctx *ctx, void *hip_stream);  /* NULL: own stream */
+int ugmg_sync(ugmg_ctx *ctx);
+
+/* ---- hierarchy (LmgcPreProcess input) ----------------------------------
+ * ugmg_set_nlevels first, then one ugmg_set_level per level (0 = coarsest)
+ * and one ugmg_set_transfer per level >= 1, then ugmg_set_params and
+ * ugmg_preprocess.  row_ptr/col_idx/vals: ConvertMatrix order, diagonal
+ * block first in each row, vals = nnzb*b*b row-major blocks.  skip =
+ * VECSKIP bits, vclass/vnclass = VCLASS/VNCLASS (gm/gm.h:2200-2288).
+ * Interpolation rows: coarse corners of the father element with their GNs
+ * weights (StandardIntCorNodeVector, transgrid.cc:235-336). */
+int ugmg_set_nlevels(ugmg_ctx *ctx, int nlevels, int b);
+int ugmg_set_level(ugmg_ctx *ctx, int level, int n, const int *row_ptr,
+                   const int *col_idx, const double *vals,
+                   const uint32_t *skip, const uint8_t *vclass,
+                   const uint8_t *vnclass);
+int ugmg_set_transfer(ugmg_ctx *ctx, int level, const int *p_ptr,
+                      const int *p_idx, const double *p_w);
+int ugmg_set_params(ugmg_ctx *ctx, const ugmg_params *p);
+/* LmgcPreProcess (iter.cc:7707-7739): device layouts of levels bl..level,
+ * colouring (GS), coarse band LU at the base level (EXPreProcess). */
+int ugmg_preprocess(ugmg_ctx *ctx, int level);
+int ugmg_postprocess(ugmg_ctx *ctx);         /* LmgcPostProcess            */
+int ugmg_level_size(const ugmg_ctx *ctx, int level, int *n, int *nnzb);
+int ugmg_num_colors(const ugmg_ctx *ctx, int level);
+/* greedy colour of every vector of a level (host array of n ints) */
+int ugmg_get_colors(ugmg_ctx *ctx, int level, int *colors);
+/* decompress the device matrix layout back to CSR (host arrays sized like
+ * the input); checks that the device index arrays carry UG's pattern */
+int ugmg_get_pattern(ugmg_ctx *ctx, int level, int *row_ptr, int *col_idx,
+                     double *vals);
+
+/* ---- ugblas on one level, ALL_VECTORS, device pointers ----------------- */
+int ugmg_dmatmul_minus(ugmg_ctx *ctx, int level, double *x, const double *y); /* x -= A y */
+int ugmg_dmatmul(ugmg_ctx *ctx, int level, double *x, const double *y);       /* x  = A y */
+int ugmg_smatmul_minus(ugmg_ctx *ctx, int level, float *x, const float *y);   /* FP32 copy of A */
+int ugmg_dnrm2x(ugmg_ctx *ctx, int level, const double *x, double *out_host);
+int ugmg_ddot(ugmg_ctx *ctx, int level, const double *x, const double *y,
+              double *out_host);
+
+/* ---- smoothers: Smoother (iter.cc:817-842) = Step; x *= damp; b -= A x -- */
+int ugmg_jacobi_step(ugmg_ctx *ctx, int level, double *x, double *b);
+int ugmg_gs_step(ugmg_ctx *ctx, int level, double *x, double *b);
+
+/* ---- transfer (transgrid.cc:117-336); level = the fine level ---------- */
+int ugmg_restrict(ugmg_ctx *ctx, int level, double *to_coarse,
+                  const double *from_fine, const double *damp);
+int ugmg_interpolate(ugmg_ctx *ctx, int level, double *to_fine,
+                     const double *from_coarse, const double *damp);
+
+/* ---- base solver: ls($m,$red,$abslimit) over ex, on the base level ----- */
+int ugmg_base_solve(ugmg_ctx *ctx, double *c, double *b);
+
+/* ---- NP_ITER: one lmgc cycle on `level`: c := correction, b -= A c ----- */
+int ugmg_iter(ugmg_ctx *ctx, int level, double *c, double *b);
+int ugmg_iter_host(ugmg_ctx *ctx, int level, double *c, double *b);
+
+/* ---- NP_LINEAR_SOLVER: ls with lmgc (ls.cc:562-749) ---------------------
+ * do_defect: LinearDefect (b -= A x) before the first residuum.  b holds the
+ * defect on return, x the solution.  cb may be NULL. */
+int ugmg_solve(ugmg_ctx *ctx, int level, double *x, double *b,
+               const double *abslimit, const double *reduction, int maxiter,
+               int do_defect, ugmg_lresult *res, ugmg_defect_cb cb, void *user);
+int ugmg_solve_host(ugmg_ctx *ctx, int level, double *x, double *b,
+                    const double *abslimit, const double *reduction,
+                    int maxiter, int do_defect, ugmg_lresult *res,
+                    ugmg_defect_cb cb, void *user);
+
+/* ---- measurement --------------------------------------------------------
+ * kind: 0 dmatmul_minus FP64, 1 smatmul_minus FP32, 2 jacobi step FP64,
+ *       3 gs step FP64, 4 restrict, 5 interpolate, 6 one cycle (current
+ *       precision), 7 gs step FP32.  Runs `reps` back-to-back launches on
+ *       the context stream between two hipEvents and returns the mean
+ *       duration in ms, plus the algorithmic bytes of one launch. */
+int ugmg_time_kernel(ugmg_ctx *ctx, int level, int kind, int reps,
+                     double *ms_per_launch, double *bytes_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,82 @@
+/*
+ * ugmg_problems.h -- synthetic multigrid problems in UG's per-level layout.
+ *
+ * UG ships no discretisation: an application assembles one stiffness
+ * matrix per grid level into the MATRIX lists of the multigrid
+ * (gm/gm.h:601-674), applies Dirichlet rows with AssembleDirichletBoundary
+ * (np/udm/disctools.cc:1837-1875) and lets the standard transfer
+ * interpolate with father-element shape functions (np/algebra/transgrid.cc:
+ * 117-336, gm/shapes.cc:748-819).  This library is that application for
+ * the solve-path workloads: uniformly refined Q1 grids on the unit square
+ * and cube, assembled per level, flattened in ConvertMatrix order
+ * (np/udm/disctools.cc:2061-2128: vector list order, diagonal block first
+ * in each row).
+ *
+ * It is host-only C++ and has no GPU dependency.  All arrays returned are
+ * owned by the problem object and live until ugmg_problem_free().
+ */
+#ifndef UGMG_PROBLEMS_H
+#define UGMG_PROBLEMS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ugmg_problem ugmg_problem;
+
+/* kappa_mode */
+#define UGMG_KAPPA_ONE   0   /* -Laplace u = f                                  */
+#define UGMG_KAPPA_JUMP  1   /* kappa per finest element = 10^U[-2,2]            */
+
+/* Uniformly refined Q1 hierarchy.
+ *   dim          2 (bilinear quads) or 3 (trilinear hexes)
+ *   coarse_cells cells per direction on level 0 (2 for the 2x2 quad grid,
+ *                1 for the single hex)
+ *   nref         number of uniform refinements; levels 0..nref
+ *   kappa_mode   UGMG_KAPPA_ONE or UGMG_KAPPA_JUMP
+ *   seed         SplitMix64 seed: kappa of every finest element (x fastest),
+ *                then the nodal f of every finest node, both drawn in order
+ * Coarse-level matrices integrate the finest-level coefficient exactly
+ * (element-local Galerkin products), so every level is the FE stiffness
+ * matrix of the same bilinear form.  Zero Dirichlet data on the whole
+ * boundary; rhs = consistent Q1 mass matrix times nodal f, zero on
+ * Dirichlet nodes.  Returns 0 or a negative error code. */
+int ugmg_problem_q1(int dim, int coarse_cells, int nref, int kappa_mode,
+                    uint64_t seed, ugmg_problem **out);
+
+void ugmg_problem_free(ugmg_problem *p);
+
+int ugmg_problem_nlevels(const ugmg_problem *p);     /* nref + 1            */
+int ugmg_problem_block(const ugmg_problem *p);       /* components per node */
+
+/* Level l matrix in ConvertMatrix order: n block rows, row_ptr[n+1],
+ * col_idx[nnzb], vals[nnzb*b*b] (row-major b x b blocks); per vector the
+ * VECSKIP bitmask, VCLASS and VNCLASS (gm/gm.h:2200-2208, 2282-2288). */
+int ugmg_problem_level(const ugmg_problem *p, int level, int *n, int *nnzb,
+                       const int **row_ptr, const int **col_idx,
+                       const double **vals, const uint32_t **skip,
+                       const uint8_t **vclass, const uint8_t **vnclass);
+
+/* Interpolation from level-1 to level (level >= 1) as CSR over fine
+ * vectors: row v lists the coarse corners of v's father element with their
+ * nonzero GNs weights, in element corner order (a corner node has the one
+ * entry (father, 1.0)). */
+int ugmg_problem_transfer(const ugmg_problem *p, int level, int *nnz,
+                          const int **p_ptr, const int **p_idx,
+                          const double **p_w);
+
+/* Finest-level right-hand side (n*b doubles) and grid geometry. */
+int ugmg_problem_rhs(const ugmg_problem *p, const double **b);
+int ugmg_problem_grid(const ugmg_problem *p, int level, int *dim,
+                      int *nodes_per_dir);
+
+/* SplitMix64 stream shared with the tests: fills out[i] = a+(b-a)*u_i. */
+void ugmg_splitmix_uniform(uint64_t seed, int64_t count, double a, double b,
+                           double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,246 @@
+"""ctypes front end of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker or the timed CPU
+baseline.  The product (ug_amd, libugmg) never imports it.
+
+The oracle restates UG's solve path (see ugmg_oracle.h for the file:line of
+every routine) over the flattened per-level arrays of ug_amd.problems.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+JAC, GS, MCGS = 0, 1, 2
+
+
+class OrcLevel(C.Structure):
+    _fields_ = [("n", C.c_int), ("b", C.c_int),
+                ("rp", C.c_void_p), ("ci", C.c_void_p), ("a", C.c_void_p),
+                ("skip", C.c_void_p), ("vclass", C.c_void_p), ("vnclass", C.c_void_p),
+                ("pp", C.c_void_p), ("pi", C.c_void_p), ("pw", C.c_void_p)]
+
+
+class OrcParams(C.Structure):
+    _fields_ = [("nu1", C.c_int), ("nu2", C.c_int), ("gamma", C.c_int), ("baselevel", C.c_int),
+                ("smoother", C.c_int),
+                ("damp", C.c_double * 3), ("lmgc_damp", C.c_double * 3), ("base_damp", C.c_double * 3),
+                ("base_abslimit", C.c_double * 3), ("base_red", C.c_double * 3),
+                ("base_maxiter", C.c_int), ("ex_reorder", C.c_int)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} not built (make -C oracle)")
+        L = C.CDLL(LIB_PATH)
+        vp, dp = C.c_void_p, C.c_void_p
+        L.orc_dmatmul_minus.argtypes = [vp, dp, dp]
+        L.orc_dmatmul.argtypes = [vp, dp, dp]
+        L.orc_dnrm2x.argtypes = [vp, dp, dp]
+        L.orc_ddot.argtypes = [vp, dp, dp, dp]
+        L.orc_solve_small_block.argtypes = [C.c_int, dp, dp, dp]
+        L.orc_l_jac.argtypes = [vp, dp, dp]
+        L.orc_l_lgs.argtypes = [vp, vp, dp, dp]
+        L.orc_greedy_color.argtypes = [vp, vp]
+        L.orc_smoother.argtypes = [vp, C.c_int, vp, dp, dp, dp]
+        L.orc_restrict.argtypes = [vp, vp, dp, dp, dp]
+        L.orc_interpolate.argtypes = [vp, vp, dp, dp, dp]
+        L.orc_ex_create.argtypes = [vp, C.c_int, C.POINTER(C.c_void_p)]
+        L.orc_ex_free.argtypes = [vp]
+        L.orc_ex_free.restype = None
+        L.orc_ex_bandwidth.argtypes = [vp]
+        L.orc_ex_order.argtypes = [vp]
+        L.orc_ex_order.restype = C.POINTER(C.c_int)
+        L.orc_ex_smoother.argtypes = [vp, vp, dp, dp, dp]
+        L.orc_ls_lmgc_solve.argtypes = [vp, C.c_int, vp, dp, dp, C.c_int, dp, dp, C.c_int, dp,
+                                        C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.orc_lmgc_iter.argtypes = [vp, C.c_int, vp, dp, dp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+class Hierarchy:
+    """Keeps numpy arrays alive and exposes an OrcLevel array."""
+
+    def __init__(self, levels):
+        self._keep = []
+        arr = (OrcLevel * len(levels))()
+        for i, lv in enumerate(levels):
+            fields = dict(rp=np.ascontiguousarray(lv.row_ptr, np.int32),
+                          ci=np.ascontiguousarray(lv.col_idx, np.int32),
+                          a=np.ascontiguousarray(lv.vals, np.float64),
+                          skip=np.ascontiguousarray(lv.skip, np.uint32),
+                          vclass=np.ascontiguousarray(lv.vclass, np.uint8),
+                          vnclass=np.ascontiguousarray(lv.vnclass, np.uint8))
+            if lv.p_ptr is not None:
+                fields.update(pp=np.ascontiguousarray(lv.p_ptr, np.int32),
+                              pi=np.ascontiguousarray(lv.p_idx, np.int32),
+                              pw=np.ascontiguousarray(lv.p_w, np.float64))
+            self._keep.append(fields)
+            arr[i].n, arr[i].b = lv.n, lv.b
+            for k, v in fields.items():
+                setattr(arr[i], k, v.ctypes.data)
+        self.arr = arr
+        self.levels = levels
+
+    def ptr(self, l=0):
+        return C.addressof(self.arr) + l * C.sizeof(OrcLevel)
+
+
+def params(nu1=2, nu2=2, gamma=1, baselevel=0, smoother=JAC, damp=0.8, lmgc_damp=1.0,
+           base_damp=1.0, base_abslimit=1e-300, base_red=1e-12, base_maxiter=1, ex_reorder=1):
+    p = OrcParams()
+    p.nu1, p.nu2, p.gamma, p.baselevel, p.smoother = nu1, nu2, gamma, baselevel, smoother
+    for i in range(3):
+        p.damp[i] = damp
+        p.lmgc_damp[i] = lmgc_damp
+        p.base_damp[i] = base_damp
+        p.base_abslimit[i] = base_abslimit
+        p.base_red[i] = base_red
+    p.base_maxiter, p.ex_reorder = base_maxiter, ex_reorder
+    return p
+
+
+def dmatmul_minus(H, l, x, y):
+    x = np.array(x, dtype=np.float64, copy=True)
+    lib().orc_dmatmul_minus(H.ptr(l), x.ctypes.data, np.ascontiguousarray(y, np.float64).ctypes.data)
+    return x
+
+
+def dmatmul(H, l, y):
+    lv = H.levels[l]
+    x = np.zeros(lv.n * lv.b)
+    lib().orc_dmatmul(H.ptr(l), x.ctypes.data, np.ascontiguousarray(y, np.float64).ctypes.data)
+    return x
+
+
+def dnrm2x(H, l, x):
+    out = np.zeros(3)
+    lib().orc_dnrm2x(H.ptr(l), np.ascontiguousarray(x, np.float64).ctypes.data, out.ctypes.data)
+    return out[:H.levels[l].b]
+
+
+def l_jac(H, l, d):
+    lv = H.levels[l]
+    v = np.zeros(lv.n * lv.b)
+    rc = lib().orc_l_jac(H.ptr(l), v.ctypes.data, np.ascontiguousarray(d, np.float64).ctypes.data)
+    return rc, v
+
+
+def l_lgs(H, l, d, rank=None):
+    lv = H.levels[l]
+    v = np.zeros(lv.n * lv.b)
+    rk = None if rank is None else np.ascontiguousarray(rank, np.int32)
+    rc = lib().orc_l_lgs(H.ptr(l), _p(rk), v.ctypes.data, np.ascontiguousarray(d, np.float64).ctypes.data)
+    return rc, v
+
+
+def greedy_color(H, l):
+    col = np.zeros(H.levels[l].n, dtype=np.int32)
+    nc = lib().orc_greedy_color(H.ptr(l), col.ctypes.data)
+    return nc, col
+
+
+def smoother(H, l, kind, damp, b, rank=None):
+    """UG Smoother: returns (x, b_new)."""
+    lv = H.levels[l]
+    x = np.zeros(lv.n * lv.b)
+    b = np.array(b, dtype=np.float64, copy=True)
+    dv = np.full(3, damp, dtype=np.float64) if np.isscalar(damp) else np.asarray(damp, np.float64)
+    rk = None if rank is None else np.ascontiguousarray(rank, np.int32)
+    rc = lib().orc_smoother(H.ptr(l), kind, _p(rk), dv.ctypes.data, x.ctypes.data, b.ctypes.data)
+    if rc:
+        raise RuntimeError("smoother failed")
+    return x, b
+
+
+def restrict(H, l, bf, damp=1.0, to_c=None):
+    lc = H.levels[l - 1]
+    out = np.zeros(lc.n * lc.b) if to_c is None else np.array(to_c, dtype=np.float64, copy=True)
+    dv = np.full(3, damp, dtype=np.float64)
+    lib().orc_restrict(H.ptr(l), H.ptr(l - 1), out.ctypes.data, np.ascontiguousarray(bf, np.float64).ctypes.data,
+                       dv.ctypes.data)
+    return out
+
+
+def interpolate(H, l, cc, damp=1.0):
+    lf = H.levels[l]
+    out = np.zeros(lf.n * lf.b)
+    dv = np.full(3, damp, dtype=np.float64)
+    lib().orc_interpolate(H.ptr(l), H.ptr(l - 1), out.ctypes.data, np.ascontiguousarray(cc, np.float64).ctypes.data,
+                          dv.ctypes.data)
+    return out
+
+
+class Ex:
+    def __init__(self, H, l, reorder=1):
+        self.H, self.l = H, l
+        h = C.c_void_p()
+        self.rc = lib().orc_ex_create(H.ptr(l), reorder, C.byref(h))
+        self.h = h
+
+    @property
+    def bandwidth(self):
+        return lib().orc_ex_bandwidth(self.h)
+
+    @property
+    def order(self):
+        n = self.H.levels[self.l].n
+        return np.ctypeslib.as_array(lib().orc_ex_order(self.h), shape=(n,)).copy()
+
+    def smooth(self, b, damp=1.0):
+        lv = self.H.levels[self.l]
+        x = np.zeros(lv.n * lv.b)
+        b = np.array(b, dtype=np.float64, copy=True)
+        dv = np.full(3, damp, dtype=np.float64)
+        rc = lib().orc_ex_smoother(self.h, self.H.ptr(self.l), dv.ctypes.data, x.ctypes.data, b.ctypes.data)
+        return rc, x, b
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_ex_free(self.h)
+            self.h = None
+
+
+def lmgc_iter(H, top, p, b):
+    lv = H.levels[top]
+    c = np.zeros(lv.n * lv.b)
+    b = np.array(b, dtype=np.float64, copy=True)
+    rc = lib().orc_lmgc_iter(H.ptr(0), top, C.byref(p), c.ctypes.data, b.ctypes.data)
+    if rc:
+        raise RuntimeError(f"lmgc failed ({rc})")
+    return c, b
+
+
+def ls_solve(H, top, p, rhs, x0=None, red=1e-10, abslimit=1e-300, maxiter=50, do_defect=True):
+    """ls + lmgc solve; returns dict(x, b, defects[nit+1, b], nit, converged)."""
+    lv = H.levels[top]
+    nb = lv.b
+    x = np.zeros(lv.n * nb) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
+    b = np.array(rhs, dtype=np.float64, copy=True)
+    ab = np.full(3, abslimit)
+    rd = np.full(3, red)
+    defects = np.zeros((maxiter + 1) * nb)
+    nit, conv = C.c_int(), C.c_int()
+    rc = lib().orc_ls_lmgc_solve(H.ptr(0), top, C.byref(p), x.ctypes.data, b.ctypes.data, int(do_defect),
+                                 ab.ctypes.data, rd.ctypes.data, maxiter, defects.ctypes.data,
+                                 C.byref(nit), C.byref(conv))
+    if rc:
+        raise RuntimeError(f"oracle solve failed ({rc})")
+    k = nit.value
+    return dict(x=x, b=b, defects=defects[:(k + 1) * nb].reshape(k + 1, nb), nit=k, converged=bool(conv.value))

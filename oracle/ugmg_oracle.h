@@ -1,0 +1,124 @@
+/*
+ * ugmg_oracle.h -- CPU restatement of UG 3.12.1's linear multigrid solve
+ * path, for TESTS ONLY.
+ *
+ * This is test infrastructure: only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg load it, and only as the checker (or the timed
+ * CPU baseline).  The product library (libugmg) never links or calls it.
+ *
+ * Every function restates the UG routine named in its comment, in UG's loop
+ * order (vector list order, row lists diagonal first), in FP64, single
+ * threaded.  Data come in ConvertMatrix layout (np/udm/disctools.cc:
+ * 2061-2128): per level n vectors of b components, row_ptr/col_idx over
+ * block entries with the diagonal block first in each row, b x b row-major
+ * blocks; vector flags VECSKIP / VCLASS / VNCLASS; the standard node
+ * interpolation from level-1 as CSR rows of (coarse corner, GNs weight).
+ *
+ * Parity pins (tests/test_oracle_*.py): dense numpy products, closed-form
+ * Q1 transfer weights and invariants, scipy triangular and dense solves,
+ * and the textbook multigrid error-propagation operator
+ * E = S^nu2 (I - P (I - E_c) A_c^{-1} R A) S^nu1.
+ */
+#ifndef UGMG_ORACLE_H
+#define UGMG_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_MAXB 3
+
+typedef struct {
+  int n;                  /* vectors on the level                         */
+  int b;                  /* components per vector (1 or 3)               */
+  const int *rp;          /* n+1                                          */
+  const int *ci;          /* nnzb; diagonal first in every row            */
+  const double *a;        /* nnzb*b*b                                     */
+  const uint32_t *skip;   /* VECSKIP bit j = component j Dirichlet        */
+  const uint8_t *vclass;  /* VCLASS                                       */
+  const uint8_t *vnclass; /* VNCLASS                                      */
+  const int *pp;          /* interpolation from level-1 (NULL on level 0) */
+  const int *pi;
+  const double *pw;
+} orc_level;
+
+enum { ORC_JAC = 0, ORC_GS = 1, ORC_MCGS = 2 };
+
+typedef struct {
+  int nu1, nu2, gamma, baselevel;
+  int smoother;                  /* ORC_JAC, ORC_GS (l_lgs), ORC_MCGS     */
+  double damp[ORC_MAXB];         /* smoother $damp                         */
+  double lmgc_damp[ORC_MAXB];    /* lmgc $damp (interpolation damping)     */
+  double base_damp[ORC_MAXB];    /* ex $damp                               */
+  double base_abslimit[ORC_MAXB];/* base ls $abslimit                      */
+  double base_red[ORC_MAXB];     /* base ls $red                           */
+  int base_maxiter;              /* base ls $m                             */
+  int ex_reorder;                /* ex $o                                  */
+} orc_params;
+
+/* ugblas (np/algebra/ugblas.cc) on one level, ALL_VECTORS */
+void orc_dmatmul_minus(const orc_level *L, double *x, const double *y);
+void orc_dmatmul(const orc_level *L, double *x, const double *y);
+void orc_dnrm2x(const orc_level *L, const double *x, double *out /* b */);
+void orc_ddot(const orc_level *L, const double *x, const double *y, double *out);
+
+/* SolveSmallBlock (np/algebra/block.cc:104-205); 0 ok, 1 singular */
+int orc_solve_small_block(int n, const double *mat, const double *rhs,
+                          double *sol);
+
+/* l_jac (np/algebra/ugiter.cc:271-335) */
+int orc_l_jac(const orc_level *L, double *v, const double *d);
+/* l_lgs (np/algebra/ugiter.cc:412-615) with VINDEX replaced by rank[]:
+ * rank = list position gives UG's lexicographic GS (l_setindex,
+ * ugiter.cc:169-186); rank = colour*n + position gives the multicolour
+ * ordering of the GPU smoother.  rank == NULL means list position. */
+int orc_l_lgs(const orc_level *L, const int *rank, double *v, const double *d);
+/* greedy first-fit colouring in list order over the matrix pattern;
+ * returns the number of colours */
+int orc_greedy_color(const orc_level *L, int *color);
+
+/* Smoother (np/procs/iter.cc:817-842): Step, x *= damp, b -= A x */
+int orc_smoother(const orc_level *L, int kind, const int *rank,
+                 const double *damp, double *x, double *b);
+
+/* StandardRestrictNodeVector / StandardIntCorNodeVector
+ * (np/algebra/transgrid.cc:117-215, 235-336) */
+void orc_restrict(const orc_level *F, const orc_level *C, double *to_c,
+                  const double *from_f, const double *damp);
+void orc_interpolate(const orc_level *F, const orc_level *C, double *to_f,
+                     const double *from_c, const double *damp);
+
+/* ex band LU (np/procs/iter.cc:8685-9071, gm/evm.cc:1400-1510) */
+typedef struct orc_ex orc_ex;
+int orc_ex_create(const orc_level *L, int reorder, orc_ex **out); /* 0 ok, 1 zero pivot */
+void orc_ex_free(orc_ex *ex);
+int orc_ex_bandwidth(const orc_ex *ex);
+const int *orc_ex_order(const orc_ex *ex);       /* new list order        */
+/* EXSmoother: x = damp * LU^{-1} b; b -= A x */
+int orc_ex_smoother(const orc_ex *ex, const orc_level *L, const double *damp,
+                    double *x, double *b);
+
+/* The ls + lmgc + smoother + transfer + ex solve (np/procs/ls.cc:637-749,
+ * np/procs/iter.cc:7741-7949).  levels[0..top].  x,b are level `top`
+ * vectors; b is the right-hand side on entry (the defect recurrence runs
+ * in it).  do_defect: apply LinearDefect (b -= A x) first.
+ * defects receives (maxiter+1)*b norms: first defect then one per
+ * iteration; *nit the iteration count, *converged the flag.
+ * Returns 0 or a nonzero error. */
+int orc_ls_lmgc_solve(const orc_level *levels, int top, const orc_params *p,
+                      double *x, double *b, int do_defect,
+                      const double *abslimit, const double *reduction,
+                      int maxiter, double *defects, int *nit, int *converged);
+
+/* One lmgc iteration on `top` (NP_ITER semantics): c overwritten with the
+ * correction, b := b - A c.  Work vectors for the coarse levels are
+ * allocated internally. */
+int orc_lmgc_iter(const orc_level *levels, int top, const orc_params *p,
+                  double *c, double *b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

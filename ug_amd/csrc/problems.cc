@@ -1,0 +1,421 @@
+// Synthetic UG-layout multigrid problems (host only).
+//
+// Plays the part of the UG application that assembles one matrix per level
+// (UG has no discretisation of its own; the solve path reads whatever the
+// application put into the MATRIX lists).  Conventions follow UG:
+//   * vector list per level, one vector per node, listed lexicographically
+//     (x fastest), flattened as in ConvertMatrix (np/udm/disctools.cc:
+//     2061-2128): VINDEX = list position, each row starts with its diagonal
+//     MATRIX (VSTART, gm/gm.h:2298) followed by the off-diagonal entries,
+//     here in ascending column order;
+//   * Dirichlet rows as AssembleDirichletBoundary (np/udm/disctools.cc:
+//     1837-1875): row zeroed, diagonal 1, column kept, rhs = solution = 0;
+//   * interpolation weights = GNs of the father element at the node's local
+//     coordinate (gm/shapes.cc:748-819; corner order (0,0),(1,0),(1,1),(0,1)
+//     and the hexahedron's bottom then top face), as used by
+//     StandardIntCorNodeVector (np/algebra/transgrid.cc:235-336).
+#include "../../include/ugmg_problems.h"
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <new>
+#include <vector>
+
+namespace {
+
+struct Level {
+  int N = 0;        // nodes per direction
+  int C = 0;        // cells per direction
+  int n = 0;        // vectors
+  std::vector<int> rp, ci;
+  std::vector<double> v;
+  std::vector<uint32_t> skip;
+  std::vector<uint8_t> vc, vnc;
+  std::vector<int> pp, pi;      // interpolation from level-1 (empty on 0)
+  std::vector<double> pw;
+};
+
+inline uint64_t splitmix_next(uint64_t &s) {
+  uint64_t z = (s += 0x9e3779b97f4a7c15ULL);
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+inline double splitmix_u01(uint64_t &s) {
+  return (double)(splitmix_next(s) >> 11) * 0x1.0p-53;
+}
+
+// UG corner order of the reference quadrilateral / hexahedron, as tensor bits
+// (bit0 = x, bit1 = y, bit2 = z).
+const int kQuadCorner[4] = {0, 1, 3, 2};
+const int kHexCorner[8] = {0, 1, 3, 2, 4, 5, 7, 6};
+
+}  // namespace
+
+struct ugmg_problem {
+  int dim = 0;
+  int b = 1;
+  std::vector<Level> lv;
+  std::vector<double> rhs;
+};
+
+extern "C" void ugmg_splitmix_uniform(uint64_t seed, int64_t count, double a,
+                                      double b, double *out) {
+  uint64_t s = seed;
+  for (int64_t i = 0; i < count; ++i) out[i] = a + (b - a) * splitmix_u01(s);
+}
+
+namespace {
+
+// Q1 element stiffness of -div(grad) on a cube/square of side h, tensor
+// local numbering: K[a][b] = sum_d s(a_d,b_d) prod_{e!=d} m(a_e,b_e) with the
+// 1D stiffness s = (1/h)[[1,-1],[-1,1]] and mass m = (h/6)[[2,1],[1,2]].
+void q1_stiffness(int dim, double h, double *K) {
+  const int nc = 1 << dim;
+  for (int a = 0; a < nc; ++a)
+    for (int bb = 0; bb < nc; ++bb) {
+      double sum = 0.0;
+      for (int d = 0; d < dim; ++d) {
+        int ad = (a >> d) & 1, bd = (bb >> d) & 1;
+        double t = (ad == bd ? 1.0 : -1.0) / h;
+        for (int e = 0; e < dim; ++e) {
+          if (e == d) continue;
+          int ae = (a >> e) & 1, be = (bb >> e) & 1;
+          t *= (ae == be ? 2.0 : 1.0) * h / 6.0;
+        }
+        sum += t;
+      }
+      K[a * nc + bb] = sum;
+    }
+}
+
+// Q[c][a][J]: coarse basis J (tensor numbering) at corner a of child c.
+void child_interp(int dim, std::vector<double> &Q) {
+  const int nc = 1 << dim;
+  Q.assign((size_t)nc * nc * nc, 0.0);
+  for (int c = 0; c < nc; ++c)
+    for (int a = 0; a < nc; ++a)
+      for (int J = 0; J < nc; ++J) {
+        double w = 1.0;
+        for (int d = 0; d < dim; ++d) {
+          double p = 0.5 * (((c >> d) & 1) + ((a >> d) & 1));
+          w *= ((J >> d) & 1) ? p : 1.0 - p;
+        }
+        Q[((size_t)c * nc + a) * nc + J] = w;
+      }
+}
+
+// Kc += Q^T Kf Q  (nc x nc)
+void galerkin_add(int nc, const double *Q, const double *Kf, double scale,
+                  double *Kc) {
+  double T[64];
+  for (int a = 0; a < nc; ++a)
+    for (int J = 0; J < nc; ++J) {
+      double s = 0.0;
+      for (int bb = 0; bb < nc; ++bb) s += Kf[a * nc + bb] * Q[bb * nc + J];
+      T[a * nc + J] = s;
+    }
+  for (int I = 0; I < nc; ++I)
+    for (int J = 0; J < nc; ++J) {
+      double s = 0.0;
+      for (int a = 0; a < nc; ++a) s += Q[a * nc + I] * T[a * nc + J];
+      Kc[I * nc + J] += scale * s;
+    }
+}
+
+inline int64_t node_id(int dim, int N, int x, int y, int z) {
+  return dim == 2 ? (int64_t)y * N + x : ((int64_t)z * N + y) * N + x;
+}
+
+// Row-wise assembly of one level from element matrices.
+// emat(e) returns a pointer to the element matrix (tensor numbering) of cell e
+// or fills scratch for the finest level.
+template <class EMat>
+void assemble_level(int dim, Level &L, EMat emat) {
+  const int N = L.N, C = L.C;
+  const int64_t n = L.n;
+  const int nsl = dim == 2 ? 9 : 27;
+  // pattern: all stencil neighbours inside the grid
+  L.rp.assign(n + 1, 0);
+  for (int64_t i = 0; i < n; ++i) {
+    int x = (int)(i % N), y = (int)((i / N) % N), z = dim == 3 ? (int)(i / ((int64_t)N * N)) : 0;
+    int cx = 1 + (x > 0) + (x < N - 1), cy = 1 + (y > 0) + (y < N - 1);
+    int cz = dim == 3 ? 1 + (z > 0) + (z < N - 1) : 1;
+    L.rp[i + 1] = cx * cy * cz;
+  }
+  for (int64_t i = 0; i < n; ++i) L.rp[i + 1] += L.rp[i];
+  const int64_t nnz = L.rp[n];
+  L.ci.assign(nnz, 0);
+  L.v.assign(nnz, 0.0);
+  L.skip.assign(n, 0);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) {
+    int x = (int)(i % N), y = (int)((i / N) % N), z = dim == 3 ? (int)(i / ((int64_t)N * N)) : 0;
+    double acc[27];
+    bool valid[27];
+    for (int k = 0; k < nsl; ++k) { acc[k] = 0.0; valid[k] = false; }
+    const int zlo = dim == 3 ? -1 : 0, zhi = dim == 3 ? 0 : 0;
+    double scratch[64];
+    // every element touching node i
+    for (int ez = zlo; ez <= zhi; ++ez)
+      for (int ey = -1; ey <= 0; ++ey)
+        for (int ex = -1; ex <= 0; ++ex) {
+          int exx = x + ex, eyy = y + ey, ezz = dim == 3 ? z + ez : 0;
+          if (exx < 0 || exx >= C || eyy < 0 || eyy >= C) continue;
+          if (dim == 3 && (ezz < 0 || ezz >= C)) continue;
+          const int64_t e = dim == 2 ? (int64_t)eyy * C + exx
+                                     : ((int64_t)ezz * C + eyy) * C + exx;
+          const double *K = emat(e, scratch);
+          const int nc = 1 << dim;
+          int a = (-ex) | ((-ey) << 1) | (dim == 3 ? ((-ez) << 2) : 0);
+          for (int bb = 0; bb < nc; ++bb) {
+            int dx = (bb & 1) + ex, dy = ((bb >> 1) & 1) + ey;
+            int dz = dim == 3 ? ((bb >> 2) & 1) + ez : 0;
+            int k = dim == 2 ? (dy + 1) * 3 + (dx + 1)
+                             : (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
+            acc[k] += K[a * nc + bb];
+          }
+        }
+    for (int dz = (dim == 3 ? -1 : 0); dz <= (dim == 3 ? 1 : 0); ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          int xx = x + dx, yy = y + dy, zz = z + dz;
+          if (xx < 0 || xx >= N || yy < 0 || yy >= N) continue;
+          if (dim == 3 && (zz < 0 || zz >= N)) continue;
+          int k = dim == 2 ? (dy + 1) * 3 + (dx + 1)
+                           : (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
+          valid[k] = true;
+        }
+    const int kdiag = dim == 2 ? 4 : 13;
+    int64_t p = L.rp[i];
+    L.ci[p] = (int)i;
+    L.v[p] = acc[kdiag];
+    ++p;
+    for (int k = 0; k < nsl; ++k) {
+      if (k == kdiag || !valid[k]) continue;
+      int dx = k % 3 - 1, dy = (k / 3) % 3 - 1, dz = dim == 3 ? k / 9 - 1 : 0;
+      L.ci[p] = (int)node_id(dim, N, x + dx, y + dy, z + dz);
+      L.v[p] = acc[k];
+      ++p;
+    }
+    const bool bnd = x == 0 || x == N - 1 || y == 0 || y == N - 1 ||
+                     (dim == 3 && (z == 0 || z == N - 1));
+    if (bnd) {
+      // AssembleDirichletBoundary: zero the row, unit diagonal.
+      L.skip[i] = 1u;
+      for (int64_t q = L.rp[i]; q < L.rp[i + 1]; ++q) L.v[q] = 0.0;
+      L.v[L.rp[i]] = 1.0;
+    }
+  }
+  L.vc.assign(n, 3);
+}
+
+void build_transfer(int dim, const Level &coarse, Level &fine) {
+  const int Nf = fine.N, Nc = coarse.N, Cc = coarse.C;
+  const int64_t n = fine.n;
+  const int nc = 1 << dim;
+  const int *corner = dim == 2 ? kQuadCorner : kHexCorner;
+  fine.pp.assign(n + 1, 0);
+  std::vector<int> tmp_idx((size_t)n * nc);
+  std::vector<double> tmp_w((size_t)n * nc);
+  std::vector<int> cnt(n);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) {
+    int c[3] = {(int)(i % Nf), (int)((i / Nf) % Nf),
+                dim == 3 ? (int)(i / ((int64_t)Nf * Nf)) : 0};
+    bool corner_node = true;
+    for (int d = 0; d < dim; ++d) corner_node &= (c[d] % 2 == 0);
+    int k = 0;
+    if (corner_node) {
+      tmp_idx[i * nc] = (int)node_id(dim, Nc, c[0] / 2, c[1] / 2, c[2] / 2);
+      tmp_w[i * nc] = 1.0;
+      k = 1;
+    } else {
+      int e[3] = {0, 0, 0};
+      double xi[3] = {0, 0, 0};
+      for (int d = 0; d < dim; ++d) {
+        if (c[d] % 2) { e[d] = (c[d] - 1) / 2; xi[d] = 0.5; }
+        else {
+          int X = c[d] / 2;
+          e[d] = X < Cc ? X : Cc - 1;
+          xi[d] = (double)(X - e[d]);
+        }
+      }
+      for (int q = 0; q < nc; ++q) {
+        int t = corner[q];
+        double w = 1.0;
+        for (int d = 0; d < dim; ++d) w *= ((t >> d) & 1) ? xi[d] : 1.0 - xi[d];
+        if (w == 0.0) continue;
+        tmp_idx[i * nc + k] = (int)node_id(dim, Nc, e[0] + (t & 1),
+                                          e[1] + ((t >> 1) & 1),
+                                          e[2] + ((t >> 2) & 1));
+        tmp_w[i * nc + k] = w;
+        ++k;
+      }
+    }
+    cnt[i] = k;
+  }
+  for (int64_t i = 0; i < n; ++i) fine.pp[i + 1] = fine.pp[i] + cnt[i];
+  fine.pi.resize(fine.pp[n]);
+  fine.pw.resize(fine.pp[n]);
+  for (int64_t i = 0; i < n; ++i)
+    for (int q = 0; q < cnt[i]; ++q) {
+      fine.pi[fine.pp[i] + q] = tmp_idx[i * nc + q];
+      fine.pw[fine.pp[i] + q] = tmp_w[i * nc + q];
+    }
+}
+
+}  // namespace
+
+extern "C" int ugmg_problem_q1(int dim, int coarse_cells, int nref,
+                               int kappa_mode, uint64_t seed,
+                               ugmg_problem **out) {
+  if (!out || (dim != 2 && dim != 3) || coarse_cells < 1 || nref < 0 ||
+      nref > 12 || (kappa_mode != UGMG_KAPPA_ONE && kappa_mode != UGMG_KAPPA_JUMP))
+    return -1;
+  // int32 indices: keep nnz below 2^31
+  {
+    double Nf = (double)coarse_cells * std::ldexp(1.0, nref) + 1.0;
+    double nnz = std::pow(3.0 * (Nf - 1) + 1, dim);
+    if (nnz > 2.0e9) return -1;
+  }
+  ugmg_problem *p = new (std::nothrow) ugmg_problem;
+  if (!p) return -4;
+  p->dim = dim;
+  p->b = 1;
+  const int L = nref;
+  p->lv.resize(L + 1);
+  for (int l = 0; l <= L; ++l) {
+    Level &lv = p->lv[l];
+    lv.C = coarse_cells << l;
+    lv.N = lv.C + 1;
+    lv.n = dim == 2 ? lv.N * lv.N : lv.N * lv.N * lv.N;
+  }
+  const int nc = 1 << dim;
+  const int64_t nel_f = dim == 2 ? (int64_t)p->lv[L].C * p->lv[L].C
+                                 : (int64_t)p->lv[L].C * p->lv[L].C * p->lv[L].C;
+  uint64_t s = seed;
+  std::vector<double> kappa(nel_f, 1.0);
+  if (kappa_mode == UGMG_KAPPA_JUMP)
+    for (int64_t e = 0; e < nel_f; ++e)
+      kappa[e] = std::pow(10.0, -2.0 + 4.0 * splitmix_u01(s));
+  std::vector<double> f(p->lv[L].n);
+  for (auto &fi : f) fi = -1.0 + 2.0 * splitmix_u01(s);
+
+  // finest level: kappa_e * Kref(h)
+  std::vector<double> Kref(nc * nc);
+  q1_stiffness(dim, 1.0 / p->lv[L].C, Kref.data());
+  assemble_level(dim, p->lv[L], [&](int64_t e, double *scratch) -> const double * {
+    for (int q = 0; q < nc * nc; ++q) scratch[q] = kappa[e] * Kref[q];
+    return scratch;
+  });
+  // coarser levels: element-local Galerkin products of the children
+  std::vector<double> Q;
+  child_interp(dim, Q);
+  std::vector<double> emat_child, emat;
+  for (int l = L - 1; l >= 0; --l) {
+    const int Cc = p->lv[l].C, Cf = p->lv[l + 1].C;
+    const int64_t nel = dim == 2 ? (int64_t)Cc * Cc : (int64_t)Cc * Cc * Cc;
+    emat.assign((size_t)nel * nc * nc, 0.0);
+#pragma omp parallel for schedule(static)
+    for (int64_t E = 0; E < nel; ++E) {
+      int X = (int)(E % Cc), Y = (int)((E / Cc) % Cc), Z = dim == 3 ? (int)(E / ((int64_t)Cc * Cc)) : 0;
+      double Kf[64];
+      for (int c = 0; c < nc; ++c) {
+        int fx = 2 * X + (c & 1), fy = 2 * Y + ((c >> 1) & 1), fz = 2 * Z + ((c >> 2) & 1);
+        int64_t fe = dim == 2 ? (int64_t)fy * Cf + fx : ((int64_t)fz * Cf + fy) * Cf + fx;
+        const double *Kc;
+        if (l == L - 1) {
+          for (int q = 0; q < nc * nc; ++q) Kf[q] = kappa[fe] * Kref[q];
+          Kc = Kf;
+        } else {
+          Kc = &emat_child[(size_t)fe * nc * nc];
+        }
+        galerkin_add(nc, &Q[(size_t)c * nc * nc], Kc, 1.0, &emat[(size_t)E * nc * nc]);
+      }
+    }
+    assemble_level(dim, p->lv[l], [&](int64_t e, double *) -> const double * {
+      return &emat[(size_t)e * nc * nc];
+    });
+    emat_child.swap(emat);
+  }
+  for (int l = 0; l <= L; ++l) p->lv[l].vnc.assign(p->lv[l].n, l < L ? 3 : 0);
+  for (int l = 1; l <= L; ++l) build_transfer(dim, p->lv[l - 1], p->lv[l]);
+
+  // rhs = M f (consistent Q1 mass, tensor product of (h/6)[1 4 1]); 0 on
+  // Dirichlet vectors (rhs = sol = 0).
+  {
+    const Level &F = p->lv[L];
+    const int N = F.N;
+    const double h = 1.0 / F.C;
+    const double m1[3] = {h / 6.0, 4.0 * h / 6.0, h / 6.0};
+    p->rhs.assign(F.n, 0.0);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < F.n; ++i) {
+      if (F.skip[i]) continue;
+      int x = (int)(i % N), y = (int)((i / N) % N), z = dim == 3 ? (int)(i / ((int64_t)N * N)) : 0;
+      double sum = 0.0;
+      for (int dz = (dim == 3 ? -1 : 0); dz <= (dim == 3 ? 1 : 0); ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+          for (int dx = -1; dx <= 1; ++dx) {
+            double w = m1[dx + 1] * m1[dy + 1] * (dim == 3 ? m1[dz + 1] : 1.0);
+            sum += w * f[node_id(dim, N, x + dx, y + dy, z + dz)];
+          }
+      p->rhs[i] = sum;
+    }
+  }
+  *out = p;
+  return 0;
+}
+
+extern "C" void ugmg_problem_free(ugmg_problem *p) { delete p; }
+
+extern "C" int ugmg_problem_nlevels(const ugmg_problem *p) {
+  return p ? (int)p->lv.size() : -1;
+}
+extern "C" int ugmg_problem_block(const ugmg_problem *p) { return p ? p->b : -1; }
+
+extern "C" int ugmg_problem_level(const ugmg_problem *p, int level, int *n,
+                                  int *nnzb, const int **row_ptr,
+                                  const int **col_idx, const double **vals,
+                                  const uint32_t **skip, const uint8_t **vclass,
+                                  const uint8_t **vnclass) {
+  if (!p || level < 0 || level >= (int)p->lv.size()) return -1;
+  const Level &L = p->lv[level];
+  if (n) *n = L.n;
+  if (nnzb) *nnzb = L.rp[L.n];
+  if (row_ptr) *row_ptr = L.rp.data();
+  if (col_idx) *col_idx = L.ci.data();
+  if (vals) *vals = L.v.data();
+  if (skip) *skip = L.skip.data();
+  if (vclass) *vclass = L.vc.data();
+  if (vnclass) *vnclass = L.vnc.data();
+  return 0;
+}
+
+extern "C" int ugmg_problem_transfer(const ugmg_problem *p, int level, int *nnz,
+                                     const int **p_ptr, const int **p_idx,
+                                     const double **p_w) {
+  if (!p || level < 1 || level >= (int)p->lv.size()) return -1;
+  const Level &L = p->lv[level];
+  if (nnz) *nnz = L.pp[L.n];
+  if (p_ptr) *p_ptr = L.pp.data();
+  if (p_idx) *p_idx = L.pi.data();
+  if (p_w) *p_w = L.pw.data();
+  return 0;
+}
+
+extern "C" int ugmg_problem_rhs(const ugmg_problem *p, const double **b) {
+  if (!p || !b) return -1;
+  *b = p->rhs.data();
+  return 0;
+}
+
+extern "C" int ugmg_problem_grid(const ugmg_problem *p, int level, int *dim,
+                                 int *nodes_per_dir) {
+  if (!p || level < 0 || level >= (int)p->lv.size()) return -1;
+  if (dim) *dim = p->dim;
+  if (nodes_per_dir) *nodes_per_dir = p->lv[level].N;
+  return 0;
+}

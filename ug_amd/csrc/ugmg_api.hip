@@ -1,0 +1,926 @@
+// C ABI of libugmg: context, device hierarchy, lmgc / ls drivers.
+//
+// The drivers issue the same operation sequence as UG's numprocs:
+//   Lmgc            np/procs/iter.cc:7741-7949
+//   Smoother        np/procs/iter.cc:817-842 (JacobiStep 917, GSStep 1039)
+//   LinearSolver    np/procs/ls.cc:637-749 (+ LinearDefect 562, Residuum 577)
+//   base ls over ex np/procs/ls.cc:637-749, np/procs/iter.cc:8972-9071
+// with three liberties that change no reported number:
+//   * the top-level correction c of a linear-solver iteration is accumulated
+//     straight into x (dset(c,0) + Lmgc + dadd(x,c) == Lmgc into x, up to
+//     the rounding of the additions; the defect recurrence never reads x);
+//   * the coarse-level dset(c,0) before the recursion is folded into the
+//     first kernel that writes c on that level (it stores instead of adds);
+//   * the ls stop test runs on the host after a one-double device-to-host
+//     copy per iteration, as UG's LinearSolver does after dnrm2x.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/ugmg.h"
+#include "ugmg_internal.h"
+#include "ugmg_kernels.h"
+
+using namespace ugmg;
+
+namespace {
+
+struct DevSellBuf {
+  int nslices = 0;
+  size_t nent = 0;
+  int *soff = nullptr, *rows = nullptr, *col = nullptr;
+  double *v64 = nullptr;
+  float *v32 = nullptr;
+  DSell<double> d64() const { return {nslices, soff, rows, col, v64}; }
+  DSell<float> d32() const { return {nslices, soff, rows, col, v32}; }
+  template <typename T>
+  DSell<T> d() const {
+    if constexpr (std::is_same_v<T, double>) return d64();
+    else return d32();
+  }
+};
+
+struct DevLevel {
+  int n = 0, nnzb = 0;
+  DevSellBuf A, L, U;
+  int ncolors = 0;
+  std::vector<int> color_slices, color;
+  double *diag64 = nullptr;
+  float *diag32 = nullptr;
+  uint8_t *vclass = nullptr, *vnclass = nullptr;
+  uint32_t *skip = nullptr;
+  int p_nnz = 0, pt_nnz = 0;
+  int *p_ptr = nullptr, *p_idx = nullptr, *pt_ptr = nullptr, *pt_idx = nullptr;
+  double *p_w64 = nullptr, *pt_w64 = nullptr;
+  float *p_w32 = nullptr, *pt_w32 = nullptr;
+  double *c = nullptr, *b = nullptr, *t = nullptr;
+  float *c32 = nullptr, *b32 = nullptr, *t32 = nullptr;
+
+  template <typename T> T *cv() { if constexpr (std::is_same_v<T, double>) return c; else return c32; }
+  template <typename T> T *bv() { if constexpr (std::is_same_v<T, double>) return b; else return b32; }
+  template <typename T> T *tv() { if constexpr (std::is_same_v<T, double>) return t; else return t32; }
+  template <typename T> const T *diag() { if constexpr (std::is_same_v<T, double>) return diag64; else return diag32; }
+  template <typename T> const T *pw() { if constexpr (std::is_same_v<T, double>) return p_w64; else return p_w32; }
+  template <typename T> const T *ptw() { if constexpr (std::is_same_v<T, double>) return pt_w64; else return pt_w32; }
+};
+
+}  // namespace
+
+struct ugmg_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  std::string err;
+  int nlevels = 0, b = 1;
+  std::vector<HostLevel> host;
+  std::vector<char> level_set, transfer_set;
+  ugmg_params p{};
+  bool params_set = false;
+  bool prepared = false;
+  int top = -1, bl = 0;
+  std::vector<DevLevel> dev;
+  std::vector<void *> allocs;
+  HostBase base;
+  double *base_inv = nullptr;
+  double *part = nullptr;
+  int part_cap = 0;
+  double *red = nullptr;      // device scalars
+  double *red_host = nullptr; // pinned
+  // cached graph of one solver iteration
+  hipGraphExec_t gexec = nullptr;
+  int gkind = -1, glevel = -1;
+  const void *gx = nullptr, *gb = nullptr;
+};
+
+namespace {
+
+int fail(ugmg_ctx *ctx, int code, const std::string &msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+#define HIPCHK(call)                                                                 \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      return fail(ctx, UGMG_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+int dalloc(ugmg_ctx *ctx, T **out, size_t n) {
+  *out = nullptr;
+  if (n == 0) n = 1;
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, n * sizeof(T));
+  if (e != hipSuccess) return fail(ctx, UGMG_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  ctx->allocs.push_back(p);
+  *out = (T *)p;
+  return 0;
+}
+
+template <typename T>
+int upload(ugmg_ctx *ctx, T **out, const T *src, size_t n) {
+  int rc = dalloc(ctx, out, n);
+  if (rc) return rc;
+  if (n) HIPCHK(hipMemcpy(*out, src, n * sizeof(T), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int upload_sell(ugmg_ctx *ctx, const HostSell &S, DevSellBuf &D) {
+  D.nslices = S.nslices;
+  D.nent = S.col.size();
+  int rc;
+  if ((rc = upload(ctx, &D.soff, S.soff.data(), S.soff.size()))) return rc;
+  if ((rc = upload(ctx, &D.rows, S.rows.data(), S.rows.size()))) return rc;
+  if ((rc = upload(ctx, &D.col, S.col.data(), S.col.size()))) return rc;
+  if ((rc = upload(ctx, &D.v64, S.val.data(), S.val.size()))) return rc;
+  std::vector<float> v32(S.val.begin(), S.val.end());
+  if ((rc = upload(ctx, &D.v32, v32.data(), v32.size()))) return rc;
+  return 0;
+}
+
+void free_device(ugmg_ctx *ctx) {
+  if (ctx->gexec) {
+    hipGraphExecDestroy(ctx->gexec);
+    ctx->gexec = nullptr;
+  }
+  ctx->gkind = -1;
+  for (void *p : ctx->allocs) hipFree(p);
+  ctx->allocs.clear();
+  ctx->dev.clear();
+  ctx->base_inv = nullptr;
+  ctx->part = nullptr;
+  ctx->red = nullptr;
+  ctx->prepared = false;
+  ctx->top = -1;
+}
+
+void drop_graph(ugmg_ctx *ctx) {
+  if (ctx->gexec) hipGraphExecDestroy(ctx->gexec);
+  ctx->gexec = nullptr;
+  ctx->gkind = -1;
+}
+
+// ------------------------------------------------------------ cycle drivers
+
+template <typename T>
+void smooth_step(ugmg_ctx *ctx, int l, T *t, T *b, T *c, int cmode, T *xout) {
+  DevLevel &d = ctx->dev[l];
+  hipStream_t st = ctx->stream;
+  const T damp = (T)ctx->p.damp[0];
+  if (ctx->p.smoother == UGMG_SMOOTH_JAC) {
+    launch_jac<T>(st, d.n, b, d.diag<T>(), d.vclass, damp, t);
+    const int mode = cmode < 0 ? 0 : (cmode == 0 ? 3 : 2);
+    launch_spmv<T, T>(st, d.A.d<T>(), 0, d.A.nslices, t, b, c, mode);
+  } else {
+    for (int k = 0; k < d.ncolors; ++k)
+      launch_gs_color<T, T>(st, d.L.d<T>(), d.color_slices[k],
+                            d.color_slices[k + 1] - d.color_slices[k], b, d.diag<T>(),
+                            d.vclass, t);
+    launch_gs_defect<T, T>(st, d.U.d<T>(), t, b, d.vclass, damp, cmode < 0 ? nullptr : c,
+                           cmode < 0 ? 1 : cmode, xout);
+  }
+}
+
+template <typename T>
+void base_solve(ugmg_ctx *ctx, T *c, T *b, int cmode) {
+  DevLevel &d = ctx->dev[ctx->bl];
+  BaseArgs ba;
+  ba.n = d.n;
+  ba.maxiter = ctx->p.base_maxiter;
+  ba.failed = ctx->base.failed ? 1 : 0;
+  ba.damp = ctx->p.base_damp[0];
+  ba.abslimit = ctx->p.base_abslimit[0];
+  ba.red = ctx->p.base_red[0];
+  ba.inv = ctx->base_inv;
+  launch_base_solve<T>(ctx->stream, ba, d.A.d64(), d.vclass, b, c, cmode);
+}
+
+// Lmgc on level l; cmode 0: c := correction, 1: c += correction
+template <typename T>
+void lmgc(ugmg_ctx *ctx, int l, T *c, T *b, int cmode) {
+  const ugmg_params &p = ctx->p;
+  hipStream_t st = ctx->stream;
+  if (l <= ctx->bl) {
+    base_solve<T>(ctx, c, b, cmode);
+    return;
+  }
+  DevLevel &d = ctx->dev[l];
+  DevLevel &dc = ctx->dev[l - 1];
+  T *t = d.tv<T>();
+  for (int i = 0; i < p.nu1; ++i) smooth_step<T>(ctx, l, t, b, c, i == 0 ? cmode : 1, nullptr);
+  if (p.nu1 == 0 && cmode == 0) launch_fill<T>(st, d.n, c, (T)0);
+  // RestrictDefect(level, b, b, Factor_One)
+  launch_restrict<T>(st, dc.n, d.pt_ptr, d.pt_idx, d.ptw<T>(), dc.skip, dc.vnclass, b, (T)1,
+                     dc.bv<T>());
+  if (p.gamma <= 0) launch_fill<T>(st, dc.n, dc.cv<T>(), (T)0);
+  for (int g = 0; g < p.gamma; ++g) lmgc<T>(ctx, l - 1, dc.cv<T>(), dc.bv<T>(), g == 0 ? 0 : 1);
+  // InterpolateCorrection(level, t, c, damp); dadd(c, t); dmatmul_minus(b, A, t)
+  launch_interp<T>(st, d.n, d.p_ptr, d.p_idx, d.pw<T>(), d.skip, dc.cv<T>(), (T)p.lmgc_damp[0],
+                   t, c);
+  launch_spmv<T, T>(st, d.A.d<T>(), 0, d.A.nslices, t, b, (T *)nullptr, 0);
+  for (int i = 0; i < p.nu2; ++i) smooth_step<T>(ctx, l, t, b, c, 1, nullptr);
+}
+
+int check_ready(ugmg_ctx *ctx, int level) {
+  if (!ctx) return UGMG_E_ARG;
+  if (!ctx->prepared) return fail(ctx, UGMG_E_STATE, "ugmg_preprocess has not been called");
+  if (level < ctx->bl || level > ctx->top)
+    return fail(ctx, UGMG_E_ARG, "level outside the preprocessed range");
+  return 0;
+}
+
+int surface_norm(ugmg_ctx *ctx, int level, const double *x, double *out) {
+  DevLevel &d = ctx->dev[level];
+  launch_dot<double>(ctx->stream, d.n, x, nullptr, d.vclass, ctx->part, ctx->red, 1);
+  HIPCHK(hipMemcpyAsync(ctx->red_host, ctx->red, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  *out = ctx->red_host[0];
+  return 0;
+}
+
+// one iteration of the linear solver, enqueued on the stream; leaves the
+// defect norm in ctx->red[0]
+void enqueue_iteration(ugmg_ctx *ctx, int level, double *x, double *b) {
+  DevLevel &d = ctx->dev[level];
+  if (ctx->p.precision == UGMG_MIXED) {
+    const int k = std::max(1, ctx->p.inner_cycles);
+    for (int j = 0; j < k; ++j) lmgc<float>(ctx, level, d.c32, d.b32, j == 0 ? 0 : 1);
+    int np = 0;
+    launch_ir_update(ctx->stream, d.A.d64(), d.c32, b, x, d.b32, d.vclass, ctx->part, &np);
+    launch_reduce_final(ctx->stream, np, ctx->part, ctx->red, 1);
+  } else {
+    lmgc<double>(ctx, level, x, b, 1);
+    launch_dot<double>(ctx->stream, d.n, b, nullptr, d.vclass, ctx->part, ctx->red, 1);
+  }
+}
+
+bool can_capture(ugmg_ctx *ctx) { return ctx->p.use_graph && ctx->stream != nullptr; }
+
+int run_iteration(ugmg_ctx *ctx, int level, double *x, double *b) {
+  if (!can_capture(ctx)) {
+    enqueue_iteration(ctx, level, x, b);
+    HIPCHK(hipGetLastError());
+  } else {
+    const int kind = ctx->p.precision;
+    if (!ctx->gexec || ctx->gkind != kind || ctx->glevel != level || ctx->gx != x ||
+        ctx->gb != b) {
+      drop_graph(ctx);
+      hipGraph_t g = nullptr;
+      HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      enqueue_iteration(ctx, level, x, b);
+      HIPCHK(hipStreamEndCapture(ctx->stream, &g));
+      hipError_t e = hipGraphInstantiate(&ctx->gexec, g, nullptr, nullptr, 0);
+      hipGraphDestroy(g);
+      if (e != hipSuccess) {
+        ctx->gexec = nullptr;
+        return fail(ctx, UGMG_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+      }
+      ctx->gkind = kind;
+      ctx->glevel = level;
+      ctx->gx = x;
+      ctx->gb = b;
+    }
+    HIPCHK(hipGraphLaunch(ctx->gexec, ctx->stream));
+  }
+  HIPCHK(hipMemcpyAsync(ctx->red_host, ctx->red, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+inline bool sc_cmp1(double x, double y) { return std::fabs(x) < std::fabs(y); }
+
+double spmv_bytes(const DevLevel &d, int sv, int sx) {
+  return (double)d.nnzb * (sv + 4) + 4.0 * (d.n + 1) + (double)d.n * (sx + 2.0 * sx);
+}
+
+}  // namespace
+
+// ================================================================== C ABI
+
+extern "C" {
+
+void ugmg_params_default(ugmg_params *p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->nu1 = p->nu2 = p->gamma = 1;
+  p->baselevel = 0;
+  p->smoother = UGMG_SMOOTH_JAC;
+  for (int i = 0; i < UGMG_MAXB; ++i) {
+    p->damp[i] = p->lmgc_damp[i] = p->base_damp[i] = 1.0;
+    p->base_abslimit[i] = 1e-10;
+    p->base_red[i] = 1e-12;
+  }
+  p->base_maxiter = 1;
+  p->ex_reorder = 1;
+  p->precision = UGMG_FP64;
+  p->inner_cycles = 1;
+  p->use_graph = 0;
+}
+
+int ugmg_create(int device, ugmg_ctx **out) {
+  if (!out) return UGMG_E_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return UGMG_E_NODEV;
+  if (device < 0 || device >= ndev) return UGMG_E_ARG;
+  ugmg_ctx *ctx = new (std::nothrow) ugmg_ctx;
+  if (!ctx) return UGMG_E_NOMEM;
+  ctx->device = device;
+  if (hipSetDevice(device) != hipSuccess ||
+      hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipHostMalloc((void **)&ctx->red_host, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+    delete ctx;
+    return UGMG_E_HIP;
+  }
+  ctx->own_stream = true;
+  ugmg_params_default(&ctx->p);
+  *out = ctx;
+  return UGMG_OK;
+}
+
+int ugmg_destroy(ugmg_ctx *ctx) {
+  if (!ctx) return UGMG_E_ARG;
+  hipSetDevice(ctx->device);
+  if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  free_device(ctx);
+  if (ctx->red_host) hipHostFree(ctx->red_host);
+  if (ctx->own_stream && ctx->stream) hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return UGMG_OK;
+}
+
+const char *ugmg_last_error(const ugmg_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int ugmg_set_stream(ugmg_ctx *ctx, void *stream) {
+  if (!ctx) return UGMG_E_ARG;
+  drop_graph(ctx);
+  if (ctx->own_stream && ctx->stream) {
+    hipStreamSynchronize(ctx->stream);
+    hipStreamDestroy(ctx->stream);
+    ctx->own_stream = false;
+    ctx->stream = nullptr;
+  }
+  if (stream) {
+    ctx->stream = (hipStream_t)stream;
+  } else {
+    HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    ctx->own_stream = true;
+  }
+  return UGMG_OK;
+}
+
+int ugmg_sync(ugmg_ctx *ctx) {
+  if (!ctx) return UGMG_E_ARG;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return UGMG_OK;
+}
+
+int ugmg_set_nlevels(ugmg_ctx *ctx, int nlevels, int b) {
+  if (!ctx || nlevels < 1 || nlevels > 32) return ctx ? fail(ctx, UGMG_E_ARG, "nlevels must be 1..32") : UGMG_E_ARG;
+  if (b != 1) return fail(ctx, UGMG_E_ARG, "only scalar (b=1) hierarchies are supported on the GPU");
+  free_device(ctx);
+  ctx->nlevels = nlevels;
+  ctx->b = b;
+  ctx->host.assign(nlevels, HostLevel());
+  ctx->level_set.assign(nlevels, 0);
+  ctx->transfer_set.assign(nlevels, 0);
+  return UGMG_OK;
+}
+
+int ugmg_set_level(ugmg_ctx *ctx, int level, int n, const int *row_ptr, const int *col_idx,
+                   const double *vals, const uint32_t *skip, const uint8_t *vclass,
+                   const uint8_t *vnclass) {
+  if (!ctx) return UGMG_E_ARG;
+  if (level < 0 || level >= ctx->nlevels) return fail(ctx, UGMG_E_ARG, "level out of range");
+  if (n < 0 || (n > 0 && (!row_ptr || !col_idx || !vals || !skip || !vclass || !vnclass)))
+    return fail(ctx, UGMG_E_ARG, "null array");
+  try {
+    HostLevel &L = ctx->host[level];
+    L.n = n;
+    L.rp.assign(row_ptr, row_ptr + n + 1);
+    if (L.rp[0] != 0) return fail(ctx, UGMG_E_ARG, "row_ptr[0] != 0");
+    for (int i = 0; i < n; ++i)
+      if (L.rp[i + 1] <= L.rp[i]) return fail(ctx, UGMG_E_ARG, "every row needs its diagonal entry");
+    L.nnzb = L.rp[n];
+    L.ci.assign(col_idx, col_idx + L.nnzb);
+    for (int i = 0; i < n; ++i) {
+      if (L.ci[L.rp[i]] != i) return fail(ctx, UGMG_E_ARG, "row does not start with its diagonal");
+      for (int m = L.rp[i]; m < L.rp[i + 1]; ++m)
+        if (L.ci[m] < 0 || L.ci[m] >= n) return fail(ctx, UGMG_E_ARG, "column index out of range");
+    }
+    L.a.assign(vals, vals + (size_t)L.nnzb);
+    L.skip.assign(skip, skip + n);
+    L.vc.assign(vclass, vclass + n);
+    L.vnc.assign(vnclass, vnclass + n);
+  } catch (const std::bad_alloc &) {
+    return fail(ctx, UGMG_E_NOMEM, "host allocation");
+  }
+  ctx->level_set[level] = 1;
+  free_device(ctx);
+  return UGMG_OK;
+}
+
+int ugmg_set_transfer(ugmg_ctx *ctx, int level, const int *p_ptr, const int *p_idx,
+                      const double *p_w) {
+  if (!ctx) return UGMG_E_ARG;
+  if (level < 1 || level >= ctx->nlevels) return fail(ctx, UGMG_E_ARG, "transfer level out of range");
+  if (!ctx->level_set[level] || !ctx->level_set[level - 1])
+    return fail(ctx, UGMG_E_STATE, "set both levels before their transfer");
+  HostLevel &L = ctx->host[level];
+  const int nc = ctx->host[level - 1].n;
+  if (!p_ptr || (L.n > 0 && (!p_idx || !p_w))) return fail(ctx, UGMG_E_ARG, "null array");
+  try {
+    L.pp.assign(p_ptr, p_ptr + L.n + 1);
+    if (L.pp[0] != 0) return fail(ctx, UGMG_E_ARG, "p_ptr[0] != 0");
+    for (int i = 0; i < L.n; ++i)
+      if (L.pp[i + 1] < L.pp[i]) return fail(ctx, UGMG_E_ARG, "p_ptr not monotone");
+    const int nnz = L.pp[L.n];
+    L.pi.assign(p_idx, p_idx + nnz);
+    for (int q = 0; q < nnz; ++q)
+      if (L.pi[q] < 0 || L.pi[q] >= nc) return fail(ctx, UGMG_E_ARG, "coarse index out of range");
+    L.pw.assign(p_w, p_w + nnz);
+  } catch (const std::bad_alloc &) {
+    return fail(ctx, UGMG_E_NOMEM, "host allocation");
+  }
+  L.has_p = true;
+  ctx->transfer_set[level] = 1;
+  free_device(ctx);
+  return UGMG_OK;
+}
+
+int ugmg_set_params(ugmg_ctx *ctx, const ugmg_params *p) {
+  if (!ctx || !p) return UGMG_E_ARG;
+  if (p->nu1 < 0 || p->nu2 < 0 || p->gamma < 0 || p->base_maxiter < 0 ||
+      (p->smoother != UGMG_SMOOTH_JAC && p->smoother != UGMG_SMOOTH_GS) ||
+      (p->precision != UGMG_FP64 && p->precision != UGMG_MIXED) || p->inner_cycles < 0)
+    return fail(ctx, UGMG_E_ARG, "invalid parameter");
+  const bool relayout = !ctx->params_set || p->smoother != ctx->p.smoother ||
+                        p->baselevel != ctx->p.baselevel || p->ex_reorder != ctx->p.ex_reorder;
+  ctx->p = *p;
+  ctx->params_set = true;
+  drop_graph(ctx);
+  if (relayout && ctx->prepared) free_device(ctx);
+  return UGMG_OK;
+}
+
+int ugmg_preprocess(ugmg_ctx *ctx, int level) {
+  if (!ctx) return UGMG_E_ARG;
+  if (ctx->nlevels <= 0) return fail(ctx, UGMG_E_STATE, "ugmg_set_nlevels first");
+  if (level < 0 || level >= ctx->nlevels) return fail(ctx, UGMG_E_ARG, "level out of range");
+  if (!ctx->params_set) return fail(ctx, UGMG_E_STATE, "ugmg_set_params first");
+  HIPCHK(hipSetDevice(ctx->device));
+  free_device(ctx);
+  const ugmg_params &p = ctx->p;
+  int bl = p.baselevel;
+  if (bl < 0) {  // LmgcInit: deepest level with at most -baselevel vectors
+    int i;
+    for (i = level; i > 0; --i)
+      if (ctx->level_set[i] && ctx->host[i].n <= -bl) break;
+    bl = i;
+  }
+  bl = std::min(bl, level);
+  for (int l = bl; l <= level; ++l) {
+    if (!ctx->level_set[l]) return fail(ctx, UGMG_E_STATE, "level " + std::to_string(l) + " not set");
+    if (l > bl && !ctx->transfer_set[l])
+      return fail(ctx, UGMG_E_STATE, "transfer of level " + std::to_string(l) + " not set");
+  }
+  if (ctx->host[bl].n > 2048)
+    return fail(ctx, UGMG_E_ARG, "base level has more than 2048 vectors; choose a coarser baselevel");
+  try {
+    ctx->dev.assign(level + 1, DevLevel());
+    ctx->bl = bl;
+    int rc;
+    int max_slices = 0;
+    for (int l = bl; l <= level; ++l) {
+      const HostLevel &H = ctx->host[l];
+      DevLevel &D = ctx->dev[l];
+      HostPrep hp;
+      const bool gs = p.smoother == UGMG_SMOOTH_GS && l > bl;
+      build_prep(H, l > bl ? &ctx->host[l - 1] : nullptr, gs, hp);
+      D.n = H.n;
+      D.nnzb = H.nnzb;
+      max_slices = std::max(max_slices, hp.A.nslices);
+      if ((rc = upload_sell(ctx, hp.A, D.A))) return rc;
+      if ((rc = upload(ctx, &D.diag64, hp.diag.data(), hp.diag.size()))) return rc;
+      std::vector<float> d32(hp.diag.begin(), hp.diag.end());
+      if ((rc = upload(ctx, &D.diag32, d32.data(), d32.size()))) return rc;
+      if ((rc = upload(ctx, &D.vclass, H.vc.data(), H.vc.size()))) return rc;
+      if ((rc = upload(ctx, &D.vnclass, H.vnc.data(), H.vnc.size()))) return rc;
+      if ((rc = upload(ctx, &D.skip, H.skip.data(), H.skip.size()))) return rc;
+      if (gs) {
+        D.ncolors = hp.ncolors;
+        D.color_slices = hp.color_slices;
+        D.color = hp.color;
+        if ((rc = upload_sell(ctx, hp.L, D.L))) return rc;
+        if ((rc = upload_sell(ctx, hp.U, D.U))) return rc;
+      }
+      if (l > bl) {
+        D.p_nnz = H.pp[H.n];
+        if ((rc = upload(ctx, &D.p_ptr, H.pp.data(), H.pp.size()))) return rc;
+        if ((rc = upload(ctx, &D.p_idx, H.pi.data(), H.pi.size()))) return rc;
+        if ((rc = upload(ctx, &D.p_w64, H.pw.data(), H.pw.size()))) return rc;
+        std::vector<float> w32(H.pw.begin(), H.pw.end());
+        if ((rc = upload(ctx, &D.p_w32, w32.data(), w32.size()))) return rc;
+        D.pt_nnz = (int)hp.pt_idx.size();
+        if ((rc = upload(ctx, &D.pt_ptr, hp.pt_ptr.data(), hp.pt_ptr.size()))) return rc;
+        if ((rc = upload(ctx, &D.pt_idx, hp.pt_idx.data(), hp.pt_idx.size()))) return rc;
+        if ((rc = upload(ctx, &D.pt_w64, hp.pt_w.data(), hp.pt_w.size()))) return rc;
+        std::vector<float> tw32(hp.pt_w.begin(), hp.pt_w.end());
+        if ((rc = upload(ctx, &D.pt_w32, tw32.data(), tw32.size()))) return rc;
+      }
+      if ((rc = dalloc(ctx, &D.c, D.n)) || (rc = dalloc(ctx, &D.b, D.n)) ||
+          (rc = dalloc(ctx, &D.t, D.n)) || (rc = dalloc(ctx, &D.c32, D.n)) ||
+          (rc = dalloc(ctx, &D.b32, D.n)) || (rc = dalloc(ctx, &D.t32, D.n)))
+        return rc;
+      HIPCHK(hipMemset(D.c, 0, sizeof(double) * std::max(D.n, 1)));
+      HIPCHK(hipMemset(D.c32, 0, sizeof(float) * std::max(D.n, 1)));
+    }
+    build_base(ctx->host[bl], p.ex_reorder, ctx->base);
+    if ((rc = upload(ctx, &ctx->base_inv, ctx->base.inv.data(), ctx->base.inv.size()))) return rc;
+    ctx->part_cap = std::max(1024, (max_slices + 3) / 4 + 1);
+    if ((rc = dalloc(ctx, &ctx->part, ctx->part_cap))) return rc;
+    if ((rc = dalloc(ctx, &ctx->red, 8))) return rc;
+  } catch (const std::bad_alloc &) {
+    free_device(ctx);
+    return fail(ctx, UGMG_E_NOMEM, "host allocation during preprocess");
+  }
+  ctx->top = level;
+  ctx->prepared = true;
+  return UGMG_OK;
+}
+
+int ugmg_postprocess(ugmg_ctx *ctx) {
+  if (!ctx) return UGMG_E_ARG;
+  if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  free_device(ctx);
+  return UGMG_OK;
+}
+
+int ugmg_level_size(const ugmg_ctx *ctx, int level, int *n, int *nnzb) {
+  if (!ctx || level < 0 || level >= ctx->nlevels || !ctx->level_set[level]) return UGMG_E_ARG;
+  if (n) *n = ctx->host[level].n;
+  if (nnzb) *nnzb = ctx->host[level].nnzb;
+  return UGMG_OK;
+}
+
+int ugmg_num_colors(const ugmg_ctx *ctx, int level) {
+  if (!ctx || !ctx->prepared || level < 0 || level > ctx->top) return UGMG_E_ARG;
+  return ctx->dev[level].ncolors;
+}
+
+int ugmg_get_colors(ugmg_ctx *ctx, int level, int *colors) {
+  int rc = check_ready(ctx, level);
+  if (rc) return rc;
+  const DevLevel &d = ctx->dev[level];
+  if (d.color.empty()) return fail(ctx, UGMG_E_STATE, "level has no colouring (not a GS level)");
+  std::copy(d.color.begin(), d.color.end(), colors);
+  return UGMG_OK;
+}
+
+int ugmg_get_pattern(ugmg_ctx *ctx, int level, int *row_ptr, int *col_idx, double *vals) {
+  int rc = check_ready(ctx, level);
+  if (rc) return rc;
+  const DevLevel &d = ctx->dev[level];
+  const int ns = d.A.nslices;
+  std::vector<int> soff(ns + 1), rows((size_t)ns * kSlice), col(d.A.nent);
+  std::vector<double> val(d.A.nent);
+  HIPCHK(hipMemcpy(soff.data(), d.A.soff, soff.size() * sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(rows.data(), d.A.rows, rows.size() * sizeof(int), hipMemcpyDeviceToHost));
+  if (d.A.nent) {
+    HIPCHK(hipMemcpy(col.data(), d.A.col, col.size() * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(val.data(), d.A.v64, val.size() * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  // the host copy's row lengths separate entries from slice padding
+  const HostLevel &H = ctx->host[level];
+  row_ptr[0] = 0;
+  for (int s = 0; s < ns; ++s)
+    for (int lane = 0; lane < kSlice; ++lane) {
+      const int r = rows[(size_t)s * kSlice + lane];
+      if (r < 0) continue;
+      const int len = H.rp[r + 1] - H.rp[r];
+      for (int k = 0; k < len; ++k) {
+        const size_t e = (size_t)soff[s] + (size_t)k * kSlice + lane;
+        col_idx[H.rp[r] + k] = col[e];
+        if (vals) vals[H.rp[r] + k] = val[e];
+      }
+      row_ptr[r + 1] = len;
+    }
+  for (int i = 0; i < d.n; ++i) row_ptr[i + 1] += row_ptr[i];
+  return UGMG_OK;
+}
+
+int ugmg_dmatmul_minus(ugmg_ctx *ctx, int level, double *x, const double *y) {
+  int rc = check_ready(ctx, level);
+  if (rc) return rc;
+  DevLevel &d = ctx->dev[level];
+  launch_spmv<double, double>(ctx->stream, d.A.d64(), 0, d.A.nslices, y, x, nullptr, 0);
+  HIPCHK(hipGetLastError());
+  return UGMG_OK;
+}
+
+int ugmg_dmatmul(ugmg_ctx *ctx, int level, double *x, const double *y) {
+  int rc = check_ready(ctx, level);
+  if (rc) return rc;
+  DevLevel &d = ctx->dev[level];
+  launch_spmv<double, double>(ctx->stream, d.A.d64(), 0, d.A.nslices, y, x, nullptr, 1);
+  HIPCHK(hipGetLastError());
+  return UGMG_OK;
+}
+
+int ugmg_smatmul_minus(ugmg_ctx *ctx, int level, float *x, const float *y) {
+  int rc = check_ready(ctx, level);
+  if (rc) return rc;
+  DevLevel &d = ctx->dev[level];
+  launch_spmv<float, float>(ctx->stream, d.A.d32(), 0, d.A.nslices, y, x, nullptr, 0);
+  HIPCHK(hipGetLastError());
+  return UGMG_OK;
+}
+
+int ugmg_dnrm2x(ugmg_ctx *ctx, int level, const double *x, double *out) {
+  int rc = check_ready(ctx, level);
+  if (rc) return rc;
+  if (!out) return fail(ctx, UGMG_E_ARG, "null output");
+  DevLevel &d = ctx->dev[level];
+  launch_dot<double>(ctx->stream, d.n, x, nullptr, nullptr, ctx->part, ctx->red, 1);
+  HIPCHK(hipMemcpyAsync(ctx->red_host, ctx->red, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  out[0] = ctx->red_host[0];
+  return UGMG_OK;
+}
+
+int ugmg_ddot(ugmg_ctx *ctx, int level, const double *x, const double *y, double *out) {
+  int rc = check_ready(ctx, level);
+  if (rc) return rc;
+  if (!out || !y) return fail(ctx, UGMG_E_ARG, "null argument");
+  DevLevel &d = ctx->dev[level];
+  launch_dot<double>(ctx->stream, d.n, x, y, nullptr, ctx->part, ctx->red, 0);
+  HIPCHK(hipMemcpyAsync(ctx->red_host, ctx->red, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  out[0] = ctx->red_host[0];
+  return UGMG_OK;
+}
+
+int ugmg_jacobi_step(ugmg_ctx *ctx, int level, double *x, double *b) {
+  int rc = check_ready(ctx, level);
+  if (rc) return rc;
+  DevLevel &d = ctx->dev[level];
+  launch_jac<double>(ctx->stream, d.n, b, d.diag64, d.vclass, ctx->p.damp[0], x);
+  launch_spmv<double, double>(ctx->stream, d.A.d64(), 0, d.A.nslices, x, b, nullptr, 0);
+  HIPCHK(hipGetLastError());
+  return UGMG_OK;
+}
+
+int ugmg_gs_step(ugmg_ctx *ctx, int level, double *x, double *b) {
+  int rc = check_ready(ctx, level);
+  if (rc) return rc;
+  DevLevel &d = ctx->dev[level];
+  if (d.ncolors == 0) return fail(ctx, UGMG_E_STATE, "level was not preprocessed for GS");
+  const int saved = ctx->p.smoother;
+  ctx->p.smoother = UGMG_SMOOTH_GS;
+  smooth_step<double>(ctx, level, d.t, b, nullptr, -1, x);
+  ctx->p.smoother = saved;
+  HIPCHK(hipGetLastError());
+  return UGMG_OK;
+}
+
+int ugmg_restrict(ugmg_ctx *ctx, int level, double *to_c, const double *from_f,
+                  const double *damp) {
+  int rc = check_ready(ctx, level);
+  if (rc) return rc;
+  if (level <= ctx->bl) return fail(ctx, UGMG_E_ARG, "restrict needs a level above the base level");
+  DevLevel &d = ctx->dev[level], &dc = ctx->dev[level - 1];
+  launch_restrict<double>(ctx->stream, dc.n, d.pt_ptr, d.pt_idx, d.pt_w64, dc.skip, dc.vnclass,
+                          from_f, damp ? damp[0] : 1.0, to_c);
+  HIPCHK(hipGetLastError());
+  return UGMG_OK;
+}
+
+int ugmg_interpolate(ugmg_ctx *ctx, int level, double *to_f, const double *from_c,
+                     const double *damp) {
+  int rc = check_ready(ctx, level);
+  if (rc) return rc;
+  if (level <= ctx->bl) return fail(ctx, UGMG_E_ARG, "interpolate needs a level above the base level");
+  DevLevel &d = ctx->dev[level];
+  launch_interp<double>(ctx->stream, d.n, d.p_ptr, d.p_idx, d.p_w64, d.skip, from_c,
+                        damp ? damp[0] : 1.0, to_f, nullptr);
+  HIPCHK(hipGetLastError());
+  return UGMG_OK;
+}
+
+int ugmg_base_solve(ugmg_ctx *ctx, double *c, double *b) {
+  int rc = check_ready(ctx, ctx ? ctx->bl : 0);
+  if (rc) return rc;
+  if (ctx->base.failed) return fail(ctx, UGMG_E_SINGULAR, "zero pivot in the band LU of the base level");
+  base_solve<double>(ctx, c, b, 0);
+  HIPCHK(hipGetLastError());
+  return UGMG_OK;
+}
+
+int ugmg_iter(ugmg_ctx *ctx, int level, double *c, double *b) {
+  int rc = check_ready(ctx, level);
+  if (rc) return rc;
+  if (ctx->base.failed) return fail(ctx, UGMG_E_SINGULAR, "zero pivot in the band LU of the base level");
+  DevLevel &d = ctx->dev[level];
+  if (ctx->p.precision == UGMG_MIXED) {
+    launch_convert<float, double>(ctx->stream, d.n, b, d.b32);
+    const int k = std::max(1, ctx->p.inner_cycles);
+    for (int j = 0; j < k; ++j) lmgc<float>(ctx, level, d.c32, d.b32, j == 0 ? 0 : 1);
+    launch_convert<double, float>(ctx->stream, d.n, d.c32, c);
+    launch_spmv<double, double>(ctx->stream, d.A.d64(), 0, d.A.nslices, c, b, nullptr, 0);
+  } else {
+    lmgc<double>(ctx, level, c, b, 0);
+  }
+  HIPCHK(hipGetLastError());
+  return UGMG_OK;
+}
+
+int ugmg_iter_host(ugmg_ctx *ctx, int level, double *c, double *b) {
+  int rc = check_ready(ctx, level);
+  if (rc) return rc;
+  const size_t bytes = sizeof(double) * ctx->dev[level].n;
+  double *dc = nullptr, *db = nullptr;
+  HIPCHK(hipMalloc(&dc, bytes));
+  if (hipMalloc(&db, bytes) != hipSuccess) {
+    hipFree(dc);
+    return fail(ctx, UGMG_E_NOMEM, "hipMalloc");
+  }
+  rc = UGMG_OK;
+  if (hipMemcpy(db, b, bytes, hipMemcpyHostToDevice) != hipSuccess) rc = fail(ctx, UGMG_E_HIP, "copy in");
+  if (!rc) rc = ugmg_iter(ctx, level, dc, db);
+  if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail(ctx, UGMG_E_HIP, "sync");
+  if (!rc && (hipMemcpy(c, dc, bytes, hipMemcpyDeviceToHost) != hipSuccess ||
+              hipMemcpy(b, db, bytes, hipMemcpyDeviceToHost) != hipSuccess))
+    rc = fail(ctx, UGMG_E_HIP, "copy out");
+  hipFree(dc);
+  hipFree(db);
+  return rc;
+}
+
+int ugmg_solve(ugmg_ctx *ctx, int level, double *x, double *b, const double *abslimit,
+               const double *reduction, int maxiter, int do_defect, ugmg_lresult *res,
+               ugmg_defect_cb cb, void *user) {
+  int rc = check_ready(ctx, level);
+  if (rc) return rc;
+  if (!x || !b || !abslimit || !reduction || !res || maxiter < 0)
+    return fail(ctx, UGMG_E_ARG, "null argument");
+  std::memset(res, 0, sizeof(*res));
+  if (ctx->base.failed) {
+    res->error_code = UGMG_E_SINGULAR;
+    return fail(ctx, UGMG_E_SINGULAR, "zero pivot in the band LU of the base level");
+  }
+  DevLevel &d = ctx->dev[level];
+  if (do_defect)  // LinearDefect: b -= A x
+    launch_spmv<double, double>(ctx->stream, d.A.d64(), 0, d.A.nslices, x, b, nullptr, 0);
+  double first = 0.0;
+  if ((rc = surface_norm(ctx, level, b, &first))) return rc;
+  res->first_defect[0] = res->last_defect[0] = first;
+  double target = first * reduction[0];  // sc_mul_check
+  if (target == 0.0) target = reduction[0];
+  if (cb) cb(user, 0, 1, &first);
+  int conv = sc_cmp1(first, abslimit[0]);
+  if (ctx->p.precision == UGMG_MIXED) launch_convert<float, double>(ctx->stream, d.n, b, d.b32);
+  int i;
+  for (i = 0; i < maxiter; ++i) {
+    if (conv) break;
+    if ((rc = run_iteration(ctx, level, x, b))) {
+      res->error_code = rc;
+      return rc;
+    }
+    const double last = ctx->red_host[0];
+    res->last_defect[0] = last;
+    if (cb) cb(user, i + 1, 1, &last);
+    if (sc_cmp1(last, abslimit[0]) || sc_cmp1(last, target)) {
+      conv = 1;
+      res->number_of_linear_iterations = i + 1;
+      break;
+    }
+  }
+  if (!conv) res->number_of_linear_iterations = i;
+  res->converged = conv;
+  return UGMG_OK;
+}
+
+int ugmg_solve_host(ugmg_ctx *ctx, int level, double *x, double *b, const double *abslimit,
+                    const double *reduction, int maxiter, int do_defect, ugmg_lresult *res,
+                    ugmg_defect_cb cb, void *user) {
+  int rc = check_ready(ctx, level);
+  if (rc) return rc;
+  const size_t bytes = sizeof(double) * ctx->dev[level].n;
+  double *dx = nullptr, *db = nullptr;
+  HIPCHK(hipMalloc(&dx, bytes));
+  if (hipMalloc(&db, bytes) != hipSuccess) {
+    hipFree(dx);
+    return fail(ctx, UGMG_E_NOMEM, "hipMalloc");
+  }
+  rc = UGMG_OK;
+  if (hipMemcpy(dx, x, bytes, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(db, b, bytes, hipMemcpyHostToDevice) != hipSuccess)
+    rc = fail(ctx, UGMG_E_HIP, "copy in");
+  if (!rc) rc = ugmg_solve(ctx, level, dx, db, abslimit, reduction, maxiter, do_defect, res, cb, user);
+  if (!rc && (hipMemcpy(x, dx, bytes, hipMemcpyDeviceToHost) != hipSuccess ||
+              hipMemcpy(b, db, bytes, hipMemcpyDeviceToHost) != hipSuccess))
+    rc = fail(ctx, UGMG_E_HIP, "copy out");
+  drop_graph(ctx);
+  hipFree(dx);
+  hipFree(db);
+  return rc;
+}
+
+int ugmg_time_kernel(ugmg_ctx *ctx, int level, int kind, int reps, double *ms, double *bytes) {
+  int rc = check_ready(ctx, level);
+  if (rc) return rc;
+  if (reps <= 0 || !ms || !bytes) return fail(ctx, UGMG_E_ARG, "bad argument");
+  DevLevel &d = ctx->dev[level];
+  if ((kind == 4 || kind == 5) && level <= ctx->bl) return fail(ctx, UGMG_E_ARG, "transfer needs level > base");
+  if ((kind == 3 || kind == 7) && d.ncolors == 0) return fail(ctx, UGMG_E_STATE, "not a GS level");
+  hipStream_t st = ctx->stream;
+  // deterministic nonzero vectors
+  launch_fill<double>(st, d.n, d.t, 1.0);
+  launch_fill<double>(st, d.n, d.b, 0.5);
+  launch_fill<float>(st, d.n, d.t32, 1.0f);
+  launch_fill<float>(st, d.n, d.b32, 0.5f);
+  const double n = d.n, nnz = d.nnzb;
+  double by = 0.0;
+  auto once = [&]() {
+    switch (kind) {
+      case 0: launch_spmv<double, double>(st, d.A.d64(), 0, d.A.nslices, d.t, d.b, nullptr, 0); break;
+      case 1: launch_spmv<float, float>(st, d.A.d32(), 0, d.A.nslices, d.t32, d.b32, nullptr, 0); break;
+      case 2: smooth_step<double>(ctx, level, d.t, d.b, d.c, 1, nullptr); break;
+      case 3: {
+        const int s = ctx->p.smoother;
+        ctx->p.smoother = UGMG_SMOOTH_GS;
+        smooth_step<double>(ctx, level, d.t, d.b, d.c, 1, nullptr);
+        ctx->p.smoother = s;
+        break;
+      }
+      case 4: {
+        DevLevel &dc = ctx->dev[level - 1];
+        launch_restrict<double>(st, dc.n, d.pt_ptr, d.pt_idx, d.pt_w64, dc.skip, dc.vnclass, d.b, 1.0, dc.b);
+        break;
+      }
+      case 5: {
+        DevLevel &dc = ctx->dev[level - 1];
+        launch_interp<double>(st, d.n, d.p_ptr, d.p_idx, d.p_w64, d.skip, dc.c, 1.0, d.t, d.c);
+        break;
+      }
+      case 6:
+        if (ctx->p.precision == UGMG_MIXED) lmgc<float>(ctx, level, d.c32, d.b32, 0);
+        else lmgc<double>(ctx, level, d.c, d.b, 0);
+        break;
+      case 7: {
+        const int s = ctx->p.smoother;
+        ctx->p.smoother = UGMG_SMOOTH_GS;
+        smooth_step<float>(ctx, level, d.t32, d.b32, d.c32, 1, nullptr);
+        ctx->p.smoother = s;
+        break;
+      }
+      default: break;
+    }
+  };
+  switch (kind) {
+    case 0: by = spmv_bytes(d, 8, 8); break;
+    case 1: by = spmv_bytes(d, 4, 4); break;
+    case 2: by = spmv_bytes(d, 8, 8) + n * (8 + 8 + 16); break;
+    // colour sweep (lower entries, b, diag, v) + defect (upper entries, b rmw, v, c rmw)
+    case 3: by = (nnz - n) * 12.0 + 8.0 * n * (1 + 1 + 1 + 2 + 1 + 2) + 8.0 * n; break;
+    case 7: by = (nnz - n) * 8.0 + 4.0 * n * (1 + 1 + 1 + 2 + 1 + 2) + 8.0 * n; break;
+    case 4: {
+      DevLevel &dc = ctx->dev[level - 1];
+      by = d.pt_nnz * 12.0 + 4.0 * (dc.n + 1) + 8.0 * d.n + 8.0 * dc.n;
+      break;
+    }
+    case 5: {
+      DevLevel &dc = ctx->dev[level - 1];
+      by = d.p_nnz * 12.0 + 4.0 * (d.n + 1) + 8.0 * dc.n + 8.0 * d.n + 16.0 * d.n;
+      break;
+    }
+    default: by = 0.0;
+  }
+  if (kind < 0 || kind > 7) return fail(ctx, UGMG_E_ARG, "unknown kernel kind");
+  once();  // warm
+  HIPCHK(hipStreamSynchronize(st));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, st));
+  for (int r = 0; r < reps; ++r) once();
+  HIPCHK(hipEventRecord(e1, st));
+  HIPCHK(hipEventSynchronize(e1));
+  float msf = 0.0f;
+  HIPCHK(hipEventElapsedTime(&msf, e0, e1));
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  *ms = (double)msf / reps;
+  *bytes = by;
+  return UGMG_OK;
+}
+
+}  // extern "C"

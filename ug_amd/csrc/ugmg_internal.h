@@ -1,0 +1,71 @@
+// Internal data structures of libugmg (host setup + device state).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "ugmg_common.h"
+
+namespace ugmg {
+
+// Host copy of one level as it crosses the C ABI (ConvertMatrix layout).
+struct HostLevel {
+  int n = 0, nnzb = 0;
+  std::vector<int> rp, ci;
+  std::vector<double> a;
+  std::vector<uint32_t> skip;
+  std::vector<uint8_t> vc, vnc;
+  bool has_p = false;
+  std::vector<int> pp, pi;
+  std::vector<double> pw;
+};
+
+// SELL-64 layout of a set of rows, built on the host.
+//   slice s covers slots [64 s, 64 s + 64); slot -> row via rows[] (-1 pad)
+//   entry (s, k, lane) at soff[s] + 64 k + lane, k < width(s)
+struct HostSell {
+  int nrows = 0;         // rows covered (slots that are not padding)
+  int nslices = 0;
+  std::vector<int> soff;  // nslices + 1, entry offsets (multiples of 64)
+  std::vector<int> rows;  // nslices * 64
+  std::vector<int> col;
+  std::vector<double> val;
+  bool identity_rows = false;
+};
+
+// Host-side products of LmgcPreProcess for one level.
+struct HostPrep {
+  HostSell A;                // all rows, list order
+  std::vector<double> diag;  // diagonal entry of every row
+  // multicolour Gauss-Seidel
+  int ncolors = 0;
+  std::vector<int> color;          // per vector
+  std::vector<int> color_slices;   // ncolors + 1, slice ranges in L
+  HostSell L;                      // rows grouped by colour, lower-colour entries
+  HostSell U;                      // same rows, higher-colour entries (all for inactive rows)
+  // restriction as a gather: coarse rows of P^T from fine rows with VCLASS >= 2
+  std::vector<int> pt_ptr, pt_idx;
+  std::vector<double> pt_w;
+};
+
+// Base level: band LU as in EXPreProcess, then the explicit inverse.
+struct HostBase {
+  int n = 0;
+  int bw = 0;
+  bool failed = false;
+  std::vector<int> order;      // BFS list order
+  std::vector<double> inv;     // n x n row-major, natural vector order
+};
+
+// filters for build_sell
+enum class Part { All, Lower, Upper };
+
+void build_sell(const HostLevel &L, const std::vector<int> *rows, Part part,
+                const std::vector<int> *color, HostSell &out);
+int greedy_color(const HostLevel &L, std::vector<int> &color);
+void build_prep(const HostLevel &L, const HostLevel *coarse, bool want_gs,
+                HostPrep &out);
+void build_base(const HostLevel &L, int reorder, HostBase &out);
+
+}  // namespace ugmg

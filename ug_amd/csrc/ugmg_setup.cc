@@ -1,0 +1,247 @@
+// Host half of LmgcPreProcess: device layouts, colouring, coarse LU.
+//
+//  * SELL-64 row slabs of the ConvertMatrix arrays (np/udm/disctools.cc:
+//    2061-2128); within a row the entries keep their list order (diagonal
+//    first), so every row sum runs in UG's order (matloop.ct:75-99).
+//  * greedy first-fit colouring in vector-list order over the symmetrised
+//    matrix pattern; colour-ordered forward substitution is l_lgs
+//    (np/algebra/ugiter.cc:412-615) with VINDEX replaced by (colour, list
+//    position).
+//  * P^T rows for the gather form of StandardRestrictNodeVector
+//    (np/algebra/transgrid.cc:117-215): only fine vectors with VCLASS >= 2
+//    contribute, entries ordered by fine list position (UG's scatter order).
+//  * EXPreProcess (np/procs/iter.cc:8685-8970): breadth-first band reorder,
+//    band copy (EX_MAT, gm/evm.h:392), LU without pivoting
+//    (gm/evm.cc:1400-1418); the factors are then applied to unit vectors to
+//    form the dense inverse that the GPU multiplies with.
+#include "ugmg_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace ugmg {
+
+void build_sell(const HostLevel &L, const std::vector<int> *rows, Part part,
+                const std::vector<int> *color, HostSell &out) {
+  const int nr = rows ? (int)rows->size() : L.n;
+  out.nrows = nr;
+  out.identity_rows = rows == nullptr;
+  out.nslices = (nr + kSlice - 1) / kSlice;
+  out.soff.assign(out.nslices + 1, 0);
+  out.rows.assign((size_t)out.nslices * kSlice, -1);
+  auto keep = [&](int r, int m) {
+    if (part == Part::All) return true;
+    int w = L.ci[m];
+    if (w == r) return false;
+    int cr = (*color)[r], cw = (*color)[w];
+    if (part == Part::Lower) return cw < cr && L.vc[w] >= 3;
+    // Upper: higher colours for active rows, every off-diagonal entry for
+    // rows that the smoother leaves at zero (VCLASS < 3)
+    if (L.vc[r] < 3) return true;
+    return cw > cr;
+  };
+  std::vector<int> width(out.nslices, 0);
+  for (int s = 0; s < out.nslices; ++s) {
+    int wmax = 0;
+    for (int lane = 0; lane < kSlice; ++lane) {
+      int slot = s * kSlice + lane;
+      if (slot >= nr) break;
+      int r = rows ? (*rows)[slot] : slot;
+      out.rows[slot] = r;
+      if (r < 0) continue;  // padding slot
+      int cnt = 0;
+      for (int m = L.rp[r]; m < L.rp[r + 1]; ++m) cnt += keep(r, m);
+      wmax = std::max(wmax, cnt);
+    }
+    width[s] = wmax;
+    out.soff[s + 1] = out.soff[s] + wmax * kSlice;
+  }
+  const size_t nent = (size_t)out.soff[out.nslices];
+  out.col.assign(nent, 0);
+  out.val.assign(nent, 0.0);
+  for (int s = 0; s < out.nslices; ++s) {
+    for (int lane = 0; lane < kSlice; ++lane) {
+      int slot = s * kSlice + lane;
+      int r = slot < nr ? out.rows[slot] : -1;
+      int k = 0;
+      if (r >= 0) {
+        for (int m = L.rp[r]; m < L.rp[r + 1]; ++m) {
+          if (!keep(r, m)) continue;
+          size_t e = (size_t)out.soff[s] + (size_t)k * kSlice + lane;
+          out.col[e] = L.ci[m];
+          out.val[e] = L.a[m];
+          ++k;
+        }
+      }
+      // padding: value 0 against a valid column
+      int padcol = r >= 0 ? r : 0;
+      for (; k < width[s]; ++k) {
+        size_t e = (size_t)out.soff[s] + (size_t)k * kSlice + lane;
+        out.col[e] = padcol;
+        out.val[e] = 0.0;
+      }
+    }
+  }
+}
+
+int greedy_color(const HostLevel &L, std::vector<int> &color) {
+  const int n = L.n;
+  // symmetrised adjacency
+  std::vector<int> deg(n + 1, 0);
+  for (int i = 0; i < n; ++i)
+    for (int m = L.rp[i]; m < L.rp[i + 1]; ++m) {
+      int w = L.ci[m];
+      if (w == i) continue;
+      ++deg[i + 1];
+      ++deg[w + 1];
+    }
+  for (int i = 0; i < n; ++i) deg[i + 1] += deg[i];
+  std::vector<int> adj(deg[n]), fill(deg.begin(), deg.end() - 1);
+  for (int i = 0; i < n; ++i)
+    for (int m = L.rp[i]; m < L.rp[i + 1]; ++m) {
+      int w = L.ci[m];
+      if (w == i) continue;
+      adj[fill[i]++] = w;
+      adj[fill[w]++] = i;
+    }
+  color.assign(n, -1);
+  int ncol = 0;
+  std::vector<int> seen(64, -1);
+  for (int i = 0; i < n; ++i) {
+    for (int q = deg[i]; q < deg[i + 1]; ++q) {
+      int c = color[adj[q]];
+      if (c >= 0) {
+        if (c >= (int)seen.size()) seen.resize(c + 1, -1);
+        seen[c] = i;
+      }
+    }
+    int c = 0;
+    while (c < (int)seen.size() && seen[c] == i) ++c;
+    color[i] = c;
+    ncol = std::max(ncol, c + 1);
+  }
+  return ncol;
+}
+
+void build_prep(const HostLevel &L, const HostLevel *coarse, bool want_gs,
+                HostPrep &out) {
+  build_sell(L, nullptr, Part::All, nullptr, out.A);
+  out.diag.resize(L.n);
+  for (int i = 0; i < L.n; ++i) out.diag[i] = L.a[L.rp[i]];
+  if (want_gs) {
+    out.ncolors = greedy_color(L, out.color);
+    // rows grouped by colour, list order inside a colour; each colour padded
+    // to whole slices so one launch per colour covers whole wavefronts
+    std::vector<int> rows;
+    out.color_slices.assign(out.ncolors + 1, 0);
+    std::vector<std::vector<int>> by(out.ncolors);
+    for (int i = 0; i < L.n; ++i) by[out.color[i]].push_back(i);
+    for (int c = 0; c < out.ncolors; ++c) {
+      for (int r : by[c]) rows.push_back(r);
+      while (rows.size() % kSlice) rows.push_back(-1);
+      out.color_slices[c + 1] = (int)(rows.size() / kSlice);
+    }
+    build_sell(L, &rows, Part::Lower, &out.color, out.L);
+    build_sell(L, &rows, Part::Upper, &out.color, out.U);
+  }
+  if (coarse && L.has_p) {
+    const int nc = coarse->n;
+    out.pt_ptr.assign(nc + 1, 0);
+    for (int v = 0; v < L.n; ++v) {
+      if (L.vc[v] < 2) continue;
+      for (int q = L.pp[v]; q < L.pp[v + 1]; ++q) ++out.pt_ptr[L.pi[q] + 1];
+    }
+    for (int i = 0; i < nc; ++i) out.pt_ptr[i + 1] += out.pt_ptr[i];
+    out.pt_idx.resize(out.pt_ptr[nc]);
+    out.pt_w.resize(out.pt_ptr[nc]);
+    std::vector<int> fill(out.pt_ptr.begin(), out.pt_ptr.end() - 1);
+    for (int v = 0; v < L.n; ++v) {
+      if (L.vc[v] < 2) continue;
+      for (int q = L.pp[v]; q < L.pp[v + 1]; ++q) {
+        int c = L.pi[q];
+        out.pt_idx[fill[c]] = v;
+        out.pt_w[fill[c]] = L.pw[q];
+        ++fill[c];
+      }
+    }
+  }
+}
+
+#define EXM(m, bw, i, j) ((m)[2 * (size_t)(bw) * (i) + (j)])
+
+void build_base(const HostLevel &L, int reorder, HostBase &out) {
+  const int n = L.n;
+  out.n = n;
+  out.order.resize(n);
+  for (int i = 0; i < n; ++i) out.order[i] = i;
+  if (reorder && n > 0) {
+    std::vector<int> fifo(n);
+    std::vector<char> used(n, 0);
+    int head = 0, tail = 0, v = 0;
+    fifo[tail++] = 0;
+    used[0] = 1;
+    while (head < tail) {
+      v = fifo[head++];
+      for (int m = L.rp[v] + 1; m < L.rp[v + 1]; ++m) {
+        int w = L.ci[m];
+        if (!used[w]) { fifo[tail++] = w; used[w] = 1; }
+      }
+    }
+    head = tail = 0;
+    fifo[tail++] = v;
+    used[v] = 0;
+    int i = 0;
+    while (head < tail) {
+      int u = fifo[head++];
+      out.order[i++] = u;
+      for (int m = L.rp[u] + 1; m < L.rp[u + 1]; ++m) {
+        int w = L.ci[m];
+        if (used[w]) { fifo[tail++] = w; used[w] = 0; }
+      }
+    }
+    if (i != n) {  // disconnected pattern: keep the list order
+      for (int k = 0; k < n; ++k) out.order[k] = k;
+    }
+  }
+  std::vector<int> index(n);
+  for (int k = 0; k < n; ++k) index[out.order[k]] = k;
+  int bw = 0;
+  for (int v = 0; v < n; ++v)
+    for (int m = L.rp[v] + 1; m < L.rp[v + 1]; ++m)
+      bw = std::max(bw, std::abs(index[v] - index[L.ci[m]]));
+  out.bw = bw;
+  std::vector<double> band((size_t)n * (2 * (size_t)bw + 1) + 1, 0.0);
+  for (int v = 0; v < n; ++v)
+    for (int m = L.rp[v]; m < L.rp[v + 1]; ++m)
+      EXM(band.data(), bw, index[v], index[L.ci[m]]) = L.a[m];
+  out.failed = false;
+  for (int i = 0; i < n - 1; ++i) {
+    double d = EXM(band.data(), bw, i, i);
+    if (d == 0.0) { out.failed = true; break; }
+    int jmax = std::min(i + bw, n - 1);
+    for (int j = i + 1; j <= jmax; ++j) {
+      double f = EXM(band.data(), bw, j, i) / d;
+      EXM(band.data(), bw, j, i) = f;
+      for (int k = i + 1; k <= jmax; ++k)
+        EXM(band.data(), bw, j, k) -= f * EXM(band.data(), bw, i, k);
+    }
+  }
+  out.inv.assign((size_t)n * n, 0.0);
+  if (out.failed) return;
+  std::vector<double> vec(n);
+  for (int j = 0; j < n; ++j) {
+    std::fill(vec.begin(), vec.end(), 0.0);
+    vec[index[j]] = 1.0;
+    for (int i = 1; i < n; ++i)
+      for (int k = std::max(i - bw, 0); k < i; ++k) vec[i] -= EXM(band.data(), bw, i, k) * vec[k];
+    for (int i = n - 1; i >= 0; --i) {
+      int kmax = std::min(i + bw, n - 1);
+      for (int k = i + 1; k <= kmax; ++k) vec[i] -= EXM(band.data(), bw, i, k) * vec[k];
+      vec[i] /= EXM(band.data(), bw, i, i);
+    }
+    for (int k = 0; k < n; ++k) out.inv[(size_t)out.order[k] * n + j] = vec[k];
+  }
+}
+
+}  // namespace ugmg

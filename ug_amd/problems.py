@@ -1,0 +1,170 @@
+"""Synthetic UG-layout multigrid problems (ctypes view of libugmg_problems).
+
+The C++ generator (``ug_amd/csrc/problems.cc``) stands in for the UG
+application that assembles one matrix per level; this module exposes its
+arrays as numpy arrays (copies, so they outlive the C object).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libugmg_problems.so")
+
+KAPPA_ONE = 0
+KAPPA_JUMP = 1
+
+_lib = None
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        lib = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        ip, dp = C.POINTER(C.c_int), C.POINTER(C.c_double)
+        lib.ugmg_problem_q1.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, C.POINTER(P)]
+        lib.ugmg_problem_q1.restype = C.c_int
+        lib.ugmg_problem_free.argtypes = [P]
+        lib.ugmg_problem_nlevels.argtypes = [P]
+        lib.ugmg_problem_block.argtypes = [P]
+        lib.ugmg_problem_level.argtypes = [P, C.c_int, ip, ip, C.POINTER(ip), C.POINTER(ip), C.POINTER(dp),
+                                           C.POINTER(C.POINTER(C.c_uint32)), C.POINTER(C.POINTER(C.c_uint8)),
+                                           C.POINTER(C.POINTER(C.c_uint8))]
+        lib.ugmg_problem_transfer.argtypes = [P, C.c_int, ip, C.POINTER(ip), C.POINTER(ip), C.POINTER(dp)]
+        lib.ugmg_problem_rhs.argtypes = [P, C.POINTER(dp)]
+        lib.ugmg_problem_grid.argtypes = [P, C.c_int, ip, ip]
+        lib.ugmg_splitmix_uniform.argtypes = [C.c_uint64, C.c_int64, C.c_double, C.c_double, dp]
+        lib.ugmg_splitmix_uniform.restype = None
+        _lib = lib
+    return _lib
+
+
+def _arr(ptr, n, dtype):
+    if n == 0:
+        return np.zeros(0, dtype=dtype)
+    return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
+
+
+@dataclass
+class Level:
+    n: int
+    b: int
+    row_ptr: np.ndarray
+    col_idx: np.ndarray
+    vals: np.ndarray
+    skip: np.ndarray
+    vclass: np.ndarray
+    vnclass: np.ndarray
+    nodes_per_dir: int = 0
+    p_ptr: np.ndarray | None = None     # interpolation from level-1
+    p_idx: np.ndarray | None = None
+    p_w: np.ndarray | None = None
+
+    @property
+    def nnzb(self) -> int:
+        return int(self.row_ptr[-1])
+
+    def dense(self) -> np.ndarray:
+        """Dense matrix (n*b x n*b); small levels only."""
+        b = self.b
+        A = np.zeros((self.n * b, self.n * b))
+        blocks = self.vals.reshape(-1, b, b)
+        for v in range(self.n):
+            for m in range(self.row_ptr[v], self.row_ptr[v + 1]):
+                w = self.col_idx[m]
+                A[v * b:(v + 1) * b, w * b:(w + 1) * b] += blocks[m]
+        return A
+
+    def dense_p(self, n_coarse: int) -> np.ndarray:
+        P = np.zeros((self.n, n_coarse))
+        for v in range(self.n):
+            for q in range(self.p_ptr[v], self.p_ptr[v + 1]):
+                P[v, self.p_idx[q]] += self.p_w[q]
+        return P
+
+
+@dataclass
+class Problem:
+    dim: int
+    levels: list = field(default_factory=list)
+    rhs: np.ndarray | None = None
+    name: str = ""
+
+    @property
+    def top(self) -> int:
+        return len(self.levels) - 1
+
+    @property
+    def b(self) -> int:
+        return self.levels[0].b
+
+
+def q1(dim: int, coarse_cells: int, nref: int, kappa: int = KAPPA_ONE, seed: int = 0,
+       name: str = "") -> Problem:
+    """Uniformly refined Q1 problem (see include/ugmg_problems.h)."""
+    lib = _load()
+    h = C.c_void_p()
+    rc = lib.ugmg_problem_q1(dim, coarse_cells, nref, kappa, C.c_uint64(seed), C.byref(h))
+    if rc != 0:
+        raise ValueError(f"ugmg_problem_q1 failed ({rc})")
+    try:
+        nl = lib.ugmg_problem_nlevels(h)
+        b = lib.ugmg_problem_block(h)
+        levels = []
+        for l in range(nl):
+            n, nnz = C.c_int(), C.c_int()
+            rp, ci = C.POINTER(C.c_int)(), C.POINTER(C.c_int)()
+            v = C.POINTER(C.c_double)()
+            sk = C.POINTER(C.c_uint32)()
+            vc, vnc = C.POINTER(C.c_uint8)(), C.POINTER(C.c_uint8)()
+            lib.ugmg_problem_level(h, l, C.byref(n), C.byref(nnz), C.byref(rp), C.byref(ci), C.byref(v),
+                                   C.byref(sk), C.byref(vc), C.byref(vnc))
+            d, N = C.c_int(), C.c_int()
+            lib.ugmg_problem_grid(h, l, C.byref(d), C.byref(N))
+            lv = Level(n=n.value, b=b,
+                       row_ptr=_arr(rp, n.value + 1, np.int32), col_idx=_arr(ci, nnz.value, np.int32),
+                       vals=_arr(v, nnz.value * b * b, np.float64), skip=_arr(sk, n.value, np.uint32),
+                       vclass=_arr(vc, n.value, np.uint8), vnclass=_arr(vnc, n.value, np.uint8),
+                       nodes_per_dir=N.value)
+            if l >= 1:
+                pn = C.c_int()
+                pp, pi = C.POINTER(C.c_int)(), C.POINTER(C.c_int)()
+                pw = C.POINTER(C.c_double)()
+                lib.ugmg_problem_transfer(h, l, C.byref(pn), C.byref(pp), C.byref(pi), C.byref(pw))
+                lv.p_ptr = _arr(pp, n.value + 1, np.int32)
+                lv.p_idx = _arr(pi, pn.value, np.int32)
+                lv.p_w = _arr(pw, pn.value, np.float64)
+            levels.append(lv)
+        rb = C.POINTER(C.c_double)()
+        lib.ugmg_problem_rhs(h, C.byref(rb))
+        rhs = _arr(rb, levels[-1].n * b, np.float64)
+    finally:
+        lib.ugmg_problem_free(h)
+    return Problem(dim=dim, levels=levels, rhs=rhs, name=name or f"q1_{dim}d_{coarse_cells}c_{nref}r")
+
+
+def splitmix_uniform(seed: int, count: int, a: float = 0.0, b: float = 1.0) -> np.ndarray:
+    lib = _load()
+    out = np.empty(count, dtype=np.float64)
+    lib.ugmg_splitmix_uniform(C.c_uint64(seed), count, a, b, out.ctypes.data_as(C.POINTER(C.c_double)))
+    return out
+
+
+# Workloads named by what they are (sizes in the docstrings).
+def poisson2d_129(seed: int = 0x5EED0001) -> Problem:
+    """-Laplace u = f, Q1 on the unit square, 2x2 coarse quads refined 6 times: 129x129 nodes."""
+    return q1(2, 2, 6, KAPPA_ONE, seed, name="poisson2d_q1_129x129")
+
+
+def jump3d(nref: int = 7, seed: int = 0x5EED0002) -> Problem:
+    """-div(kappa grad u) = f, Q1 hexes on the unit cube from 1 hex refined nref times
+    (nref=7: 129^3 = 2,146,689 nodes), kappa = 10^U[-2,2] per finest element."""
+    n = 2 ** nref + 1
+    return q1(3, 1, nref, KAPPA_JUMP, seed, name=f"jump3d_q1_{n}^3")

@@ -1,0 +1,291 @@
+"""Python host side of libugmg (ctypes over include/ugmg.h).
+
+Mirrors the numproc calls a UG script makes (npinit options become Params
+fields; PreProcess/Iter/Solver map to preprocess/iter/solve).  Device memory
+comes from torch tensors (plumbing only); every compute call goes to the HIP
+library, and loading fails loudly when libugmg.so is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libugmg.so")
+HEADER = os.path.join(os.path.dirname(_HERE), "include", "ugmg.h")
+
+SMOOTH_JAC, SMOOTH_GS = 0, 1
+FP64, MIXED = 0, 1
+ERRORS = {0: "OK", -1: "E_ARG", -2: "E_HIP", -3: "E_STATE", -4: "E_NOMEM", -5: "E_SINGULAR",
+          -6: "E_BUDGET", -7: "E_NODEV"}
+
+# time_kernel kinds
+K_SPMV64, K_SPMV32, K_JAC64, K_GS64, K_RESTRICT, K_INTERP, K_CYCLE, K_GS32 = range(8)
+
+
+class UgmgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Params(C.Structure):
+    _fields_ = [("nu1", C.c_int), ("nu2", C.c_int), ("gamma", C.c_int), ("baselevel", C.c_int),
+                ("smoother", C.c_int),
+                ("damp", C.c_double * 3), ("lmgc_damp", C.c_double * 3), ("base_damp", C.c_double * 3),
+                ("base_abslimit", C.c_double * 3), ("base_red", C.c_double * 3),
+                ("base_maxiter", C.c_int), ("ex_reorder", C.c_int), ("precision", C.c_int),
+                ("inner_cycles", C.c_int), ("use_graph", C.c_int)]
+
+
+class LResult(C.Structure):
+    _fields_ = [("error_code", C.c_int), ("converged", C.c_int), ("first_defect", C.c_double * 3),
+                ("last_defect", C.c_double * 3), ("number_of_linear_iterations", C.c_int)]
+
+
+DEFECT_CB = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double))
+
+_lib = None
+
+
+def declared_symbols(header=HEADER):
+    """Function names declared in include/ugmg.h."""
+    text = open(header).read()
+    return sorted(set(re.findall(r"\b(ugmg_[a-z0-9_]+)\s*\(", text)))
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} missing: the HIP library must be built (__graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    vp, ip, dp = C.c_void_p, C.c_int, C.c_void_p
+    sig = {
+        "ugmg_params_default": ([C.POINTER(Params)], None),
+        "ugmg_create": ([ip, C.POINTER(vp)], ip),
+        "ugmg_destroy": ([vp], ip),
+        "ugmg_last_error": ([vp], C.c_char_p),
+        "ugmg_set_stream": ([vp, vp], ip),
+        "ugmg_sync": ([vp], ip),
+        "ugmg_set_nlevels": ([vp, ip, ip], ip),
+        "ugmg_set_level": ([vp, ip, ip, vp, vp, vp, vp, vp, vp], ip),
+        "ugmg_set_transfer": ([vp, ip, vp, vp, vp], ip),
+        "ugmg_set_params": ([vp, C.POINTER(Params)], ip),
+        "ugmg_preprocess": ([vp, ip], ip),
+        "ugmg_postprocess": ([vp], ip),
+        "ugmg_level_size": ([vp, ip, C.POINTER(ip), C.POINTER(ip)], ip),
+        "ugmg_num_colors": ([vp, ip], ip),
+        "ugmg_get_colors": ([vp, ip, vp], ip),
+        "ugmg_get_pattern": ([vp, ip, vp, vp, vp], ip),
+        "ugmg_dmatmul_minus": ([vp, ip, dp, dp], ip),
+        "ugmg_dmatmul": ([vp, ip, dp, dp], ip),
+        "ugmg_smatmul_minus": ([vp, ip, dp, dp], ip),
+        "ugmg_dnrm2x": ([vp, ip, dp, dp], ip),
+        "ugmg_ddot": ([vp, ip, dp, dp, dp], ip),
+        "ugmg_jacobi_step": ([vp, ip, dp, dp], ip),
+        "ugmg_gs_step": ([vp, ip, dp, dp], ip),
+        "ugmg_restrict": ([vp, ip, dp, dp, dp], ip),
+        "ugmg_interpolate": ([vp, ip, dp, dp, dp], ip),
+        "ugmg_base_solve": ([vp, dp, dp], ip),
+        "ugmg_iter": ([vp, ip, dp, dp], ip),
+        "ugmg_iter_host": ([vp, ip, dp, dp], ip),
+        "ugmg_solve": ([vp, ip, dp, dp, dp, dp, ip, ip, C.POINTER(LResult), DEFECT_CB, vp], ip),
+        "ugmg_solve_host": ([vp, ip, dp, dp, dp, dp, ip, ip, C.POINTER(LResult), DEFECT_CB, vp], ip),
+        "ugmg_time_kernel": ([vp, ip, ip, ip, C.POINTER(C.c_double), C.POINTER(C.c_double)], ip),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def default_params(**kw) -> Params:
+    p = Params()
+    load().ugmg_params_default(C.byref(p))
+    for k, v in kw.items():
+        if k in ("damp", "lmgc_damp", "base_damp", "base_abslimit", "base_red"):
+            arr = getattr(p, k)
+            vals = [v] * 3 if np.isscalar(v) else list(v)
+            for i in range(3):
+                arr[i] = vals[min(i, len(vals) - 1)]
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def _ptr(t):
+    """Device pointer of a torch tensor (or None)."""
+    return None if t is None else t.data_ptr()
+
+
+class Context:
+    """One ugmg_ctx (device hierarchy + settings) on one GPU."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        h = C.c_void_p()
+        rc = self.lib.ugmg_create(device, C.byref(h))
+        if rc != 0:
+            raise UgmgError(rc, "ugmg_create failed (no GPU?)")
+        self.h = h
+        self.params = default_params()
+        self.problem = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.ugmg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise UgmgError(rc, self.lib.ugmg_last_error(self.h).decode())
+        return rc
+
+    # ---------------------------------------------------------------- setup
+    def set_stream(self, stream=None):
+        """Use a torch stream (or None for the context's own stream)."""
+        s = None if stream is None else C.c_void_p(stream.cuda_stream)
+        self._chk(self.lib.ugmg_set_stream(self.h, s))
+
+    def set_hierarchy(self, problem, top=None):
+        levels = problem.levels if top is None else problem.levels[:top + 1]
+        self._keep = []
+        self._chk(self.lib.ugmg_set_nlevels(self.h, len(levels), levels[0].b))
+        for l, lv in enumerate(levels):
+            arrs = [np.ascontiguousarray(lv.row_ptr, np.int32), np.ascontiguousarray(lv.col_idx, np.int32),
+                    np.ascontiguousarray(lv.vals, np.float64), np.ascontiguousarray(lv.skip, np.uint32),
+                    np.ascontiguousarray(lv.vclass, np.uint8), np.ascontiguousarray(lv.vnclass, np.uint8)]
+            self._chk(self.lib.ugmg_set_level(self.h, l, lv.n, *[a.ctypes.data for a in arrs]))
+        for l in range(1, len(levels)):
+            lv = levels[l]
+            arrs = [np.ascontiguousarray(lv.p_ptr, np.int32), np.ascontiguousarray(lv.p_idx, np.int32),
+                    np.ascontiguousarray(lv.p_w, np.float64)]
+            self._chk(self.lib.ugmg_set_transfer(self.h, l, *[a.ctypes.data for a in arrs]))
+        self.problem = problem
+        self.nlevels = len(levels)
+
+    def set_params(self, **kw):
+        self.params = default_params(**kw)
+        self._chk(self.lib.ugmg_set_params(self.h, C.byref(self.params)))
+
+    def preprocess(self, level=None):
+        level = self.nlevels - 1 if level is None else level
+        self._chk(self.lib.ugmg_preprocess(self.h, level))
+        self.top = level
+
+    def postprocess(self):
+        self._chk(self.lib.ugmg_postprocess(self.h))
+
+    def sync(self):
+        self._chk(self.lib.ugmg_sync(self.h))
+
+    def colors(self, level):
+        n = self.problem.levels[level].n
+        out = np.zeros(n, dtype=np.int32)
+        self._chk(self.lib.ugmg_get_colors(self.h, level, out.ctypes.data))
+        return out
+
+    def pattern(self, level):
+        lv = self.problem.levels[level]
+        rp = np.zeros(lv.n + 1, np.int32)
+        ci = np.zeros(lv.nnzb, np.int32)
+        va = np.zeros(lv.nnzb, np.float64)
+        self._chk(self.lib.ugmg_get_pattern(self.h, level, rp.ctypes.data, ci.ctypes.data, va.ctypes.data))
+        return rp, ci, va
+
+    # ------------------------------------------------------ device kernels
+    def dmatmul_minus(self, level, x, y):
+        self._chk(self.lib.ugmg_dmatmul_minus(self.h, level, _ptr(x), _ptr(y)))
+
+    def dmatmul(self, level, x, y):
+        self._chk(self.lib.ugmg_dmatmul(self.h, level, _ptr(x), _ptr(y)))
+
+    def smatmul_minus(self, level, x, y):
+        self._chk(self.lib.ugmg_smatmul_minus(self.h, level, _ptr(x), _ptr(y)))
+
+    def dnrm2x(self, level, x):
+        out = C.c_double()
+        self._chk(self.lib.ugmg_dnrm2x(self.h, level, _ptr(x), C.addressof(out)))
+        return out.value
+
+    def ddot(self, level, x, y):
+        out = C.c_double()
+        self._chk(self.lib.ugmg_ddot(self.h, level, _ptr(x), _ptr(y), C.addressof(out)))
+        return out.value
+
+    def jacobi_step(self, level, x, b):
+        self._chk(self.lib.ugmg_jacobi_step(self.h, level, _ptr(x), _ptr(b)))
+
+    def gs_step(self, level, x, b):
+        self._chk(self.lib.ugmg_gs_step(self.h, level, _ptr(x), _ptr(b)))
+
+    def restrict(self, level, to_c, from_f, damp=1.0):
+        d = (C.c_double * 3)(damp, damp, damp)
+        self._chk(self.lib.ugmg_restrict(self.h, level, _ptr(to_c), _ptr(from_f), C.addressof(d)))
+
+    def interpolate(self, level, to_f, from_c, damp=1.0):
+        d = (C.c_double * 3)(damp, damp, damp)
+        self._chk(self.lib.ugmg_interpolate(self.h, level, _ptr(to_f), _ptr(from_c), C.addressof(d)))
+
+    def base_solve(self, c, b):
+        self._chk(self.lib.ugmg_base_solve(self.h, _ptr(c), _ptr(b)))
+
+    def iter(self, level, c, b):
+        self._chk(self.lib.ugmg_iter(self.h, level, _ptr(c), _ptr(b)))
+
+    def iter_host(self, level, b):
+        lv = self.problem.levels[level]
+        c = np.zeros(lv.n * lv.b)
+        b = np.array(b, dtype=np.float64, copy=True)
+        self._chk(self.lib.ugmg_iter_host(self.h, level, c.ctypes.data, b.ctypes.data))
+        return c, b
+
+    def solve(self, level, x, b, red=1e-10, abslimit=1e-300, maxiter=50, do_defect=True, record=True):
+        """ls solve on device tensors x, b; returns (LResult, defects list)."""
+        res = LResult()
+        defects = []
+
+        def _cb(user, it, ncomp, d):
+            defects.append([d[i] for i in range(ncomp)])
+
+        cb = DEFECT_CB(_cb) if record else DEFECT_CB()
+        ab = (C.c_double * 3)(abslimit, abslimit, abslimit)
+        rd = (C.c_double * 3)(red, red, red)
+        self._chk(self.lib.ugmg_solve(self.h, level, _ptr(x), _ptr(b), C.addressof(ab), C.addressof(rd),
+                                      maxiter, int(do_defect), C.byref(res), cb, None))
+        return res, defects
+
+    def solve_host(self, level, rhs, x0=None, red=1e-10, abslimit=1e-300, maxiter=50, do_defect=True):
+        lv = self.problem.levels[level]
+        x = np.zeros(lv.n * lv.b) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
+        b = np.array(rhs, dtype=np.float64, copy=True)
+        res = LResult()
+        defects = []
+
+        def _cb(user, it, ncomp, d):
+            defects.append([d[i] for i in range(ncomp)])
+
+        cb = DEFECT_CB(_cb)
+        ab = (C.c_double * 3)(abslimit, abslimit, abslimit)
+        rd = (C.c_double * 3)(red, red, red)
+        self._chk(self.lib.ugmg_solve_host(self.h, level, x.ctypes.data, b.ctypes.data, C.addressof(ab),
+                                           C.addressof(rd), maxiter, int(do_defect), C.byref(res), cb, None))
+        return x, b, res, np.array(defects)
+
+    def time_kernel(self, level, kind, reps=20):
+        ms, by = C.c_double(), C.c_double()
+        self._chk(self.lib.ugmg_time_kernel(self.h, level, kind, reps, C.byref(ms), C.byref(by)))
+        return ms.value, by.value

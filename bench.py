@@ -1,0 +1,231 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X multigrid solve path.
+
+Workload (one GPU): 3D jump-coefficient diffusion -div(kappa grad u) = f on
+the unit cube, trilinear Q1 hexes from one hex refined 7 times (129^3 =
+2,146,689 unknowns, 27-point stencil), kappa = 10^U[-2,2] per finest
+element, f ~ U[-1,1]; ls + lmgc V(2,2) with multicolour Gauss-Seidel,
+base level 2 (5^3 vectors, band LU), stop at 1e-10 relative defect.
+
+A "step" is one complete solve to 1e-10 from x = 0 with the hierarchy
+resident in HBM.  The headline value is the mixed-precision form (FP32
+V-cycles inside FP64 refinement); the plain FP64 form is timed beside it.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "time-to-1e-10 defect reduction, mixed vs FP64 GPU; p50/p99 solve latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_info():
+    model, cores = "unknown", os.cpu_count()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, cores
+
+
+def load_traffic(name):
+    """HBM bytes per launch from a committed rocprofv3 --pmc summary, or None."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            return json.load(f).get("traffic_bytes")
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nref", type=int, default=7, help="refinements of the unit hex (7: 129^3)")
+    ap.add_argument("--inner", type=int, default=3, help="FP32 V-cycles per FP64 refinement step")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
+    ap.add_argument("--cpu-cycles", type=int, default=2)
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="run only the roofline kernels and a copy kernel of known size (for rocprofv3 --pmc)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the sharded form is described in DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+    from ug_amd import problems, ugmg
+
+    t0 = time.time()
+    P = problems.jump3d(args.nref)
+    gen_s = time.time() - t0
+    top = P.top
+    n = P.levels[-1].n
+    base = dict(nu1=2, nu2=2, gamma=1, baselevel=2, smoother=ugmg.SMOOTH_GS, damp=1.0,
+                base_abslimit=1e-300, base_red=1e-12, base_maxiter=1, use_graph=1)
+
+    ctx64 = ugmg.Context(0)
+    ctx64.set_hierarchy(P)
+    ctx64.set_params(**base, precision=ugmg.FP64)
+    ctx64.preprocess()
+    if args.kernels_only:
+        # 10 FP64 SpMV launches on the finest level, then a copy kernel of
+        # known size (torch elementwise: reads 8*m bytes, writes 8*m bytes)
+        ms, by = ctx64.time_kernel(top, ugmg.K_SPMV64, 10)
+        m = 1 << 26
+        src = torch.ones(m, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        print(json.dumps({"spmv_fp64_us": ms * 1e3, "spmv_fp64_algorithmic_bytes": by,
+                          "copy_kernel_bytes_read": 8 * m, "copy_kernel_bytes_written": 8 * m}))
+        ctx64.close()
+        return 0
+    ctxmx = ugmg.Context(0)
+    ctxmx.set_hierarchy(P)
+    ctxmx.set_params(**base, precision=ugmg.MIXED, inner_cycles=args.inner)
+    ctxmx.preprocess()
+
+    rhs = torch.from_numpy(P.rhs).cuda()
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(rhs)
+    torch.cuda.synchronize()
+
+    def solve(ctx):
+        x.zero_()
+        b.copy_(rhs)
+        torch.cuda.synchronize()
+        res, d = ctx.solve(top, x, b, red=1e-10, abslimit=1e-300, maxiter=500, record=True)
+        return res, d
+
+    def timed(ctx, k):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        out = None
+        lat = []
+        for _ in range(k):
+            s = time.perf_counter()
+            out = solve(ctx)
+            torch.cuda.synchronize()
+            lat.append(time.perf_counter() - s)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / k, out, lat
+
+    for _ in range(args.warmup):
+        solve(ctx64)
+        solve(ctxmx)
+    t64, (r64, d64), _ = timed(ctx64, args.steps)
+    x64 = x.detach().clone()
+    tmx, (rmx, dmx), lat = timed(ctxmx, args.steps)
+    xmx = x.detach().clone()
+
+    # equal final accuracy: true defect |f - A x| / |f| of both solutions in FP64
+    def true_rel(xs):
+        bb = rhs.clone()
+        torch.cuda.synchronize()
+        ctx64.dmatmul_minus(top, bb, xs)
+        ctx64.sync()
+        return float(torch.linalg.norm(bb) / torch.linalg.norm(rhs))
+
+    acc64, accmx = true_rel(x64), true_rel(xmx)
+    diff = float(torch.linalg.norm(xmx - x64) / torch.linalg.norm(x64))
+
+    # roofline: FP64 SpMV (dmatmul_minus) and the FP64 GS step on the finest level
+    ms_spmv, by_spmv = ctx64.time_kernel(top, ugmg.K_SPMV64, 20)
+    ms_gs, by_gs = ctx64.time_kernel(top, ugmg.K_GS64, 10)
+    ms_spmv32, by_spmv32 = ctxmx.time_kernel(top, ugmg.K_SPMV32, 20)
+    ach = by_spmv / (ms_spmv * 1e-3) / 1e9
+    ach_gs = by_gs / (ms_gs * 1e-3) / 1e9
+    ach32 = by_spmv32 / (ms_spmv32 * 1e-3) / 1e9
+
+    out = {
+        "metric": METRIC,
+        "value": round(tmx * 1e3, 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(tmx * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 V-cycle in f64 refinement",
+        "data": "synthetic (seeded SplitMix64 kappa and f)",
+        "config": {
+            "workload": f"jump3d_q1_{2 ** args.nref + 1}^3: -div(kappa grad u)=f, unit cube, Q1 hex, "
+                        f"kappa=10^U[-2,2] per element, V(2,2) multicolour GS, base level 2, tol 1e-10",
+            "unknowns": n,
+            "nnz_fine": int(P.levels[-1].nnzb),
+            "levels": top + 1,
+            "inner_cycles": args.inner,
+        },
+        "fp64_ms": round(t64 * 1e3, 3),
+        "speedup_mixed_vs_fp64": round(t64 / tmx, 3),
+        "fp64_iterations": r64.number_of_linear_iterations,
+        "mixed_outer_iterations": rmx.number_of_linear_iterations,
+        "final_true_rel_defect": {"fp64": acc64, "mixed": accmx},
+        "mixed_vs_fp64_solution_rel_diff": diff,
+        "mixed_latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                             "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)},
+        "roofline": {
+            "kernel": "k_spmv<double> (dmatmul_minus, b -= A x) on the 129^3 level",
+            "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": load_traffic("spmv_fp64_jump3d_129_traffic.json"),
+            "algorithmic_bytes": by_spmv, "avg_launch_us": round(ms_spmv * 1e3, 2),
+        },
+        "roofline_smoother": {
+            "kernel": "multicolour GS step (8 colour sweeps + defect) FP64 on the 129^3 level",
+            "bound": "hbm", "achieved": round(ach_gs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach_gs / HBM_PEAK_GBS, 4), "algorithmic_bytes": by_gs,
+            "avg_step_us": round(ms_gs * 1e3, 2),
+        },
+        "roofline_spmv_fp32": {"achieved": round(ach32, 1), "frac": round(ach32 / HBM_PEAK_GBS, 4),
+                               "avg_launch_us": round(ms_spmv32 * 1e3, 2)},
+        "setup_s": {"generate": round(gen_s, 2)},
+    }
+
+    if not args.no_cpu:
+        from oracle import oracle as O
+        model, cores = cpu_info()
+        H = O.Hierarchy(P.levels)
+        p = O.params(smoother=O.GS, damp=1.0, baselevel=2)
+        t = time.perf_counter()
+        O.ls_solve(H, top, p, P.rhs, red=1e-300, maxiter=args.cpu_cycles)
+        cyc = (time.perf_counter() - t) / args.cpu_cycles
+        out["cpu_baseline"] = {
+            "value": round(cyc * r64.number_of_linear_iterations * 1e3, 1),
+            "unit": "ms",
+            "cores": 1,
+            "kind": "port",
+            "sample": f"{args.cpu_cycles} ls+lmgc V(2,2) cycles with UG's lexicographic GS on the same "
+                      f"{n}-unknown hierarchy, 1 thread of {cores} ({model}); per-cycle time "
+                      f"{cyc * 1e3:.0f} ms scaled by the FP64 GPU cycle count "
+                      f"({r64.number_of_linear_iterations})",
+        }
+    out["host"] = platform.node()
+    print(json.dumps(out))
+    ctx64.close()
+    ctxmx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

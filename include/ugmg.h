@@ -156,6 +156,19 @@ int ugmg_solve_host(ugmg_ctx *ctx, int level, double *x, double *b,
                     int maxiter, int do_defect, ugmg_lresult *res,
                     ugmg_defect_cb cb, void *user);
 
+/* ---- host-only setup routines (no GPU needed) ---------------------------
+ * The pieces of PreProcess that run on the host, callable on their own.
+ * ugmg_color: greedy first-fit colouring in list order over the symmetrised
+ *   pattern (the multicolour GS ordering); returns the colour count or < 0.
+ * ugmg_coarse_inverse: EXPreProcess (iter.cc:8685-8970) -- optional BFS
+ *   band reorder, band LU without pivoting (evm.cc:1400-1418) -- followed by
+ *   LU solves of the unit vectors: inv (n*n, row-major, list order), order
+ *   (new list order), bandwidth.  UGMG_E_SINGULAR on a zero pivot. */
+int ugmg_color(int n, const int *row_ptr, const int *col_idx, int *colors);
+int ugmg_coarse_inverse(int n, const int *row_ptr, const int *col_idx,
+                        const double *vals, int reorder, double *inv,
+                        int *order, int *bandwidth);
+
 /* ---- measurement --------------------------------------------------------
  * kind: 0 dmatmul_minus FP64, 1 smatmul_minus FP32, 2 jacobi step FP64,
  *       3 gs step FP64, 4 restrict, 5 interpolate, 6 one cycle (current

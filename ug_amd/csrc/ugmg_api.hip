@@ -834,6 +834,47 @@ int ugmg_solve_host(ugmg_ctx *ctx, int level, double *x, double *b, const double
   return rc;
 }
 
+int ugmg_color(int n, const int *row_ptr, const int *col_idx, int *colors) {
+  if (n < 0 || (n > 0 && (!row_ptr || !col_idx || !colors))) return UGMG_E_ARG;
+  try {
+    HostLevel L;
+    L.n = n;
+    L.rp.assign(row_ptr, row_ptr + n + 1);
+    L.ci.assign(col_idx, col_idx + L.rp[n]);
+    for (int c : L.ci)
+      if (c < 0 || c >= n) return UGMG_E_ARG;
+    std::vector<int> col;
+    const int nc = greedy_color(L, col);
+    std::copy(col.begin(), col.end(), colors);
+    return nc;
+  } catch (const std::bad_alloc &) {
+    return UGMG_E_NOMEM;
+  }
+}
+
+int ugmg_coarse_inverse(int n, const int *row_ptr, const int *col_idx, const double *vals,
+                        int reorder, double *inv, int *order, int *bandwidth) {
+  if (n < 0 || (n > 0 && (!row_ptr || !col_idx || !vals || !inv))) return UGMG_E_ARG;
+  try {
+    HostLevel L;
+    L.n = n;
+    L.rp.assign(row_ptr, row_ptr + n + 1);
+    L.ci.assign(col_idx, col_idx + L.rp[n]);
+    L.a.assign(vals, vals + L.rp[n]);
+    for (int c : L.ci)
+      if (c < 0 || c >= n) return UGMG_E_ARG;
+    HostBase B;
+    build_base(L, reorder, B);
+    if (order) std::copy(B.order.begin(), B.order.end(), order);
+    if (bandwidth) *bandwidth = B.bw;
+    if (B.failed) return UGMG_E_SINGULAR;
+    std::copy(B.inv.begin(), B.inv.end(), inv);
+    return UGMG_OK;
+  } catch (const std::bad_alloc &) {
+    return UGMG_E_NOMEM;
+  }
+}
+
 int ugmg_time_kernel(ugmg_ctx *ctx, int level, int kind, int reps, double *ms, double *bytes) {
   int rc = check_ready(ctx, level);
   if (rc) return rc;

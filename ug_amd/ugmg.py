@@ -97,6 +97,8 @@ def load():
         "ugmg_solve": ([vp, ip, dp, dp, dp, dp, ip, ip, C.POINTER(LResult), DEFECT_CB, vp], ip),
         "ugmg_solve_host": ([vp, ip, dp, dp, dp, dp, ip, ip, C.POINTER(LResult), DEFECT_CB, vp], ip),
         "ugmg_time_kernel": ([vp, ip, ip, ip, C.POINTER(C.c_double), C.POINTER(C.c_double)], ip),
+        "ugmg_color": ([ip, vp, vp, vp], ip),
+        "ugmg_coarse_inverse": ([ip, vp, vp, vp, ip, vp, vp, C.POINTER(ip)], ip),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -118,6 +120,30 @@ def default_params(**kw) -> Params:
         else:
             setattr(p, k, v)
     return p
+
+
+def color(level):
+    """Greedy colouring of a problems.Level (host routine of libugmg)."""
+    rp = np.ascontiguousarray(level.row_ptr, np.int32)
+    ci = np.ascontiguousarray(level.col_idx, np.int32)
+    out = np.zeros(level.n, np.int32)
+    nc = load().ugmg_color(level.n, rp.ctypes.data, ci.ctypes.data, out.ctypes.data)
+    if nc < 0:
+        raise UgmgError(nc, "ugmg_color")
+    return nc, out
+
+
+def coarse_inverse(level, reorder=1):
+    """(rc, inverse, order, bandwidth) of a scalar level (host routine)."""
+    rp = np.ascontiguousarray(level.row_ptr, np.int32)
+    ci = np.ascontiguousarray(level.col_idx, np.int32)
+    va = np.ascontiguousarray(level.vals, np.float64)
+    inv = np.zeros((level.n, level.n))
+    order = np.zeros(level.n, np.int32)
+    bw = C.c_int()
+    rc = load().ugmg_coarse_inverse(level.n, rp.ctypes.data, ci.ctypes.data, va.ctypes.data, reorder,
+                                    inv.ctypes.data, order.ctypes.data, C.byref(bw))
+    return rc, inv, order, bw.value
 
 
 def _ptr(t):

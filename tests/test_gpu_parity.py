@@ -305,3 +305,51 @@ def test_errors_are_reported(poisson2d):
     with pytest.raises(ugmg.UgmgError):
         ctx.preprocess()
     ctx.close()
+
+
+@pytest.mark.parametrize("bl,maxit", [(3, 1), (4, 1), (2, 2)])
+def test_base_levels_and_base_iterations(jump3d_33, bl, maxit):
+    """Dense-inverse base solve at several base levels, and a base ls with
+    $m 2 (two ex applications), against the oracle's band-LU restatement."""
+    from ug_amd import ugmg
+    P = jump3d_33
+    H = O.Hierarchy(P.levels)
+    want = O.ls_solve(H, P.top, O.params(smoother=O.MCGS, damp=1.0, baselevel=bl, base_maxiter=maxit,
+                                         base_red=1e-3), P.rhs, red=1e-10, maxiter=200)
+    ctx = ugmg.Context(0)
+    ctx.set_hierarchy(P)
+    ctx.set_params(nu1=2, nu2=2, gamma=1, baselevel=bl, smoother=ugmg.SMOOTH_GS, damp=1.0,
+                   base_abslimit=1e-300, base_red=1e-3, base_maxiter=maxit, use_graph=1)
+    ctx.preprocess()
+    n = P.levels[-1].n
+    x, b = _full(n), _dev(P.rhs)
+    res, defects = ctx.solve(P.top, x, b, red=1e-10, maxiter=200)
+    d = np.array(defects)[:, 0]
+    assert res.number_of_linear_iterations == want["nit"]
+    assert np.max(np.abs(d - want["defects"][:, 0]) / want["defects"][:, 0]) < 1e-9
+    ctx.close()
+
+
+def test_fused_coarse_cycle_matches(poisson2d, monkeypatch):
+    """The single-workgroup coarse-cycle kernel (levels up to UGMG_FUSE_ROWS
+    vectors) gives the oracle's defects, W-cycle (gamma 2) included."""
+    from ug_amd import ugmg
+    H = O.Hierarchy(poisson2d.levels)
+    monkeypatch.setenv("UGMG_FUSE_ROWS", "5000")
+    for gamma, sm in ((1, ugmg.SMOOTH_JAC), (2, ugmg.SMOOTH_GS)):
+        osm = O.JAC if sm == ugmg.SMOOTH_JAC else O.MCGS
+        damp = 0.8 if sm == ugmg.SMOOTH_JAC else 1.0
+        want = O.ls_solve(H, poisson2d.top, O.params(smoother=osm, damp=damp, baselevel=0, gamma=gamma),
+                          poisson2d.rhs, red=1e-10, maxiter=50)
+        ctx = ugmg.Context(0)
+        ctx.set_hierarchy(poisson2d)
+        ctx.set_params(nu1=2, nu2=2, gamma=gamma, baselevel=0, smoother=sm, damp=damp, base_abslimit=1e-300,
+                       use_graph=1)
+        ctx.preprocess()
+        n = poisson2d.levels[-1].n
+        x, b = _full(n), _dev(poisson2d.rhs)
+        res, defects = ctx.solve(poisson2d.top, x, b, red=1e-10, maxiter=50)
+        d = np.array(defects)[:, 0]
+        assert res.number_of_linear_iterations == want["nit"], (gamma, sm)
+        assert np.max(np.abs(d - want["defects"][:, 0]) / want["defects"][:, 0]) < 1e-10
+        ctx.close()

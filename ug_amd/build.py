@@ -46,11 +46,11 @@ def build_ugmg():
         _run([HIPCC, *HIP_FLAGS, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     obj = os.path.join(BUILD, "ugmg_setup.o")
-    _run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, "ugmg_setup.cc"), "-o", obj])
+    _run(["g++", *CXX_FLAGS, "-fopenmp", "-c", os.path.join(CSRC, "ugmg_setup.cc"), "-o", obj])
     objs.append(obj)
     out = os.path.join(PKG, "libugmg.so")
     tmp = out + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lgomp"])
     os.replace(tmp, out)
 
 

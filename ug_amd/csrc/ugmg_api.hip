@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -57,18 +58,15 @@ struct DevLevel {
   uint8_t *vclass = nullptr, *vnclass = nullptr;
   uint32_t *skip = nullptr;
   int p_nnz = 0, pt_nnz = 0;
-  int *p_ptr = nullptr, *p_idx = nullptr, *pt_ptr = nullptr, *pt_idx = nullptr;
-  double *p_w64 = nullptr, *pt_w64 = nullptr;
-  float *p_w32 = nullptr, *pt_w32 = nullptr;
+  DevSellBuf P, PT;   // interpolation from / restriction to level-1
+  bool nt = false;    // stream the matrix with non-temporal loads
   double *c = nullptr, *b = nullptr, *t = nullptr;
   float *c32 = nullptr, *b32 = nullptr, *t32 = nullptr;
 
-  template <typename T> T *cv() { if constexpr (std::is_same_v<T, double>) return c; else return c32; }
-  template <typename T> T *bv() { if constexpr (std::is_same_v<T, double>) return b; else return b32; }
-  template <typename T> T *tv() { if constexpr (std::is_same_v<T, double>) return t; else return t32; }
-  template <typename T> const T *diag() { if constexpr (std::is_same_v<T, double>) return diag64; else return diag32; }
-  template <typename T> const T *pw() { if constexpr (std::is_same_v<T, double>) return p_w64; else return p_w32; }
-  template <typename T> const T *ptw() { if constexpr (std::is_same_v<T, double>) return pt_w64; else return pt_w32; }
+  template <typename T> T *cv() const { if constexpr (std::is_same_v<T, double>) return c; else return c32; }
+  template <typename T> T *bv() const { if constexpr (std::is_same_v<T, double>) return b; else return b32; }
+  template <typename T> T *tv() const { if constexpr (std::is_same_v<T, double>) return t; else return t32; }
+  template <typename T> const T *diag() const { if constexpr (std::is_same_v<T, double>) return diag64; else return diag32; }
 };
 
 }  // namespace
@@ -97,6 +95,15 @@ struct ugmg_ctx {
   hipGraphExec_t gexec = nullptr;
   int gkind = -1, glevel = -1;
   const void *gx = nullptr, *gb = nullptr;
+  // levels bl..fuse_top run inside one fused coarse-cycle kernel
+  int fuse_top = -1;
+  CLevel<double> *clev64 = nullptr;
+  CLevel<float> *clev32 = nullptr;
+  // base solver: partial sums of the base defect norm, first norm
+  double *base_part = nullptr;
+  double *base_first = nullptr;
+  int base_nparts = 0;
+  bool base_b_dead = false;   // every base vector has VNCLASS >= 2
 };
 
 namespace {
@@ -156,6 +163,11 @@ void free_device(ugmg_ctx *ctx) {
   ctx->allocs.clear();
   ctx->dev.clear();
   ctx->base_inv = nullptr;
+  ctx->clev64 = nullptr;
+  ctx->clev32 = nullptr;
+  ctx->fuse_top = -1;
+  ctx->base_part = nullptr;
+  ctx->base_first = nullptr;
   ctx->part = nullptr;
   ctx->red = nullptr;
   ctx->prepared = false;
@@ -178,29 +190,74 @@ void smooth_step(ugmg_ctx *ctx, int l, T *t, T *b, T *c, int cmode, T *xout) {
   if (ctx->p.smoother == UGMG_SMOOTH_JAC) {
     launch_jac<T>(st, d.n, b, d.diag<T>(), d.vclass, damp, t);
     const int mode = cmode < 0 ? 0 : (cmode == 0 ? 3 : 2);
-    launch_spmv<T, T>(st, d.A.d<T>(), 0, d.A.nslices, t, b, c, mode);
+    launch_spmv<T, T>(st, d.A.d<T>(), 0, d.A.nslices, t, b, c, mode, d.nt);
   } else {
     for (int k = 0; k < d.ncolors; ++k)
       launch_gs_color<T, T>(st, d.L.d<T>(), d.color_slices[k],
                             d.color_slices[k + 1] - d.color_slices[k], b, d.diag<T>(),
-                            d.vclass, t);
+                            d.vclass, t, d.nt);
     launch_gs_defect<T, T>(st, d.U.d<T>(), t, b, d.vclass, damp, cmode < 0 ? nullptr : c,
-                           cmode < 0 ? 1 : cmode, xout);
+                           cmode < 0 ? 1 : cmode, xout, d.nt);
   }
 }
 
-template <typename T>
-void base_solve(ugmg_ctx *ctx, T *c, T *b, int cmode) {
-  DevLevel &d = ctx->dev[ctx->bl];
+BaseArgs base_args(ugmg_ctx *ctx) {
   BaseArgs ba;
-  ba.n = d.n;
+  ba.n = ctx->dev[ctx->bl].n;
   ba.maxiter = ctx->p.base_maxiter;
   ba.failed = ctx->base.failed ? 1 : 0;
   ba.damp = ctx->p.base_damp[0];
   ba.abslimit = ctx->p.base_abslimit[0];
   ba.red = ctx->p.base_red[0];
   ba.inv = ctx->base_inv;
-  launch_base_solve<T>(ctx->stream, ba, d.A.d64(), d.vclass, b, c, cmode);
+  return ba;
+}
+
+// Lmgc on levels top..bl in one workgroup (levels at most a few thousand
+// vectors, where a separate launch per operation would be latency bound)
+template <typename T>
+void coarse_cycle(ugmg_ctx *ctx, int top, T *c, T *b, int cmode) {
+  CoarseArgs<T> a;
+  if constexpr (std::is_same_v<T, double>) a.lv = ctx->clev64;
+  else a.lv = ctx->clev32;
+  a.bl = ctx->bl;
+  a.top = top;
+  const ugmg_params &p = ctx->p;
+  a.cp = CycleParams{p.nu1, p.nu2, p.gamma, p.smoother, p.damp[0], p.lmgc_damp[0]};
+  a.base = base_args(ctx);
+  a.baseA = ctx->dev[ctx->bl].A.d64();
+  a.base_vclass = ctx->dev[ctx->bl].vclass;
+  a.c_top = c;
+  a.b_top = b;
+  a.cmode_top = cmode;
+  launch_coarse_cycle<T>(ctx->stream, a, ctx->dev[ctx->bl].n);
+}
+
+// ls over ex on the base level: `maxiter` GEMVs with the dense inverse.  The
+// first defect norm comes from the restriction's partial sums (have_parts)
+// or from a separate reduction.  Inside a cycle the base-level defect b is
+// dead once the coarse correction is known (the next restriction overwrites
+// every entry when all base vectors have VNCLASS >= 2), so b -= A x is only
+// formed when a later iteration or the caller needs it.
+template <typename T>
+void base_solve(ugmg_ctx *ctx, T *c, T *b, int cmode, bool have_parts, bool need_b) {
+  DevLevel &d = ctx->dev[ctx->bl];
+  hipStream_t st = ctx->stream;
+  const ugmg_params &p = ctx->p;
+  int np = have_parts ? ctx->base_nparts : launch_dot_part<T>(st, d.n, b, d.vclass, ctx->base_part);
+  T *x = d.tv<T>();
+  const int m = std::max(0, p.base_maxiter);
+  if (m == 0 && cmode == 0) launch_fill<T>(st, d.n, c, (T)0);
+  for (int it = 0; it < m; ++it) {
+    if (!ctx->base.failed)
+      launch_base_gemv<T>(st, d.n, ctx->base_inv, b, c, x, it == 0 ? cmode : 1, p.base_damp[0],
+                          p.base_abslimit[0], p.base_red[0], ctx->base_part, np,
+                          ctx->base_first, it);
+    const bool more = it + 1 < m;
+    if (more || need_b || !ctx->base_b_dead)
+      launch_spmv<T, T>(st, d.A.d<T>(), 0, d.A.nslices, x, b, (T *)nullptr, 0, d.nt);
+    if (more) np = launch_dot_part<T>(st, d.n, b, d.vclass, ctx->base_part);
+  }
 }
 
 // Lmgc on level l; cmode 0: c := correction, 1: c += correction
@@ -208,8 +265,12 @@ template <typename T>
 void lmgc(ugmg_ctx *ctx, int l, T *c, T *b, int cmode) {
   const ugmg_params &p = ctx->p;
   hipStream_t st = ctx->stream;
+  if (l <= ctx->fuse_top) {
+    coarse_cycle<T>(ctx, l, c, b, cmode);
+    return;
+  }
   if (l <= ctx->bl) {
-    base_solve<T>(ctx, c, b, cmode);
+    base_solve<T>(ctx, c, b, cmode, false, true);
     return;
   }
   DevLevel &d = ctx->dev[l];
@@ -218,14 +279,23 @@ void lmgc(ugmg_ctx *ctx, int l, T *c, T *b, int cmode) {
   for (int i = 0; i < p.nu1; ++i) smooth_step<T>(ctx, l, t, b, c, i == 0 ? cmode : 1, nullptr);
   if (p.nu1 == 0 && cmode == 0) launch_fill<T>(st, d.n, c, (T)0);
   // RestrictDefect(level, b, b, Factor_One)
-  launch_restrict<T>(st, dc.n, d.pt_ptr, d.pt_idx, d.ptw<T>(), dc.skip, dc.vnclass, b, (T)1,
-                     dc.bv<T>());
+  const bool to_base = l - 1 == ctx->bl && ctx->fuse_top < ctx->bl;
+  const int np = launch_restrict<T>(st, d.PT.d<T>(), dc.skip, dc.vnclass, dc.vclass, b, (T)1,
+                                    dc.bv<T>(), to_base ? ctx->base_part : nullptr, d.nt);
   if (p.gamma <= 0) launch_fill<T>(st, dc.n, dc.cv<T>(), (T)0);
-  for (int g = 0; g < p.gamma; ++g) lmgc<T>(ctx, l - 1, dc.cv<T>(), dc.bv<T>(), g == 0 ? 0 : 1);
+  for (int g = 0; g < p.gamma; ++g) {
+    if (to_base) {
+      // the first visit uses the restriction's partial norms; after a
+      // base solve b has changed, so later visits recompute them
+      if (g == 0) ctx->base_nparts = np;
+      base_solve<T>(ctx, dc.cv<T>(), dc.bv<T>(), g == 0 ? 0 : 1, g == 0, false);
+    } else {
+      lmgc<T>(ctx, l - 1, dc.cv<T>(), dc.bv<T>(), g == 0 ? 0 : 1);
+    }
+  }
   // InterpolateCorrection(level, t, c, damp); dadd(c, t); dmatmul_minus(b, A, t)
-  launch_interp<T>(st, d.n, d.p_ptr, d.p_idx, d.pw<T>(), d.skip, dc.cv<T>(), (T)p.lmgc_damp[0],
-                   t, c);
-  launch_spmv<T, T>(st, d.A.d<T>(), 0, d.A.nslices, t, b, (T *)nullptr, 0);
+  launch_interp<T>(st, d.P.d<T>(), d.skip, dc.cv<T>(), (T)p.lmgc_damp[0], t, c, d.nt);
+  launch_spmv<T, T>(st, d.A.d<T>(), 0, d.A.nslices, t, b, (T *)nullptr, 0, d.nt);
   for (int i = 0; i < p.nu2; ++i) smooth_step<T>(ctx, l, t, b, c, 1, nullptr);
 }
 
@@ -254,7 +324,7 @@ void enqueue_iteration(ugmg_ctx *ctx, int level, double *x, double *b) {
     const int k = std::max(1, ctx->p.inner_cycles);
     for (int j = 0; j < k; ++j) lmgc<float>(ctx, level, d.c32, d.b32, j == 0 ? 0 : 1);
     int np = 0;
-    launch_ir_update(ctx->stream, d.A.d64(), d.c32, b, x, d.b32, d.vclass, ctx->part, &np);
+    launch_ir_update(ctx->stream, d.A.d64(), d.c32, b, x, d.b32, d.vclass, ctx->part, &np, d.nt);
     launch_reduce_final(ctx->stream, np, ctx->part, ctx->red, 1);
   } else {
     lmgc<double>(ctx, level, x, b, 1);
@@ -492,11 +562,15 @@ int ugmg_preprocess(ugmg_ctx *ctx, int level) {
     if (l > bl && !ctx->transfer_set[l])
       return fail(ctx, UGMG_E_STATE, "transfer of level " + std::to_string(l) + " not set");
   }
-  if (ctx->host[bl].n > 2048)
-    return fail(ctx, UGMG_E_ARG, "base level has more than 2048 vectors; choose a coarser baselevel");
+  if (ctx->host[bl].n > 8192)
+    return fail(ctx, UGMG_E_ARG, "base level has more than 8192 vectors; choose a coarser baselevel");
   try {
     ctx->dev.assign(level + 1, DevLevel());
     ctx->bl = bl;
+    const char *env_nt = std::getenv("UGMG_NT_BYTES");
+    const double nt_bytes = env_nt ? std::atof(env_nt) : 64.0e6;
+    const char *env_fuse = std::getenv("UGMG_FUSE_ROWS");
+    const int fuse_rows = env_fuse ? std::atoi(env_fuse) : 0;
     int rc;
     int max_slices = 0;
     for (int l = bl; l <= level; ++l) {
@@ -524,18 +598,11 @@ int ugmg_preprocess(ugmg_ctx *ctx, int level) {
       }
       if (l > bl) {
         D.p_nnz = H.pp[H.n];
-        if ((rc = upload(ctx, &D.p_ptr, H.pp.data(), H.pp.size()))) return rc;
-        if ((rc = upload(ctx, &D.p_idx, H.pi.data(), H.pi.size()))) return rc;
-        if ((rc = upload(ctx, &D.p_w64, H.pw.data(), H.pw.size()))) return rc;
-        std::vector<float> w32(H.pw.begin(), H.pw.end());
-        if ((rc = upload(ctx, &D.p_w32, w32.data(), w32.size()))) return rc;
         D.pt_nnz = (int)hp.pt_idx.size();
-        if ((rc = upload(ctx, &D.pt_ptr, hp.pt_ptr.data(), hp.pt_ptr.size()))) return rc;
-        if ((rc = upload(ctx, &D.pt_idx, hp.pt_idx.data(), hp.pt_idx.size()))) return rc;
-        if ((rc = upload(ctx, &D.pt_w64, hp.pt_w.data(), hp.pt_w.size()))) return rc;
-        std::vector<float> tw32(hp.pt_w.begin(), hp.pt_w.end());
-        if ((rc = upload(ctx, &D.pt_w32, tw32.data(), tw32.size()))) return rc;
+        if ((rc = upload_sell(ctx, hp.P, D.P))) return rc;
+        if ((rc = upload_sell(ctx, hp.PT, D.PT))) return rc;
       }
+      D.nt = (double)hp.A.col.size() * 12.0 > nt_bytes;
       if ((rc = dalloc(ctx, &D.c, D.n)) || (rc = dalloc(ctx, &D.b, D.n)) ||
           (rc = dalloc(ctx, &D.t, D.n)) || (rc = dalloc(ctx, &D.c32, D.n)) ||
           (rc = dalloc(ctx, &D.b32, D.n)) || (rc = dalloc(ctx, &D.t32, D.n)))
@@ -545,9 +612,52 @@ int ugmg_preprocess(ugmg_ctx *ctx, int level) {
     }
     build_base(ctx->host[bl], p.ex_reorder, ctx->base);
     if ((rc = upload(ctx, &ctx->base_inv, ctx->base.inv.data(), ctx->base.inv.size()))) return rc;
+    // fused coarse levels: the base level and every level above it up to
+    // fuse_rows vectors (and at most kMaxColors colours)
+    ctx->fuse_top = -1;
+    if (fuse_rows > 0 && ctx->dev[bl].n <= 2048) {
+      ctx->fuse_top = bl;
+      for (int l = bl + 1; l <= level; ++l) {
+        if (ctx->dev[l].n > fuse_rows || ctx->dev[l].ncolors > kMaxColors) break;
+        ctx->fuse_top = l;
+      }
+    }
+    ctx->base_b_dead = true;
+    for (uint8_t v : ctx->host[bl].vnc) ctx->base_b_dead &= v >= 2;
+    std::vector<CLevel<double>> c64(std::max(ctx->fuse_top + 1, 1));
+    std::vector<CLevel<float>> c32(std::max(ctx->fuse_top + 1, 1));
+    for (int l = bl; l <= ctx->fuse_top; ++l) {
+      const DevLevel &D = ctx->dev[l];
+      auto fill = [&](auto &C, auto tag) {
+        using T = decltype(tag);
+        std::memset((void *)&C, 0, sizeof(C));
+        C.n = D.n;
+        C.ncolors = D.ncolors;
+        for (int k = 0; k < (int)D.color_slices.size() && k <= kMaxColors; ++k)
+          C.color_slices[k] = D.color_slices[k];
+        C.A = D.A.d<T>();
+        C.L = D.L.d<T>();
+        C.U = D.U.d<T>();
+        C.P = D.P.d<T>();
+        C.PT = D.PT.d<T>();
+        C.diag = D.diag<T>();
+        C.vclass = D.vclass;
+        C.vnclass = D.vnclass;
+        C.skip = D.skip;
+        C.c = ctx->dev[l].cv<T>();
+        C.b = ctx->dev[l].bv<T>();
+        C.t = ctx->dev[l].tv<T>();
+      };
+      fill(c64[l], double());
+      fill(c32[l], float());
+    }
+    if ((rc = upload(ctx, &ctx->clev64, c64.data(), c64.size()))) return rc;
+    if ((rc = upload(ctx, &ctx->clev32, c32.data(), c32.size()))) return rc;
     ctx->part_cap = std::max(1024, (max_slices + 3) / 4 + 1);
     if ((rc = dalloc(ctx, &ctx->part, ctx->part_cap))) return rc;
     if ((rc = dalloc(ctx, &ctx->red, 8))) return rc;
+    if ((rc = dalloc(ctx, &ctx->base_part, 1024))) return rc;
+    if ((rc = dalloc(ctx, &ctx->base_first, 8))) return rc;
   } catch (const std::bad_alloc &) {
     free_device(ctx);
     return fail(ctx, UGMG_E_NOMEM, "host allocation during preprocess");
@@ -621,7 +731,7 @@ int ugmg_dmatmul_minus(ugmg_ctx *ctx, int level, double *x, const double *y) {
   int rc = check_ready(ctx, level);
   if (rc) return rc;
   DevLevel &d = ctx->dev[level];
-  launch_spmv<double, double>(ctx->stream, d.A.d64(), 0, d.A.nslices, y, x, nullptr, 0);
+  launch_spmv<double, double>(ctx->stream, d.A.d64(), 0, d.A.nslices, y, x, nullptr, 0, d.nt);
   HIPCHK(hipGetLastError());
   return UGMG_OK;
 }
@@ -630,7 +740,7 @@ int ugmg_dmatmul(ugmg_ctx *ctx, int level, double *x, const double *y) {
   int rc = check_ready(ctx, level);
   if (rc) return rc;
   DevLevel &d = ctx->dev[level];
-  launch_spmv<double, double>(ctx->stream, d.A.d64(), 0, d.A.nslices, y, x, nullptr, 1);
+  launch_spmv<double, double>(ctx->stream, d.A.d64(), 0, d.A.nslices, y, x, nullptr, 1, d.nt);
   HIPCHK(hipGetLastError());
   return UGMG_OK;
 }
@@ -639,7 +749,7 @@ int ugmg_smatmul_minus(ugmg_ctx *ctx, int level, float *x, const float *y) {
   int rc = check_ready(ctx, level);
   if (rc) return rc;
   DevLevel &d = ctx->dev[level];
-  launch_spmv<float, float>(ctx->stream, d.A.d32(), 0, d.A.nslices, y, x, nullptr, 0);
+  launch_spmv<float, float>(ctx->stream, d.A.d32(), 0, d.A.nslices, y, x, nullptr, 0, d.nt);
   HIPCHK(hipGetLastError());
   return UGMG_OK;
 }
@@ -673,7 +783,7 @@ int ugmg_jacobi_step(ugmg_ctx *ctx, int level, double *x, double *b) {
   if (rc) return rc;
   DevLevel &d = ctx->dev[level];
   launch_jac<double>(ctx->stream, d.n, b, d.diag64, d.vclass, ctx->p.damp[0], x);
-  launch_spmv<double, double>(ctx->stream, d.A.d64(), 0, d.A.nslices, x, b, nullptr, 0);
+  launch_spmv<double, double>(ctx->stream, d.A.d64(), 0, d.A.nslices, x, b, nullptr, 0, d.nt);
   HIPCHK(hipGetLastError());
   return UGMG_OK;
 }
@@ -697,8 +807,8 @@ int ugmg_restrict(ugmg_ctx *ctx, int level, double *to_c, const double *from_f,
   if (rc) return rc;
   if (level <= ctx->bl) return fail(ctx, UGMG_E_ARG, "restrict needs a level above the base level");
   DevLevel &d = ctx->dev[level], &dc = ctx->dev[level - 1];
-  launch_restrict<double>(ctx->stream, dc.n, d.pt_ptr, d.pt_idx, d.pt_w64, dc.skip, dc.vnclass,
-                          from_f, damp ? damp[0] : 1.0, to_c);
+  launch_restrict<double>(ctx->stream, d.PT.d64(), dc.skip, dc.vnclass, dc.vclass, from_f,
+                          damp ? damp[0] : 1.0, to_c, nullptr, d.nt);
   HIPCHK(hipGetLastError());
   return UGMG_OK;
 }
@@ -709,8 +819,8 @@ int ugmg_interpolate(ugmg_ctx *ctx, int level, double *to_f, const double *from_
   if (rc) return rc;
   if (level <= ctx->bl) return fail(ctx, UGMG_E_ARG, "interpolate needs a level above the base level");
   DevLevel &d = ctx->dev[level];
-  launch_interp<double>(ctx->stream, d.n, d.p_ptr, d.p_idx, d.p_w64, d.skip, from_c,
-                        damp ? damp[0] : 1.0, to_f, nullptr);
+  launch_interp<double>(ctx->stream, d.P.d64(), d.skip, from_c, damp ? damp[0] : 1.0, to_f,
+                        nullptr, d.nt);
   HIPCHK(hipGetLastError());
   return UGMG_OK;
 }
@@ -719,7 +829,7 @@ int ugmg_base_solve(ugmg_ctx *ctx, double *c, double *b) {
   int rc = check_ready(ctx, ctx ? ctx->bl : 0);
   if (rc) return rc;
   if (ctx->base.failed) return fail(ctx, UGMG_E_SINGULAR, "zero pivot in the band LU of the base level");
-  base_solve<double>(ctx, c, b, 0);
+  base_solve<double>(ctx, c, b, 0, false, true);
   HIPCHK(hipGetLastError());
   return UGMG_OK;
 }
@@ -734,7 +844,7 @@ int ugmg_iter(ugmg_ctx *ctx, int level, double *c, double *b) {
     const int k = std::max(1, ctx->p.inner_cycles);
     for (int j = 0; j < k; ++j) lmgc<float>(ctx, level, d.c32, d.b32, j == 0 ? 0 : 1);
     launch_convert<double, float>(ctx->stream, d.n, d.c32, c);
-    launch_spmv<double, double>(ctx->stream, d.A.d64(), 0, d.A.nslices, c, b, nullptr, 0);
+    launch_spmv<double, double>(ctx->stream, d.A.d64(), 0, d.A.nslices, c, b, nullptr, 0, d.nt);
   } else {
     lmgc<double>(ctx, level, c, b, 0);
   }
@@ -778,7 +888,7 @@ int ugmg_solve(ugmg_ctx *ctx, int level, double *x, double *b, const double *abs
   }
   DevLevel &d = ctx->dev[level];
   if (do_defect)  // LinearDefect: b -= A x
-    launch_spmv<double, double>(ctx->stream, d.A.d64(), 0, d.A.nslices, x, b, nullptr, 0);
+    launch_spmv<double, double>(ctx->stream, d.A.d64(), 0, d.A.nslices, x, b, nullptr, 0, d.nt);
   double first = 0.0;
   if ((rc = surface_norm(ctx, level, b, &first))) return rc;
   res->first_defect[0] = res->last_defect[0] = first;
@@ -892,8 +1002,8 @@ int ugmg_time_kernel(ugmg_ctx *ctx, int level, int kind, int reps, double *ms, d
   double by = 0.0;
   auto once = [&]() {
     switch (kind) {
-      case 0: launch_spmv<double, double>(st, d.A.d64(), 0, d.A.nslices, d.t, d.b, nullptr, 0); break;
-      case 1: launch_spmv<float, float>(st, d.A.d32(), 0, d.A.nslices, d.t32, d.b32, nullptr, 0); break;
+      case 0: launch_spmv<double, double>(st, d.A.d64(), 0, d.A.nslices, d.t, d.b, nullptr, 0, d.nt); break;
+      case 1: launch_spmv<float, float>(st, d.A.d32(), 0, d.A.nslices, d.t32, d.b32, nullptr, 0, d.nt); break;
       case 2: smooth_step<double>(ctx, level, d.t, d.b, d.c, 1, nullptr); break;
       case 3: {
         const int s = ctx->p.smoother;
@@ -904,12 +1014,13 @@ int ugmg_time_kernel(ugmg_ctx *ctx, int level, int kind, int reps, double *ms, d
       }
       case 4: {
         DevLevel &dc = ctx->dev[level - 1];
-        launch_restrict<double>(st, dc.n, d.pt_ptr, d.pt_idx, d.pt_w64, dc.skip, dc.vnclass, d.b, 1.0, dc.b);
+        launch_restrict<double>(st, d.PT.d64(), dc.skip, dc.vnclass, dc.vclass, d.b, 1.0, dc.b,
+                                nullptr, d.nt);
         break;
       }
       case 5: {
         DevLevel &dc = ctx->dev[level - 1];
-        launch_interp<double>(st, d.n, d.p_ptr, d.p_idx, d.p_w64, d.skip, dc.c, 1.0, d.t, d.c);
+        launch_interp<double>(st, d.P.d64(), d.skip, dc.c, 1.0, d.t, d.c, d.nt);
         break;
       }
       case 6:

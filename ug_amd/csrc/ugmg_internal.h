@@ -47,6 +47,8 @@ struct HostPrep {
   // restriction as a gather: coarse rows of P^T from fine rows with VCLASS >= 2
   std::vector<int> pt_ptr, pt_idx;
   std::vector<double> pt_w;
+  HostSell P;    // interpolation rows (fine vectors)
+  HostSell PT;   // restriction rows (coarse vectors)
 };
 
 // Base level: band LU as in EXPreProcess, then the explicit inverse.
@@ -63,6 +65,7 @@ enum class Part { All, Lower, Upper };
 
 void build_sell(const HostLevel &L, const std::vector<int> *rows, Part part,
                 const std::vector<int> *color, HostSell &out);
+void build_sell_csr(int n, const int *rp, const int *ci, const double *a, HostSell &out);
 int greedy_color(const HostLevel &L, std::vector<int> &color);
 void build_prep(const HostLevel &L, const HostLevel *coarse, bool want_gs,
                 HostPrep &out);

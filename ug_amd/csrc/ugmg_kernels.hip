@@ -1,13 +1,19 @@
 // CDNA4 (gfx950) kernels of the multigrid solve path.
 //
-// All matrix kernels read SELL-64 slabs: one wavefront owns 64 consecutive
-// slots (rows) of a slice and walks the slice's entries column-major, so
-// every value/column load of a wavefront is one contiguous 256/512-byte
-// stretch; the x gathers of neighbouring rows of a stencil hit the same
-// cache lines.  Blocks are 4 wavefronts; block ids are remapped so that each
-// of the 8 XCDs (round-robin dispatch) walks one contiguous range of slices
-// and keeps its share of x in its own L2.  No MFMA: these are sparse
-// contractions at ~0.15 flop/byte, bound by HBM bandwidth.
+// Matrix layout: SELL-64 slabs (ugmg_internal.h).  One wavefront owns the 64
+// slots of a slice and walks the slice's entries column-major, so every
+// value/column load of a wavefront is one contiguous 256/512-byte stretch and
+// the x gathers of neighbouring stencil rows share cache lines.  Row sums are
+// formed in the entries' list order (diagonal first), i.e. UG's order.
+// Loads are issued in chunks of 8 entries (clamped to the row width, so the
+// chunk needs no branch) to keep eight independent memory requests in flight
+// per lane: small levels are latency bound, not bandwidth bound.
+// Streams over the large levels use non-temporal loads so that the matrix
+// does not evict the vectors from L2; small levels use cached loads.
+// Blocks are 4 wavefronts; block ids are remapped so that each of the 8 XCDs
+// (round-robin dispatch) walks one contiguous range of slices and keeps its
+// share of x in its own L2.  No MFMA anywhere: these are sparse contractions
+// at ~0.15 flop/byte, bounded by HBM bandwidth.
 #include <hip/hip_runtime.h>
 
 #include "ugmg_kernels.h"
@@ -21,46 +27,53 @@ __device__ __forceinline__ int xcd_remap(int bid, int nb) {
   return xcd * per + (xcd < rem ? xcd : rem) + idx;
 }
 
-template <typename T>
-__device__ __forceinline__ T ntload(const T *p) {
-  return __builtin_nontemporal_load(p);
+template <int NT, typename T>
+__device__ __forceinline__ T ld(const T *p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
 }
 
-template <typename T> struct Acc { using type = T; };
+// sum_{k<w} val[64 k] * x[col[64 k]], entry order, 8 loads in flight
+template <int NT, typename TV, typename TX>
+__device__ __forceinline__ TX sell_dot(const TV *__restrict__ v, const int *__restrict__ ci,
+                                       int w, const TX *__restrict__ x) {
+  TX sum = 0;
+  for (int k = 0; k < w; k += 8) {
+    TV a[8];
+    int c[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int kk = (k + j < w) ? k + j : w - 1;
+      a[j] = ld<NT>(v + 64 * kk);
+      c[j] = ld<NT>(ci + 64 * kk);
+    }
+    TX xv[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) xv[j] = x[c[j]];
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (k + j < w) sum += (TX)a[j] * xv[j];
+  }
+  return sum;
+}
+
+#define SLICE_PROLOGUE(S, slice0, nsl)                                       \
+  const int bid_ = xcd_remap(blockIdx.x, gridDim.x);                         \
+  const int s = (slice0) + bid_ * (kBlock / kSlice) + (threadIdx.x >> 6);   \
+  if (s >= (slice0) + (nsl)) return;                                         \
+  const int lane = threadIdx.x & 63;                                         \
+  const int r = (S).rows[s * kSlice + lane];                                 \
+  const int e0 = (S).soff[s];                                                \
+  const int w = ((S).soff[s + 1] - e0) >> 6;
 
 // ------------------------------------------------------------------- SpMV
 // MODE: 0 y -= A x, 1 y = A x, 2 y -= A x and c += x, 3 y -= A x and c = x
-template <typename TV, typename TX, int MODE>
-__global__ __launch_bounds__(kBlock) void k_spmv(DSell<TV> A, int slice0,
-                                                  int nsl, const TX *__restrict__ x,
-                                                  TX *__restrict__ y,
+template <typename TV, typename TX, int MODE, int NT>
+__global__ __launch_bounds__(kBlock) void k_spmv(DSell<TV> A, int slice0, int nsl,
+                                                  const TX *__restrict__ x, TX *__restrict__ y,
                                                   TX *__restrict__ c) {
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const int s = slice0 + b * (kBlock / kSlice) + (threadIdx.x >> 6);
-  if (s >= slice0 + nsl) return;
-  const int lane = threadIdx.x & 63;
-  const int r = A.rows[s * kSlice + lane];
-  const int e0 = A.soff[s];
-  const int w = (A.soff[s + 1] - e0) >> 6;
-  const TV *v = A.val + e0 + lane;
-  const int *ci = A.col + e0 + lane;
-  TX sum = 0;
-  int k = 0;
-  for (; k + 4 <= w; k += 4) {
-    const TV a0 = ntload(v), a1 = ntload(v + 64), a2 = ntload(v + 128), a3 = ntload(v + 192);
-    const int c0 = ntload(ci), c1 = ntload(ci + 64), c2 = ntload(ci + 128), c3 = ntload(ci + 192);
-    sum += (TX)a0 * x[c0];
-    sum += (TX)a1 * x[c1];
-    sum += (TX)a2 * x[c2];
-    sum += (TX)a3 * x[c3];
-    v += 256;
-    ci += 256;
-  }
-  for (; k < w; ++k) {
-    sum += (TX)ntload(v) * x[ntload(ci)];
-    v += 64;
-    ci += 64;
-  }
+  SLICE_PROLOGUE(A, slice0, nsl)
+  const TX sum = sell_dot<NT>(A.val + e0 + lane, A.col + e0 + lane, w, x);
   if (r < 0) return;
   if (MODE == 1) {
     y[r] = sum;
@@ -71,35 +84,49 @@ __global__ __launch_bounds__(kBlock) void k_spmv(DSell<TV> A, int slice0,
   }
 }
 
-// FP64 defect of the mixed-precision refinement step:
-//   r -= A c32 (FP64 values, FP64 sums);  x += c32;  r32 = (float) r;
-//   per-block partial sum of r^2 over NEW_DEFECT rows.
+// FP64 defect of a mixed-precision refinement step:
+//   r -= A c32 (FP64 values and sums); x += c32; r32 = (float) r;
+//   per-block partial sums of r^2 over NEW_DEFECT rows.
+template <int NT>
 __global__ __launch_bounds__(kBlock) void k_ir_update(DSell<double> A, int nsl,
                                                        const float *__restrict__ c32,
-                                                       double *__restrict__ r,
+                                                       double *__restrict__ rv,
                                                        double *__restrict__ x,
                                                        float *__restrict__ r32,
                                                        const uint8_t *__restrict__ vclass,
                                                        double *__restrict__ part) {
   __shared__ double red[kBlock / 64];
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const int s = b * (kBlock / kSlice) + (threadIdx.x >> 6);
+  const int bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int s = bid * (kBlock / kSlice) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   double sq = 0.0;
   if (s < nsl) {
-    const int rr = A.rows[s * kSlice + lane];
+    const int r = A.rows[s * kSlice + lane];
     const int e0 = A.soff[s];
     const int w = (A.soff[s + 1] - e0) >> 6;
-    const double *v = A.val + e0 + lane;
-    const int *ci = A.col + e0 + lane;
     double sum = 0.0;
-    for (int k = 0; k < w; ++k) sum += ntload(v + 64 * k) * (double)c32[ntload(ci + 64 * k)];
-    if (rr >= 0) {
-      double nr = r[rr] - sum;
-      r[rr] = nr;
-      x[rr] += (double)c32[rr];
-      r32[rr] = (float)nr;
-      if (vclass[rr] >= 2) sq = nr * nr;
+    for (int k = 0; k < w; k += 8) {
+      double a[8];
+      int c[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int kk = (k + j < w) ? k + j : w - 1;
+        a[j] = ld<NT>(A.val + e0 + lane + 64 * kk);
+        c[j] = ld<NT>(A.col + e0 + lane + 64 * kk);
+      }
+      float xv[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) xv[j] = c32[c[j]];
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (k + j < w) sum += a[j] * (double)xv[j];
+    }
+    if (r >= 0) {
+      const double nr = rv[r] - sum;
+      rv[r] = nr;
+      x[r] += (double)c32[r];
+      r32[r] = (float)nr;
+      if (vclass[r] >= 2) sq = nr * nr;
     }
   }
   for (int off = 32; off > 0; off >>= 1) sq += __shfl_down(sq, off, 64);
@@ -117,8 +144,8 @@ __global__ __launch_bounds__(kBlock) void k_ir_update(DSell<double> A, int nsl,
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_jac(int n, const T *__restrict__ b,
                                                  const T *__restrict__ diag,
-                                                 const uint8_t *__restrict__ vclass,
-                                                 T damp, T *__restrict__ t) {
+                                                 const uint8_t *__restrict__ vclass, T damp,
+                                                 T *__restrict__ t) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   t[i] = vclass[i] < 3 ? (T)0 : (b[i] / diag[i]) * damp;
@@ -126,49 +153,31 @@ __global__ __launch_bounds__(kBlock) void k_jac(int n, const T *__restrict__ b,
 
 // ------------------------------------------------------ multicolour GS
 // forward substitution of one colour: v_r = (b_r - sum_{lower} a v) / a_rr
-template <typename TV, typename T>
+template <typename TV, typename T, int NT>
 __global__ __launch_bounds__(kBlock) void k_gs_color(DSell<TV> L, int slice0, int nsl,
                                                       const T *__restrict__ b,
                                                       const T *__restrict__ diag,
                                                       const uint8_t *__restrict__ vclass,
                                                       T *__restrict__ v) {
-  const int bb = xcd_remap(blockIdx.x, gridDim.x);
-  const int s = slice0 + bb * (kBlock / kSlice) + (threadIdx.x >> 6);
-  if (s >= slice0 + nsl) return;
-  const int lane = threadIdx.x & 63;
-  const int r = L.rows[s * kSlice + lane];
-  const int e0 = L.soff[s];
-  const int w = (L.soff[s + 1] - e0) >> 6;
-  const TV *a = L.val + e0 + lane;
-  const int *ci = L.col + e0 + lane;
-  T sum = 0;
-  for (int k = 0; k < w; ++k) sum += (T)ntload(a + 64 * k) * v[ntload(ci + 64 * k)];
+  SLICE_PROLOGUE(L, slice0, nsl)
+  const T sum = sell_dot<NT>(L.val + e0 + lane, L.col + e0 + lane, w, v);
   if (r < 0) return;
   v[r] = vclass[r] < 3 ? (T)0 : (b[r] - sum) / diag[r];
 }
 
-// defect after the sweep, b -= A (damp v), using (D+L) v = b on active rows:
+// defect after the sweep, b -= A (damp v), with (D+L) v = b on active rows:
 //   active   b <- b - damp (b + U v)
-//   inactive b <- b - damp (A v)      (v_r = 0, U holds the whole row)
+//   inactive b <- b - damp (A v)      (v_r = 0; U holds the whole row)
 // and the correction: CMODE 0 c = damp v, 1 c += damp v; xout = damp v.
-template <typename TV, typename T, int CMODE>
+template <typename TV, typename T, int CMODE, int NT>
 __global__ __launch_bounds__(kBlock) void k_gs_defect(DSell<TV> U, int nsl,
                                                        const T *__restrict__ v,
                                                        T *__restrict__ b,
                                                        const uint8_t *__restrict__ vclass,
                                                        T damp, T *__restrict__ c,
                                                        T *__restrict__ xout) {
-  const int bb = xcd_remap(blockIdx.x, gridDim.x);
-  const int s = bb * (kBlock / kSlice) + (threadIdx.x >> 6);
-  if (s >= nsl) return;
-  const int lane = threadIdx.x & 63;
-  const int r = U.rows[s * kSlice + lane];
-  const int e0 = U.soff[s];
-  const int w = (U.soff[s + 1] - e0) >> 6;
-  const TV *a = U.val + e0 + lane;
-  const int *ci = U.col + e0 + lane;
-  T sum = 0;
-  for (int k = 0; k < w; ++k) sum += (T)ntload(a + 64 * k) * v[ntload(ci + 64 * k)];
+  SLICE_PROLOGUE(U, 0, nsl)
+  const T sum = sell_dot<NT>(U.val + e0 + lane, U.col + e0 + lane, w, v);
   if (r < 0) return;
   const T br = b[r];
   b[r] = vclass[r] >= 3 ? br - damp * (br + sum) : br - damp * sum;
@@ -181,38 +190,58 @@ __global__ __launch_bounds__(kBlock) void k_gs_defect(DSell<TV> U, int nsl,
 }
 
 // --------------------------------------------------------------- transfer
-// StandardRestrictNodeVector as a gather over P^T rows
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_restrict(int nc, const int *__restrict__ ptr,
-                                                      const int *__restrict__ idx,
-                                                      const T *__restrict__ w,
+// StandardRestrictNodeVector as a gather over the slabs of P^T:
+//   bc_I = (VNCLASS_I >= 2 ? 0 : bc_I) + (Dirichlet_I ? 0 : damp sum w b_f)
+// With `part`, also the per-block partial sums of the new bc^2 over NEW_DEFECT
+// vectors (the base solver's first LinearResiduum).
+template <typename T, int NT>
+__global__ __launch_bounds__(kBlock) void k_restrict(DSell<T> PT, int nsl,
                                                       const uint32_t *__restrict__ cskip,
                                                       const uint8_t *__restrict__ cvnclass,
+                                                      const uint8_t *__restrict__ cvclass,
                                                       const T *__restrict__ bf, T damp,
-                                                      T *__restrict__ bc) {
-  const int I = blockIdx.x * kBlock + threadIdx.x;
-  if (I >= nc) return;
-  T acc = cvnclass[I] >= 2 ? (T)0 : bc[I];
-  if (!(cskip[I] & 1u))
-    for (int q = ptr[I]; q < ptr[I + 1]; ++q) acc += w[q] * (damp * bf[idx[q]]);
-  bc[I] = acc;
+                                                      T *__restrict__ bc,
+                                                      double *__restrict__ part) {
+  __shared__ double red[kBlock / 64];
+  const int bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int s = bid * (kBlock / kSlice) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  double sq = 0.0;
+  if (s < nsl) {
+    const int r = PT.rows[s * kSlice + lane];
+    const int e0 = PT.soff[s];
+    const int w = (PT.soff[s + 1] - e0) >> 6;
+    const T sum = sell_dot<NT>(PT.val + e0 + lane, PT.col + e0 + lane, w, bf);
+    if (r >= 0) {
+      const T base = cvnclass[r] >= 2 ? (T)0 : bc[r];
+      const T v = (cskip[r] & 1u) ? base : base + damp * sum;
+      bc[r] = v;
+      if (part && cvclass[r] >= 2) sq = (double)v * (double)v;
+    }
+  }
+  if (!part) return;
+  for (int off = 32; off > 0; off >>= 1) sq += __shfl_down(sq, off, 64);
+  if (lane == 0) red[threadIdx.x >> 6] = sq;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int i = 0; i < kBlock / 64; ++i) t += red[i];
+    part[blockIdx.x] = t;
+  }
 }
 
 // StandardIntCorNodeVector (+ dadd): t = damp P c_c on non-Dirichlet rows
-template <typename T, int CADD>
-__global__ __launch_bounds__(kBlock) void k_interp(int nf, const int *__restrict__ ptr,
-                                                    const int *__restrict__ idx,
-                                                    const T *__restrict__ w,
+template <typename T, int CADD, int NT>
+__global__ __launch_bounds__(kBlock) void k_interp(DSell<T> P, int nsl,
                                                     const uint32_t *__restrict__ fskip,
                                                     const T *__restrict__ cc, T damp,
                                                     T *__restrict__ t, T *__restrict__ c) {
-  const int v = blockIdx.x * kBlock + threadIdx.x;
-  if (v >= nf) return;
-  T acc = 0;
-  if (!(fskip[v] & 1u))
-    for (int q = ptr[v]; q < ptr[v + 1]; ++q) acc += (w[q] * damp) * cc[idx[q]];
-  t[v] = acc;
-  if (CADD) c[v] += acc;
+  SLICE_PROLOGUE(P, 0, nsl)
+  const T sum = sell_dot<NT>(P.val + e0 + lane, P.col + e0 + lane, w, cc);
+  if (r < 0) return;
+  const T tv = (fskip[r] & 1u) ? (T)0 : damp * sum;
+  t[r] = tv;
+  if (CADD) c[r] += tv;
 }
 
 // -------------------------------------------------------------- reductions
@@ -238,7 +267,8 @@ __global__ __launch_bounds__(kBlock) void k_dot_part(int n, const T *__restrict_
   }
 }
 
-__global__ __launch_bounds__(1024) void k_reduce_final(int nparts, const double *__restrict__ part,
+__global__ __launch_bounds__(1024) void k_reduce_final(int nparts,
+                                                        const double *__restrict__ part,
                                                         double *__restrict__ out, int do_sqrt) {
   __shared__ double red[16];
   double s = 0.0;
@@ -260,7 +290,8 @@ __global__ __launch_bounds__(kBlock) void k_fill(int n, T *x, T val) {
   if (i < n) x[i] = val;
 }
 template <typename TD, typename TS>
-__global__ __launch_bounds__(kBlock) void k_convert(int n, const TS *__restrict__ s, TD *__restrict__ d) {
+__global__ __launch_bounds__(kBlock) void k_convert(int n, const TS *__restrict__ s,
+                                                     TD *__restrict__ d) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i < n) d[i] = (TD)s[i];
 }
@@ -270,176 +301,353 @@ __global__ __launch_bounds__(kBlock) void k_add(int n, T *__restrict__ x, const 
   if (i < n) x[i] += y[i];
 }
 
-// ------------------------------------------------------------- base solve
-// ls($m, $red, $abslimit) over ex on the base level, one workgroup:
-//   first = |b|; converged = first < abslimit
-//   repeat m: x = damp * inv b; b -= A x; c += x; test |b|
-// The dense inverse equals the band LU solve of EXApplyLUDOUBLE; every
-// product runs in FP64 whatever T is.
-template <typename T, int CMODE>
-__global__ __launch_bounds__(1024) void k_base_solve(BaseArgs ba, DSell<double> A,
-                                                      const uint8_t *__restrict__ vclass,
-                                                      T *__restrict__ b, T *__restrict__ c) {
-  extern __shared__ double sh[];
-  const int n = ba.n;
-  double *bl = sh;          // n
-  double *xl = sh + n;      // n
-  double *cl = sh + 2 * n;  // n
-  __shared__ double red[16];
-  __shared__ int conv;
-  const int tid = threadIdx.x;
-  for (int i = tid; i < n; i += 1024) {
-    bl[i] = (double)b[i];
-    cl[i] = CMODE ? (double)c[i] : 0.0;
+// ==================================================== fused coarse cycle
+// One workgroup runs Lmgc on every level from `top` down to the base level:
+// each operation is a loop over rows or slabs followed by a workgroup
+// barrier, so the levels below a few thousand vectors cost one launch per
+// cycle instead of dozens of latency-bound launches.
+
+__device__ __forceinline__ double wg_sum(double s, double *red) {
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  double t = 0.0;
+  for (int i = 0; i < kCoarseThreads / 64; ++i) t += red[i];
+  return t;
+}
+
+template <typename TV, typename TX, class Epi>
+__device__ __forceinline__ void wg_sell(const DSell<TV> &S, int s0, int s1,
+                                        const TX *__restrict__ x, Epi epi) {
+  const int nslot = (s1 - s0) * kSlice;
+  for (int slot = threadIdx.x; slot < nslot; slot += kCoarseThreads) {
+    const int s = s0 + (slot >> 6), lane = slot & 63;
+    const int r = S.rows[s * kSlice + lane];
+    const int e0 = S.soff[s], w = (S.soff[s + 1] - e0) >> 6;
+    const TX sum = sell_dot<0>(S.val + e0 + lane, S.col + e0 + lane, w, x);
+    if (r >= 0) epi(r, sum);
   }
   __syncthreads();
-  auto norm = [&]() -> double {
+}
+
+// ls over ex on the base level: first = |b|; converged = first < abslimit;
+// repeat m: x = damp inv b; b -= A x; c += x; stop on abslimit or red.
+template <typename T>
+__device__ void wg_base_solve(const BaseArgs &ba, const DSell<double> &A,
+                              const uint8_t *__restrict__ vclass, T *__restrict__ b,
+                              T *__restrict__ c, int cmode, double *sh, double *red) {
+  const int n = ba.n;
+  double *bl = sh, *xl = sh + n, *cl = sh + 2 * n;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < n; i += kCoarseThreads) {
+    bl[i] = (double)b[i];
+    cl[i] = cmode ? (double)c[i] : 0.0;
+  }
+  __syncthreads();
+  auto norm = [&]() {
     double s = 0.0;
-    for (int i = tid; i < n; i += 1024)
+    for (int i = tid; i < n; i += kCoarseThreads)
       if (vclass[i] >= 2) s += bl[i] * bl[i];
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-    __syncthreads();
-    if ((tid & 63) == 0) red[tid >> 6] = s;
-    __syncthreads();
-    double t = 0.0;
-    for (int i = 0; i < 16; ++i) t += red[i];
-    return sqrt(t);
+    return sqrt(wg_sum(s, red));
   };
   const double first = norm();
   double target = first * ba.red;
   if (target == 0.0) target = ba.red;
-  if (tid == 0) conv = fabs(first) < fabs(ba.abslimit);
-  __syncthreads();
-  for (int it = 0; it < ba.maxiter; ++it) {
-    if (conv) break;
-    if (ba.failed) break;
-    for (int i = tid; i < n; i += 1024) {
+  bool conv = fabs(first) < fabs(ba.abslimit);
+  for (int it = 0; it < ba.maxiter && !conv && !ba.failed; ++it) {
+    // x = damp * inv b: one wavefront per row, lanes across the columns
+    const int wave = tid >> 6, lane = tid & 63;
+    for (int i = wave; i < n; i += kCoarseThreads / 64) {
       const double *row = ba.inv + (size_t)i * n;
       double s = 0.0;
-      for (int j = 0; j < n; ++j) s += row[j] * bl[j];
-      xl[i] = s * ba.damp;
+      for (int j = lane; j < n; j += 64) s += row[j] * bl[j];
+      for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+      if (lane == 0) xl[i] = s * ba.damp;
     }
     __syncthreads();
-    const int nslot = A.nslices * kSlice;
-    for (int slot = tid; slot < nslot; slot += 1024) {
-      const int r = A.rows[slot];
-      if (r < 0) continue;
-      const int s = slot >> 6, lane = slot & 63;
-      const int e0 = A.soff[s], w = (A.soff[s + 1] - e0) >> 6;
-      double sum = 0.0;
-      for (int k = 0; k < w; ++k) sum += A.val[e0 + 64 * k + lane] * xl[A.col[e0 + 64 * k + lane]];
-      bl[r] -= sum;
-    }
-    __syncthreads();
-    for (int i = tid; i < n; i += 1024) cl[i] += xl[i];
+    wg_sell(A, 0, A.nslices, xl, [&](int r, double s) { bl[r] -= s; });
+    for (int i = tid; i < n; i += kCoarseThreads) cl[i] += xl[i];
     const double last = norm();
-    if (tid == 0) conv = fabs(last) < fabs(ba.abslimit) || fabs(last) < fabs(target);
-    __syncthreads();
+    conv = fabs(last) < fabs(ba.abslimit) || fabs(last) < fabs(target);
   }
-  for (int i = tid; i < n; i += 1024) {
+  __syncthreads();
+  for (int i = tid; i < n; i += kCoarseThreads) {
     b[i] = (T)bl[i];
     c[i] = (T)cl[i];
+  }
+  __syncthreads();
+}
+
+template <typename T>
+__device__ void wg_smooth(const CLevel<T> &L, const CycleParams &cp, T *__restrict__ c,
+                          T *__restrict__ b, int cmode) {
+  const T damp = (T)cp.damp;
+  T *__restrict__ t = L.t;
+  if (cp.smoother == 0) {
+    for (int i = threadIdx.x; i < L.n; i += kCoarseThreads)
+      t[i] = L.vclass[i] < 3 ? (T)0 : (b[i] / L.diag[i]) * damp;
+    __syncthreads();
+    wg_sell(L.A, 0, L.A.nslices, t, [&](int r, T s) {
+      b[r] -= s;
+      if (cmode) c[r] += t[r];
+      else c[r] = t[r];
+    });
+  } else {
+    for (int k = 0; k < L.ncolors; ++k)
+      wg_sell(L.L, L.color_slices[k], L.color_slices[k + 1], t, [&](int r, T s) {
+        t[r] = L.vclass[r] < 3 ? (T)0 : (b[r] - s) / L.diag[r];
+      });
+    wg_sell(L.U, 0, L.U.nslices, t, [&](int r, T s) {
+      const T br = b[r];
+      b[r] = L.vclass[r] >= 3 ? br - damp * (br + s) : br - damp * s;
+      const T tt = damp * t[r];
+      if (cmode) c[r] += tt;
+      else c[r] = tt;
+    });
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kCoarseThreads) void k_coarse_cycle(CoarseArgs<T> a) {
+  extern __shared__ double sh[];
+  __shared__ double red[kCoarseThreads / 64];
+  const CycleParams cp = a.cp;
+  int phase[kMaxLevels], gcount[kMaxLevels], cm[kMaxLevels];
+  int l = a.top;
+  phase[l] = 0;
+  cm[l] = a.cmode_top;
+  while (l <= a.top) {
+    const CLevel<T> &L = a.lv[l];
+    T *c = l == a.top ? a.c_top : L.c;
+    T *b = l == a.top ? a.b_top : L.b;
+    if (l <= a.bl) {
+      wg_base_solve<T>(a.base, a.baseA, a.base_vclass, b, c, cm[l], sh, red);
+      ++l;
+      continue;
+    }
+    const CLevel<T> &C = a.lv[l - 1];
+    if (phase[l] == 0) {
+      for (int i = 0; i < cp.nu1; ++i) wg_smooth<T>(L, cp, c, b, i == 0 ? cm[l] : 1);
+      if (cp.nu1 == 0 && cm[l] == 0) {
+        for (int i = threadIdx.x; i < L.n; i += kCoarseThreads) c[i] = (T)0;
+        __syncthreads();
+      }
+      wg_sell(L.PT, 0, L.PT.nslices, b, [&](int I, T s) {
+        const T base = C.vnclass[I] >= 2 ? (T)0 : C.b[I];
+        C.b[I] = (C.skip[I] & 1u) ? base : base + s;
+      });
+      gcount[l] = 0;
+      phase[l] = 1;
+      if (cp.gamma > 0) {
+        --l;
+        phase[l] = 0;
+        cm[l] = 0;
+        continue;
+      }
+      for (int i = threadIdx.x; i < C.n; i += kCoarseThreads) C.c[i] = (T)0;
+      __syncthreads();
+    } else {
+      if (++gcount[l] < cp.gamma) {
+        --l;
+        phase[l] = 0;
+        cm[l] = 1;
+        continue;
+      }
+    }
+    const T ld = (T)cp.lmgc_damp;
+    wg_sell(L.P, 0, L.P.nslices, C.c, [&](int v, T s) {
+      const T tv = (L.skip[v] & 1u) ? (T)0 : ld * s;
+      L.t[v] = tv;
+      c[v] += tv;
+    });
+    wg_sell(L.A, 0, L.A.nslices, L.t, [&](int r, T s) { b[r] -= s; });
+    for (int i = 0; i < cp.nu2; ++i) wg_smooth<T>(L, cp, c, b, 1);
+    if (l == a.top) break;
+    ++l;
   }
 }
 
 // ----------------------------------------------------------------- launchers
+int reduce_parts(int n);
 static inline int nblk(int n) { return (n + kBlock - 1) / kBlock; }
 static inline int nblk_slices(int nsl) { return (nsl + kBlock / kSlice - 1) / (kBlock / kSlice); }
 
+#define NT_DISPATCH(nt, ...) \
+  do {                       \
+    if (nt) {                \
+      constexpr int NTV = 1; \
+      __VA_ARGS__;           \
+    } else {                 \
+      constexpr int NTV = 0; \
+      __VA_ARGS__;           \
+    }                        \
+  } while (0)
+
 template <typename TV, typename TX>
-void launch_spmv(hipStream_t st, const DSell<TV> &A, int slice0, int nsl, const TX *x,
-                 TX *y, TX *c, int mode) {
+void launch_spmv(hipStream_t st, const DSell<TV> &A, int slice0, int nsl, const TX *x, TX *y,
+                 TX *c, int mode, bool nt) {
   if (nsl <= 0) return;
   dim3 g(nblk_slices(nsl)), blk(kBlock);
-  switch (mode) {
-    case 0: k_spmv<TV, TX, 0><<<g, blk, 0, st>>>(A, slice0, nsl, x, y, c); break;
-    case 1: k_spmv<TV, TX, 1><<<g, blk, 0, st>>>(A, slice0, nsl, x, y, c); break;
-    case 2: k_spmv<TV, TX, 2><<<g, blk, 0, st>>>(A, slice0, nsl, x, y, c); break;
-    default: k_spmv<TV, TX, 3><<<g, blk, 0, st>>>(A, slice0, nsl, x, y, c); break;
-  }
+  NT_DISPATCH(nt, switch (mode) {
+    case 0: k_spmv<TV, TX, 0, NTV><<<g, blk, 0, st>>>(A, slice0, nsl, x, y, c); break;
+    case 1: k_spmv<TV, TX, 1, NTV><<<g, blk, 0, st>>>(A, slice0, nsl, x, y, c); break;
+    case 2: k_spmv<TV, TX, 2, NTV><<<g, blk, 0, st>>>(A, slice0, nsl, x, y, c); break;
+    default: k_spmv<TV, TX, 3, NTV><<<g, blk, 0, st>>>(A, slice0, nsl, x, y, c); break;
+  });
 }
 template void launch_spmv<double, double>(hipStream_t, const DSell<double> &, int, int,
-                                          const double *, double *, double *, int);
+                                          const double *, double *, double *, int, bool);
 template void launch_spmv<float, float>(hipStream_t, const DSell<float> &, int, int,
-                                        const float *, float *, float *, int);
+                                        const float *, float *, float *, int, bool);
 
 void launch_ir_update(hipStream_t st, const DSell<double> &A, const float *c32, double *r,
-                      double *x, float *r32, const uint8_t *vclass, double *part, int *nparts) {
+                      double *x, float *r32, const uint8_t *vclass, double *part, int *nparts,
+                      bool nt) {
   const int g = nblk_slices(A.nslices);
   *nparts = g;
   if (g == 0) return;
-  k_ir_update<<<g, kBlock, 0, st>>>(A, A.nslices, c32, r, x, r32, vclass, part);
+  NT_DISPATCH(nt, k_ir_update<NTV><<<g, kBlock, 0, st>>>(A, A.nslices, c32, r, x, r32, vclass, part));
 }
 
 template <typename T>
-void launch_jac(hipStream_t st, int n, const T *b, const T *diag, const uint8_t *vclass,
-                T damp, T *t) {
+void launch_jac(hipStream_t st, int n, const T *b, const T *diag, const uint8_t *vclass, T damp,
+                T *t) {
   if (n <= 0) return;
   k_jac<T><<<nblk(n), kBlock, 0, st>>>(n, b, diag, vclass, damp, t);
 }
 template void launch_jac<double>(hipStream_t, int, const double *, const double *,
                                  const uint8_t *, double, double *);
-template void launch_jac<float>(hipStream_t, int, const float *, const float *,
-                                const uint8_t *, float, float *);
+template void launch_jac<float>(hipStream_t, int, const float *, const float *, const uint8_t *,
+                                float, float *);
 
 template <typename TV, typename T>
 void launch_gs_color(hipStream_t st, const DSell<TV> &L, int slice0, int nsl, const T *b,
-                     const T *diag, const uint8_t *vclass, T *v) {
+                     const T *diag, const uint8_t *vclass, T *v, bool nt) {
   if (nsl <= 0) return;
-  k_gs_color<TV, T><<<nblk_slices(nsl), kBlock, 0, st>>>(L, slice0, nsl, b, diag, vclass, v);
+  NT_DISPATCH(nt, k_gs_color<TV, T, NTV><<<nblk_slices(nsl), kBlock, 0, st>>>(L, slice0, nsl, b,
+                                                                             diag, vclass, v));
 }
 template void launch_gs_color<double, double>(hipStream_t, const DSell<double> &, int, int,
-                                              const double *, const double *,
-                                              const uint8_t *, double *);
+                                              const double *, const double *, const uint8_t *,
+                                              double *, bool);
 template void launch_gs_color<float, float>(hipStream_t, const DSell<float> &, int, int,
                                             const float *, const float *, const uint8_t *,
-                                            float *);
+                                            float *, bool);
 
 template <typename TV, typename T>
-void launch_gs_defect(hipStream_t st, const DSell<TV> &U, const T *v, T *b,
-                      const uint8_t *vclass, T damp, T *c, int cmode, T *xout) {
+void launch_gs_defect(hipStream_t st, const DSell<TV> &U, const T *v, T *b, const uint8_t *vclass,
+                      T damp, T *c, int cmode, T *xout, bool nt) {
   if (U.nslices <= 0) return;
   dim3 g(nblk_slices(U.nslices));
-  if (cmode == 0)
-    k_gs_defect<TV, T, 0><<<g, kBlock, 0, st>>>(U, U.nslices, v, b, vclass, damp, c, xout);
-  else
-    k_gs_defect<TV, T, 1><<<g, kBlock, 0, st>>>(U, U.nslices, v, b, vclass, damp, c, xout);
+  NT_DISPATCH(nt, if (cmode == 0) k_gs_defect<TV, T, 0, NTV><<<g, kBlock, 0, st>>>(
+                      U, U.nslices, v, b, vclass, damp, c, xout);
+              else k_gs_defect<TV, T, 1, NTV><<<g, kBlock, 0, st>>>(U, U.nslices, v, b, vclass,
+                                                                    damp, c, xout));
 }
 template void launch_gs_defect<double, double>(hipStream_t, const DSell<double> &,
-                                               const double *, double *, const uint8_t *,
-                                               double, double *, int, double *);
+                                               const double *, double *, const uint8_t *, double,
+                                               double *, int, double *, bool);
 template void launch_gs_defect<float, float>(hipStream_t, const DSell<float> &, const float *,
                                              float *, const uint8_t *, float, float *, int,
-                                             float *);
+                                             float *, bool);
 
 template <typename T>
-void launch_restrict(hipStream_t st, int nc, const int *ptr, const int *idx, const T *w,
-                     const uint32_t *cskip, const uint8_t *cvnclass, const T *bf, T damp,
-                     T *bc) {
-  if (nc <= 0) return;
-  k_restrict<T><<<nblk(nc), kBlock, 0, st>>>(nc, ptr, idx, w, cskip, cvnclass, bf, damp, bc);
+int launch_restrict(hipStream_t st, const DSell<T> &PT, const uint32_t *cskip,
+                    const uint8_t *cvnclass, const uint8_t *cvclass, const T *bf, T damp, T *bc,
+                    double *part, bool nt) {
+  if (PT.nslices <= 0) return 0;
+  const int g = nblk_slices(PT.nslices);
+  NT_DISPATCH(nt, k_restrict<T, NTV><<<g, kBlock, 0, st>>>(PT, PT.nslices, cskip, cvnclass,
+                                                          cvclass, bf, damp, bc, part));
+  return g;
 }
-template void launch_restrict<double>(hipStream_t, int, const int *, const int *,
-                                      const double *, const uint32_t *, const uint8_t *,
-                                      const double *, double, double *);
-template void launch_restrict<float>(hipStream_t, int, const int *, const int *, const float *,
-                                     const uint32_t *, const uint8_t *, const float *, float,
-                                     float *);
+template int launch_restrict<double>(hipStream_t, const DSell<double> &, const uint32_t *,
+                                     const uint8_t *, const uint8_t *, const double *, double,
+                                     double *, double *, bool);
+template int launch_restrict<float>(hipStream_t, const DSell<float> &, const uint32_t *,
+                                    const uint8_t *, const uint8_t *, const float *, float,
+                                    float *, double *, bool);
+
+// --------------------------------------------------- base solve (GEMV)
+// ls over ex on the base level, one wavefront per row of the dense inverse:
+//   cur = |b| from the block partials of the defect norm;
+//   converged: cur < abslimit (it == 0), cur < abslimit or cur < red*first;
+//   x = converged ? 0 : damp * inv b;  c (+)= x;  xout = x.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_base_gemv(int n, const double *__restrict__ inv,
+                                                       const T *__restrict__ b,
+                                                       T *__restrict__ c, T *__restrict__ xout,
+                                                       int cmode, double damp, double abslimit,
+                                                       double red, const double *__restrict__ part,
+                                                       int nparts, double *__restrict__ first,
+                                                       int it) {
+  const int row = (blockIdx.x * kBlock + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  double s = 0.0;
+  for (int i = lane; i < nparts; i += 64) s += part[i];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  const double cur = sqrt(s);
+  bool conv;
+  if (it == 0) {
+    conv = fabs(cur) < fabs(abslimit);
+    if (row == 0 && lane == 0) *first = cur;
+  } else {
+    double target = *first * red;
+    if (target == 0.0) target = red;
+    conv = fabs(cur) < fabs(abslimit) || fabs(cur) < fabs(target);
+  }
+  if (row >= n) return;
+  double x = 0.0;
+  if (!conv) {
+    const double *__restrict__ rr = inv + (size_t)row * n;
+    for (int j = lane; j < n; j += 64) x += rr[j] * (double)b[j];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    x *= damp;
+  }
+  if (lane == 0) {
+    const T xv = (T)x;
+    if (!conv || cmode == 0) c[row] = cmode ? c[row] + xv : xv;
+    if (xout) xout[row] = xv;
+  }
+}
 
 template <typename T>
-void launch_interp(hipStream_t st, int nf, const int *ptr, const int *idx, const T *w,
-                   const uint32_t *fskip, const T *cc, T damp, T *t, T *c) {
-  if (nf <= 0) return;
-  if (c)
-    k_interp<T, 1><<<nblk(nf), kBlock, 0, st>>>(nf, ptr, idx, w, fskip, cc, damp, t, c);
-  else
-    k_interp<T, 0><<<nblk(nf), kBlock, 0, st>>>(nf, ptr, idx, w, fskip, cc, damp, t, c);
+void launch_base_gemv(hipStream_t st, int n, const double *inv, const T *b, T *c, T *xout,
+                      int cmode, double damp, double abslimit, double red, const double *part,
+                      int nparts, double *first, int it) {
+  const int g = (n * 64 + kBlock - 1) / kBlock;
+  k_base_gemv<T><<<g, kBlock, 0, st>>>(n, inv, b, c, xout, cmode, damp, abslimit, red, part,
+                                       nparts, first, it);
 }
-template void launch_interp<double>(hipStream_t, int, const int *, const int *, const double *,
-                                    const uint32_t *, const double *, double, double *,
-                                    double *);
-template void launch_interp<float>(hipStream_t, int, const int *, const int *, const float *,
-                                   const uint32_t *, const float *, float, float *, float *);
+template void launch_base_gemv<double>(hipStream_t, int, const double *, const double *,
+                                       double *, double *, int, double, double, double,
+                                       const double *, int, double *, int);
+template void launch_base_gemv<float>(hipStream_t, int, const double *, const float *, float *,
+                                      float *, int, double, double, double, const double *, int,
+                                      double *, int);
+
+template <typename T>
+int launch_dot_part(hipStream_t st, int n, const T *x, const uint8_t *vclass, double *part) {
+  const int g = reduce_parts(n);
+  k_dot_part<T, 0><<<g, kBlock, 0, st>>>(n, x, nullptr, vclass, part);
+  return g;
+}
+
+template <typename T>
+void launch_interp(hipStream_t st, const DSell<T> &P, const uint32_t *fskip, const T *cc, T damp,
+                   T *t, T *c, bool nt) {
+  if (P.nslices <= 0) return;
+  dim3 g(nblk_slices(P.nslices));
+  NT_DISPATCH(nt, if (c) k_interp<T, 1, NTV><<<g, kBlock, 0, st>>>(P, P.nslices, fskip, cc,
+                                                                   damp, t, c);
+              else k_interp<T, 0, NTV><<<g, kBlock, 0, st>>>(P, P.nslices, fskip, cc, damp, t,
+                                                             c));
+}
+template void launch_interp<double>(hipStream_t, const DSell<double> &, const uint32_t *,
+                                    const double *, double, double *, double *, bool);
+template void launch_interp<float>(hipStream_t, const DSell<float> &, const uint32_t *,
+                                   const float *, float, float *, float *, bool);
 
 int reduce_parts(int n) {
   int g = nblk(n);
@@ -450,16 +658,16 @@ template <typename T>
 void launch_dot(hipStream_t st, int n, const T *x, const T *y, const uint8_t *vclass,
                 double *part, double *out, int do_sqrt) {
   const int g = reduce_parts(n);
-  if (y)
-    k_dot_part<T, 1><<<g, kBlock, 0, st>>>(n, x, y, vclass, part);
-  else
-    k_dot_part<T, 0><<<g, kBlock, 0, st>>>(n, x, y, vclass, part);
+  if (y) k_dot_part<T, 1><<<g, kBlock, 0, st>>>(n, x, y, vclass, part);
+  else k_dot_part<T, 0><<<g, kBlock, 0, st>>>(n, x, y, vclass, part);
   k_reduce_final<<<1, 1024, 0, st>>>(g, part, out, do_sqrt);
 }
 template void launch_dot<double>(hipStream_t, int, const double *, const double *,
                                  const uint8_t *, double *, double *, int);
-template void launch_dot<float>(hipStream_t, int, const float *, const float *,
-                                const uint8_t *, double *, double *, int);
+template void launch_dot<float>(hipStream_t, int, const float *, const float *, const uint8_t *,
+                                double *, double *, int);
+template int launch_dot_part<double>(hipStream_t, int, const double *, const uint8_t *, double *);
+template int launch_dot_part<float>(hipStream_t, int, const float *, const uint8_t *, double *);
 
 void launch_reduce_final(hipStream_t st, int nparts, const double *part, double *out,
                          int do_sqrt) {
@@ -488,17 +696,11 @@ template void launch_add<double>(hipStream_t, int, double *, const double *);
 template void launch_add<float>(hipStream_t, int, float *, const float *);
 
 template <typename T>
-void launch_base_solve(hipStream_t st, const BaseArgs &ba, const DSell<double> &A,
-                       const uint8_t *vclass, T *b, T *c, int cmode) {
-  const size_t shmem = 3 * (size_t)ba.n * sizeof(double);
-  if (cmode)
-    k_base_solve<T, 1><<<1, 1024, shmem, st>>>(ba, A, vclass, b, c);
-  else
-    k_base_solve<T, 0><<<1, 1024, shmem, st>>>(ba, A, vclass, b, c);
+void launch_coarse_cycle(hipStream_t st, const CoarseArgs<T> &args, int base_n) {
+  const size_t shmem = 3 * (size_t)base_n * sizeof(double);
+  k_coarse_cycle<T><<<1, kCoarseThreads, shmem, st>>>(args);
 }
-template void launch_base_solve<double>(hipStream_t, const BaseArgs &, const DSell<double> &,
-                                        const uint8_t *, double *, double *, int);
-template void launch_base_solve<float>(hipStream_t, const BaseArgs &, const DSell<double> &,
-                                       const uint8_t *, float *, float *, int);
+template void launch_coarse_cycle<double>(hipStream_t, const CoarseArgs<double> &, int);
+template void launch_coarse_cycle<float>(hipStream_t, const CoarseArgs<float> &, int);
 
 }  // namespace ugmg

@@ -22,36 +22,28 @@
 
 namespace ugmg {
 
-void build_sell(const HostLevel &L, const std::vector<int> *rows, Part part,
-                const std::vector<int> *color, HostSell &out) {
-  const int nr = rows ? (int)rows->size() : L.n;
+// SELL-64 slabs of the rows `rows` (or 0..n-1) of a CSR matrix, keeping the
+// entries for which keep(row, entry) holds, in CSR order.
+template <class Keep>
+static void sell_from_csr(int n, const int *rp, const int *ci, const double *a,
+                          const std::vector<int> *rows, Keep keep, HostSell &out) {
+  const int nr = rows ? (int)rows->size() : n;
   out.nrows = nr;
   out.identity_rows = rows == nullptr;
   out.nslices = (nr + kSlice - 1) / kSlice;
   out.soff.assign(out.nslices + 1, 0);
   out.rows.assign((size_t)out.nslices * kSlice, -1);
-  auto keep = [&](int r, int m) {
-    if (part == Part::All) return true;
-    int w = L.ci[m];
-    if (w == r) return false;
-    int cr = (*color)[r], cw = (*color)[w];
-    if (part == Part::Lower) return cw < cr && L.vc[w] >= 3;
-    // Upper: higher colours for active rows, every off-diagonal entry for
-    // rows that the smoother leaves at zero (VCLASS < 3)
-    if (L.vc[r] < 3) return true;
-    return cw > cr;
-  };
   std::vector<int> width(out.nslices, 0);
   for (int s = 0; s < out.nslices; ++s) {
     int wmax = 0;
     for (int lane = 0; lane < kSlice; ++lane) {
-      int slot = s * kSlice + lane;
+      const int slot = s * kSlice + lane;
       if (slot >= nr) break;
-      int r = rows ? (*rows)[slot] : slot;
+      const int r = rows ? (*rows)[slot] : slot;
       out.rows[slot] = r;
       if (r < 0) continue;  // padding slot
       int cnt = 0;
-      for (int m = L.rp[r]; m < L.rp[r + 1]; ++m) cnt += keep(r, m);
+      for (int m = rp[r]; m < rp[r + 1]; ++m) cnt += keep(r, m) ? 1 : 0;
       wmax = std::max(wmax, cnt);
     }
     width[s] = wmax;
@@ -62,27 +54,48 @@ void build_sell(const HostLevel &L, const std::vector<int> *rows, Part part,
   out.val.assign(nent, 0.0);
   for (int s = 0; s < out.nslices; ++s) {
     for (int lane = 0; lane < kSlice; ++lane) {
-      int slot = s * kSlice + lane;
-      int r = slot < nr ? out.rows[slot] : -1;
+      const int slot = s * kSlice + lane;
+      const int r = slot < nr ? out.rows[slot] : -1;
       int k = 0;
+      int last = 0;
       if (r >= 0) {
-        for (int m = L.rp[r]; m < L.rp[r + 1]; ++m) {
+        for (int m = rp[r]; m < rp[r + 1]; ++m) {
           if (!keep(r, m)) continue;
-          size_t e = (size_t)out.soff[s] + (size_t)k * kSlice + lane;
-          out.col[e] = L.ci[m];
-          out.val[e] = L.a[m];
+          const size_t e = (size_t)out.soff[s] + (size_t)k * kSlice + lane;
+          out.col[e] = ci[m];
+          out.val[e] = a[m];
+          last = ci[m];
           ++k;
         }
       }
-      // padding: value 0 against a valid column
-      int padcol = r >= 0 ? r : 0;
+      // padding: value 0 against the row's last column (a line already loaded)
       for (; k < width[s]; ++k) {
-        size_t e = (size_t)out.soff[s] + (size_t)k * kSlice + lane;
-        out.col[e] = padcol;
+        const size_t e = (size_t)out.soff[s] + (size_t)k * kSlice + lane;
+        out.col[e] = last;
         out.val[e] = 0.0;
       }
     }
   }
+}
+
+void build_sell(const HostLevel &L, const std::vector<int> *rows, Part part,
+                const std::vector<int> *color, HostSell &out) {
+  auto keep = [&](int r, int m) {
+    if (part == Part::All) return true;
+    const int w = L.ci[m];
+    if (w == r) return false;
+    const int cr = (*color)[r], cw = (*color)[w];
+    if (part == Part::Lower) return cw < cr && L.vc[w] >= 3;
+    // Upper: higher colours for active rows, every off-diagonal entry for
+    // rows that the smoother leaves at zero (VCLASS < 3)
+    if (L.vc[r] < 3) return true;
+    return cw > cr;
+  };
+  sell_from_csr(L.n, L.rp.data(), L.ci.data(), L.a.data(), rows, keep, out);
+}
+
+void build_sell_csr(int n, const int *rp, const int *ci, const double *a, HostSell &out) {
+  sell_from_csr(n, rp, ci, a, nullptr, [](int, int) { return true; }, out);
 }
 
 int greedy_color(const HostLevel &L, std::vector<int> &color) {
@@ -165,6 +178,8 @@ void build_prep(const HostLevel &L, const HostLevel *coarse, bool want_gs,
         ++fill[c];
       }
     }
+    build_sell_csr(L.n, L.pp.data(), L.pi.data(), L.pw.data(), out.P);
+    build_sell_csr(nc, out.pt_ptr.data(), out.pt_idx.data(), out.pt_w.data(), out.PT);
   }
 }
 
@@ -229,18 +244,25 @@ void build_base(const HostLevel &L, int reorder, HostBase &out) {
   }
   out.inv.assign((size_t)n * n, 0.0);
   if (out.failed) return;
-  std::vector<double> vec(n);
-  for (int j = 0; j < n; ++j) {
-    std::fill(vec.begin(), vec.end(), 0.0);
-    vec[index[j]] = 1.0;
-    for (int i = 1; i < n; ++i)
-      for (int k = std::max(i - bw, 0); k < i; ++k) vec[i] -= EXM(band.data(), bw, i, k) * vec[k];
-    for (int i = n - 1; i >= 0; --i) {
-      int kmax = std::min(i + bw, n - 1);
-      for (int k = i + 1; k <= kmax; ++k) vec[i] -= EXM(band.data(), bw, i, k) * vec[k];
-      vec[i] /= EXM(band.data(), bw, i, i);
+  // column j of the inverse: EXApplyLUDOUBLE on the unit vector e_j (the
+  // forward substitution starts at the unit's position)
+#pragma omp parallel
+  {
+    std::vector<double> vec(n);
+#pragma omp for schedule(dynamic, 16)
+    for (int j = 0; j < n; ++j) {
+      std::fill(vec.begin(), vec.end(), 0.0);
+      const int p0 = index[j];
+      vec[p0] = 1.0;
+      for (int i = p0 + 1; i < n; ++i)
+        for (int k = std::max(i - bw, p0); k < i; ++k) vec[i] -= EXM(band.data(), bw, i, k) * vec[k];
+      for (int i = n - 1; i >= 0; --i) {
+        int kmax = std::min(i + bw, n - 1);
+        for (int k = i + 1; k <= kmax; ++k) vec[i] -= EXM(band.data(), bw, i, k) * vec[k];
+        vec[i] /= EXM(band.data(), bw, i, i);
+      }
+      for (int k = 0; k < n; ++k) out.inv[(size_t)out.order[k] * n + j] = vec[k];
     }
-    for (int k = 0; k < n; ++k) out.inv[(size_t)out.order[k] * n + j] = vec[k];
   }
 }
 

@@ -53,6 +53,74 @@ def load_traffic(name):
         return None
 
 
+def serving_run(n_req, rate, seed, budget_gb, slots):
+    """Solve-serving stream: Poisson arrivals of 2D Q1 Poisson and 3D Q1
+    jump-coefficient (kappa = 10^U[-2,2]) solves with unknowns log-uniform in
+    [1e3, 1e6] (the largest uniformly refined level not above the drawn
+    size), relative defect 1e-8, FP64 MC-GS V(2,2)."""
+    import numpy as np
+    from ug_amd import problems, ugmg
+
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i in range(n_req):
+        dim = 2 if rng.random() < 0.5 else 3
+        target = 10.0 ** rng.uniform(3.0, 6.0)
+        cells = 2 if dim == 2 else 1
+        size = lambda r: (cells * 2 ** r + 1) ** dim
+        nref = 1
+        while size(nref + 1) <= target:
+            nref += 1
+        while size(nref) < 1000:
+            nref += 1
+        kappa = problems.KAPPA_ONE if dim == 2 else problems.KAPPA_JUMP
+        P = problems.q1(dim, cells, nref, kappa, seed=int(seed * 1000 + i))
+        # base level: the largest level with at most 729 vectors
+        bl = max(l for l in range(P.top + 1) if P.levels[l].n <= 729)
+        prm = ugmg.default_params(nu1=2, nu2=2, gamma=1, baselevel=bl, smoother=ugmg.SMOOTH_GS, damp=1.0,
+                                  base_abslimit=1e-300, use_graph=1)
+        reqs.append((P, prm))
+    gaps = rng.exponential(1.0 / rate, n_req)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    srv = ugmg.Server(0, budget_bytes=budget_gb * 1e9, max_slots=slots)
+    ids = []
+    t0 = time.perf_counter()
+    k = 0
+    while True:
+        now = time.perf_counter() - t0
+        while k < n_req and arrivals[k] <= now:
+            P, prm = reqs[k]
+            ids.append(srv.submit(P, prm, red=1e-8, maxiter=500))
+            k += 1
+        a, q, d = srv.step()
+        if k == n_req and d >= n_req:
+            break
+        if a == 0 and k < n_req:
+            time.sleep(min(0.001, max(0.0, arrivals[k] - now)))
+    lat, gpu, ok = [], [], 0
+    t_first, t_last = None, 0.0
+    for rid in ids:
+        st, res, (ts, ta, td) = srv.status(rid)
+        if st == ugmg.REQ_DONE and res.converged:
+            ok += 1
+        lat.append(td - ts)
+        gpu.append(td - ta)
+        t_first = ts if t_first is None else min(t_first, ts)
+        t_last = max(t_last, td)
+    srv.close()
+    sizes = [r[0].levels[-1].n for r in reqs]
+    return {
+        "requests": n_req, "converged": ok, "arrival_rate_per_s": rate, "slots": slots,
+        "hbm_budget_gb": budget_gb,
+        "unknowns_min_median_max": [int(min(sizes)), int(np.median(sizes)), int(max(sizes))],
+        "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
+        "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2),
+        "admitted_to_done_p50_ms": round(float(np.percentile(gpu, 50)) * 1e3, 2),
+        "solves_per_s": round(n_req / (t_last - t_first), 2),
+        "note": "latency = completion - submission, including the server's host layout build",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -62,6 +130,10 @@ def main():
     ap.add_argument("--inner", type=int, default=3, help="FP32 V-cycles per FP64 refinement step")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--cpu-cycles", type=int, default=2)
+    ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--serve-rate", type=float, default=10.0, help="Poisson arrival rate (1/s)")
+    ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--kernels-only", action="store_true",
                     help="run only the roofline kernels and a copy kernel of known size (for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -202,6 +274,9 @@ def main():
                                "avg_launch_us": round(ms_spmv32 * 1e3, 2)},
         "setup_s": {"generate": round(gen_s, 2)},
     }
+
+    if not args.no_serve:
+        out["serving"] = serving_run(args.serve_requests, args.serve_rate, 0x5EED0005, 32.0, args.serve_slots)
 
     if not args.no_cpu:
         from oracle import oracle as O

@@ -169,6 +169,39 @@ int ugmg_coarse_inverse(int n, const int *row_ptr, const int *col_idx,
                         const double *vals, int reorder, double *inv,
                         int *order, int *bandwidth);
 
+/* ---- serving: continuous batching of independent solves ----------------
+ * No counterpart in UG (its ugd daemon is a remote UI, ugd.cc:528-608).
+ * Each request is a whole hierarchy + rhs; the server builds its layouts on
+ * a worker thread, admits it (FIFO) when a slot and the HBM budget allow,
+ * and keeps one ls iteration of every admitted request in flight on that
+ * request's own stream.  The caller drives progress with ugmg_server_step;
+ * results equal those of a stand-alone ugmg_solve of the same request. */
+typedef struct {
+  int n;
+  const int *row_ptr, *col_idx;
+  const double *vals;
+  const uint32_t *skip;
+  const uint8_t *vclass, *vnclass;
+  const int *p_ptr, *p_idx;        /* interpolation from level-1 (level >= 1) */
+  const double *p_w;
+} ugmg_level_desc;
+
+enum { UGMG_REQ_QUEUED = 0, UGMG_REQ_ACTIVE = 1, UGMG_REQ_DONE = 2, UGMG_REQ_FAILED = 3 };
+
+typedef struct ugmg_server ugmg_server;
+int ugmg_server_create(int device, double hbm_budget_bytes, int max_slots,
+                       ugmg_server **out);
+int ugmg_server_destroy(ugmg_server *s);
+const char *ugmg_server_last_error(const ugmg_server *s);
+int ugmg_server_submit(ugmg_server *s, int nlevels, const ugmg_level_desc *levels,
+                       const ugmg_params *p, const double *rhs, double red,
+                       double abslimit, int maxiter, int *id);
+int ugmg_server_step(ugmg_server *s, int *nactive, int *nqueued, int *ndone);
+/* times[3]: submit, admission, completion in seconds since creation */
+int ugmg_server_status(ugmg_server *s, int id, int *state, ugmg_lresult *res,
+                       double *times);
+int ugmg_server_result(ugmg_server *s, int id, double *x, int n);
+
 /* ---- measurement --------------------------------------------------------
  * kind: 0 dmatmul_minus FP64, 1 smatmul_minus FP32, 2 jacobi step FP64,
  *       3 gs step FP64, 4 restrict, 5 interpolate, 6 one cycle (current

@@ -353,3 +353,46 @@ def test_fused_coarse_cycle_matches(poisson2d, monkeypatch):
         assert res.number_of_linear_iterations == want["nit"], (gamma, sm)
         assert np.max(np.abs(d - want["defects"][:, 0]) / want["defects"][:, 0]) < 1e-10
         ctx.close()
+
+
+def test_server_results_equal_standalone_solves():
+    """Continuous batching: four requests of different sizes and kinds
+    admitted together (two slots, so two join while others run) give exactly
+    the solutions of stand-alone solves."""
+    from ug_amd import problems, ugmg
+    probs = [problems.q1(2, 2, 5, problems.KAPPA_ONE, seed=1), problems.jump3d(4, seed=2),
+             problems.q1(2, 2, 6, problems.KAPPA_ONE, seed=3), problems.jump3d(5, seed=4)]
+    kw = dict(nu1=2, nu2=2, gamma=1, baselevel=1, smoother=ugmg.SMOOTH_GS, damp=1.0, base_abslimit=1e-300,
+              use_graph=1)
+    alone = []
+    for P in probs:
+        ctx = ugmg.Context(0)
+        ctx.set_hierarchy(P)
+        ctx.set_params(**kw)
+        ctx.preprocess()
+        x, b, res, d = ctx.solve_host(P.top, P.rhs, red=1e-8, maxiter=200)
+        assert res.converged
+        alone.append((x, res.number_of_linear_iterations))
+        ctx.close()
+    srv = ugmg.Server(0, budget_bytes=8e9, max_slots=2)
+    ids = [srv.submit(P, ugmg.default_params(**kw), red=1e-8, maxiter=200) for P in probs]
+    srv.drain(len(ids))
+    for P, rid, (xa, nit) in zip(probs, ids, alone):
+        st, res, times = srv.status(rid)
+        assert st == ugmg.REQ_DONE and res.converged
+        assert res.number_of_linear_iterations == nit
+        assert times[0] <= times[1] <= times[2]
+        xs = srv.result(rid, P.levels[-1].n)
+        assert np.array_equal(xs, xa)
+    srv.close()
+
+
+def test_server_budget_rejects_oversized_request():
+    from ug_amd import problems, ugmg
+    P = problems.jump3d(4, seed=5)
+    srv = ugmg.Server(0, budget_bytes=1e5, max_slots=2)
+    rid = srv.submit(P, ugmg.default_params(baselevel=1, base_abslimit=1e-300), red=1e-8)
+    srv.drain(1)
+    st, res, _ = srv.status(rid)
+    assert st == ugmg.REQ_FAILED and res.error_code == -6
+    srv.close()

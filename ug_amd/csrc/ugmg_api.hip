@@ -85,6 +85,7 @@ struct ugmg_ctx {
   int top = -1, bl = 0;
   std::vector<DevLevel> dev;
   std::vector<void *> allocs;
+  double dev_bytes = 0.0;
   HostBase base;
   double *base_inv = nullptr;
   double *part = nullptr;
@@ -128,6 +129,7 @@ int dalloc(ugmg_ctx *ctx, T **out, size_t n) {
   hipError_t e = hipMalloc(&p, n * sizeof(T));
   if (e != hipSuccess) return fail(ctx, UGMG_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
   ctx->allocs.push_back(p);
+  ctx->dev_bytes += (double)n * sizeof(T);
   *out = (T *)p;
   return 0;
 }
@@ -148,7 +150,8 @@ int upload_sell(ugmg_ctx *ctx, const HostSell &S, DevSellBuf &D) {
   if ((rc = upload(ctx, &D.rows, S.rows.data(), S.rows.size()))) return rc;
   if ((rc = upload(ctx, &D.col, S.col.data(), S.col.size()))) return rc;
   if ((rc = upload(ctx, &D.v64, S.val.data(), S.val.size()))) return rc;
-  std::vector<float> v32(S.val.begin(), S.val.end());
+  std::vector<float> v32;
+  to_float(S.val, v32);
   if ((rc = upload(ctx, &D.v32, v32.data(), v32.size()))) return rc;
   return 0;
 }
@@ -161,6 +164,7 @@ void free_device(ugmg_ctx *ctx) {
   ctx->gkind = -1;
   for (void *p : ctx->allocs) hipFree(p);
   ctx->allocs.clear();
+  ctx->dev_bytes = 0.0;
   ctx->dev.clear();
   ctx->base_inv = nullptr;
   ctx->clev64 = nullptr;
@@ -334,7 +338,9 @@ void enqueue_iteration(ugmg_ctx *ctx, int level, double *x, double *b) {
 
 bool can_capture(ugmg_ctx *ctx) { return ctx->p.use_graph && ctx->stream != nullptr; }
 
-int run_iteration(ugmg_ctx *ctx, int level, double *x, double *b) {
+// enqueue one solver iteration (from the cached graph when enabled) and the
+// asynchronous copy of its defect norm to ctx->red_host[0]
+int launch_iteration(ugmg_ctx *ctx, int level, double *x, double *b) {
   if (!can_capture(ctx)) {
     enqueue_iteration(ctx, level, x, b);
     HIPCHK(hipGetLastError());
@@ -361,6 +367,12 @@ int run_iteration(ugmg_ctx *ctx, int level, double *x, double *b) {
     HIPCHK(hipGraphLaunch(ctx->gexec, ctx->stream));
   }
   HIPCHK(hipMemcpyAsync(ctx->red_host, ctx->red, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  return 0;
+}
+
+int run_iteration(ugmg_ctx *ctx, int level, double *x, double *b) {
+  int rc = launch_iteration(ctx, level, x, b);
+  if (rc) return rc;
   HIPCHK(hipStreamSynchronize(ctx->stream));
   return 0;
 }
@@ -372,6 +384,156 @@ double spmv_bytes(const DevLevel &d, int sv, int sx) {
 }
 
 }  // namespace
+
+// Host half of LmgcPreProcess (CPU only, no HIP call; safe on a worker
+// thread while the context is not otherwise used).
+struct PrepState {
+  int level = -1, bl = 0, fuse_rows = 0;
+  double nt_bytes = 64.0e6;
+  std::vector<HostPrep> hp;
+  HostBase base;
+};
+
+int prep_host(ugmg_ctx *ctx, int level, PrepState &ps) {
+  if (ctx->nlevels <= 0) return fail(ctx, UGMG_E_STATE, "ugmg_set_nlevels first");
+  if (level < 0 || level >= ctx->nlevels) return fail(ctx, UGMG_E_ARG, "level out of range");
+  if (!ctx->params_set) return fail(ctx, UGMG_E_STATE, "ugmg_set_params first");
+  const ugmg_params &p = ctx->p;
+  int bl = p.baselevel;
+  if (bl < 0) {  // LmgcInit: deepest level with at most -baselevel vectors
+    int i;
+    for (i = level; i > 0; --i)
+      if (ctx->level_set[i] && ctx->host[i].n <= -bl) break;
+    bl = i;
+  }
+  bl = std::min(bl, level);
+  for (int l = bl; l <= level; ++l) {
+    if (!ctx->level_set[l]) return fail(ctx, UGMG_E_STATE, "level " + std::to_string(l) + " not set");
+    if (l > bl && !ctx->transfer_set[l])
+      return fail(ctx, UGMG_E_STATE, "transfer of level " + std::to_string(l) + " not set");
+  }
+  if (ctx->host[bl].n > 8192)
+    return fail(ctx, UGMG_E_ARG, "base level has more than 8192 vectors; choose a coarser baselevel");
+  const char *env_nt = std::getenv("UGMG_NT_BYTES");
+  ps.nt_bytes = env_nt ? std::atof(env_nt) : 64.0e6;
+  const char *env_fuse = std::getenv("UGMG_FUSE_ROWS");
+  ps.fuse_rows = env_fuse ? std::atoi(env_fuse) : 0;
+  ps.level = level;
+  ps.bl = bl;
+  try {
+    ps.hp.assign(level + 1, HostPrep());
+    for (int l = bl; l <= level; ++l) {
+      const bool gs = p.smoother == UGMG_SMOOTH_GS && l > bl;
+      build_prep(ctx->host[l], l > bl ? &ctx->host[l - 1] : nullptr, gs, ps.hp[l]);
+    }
+    build_base(ctx->host[bl], p.ex_reorder, ps.base);
+  } catch (const std::bad_alloc &) {
+    return fail(ctx, UGMG_E_NOMEM, "host allocation during preprocess");
+  }
+  return UGMG_OK;
+}
+
+// Device half: uploads the layouts built by prep_host.
+int prep_device(ugmg_ctx *ctx, PrepState &ps) {
+  free_device(ctx);
+  const int level = ps.level, bl = ps.bl;
+  try {
+    ctx->dev.assign(level + 1, DevLevel());
+    ctx->bl = bl;
+    int rc;
+    int max_slices = 0;
+    for (int l = bl; l <= level; ++l) {
+      const HostLevel &H = ctx->host[l];
+      DevLevel &D = ctx->dev[l];
+      HostPrep &hp = ps.hp[l];
+      D.n = H.n;
+      D.nnzb = H.nnzb;
+      max_slices = std::max(max_slices, hp.A.nslices);
+      if ((rc = upload_sell(ctx, hp.A, D.A))) return rc;
+      if ((rc = upload(ctx, &D.diag64, hp.diag.data(), hp.diag.size()))) return rc;
+      std::vector<float> d32;
+      to_float(hp.diag, d32);
+      if ((rc = upload(ctx, &D.diag32, d32.data(), d32.size()))) return rc;
+      if ((rc = upload(ctx, &D.vclass, H.vc.data(), H.vc.size()))) return rc;
+      if ((rc = upload(ctx, &D.vnclass, H.vnc.data(), H.vnc.size()))) return rc;
+      if ((rc = upload(ctx, &D.skip, H.skip.data(), H.skip.size()))) return rc;
+      if (hp.ncolors > 0) {
+        D.ncolors = hp.ncolors;
+        D.color_slices = hp.color_slices;
+        D.color = hp.color;
+        if ((rc = upload_sell(ctx, hp.L, D.L))) return rc;
+        if ((rc = upload_sell(ctx, hp.U, D.U))) return rc;
+      }
+      if (l > bl) {
+        D.p_nnz = H.pp[H.n];
+        D.pt_nnz = (int)hp.pt_idx.size();
+        if ((rc = upload_sell(ctx, hp.P, D.P))) return rc;
+        if ((rc = upload_sell(ctx, hp.PT, D.PT))) return rc;
+      }
+      D.nt = (double)hp.A.col.size() * 12.0 > ps.nt_bytes;
+      if ((rc = dalloc(ctx, &D.c, D.n)) || (rc = dalloc(ctx, &D.b, D.n)) ||
+          (rc = dalloc(ctx, &D.t, D.n)) || (rc = dalloc(ctx, &D.c32, D.n)) ||
+          (rc = dalloc(ctx, &D.b32, D.n)) || (rc = dalloc(ctx, &D.t32, D.n)))
+        return rc;
+      HIPCHK(hipMemset(D.c, 0, sizeof(double) * std::max(D.n, 1)));
+      HIPCHK(hipMemset(D.c32, 0, sizeof(float) * std::max(D.n, 1)));
+    }
+    ctx->base = std::move(ps.base);
+    if ((rc = upload(ctx, &ctx->base_inv, ctx->base.inv.data(), ctx->base.inv.size()))) return rc;
+    // fused coarse levels (opt-in): the base level and every level above it
+    // up to fuse_rows vectors (and at most kMaxColors colours)
+    ctx->fuse_top = -1;
+    if (ps.fuse_rows > 0 && ctx->dev[bl].n <= 2048) {
+      ctx->fuse_top = bl;
+      for (int l = bl + 1; l <= level; ++l) {
+        if (ctx->dev[l].n > ps.fuse_rows || ctx->dev[l].ncolors > kMaxColors) break;
+        ctx->fuse_top = l;
+      }
+    }
+    ctx->base_b_dead = true;
+    for (uint8_t v : ctx->host[bl].vnc) ctx->base_b_dead &= v >= 2;
+    std::vector<CLevel<double>> c64(std::max(ctx->fuse_top + 1, 1));
+    std::vector<CLevel<float>> c32(std::max(ctx->fuse_top + 1, 1));
+    for (int l = bl; l <= ctx->fuse_top; ++l) {
+      const DevLevel &D = ctx->dev[l];
+      auto fill = [&](auto &C, auto tag) {
+        using T = decltype(tag);
+        std::memset((void *)&C, 0, sizeof(C));
+        C.n = D.n;
+        C.ncolors = D.ncolors;
+        for (int k = 0; k < (int)D.color_slices.size() && k <= kMaxColors; ++k)
+          C.color_slices[k] = D.color_slices[k];
+        C.A = D.A.d<T>();
+        C.L = D.L.d<T>();
+        C.U = D.U.d<T>();
+        C.P = D.P.d<T>();
+        C.PT = D.PT.d<T>();
+        C.diag = D.diag<T>();
+        C.vclass = D.vclass;
+        C.vnclass = D.vnclass;
+        C.skip = D.skip;
+        C.c = D.cv<T>();
+        C.b = D.bv<T>();
+        C.t = D.tv<T>();
+      };
+      fill(c64[l], double());
+      fill(c32[l], float());
+    }
+    if ((rc = upload(ctx, &ctx->clev64, c64.data(), c64.size()))) return rc;
+    if ((rc = upload(ctx, &ctx->clev32, c32.data(), c32.size()))) return rc;
+    ctx->part_cap = std::max(1024, (max_slices + 3) / 4 + 1);
+    if ((rc = dalloc(ctx, &ctx->part, ctx->part_cap))) return rc;
+    if ((rc = dalloc(ctx, &ctx->red, 8))) return rc;
+    if ((rc = dalloc(ctx, &ctx->base_part, 1024))) return rc;
+    if ((rc = dalloc(ctx, &ctx->base_first, 8))) return rc;
+  } catch (const std::bad_alloc &) {
+    free_device(ctx);
+    return fail(ctx, UGMG_E_NOMEM, "host allocation during preprocess");
+  }
+  ctx->top = level;
+  ctx->prepared = true;
+  return UGMG_OK;
+}
 
 // ================================================================== C ABI
 
@@ -543,128 +705,11 @@ int ugmg_set_params(ugmg_ctx *ctx, const ugmg_params *p) {
 
 int ugmg_preprocess(ugmg_ctx *ctx, int level) {
   if (!ctx) return UGMG_E_ARG;
-  if (ctx->nlevels <= 0) return fail(ctx, UGMG_E_STATE, "ugmg_set_nlevels first");
-  if (level < 0 || level >= ctx->nlevels) return fail(ctx, UGMG_E_ARG, "level out of range");
-  if (!ctx->params_set) return fail(ctx, UGMG_E_STATE, "ugmg_set_params first");
+  PrepState ps;
+  int rc = prep_host(ctx, level, ps);
+  if (rc) return rc;
   HIPCHK(hipSetDevice(ctx->device));
-  free_device(ctx);
-  const ugmg_params &p = ctx->p;
-  int bl = p.baselevel;
-  if (bl < 0) {  // LmgcInit: deepest level with at most -baselevel vectors
-    int i;
-    for (i = level; i > 0; --i)
-      if (ctx->level_set[i] && ctx->host[i].n <= -bl) break;
-    bl = i;
-  }
-  bl = std::min(bl, level);
-  for (int l = bl; l <= level; ++l) {
-    if (!ctx->level_set[l]) return fail(ctx, UGMG_E_STATE, "level " + std::to_string(l) + " not set");
-    if (l > bl && !ctx->transfer_set[l])
-      return fail(ctx, UGMG_E_STATE, "transfer of level " + std::to_string(l) + " not set");
-  }
-  if (ctx->host[bl].n > 8192)
-    return fail(ctx, UGMG_E_ARG, "base level has more than 8192 vectors; choose a coarser baselevel");
-  try {
-    ctx->dev.assign(level + 1, DevLevel());
-    ctx->bl = bl;
-    const char *env_nt = std::getenv("UGMG_NT_BYTES");
-    const double nt_bytes = env_nt ? std::atof(env_nt) : 64.0e6;
-    const char *env_fuse = std::getenv("UGMG_FUSE_ROWS");
-    const int fuse_rows = env_fuse ? std::atoi(env_fuse) : 0;
-    int rc;
-    int max_slices = 0;
-    for (int l = bl; l <= level; ++l) {
-      const HostLevel &H = ctx->host[l];
-      DevLevel &D = ctx->dev[l];
-      HostPrep hp;
-      const bool gs = p.smoother == UGMG_SMOOTH_GS && l > bl;
-      build_prep(H, l > bl ? &ctx->host[l - 1] : nullptr, gs, hp);
-      D.n = H.n;
-      D.nnzb = H.nnzb;
-      max_slices = std::max(max_slices, hp.A.nslices);
-      if ((rc = upload_sell(ctx, hp.A, D.A))) return rc;
-      if ((rc = upload(ctx, &D.diag64, hp.diag.data(), hp.diag.size()))) return rc;
-      std::vector<float> d32(hp.diag.begin(), hp.diag.end());
-      if ((rc = upload(ctx, &D.diag32, d32.data(), d32.size()))) return rc;
-      if ((rc = upload(ctx, &D.vclass, H.vc.data(), H.vc.size()))) return rc;
-      if ((rc = upload(ctx, &D.vnclass, H.vnc.data(), H.vnc.size()))) return rc;
-      if ((rc = upload(ctx, &D.skip, H.skip.data(), H.skip.size()))) return rc;
-      if (gs) {
-        D.ncolors = hp.ncolors;
-        D.color_slices = hp.color_slices;
-        D.color = hp.color;
-        if ((rc = upload_sell(ctx, hp.L, D.L))) return rc;
-        if ((rc = upload_sell(ctx, hp.U, D.U))) return rc;
-      }
-      if (l > bl) {
-        D.p_nnz = H.pp[H.n];
-        D.pt_nnz = (int)hp.pt_idx.size();
-        if ((rc = upload_sell(ctx, hp.P, D.P))) return rc;
-        if ((rc = upload_sell(ctx, hp.PT, D.PT))) return rc;
-      }
-      D.nt = (double)hp.A.col.size() * 12.0 > nt_bytes;
-      if ((rc = dalloc(ctx, &D.c, D.n)) || (rc = dalloc(ctx, &D.b, D.n)) ||
-          (rc = dalloc(ctx, &D.t, D.n)) || (rc = dalloc(ctx, &D.c32, D.n)) ||
-          (rc = dalloc(ctx, &D.b32, D.n)) || (rc = dalloc(ctx, &D.t32, D.n)))
-        return rc;
-      HIPCHK(hipMemset(D.c, 0, sizeof(double) * std::max(D.n, 1)));
-      HIPCHK(hipMemset(D.c32, 0, sizeof(float) * std::max(D.n, 1)));
-    }
-    build_base(ctx->host[bl], p.ex_reorder, ctx->base);
-    if ((rc = upload(ctx, &ctx->base_inv, ctx->base.inv.data(), ctx->base.inv.size()))) return rc;
-    // fused coarse levels: the base level and every level above it up to
-    // fuse_rows vectors (and at most kMaxColors colours)
-    ctx->fuse_top = -1;
-    if (fuse_rows > 0 && ctx->dev[bl].n <= 2048) {
-      ctx->fuse_top = bl;
-      for (int l = bl + 1; l <= level; ++l) {
-        if (ctx->dev[l].n > fuse_rows || ctx->dev[l].ncolors > kMaxColors) break;
-        ctx->fuse_top = l;
-      }
-    }
-    ctx->base_b_dead = true;
-    for (uint8_t v : ctx->host[bl].vnc) ctx->base_b_dead &= v >= 2;
-    std::vector<CLevel<double>> c64(std::max(ctx->fuse_top + 1, 1));
-    std::vector<CLevel<float>> c32(std::max(ctx->fuse_top + 1, 1));
-    for (int l = bl; l <= ctx->fuse_top; ++l) {
-      const DevLevel &D = ctx->dev[l];
-      auto fill = [&](auto &C, auto tag) {
-        using T = decltype(tag);
-        std::memset((void *)&C, 0, sizeof(C));
-        C.n = D.n;
-        C.ncolors = D.ncolors;
-        for (int k = 0; k < (int)D.color_slices.size() && k <= kMaxColors; ++k)
-          C.color_slices[k] = D.color_slices[k];
-        C.A = D.A.d<T>();
-        C.L = D.L.d<T>();
-        C.U = D.U.d<T>();
-        C.P = D.P.d<T>();
-        C.PT = D.PT.d<T>();
-        C.diag = D.diag<T>();
-        C.vclass = D.vclass;
-        C.vnclass = D.vnclass;
-        C.skip = D.skip;
-        C.c = ctx->dev[l].cv<T>();
-        C.b = ctx->dev[l].bv<T>();
-        C.t = ctx->dev[l].tv<T>();
-      };
-      fill(c64[l], double());
-      fill(c32[l], float());
-    }
-    if ((rc = upload(ctx, &ctx->clev64, c64.data(), c64.size()))) return rc;
-    if ((rc = upload(ctx, &ctx->clev32, c32.data(), c32.size()))) return rc;
-    ctx->part_cap = std::max(1024, (max_slices + 3) / 4 + 1);
-    if ((rc = dalloc(ctx, &ctx->part, ctx->part_cap))) return rc;
-    if ((rc = dalloc(ctx, &ctx->red, 8))) return rc;
-    if ((rc = dalloc(ctx, &ctx->base_part, 1024))) return rc;
-    if ((rc = dalloc(ctx, &ctx->base_first, 8))) return rc;
-  } catch (const std::bad_alloc &) {
-    free_device(ctx);
-    return fail(ctx, UGMG_E_NOMEM, "host allocation during preprocess");
-  }
-  ctx->top = level;
-  ctx->prepared = true;
-  return UGMG_OK;
+  return prep_device(ctx, ps);
 }
 
 int ugmg_postprocess(ugmg_ctx *ctx) {
@@ -1072,6 +1117,376 @@ int ugmg_time_kernel(ugmg_ctx *ctx, int level, int kind, int reps, double *ms, d
   hipEventDestroy(e1);
   *ms = (double)msf / reps;
   *bytes = by;
+  return UGMG_OK;
+}
+
+}  // extern "C"
+
+// ================================================================= serving
+//
+// Continuous batching of independent solves.  Every submitted request gets
+// its own context (its own stream); a worker thread builds the host layouts
+// (prep_host) in submission order while the GPU works on admitted requests;
+// ugmg_server_step() admits prepared requests in FIFO order while slots and
+// the HBM budget allow, uploads them, and keeps one solver iteration of every
+// admitted request in flight, polling each request's norm copy with an
+// event.  Requests therefore overlap on the device (one stream each), join as
+// soon as a slot and memory free up, and run exactly the kernels of a
+// stand-alone ugmg_solve, so every result is the same alone and in the mix.
+
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
+
+namespace {
+
+struct SrvReq {
+  int id = 0;
+  int state = UGMG_REQ_QUEUED;
+  double t_submit = 0.0, t_admit = -1.0, t_done = -1.0;
+  ugmg_ctx *ctx = nullptr;
+  int top = 0;
+  std::vector<double> rhs;
+  double red = 1e-8, abslimit = 1e-300;
+  int maxiter = 100;
+  double est_bytes = 0.0, charged = 0.0;
+  PrepState ps;
+  bool host_ready = false;
+  int prep_rc = 0;
+  double *x = nullptr, *b = nullptr;
+  int phase = 0, it = 0;
+  double target = 0.0;
+  bool pending = false;
+  hipEvent_t ev = nullptr;
+  ugmg_lresult res{};
+  std::vector<double> xout;
+  std::string err;
+};
+
+double est_device_bytes(const ugmg_ctx *ctx, int top, int bl) {
+  double by = 0.0;
+  const bool gs = ctx->p.smoother == UGMG_SMOOTH_GS;
+  for (int l = bl; l <= top; ++l) {
+    const HostLevel &H = ctx->host[l];
+    by += (double)H.nnzb * 16.0 * 1.1;                      // A: col + FP64 + FP32, padding
+    if (gs && l > bl) by += (double)(H.nnzb - H.n) * 16.0 * 1.1;  // L + U
+    if (l > bl) by += 2.0 * (double)H.pp[H.n] * 16.0 * 1.1;      // P and P^T
+    by += (double)H.n * 70.0;                                 // vectors, diag, flags
+  }
+  by += (double)ctx->host[bl].n * ctx->host[bl].n * 8.0;     // dense inverse
+  return by;
+}
+
+}  // namespace
+
+struct ugmg_server {
+  int device = 0;
+  double budget = 0.0, used = 0.0;
+  int max_slots = 1;
+  std::vector<std::unique_ptr<SrvReq>> reqs;
+  std::deque<int> queue;
+  std::vector<int> active;
+  int ndone = 0;
+  std::string err;
+  std::chrono::steady_clock::time_point t0;
+  std::thread worker;
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<int> prep_queue;
+  bool stop = false;
+
+  double now() const {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+};
+
+namespace {
+
+void server_worker(ugmg_server *s) {
+  for (;;) {
+    SrvReq *r = nullptr;
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      s->cv.wait(lk, [&] { return s->stop || !s->prep_queue.empty(); });
+      if (s->stop) return;
+      r = s->reqs[s->prep_queue.front()].get();
+      s->prep_queue.pop_front();
+    }
+    const int rc = prep_host(r->ctx, r->top, r->ps);
+    std::lock_guard<std::mutex> lk(s->mu);
+    r->prep_rc = rc;
+    if (rc) r->err = r->ctx->err;
+    r->host_ready = true;
+  }
+}
+
+int srv_fail(ugmg_server *s, int code, const std::string &msg) {
+  s->err = msg;
+  return code;
+}
+
+void srv_finish(ugmg_server *s, SrvReq *r, int state) {
+  ugmg_ctx *ctx = r->ctx;
+  if (ctx && ctx->stream) hipStreamSynchronize(ctx->stream);
+  if (state == UGMG_REQ_DONE && ctx && r->x) {
+    r->xout.resize(ctx->dev[r->top].n);
+    if (hipMemcpy(r->xout.data(), r->x, sizeof(double) * r->xout.size(), hipMemcpyDeviceToHost) !=
+        hipSuccess) {
+      state = UGMG_REQ_FAILED;
+      r->err = "copy of the solution failed";
+    }
+  }
+  if (r->ev) hipEventDestroy(r->ev);
+  r->ev = nullptr;
+  if (ctx) ugmg_destroy(ctx);
+  r->ctx = nullptr;
+  r->x = r->b = nullptr;
+  r->ps = PrepState();
+  std::vector<double>().swap(r->rhs);
+  s->used -= r->charged;
+  r->charged = 0.0;
+  r->state = state;
+  r->t_done = s->now();
+  ++s->ndone;
+}
+
+// admit one prepared request: uploads, first defect norm in flight
+int srv_admit(ugmg_server *s, SrvReq *r) {
+  ugmg_ctx *ctx = r->ctx;
+  if (hipSetDevice(s->device) != hipSuccess) return UGMG_E_HIP;
+  int rc = prep_device(ctx, r->ps);
+  r->ps = PrepState();
+  if (rc) {
+    r->err = ctx->err;
+    return rc;
+  }
+  DevLevel &d = ctx->dev[r->top];
+  if ((rc = dalloc(ctx, &r->x, d.n)) || (rc = dalloc(ctx, &r->b, d.n))) {
+    r->err = ctx->err;
+    return rc;
+  }
+  r->charged = ctx->dev_bytes;
+  s->used += r->charged;
+  if (hipMemcpyAsync(r->b, r->rhs.data(), sizeof(double) * d.n, hipMemcpyHostToDevice,
+                     ctx->stream) != hipSuccess ||
+      hipEventCreateWithFlags(&r->ev, hipEventDisableTiming) != hipSuccess)
+    return UGMG_E_HIP;
+  launch_fill<double>(ctx->stream, d.n, r->x, 0.0);
+  if (ctx->p.precision == UGMG_MIXED) launch_convert<float, double>(ctx->stream, d.n, r->b, d.b32);
+  launch_dot<double>(ctx->stream, d.n, r->b, nullptr, d.vclass, ctx->part, ctx->red, 1);
+  if (hipMemcpyAsync(ctx->red_host, ctx->red, sizeof(double), hipMemcpyDeviceToHost,
+                     ctx->stream) != hipSuccess ||
+      hipEventRecord(r->ev, ctx->stream) != hipSuccess)
+    return UGMG_E_HIP;
+  r->pending = true;
+  r->phase = 0;
+  r->it = 0;
+  r->state = UGMG_REQ_ACTIVE;
+  r->t_admit = s->now();
+  return UGMG_OK;
+}
+
+// one completed norm: stop test (sc_cmp) and, if not done, the next iteration
+int srv_advance(ugmg_server *s, SrvReq *r) {
+  ugmg_ctx *ctx = r->ctx;
+  const double val = ctx->red_host[0];
+  bool conv;
+  if (r->phase == 0) {
+    r->res.first_defect[0] = r->res.last_defect[0] = val;
+    r->target = val * r->red;
+    if (r->target == 0.0) r->target = r->red;
+    conv = sc_cmp1(val, r->abslimit);
+    r->phase = 1;
+  } else {
+    ++r->it;
+    r->res.last_defect[0] = val;
+    conv = sc_cmp1(val, r->abslimit) || sc_cmp1(val, r->target);
+  }
+  if (conv || r->it >= r->maxiter) {
+    r->res.converged = conv ? 1 : 0;
+    r->res.number_of_linear_iterations = r->it;
+    srv_finish(s, r, UGMG_REQ_DONE);
+    return 1;
+  }
+  int rc = launch_iteration(ctx, r->top, r->x, r->b);
+  if (rc || hipEventRecord(r->ev, ctx->stream) != hipSuccess) {
+    r->err = ctx->err;
+    r->res.error_code = rc ? rc : UGMG_E_HIP;
+    srv_finish(s, r, UGMG_REQ_FAILED);
+    return 1;
+  }
+  r->pending = true;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ugmg_server_create(int device, double hbm_budget_bytes, int max_slots, ugmg_server **out) {
+  if (!out || max_slots < 1 || hbm_budget_bytes <= 0.0) return UGMG_E_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return UGMG_E_NODEV;
+  if (device < 0 || device >= ndev) return UGMG_E_ARG;
+  ugmg_server *s = new (std::nothrow) ugmg_server;
+  if (!s) return UGMG_E_NOMEM;
+  s->device = device;
+  s->budget = hbm_budget_bytes;
+  s->max_slots = max_slots;
+  s->t0 = std::chrono::steady_clock::now();
+  s->worker = std::thread(server_worker, s);
+  *out = s;
+  return UGMG_OK;
+}
+
+int ugmg_server_destroy(ugmg_server *s) {
+  if (!s) return UGMG_E_ARG;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  s->cv.notify_all();
+  if (s->worker.joinable()) s->worker.join();
+  for (auto &r : s->reqs)
+    if (r && r->ctx) {
+      if (r->ctx->stream) hipStreamSynchronize(r->ctx->stream);
+      if (r->ev) hipEventDestroy(r->ev);
+      ugmg_destroy(r->ctx);
+      r->ctx = nullptr;
+    }
+  delete s;
+  return UGMG_OK;
+}
+
+const char *ugmg_server_last_error(const ugmg_server *s) { return s ? s->err.c_str() : "null server"; }
+
+int ugmg_server_submit(ugmg_server *s, int nlevels, const ugmg_level_desc *levels,
+                       const ugmg_params *p, const double *rhs, double red, double abslimit,
+                       int maxiter, int *id) {
+  if (!s || !levels || !p || !rhs || !id || nlevels < 1 || maxiter < 0) return UGMG_E_ARG;
+  auto r = std::make_unique<SrvReq>();
+  ugmg_ctx *ctx = nullptr;
+  int rc = ugmg_create(s->device, &ctx);
+  if (rc) return srv_fail(s, rc, "ugmg_create failed");
+  r->ctx = ctx;
+  auto bail = [&](int code) {
+    s->err = ctx->err;
+    ugmg_destroy(ctx);
+    return code;
+  };
+  if ((rc = ugmg_set_nlevels(ctx, nlevels, 1))) return bail(rc);
+  for (int l = 0; l < nlevels; ++l) {
+    const ugmg_level_desc &L = levels[l];
+    if ((rc = ugmg_set_level(ctx, l, L.n, L.row_ptr, L.col_idx, L.vals, L.skip, L.vclass, L.vnclass)))
+      return bail(rc);
+  }
+  for (int l = 1; l < nlevels; ++l)
+    if ((rc = ugmg_set_transfer(ctx, l, levels[l].p_ptr, levels[l].p_idx, levels[l].p_w)))
+      return bail(rc);
+  ugmg_params q = *p;
+  if ((rc = ugmg_set_params(ctx, &q))) return bail(rc);
+  r->top = nlevels - 1;
+  const int n = levels[r->top].n;
+  r->rhs.assign(rhs, rhs + n);
+  r->red = red;
+  r->abslimit = abslimit;
+  r->maxiter = maxiter;
+  const int bl = std::min(std::max(q.baselevel, 0), r->top);
+  r->est_bytes = est_device_bytes(ctx, r->top, bl);
+  r->t_submit = s->now();
+  std::lock_guard<std::mutex> lk(s->mu);
+  r->id = (int)s->reqs.size();
+  *id = r->id;
+  s->queue.push_back(r->id);
+  s->prep_queue.push_back(r->id);
+  s->reqs.push_back(std::move(r));
+  s->cv.notify_one();
+  return UGMG_OK;
+}
+
+int ugmg_server_step(ugmg_server *s, int *nactive, int *nqueued, int *ndone) {
+  if (!s) return UGMG_E_ARG;
+  if (hipSetDevice(s->device) != hipSuccess) return srv_fail(s, UGMG_E_HIP, "hipSetDevice");
+  // progress: every request whose norm copy has landed takes its stop test
+  for (size_t k = 0; k < s->active.size();) {
+    SrvReq *r = s->reqs[s->active[k]].get();
+    bool finished = false;
+    if (r->pending) {
+      hipError_t q = hipEventQuery(r->ev);
+      if (q == hipSuccess) {
+        r->pending = false;
+        finished = srv_advance(s, r) != 0;
+      } else if (q != hipErrorNotReady) {
+        r->err = std::string("hipEventQuery: ") + hipGetErrorString(q);
+        r->res.error_code = UGMG_E_HIP;
+        srv_finish(s, r, UGMG_REQ_FAILED);
+        finished = true;
+      }
+    }
+    if (finished) s->active.erase(s->active.begin() + k);
+    else ++k;
+  }
+  // admission: FIFO, prepared, within slots and HBM budget (a request that
+  // alone exceeds the budget is rejected)
+  for (;;) {
+    if (s->queue.empty() || (int)s->active.size() >= s->max_slots) break;
+    SrvReq *r = s->reqs[s->queue.front()].get();
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      if (!r->host_ready) break;
+    }
+    if (r->prep_rc) {
+      s->queue.pop_front();
+      r->res.error_code = r->prep_rc;
+      srv_finish(s, r, UGMG_REQ_FAILED);
+      continue;
+    }
+    if (r->est_bytes > s->budget) {
+      s->queue.pop_front();
+      r->err = "request exceeds the HBM budget";
+      r->res.error_code = UGMG_E_BUDGET;
+      srv_finish(s, r, UGMG_REQ_FAILED);
+      continue;
+    }
+    if (s->used + r->est_bytes > s->budget) break;
+    s->queue.pop_front();
+    const int rc = srv_admit(s, r);
+    if (rc) {
+      r->res.error_code = rc;
+      srv_finish(s, r, UGMG_REQ_FAILED);
+      continue;
+    }
+    s->active.push_back(r->id);
+  }
+  if (nactive) *nactive = (int)s->active.size();
+  if (nqueued) *nqueued = (int)s->queue.size();
+  if (ndone) *ndone = s->ndone;
+  return UGMG_OK;
+}
+
+int ugmg_server_status(ugmg_server *s, int id, int *state, ugmg_lresult *res, double *times) {
+  if (!s || id < 0 || id >= (int)s->reqs.size()) return UGMG_E_ARG;
+  const SrvReq *r = s->reqs[id].get();
+  if (state) *state = r->state;
+  if (res) *res = r->res;
+  if (times) {
+    times[0] = r->t_submit;
+    times[1] = r->t_admit;
+    times[2] = r->t_done;
+  }
+  return UGMG_OK;
+}
+
+int ugmg_server_result(ugmg_server *s, int id, double *x, int n) {
+  if (!s || id < 0 || id >= (int)s->reqs.size() || !x) return UGMG_E_ARG;
+  SrvReq *r = s->reqs[id].get();
+  if (r->state != UGMG_REQ_DONE) return srv_fail(s, UGMG_E_STATE, "request not done: " + r->err);
+  if ((int)r->xout.size() != n) return srv_fail(s, UGMG_E_ARG, "wrong solution length");
+  std::copy(r->xout.begin(), r->xout.end(), x);
   return UGMG_OK;
 }
 

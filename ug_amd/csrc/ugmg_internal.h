@@ -67,6 +67,7 @@ void build_sell(const HostLevel &L, const std::vector<int> *rows, Part part,
                 const std::vector<int> *color, HostSell &out);
 void build_sell_csr(int n, const int *rp, const int *ci, const double *a, HostSell &out);
 int greedy_color(const HostLevel &L, std::vector<int> &color);
+void to_float(const std::vector<double> &in, std::vector<float> &out);
 void build_prep(const HostLevel &L, const HostLevel *coarse, bool want_gs,
                 HostPrep &out);
 void build_base(const HostLevel &L, int reorder, HostBase &out);

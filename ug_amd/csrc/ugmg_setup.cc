@@ -31,10 +31,12 @@ static void sell_from_csr(int n, const int *rp, const int *ci, const double *a,
   out.nrows = nr;
   out.identity_rows = rows == nullptr;
   out.nslices = (nr + kSlice - 1) / kSlice;
-  out.soff.assign(out.nslices + 1, 0);
-  out.rows.assign((size_t)out.nslices * kSlice, -1);
-  std::vector<int> width(out.nslices, 0);
-  for (int s = 0; s < out.nslices; ++s) {
+  const int ns = out.nslices;
+  out.soff.assign(ns + 1, 0);
+  out.rows.assign((size_t)ns * kSlice, -1);
+  std::vector<int> width(ns, 0);
+#pragma omp parallel for schedule(static)
+  for (int s = 0; s < ns; ++s) {
     int wmax = 0;
     for (int lane = 0; lane < kSlice; ++lane) {
       const int slot = s * kSlice + lane;
@@ -47,12 +49,13 @@ static void sell_from_csr(int n, const int *rp, const int *ci, const double *a,
       wmax = std::max(wmax, cnt);
     }
     width[s] = wmax;
-    out.soff[s + 1] = out.soff[s] + wmax * kSlice;
   }
-  const size_t nent = (size_t)out.soff[out.nslices];
-  out.col.assign(nent, 0);
-  out.val.assign(nent, 0.0);
-  for (int s = 0; s < out.nslices; ++s) {
+  for (int s = 0; s < ns; ++s) out.soff[s + 1] = out.soff[s] + width[s] * kSlice;
+  const size_t nent = (size_t)out.soff[ns];
+  out.col.resize(nent);
+  out.val.resize(nent);
+#pragma omp parallel for schedule(static)
+  for (int s = 0; s < ns; ++s) {
     for (int lane = 0; lane < kSlice; ++lane) {
       const int slot = s * kSlice + lane;
       const int r = slot < nr ? out.rows[slot] : -1;
@@ -96,6 +99,13 @@ void build_sell(const HostLevel &L, const std::vector<int> *rows, Part part,
 
 void build_sell_csr(int n, const int *rp, const int *ci, const double *a, HostSell &out) {
   sell_from_csr(n, rp, ci, a, nullptr, [](int, int) { return true; }, out);
+}
+
+void to_float(const std::vector<double> &in, std::vector<float> &out) {
+  out.resize(in.size());
+  const long long ne = (long long)in.size();
+#pragma omp parallel for schedule(static)
+  for (long long i = 0; i < ne; ++i) out[i] = (float)in[i];
 }
 
 int greedy_color(const HostLevel &L, std::vector<int> &color) {

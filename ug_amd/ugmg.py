@@ -315,3 +315,112 @@ class Context:
         ms, by = C.c_double(), C.c_double()
         self._chk(self.lib.ugmg_time_kernel(self.h, level, kind, reps, C.byref(ms), C.byref(by)))
         return ms.value, by.value
+
+
+# ---------------------------------------------------------------- serving
+
+class LevelDesc(C.Structure):
+    _fields_ = [("n", C.c_int), ("row_ptr", C.c_void_p), ("col_idx", C.c_void_p), ("vals", C.c_void_p),
+                ("skip", C.c_void_p), ("vclass", C.c_void_p), ("vnclass", C.c_void_p),
+                ("p_ptr", C.c_void_p), ("p_idx", C.c_void_p), ("p_w", C.c_void_p)]
+
+
+REQ_QUEUED, REQ_ACTIVE, REQ_DONE, REQ_FAILED = range(4)
+
+
+def _load_server(L):
+    if getattr(L, "_server_sigs", False):
+        return
+    vp, ip = C.c_void_p, C.c_int
+    sig = {
+        "ugmg_server_create": ([ip, C.c_double, ip, C.POINTER(vp)], ip),
+        "ugmg_server_destroy": ([vp], ip),
+        "ugmg_server_last_error": ([vp], C.c_char_p),
+        "ugmg_server_submit": ([vp, ip, C.POINTER(LevelDesc), C.POINTER(Params), vp, C.c_double, C.c_double,
+                                ip, C.POINTER(ip)], ip),
+        "ugmg_server_step": ([vp, C.POINTER(ip), C.POINTER(ip), C.POINTER(ip)], ip),
+        "ugmg_server_status": ([vp, ip, C.POINTER(ip), C.POINTER(LResult), vp], ip),
+        "ugmg_server_result": ([vp, ip, vp, ip], ip),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    L._server_sigs = True
+
+
+class Server:
+    """Continuous-batching solve server (ugmg_server_* in libugmg)."""
+
+    def __init__(self, device=0, budget_bytes=64e9, max_slots=8):
+        self.lib = load()
+        _load_server(self.lib)
+        h = C.c_void_p()
+        rc = self.lib.ugmg_server_create(device, float(budget_bytes), max_slots, C.byref(h))
+        if rc != 0:
+            raise UgmgError(rc, "ugmg_server_create failed")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.ugmg_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise UgmgError(rc, self.lib.ugmg_server_last_error(self.h).decode())
+
+    def submit(self, problem, params: Params, rhs=None, red=1e-8, abslimit=1e-300, maxiter=200):
+        levels = problem.levels
+        descs = (LevelDesc * len(levels))()
+        keep = []
+        for l, lv in enumerate(levels):
+            arrs = [np.ascontiguousarray(lv.row_ptr, np.int32), np.ascontiguousarray(lv.col_idx, np.int32),
+                    np.ascontiguousarray(lv.vals, np.float64), np.ascontiguousarray(lv.skip, np.uint32),
+                    np.ascontiguousarray(lv.vclass, np.uint8), np.ascontiguousarray(lv.vnclass, np.uint8)]
+            if lv.p_ptr is not None:
+                arrs += [np.ascontiguousarray(lv.p_ptr, np.int32), np.ascontiguousarray(lv.p_idx, np.int32),
+                         np.ascontiguousarray(lv.p_w, np.float64)]
+            keep.append(arrs)
+            d = descs[l]
+            d.n = lv.n
+            d.row_ptr, d.col_idx, d.vals, d.skip, d.vclass, d.vnclass = [a.ctypes.data for a in arrs[:6]]
+            if len(arrs) > 6:
+                d.p_ptr, d.p_idx, d.p_w = [a.ctypes.data for a in arrs[6:]]
+        b = np.ascontiguousarray(problem.rhs if rhs is None else rhs, np.float64)
+        rid = C.c_int()
+        self._chk(self.lib.ugmg_server_submit(self.h, len(levels), descs, C.byref(params), b.ctypes.data,
+                                              red, abslimit, maxiter, C.byref(rid)))
+        return rid.value
+
+    def step(self):
+        a, q, d = C.c_int(), C.c_int(), C.c_int()
+        self._chk(self.lib.ugmg_server_step(self.h, C.byref(a), C.byref(q), C.byref(d)))
+        return a.value, q.value, d.value
+
+    def status(self, rid):
+        st = C.c_int()
+        res = LResult()
+        times = (C.c_double * 3)()
+        self._chk(self.lib.ugmg_server_status(self.h, rid, C.byref(st), C.byref(res), C.addressof(times)))
+        return st.value, res, tuple(times)
+
+    def result(self, rid, n):
+        x = np.zeros(n)
+        self._chk(self.lib.ugmg_server_result(self.h, rid, x.ctypes.data, n))
+        return x
+
+    def drain(self, total, sleep=0.0):
+        import time
+        while True:
+            a, q, d = self.step()
+            if d >= total:
+                return
+            if sleep:
+                time.sleep(sleep)

@@ -36,11 +36,12 @@ namespace {
 struct DevSellBuf {
   int nslices = 0;
   size_t nent = 0;
-  int *soff = nullptr, *rows = nullptr, *col = nullptr;
+  int nuniform = 0;
+  int *soff = nullptr, *rows = nullptr, *col = nullptr, *sdoff = nullptr, *dtab = nullptr;
   double *v64 = nullptr;
   float *v32 = nullptr;
-  DSell<double> d64() const { return {nslices, soff, rows, col, v64}; }
-  DSell<float> d32() const { return {nslices, soff, rows, col, v32}; }
+  DSell<double> d64() const { return {nslices, soff, rows, col, v64, sdoff, dtab}; }
+  DSell<float> d32() const { return {nslices, soff, rows, col, v32, sdoff, dtab}; }
   template <typename T>
   DSell<T> d() const {
     if constexpr (std::is_same_v<T, double>) return d64();
@@ -149,6 +150,10 @@ int upload_sell(ugmg_ctx *ctx, const HostSell &S, DevSellBuf &D) {
   if ((rc = upload(ctx, &D.soff, S.soff.data(), S.soff.size()))) return rc;
   if ((rc = upload(ctx, &D.rows, S.rows.data(), S.rows.size()))) return rc;
   if ((rc = upload(ctx, &D.col, S.col.data(), S.col.size()))) return rc;
+  if ((rc = upload(ctx, &D.sdoff, S.sdoff.data(), S.sdoff.size()))) return rc;
+  if ((rc = upload(ctx, &D.dtab, S.dtab.data(), S.dtab.size()))) return rc;
+  D.nuniform = 0;
+  for (int o : S.sdoff) D.nuniform += o >= 0;
   if ((rc = upload(ctx, &D.v64, S.val.data(), S.val.size()))) return rc;
   std::vector<float> v32;
   to_float(S.val, v32);
@@ -475,8 +480,10 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
           (rc = dalloc(ctx, &D.t, D.n)) || (rc = dalloc(ctx, &D.c32, D.n)) ||
           (rc = dalloc(ctx, &D.b32, D.n)) || (rc = dalloc(ctx, &D.t32, D.n)))
         return rc;
-      HIPCHK(hipMemset(D.c, 0, sizeof(double) * std::max(D.n, 1)));
-      HIPCHK(hipMemset(D.c32, 0, sizeof(float) * std::max(D.n, 1)));
+      // every level vector starts at zero: the zero-valued phantom entries of
+      // uniform slices gather from them before a sweep has written them
+      for (double *v : {D.c, D.b, D.t}) HIPCHK(hipMemset(v, 0, sizeof(double) * std::max(D.n, 1)));
+      for (float *v : {D.c32, D.b32, D.t32}) HIPCHK(hipMemset(v, 0, sizeof(float) * std::max(D.n, 1)));
     }
     ctx->base = std::move(ps.base);
     if ((rc = upload(ctx, &ctx->base_inv, ctx->base.inv.data(), ctx->base.inv.size()))) return rc;
@@ -745,29 +752,36 @@ int ugmg_get_pattern(ugmg_ctx *ctx, int level, int *row_ptr, int *col_idx, doubl
   if (rc) return rc;
   const DevLevel &d = ctx->dev[level];
   const int ns = d.A.nslices;
-  std::vector<int> soff(ns + 1), rows((size_t)ns * kSlice), col(d.A.nent);
+  std::vector<int> soff(ns + 1), rows((size_t)ns * kSlice), col(d.A.nent), sdoff(std::max(ns, 1));
   std::vector<double> val(d.A.nent);
   HIPCHK(hipMemcpy(soff.data(), d.A.soff, soff.size() * sizeof(int), hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(rows.data(), d.A.rows, rows.size() * sizeof(int), hipMemcpyDeviceToHost));
+  if (ns) HIPCHK(hipMemcpy(sdoff.data(), d.A.sdoff, ns * sizeof(int), hipMemcpyDeviceToHost));
   if (d.A.nent) {
     HIPCHK(hipMemcpy(col.data(), d.A.col, col.size() * sizeof(int), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(val.data(), d.A.v64, val.size() * sizeof(double), hipMemcpyDeviceToHost));
   }
-  // the host copy's row lengths separate entries from slice padding
+  // explicit slices: the host copy's row lengths separate entries from slice
+  // padding; uniform slices: phantom entries carry column -1
   const HostLevel &H = ctx->host[level];
   row_ptr[0] = 0;
-  for (int s = 0; s < ns; ++s)
+  for (int s = 0; s < ns; ++s) {
+    const int w = (soff[s + 1] - soff[s]) / kSlice;
     for (int lane = 0; lane < kSlice; ++lane) {
       const int r = rows[(size_t)s * kSlice + lane];
       if (r < 0) continue;
       const int len = H.rp[r + 1] - H.rp[r];
-      for (int k = 0; k < len; ++k) {
+      int q = 0;
+      for (int k = 0; k < w && q < len; ++k) {
         const size_t e = (size_t)soff[s] + (size_t)k * kSlice + lane;
-        col_idx[H.rp[r] + k] = col[e];
-        if (vals) vals[H.rp[r] + k] = val[e];
+        if (sdoff[s] >= 0 && col[e] < 0) continue;
+        col_idx[H.rp[r] + q] = col[e];
+        if (vals) vals[H.rp[r] + q] = val[e];
+        ++q;
       }
-      row_ptr[r + 1] = len;
+      row_ptr[r + 1] = q;
     }
+  }
   for (int i = 0; i < d.n; ++i) row_ptr[i + 1] += row_ptr[i];
   return UGMG_OK;
 }

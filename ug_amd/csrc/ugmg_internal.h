@@ -32,6 +32,12 @@ struct HostSell {
   std::vector<int> col;
   std::vector<double> val;
   bool identity_rows = false;
+  // uniform-delta slices: every row of the slice has the column offsets
+  // dtab[sdoff[s] + k] (k < width), entries missing in a row are stored as
+  // value 0 ("phantoms", col = -1 in `col`); sdoff[s] = -1 for an explicit
+  // slice, whose columns are read from `col`.
+  std::vector<int> sdoff;
+  std::vector<int> dtab;
 };
 
 // Host-side products of LmgcPreProcess for one level.

@@ -21,6 +21,8 @@ struct DSell {
   const int *rows;
   const int *col;
   const TV *val;
+  const int *sdoff;   // per slice: offset into dtab (uniform slice) or -1
+  const int *dtab;    // shared column-offset lists of uniform slices
 };
 
 // base-level solver settings: ls($m,$red,$abslimit) over ex($damp)

@@ -33,11 +33,32 @@ __device__ __forceinline__ T ld(const T *p) {
   else return *p;
 }
 
-// sum_{k<w} val[64 k] * x[col[64 k]], entry order, 8 loads in flight
-template <int NT, typename TV, typename TX>
-__device__ __forceinline__ TX sell_dot(const TV *__restrict__ v, const int *__restrict__ ci,
-                                       int w, const TX *__restrict__ x) {
-  TX sum = 0;
+// sum_{k<w} val[64 k] * x[col_k], entry order, 8 entries in flight per lane.
+// Uniform slice (sdo >= 0): col_k = rb + dtab[sdo + k] (wave-uniform offsets
+// in scalar registers), so the x gathers issue together with the value loads;
+// explicit slice: col_k = ci[64 k].  TA is the accumulation type.
+template <int NT, typename TV, typename TX, typename TA>
+__device__ __forceinline__ TA sell_dot(const TV *__restrict__ v, const int *__restrict__ ci,
+                                       int w, const TX *__restrict__ x, int sdo,
+                                       const int *__restrict__ dtab, int rb) {
+  TA sum = 0;
+  if (sdo >= 0) {
+    const int *__restrict__ dt = dtab + sdo;
+    for (int k = 0; k < w; k += 8) {
+      TV a[8];
+      TX xv[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int kk = (k + j < w) ? k + j : w - 1;
+        a[j] = ld<NT>(v + 64 * kk);
+        xv[j] = x[rb + dt[kk]];
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (k + j < w) sum += (TA)a[j] * (TA)xv[j];
+    }
+    return sum;
+  }
   for (int k = 0; k < w; k += 8) {
     TV a[8];
     int c[8];
@@ -52,19 +73,24 @@ __device__ __forceinline__ TX sell_dot(const TV *__restrict__ v, const int *__re
     for (int j = 0; j < 8; ++j) xv[j] = x[c[j]];
 #pragma unroll
     for (int j = 0; j < 8; ++j)
-      if (k + j < w) sum += (TX)a[j] * xv[j];
+      if (k + j < w) sum += (TA)a[j] * (TA)xv[j];
   }
   return sum;
 }
 
-#define SLICE_PROLOGUE(S, slice0, nsl)                                       \
-  const int bid_ = xcd_remap(blockIdx.x, gridDim.x);                         \
-  const int s = (slice0) + bid_ * (kBlock / kSlice) + (threadIdx.x >> 6);   \
-  if (s >= (slice0) + (nsl)) return;                                         \
-  const int lane = threadIdx.x & 63;                                         \
-  const int r = (S).rows[s * kSlice + lane];                                 \
-  const int e0 = (S).soff[s];                                                \
-  const int w = ((S).soff[s + 1] - e0) >> 6;
+// slice s (wave-uniform, in scalar registers), lane, row r (-1 = padding),
+// entry base e0, width w, uniform-offset list sdo, index base rb
+#define SLICE_PROLOGUE(S, slice0, nsl)                                                   \
+  const int bid_ = xcd_remap(blockIdx.x, gridDim.x);                                     \
+  const int s = __builtin_amdgcn_readfirstlane((slice0) + bid_ * (kBlock / kSlice) +     \
+                                               (int)(threadIdx.x >> 6));                 \
+  if (s >= (slice0) + (nsl)) return;                                                     \
+  const int lane = threadIdx.x & 63;                                                     \
+  const int r = (S).rows[s * kSlice + lane];                                             \
+  const int e0 = (S).soff[s];                                                            \
+  const int w = ((S).soff[s + 1] - e0) >> 6;                                             \
+  const int sdo = (S).sdoff[s];                                                          \
+  const int rb = r >= 0 ? r : (S).rows[s * kSlice];
 
 // ------------------------------------------------------------------- SpMV
 // MODE: 0 y -= A x, 1 y = A x, 2 y -= A x and c += x, 3 y -= A x and c = x
@@ -73,7 +99,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(DSell<TV> A, int slice0, int ns
                                                   const TX *__restrict__ x, TX *__restrict__ y,
                                                   TX *__restrict__ c) {
   SLICE_PROLOGUE(A, slice0, nsl)
-  const TX sum = sell_dot<NT>(A.val + e0 + lane, A.col + e0 + lane, w, x);
+  const TX sum = sell_dot<NT, TV, TX, TX>(A.val + e0 + lane, A.col + e0 + lane, w, x, sdo, A.dtab, rb);
   if (r < 0) return;
   if (MODE == 1) {
     y[r] = sum;
@@ -97,30 +123,17 @@ __global__ __launch_bounds__(kBlock) void k_ir_update(DSell<double> A, int nsl,
                                                        double *__restrict__ part) {
   __shared__ double red[kBlock / 64];
   const int bid = xcd_remap(blockIdx.x, gridDim.x);
-  const int s = bid * (kBlock / kSlice) + (threadIdx.x >> 6);
+  const int s = __builtin_amdgcn_readfirstlane(bid * (kBlock / kSlice) + (int)(threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
   double sq = 0.0;
   if (s < nsl) {
     const int r = A.rows[s * kSlice + lane];
     const int e0 = A.soff[s];
     const int w = (A.soff[s + 1] - e0) >> 6;
-    double sum = 0.0;
-    for (int k = 0; k < w; k += 8) {
-      double a[8];
-      int c[8];
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const int kk = (k + j < w) ? k + j : w - 1;
-        a[j] = ld<NT>(A.val + e0 + lane + 64 * kk);
-        c[j] = ld<NT>(A.col + e0 + lane + 64 * kk);
-      }
-      float xv[8];
-#pragma unroll
-      for (int j = 0; j < 8; ++j) xv[j] = c32[c[j]];
-#pragma unroll
-      for (int j = 0; j < 8; ++j)
-        if (k + j < w) sum += a[j] * (double)xv[j];
-    }
+    const int sdo = A.sdoff[s];
+    const int rb = r >= 0 ? r : A.rows[s * kSlice];
+    const double sum =
+        sell_dot<NT, double, float, double>(A.val + e0 + lane, A.col + e0 + lane, w, c32, sdo, A.dtab, rb);
     if (r >= 0) {
       const double nr = rv[r] - sum;
       rv[r] = nr;
@@ -160,7 +173,7 @@ __global__ __launch_bounds__(kBlock) void k_gs_color(DSell<TV> L, int slice0, in
                                                       const uint8_t *__restrict__ vclass,
                                                       T *__restrict__ v) {
   SLICE_PROLOGUE(L, slice0, nsl)
-  const T sum = sell_dot<NT>(L.val + e0 + lane, L.col + e0 + lane, w, v);
+  const T sum = sell_dot<NT, TV, T, T>(L.val + e0 + lane, L.col + e0 + lane, w, v, sdo, L.dtab, rb);
   if (r < 0) return;
   v[r] = vclass[r] < 3 ? (T)0 : (b[r] - sum) / diag[r];
 }
@@ -177,7 +190,7 @@ __global__ __launch_bounds__(kBlock) void k_gs_defect(DSell<TV> U, int nsl,
                                                        T damp, T *__restrict__ c,
                                                        T *__restrict__ xout) {
   SLICE_PROLOGUE(U, 0, nsl)
-  const T sum = sell_dot<NT>(U.val + e0 + lane, U.col + e0 + lane, w, v);
+  const T sum = sell_dot<NT, TV, T, T>(U.val + e0 + lane, U.col + e0 + lane, w, v, sdo, U.dtab, rb);
   if (r < 0) return;
   const T br = b[r];
   b[r] = vclass[r] >= 3 ? br - damp * (br + sum) : br - damp * sum;
@@ -204,14 +217,16 @@ __global__ __launch_bounds__(kBlock) void k_restrict(DSell<T> PT, int nsl,
                                                       double *__restrict__ part) {
   __shared__ double red[kBlock / 64];
   const int bid = xcd_remap(blockIdx.x, gridDim.x);
-  const int s = bid * (kBlock / kSlice) + (threadIdx.x >> 6);
+  const int s = __builtin_amdgcn_readfirstlane(bid * (kBlock / kSlice) + (int)(threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
   double sq = 0.0;
   if (s < nsl) {
     const int r = PT.rows[s * kSlice + lane];
     const int e0 = PT.soff[s];
     const int w = (PT.soff[s + 1] - e0) >> 6;
-    const T sum = sell_dot<NT>(PT.val + e0 + lane, PT.col + e0 + lane, w, bf);
+    const int sdo = PT.sdoff[s];
+    const int rb = r >= 0 ? r : PT.rows[s * kSlice];
+    const T sum = sell_dot<NT, T, T, T>(PT.val + e0 + lane, PT.col + e0 + lane, w, bf, sdo, PT.dtab, rb);
     if (r >= 0) {
       const T base = cvnclass[r] >= 2 ? (T)0 : bc[r];
       const T v = (cskip[r] & 1u) ? base : base + damp * sum;
@@ -237,7 +252,7 @@ __global__ __launch_bounds__(kBlock) void k_interp(DSell<T> P, int nsl,
                                                     const T *__restrict__ cc, T damp,
                                                     T *__restrict__ t, T *__restrict__ c) {
   SLICE_PROLOGUE(P, 0, nsl)
-  const T sum = sell_dot<NT>(P.val + e0 + lane, P.col + e0 + lane, w, cc);
+  const T sum = sell_dot<NT, T, T, T>(P.val + e0 + lane, P.col + e0 + lane, w, cc, sdo, P.dtab, rb);
   if (r < 0) return;
   const T tv = (fskip[r] & 1u) ? (T)0 : damp * sum;
   t[r] = tv;
@@ -322,10 +337,13 @@ __device__ __forceinline__ void wg_sell(const DSell<TV> &S, int s0, int s1,
                                         const TX *__restrict__ x, Epi epi) {
   const int nslot = (s1 - s0) * kSlice;
   for (int slot = threadIdx.x; slot < nslot; slot += kCoarseThreads) {
-    const int s = s0 + (slot >> 6), lane = slot & 63;
+    const int s = __builtin_amdgcn_readfirstlane(s0 + (slot >> 6));
+    const int lane = slot & 63;
     const int r = S.rows[s * kSlice + lane];
     const int e0 = S.soff[s], w = (S.soff[s + 1] - e0) >> 6;
-    const TX sum = sell_dot<0>(S.val + e0 + lane, S.col + e0 + lane, w, x);
+    const int sdo = S.sdoff[s];
+    const int rb = r >= 0 ? r : S.rows[s * kSlice];
+    const TX sum = sell_dot<0, TV, TX, TX>(S.val + e0 + lane, S.col + e0 + lane, w, x, sdo, S.dtab, rb);
     if (r >= 0) epi(r, sum);
   }
   __syncthreads();

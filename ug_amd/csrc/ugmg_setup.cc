@@ -24,9 +24,15 @@ namespace ugmg {
 
 // SELL-64 slabs of the rows `rows` (or 0..n-1) of a CSR matrix, keeping the
 // entries for which keep(row, entry) holds, in CSR order.
+// SELL-64 slabs of the rows `rows` (or 0..n-1) of a CSR matrix with ncols
+// columns, keeping the entries for which keep(row, entry) holds, in CSR
+// order.  With `uniform`, a slice whose rows all list their columns as an
+// in-order subsequence of one common offset sequence (column - row) is stored
+// as values only against that sequence.
 template <class Keep>
-static void sell_from_csr(int n, const int *rp, const int *ci, const double *a,
-                          const std::vector<int> *rows, Keep keep, HostSell &out) {
+static void sell_from_csr(int n, int ncols, const int *rp, const int *ci, const double *a,
+                          const std::vector<int> *rows, Keep keep, bool uniform,
+                          HostSell &out) {
   const int nr = rows ? (int)rows->size() : n;
   out.nrows = nr;
   out.identity_rows = rows == nullptr;
@@ -34,10 +40,12 @@ static void sell_from_csr(int n, const int *rp, const int *ci, const double *a,
   const int ns = out.nslices;
   out.soff.assign(ns + 1, 0);
   out.rows.assign((size_t)ns * kSlice, -1);
+  out.sdoff.assign(ns, -1);
   std::vector<int> width(ns, 0);
+  std::vector<std::vector<int>> sdel(ns);   // common offsets of uniform slices
 #pragma omp parallel for schedule(static)
   for (int s = 0; s < ns; ++s) {
-    int wmax = 0;
+    int wmax = 0, longest = -1;
     for (int lane = 0; lane < kSlice; ++lane) {
       const int slot = s * kSlice + lane;
       if (slot >= nr) break;
@@ -46,19 +54,83 @@ static void sell_from_csr(int n, const int *rp, const int *ci, const double *a,
       if (r < 0) continue;  // padding slot
       int cnt = 0;
       for (int m = rp[r]; m < rp[r + 1]; ++m) cnt += keep(r, m) ? 1 : 0;
-      wmax = std::max(wmax, cnt);
+      if (cnt > wmax) { wmax = cnt; longest = r; }
     }
     width[s] = wmax;
+    if (!uniform || wmax == 0 || out.rows[(size_t)s * kSlice] < 0) continue;
+    std::vector<int> U;
+    for (int m = rp[longest]; m < rp[longest + 1]; ++m)
+      if (keep(longest, m)) U.push_back(ci[m] - longest);
+    bool ok = true;
+    for (int lane = 0; lane < kSlice && ok; ++lane) {
+      const int r = out.rows[(size_t)s * kSlice + lane];
+      if (r < 0) continue;
+      for (int u : U)
+        if (r + u < 0 || r + u >= ncols) { ok = false; break; }
+      size_t k = 0;
+      for (int m = rp[r]; m < rp[r + 1] && ok; ++m) {
+        if (!keep(r, m)) continue;
+        const int d = ci[m] - r;
+        while (k < U.size() && U[k] != d) ++k;
+        if (k == U.size()) ok = false;
+        else ++k;
+      }
+    }
+    if (ok) sdel[s].swap(U);
   }
   for (int s = 0; s < ns; ++s) out.soff[s + 1] = out.soff[s] + width[s] * kSlice;
+  // shared offset lists (most slices of a stencil matrix share one)
+  out.dtab.clear();
+  {
+    std::vector<std::pair<std::vector<int>, int>> seen;
+    for (int s = 0; s < ns; ++s) {
+      if (sdel[s].empty()) continue;
+      int off = -1;
+      for (auto &e : seen)
+        if (e.first == sdel[s]) { off = e.second; break; }
+      if (off < 0) {
+        off = (int)out.dtab.size();
+        out.dtab.insert(out.dtab.end(), sdel[s].begin(), sdel[s].end());
+        if (seen.size() < 64) seen.emplace_back(sdel[s], off);
+      }
+      out.sdoff[s] = off;
+    }
+    if (out.dtab.empty()) out.dtab.push_back(0);
+  }
   const size_t nent = (size_t)out.soff[ns];
   out.col.resize(nent);
   out.val.resize(nent);
 #pragma omp parallel for schedule(static)
   for (int s = 0; s < ns; ++s) {
+    const int w = width[s];
     for (int lane = 0; lane < kSlice; ++lane) {
       const int slot = s * kSlice + lane;
       const int r = slot < nr ? out.rows[slot] : -1;
+      if (out.sdoff[s] >= 0) {
+        const int *U = out.dtab.data() + out.sdoff[s];
+        int k = 0;
+        if (r >= 0)
+          for (int m = rp[r]; m < rp[r + 1]; ++m) {
+            if (!keep(r, m)) continue;
+            const int d = ci[m] - r;
+            while (U[k] != d) {  // phantom entries before this one
+              const size_t e = (size_t)out.soff[s] + (size_t)k * kSlice + lane;
+              out.col[e] = -1;
+              out.val[e] = 0.0;
+              ++k;
+            }
+            const size_t e = (size_t)out.soff[s] + (size_t)k * kSlice + lane;
+            out.col[e] = ci[m];
+            out.val[e] = a[m];
+            ++k;
+          }
+        for (; k < w; ++k) {
+          const size_t e = (size_t)out.soff[s] + (size_t)k * kSlice + lane;
+          out.col[e] = -1;
+          out.val[e] = 0.0;
+        }
+        continue;
+      }
       int k = 0;
       int last = 0;
       if (r >= 0) {
@@ -72,7 +144,7 @@ static void sell_from_csr(int n, const int *rp, const int *ci, const double *a,
         }
       }
       // padding: value 0 against the row's last column (a line already loaded)
-      for (; k < width[s]; ++k) {
+      for (; k < w; ++k) {
         const size_t e = (size_t)out.soff[s] + (size_t)k * kSlice + lane;
         out.col[e] = last;
         out.val[e] = 0.0;
@@ -94,11 +166,11 @@ void build_sell(const HostLevel &L, const std::vector<int> *rows, Part part,
     if (L.vc[r] < 3) return true;
     return cw > cr;
   };
-  sell_from_csr(L.n, L.rp.data(), L.ci.data(), L.a.data(), rows, keep, out);
+  sell_from_csr(L.n, L.n, L.rp.data(), L.ci.data(), L.a.data(), rows, keep, true, out);
 }
 
 void build_sell_csr(int n, const int *rp, const int *ci, const double *a, HostSell &out) {
-  sell_from_csr(n, rp, ci, a, nullptr, [](int, int) { return true; }, out);
+  sell_from_csr(n, 0, rp, ci, a, nullptr, [](int, int) { return true; }, false, out);
 }
 
 void to_float(const std::vector<double> &in, std::vector<float> &out) {

@@ -46,6 +46,16 @@ typedef struct ugmg_problem ugmg_problem;
 int ugmg_problem_q1(int dim, int coarse_cells, int nref, int kappa_mode,
                     uint64_t seed, ugmg_problem **out);
 
+/* 2D convection-diffusion -eps Laplace u + b.grad u = f, b = (cos t, sin t),
+ * streamline-upwind Petrov-Galerkin P1 triangles: 2x2 coarse squares split
+ * by the (0,0)-(1,1) diagonal (8 triangles) refined nref times (level l has
+ * (2^(l+1)+1)^2 nodes), every level rediscretised, zero Dirichlet boundary.
+ * theta = NaN draws t ~ U[0, 2 pi) first from the SplitMix64 stream; then f
+ * nodal U[-1,1] on the finest level.  Nonsymmetric. */
+int ugmg_problem_p1_cd(int nref, double eps, double theta, uint64_t seed,
+                       ugmg_problem **out);
+double ugmg_problem_theta(const ugmg_problem *p);
+
 void ugmg_problem_free(ugmg_problem *p);
 
 int ugmg_problem_nlevels(const ugmg_problem *p);     /* nref + 1            */

@@ -66,6 +66,8 @@ def lib():
         L.orc_ls_lmgc_solve.argtypes = [vp, C.c_int, vp, dp, dp, C.c_int, dp, dp, C.c_int, dp,
                                         C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.orc_lmgc_iter.argtypes = [vp, C.c_int, vp, dp, dp]
+        L.orc_bcgs_solve.argtypes = [vp, C.c_int, vp, C.c_int, dp, dp, C.c_int, dp, dp, C.c_int, C.c_int, dp,
+                                     C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
         _lib = L
     return _lib
 
@@ -244,3 +246,23 @@ def ls_solve(H, top, p, rhs, x0=None, red=1e-10, abslimit=1e-300, maxiter=50, do
         raise RuntimeError(f"oracle solve failed ({rc})")
     k = nit.value
     return dict(x=x, b=b, defects=defects[:(k + 1) * nb].reshape(k + 1, nb), nit=k, converged=bool(conv.value))
+
+
+def bcgs_solve(H, top, p, rhs, x0=None, red=1e-8, abslimit=1e-300, maxiter=200, restart=0, use_iter=True,
+               do_defect=True):
+    """BCGSSolver restated; returns dict(x, b, defects[nd, b], nit, converged)."""
+    lv = H.levels[top]
+    nb = lv.b
+    x = np.zeros(lv.n * nb) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
+    b = np.array(rhs, dtype=np.float64, copy=True)
+    ab, rd = np.full(3, abslimit), np.full(3, red)
+    maxdef = 2 * maxiter + 1
+    defects = np.zeros(maxdef * nb)
+    nd, nit, conv = C.c_int(), C.c_int(), C.c_int()
+    rc = lib().orc_bcgs_solve(H.ptr(0), top, C.byref(p), int(use_iter), x.ctypes.data, b.ctypes.data,
+                              int(do_defect), ab.ctypes.data, rd.ctypes.data, maxiter, restart,
+                              defects.ctypes.data, maxdef, C.byref(nd), C.byref(nit), C.byref(conv))
+    if rc:
+        raise RuntimeError(f"oracle bcgs failed ({rc})")
+    k = min(nd.value, maxdef)
+    return dict(x=x, b=b, defects=defects[:k * nb].reshape(k, nb), nit=nit.value, converged=bool(conv.value))

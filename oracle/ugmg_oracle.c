@@ -678,3 +678,132 @@ int orc_ls_lmgc_solve(const orc_level *levels, int top, const orc_params *p,
   mg_free(&mg);
   return err;
 }
+
+/* ------------------------------------------------------------- BiCGStab */
+
+static double ddotw_surface(const orc_level *L, const double *x, const double *y) {
+  /* ddotw (ugblas.cc:3023-3044) with unit weights, ON_SURFACE */
+  const int b = L->b;
+  double a[ORC_MAXB] = {0.0, 0.0, 0.0};
+  for (int v = 0; v < L->n; ++v) {
+    if (L->vclass[v] < NEWDEF_CLASS) continue;
+    for (int i = 0; i < b; ++i) a[i] += x[(size_t)v * b + i] * y[(size_t)v * b + i];
+  }
+  double s = 0.0;
+  for (int i = 0; i < b; ++i) s += a[i];
+  return s;
+}
+
+/* sc_eq (npscan.cc:1095-1104) */
+static int sc_eq(int b, const double *x, const double *y, double ac) {
+  for (int i = 0; i < b; ++i)
+    if (x[i] < 0.0 || y[i] < 0.0 || fabs(x[i] - y[i]) > ac * sqrt(x[i] * y[i])) return 0;
+  return 1;
+}
+
+/* BCGSSolver (np/procs/ls.cc:1864-2062) with Iter = lmgc (or none).
+ * defects receives the first defect and every LinearResiduum of the loop
+ * (two per iteration), ndef their count. */
+int orc_bcgs_solve(const orc_level *levels, int top, const orc_params *p, int use_iter,
+                   double *x, double *b, int do_defect, const double *abslimit,
+                   const double *reduction, int maxiter, int restart, double *defects,
+                   int maxdef, int *ndef, int *nit, int *converged) {
+  orc_mg mg;
+  const orc_level *L = &levels[top];
+  const int nb = L->b;
+  const size_t len = (size_t)L->n * nb;
+  int err = mg_setup(&mg, levels, top, p);
+  if (err) { mg_free(&mg); return err; }
+  double *r = (double *)calloc(len + 1, sizeof(double));
+  double *pv = (double *)calloc(len + 1, sizeof(double));
+  double *v = (double *)calloc(len + 1, sizeof(double));
+  double *sv = (double *)calloc(len + 1, sizeof(double));
+  double *t = (double *)calloc(len + 1, sizeof(double));
+  double *q = (double *)calloc(len + 1, sizeof(double));
+  double first[ORC_MAXB], last[ORC_MAXB], old[ORC_MAXB], d2r[ORC_MAXB];
+  int nd = 0, its = 0, conv = 0;
+  if (do_defect) surface_dmatmul_minus(L, b, x);
+  surface_nrm2x(L, b, first);
+  for (int i = 0; i < nb; ++i) {
+    last[i] = first[i];
+    old[i] = -1.0;
+    d2r[i] = first[i] * reduction[i];
+    if (d2r[i] == 0.0) d2r[i] = reduction[i];
+    if (nd < maxdef) defects[nd * nb + i] = first[i];
+  }
+  ++nd;
+  conv = sc_cmp(nb, first, abslimit);
+  double alpha = 0.0, rho = 0.0, omega = 0.0, rho_new = 0.0, beta = 0.0, tt = 0.0;
+  int need_restart = 1, eq_count = 0;
+  /* Iter(q, z): q = lmgc correction of defect z (z left untouched) */
+  #define BCGS_ITER(Z)                                                  \
+    do {                                                                \
+      memcpy(mg.b[top], (Z), len * sizeof(double));                     \
+      memset(mg.c[top], 0, len * sizeof(double));                       \
+      if (lmgc(&mg, top)) { err = 1; goto done; }                       \
+      memcpy(q, mg.c[top], len * sizeof(double));                       \
+    } while (0)
+  for (int i = 0; i < maxiter; ++i) {
+    if (conv) break;
+    if ((restart > 0 && i % restart == 0) || need_restart) {
+      memset(pv, 0, len * sizeof(double));
+      memset(v, 0, len * sizeof(double));
+      memcpy(r, b, len * sizeof(double));
+      alpha = rho = omega = 1.0;
+      need_restart = 0;
+    }
+    rho_new = ddotw_surface(L, b, r);
+    if (rho != 0.0 && omega != 0.0) beta = rho_new * alpha / rho / omega;
+    for (size_t k = 0; k < len; ++k) pv[k] *= beta;                    /* dscal */
+    for (size_t k = 0; k < len; ++k) pv[k] += b[k];                    /* dadd  */
+    for (size_t k = 0; k < len; ++k) pv[k] += -beta * omega * v[k];   /* daxpy */
+    if (use_iter) {
+      BCGS_ITER(pv);
+      orc_dmatmul(L, v, q);   /* v = A q (all rows = surface here) */
+      alpha = ddotw_surface(L, v, r);
+      if (alpha != 0.0) alpha = rho_new / alpha;
+      for (size_t k = 0; k < len; ++k) x[k] += alpha * q[k];
+    } else {
+      orc_dmatmul(L, v, pv);
+      alpha = ddotw_surface(L, v, r);
+      if (alpha != 0.0) alpha = rho_new / alpha;
+      for (size_t k = 0; k < len; ++k) x[k] += alpha * pv[k];
+    }
+    ++its;
+    for (size_t k = 0; k < len; ++k) sv[k] = b[k] + -alpha * v[k];
+    surface_nrm2x(L, sv, last);
+    if (nd < maxdef) for (int c = 0; c < nb; ++c) defects[nd * nb + c] = last[c];
+    ++nd;
+    if (sc_cmp(nb, last, abslimit) || sc_cmp(nb, last, d2r)) {
+      memcpy(b, sv, len * sizeof(double));
+      conv = 1;
+      break;
+    }
+    if (use_iter) BCGS_ITER(sv);
+    else memcpy(q, sv, len * sizeof(double));
+    orc_dmatmul(L, t, q);
+    tt = ddotw_surface(L, t, t);
+    omega = ddotw_surface(L, sv, t);
+    if (tt != 0.0) omega /= tt;
+    for (size_t k = 0; k < len; ++k) x[k] += omega * q[k];
+    for (size_t k = 0; k < len; ++k) b[k] = sv[k] + -omega * t[k];
+    rho = rho_new;
+    surface_nrm2x(L, b, last);
+    if (nd < maxdef) for (int c = 0; c < nb; ++c) defects[nd * nb + c] = last[c];
+    ++nd;
+    ++its;
+    if (sc_cmp(nb, last, abslimit) || sc_cmp(nb, last, d2r)) { conv = 1; break; }
+    if (sc_eq(nb, last, old, 1e-4)) ++eq_count;
+    else eq_count = 0;
+    for (int c = 0; c < nb; ++c) old[c] = last[c];
+    if (eq_count > 4) { conv = 0; break; }
+  }
+  #undef BCGS_ITER
+done:
+  *ndef = nd;
+  *nit = its;
+  *converged = conv;
+  free(r); free(pv); free(v); free(sv); free(t); free(q);
+  mg_free(&mg);
+  return err;
+}

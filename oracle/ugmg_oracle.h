@@ -118,6 +118,14 @@ int orc_ls_lmgc_solve(const orc_level *levels, int top, const orc_params *p,
 int orc_lmgc_iter(const orc_level *levels, int top, const orc_params *p,
                   double *c, double *b);
 
+/* BCGSSolver (np/procs/ls.cc:1864-2062), right-preconditioned by one
+ * lmgc cycle per application (use_iter) or unpreconditioned.  defects gets
+ * the first defect and each LinearResiduum of the loop (up to maxdef). */
+int orc_bcgs_solve(const orc_level *levels, int top, const orc_params *p, int use_iter,
+                   double *x, double *b, int do_defect, const double *abslimit,
+                   const double *reduction, int maxiter, int restart, double *defects,
+                   int maxdef, int *ndef, int *nit, int *converged);
+
 #ifdef __cplusplus
 }
 #endif

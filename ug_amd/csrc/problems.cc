@@ -56,6 +56,7 @@ const int kHexCorner[8] = {0, 1, 3, 2, 4, 5, 7, 6};
 struct ugmg_problem {
   int dim = 0;
   int b = 1;
+  double theta = 0.0;
   std::vector<Level> lv;
   std::vector<double> rhs;
 };
@@ -368,6 +369,179 @@ extern "C" int ugmg_problem_q1(int dim, int coarse_cells, int nref,
   *out = p;
   return 0;
 }
+
+// ----------------------------------------------------------------------
+// P1 triangles, streamline-upwind Petrov-Galerkin convection-diffusion
+//   -eps Laplace u + b . grad u = f,  b = (cos theta, sin theta)
+// Unit square, 2x2 coarse squares each split by the (0,0)-(1,1) diagonal
+// (8 triangles), uniform red refinement (every triangle into 4), so level l
+// has (2*2^l+1)^2 nodes; every level is rediscretised with its own h.
+// Element matrix: eps (grad phi_j, grad phi_i) + (b.grad phi_j, phi_i)
+//               + tau (b.grad phi_j, b.grad phi_i),
+// tau = h/(2|b|) (coth Pe - 1/Pe), Pe = |b| h / (2 eps);
+// rhs: (f_h, phi_i) + tau (f_h, b.grad phi_i) with f_h the P1 interpolant.
+namespace {
+
+void p1_element(const double xy[3][2], double eps, const double bvec[2], double tau,
+                double K[3][3], double M[3][3], double G[3]) {
+  const double x0 = xy[0][0], y0 = xy[0][1], x1 = xy[1][0], y1 = xy[1][1], x2 = xy[2][0],
+               y2 = xy[2][1];
+  const double det = (x1 - x0) * (y2 - y0) - (x2 - x0) * (y1 - y0);
+  const double area = 0.5 * std::fabs(det);
+  double gr[3][2] = {{(y1 - y2) / det, (x2 - x1) / det},
+                     {(y2 - y0) / det, (x0 - x2) / det},
+                     {(y0 - y1) / det, (x1 - x0) / det}};
+  for (int i = 0; i < 3; ++i) {
+    const double bgi = bvec[0] * gr[i][0] + bvec[1] * gr[i][1];
+    G[i] = tau * bgi * area;  // weight of mean(f) in the SUPG rhs term
+    for (int j = 0; j < 3; ++j) {
+      const double bgj = bvec[0] * gr[j][0] + bvec[1] * gr[j][1];
+      K[i][j] = eps * area * (gr[i][0] * gr[j][0] + gr[i][1] * gr[j][1]) + area / 3.0 * bgj +
+                tau * area * bgi * bgj;
+      M[i][j] = area / 12.0 * (i == j ? 2.0 : 1.0);
+    }
+  }
+}
+
+void assemble_p1_level(Level &L, double eps, const double bvec[2], std::vector<double> *rhs,
+                       const std::vector<double> *f) {
+  const int N = L.N, C = L.C;
+  const double h = 1.0 / C;
+  const double pe = h / (2.0 * eps);
+  const double tau = pe > 1e-8 ? h / 2.0 * (1.0 / std::tanh(pe) - 1.0 / pe) : 0.0;
+  const int64_t n = (int64_t)N * N;
+  // offsets (dx,dy) coupled by a triangle of this split, ascending column order
+  const int offs[7][2] = {{-1, -1}, {0, -1}, {-1, 0}, {0, 0}, {1, 0}, {0, 1}, {1, 1}};
+  std::vector<double> st((size_t)n * 7, 0.0);
+  if (rhs) rhs->assign(n, 0.0);
+  auto slot = [&](int dx, int dy) {
+    for (int k = 0; k < 7; ++k)
+      if (offs[k][0] == dx && offs[k][1] == dy) return k;
+    return -1;
+  };
+  for (int cy = 0; cy < C; ++cy)
+    for (int cx = 0; cx < C; ++cx)
+      for (int t = 0; t < 2; ++t) {
+        // lower triangle (a,b,c), upper (a,c,d); a = (cx,cy) ... d = (cx,cy+1)
+        const int vx[2][3] = {{cx, cx + 1, cx + 1}, {cx, cx + 1, cx}};
+        const int vy[2][3] = {{cy, cy, cy + 1}, {cy, cy + 1, cy + 1}};
+        double xy[3][2];
+        int64_t id[3];
+        for (int q = 0; q < 3; ++q) {
+          xy[q][0] = vx[t][q] * h;
+          xy[q][1] = vy[t][q] * h;
+          id[q] = (int64_t)vy[t][q] * N + vx[t][q];
+        }
+        double K[3][3], M[3][3], G[3];
+        p1_element(xy, eps, bvec, tau, K, M, G);
+        for (int i = 0; i < 3; ++i) {
+          for (int j = 0; j < 3; ++j) {
+            const int k = slot(vx[t][j] - vx[t][i], vy[t][j] - vy[t][i]);
+            st[(size_t)id[i] * 7 + k] += K[i][j];
+          }
+          if (rhs) {
+            double fm = 0.0;
+            for (int j = 0; j < 3; ++j) {
+              (*rhs)[id[i]] += M[i][j] * (*f)[id[j]];
+              fm += (*f)[id[j]];
+            }
+            (*rhs)[id[i]] += G[i] * fm / 3.0;
+          }
+        }
+      }
+  L.rp.assign(n + 1, 0);
+  for (int64_t i = 0; i < n; ++i) {
+    const int x = (int)(i % N), y = (int)(i / N);
+    int cnt = 0;
+    for (int k = 0; k < 7; ++k) {
+      const int xx = x + offs[k][0], yy = y + offs[k][1];
+      cnt += (xx >= 0 && xx < N && yy >= 0 && yy < N);
+    }
+    L.rp[i + 1] = L.rp[i] + cnt;
+  }
+  L.ci.resize(L.rp[n]);
+  L.v.resize(L.rp[n]);
+  L.skip.assign(n, 0);
+  for (int64_t i = 0; i < n; ++i) {
+    const int x = (int)(i % N), y = (int)(i / N);
+    int64_t q = L.rp[i];
+    L.ci[q] = (int)i;
+    L.v[q] = st[(size_t)i * 7 + 3];
+    ++q;
+    for (int k = 0; k < 7; ++k) {
+      if (k == 3) continue;
+      const int xx = x + offs[k][0], yy = y + offs[k][1];
+      if (xx < 0 || xx >= N || yy < 0 || yy >= N) continue;
+      L.ci[q] = yy * N + xx;
+      L.v[q] = st[(size_t)i * 7 + k];
+      ++q;
+    }
+    if (x == 0 || x == N - 1 || y == 0 || y == N - 1) {
+      L.skip[i] = 1u;
+      for (int64_t m = L.rp[i]; m < L.rp[i + 1]; ++m) L.v[m] = 0.0;
+      L.v[L.rp[i]] = 1.0;
+      if (rhs) (*rhs)[i] = 0.0;
+    }
+  }
+  L.vc.assign(n, 3);
+}
+
+// P1 interpolation: corner nodes take their father, edge midpoints 1/2 of
+// the edge's two end points (GNs of the father triangle, zero weight dropped)
+void build_p1_transfer(const Level &coarse, Level &fine) {
+  const int Nf = fine.N, Nc = coarse.N;
+  const int64_t n = (int64_t)Nf * Nf;
+  fine.pp.assign(n + 1, 0);
+  fine.pi.clear();
+  fine.pw.clear();
+  for (int64_t i = 0; i < n; ++i) {
+    const int x = (int)(i % Nf), y = (int)(i / Nf);
+    if (x % 2 == 0 && y % 2 == 0) {
+      fine.pi.push_back((y / 2) * Nc + x / 2);
+      fine.pw.push_back(1.0);
+    } else {
+      const int x0 = x / 2, y0 = y / 2;                 // lower end point
+      const int x1 = (x + 1) / 2, y1 = (y + 1) / 2;     // upper end point
+      fine.pi.push_back(y0 * Nc + x0);
+      fine.pw.push_back(0.5);
+      fine.pi.push_back(y1 * Nc + x1);
+      fine.pw.push_back(0.5);
+    }
+    fine.pp[i + 1] = (int)fine.pi.size();
+  }
+}
+
+}  // namespace
+
+extern "C" int ugmg_problem_p1_cd(int nref, double eps, double theta, uint64_t seed,
+                                  ugmg_problem **out) {
+  if (!out || nref < 0 || nref > 11 || !(eps > 0.0)) return -1;
+  ugmg_problem *p = new (std::nothrow) ugmg_problem;
+  if (!p) return -4;
+  p->dim = 2;
+  p->b = 1;
+  uint64_t s = seed;
+  if (std::isnan(theta)) theta = 2.0 * M_PI * splitmix_u01(s);
+  const double bvec[2] = {std::cos(theta), std::sin(theta)};
+  const int L = nref;
+  p->lv.resize(L + 1);
+  for (int l = 0; l <= L; ++l) {
+    p->lv[l].C = 2 << l;
+    p->lv[l].N = p->lv[l].C + 1;
+    p->lv[l].n = p->lv[l].N * p->lv[l].N;
+  }
+  std::vector<double> f(p->lv[L].n);
+  for (auto &fi : f) fi = -1.0 + 2.0 * splitmix_u01(s);
+  for (int l = 0; l <= L; ++l)
+    assemble_p1_level(p->lv[l], eps, bvec, l == L ? &p->rhs : nullptr, l == L ? &f : nullptr);
+  for (int l = 0; l <= L; ++l) p->lv[l].vnc.assign(p->lv[l].n, l < L ? 3 : 0);
+  for (int l = 1; l <= L; ++l) build_p1_transfer(p->lv[l - 1], p->lv[l]);
+  p->theta = theta;
+  *out = p;
+  return 0;
+}
+
+extern "C" double ugmg_problem_theta(const ugmg_problem *p) { return p ? p->theta : 0.0; }
 
 extern "C" void ugmg_problem_free(ugmg_problem *p) { delete p; }
 

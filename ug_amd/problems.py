@@ -41,6 +41,10 @@ def _load():
         lib.ugmg_problem_rhs.argtypes = [P, C.POINTER(dp)]
         lib.ugmg_problem_grid.argtypes = [P, C.c_int, ip, ip]
         lib.ugmg_splitmix_uniform.argtypes = [C.c_uint64, C.c_int64, C.c_double, C.c_double, dp]
+        lib.ugmg_problem_p1_cd.argtypes = [C.c_int, C.c_double, C.c_double, C.c_uint64, C.POINTER(P)]
+        lib.ugmg_problem_p1_cd.restype = C.c_int
+        lib.ugmg_problem_theta.argtypes = [P]
+        lib.ugmg_problem_theta.restype = C.c_double
         lib.ugmg_splitmix_uniform.restype = None
         _lib = lib
     return _lib
@@ -96,6 +100,7 @@ class Problem:
     levels: list = field(default_factory=list)
     rhs: np.ndarray | None = None
     name: str = ""
+    theta: float = 0.0
 
     @property
     def top(self) -> int:
@@ -114,6 +119,24 @@ def q1(dim: int, coarse_cells: int, nref: int, kappa: int = KAPPA_ONE, seed: int
     rc = lib.ugmg_problem_q1(dim, coarse_cells, nref, kappa, C.c_uint64(seed), C.byref(h))
     if rc != 0:
         raise ValueError(f"ugmg_problem_q1 failed ({rc})")
+    return _extract(lib, h, dim, name or f"q1_{dim}d_{coarse_cells}c_{nref}r")
+
+
+def p1_cd(nref: int, eps: float = 1e-3, theta: float = float("nan"), seed: int = 0, name: str = "") -> Problem:
+    """2D SUPG P1 convection-diffusion (see include/ugmg_problems.h); level
+    nref has (2^(nref+1)+1)^2 nodes.  theta = NaN draws the wind angle."""
+    lib = _load()
+    h = C.c_void_p()
+    rc = lib.ugmg_problem_p1_cd(nref, eps, theta, C.c_uint64(seed), C.byref(h))
+    if rc != 0:
+        raise ValueError(f"ugmg_problem_p1_cd failed ({rc})")
+    th = lib.ugmg_problem_theta(h)
+    P = _extract(lib, h, 2, name or f"p1cd_{2 ** (nref + 1) + 1}^2")
+    P.theta = th
+    return P
+
+
+def _extract(lib, h, dim, name):
     try:
         nl = lib.ugmg_problem_nlevels(h)
         b = lib.ugmg_problem_block(h)
@@ -147,7 +170,7 @@ def q1(dim: int, coarse_cells: int, nref: int, kappa: int = KAPPA_ONE, seed: int
         rhs = _arr(rb, levels[-1].n * b, np.float64)
     finally:
         lib.ugmg_problem_free(h)
-    return Problem(dim=dim, levels=levels, rhs=rhs, name=name or f"q1_{dim}d_{coarse_cells}c_{nref}r")
+    return Problem(dim=dim, levels=levels, rhs=rhs, name=name)
 
 
 def splitmix_uniform(seed: int, count: int, a: float = 0.0, b: float = 1.0) -> np.ndarray:

@@ -156,6 +156,39 @@ int ugmg_solve_host(ugmg_ctx *ctx, int level, double *x, double *b,
                     int maxiter, int do_defect, ugmg_lresult *res,
                     ugmg_defect_cb cb, void *user);
 
+/* ---- several right-hand sides per launch --------------------------------
+ * UG solves one right-hand side per call; these entry points run nrhs
+ * independent columns through the same kernels (one launch per operation,
+ * the column index in the grid), so the matrix of each level is read once
+ * per operation for all columns.  Column j of a vector argument starts at
+ * j * ld (ld >= n), device memory.  Every column follows exactly the
+ * recurrence of a single-column call.
+ * ugmg_set_batch: columns the level vectors are laid out for (default 1);
+ *   changing it frees the device hierarchy (call ugmg_preprocess again).
+ * ugmg_iter_batch: NP_ITER on each column (lmgc, iter.cc:7741-7949). */
+int ugmg_set_batch(ugmg_ctx *ctx, int nrhs);
+int ugmg_iter_batch(ugmg_ctx *ctx, int level, int nrhs, double *c, double *b,
+                    int ld);
+
+/* ---- NP_LINEAR_SOLVER bcgs: BCGSSolver (ls.cc:1864-2062) ----------------
+ * Right-preconditioned BiCGStab with the lmgc cycle as $I (use_iter; FP32
+ * cycle when the precision is UGMG_MIXED, FP64 vectors, dots and norms
+ * throughout), restart every $R iterations (restart > 0), unit weights $w,
+ * stop on abslimit / reduction per column or after 5 near-equal defects
+ * (sc_eq 1e-4).  nrhs columns at once (see above); res has nrhs entries
+ * (number_of_linear_iterations counts both half steps, as UG does).
+ * defects (host, may be NULL): column j's defect sequence -- the first and
+ * every LinearResiduum -- at defects[j * maxdef ...]. */
+int ugmg_bcgs_solve(ugmg_ctx *ctx, int level, int nrhs, double *x, double *b,
+                    int ld, const double *abslimit, const double *reduction,
+                    int maxiter, int restart, int use_iter, int do_defect,
+                    ugmg_lresult *res, double *defects, int maxdef);
+int ugmg_bcgs_solve_host(ugmg_ctx *ctx, int level, int nrhs, double *x,
+                         double *b, int ld, const double *abslimit,
+                         const double *reduction, int maxiter, int restart,
+                         int use_iter, int do_defect, ugmg_lresult *res,
+                         double *defects, int maxdef);
+
 /* ---- host-only setup routines (no GPU needed) ---------------------------
  * The pieces of PreProcess that run on the host, callable on their own.
  * ugmg_color: greedy first-fit colouring in list order over the symmetrised

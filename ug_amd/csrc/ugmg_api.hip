@@ -61,6 +61,7 @@ struct DevLevel {
   int p_nnz = 0, pt_nnz = 0;
   DevSellBuf P, PT;   // interpolation from / restriction to level-1
   bool nt = false;    // stream the matrix with non-temporal loads
+  size_t vs = 0;      // column stride of the level vectors (batched launches)
   double *c = nullptr, *b = nullptr, *t = nullptr;
   float *c32 = nullptr, *b32 = nullptr, *t32 = nullptr;
 
@@ -106,6 +107,20 @@ struct ugmg_ctx {
   double *base_first = nullptr;
   int base_nparts = 0;
   bool base_b_dead = false;   // every base vector has VNCLASS >= 2
+  // batched launches: level vectors hold nrhs_cap columns; nb = columns of
+  // the work being enqueued
+  int nrhs_cap = 1, nb = 1;
+  // BCGSSolver work (allocated on first use, freed with the hierarchy)
+  struct Bcgs {
+    int nrhs = 0, maxdef = 0, npart = 0;
+    double *x = nullptr, *b = nullptr, *r = nullptr, *p = nullptr, *v = nullptr, *s = nullptr,
+           *t = nullptr, *partA = nullptr, *partB = nullptr, *defects = nullptr;
+    BcgsState *state = nullptr;
+    hipGraphExec_t g[2] = {nullptr, nullptr};   // iteration without / with restart
+    int glevel = -1, gnrhs = 0, gprec = -1, giter = -1, gk = -1, gmaxdef = -1;
+  } bc;
+  BcgsState *state_host = nullptr;   // pinned
+  int state_host_cap = 0;
 };
 
 namespace {
@@ -161,12 +176,22 @@ int upload_sell(ugmg_ctx *ctx, const HostSell &S, DevSellBuf &D) {
   return 0;
 }
 
+void drop_bcgs_graphs(ugmg_ctx *ctx) {
+  for (auto &g : ctx->bc.g) {
+    if (g) hipGraphExecDestroy(g);
+    g = nullptr;
+  }
+  ctx->bc.glevel = -1;
+}
+
 void free_device(ugmg_ctx *ctx) {
   if (ctx->gexec) {
     hipGraphExecDestroy(ctx->gexec);
     ctx->gexec = nullptr;
   }
   ctx->gkind = -1;
+  drop_bcgs_graphs(ctx);
+  ctx->bc = ugmg_ctx::Bcgs();
   for (void *p : ctx->allocs) hipFree(p);
   ctx->allocs.clear();
   ctx->dev_bytes = 0.0;
@@ -187,7 +212,11 @@ void drop_graph(ugmg_ctx *ctx) {
   if (ctx->gexec) hipGraphExecDestroy(ctx->gexec);
   ctx->gexec = nullptr;
   ctx->gkind = -1;
+  drop_bcgs_graphs(ctx);
 }
+
+inline Batch batch(const ugmg_ctx *ctx, int l) { return Batch{ctx->nb, ctx->dev[l].vs}; }
+inline Batch B1(const DevLevel &d) { return Batch{1, d.vs}; }
 
 // ------------------------------------------------------------ cycle drivers
 
@@ -196,17 +225,18 @@ void smooth_step(ugmg_ctx *ctx, int l, T *t, T *b, T *c, int cmode, T *xout) {
   DevLevel &d = ctx->dev[l];
   hipStream_t st = ctx->stream;
   const T damp = (T)ctx->p.damp[0];
+  const Batch bt = batch(ctx, l);
   if (ctx->p.smoother == UGMG_SMOOTH_JAC) {
-    launch_jac<T>(st, d.n, b, d.diag<T>(), d.vclass, damp, t);
+    launch_jac<T>(st, d.n, b, d.diag<T>(), d.vclass, damp, t, bt);
     const int mode = cmode < 0 ? 0 : (cmode == 0 ? 3 : 2);
-    launch_spmv<T, T>(st, d.A.d<T>(), 0, d.A.nslices, t, b, c, mode, d.nt);
+    launch_spmv<T, T, T>(st, d.A.d<T>(), 0, d.A.nslices, t, b, c, mode, d.nt, bt);
   } else {
     for (int k = 0; k < d.ncolors; ++k)
       launch_gs_color<T, T>(st, d.L.d<T>(), d.color_slices[k],
                             d.color_slices[k + 1] - d.color_slices[k], b, d.diag<T>(),
-                            d.vclass, t, d.nt);
+                            d.vclass, t, d.nt, bt);
     launch_gs_defect<T, T>(st, d.U.d<T>(), t, b, d.vclass, damp, cmode < 0 ? nullptr : c,
-                           cmode < 0 ? 1 : cmode, xout, d.nt);
+                           cmode < 0 ? 1 : cmode, xout, d.nt, bt);
   }
 }
 
@@ -253,19 +283,21 @@ void base_solve(ugmg_ctx *ctx, T *c, T *b, int cmode, bool have_parts, bool need
   DevLevel &d = ctx->dev[ctx->bl];
   hipStream_t st = ctx->stream;
   const ugmg_params &p = ctx->p;
-  int np = have_parts ? ctx->base_nparts : launch_dot_part<T>(st, d.n, b, d.vclass, ctx->base_part);
+  const Batch bt = batch(ctx, ctx->bl);
+  int np = have_parts ? ctx->base_nparts
+                      : launch_dot_part<T>(st, d.n, b, d.vclass, ctx->base_part, bt);
   T *x = d.tv<T>();
   const int m = std::max(0, p.base_maxiter);
-  if (m == 0 && cmode == 0) launch_fill<T>(st, d.n, c, (T)0);
+  if (m == 0 && cmode == 0) launch_fill<T>(st, d.n, c, (T)0, bt);
   for (int it = 0; it < m; ++it) {
     if (!ctx->base.failed)
       launch_base_gemv<T>(st, d.n, ctx->base_inv, b, c, x, it == 0 ? cmode : 1, p.base_damp[0],
                           p.base_abslimit[0], p.base_red[0], ctx->base_part, np,
-                          ctx->base_first, it);
+                          ctx->base_first, it, bt);
     const bool more = it + 1 < m;
     if (more || need_b || !ctx->base_b_dead)
-      launch_spmv<T, T>(st, d.A.d<T>(), 0, d.A.nslices, x, b, (T *)nullptr, 0, d.nt);
-    if (more) np = launch_dot_part<T>(st, d.n, b, d.vclass, ctx->base_part);
+      launch_spmv<T, T, T>(st, d.A.d<T>(), 0, d.A.nslices, x, b, (T *)nullptr, 0, d.nt, bt);
+    if (more) np = launch_dot_part<T>(st, d.n, b, d.vclass, ctx->base_part, bt);
   }
 }
 
@@ -274,7 +306,7 @@ template <typename T>
 void lmgc(ugmg_ctx *ctx, int l, T *c, T *b, int cmode) {
   const ugmg_params &p = ctx->p;
   hipStream_t st = ctx->stream;
-  if (l <= ctx->fuse_top) {
+  if (l <= ctx->fuse_top && ctx->nb == 1) {
     coarse_cycle<T>(ctx, l, c, b, cmode);
     return;
   }
@@ -285,13 +317,15 @@ void lmgc(ugmg_ctx *ctx, int l, T *c, T *b, int cmode) {
   DevLevel &d = ctx->dev[l];
   DevLevel &dc = ctx->dev[l - 1];
   T *t = d.tv<T>();
+  const int nb = ctx->nb;
   for (int i = 0; i < p.nu1; ++i) smooth_step<T>(ctx, l, t, b, c, i == 0 ? cmode : 1, nullptr);
-  if (p.nu1 == 0 && cmode == 0) launch_fill<T>(st, d.n, c, (T)0);
+  if (p.nu1 == 0 && cmode == 0) launch_fill<T>(st, d.n, c, (T)0, batch(ctx, l));
   // RestrictDefect(level, b, b, Factor_One)
-  const bool to_base = l - 1 == ctx->bl && ctx->fuse_top < ctx->bl;
+  const bool to_base = l - 1 == ctx->bl && (ctx->fuse_top < ctx->bl || nb > 1);
   const int np = launch_restrict<T>(st, d.PT.d<T>(), dc.skip, dc.vnclass, dc.vclass, b, (T)1,
-                                    dc.bv<T>(), to_base ? ctx->base_part : nullptr, d.nt);
-  if (p.gamma <= 0) launch_fill<T>(st, dc.n, dc.cv<T>(), (T)0);
+                                    dc.bv<T>(), to_base ? ctx->base_part : nullptr, d.nt, nb,
+                                    d.vs, dc.vs);
+  if (p.gamma <= 0) launch_fill<T>(st, dc.n, dc.cv<T>(), (T)0, batch(ctx, l - 1));
   for (int g = 0; g < p.gamma; ++g) {
     if (to_base) {
       // the first visit uses the restriction's partial norms; after a
@@ -303,8 +337,10 @@ void lmgc(ugmg_ctx *ctx, int l, T *c, T *b, int cmode) {
     }
   }
   // InterpolateCorrection(level, t, c, damp); dadd(c, t); dmatmul_minus(b, A, t)
-  launch_interp<T>(st, d.P.d<T>(), d.skip, dc.cv<T>(), (T)p.lmgc_damp[0], t, c, d.nt);
-  launch_spmv<T, T>(st, d.A.d<T>(), 0, d.A.nslices, t, b, (T *)nullptr, 0, d.nt);
+  launch_interp<T>(st, d.P.d<T>(), d.skip, dc.cv<T>(), (T)p.lmgc_damp[0], t, c, d.nt, nb, d.vs,
+                   dc.vs);
+  launch_spmv<T, T, T>(st, d.A.d<T>(), 0, d.A.nslices, t, b, (T *)nullptr, 0, d.nt,
+                       batch(ctx, l));
   for (int i = 0; i < p.nu2; ++i) smooth_step<T>(ctx, l, t, b, c, 1, nullptr);
 }
 
@@ -318,7 +354,7 @@ int check_ready(ugmg_ctx *ctx, int level) {
 
 int surface_norm(ugmg_ctx *ctx, int level, const double *x, double *out) {
   DevLevel &d = ctx->dev[level];
-  launch_dot<double>(ctx->stream, d.n, x, nullptr, d.vclass, ctx->part, ctx->red, 1);
+  launch_dot<double>(ctx->stream, d.n, x, nullptr, d.vclass, ctx->part, ctx->red, 1, Batch{1, d.vs});
   HIPCHK(hipMemcpyAsync(ctx->red_host, ctx->red, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   *out = ctx->red_host[0];
@@ -333,11 +369,13 @@ void enqueue_iteration(ugmg_ctx *ctx, int level, double *x, double *b) {
     const int k = std::max(1, ctx->p.inner_cycles);
     for (int j = 0; j < k; ++j) lmgc<float>(ctx, level, d.c32, d.b32, j == 0 ? 0 : 1);
     int np = 0;
-    launch_ir_update(ctx->stream, d.A.d64(), d.c32, b, x, d.b32, d.vclass, ctx->part, &np, d.nt);
-    launch_reduce_final(ctx->stream, np, ctx->part, ctx->red, 1);
+    launch_ir_update(ctx->stream, d.A.d64(), d.c32, b, x, d.b32, d.vclass, ctx->part, &np, d.nt,
+                     Batch{1, d.vs});
+    launch_reduce_final(ctx->stream, np, ctx->part, ctx->red, 1, 1);
   } else {
     lmgc<double>(ctx, level, x, b, 1);
-    launch_dot<double>(ctx->stream, d.n, b, nullptr, d.vclass, ctx->part, ctx->red, 1);
+    launch_dot<double>(ctx->stream, d.n, b, nullptr, d.vclass, ctx->part, ctx->red, 1,
+                       Batch{1, d.vs});
   }
 }
 
@@ -386,6 +424,122 @@ inline bool sc_cmp1(double x, double y) { return std::fabs(x) < std::fabs(y); }
 
 double spmv_bytes(const DevLevel &d, int sv, int sx) {
   return (double)d.nnzb * (sv + 4) + 4.0 * (d.n + 1) + (double)d.n * (sx + 2.0 * sx);
+}
+
+// ------------------------------------------------------------ BCGSSolver
+// Work of BCGSSolver (np/procs/ls.cc:1864-2062) on nb columns of level
+// `level`: vectors x, b, r, p, v, s, t in the level's column stride; the
+// cycle's input and output are the level's own c/b vectors (FP64) or
+// c32/b32 (FP32 cycle), so q lives in c (c32).
+int bcgs_alloc(ugmg_ctx *ctx, int level, int nrhs, int maxdef) {
+  auto &w = ctx->bc;
+  const DevLevel &d = ctx->dev[level];
+  const int np = std::max(bcgs_nparts(d.n), bcgs_spmv_nparts(d.A.nslices));
+  if (w.x && w.nrhs >= nrhs && w.maxdef >= maxdef && w.npart >= np) return 0;
+  drop_bcgs_graphs(ctx);
+  // earlier buffers stay in ctx->allocs until the hierarchy is freed
+  const int nr = std::max(nrhs, w.nrhs);
+  const int md = std::max(std::max(maxdef, w.maxdef), 1);
+  const size_t nv = d.vs * (size_t)nr;
+  int rc;
+  if ((rc = dalloc(ctx, &w.x, nv)) || (rc = dalloc(ctx, &w.b, nv)) || (rc = dalloc(ctx, &w.r, nv)) ||
+      (rc = dalloc(ctx, &w.p, nv)) || (rc = dalloc(ctx, &w.v, nv)) || (rc = dalloc(ctx, &w.s, nv)) ||
+      (rc = dalloc(ctx, &w.t, nv)) || (rc = dalloc(ctx, &w.partA, (size_t)2 * np * nr)) ||
+      (rc = dalloc(ctx, &w.partB, (size_t)2 * np * nr)) ||
+      (rc = dalloc(ctx, &w.defects, (size_t)md * nr)) || (rc = dalloc(ctx, &w.state, nr)))
+    return rc;
+  for (double *v : {w.x, w.b, w.r, w.p, w.v, w.s, w.t}) HIPCHK(hipMemset(v, 0, sizeof(double) * nv));
+  w.nrhs = nr;
+  w.maxdef = md;
+  w.npart = np;
+  if (ctx->state_host_cap < nr) {
+    if (ctx->state_host) hipHostFree(ctx->state_host);
+    ctx->state_host = nullptr;
+    ctx->state_host_cap = 0;
+    HIPCHK(hipHostMalloc((void **)&ctx->state_host, sizeof(BcgsState) * nr, hipHostMallocDefault));
+    ctx->state_host_cap = nr;
+  }
+  return 0;
+}
+
+// the preconditioner Iter(q, w): c := cycle correction of the input held
+// in the level's b (b32), which the cycle overwrites
+void bcgs_precond(ugmg_ctx *ctx, int level, bool mixed) {
+  DevLevel &d = ctx->dev[level];
+  if (mixed) {
+    const int k = std::max(1, ctx->p.inner_cycles);
+    for (int j = 0; j < k; ++j) lmgc<float>(ctx, level, d.c32, d.b32, j == 0 ? 0 : 1);
+  } else {
+    lmgc<double>(ctx, level, d.c, d.b, 0);
+  }
+}
+
+// one loop pass of BCGSSolver on every active column
+void enqueue_bcgs_iteration(ugmg_ctx *ctx, int level, bool use_iter, int restart,
+                            double abslimit, double red, int maxdef) {
+  auto &w = ctx->bc;
+  DevLevel &d = ctx->dev[level];
+  hipStream_t st = ctx->stream;
+  const Batch bt = batch(ctx, level);
+  const int nb = ctx->nb, n = d.n;
+  const bool mixed = use_iter && ctx->p.precision == UGMG_MIXED;
+  const int npv = bcgs_nparts(n), nps = bcgs_spmv_nparts(d.A.nslices);
+  const DSell<double> A = d.A.d64();
+  // no preconditioner: q = p and q = s, held in c
+  double *w64 = !use_iter ? d.c : (mixed ? nullptr : d.b);
+  float *w32 = mixed ? d.b32 : nullptr;
+  launch_bcgs_scalar(st, BCGS_RHO, npv, w.partA, w.state, nb, restart, abslimit, red, w.defects, maxdef);
+  launch_bcgs_p(st, n, w.b, w.r, w.v, w.p, w64, w32, w.state, restart, bt);
+  if (use_iter) bcgs_precond(ctx, level, mixed);
+  if (mixed) launch_bcgs_spmv<float>(st, A, d.c32, w.v, w.r, d.vclass, w.partB, 0, 0, w.state, d.nt, bt);
+  else launch_bcgs_spmv<double>(st, A, d.c, w.v, w.r, d.vclass, w.partB, 0, 0, w.state, d.nt, bt);
+  launch_bcgs_scalar(st, BCGS_ALPHA, nps, w.partB, w.state, nb, restart, abslimit, red, w.defects, maxdef);
+  if (mixed) launch_bcgs_s<float>(st, n, w.b, w.v, d.c32, w.x, w.s, w64, w32, d.vclass, w.partB, w.state, bt);
+  else launch_bcgs_s<double>(st, n, w.b, w.v, d.c, w.x, w.s, w64, w32, d.vclass, w.partB, w.state, bt);
+  launch_bcgs_scalar(st, BCGS_RES1, npv, w.partB, w.state, nb, restart, abslimit, red, w.defects, maxdef);
+  if (use_iter) bcgs_precond(ctx, level, mixed);
+  if (mixed) launch_bcgs_spmv<float>(st, A, d.c32, w.t, w.s, d.vclass, w.partB, 1, 1, w.state, d.nt, bt);
+  else launch_bcgs_spmv<double>(st, A, d.c, w.t, w.s, d.vclass, w.partB, 1, 1, w.state, d.nt, bt);
+  launch_bcgs_scalar(st, BCGS_OMEGA, nps, w.partB, w.state, nb, restart, abslimit, red, w.defects, maxdef);
+  if (mixed) launch_bcgs_b<float>(st, n, w.s, w.t, d.c32, w.r, w.x, w.b, d.vclass, w.partA, w.state, bt);
+  else launch_bcgs_b<double>(st, n, w.s, w.t, d.c, w.r, w.x, w.b, d.vclass, w.partA, w.state, bt);
+  launch_bcgs_scalar(st, BCGS_RES2, npv, w.partA, w.state, nb, restart, abslimit, red, w.defects, maxdef);
+}
+
+int launch_bcgs_iteration(ugmg_ctx *ctx, int level, bool use_iter, int restart, double abslimit,
+                          double red, int maxdef) {
+  auto &w = ctx->bc;
+  if (!can_capture(ctx)) {
+    enqueue_bcgs_iteration(ctx, level, use_iter, restart, abslimit, red, maxdef);
+    HIPCHK(hipGetLastError());
+    return 0;
+  }
+  const int k = ctx->p.inner_cycles;
+  if (w.glevel != level || w.gnrhs != ctx->nb || w.gprec != ctx->p.precision ||
+      w.giter != (int)use_iter || w.gk != k || w.gmaxdef != maxdef) {
+    drop_bcgs_graphs(ctx);
+    w.gmaxdef = maxdef;
+    w.glevel = level;
+    w.gnrhs = ctx->nb;
+    w.gprec = ctx->p.precision;
+    w.giter = use_iter;
+    w.gk = k;
+  }
+  hipGraphExec_t &ge = w.g[restart ? 1 : 0];
+  if (!ge) {
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_bcgs_iteration(ctx, level, use_iter, restart, abslimit, red, maxdef);
+    HIPCHK(hipStreamEndCapture(ctx->stream, &g));
+    hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      ge = nullptr;
+      return fail(ctx, UGMG_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+  }
+  HIPCHK(hipGraphLaunch(ge, ctx->stream));
+  return 0;
 }
 
 }  // namespace
@@ -476,14 +630,17 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
         if ((rc = upload_sell(ctx, hp.PT, D.PT))) return rc;
       }
       D.nt = (double)hp.A.col.size() * 12.0 > ps.nt_bytes;
-      if ((rc = dalloc(ctx, &D.c, D.n)) || (rc = dalloc(ctx, &D.b, D.n)) ||
-          (rc = dalloc(ctx, &D.t, D.n)) || (rc = dalloc(ctx, &D.c32, D.n)) ||
-          (rc = dalloc(ctx, &D.b32, D.n)) || (rc = dalloc(ctx, &D.t32, D.n)))
+      // columns padded to 64 entries (512-byte aligned FP64 columns)
+      D.vs = ((size_t)std::max(D.n, 1) + 63) / 64 * 64;
+      const size_t nv = D.vs * (size_t)ctx->nrhs_cap;
+      if ((rc = dalloc(ctx, &D.c, nv)) || (rc = dalloc(ctx, &D.b, nv)) ||
+          (rc = dalloc(ctx, &D.t, nv)) || (rc = dalloc(ctx, &D.c32, nv)) ||
+          (rc = dalloc(ctx, &D.b32, nv)) || (rc = dalloc(ctx, &D.t32, nv)))
         return rc;
       // every level vector starts at zero: the zero-valued phantom entries of
       // uniform slices gather from them before a sweep has written them
-      for (double *v : {D.c, D.b, D.t}) HIPCHK(hipMemset(v, 0, sizeof(double) * std::max(D.n, 1)));
-      for (float *v : {D.c32, D.b32, D.t32}) HIPCHK(hipMemset(v, 0, sizeof(float) * std::max(D.n, 1)));
+      for (double *v : {D.c, D.b, D.t}) HIPCHK(hipMemset(v, 0, sizeof(double) * nv));
+      for (float *v : {D.c32, D.b32, D.t32}) HIPCHK(hipMemset(v, 0, sizeof(float) * nv));
     }
     ctx->base = std::move(ps.base);
     if ((rc = upload(ctx, &ctx->base_inv, ctx->base.inv.data(), ctx->base.inv.size()))) return rc;
@@ -529,10 +686,13 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
     if ((rc = upload(ctx, &ctx->clev64, c64.data(), c64.size()))) return rc;
     if ((rc = upload(ctx, &ctx->clev32, c32.data(), c32.size()))) return rc;
     ctx->part_cap = std::max(1024, (max_slices + 3) / 4 + 1);
-    if ((rc = dalloc(ctx, &ctx->part, ctx->part_cap))) return rc;
-    if ((rc = dalloc(ctx, &ctx->red, 8))) return rc;
-    if ((rc = dalloc(ctx, &ctx->base_part, 1024))) return rc;
-    if ((rc = dalloc(ctx, &ctx->base_first, 8))) return rc;
+    const int nc = ctx->nrhs_cap;
+    if ((rc = dalloc(ctx, &ctx->part, (size_t)ctx->part_cap * nc))) return rc;
+    if ((rc = dalloc(ctx, &ctx->red, std::max(8, nc)))) return rc;
+    // restriction partials into the base level: one per block of 4 slices
+    const int base_parts = std::max(1024, (ctx->dev[std::min(bl + 1, level)].PT.nslices + 3) / 4 + 1);
+    if ((rc = dalloc(ctx, &ctx->base_part, (size_t)base_parts * nc))) return rc;
+    if ((rc = dalloc(ctx, &ctx->base_first, std::max(8, nc)))) return rc;
   } catch (const std::bad_alloc &) {
     free_device(ctx);
     return fail(ctx, UGMG_E_NOMEM, "host allocation during preprocess");
@@ -591,6 +751,7 @@ int ugmg_destroy(ugmg_ctx *ctx) {
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
   free_device(ctx);
   if (ctx->red_host) hipHostFree(ctx->red_host);
+  if (ctx->state_host) hipHostFree(ctx->state_host);
   if (ctx->own_stream && ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
   return UGMG_OK;
@@ -790,7 +951,7 @@ int ugmg_dmatmul_minus(ugmg_ctx *ctx, int level, double *x, const double *y) {
   int rc = check_ready(ctx, level);
   if (rc) return rc;
   DevLevel &d = ctx->dev[level];
-  launch_spmv<double, double>(ctx->stream, d.A.d64(), 0, d.A.nslices, y, x, nullptr, 0, d.nt);
+  launch_spmv<double, double, double>(ctx->stream, d.A.d64(), 0, d.A.nslices, y, x, nullptr, 0, d.nt, B1(d));
   HIPCHK(hipGetLastError());
   return UGMG_OK;
 }
@@ -799,7 +960,7 @@ int ugmg_dmatmul(ugmg_ctx *ctx, int level, double *x, const double *y) {
   int rc = check_ready(ctx, level);
   if (rc) return rc;
   DevLevel &d = ctx->dev[level];
-  launch_spmv<double, double>(ctx->stream, d.A.d64(), 0, d.A.nslices, y, x, nullptr, 1, d.nt);
+  launch_spmv<double, double, double>(ctx->stream, d.A.d64(), 0, d.A.nslices, y, x, nullptr, 1, d.nt, B1(d));
   HIPCHK(hipGetLastError());
   return UGMG_OK;
 }
@@ -808,7 +969,7 @@ int ugmg_smatmul_minus(ugmg_ctx *ctx, int level, float *x, const float *y) {
   int rc = check_ready(ctx, level);
   if (rc) return rc;
   DevLevel &d = ctx->dev[level];
-  launch_spmv<float, float>(ctx->stream, d.A.d32(), 0, d.A.nslices, y, x, nullptr, 0, d.nt);
+  launch_spmv<float, float, float>(ctx->stream, d.A.d32(), 0, d.A.nslices, y, x, nullptr, 0, d.nt, B1(d));
   HIPCHK(hipGetLastError());
   return UGMG_OK;
 }
@@ -818,7 +979,7 @@ int ugmg_dnrm2x(ugmg_ctx *ctx, int level, const double *x, double *out) {
   if (rc) return rc;
   if (!out) return fail(ctx, UGMG_E_ARG, "null output");
   DevLevel &d = ctx->dev[level];
-  launch_dot<double>(ctx->stream, d.n, x, nullptr, nullptr, ctx->part, ctx->red, 1);
+  launch_dot<double>(ctx->stream, d.n, x, nullptr, nullptr, ctx->part, ctx->red, 1, B1(d));
   HIPCHK(hipMemcpyAsync(ctx->red_host, ctx->red, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   out[0] = ctx->red_host[0];
@@ -830,7 +991,7 @@ int ugmg_ddot(ugmg_ctx *ctx, int level, const double *x, const double *y, double
   if (rc) return rc;
   if (!out || !y) return fail(ctx, UGMG_E_ARG, "null argument");
   DevLevel &d = ctx->dev[level];
-  launch_dot<double>(ctx->stream, d.n, x, y, nullptr, ctx->part, ctx->red, 0);
+  launch_dot<double>(ctx->stream, d.n, x, y, nullptr, ctx->part, ctx->red, 0, B1(d));
   HIPCHK(hipMemcpyAsync(ctx->red_host, ctx->red, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   out[0] = ctx->red_host[0];
@@ -841,8 +1002,8 @@ int ugmg_jacobi_step(ugmg_ctx *ctx, int level, double *x, double *b) {
   int rc = check_ready(ctx, level);
   if (rc) return rc;
   DevLevel &d = ctx->dev[level];
-  launch_jac<double>(ctx->stream, d.n, b, d.diag64, d.vclass, ctx->p.damp[0], x);
-  launch_spmv<double, double>(ctx->stream, d.A.d64(), 0, d.A.nslices, x, b, nullptr, 0, d.nt);
+  launch_jac<double>(ctx->stream, d.n, b, d.diag64, d.vclass, ctx->p.damp[0], x, B1(d));
+  launch_spmv<double, double, double>(ctx->stream, d.A.d64(), 0, d.A.nslices, x, b, nullptr, 0, d.nt, B1(d));
   HIPCHK(hipGetLastError());
   return UGMG_OK;
 }
@@ -867,7 +1028,7 @@ int ugmg_restrict(ugmg_ctx *ctx, int level, double *to_c, const double *from_f,
   if (level <= ctx->bl) return fail(ctx, UGMG_E_ARG, "restrict needs a level above the base level");
   DevLevel &d = ctx->dev[level], &dc = ctx->dev[level - 1];
   launch_restrict<double>(ctx->stream, d.PT.d64(), dc.skip, dc.vnclass, dc.vclass, from_f,
-                          damp ? damp[0] : 1.0, to_c, nullptr, d.nt);
+                          damp ? damp[0] : 1.0, to_c, nullptr, d.nt, 1, d.vs, dc.vs);
   HIPCHK(hipGetLastError());
   return UGMG_OK;
 }
@@ -879,7 +1040,7 @@ int ugmg_interpolate(ugmg_ctx *ctx, int level, double *to_f, const double *from_
   if (level <= ctx->bl) return fail(ctx, UGMG_E_ARG, "interpolate needs a level above the base level");
   DevLevel &d = ctx->dev[level];
   launch_interp<double>(ctx->stream, d.P.d64(), d.skip, from_c, damp ? damp[0] : 1.0, to_f,
-                        nullptr, d.nt);
+                        nullptr, d.nt, 1, d.vs, ctx->dev[level - 1].vs);
   HIPCHK(hipGetLastError());
   return UGMG_OK;
 }
@@ -899,11 +1060,11 @@ int ugmg_iter(ugmg_ctx *ctx, int level, double *c, double *b) {
   if (ctx->base.failed) return fail(ctx, UGMG_E_SINGULAR, "zero pivot in the band LU of the base level");
   DevLevel &d = ctx->dev[level];
   if (ctx->p.precision == UGMG_MIXED) {
-    launch_convert<float, double>(ctx->stream, d.n, b, d.b32);
+    launch_convert<float, double>(ctx->stream, d.n, b, d.b32, 1, d.vs, d.vs);
     const int k = std::max(1, ctx->p.inner_cycles);
     for (int j = 0; j < k; ++j) lmgc<float>(ctx, level, d.c32, d.b32, j == 0 ? 0 : 1);
-    launch_convert<double, float>(ctx->stream, d.n, d.c32, c);
-    launch_spmv<double, double>(ctx->stream, d.A.d64(), 0, d.A.nslices, c, b, nullptr, 0, d.nt);
+    launch_convert<double, float>(ctx->stream, d.n, d.c32, c, 1, d.vs, d.vs);
+    launch_spmv<double, double, double>(ctx->stream, d.A.d64(), 0, d.A.nslices, c, b, nullptr, 0, d.nt, B1(d));
   } else {
     lmgc<double>(ctx, level, c, b, 0);
   }
@@ -947,7 +1108,7 @@ int ugmg_solve(ugmg_ctx *ctx, int level, double *x, double *b, const double *abs
   }
   DevLevel &d = ctx->dev[level];
   if (do_defect)  // LinearDefect: b -= A x
-    launch_spmv<double, double>(ctx->stream, d.A.d64(), 0, d.A.nslices, x, b, nullptr, 0, d.nt);
+    launch_spmv<double, double, double>(ctx->stream, d.A.d64(), 0, d.A.nslices, x, b, nullptr, 0, d.nt, B1(d));
   double first = 0.0;
   if ((rc = surface_norm(ctx, level, b, &first))) return rc;
   res->first_defect[0] = res->last_defect[0] = first;
@@ -955,7 +1116,7 @@ int ugmg_solve(ugmg_ctx *ctx, int level, double *x, double *b, const double *abs
   if (target == 0.0) target = reduction[0];
   if (cb) cb(user, 0, 1, &first);
   int conv = sc_cmp1(first, abslimit[0]);
-  if (ctx->p.precision == UGMG_MIXED) launch_convert<float, double>(ctx->stream, d.n, b, d.b32);
+  if (ctx->p.precision == UGMG_MIXED) launch_convert<float, double>(ctx->stream, d.n, b, d.b32, 1, d.vs, d.vs);
   int i;
   for (i = 0; i < maxiter; ++i) {
     if (conv) break;
@@ -998,6 +1159,145 @@ int ugmg_solve_host(ugmg_ctx *ctx, int level, double *x, double *b, const double
               hipMemcpy(b, db, bytes, hipMemcpyDeviceToHost) != hipSuccess))
     rc = fail(ctx, UGMG_E_HIP, "copy out");
   drop_graph(ctx);
+  hipFree(dx);
+  hipFree(db);
+  return rc;
+}
+
+int ugmg_set_batch(ugmg_ctx *ctx, int nrhs) {
+  if (!ctx) return UGMG_E_ARG;
+  if (nrhs < 1 || nrhs > 65535) return fail(ctx, UGMG_E_ARG, "nrhs must be 1..65535");
+  if (nrhs != ctx->nrhs_cap) {
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    free_device(ctx);
+  }
+  ctx->nrhs_cap = nrhs;
+  return UGMG_OK;
+}
+
+int ugmg_iter_batch(ugmg_ctx *ctx, int level, int nrhs, double *c, double *b, int ld) {
+  int rc = check_ready(ctx, level);
+  if (rc) return rc;
+  if (nrhs < 1 || nrhs > ctx->nrhs_cap)
+    return fail(ctx, UGMG_E_ARG, "nrhs exceeds the batch the hierarchy was laid out for (ugmg_set_batch)");
+  if (!c || !b || ld < ctx->dev[level].n) return fail(ctx, UGMG_E_ARG, "null vector or ld < n");
+  if (ctx->base.failed) return fail(ctx, UGMG_E_SINGULAR, "zero pivot in the band LU of the base level");
+  DevLevel &d = ctx->dev[level];
+  hipStream_t st = ctx->stream;
+  ctx->nb = nrhs;
+  const Batch bt = batch(ctx, level);
+  if (ctx->p.precision == UGMG_MIXED) {
+    launch_convert<float, double>(st, d.n, b, d.b32, nrhs, ld, d.vs);
+    const int k = std::max(1, ctx->p.inner_cycles);
+    for (int j = 0; j < k; ++j) lmgc<float>(ctx, level, d.c32, d.b32, j == 0 ? 0 : 1);
+    launch_convert<double, float>(st, d.n, d.c32, c, nrhs, d.vs, ld);
+    launch_spmv<double, double, double>(st, d.A.d64(), 0, d.A.nslices, c, b, nullptr, 0, d.nt,
+                                        Batch{nrhs, (size_t)ld});
+  } else {
+    launch_convert<double, double>(st, d.n, b, d.b, nrhs, ld, d.vs);
+    lmgc<double>(ctx, level, d.c, d.b, 0);
+    launch_convert<double, double>(st, d.n, d.c, c, nrhs, d.vs, ld);
+    launch_convert<double, double>(st, d.n, d.b, b, nrhs, d.vs, ld);
+  }
+  (void)bt;
+  ctx->nb = 1;
+  HIPCHK(hipGetLastError());
+  return UGMG_OK;
+}
+
+int ugmg_bcgs_solve(ugmg_ctx *ctx, int level, int nrhs, double *x, double *b, int ld,
+                    const double *abslimit, const double *reduction, int maxiter, int restart,
+                    int use_iter, int do_defect, ugmg_lresult *res, double *defects, int maxdef) {
+  int rc = check_ready(ctx, level);
+  if (rc) return rc;
+  if (nrhs < 1 || nrhs > ctx->nrhs_cap)
+    return fail(ctx, UGMG_E_ARG, "nrhs exceeds the batch the hierarchy was laid out for (ugmg_set_batch)");
+  if (!x || !b || !abslimit || !reduction || !res || maxiter < 0 || restart < 0 ||
+      ld < ctx->dev[level].n || (defects && maxdef < 1))
+    return fail(ctx, UGMG_E_ARG, "bad argument");
+  std::memset(res, 0, sizeof(*res) * nrhs);
+  if (use_iter && ctx->base.failed) {
+    for (int j = 0; j < nrhs; ++j) res[j].error_code = UGMG_E_SINGULAR;
+    return fail(ctx, UGMG_E_SINGULAR, "zero pivot in the band LU of the base level");
+  }
+  const int md = defects ? maxdef : 1;
+  if ((rc = bcgs_alloc(ctx, level, nrhs, md))) return rc;
+  auto &w = ctx->bc;
+  DevLevel &d = ctx->dev[level];
+  hipStream_t st = ctx->stream;
+  ctx->nb = nrhs;
+  const Batch bt = batch(ctx, level);
+  launch_convert<double, double>(st, d.n, x, w.x, nrhs, ld, d.vs);
+  launch_convert<double, double>(st, d.n, b, w.b, nrhs, ld, d.vs);
+  if (do_defect)  // BCGSDefect: b -= A x
+    launch_spmv<double, double, double>(st, d.A.d64(), 0, d.A.nslices, w.x, w.b, nullptr, 0, d.nt, bt);
+  // first defect (the solver reads LinearResiduum's last_defect)
+  launch_bcgs_norm(st, d.n, w.b, d.vclass, w.partA, bt);
+  launch_bcgs_scalar(st, BCGS_FIRST, bcgs_nparts(d.n), w.partA, w.state, nrhs, 1, abslimit[0],
+                     reduction[0], defects ? w.defects : nullptr, md);
+  rc = 0;
+  int it = 0;
+  auto any_active = [&](bool &any) -> int {
+    HIPCHK(hipMemcpyAsync(ctx->state_host, w.state, sizeof(BcgsState) * nrhs, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    any = false;
+    for (int j = 0; j < nrhs; ++j) any |= ctx->state_host[j].active != 0;
+    return 0;
+  };
+  bool any = false;
+  if ((rc = any_active(any))) {
+    ctx->nb = 1;
+    return rc;
+  }
+  for (it = 0; it < maxiter && any; ++it) {
+    const int rs = it == 0 || (restart > 0 && it % restart == 0);
+    if ((rc = launch_bcgs_iteration(ctx, level, use_iter != 0, rs, abslimit[0], reduction[0],
+                                    defects ? md : 0)))
+      break;
+    if ((rc = any_active(any))) break;
+  }
+  ctx->nb = 1;
+  if (rc) return rc;
+  launch_convert<double, double>(st, d.n, w.x, x, nrhs, d.vs, ld);
+  launch_convert<double, double>(st, d.n, w.b, b, nrhs, d.vs, ld);
+  HIPCHK(hipGetLastError());
+  if (defects)
+    HIPCHK(hipMemcpyAsync(defects, w.defects, sizeof(double) * (size_t)md * nrhs, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  for (int j = 0; j < nrhs; ++j) {
+    const BcgsState &S = ctx->state_host[j];
+    res[j].converged = S.conv && !S.active;
+    res[j].first_defect[0] = S.first;
+    res[j].last_defect[0] = S.last;
+    res[j].number_of_linear_iterations = S.nit;
+  }
+  return UGMG_OK;
+}
+
+int ugmg_bcgs_solve_host(ugmg_ctx *ctx, int level, int nrhs, double *x, double *b, int ld,
+                         const double *abslimit, const double *reduction, int maxiter,
+                         int restart, int use_iter, int do_defect, ugmg_lresult *res,
+                         double *defects, int maxdef) {
+  int rc = check_ready(ctx, level);
+  if (rc) return rc;
+  if (nrhs < 1 || ld < ctx->dev[level].n || !x || !b) return fail(ctx, UGMG_E_ARG, "bad argument");
+  const size_t bytes = sizeof(double) * (size_t)ld * nrhs;
+  double *dx = nullptr, *db = nullptr;
+  HIPCHK(hipMalloc(&dx, bytes));
+  if (hipMalloc(&db, bytes) != hipSuccess) {
+    hipFree(dx);
+    return fail(ctx, UGMG_E_NOMEM, "hipMalloc");
+  }
+  rc = UGMG_OK;
+  if (hipMemcpy(dx, x, bytes, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(db, b, bytes, hipMemcpyHostToDevice) != hipSuccess)
+    rc = fail(ctx, UGMG_E_HIP, "copy in");
+  if (!rc)
+    rc = ugmg_bcgs_solve(ctx, level, nrhs, dx, db, ld, abslimit, reduction, maxiter, restart,
+                         use_iter, do_defect, res, defects, maxdef);
+  if (!rc && (hipMemcpy(x, dx, bytes, hipMemcpyDeviceToHost) != hipSuccess ||
+              hipMemcpy(b, db, bytes, hipMemcpyDeviceToHost) != hipSuccess))
+    rc = fail(ctx, UGMG_E_HIP, "copy out");
   hipFree(dx);
   hipFree(db);
   return rc;
@@ -1053,16 +1353,16 @@ int ugmg_time_kernel(ugmg_ctx *ctx, int level, int kind, int reps, double *ms, d
   if ((kind == 3 || kind == 7) && d.ncolors == 0) return fail(ctx, UGMG_E_STATE, "not a GS level");
   hipStream_t st = ctx->stream;
   // deterministic nonzero vectors
-  launch_fill<double>(st, d.n, d.t, 1.0);
-  launch_fill<double>(st, d.n, d.b, 0.5);
-  launch_fill<float>(st, d.n, d.t32, 1.0f);
-  launch_fill<float>(st, d.n, d.b32, 0.5f);
+  launch_fill<double>(st, d.n, d.t, 1.0, B1(d));
+  launch_fill<double>(st, d.n, d.b, 0.5, B1(d));
+  launch_fill<float>(st, d.n, d.t32, 1.0f, B1(d));
+  launch_fill<float>(st, d.n, d.b32, 0.5f, B1(d));
   const double n = d.n, nnz = d.nnzb;
   double by = 0.0;
   auto once = [&]() {
     switch (kind) {
-      case 0: launch_spmv<double, double>(st, d.A.d64(), 0, d.A.nslices, d.t, d.b, nullptr, 0, d.nt); break;
-      case 1: launch_spmv<float, float>(st, d.A.d32(), 0, d.A.nslices, d.t32, d.b32, nullptr, 0, d.nt); break;
+      case 0: launch_spmv<double, double, double>(st, d.A.d64(), 0, d.A.nslices, d.t, d.b, nullptr, 0, d.nt, B1(d)); break;
+      case 1: launch_spmv<float, float, float>(st, d.A.d32(), 0, d.A.nslices, d.t32, d.b32, nullptr, 0, d.nt, B1(d)); break;
       case 2: smooth_step<double>(ctx, level, d.t, d.b, d.c, 1, nullptr); break;
       case 3: {
         const int s = ctx->p.smoother;
@@ -1074,12 +1374,12 @@ int ugmg_time_kernel(ugmg_ctx *ctx, int level, int kind, int reps, double *ms, d
       case 4: {
         DevLevel &dc = ctx->dev[level - 1];
         launch_restrict<double>(st, d.PT.d64(), dc.skip, dc.vnclass, dc.vclass, d.b, 1.0, dc.b,
-                                nullptr, d.nt);
+                                nullptr, d.nt, 1, d.vs, dc.vs);
         break;
       }
       case 5: {
         DevLevel &dc = ctx->dev[level - 1];
-        launch_interp<double>(st, d.P.d64(), d.skip, dc.c, 1.0, d.t, d.c, d.nt);
+        launch_interp<double>(st, d.P.d64(), d.skip, dc.c, 1.0, d.t, d.c, d.nt, 1, d.vs, dc.vs);
         break;
       }
       case 6:
@@ -1288,9 +1588,10 @@ int srv_admit(ugmg_server *s, SrvReq *r) {
                      ctx->stream) != hipSuccess ||
       hipEventCreateWithFlags(&r->ev, hipEventDisableTiming) != hipSuccess)
     return UGMG_E_HIP;
-  launch_fill<double>(ctx->stream, d.n, r->x, 0.0);
-  if (ctx->p.precision == UGMG_MIXED) launch_convert<float, double>(ctx->stream, d.n, r->b, d.b32);
-  launch_dot<double>(ctx->stream, d.n, r->b, nullptr, d.vclass, ctx->part, ctx->red, 1);
+  launch_fill<double>(ctx->stream, d.n, r->x, 0.0, B1(d));
+  if (ctx->p.precision == UGMG_MIXED)
+    launch_convert<float, double>(ctx->stream, d.n, r->b, d.b32, 1, d.vs, d.vs);
+  launch_dot<double>(ctx->stream, d.n, r->b, nullptr, d.vclass, ctx->part, ctx->red, 1, B1(d));
   if (hipMemcpyAsync(ctx->red_host, ctx->red, sizeof(double), hipMemcpyDeviceToHost,
                      ctx->stream) != hipSuccess ||
       hipEventRecord(r->ev, ctx->stream) != hipSuccess)

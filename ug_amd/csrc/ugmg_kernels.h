@@ -66,47 +66,107 @@ struct CoarseArgs {
   int cmode_top;         // 0: c := correction, 1: c += correction
 };
 
-template <typename TV, typename TX>
-void launch_spmv(hipStream_t st, const DSell<TV> &A, int slice0, int nsl, const TX *x, TX *y,
-                 TX *c, int mode, bool nt);
+// Batched launches: every vector argument holds `nrhs` columns, column j at
+// offset j * vs (vs = the level's column stride, >= n); the column index of a
+// workgroup is its second grid coordinate.  nrhs = 1 is the plain case.
+struct Batch {
+  int nrhs;
+  size_t vs;
+};
+
+// per right-hand side state of the batched BCGSSolver (np/procs/ls.cc:
+// 1864-2062); written only by the scalar kernels, read by the vector kernels
+struct BcgsState {
+  double rho, omega, alpha, beta, rho_new;
+  double first, target, last, old;
+  double abslimit;
+  int eq_count;   // consecutive sc_eq(last, old, 1e-4) hits
+  int active;     // still iterating
+  int half;       // converged after the first half step of this iteration
+  int conv;
+  int nit;        // LinearResiduum calls after the first defect
+  int nd;         // defects recorded
+};
+
+enum BcgsOp { BCGS_FIRST = 0, BCGS_RHO = 1, BCGS_ALPHA = 2, BCGS_RES1 = 3, BCGS_OMEGA = 4, BCGS_RES2 = 5 };
+
+template <typename TV, typename TX, typename TY>
+void launch_spmv(hipStream_t st, const DSell<TV> &A, int slice0, int nsl, const TX *x, TY *y,
+                 TY *c, int mode, bool nt, Batch bt);
 void launch_ir_update(hipStream_t st, const DSell<double> &A, const float *c32, double *r,
                       double *x, float *r32, const uint8_t *vclass, double *part, int *nparts,
-                      bool nt);
+                      bool nt, Batch bt);
 template <typename T>
 void launch_jac(hipStream_t st, int n, const T *b, const T *diag, const uint8_t *vclass, T damp,
-                T *t);
+                T *t, Batch bt);
 template <typename TV, typename T>
 void launch_gs_color(hipStream_t st, const DSell<TV> &L, int slice0, int nsl, const T *b,
-                     const T *diag, const uint8_t *vclass, T *v, bool nt);
+                     const T *diag, const uint8_t *vclass, T *v, bool nt, Batch bt);
 template <typename TV, typename T>
 void launch_gs_defect(hipStream_t st, const DSell<TV> &U, const T *v, T *b, const uint8_t *vclass,
-                      T damp, T *c, int cmode, T *xout, bool nt);
-// returns the number of blocks (= partial sums written when part != null)
+                      T damp, T *c, int cmode, T *xout, bool nt, Batch bt);
+// returns the number of blocks per column (= partial sums per column when
+// part != null; column j's partials start at part + j * return value)
 template <typename T>
 int launch_restrict(hipStream_t st, const DSell<T> &PT, const uint32_t *cskip,
                     const uint8_t *cvnclass, const uint8_t *cvclass, const T *bf, T damp, T *bc,
-                    double *part, bool nt);
+                    double *part, bool nt, int nrhs, size_t vsf, size_t vsc);
 template <typename T>
 void launch_base_gemv(hipStream_t st, int n, const double *inv, const T *b, T *c, T *xout,
                       int cmode, double damp, double abslimit, double red, const double *part,
-                      int nparts, double *first, int it);
+                      int nparts, double *first, int it, Batch bt);
 template <typename T>
-int launch_dot_part(hipStream_t st, int n, const T *x, const uint8_t *vclass, double *part);
+int launch_dot_part(hipStream_t st, int n, const T *x, const uint8_t *vclass, double *part,
+                    Batch bt);
 template <typename T>
 void launch_interp(hipStream_t st, const DSell<T> &P, const uint32_t *fskip, const T *cc, T damp,
-                   T *t, T *c, bool nt);
+                   T *t, T *c, bool nt, int nrhs, size_t vsf, size_t vsc);
 int reduce_parts(int n);
 template <typename T>
 void launch_dot(hipStream_t st, int n, const T *x, const T *y, const uint8_t *vclass,
-                double *part, double *out, int do_sqrt);
-void launch_reduce_final(hipStream_t st, int nparts, const double *part, double *out, int do_sqrt);
+                double *part, double *out, int do_sqrt, Batch bt);
+// out[j] = sum of column j's nparts partials (sqrt if do_sqrt)
+void launch_reduce_final(hipStream_t st, int nparts, const double *part, double *out, int do_sqrt,
+                         int nrhs);
 template <typename T>
-void launch_fill(hipStream_t st, int n, T *x, T val);
+void launch_fill(hipStream_t st, int n, T *x, T val, Batch bt);
 template <typename TD, typename TS>
-void launch_convert(hipStream_t st, int n, const TS *s, TD *d);
+void launch_convert(hipStream_t st, int n, const TS *s, TD *d, int nrhs, size_t vss, size_t vsd);
 template <typename T>
-void launch_add(hipStream_t st, int n, T *x, const T *y);
+void launch_add(hipStream_t st, int n, T *x, const T *y, Batch bt);
 template <typename T>
 void launch_coarse_cycle(hipStream_t st, const CoarseArgs<T> &args, int base_n);
+
+// ---- batched BCGSSolver pieces (FP64 outer vectors; TQ = type of q, the
+// preconditioned direction: double, or float when the cycle runs in FP32)
+// partial sums per column: two per block, layout part[(j * 2 + k) * nparts + blk]
+int bcgs_nparts(int n);
+void launch_bcgs_scalar(hipStream_t st, int op, int nparts, const double *part, BcgsState *s,
+                        int nrhs, int restart, double abslimit, double red, double *defects,
+                        int maxdef);
+// restart: r = b, v = 0, p = 0 first.  p = beta p + b - beta omega v;
+// then the cycle's input: w64 = p or w32 = (float) p (whichever is non-null)
+void launch_bcgs_p(hipStream_t st, int n, const double *b, double *r, double *v, double *p,
+                   double *w64, float *w32, const BcgsState *s, int restart, Batch bt);
+// y = A q on active columns (skip_half: and not converged at the half step),
+// with the partial sums of y.z (k = 0) and y.y (k = 1, when yy) over NEW_DEFECT rows
+template <typename TQ>
+void launch_bcgs_spmv(hipStream_t st, const DSell<double> &A, const TQ *q, double *y,
+                      const double *z, const uint8_t *vclass, double *part, int yy, int skip_half, const BcgsState *s, bool nt, Batch bt);
+int bcgs_spmv_nparts(int nslices);
+// x += alpha q; s = b - alpha v; cycle input w = s; partials of s.s (k = 0)
+template <typename TQ>
+void launch_bcgs_s(hipStream_t st, int n, const double *b, const double *v, const TQ *q,
+                   double *x, double *sv, double *w64, float *w32, const uint8_t *vclass,
+                   double *part, const BcgsState *s, Batch bt);
+// half-converged: b = s; else x += omega q; b = s - omega t.  partials of
+// b.b (k = 0) and b.r (k = 1)
+template <typename TQ>
+void launch_bcgs_b(hipStream_t st, int n, const double *sv, const double *t, const TQ *q,
+                   const double *r, double *x, double *b, const uint8_t *vclass, double *part,
+                   const BcgsState *s, Batch bt);
+// partials of x.x (k = 0, and k = 1) for the first defect
+void launch_bcgs_norm(hipStream_t st, int n, const double *x, const uint8_t *vclass, double *part,
+                      Batch bt);
 
 }  // namespace ugmg

@@ -12,7 +12,10 @@
 // does not evict the vectors from L2; small levels use cached loads.
 // Blocks are 4 wavefronts; block ids are remapped so that each of the 8 XCDs
 // (round-robin dispatch) walks one contiguous range of slices and keeps its
-// share of x in its own L2.  No MFMA anywhere: these are sparse contractions
+// share of x in its own L2.  Batched launches (several right-hand sides,
+// column index = second grid coordinate) remap slice-major: an XCD takes a
+// range of slice blocks and runs every column of one block before the next,
+// so that block's matrix entries are read from HBM once and then hit in L2.  No MFMA anywhere: these are sparse contractions
 // at ~0.15 flop/byte, bounded by HBM bandwidth.
 #include <hip/hip_runtime.h>
 
@@ -78,11 +81,28 @@ __device__ __forceinline__ TA sell_dot(const TV *__restrict__ v, const int *__re
   return sum;
 }
 
+// Slice block and column of this workgroup.  Single column: the XCD-aware
+// remap of blockIdx.x.  Several columns: the linear workgroup id is remapped
+// over the whole grid and split slice-block-major.
+__device__ __forceinline__ void batch_map(int &sb, int &col) {
+  if (gridDim.y == 1) {
+    sb = xcd_remap(blockIdx.x, gridDim.x);
+    col = 0;
+    return;
+  }
+  const int nb = gridDim.x * gridDim.y;
+  const int lr = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x, nb);
+  sb = lr / gridDim.y;
+  col = lr % gridDim.y;
+}
+
 // slice s (wave-uniform, in scalar registers), lane, row r (-1 = padding),
-// entry base e0, width w, uniform-offset list sdo, index base rb
+// entry base e0, width w, uniform-offset list sdo, index base rb, slice
+// block sb_, column col_
 #define SLICE_PROLOGUE(S, slice0, nsl)                                                   \
-  const int bid_ = xcd_remap(blockIdx.x, gridDim.x);                                     \
-  const int s = __builtin_amdgcn_readfirstlane((slice0) + bid_ * (kBlock / kSlice) +     \
+  int sb_, col_;                                                                         \
+  batch_map(sb_, col_);                                                                  \
+  const int s = __builtin_amdgcn_readfirstlane((slice0) + sb_ * (kBlock / kSlice) +      \
                                                (int)(threadIdx.x >> 6));                 \
   if (s >= (slice0) + (nsl)) return;                                                     \
   const int lane = threadIdx.x & 63;                                                     \
@@ -92,21 +112,49 @@ __device__ __forceinline__ TA sell_dot(const TV *__restrict__ v, const int *__re
   const int sdo = (S).sdoff[s];                                                          \
   const int rb = r >= 0 ? r : (S).rows[s * kSlice];
 
+// same, for kernels that end in a block reduction (no early return)
+#define SLICE_PROLOGUE_RED(S, nsl)                                                       \
+  int sb_, col_;                                                                         \
+  batch_map(sb_, col_);                                                                  \
+  const int s = __builtin_amdgcn_readfirstlane(sb_ * (kBlock / kSlice) +                 \
+                                               (int)(threadIdx.x >> 6));                 \
+  const int lane = threadIdx.x & 63;                                                     \
+  const bool sok = s < (nsl);                                                            \
+  const int r = sok ? (S).rows[s * kSlice + lane] : -1;                                  \
+  const int e0 = sok ? (S).soff[s] : 0;                                                  \
+  const int w = sok ? ((S).soff[s + 1] - e0) >> 6 : 0;                                   \
+  const int sdo = sok ? (S).sdoff[s] : -1;                                               \
+  const int rb = r >= 0 ? r : (sok ? (S).rows[s * kSlice] : 0);
+
+// block sum of one double per thread; thread 0 gets the total
+__device__ __forceinline__ double block_sum(double v, double *red) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < kBlock / 64; ++i) t += red[i];
+  return t;
+}
+
 // ------------------------------------------------------------------- SpMV
 // MODE: 0 y -= A x, 1 y = A x, 2 y -= A x and c += x, 3 y -= A x and c = x
-template <typename TV, typename TX, int MODE, int NT>
+template <typename TV, typename TX, typename TY, int MODE, int NT>
 __global__ __launch_bounds__(kBlock) void k_spmv(DSell<TV> A, int slice0, int nsl,
-                                                  const TX *__restrict__ x, TX *__restrict__ y,
-                                                  TX *__restrict__ c) {
+                                                  const TX *__restrict__ x, TY *__restrict__ y,
+                                                  TY *__restrict__ c, size_t vs) {
   SLICE_PROLOGUE(A, slice0, nsl)
-  const TX sum = sell_dot<NT, TV, TX, TX>(A.val + e0 + lane, A.col + e0 + lane, w, x, sdo, A.dtab, rb);
+  const size_t o = (size_t)col_ * vs;
+  x += o;
+  y += o;
+  const TY sum = sell_dot<NT, TV, TX, TY>(A.val + e0 + lane, A.col + e0 + lane, w, x, sdo, A.dtab, rb);
   if (r < 0) return;
   if (MODE == 1) {
     y[r] = sum;
   } else {
     y[r] -= sum;
-    if (MODE == 2) c[r] += x[r];
-    if (MODE == 3) c[r] = x[r];
+    if (MODE == 2) c[o + r] += (TY)x[r];
+    if (MODE == 3) c[o + r] = (TY)x[r];
   }
 }
 
@@ -120,36 +168,24 @@ __global__ __launch_bounds__(kBlock) void k_ir_update(DSell<double> A, int nsl,
                                                        double *__restrict__ x,
                                                        float *__restrict__ r32,
                                                        const uint8_t *__restrict__ vclass,
-                                                       double *__restrict__ part) {
+                                                       double *__restrict__ part, size_t vs) {
   __shared__ double red[kBlock / 64];
-  const int bid = xcd_remap(blockIdx.x, gridDim.x);
-  const int s = __builtin_amdgcn_readfirstlane(bid * (kBlock / kSlice) + (int)(threadIdx.x >> 6));
-  const int lane = threadIdx.x & 63;
+  SLICE_PROLOGUE_RED(A, nsl)
+  const size_t o = (size_t)col_ * vs;
   double sq = 0.0;
-  if (s < nsl) {
-    const int r = A.rows[s * kSlice + lane];
-    const int e0 = A.soff[s];
-    const int w = (A.soff[s + 1] - e0) >> 6;
-    const int sdo = A.sdoff[s];
-    const int rb = r >= 0 ? r : A.rows[s * kSlice];
+  if (sok) {
     const double sum =
-        sell_dot<NT, double, float, double>(A.val + e0 + lane, A.col + e0 + lane, w, c32, sdo, A.dtab, rb);
+        sell_dot<NT, double, float, double>(A.val + e0 + lane, A.col + e0 + lane, w, c32 + o, sdo, A.dtab, rb);
     if (r >= 0) {
-      const double nr = rv[r] - sum;
-      rv[r] = nr;
-      x[r] += (double)c32[r];
-      r32[r] = (float)nr;
+      const double nr = rv[o + r] - sum;
+      rv[o + r] = nr;
+      x[o + r] += (double)c32[o + r];
+      r32[o + r] = (float)nr;
       if (vclass[r] >= 2) sq = nr * nr;
     }
   }
-  for (int off = 32; off > 0; off >>= 1) sq += __shfl_down(sq, off, 64);
-  if (lane == 0) red[threadIdx.x >> 6] = sq;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double t = 0.0;
-    for (int i = 0; i < kBlock / 64; ++i) t += red[i];
-    part[blockIdx.x] = t;
-  }
+  const double t = block_sum(sq, red);
+  if (threadIdx.x == 0) part[(size_t)col_ * gridDim.x + sb_] = t;
 }
 
 // ----------------------------------------------------------------- Jacobi
@@ -158,10 +194,11 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_jac(int n, const T *__restrict__ b,
                                                  const T *__restrict__ diag,
                                                  const uint8_t *__restrict__ vclass, T damp,
-                                                 T *__restrict__ t) {
+                                                 T *__restrict__ t, size_t vs) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
-  t[i] = vclass[i] < 3 ? (T)0 : (b[i] / diag[i]) * damp;
+  const size_t o = (size_t)blockIdx.y * vs;
+  t[o + i] = vclass[i] < 3 ? (T)0 : (b[o + i] / diag[i]) * damp;
 }
 
 // ------------------------------------------------------ multicolour GS
@@ -171,11 +208,13 @@ __global__ __launch_bounds__(kBlock) void k_gs_color(DSell<TV> L, int slice0, in
                                                       const T *__restrict__ b,
                                                       const T *__restrict__ diag,
                                                       const uint8_t *__restrict__ vclass,
-                                                      T *__restrict__ v) {
+                                                      T *__restrict__ v, size_t vs) {
   SLICE_PROLOGUE(L, slice0, nsl)
+  const size_t o = (size_t)col_ * vs;
+  v += o;
   const T sum = sell_dot<NT, TV, T, T>(L.val + e0 + lane, L.col + e0 + lane, w, v, sdo, L.dtab, rb);
   if (r < 0) return;
-  v[r] = vclass[r] < 3 ? (T)0 : (b[r] - sum) / diag[r];
+  v[r] = vclass[r] < 3 ? (T)0 : (b[o + r] - sum) / diag[r];
 }
 
 // defect after the sweep, b -= A (damp v), with (D+L) v = b on active rows:
@@ -188,18 +227,20 @@ __global__ __launch_bounds__(kBlock) void k_gs_defect(DSell<TV> U, int nsl,
                                                        T *__restrict__ b,
                                                        const uint8_t *__restrict__ vclass,
                                                        T damp, T *__restrict__ c,
-                                                       T *__restrict__ xout) {
+                                                       T *__restrict__ xout, size_t vs) {
   SLICE_PROLOGUE(U, 0, nsl)
+  const size_t o = (size_t)col_ * vs;
+  v += o;
   const T sum = sell_dot<NT, TV, T, T>(U.val + e0 + lane, U.col + e0 + lane, w, v, sdo, U.dtab, rb);
   if (r < 0) return;
-  const T br = b[r];
-  b[r] = vclass[r] >= 3 ? br - damp * (br + sum) : br - damp * sum;
+  const T br = b[o + r];
+  b[o + r] = vclass[r] >= 3 ? br - damp * (br + sum) : br - damp * sum;
   const T t = damp * v[r];
   if (c) {
-    if (CMODE == 0) c[r] = t;
-    else c[r] += t;
+    if (CMODE == 0) c[o + r] = t;
+    else c[o + r] += t;
   }
-  if (xout) xout[r] = t;
+  if (xout) xout[o + r] = t;
 }
 
 // --------------------------------------------------------------- transfer
@@ -214,18 +255,14 @@ __global__ __launch_bounds__(kBlock) void k_restrict(DSell<T> PT, int nsl,
                                                       const uint8_t *__restrict__ cvclass,
                                                       const T *__restrict__ bf, T damp,
                                                       T *__restrict__ bc,
-                                                      double *__restrict__ part) {
+                                                      double *__restrict__ part, size_t vsf,
+                                                      size_t vsc) {
   __shared__ double red[kBlock / 64];
-  const int bid = xcd_remap(blockIdx.x, gridDim.x);
-  const int s = __builtin_amdgcn_readfirstlane(bid * (kBlock / kSlice) + (int)(threadIdx.x >> 6));
-  const int lane = threadIdx.x & 63;
+  SLICE_PROLOGUE_RED(PT, nsl)
+  bf += (size_t)col_ * vsf;
+  bc += (size_t)col_ * vsc;
   double sq = 0.0;
-  if (s < nsl) {
-    const int r = PT.rows[s * kSlice + lane];
-    const int e0 = PT.soff[s];
-    const int w = (PT.soff[s + 1] - e0) >> 6;
-    const int sdo = PT.sdoff[s];
-    const int rb = r >= 0 ? r : PT.rows[s * kSlice];
+  if (sok) {
     const T sum = sell_dot<NT, T, T, T>(PT.val + e0 + lane, PT.col + e0 + lane, w, bf, sdo, PT.dtab, rb);
     if (r >= 0) {
       const T base = cvnclass[r] >= 2 ? (T)0 : bc[r];
@@ -235,14 +272,8 @@ __global__ __launch_bounds__(kBlock) void k_restrict(DSell<T> PT, int nsl,
     }
   }
   if (!part) return;
-  for (int off = 32; off > 0; off >>= 1) sq += __shfl_down(sq, off, 64);
-  if (lane == 0) red[threadIdx.x >> 6] = sq;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double t = 0.0;
-    for (int i = 0; i < kBlock / 64; ++i) t += red[i];
-    part[blockIdx.x] = t;
-  }
+  const double t = block_sum(sq, red);
+  if (threadIdx.x == 0) part[(size_t)col_ * gridDim.x + sb_] = t;
 }
 
 // StandardIntCorNodeVector (+ dadd): t = damp P c_c on non-Dirichlet rows
@@ -250,13 +281,16 @@ template <typename T, int CADD, int NT>
 __global__ __launch_bounds__(kBlock) void k_interp(DSell<T> P, int nsl,
                                                     const uint32_t *__restrict__ fskip,
                                                     const T *__restrict__ cc, T damp,
-                                                    T *__restrict__ t, T *__restrict__ c) {
+                                                    T *__restrict__ t, T *__restrict__ c,
+                                                    size_t vsf, size_t vsc) {
   SLICE_PROLOGUE(P, 0, nsl)
+  cc += (size_t)col_ * vsc;
+  const size_t o = (size_t)col_ * vsf;
   const T sum = sell_dot<NT, T, T, T>(P.val + e0 + lane, P.col + e0 + lane, w, cc, sdo, P.dtab, rb);
   if (r < 0) return;
   const T tv = (fskip[r] & 1u) ? (T)0 : damp * sum;
-  t[r] = tv;
-  if (CADD) c[r] += tv;
+  t[o + r] = tv;
+  if (CADD) c[o + r] += tv;
 }
 
 // -------------------------------------------------------------- reductions
@@ -264,28 +298,24 @@ template <typename T, int DOT>
 __global__ __launch_bounds__(kBlock) void k_dot_part(int n, const T *__restrict__ x,
                                                       const T *__restrict__ y,
                                                       const uint8_t *__restrict__ vclass,
-                                                      double *__restrict__ part) {
+                                                      double *__restrict__ part, size_t vs) {
   __shared__ double red[kBlock / 64];
+  const size_t o = (size_t)blockIdx.y * vs;
   double s = 0.0;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     if (vclass && vclass[i] < 2) continue;
-    const double xi = (double)x[i];
-    s += DOT ? xi * (double)y[i] : xi * xi;
+    const double xi = (double)x[o + i];
+    s += DOT ? xi * (double)y[o + i] : xi * xi;
   }
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double t = 0.0;
-    for (int i = 0; i < kBlock / 64; ++i) t += red[i];
-    part[blockIdx.x] = t;
-  }
+  const double t = block_sum(s, red);
+  if (threadIdx.x == 0) part[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = t;
 }
 
 __global__ __launch_bounds__(1024) void k_reduce_final(int nparts,
                                                         const double *__restrict__ part,
                                                         double *__restrict__ out, int do_sqrt) {
   __shared__ double red[16];
+  part += (size_t)blockIdx.x * nparts;
   double s = 0.0;
   for (int i = threadIdx.x; i < nparts; i += 1024) s += part[i];
   for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
@@ -294,26 +324,28 @@ __global__ __launch_bounds__(1024) void k_reduce_final(int nparts,
   if (threadIdx.x == 0) {
     double t = 0.0;
     for (int i = 0; i < 16; ++i) t += red[i];
-    out[0] = do_sqrt ? sqrt(t) : t;
+    out[blockIdx.x] = do_sqrt ? sqrt(t) : t;
   }
 }
 
 // ------------------------------------------------------------------ BLAS-1
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_fill(int n, T *x, T val) {
+__global__ __launch_bounds__(kBlock) void k_fill(int n, T *x, T val, size_t vs) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i < n) x[i] = val;
+  if (i < n) x[(size_t)blockIdx.y * vs + i] = val;
 }
 template <typename TD, typename TS>
 __global__ __launch_bounds__(kBlock) void k_convert(int n, const TS *__restrict__ s,
-                                                     TD *__restrict__ d) {
+                                                     TD *__restrict__ d, size_t vss, size_t vsd) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i < n) d[i] = (TD)s[i];
+  if (i < n) d[(size_t)blockIdx.y * vsd + i] = (TD)s[(size_t)blockIdx.y * vss + i];
 }
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_add(int n, T *__restrict__ x, const T *__restrict__ y) {
+__global__ __launch_bounds__(kBlock) void k_add(int n, T *__restrict__ x, const T *__restrict__ y,
+                                                 size_t vs) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i < n) x[i] += y[i];
+  const size_t o = (size_t)blockIdx.y * vs;
+  if (i < n) x[o + i] += y[o + i];
 }
 
 // ==================================================== fused coarse cycle
@@ -490,6 +522,7 @@ __global__ __launch_bounds__(kCoarseThreads) void k_coarse_cycle(CoarseArgs<T> a
 int reduce_parts(int n);
 static inline int nblk(int n) { return (n + kBlock - 1) / kBlock; }
 static inline int nblk_slices(int nsl) { return (nsl + kBlock / kSlice - 1) / (kBlock / kSlice); }
+static inline dim3 grid2(int gx, int nrhs) { return dim3(gx, nrhs < 1 ? 1 : nrhs); }
 
 #define NT_DISPATCH(nt, ...) \
   do {                       \
@@ -502,90 +535,92 @@ static inline int nblk_slices(int nsl) { return (nsl + kBlock / kSlice - 1) / (k
     }                        \
   } while (0)
 
-template <typename TV, typename TX>
-void launch_spmv(hipStream_t st, const DSell<TV> &A, int slice0, int nsl, const TX *x, TX *y,
-                 TX *c, int mode, bool nt) {
+template <typename TV, typename TX, typename TY>
+void launch_spmv(hipStream_t st, const DSell<TV> &A, int slice0, int nsl, const TX *x, TY *y,
+                 TY *c, int mode, bool nt, Batch bt) {
   if (nsl <= 0) return;
-  dim3 g(nblk_slices(nsl)), blk(kBlock);
+  const dim3 g = grid2(nblk_slices(nsl), bt.nrhs), blk(kBlock);
   NT_DISPATCH(nt, switch (mode) {
-    case 0: k_spmv<TV, TX, 0, NTV><<<g, blk, 0, st>>>(A, slice0, nsl, x, y, c); break;
-    case 1: k_spmv<TV, TX, 1, NTV><<<g, blk, 0, st>>>(A, slice0, nsl, x, y, c); break;
-    case 2: k_spmv<TV, TX, 2, NTV><<<g, blk, 0, st>>>(A, slice0, nsl, x, y, c); break;
-    default: k_spmv<TV, TX, 3, NTV><<<g, blk, 0, st>>>(A, slice0, nsl, x, y, c); break;
+    case 0: k_spmv<TV, TX, TY, 0, NTV><<<g, blk, 0, st>>>(A, slice0, nsl, x, y, c, bt.vs); break;
+    case 1: k_spmv<TV, TX, TY, 1, NTV><<<g, blk, 0, st>>>(A, slice0, nsl, x, y, c, bt.vs); break;
+    case 2: k_spmv<TV, TX, TY, 2, NTV><<<g, blk, 0, st>>>(A, slice0, nsl, x, y, c, bt.vs); break;
+    default: k_spmv<TV, TX, TY, 3, NTV><<<g, blk, 0, st>>>(A, slice0, nsl, x, y, c, bt.vs); break;
   });
 }
-template void launch_spmv<double, double>(hipStream_t, const DSell<double> &, int, int,
-                                          const double *, double *, double *, int, bool);
-template void launch_spmv<float, float>(hipStream_t, const DSell<float> &, int, int,
-                                        const float *, float *, float *, int, bool);
+template void launch_spmv<double, double, double>(hipStream_t, const DSell<double> &, int, int,
+                                                  const double *, double *, double *, int, bool,
+                                                  Batch);
+template void launch_spmv<float, float, float>(hipStream_t, const DSell<float> &, int, int,
+                                               const float *, float *, float *, int, bool, Batch);
 
 void launch_ir_update(hipStream_t st, const DSell<double> &A, const float *c32, double *r,
                       double *x, float *r32, const uint8_t *vclass, double *part, int *nparts,
-                      bool nt) {
+                      bool nt, Batch bt) {
   const int g = nblk_slices(A.nslices);
   *nparts = g;
   if (g == 0) return;
-  NT_DISPATCH(nt, k_ir_update<NTV><<<g, kBlock, 0, st>>>(A, A.nslices, c32, r, x, r32, vclass, part));
+  NT_DISPATCH(nt, k_ir_update<NTV><<<grid2(g, bt.nrhs), kBlock, 0, st>>>(A, A.nslices, c32, r, x,
+                                                                         r32, vclass, part, bt.vs));
 }
 
 template <typename T>
 void launch_jac(hipStream_t st, int n, const T *b, const T *diag, const uint8_t *vclass, T damp,
-                T *t) {
+                T *t, Batch bt) {
   if (n <= 0) return;
-  k_jac<T><<<nblk(n), kBlock, 0, st>>>(n, b, diag, vclass, damp, t);
+  k_jac<T><<<grid2(nblk(n), bt.nrhs), kBlock, 0, st>>>(n, b, diag, vclass, damp, t, bt.vs);
 }
 template void launch_jac<double>(hipStream_t, int, const double *, const double *,
-                                 const uint8_t *, double, double *);
+                                 const uint8_t *, double, double *, Batch);
 template void launch_jac<float>(hipStream_t, int, const float *, const float *, const uint8_t *,
-                                float, float *);
+                                float, float *, Batch);
 
 template <typename TV, typename T>
 void launch_gs_color(hipStream_t st, const DSell<TV> &L, int slice0, int nsl, const T *b,
-                     const T *diag, const uint8_t *vclass, T *v, bool nt) {
+                     const T *diag, const uint8_t *vclass, T *v, bool nt, Batch bt) {
   if (nsl <= 0) return;
-  NT_DISPATCH(nt, k_gs_color<TV, T, NTV><<<nblk_slices(nsl), kBlock, 0, st>>>(L, slice0, nsl, b,
-                                                                             diag, vclass, v));
+  NT_DISPATCH(nt, k_gs_color<TV, T, NTV><<<grid2(nblk_slices(nsl), bt.nrhs), kBlock, 0, st>>>(
+                      L, slice0, nsl, b, diag, vclass, v, bt.vs));
 }
 template void launch_gs_color<double, double>(hipStream_t, const DSell<double> &, int, int,
                                               const double *, const double *, const uint8_t *,
-                                              double *, bool);
+                                              double *, bool, Batch);
 template void launch_gs_color<float, float>(hipStream_t, const DSell<float> &, int, int,
                                             const float *, const float *, const uint8_t *,
-                                            float *, bool);
+                                            float *, bool, Batch);
 
 template <typename TV, typename T>
 void launch_gs_defect(hipStream_t st, const DSell<TV> &U, const T *v, T *b, const uint8_t *vclass,
-                      T damp, T *c, int cmode, T *xout, bool nt) {
+                      T damp, T *c, int cmode, T *xout, bool nt, Batch bt) {
   if (U.nslices <= 0) return;
-  dim3 g(nblk_slices(U.nslices));
+  const dim3 g = grid2(nblk_slices(U.nslices), bt.nrhs);
   NT_DISPATCH(nt, if (cmode == 0) k_gs_defect<TV, T, 0, NTV><<<g, kBlock, 0, st>>>(
-                      U, U.nslices, v, b, vclass, damp, c, xout);
+                      U, U.nslices, v, b, vclass, damp, c, xout, bt.vs);
               else k_gs_defect<TV, T, 1, NTV><<<g, kBlock, 0, st>>>(U, U.nslices, v, b, vclass,
-                                                                    damp, c, xout));
+                                                                    damp, c, xout, bt.vs));
 }
 template void launch_gs_defect<double, double>(hipStream_t, const DSell<double> &,
                                                const double *, double *, const uint8_t *, double,
-                                               double *, int, double *, bool);
+                                               double *, int, double *, bool, Batch);
 template void launch_gs_defect<float, float>(hipStream_t, const DSell<float> &, const float *,
                                              float *, const uint8_t *, float, float *, int,
-                                             float *, bool);
+                                             float *, bool, Batch);
 
 template <typename T>
 int launch_restrict(hipStream_t st, const DSell<T> &PT, const uint32_t *cskip,
                     const uint8_t *cvnclass, const uint8_t *cvclass, const T *bf, T damp, T *bc,
-                    double *part, bool nt) {
+                    double *part, bool nt, int nrhs, size_t vsf, size_t vsc) {
   if (PT.nslices <= 0) return 0;
   const int g = nblk_slices(PT.nslices);
-  NT_DISPATCH(nt, k_restrict<T, NTV><<<g, kBlock, 0, st>>>(PT, PT.nslices, cskip, cvnclass,
-                                                          cvclass, bf, damp, bc, part));
+  NT_DISPATCH(nt, k_restrict<T, NTV><<<grid2(g, nrhs), kBlock, 0, st>>>(
+                      PT, PT.nslices, cskip, cvnclass, cvclass, bf, damp, bc, part, vsf, vsc));
   return g;
 }
 template int launch_restrict<double>(hipStream_t, const DSell<double> &, const uint32_t *,
                                      const uint8_t *, const uint8_t *, const double *, double,
-                                     double *, double *, bool);
+                                     double *, double *, bool, int, size_t, size_t);
 template int launch_restrict<float>(hipStream_t, const DSell<float> &, const uint32_t *,
                                     const uint8_t *, const uint8_t *, const float *, float,
-                                    float *, double *, bool);
+                                    float *, double *, bool, int, size_t, size_t);
 
 // --------------------------------------------------- base solve (GEMV)
 // ls over ex on the base level, one wavefront per row of the dense inverse:
@@ -599,9 +634,12 @@ __global__ __launch_bounds__(kBlock) void k_base_gemv(int n, const double *__res
                                                        int cmode, double damp, double abslimit,
                                                        double red, const double *__restrict__ part,
                                                        int nparts, double *__restrict__ first,
-                                                       int it) {
+                                                       int it, size_t vs) {
   const int row = (blockIdx.x * kBlock + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
+  const int cj = blockIdx.y;
+  const size_t o = (size_t)cj * vs;
+  part += (size_t)cj * nparts;
   double s = 0.0;
   for (int i = lane; i < nparts; i += 64) s += part[i];
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
@@ -609,9 +647,9 @@ __global__ __launch_bounds__(kBlock) void k_base_gemv(int n, const double *__res
   bool conv;
   if (it == 0) {
     conv = fabs(cur) < fabs(abslimit);
-    if (row == 0 && lane == 0) *first = cur;
+    if (row == 0 && lane == 0) first[cj] = cur;
   } else {
-    double target = *first * red;
+    double target = first[cj] * red;
     if (target == 0.0) target = red;
     conv = fabs(cur) < fabs(abslimit) || fabs(cur) < fabs(target);
   }
@@ -619,53 +657,57 @@ __global__ __launch_bounds__(kBlock) void k_base_gemv(int n, const double *__res
   double x = 0.0;
   if (!conv) {
     const double *__restrict__ rr = inv + (size_t)row * n;
-    for (int j = lane; j < n; j += 64) x += rr[j] * (double)b[j];
+    const T *__restrict__ bb = b + o;
+    for (int j = lane; j < n; j += 64) x += rr[j] * (double)bb[j];
     for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
     x *= damp;
   }
   if (lane == 0) {
     const T xv = (T)x;
-    if (!conv || cmode == 0) c[row] = cmode ? c[row] + xv : xv;
-    if (xout) xout[row] = xv;
+    if (!conv || cmode == 0) c[o + row] = cmode ? c[o + row] + xv : xv;
+    if (xout) xout[o + row] = xv;
   }
 }
 
 template <typename T>
 void launch_base_gemv(hipStream_t st, int n, const double *inv, const T *b, T *c, T *xout,
                       int cmode, double damp, double abslimit, double red, const double *part,
-                      int nparts, double *first, int it) {
+                      int nparts, double *first, int it, Batch bt) {
   const int g = (n * 64 + kBlock - 1) / kBlock;
-  k_base_gemv<T><<<g, kBlock, 0, st>>>(n, inv, b, c, xout, cmode, damp, abslimit, red, part,
-                                       nparts, first, it);
+  k_base_gemv<T><<<grid2(g, bt.nrhs), kBlock, 0, st>>>(n, inv, b, c, xout, cmode, damp, abslimit,
+                                                       red, part, nparts, first, it, bt.vs);
 }
 template void launch_base_gemv<double>(hipStream_t, int, const double *, const double *,
                                        double *, double *, int, double, double, double,
-                                       const double *, int, double *, int);
+                                       const double *, int, double *, int, Batch);
 template void launch_base_gemv<float>(hipStream_t, int, const double *, const float *, float *,
                                       float *, int, double, double, double, const double *, int,
-                                      double *, int);
+                                      double *, int, Batch);
 
 template <typename T>
-int launch_dot_part(hipStream_t st, int n, const T *x, const uint8_t *vclass, double *part) {
+int launch_dot_part(hipStream_t st, int n, const T *x, const uint8_t *vclass, double *part,
+                    Batch bt) {
   const int g = reduce_parts(n);
-  k_dot_part<T, 0><<<g, kBlock, 0, st>>>(n, x, nullptr, vclass, part);
+  k_dot_part<T, 0><<<grid2(g, bt.nrhs), kBlock, 0, st>>>(n, x, nullptr, vclass, part, bt.vs);
   return g;
 }
 
 template <typename T>
 void launch_interp(hipStream_t st, const DSell<T> &P, const uint32_t *fskip, const T *cc, T damp,
-                   T *t, T *c, bool nt) {
+                   T *t, T *c, bool nt, int nrhs, size_t vsf, size_t vsc) {
   if (P.nslices <= 0) return;
-  dim3 g(nblk_slices(P.nslices));
+  const dim3 g = grid2(nblk_slices(P.nslices), nrhs);
   NT_DISPATCH(nt, if (c) k_interp<T, 1, NTV><<<g, kBlock, 0, st>>>(P, P.nslices, fskip, cc,
-                                                                   damp, t, c);
+                                                                   damp, t, c, vsf, vsc);
               else k_interp<T, 0, NTV><<<g, kBlock, 0, st>>>(P, P.nslices, fskip, cc, damp, t,
-                                                             c));
+                                                             c, vsf, vsc));
 }
 template void launch_interp<double>(hipStream_t, const DSell<double> &, const uint32_t *,
-                                    const double *, double, double *, double *, bool);
+                                    const double *, double, double *, double *, bool, int,
+                                    size_t, size_t);
 template void launch_interp<float>(hipStream_t, const DSell<float> &, const uint32_t *,
-                                   const float *, float, float *, float *, bool);
+                                   const float *, float, float *, float *, bool, int, size_t,
+                                   size_t);
 
 int reduce_parts(int n) {
   int g = nblk(n);
@@ -674,44 +716,51 @@ int reduce_parts(int n) {
 
 template <typename T>
 void launch_dot(hipStream_t st, int n, const T *x, const T *y, const uint8_t *vclass,
-                double *part, double *out, int do_sqrt) {
+                double *part, double *out, int do_sqrt, Batch bt) {
   const int g = reduce_parts(n);
-  if (y) k_dot_part<T, 1><<<g, kBlock, 0, st>>>(n, x, y, vclass, part);
-  else k_dot_part<T, 0><<<g, kBlock, 0, st>>>(n, x, y, vclass, part);
-  k_reduce_final<<<1, 1024, 0, st>>>(g, part, out, do_sqrt);
+  const dim3 gg = grid2(g, bt.nrhs);
+  if (y) k_dot_part<T, 1><<<gg, kBlock, 0, st>>>(n, x, y, vclass, part, bt.vs);
+  else k_dot_part<T, 0><<<gg, kBlock, 0, st>>>(n, x, y, vclass, part, bt.vs);
+  k_reduce_final<<<gg.y, 1024, 0, st>>>(g, part, out, do_sqrt);
 }
 template void launch_dot<double>(hipStream_t, int, const double *, const double *,
-                                 const uint8_t *, double *, double *, int);
+                                 const uint8_t *, double *, double *, int, Batch);
 template void launch_dot<float>(hipStream_t, int, const float *, const float *, const uint8_t *,
-                                double *, double *, int);
-template int launch_dot_part<double>(hipStream_t, int, const double *, const uint8_t *, double *);
-template int launch_dot_part<float>(hipStream_t, int, const float *, const uint8_t *, double *);
+                                double *, double *, int, Batch);
+template int launch_dot_part<double>(hipStream_t, int, const double *, const uint8_t *, double *,
+                                     Batch);
+template int launch_dot_part<float>(hipStream_t, int, const float *, const uint8_t *, double *,
+                                    Batch);
 
-void launch_reduce_final(hipStream_t st, int nparts, const double *part, double *out,
-                         int do_sqrt) {
-  k_reduce_final<<<1, 1024, 0, st>>>(nparts, part, out, do_sqrt);
+void launch_reduce_final(hipStream_t st, int nparts, const double *part, double *out, int do_sqrt,
+                         int nrhs) {
+  k_reduce_final<<<nrhs < 1 ? 1 : nrhs, 1024, 0, st>>>(nparts, part, out, do_sqrt);
 }
 
 template <typename T>
-void launch_fill(hipStream_t st, int n, T *x, T val) {
-  if (n > 0) k_fill<T><<<nblk(n), kBlock, 0, st>>>(n, x, val);
+void launch_fill(hipStream_t st, int n, T *x, T val, Batch bt) {
+  if (n > 0) k_fill<T><<<grid2(nblk(n), bt.nrhs), kBlock, 0, st>>>(n, x, val, bt.vs);
 }
-template void launch_fill<double>(hipStream_t, int, double *, double);
-template void launch_fill<float>(hipStream_t, int, float *, float);
+template void launch_fill<double>(hipStream_t, int, double *, double, Batch);
+template void launch_fill<float>(hipStream_t, int, float *, float, Batch);
 
 template <typename TD, typename TS>
-void launch_convert(hipStream_t st, int n, const TS *s, TD *d) {
-  if (n > 0) k_convert<TD, TS><<<nblk(n), kBlock, 0, st>>>(n, s, d);
+void launch_convert(hipStream_t st, int n, const TS *s, TD *d, int nrhs, size_t vss, size_t vsd) {
+  if (n > 0) k_convert<TD, TS><<<grid2(nblk(n), nrhs), kBlock, 0, st>>>(n, s, d, vss, vsd);
 }
-template void launch_convert<float, double>(hipStream_t, int, const double *, float *);
-template void launch_convert<double, float>(hipStream_t, int, const float *, double *);
+template void launch_convert<float, double>(hipStream_t, int, const double *, float *, int, size_t,
+                                            size_t);
+template void launch_convert<double, float>(hipStream_t, int, const float *, double *, int, size_t,
+                                            size_t);
+template void launch_convert<double, double>(hipStream_t, int, const double *, double *, int,
+                                             size_t, size_t);
 
 template <typename T>
-void launch_add(hipStream_t st, int n, T *x, const T *y) {
-  if (n > 0) k_add<T><<<nblk(n), kBlock, 0, st>>>(n, x, y);
+void launch_add(hipStream_t st, int n, T *x, const T *y, Batch bt) {
+  if (n > 0) k_add<T><<<grid2(nblk(n), bt.nrhs), kBlock, 0, st>>>(n, x, y, bt.vs);
 }
-template void launch_add<double>(hipStream_t, int, double *, const double *);
-template void launch_add<float>(hipStream_t, int, float *, const float *);
+template void launch_add<double>(hipStream_t, int, double *, const double *, Batch);
+template void launch_add<float>(hipStream_t, int, float *, const float *, Batch);
 
 template <typename T>
 void launch_coarse_cycle(hipStream_t st, const CoarseArgs<T> &args, int base_n) {
@@ -720,5 +769,371 @@ void launch_coarse_cycle(hipStream_t st, const CoarseArgs<T> &args, int base_n) 
 }
 template void launch_coarse_cycle<double>(hipStream_t, const CoarseArgs<double> &, int);
 template void launch_coarse_cycle<float>(hipStream_t, const CoarseArgs<float> &, int);
+
+// ============================================================ BCGSSolver
+// Right-preconditioned BiCGStab of np/procs/ls.cc:1864-2062 for several
+// right-hand sides at once.  Every column runs its own recurrence: the
+// scalars live in BcgsState on the device and change only in
+// k_bcgs_scalar (one workgroup per column), so a whole iteration is
+// stream-ordered device work with no host round trip.  UG's operation order
+// is kept per entry: p = ((beta p) + b) + (-beta omega) v, x = x + alpha q,
+// s = b + (-alpha) v, x = x + omega q, b = s + (-omega) t.
+
+int bcgs_nparts(int n) { return reduce_parts(n); }
+int bcgs_spmv_nparts(int nslices) { return nblk_slices(nslices) < 1 ? 1 : nblk_slices(nslices); }
+
+// sc_eq (np/udm/npscan.cc:1095-1104) for one component
+__device__ __forceinline__ bool sc_eq1(double x, double y, double ac) {
+  return !(x < 0.0 || y < 0.0 || fabs(x - y) > ac * sqrt(x * y));
+}
+
+__device__ __forceinline__ void record_defect(BcgsState &S, double *defects, int maxdef, int cj,
+                                              double d) {
+  if (defects && S.nd < maxdef) defects[(size_t)cj * maxdef + S.nd] = d;
+  ++S.nd;
+}
+
+__global__ __launch_bounds__(1024) void k_bcgs_scalar(int op, int nparts,
+                                                       const double *__restrict__ part,
+                                                       BcgsState *__restrict__ st, int restart,
+                                                       double abslimit, double red,
+                                                       double *__restrict__ defects, int maxdef) {
+  __shared__ double sh[2][16];
+  const int cj = blockIdx.x;
+  const double *p0 = part + (size_t)(cj * 2) * nparts;
+  const double *p1 = p0 + nparts;
+  // RHO after a restart reads b.b (k = 0): r was just set to b
+  const bool two = op == BCGS_OMEGA;
+  const double *pa = (op == BCGS_RHO && !restart) ? p1 : p0;
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += 1024) {
+    a += pa[i];
+    if (two) b += p1[i];
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_down(a, off, 64);
+    b += __shfl_down(b, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    sh[0][threadIdx.x >> 6] = a;
+    sh[1][threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  a = 0.0;
+  b = 0.0;
+  for (int i = 0; i < 16; ++i) {
+    a += sh[0][i];
+    b += sh[1][i];
+  }
+  BcgsState S = st[cj];
+  switch (op) {
+    case BCGS_FIRST: {
+      const double first = sqrt(a);
+      S = BcgsState{};
+      S.first = S.last = first;
+      S.target = first * red;   // sc_mul_check
+      if (S.target == 0.0) S.target = red;
+      S.old = -1.0;
+      S.abslimit = abslimit;
+      S.conv = fabs(first) < fabs(abslimit);
+      S.active = !S.conv;
+      record_defect(S, defects, maxdef, cj, first);
+      break;
+    }
+    case BCGS_RHO:
+      if (!S.active) return;
+      if (restart) S.alpha = S.rho = S.omega = 1.0;
+      S.rho_new = a;
+      if (S.rho != 0.0 && S.omega != 0.0) S.beta = S.rho_new * S.alpha / S.rho / S.omega;
+      break;
+    case BCGS_ALPHA:
+      if (!S.active) return;
+      S.alpha = a != 0.0 ? S.rho_new / a : a;
+      break;
+    case BCGS_RES1:
+      if (!S.active) return;
+      S.last = sqrt(a);
+      ++S.nit;
+      record_defect(S, defects, maxdef, cj, S.last);
+      if (fabs(S.last) < fabs(S.abslimit) || fabs(S.last) < fabs(S.target)) {
+        S.half = 1;
+        S.conv = 1;
+      }
+      break;
+    case BCGS_OMEGA:
+      if (!S.active || S.half) return;
+      S.omega = a != 0.0 ? b / a : b;   // a = (t,t), b = (s,t)
+      break;
+    default: {  // BCGS_RES2
+      if (!S.active) return;
+      if (S.half) {
+        S.active = 0;
+        break;
+      }
+      S.rho = S.rho_new;
+      S.last = sqrt(a);
+      record_defect(S, defects, maxdef, cj, S.last);
+      ++S.nit;
+      if (fabs(S.last) < fabs(S.abslimit) || fabs(S.last) < fabs(S.target)) {
+        S.conv = 1;
+        S.active = 0;
+        break;
+      }
+      if (sc_eq1(S.last, S.old, 1e-4)) ++S.eq_count;
+      else S.eq_count = 0;
+      S.old = S.last;
+      if (S.eq_count > 4) {
+        S.conv = 0;
+        S.active = 0;
+      }
+      break;
+    }
+  }
+  st[cj] = S;
+}
+
+void launch_bcgs_scalar(hipStream_t st, int op, int nparts, const double *part, BcgsState *s,
+                        int nrhs, int restart, double abslimit, double red, double *defects,
+                        int maxdef) {
+  k_bcgs_scalar<<<nrhs, 1024, 0, st>>>(op, nparts, part, s, restart, abslimit, red, defects,
+                                       maxdef);
+}
+
+__global__ __launch_bounds__(kBlock) void k_bcgs_p(int n, const double *__restrict__ b,
+                                                    double *__restrict__ r, double *__restrict__ v,
+                                                    double *__restrict__ p,
+                                                    double *__restrict__ w64,
+                                                    float *__restrict__ w32,
+                                                    const BcgsState *__restrict__ st, int restart,
+                                                    size_t vs) {
+  const int cj = blockIdx.y;
+  if (!st[cj].active) return;
+  const double beta = st[cj].beta, mbo = -beta * st[cj].omega;
+  const size_t o = (size_t)cj * vs;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const double bi = b[o + i];
+    double pi, vi;
+    if (restart) {
+      r[o + i] = bi;
+      v[o + i] = 0.0;
+      pi = 0.0;
+      vi = 0.0;
+    } else {
+      pi = p[o + i];
+      vi = v[o + i];
+    }
+    double t = beta * pi;
+    t = t + bi;
+    t = t + mbo * vi;
+    p[o + i] = t;
+    if (w64) w64[o + i] = t;
+    if (w32) w32[o + i] = (float)t;
+  }
+}
+
+void launch_bcgs_p(hipStream_t st, int n, const double *b, double *r, double *v, double *p,
+                   double *w64, float *w32, const BcgsState *s, int restart, Batch bt) {
+  k_bcgs_p<<<grid2(bcgs_nparts(n), bt.nrhs), kBlock, 0, st>>>(n, b, r, v, p, w64, w32, s, restart,
+                                                              bt.vs);
+}
+
+template <typename TQ, int NT>
+__global__ __launch_bounds__(kBlock) void k_bcgs_spmv(DSell<double> A, int nsl,
+                                                       const TQ *__restrict__ q,
+                                                       double *__restrict__ y,
+                                                       const double *__restrict__ z,
+                                                       const uint8_t *__restrict__ vclass,
+                                                       double *__restrict__ part, int yy,
+                                                       int skip_half,
+                                                       const BcgsState *__restrict__ st,
+                                                       size_t vs) {
+  __shared__ double red[2][kBlock / 64];
+  SLICE_PROLOGUE_RED(A, nsl)
+  if (!st[col_].active || (skip_half && st[col_].half)) return;   // uniform per block
+  const size_t o = (size_t)col_ * vs;
+  double yz = 0.0, y2 = 0.0;
+  if (sok) {
+    const double sum =
+        sell_dot<NT, double, TQ, double>(A.val + e0 + lane, A.col + e0 + lane, w, q + o, sdo, A.dtab, rb);
+    if (r >= 0) {
+      y[o + r] = sum;
+      if (vclass[r] >= 2) {
+        yz = sum * z[o + r];
+        y2 = sum * sum;
+      }
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    yz += __shfl_down(yz, off, 64);
+    y2 += __shfl_down(y2, off, 64);
+  }
+  if (lane == 0) {
+    red[0][threadIdx.x >> 6] = yz;
+    red[1][threadIdx.x >> 6] = y2;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0, b2 = 0.0;
+    for (int i = 0; i < kBlock / 64; ++i) {
+      a += red[0][i];
+      b2 += red[1][i];
+    }
+    // OMEGA wants (t,t) in k = 0 and (s,t) in k = 1; ALPHA wants (v,r) in k = 0
+    double *pp = part + (size_t)(col_ * 2) * gridDim.x;
+    if (yy) {
+      pp[sb_] = b2;
+      pp[gridDim.x + sb_] = a;
+    } else {
+      pp[sb_] = a;
+    }
+  }
+}
+
+template <typename TQ>
+void launch_bcgs_spmv(hipStream_t st, const DSell<double> &A, const TQ *q, double *y,
+                      const double *z, const uint8_t *vclass, double *part, int yy, int skip_half, const BcgsState *s, bool nt, Batch bt) {
+  if (A.nslices <= 0) return;
+  const dim3 g = grid2(nblk_slices(A.nslices), bt.nrhs);
+  NT_DISPATCH(nt, k_bcgs_spmv<TQ, NTV><<<g, kBlock, 0, st>>>(A, A.nslices, q, y, z, vclass, part,
+                                                             yy, skip_half, s, bt.vs));
+}
+template void launch_bcgs_spmv<double>(hipStream_t, const DSell<double> &, const double *,
+                                       double *, const double *, const uint8_t *, double *, int,
+                                       int, const BcgsState *, bool, Batch);
+template void launch_bcgs_spmv<float>(hipStream_t, const DSell<double> &, const float *, double *,
+                                      const double *, const uint8_t *, double *, int, int,
+                                      const BcgsState *, bool, Batch);
+
+template <typename TQ>
+__global__ __launch_bounds__(kBlock) void k_bcgs_s(int n, const double *__restrict__ b,
+                                                    const double *__restrict__ v,
+                                                    const TQ *__restrict__ q,
+                                                    double *__restrict__ x,
+                                                    double *__restrict__ sv,
+                                                    double *__restrict__ w64,
+                                                    float *__restrict__ w32,
+                                                    const uint8_t *__restrict__ vclass,
+                                                    double *__restrict__ part,
+                                                    const BcgsState *__restrict__ st, size_t vs) {
+  __shared__ double red[kBlock / 64];
+  const int cj = blockIdx.y;
+  if (!st[cj].active) return;   // uniform per block
+  const double alpha = st[cj].alpha, malpha = -alpha;
+  const size_t o = (size_t)cj * vs;
+  double s2 = 0.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    x[o + i] = x[o + i] + alpha * (double)q[o + i];
+    const double si = b[o + i] + malpha * v[o + i];
+    sv[o + i] = si;
+    if (w64) w64[o + i] = si;
+    if (w32) w32[o + i] = (float)si;
+    if (vclass[i] >= 2) s2 += si * si;
+  }
+  const double t = block_sum(s2, red);
+  if (threadIdx.x == 0) part[(size_t)(cj * 2) * gridDim.x + blockIdx.x] = t;
+}
+
+template <typename TQ>
+void launch_bcgs_s(hipStream_t st, int n, const double *b, const double *v, const TQ *q,
+                   double *x, double *sv, double *w64, float *w32, const uint8_t *vclass,
+                   double *part, const BcgsState *s, Batch bt) {
+  k_bcgs_s<TQ><<<grid2(bcgs_nparts(n), bt.nrhs), kBlock, 0, st>>>(n, b, v, q, x, sv, w64, w32,
+                                                                  vclass, part, s, bt.vs);
+}
+template void launch_bcgs_s<double>(hipStream_t, int, const double *, const double *,
+                                    const double *, double *, double *, double *, float *,
+                                    const uint8_t *, double *, const BcgsState *, Batch);
+template void launch_bcgs_s<float>(hipStream_t, int, const double *, const double *,
+                                   const float *, double *, double *, double *, float *,
+                                   const uint8_t *, double *, const BcgsState *, Batch);
+
+template <typename TQ>
+__global__ __launch_bounds__(kBlock) void k_bcgs_b(int n, const double *__restrict__ sv,
+                                                    const double *__restrict__ t,
+                                                    const TQ *__restrict__ q,
+                                                    const double *__restrict__ r,
+                                                    double *__restrict__ x,
+                                                    double *__restrict__ b,
+                                                    const uint8_t *__restrict__ vclass,
+                                                    double *__restrict__ part,
+                                                    const BcgsState *__restrict__ st, size_t vs) {
+  __shared__ double red[2][kBlock / 64];
+  const int cj = blockIdx.y;
+  if (!st[cj].active) return;   // uniform per block
+  const bool half = st[cj].half;
+  const double omega = st[cj].omega, momega = -omega;
+  const size_t o = (size_t)cj * vs;
+  double b2 = 0.0, br = 0.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    double bi;
+    if (half) {
+      bi = sv[o + i];
+    } else {
+      x[o + i] = x[o + i] + omega * (double)q[o + i];
+      bi = sv[o + i] + momega * t[o + i];
+    }
+    b[o + i] = bi;
+    if (vclass[i] >= 2) {
+      b2 += bi * bi;
+      br += bi * r[o + i];
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    b2 += __shfl_down(b2, off, 64);
+    br += __shfl_down(br, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = b2;
+    red[1][threadIdx.x >> 6] = br;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0, c = 0.0;
+    for (int i = 0; i < kBlock / 64; ++i) {
+      a += red[0][i];
+      c += red[1][i];
+    }
+    double *pp = part + (size_t)(cj * 2) * gridDim.x;
+    pp[blockIdx.x] = a;
+    pp[gridDim.x + blockIdx.x] = c;
+  }
+}
+
+template <typename TQ>
+void launch_bcgs_b(hipStream_t st, int n, const double *sv, const double *t, const TQ *q,
+                   const double *r, double *x, double *b, const uint8_t *vclass, double *part,
+                   const BcgsState *s, Batch bt) {
+  k_bcgs_b<TQ><<<grid2(bcgs_nparts(n), bt.nrhs), kBlock, 0, st>>>(n, sv, t, q, r, x, b, vclass,
+                                                                  part, s, bt.vs);
+}
+template void launch_bcgs_b<double>(hipStream_t, int, const double *, const double *,
+                                    const double *, const double *, double *, double *,
+                                    const uint8_t *, double *, const BcgsState *, Batch);
+template void launch_bcgs_b<float>(hipStream_t, int, const double *, const double *,
+                                   const float *, const double *, double *, double *,
+                                   const uint8_t *, double *, const BcgsState *, Batch);
+
+__global__ __launch_bounds__(kBlock) void k_bcgs_norm(int n, const double *__restrict__ x,
+                                                       const uint8_t *__restrict__ vclass,
+                                                       double *__restrict__ part, size_t vs) {
+  __shared__ double red[kBlock / 64];
+  const int cj = blockIdx.y;
+  const size_t o = (size_t)cj * vs;
+  double s = 0.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+    if (vclass[i] >= 2) s += x[o + i] * x[o + i];
+  const double t = block_sum(s, red);
+  if (threadIdx.x == 0) {
+    double *pp = part + (size_t)(cj * 2) * gridDim.x;
+    pp[blockIdx.x] = t;
+    pp[gridDim.x + blockIdx.x] = t;
+  }
+}
+
+void launch_bcgs_norm(hipStream_t st, int n, const double *x, const uint8_t *vclass, double *part,
+                      Batch bt) {
+  k_bcgs_norm<<<grid2(bcgs_nparts(n), bt.nrhs), kBlock, 0, st>>>(n, x, vclass, part, bt.vs);
+}
 
 }  // namespace ugmg

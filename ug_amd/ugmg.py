@@ -97,6 +97,11 @@ def load():
         "ugmg_solve": ([vp, ip, dp, dp, dp, dp, ip, ip, C.POINTER(LResult), DEFECT_CB, vp], ip),
         "ugmg_solve_host": ([vp, ip, dp, dp, dp, dp, ip, ip, C.POINTER(LResult), DEFECT_CB, vp], ip),
         "ugmg_time_kernel": ([vp, ip, ip, ip, C.POINTER(C.c_double), C.POINTER(C.c_double)], ip),
+        "ugmg_set_batch": ([vp, ip], ip),
+        "ugmg_iter_batch": ([vp, ip, ip, dp, dp, ip], ip),
+        "ugmg_bcgs_solve": ([vp, ip, ip, dp, dp, ip, dp, dp, ip, ip, ip, ip, C.POINTER(LResult), dp, ip], ip),
+        "ugmg_bcgs_solve_host": ([vp, ip, ip, dp, dp, ip, dp, dp, ip, ip, ip, ip, C.POINTER(LResult), dp, ip],
+                                 ip),
         "ugmg_color": ([ip, vp, vp, vp], ip),
         "ugmg_coarse_inverse": ([ip, vp, vp, vp, ip, vp, vp, C.POINTER(ip)], ip),
     }
@@ -310,6 +315,49 @@ class Context:
         self._chk(self.lib.ugmg_solve_host(self.h, level, x.ctypes.data, b.ctypes.data, C.addressof(ab),
                                            C.addressof(rd), maxiter, int(do_defect), C.byref(res), cb, None))
         return x, b, res, np.array(defects)
+
+    # ------------------------------------------------------- several columns
+    def set_batch(self, nrhs):
+        """Lay the level vectors out for nrhs columns (before preprocess)."""
+        self._chk(self.lib.ugmg_set_batch(self.h, nrhs))
+
+    def iter_batch(self, level, c, b):
+        """lmgc on every column of the (nrhs, ld) device tensors c, b."""
+        nrhs, ld = b.shape
+        self._chk(self.lib.ugmg_iter_batch(self.h, level, nrhs, _ptr(c), _ptr(b), ld))
+
+    def bcgs(self, level, x, b, red=1e-8, abslimit=1e-300, maxiter=100, restart=0, use_iter=True,
+             do_defect=True, maxdef=None):
+        """BCGSSolver on the (nrhs, ld) device tensors x, b (updated in
+        place); returns (list of LResult, defects array (nrhs, maxdef))."""
+        nrhs, ld = b.shape
+        md = 2 * maxiter + 1 if maxdef is None else maxdef
+        res = (LResult * nrhs)()
+        dfs = np.zeros((nrhs, md))
+        ab = (C.c_double * 3)(abslimit, abslimit, abslimit)
+        rd = (C.c_double * 3)(red, red, red)
+        self._chk(self.lib.ugmg_bcgs_solve(self.h, level, nrhs, _ptr(x), _ptr(b), ld, C.addressof(ab),
+                                           C.addressof(rd), maxiter, restart, int(use_iter), int(do_defect),
+                                           res, dfs.ctypes.data, md))
+        return list(res), dfs
+
+    def bcgs_host(self, level, rhs, x0=None, red=1e-8, abslimit=1e-300, maxiter=100, restart=0,
+                  use_iter=True, do_defect=True):
+        """BCGSSolver on host arrays rhs (nrhs, n) -> (x, b, results, defect lists)."""
+        rhs = np.atleast_2d(np.asarray(rhs, dtype=np.float64))
+        nrhs, n = rhs.shape
+        x = np.zeros((nrhs, n)) if x0 is None else np.array(np.atleast_2d(x0), dtype=np.float64, copy=True)
+        b = np.ascontiguousarray(rhs.copy())
+        md = 2 * maxiter + 1
+        res = (LResult * nrhs)()
+        dfs = np.zeros((nrhs, md))
+        ab = (C.c_double * 3)(abslimit, abslimit, abslimit)
+        rd = (C.c_double * 3)(red, red, red)
+        self._chk(self.lib.ugmg_bcgs_solve_host(self.h, level, nrhs, x.ctypes.data, b.ctypes.data, n,
+                                                C.addressof(ab), C.addressof(rd), maxiter, restart,
+                                                int(use_iter), int(do_defect), res, dfs.ctypes.data, md))
+        seqs = [dfs[j, :min(res[j].number_of_linear_iterations + 1, md)] for j in range(nrhs)]
+        return x, b, list(res), seqs
 
     def time_kernel(self, level, kind, reps=20):
         ms, by = C.c_double(), C.c_double()

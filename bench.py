@@ -121,6 +121,80 @@ def serving_run(n_req, rate, seed, budget_gb, slots):
     }
 
 
+def bcgs_run(nref, nrhs, steps, warmup, cpu_sample):
+    """BiCGStab preconditioned by one V(1,1) multicolour-GS cycle on the SUPG
+    P1 convection-diffusion hierarchy (eps 1e-3, wind angle from the seed),
+    nrhs right-hand sides ~U[-1,1] solved together to 1e-8: FP32 cycle with
+    FP64 BiCGStab vectors, dots and norms, beside the all-FP64 form and a
+    single right-hand side."""
+    import numpy as np
+    import torch
+    from ug_amd import problems, ugmg
+
+    P = problems.p1_cd(nref, 1e-3, float("nan"), seed=0x5EED0003)
+    top, n = P.top, P.levels[-1].n
+    R = problems.splitmix_uniform(0x5EED0013, nrhs * n, -1.0, 1.0).reshape(nrhs, n)
+    R[:, P.levels[-1].skip == 1] = 0.0
+    bl = max(l for l, lv in enumerate(P.levels) if lv.n <= 729)
+    prm = dict(nu1=1, nu2=1, gamma=1, baselevel=bl, smoother=ugmg.SMOOTH_GS, damp=1.0,
+               base_abslimit=1e-300, base_red=1e-12, base_maxiter=1, use_graph=1)
+    Rt = torch.from_numpy(R).cuda()
+    X = torch.zeros_like(Rt)
+    B = torch.empty_like(Rt)
+    out = {"workload": f"p1cd_{2 ** (nref + 1) + 1}^2: -1e-3 Lap u + b.grad u = f, SUPG P1, "
+                       f"theta={P.theta:.4f}, BiCGStab + V(1,1) MC-GS, base level {bl}, {nrhs} rhs, tol 1e-8",
+           "unknowns": n, "nnz_fine": int(P.levels[-1].nnzb), "levels": top + 1, "nrhs": nrhs}
+
+    def run(ctx, k, cols):
+        lat, res = [], None
+        for _ in range(k):
+            X[:cols].zero_()
+            B[:cols].copy_(Rt[:cols])
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            res, _ = ctx.bcgs(top, X[:cols], B[:cols], red=1e-8, maxiter=200, maxdef=1)
+            torch.cuda.synchronize()
+            lat.append(time.perf_counter() - t)
+        return float(np.mean(lat)), res
+
+    for name, prec in (("mixed", ugmg.MIXED), ("fp64", ugmg.FP64)):
+        ctx = ugmg.Context(0)
+        ctx.set_hierarchy(P)
+        ctx.set_params(**prm, precision=prec, inner_cycles=1)
+        ctx.set_batch(nrhs)
+        ctx.preprocess()
+        run(ctx, warmup, nrhs)
+        t_all, res = run(ctx, steps, nrhs)
+        # true defect of every column, FP64 on the GPU
+        Bt = Rt.clone()
+        torch.cuda.synchronize()
+        for j in range(nrhs):
+            ctx.dmatmul_minus(top, Bt[j], X[j])
+        ctx.sync()
+        rel = (torch.linalg.norm(Bt, dim=1) / torch.linalg.norm(Rt, dim=1)).max().item()
+        run(ctx, warmup, 1)
+        t_one, _ = run(ctx, steps, 1)
+        its = [r.number_of_linear_iterations for r in res]
+        out[name] = {"ms": round(t_all * 1e3, 3), "ms_single_rhs": round(t_one * 1e3, 3),
+                     "batch_gain_per_rhs": round(t_one * nrhs / t_all, 2),
+                     "half_steps_max": max(its), "half_steps_mean": round(float(np.mean(its)), 1),
+                     "all_converged": all(r.converged for r in res), "max_true_rel_defect": rel}
+        ctx.close()
+    out["speedup_mixed_vs_fp64"] = round(out["fp64"]["ms"] / out["mixed"]["ms"], 3)
+    if cpu_sample > 0:
+        from oracle import oracle as O
+        H = O.Hierarchy(P.levels)
+        p = O.params(smoother=O.MCGS, damp=1.0, baselevel=bl, nu1=1, nu2=1)
+        t = time.perf_counter()
+        for j in range(cpu_sample):
+            O.bcgs_solve(H, top, p, R[j], red=1e-8, maxiter=200)
+        per = (time.perf_counter() - t) / cpu_sample
+        out["cpu_baseline"] = {"value": round(per * nrhs * 1e3, 1), "unit": "ms", "cores": 1, "kind": "port",
+                               "sample": f"{cpu_sample} of the {nrhs} right-hand sides solved one at a time by the "
+                                         f"oracle's BCGSSolver + lmgc (FP64, 1 thread), scaled to {nrhs}"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -134,6 +208,9 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=10.0, help="Poisson arrival rate (1/s)")
     ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--bcgs-nref", type=int, default=8, help="P1 refinements of the BiCGStab leg (8: 513^2)")
+    ap.add_argument("--bcgs-rhs", type=int, default=32)
+    ap.add_argument("--no-bcgs", action="store_true")
     ap.add_argument("--kernels-only", action="store_true",
                     help="run only the roofline kernels and a copy kernel of known size (for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -274,6 +351,10 @@ def main():
                                "avg_launch_us": round(ms_spmv32 * 1e3, 2)},
         "setup_s": {"generate": round(gen_s, 2)},
     }
+
+    if not args.no_bcgs:
+        out["bcgs_cd2d"] = bcgs_run(args.bcgs_nref, args.bcgs_rhs, args.steps, args.warmup,
+                                    0 if args.no_cpu else 2)
 
     if not args.no_serve:
         out["serving"] = serving_run(args.serve_requests, args.serve_rate, 0x5EED0005, 32.0, args.serve_slots)

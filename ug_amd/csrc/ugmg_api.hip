@@ -40,8 +40,15 @@ struct DevSellBuf {
   int *soff = nullptr, *rows = nullptr, *col = nullptr, *sdoff = nullptr, *dtab = nullptr;
   double *v64 = nullptr;
   float *v32 = nullptr;
-  DSell<double> d64() const { return {nslices, soff, rows, col, v64, sdoff, dtab}; }
-  DSell<float> d32() const { return {nslices, soff, rows, col, v32, sdoff, dtab}; }
+  int wmax = 0;                 // widest slice
+  std::vector<int> width_h;     // per-slice width (host)
+  int max_width(int s0, int s1) const {
+    int m = 0;
+    for (int s = s0; s < s1; ++s) m = std::max(m, width_h[s]);
+    return m;
+  }
+  DSell<double> d64() const { return {nslices, soff, rows, col, v64, sdoff, dtab, wmax}; }
+  DSell<float> d32() const { return {nslices, soff, rows, col, v32, sdoff, dtab, wmax}; }
   template <typename T>
   DSell<T> d() const {
     if constexpr (std::is_same_v<T, double>) return d64();
@@ -53,7 +60,7 @@ struct DevLevel {
   int n = 0, nnzb = 0;
   DevSellBuf A, L, U;
   int ncolors = 0;
-  std::vector<int> color_slices, color;
+  std::vector<int> color_slices, color, color_wmax;
   double *diag64 = nullptr;
   float *diag32 = nullptr;
   uint8_t *vclass = nullptr, *vnclass = nullptr;
@@ -62,6 +69,10 @@ struct DevLevel {
   DevSellBuf P, PT;   // interpolation from / restriction to level-1
   bool nt = false;    // stream the matrix with non-temporal loads
   size_t vs = 0;      // column stride of the level vectors (batched launches)
+  // device order (GS levels): perm_h[p] = list index of device position p;
+  // empty / null: list order
+  std::vector<int> perm_h;
+  int *perm = nullptr;
   double *c = nullptr, *b = nullptr, *t = nullptr;
   float *c32 = nullptr, *b32 = nullptr, *t32 = nullptr;
 
@@ -161,6 +172,12 @@ int upload(ugmg_ctx *ctx, T **out, const T *src, size_t n) {
 int upload_sell(ugmg_ctx *ctx, const HostSell &S, DevSellBuf &D) {
   D.nslices = S.nslices;
   D.nent = S.col.size();
+  D.width_h.resize(S.nslices);
+  D.wmax = 0;
+  for (int q = 0; q < S.nslices; ++q) {
+    D.width_h[q] = (S.soff[q + 1] - S.soff[q]) / kSlice;
+    D.wmax = std::max(D.wmax, D.width_h[q]);
+  }
   int rc;
   if ((rc = upload(ctx, &D.soff, S.soff.data(), S.soff.size()))) return rc;
   if ((rc = upload(ctx, &D.rows, S.rows.data(), S.rows.size()))) return rc;
@@ -218,6 +235,21 @@ void drop_graph(ugmg_ctx *ctx) {
 inline Batch batch(const ugmg_ctx *ctx, int l) { return Batch{ctx->nb, ctx->dev[l].vs}; }
 inline Batch B1(const DevLevel &d) { return Batch{1, d.vs}; }
 
+// the caller's vectors (list order, column stride ld) to the level's device
+// order and back; a plain (converting) copy on levels kept in list order
+template <typename TD, typename TS>
+void to_dev(ugmg_ctx *ctx, int l, const TS *src, size_t lds, TD *dst, int nrhs = 1) {
+  const DevLevel &d = ctx->dev[l];
+  if (d.perm) launch_gather<TD, TS>(ctx->stream, d.n, d.perm, src, dst, nrhs, lds, d.vs);
+  else launch_convert<TD, TS>(ctx->stream, d.n, src, dst, nrhs, lds, d.vs);
+}
+template <typename TD, typename TS>
+void to_user(ugmg_ctx *ctx, int l, const TS *src, TD *dst, size_t ldd, int nrhs = 1) {
+  const DevLevel &d = ctx->dev[l];
+  if (d.perm) launch_scatter<TD, TS>(ctx->stream, d.n, d.perm, src, dst, nrhs, d.vs, ldd);
+  else launch_convert<TD, TS>(ctx->stream, d.n, src, dst, nrhs, d.vs, ldd);
+}
+
 // ------------------------------------------------------------ cycle drivers
 
 template <typename T>
@@ -233,8 +265,8 @@ void smooth_step(ugmg_ctx *ctx, int l, T *t, T *b, T *c, int cmode, T *xout) {
   } else {
     for (int k = 0; k < d.ncolors; ++k)
       launch_gs_color<T, T>(st, d.L.d<T>(), d.color_slices[k],
-                            d.color_slices[k + 1] - d.color_slices[k], b, d.diag<T>(),
-                            d.vclass, t, d.nt, bt);
+                            d.color_slices[k + 1] - d.color_slices[k], d.color_wmax[k], b,
+                            d.diag<T>(), d.vclass, t, d.nt, bt);
     launch_gs_defect<T, T>(st, d.U.d<T>(), t, b, d.vclass, damp, cmode < 0 ? nullptr : c,
                            cmode < 0 ? 1 : cmode, xout, d.nt, bt);
   }
@@ -550,6 +582,7 @@ struct PrepState {
   int level = -1, bl = 0, fuse_rows = 0;
   double nt_bytes = 64.0e6;
   std::vector<HostPrep> hp;
+  std::vector<HostLevel> plv;   // levels in device order (GS levels)
   HostBase base;
 };
 
@@ -579,11 +612,37 @@ int prep_host(ugmg_ctx *ctx, int level, PrepState &ps) {
   ps.fuse_rows = env_fuse ? std::atoi(env_fuse) : 0;
   ps.level = level;
   ps.bl = bl;
+  // UGMG_GS_WINDOW: window of the GS device order (unset: chosen per level;
+  // 1: list order)
+  const char *env_win = std::getenv("UGMG_GS_WINDOW");
+  const int force_win = env_win ? std::atoi(env_win) : -1;
   try {
     ps.hp.assign(level + 1, HostPrep());
+    std::vector<HostLevel> &plv = ps.plv;          // levels in device order
+    plv.assign(level + 1, HostLevel());
+    std::vector<std::vector<int>> pos(level + 1);   // list index -> device position
     for (int l = bl; l <= level; ++l) {
       const bool gs = p.smoother == UGMG_SMOOTH_GS && l > bl;
-      build_prep(ctx->host[l], l > bl ? &ctx->host[l - 1] : nullptr, gs, ps.hp[l]);
+      HostPrep &hp = ps.hp[l];
+      const bool cperm = l > bl && !pos[l - 1].empty();
+      const HostLevel *coarse = l > bl ? (cperm ? &plv[l - 1] : &ctx->host[l - 1]) : nullptr;
+      if (!gs) {
+        build_prep(ctx->host[l], coarse, nullptr, 0, hp);
+        continue;
+      }
+      const HostLevel &H = ctx->host[l];
+      std::vector<int> col;
+      const int nc = greedy_color(H, col);
+      const int win = force_win >= 0 ? force_win : choose_gs_window(H, col, nc);
+      hp.window = win;
+      gs_device_order(col, nc, win, hp.perm);
+      pos[l].resize(H.n);
+      for (int q = 0; q < H.n; ++q) pos[l][hp.perm[q]] = q;
+      permute_level(H, hp.perm, pos[l], cperm ? &pos[l - 1] : nullptr, plv[l]);
+      std::vector<int> colp(H.n);
+      for (int q = 0; q < H.n; ++q) colp[q] = col[hp.perm[q]];
+      build_prep(plv[l], coarse, &colp, nc, hp);
+      hp.color_ug = col;
     }
     build_base(ctx->host[bl], p.ex_reorder, ps.base);
   } catch (const std::bad_alloc &) {
@@ -602,9 +661,14 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
     int rc;
     int max_slices = 0;
     for (int l = bl; l <= level; ++l) {
-      const HostLevel &H = ctx->host[l];
-      DevLevel &D = ctx->dev[l];
       HostPrep &hp = ps.hp[l];
+      const bool permuted = !hp.perm.empty();
+      const HostLevel &H = permuted ? ps.plv[l] : ctx->host[l];
+      DevLevel &D = ctx->dev[l];
+      if (permuted) {
+        D.perm_h = hp.perm;
+        if ((rc = upload(ctx, &D.perm, hp.perm.data(), hp.perm.size()))) return rc;
+      }
       D.n = H.n;
       D.nnzb = H.nnzb;
       max_slices = std::max(max_slices, hp.A.nslices);
@@ -619,9 +683,12 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
       if (hp.ncolors > 0) {
         D.ncolors = hp.ncolors;
         D.color_slices = hp.color_slices;
-        D.color = hp.color;
+        D.color = hp.color_ug;
         if ((rc = upload_sell(ctx, hp.L, D.L))) return rc;
         if ((rc = upload_sell(ctx, hp.U, D.U))) return rc;
+        D.color_wmax.resize(D.ncolors);
+        for (int k = 0; k < D.ncolors; ++k)
+          D.color_wmax[k] = D.L.max_width(D.color_slices[k], D.color_slices[k + 1]);
       }
       if (l > bl) {
         D.p_nnz = H.pp[H.n];
@@ -923,20 +990,23 @@ int ugmg_get_pattern(ugmg_ctx *ctx, int level, int *row_ptr, int *col_idx, doubl
     HIPCHK(hipMemcpy(val.data(), d.A.v64, val.size() * sizeof(double), hipMemcpyDeviceToHost));
   }
   // explicit slices: the host copy's row lengths separate entries from slice
-  // padding; uniform slices: phantom entries carry column -1
+  // padding; uniform slices: phantom entries carry column -1.  Device rows
+  // and columns map back to list indices through the device order.
   const HostLevel &H = ctx->host[level];
+  const bool pm = !d.perm_h.empty();
   row_ptr[0] = 0;
   for (int s = 0; s < ns; ++s) {
     const int w = (soff[s + 1] - soff[s]) / kSlice;
     for (int lane = 0; lane < kSlice; ++lane) {
-      const int r = rows[(size_t)s * kSlice + lane];
-      if (r < 0) continue;
+      const int rd = rows[(size_t)s * kSlice + lane];
+      if (rd < 0) continue;
+      const int r = pm ? d.perm_h[rd] : rd;
       const int len = H.rp[r + 1] - H.rp[r];
       int q = 0;
       for (int k = 0; k < w && q < len; ++k) {
         const size_t e = (size_t)soff[s] + (size_t)k * kSlice + lane;
         if (sdoff[s] >= 0 && col[e] < 0) continue;
-        col_idx[H.rp[r] + q] = col[e];
+        col_idx[H.rp[r] + q] = pm ? d.perm_h[col[e]] : col[e];
         if (vals) vals[H.rp[r] + q] = val[e];
         ++q;
       }
@@ -947,11 +1017,21 @@ int ugmg_get_pattern(ugmg_ctx *ctx, int level, int *row_ptr, int *col_idx, doubl
   return UGMG_OK;
 }
 
+// On a level kept in device order the single-level entry points move their
+// list-order arguments through the level's own vectors (c, b, t), which are
+// scratch outside a cycle.
 int ugmg_dmatmul_minus(ugmg_ctx *ctx, int level, double *x, const double *y) {
   int rc = check_ready(ctx, level);
   if (rc) return rc;
   DevLevel &d = ctx->dev[level];
-  launch_spmv<double, double, double>(ctx->stream, d.A.d64(), 0, d.A.nslices, y, x, nullptr, 0, d.nt, B1(d));
+  if (!d.perm) {
+    launch_spmv<double, double, double>(ctx->stream, d.A.d64(), 0, d.A.nslices, y, x, nullptr, 0, d.nt, B1(d));
+  } else {
+    to_dev<double, double>(ctx, level, y, d.vs, d.t);
+    to_dev<double, double>(ctx, level, x, d.vs, d.b);
+    launch_spmv<double, double, double>(ctx->stream, d.A.d64(), 0, d.A.nslices, d.t, d.b, nullptr, 0, d.nt, B1(d));
+    to_user<double, double>(ctx, level, d.b, x, d.vs);
+  }
   HIPCHK(hipGetLastError());
   return UGMG_OK;
 }
@@ -960,7 +1040,13 @@ int ugmg_dmatmul(ugmg_ctx *ctx, int level, double *x, const double *y) {
   int rc = check_ready(ctx, level);
   if (rc) return rc;
   DevLevel &d = ctx->dev[level];
-  launch_spmv<double, double, double>(ctx->stream, d.A.d64(), 0, d.A.nslices, y, x, nullptr, 1, d.nt, B1(d));
+  if (!d.perm) {
+    launch_spmv<double, double, double>(ctx->stream, d.A.d64(), 0, d.A.nslices, y, x, nullptr, 1, d.nt, B1(d));
+  } else {
+    to_dev<double, double>(ctx, level, y, d.vs, d.t);
+    launch_spmv<double, double, double>(ctx->stream, d.A.d64(), 0, d.A.nslices, d.t, d.b, nullptr, 1, d.nt, B1(d));
+    to_user<double, double>(ctx, level, d.b, x, d.vs);
+  }
   HIPCHK(hipGetLastError());
   return UGMG_OK;
 }
@@ -969,7 +1055,14 @@ int ugmg_smatmul_minus(ugmg_ctx *ctx, int level, float *x, const float *y) {
   int rc = check_ready(ctx, level);
   if (rc) return rc;
   DevLevel &d = ctx->dev[level];
-  launch_spmv<float, float, float>(ctx->stream, d.A.d32(), 0, d.A.nslices, y, x, nullptr, 0, d.nt, B1(d));
+  if (!d.perm) {
+    launch_spmv<float, float, float>(ctx->stream, d.A.d32(), 0, d.A.nslices, y, x, nullptr, 0, d.nt, B1(d));
+  } else {
+    to_dev<float, float>(ctx, level, y, d.vs, d.t32);
+    to_dev<float, float>(ctx, level, x, d.vs, d.b32);
+    launch_spmv<float, float, float>(ctx->stream, d.A.d32(), 0, d.A.nslices, d.t32, d.b32, nullptr, 0, d.nt, B1(d));
+    to_user<float, float>(ctx, level, d.b32, x, d.vs);
+  }
   HIPCHK(hipGetLastError());
   return UGMG_OK;
 }
@@ -1002,8 +1095,14 @@ int ugmg_jacobi_step(ugmg_ctx *ctx, int level, double *x, double *b) {
   int rc = check_ready(ctx, level);
   if (rc) return rc;
   DevLevel &d = ctx->dev[level];
-  launch_jac<double>(ctx->stream, d.n, b, d.diag64, d.vclass, ctx->p.damp[0], x, B1(d));
-  launch_spmv<double, double, double>(ctx->stream, d.A.d64(), 0, d.A.nslices, x, b, nullptr, 0, d.nt, B1(d));
+  double *xd = d.perm ? d.t : x, *bd = d.perm ? d.b : b;
+  if (d.perm) to_dev<double, double>(ctx, level, b, d.vs, bd);
+  launch_jac<double>(ctx->stream, d.n, bd, d.diag64, d.vclass, ctx->p.damp[0], xd, B1(d));
+  launch_spmv<double, double, double>(ctx->stream, d.A.d64(), 0, d.A.nslices, xd, bd, nullptr, 0, d.nt, B1(d));
+  if (d.perm) {
+    to_user<double, double>(ctx, level, xd, x, d.vs);
+    to_user<double, double>(ctx, level, bd, b, d.vs);
+  }
   HIPCHK(hipGetLastError());
   return UGMG_OK;
 }
@@ -1013,10 +1112,16 @@ int ugmg_gs_step(ugmg_ctx *ctx, int level, double *x, double *b) {
   if (rc) return rc;
   DevLevel &d = ctx->dev[level];
   if (d.ncolors == 0) return fail(ctx, UGMG_E_STATE, "level was not preprocessed for GS");
+  double *xd = d.perm ? d.c : x, *bd = d.perm ? d.b : b;
+  if (d.perm) to_dev<double, double>(ctx, level, b, d.vs, bd);
   const int saved = ctx->p.smoother;
   ctx->p.smoother = UGMG_SMOOTH_GS;
-  smooth_step<double>(ctx, level, d.t, b, nullptr, -1, x);
+  smooth_step<double>(ctx, level, d.t, bd, nullptr, -1, xd);
   ctx->p.smoother = saved;
+  if (d.perm) {
+    to_user<double, double>(ctx, level, xd, x, d.vs);
+    to_user<double, double>(ctx, level, bd, b, d.vs);
+  }
   HIPCHK(hipGetLastError());
   return UGMG_OK;
 }
@@ -1027,8 +1132,19 @@ int ugmg_restrict(ugmg_ctx *ctx, int level, double *to_c, const double *from_f,
   if (rc) return rc;
   if (level <= ctx->bl) return fail(ctx, UGMG_E_ARG, "restrict needs a level above the base level");
   DevLevel &d = ctx->dev[level], &dc = ctx->dev[level - 1];
-  launch_restrict<double>(ctx->stream, d.PT.d64(), dc.skip, dc.vnclass, dc.vclass, from_f,
-                          damp ? damp[0] : 1.0, to_c, nullptr, d.nt, 1, d.vs, dc.vs);
+  const double *fd = from_f;
+  double *cd = to_c;
+  if (d.perm) {
+    to_dev<double, double>(ctx, level, from_f, d.vs, d.b);
+    fd = d.b;
+  }
+  if (dc.perm) {
+    to_dev<double, double>(ctx, level - 1, to_c, dc.vs, dc.b);
+    cd = dc.b;
+  }
+  launch_restrict<double>(ctx->stream, d.PT.d64(), dc.skip, dc.vnclass, dc.vclass, fd,
+                          damp ? damp[0] : 1.0, cd, nullptr, d.nt, 1, d.vs, dc.vs);
+  if (dc.perm) to_user<double, double>(ctx, level - 1, cd, to_c, dc.vs);
   HIPCHK(hipGetLastError());
   return UGMG_OK;
 }
@@ -1038,9 +1154,16 @@ int ugmg_interpolate(ugmg_ctx *ctx, int level, double *to_f, const double *from_
   int rc = check_ready(ctx, level);
   if (rc) return rc;
   if (level <= ctx->bl) return fail(ctx, UGMG_E_ARG, "interpolate needs a level above the base level");
-  DevLevel &d = ctx->dev[level];
-  launch_interp<double>(ctx->stream, d.P.d64(), d.skip, from_c, damp ? damp[0] : 1.0, to_f,
-                        nullptr, d.nt, 1, d.vs, ctx->dev[level - 1].vs);
+  DevLevel &d = ctx->dev[level], &dc = ctx->dev[level - 1];
+  const double *cd = from_c;
+  double *fd = d.perm ? d.t : to_f;
+  if (dc.perm) {
+    to_dev<double, double>(ctx, level - 1, from_c, dc.vs, dc.c);
+    cd = dc.c;
+  }
+  launch_interp<double>(ctx->stream, d.P.d64(), d.skip, cd, damp ? damp[0] : 1.0, fd,
+                        nullptr, d.nt, 1, d.vs, dc.vs);
+  if (d.perm) to_user<double, double>(ctx, level, fd, to_f, d.vs);
   HIPCHK(hipGetLastError());
   return UGMG_OK;
 }
@@ -1060,13 +1183,26 @@ int ugmg_iter(ugmg_ctx *ctx, int level, double *c, double *b) {
   if (ctx->base.failed) return fail(ctx, UGMG_E_SINGULAR, "zero pivot in the band LU of the base level");
   DevLevel &d = ctx->dev[level];
   if (ctx->p.precision == UGMG_MIXED) {
-    launch_convert<float, double>(ctx->stream, d.n, b, d.b32, 1, d.vs, d.vs);
+    to_dev<float, double>(ctx, level, b, d.vs, d.b32);
     const int k = std::max(1, ctx->p.inner_cycles);
     for (int j = 0; j < k; ++j) lmgc<float>(ctx, level, d.c32, d.b32, j == 0 ? 0 : 1);
-    launch_convert<double, float>(ctx->stream, d.n, d.c32, c, 1, d.vs, d.vs);
-    launch_spmv<double, double, double>(ctx->stream, d.A.d64(), 0, d.A.nslices, c, b, nullptr, 0, d.nt, B1(d));
-  } else {
+    if (!d.perm) {
+      launch_convert<double, float>(ctx->stream, d.n, d.c32, c, 1, d.vs, d.vs);
+      launch_spmv<double, double, double>(ctx->stream, d.A.d64(), 0, d.A.nslices, c, b, nullptr, 0, d.nt, B1(d));
+    } else {
+      launch_convert<double, float>(ctx->stream, d.n, d.c32, d.c, 1, d.vs, d.vs);
+      to_dev<double, double>(ctx, level, b, d.vs, d.b);
+      launch_spmv<double, double, double>(ctx->stream, d.A.d64(), 0, d.A.nslices, d.c, d.b, nullptr, 0, d.nt, B1(d));
+      to_user<double, double>(ctx, level, d.c, c, d.vs);
+      to_user<double, double>(ctx, level, d.b, b, d.vs);
+    }
+  } else if (!d.perm) {
     lmgc<double>(ctx, level, c, b, 0);
+  } else {
+    to_dev<double, double>(ctx, level, b, d.vs, d.b);
+    lmgc<double>(ctx, level, d.c, d.b, 0);
+    to_user<double, double>(ctx, level, d.c, c, d.vs);
+    to_user<double, double>(ctx, level, d.b, b, d.vs);
   }
   HIPCHK(hipGetLastError());
   return UGMG_OK;
@@ -1107,6 +1243,22 @@ int ugmg_solve(ugmg_ctx *ctx, int level, double *x, double *b, const double *abs
     return fail(ctx, UGMG_E_SINGULAR, "zero pivot in the band LU of the base level");
   }
   DevLevel &d = ctx->dev[level];
+  // a level in device order iterates on its own c (x) and b, moved in and
+  // out once per solve
+  double *const xu = x, *const bu = b;
+  if (d.perm) {
+    to_dev<double, double>(ctx, level, xu, d.vs, d.c);
+    to_dev<double, double>(ctx, level, bu, d.vs, d.b);
+    x = d.c;
+    b = d.b;
+  }
+  auto move_out = [&]() -> int {
+    if (!d.perm) return 0;
+    to_user<double, double>(ctx, level, x, xu, d.vs);
+    to_user<double, double>(ctx, level, b, bu, d.vs);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return 0;
+  };
   if (do_defect)  // LinearDefect: b -= A x
     launch_spmv<double, double, double>(ctx->stream, d.A.d64(), 0, d.A.nslices, x, b, nullptr, 0, d.nt, B1(d));
   double first = 0.0;
@@ -1122,6 +1274,7 @@ int ugmg_solve(ugmg_ctx *ctx, int level, double *x, double *b, const double *abs
     if (conv) break;
     if ((rc = run_iteration(ctx, level, x, b))) {
       res->error_code = rc;
+      move_out();
       return rc;
     }
     const double last = ctx->red_host[0];
@@ -1135,7 +1288,7 @@ int ugmg_solve(ugmg_ctx *ctx, int level, double *x, double *b, const double *abs
   }
   if (!conv) res->number_of_linear_iterations = i;
   res->converged = conv;
-  return UGMG_OK;
+  return move_out();
 }
 
 int ugmg_solve_host(ugmg_ctx *ctx, int level, double *x, double *b, const double *abslimit,
@@ -1187,19 +1340,18 @@ int ugmg_iter_batch(ugmg_ctx *ctx, int level, int nrhs, double *c, double *b, in
   ctx->nb = nrhs;
   const Batch bt = batch(ctx, level);
   if (ctx->p.precision == UGMG_MIXED) {
-    launch_convert<float, double>(st, d.n, b, d.b32, nrhs, ld, d.vs);
+    to_dev<float, double>(ctx, level, b, ld, d.b32, nrhs);
     const int k = std::max(1, ctx->p.inner_cycles);
     for (int j = 0; j < k; ++j) lmgc<float>(ctx, level, d.c32, d.b32, j == 0 ? 0 : 1);
-    launch_convert<double, float>(st, d.n, d.c32, c, nrhs, d.vs, ld);
-    launch_spmv<double, double, double>(st, d.A.d64(), 0, d.A.nslices, c, b, nullptr, 0, d.nt,
-                                        Batch{nrhs, (size_t)ld});
+    launch_convert<double, float>(st, d.n, d.c32, d.c, nrhs, d.vs, d.vs);
+    to_dev<double, double>(ctx, level, b, ld, d.b, nrhs);
+    launch_spmv<double, double, double>(st, d.A.d64(), 0, d.A.nslices, d.c, d.b, nullptr, 0, d.nt, bt);
   } else {
-    launch_convert<double, double>(st, d.n, b, d.b, nrhs, ld, d.vs);
+    to_dev<double, double>(ctx, level, b, ld, d.b, nrhs);
     lmgc<double>(ctx, level, d.c, d.b, 0);
-    launch_convert<double, double>(st, d.n, d.c, c, nrhs, d.vs, ld);
-    launch_convert<double, double>(st, d.n, d.b, b, nrhs, d.vs, ld);
   }
-  (void)bt;
+  to_user<double, double>(ctx, level, d.c, c, ld, nrhs);
+  to_user<double, double>(ctx, level, d.b, b, ld, nrhs);
   ctx->nb = 1;
   HIPCHK(hipGetLastError());
   return UGMG_OK;
@@ -1227,8 +1379,8 @@ int ugmg_bcgs_solve(ugmg_ctx *ctx, int level, int nrhs, double *x, double *b, in
   hipStream_t st = ctx->stream;
   ctx->nb = nrhs;
   const Batch bt = batch(ctx, level);
-  launch_convert<double, double>(st, d.n, x, w.x, nrhs, ld, d.vs);
-  launch_convert<double, double>(st, d.n, b, w.b, nrhs, ld, d.vs);
+  to_dev<double, double>(ctx, level, x, ld, w.x, nrhs);
+  to_dev<double, double>(ctx, level, b, ld, w.b, nrhs);
   if (do_defect)  // BCGSDefect: b -= A x
     launch_spmv<double, double, double>(st, d.A.d64(), 0, d.A.nslices, w.x, w.b, nullptr, 0, d.nt, bt);
   // first defect (the solver reads LinearResiduum's last_defect)
@@ -1258,8 +1410,8 @@ int ugmg_bcgs_solve(ugmg_ctx *ctx, int level, int nrhs, double *x, double *b, in
   }
   ctx->nb = 1;
   if (rc) return rc;
-  launch_convert<double, double>(st, d.n, w.x, x, nrhs, d.vs, ld);
-  launch_convert<double, double>(st, d.n, w.b, b, nrhs, d.vs, ld);
+  to_user<double, double>(ctx, level, w.x, x, ld, nrhs);
+  to_user<double, double>(ctx, level, w.b, b, ld, nrhs);
   HIPCHK(hipGetLastError());
   if (defects)
     HIPCHK(hipMemcpyAsync(defects, w.defects, sizeof(double) * (size_t)md * nrhs, hipMemcpyDeviceToHost, st));
@@ -1546,11 +1698,16 @@ void srv_finish(ugmg_server *s, SrvReq *r, int state) {
   ugmg_ctx *ctx = r->ctx;
   if (ctx && ctx->stream) hipStreamSynchronize(ctx->stream);
   if (state == UGMG_REQ_DONE && ctx && r->x) {
-    r->xout.resize(ctx->dev[r->top].n);
+    const DevLevel &d = ctx->dev[r->top];
+    r->xout.resize(d.n);
     if (hipMemcpy(r->xout.data(), r->x, sizeof(double) * r->xout.size(), hipMemcpyDeviceToHost) !=
         hipSuccess) {
       state = UGMG_REQ_FAILED;
       r->err = "copy of the solution failed";
+    } else if (!d.perm_h.empty()) {   // device order -> list order
+      std::vector<double> xl(d.n);
+      for (int q = 0; q < d.n; ++q) xl[d.perm_h[q]] = r->xout[q];
+      r->xout.swap(xl);
     }
   }
   if (r->ev) hipEventDestroy(r->ev);
@@ -1584,6 +1741,11 @@ int srv_admit(ugmg_server *s, SrvReq *r) {
   }
   r->charged = ctx->dev_bytes;
   s->used += r->charged;
+  if (!d.perm_h.empty()) {   // list order -> device order
+    std::vector<double> rp(d.n);
+    for (int q = 0; q < d.n; ++q) rp[q] = r->rhs[d.perm_h[q]];
+    r->rhs.swap(rp);
+  }
   if (hipMemcpyAsync(r->b, r->rhs.data(), sizeof(double) * d.n, hipMemcpyHostToDevice,
                      ctx->stream) != hipSuccess ||
       hipEventCreateWithFlags(&r->ev, hipEventDisableTiming) != hipSuccess)

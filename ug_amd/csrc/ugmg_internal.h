@@ -42,6 +42,11 @@ struct HostSell {
 
 // Host-side products of LmgcPreProcess for one level.
 struct HostPrep {
+  // device order of the level's vectors: perm[p] = list index of device
+  // position p (empty: list order); color in device order
+  std::vector<int> perm;
+  int window = 0;
+  std::vector<int> color_ug;   // colouring in list order
   HostSell A;                // all rows, list order
   std::vector<double> diag;  // diagonal entry of every row
   // multicolour Gauss-Seidel
@@ -74,8 +79,24 @@ void build_sell(const HostLevel &L, const std::vector<int> *rows, Part part,
 void build_sell_csr(int n, const int *rp, const int *ci, const double *a, HostSell &out);
 int greedy_color(const HostLevel &L, std::vector<int> &color);
 void to_float(const std::vector<double> &in, std::vector<float> &out);
-void build_prep(const HostLevel &L, const HostLevel *coarse, bool want_gs,
-                HostPrep &out);
+// L (and its transfer) in device order: row p of out is row perm[p] of L,
+// columns through pos; interpolation columns through cpos (coarse list
+// index -> coarse device position; null: identity)
+void permute_level(const HostLevel &L, const std::vector<int> &perm, const std::vector<int> &pos,
+                   const std::vector<int> *cpos, HostLevel &out);
+// device order of a multicolour-GS level: list order stably regrouped by
+// colour inside windows of `window` consecutive vectors (one grid line of a
+// lexicographic lattice), so each colour's vectors are contiguous runs and
+// the column offsets of a stencil stay uniform along a run.
+void gs_device_order(const std::vector<int> &color, int ncolors, int window,
+                     std::vector<int> &perm);
+// window for gs_device_order: the candidate (grid-line lengths guessed from
+// the stencil offsets, or the whole level) whose colour runs give the most
+// uniform-offset slices on a sample
+int choose_gs_window(const HostLevel &L, const std::vector<int> &color, int ncolors);
+// color: the level's colouring in the row order of L (null: Jacobi level)
+void build_prep(const HostLevel &L, const HostLevel *coarse, const std::vector<int> *color,
+                int ncolors, HostPrep &out);
 void build_base(const HostLevel &L, int reorder, HostBase &out);
 
 }  // namespace ugmg

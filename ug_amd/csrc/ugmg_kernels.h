@@ -23,6 +23,7 @@ struct DSell {
   const TV *val;
   const int *sdoff;   // per slice: offset into dtab (uniform slice) or -1
   const int *dtab;    // shared column-offset lists of uniform slices
+  int wmax;           // widest slice (entries per row)
 };
 
 // base-level solver settings: ls($m,$red,$abslimit) over ex($damp)
@@ -100,7 +101,7 @@ template <typename T>
 void launch_jac(hipStream_t st, int n, const T *b, const T *diag, const uint8_t *vclass, T damp,
                 T *t, Batch bt);
 template <typename TV, typename T>
-void launch_gs_color(hipStream_t st, const DSell<TV> &L, int slice0, int nsl, const T *b,
+void launch_gs_color(hipStream_t st, const DSell<TV> &L, int slice0, int nsl, int wmax, const T *b,
                      const T *diag, const uint8_t *vclass, T *v, bool nt, Batch bt);
 template <typename TV, typename T>
 void launch_gs_defect(hipStream_t st, const DSell<TV> &U, const T *v, T *b, const uint8_t *vclass,
@@ -134,6 +135,14 @@ template <typename TD, typename TS>
 void launch_convert(hipStream_t st, int n, const TS *s, TD *d, int nrhs, size_t vss, size_t vsd);
 template <typename T>
 void launch_add(hipStream_t st, int n, T *x, const T *y, Batch bt);
+// list order <-> device order of a level: gather d[p] = s[perm[p]],
+// scatter d[perm[p]] = s[p]; column strides vss (source), vsd (destination)
+template <typename TD, typename TS>
+void launch_gather(hipStream_t st, int n, const int *perm, const TS *s, TD *d, int nrhs,
+                   size_t vss, size_t vsd);
+template <typename TD, typename TS>
+void launch_scatter(hipStream_t st, int n, const int *perm, const TS *s, TD *d, int nrhs,
+                    size_t vss, size_t vsd);
 template <typename T>
 void launch_coarse_cycle(hipStream_t st, const CoarseArgs<T> &args, int base_n);
 

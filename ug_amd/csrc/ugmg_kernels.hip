@@ -36,46 +36,50 @@ __device__ __forceinline__ T ld(const T *p) {
   else return *p;
 }
 
-// sum_{k<w} val[64 k] * x[col_k], entry order, 8 entries in flight per lane.
+// sum_{k<w} val[64 k] * x[col_k] in entry order, in chunks of CH entries:
+// the loads of a chunk are clamped to the row width (no branch, so they all
+// issue before the first product) and a row of at most CH entries costs one
+// round of loads (two for explicit columns).  CH is chosen per launch from
+// the widest slice.
 // Uniform slice (sdo >= 0): col_k = rb + dtab[sdo + k] (wave-uniform offsets
 // in scalar registers), so the x gathers issue together with the value loads;
 // explicit slice: col_k = ci[64 k].  TA is the accumulation type.
-template <int NT, typename TV, typename TX, typename TA>
+template <int CH, int NT, typename TV, typename TX, typename TA>
 __device__ __forceinline__ TA sell_dot(const TV *__restrict__ v, const int *__restrict__ ci,
                                        int w, const TX *__restrict__ x, int sdo,
                                        const int *__restrict__ dtab, int rb) {
   TA sum = 0;
   if (sdo >= 0) {
     const int *__restrict__ dt = dtab + sdo;
-    for (int k = 0; k < w; k += 8) {
-      TV a[8];
-      TX xv[8];
+    for (int k = 0; k < w; k += CH) {
+      TV a[CH];
+      TX xv[CH];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
+      for (int j = 0; j < CH; ++j) {
         const int kk = (k + j < w) ? k + j : w - 1;
         a[j] = ld<NT>(v + 64 * kk);
         xv[j] = x[rb + dt[kk]];
       }
 #pragma unroll
-      for (int j = 0; j < 8; ++j)
+      for (int j = 0; j < CH; ++j)
         if (k + j < w) sum += (TA)a[j] * (TA)xv[j];
     }
     return sum;
   }
-  for (int k = 0; k < w; k += 8) {
-    TV a[8];
-    int c[8];
+  for (int k = 0; k < w; k += CH) {
+    TV a[CH];
+    int c[CH];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
+    for (int j = 0; j < CH; ++j) {
       const int kk = (k + j < w) ? k + j : w - 1;
       a[j] = ld<NT>(v + 64 * kk);
       c[j] = ld<NT>(ci + 64 * kk);
     }
-    TX xv[8];
+    TX xv[CH];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) xv[j] = x[c[j]];
+    for (int j = 0; j < CH; ++j) xv[j] = x[c[j]];
 #pragma unroll
-    for (int j = 0; j < 8; ++j)
+    for (int j = 0; j < CH; ++j)
       if (k + j < w) sum += (TA)a[j] * (TA)xv[j];
   }
   return sum;
@@ -139,7 +143,7 @@ __device__ __forceinline__ double block_sum(double v, double *red) {
 
 // ------------------------------------------------------------------- SpMV
 // MODE: 0 y -= A x, 1 y = A x, 2 y -= A x and c += x, 3 y -= A x and c = x
-template <typename TV, typename TX, typename TY, int MODE, int NT>
+template <int MW, typename TV, typename TX, typename TY, int MODE, int NT>
 __global__ __launch_bounds__(kBlock) void k_spmv(DSell<TV> A, int slice0, int nsl,
                                                   const TX *__restrict__ x, TY *__restrict__ y,
                                                   TY *__restrict__ c, size_t vs) {
@@ -147,7 +151,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(DSell<TV> A, int slice0, int ns
   const size_t o = (size_t)col_ * vs;
   x += o;
   y += o;
-  const TY sum = sell_dot<NT, TV, TX, TY>(A.val + e0 + lane, A.col + e0 + lane, w, x, sdo, A.dtab, rb);
+  const TY sum = sell_dot<MW, NT, TV, TX, TY>(A.val + e0 + lane, A.col + e0 + lane, w, x, sdo, A.dtab, rb);
   if (r < 0) return;
   if (MODE == 1) {
     y[r] = sum;
@@ -161,7 +165,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(DSell<TV> A, int slice0, int ns
 // FP64 defect of a mixed-precision refinement step:
 //   r -= A c32 (FP64 values and sums); x += c32; r32 = (float) r;
 //   per-block partial sums of r^2 over NEW_DEFECT rows.
-template <int NT>
+template <int MW, int NT>
 __global__ __launch_bounds__(kBlock) void k_ir_update(DSell<double> A, int nsl,
                                                        const float *__restrict__ c32,
                                                        double *__restrict__ rv,
@@ -175,7 +179,7 @@ __global__ __launch_bounds__(kBlock) void k_ir_update(DSell<double> A, int nsl,
   double sq = 0.0;
   if (sok) {
     const double sum =
-        sell_dot<NT, double, float, double>(A.val + e0 + lane, A.col + e0 + lane, w, c32 + o, sdo, A.dtab, rb);
+        sell_dot<MW, NT, double, float, double>(A.val + e0 + lane, A.col + e0 + lane, w, c32 + o, sdo, A.dtab, rb);
     if (r >= 0) {
       const double nr = rv[o + r] - sum;
       rv[o + r] = nr;
@@ -203,7 +207,7 @@ __global__ __launch_bounds__(kBlock) void k_jac(int n, const T *__restrict__ b,
 
 // ------------------------------------------------------ multicolour GS
 // forward substitution of one colour: v_r = (b_r - sum_{lower} a v) / a_rr
-template <typename TV, typename T, int NT>
+template <int MW, typename TV, typename T, int NT>
 __global__ __launch_bounds__(kBlock) void k_gs_color(DSell<TV> L, int slice0, int nsl,
                                                       const T *__restrict__ b,
                                                       const T *__restrict__ diag,
@@ -212,16 +216,24 @@ __global__ __launch_bounds__(kBlock) void k_gs_color(DSell<TV> L, int slice0, in
   SLICE_PROLOGUE(L, slice0, nsl)
   const size_t o = (size_t)col_ * vs;
   v += o;
-  const T sum = sell_dot<NT, TV, T, T>(L.val + e0 + lane, L.col + e0 + lane, w, v, sdo, L.dtab, rb);
+  // the row's own data load alongside the entries
+  T br = 0, dr = 1;
+  int act = 0;
+  if (r >= 0) {
+    br = b[o + r];
+    dr = diag[r];
+    act = vclass[r] >= 3;
+  }
+  const T sum = sell_dot<MW, NT, TV, T, T>(L.val + e0 + lane, L.col + e0 + lane, w, v, sdo, L.dtab, rb);
   if (r < 0) return;
-  v[r] = vclass[r] < 3 ? (T)0 : (b[o + r] - sum) / diag[r];
+  v[r] = act ? (br - sum) / dr : (T)0;
 }
 
 // defect after the sweep, b -= A (damp v), with (D+L) v = b on active rows:
 //   active   b <- b - damp (b + U v)
 //   inactive b <- b - damp (A v)      (v_r = 0; U holds the whole row)
 // and the correction: CMODE 0 c = damp v, 1 c += damp v; xout = damp v.
-template <typename TV, typename T, int CMODE, int NT>
+template <int MW, typename TV, typename T, int CMODE, int NT>
 __global__ __launch_bounds__(kBlock) void k_gs_defect(DSell<TV> U, int nsl,
                                                        const T *__restrict__ v,
                                                        T *__restrict__ b,
@@ -231,14 +243,21 @@ __global__ __launch_bounds__(kBlock) void k_gs_defect(DSell<TV> U, int nsl,
   SLICE_PROLOGUE(U, 0, nsl)
   const size_t o = (size_t)col_ * vs;
   v += o;
-  const T sum = sell_dot<NT, TV, T, T>(U.val + e0 + lane, U.col + e0 + lane, w, v, sdo, U.dtab, rb);
+  T br = 0, vr = 0, cr = 0;
+  int act = 0;
+  if (r >= 0) {
+    br = b[o + r];
+    vr = v[r];
+    act = vclass[r] >= 3;
+    if (CMODE == 1 && c) cr = c[o + r];
+  }
+  const T sum = sell_dot<MW, NT, TV, T, T>(U.val + e0 + lane, U.col + e0 + lane, w, v, sdo, U.dtab, rb);
   if (r < 0) return;
-  const T br = b[o + r];
-  b[o + r] = vclass[r] >= 3 ? br - damp * (br + sum) : br - damp * sum;
-  const T t = damp * v[r];
+  b[o + r] = act ? br - damp * (br + sum) : br - damp * sum;
+  const T t = damp * vr;
   if (c) {
     if (CMODE == 0) c[o + r] = t;
-    else c[o + r] += t;
+    else c[o + r] = cr + t;
   }
   if (xout) xout[o + r] = t;
 }
@@ -248,7 +267,7 @@ __global__ __launch_bounds__(kBlock) void k_gs_defect(DSell<TV> U, int nsl,
 //   bc_I = (VNCLASS_I >= 2 ? 0 : bc_I) + (Dirichlet_I ? 0 : damp sum w b_f)
 // With `part`, also the per-block partial sums of the new bc^2 over NEW_DEFECT
 // vectors (the base solver's first LinearResiduum).
-template <typename T, int NT>
+template <int MW, typename T, int NT>
 __global__ __launch_bounds__(kBlock) void k_restrict(DSell<T> PT, int nsl,
                                                       const uint32_t *__restrict__ cskip,
                                                       const uint8_t *__restrict__ cvnclass,
@@ -263,7 +282,7 @@ __global__ __launch_bounds__(kBlock) void k_restrict(DSell<T> PT, int nsl,
   bc += (size_t)col_ * vsc;
   double sq = 0.0;
   if (sok) {
-    const T sum = sell_dot<NT, T, T, T>(PT.val + e0 + lane, PT.col + e0 + lane, w, bf, sdo, PT.dtab, rb);
+    const T sum = sell_dot<MW, NT, T, T, T>(PT.val + e0 + lane, PT.col + e0 + lane, w, bf, sdo, PT.dtab, rb);
     if (r >= 0) {
       const T base = cvnclass[r] >= 2 ? (T)0 : bc[r];
       const T v = (cskip[r] & 1u) ? base : base + damp * sum;
@@ -277,7 +296,7 @@ __global__ __launch_bounds__(kBlock) void k_restrict(DSell<T> PT, int nsl,
 }
 
 // StandardIntCorNodeVector (+ dadd): t = damp P c_c on non-Dirichlet rows
-template <typename T, int CADD, int NT>
+template <int MW, typename T, int CADD, int NT>
 __global__ __launch_bounds__(kBlock) void k_interp(DSell<T> P, int nsl,
                                                     const uint32_t *__restrict__ fskip,
                                                     const T *__restrict__ cc, T damp,
@@ -286,7 +305,7 @@ __global__ __launch_bounds__(kBlock) void k_interp(DSell<T> P, int nsl,
   SLICE_PROLOGUE(P, 0, nsl)
   cc += (size_t)col_ * vsc;
   const size_t o = (size_t)col_ * vsf;
-  const T sum = sell_dot<NT, T, T, T>(P.val + e0 + lane, P.col + e0 + lane, w, cc, sdo, P.dtab, rb);
+  const T sum = sell_dot<MW, NT, T, T, T>(P.val + e0 + lane, P.col + e0 + lane, w, cc, sdo, P.dtab, rb);
   if (r < 0) return;
   const T tv = (fskip[r] & 1u) ? (T)0 : damp * sum;
   t[o + r] = tv;
@@ -375,7 +394,7 @@ __device__ __forceinline__ void wg_sell(const DSell<TV> &S, int s0, int s1,
     const int e0 = S.soff[s], w = (S.soff[s + 1] - e0) >> 6;
     const int sdo = S.sdoff[s];
     const int rb = r >= 0 ? r : S.rows[s * kSlice];
-    const TX sum = sell_dot<0, TV, TX, TX>(S.val + e0 + lane, S.col + e0 + lane, w, x, sdo, S.dtab, rb);
+    const TX sum = sell_dot<8, 0, TV, TX, TX>(S.val + e0 + lane, S.col + e0 + lane, w, x, sdo, S.dtab, rb);
     if (r >= 0) epi(r, sum);
   }
   __syncthreads();
@@ -524,6 +543,21 @@ static inline int nblk(int n) { return (n + kBlock - 1) / kBlock; }
 static inline int nblk_slices(int nsl) { return (nsl + kBlock / kSlice - 1) / (kBlock / kSlice); }
 static inline dim3 grid2(int gx, int nrhs) { return dim3(gx, nrhs < 1 ? 1 : nrhs); }
 
+// widest row of a launch -> chunk MW (one round of loads up to 28 entries)
+#define W_DISPATCH(wmax, ...)     \
+  do {                            \
+    if ((wmax) <= 8 || (wmax) > 28) { \
+      constexpr int MWV = 8;      \
+      __VA_ARGS__;                \
+    } else if ((wmax) <= 16) {    \
+      constexpr int MWV = 16;     \
+      __VA_ARGS__;                \
+    } else {                      \
+      constexpr int MWV = 28;     \
+      __VA_ARGS__;                \
+    }                             \
+  } while (0)
+
 #define NT_DISPATCH(nt, ...) \
   do {                       \
     if (nt) {                \
@@ -540,12 +574,12 @@ void launch_spmv(hipStream_t st, const DSell<TV> &A, int slice0, int nsl, const 
                  TY *c, int mode, bool nt, Batch bt) {
   if (nsl <= 0) return;
   const dim3 g = grid2(nblk_slices(nsl), bt.nrhs), blk(kBlock);
-  NT_DISPATCH(nt, switch (mode) {
-    case 0: k_spmv<TV, TX, TY, 0, NTV><<<g, blk, 0, st>>>(A, slice0, nsl, x, y, c, bt.vs); break;
-    case 1: k_spmv<TV, TX, TY, 1, NTV><<<g, blk, 0, st>>>(A, slice0, nsl, x, y, c, bt.vs); break;
-    case 2: k_spmv<TV, TX, TY, 2, NTV><<<g, blk, 0, st>>>(A, slice0, nsl, x, y, c, bt.vs); break;
-    default: k_spmv<TV, TX, TY, 3, NTV><<<g, blk, 0, st>>>(A, slice0, nsl, x, y, c, bt.vs); break;
-  });
+  W_DISPATCH(A.wmax, NT_DISPATCH(nt, switch (mode) {
+    case 0: k_spmv<MWV, TV, TX, TY, 0, NTV><<<g, blk, 0, st>>>(A, slice0, nsl, x, y, c, bt.vs); break;
+    case 1: k_spmv<MWV, TV, TX, TY, 1, NTV><<<g, blk, 0, st>>>(A, slice0, nsl, x, y, c, bt.vs); break;
+    case 2: k_spmv<MWV, TV, TX, TY, 2, NTV><<<g, blk, 0, st>>>(A, slice0, nsl, x, y, c, bt.vs); break;
+    default: k_spmv<MWV, TV, TX, TY, 3, NTV><<<g, blk, 0, st>>>(A, slice0, nsl, x, y, c, bt.vs); break;
+  }));
 }
 template void launch_spmv<double, double, double>(hipStream_t, const DSell<double> &, int, int,
                                                   const double *, double *, double *, int, bool,
@@ -559,8 +593,8 @@ void launch_ir_update(hipStream_t st, const DSell<double> &A, const float *c32, 
   const int g = nblk_slices(A.nslices);
   *nparts = g;
   if (g == 0) return;
-  NT_DISPATCH(nt, k_ir_update<NTV><<<grid2(g, bt.nrhs), kBlock, 0, st>>>(A, A.nslices, c32, r, x,
-                                                                         r32, vclass, part, bt.vs));
+  W_DISPATCH(A.wmax, NT_DISPATCH(nt, k_ir_update<MWV, NTV><<<grid2(g, bt.nrhs), kBlock, 0, st>>>(
+                                         A, A.nslices, c32, r, x, r32, vclass, part, bt.vs)));
 }
 
 template <typename T>
@@ -575,16 +609,19 @@ template void launch_jac<float>(hipStream_t, int, const float *, const float *, 
                                 float, float *, Batch);
 
 template <typename TV, typename T>
-void launch_gs_color(hipStream_t st, const DSell<TV> &L, int slice0, int nsl, const T *b,
+void launch_gs_color(hipStream_t st, const DSell<TV> &L, int slice0, int nsl, int wmax, const T *b,
                      const T *diag, const uint8_t *vclass, T *v, bool nt, Batch bt) {
   if (nsl <= 0) return;
-  NT_DISPATCH(nt, k_gs_color<TV, T, NTV><<<grid2(nblk_slices(nsl), bt.nrhs), kBlock, 0, st>>>(
+  // colour sweeps stream in chunks of 8 entries whatever the width: their
+  // time follows the entries read, and a wider chunk only adds clamped loads
+  (void)wmax;
+  NT_DISPATCH(nt, k_gs_color<8, TV, T, NTV><<<grid2(nblk_slices(nsl), bt.nrhs), kBlock, 0, st>>>(
                       L, slice0, nsl, b, diag, vclass, v, bt.vs));
 }
-template void launch_gs_color<double, double>(hipStream_t, const DSell<double> &, int, int,
+template void launch_gs_color<double, double>(hipStream_t, const DSell<double> &, int, int, int,
                                               const double *, const double *, const uint8_t *,
                                               double *, bool, Batch);
-template void launch_gs_color<float, float>(hipStream_t, const DSell<float> &, int, int,
+template void launch_gs_color<float, float>(hipStream_t, const DSell<float> &, int, int, int,
                                             const float *, const float *, const uint8_t *,
                                             float *, bool, Batch);
 
@@ -593,10 +630,10 @@ void launch_gs_defect(hipStream_t st, const DSell<TV> &U, const T *v, T *b, cons
                       T damp, T *c, int cmode, T *xout, bool nt, Batch bt) {
   if (U.nslices <= 0) return;
   const dim3 g = grid2(nblk_slices(U.nslices), bt.nrhs);
-  NT_DISPATCH(nt, if (cmode == 0) k_gs_defect<TV, T, 0, NTV><<<g, kBlock, 0, st>>>(
-                      U, U.nslices, v, b, vclass, damp, c, xout, bt.vs);
-              else k_gs_defect<TV, T, 1, NTV><<<g, kBlock, 0, st>>>(U, U.nslices, v, b, vclass,
-                                                                    damp, c, xout, bt.vs));
+  W_DISPATCH(U.wmax, NT_DISPATCH(nt, if (cmode == 0) k_gs_defect<MWV, TV, T, 0, NTV><<<g, kBlock, 0, st>>>(
+                                         U, U.nslices, v, b, vclass, damp, c, xout, bt.vs);
+                                 else k_gs_defect<MWV, TV, T, 1, NTV><<<g, kBlock, 0, st>>>(
+                                         U, U.nslices, v, b, vclass, damp, c, xout, bt.vs)));
 }
 template void launch_gs_defect<double, double>(hipStream_t, const DSell<double> &,
                                                const double *, double *, const uint8_t *, double,
@@ -611,8 +648,9 @@ int launch_restrict(hipStream_t st, const DSell<T> &PT, const uint32_t *cskip,
                     double *part, bool nt, int nrhs, size_t vsf, size_t vsc) {
   if (PT.nslices <= 0) return 0;
   const int g = nblk_slices(PT.nslices);
-  NT_DISPATCH(nt, k_restrict<T, NTV><<<grid2(g, nrhs), kBlock, 0, st>>>(
-                      PT, PT.nslices, cskip, cvnclass, cvclass, bf, damp, bc, part, vsf, vsc));
+  W_DISPATCH(PT.wmax, NT_DISPATCH(nt, k_restrict<MWV, T, NTV><<<grid2(g, nrhs), kBlock, 0, st>>>(
+                                          PT, PT.nslices, cskip, cvnclass, cvclass, bf, damp, bc,
+                                          part, vsf, vsc)));
   return g;
 }
 template int launch_restrict<double>(hipStream_t, const DSell<double> &, const uint32_t *,
@@ -697,10 +735,10 @@ void launch_interp(hipStream_t st, const DSell<T> &P, const uint32_t *fskip, con
                    T *t, T *c, bool nt, int nrhs, size_t vsf, size_t vsc) {
   if (P.nslices <= 0) return;
   const dim3 g = grid2(nblk_slices(P.nslices), nrhs);
-  NT_DISPATCH(nt, if (c) k_interp<T, 1, NTV><<<g, kBlock, 0, st>>>(P, P.nslices, fskip, cc,
-                                                                   damp, t, c, vsf, vsc);
-              else k_interp<T, 0, NTV><<<g, kBlock, 0, st>>>(P, P.nslices, fskip, cc, damp, t,
-                                                             c, vsf, vsc));
+  W_DISPATCH(P.wmax, NT_DISPATCH(nt, if (c) k_interp<MWV, T, 1, NTV><<<g, kBlock, 0, st>>>(
+                                                P, P.nslices, fskip, cc, damp, t, c, vsf, vsc);
+                                 else k_interp<MWV, T, 0, NTV><<<g, kBlock, 0, st>>>(
+                                                P, P.nslices, fskip, cc, damp, t, c, vsf, vsc)));
 }
 template void launch_interp<double>(hipStream_t, const DSell<double> &, const uint32_t *,
                                     const double *, double, double *, double *, bool, int,
@@ -754,6 +792,8 @@ template void launch_convert<double, float>(hipStream_t, int, const float *, dou
                                             size_t);
 template void launch_convert<double, double>(hipStream_t, int, const double *, double *, int,
                                              size_t, size_t);
+template void launch_convert<float, float>(hipStream_t, int, const float *, float *, int, size_t,
+                                           size_t);
 
 template <typename T>
 void launch_add(hipStream_t st, int n, T *x, const T *y, Batch bt) {
@@ -761,6 +801,40 @@ void launch_add(hipStream_t st, int n, T *x, const T *y, Batch bt) {
 }
 template void launch_add<double>(hipStream_t, int, double *, const double *, Batch);
 template void launch_add<float>(hipStream_t, int, float *, const float *, Batch);
+
+template <typename TD, typename TS, int SCATTER>
+__global__ __launch_bounds__(kBlock) void k_permute(int n, const int *__restrict__ perm,
+                                                     const TS *__restrict__ s, TD *__restrict__ d,
+                                                     size_t vss, size_t vsd) {
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= n) return;
+  const int q = perm[p];
+  s += (size_t)blockIdx.y * vss;
+  d += (size_t)blockIdx.y * vsd;
+  if (SCATTER) d[q] = (TD)s[p];
+  else d[p] = (TD)s[q];
+}
+
+template <typename TD, typename TS>
+void launch_gather(hipStream_t st, int n, const int *perm, const TS *s, TD *d, int nrhs,
+                   size_t vss, size_t vsd) {
+  if (n > 0) k_permute<TD, TS, 0><<<grid2(nblk(n), nrhs), kBlock, 0, st>>>(n, perm, s, d, vss, vsd);
+}
+template <typename TD, typename TS>
+void launch_scatter(hipStream_t st, int n, const int *perm, const TS *s, TD *d, int nrhs,
+                    size_t vss, size_t vsd) {
+  if (n > 0) k_permute<TD, TS, 1><<<grid2(nblk(n), nrhs), kBlock, 0, st>>>(n, perm, s, d, vss, vsd);
+}
+#define PERM_INST(TD, TS)                                                                      \
+  template void launch_gather<TD, TS>(hipStream_t, int, const int *, const TS *, TD *, int,    \
+                                      size_t, size_t);                                         \
+  template void launch_scatter<TD, TS>(hipStream_t, int, const int *, const TS *, TD *, int,   \
+                                       size_t, size_t);
+PERM_INST(double, double)
+PERM_INST(float, double)
+PERM_INST(double, float)
+PERM_INST(float, float)
+#undef PERM_INST
 
 template <typename T>
 void launch_coarse_cycle(hipStream_t st, const CoarseArgs<T> &args, int base_n) {
@@ -938,7 +1012,7 @@ void launch_bcgs_p(hipStream_t st, int n, const double *b, double *r, double *v,
                                                               bt.vs);
 }
 
-template <typename TQ, int NT>
+template <int MW, typename TQ, int NT>
 __global__ __launch_bounds__(kBlock) void k_bcgs_spmv(DSell<double> A, int nsl,
                                                        const TQ *__restrict__ q,
                                                        double *__restrict__ y,
@@ -955,7 +1029,7 @@ __global__ __launch_bounds__(kBlock) void k_bcgs_spmv(DSell<double> A, int nsl,
   double yz = 0.0, y2 = 0.0;
   if (sok) {
     const double sum =
-        sell_dot<NT, double, TQ, double>(A.val + e0 + lane, A.col + e0 + lane, w, q + o, sdo, A.dtab, rb);
+        sell_dot<MW, NT, double, TQ, double>(A.val + e0 + lane, A.col + e0 + lane, w, q + o, sdo, A.dtab, rb);
     if (r >= 0) {
       y[o + r] = sum;
       if (vclass[r] >= 2) {
@@ -995,8 +1069,9 @@ void launch_bcgs_spmv(hipStream_t st, const DSell<double> &A, const TQ *q, doubl
                       const double *z, const uint8_t *vclass, double *part, int yy, int skip_half, const BcgsState *s, bool nt, Batch bt) {
   if (A.nslices <= 0) return;
   const dim3 g = grid2(nblk_slices(A.nslices), bt.nrhs);
-  NT_DISPATCH(nt, k_bcgs_spmv<TQ, NTV><<<g, kBlock, 0, st>>>(A, A.nslices, q, y, z, vclass, part,
-                                                             yy, skip_half, s, bt.vs));
+  W_DISPATCH(A.wmax, NT_DISPATCH(nt, k_bcgs_spmv<MWV, TQ, NTV><<<g, kBlock, 0, st>>>(
+                                         A, A.nslices, q, y, z, vclass, part, yy, skip_half, s,
+                                         bt.vs)));
 }
 template void launch_bcgs_spmv<double>(hipStream_t, const DSell<double> &, const double *,
                                        double *, const double *, const uint8_t *, double *, int,

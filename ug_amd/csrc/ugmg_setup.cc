@@ -219,13 +219,150 @@ int greedy_color(const HostLevel &L, std::vector<int> &color) {
   return ncol;
 }
 
-void build_prep(const HostLevel &L, const HostLevel *coarse, bool want_gs,
-                HostPrep &out) {
-  build_sell(L, nullptr, Part::All, nullptr, out.A);
+void permute_level(const HostLevel &L, const std::vector<int> &perm, const std::vector<int> &pos,
+                   const std::vector<int> *cpos, HostLevel &out) {
+  const int n = L.n;
+  out.n = n;
+  out.nnzb = L.nnzb;
+  out.rp.assign(n + 1, 0);
+  for (int p = 0; p < n; ++p) out.rp[p + 1] = out.rp[p] + (L.rp[perm[p] + 1] - L.rp[perm[p]]);
+  out.ci.resize(L.ci.size());
+  out.a.resize(L.a.size());
+  out.skip.resize(n);
+  out.vc.resize(n);
+  out.vnc.resize(n);
+#pragma omp parallel for schedule(static)
+  for (int p = 0; p < n; ++p) {
+    const int r = perm[p];
+    int q = out.rp[p];
+    for (int m = L.rp[r]; m < L.rp[r + 1]; ++m, ++q) {
+      out.ci[q] = pos[L.ci[m]];
+      out.a[q] = L.a[m];
+    }
+    out.skip[p] = L.skip[r];
+    out.vc[p] = L.vc[r];
+    out.vnc[p] = L.vnc[r];
+  }
+  out.has_p = L.has_p;
+  if (L.has_p) {
+    out.pp.assign(n + 1, 0);
+    for (int p = 0; p < n; ++p) out.pp[p + 1] = out.pp[p] + (L.pp[perm[p] + 1] - L.pp[perm[p]]);
+    out.pi.resize(L.pi.size());
+    out.pw.resize(L.pw.size());
+    for (int p = 0; p < n; ++p) {
+      const int r = perm[p];
+      int q = out.pp[p];
+      for (int m = L.pp[r]; m < L.pp[r + 1]; ++m, ++q) {
+        out.pi[q] = cpos ? (*cpos)[L.pi[m]] : L.pi[m];
+        out.pw[q] = L.pw[m];
+      }
+    }
+  }
+}
+
+void gs_device_order(const std::vector<int> &color, int ncolors, int window,
+                     std::vector<int> &perm) {
+  const int n = (int)color.size();
+  const int W = window > 0 ? window : std::max(n, 1);
+  perm.resize(n);
+  int p = 0;
+  std::vector<int> cnt(ncolors + 1);
+  for (int w0 = 0; w0 < n; w0 += W) {
+    const int w1 = std::min(n, w0 + W);
+    std::fill(cnt.begin(), cnt.end(), 0);
+    for (int i = w0; i < w1; ++i) ++cnt[color[i] + 1];
+    for (int c = 0; c < ncolors; ++c) cnt[c + 1] += cnt[c];
+    for (int i = w0; i < w1; ++i) perm[p + cnt[color[i]]++] = i;
+    p += w1 - w0;
+  }
+}
+
+int choose_gs_window(const HostLevel &L, const std::vector<int> &color, int ncolors) {
+  const int n = L.n;
+  if (n < 4 * kSlice) return 0;
+  // most common smallest column offset >= 2 over a sample of rows
+  std::vector<int> cand;
+  {
+    std::vector<std::pair<int, int>> hist;
+    for (int k = 0; k < 512; ++k) {
+      const int r = (int)((long long)k * (n - 1) / 511);
+      int s = 0;
+      for (int m = L.rp[r]; m < L.rp[r + 1]; ++m) {
+        const int d = L.ci[m] - r;
+        if (d >= 2 && (s == 0 || d < s)) s = d;
+      }
+      if (s == 0) continue;
+      bool f = false;
+      for (auto &h : hist)
+        if (h.first == s) { ++h.second; f = true; }
+      if (!f) hist.emplace_back(s, 1);
+    }
+    int best = 0, bc = 0;
+    for (auto &h : hist)
+      if (h.second > bc) { bc = h.second; best = h.first; }
+    if (best > 0) {
+      cand.push_back(best);
+      cand.push_back(best + 1);
+    }
+    cand.push_back(0);   // whole level
+  }
+  int win = 0;
+  double best_score = -1.0;
+  std::vector<int> perm, pos(n), list;
+  for (int W : cand) {
+    gs_device_order(color, ncolors, W, perm);
+    for (int p = 0; p < n; ++p) pos[perm[p]] = p;
+    // colour-grouped device positions (the GS slab order)
+    list.clear();
+    for (int c = 0; c < ncolors; ++c)
+      for (int p = 0; p < n; ++p)
+        if (color[perm[p]] == c) list.push_back(p);
+    const int nsl = (int)list.size() / kSlice;
+    int tried = 0, uni = 0;
+    for (int k = 0; k < 256 && nsl > 0; ++k) {
+      const int s0 = (int)((long long)k * nsl / 256) * kSlice;
+      ++tried;
+      // uniform: every row's offsets (device positions) an in-order
+      // subsequence of the longest row's
+      int longest = -1, lw = -1;
+      for (int j = 0; j < kSlice; ++j) {
+        const int r = perm[list[s0 + j]];
+        const int w = L.rp[r + 1] - L.rp[r];
+        if (w > lw) { lw = w; longest = list[s0 + j]; }
+      }
+      std::vector<int> U;
+      const int rl = perm[longest];
+      for (int m = L.rp[rl]; m < L.rp[rl + 1]; ++m) U.push_back(pos[L.ci[m]] - longest);
+      bool ok = true;
+      for (int j = 0; j < kSlice && ok; ++j) {
+        const int p = list[s0 + j], r = perm[p];
+        size_t q = 0;
+        for (int m = L.rp[r]; m < L.rp[r + 1] && ok; ++m) {
+          const int d = pos[L.ci[m]] - p;
+          while (q < U.size() && U[q] != d) ++q;
+          if (q == U.size()) ok = false;
+          else ++q;
+        }
+      }
+      uni += ok;
+    }
+    const double score = tried ? (double)uni / tried : 0.0;
+    if (score > best_score + 1e-9) {
+      best_score = score;
+      win = W;
+    }
+  }
+  return win;
+}
+
+void build_prep(const HostLevel &L, const HostLevel *coarse, const std::vector<int> *color,
+                int ncolors, HostPrep &out) {
   out.diag.resize(L.n);
   for (int i = 0; i < L.n; ++i) out.diag[i] = L.a[L.rp[i]];
-  if (want_gs) {
-    out.ncolors = greedy_color(L, out.color);
+  if (!color) build_sell(L, nullptr, Part::All, nullptr, out.A);
+  if (color) {
+    out.ncolors = ncolors;
+    out.color = *color;
     // rows grouped by colour, list order inside a colour; each colour padded
     // to whole slices so one launch per colour covers whole wavefronts
     std::vector<int> rows;
@@ -239,6 +376,9 @@ void build_prep(const HostLevel &L, const HostLevel *coarse, bool want_gs,
     }
     build_sell(L, &rows, Part::Lower, &out.color, out.L);
     build_sell(L, &rows, Part::Upper, &out.color, out.U);
+    // the whole matrix over the same colour-grouped rows: a slice then holds
+    // one colour run, whose stencil offsets are uniform
+    build_sell(L, &rows, Part::All, nullptr, out.A);
   }
   if (coarse && L.has_p) {
     const int nc = coarse->n;

@@ -56,6 +56,17 @@ int ugmg_problem_p1_cd(int nref, double eps, double theta, uint64_t seed,
                        ugmg_problem **out);
 double ugmg_problem_theta(const ugmg_problem *p);
 
+/* 3D linear elasticity -div(2 mu eps(u) + lambda div u I) = f, P1 on the
+ * Kuhn split of the unit cube (6 tetrahedra) refined nref times by red
+ * refinement (level l: (2^l+1)^3 nodes, 15-block stencil), 3x3 blocks
+ * (b = 3, displacement components x, y, z), E and nu given, clamped
+ * (all components Dirichlet) on the face x = 0, traction-free elsewhere,
+ * every level rediscretised.  Body force f ~ U[-1,1]^3 per finest node
+ * from the SplitMix64 stream (node order, components fastest); rhs =
+ * consistent P1 mass times f. */
+int ugmg_problem_elasticity(int nref, double E, double nu, uint64_t seed,
+                            ugmg_problem **out);
+
 void ugmg_problem_free(ugmg_problem *p);
 
 int ugmg_problem_nlevels(const ugmg_problem *p);     /* nref + 1            */

@@ -355,6 +355,34 @@ def test_fused_coarse_cycle_matches(poisson2d, monkeypatch):
         ctx.close()
 
 
+@pytest.mark.parametrize("precision,nu", [(0, 2), (1, 3), (0, 1)])
+def test_folded_gs_defects_are_bitwise_the_separate_passes(jump3d_17, monkeypatch, precision, nu):
+    """Folding each GS step's defect into the next step's colour sweeps
+    (the default) changes no bit of the cycle against separate defect
+    passes (UGMG_GS_FUSE=0), in FP64 and in the FP32 cycle; and the list
+    order (UGMG_GS_WINDOW=1) gives the device order's results to rounding."""
+    from ug_amd import ugmg
+    P = jump3d_17
+    n = P.levels[-1].n
+    b0 = P.rhs.copy()
+    out = {}
+    for tag, env in (("fused", {}), ("split", {"UGMG_GS_FUSE": "0"}), ("list", {"UGMG_GS_WINDOW": "1"})):
+        for k in ("UGMG_GS_FUSE", "UGMG_GS_WINDOW"):
+            monkeypatch.delenv(k, raising=False)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        ctx = ugmg.Context(0)
+        ctx.set_hierarchy(P)
+        ctx.set_params(nu1=nu, nu2=nu, gamma=1, baselevel=1, smoother=ugmg.SMOOTH_GS, damp=1.0,
+                       base_abslimit=1e-300, precision=precision, inner_cycles=1)
+        ctx.preprocess()
+        out[tag] = ctx.iter_host(P.top, b0)
+        ctx.close()
+    assert np.array_equal(out["fused"][0], out["split"][0]) and np.array_equal(out["fused"][1], out["split"][1])
+    tol = 1e-12 if precision == 0 else 1e-5
+    assert _rel(out["fused"][0], out["list"][0]) < tol
+
+
 def test_server_results_equal_standalone_solves():
     """Continuous batching: four requests of different sizes and kinds
     admitted together (two slots, so two join while others run) give exactly

@@ -16,6 +16,8 @@
 //     StandardIntCorNodeVector (np/algebra/transgrid.cc:235-336).
 #include "../../include/ugmg_problems.h"
 
+#include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -542,6 +544,215 @@ extern "C" int ugmg_problem_p1_cd(int nref, double eps, double theta, uint64_t s
 }
 
 extern "C" double ugmg_problem_theta(const ugmg_problem *p) { return p ? p->theta : 0.0; }
+
+namespace {
+
+// ---------------------------------------------------------------- elasticity
+// Kuhn (Freudenthal) split of the unit cube into the 6 tetrahedra
+// 0 -> e_a -> e_a + e_b -> (1,1,1) over the permutations (a,b,c) of the axes;
+// refining every cube of a level into 8 and splitting each the same way is a
+// red refinement of the coarse tetrahedra, so the P1 spaces are nested and a
+// fine node either is a coarse node or bisects a coarse edge e_S (S the set
+// of its odd coordinates).
+const int kPerm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+
+// stencil offsets of a Kuhn lattice node: +-e_S for every nonempty S, and 0
+void kuhn_offsets(std::vector<std::array<int, 3>> &offs, int N) {
+  offs.clear();
+  for (int sgn = -1; sgn <= 1; sgn += 2)
+    for (int S = 1; S < 8; ++S) offs.push_back({sgn * (S & 1), sgn * ((S >> 1) & 1), sgn * ((S >> 2) & 1)});
+  offs.push_back({0, 0, 0});
+  // ascending column order for lexicographic ids (x fastest)
+  std::sort(offs.begin(), offs.end(), [N](const std::array<int, 3> &a, const std::array<int, 3> &b) {
+    return a[0] + (long long)N * (a[1] + (long long)N * a[2]) < b[0] + (long long)N * (b[1] + (long long)N * b[2]);
+  });
+}
+
+// P1 linear elasticity on one tetrahedron: K[i][j] (3x3 block, row-major) =
+// V (lambda d_a phi_i d_b phi_j + mu d_b phi_i d_a phi_j + mu delta_ab
+// grad phi_i . grad phi_j), consistent mass V/20 (1 + delta_ij)
+void p1_elasticity_element(const double X[4][3], double lam, double mu, double K[4][4][9],
+                           double M[4][4]) {
+  double J[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) J[r][c] = X[c + 1][r] - X[0][r];
+  const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                     J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                     J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+  const double V = std::fabs(det) / 6.0;
+  // inverse of J: rows are the gradients of the barycentric coordinates 1..3
+  double Ji[3][3];
+  Ji[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / det;
+  Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
+  Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
+  Ji[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) / det;
+  Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
+  Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
+  Ji[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / det;
+  Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
+  Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
+  double g[4][3];
+  for (int d = 0; d < 3; ++d) {
+    g[1][d] = Ji[0][d];
+    g[2][d] = Ji[1][d];
+    g[3][d] = Ji[2][d];
+    g[0][d] = -(g[1][d] + g[2][d] + g[3][d]);
+  }
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      const double gg = g[i][0] * g[j][0] + g[i][1] * g[j][1] + g[i][2] * g[j][2];
+      for (int a = 0; a < 3; ++a)
+        for (int bb = 0; bb < 3; ++bb)
+          K[i][j][a * 3 + bb] =
+              V * (lam * g[i][a] * g[j][bb] + mu * g[i][bb] * g[j][a] + (a == bb ? mu * gg : 0.0));
+      M[i][j] = V / 20.0 * (i == j ? 2.0 : 1.0);
+    }
+}
+
+void assemble_elasticity_level(Level &L, double lam, double mu, std::vector<double> *rhs,
+                               const std::vector<double> *f) {
+  const int N = L.N, C = L.C;
+  const double h = 1.0 / C;
+  const int64_t n = (int64_t)N * N * N;
+  std::vector<std::array<int, 3>> offs;
+  kuhn_offsets(offs, N);
+  const int no = (int)offs.size();   // 15
+  auto slot = [&](int dx, int dy, int dz) {
+    for (int k = 0; k < no; ++k)
+      if (offs[k][0] == dx && offs[k][1] == dy && offs[k][2] == dz) return k;
+    return -1;
+  };
+  std::vector<double> st((size_t)n * no * 9, 0.0);
+  if (rhs) rhs->assign((size_t)n * 3, 0.0);
+  for (int cz = 0; cz < C; ++cz)
+    for (int cy = 0; cy < C; ++cy)
+      for (int cx = 0; cx < C; ++cx)
+        for (int t = 0; t < 6; ++t) {
+          int vx[4][3] = {{0, 0, 0}};
+          for (int q = 1; q < 4; ++q) {
+            for (int d = 0; d < 3; ++d) vx[q][d] = vx[q - 1][d];
+            vx[q][kPerm[t][q - 1]] = 1;
+          }
+          double X[4][3];
+          int64_t id[4];
+          int cc[4][3];
+          for (int q = 0; q < 4; ++q) {
+            cc[q][0] = cx + vx[q][0];
+            cc[q][1] = cy + vx[q][1];
+            cc[q][2] = cz + vx[q][2];
+            for (int d = 0; d < 3; ++d) X[q][d] = cc[q][d] * h;
+            id[q] = cc[q][0] + (int64_t)N * (cc[q][1] + (int64_t)N * cc[q][2]);
+          }
+          double K[4][4][9], M[4][4];
+          p1_elasticity_element(X, lam, mu, K, M);
+          for (int i = 0; i < 4; ++i) {
+            for (int j = 0; j < 4; ++j) {
+              const int k = slot(cc[j][0] - cc[i][0], cc[j][1] - cc[i][1], cc[j][2] - cc[i][2]);
+              double *dst = &st[((size_t)id[i] * no + k) * 9];
+              for (int e = 0; e < 9; ++e) dst[e] += K[i][j][e];
+            }
+            if (rhs)
+              for (int j = 0; j < 4; ++j)
+                for (int a = 0; a < 3; ++a) (*rhs)[(size_t)id[i] * 3 + a] += M[i][j] * (*f)[(size_t)id[j] * 3 + a];
+          }
+        }
+  const int kd = slot(0, 0, 0);
+  L.rp.assign(n + 1, 0);
+  for (int64_t i = 0; i < n; ++i) {
+    const int x = (int)(i % N), y = (int)((i / N) % N), z = (int)(i / ((int64_t)N * N));
+    int cnt = 0;
+    for (int k = 0; k < no; ++k) {
+      const int xx = x + offs[k][0], yy = y + offs[k][1], zz = z + offs[k][2];
+      cnt += (xx >= 0 && xx < N && yy >= 0 && yy < N && zz >= 0 && zz < N);
+    }
+    L.rp[i + 1] = L.rp[i] + cnt;
+  }
+  L.ci.resize(L.rp[n]);
+  L.v.resize((size_t)L.rp[n] * 9);
+  L.skip.assign(n, 0);
+  for (int64_t i = 0; i < n; ++i) {
+    const int x = (int)(i % N), y = (int)((i / N) % N), z = (int)(i / ((int64_t)N * N));
+    int64_t q = L.rp[i];
+    auto put = [&](int64_t col, int k) {
+      L.ci[q] = (int)col;
+      std::memcpy(&L.v[(size_t)q * 9], &st[((size_t)i * no + k) * 9], 9 * sizeof(double));
+      ++q;
+    };
+    put(i, kd);
+    for (int k = 0; k < no; ++k) {
+      if (k == kd) continue;
+      const int xx = x + offs[k][0], yy = y + offs[k][1], zz = z + offs[k][2];
+      if (xx < 0 || xx >= N || yy < 0 || yy >= N || zz < 0 || zz >= N) continue;
+      put(xx + (int64_t)N * (yy + (int64_t)N * zz), k);
+    }
+    if (x == 0) {   // clamped face: every component Dirichlet
+      L.skip[i] = 7u;
+      for (int64_t m = L.rp[i]; m < L.rp[i + 1]; ++m)
+        for (int e = 0; e < 9; ++e) L.v[(size_t)m * 9 + e] = 0.0;
+      for (int a = 0; a < 3; ++a) L.v[(size_t)L.rp[i] * 9 + a * 4] = 1.0;
+      if (rhs)
+        for (int a = 0; a < 3; ++a) (*rhs)[(size_t)i * 3 + a] = 0.0;
+    }
+  }
+  L.vc.assign(n, 3);
+}
+
+// P1 interpolation on the Kuhn lattice: coarse nodes 1, bisecting nodes 1/2
+// to the two end points of the coarse edge (lower end point first)
+void build_kuhn_transfer(const Level &coarse, Level &fine) {
+  const int Nf = fine.N, Nc = coarse.N;
+  const int64_t n = (int64_t)Nf * Nf * Nf;
+  fine.pp.assign(n + 1, 0);
+  fine.pi.clear();
+  fine.pw.clear();
+  auto cid = [Nc](int x, int y, int z) { return x + Nc * (y + Nc * z); };
+  for (int64_t i = 0; i < n; ++i) {
+    const int x = (int)(i % Nf), y = (int)((i / Nf) % Nf), z = (int)(i / ((int64_t)Nf * Nf));
+    if (x % 2 == 0 && y % 2 == 0 && z % 2 == 0) {
+      fine.pi.push_back(cid(x / 2, y / 2, z / 2));
+      fine.pw.push_back(1.0);
+    } else {
+      fine.pi.push_back(cid(x / 2, y / 2, z / 2));
+      fine.pw.push_back(0.5);
+      fine.pi.push_back(cid((x + 1) / 2, (y + 1) / 2, (z + 1) / 2));
+      fine.pw.push_back(0.5);
+    }
+    fine.pp[i + 1] = (int)fine.pi.size();
+  }
+}
+
+}  // namespace
+
+extern "C" int ugmg_problem_elasticity(int nref, double E, double nu, uint64_t seed,
+                                       ugmg_problem **out) {
+  if (!out || nref < 0 || nref > 8 || !(E > 0.0) || !(nu > -1.0 && nu < 0.5)) return -1;
+  ugmg_problem *p = new (std::nothrow) ugmg_problem;
+  if (!p) return -4;
+  p->dim = 3;
+  p->b = 3;
+  const double lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)), mu = E / (2.0 * (1.0 + nu));
+  const int L = nref;
+  try {
+    p->lv.resize(L + 1);
+    for (int l = 0; l <= L; ++l) {
+      p->lv[l].C = 1 << l;
+      p->lv[l].N = p->lv[l].C + 1;
+      p->lv[l].n = p->lv[l].N * p->lv[l].N * p->lv[l].N;
+    }
+    uint64_t s = seed;
+    std::vector<double> f((size_t)p->lv[L].n * 3);
+    for (auto &fi : f) fi = -1.0 + 2.0 * splitmix_u01(s);
+    for (int l = 0; l <= L; ++l)
+      assemble_elasticity_level(p->lv[l], lam, mu, l == L ? &p->rhs : nullptr, l == L ? &f : nullptr);
+    for (int l = 0; l <= L; ++l) p->lv[l].vnc.assign(p->lv[l].n, l < L ? 3 : 0);
+    for (int l = 1; l <= L; ++l) build_kuhn_transfer(p->lv[l - 1], p->lv[l]);
+  } catch (const std::bad_alloc &) {
+    delete p;
+    return -4;
+  }
+  *out = p;
+  return 0;
+}
 
 extern "C" void ugmg_problem_free(ugmg_problem *p) { delete p; }
 

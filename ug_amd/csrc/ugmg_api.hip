@@ -61,6 +61,7 @@ struct DevLevel {
   DevSellBuf A, L, U;
   int ncolors = 0;
   std::vector<int> color_slices, color, color_wmax;
+  bool all_active = false;   // every vector has VCLASS >= 3
   double *diag64 = nullptr;
   float *diag32 = nullptr;
   uint8_t *vclass = nullptr, *vnclass = nullptr;
@@ -121,6 +122,7 @@ struct ugmg_ctx {
   // batched launches: level vectors hold nrhs_cap columns; nb = columns of
   // the work being enqueued
   int nrhs_cap = 1, nb = 1;
+  bool fuse_steps = true;   // fold GS defects into the next sweep (UGMG_GS_FUSE=0: off)
   // BCGSSolver work (allocated on first use, freed with the hierarchy)
   struct Bcgs {
     int nrhs = 0, maxdef = 0, npart = 0;
@@ -272,6 +274,31 @@ void smooth_step(ugmg_ctx *ctx, int l, T *t, T *b, T *c, int cmode, T *xout) {
   }
 }
 
+// nsteps smoother steps (cmode: correction mode of the first).  On a GS
+// level whose rows are all active, the defect of every step but the last is
+// folded into the next step's colour sweeps.
+template <typename T>
+void smooth(ugmg_ctx *ctx, int l, T *t, T *b, T *c, int cmode, int nsteps) {
+  DevLevel &d = ctx->dev[l];
+  if (ctx->p.smoother != UGMG_SMOOTH_GS || !d.all_active || nsteps < 2 || !ctx->fuse_steps) {
+    for (int i = 0; i < nsteps; ++i) smooth_step<T>(ctx, l, t, b, c, i == 0 ? cmode : 1, nullptr);
+    return;
+  }
+  hipStream_t st = ctx->stream;
+  const T damp = (T)ctx->p.damp[0];
+  const Batch bt = batch(ctx, l);
+  for (int k = 0; k < d.ncolors; ++k)
+    launch_gs_color<T, T>(st, d.L.d<T>(), d.color_slices[k],
+                          d.color_slices[k + 1] - d.color_slices[k], d.color_wmax[k], b,
+                          d.diag<T>(), d.vclass, t, d.nt, bt);
+  for (int i = 1; i < nsteps; ++i)
+    for (int k = 0; k < d.ncolors; ++k)
+      launch_gs_color_fused<T, T>(st, d.L.d<T>(), d.U.d<T>(), d.color_slices[k],
+                                  d.color_slices[k + 1] - d.color_slices[k], b, d.diag<T>(),
+                                  damp, t, c, i == 1 ? cmode : 1, d.nt, bt);
+  launch_gs_defect<T, T>(st, d.U.d<T>(), t, b, d.vclass, damp, c, 1, nullptr, d.nt, bt);
+}
+
 BaseArgs base_args(ugmg_ctx *ctx) {
   BaseArgs ba;
   ba.n = ctx->dev[ctx->bl].n;
@@ -350,7 +377,7 @@ void lmgc(ugmg_ctx *ctx, int l, T *c, T *b, int cmode) {
   DevLevel &dc = ctx->dev[l - 1];
   T *t = d.tv<T>();
   const int nb = ctx->nb;
-  for (int i = 0; i < p.nu1; ++i) smooth_step<T>(ctx, l, t, b, c, i == 0 ? cmode : 1, nullptr);
+  smooth<T>(ctx, l, t, b, c, cmode, p.nu1);
   if (p.nu1 == 0 && cmode == 0) launch_fill<T>(st, d.n, c, (T)0, batch(ctx, l));
   // RestrictDefect(level, b, b, Factor_One)
   const bool to_base = l - 1 == ctx->bl && (ctx->fuse_top < ctx->bl || nb > 1);
@@ -373,7 +400,7 @@ void lmgc(ugmg_ctx *ctx, int l, T *c, T *b, int cmode) {
                    dc.vs);
   launch_spmv<T, T, T>(st, d.A.d<T>(), 0, d.A.nslices, t, b, (T *)nullptr, 0, d.nt,
                        batch(ctx, l));
-  for (int i = 0; i < p.nu2; ++i) smooth_step<T>(ctx, l, t, b, c, 1, nullptr);
+  smooth<T>(ctx, l, t, b, c, 1, p.nu2);
 }
 
 int check_ready(ugmg_ctx *ctx, int level) {
@@ -608,6 +635,8 @@ int prep_host(ugmg_ctx *ctx, int level, PrepState &ps) {
     return fail(ctx, UGMG_E_ARG, "base level has more than 8192 vectors; choose a coarser baselevel");
   const char *env_nt = std::getenv("UGMG_NT_BYTES");
   ps.nt_bytes = env_nt ? std::atof(env_nt) : 64.0e6;
+  const char *env_gsf = std::getenv("UGMG_GS_FUSE");
+  ctx->fuse_steps = !(env_gsf && std::atoi(env_gsf) == 0);
   const char *env_fuse = std::getenv("UGMG_FUSE_ROWS");
   ps.fuse_rows = env_fuse ? std::atoi(env_fuse) : 0;
   ps.level = level;
@@ -686,6 +715,8 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
         D.color = hp.color_ug;
         if ((rc = upload_sell(ctx, hp.L, D.L))) return rc;
         if ((rc = upload_sell(ctx, hp.U, D.U))) return rc;
+        D.all_active = true;
+        for (uint8_t v : H.vc) D.all_active &= v >= 3;
         D.color_wmax.resize(D.ncolors);
         for (int k = 0; k < D.ncolors; ++k)
           D.color_wmax[k] = D.L.max_width(D.color_slices[k], D.color_slices[k + 1]);

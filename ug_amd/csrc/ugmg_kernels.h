@@ -103,6 +103,12 @@ void launch_jac(hipStream_t st, int n, const T *b, const T *diag, const uint8_t 
 template <typename TV, typename T>
 void launch_gs_color(hipStream_t st, const DSell<TV> &L, int slice0, int nsl, int wmax, const T *b,
                      const T *diag, const uint8_t *vclass, T *v, bool nt, Batch bt);
+// sweep of one colour with the previous step's defect folded in (all rows
+// active); cmode is the correction mode of the previous step
+template <typename TV, typename T>
+void launch_gs_color_fused(hipStream_t st, const DSell<TV> &L, const DSell<TV> &U, int slice0,
+                           int nsl, T *b, const T *diag, T damp, T *v, T *c, int cmode, bool nt,
+                           Batch bt);
 template <typename TV, typename T>
 void launch_gs_defect(hipStream_t st, const DSell<TV> &U, const T *v, T *b, const uint8_t *vclass,
                       T damp, T *c, int cmode, T *xout, bool nt, Batch bt);

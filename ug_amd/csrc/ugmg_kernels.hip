@@ -229,6 +229,42 @@ __global__ __launch_bounds__(kBlock) void k_gs_color(DSell<TV> L, int slice0, in
   v[r] = act ? (br - sum) / dr : (T)0;
 }
 
+// Step s+1 of the sweep with step s's defect folded in (levels whose rows
+// are all active).  When colour k runs, v still holds step s at the higher
+// colours and step s+1 at the lower ones, so one pass over the row gives
+//   b_r <- b_r - damp (b_r + (U v_s)_r)      (the defect of step s)
+//   c_r (+)= damp v_s,r                      (CMODE 0: =, 1: +=)
+//   v_r <- (b_r - (L v_{s+1})_r) / a_rr      (the sweep of step s+1)
+// with the same operations, in the same order, as the separate kernels.
+template <typename TV, typename T, int CMODE, int NT>
+__global__ __launch_bounds__(kBlock) void k_gs_color_fused(DSell<TV> L, DSell<TV> U, int slice0,
+                                                            int nsl, T *__restrict__ b,
+                                                            const T *__restrict__ diag, T damp,
+                                                            T *__restrict__ v,
+                                                            T *__restrict__ c, size_t vs) {
+  SLICE_PROLOGUE(L, slice0, nsl)
+  const size_t o = (size_t)col_ * vs;
+  v += o;
+  const int eu = U.soff[s];
+  const int wu = (U.soff[s + 1] - eu) >> 6;
+  const int sdu = U.sdoff[s];
+  T br = 0, dr = 1, vo = 0, cr = 0;
+  if (r >= 0) {
+    br = b[o + r];
+    dr = diag[r];
+    vo = v[r];
+    if (CMODE == 1) cr = c[o + r];
+  }
+  const T su = sell_dot<8, NT, TV, T, T>(U.val + eu + lane, U.col + eu + lane, wu, v, sdu, U.dtab, rb);
+  const T sl = sell_dot<8, NT, TV, T, T>(L.val + e0 + lane, L.col + e0 + lane, w, v, sdo, L.dtab, rb);
+  if (r < 0) return;
+  const T bn = br - damp * (br + su);
+  b[o + r] = bn;
+  const T t = damp * vo;
+  c[o + r] = CMODE == 0 ? t : cr + t;
+  v[r] = (bn - sl) / dr;
+}
+
 // defect after the sweep, b -= A (damp v), with (D+L) v = b on active rows:
 //   active   b <- b - damp (b + U v)
 //   inactive b <- b - damp (A v)      (v_r = 0; U holds the whole row)
@@ -624,6 +660,26 @@ template void launch_gs_color<double, double>(hipStream_t, const DSell<double> &
 template void launch_gs_color<float, float>(hipStream_t, const DSell<float> &, int, int, int,
                                             const float *, const float *, const uint8_t *,
                                             float *, bool, Batch);
+
+template <typename TV, typename T>
+void launch_gs_color_fused(hipStream_t st, const DSell<TV> &L, const DSell<TV> &U, int slice0,
+                           int nsl, T *b, const T *diag, T damp, T *v, T *c, int cmode, bool nt,
+                           Batch bt) {
+  if (nsl <= 0) return;
+  const dim3 g = grid2(nblk_slices(nsl), bt.nrhs);
+  NT_DISPATCH(nt, if (cmode == 0) k_gs_color_fused<TV, T, 0, NTV><<<g, kBlock, 0, st>>>(
+                      L, U, slice0, nsl, b, diag, damp, v, c, bt.vs);
+              else k_gs_color_fused<TV, T, 1, NTV><<<g, kBlock, 0, st>>>(
+                      L, U, slice0, nsl, b, diag, damp, v, c, bt.vs));
+}
+template void launch_gs_color_fused<double, double>(hipStream_t, const DSell<double> &,
+                                                    const DSell<double> &, int, int, double *,
+                                                    const double *, double, double *, double *,
+                                                    int, bool, Batch);
+template void launch_gs_color_fused<float, float>(hipStream_t, const DSell<float> &,
+                                                  const DSell<float> &, int, int, float *,
+                                                  const float *, float, float *, float *, int,
+                                                  bool, Batch);
 
 template <typename TV, typename T>
 void launch_gs_defect(hipStream_t st, const DSell<TV> &U, const T *v, T *b, const uint8_t *vclass,

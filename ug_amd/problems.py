@@ -43,6 +43,8 @@ def _load():
         lib.ugmg_splitmix_uniform.argtypes = [C.c_uint64, C.c_int64, C.c_double, C.c_double, dp]
         lib.ugmg_problem_p1_cd.argtypes = [C.c_int, C.c_double, C.c_double, C.c_uint64, C.POINTER(P)]
         lib.ugmg_problem_p1_cd.restype = C.c_int
+        lib.ugmg_problem_elasticity.argtypes = [C.c_int, C.c_double, C.c_double, C.c_uint64, C.POINTER(P)]
+        lib.ugmg_problem_elasticity.restype = C.c_int
         lib.ugmg_problem_theta.argtypes = [P]
         lib.ugmg_problem_theta.restype = C.c_double
         lib.ugmg_splitmix_uniform.restype = None
@@ -134,6 +136,17 @@ def p1_cd(nref: int, eps: float = 1e-3, theta: float = float("nan"), seed: int =
     P = _extract(lib, h, 2, name or f"p1cd_{2 ** (nref + 1) + 1}^2")
     P.theta = th
     return P
+
+
+def elasticity(nref: int, E: float = 1.0, nu: float = 0.3, seed: int = 0x5EED0004, name: str = "") -> Problem:
+    """3D P1 linear elasticity on the Kuhn-split unit cube (see
+    include/ugmg_problems.h): b = 3, level nref has (2^nref+1)^3 nodes."""
+    lib = _load()
+    h = C.c_void_p()
+    rc = lib.ugmg_problem_elasticity(nref, E, nu, C.c_uint64(seed), C.byref(h))
+    if rc != 0:
+        raise ValueError(f"ugmg_problem_elasticity failed ({rc})")
+    return _extract(lib, h, 3, name or f"elasticity_p1_{2 ** nref + 1}^3")
 
 
 def _extract(lib, h, dim, name):

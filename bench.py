@@ -195,6 +195,72 @@ def bcgs_run(nref, nrhs, steps, warmup, cpu_sample):
     return out
 
 
+def elasticity_run(nref, steps, warmup, inner, cpu_cycles):
+    """3D P1 linear elasticity (3x3 blocks) on the Kuhn-split unit cube,
+    clamped at x = 0: ls + lmgc V(2,2) multicolour block GS to 1e-10 per
+    component, FP64 and FP32 cycle in FP64 refinement."""
+    import numpy as np
+    import torch
+    from ug_amd import problems, ugmg
+
+    t0 = time.time()
+    P = problems.elasticity(nref, E=1.0, nu=0.3, seed=0x5EED0004)
+    gen_s = time.time() - t0
+    top, n = P.top, P.levels[-1].n
+    bl = max(l for l, lv in enumerate(P.levels) if 3 * lv.n <= 8192)
+    prm = dict(nu1=2, nu2=2, gamma=1, baselevel=bl, smoother=ugmg.SMOOTH_GS, damp=1.0,
+               base_abslimit=1e-300, base_red=1e-12, base_maxiter=1, use_graph=1)
+    rhs = torch.from_numpy(P.rhs).cuda()
+    x = torch.zeros_like(rhs)
+    b = torch.empty_like(rhs)
+    out = {"workload": f"elasticity_p1_{2 ** nref + 1}^3: P1 tets (Kuhn split), E=1 nu=0.3, clamped x=0, "
+                       f"3x3 blocks, V(2,2) multicolour block GS, base level {bl}, tol 1e-10 per component",
+           "nodes": n, "unknowns": 3 * n, "nnz_blocks_fine": int(P.levels[-1].nnzb), "levels": top + 1,
+           "generate_s": round(gen_s, 2)}
+    A64 = None
+    for name, prec in (("fp64", ugmg.FP64), ("mixed", ugmg.MIXED)):
+        ctx = ugmg.Context(0)
+        ctx.set_hierarchy(P)
+        ctx.set_params(**prm, precision=prec, inner_cycles=inner)
+        ctx.preprocess()
+        lat, res = [], None
+        for k in range(warmup + steps):
+            x.zero_()
+            b.copy_(rhs)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            res, _ = ctx.solve(top, x, b, red=1e-10, abslimit=1e-300, maxiter=500, record=False)
+            torch.cuda.synchronize()
+            if k >= warmup:
+                lat.append(time.perf_counter() - t)
+        bb = rhs.clone()
+        torch.cuda.synchronize()
+        ctx.dmatmul_minus(top, bb, x)
+        ctx.sync()
+        tr = (bb.view(-1, 3).norm(dim=0) / rhs.view(-1, 3).norm(dim=0)).cpu().numpy()
+        out[name] = {"ms": round(float(np.mean(lat)) * 1e3, 3), "iterations": res.number_of_linear_iterations,
+                     "converged": bool(res.converged), "true_rel_defect_per_component": [float(v) for v in tr]}
+        if name == "fp64":
+            A64 = ctx
+        else:
+            ctx.close()
+    out["speedup_mixed_vs_fp64"] = round(out["fp64"]["ms"] / out["mixed"]["ms"], 3)
+    A64.close()
+    if cpu_cycles > 0:
+        from oracle import oracle as O
+        H = O.Hierarchy(P.levels)
+        p = O.params(smoother=O.GS, damp=1.0, baselevel=bl)
+        t = time.perf_counter()
+        O.ls_solve(H, top, p, P.rhs, red=1e-300, maxiter=cpu_cycles)
+        cyc = (time.perf_counter() - t) / cpu_cycles
+        its = out["fp64"]["iterations"]
+        out["cpu_baseline"] = {"value": round(cyc * its * 1e3, 1), "unit": "ms", "cores": 1, "kind": "port",
+                               "sample": f"{cpu_cycles} ls+lmgc V(2,2) cycles of the oracle with UG's lexicographic "
+                                         f"block GS on the same hierarchy, 1 thread, scaled by the FP64 GPU cycle "
+                                         f"count ({its})"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -211,6 +277,8 @@ def main():
     ap.add_argument("--bcgs-nref", type=int, default=8, help="P1 refinements of the BiCGStab leg (8: 513^2)")
     ap.add_argument("--bcgs-rhs", type=int, default=32)
     ap.add_argument("--no-bcgs", action="store_true")
+    ap.add_argument("--elasticity-nref", type=int, default=6, help="Kuhn-cube refinements (6: 65^3 nodes)")
+    ap.add_argument("--no-elasticity", action="store_true")
     ap.add_argument("--kernels-only", action="store_true",
                     help="run only the roofline kernels and a copy kernel of known size (for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -355,6 +423,10 @@ def main():
     if not args.no_bcgs:
         out["bcgs_cd2d"] = bcgs_run(args.bcgs_nref, args.bcgs_rhs, args.steps, args.warmup,
                                     0 if args.no_cpu else 2)
+
+    if not args.no_elasticity:
+        out["elasticity_3d"] = elasticity_run(args.elasticity_nref, args.steps, args.warmup, args.inner,
+                                              0 if args.no_cpu else 1)
 
     if not args.no_serve:
         out["serving"] = serving_run(args.serve_requests, args.serve_rate, 0x5EED0005, 32.0, args.serve_slots)

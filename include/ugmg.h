@@ -99,6 +99,10 @@ int ugmg_sync(ugmg_ctx *ctx);
  * VECSKIP bits, vclass/vnclass = VCLASS/VNCLASS (gm/gm.h:2200-2288).
  * Interpolation rows: coarse corners of the father element with their GNs
  * weights (StandardIntCorNodeVector, transgrid.cc:235-336). */
+/* b: components per vector, 1 or 3 (3x3 blocks: SolveSmallBlock-based
+ * Jacobi and GS, per-component Dirichlet bits, norms and stop tests; one
+ * damping factor and base limit for all components; bcgs and the server
+ * take b = 1) */
 int ugmg_set_nlevels(ugmg_ctx *ctx, int nlevels, int b);
 int ugmg_set_level(ugmg_ctx *ctx, int level, int n, const int *row_ptr,
                    const int *col_idx, const double *vals,
@@ -124,6 +128,7 @@ int ugmg_get_pattern(ugmg_ctx *ctx, int level, int *row_ptr, int *col_idx,
 int ugmg_dmatmul_minus(ugmg_ctx *ctx, int level, double *x, const double *y); /* x -= A y */
 int ugmg_dmatmul(ugmg_ctx *ctx, int level, double *x, const double *y);       /* x  = A y */
 int ugmg_smatmul_minus(ugmg_ctx *ctx, int level, float *x, const float *y);   /* FP32 copy of A */
+/* dnrm2x / ddot: one value per component (out_host holds b doubles) */
 int ugmg_dnrm2x(ugmg_ctx *ctx, int level, const double *x, double *out_host);
 int ugmg_ddot(ugmg_ctx *ctx, int level, const double *x, const double *y,
               double *out_host);
@@ -188,6 +193,16 @@ int ugmg_bcgs_solve_host(ugmg_ctx *ctx, int level, int nrhs, double *x,
                          const double *reduction, int maxiter, int restart,
                          int use_iter, int do_defect, ugmg_lresult *res,
                          double *defects, int maxdef);
+
+/* ---- Galerkin coarse matrices: AssembleGalerkinByMatrix ------------------
+ * (np/algebra/transgrid.cc:1575-1700) on the GPU.  The values of level-1
+ * become P^T A P from level's matrix and its interpolation (scalar weights;
+ * 3x3 blocks componentwise), in the pattern set for level-1; a product
+ * whose connection is missing from that pattern is added to the diagonal
+ * (UG's branch when CreateExtraConnection fails).  Replaces level-1's
+ * values in the context; call ugmg_preprocess afterwards.  Applied from the
+ * top down it builds a rediscretisation-free hierarchy. */
+int ugmg_galerkin(ugmg_ctx *ctx, int level);
 
 /* ---- host-only setup routines (no GPU needed) ---------------------------
  * The pieces of PreProcess that run on the host, callable on their own.

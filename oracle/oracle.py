@@ -66,6 +66,8 @@ def lib():
         L.orc_ls_lmgc_solve.argtypes = [vp, C.c_int, vp, dp, dp, C.c_int, dp, dp, C.c_int, dp,
                                         C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.orc_lmgc_iter.argtypes = [vp, C.c_int, vp, dp, dp]
+        L.orc_galerkin.argtypes = [vp, vp, dp]
+        L.orc_galerkin.restype = None
         L.orc_bcgs_solve.argtypes = [vp, C.c_int, vp, C.c_int, dp, dp, C.c_int, dp, dp, C.c_int, C.c_int, dp,
                                      C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
         _lib = L
@@ -266,3 +268,12 @@ def bcgs_solve(H, top, p, rhs, x0=None, red=1e-8, abslimit=1e-300, maxiter=200, 
         raise RuntimeError(f"oracle bcgs failed ({rc})")
     k = min(nd.value, maxdef)
     return dict(x=x, b=b, defects=defects[:k * nb].reshape(k, nb), nit=nit.value, converged=bool(conv.value))
+
+
+def galerkin(H, l):
+    """AssembleGalerkinByMatrix restated: values of level l-1 (in its
+    pattern) from level l's matrix and interpolation."""
+    lc = H.levels[l - 1]
+    out = np.zeros(lc.nnzb * lc.b * lc.b)
+    lib().orc_galerkin(H.ptr(l), H.ptr(l - 1), out.ctypes.data)
+    return out

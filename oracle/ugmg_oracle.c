@@ -807,3 +807,38 @@ done:
   mg_free(&mg);
   return err;
 }
+
+/* -------------------------------------------------------------- Galerkin */
+
+/* AssembleGalerkinByMatrix (np/algebra/transgrid.cc:1575-1700) with the
+ * interpolation of F as the interpolation matrices: for every fine vector v
+ * (list order), every MATRIX m = (v, w) of its row (diagonal first), every
+ * interpolation entry (I, wi) of v and (J, wj) of w:
+ *   A_c[I][J] += (a_m wi) wj      (blocks: componentwise)
+ * into the pattern of C; a product whose connection is missing from the
+ * pattern goes to the diagonal of I (the branch taken when
+ * CreateExtraConnection fails).  ac (nnzb_C * b * b) is cleared first. */
+void orc_galerkin(const orc_level *F, const orc_level *C, double *ac) {
+  const int b = F->b, bb = b * b;
+  const size_t nz = (size_t)C->rp[C->n] * bb;
+  for (size_t i = 0; i < nz; ++i) ac[i] = 0.0;
+  for (int v = 0; v < F->n; ++v)
+    for (int m = F->rp[v]; m < F->rp[v + 1]; ++m) {
+      const int w = F->ci[m];
+      const double *M = F->a + (size_t)m * bb;
+      for (int q = F->pp[v]; q < F->pp[v + 1]; ++q) {
+        const int I = F->pi[q];
+        const double wi = F->pw[q];
+        for (int r = F->pp[w]; r < F->pp[w + 1]; ++r) {
+          const int J = F->pi[r];
+          const double wj = F->pw[r];
+          int slot = -1;
+          for (int k = C->rp[I]; k < C->rp[I + 1]; ++k)
+            if (C->ci[k] == J) { slot = k; break; }
+          if (slot < 0) slot = C->rp[I];   /* diagonal first */
+          double *T = ac + (size_t)slot * bb;
+          for (int e = 0; e < bb; ++e) T[e] += (M[e] * wi) * wj;
+        }
+      }
+    }
+}

@@ -126,6 +126,12 @@ int orc_bcgs_solve(const orc_level *levels, int top, const orc_params *p, int us
                    const double *reduction, int maxiter, int restart, double *defects,
                    int maxdef, int *ndef, int *nit, int *converged);
 
+/* AssembleGalerkinByMatrix (np/algebra/transgrid.cc:1575-1700): coarse
+ * values A_c = P^T A_f P (scalar interpolation weights, blocks
+ * componentwise) in C's pattern, missing connections lumped to the
+ * diagonal; ac has nnzb_C * b * b doubles. */
+void orc_galerkin(const orc_level *F, const orc_level *C, double *ac);
+
 #ifdef __cplusplus
 }
 #endif

@@ -399,3 +399,51 @@ def test_poisson_convergence_rate_is_grid_independent():
         d = r["defects"][:, 0]
         rates.append((d[-1] / d[0]) ** (1.0 / r["nit"]))
     assert max(rates) < 0.12 and max(rates) - min(rates) < 0.03, rates
+
+
+# ------------------------------------------------------------ Galerkin RAP
+
+@pytest.mark.parametrize("which", ["q1_3d_jump", "p1_cd", "elasticity"])
+def test_galerkin_is_ptap_in_the_coarse_pattern(which):
+    """AssembleGalerkinByMatrix restated = dense P^T A P on the coarse
+    pattern, with products outside it lumped into the diagonal."""
+    if which == "q1_3d_jump":
+        P = problems.q1(3, 1, 3, problems.KAPPA_JUMP, seed=5)
+    elif which == "p1_cd":
+        P = problems.p1_cd(3, 1e-2, 1.0, seed=5)
+    else:
+        P = problems.elasticity(2, seed=5)
+    H = O.Hierarchy(P.levels)
+    for l in range(1, P.top + 1):
+        f, c = P.levels[l], P.levels[l - 1]
+        b = f.b
+        Pb = np.kron(f.dense_p(c.n), np.eye(b))
+        G = Pb.T @ f.dense() @ Pb
+        got = O.galerkin(H, l).reshape(-1, b, b)
+        mask = np.zeros((c.n, c.n), dtype=bool)
+        for v in range(c.n):
+            mask[v, c.col_idx[c.row_ptr[v]:c.row_ptr[v + 1]]] = True
+        want = np.zeros_like(got)
+        for v in range(c.n):
+            lump = np.zeros((b, b))
+            for w in range(c.n):
+                blk = G[v * b:(v + 1) * b, w * b:(w + 1) * b]
+                if not mask[v, w]:
+                    lump += blk
+            for k in range(c.row_ptr[v], c.row_ptr[v + 1]):
+                w = c.col_idx[k]
+                want[k] = G[v * b:(v + 1) * b, w * b:(w + 1) * b] + (lump if w == v else 0.0)
+        assert np.max(np.abs(got - want)) < 1e-13 * np.max(np.abs(want)), (which, l)
+
+
+def test_galerkin_matches_exact_coarse_operator_inside():
+    """On the Q1 jump problem the generator's coarse matrices integrate the
+    fine coefficient exactly, so RAP reproduces them away from Dirichlet rows."""
+    P = problems.q1(3, 1, 3, problems.KAPPA_JUMP, seed=6)
+    H = O.Hierarchy(P.levels)
+    for l in range(2, P.top + 1):
+        c = P.levels[l - 1]
+        got = O.galerkin(H, l)
+        rows = np.repeat(np.arange(c.n), np.diff(c.row_ptr))
+        inner = (c.skip[rows] == 0) & (c.skip[c.col_idx] == 0)
+        assert np.max(np.abs(got[inner] - c.vals[inner])) < 1e-12 * np.max(np.abs(c.vals))

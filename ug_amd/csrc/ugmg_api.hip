@@ -47,8 +47,8 @@ struct DevSellBuf {
     for (int s = s0; s < s1; ++s) m = std::max(m, width_h[s]);
     return m;
   }
-  DSell<double> d64() const { return {nslices, soff, rows, col, v64, sdoff, dtab, wmax}; }
-  DSell<float> d32() const { return {nslices, soff, rows, col, v32, sdoff, dtab, wmax}; }
+  DSell<double> d64() const { return {nslices, soff, rows, col, v64, sdoff, dtab, wmax, nent}; }
+  DSell<float> d32() const { return {nslices, soff, rows, col, v32, sdoff, dtab, wmax, nent}; }
   template <typename T>
   DSell<T> d() const {
     if constexpr (std::is_same_v<T, double>) return d64();
@@ -69,7 +69,8 @@ struct DevLevel {
   int p_nnz = 0, pt_nnz = 0;
   DevSellBuf P, PT;   // interpolation from / restriction to level-1
   bool nt = false;    // stream the matrix with non-temporal loads
-  size_t vs = 0;      // column stride of the level vectors (batched launches)
+  int nb = 1;         // components per vector (block size)
+  size_t vs = 0;      // column stride of the level vectors (scalars; batched launches)
   // device order (GS levels): perm_h[p] = list index of device position p;
   // empty / null: list order
   std::vector<int> perm_h;
@@ -234,22 +235,22 @@ void drop_graph(ugmg_ctx *ctx) {
   drop_bcgs_graphs(ctx);
 }
 
-inline Batch batch(const ugmg_ctx *ctx, int l) { return Batch{ctx->nb, ctx->dev[l].vs}; }
-inline Batch B1(const DevLevel &d) { return Batch{1, d.vs}; }
+inline Batch batch(const ugmg_ctx *ctx, int l) { return Batch{ctx->nb, ctx->dev[l].vs, ctx->dev[l].nb}; }
+inline Batch B1(const DevLevel &d) { return Batch{1, d.vs, d.nb}; }
 
 // the caller's vectors (list order, column stride ld) to the level's device
 // order and back; a plain (converting) copy on levels kept in list order
 template <typename TD, typename TS>
 void to_dev(ugmg_ctx *ctx, int l, const TS *src, size_t lds, TD *dst, int nrhs = 1) {
   const DevLevel &d = ctx->dev[l];
-  if (d.perm) launch_gather<TD, TS>(ctx->stream, d.n, d.perm, src, dst, nrhs, lds, d.vs);
-  else launch_convert<TD, TS>(ctx->stream, d.n, src, dst, nrhs, lds, d.vs);
+  if (d.perm) launch_gather<TD, TS>(ctx->stream, d.n, d.perm, src, dst, nrhs, lds, d.vs, d.nb);
+  else launch_convert<TD, TS>(ctx->stream, d.n * d.nb, src, dst, nrhs, lds, d.vs);
 }
 template <typename TD, typename TS>
 void to_user(ugmg_ctx *ctx, int l, const TS *src, TD *dst, size_t ldd, int nrhs = 1) {
   const DevLevel &d = ctx->dev[l];
-  if (d.perm) launch_scatter<TD, TS>(ctx->stream, d.n, d.perm, src, dst, nrhs, d.vs, ldd);
-  else launch_convert<TD, TS>(ctx->stream, d.n, src, dst, nrhs, d.vs, ldd);
+  if (d.perm) launch_scatter<TD, TS>(ctx->stream, d.n, d.perm, src, dst, nrhs, d.vs, ldd, d.nb);
+  else launch_convert<TD, TS>(ctx->stream, d.n * d.nb, src, dst, nrhs, d.vs, ldd);
 }
 
 // ------------------------------------------------------------ cycle drivers
@@ -350,7 +351,7 @@ void base_solve(ugmg_ctx *ctx, T *c, T *b, int cmode, bool have_parts, bool need
   if (m == 0 && cmode == 0) launch_fill<T>(st, d.n, c, (T)0, bt);
   for (int it = 0; it < m; ++it) {
     if (!ctx->base.failed)
-      launch_base_gemv<T>(st, d.n, ctx->base_inv, b, c, x, it == 0 ? cmode : 1, p.base_damp[0],
+      launch_base_gemv<T>(st, d.n * d.nb, ctx->base_inv, b, c, x, it == 0 ? cmode : 1, p.base_damp[0],
                           p.base_abslimit[0], p.base_red[0], ctx->base_part, np,
                           ctx->base_first, it, bt);
     const bool more = it + 1 < m;
@@ -383,7 +384,7 @@ void lmgc(ugmg_ctx *ctx, int l, T *c, T *b, int cmode) {
   const bool to_base = l - 1 == ctx->bl && (ctx->fuse_top < ctx->bl || nb > 1);
   const int np = launch_restrict<T>(st, d.PT.d<T>(), dc.skip, dc.vnclass, dc.vclass, b, (T)1,
                                     dc.bv<T>(), to_base ? ctx->base_part : nullptr, d.nt, nb,
-                                    d.vs, dc.vs);
+                                    d.vs, dc.vs, d.nb);
   if (p.gamma <= 0) launch_fill<T>(st, dc.n, dc.cv<T>(), (T)0, batch(ctx, l - 1));
   for (int g = 0; g < p.gamma; ++g) {
     if (to_base) {
@@ -397,7 +398,7 @@ void lmgc(ugmg_ctx *ctx, int l, T *c, T *b, int cmode) {
   }
   // InterpolateCorrection(level, t, c, damp); dadd(c, t); dmatmul_minus(b, A, t)
   launch_interp<T>(st, d.P.d<T>(), d.skip, dc.cv<T>(), (T)p.lmgc_damp[0], t, c, d.nt, nb, d.vs,
-                   dc.vs);
+                   dc.vs, d.nb);
   launch_spmv<T, T, T>(st, d.A.d<T>(), 0, d.A.nslices, t, b, (T *)nullptr, 0, d.nt,
                        batch(ctx, l));
   smooth<T>(ctx, l, t, b, c, 1, p.nu2);
@@ -413,10 +414,10 @@ int check_ready(ugmg_ctx *ctx, int level) {
 
 int surface_norm(ugmg_ctx *ctx, int level, const double *x, double *out) {
   DevLevel &d = ctx->dev[level];
-  launch_dot<double>(ctx->stream, d.n, x, nullptr, d.vclass, ctx->part, ctx->red, 1, Batch{1, d.vs});
-  HIPCHK(hipMemcpyAsync(ctx->red_host, ctx->red, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  launch_dot<double>(ctx->stream, d.n, x, nullptr, d.vclass, ctx->part, ctx->red, 1, B1(d));
+  HIPCHK(hipMemcpyAsync(ctx->red_host, ctx->red, sizeof(double) * d.nb, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
-  *out = ctx->red_host[0];
+  for (int i = 0; i < d.nb; ++i) out[i] = ctx->red_host[i];
   return 0;
 }
 
@@ -429,12 +430,11 @@ void enqueue_iteration(ugmg_ctx *ctx, int level, double *x, double *b) {
     for (int j = 0; j < k; ++j) lmgc<float>(ctx, level, d.c32, d.b32, j == 0 ? 0 : 1);
     int np = 0;
     launch_ir_update(ctx->stream, d.A.d64(), d.c32, b, x, d.b32, d.vclass, ctx->part, &np, d.nt,
-                     Batch{1, d.vs});
-    launch_reduce_final(ctx->stream, np, ctx->part, ctx->red, 1, 1);
+                     B1(d));
+    launch_reduce_final(ctx->stream, np, ctx->part, ctx->red, 1, d.nb);
   } else {
     lmgc<double>(ctx, level, x, b, 1);
-    launch_dot<double>(ctx->stream, d.n, b, nullptr, d.vclass, ctx->part, ctx->red, 1,
-                       Batch{1, d.vs});
+    launch_dot<double>(ctx->stream, d.n, b, nullptr, d.vclass, ctx->part, ctx->red, 1, B1(d));
   }
 }
 
@@ -468,7 +468,8 @@ int launch_iteration(ugmg_ctx *ctx, int level, double *x, double *b) {
     }
     HIPCHK(hipGraphLaunch(ctx->gexec, ctx->stream));
   }
-  HIPCHK(hipMemcpyAsync(ctx->red_host, ctx->red, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(ctx->red_host, ctx->red, sizeof(double) * ctx->b, hipMemcpyDeviceToHost,
+                        ctx->stream));
   return 0;
 }
 
@@ -631,8 +632,8 @@ int prep_host(ugmg_ctx *ctx, int level, PrepState &ps) {
     if (l > bl && !ctx->transfer_set[l])
       return fail(ctx, UGMG_E_STATE, "transfer of level " + std::to_string(l) + " not set");
   }
-  if (ctx->host[bl].n > 8192)
-    return fail(ctx, UGMG_E_ARG, "base level has more than 8192 vectors; choose a coarser baselevel");
+  if ((long long)ctx->host[bl].n * ctx->b > 8192)
+    return fail(ctx, UGMG_E_ARG, "base level has more than 8192 unknowns; choose a coarser baselevel");
   const char *env_nt = std::getenv("UGMG_NT_BYTES");
   ps.nt_bytes = env_nt ? std::atof(env_nt) : 64.0e6;
   const char *env_gsf = std::getenv("UGMG_GS_FUSE");
@@ -699,6 +700,7 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
         if ((rc = upload(ctx, &D.perm, hp.perm.data(), hp.perm.size()))) return rc;
       }
       D.n = H.n;
+      D.nb = ctx->b;
       D.nnzb = H.nnzb;
       max_slices = std::max(max_slices, hp.A.nslices);
       if ((rc = upload_sell(ctx, hp.A, D.A))) return rc;
@@ -729,7 +731,7 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
       }
       D.nt = (double)hp.A.col.size() * 12.0 > ps.nt_bytes;
       // columns padded to 64 entries (512-byte aligned FP64 columns)
-      D.vs = ((size_t)std::max(D.n, 1) + 63) / 64 * 64;
+      D.vs = ((size_t)std::max(D.n, 1) * D.nb + 63) / 64 * 64;
       const size_t nv = D.vs * (size_t)ctx->nrhs_cap;
       if ((rc = dalloc(ctx, &D.c, nv)) || (rc = dalloc(ctx, &D.b, nv)) ||
           (rc = dalloc(ctx, &D.t, nv)) || (rc = dalloc(ctx, &D.c32, nv)) ||
@@ -745,7 +747,7 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
     // fused coarse levels (opt-in): the base level and every level above it
     // up to fuse_rows vectors (and at most kMaxColors colours)
     ctx->fuse_top = -1;
-    if (ps.fuse_rows > 0 && ctx->dev[bl].n <= 2048) {
+    if (ps.fuse_rows > 0 && ctx->dev[bl].n <= 2048 && ctx->b == 1) {
       ctx->fuse_top = bl;
       for (int l = bl + 1; l <= level; ++l) {
         if (ctx->dev[l].n > ps.fuse_rows || ctx->dev[l].ncolors > kMaxColors) break;
@@ -784,7 +786,7 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
     if ((rc = upload(ctx, &ctx->clev64, c64.data(), c64.size()))) return rc;
     if ((rc = upload(ctx, &ctx->clev32, c32.data(), c32.size()))) return rc;
     ctx->part_cap = std::max(1024, (max_slices + 3) / 4 + 1);
-    const int nc = ctx->nrhs_cap;
+    const int nc = ctx->nrhs_cap * ctx->b;   // partial-sum columns: rhs x component
     if ((rc = dalloc(ctx, &ctx->part, (size_t)ctx->part_cap * nc))) return rc;
     if ((rc = dalloc(ctx, &ctx->red, std::max(8, nc)))) return rc;
     // restriction partials into the base level: one per block of 4 slices
@@ -883,7 +885,8 @@ int ugmg_sync(ugmg_ctx *ctx) {
 
 int ugmg_set_nlevels(ugmg_ctx *ctx, int nlevels, int b) {
   if (!ctx || nlevels < 1 || nlevels > 32) return ctx ? fail(ctx, UGMG_E_ARG, "nlevels must be 1..32") : UGMG_E_ARG;
-  if (b != 1) return fail(ctx, UGMG_E_ARG, "only scalar (b=1) hierarchies are supported on the GPU");
+  if (b != 1 && b != 3)
+    return fail(ctx, UGMG_E_ARG, "block size must be 1 (scalar) or 3 (3x3 blocks)");
   free_device(ctx);
   ctx->nlevels = nlevels;
   ctx->b = b;
@@ -914,7 +917,8 @@ int ugmg_set_level(ugmg_ctx *ctx, int level, int n, const int *row_ptr, const in
       for (int m = L.rp[i]; m < L.rp[i + 1]; ++m)
         if (L.ci[m] < 0 || L.ci[m] >= n) return fail(ctx, UGMG_E_ARG, "column index out of range");
     }
-    L.a.assign(vals, vals + (size_t)L.nnzb);
+    L.b = ctx->b;
+    L.a.assign(vals, vals + (size_t)L.nnzb * ctx->b * ctx->b);
     L.skip.assign(skip, skip + n);
     L.vc.assign(vclass, vclass + n);
     L.vnc.assign(vnclass, vnclass + n);
@@ -960,6 +964,13 @@ int ugmg_set_params(ugmg_ctx *ctx, const ugmg_params *p) {
       (p->smoother != UGMG_SMOOTH_JAC && p->smoother != UGMG_SMOOTH_GS) ||
       (p->precision != UGMG_FP64 && p->precision != UGMG_MIXED) || p->inner_cycles < 0)
     return fail(ctx, UGMG_E_ARG, "invalid parameter");
+  if (ctx->b > 1)
+    for (int i = 1; i < ctx->b; ++i)
+      if (p->damp[i] != p->damp[0] || p->lmgc_damp[i] != p->lmgc_damp[0] ||
+          p->base_damp[i] != p->base_damp[0] || p->base_abslimit[i] != p->base_abslimit[0] ||
+          p->base_red[i] != p->base_red[0])
+        return fail(ctx, UGMG_E_ARG,
+                    "block hierarchies take one damping / base limit for all components");
   const bool relayout = !ctx->params_set || p->smoother != ctx->p.smoother ||
                         p->baselevel != ctx->p.baselevel || p->ex_reorder != ctx->p.ex_reorder;
   ctx->p = *p;
@@ -1011,8 +1022,9 @@ int ugmg_get_pattern(ugmg_ctx *ctx, int level, int *row_ptr, int *col_idx, doubl
   if (rc) return rc;
   const DevLevel &d = ctx->dev[level];
   const int ns = d.A.nslices;
+  const int bb = d.nb * d.nb;
   std::vector<int> soff(ns + 1), rows((size_t)ns * kSlice), col(d.A.nent), sdoff(std::max(ns, 1));
-  std::vector<double> val(d.A.nent);
+  std::vector<double> val(d.A.nent * bb);
   HIPCHK(hipMemcpy(soff.data(), d.A.soff, soff.size() * sizeof(int), hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(rows.data(), d.A.rows, rows.size() * sizeof(int), hipMemcpyDeviceToHost));
   if (ns) HIPCHK(hipMemcpy(sdoff.data(), d.A.sdoff, ns * sizeof(int), hipMemcpyDeviceToHost));
@@ -1038,7 +1050,9 @@ int ugmg_get_pattern(ugmg_ctx *ctx, int level, int *row_ptr, int *col_idx, doubl
         const size_t e = (size_t)soff[s] + (size_t)k * kSlice + lane;
         if (sdoff[s] >= 0 && col[e] < 0) continue;
         col_idx[H.rp[r] + q] = pm ? d.perm_h[col[e]] : col[e];
-        if (vals) vals[H.rp[r] + q] = val[e];
+        if (vals)
+          for (int k2 = 0; k2 < bb; ++k2)
+            vals[(size_t)(H.rp[r] + q) * bb + k2] = val[(size_t)k2 * d.A.nent + e];
         ++q;
       }
       row_ptr[r + 1] = q;
@@ -1104,9 +1118,9 @@ int ugmg_dnrm2x(ugmg_ctx *ctx, int level, const double *x, double *out) {
   if (!out) return fail(ctx, UGMG_E_ARG, "null output");
   DevLevel &d = ctx->dev[level];
   launch_dot<double>(ctx->stream, d.n, x, nullptr, nullptr, ctx->part, ctx->red, 1, B1(d));
-  HIPCHK(hipMemcpyAsync(ctx->red_host, ctx->red, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(ctx->red_host, ctx->red, sizeof(double) * d.nb, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
-  out[0] = ctx->red_host[0];
+  for (int i = 0; i < d.nb; ++i) out[i] = ctx->red_host[i];
   return UGMG_OK;
 }
 
@@ -1116,9 +1130,9 @@ int ugmg_ddot(ugmg_ctx *ctx, int level, const double *x, const double *y, double
   if (!out || !y) return fail(ctx, UGMG_E_ARG, "null argument");
   DevLevel &d = ctx->dev[level];
   launch_dot<double>(ctx->stream, d.n, x, y, nullptr, ctx->part, ctx->red, 0, B1(d));
-  HIPCHK(hipMemcpyAsync(ctx->red_host, ctx->red, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(ctx->red_host, ctx->red, sizeof(double) * d.nb, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
-  out[0] = ctx->red_host[0];
+  for (int i = 0; i < d.nb; ++i) out[i] = ctx->red_host[i];
   return UGMG_OK;
 }
 
@@ -1218,10 +1232,10 @@ int ugmg_iter(ugmg_ctx *ctx, int level, double *c, double *b) {
     const int k = std::max(1, ctx->p.inner_cycles);
     for (int j = 0; j < k; ++j) lmgc<float>(ctx, level, d.c32, d.b32, j == 0 ? 0 : 1);
     if (!d.perm) {
-      launch_convert<double, float>(ctx->stream, d.n, d.c32, c, 1, d.vs, d.vs);
+      launch_convert<double, float>(ctx->stream, d.n * d.nb, d.c32, c, 1, d.vs, d.vs);
       launch_spmv<double, double, double>(ctx->stream, d.A.d64(), 0, d.A.nslices, c, b, nullptr, 0, d.nt, B1(d));
     } else {
-      launch_convert<double, float>(ctx->stream, d.n, d.c32, d.c, 1, d.vs, d.vs);
+      launch_convert<double, float>(ctx->stream, d.n * d.nb, d.c32, d.c, 1, d.vs, d.vs);
       to_dev<double, double>(ctx, level, b, d.vs, d.b);
       launch_spmv<double, double, double>(ctx->stream, d.A.d64(), 0, d.A.nslices, d.c, d.b, nullptr, 0, d.nt, B1(d));
       to_user<double, double>(ctx, level, d.c, c, d.vs);
@@ -1242,7 +1256,7 @@ int ugmg_iter(ugmg_ctx *ctx, int level, double *c, double *b) {
 int ugmg_iter_host(ugmg_ctx *ctx, int level, double *c, double *b) {
   int rc = check_ready(ctx, level);
   if (rc) return rc;
-  const size_t bytes = sizeof(double) * ctx->dev[level].n;
+  const size_t bytes = sizeof(double) * ctx->dev[level].n * ctx->b;
   double *dc = nullptr, *db = nullptr;
   HIPCHK(hipMalloc(&dc, bytes));
   if (hipMalloc(&db, bytes) != hipSuccess) {
@@ -1292,14 +1306,24 @@ int ugmg_solve(ugmg_ctx *ctx, int level, double *x, double *b, const double *abs
   };
   if (do_defect)  // LinearDefect: b -= A x
     launch_spmv<double, double, double>(ctx->stream, d.A.d64(), 0, d.A.nslices, x, b, nullptr, 0, d.nt, B1(d));
-  double first = 0.0;
-  if ((rc = surface_norm(ctx, level, b, &first))) return rc;
-  res->first_defect[0] = res->last_defect[0] = first;
-  double target = first * reduction[0];  // sc_mul_check
-  if (target == 0.0) target = reduction[0];
-  if (cb) cb(user, 0, 1, &first);
-  int conv = sc_cmp1(first, abslimit[0]);
-  if (ctx->p.precision == UGMG_MIXED) launch_convert<float, double>(ctx->stream, d.n, b, d.b32, 1, d.vs, d.vs);
+  const int nc = d.nb;
+  // sc_cmp (npscan.cc:1027-1057): every component below its limit
+  auto sc_cmp = [nc](const double *x, const double *y) {
+    for (int i = 0; i < nc; ++i)
+      if (!sc_cmp1(x[i], y[i])) return false;
+    return true;
+  };
+  double first[UGMG_MAXB] = {0.0, 0.0, 0.0}, target[UGMG_MAXB], last[UGMG_MAXB];
+  if ((rc = surface_norm(ctx, level, b, first))) return rc;
+  for (int i = 0; i < nc; ++i) {
+    res->first_defect[i] = res->last_defect[i] = first[i];
+    target[i] = first[i] * reduction[i];  // sc_mul_check
+    if (target[i] == 0.0) target[i] = reduction[i];
+  }
+  if (cb) cb(user, 0, nc, first);
+  int conv = sc_cmp(first, abslimit);
+  if (ctx->p.precision == UGMG_MIXED)
+    launch_convert<float, double>(ctx->stream, d.n * d.nb, b, d.b32, 1, d.vs, d.vs);
   int i;
   for (i = 0; i < maxiter; ++i) {
     if (conv) break;
@@ -1308,10 +1332,9 @@ int ugmg_solve(ugmg_ctx *ctx, int level, double *x, double *b, const double *abs
       move_out();
       return rc;
     }
-    const double last = ctx->red_host[0];
-    res->last_defect[0] = last;
-    if (cb) cb(user, i + 1, 1, &last);
-    if (sc_cmp1(last, abslimit[0]) || sc_cmp1(last, target)) {
+    for (int k = 0; k < nc; ++k) res->last_defect[k] = last[k] = ctx->red_host[k];
+    if (cb) cb(user, i + 1, nc, last);
+    if (sc_cmp(last, abslimit) || sc_cmp(last, target)) {
       conv = 1;
       res->number_of_linear_iterations = i + 1;
       break;
@@ -1327,7 +1350,7 @@ int ugmg_solve_host(ugmg_ctx *ctx, int level, double *x, double *b, const double
                     ugmg_defect_cb cb, void *user) {
   int rc = check_ready(ctx, level);
   if (rc) return rc;
-  const size_t bytes = sizeof(double) * ctx->dev[level].n;
+  const size_t bytes = sizeof(double) * ctx->dev[level].n * ctx->b;
   double *dx = nullptr, *db = nullptr;
   HIPCHK(hipMalloc(&dx, bytes));
   if (hipMalloc(&db, bytes) != hipSuccess) {
@@ -1364,7 +1387,7 @@ int ugmg_iter_batch(ugmg_ctx *ctx, int level, int nrhs, double *c, double *b, in
   if (rc) return rc;
   if (nrhs < 1 || nrhs > ctx->nrhs_cap)
     return fail(ctx, UGMG_E_ARG, "nrhs exceeds the batch the hierarchy was laid out for (ugmg_set_batch)");
-  if (!c || !b || ld < ctx->dev[level].n) return fail(ctx, UGMG_E_ARG, "null vector or ld < n");
+  if (!c || !b || ld < ctx->dev[level].n * ctx->b) return fail(ctx, UGMG_E_ARG, "null vector or ld < n*b");
   if (ctx->base.failed) return fail(ctx, UGMG_E_SINGULAR, "zero pivot in the band LU of the base level");
   DevLevel &d = ctx->dev[level];
   hipStream_t st = ctx->stream;
@@ -1374,7 +1397,7 @@ int ugmg_iter_batch(ugmg_ctx *ctx, int level, int nrhs, double *c, double *b, in
     to_dev<float, double>(ctx, level, b, ld, d.b32, nrhs);
     const int k = std::max(1, ctx->p.inner_cycles);
     for (int j = 0; j < k; ++j) lmgc<float>(ctx, level, d.c32, d.b32, j == 0 ? 0 : 1);
-    launch_convert<double, float>(st, d.n, d.c32, d.c, nrhs, d.vs, d.vs);
+    launch_convert<double, float>(st, d.n * d.nb, d.c32, d.c, nrhs, d.vs, d.vs);
     to_dev<double, double>(ctx, level, b, ld, d.b, nrhs);
     launch_spmv<double, double, double>(st, d.A.d64(), 0, d.A.nslices, d.c, d.b, nullptr, 0, d.nt, bt);
   } else {
@@ -1398,6 +1421,7 @@ int ugmg_bcgs_solve(ugmg_ctx *ctx, int level, int nrhs, double *x, double *b, in
   if (!x || !b || !abslimit || !reduction || !res || maxiter < 0 || restart < 0 ||
       ld < ctx->dev[level].n || (defects && maxdef < 1))
     return fail(ctx, UGMG_E_ARG, "bad argument");
+  if (ctx->b != 1) return fail(ctx, UGMG_E_ARG, "bcgs runs on scalar (b = 1) hierarchies");
   std::memset(res, 0, sizeof(*res) * nrhs);
   if (use_iter && ctx->base.failed) {
     for (int j = 0; j < nrhs; ++j) res[j].error_code = UGMG_E_SINGULAR;
@@ -1463,7 +1487,7 @@ int ugmg_bcgs_solve_host(ugmg_ctx *ctx, int level, int nrhs, double *x, double *
                          double *defects, int maxdef) {
   int rc = check_ready(ctx, level);
   if (rc) return rc;
-  if (nrhs < 1 || ld < ctx->dev[level].n || !x || !b) return fail(ctx, UGMG_E_ARG, "bad argument");
+  if (nrhs < 1 || ld < ctx->dev[level].n * ctx->b || !x || !b) return fail(ctx, UGMG_E_ARG, "bad argument");
   const size_t bytes = sizeof(double) * (size_t)ld * nrhs;
   double *dx = nullptr, *db = nullptr;
   HIPCHK(hipMalloc(&dx, bytes));
@@ -1484,6 +1508,67 @@ int ugmg_bcgs_solve_host(ugmg_ctx *ctx, int level, int nrhs, double *x, double *
   hipFree(dx);
   hipFree(db);
   return rc;
+}
+
+int ugmg_galerkin(ugmg_ctx *ctx, int level) {
+  if (!ctx) return UGMG_E_ARG;
+  if (level < 1 || level >= ctx->nlevels) return fail(ctx, UGMG_E_ARG, "level must be 1..nlevels-1");
+  if (!ctx->level_set[level] || !ctx->level_set[level - 1] || !ctx->transfer_set[level])
+    return fail(ctx, UGMG_E_STATE, "set both levels and the transfer first");
+  HostLevel &F = ctx->host[level], &Cl = ctx->host[level - 1];
+  const int bb = ctx->b * ctx->b;
+  // P^T over every fine vector (the coarse matrix takes all of them)
+  std::vector<int> ptp(Cl.n + 1, 0), pti, fillp;
+  std::vector<double> ptw;
+  try {
+    for (int q = 0; q < F.pp[F.n]; ++q) ++ptp[F.pi[q] + 1];
+    for (int i = 0; i < Cl.n; ++i) ptp[i + 1] += ptp[i];
+    pti.resize(ptp[Cl.n]);
+    ptw.resize(ptp[Cl.n]);
+    fillp.assign(ptp.begin(), ptp.end() - 1);
+    for (int v = 0; v < F.n; ++v)
+      for (int q = F.pp[v]; q < F.pp[v + 1]; ++q) {
+        const int I = F.pi[q];
+        pti[fillp[I]] = v;
+        ptw[fillp[I]] = F.pw[q];
+        ++fillp[I];
+      }
+  } catch (const std::bad_alloc &) {
+    return fail(ctx, UGMG_E_NOMEM, "host allocation");
+  }
+  HIPCHK(hipSetDevice(ctx->device));
+  if (ctx->stream) HIPCHK(hipStreamSynchronize(ctx->stream));
+  free_device(ctx);   // the coarse values change: layouts are rebuilt by ugmg_preprocess
+  std::vector<void *> tmp;
+  auto up = [&](const void *src, size_t bytes, void **dst) -> int {
+    *dst = nullptr;
+    if (hipMalloc(dst, bytes ? bytes : 1) != hipSuccess) return UGMG_E_NOMEM;
+    tmp.push_back(*dst);
+    if (bytes && src && hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) != hipSuccess) return UGMG_E_HIP;
+    return 0;
+  };
+  void *d_ptp, *d_pti, *d_ptw, *d_frp, *d_fci, *d_fa, *d_pp, *d_pi, *d_pw, *d_crp, *d_cci, *d_ca;
+  int rc = 0;
+  const size_t nzc = (size_t)Cl.nnzb;
+  if ((rc = up(ptp.data(), ptp.size() * 4, &d_ptp)) || (rc = up(pti.data(), pti.size() * 4, &d_pti)) ||
+      (rc = up(ptw.data(), ptw.size() * 8, &d_ptw)) || (rc = up(F.rp.data(), F.rp.size() * 4, &d_frp)) ||
+      (rc = up(F.ci.data(), F.ci.size() * 4, &d_fci)) || (rc = up(F.a.data(), F.a.size() * 8, &d_fa)) ||
+      (rc = up(F.pp.data(), F.pp.size() * 4, &d_pp)) || (rc = up(F.pi.data(), F.pi.size() * 4, &d_pi)) ||
+      (rc = up(F.pw.data(), F.pw.size() * 8, &d_pw)) || (rc = up(Cl.rp.data(), Cl.rp.size() * 4, &d_crp)) ||
+      (rc = up(Cl.ci.data(), Cl.ci.size() * 4, &d_cci)) || (rc = up(nullptr, nzc * bb * 8, &d_ca))) {
+    for (void *p : tmp) hipFree(p);
+    return fail(ctx, rc, "device buffers for the Galerkin product");
+  }
+  hipStream_t st = ctx->stream;
+  launch_galerkin(st, Cl.n, bb, (int *)d_ptp, (int *)d_pti, (double *)d_ptw, (int *)d_frp,
+                  (int *)d_fci, (double *)d_fa, (int *)d_pp, (int *)d_pi, (double *)d_pw,
+                  (int *)d_crp, (int *)d_cci, (double *)d_ca);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(Cl.a.data(), d_ca, nzc * bb * 8, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  for (void *p : tmp) hipFree(p);
+  if (e != hipSuccess) return fail(ctx, UGMG_E_HIP, std::string("Galerkin product: ") + hipGetErrorString(e));
+  return UGMG_OK;
 }
 
 int ugmg_color(int n, const int *row_ptr, const int *col_idx, int *colors) {

@@ -12,6 +12,7 @@ namespace ugmg {
 // Host copy of one level as it crosses the C ABI (ConvertMatrix layout).
 struct HostLevel {
   int n = 0, nnzb = 0;
+  int b = 1;   // components per vector; a holds b*b values per entry
   std::vector<int> rp, ci;
   std::vector<double> a;
   std::vector<uint32_t> skip;
@@ -30,7 +31,9 @@ struct HostSell {
   std::vector<int> soff;  // nslices + 1, entry offsets (multiples of 64)
   std::vector<int> rows;  // nslices * 64
   std::vector<int> col;
+  // bb planes of soff[nslices] values: component q of entry e at q*nent + e
   std::vector<double> val;
+  int bb = 1;
   bool identity_rows = false;
   // uniform-delta slices: every row of the slice has the column offsets
   // dtab[sdoff[s] + k] (k < width), entries missing in a row are stored as
@@ -48,7 +51,7 @@ struct HostPrep {
   int window = 0;
   std::vector<int> color_ug;   // colouring in list order
   HostSell A;                // all rows, list order
-  std::vector<double> diag;  // diagonal entry of every row
+  std::vector<double> diag;  // diagonal block (b*b, row-major) of every row
   // multicolour Gauss-Seidel
   int ncolors = 0;
   std::vector<int> color;          // per vector
@@ -64,11 +67,11 @@ struct HostPrep {
 
 // Base level: band LU as in EXPreProcess, then the explicit inverse.
 struct HostBase {
-  int n = 0;
+  int n = 0;                   // scalar unknowns (vectors * b)
   int bw = 0;
   bool failed = false;
   std::vector<int> order;      // BFS list order
-  std::vector<double> inv;     // n x n row-major, natural vector order
+  std::vector<double> inv;     // n x n row-major, list order (vector-major)
 };
 
 // filters for build_sell

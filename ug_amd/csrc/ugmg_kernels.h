@@ -24,6 +24,7 @@ struct DSell {
   const int *sdoff;   // per slice: offset into dtab (uniform slice) or -1
   const int *dtab;    // shared column-offset lists of uniform slices
   int wmax;           // widest slice (entries per row)
+  size_t pst;         // entries per value plane (b x b blocks: b*b planes)
 };
 
 // base-level solver settings: ls($m,$red,$abslimit) over ex($damp)
@@ -70,9 +71,12 @@ struct CoarseArgs {
 // Batched launches: every vector argument holds `nrhs` columns, column j at
 // offset j * vs (vs = the level's column stride, >= n); the column index of a
 // workgroup is its second grid coordinate.  nrhs = 1 is the plain case.
+// b = components per vector (1, or 3 with 3x3 blocks): vector v's
+// components are consecutive (UG's layout), column strides count scalars.
 struct Batch {
   int nrhs;
   size_t vs;
+  int b = 1;
 };
 
 // per right-hand side state of the batched BCGSSolver (np/procs/ls.cc:
@@ -117,17 +121,19 @@ void launch_gs_defect(hipStream_t st, const DSell<TV> &U, const T *v, T *b, cons
 template <typename T>
 int launch_restrict(hipStream_t st, const DSell<T> &PT, const uint32_t *cskip,
                     const uint8_t *cvnclass, const uint8_t *cvclass, const T *bf, T damp, T *bc,
-                    double *part, bool nt, int nrhs, size_t vsf, size_t vsc);
+                    double *part, bool nt, int nrhs, size_t vsf, size_t vsc, int b = 1);
 template <typename T>
 void launch_base_gemv(hipStream_t st, int n, const double *inv, const T *b, T *c, T *xout,
                       int cmode, double damp, double abslimit, double red, const double *part,
                       int nparts, double *first, int it, Batch bt);
+// columns of the partial sums: one per component of every right-hand side
+// (column j * b + i), nparts each
 template <typename T>
 int launch_dot_part(hipStream_t st, int n, const T *x, const uint8_t *vclass, double *part,
                     Batch bt);
 template <typename T>
 void launch_interp(hipStream_t st, const DSell<T> &P, const uint32_t *fskip, const T *cc, T damp,
-                   T *t, T *c, bool nt, int nrhs, size_t vsf, size_t vsc);
+                   T *t, T *c, bool nt, int nrhs, size_t vsf, size_t vsc, int b = 1);
 int reduce_parts(int n);
 template <typename T>
 void launch_dot(hipStream_t st, int n, const T *x, const T *y, const uint8_t *vclass,
@@ -145,12 +151,18 @@ void launch_add(hipStream_t st, int n, T *x, const T *y, Batch bt);
 // scatter d[perm[p]] = s[p]; column strides vss (source), vsd (destination)
 template <typename TD, typename TS>
 void launch_gather(hipStream_t st, int n, const int *perm, const TS *s, TD *d, int nrhs,
-                   size_t vss, size_t vsd);
+                   size_t vss, size_t vsd, int b = 1);
 template <typename TD, typename TS>
 void launch_scatter(hipStream_t st, int n, const int *perm, const TS *s, TD *d, int nrhs,
-                    size_t vss, size_t vsd);
+                    size_t vss, size_t vsd, int b = 1);
 template <typename T>
 void launch_coarse_cycle(hipStream_t st, const CoarseArgs<T> &args, int base_n);
+// coarse values ca (pattern crp/cci, bb values per entry) = P^T A P; pt*:
+// every fine vector that interpolates from coarse row I, in list order
+void launch_galerkin(hipStream_t st, int nc, int bb, const int *ptp, const int *pti,
+                     const double *ptw, const int *frp, const int *fci, const double *fa,
+                     const int *pp, const int *pi, const double *pw, const int *crp,
+                     const int *cci, double *ca);
 
 // ---- batched BCGSSolver pieces (FP64 outer vectors; TQ = type of q, the
 // preconditioned direction: double, or float when the cycle runs in FP32)

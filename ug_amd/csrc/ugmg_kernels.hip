@@ -85,6 +85,103 @@ __device__ __forceinline__ TA sell_dot(const TV *__restrict__ v, const int *__re
   return sum;
 }
 
+// b x b blocks (b = 3): out_i = sum_k sum_j A_k[i][j] x[b col_k + j] over the
+// row's entries in order, entry k's block in b*b planes of pst values;
+// chunks of 2 entries (18 values and 6 gathers in flight per lane).
+template <int B, int NT, typename TV, typename TX, typename TA>
+__device__ __forceinline__ void sell_dot_blk(const TV *__restrict__ v, size_t pst,
+                                             const int *__restrict__ ci, int w,
+                                             const TX *__restrict__ x, int sdo,
+                                             const int *__restrict__ dtab, int rb, TA (&out)[B]) {
+  constexpr int CH = 2;
+#pragma unroll
+  for (int i = 0; i < B; ++i) out[i] = 0;
+  for (int k = 0; k < w; k += CH) {
+    TV a[CH][B * B];
+    TX xv[CH][B];
+#pragma unroll
+    for (int j = 0; j < CH; ++j) {
+      const int kk = (k + j < w) ? k + j : w - 1;
+      const int c = sdo >= 0 ? rb + dtab[sdo + kk] : ld<NT>(ci + 64 * kk);
+#pragma unroll
+      for (int q = 0; q < B * B; ++q) a[j][q] = ld<NT>(v + q * pst + 64 * kk);
+#pragma unroll
+      for (int i = 0; i < B; ++i) xv[j][i] = x[(size_t)B * c + i];
+    }
+#pragma unroll
+    for (int j = 0; j < CH; ++j)
+      if (k + j < w) {
+#pragma unroll
+        for (int i = 0; i < B; ++i) {
+          TA t = 0;
+#pragma unroll
+          for (int jj = 0; jj < B; ++jj) t += (TA)a[j][i * B + jj] * (TA)xv[j][jj];
+          out[i] += t;
+        }
+      }
+  }
+}
+
+// scalar weights on vectors of B components (the transfer): out_i =
+// sum_k w_k x[B col_k + i]
+template <int B, int NT, typename TV, typename TX, typename TA>
+__device__ __forceinline__ void sell_dot_diag(const TV *__restrict__ v, const int *__restrict__ ci,
+                                              int w, const TX *__restrict__ x, int sdo,
+                                              const int *__restrict__ dtab, int rb, TA (&out)[B]) {
+  constexpr int CH = 4;
+#pragma unroll
+  for (int i = 0; i < B; ++i) out[i] = 0;
+  for (int k = 0; k < w; k += CH) {
+    TV a[CH];
+    TX xv[CH][B];
+#pragma unroll
+    for (int j = 0; j < CH; ++j) {
+      const int kk = (k + j < w) ? k + j : w - 1;
+      const int c = sdo >= 0 ? rb + dtab[sdo + kk] : ld<NT>(ci + 64 * kk);
+      a[j] = ld<NT>(v + 64 * kk);
+#pragma unroll
+      for (int i = 0; i < B; ++i) xv[j][i] = x[(size_t)B * c + i];
+    }
+#pragma unroll
+    for (int j = 0; j < CH; ++j)
+      if (k + j < w) {
+#pragma unroll
+        for (int i = 0; i < B; ++i) out[i] += (TA)a[j] * (TA)xv[j][i];
+      }
+  }
+}
+
+// SolveSmallBlock for n = 3 (np/algebra/block.cc:131-142): elimination
+// without pivoting, in UG's operation order; m row-major
+template <typename T>
+__device__ __forceinline__ void solve3(const T (&m)[9], const T (&r)[3], T (&s)[3]) {
+  const T M3div0 = m[3] / m[0];
+  const T M6div0 = m[6] / m[0];
+  const T aux = (m[7] - M6div0 * m[1]) / (m[4] - M3div0 * m[1]);
+  s[2] = (r[2] - M6div0 * r[0] - aux * (r[1] - M3div0 * r[0])) /
+         (m[8] - M6div0 * m[2] - aux * (m[5] - M3div0 * m[2]));
+  s[1] = (r[1] - m[3] / m[0] * r[0] - (m[5] - M3div0 * m[2]) * s[2]) / (m[4] - M3div0 * m[1]);
+  s[0] = (r[0] - m[1] * s[1] - m[2] * s[2]) / m[0];
+}
+
+// block sum of B doubles per thread; thread 0 gets the totals
+template <int B>
+__device__ __forceinline__ void block_sum_b(double (&v)[B], double (*red)[kBlock / 64]) {
+#pragma unroll
+  for (int i = 0; i < B; ++i) {
+    for (int off = 32; off > 0; off >>= 1) v[i] += __shfl_down(v[i], off, 64);
+    if ((threadIdx.x & 63) == 0) red[i][threadIdx.x >> 6] = v[i];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int i = 0; i < B; ++i) {
+      double t = 0.0;
+      for (int k = 0; k < kBlock / 64; ++k) t += red[i][k];
+      v[i] = t;
+    }
+}
+
 // Slice block and column of this workgroup.  Single column: the XCD-aware
 // remap of blockIdx.x.  Several columns: the linear workgroup id is remapped
 // over the whole grid and split slice-block-major.
@@ -403,6 +500,371 @@ __global__ __launch_bounds__(kBlock) void k_add(int n, T *__restrict__ x, const 
   if (i < n) x[o + i] += y[o + i];
 }
 
+// ======================================================== 3x3 blocks
+// The same operations on vectors of B = 3 components with b x b matrix
+// blocks (linear elasticity); diag holds the diagonal block of every row.
+
+template <typename TV, typename TX, typename TY, int MODE, int NT>
+__global__ __launch_bounds__(kBlock) void k_spmv_b3(DSell<TV> A, int slice0, int nsl,
+                                                     const TX *__restrict__ x,
+                                                     TY *__restrict__ y, TY *__restrict__ c,
+                                                     size_t vs) {
+  constexpr int B = 3;
+  SLICE_PROLOGUE(A, slice0, nsl)
+  const size_t o = (size_t)col_ * vs;
+  x += o;
+  y += o;
+  TY sum[B];
+  sell_dot_blk<B, NT, TV, TX, TY>(A.val + e0 + lane, A.pst, A.col + e0 + lane, w, x, sdo, A.dtab, rb, sum);
+  if (r < 0) return;
+#pragma unroll
+  for (int i = 0; i < B; ++i) {
+    const size_t q = (size_t)B * r + i;
+    if (MODE == 1) {
+      y[q] = sum[i];
+    } else {
+      y[q] -= sum[i];
+      if (MODE == 2) c[o + q] += (TY)x[q];
+      if (MODE == 3) c[o + q] = (TY)x[q];
+    }
+  }
+}
+
+template <int NT>
+__global__ __launch_bounds__(kBlock) void k_ir_update_b3(DSell<double> A, int nsl,
+                                                          const float *__restrict__ c32,
+                                                          double *__restrict__ rv,
+                                                          double *__restrict__ x,
+                                                          float *__restrict__ r32,
+                                                          const uint8_t *__restrict__ vclass,
+                                                          double *__restrict__ part, size_t vs) {
+  constexpr int B = 3;
+  __shared__ double red[B][kBlock / 64];
+  SLICE_PROLOGUE_RED(A, nsl)
+  const size_t o = (size_t)col_ * vs;
+  double sq[B] = {0.0, 0.0, 0.0};
+  if (sok) {
+    double sum[B];
+    sell_dot_blk<B, NT, double, float, double>(A.val + e0 + lane, A.pst, A.col + e0 + lane, w,
+                                               c32 + o, sdo, A.dtab, rb, sum);
+    if (r >= 0)
+#pragma unroll
+      for (int i = 0; i < B; ++i) {
+        const size_t q = o + (size_t)B * r + i;
+        const double nr = rv[q] - sum[i];
+        rv[q] = nr;
+        x[q] += (double)c32[q];
+        r32[q] = (float)nr;
+        if (vclass[r] >= 2) sq[i] = nr * nr;
+      }
+  }
+  block_sum_b<B>(sq, red);
+  if (threadIdx.x == 0)
+    for (int i = 0; i < B; ++i) part[((size_t)col_ * B + i) * gridDim.x + sb_] = sq[i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_jac_b3(int n, const T *__restrict__ b,
+                                                    const T *__restrict__ diag,
+                                                    const uint8_t *__restrict__ vclass, T damp,
+                                                    T *__restrict__ t, size_t vs) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const size_t o = (size_t)blockIdx.y * vs + 3 * (size_t)i;
+  if (vclass[i] < 3) {
+    t[o] = t[o + 1] = t[o + 2] = (T)0;
+    return;
+  }
+  T m[9], r[3], x[3];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) m[q] = diag[9 * (size_t)i + q];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) r[q] = b[o + q];
+  solve3<T>(m, r, x);
+#pragma unroll
+  for (int q = 0; q < 3; ++q) t[o + q] = x[q] * damp;
+}
+
+template <typename TV, typename T, int NT>
+__global__ __launch_bounds__(kBlock) void k_gs_color_b3(DSell<TV> L, int slice0, int nsl,
+                                                         const T *__restrict__ b,
+                                                         const T *__restrict__ diag,
+                                                         const uint8_t *__restrict__ vclass,
+                                                         T *__restrict__ v, size_t vs) {
+  constexpr int B = 3;
+  SLICE_PROLOGUE(L, slice0, nsl)
+  const size_t o = (size_t)col_ * vs;
+  v += o;
+  T sum[B];
+  sell_dot_blk<B, NT, TV, T, T>(L.val + e0 + lane, L.pst, L.col + e0 + lane, w, v, sdo, L.dtab, rb, sum);
+  if (r < 0) return;
+  T x[B] = {0, 0, 0};
+  if (vclass[r] >= 3) {
+    T m[9], rr[B];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) m[q] = diag[9 * (size_t)r + q];
+#pragma unroll
+    for (int q = 0; q < B; ++q) rr[q] = b[o + (size_t)B * r + q] - sum[q];
+    solve3<T>(m, rr, x);
+  }
+#pragma unroll
+  for (int q = 0; q < B; ++q) v[(size_t)B * r + q] = x[q];
+}
+
+template <typename TV, typename T, int CMODE, int NT>
+__global__ __launch_bounds__(kBlock) void k_gs_defect_b3(DSell<TV> U, int nsl,
+                                                          const T *__restrict__ v,
+                                                          T *__restrict__ b,
+                                                          const uint8_t *__restrict__ vclass,
+                                                          T damp, T *__restrict__ c,
+                                                          T *__restrict__ xout, size_t vs) {
+  constexpr int B = 3;
+  SLICE_PROLOGUE(U, 0, nsl)
+  const size_t o = (size_t)col_ * vs;
+  v += o;
+  T sum[B];
+  sell_dot_blk<B, NT, TV, T, T>(U.val + e0 + lane, U.pst, U.col + e0 + lane, w, v, sdo, U.dtab, rb, sum);
+  if (r < 0) return;
+  const bool act = vclass[r] >= 3;
+#pragma unroll
+  for (int q = 0; q < B; ++q) {
+    const size_t k = (size_t)B * r + q;
+    const T br = b[o + k];
+    b[o + k] = act ? br - damp * (br + sum[q]) : br - damp * sum[q];
+    const T t = damp * v[k];
+    if (c) {
+      if (CMODE == 0) c[o + k] = t;
+      else c[o + k] += t;
+    }
+    if (xout) xout[o + k] = t;
+  }
+}
+
+template <typename TV, typename T, int CMODE, int NT>
+__global__ __launch_bounds__(kBlock) void k_gs_color_fused_b3(DSell<TV> L, DSell<TV> U,
+                                                               int slice0, int nsl,
+                                                               T *__restrict__ b,
+                                                               const T *__restrict__ diag, T damp,
+                                                               T *__restrict__ v,
+                                                               T *__restrict__ c, size_t vs) {
+  constexpr int B = 3;
+  SLICE_PROLOGUE(L, slice0, nsl)
+  const size_t o = (size_t)col_ * vs;
+  v += o;
+  const int eu = U.soff[s];
+  const int wu = (U.soff[s + 1] - eu) >> 6;
+  const int sdu = U.sdoff[s];
+  T su[B], sl[B];
+  sell_dot_blk<B, NT, TV, T, T>(U.val + eu + lane, U.pst, U.col + eu + lane, wu, v, sdu, U.dtab, rb, su);
+  sell_dot_blk<B, NT, TV, T, T>(L.val + e0 + lane, L.pst, L.col + e0 + lane, w, v, sdo, L.dtab, rb, sl);
+  if (r < 0) return;
+  T m[9], rr[B], x[B];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) m[q] = diag[9 * (size_t)r + q];
+#pragma unroll
+  for (int q = 0; q < B; ++q) {
+    const size_t k = (size_t)B * r + q;
+    const T br = b[o + k];
+    const T bn = br - damp * (br + su[q]);
+    b[o + k] = bn;
+    const T t = damp * v[k];
+    c[o + k] = CMODE == 0 ? t : c[o + k] + t;
+    rr[q] = bn - sl[q];
+  }
+  solve3<T>(m, rr, x);
+#pragma unroll
+  for (int q = 0; q < B; ++q) v[(size_t)B * r + q] = x[q];
+}
+
+template <typename T, int NT>
+__global__ __launch_bounds__(kBlock) void k_restrict_b3(DSell<T> PT, int nsl,
+                                                         const uint32_t *__restrict__ cskip,
+                                                         const uint8_t *__restrict__ cvnclass,
+                                                         const uint8_t *__restrict__ cvclass,
+                                                         const T *__restrict__ bf, T damp,
+                                                         T *__restrict__ bc,
+                                                         double *__restrict__ part, size_t vsf,
+                                                         size_t vsc) {
+  constexpr int B = 3;
+  __shared__ double red[B][kBlock / 64];
+  SLICE_PROLOGUE_RED(PT, nsl)
+  bf += (size_t)col_ * vsf;
+  bc += (size_t)col_ * vsc;
+  double sq[B] = {0.0, 0.0, 0.0};
+  if (sok) {
+    T sum[B];
+    sell_dot_diag<B, NT, T, T, T>(PT.val + e0 + lane, PT.col + e0 + lane, w, bf, sdo, PT.dtab, rb, sum);
+    if (r >= 0) {
+      const bool zero = cvnclass[r] >= 2;
+      const uint32_t sk = cskip[r];
+#pragma unroll
+      for (int q = 0; q < B; ++q) {
+        const size_t k = (size_t)B * r + q;
+        const T base = zero ? (T)0 : bc[k];
+        const T v = ((sk >> q) & 1u) ? base : base + damp * sum[q];
+        bc[k] = v;
+        if (part && cvclass[r] >= 2) sq[q] = (double)v * (double)v;
+      }
+    }
+  }
+  if (!part) return;
+  block_sum_b<B>(sq, red);
+  if (threadIdx.x == 0)
+    for (int i = 0; i < B; ++i) part[((size_t)col_ * B + i) * gridDim.x + sb_] = sq[i];
+}
+
+template <typename T, int CADD, int NT>
+__global__ __launch_bounds__(kBlock) void k_interp_b3(DSell<T> P, int nsl,
+                                                       const uint32_t *__restrict__ fskip,
+                                                       const T *__restrict__ cc, T damp,
+                                                       T *__restrict__ t, T *__restrict__ c,
+                                                       size_t vsf, size_t vsc) {
+  constexpr int B = 3;
+  SLICE_PROLOGUE(P, 0, nsl)
+  cc += (size_t)col_ * vsc;
+  const size_t o = (size_t)col_ * vsf;
+  T sum[B];
+  sell_dot_diag<B, NT, T, T, T>(P.val + e0 + lane, P.col + e0 + lane, w, cc, sdo, P.dtab, rb, sum);
+  if (r < 0) return;
+  const uint32_t sk = fskip[r];
+#pragma unroll
+  for (int q = 0; q < B; ++q) {
+    const size_t k = o + (size_t)B * r + q;
+    const T tv = ((sk >> q) & 1u) ? (T)0 : damp * sum[q];
+    t[k] = tv;
+    if (CADD) c[k] += tv;
+  }
+}
+
+// partial sums per component: part[(column * 3 + i) * gridDim.x + block]
+template <typename T, int DOT>
+__global__ __launch_bounds__(kBlock) void k_dot_part_b3(int n, const T *__restrict__ x,
+                                                         const T *__restrict__ y,
+                                                         const uint8_t *__restrict__ vclass,
+                                                         double *__restrict__ part, size_t vs) {
+  constexpr int B = 3;
+  __shared__ double red[B][kBlock / 64];
+  const size_t o = (size_t)blockIdx.y * vs;
+  double s[B] = {0.0, 0.0, 0.0};
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    if (vclass && vclass[i] < 2) continue;
+#pragma unroll
+    for (int q = 0; q < B; ++q) {
+      const double xi = (double)x[o + (size_t)B * i + q];
+      s[q] += DOT ? xi * (double)y[o + (size_t)B * i + q] : xi * xi;
+    }
+  }
+  block_sum_b<B>(s, red);
+  if (threadIdx.x == 0)
+    for (int q = 0; q < B; ++q) part[((size_t)blockIdx.y * B + q) * gridDim.x + blockIdx.x] = s[q];
+}
+
+// ls over ex on the base level with per-component stop tests (sc_cmp:
+// every component below its limit); inv is (3n) x (3n)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_base_gemv_b3(int n3, const double *__restrict__ inv,
+                                                          const T *__restrict__ b,
+                                                          T *__restrict__ c, T *__restrict__ xout,
+                                                          int cmode, double damp, double abslimit,
+                                                          double red, const double *__restrict__ part,
+                                                          int nparts, double *__restrict__ first,
+                                                          int it, size_t vs) {
+  constexpr int B = 3;
+  const int row = (blockIdx.x * kBlock + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  const int cj = blockIdx.y;
+  const size_t o = (size_t)cj * vs;
+  double cur[B];
+#pragma unroll
+  for (int q = 0; q < B; ++q) {
+    const double *p = part + ((size_t)cj * B + q) * nparts;
+    double s = 0.0;
+    for (int i = lane; i < nparts; i += 64) s += p[i];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    cur[q] = sqrt(s);
+  }
+  bool below_abs = true, below_red = true;
+  if (it == 0) {
+    if (row == 0 && lane == 0)
+      for (int q = 0; q < B; ++q) first[(size_t)cj * B + q] = cur[q];
+    for (int q = 0; q < B; ++q) below_abs &= fabs(cur[q]) < fabs(abslimit);
+    below_red = false;
+  } else {
+    for (int q = 0; q < B; ++q) {
+      double target = first[(size_t)cj * B + q] * red;
+      if (target == 0.0) target = red;
+      below_abs &= fabs(cur[q]) < fabs(abslimit);
+      below_red &= fabs(cur[q]) < fabs(target);
+    }
+  }
+  const bool conv = below_abs || below_red;
+  if (row >= n3) return;
+  double x = 0.0;
+  if (!conv) {
+    const double *__restrict__ rr = inv + (size_t)row * n3;
+    const T *__restrict__ bb = b + o;
+    for (int j = lane; j < n3; j += 64) x += rr[j] * (double)bb[j];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    x *= damp;
+  }
+  if (lane == 0) {
+    const T xv = (T)x;
+    if (!conv || cmode == 0) c[o + row] = cmode ? c[o + row] + xv : xv;
+    if (xout) xout[o + row] = xv;
+  }
+}
+
+// ================================================================ Galerkin
+// AssembleGalerkinByMatrix (np/algebra/transgrid.cc:1575-1700) as a gather:
+// one thread per coarse row I walks the fine vectors v that interpolate
+// from I (fine list order), their matrix rows and the interpolation rows of
+// the columns, and adds (a wi) wj to the coarse entry (I, J) of the given
+// pattern (the diagonal when (I, J) is missing).  For a fixed coarse entry
+// the contributions arrive in UG's scatter order (v, then m, then J).
+template <int BB>
+__global__ __launch_bounds__(kBlock) void k_galerkin(
+    int nc, const int *__restrict__ ptp, const int *__restrict__ pti,
+    const double *__restrict__ ptw, const int *__restrict__ frp, const int *__restrict__ fci,
+    const double *__restrict__ fa, const int *__restrict__ pp, const int *__restrict__ pi,
+    const double *__restrict__ pw, const int *__restrict__ crp, const int *__restrict__ cci,
+    double *__restrict__ ca) {
+  const int I = blockIdx.x * kBlock + threadIdx.x;
+  if (I >= nc) return;
+  const int c0 = crp[I], c1 = crp[I + 1];
+  for (size_t k = (size_t)c0 * BB; k < (size_t)c1 * BB; ++k) ca[k] = 0.0;
+  for (int q = ptp[I]; q < ptp[I + 1]; ++q) {
+    const int v = pti[q];
+    const double wi = ptw[q];
+    for (int m = frp[v]; m < frp[v + 1]; ++m) {
+      const int w = fci[m];
+      for (int r = pp[w]; r < pp[w + 1]; ++r) {
+        const int J = pi[r];
+        const double wj = pw[r];
+        int slot = c0;
+        for (int k = c0; k < c1; ++k)
+          if (cci[k] == J) {
+            slot = k;
+            break;
+          }
+#pragma unroll
+        for (int e = 0; e < BB; ++e)
+          ca[(size_t)slot * BB + e] += (fa[(size_t)m * BB + e] * wi) * wj;
+      }
+    }
+  }
+}
+
+void launch_galerkin(hipStream_t st, int nc, int bb, const int *ptp, const int *pti,
+                     const double *ptw, const int *frp, const int *fci, const double *fa,
+                     const int *pp, const int *pi, const double *pw, const int *crp,
+                     const int *cci, double *ca) {
+  if (nc <= 0) return;
+  if (bb == 9)
+    k_galerkin<9><<<(nc + kBlock - 1) / kBlock, kBlock, 0, st>>>(nc, ptp, pti, ptw, frp, fci, fa, pp, pi, pw, crp, cci, ca);
+  else
+    k_galerkin<1><<<(nc + kBlock - 1) / kBlock, kBlock, 0, st>>>(nc, ptp, pti, ptw, frp, fci, fa, pp, pi, pw, crp, cci, ca);
+}
+
 // ==================================================== fused coarse cycle
 // One workgroup runs Lmgc on every level from `top` down to the base level:
 // each operation is a loop over rows or slabs followed by a workgroup
@@ -610,6 +1072,15 @@ void launch_spmv(hipStream_t st, const DSell<TV> &A, int slice0, int nsl, const 
                  TY *c, int mode, bool nt, Batch bt) {
   if (nsl <= 0) return;
   const dim3 g = grid2(nblk_slices(nsl), bt.nrhs), blk(kBlock);
+  if (bt.b == 3) {
+    NT_DISPATCH(nt, switch (mode) {
+      case 0: k_spmv_b3<TV, TX, TY, 0, NTV><<<g, blk, 0, st>>>(A, slice0, nsl, x, y, c, bt.vs); break;
+      case 1: k_spmv_b3<TV, TX, TY, 1, NTV><<<g, blk, 0, st>>>(A, slice0, nsl, x, y, c, bt.vs); break;
+      case 2: k_spmv_b3<TV, TX, TY, 2, NTV><<<g, blk, 0, st>>>(A, slice0, nsl, x, y, c, bt.vs); break;
+      default: k_spmv_b3<TV, TX, TY, 3, NTV><<<g, blk, 0, st>>>(A, slice0, nsl, x, y, c, bt.vs); break;
+    });
+    return;
+  }
   W_DISPATCH(A.wmax, NT_DISPATCH(nt, switch (mode) {
     case 0: k_spmv<MWV, TV, TX, TY, 0, NTV><<<g, blk, 0, st>>>(A, slice0, nsl, x, y, c, bt.vs); break;
     case 1: k_spmv<MWV, TV, TX, TY, 1, NTV><<<g, blk, 0, st>>>(A, slice0, nsl, x, y, c, bt.vs); break;
@@ -629,6 +1100,11 @@ void launch_ir_update(hipStream_t st, const DSell<double> &A, const float *c32, 
   const int g = nblk_slices(A.nslices);
   *nparts = g;
   if (g == 0) return;
+  if (bt.b == 3) {
+    NT_DISPATCH(nt, k_ir_update_b3<NTV><<<grid2(g, bt.nrhs), kBlock, 0, st>>>(
+                        A, A.nslices, c32, r, x, r32, vclass, part, bt.vs));
+    return;
+  }
   W_DISPATCH(A.wmax, NT_DISPATCH(nt, k_ir_update<MWV, NTV><<<grid2(g, bt.nrhs), kBlock, 0, st>>>(
                                          A, A.nslices, c32, r, x, r32, vclass, part, bt.vs)));
 }
@@ -637,7 +1113,8 @@ template <typename T>
 void launch_jac(hipStream_t st, int n, const T *b, const T *diag, const uint8_t *vclass, T damp,
                 T *t, Batch bt) {
   if (n <= 0) return;
-  k_jac<T><<<grid2(nblk(n), bt.nrhs), kBlock, 0, st>>>(n, b, diag, vclass, damp, t, bt.vs);
+  if (bt.b == 3) k_jac_b3<T><<<grid2(nblk(n), bt.nrhs), kBlock, 0, st>>>(n, b, diag, vclass, damp, t, bt.vs);
+  else k_jac<T><<<grid2(nblk(n), bt.nrhs), kBlock, 0, st>>>(n, b, diag, vclass, damp, t, bt.vs);
 }
 template void launch_jac<double>(hipStream_t, int, const double *, const double *,
                                  const uint8_t *, double, double *, Batch);
@@ -651,6 +1128,11 @@ void launch_gs_color(hipStream_t st, const DSell<TV> &L, int slice0, int nsl, in
   // colour sweeps stream in chunks of 8 entries whatever the width: their
   // time follows the entries read, and a wider chunk only adds clamped loads
   (void)wmax;
+  if (bt.b == 3) {
+    NT_DISPATCH(nt, k_gs_color_b3<TV, T, NTV><<<grid2(nblk_slices(nsl), bt.nrhs), kBlock, 0, st>>>(
+                        L, slice0, nsl, b, diag, vclass, v, bt.vs));
+    return;
+  }
   NT_DISPATCH(nt, k_gs_color<8, TV, T, NTV><<<grid2(nblk_slices(nsl), bt.nrhs), kBlock, 0, st>>>(
                       L, slice0, nsl, b, diag, vclass, v, bt.vs));
 }
@@ -667,6 +1149,13 @@ void launch_gs_color_fused(hipStream_t st, const DSell<TV> &L, const DSell<TV> &
                            Batch bt) {
   if (nsl <= 0) return;
   const dim3 g = grid2(nblk_slices(nsl), bt.nrhs);
+  if (bt.b == 3) {
+    NT_DISPATCH(nt, if (cmode == 0) k_gs_color_fused_b3<TV, T, 0, NTV><<<g, kBlock, 0, st>>>(
+                        L, U, slice0, nsl, b, diag, damp, v, c, bt.vs);
+                else k_gs_color_fused_b3<TV, T, 1, NTV><<<g, kBlock, 0, st>>>(
+                        L, U, slice0, nsl, b, diag, damp, v, c, bt.vs));
+    return;
+  }
   NT_DISPATCH(nt, if (cmode == 0) k_gs_color_fused<TV, T, 0, NTV><<<g, kBlock, 0, st>>>(
                       L, U, slice0, nsl, b, diag, damp, v, c, bt.vs);
               else k_gs_color_fused<TV, T, 1, NTV><<<g, kBlock, 0, st>>>(
@@ -686,6 +1175,13 @@ void launch_gs_defect(hipStream_t st, const DSell<TV> &U, const T *v, T *b, cons
                       T damp, T *c, int cmode, T *xout, bool nt, Batch bt) {
   if (U.nslices <= 0) return;
   const dim3 g = grid2(nblk_slices(U.nslices), bt.nrhs);
+  if (bt.b == 3) {
+    NT_DISPATCH(nt, if (cmode == 0) k_gs_defect_b3<TV, T, 0, NTV><<<g, kBlock, 0, st>>>(
+                        U, U.nslices, v, b, vclass, damp, c, xout, bt.vs);
+                else k_gs_defect_b3<TV, T, 1, NTV><<<g, kBlock, 0, st>>>(
+                        U, U.nslices, v, b, vclass, damp, c, xout, bt.vs));
+    return;
+  }
   W_DISPATCH(U.wmax, NT_DISPATCH(nt, if (cmode == 0) k_gs_defect<MWV, TV, T, 0, NTV><<<g, kBlock, 0, st>>>(
                                          U, U.nslices, v, b, vclass, damp, c, xout, bt.vs);
                                  else k_gs_defect<MWV, TV, T, 1, NTV><<<g, kBlock, 0, st>>>(
@@ -701,9 +1197,14 @@ template void launch_gs_defect<float, float>(hipStream_t, const DSell<float> &, 
 template <typename T>
 int launch_restrict(hipStream_t st, const DSell<T> &PT, const uint32_t *cskip,
                     const uint8_t *cvnclass, const uint8_t *cvclass, const T *bf, T damp, T *bc,
-                    double *part, bool nt, int nrhs, size_t vsf, size_t vsc) {
+                    double *part, bool nt, int nrhs, size_t vsf, size_t vsc, int b) {
   if (PT.nslices <= 0) return 0;
   const int g = nblk_slices(PT.nslices);
+  if (b == 3) {
+    NT_DISPATCH(nt, k_restrict_b3<T, NTV><<<grid2(g, nrhs), kBlock, 0, st>>>(
+                        PT, PT.nslices, cskip, cvnclass, cvclass, bf, damp, bc, part, vsf, vsc));
+    return g;
+  }
   W_DISPATCH(PT.wmax, NT_DISPATCH(nt, k_restrict<MWV, T, NTV><<<grid2(g, nrhs), kBlock, 0, st>>>(
                                           PT, PT.nslices, cskip, cvnclass, cvclass, bf, damp, bc,
                                           part, vsf, vsc)));
@@ -711,10 +1212,10 @@ int launch_restrict(hipStream_t st, const DSell<T> &PT, const uint32_t *cskip,
 }
 template int launch_restrict<double>(hipStream_t, const DSell<double> &, const uint32_t *,
                                      const uint8_t *, const uint8_t *, const double *, double,
-                                     double *, double *, bool, int, size_t, size_t);
+                                     double *, double *, bool, int, size_t, size_t, int);
 template int launch_restrict<float>(hipStream_t, const DSell<float> &, const uint32_t *,
                                     const uint8_t *, const uint8_t *, const float *, float,
-                                    float *, double *, bool, int, size_t, size_t);
+                                    float *, double *, bool, int, size_t, size_t, int);
 
 // --------------------------------------------------- base solve (GEMV)
 // ls over ex on the base level, one wavefront per row of the dense inverse:
@@ -768,8 +1269,13 @@ void launch_base_gemv(hipStream_t st, int n, const double *inv, const T *b, T *c
                       int cmode, double damp, double abslimit, double red, const double *part,
                       int nparts, double *first, int it, Batch bt) {
   const int g = (n * 64 + kBlock - 1) / kBlock;
-  k_base_gemv<T><<<grid2(g, bt.nrhs), kBlock, 0, st>>>(n, inv, b, c, xout, cmode, damp, abslimit,
-                                                       red, part, nparts, first, it, bt.vs);
+  if (bt.b == 3)   // n counts scalar unknowns
+    k_base_gemv_b3<T><<<grid2(g, bt.nrhs), kBlock, 0, st>>>(n, inv, b, c, xout, cmode, damp,
+                                                            abslimit, red, part, nparts, first, it,
+                                                            bt.vs);
+  else
+    k_base_gemv<T><<<grid2(g, bt.nrhs), kBlock, 0, st>>>(n, inv, b, c, xout, cmode, damp, abslimit,
+                                                         red, part, nparts, first, it, bt.vs);
 }
 template void launch_base_gemv<double>(hipStream_t, int, const double *, const double *,
                                        double *, double *, int, double, double, double,
@@ -782,15 +1288,23 @@ template <typename T>
 int launch_dot_part(hipStream_t st, int n, const T *x, const uint8_t *vclass, double *part,
                     Batch bt) {
   const int g = reduce_parts(n);
-  k_dot_part<T, 0><<<grid2(g, bt.nrhs), kBlock, 0, st>>>(n, x, nullptr, vclass, part, bt.vs);
+  if (bt.b == 3) k_dot_part_b3<T, 0><<<grid2(g, bt.nrhs), kBlock, 0, st>>>(n, x, nullptr, vclass, part, bt.vs);
+  else k_dot_part<T, 0><<<grid2(g, bt.nrhs), kBlock, 0, st>>>(n, x, nullptr, vclass, part, bt.vs);
   return g;
 }
 
 template <typename T>
 void launch_interp(hipStream_t st, const DSell<T> &P, const uint32_t *fskip, const T *cc, T damp,
-                   T *t, T *c, bool nt, int nrhs, size_t vsf, size_t vsc) {
+                   T *t, T *c, bool nt, int nrhs, size_t vsf, size_t vsc, int b) {
   if (P.nslices <= 0) return;
   const dim3 g = grid2(nblk_slices(P.nslices), nrhs);
+  if (b == 3) {
+    NT_DISPATCH(nt, if (c) k_interp_b3<T, 1, NTV><<<g, kBlock, 0, st>>>(P, P.nslices, fskip, cc,
+                                                                        damp, t, c, vsf, vsc);
+                else k_interp_b3<T, 0, NTV><<<g, kBlock, 0, st>>>(P, P.nslices, fskip, cc, damp,
+                                                                  t, c, vsf, vsc));
+    return;
+  }
   W_DISPATCH(P.wmax, NT_DISPATCH(nt, if (c) k_interp<MWV, T, 1, NTV><<<g, kBlock, 0, st>>>(
                                                 P, P.nslices, fskip, cc, damp, t, c, vsf, vsc);
                                  else k_interp<MWV, T, 0, NTV><<<g, kBlock, 0, st>>>(
@@ -798,10 +1312,10 @@ void launch_interp(hipStream_t st, const DSell<T> &P, const uint32_t *fskip, con
 }
 template void launch_interp<double>(hipStream_t, const DSell<double> &, const uint32_t *,
                                     const double *, double, double *, double *, bool, int,
-                                    size_t, size_t);
+                                    size_t, size_t, int);
 template void launch_interp<float>(hipStream_t, const DSell<float> &, const uint32_t *,
                                    const float *, float, float *, float *, bool, int, size_t,
-                                   size_t);
+                                   size_t, int);
 
 int reduce_parts(int n) {
   int g = nblk(n);
@@ -813,6 +1327,12 @@ void launch_dot(hipStream_t st, int n, const T *x, const T *y, const uint8_t *vc
                 double *part, double *out, int do_sqrt, Batch bt) {
   const int g = reduce_parts(n);
   const dim3 gg = grid2(g, bt.nrhs);
+  if (bt.b == 3) {
+    if (y) k_dot_part_b3<T, 1><<<gg, kBlock, 0, st>>>(n, x, y, vclass, part, bt.vs);
+    else k_dot_part_b3<T, 0><<<gg, kBlock, 0, st>>>(n, x, y, vclass, part, bt.vs);
+    k_reduce_final<<<gg.y * 3, 1024, 0, st>>>(g, part, out, do_sqrt);
+    return;
+  }
   if (y) k_dot_part<T, 1><<<gg, kBlock, 0, st>>>(n, x, y, vclass, part, bt.vs);
   else k_dot_part<T, 0><<<gg, kBlock, 0, st>>>(n, x, y, vclass, part, bt.vs);
   k_reduce_final<<<gg.y, 1024, 0, st>>>(g, part, out, do_sqrt);
@@ -833,6 +1353,7 @@ void launch_reduce_final(hipStream_t st, int nparts, const double *part, double 
 
 template <typename T>
 void launch_fill(hipStream_t st, int n, T *x, T val, Batch bt) {
+  n *= bt.b;
   if (n > 0) k_fill<T><<<grid2(nblk(n), bt.nrhs), kBlock, 0, st>>>(n, x, val, bt.vs);
 }
 template void launch_fill<double>(hipStream_t, int, double *, double, Batch);
@@ -853,6 +1374,7 @@ template void launch_convert<float, float>(hipStream_t, int, const float *, floa
 
 template <typename T>
 void launch_add(hipStream_t st, int n, T *x, const T *y, Batch bt) {
+  n *= bt.b;
   if (n > 0) k_add<T><<<grid2(nblk(n), bt.nrhs), kBlock, 0, st>>>(n, x, y, bt.vs);
 }
 template void launch_add<double>(hipStream_t, int, double *, const double *, Batch);
@@ -861,10 +1383,10 @@ template void launch_add<float>(hipStream_t, int, float *, const float *, Batch)
 template <typename TD, typename TS, int SCATTER>
 __global__ __launch_bounds__(kBlock) void k_permute(int n, const int *__restrict__ perm,
                                                      const TS *__restrict__ s, TD *__restrict__ d,
-                                                     size_t vss, size_t vsd) {
-  const int p = blockIdx.x * kBlock + threadIdx.x;
-  if (p >= n) return;
-  const int q = perm[p];
+                                                     size_t vss, size_t vsd, int b) {
+  const int e = blockIdx.x * kBlock + threadIdx.x;   // scalar entry: vector e / b
+  if (e >= n * b) return;
+  const int p = e, q = perm[e / b] * b + e % b;
   s += (size_t)blockIdx.y * vss;
   d += (size_t)blockIdx.y * vsd;
   if (SCATTER) d[q] = (TD)s[p];
@@ -873,19 +1395,21 @@ __global__ __launch_bounds__(kBlock) void k_permute(int n, const int *__restrict
 
 template <typename TD, typename TS>
 void launch_gather(hipStream_t st, int n, const int *perm, const TS *s, TD *d, int nrhs,
-                   size_t vss, size_t vsd) {
-  if (n > 0) k_permute<TD, TS, 0><<<grid2(nblk(n), nrhs), kBlock, 0, st>>>(n, perm, s, d, vss, vsd);
+                   size_t vss, size_t vsd, int b) {
+  if (n > 0)
+    k_permute<TD, TS, 0><<<grid2(nblk(n * b), nrhs), kBlock, 0, st>>>(n, perm, s, d, vss, vsd, b);
 }
 template <typename TD, typename TS>
 void launch_scatter(hipStream_t st, int n, const int *perm, const TS *s, TD *d, int nrhs,
-                    size_t vss, size_t vsd) {
-  if (n > 0) k_permute<TD, TS, 1><<<grid2(nblk(n), nrhs), kBlock, 0, st>>>(n, perm, s, d, vss, vsd);
+                    size_t vss, size_t vsd, int b) {
+  if (n > 0)
+    k_permute<TD, TS, 1><<<grid2(nblk(n * b), nrhs), kBlock, 0, st>>>(n, perm, s, d, vss, vsd, b);
 }
 #define PERM_INST(TD, TS)                                                                      \
   template void launch_gather<TD, TS>(hipStream_t, int, const int *, const TS *, TD *, int,    \
-                                      size_t, size_t);                                         \
+                                      size_t, size_t, int);                                    \
   template void launch_scatter<TD, TS>(hipStream_t, int, const int *, const TS *, TD *, int,   \
-                                       size_t, size_t);
+                                       size_t, size_t, int);
 PERM_INST(double, double)
 PERM_INST(float, double)
 PERM_INST(double, float)

@@ -31,9 +31,10 @@ namespace ugmg {
 // as values only against that sequence.
 template <class Keep>
 static void sell_from_csr(int n, int ncols, const int *rp, const int *ci, const double *a,
-                          const std::vector<int> *rows, Keep keep, bool uniform,
+                          int bb, const std::vector<int> *rows, Keep keep, bool uniform,
                           HostSell &out) {
   const int nr = rows ? (int)rows->size() : n;
+  out.bb = bb;
   out.nrows = nr;
   out.identity_rows = rows == nullptr;
   out.nslices = (nr + kSlice - 1) / kSlice;
@@ -99,7 +100,10 @@ static void sell_from_csr(int n, int ncols, const int *rp, const int *ci, const 
   }
   const size_t nent = (size_t)out.soff[ns];
   out.col.resize(nent);
-  out.val.resize(nent);
+  out.val.resize(nent * bb);
+  auto put = [&](size_t e, int m) {
+    for (int q = 0; q < bb; ++q) out.val[(size_t)q * nent + e] = m < 0 ? 0.0 : a[(size_t)m * bb + q];
+  };
 #pragma omp parallel for schedule(static)
   for (int s = 0; s < ns; ++s) {
     const int w = width[s];
@@ -116,18 +120,18 @@ static void sell_from_csr(int n, int ncols, const int *rp, const int *ci, const 
             while (U[k] != d) {  // phantom entries before this one
               const size_t e = (size_t)out.soff[s] + (size_t)k * kSlice + lane;
               out.col[e] = -1;
-              out.val[e] = 0.0;
+              put(e, -1);
               ++k;
             }
             const size_t e = (size_t)out.soff[s] + (size_t)k * kSlice + lane;
             out.col[e] = ci[m];
-            out.val[e] = a[m];
+            put(e, m);
             ++k;
           }
         for (; k < w; ++k) {
           const size_t e = (size_t)out.soff[s] + (size_t)k * kSlice + lane;
           out.col[e] = -1;
-          out.val[e] = 0.0;
+          put(e, -1);
         }
         continue;
       }
@@ -138,7 +142,7 @@ static void sell_from_csr(int n, int ncols, const int *rp, const int *ci, const 
           if (!keep(r, m)) continue;
           const size_t e = (size_t)out.soff[s] + (size_t)k * kSlice + lane;
           out.col[e] = ci[m];
-          out.val[e] = a[m];
+          put(e, m);
           last = ci[m];
           ++k;
         }
@@ -147,7 +151,7 @@ static void sell_from_csr(int n, int ncols, const int *rp, const int *ci, const 
       for (; k < w; ++k) {
         const size_t e = (size_t)out.soff[s] + (size_t)k * kSlice + lane;
         out.col[e] = last;
-        out.val[e] = 0.0;
+        put(e, -1);
       }
     }
   }
@@ -166,11 +170,11 @@ void build_sell(const HostLevel &L, const std::vector<int> *rows, Part part,
     if (L.vc[r] < 3) return true;
     return cw > cr;
   };
-  sell_from_csr(L.n, L.n, L.rp.data(), L.ci.data(), L.a.data(), rows, keep, true, out);
+  sell_from_csr(L.n, L.n, L.rp.data(), L.ci.data(), L.a.data(), L.b * L.b, rows, keep, true, out);
 }
 
 void build_sell_csr(int n, const int *rp, const int *ci, const double *a, HostSell &out) {
-  sell_from_csr(n, 0, rp, ci, a, nullptr, [](int, int) { return true; }, false, out);
+  sell_from_csr(n, 0, rp, ci, a, 1, nullptr, [](int, int) { return true; }, false, out);
 }
 
 void to_float(const std::vector<double> &in, std::vector<float> &out) {
@@ -221,9 +225,10 @@ int greedy_color(const HostLevel &L, std::vector<int> &color) {
 
 void permute_level(const HostLevel &L, const std::vector<int> &perm, const std::vector<int> &pos,
                    const std::vector<int> *cpos, HostLevel &out) {
-  const int n = L.n;
+  const int n = L.n, bb = L.b * L.b;
   out.n = n;
   out.nnzb = L.nnzb;
+  out.b = L.b;
   out.rp.assign(n + 1, 0);
   for (int p = 0; p < n; ++p) out.rp[p + 1] = out.rp[p] + (L.rp[perm[p] + 1] - L.rp[perm[p]]);
   out.ci.resize(L.ci.size());
@@ -237,7 +242,7 @@ void permute_level(const HostLevel &L, const std::vector<int> &perm, const std::
     int q = out.rp[p];
     for (int m = L.rp[r]; m < L.rp[r + 1]; ++m, ++q) {
       out.ci[q] = pos[L.ci[m]];
-      out.a[q] = L.a[m];
+      for (int e = 0; e < bb; ++e) out.a[(size_t)q * bb + e] = L.a[(size_t)m * bb + e];
     }
     out.skip[p] = L.skip[r];
     out.vc[p] = L.vc[r];
@@ -357,8 +362,10 @@ int choose_gs_window(const HostLevel &L, const std::vector<int> &color, int ncol
 
 void build_prep(const HostLevel &L, const HostLevel *coarse, const std::vector<int> *color,
                 int ncolors, HostPrep &out) {
-  out.diag.resize(L.n);
-  for (int i = 0; i < L.n; ++i) out.diag[i] = L.a[L.rp[i]];
+  const int bb = L.b * L.b;
+  out.diag.resize((size_t)L.n * bb);
+  for (int i = 0; i < L.n; ++i)
+    for (int e = 0; e < bb; ++e) out.diag[(size_t)i * bb + e] = L.a[(size_t)L.rp[i] * bb + e];
   if (!color) build_sell(L, nullptr, Part::All, nullptr, out.A);
   if (color) {
     out.ncolors = ncolors;
@@ -408,13 +415,12 @@ void build_prep(const HostLevel &L, const HostLevel *coarse, const std::vector<i
 #define EXM(m, bw, i, j) ((m)[2 * (size_t)(bw) * (i) + (j)])
 
 void build_base(const HostLevel &L, int reorder, HostBase &out) {
-  const int n = L.n;
-  out.n = n;
-  out.order.resize(n);
-  for (int i = 0; i < n; ++i) out.order[i] = i;
-  if (reorder && n > 0) {
-    std::vector<int> fifo(n);
-    std::vector<char> used(n, 0);
+  const int nvec = L.n, B = L.b, bb = B * B;
+  std::vector<int> order(nvec);
+  for (int i = 0; i < nvec; ++i) order[i] = i;
+  if (reorder && nvec > 0) {
+    std::vector<int> fifo(nvec);
+    std::vector<char> used(nvec, 0);
     int head = 0, tail = 0, v = 0;
     fifo[tail++] = 0;
     used[0] = 1;
@@ -431,27 +437,35 @@ void build_base(const HostLevel &L, int reorder, HostBase &out) {
     int i = 0;
     while (head < tail) {
       int u = fifo[head++];
-      out.order[i++] = u;
+      order[i++] = u;
       for (int m = L.rp[u] + 1; m < L.rp[u + 1]; ++m) {
         int w = L.ci[m];
         if (used[w]) { fifo[tail++] = w; used[w] = 0; }
       }
     }
-    if (i != n) {  // disconnected pattern: keep the list order
-      for (int k = 0; k < n; ++k) out.order[k] = k;
+    if (i != nvec) {  // disconnected pattern: keep the list order
+      for (int k = 0; k < nvec; ++k) order[k] = k;
     }
   }
-  std::vector<int> index(n);
-  for (int k = 0; k < n; ++k) index[out.order[k]] = k;
+  out.order = order;
+  // VINDEX * b of every vector; band over scalar unknowns, bandwidth
+  // max|i - j| (+ b - 1 for blocks)
+  std::vector<int> index(nvec);
+  for (int k = 0; k < nvec; ++k) index[order[k]] = k * B;
   int bw = 0;
-  for (int v = 0; v < n; ++v)
+  for (int v = 0; v < nvec; ++v)
     for (int m = L.rp[v] + 1; m < L.rp[v + 1]; ++m)
       bw = std::max(bw, std::abs(index[v] - index[L.ci[m]]));
+  if (B > 1) bw += B - 1;
+  const int n = nvec * B;
+  out.n = n;
   out.bw = bw;
   std::vector<double> band((size_t)n * (2 * (size_t)bw + 1) + 1, 0.0);
-  for (int v = 0; v < n; ++v)
+  for (int v = 0; v < nvec; ++v)
     for (int m = L.rp[v]; m < L.rp[v + 1]; ++m)
-      EXM(band.data(), bw, index[v], index[L.ci[m]]) = L.a[m];
+      for (int i = 0; i < B; ++i)
+        for (int j = 0; j < B; ++j)
+          EXM(band.data(), bw, index[v] + i, index[L.ci[m]] + j) = L.a[(size_t)m * bb + i * B + j];
   out.failed = false;
   for (int i = 0; i < n - 1; ++i) {
     double d = EXM(band.data(), bw, i, i);
@@ -466,15 +480,15 @@ void build_base(const HostLevel &L, int reorder, HostBase &out) {
   }
   out.inv.assign((size_t)n * n, 0.0);
   if (out.failed) return;
-  // column j of the inverse: EXApplyLUDOUBLE on the unit vector e_j (the
-  // forward substitution starts at the unit's position)
+  // column u = b v + j of the inverse: EXApplyLUDOUBLE on the unit vector
+  // at band position index[v] + j (the forward substitution starts there)
 #pragma omp parallel
   {
     std::vector<double> vec(n);
 #pragma omp for schedule(dynamic, 16)
-    for (int j = 0; j < n; ++j) {
+    for (int u = 0; u < n; ++u) {
       std::fill(vec.begin(), vec.end(), 0.0);
-      const int p0 = index[j];
+      const int p0 = index[u / B] + u % B;
       vec[p0] = 1.0;
       for (int i = p0 + 1; i < n; ++i)
         for (int k = std::max(i - bw, p0); k < i; ++k) vec[i] -= EXM(band.data(), bw, i, k) * vec[k];
@@ -483,7 +497,8 @@ void build_base(const HostLevel &L, int reorder, HostBase &out) {
         for (int k = i + 1; k <= kmax; ++k) vec[i] -= EXM(band.data(), bw, i, k) * vec[k];
         vec[i] /= EXM(band.data(), bw, i, i);
       }
-      for (int k = 0; k < n; ++k) out.inv[(size_t)out.order[k] * n + j] = vec[k];
+      for (int k = 0; k < nvec; ++k)
+        for (int i = 0; i < B; ++i) out.inv[((size_t)order[k] * B + i) * n + u] = vec[(size_t)k * B + i];
     }
   }
 }

@@ -98,6 +98,7 @@ def load():
         "ugmg_solve_host": ([vp, ip, dp, dp, dp, dp, ip, ip, C.POINTER(LResult), DEFECT_CB, vp], ip),
         "ugmg_time_kernel": ([vp, ip, ip, ip, C.POINTER(C.c_double), C.POINTER(C.c_double)], ip),
         "ugmg_set_batch": ([vp, ip], ip),
+        "ugmg_galerkin": ([vp, ip], ip),
         "ugmg_iter_batch": ([vp, ip, ip, dp, dp, ip], ip),
         "ugmg_bcgs_solve": ([vp, ip, ip, dp, dp, ip, dp, dp, ip, ip, ip, ip, C.POINTER(LResult), dp, ip], ip),
         "ugmg_bcgs_solve_host": ([vp, ip, ip, dp, dp, ip, dp, dp, ip, ip, ip, ip, C.POINTER(LResult), dp, ip],
@@ -233,7 +234,7 @@ class Context:
         lv = self.problem.levels[level]
         rp = np.zeros(lv.n + 1, np.int32)
         ci = np.zeros(lv.nnzb, np.int32)
-        va = np.zeros(lv.nnzb, np.float64)
+        va = np.zeros(lv.nnzb * lv.b * lv.b, np.float64)
         self._chk(self.lib.ugmg_get_pattern(self.h, level, rp.ctypes.data, ci.ctypes.data, va.ctypes.data))
         return rp, ci, va
 
@@ -247,15 +248,21 @@ class Context:
     def smatmul_minus(self, level, x, y):
         self._chk(self.lib.ugmg_smatmul_minus(self.h, level, _ptr(x), _ptr(y)))
 
+    def _ncomp(self):
+        return self.problem.b if self.problem is not None else 1
+
     def dnrm2x(self, level, x):
-        out = C.c_double()
+        """Per-component norm (a float for scalar hierarchies)."""
+        out = (C.c_double * 3)()
         self._chk(self.lib.ugmg_dnrm2x(self.h, level, _ptr(x), C.addressof(out)))
-        return out.value
+        b = self._ncomp()
+        return out[0] if b == 1 else np.array(out[:b])
 
     def ddot(self, level, x, y):
-        out = C.c_double()
+        out = (C.c_double * 3)()
         self._chk(self.lib.ugmg_ddot(self.h, level, _ptr(x), _ptr(y), C.addressof(out)))
-        return out.value
+        b = self._ncomp()
+        return out[0] if b == 1 else np.array(out[:b])
 
     def jacobi_step(self, level, x, b):
         self._chk(self.lib.ugmg_jacobi_step(self.h, level, _ptr(x), _ptr(b)))
@@ -315,6 +322,15 @@ class Context:
         self._chk(self.lib.ugmg_solve_host(self.h, level, x.ctypes.data, b.ctypes.data, C.addressof(ab),
                                            C.addressof(rd), maxiter, int(do_defect), C.byref(res), cb, None))
         return x, b, res, np.array(defects)
+
+    def galerkin(self, level):
+        """Level-1 values := P^T A P on the GPU (then preprocess again)."""
+        self._chk(self.lib.ugmg_galerkin(self.h, level))
+
+    def get_level_values(self, level):
+        """The context's values of a level (ConvertMatrix order) via the
+        device pattern check (after preprocess)."""
+        return self.pattern(level)[2]
 
     # ------------------------------------------------------- several columns
     def set_batch(self, nrhs):

@@ -1,0 +1,70 @@
+"""A UG script through the numproc layer on the GPU: the configuration of
+UG's config-1 run (ls + lmgc V(2,2), jac $damp 0.8, ex base) executed with
+npexecute's flags gives the oracle's solve; bcgs with an lmgc $I likewise."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def test_ls_script_is_the_oracle_solve(poisson2d):
+    from ug_amd import numproc as NP
+    from ug_amd import ugmg
+    P = poisson2d
+    ctx = ugmg.Context(0)
+    ctx.set_hierarchy(P)
+    env = NP.NumProcEnv(ctx)
+    x = env.vector("sol", np.zeros(P.levels[-1].n))
+    b = env.vector("rhs", P.rhs)
+    env.create("transfer", "transfer"); env.init("transfer")
+    env.create("jac", "jac"); env.init("jac", "$damp 0.8")
+    env.create("ex", "ex"); env.init("ex")
+    env.create("base", "ls"); env.init("base", "$red 1e-12 $m 1 $I ex $abslimit 1e-300")
+    env.create("mgc", "lmgc"); env.init("mgc", "$S jac jac base $T transfer $n1 2 $n2 2 $b 0 $graph 1")
+    env.create("mgs", "ls")
+    assert env.init("mgs", "$A MAT $x sol $b rhs $m 50 $red 1e-10 $abslimit 1e-300 $I mgc") == NP.EXECUTABLE
+    print("npexecute mgs", flush=True)
+    env.execute("mgs", "$i $d $r $s $p")
+    r = env.result("mgs")
+    want = O.ls_solve(O.Hierarchy(P.levels), P.top, O.params(smoother=O.JAC, damp=0.8, baselevel=0), P.rhs,
+                      red=1e-10, maxiter=50)
+    assert r.converged and r.number_of_linear_iterations == want["nit"]
+    assert abs(r.last_defect[0] - want["defects"][-1, 0]) < 1e-10 * want["defects"][-1, 0]
+    assert np.max(np.abs(x - want["x"])) < 1e-10 * np.max(np.abs(want["x"]))
+    assert np.max(np.abs(b - want["b"])) < 1e-9 * np.max(np.abs(P.rhs))
+    env.close()
+    ctx.close()
+
+
+def test_bcgs_script_with_two_rhs():
+    from ug_amd import numproc as NP
+    from ug_amd import problems, ugmg
+    P = problems.p1_cd(5, 1e-2, 0.7, seed=3)
+    n = P.levels[-1].n
+    R = np.stack([P.rhs, np.where(P.levels[-1].skip == 1, 0.0, np.random.default_rng(2).uniform(-1, 1, n))])
+    ctx = ugmg.Context(0)
+    ctx.set_hierarchy(P)
+    env = NP.NumProcEnv(ctx)
+    X = env.vector("x", np.zeros((2, n)))
+    B = env.vector("b", R.copy())
+    env.create("transfer", "transfer"); env.init("transfer")
+    env.create("gs", "gs"); env.init("gs")
+    env.create("ex", "ex"); env.init("ex")
+    env.create("base", "ls"); env.init("base", "$red 1e-12 $m 1 $I ex $abslimit 1e-300")
+    env.create("mgc", "lmgc"); env.init("mgc", "$S gs gs base $T transfer $n1 1 $n2 1 $b 0")
+    env.create("k", "bcgs")
+    assert env.init("k", "$A MAT $x x $b b $m 100 $red 1e-10 $abslimit 1e-300 $I mgc $nrhs 2") == NP.EXECUTABLE
+    env.execute("k", "$i $d $r $s")
+    H = O.Hierarchy(P.levels)
+    for j in range(2):
+        want = O.bcgs_solve(H, P.top, O.params(smoother=O.MCGS, damp=1.0, nu1=1, nu2=1), R[j], red=1e-10,
+                            maxiter=100)
+        r = env.result("k", j)
+        assert r.converged and r.number_of_linear_iterations == want["nit"]
+        assert np.max(np.abs(X[j] - want["x"])) < 1e-8 * np.max(np.abs(want["x"]))
+    env.close()
+    ctx.close()

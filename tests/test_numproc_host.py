@@ -1,0 +1,115 @@
+"""The numproc layer on the CPU (no GPU touched): UG's option syntax,
+defaults and returned states of npinit for the classes of the solve path
+(LinearSolverInit ls.cc:771-797, NPLinearSolverInit ls.cc:382-407,
+BCGSInit ls.cc:1750-1800, LmgcInit iter.cc:7613-7656, SmootherInit
+iter.cc:763-790, EXInit iter.cc:8248-8260), and the parameters the chain
+hands to libugmg."""
+import numpy as np
+import pytest
+
+from ug_amd import numproc as NP
+from ug_amd import ugmg
+
+
+def script(env, smoother="gs", damp="1.0", extra_mg=""):
+    env.create("transfer", "transfer")
+    assert env.init("transfer") == NP.EXECUTABLE
+    for s in ("pre", "post"):
+        env.create(s, smoother)
+        assert env.init(s, f"$damp {damp}") == NP.ACTIVE          # no $A $c $r
+    env.create("ex", "ex")
+    assert env.init("ex") == NP.ACTIVE
+    env.create("base", "ls")
+    assert env.init("base", "$red 1e-12 $m 1 $I ex $abslimit 1e-300") == NP.ACTIVE   # no $x $b $A
+    env.create("mgc", "lmgc")
+    return env.init("mgc", f"$S pre post base $T transfer $n1 2 $n2 2 $b 4 $damp 1.0 {extra_mg}")
+
+
+def test_script_chain_and_params():
+    env = NP.NumProcEnv(None)
+    assert script(env) == NP.ACTIVE
+    env.vector("sol", np.zeros(10))
+    env.vector("rhs", np.ones(10))
+    env.create("mgs", "ls")
+    assert env.init("mgs", "$A MAT $x sol $b rhs $m 100 $red 1e-10 $I mgc") == NP.EXECUTABLE
+    p = env.params("mgs")
+    assert (p.nu1, p.nu2, p.gamma, p.baselevel, p.smoother) == (2, 2, 1, 4, ugmg.SMOOTH_GS)
+    assert p.damp[0] == 1.0 and p.lmgc_damp[0] == 1.0 and p.base_damp[0] == 1.0
+    assert p.base_maxiter == 1 and p.base_red[0] == 1e-12 and p.base_abslimit[0] == 1e-300
+    assert p.ex_reorder == 1 and p.precision == ugmg.FP64 and p.inner_cycles == 1
+    # execution needs a context
+    with pytest.raises(ugmg.UgmgError):
+        env.execute("mgs", "$i $s")
+
+
+def test_defaults_and_states():
+    env = NP.NumProcEnv(None)
+    env.create("t", "transfer")
+    env.init("t")
+    env.create("j", "jac")
+    assert env.init("j") == NP.ACTIVE
+    env.create("ex", "ex")
+    env.init("ex", "$o 0")
+    env.create("base", "ls")
+    env.init("base", "$m 2 $I ex $red 1e-6")
+    env.create("mg", "lmgc")
+    assert env.init("mg", "$S j j base $T t") == NP.ACTIVE
+    env.create("s", "ls")
+    env.vector("x", np.zeros(4))
+    env.vector("b", np.zeros(4))
+    assert env.init("s", "$x x $b b $A A $m 5 $I mg") == NP.ACTIVE      # $red missing
+    assert env.init("s", "$x x $b b $A A $m 5 $I mg $red 1e-8") == NP.EXECUTABLE
+    p = env.params("s")
+    # LmgcInit defaults g = n1 = n2 = 1, b = 0; lmgc and smoother $damp 1
+    assert (p.gamma, p.nu1, p.nu2, p.baselevel) == (1, 1, 1, 0)
+    assert p.damp[2] == 1.0 and p.lmgc_damp[1] == 1.0
+    assert p.ex_reorder == 0 and p.base_maxiter == 2 and p.base_red[0] == 1e-6
+    assert p.base_abslimit[0] == 1e-10            # ABS_LIMIT (ls.cc:66)
+    assert p.smoother == ugmg.SMOOTH_JAC
+
+
+def test_error_behaviour():
+    env = NP.NumProcEnv(None)
+    assert script(env) == NP.ACTIVE
+    env.create("a", "ls")
+    assert env.init("a", "$red 1e-8 $I mgc") == NP.NOT_ACTIVE        # $m missing
+    assert env.init("a", "$m 3 $red 1e-8 $I nothere") == NP.NOT_ACTIVE
+    env.create("k", "bcgs")
+    assert env.init("k", "$m 3 $R -1 $red 1e-8") == NP.NOT_ACTIVE    # $R < 0
+    assert env.init("k", "$m 3 $red 1e-8 $weight 2") == NP.NOT_ACTIVE
+    assert env.init("k", "$m 3 $red 1e-8") == NP.ACTIVE               # no vectors; no $I is allowed
+    env.create("m2", "lmgc")
+    assert env.init("m2", "$S pre post base") == NP.NOT_ACTIVE        # $T missing
+    assert env.init("m2", "$S pre post base $T transfer $precision half") == NP.NOT_ACTIVE
+    # ReadArgvINT matches the whole option name: "$nn1 5" is not $n1
+    assert env.init("m2", "$S pre post base $T transfer $nn1 5") == NP.ACTIVE
+    with pytest.raises(ugmg.UgmgError):
+        env.create("m2", "lmgc")                                      # name taken
+    with pytest.raises(ugmg.UgmgError):
+        env.create("q", "cg")                                         # not a class of this path
+
+
+def test_pre_and_post_must_match():
+    env = NP.NumProcEnv(None)
+    script(env)
+    env.create("post2", "gs")
+    env.init("post2", "$damp 0.7")
+    env.create("mg2", "lmgc")
+    env.init("mg2", "$S pre post2 base $T transfer")
+    env.vector("x", np.zeros(2))
+    env.vector("b", np.zeros(2))
+    env.create("s", "ls")
+    env.init("s", "$x x $b b $A A $m 5 $I mg2 $red 1e-8")
+    with pytest.raises(ugmg.UgmgError):
+        env.params("s")
+
+
+def test_gpu_options():
+    env = NP.NumProcEnv(None)
+    assert script(env, extra_mg="$precision mixed $inner 3 $graph 0") == NP.ACTIVE
+    env.vector("x", np.zeros(2))
+    env.vector("b", np.zeros(2))
+    env.create("s", "bcgs")
+    assert env.init("s", "$x x $b b $A A $m 50 $R 10 $I mgc $red 1e-8 $nrhs 4") == NP.EXECUTABLE
+    p = env.params("s")
+    assert p.precision == ugmg.MIXED and p.inner_cycles == 3 and p.use_graph == 0

@@ -1,6 +1,6 @@
 """Build recipe for the native parts (called by __graft_entry__.build()).
 
-  ug_amd/libugmg.so           HIP kernels + C ABI (hipcc, gfx950)
+  ug_amd/libugmg.so           HIP kernels + C ABI (hipcc, gfx950) + numproc layer
   ug_amd/libugmg_problems.so  synthetic UG-layout problems (g++, host only)
   oracle/liboracle.so         CPU restatement used by tests (gcc, test only)
 """
@@ -47,6 +47,9 @@ def build_ugmg():
         objs.append(obj)
     obj = os.path.join(BUILD, "ugmg_setup.o")
     _run(["g++", *CXX_FLAGS, "-fopenmp", "-c", os.path.join(CSRC, "ugmg_setup.cc"), "-o", obj])
+    objs.append(obj)
+    obj = os.path.join(BUILD, "ugmg_np.o")   # numproc layer over the C ABI (host only)
+    _run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, "ugmg_np.cc"), "-o", obj])
     objs.append(obj)
     out = os.path.join(PKG, "libugmg.so")
     tmp = out + ".tmp"

@@ -1,0 +1,424 @@
+// Numproc layer over the C ABI: npcreate / npinit / npexecute for the
+// solve path (include/ugmg_np.h).  Option parsing follows UG's helpers:
+//   ReadArgvINT / ReadArgvDOUBLE  low/scan.cc:103-160  (first character of
+//                                 the argument, then "%s %d" / "%s %lf")
+//   ReadArgvOption                low/scan.cc:316-333  (flag, or flag k)
+//   sc_read                       np/udm/npscan.cc:1211-1280 (one value for
+//                                 every component; absent -> 2)
+// and each class reads the options its UG Init function reads, with the
+// same defaults and the same returned state.
+#include "../../include/ugmg_np.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <new>
+#include <string>
+#include <vector>
+
+namespace {
+
+constexpr double kAbsLimit = 1e-10;   // ABS_LIMIT, np/procs/ls.cc:66
+
+using Argv = std::vector<std::string>;
+
+// the command line split at '$' (UG's option separator), trimmed
+Argv split_argv(const char *s) {
+  Argv av;
+  if (!s) return av;
+  std::string cur;
+  auto flush = [&]() {
+    size_t a = cur.find_first_not_of(" \t\n"), b = cur.find_last_not_of(" \t\n");
+    if (a != std::string::npos) av.push_back(cur.substr(a, b - a + 1));
+    cur.clear();
+  };
+  for (const char *p = s; *p; ++p) {
+    if (*p == '$') flush();
+    else cur.push_back(*p);
+  }
+  flush();
+  return av;
+}
+
+int read_int(const Argv &av, const char *name, int *out) {
+  for (const auto &a : av) {
+    if (a.empty() || a[0] != name[0]) continue;
+    char opt[128];
+    int v;
+    if (std::sscanf(a.c_str(), "%127s %d", opt, &v) != 2) continue;
+    if (std::strcmp(opt, name) == 0) {
+      *out = v;
+      return 0;
+    }
+  }
+  return 1;
+}
+
+int read_option(const Argv &av, const char *name) {
+  for (const auto &a : av) {
+    if (a.empty() || a[0] != name[0]) continue;
+    char opt[128];
+    int v;
+    if (std::sscanf(a.c_str(), "%127s %d", opt, &v) == 2 && std::strcmp(opt, name) == 0) return v;
+    if (a == name) return 1;
+  }
+  return 0;
+}
+
+// "name value": the value string (the rest of the argument)
+bool read_string(const Argv &av, const char *name, std::string &out) {
+  const size_t ln = std::strlen(name);
+  for (const auto &a : av)
+    if (a.size() > ln && a.compare(0, ln, name) == 0 && (a[ln] == ' ' || a[ln] == '\t')) {
+      size_t p = a.find_first_not_of(" \t", ln);
+      out = p == std::string::npos ? "" : a.substr(p);
+      return true;
+    }
+  return false;
+}
+
+// sc_read: 0 read, 2 absent, 3 unreadable; one value for every component
+int sc_read(const Argv &av, const char *name, double x[UGMG_MAXB]) {
+  std::string v;
+  if (!read_string(av, name, v)) return 2;
+  double d;
+  if (std::sscanf(v.c_str(), "%lf", &d) != 1) return 3;
+  for (int i = 0; i < UGMG_MAXB; ++i) x[i] = d;
+  return 0;
+}
+
+struct NumProc {
+  std::string cls;
+  int state = UGMG_NP_NOT_INIT;
+  Argv av;
+  // ls / bcgs
+  int m = 0, restart = 0, nrhs = 1;
+  double red[UGMG_MAXB] = {0, 0, 0}, abslimit[UGMG_MAXB] = {kAbsLimit, kAbsLimit, kAbsLimit};
+  double weight[UGMG_MAXB] = {1, 1, 1};
+  std::string I, x, b, A, c, r;
+  std::vector<ugmg_lresult> res;
+  bool pending_defect = false;
+  ugmg_params params{};
+  // lmgc
+  std::string T, pre, post, base;
+  int gamma = 1, nu1 = 1, nu2 = 1, bl = 0, precision = UGMG_FP64, inner = 1, graph = 1;
+  // smoothers
+  double damp[UGMG_MAXB] = {1, 1, 1};
+  int reorder = 1;
+};
+
+bool is_iter(const std::string &c) { return c == "lmgc" || c == "jac" || c == "gs" || c == "ex"; }
+bool is_ls(const std::string &c) { return c == "ls" || c == "bcgs"; }
+
+}  // namespace
+
+struct ugmg_np_env {
+  ugmg_ctx *ctx = nullptr;
+  int level = -1;
+  std::map<std::string, NumProc> np;
+  std::map<std::string, std::pair<double *, int>> vec;
+  std::string err;
+};
+
+namespace {
+
+int fail(ugmg_np_env *e, int code, const std::string &msg) {
+  e->err = msg;
+  return code;
+}
+
+NumProc *find(ugmg_np_env *e, const std::string &name) {
+  auto it = e->np.find(name);
+  return it == e->np.end() ? nullptr : &it->second;
+}
+
+// NPIterInit (iter.cc:674-684): $A $c $r
+int iter_state(const Argv &av, NumProc &o) {
+  read_string(av, "A", o.A);
+  read_string(av, "c", o.c);
+  read_string(av, "r", o.r);
+  return (o.A.empty() || o.c.empty() || o.r.empty()) ? UGMG_NP_ACTIVE : UGMG_NP_EXECUTABLE;
+}
+
+// NPLinearSolverInit (ls.cc:382-407)
+int ls_state(ugmg_np_env *e, const Argv &av, NumProc &o) {
+  read_string(av, "A", o.A);
+  read_string(av, "x", o.x);
+  read_string(av, "b", o.b);
+  if (sc_read(av, "abslimit", o.abslimit))
+    for (int i = 0; i < UGMG_MAXB; ++i) o.abslimit[i] = kAbsLimit;
+  if (sc_read(av, "red", o.red)) return UGMG_NP_ACTIVE;
+  if (o.x.empty() || o.b.empty() || o.A.empty() || !e->vec.count(o.x) || !e->vec.count(o.b))
+    return UGMG_NP_ACTIVE;
+  return UGMG_NP_EXECUTABLE;
+}
+
+int init_object(ugmg_np_env *e, NumProc &o) {
+  const Argv &av = o.av;
+  if (o.cls == "ls") {   // LinearSolverInit (ls.cc:771-797)
+    if (read_int(av, "m", &o.m)) return fail(e, UGMG_NP_NOT_ACTIVE, "ls: $m missing");
+    if (!read_string(av, "I", o.I) || !find(e, o.I) || !is_iter(find(e, o.I)->cls))
+      return fail(e, UGMG_NP_NOT_ACTIVE, "ls: $I does not name an iteration");
+    return ls_state(e, av, o);
+  }
+  if (o.cls == "bcgs") {   // BCGSInit (ls.cc:1750-1800)
+    if (sc_read(av, "weight", o.weight))
+      for (int i = 0; i < UGMG_MAXB; ++i) o.weight[i] = 1.0;
+    for (int i = 0; i < UGMG_MAXB; ++i) o.weight[i] *= o.weight[i];
+    if (read_int(av, "m", &o.m)) return fail(e, UGMG_NP_NOT_ACTIVE, "bcgs: $m missing");
+    if (read_int(av, "R", &o.restart)) o.restart = 0;
+    if (o.restart < 0) return fail(e, UGMG_NP_NOT_ACTIVE, "bcgs: $R < 0");
+    for (int i = 0; i < UGMG_MAXB; ++i)
+      if (o.weight[i] != 1.0) return fail(e, UGMG_NP_NOT_ACTIVE, "bcgs: only unit $weight runs on the GPU");
+    o.I.clear();
+    if (read_string(av, "I", o.I) && (!find(e, o.I) || find(e, o.I)->cls != "lmgc"))
+      return fail(e, UGMG_NP_NOT_ACTIVE, "bcgs: $I must name an lmgc object");
+    if (read_int(av, "nrhs", &o.nrhs)) o.nrhs = 1;
+    if (o.nrhs < 1) return fail(e, UGMG_NP_NOT_ACTIVE, "bcgs: $nrhs < 1");
+    return ls_state(e, av, o);
+  }
+  if (o.cls == "lmgc") {   // LmgcInit (iter.cc:7613-7656)
+    read_string(av, "T", o.T);
+    o.pre.clear();
+    for (const auto &a : av)
+      if (!a.empty() && a[0] == 'S') {
+        char p1[128], p2[128], p3[128];
+        if (std::sscanf(a.c_str(), "S %127s %127s %127s", p1, p2, p3) != 3) continue;
+        o.pre = p1;
+        o.post = p2;
+        o.base = p3;
+        break;
+      }
+    if (read_int(av, "g", &o.gamma)) o.gamma = 1;
+    if (read_int(av, "n1", &o.nu1)) o.nu1 = 1;
+    if (read_int(av, "n2", &o.nu2)) o.nu2 = 1;
+    if (read_int(av, "b", &o.bl)) o.bl = 0;
+    NumProc *t = find(e, o.T), *pr = find(e, o.pre), *po = find(e, o.post), *ba = find(e, o.base);
+    if (!t || t->cls != "transfer") return fail(e, UGMG_NP_NOT_ACTIVE, "lmgc: $T does not name a transfer");
+    if (!pr || !po || !is_iter(pr->cls) || !is_iter(po->cls))
+      return fail(e, UGMG_NP_NOT_ACTIVE, "lmgc: $S pre/post do not name iterations");
+    if (!ba || ba->cls != "ls") return fail(e, UGMG_NP_NOT_ACTIVE, "lmgc: $S base does not name an ls");
+    if (sc_read(av, "damp", o.damp))
+      for (int i = 0; i < UGMG_MAXB; ++i) o.damp[i] = 1.0;
+    std::string prec;
+    o.precision = UGMG_FP64;
+    if (read_string(av, "precision", prec)) {
+      if (prec == "mixed") o.precision = UGMG_MIXED;
+      else if (prec != "fp64") return fail(e, UGMG_NP_NOT_ACTIVE, "lmgc: $precision is fp64 or mixed");
+    }
+    if (read_int(av, "inner", &o.inner)) o.inner = 1;
+    if (read_int(av, "graph", &o.graph)) o.graph = 1;
+    return iter_state(av, o);
+  }
+  if (o.cls == "jac" || o.cls == "gs") {   // SmootherInit (iter.cc:763-790)
+    for (int i = 0; i < UGMG_MAXB; ++i) o.damp[i] = 1.0;
+    sc_read(av, "damp", o.damp);
+    return iter_state(av, o);
+  }
+  if (o.cls == "ex") {   // EXInit (iter.cc:8248-8260)
+    if (read_int(av, "o", &o.reorder)) o.reorder = 1;
+    for (int i = 0; i < UGMG_MAXB; ++i) o.damp[i] = 1.0;
+    sc_read(av, "damp", o.damp);
+    return iter_state(av, o);
+  }
+  if (o.cls == "transfer") return UGMG_NP_EXECUTABLE;
+  return fail(e, UGMG_E_ARG, "unknown class " + o.cls);
+}
+
+// the ugmg_params a solver object's $I chain describes
+int build_params(ugmg_np_env *e, NumProc &o, ugmg_params &p, bool &use_iter) {
+  ugmg_params_default(&p);
+  use_iter = !o.I.empty();
+  if (!use_iter) return 0;
+  NumProc *mg = find(e, o.I);
+  if (!mg || mg->cls != "lmgc" || mg->state < UGMG_NP_ACTIVE)
+    return fail(e, UGMG_E_STATE, "$I must be an initialised lmgc");
+  NumProc *pr = find(e, mg->pre), *po = find(e, mg->post), *ba = find(e, mg->base);
+  if (!pr || !po || !ba) return fail(e, UGMG_E_STATE, "lmgc parts missing");
+  if (pr->cls != po->cls || pr->cls == "ex" || pr->cls == "lmgc")
+    return fail(e, UGMG_E_ARG, "pre- and post-smoother must be the same jac or gs class");
+  for (int i = 0; i < UGMG_MAXB; ++i)
+    if (pr->damp[i] != po->damp[i]) return fail(e, UGMG_E_ARG, "pre- and post-smoother $damp differ");
+  NumProc *ex = find(e, ba->I);
+  if (!ex || ex->cls != "ex") return fail(e, UGMG_E_ARG, "the base ls must iterate with ex");
+  p.nu1 = mg->nu1;
+  p.nu2 = mg->nu2;
+  p.gamma = mg->gamma;
+  p.baselevel = mg->bl;
+  p.smoother = pr->cls == "jac" ? UGMG_SMOOTH_JAC : UGMG_SMOOTH_GS;
+  for (int i = 0; i < UGMG_MAXB; ++i) {
+    p.damp[i] = pr->damp[i];
+    p.lmgc_damp[i] = mg->damp[i];
+    p.base_damp[i] = ex->damp[i];
+    p.base_abslimit[i] = ba->abslimit[i];
+    p.base_red[i] = ba->red[i];
+  }
+  p.base_maxiter = ba->m;
+  p.ex_reorder = ex->reorder;
+  p.precision = mg->precision;
+  p.inner_cycles = mg->inner;
+  p.use_graph = mg->graph;
+  return 0;
+}
+
+int level_n(ugmg_np_env *e, int level, int *n) {
+  int nn = 0;
+  int rc = ugmg_level_size(e->ctx, level, &nn, nullptr);
+  if (rc) return fail(e, rc, "level not set");
+  *n = nn;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ugmg_np_env_create(ugmg_ctx *ctx, ugmg_np_env **out) {
+  if (!out) return UGMG_E_ARG;
+  *out = new (std::nothrow) ugmg_np_env;
+  if (!*out) return UGMG_E_NOMEM;
+  (*out)->ctx = ctx;
+  return UGMG_OK;
+}
+
+int ugmg_np_env_destroy(ugmg_np_env *env) {
+  delete env;
+  return UGMG_OK;
+}
+
+const char *ugmg_np_last_error(const ugmg_np_env *env) { return env ? env->err.c_str() : "null environment"; }
+
+int ugmg_np_set_level(ugmg_np_env *env, int level) {
+  if (!env || level < 0) return UGMG_E_ARG;
+  env->level = level;
+  return UGMG_OK;
+}
+
+int ugmg_np_set_vector(ugmg_np_env *env, const char *name, double *data, int ncols) {
+  if (!env || !name || !data || ncols < 1) return UGMG_E_ARG;
+  env->vec[name] = {data, ncols};
+  return UGMG_OK;
+}
+
+int ugmg_np_create(ugmg_np_env *env, const char *name, const char *cls) {
+  if (!env || !name || !cls) return UGMG_E_ARG;
+  const std::string c = cls;
+  if (!is_iter(c) && !is_ls(c) && c != "transfer") return fail(env, UGMG_E_ARG, "unknown class " + c);
+  if (env->np.count(name)) return fail(env, UGMG_E_ARG, std::string("numproc ") + name + " exists");
+  NumProc o;
+  o.cls = c;
+  env->np[name] = o;
+  return UGMG_OK;
+}
+
+int ugmg_np_init(ugmg_np_env *env, const char *name, const char *options) {
+  if (!env || !name) return UGMG_E_ARG;
+  NumProc *o = find(env, name);
+  if (!o) return fail(env, UGMG_E_ARG, std::string("no numproc ") + name);
+  o->av = split_argv(options);
+  env->err.clear();
+  const int st = init_object(env, *o);
+  if (st < 0) return st;
+  o->state = st;
+  return st;
+}
+
+int ugmg_np_execute(ugmg_np_env *env, const char *name, const char *flags) {
+  if (!env || !name) return UGMG_E_ARG;
+  NumProc *o = find(env, name);
+  if (!o) return fail(env, UGMG_E_ARG, std::string("no numproc ") + name);
+  if (!env->ctx) return fail(env, UGMG_E_STATE, "no context (environment created without a GPU)");
+  const Argv av = split_argv(flags);
+  int level = env->level;
+  if (level < 0) {
+    level = 0;
+    while (ugmg_level_size(env->ctx, level + 1, nullptr, nullptr) == UGMG_OK) ++level;
+  }
+  int n = 0, rc;
+  if ((rc = level_n(env, level, &n))) return rc;
+  if (is_iter(o->cls)) {   // NPIterExecute: $i PreProcess, $s Iter (c, r), $p PostProcess
+    if (o->cls != "lmgc") return fail(env, UGMG_E_ARG, "only lmgc objects run on their own");
+    NumProc fake;
+    fake.I = name;
+    ugmg_params p;
+    bool use_iter;
+    if (read_option(av, "i")) {
+      if ((rc = build_params(env, fake, p, use_iter))) return rc;
+      if ((rc = ugmg_set_params(env->ctx, &p)) || (rc = ugmg_preprocess(env->ctx, level)))
+        return fail(env, rc, ugmg_last_error(env->ctx));
+    }
+    if (read_option(av, "s")) {
+      if (o->state != UGMG_NP_EXECUTABLE || !env->vec.count(o->c) || !env->vec.count(o->r))
+        return fail(env, UGMG_E_STATE, "lmgc: $c and $r must name registered vectors");
+      if ((rc = ugmg_iter_host(env->ctx, level, env->vec[o->c].first, env->vec[o->r].first)))
+        return fail(env, rc, ugmg_last_error(env->ctx));
+    }
+    if (read_option(av, "p")) ugmg_postprocess(env->ctx);
+    return UGMG_OK;
+  }
+  if (!is_ls(o->cls)) return fail(env, UGMG_E_ARG, "transfer objects are not executed");
+  if (o->state != UGMG_NP_EXECUTABLE) return fail(env, UGMG_E_STATE, std::string(name) + " is not executable");
+  // NPLinearSolverExecute (ls.cc:436-535): $i, $d, $r, $s, $p in this order
+  double *x = env->vec[o->x].first, *b = env->vec[o->b].first;
+  {
+    ugmg_params p;
+    bool ui;
+    if (read_option(av, "i")) {
+      if ((rc = build_params(env, *o, p, ui))) return rc;
+      o->params = p;
+      if (o->cls == "bcgs" && (rc = ugmg_set_batch(env->ctx, o->nrhs)))
+        return fail(env, rc, ugmg_last_error(env->ctx));
+      if ((rc = ugmg_set_params(env->ctx, &p)) || (rc = ugmg_preprocess(env->ctx, level)))
+        return fail(env, rc, ugmg_last_error(env->ctx));
+    }
+  }
+  const int ncols = o->cls == "bcgs" ? o->nrhs : 1;
+  o->res.assign(ncols, ugmg_lresult{});
+  if (read_option(av, "d")) o->pending_defect = true;   // formed by the next Residuum/Solver
+  const bool want_r = read_option(av, "r") != 0, want_s = read_option(av, "s") != 0;
+  if (want_r && !want_s) {
+    // LinearResiduum alone: LinearDefect (if pending) and the surface norm,
+    // through a zero-iteration solve
+    ugmg_lresult r;
+    if ((rc = ugmg_solve_host(env->ctx, level, x, b, o->abslimit, o->red, 0, o->pending_defect ? 1 : 0, &r,
+                              nullptr, nullptr)))
+      return fail(env, rc, ugmg_last_error(env->ctx));
+    o->pending_defect = false;
+    o->res[0] = r;
+  }
+  if (want_s) {
+    if (o->cls == "ls") {
+      if ((rc = ugmg_solve_host(env->ctx, level, x, b, o->abslimit, o->red, o->m, o->pending_defect ? 1 : 0,
+                                &o->res[0], nullptr, nullptr)))
+        return fail(env, rc, ugmg_last_error(env->ctx));
+    } else {
+      if ((rc = ugmg_bcgs_solve_host(env->ctx, level, ncols, x, b, n, o->abslimit, o->red, o->m,
+                                     o->restart, o->I.empty() ? 0 : 1, o->pending_defect ? 1 : 0,
+                                     o->res.data(), nullptr, 0)))
+        return fail(env, rc, ugmg_last_error(env->ctx));
+    }
+    o->pending_defect = false;
+  }
+  if (read_option(av, "p")) ugmg_postprocess(env->ctx);
+  return UGMG_OK;
+}
+
+int ugmg_np_result(const ugmg_np_env *env, const char *name, int col, ugmg_lresult *res) {
+  if (!env || !name || !res) return UGMG_E_ARG;
+  auto it = env->np.find(name);
+  if (it == env->np.end() || col < 0 || col >= (int)it->second.res.size()) return UGMG_E_ARG;
+  *res = it->second.res[col];
+  return UGMG_OK;
+}
+
+int ugmg_np_params(const ugmg_np_env *env, const char *name, ugmg_params *p) {
+  if (!env || !name || !p) return UGMG_E_ARG;
+  auto it = env->np.find(name);
+  if (it == env->np.end()) return UGMG_E_ARG;
+  bool ui;
+  NumProc o = it->second;
+  return build_params(const_cast<ugmg_np_env *>(env), o, *p, ui);
+}
+
+}  // extern "C"

@@ -1188,7 +1188,7 @@ int ugmg_restrict(ugmg_ctx *ctx, int level, double *to_c, const double *from_f,
     cd = dc.b;
   }
   launch_restrict<double>(ctx->stream, d.PT.d64(), dc.skip, dc.vnclass, dc.vclass, fd,
-                          damp ? damp[0] : 1.0, cd, nullptr, d.nt, 1, d.vs, dc.vs);
+                          damp ? damp[0] : 1.0, cd, nullptr, d.nt, 1, d.vs, dc.vs, d.nb);
   if (dc.perm) to_user<double, double>(ctx, level - 1, cd, to_c, dc.vs);
   HIPCHK(hipGetLastError());
   return UGMG_OK;
@@ -1207,7 +1207,7 @@ int ugmg_interpolate(ugmg_ctx *ctx, int level, double *to_f, const double *from_
     cd = dc.c;
   }
   launch_interp<double>(ctx->stream, d.P.d64(), d.skip, cd, damp ? damp[0] : 1.0, fd,
-                        nullptr, d.nt, 1, d.vs, dc.vs);
+                        nullptr, d.nt, 1, d.vs, dc.vs, d.nb);
   if (d.perm) to_user<double, double>(ctx, level, fd, to_f, d.vs);
   HIPCHK(hipGetLastError());
   return UGMG_OK;
@@ -1642,12 +1642,12 @@ int ugmg_time_kernel(ugmg_ctx *ctx, int level, int kind, int reps, double *ms, d
       case 4: {
         DevLevel &dc = ctx->dev[level - 1];
         launch_restrict<double>(st, d.PT.d64(), dc.skip, dc.vnclass, dc.vclass, d.b, 1.0, dc.b,
-                                nullptr, d.nt, 1, d.vs, dc.vs);
+                                nullptr, d.nt, 1, d.vs, dc.vs, d.nb);
         break;
       }
       case 5: {
         DevLevel &dc = ctx->dev[level - 1];
-        launch_interp<double>(st, d.P.d64(), d.skip, dc.c, 1.0, d.t, d.c, d.nt, 1, d.vs, dc.vs);
+        launch_interp<double>(st, d.P.d64(), d.skip, dc.c, 1.0, d.t, d.c, d.nt, 1, d.vs, dc.vs, d.nb);
         break;
       }
       case 6:

@@ -121,7 +121,7 @@ void launch_gs_defect(hipStream_t st, const DSell<TV> &U, const T *v, T *b, cons
 template <typename T>
 int launch_restrict(hipStream_t st, const DSell<T> &PT, const uint32_t *cskip,
                     const uint8_t *cvnclass, const uint8_t *cvclass, const T *bf, T damp, T *bc,
-                    double *part, bool nt, int nrhs, size_t vsf, size_t vsc, int b = 1);
+                    double *part, bool nt, int nrhs, size_t vsf, size_t vsc, int b);
 template <typename T>
 void launch_base_gemv(hipStream_t st, int n, const double *inv, const T *b, T *c, T *xout,
                       int cmode, double damp, double abslimit, double red, const double *part,
@@ -133,7 +133,7 @@ int launch_dot_part(hipStream_t st, int n, const T *x, const uint8_t *vclass, do
                     Batch bt);
 template <typename T>
 void launch_interp(hipStream_t st, const DSell<T> &P, const uint32_t *fskip, const T *cc, T damp,
-                   T *t, T *c, bool nt, int nrhs, size_t vsf, size_t vsc, int b = 1);
+                   T *t, T *c, bool nt, int nrhs, size_t vsf, size_t vsc, int b);
 int reduce_parts(int n);
 template <typename T>
 void launch_dot(hipStream_t st, int n, const T *x, const T *y, const uint8_t *vclass,
@@ -151,10 +151,10 @@ void launch_add(hipStream_t st, int n, T *x, const T *y, Batch bt);
 // scatter d[perm[p]] = s[p]; column strides vss (source), vsd (destination)
 template <typename TD, typename TS>
 void launch_gather(hipStream_t st, int n, const int *perm, const TS *s, TD *d, int nrhs,
-                   size_t vss, size_t vsd, int b = 1);
+                   size_t vss, size_t vsd, int b);
 template <typename TD, typename TS>
 void launch_scatter(hipStream_t st, int n, const int *perm, const TS *s, TD *d, int nrhs,
-                    size_t vss, size_t vsd, int b = 1);
+                    size_t vss, size_t vsd, int b);
 template <typename T>
 void launch_coarse_cycle(hipStream_t st, const CoarseArgs<T> &args, int base_n);
 // coarse values ca (pattern crp/cci, bb values per entry) = P^T A P; pt*:

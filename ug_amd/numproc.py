@@ -80,8 +80,9 @@ class NumProcEnv:
         self._chk(self.lib.ugmg_np_set_level(self.h, level))
 
     def vector(self, name, array):
-        """Register a float64 host array (kept alive; updated in place)."""
-        a = np.ascontiguousarray(array, dtype=np.float64)
+        """Register a copy of `array` as the named vector; returns the copy,
+        which npexecute updates in place (the caller's array is untouched)."""
+        a = np.array(array, dtype=np.float64, copy=True, order="C")
         ncols = a.shape[0] if a.ndim == 2 else 1
         self._vecs[name] = a
         self._chk(self.lib.ugmg_np_set_vector(self.h, name.encode(), a.ctypes.data, ncols))

@@ -204,6 +204,45 @@ int ugmg_bcgs_solve_host(ugmg_ctx *ctx, int level, int nrhs, double *x,
  * top down it builds a rediscretisation-free hierarchy. */
 int ugmg_galerkin(ugmg_ctx *ctx, int level);
 
+/* ---- device-side assembly of a level from element lists -----------------
+ * The application's element loop with AddVlistMValues / AddVlistVValues
+ * (np/udm/disctools.cc:536-555, 841-950) and AssembleDirichletBoundary
+ * (disctools.cc:1837-1875) run on the GPU, in place of the host flatten
+ * (ConvertMatrix, disctools.cc:2061-2128) behind ugmg_set_level: the level
+ * is then set as ugmg_set_level would set it.  Every array in
+ * ugmg_elements is DEVICE memory.  Vector v couples with every vector
+ * sharing an element with it; a row lists its diagonal block, then the
+ * coupled vectors in ascending order; every entry is 0 plus its element
+ * contributions in element order (scale[e]*kref rounded before the add).
+ * Element matrices are (nc b) x (nc b), row-major over the corners'
+ * components (corner-major), as AddVlistMValues reads them.  Dirichlet:
+ * for each VECSKIP bit j of v, row j of every block of v's row is zeroed,
+ * the diagonal (j, j) set to 1 and rhs_j to dirichlet[v b + j] (0 when
+ * NULL).  rhs (device, n b doubles, or NULL) receives the element vectors'
+ * sums in element order.  At most 96 elements meet at a vector and 96
+ * vectors share a row (UGMG_E_ARG otherwise, as for a corner index outside
+ * [0, n)). */
+typedef struct {
+  int n;                      /* vectors of the level */
+  int nelem, nc;              /* elements, corners per element */
+  const int *elem;            /* nelem*nc vector indices */
+  const double *emat;         /* nelem*(nc*b)^2 element matrices, or NULL */
+  const double *kref;         /* (nc*b)^2 reference matrix when emat is NULL */
+  const double *scale;        /* nelem factors of kref when emat is NULL */
+  const double *evec;         /* nelem*nc*b element vectors, or NULL */
+  const uint32_t *skip;       /* n VECSKIP masks, or NULL */
+  const double *dirichlet;    /* n*b Dirichlet values, or NULL */
+  const uint8_t *vclass;      /* n VCLASS, or NULL (3) */
+  const uint8_t *vnclass;     /* n VNCLASS, or NULL (0) */
+} ugmg_elements;
+int ugmg_assemble_level(ugmg_ctx *ctx, int level, const ugmg_elements *el,
+                        double *rhs);
+/* The level as the context holds it (list order): with NULL arrays only
+ * *n and *nnzb; else also row_ptr[n+1], col_idx[nnzb], vals[nnzb*b*b] and
+ * skip[n] (host arrays; any of them may be NULL). */
+int ugmg_get_level(ugmg_ctx *ctx, int level, int *n, int *nnzb, int *row_ptr,
+                   int *col_idx, double *vals, uint32_t *skip);
+
 /* ---- host-only setup routines (no GPU needed) ---------------------------
  * The pieces of PreProcess that run on the host, callable on their own.
  * ugmg_color: greedy first-fit colouring in list order over the symmetrised

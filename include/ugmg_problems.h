@@ -93,6 +93,29 @@ int ugmg_problem_rhs(const ugmg_problem *p, const double **b);
 int ugmg_problem_grid(const ugmg_problem *p, int level, int *dim,
                       int *nodes_per_dir);
 
+/* Element data of the finest level of the generators above, as an element
+ * loop hands it to AddVlistMValues (np/udm/disctools.cc:841): nelem
+ * elements of nc corners, elem[e*nc + i] = vector of corner i; element
+ * matrix (nc b) x (nc b), row-major over corner-major scalar unknowns, either
+ * emat[e] (full) or scale[e] * kref (one reference matrix; emat NULL);
+ * skip = the VECSKIP masks of the level.  Same arithmetic and SplitMix64
+ * draws as the generators, so assembling in element order reproduces their
+ * finest level exactly.  Free with ugmg_element_data_free. */
+typedef struct ugmg_element_data {
+  int n, b, nelem, nc;
+  int *elem;
+  double *emat;           /* nelem (nc b)^2, or NULL */
+  double *scale, *kref;   /* nelem and (nc b)^2, or NULL */
+  uint32_t *skip;         /* n */
+} ugmg_element_data;
+int ugmg_problem_q1_elements(int dim, int coarse_cells, int nref, int kappa_mode,
+                             uint64_t seed, ugmg_element_data **out);
+int ugmg_problem_p1_cd_elements(int nref, double eps, double theta, uint64_t seed,
+                                ugmg_element_data **out);
+int ugmg_problem_elasticity_elements(int nref, double E, double nu, uint64_t seed,
+                                     ugmg_element_data **out);
+void ugmg_element_data_free(ugmg_element_data *d);
+
 /* SplitMix64 stream shared with the tests: fills out[i] = a+(b-a)*u_i. */
 void ugmg_splitmix_uniform(uint64_t seed, int64_t count, double a, double b,
                            double *out);

@@ -1,0 +1,90 @@
+"""Device-side EXPreProcess (np/procs/iter.cc:8685-8970): the band LU without
+pivoting (EXDecomposeMatrixDOUBLE, gm/evm.cc:1400-1418) and the explicit
+inverse from EXApplyLUDOUBLE (gm/evm.cc:1493-1510) run on the GPU during
+ugmg_preprocess (for base levels above about 2e9 multiply-adds of
+factorisation; UGMG_BASE_HOST=0 forces it for the small levels here).  The
+inverse is read back column by column through the
+base solve (ls over ex, one step, damp 1, on unit vectors: the GEMV returns
+the column exactly) and must be bit-identical to the host factorisation
+(UGMG_BASE_HOST=1, and for scalar levels ugmg_coarse_inverse, which the
+CPU tests pin against the oracle's EX).  A zero pivot fails on the device
+as on the host."""
+import os
+
+import numpy as np
+import pytest
+
+from ug_amd import problems, ugmg
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _context(P, bl, host_factor, reorder=1):
+    ctx = ugmg.Context(0)
+    ctx.set_hierarchy(P)
+    ctx.set_params(nu1=1, nu2=1, baselevel=bl, smoother=ugmg.SMOOTH_JAC, damp=0.8, base_abslimit=1e-300,
+                   base_maxiter=1, ex_reorder=reorder)
+    # small base levels are factored on the host unless forced to the device
+    os.environ["UGMG_BASE_HOST"] = "1" if host_factor else "0"
+    try:
+        ctx.preprocess()
+    finally:
+        os.environ.pop("UGMG_BASE_HOST", None)
+    return ctx
+
+
+def _inverse_columns(ctx, n):
+    inv = np.empty((n, n))
+    c = torch.zeros(n, dtype=torch.float64, device="cuda")
+    b = torch.zeros(n, dtype=torch.float64, device="cuda")
+    for u in range(n):
+        b.zero_()
+        b[u] = 1.0
+        torch.cuda.synchronize()      # the context works on its own stream
+        ctx.base_solve(c, b)
+        ctx.sync()
+        inv[:, u] = c.cpu().numpy()
+    return inv
+
+
+@pytest.mark.parametrize("which", ["jump3d", "poisson2d", "elasticity"])
+def test_device_base_inverse_is_bitwise_the_host_one(which):
+    if which == "jump3d":
+        P, bl = problems.jump3d(4, seed=11), 3          # 9^3 vectors, BFS bandwidth 259
+    elif which == "poisson2d":
+        P, bl = problems.q1(2, 2, 5, problems.KAPPA_ONE, seed=12), 4   # 33^2 vectors
+    else:
+        P, bl = problems.elasticity(3, seed=13), 2       # 5^3 vectors x 3 components
+    lv = P.levels[bl]
+    n = lv.n * lv.b
+    dev = _context(P, bl, host_factor=False)
+    host = _context(P, bl, host_factor=True)
+    got = _inverse_columns(dev, n)
+    want = _inverse_columns(host, n)
+    dev.close()
+    host.close()
+    assert np.array_equal(got, want)
+    if lv.b == 1:
+        rc, inv, _, _ = ugmg.coarse_inverse(lv, reorder=1)
+        assert rc == 0 and np.array_equal(got, inv)
+    # and it is the inverse
+    A = lv.dense()
+    assert np.max(np.abs(got @ A - np.eye(n))) < 1e-10
+
+
+@pytest.mark.parametrize("host_factor", [False, True])
+def test_device_base_zero_pivot_fails_like_the_host(host_factor):
+    P = problems.q1(2, 2, 1, problems.KAPPA_ONE, seed=1)
+    lv = P.levels[0]
+    lv.vals = lv.vals.copy()
+    lv.vals[lv.row_ptr[0]] = 0.0      # first pivot of the list order
+    ctx = _context(P, 0, host_factor=host_factor, reorder=0)
+    c = torch.zeros(lv.n, dtype=torch.float64, device="cuda")
+    b = torch.ones(lv.n, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    with pytest.raises(ugmg.UgmgError) as e:
+        ctx.base_solve(c, b)
+    assert e.value.code == -5
+    ctx.close()

@@ -754,6 +754,163 @@ extern "C" int ugmg_problem_elasticity(int nref, double E, double nu, uint64_t s
   return 0;
 }
 
+// ------------------------------------------------------------ element data
+// The finest level as its element loop sees it (the data an application
+// hands to AddVlistMValues, np/udm/disctools.cc:841): element e's corners
+// and its (nc b) x (nc b) matrix, computed by the same code, from the same
+// SplitMix64 draws, as the generators above, so that summing the element
+// matrices in element order reproduces their finest levels bit for bit.
+
+namespace {
+
+ugmg_element_data *new_element_data(int n, int b, int64_t nelem, int nc, bool full) {
+  ugmg_element_data *d = new ugmg_element_data();
+  d->n = n;
+  d->b = b;
+  d->nelem = (int)nelem;
+  d->nc = nc;
+  const size_t m = (size_t)nc * b;
+  d->elem = new int[(size_t)nelem * nc];
+  d->skip = new uint32_t[n]();
+  if (full) {
+    d->emat = new double[(size_t)nelem * m * m];
+  } else {
+    d->scale = new double[nelem];
+    d->kref = new double[m * m];
+  }
+  return d;
+}
+
+}  // namespace
+
+extern "C" void ugmg_element_data_free(ugmg_element_data *d) {
+  if (!d) return;
+  delete[] d->elem;
+  delete[] d->skip;
+  delete[] d->emat;
+  delete[] d->scale;
+  delete[] d->kref;
+  delete d;
+}
+
+extern "C" int ugmg_problem_q1_elements(int dim, int coarse_cells, int nref, int kappa_mode,
+                                        uint64_t seed, ugmg_element_data **out) {
+  if (!out || (dim != 2 && dim != 3) || coarse_cells < 1 || nref < 0 || nref > 12 ||
+      (kappa_mode != UGMG_KAPPA_ONE && kappa_mode != UGMG_KAPPA_JUMP))
+    return -1;
+  const int C = coarse_cells << nref, N = C + 1, nc = 1 << dim;
+  const int64_t nel = dim == 2 ? (int64_t)C * C : (int64_t)C * C * C;
+  const int64_t n = dim == 2 ? (int64_t)N * N : (int64_t)N * N * N;
+  if (nel * nc > 2000000000LL) return -1;
+  ugmg_element_data *d;
+  try {
+    d = new_element_data((int)n, 1, nel, nc, false);
+  } catch (const std::bad_alloc &) {
+    return -4;
+  }
+  uint64_t s = seed;
+  for (int64_t e = 0; e < nel; ++e)
+    d->scale[e] = kappa_mode == UGMG_KAPPA_JUMP ? std::pow(10.0, -2.0 + 4.0 * splitmix_u01(s)) : 1.0;
+  q1_stiffness(dim, 1.0 / C, d->kref);
+#pragma omp parallel for schedule(static)
+  for (int64_t e = 0; e < nel; ++e) {
+    const int x = (int)(e % C), y = (int)((e / C) % C), z = dim == 3 ? (int)(e / ((int64_t)C * C)) : 0;
+    for (int a = 0; a < nc; ++a)
+      d->elem[e * nc + a] = (int)node_id(dim, N, x + (a & 1), y + ((a >> 1) & 1), z + ((a >> 2) & 1));
+  }
+  for (int64_t i = 0; i < n; ++i) {
+    const int x = (int)(i % N), y = (int)((i / N) % N), z = dim == 3 ? (int)(i / ((int64_t)N * N)) : 0;
+    d->skip[i] = (x == 0 || x == N - 1 || y == 0 || y == N - 1 || (dim == 3 && (z == 0 || z == N - 1))) ? 1u : 0u;
+  }
+  *out = d;
+  return 0;
+}
+
+extern "C" int ugmg_problem_p1_cd_elements(int nref, double eps, double theta, uint64_t seed,
+                                           ugmg_element_data **out) {
+  if (!out || nref < 0 || nref > 11 || !(eps > 0.0)) return -1;
+  uint64_t s = seed;
+  if (std::isnan(theta)) theta = 2.0 * M_PI * splitmix_u01(s);
+  const double bvec[2] = {std::cos(theta), std::sin(theta)};
+  const int C = 2 << nref, N = C + 1;
+  const double h = 1.0 / C;
+  const double pe = h / (2.0 * eps);
+  const double tau = pe > 1e-8 ? h / 2.0 * (1.0 / std::tanh(pe) - 1.0 / pe) : 0.0;
+  ugmg_element_data *d;
+  try {
+    d = new_element_data(N * N, 1, (int64_t)C * C * 2, 3, true);
+  } catch (const std::bad_alloc &) {
+    return -4;
+  }
+  for (int cy = 0; cy < C; ++cy)
+    for (int cx = 0; cx < C; ++cx)
+      for (int t = 0; t < 2; ++t) {
+        const int64_t e = ((int64_t)cy * C + cx) * 2 + t;
+        const int vx[2][3] = {{cx, cx + 1, cx + 1}, {cx, cx + 1, cx}};
+        const int vy[2][3] = {{cy, cy, cy + 1}, {cy, cy + 1, cy + 1}};
+        double xy[3][2];
+        for (int q = 0; q < 3; ++q) {
+          xy[q][0] = vx[t][q] * h;
+          xy[q][1] = vy[t][q] * h;
+          d->elem[e * 3 + q] = vy[t][q] * N + vx[t][q];
+        }
+        double K[3][3], M[3][3], G[3];
+        p1_element(xy, eps, bvec, tau, K, M, G);
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j) d->emat[e * 9 + i * 3 + j] = K[i][j];
+      }
+  for (int i = 0; i < N * N; ++i) {
+    const int x = i % N, y = i / N;
+    d->skip[i] = (x == 0 || x == N - 1 || y == 0 || y == N - 1) ? 1u : 0u;
+  }
+  *out = d;
+  return 0;
+}
+
+extern "C" int ugmg_problem_elasticity_elements(int nref, double E, double nu, uint64_t seed,
+                                                ugmg_element_data **out) {
+  (void)seed;   // the element matrices do not depend on the body force
+  if (!out || nref < 0 || nref > 8 || !(E > 0.0) || !(nu > -1.0 && nu < 0.5)) return -1;
+  const double lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)), mu = E / (2.0 * (1.0 + nu));
+  const int C = 1 << nref, N = C + 1;
+  const double h = 1.0 / C;
+  const int64_t nel = (int64_t)C * C * C * 6;
+  ugmg_element_data *d;
+  try {
+    d = new_element_data(N * N * N, 3, nel, 4, true);
+  } catch (const std::bad_alloc &) {
+    return -4;
+  }
+#pragma omp parallel for schedule(static)
+  for (int64_t c = 0; c < (int64_t)C * C * C; ++c) {
+    const int cx = (int)(c % C), cy = (int)((c / C) % C), cz = (int)(c / ((int64_t)C * C));
+    for (int t = 0; t < 6; ++t) {
+      const int64_t e = c * 6 + t;
+      int vx[4][3] = {{0, 0, 0}};
+      for (int q = 1; q < 4; ++q) {
+        for (int dd = 0; dd < 3; ++dd) vx[q][dd] = vx[q - 1][dd];
+        vx[q][kPerm[t][q - 1]] = 1;
+      }
+      double X[4][3];
+      for (int q = 0; q < 4; ++q) {
+        const int cc[3] = {cx + vx[q][0], cy + vx[q][1], cz + vx[q][2]};
+        for (int dd = 0; dd < 3; ++dd) X[q][dd] = cc[dd] * h;
+        d->elem[e * 4 + q] = cc[0] + N * (cc[1] + N * cc[2]);
+      }
+      double K[4][4][9], M[4][4];
+      p1_elasticity_element(X, lam, mu, K, M);
+      double *em = d->emat + (size_t)e * 144;
+      for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j)
+          for (int a = 0; a < 3; ++a)
+            for (int bb = 0; bb < 3; ++bb) em[(i * 3 + a) * 12 + j * 3 + bb] = K[i][j][a * 3 + bb];
+    }
+  }
+  for (int i = 0; i < N * N * N; ++i) d->skip[i] = (i % N == 0) ? 7u : 0u;
+  *out = d;
+  return 0;
+}
+
 extern "C" void ugmg_problem_free(ugmg_problem *p) { delete p; }
 
 extern "C" int ugmg_problem_nlevels(const ugmg_problem *p) {

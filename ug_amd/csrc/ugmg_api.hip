@@ -606,6 +606,10 @@ int launch_bcgs_iteration(ugmg_ctx *ctx, int level, bool use_iter, int restart, 
 
 // Host half of LmgcPreProcess (CPU only, no HIP call; safe on a worker
 // thread while the context is not otherwise used).
+// multiply-adds of the base factorisation (about n bw (bw + n)) above which
+// it runs on the device: about a second of host time
+constexpr double kBaseDeviceWork = 2.0e9;
+
 struct PrepState {
   int level = -1, bl = 0, fuse_rows = 0;
   double nt_bytes = 64.0e6;
@@ -674,11 +678,64 @@ int prep_host(ugmg_ctx *ctx, int level, PrepState &ps) {
       build_prep(plv[l], coarse, &colp, nc, hp);
       hp.color_ug = col;
     }
-    build_base(ctx->host[bl], p.ex_reorder, ps.base);
+    // EXPreProcess: order and band here.  The band LU and the inverse of a
+    // large base level run on the device in prep_device; a small one is
+    // factored here, where the server overlaps it with other requests'
+    // device work (UGMG_BASE_HOST=1: always here, 0: always on the device).
+    build_base_band(ctx->host[bl], p.ex_reorder, ps.base);
+    const char *env_bh = std::getenv("UGMG_BASE_HOST");
+    const int base_mode = env_bh ? std::atoi(env_bh) : -1;
+    const double nb = ps.base.n, bwb = ps.base.bw;
+    if (base_mode == 1 || (base_mode < 0 && nb * bwb * (bwb + nb) < kBaseDeviceWork))
+      factor_base_host(ps.base);
   } catch (const std::bad_alloc &) {
     return fail(ctx, UGMG_E_NOMEM, "host allocation during preprocess");
   }
   return UGMG_OK;
+}
+
+// The band LU and inverse of the base level on the device (bit-identical to
+// factor_base_host); sets B.failed on a zero pivot.
+int factor_base_device(ugmg_ctx *ctx, HostBase &B) {
+  const int n = B.n;
+  int rc;
+  if ((rc = dalloc(ctx, &ctx->base_inv, (size_t)n * n))) return rc;
+  if (n == 0) return 0;
+  const size_t ldx = ((size_t)n + 63) / 64 * 64;
+  double *band = nullptr, *X = nullptr;
+  int *pos = nullptr, *flag = nullptr;
+  auto release = [&] {
+    hipFree(band);
+    hipFree(X);
+    hipFree(pos);
+    hipFree(flag);
+  };
+  hipError_t e = hipMalloc((void **)&band, B.band.size() * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void **)&X, (size_t)n * ldx * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void **)&pos, (size_t)n * sizeof(int));
+  if (e == hipSuccess) e = hipMalloc((void **)&flag, sizeof(int));
+  if (e != hipSuccess) {
+    release();
+    return fail(ctx, UGMG_E_NOMEM, std::string("hipMalloc (base factorisation): ") + hipGetErrorString(e));
+  }
+  const int zero = 0;
+  e = hipMemcpyAsync(band, B.band.data(), B.band.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(pos, B.pos.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(flag, &zero, sizeof(int), hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) {
+    launch_base_factor(ctx->stream, band, n, B.bw, pos, X, ldx, ctx->base_inv, flag);
+    e = hipGetLastError();
+  }
+  int failed = 0;
+  if (e == hipSuccess) e = hipMemcpyAsync(&failed, flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  release();
+  if (e != hipSuccess)
+    return fail(ctx, UGMG_E_HIP, std::string("base factorisation: ") + hipGetErrorString(e));
+  B.failed = failed != 0;
+  std::vector<double>().swap(B.band);
+  return 0;
 }
 
 // Device half: uploads the layouts built by prep_host.
@@ -743,7 +800,12 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
       for (float *v : {D.c32, D.b32, D.t32}) HIPCHK(hipMemset(v, 0, sizeof(float) * nv));
     }
     ctx->base = std::move(ps.base);
-    if ((rc = upload(ctx, &ctx->base_inv, ctx->base.inv.data(), ctx->base.inv.size()))) return rc;
+    if (!ctx->base.inv.empty() || ctx->base.n == 0) {   // factored on the host
+      if ((rc = upload(ctx, &ctx->base_inv, ctx->base.inv.data(), ctx->base.inv.size()))) return rc;
+      std::vector<double>().swap(ctx->base.inv);
+    } else if ((rc = factor_base_device(ctx, ctx->base))) {
+      return rc;
+    }
     // fused coarse levels (opt-in): the base level and every level above it
     // up to fuse_rows vectors (and at most kMaxColors colours)
     ctx->fuse_top = -1;
@@ -927,6 +989,69 @@ int ugmg_set_level(ugmg_ctx *ctx, int level, int n, const int *row_ptr, const in
   }
   ctx->level_set[level] = 1;
   free_device(ctx);
+  return UGMG_OK;
+}
+
+int ugmg_assemble_level(ugmg_ctx *ctx, int level, const ugmg_elements *el, double *rhs) {
+  if (!ctx) return UGMG_E_ARG;
+  if (level < 0 || level >= ctx->nlevels) return fail(ctx, UGMG_E_ARG, "level out of range");
+  if (!el || el->n < 0 || el->nelem < 0 || el->nc < 1 || el->nc > 64)
+    return fail(ctx, UGMG_E_ARG, "bad element description");
+  if (el->nelem > 0 && (!el->elem || (!el->emat && (!el->kref || !el->scale))))
+    return fail(ctx, UGMG_E_ARG, "null element array");
+  if ((long long)el->nelem * el->nc > 2000000000LL)
+    return fail(ctx, UGMG_E_ARG, "more than 2e9 element corners");
+  HIPCHK(hipSetDevice(ctx->device));
+  AsmInput in{el->n, ctx->b, el->nelem, el->nc, el->elem, el->emat, el->kref, el->scale, el->evec,
+              el->skip, el->dirichlet};
+  std::string err;
+  try {
+    HostLevel L;
+    const int rc = assemble_device(ctx->stream, in, rhs, L.rp, L.ci, L.a, err);
+    if (rc) return fail(ctx, rc, "ugmg_assemble_level: " + err);
+    const int n = el->n;
+    L.n = n;
+    L.b = ctx->b;
+    L.nnzb = L.rp[n];
+    L.skip.assign(n, 0u);
+    L.vc.assign(n, 3);
+    L.vnc.assign(n, 0);
+    if (n > 0) {
+      if (el->skip) HIPCHK(hipMemcpy(L.skip.data(), el->skip, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+      if (el->vclass) HIPCHK(hipMemcpy(L.vc.data(), el->vclass, n, hipMemcpyDeviceToHost));
+      if (el->vnclass) HIPCHK(hipMemcpy(L.vnc.data(), el->vnclass, n, hipMemcpyDeviceToHost));
+    }
+    HostLevel &H = ctx->host[level];
+    const bool keep_p = H.has_p && H.n == n;   // a transfer set for this level stays
+    if (keep_p) {
+      L.has_p = true;
+      L.pp.swap(H.pp);
+      L.pi.swap(H.pi);
+      L.pw.swap(H.pw);
+    } else {
+      ctx->transfer_set[level] = 0;
+    }
+    H = std::move(L);
+  } catch (const std::bad_alloc &) {
+    return fail(ctx, UGMG_E_NOMEM, "host allocation");
+  }
+  ctx->level_set[level] = 1;
+  free_device(ctx);
+  return UGMG_OK;
+}
+
+int ugmg_get_level(ugmg_ctx *ctx, int level, int *n, int *nnzb, int *row_ptr, int *col_idx,
+                   double *vals, uint32_t *skip) {
+  if (!ctx) return UGMG_E_ARG;
+  if (level < 0 || level >= ctx->nlevels || !ctx->level_set[level])
+    return fail(ctx, UGMG_E_ARG, "level not set");
+  const HostLevel &L = ctx->host[level];
+  if (n) *n = L.n;
+  if (nnzb) *nnzb = L.nnzb;
+  if (row_ptr) std::copy(L.rp.begin(), L.rp.end(), row_ptr);
+  if (col_idx) std::copy(L.ci.begin(), L.ci.end(), col_idx);
+  if (vals) std::copy(L.a.begin(), L.a.end(), vals);
+  if (skip) std::copy(L.skip.begin(), L.skip.end(), skip);
   return UGMG_OK;
 }
 

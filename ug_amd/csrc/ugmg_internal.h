@@ -71,7 +71,11 @@ struct HostBase {
   int bw = 0;
   bool failed = false;
   std::vector<int> order;      // BFS list order
-  std::vector<double> inv;     // n x n row-major, list order (vector-major)
+  std::vector<int> pos;        // scalar unknown (list order) -> band position
+  std::vector<double> band;    // EX_MAT band, n (2 bw + 1) + 1 (before factoring)
+  std::vector<double> inv;     // n x n row-major, list order (vector-major);
+                               // empty until factor_base_host (or factored
+                               // on the device)
 };
 
 // filters for build_sell
@@ -100,6 +104,10 @@ int choose_gs_window(const HostLevel &L, const std::vector<int> &color, int ncol
 // color: the level's colouring in the row order of L (null: Jacobi level)
 void build_prep(const HostLevel &L, const HostLevel *coarse, const std::vector<int> *color,
                 int ncolors, HostPrep &out);
+// EXPreProcess's order (BFS unless reorder is 0), bandwidth and band
+void build_base_band(const HostLevel &L, int reorder, HostBase &out);
+// band LU without pivoting and the inverse on the host (frees the band)
+void factor_base_host(HostBase &out);
 void build_base(const HostLevel &L, int reorder, HostBase &out);
 
 }  // namespace ugmg

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <string>
+#include <vector>
 
 #include "ugmg_common.h"
 
@@ -122,6 +124,32 @@ template <typename T>
 int launch_restrict(hipStream_t st, const DSell<T> &PT, const uint32_t *cskip,
                     const uint8_t *cvnclass, const uint8_t *cvclass, const T *bf, T damp, T *bc,
                     double *part, bool nt, int nrhs, size_t vsf, size_t vsc, int b);
+// Element lists of one level on the device (all pointers device memory):
+// nelem elements of nc corners, elem[e*nc + i] = vector of corner i; element
+// matrices (nc b) x (nc b) row-major over corner-major scalar unknowns,
+// emat (per element) or scale[e] * kref (emat null); evec (nc b per
+// element) or null; skip = VECSKIP per vector or null; dval = Dirichlet
+// values (n b) or null (0).
+struct AsmInput {
+  int n, b, nelem, nc;
+  const int *elem;
+  const double *emat, *kref, *scale, *evec;
+  const uint32_t *skip;
+  const double *dval;
+};
+// Assembles the level on the device and copies the CSR (diagonal block
+// first, then ascending columns) to the host vectors; rhs (device, n b) or
+// null.  Returns 0 or a negative UGMG_E_* code with err set.
+int assemble_device(hipStream_t st, const AsmInput &in, double *rhs, std::vector<int> &rp_h,
+                    std::vector<int> &ci_h, std::vector<double> &vals_h, std::string &err);
+
+// Band LU (no pivoting) of the base level and its explicit inverse, on the
+// device: band (n rows, bandwidth bw, UG's EX_MAT layout) is factored in
+// place, X (n x ldx, ldx >= n rounded up to 64) receives the inverse in band
+// positions, inv (n x n) the inverse in list order: inv[r][u] =
+// X[pos[r]][pos[u]].  *failed (0 on entry) is set on a zero pivot.
+void launch_base_factor(hipStream_t st, double *band, int n, int bw, const int *pos, double *X,
+                        size_t ldx, double *inv, int *failed);
 template <typename T>
 void launch_base_gemv(hipStream_t st, int n, const double *inv, const T *b, T *c, T *xout,
                       int cmode, double damp, double abslimit, double red, const double *part,

@@ -414,7 +414,7 @@ void build_prep(const HostLevel &L, const HostLevel *coarse, const std::vector<i
 
 #define EXM(m, bw, i, j) ((m)[2 * (size_t)(bw) * (i) + (j)])
 
-void build_base(const HostLevel &L, int reorder, HostBase &out) {
+void build_base_band(const HostLevel &L, int reorder, HostBase &out) {
   const int nvec = L.n, B = L.b, bb = B * B;
   std::vector<int> order(nvec);
   for (int i = 0; i < nvec; ++i) order[i] = i;
@@ -460,12 +460,22 @@ void build_base(const HostLevel &L, int reorder, HostBase &out) {
   const int n = nvec * B;
   out.n = n;
   out.bw = bw;
-  std::vector<double> band((size_t)n * (2 * (size_t)bw + 1) + 1, 0.0);
+  out.pos.resize(n);
+  for (int u = 0; u < n; ++u) out.pos[u] = index[u / B] + u % B;
+  out.band.assign((size_t)n * (2 * (size_t)bw + 1) + 1, 0.0);
+  std::vector<double> &band = out.band;
   for (int v = 0; v < nvec; ++v)
     for (int m = L.rp[v]; m < L.rp[v + 1]; ++m)
       for (int i = 0; i < B; ++i)
         for (int j = 0; j < B; ++j)
           EXM(band.data(), bw, index[v] + i, index[L.ci[m]] + j) = L.a[(size_t)m * bb + i * B + j];
+  out.failed = false;
+  out.inv.clear();
+}
+
+void factor_base_host(HostBase &out) {
+  const int n = out.n, bw = out.bw;
+  std::vector<double> &band = out.band;
   out.failed = false;
   for (int i = 0; i < n - 1; ++i) {
     double d = EXM(band.data(), bw, i, i);
@@ -479,16 +489,19 @@ void build_base(const HostLevel &L, int reorder, HostBase &out) {
     }
   }
   out.inv.assign((size_t)n * n, 0.0);
-  if (out.failed) return;
-  // column u = b v + j of the inverse: EXApplyLUDOUBLE on the unit vector
-  // at band position index[v] + j (the forward substitution starts there)
+  if (out.failed) {
+    std::vector<double>().swap(out.band);
+    return;
+  }
+  // column u of the inverse: EXApplyLUDOUBLE on the unit vector at band
+  // position pos[u] (the forward substitution starts there)
 #pragma omp parallel
   {
     std::vector<double> vec(n);
 #pragma omp for schedule(dynamic, 16)
     for (int u = 0; u < n; ++u) {
       std::fill(vec.begin(), vec.end(), 0.0);
-      const int p0 = index[u / B] + u % B;
+      const int p0 = out.pos[u];
       vec[p0] = 1.0;
       for (int i = p0 + 1; i < n; ++i)
         for (int k = std::max(i - bw, p0); k < i; ++k) vec[i] -= EXM(band.data(), bw, i, k) * vec[k];
@@ -497,10 +510,15 @@ void build_base(const HostLevel &L, int reorder, HostBase &out) {
         for (int k = i + 1; k <= kmax; ++k) vec[i] -= EXM(band.data(), bw, i, k) * vec[k];
         vec[i] /= EXM(band.data(), bw, i, i);
       }
-      for (int k = 0; k < nvec; ++k)
-        for (int i = 0; i < B; ++i) out.inv[((size_t)order[k] * B + i) * n + u] = vec[(size_t)k * B + i];
+      for (int r = 0; r < n; ++r) out.inv[(size_t)r * n + u] = vec[out.pos[r]];
     }
   }
+  std::vector<double>().swap(out.band);
+}
+
+void build_base(const HostLevel &L, int reorder, HostBase &out) {
+  build_base_band(L, reorder, out);
+  factor_base_host(out);
 }
 
 }  // namespace ugmg

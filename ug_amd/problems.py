@@ -47,9 +47,129 @@ def _load():
         lib.ugmg_problem_elasticity.restype = C.c_int
         lib.ugmg_problem_theta.argtypes = [P]
         lib.ugmg_problem_theta.restype = C.c_double
+        E = C.POINTER(C.POINTER(_ElementData))
+        lib.ugmg_problem_q1_elements.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, E]
+        lib.ugmg_problem_p1_cd_elements.argtypes = [C.c_int, C.c_double, C.c_double, C.c_uint64, E]
+        lib.ugmg_problem_elasticity_elements.argtypes = [C.c_int, C.c_double, C.c_double, C.c_uint64, E]
+        lib.ugmg_element_data_free.argtypes = [C.POINTER(_ElementData)]
+        lib.ugmg_element_data_free.restype = None
         lib.ugmg_splitmix_uniform.restype = None
         _lib = lib
     return _lib
+
+
+class _ElementData(C.Structure):
+    _fields_ = [("n", C.c_int), ("b", C.c_int), ("nelem", C.c_int), ("nc", C.c_int),
+                ("elem", C.POINTER(C.c_int)), ("emat", C.POINTER(C.c_double)),
+                ("scale", C.POINTER(C.c_double)), ("kref", C.POINTER(C.c_double)),
+                ("skip", C.POINTER(C.c_uint32))]
+
+
+@dataclass
+class Elements:
+    """Finest-level element data of a generator (include/ugmg_problems.h):
+    elem (nelem, nc) corner vectors; element matrices (nc b)^2 either emat
+    (nelem, m, m) or scale (nelem,) times kref (m, m); skip (n,) VECSKIP."""
+    n: int
+    b: int
+    elem: np.ndarray
+    skip: np.ndarray
+    emat: np.ndarray | None = None
+    scale: np.ndarray | None = None
+    kref: np.ndarray | None = None
+
+    @property
+    def nelem(self) -> int:
+        return self.elem.shape[0]
+
+    @property
+    def nc(self) -> int:
+        return self.elem.shape[1]
+
+
+def _elements(fn, *args) -> Elements:
+    lib = _load()
+    h = C.POINTER(_ElementData)()
+    rc = getattr(lib, fn)(*args, C.byref(h))
+    if rc != 0:
+        raise ValueError(f"{fn} failed ({rc})")
+    try:
+        d = h.contents
+        m = d.nc * d.b
+        el = Elements(n=d.n, b=d.b, elem=_arr(d.elem, d.nelem * d.nc, np.int32).reshape(d.nelem, d.nc),
+                      skip=_arr(d.skip, d.n, np.uint32))
+        if d.emat:
+            el.emat = _arr(d.emat, d.nelem * m * m, np.float64).reshape(d.nelem, m, m)
+        else:
+            el.scale = _arr(d.scale, d.nelem, np.float64)
+            el.kref = _arr(d.kref, m * m, np.float64).reshape(m, m)
+    finally:
+        lib.ugmg_element_data_free(h)
+    return el
+
+
+def q1_elements(dim: int, coarse_cells: int, nref: int, kappa: int = KAPPA_ONE, seed: int = 0) -> Elements:
+    """Element data of q1(...)'s finest level: scale = kappa per element, kref = Q1 stiffness."""
+    return _elements("ugmg_problem_q1_elements", dim, coarse_cells, nref, kappa, C.c_uint64(seed))
+
+
+def p1_cd_elements(nref: int, eps: float = 1e-3, theta: float = float("nan"), seed: int = 0) -> Elements:
+    """Element data (SUPG element matrices) of p1_cd(...)'s finest level."""
+    return _elements("ugmg_problem_p1_cd_elements", nref, eps, theta, C.c_uint64(seed))
+
+
+def elasticity_elements(nref: int, E: float = 1.0, nu: float = 0.3, seed: int = 0x5EED0004) -> Elements:
+    """Element data (12 x 12 P1 elasticity matrices) of elasticity(...)'s finest level."""
+    return _elements("ugmg_problem_elasticity_elements", nref, E, nu, C.c_uint64(seed))
+
+
+def assemble_host(el: Elements, evec: np.ndarray | None = None, dirichlet: np.ndarray | None = None):
+    """Element-order assembly on the host with numpy (small levels): the
+    CSR (diagonal block first, then ascending columns), each entry summed
+    in element order, then AssembleDirichletBoundary.  Returns
+    (row_ptr, col_idx, vals (nnzb*b*b,), rhs (n*b,))."""
+    n, b, nc = el.n, el.b, el.nc
+    m = nc * b
+    pairs = set()
+    for e in range(el.nelem):
+        for v in el.elem[e]:
+            for w in el.elem[e]:
+                pairs.add((int(v), int(w)))
+    rows = [[] for _ in range(n)]
+    for v, w in pairs:
+        if v != w:
+            rows[v].append(w)
+    rp = np.zeros(n + 1, np.int32)
+    cols = []
+    for v in range(n):
+        r = [v] + sorted(rows[v])
+        cols.extend(r)
+        rp[v + 1] = rp[v] + len(r)
+    ci = np.array(cols, np.int32)
+    pos = {}
+    for v in range(n):
+        for q in range(rp[v], rp[v + 1]):
+            pos[(v, int(ci[q]))] = q
+    vals = np.zeros((len(ci), b, b))
+    rhs = np.zeros((n, b))
+    for e in range(el.nelem):
+        K = el.emat[e] if el.emat is not None else None
+        for i in range(nc):
+            v = int(el.elem[e, i])
+            for j in range(nc):
+                w = int(el.elem[e, j])
+                blk = K[i * b:(i + 1) * b, j * b:(j + 1) * b] if K is not None else \
+                    el.scale[e] * el.kref[i * b:(i + 1) * b, j * b:(j + 1) * b]
+                vals[pos[(v, w)]] += blk
+            if evec is not None:
+                rhs[v] += evec[e, i]
+    for v in range(n):
+        for j in range(b):
+            if (int(el.skip[v]) >> j) & 1:
+                vals[rp[v]:rp[v + 1], j, :] = 0.0
+                vals[rp[v], j, j] = 1.0
+                rhs[v, j] = 0.0 if dirichlet is None else dirichlet[v * b + j]
+    return rp, ci, vals.reshape(-1), rhs.reshape(-1)
 
 
 def _arr(ptr, n, dtype):

@@ -57,6 +57,14 @@ def declared_symbols(header=HEADER):
     return sorted(set(re.findall(r"\b(ugmg_[a-z0-9_]+)\s*\(", text)))
 
 
+class Elements(C.Structure):
+    """ugmg_elements: element lists of one level in device memory."""
+    _fields_ = [("n", C.c_int), ("nelem", C.c_int), ("nc", C.c_int), ("elem", C.c_void_p),
+                ("emat", C.c_void_p), ("kref", C.c_void_p), ("scale", C.c_void_p), ("evec", C.c_void_p),
+                ("skip", C.c_void_p), ("dirichlet", C.c_void_p), ("vclass", C.c_void_p),
+                ("vnclass", C.c_void_p)]
+
+
 def load():
     global _lib
     if _lib is not None:
@@ -103,6 +111,8 @@ def load():
         "ugmg_bcgs_solve": ([vp, ip, ip, dp, dp, ip, dp, dp, ip, ip, ip, ip, C.POINTER(LResult), dp, ip], ip),
         "ugmg_bcgs_solve_host": ([vp, ip, ip, dp, dp, ip, dp, dp, ip, ip, ip, ip, C.POINTER(LResult), dp, ip],
                                  ip),
+        "ugmg_assemble_level": ([vp, ip, C.POINTER(Elements), dp], ip),
+        "ugmg_get_level": ([vp, ip, C.POINTER(ip), C.POINTER(ip), vp, vp, vp, vp], ip),
         "ugmg_color": ([ip, vp, vp, vp], ip),
         "ugmg_coarse_inverse": ([ip, vp, vp, vp, ip, vp, vp, C.POINTER(ip)], ip),
     }
@@ -196,6 +206,7 @@ class Context:
         levels = problem.levels if top is None else problem.levels[:top + 1]
         self._keep = []
         self._chk(self.lib.ugmg_set_nlevels(self.h, len(levels), levels[0].b))
+        self.nb = levels[0].b
         for l, lv in enumerate(levels):
             arrs = [np.ascontiguousarray(lv.row_ptr, np.int32), np.ascontiguousarray(lv.col_idx, np.int32),
                     np.ascontiguousarray(lv.vals, np.float64), np.ascontiguousarray(lv.skip, np.uint32),
@@ -208,6 +219,11 @@ class Context:
             self._chk(self.lib.ugmg_set_transfer(self.h, l, *[a.ctypes.data for a in arrs]))
         self.problem = problem
         self.nlevels = len(levels)
+
+    def set_nlevels(self, nlevels, b=1):
+        self._chk(self.lib.ugmg_set_nlevels(self.h, nlevels, b))
+        self.nb = b
+        self.nlevels = nlevels
 
     def set_params(self, **kw):
         self.params = default_params(**kw)
@@ -322,6 +338,30 @@ class Context:
         self._chk(self.lib.ugmg_solve_host(self.h, level, x.ctypes.data, b.ctypes.data, C.addressof(ab),
                                            C.addressof(rd), maxiter, int(do_defect), C.byref(res), cb, None))
         return x, b, res, np.array(defects)
+
+    def assemble_level(self, level, n, elem, emat=None, kref=None, scale=None, evec=None, skip=None,
+                       dirichlet=None, vclass=None, vnclass=None, rhs=None):
+        """ugmg_assemble_level from device tensors: elem (nelem, nc) int32,
+        emat (nelem, m, m) or kref (m, m) + scale (nelem,) float64, evec
+        (nelem, nc, b), skip (n,) int32/uint32 bits, dirichlet (n b,),
+        vclass / vnclass (n,) uint8; rhs (n b,) float64 receives the load."""
+        el = Elements(n=n, nelem=elem.shape[0], nc=elem.shape[1], elem=_ptr(elem), emat=_ptr(emat),
+                      kref=_ptr(kref), scale=_ptr(scale), evec=_ptr(evec), skip=_ptr(skip),
+                      dirichlet=_ptr(dirichlet), vclass=_ptr(vclass), vnclass=_ptr(vnclass))
+        self._chk(self.lib.ugmg_assemble_level(self.h, level, C.byref(el), _ptr(rhs)))
+
+    def get_level(self, level):
+        """(row_ptr, col_idx, vals, skip) of a level as the context holds it."""
+        n, nnzb = C.c_int(), C.c_int()
+        self._chk(self.lib.ugmg_get_level(self.h, level, C.byref(n), C.byref(nnzb), None, None, None, None))
+        rp = np.zeros(n.value + 1, np.int32)
+        ci = np.zeros(nnzb.value, np.int32)
+        sk = np.zeros(n.value, np.uint32)
+        b = self.nb
+        va = np.zeros(nnzb.value * b * b)
+        self._chk(self.lib.ugmg_get_level(self.h, level, C.byref(n), C.byref(nnzb), rp.ctypes.data,
+                                          ci.ctypes.data, va.ctypes.data, sk.ctypes.data))
+        return rp, ci, va, sk
 
     def galerkin(self, level):
         """Level-1 values := P^T A P on the GPU (then preprocess again)."""

@@ -724,7 +724,7 @@ int factor_base_device(ugmg_ctx *ctx, HostBase &B) {
     e = hipMemcpyAsync(pos, B.pos.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(flag, &zero, sizeof(int), hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess) {
-    launch_base_factor(ctx->stream, band, n, B.bw, pos, X, ldx, ctx->base_inv, flag);
+    launch_base_factor(ctx->stream, band, n, B.bw, pos, X, ctx->base_inv, flag);
     e = hipGetLastError();
   }
   int failed = 0;

@@ -61,43 +61,85 @@ __global__ __launch_bounds__(256) void k_band_lu_step(double *__restrict__ band,
   }
 }
 
+// acc - sum_{k0 <= k < k1} row[k] x[k ldx], subtracting term by term in
+// ascending k (each product rounded, then the difference), in chunks of
+// 64 terms.  The band row is wave-uniform: lane j loads row[k + j] in one
+// coalesced load and each term takes its factor with readlane, while the
+// 64 column entries of a chunk are loaded before any of them is used, so a
+// chunk costs one memory round trip; only the subtractions form the
+// dependent chain.
+template <bool TAIL>
+__device__ __forceinline__ double band_chunk_sub(int llo, int lhi, const double *__restrict__ x,
+                                                 size_t ldx, int k, int k1, double acc) {
+#pragma clang fp contract(off)
+  constexpr int kCh = 64;
+  double xv[kCh];
+#pragma unroll
+  for (int j = 0; j < kCh; ++j) xv[j] = x[(size_t)(TAIL ? min(k + j, k1 - 1) : k + j) * ldx];
+  // keep the scheduler from sinking each load next to its use
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int j = 0; j < kCh; ++j) {
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane(llo, j);
+    const long long hi = __builtin_amdgcn_readlane(lhi, j);
+    const double lj = __longlong_as_double((hi << 32) | lo);
+    const double p = lj * xv[j];
+    const double t = acc - p;
+    // terms past k1 (last chunk only) are discarded by a select, not a
+    // branch, which would let the compiler sink each load into its block
+    acc = (!TAIL || k + j < k1) ? t : acc;
+  }
+  return acc;
+}
+
+__device__ __forceinline__ double band_row_sub(const double *__restrict__ row,
+                                               const double *__restrict__ x, size_t ldx, int k0,
+                                               int k1, double acc) {
+  constexpr int kCh = 64;
+  const int lane = threadIdx.x & 63;
+  int k = k0;
+  for (; k + kCh <= k1; k += kCh) {
+    const long long lb = __double_as_longlong(row[k + lane]);
+    acc = band_chunk_sub<false>((int)lb, (int)(lb >> 32), x, ldx, k, k1, acc);
+  }
+  if (k < k1) {
+    const long long lb = __double_as_longlong(row[min(k + lane, k1 - 1)]);
+    acc = band_chunk_sub<true>((int)lb, (int)(lb >> 32), x, ldx, k, k1, acc);
+  }
+  return acc;
+}
+
 // Columns c0 .. c0 + 63 of the inverse in band positions, one lane per
-// column c (the unit vector e_c): X[k][c], row stride ldx.  Forward
+// column c (the unit vector e_c).  The 64 columns of one wave are stored
+// row by row in one contiguous panel (entry (k, c) at
+// X[(c / 64) n 64 + 64 k + c % 64]), so the rows a substitution step reads
+// are a few consecutive pages rather than one page per row.  Forward
 // substitution from row c0 + 1 (rows below a lane's own c hold zeros, so
 // the extra terms of lanes with c > c0 subtract zeros and change nothing),
 // then backward substitution over every row, as EXApplyLUDOUBLE.
 __global__ __launch_bounds__(64) void k_band_inverse(const double *__restrict__ band, int bw, int n,
-                                                     double *__restrict__ X, size_t ldx,
+                                                     double *__restrict__ X,
                                                      const int *__restrict__ failed) {
 #pragma clang fp contract(off)
   if (*failed) return;
   const int c0 = blockIdx.x * 64;
   const int c = c0 + (int)threadIdx.x;
-  double *x = X + c;   // X has ldx >= c0 + 64 columns
+  const size_t ldx = 64;
+  double *x = X + (size_t)blockIdx.x * n * 64 + threadIdx.x;
   for (int k = 0; k < n; ++k) x[(size_t)k * ldx] = k == c ? 1.0 : 0.0;
-  for (int i = c0 + 1; i < n; ++i) {
-    const double *__restrict__ Li = band + 2 * (size_t)bw * i;
-    double acc = x[(size_t)i * ldx];
-    for (int k = max(i - bw, c0); k < i; ++k) {
-      const double p = Li[k] * x[(size_t)k * ldx];
-      acc = acc - p;
-    }
-    x[(size_t)i * ldx] = acc;
-  }
+  for (int i = c0 + 1; i < n; ++i)
+    x[(size_t)i * ldx] = band_row_sub(band + 2 * (size_t)bw * i, x, ldx, max(i - bw, c0), i,
+                                      x[(size_t)i * ldx]);
   for (int i = n - 1; i >= 0; --i) {
     const double *__restrict__ Ui = band + 2 * (size_t)bw * i;
-    const int kmax = min(i + bw, n - 1);
-    double acc = x[(size_t)i * ldx];
-    for (int k = i + 1; k <= kmax; ++k) {
-      const double p = Ui[k] * x[(size_t)k * ldx];
-      acc = acc - p;
-    }
+    const double acc = band_row_sub(Ui, x, ldx, i + 1, min(i + bw, n - 1) + 1, x[(size_t)i * ldx]);
     x[(size_t)i * ldx] = acc / Ui[i];
   }
 }
 
-// inv[r][u] = X[pos[r]][pos[u]] (list order rows and columns)
-__global__ __launch_bounds__(256) void k_band_inverse_out(const double *__restrict__ X, size_t ldx,
+// inv[r][u] = entry (pos[r], pos[u]) of the panels (list order rows and
+// columns)
+__global__ __launch_bounds__(256) void k_band_inverse_out(const double *__restrict__ X,
                                                           const int *__restrict__ pos, int n,
                                                           double *__restrict__ inv,
                                                           const int *__restrict__ failed) {
@@ -105,13 +147,14 @@ __global__ __launch_bounds__(256) void k_band_inverse_out(const double *__restri
   const int u = blockIdx.x * 256 + threadIdx.x;
   const int r = blockIdx.y;
   if (u >= n) return;
-  inv[(size_t)r * n + u] = X[(size_t)pos[r] * ldx + pos[u]];
+  const int q = pos[u];
+  inv[(size_t)r * n + u] = X[(size_t)(q >> 6) * n * 64 + (size_t)pos[r] * 64 + (q & 63)];
 }
 
 }  // namespace
 
 void launch_base_factor(hipStream_t st, double *band, int n, int bw, const int *pos, double *X,
-                        size_t ldx, double *inv, int *failed) {
+                        double *inv, int *failed) {
   if (n <= 0) return;
   for (int i = 0; i < n; ++i) {
     const int m = i < n - 1 ? std::min(i + bw, n - 1) - i : 0;
@@ -119,8 +162,8 @@ void launch_base_factor(hipStream_t st, double *band, int n, int bw, const int *
     const unsigned nb = (unsigned)((work + 255) / 256);
     k_band_lu_step<<<nb, 256, 0, st>>>(band, bw, n, i, failed);
   }
-  k_band_inverse<<<(n + 63) / 64, 64, 0, st>>>(band, bw, n, X, ldx, failed);
-  k_band_inverse_out<<<dim3((n + 255) / 256, n), 256, 0, st>>>(X, ldx, pos, n, inv, failed);
+  k_band_inverse<<<(n + 63) / 64, 64, 0, st>>>(band, bw, n, X, failed);
+  k_band_inverse_out<<<dim3((n + 255) / 256, n), 256, 0, st>>>(X, pos, n, inv, failed);
 }
 
 }  // namespace ugmg

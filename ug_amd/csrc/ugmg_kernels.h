@@ -145,11 +145,12 @@ int assemble_device(hipStream_t st, const AsmInput &in, double *rhs, std::vector
 
 // Band LU (no pivoting) of the base level and its explicit inverse, on the
 // device: band (n rows, bandwidth bw, UG's EX_MAT layout) is factored in
-// place, X (n x ldx, ldx >= n rounded up to 64) receives the inverse in band
-// positions, inv (n x n) the inverse in list order: inv[r][u] =
-// X[pos[r]][pos[u]].  *failed (0 on entry) is set on a zero pivot.
+// place, X (n rounded up to 64, times n doubles) receives the inverse in
+// band positions as panels of 64 columns, inv (n x n) the inverse in list
+// order: inv[r][u] = entry (pos[r], pos[u]).  *failed (0 on entry) is set
+// on a zero pivot.
 void launch_base_factor(hipStream_t st, double *band, int n, int bw, const int *pos, double *X,
-                        size_t ldx, double *inv, int *failed);
+                        double *inv, int *failed);
 template <typename T>
 void launch_base_gemv(hipStream_t st, int n, const double *inv, const T *b, T *c, T *xout,
                       int cmode, double damp, double abslimit, double red, const double *part,

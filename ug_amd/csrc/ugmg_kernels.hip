@@ -182,17 +182,35 @@ __device__ __forceinline__ void block_sum_b(double (&v)[B], double (*red)[kBlock
     }
 }
 
+// Chunked round-robin: XCD x runs chunks x, x + 8, x + 16, ... of kChunk
+// consecutive blocks (the tail beyond whole rounds of 8 chunks keeps its
+// place).  For launches whose work per slice changes along the slice range
+// -- the defect over every colour, where rows of the first colour hold all
+// upper entries and rows of the last none; the interpolation, whose fine
+// rows hold 1, 2, 4 or 8 coarse corners by colour -- every XCD gets a share
+// of each region, while the blocks of a chunk still share their L2.
+__device__ __forceinline__ int xcd_chunked(int bid, int nb) {
+  constexpr int kChunk = 32;
+  const int nfull = nb / (8 * kChunk) * (8 * kChunk);
+  if (bid >= nfull) return bid;
+  const int x = bid % 8, k = bid / 8;
+  return ((k / kChunk) * 8 + x) * kChunk + k % kChunk;
+}
+
 // Slice block and column of this workgroup.  Single column: the XCD-aware
-// remap of blockIdx.x.  Several columns: the linear workgroup id is remapped
-// over the whole grid and split slice-block-major.
+// remap of blockIdx.x (contiguous range per XCD, or chunked round-robin
+// with CHUNKED).  Several columns: the linear workgroup id is remapped over
+// the whole grid and split slice-block-major.
+template <bool CHUNKED = false>
 __device__ __forceinline__ void batch_map(int &sb, int &col) {
   if (gridDim.y == 1) {
-    sb = xcd_remap(blockIdx.x, gridDim.x);
+    sb = CHUNKED ? xcd_chunked(blockIdx.x, gridDim.x) : xcd_remap(blockIdx.x, gridDim.x);
     col = 0;
     return;
   }
   const int nb = gridDim.x * gridDim.y;
-  const int lr = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x, nb);
+  const int id = blockIdx.y * gridDim.x + blockIdx.x;
+  const int lr = CHUNKED ? xcd_chunked(id, nb) : xcd_remap(id, nb);
   sb = lr / gridDim.y;
   col = lr % gridDim.y;
 }
@@ -200,9 +218,10 @@ __device__ __forceinline__ void batch_map(int &sb, int &col) {
 // slice s (wave-uniform, in scalar registers), lane, row r (-1 = padding),
 // entry base e0, width w, uniform-offset list sdo, index base rb, slice
 // block sb_, column col_
-#define SLICE_PROLOGUE(S, slice0, nsl)                                                   \
+#define SLICE_PROLOGUE(S, slice0, nsl) SLICE_PROLOGUE_MAP(S, slice0, nsl, false)
+#define SLICE_PROLOGUE_MAP(S, slice0, nsl, CHUNKED)                                      \
   int sb_, col_;                                                                         \
-  batch_map(sb_, col_);                                                                  \
+  batch_map<CHUNKED>(sb_, col_);                                                         \
   const int s = __builtin_amdgcn_readfirstlane((slice0) + sb_ * (kBlock / kSlice) +      \
                                                (int)(threadIdx.x >> 6));                 \
   if (s >= (slice0) + (nsl)) return;                                                     \
@@ -373,7 +392,7 @@ __global__ __launch_bounds__(kBlock) void k_gs_defect(DSell<TV> U, int nsl,
                                                        const uint8_t *__restrict__ vclass,
                                                        T damp, T *__restrict__ c,
                                                        T *__restrict__ xout, size_t vs) {
-  SLICE_PROLOGUE(U, 0, nsl)
+  SLICE_PROLOGUE_MAP(U, 0, nsl, true)
   const size_t o = (size_t)col_ * vs;
   v += o;
   T br = 0, vr = 0, cr = 0;
@@ -435,7 +454,7 @@ __global__ __launch_bounds__(kBlock) void k_interp(DSell<T> P, int nsl,
                                                     const T *__restrict__ cc, T damp,
                                                     T *__restrict__ t, T *__restrict__ c,
                                                     size_t vsf, size_t vsc) {
-  SLICE_PROLOGUE(P, 0, nsl)
+  SLICE_PROLOGUE_MAP(P, 0, nsl, true)
   cc += (size_t)col_ * vsc;
   const size_t o = (size_t)col_ * vsf;
   const T sum = sell_dot<MW, NT, T, T, T>(P.val + e0 + lane, P.col + e0 + lane, w, cc, sdo, P.dtab, rb);
@@ -619,7 +638,7 @@ __global__ __launch_bounds__(kBlock) void k_gs_defect_b3(DSell<TV> U, int nsl,
                                                           T damp, T *__restrict__ c,
                                                           T *__restrict__ xout, size_t vs) {
   constexpr int B = 3;
-  SLICE_PROLOGUE(U, 0, nsl)
+  SLICE_PROLOGUE_MAP(U, 0, nsl, true)
   const size_t o = (size_t)col_ * vs;
   v += o;
   T sum[B];
@@ -720,7 +739,7 @@ __global__ __launch_bounds__(kBlock) void k_interp_b3(DSell<T> P, int nsl,
                                                        T *__restrict__ t, T *__restrict__ c,
                                                        size_t vsf, size_t vsc) {
   constexpr int B = 3;
-  SLICE_PROLOGUE(P, 0, nsl)
+  SLICE_PROLOGUE_MAP(P, 0, nsl, true)
   cc += (size_t)col_ * vsc;
   const size_t o = (size_t)col_ * vsf;
   T sum[B];
@@ -1182,10 +1201,13 @@ void launch_gs_defect(hipStream_t st, const DSell<TV> &U, const T *v, T *b, cons
                         U, U.nslices, v, b, vclass, damp, c, xout, bt.vs));
     return;
   }
-  W_DISPATCH(U.wmax, NT_DISPATCH(nt, if (cmode == 0) k_gs_defect<MWV, TV, T, 0, NTV><<<g, kBlock, 0, st>>>(
-                                         U, U.nslices, v, b, vclass, damp, c, xout, bt.vs);
-                                 else k_gs_defect<MWV, TV, T, 1, NTV><<<g, kBlock, 0, st>>>(
-                                         U, U.nslices, v, b, vclass, damp, c, xout, bt.vs)));
+  // chunks of 8 entries: the upper widths of the colours run from the whole
+  // off-diagonal row down to 0, and a wider chunk mostly loads clamped
+  // repeats on the narrow colours (measured slower for 28 and 16)
+  NT_DISPATCH(nt, if (cmode == 0) k_gs_defect<8, TV, T, 0, NTV><<<g, kBlock, 0, st>>>(
+                      U, U.nslices, v, b, vclass, damp, c, xout, bt.vs);
+              else k_gs_defect<8, TV, T, 1, NTV><<<g, kBlock, 0, st>>>(
+                      U, U.nslices, v, b, vclass, damp, c, xout, bt.vs));
 }
 template void launch_gs_defect<double, double>(hipStream_t, const DSell<double> &,
                                                const double *, double *, const uint8_t *, double,

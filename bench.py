@@ -405,6 +405,15 @@ def main():
     ach = by_spmv / (ms_spmv * 1e-3) / 1e9
     ach_gs = by_gs / (ms_gs * 1e-3) / 1e9
     ach32 = by_spmv32 / (ms_spmv32 * 1e-3) / 1e9
+    # the damped-Jacobi step (t = w D^-1 b, b -= A t, c += t) on the same
+    # level, from a Jacobi hierarchy (list order, no colouring)
+    ctxj = ugmg.Context(0)
+    ctxj.set_hierarchy(P)
+    ctxj.set_params(**dict(base, smoother=ugmg.SMOOTH_JAC, damp=0.8), precision=ugmg.FP64)
+    ctxj.preprocess()
+    ms_jac, by_jac = ctxj.time_kernel(top, ugmg.K_JAC64, 20)
+    ctxj.close()
+    ach_jac = by_jac / (ms_jac * 1e-3) / 1e9
 
     out = {
         "metric": METRIC,
@@ -446,6 +455,12 @@ def main():
             "bound": "hbm", "achieved": round(ach_gs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(ach_gs / HBM_PEAK_GBS, 4), "algorithmic_bytes": by_gs,
             "avg_step_us": round(ms_gs * 1e3, 2),
+        },
+        "roofline_jacobi": {
+            "kernel": "damped-Jacobi step (k_jac + k_spmv mode 2: b -= A t, c += t) FP64 on the 129^3 level",
+            "bound": "hbm", "achieved": round(ach_jac, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach_jac / HBM_PEAK_GBS, 4), "algorithmic_bytes": by_jac,
+            "avg_step_us": round(ms_jac * 1e3, 2),
         },
         "roofline_spmv_fp32": {"achieved": round(ach32, 1), "frac": round(ach32 / HBM_PEAK_GBS, 4),
                                "avg_launch_us": round(ms_spmv32 * 1e3, 2)},

@@ -17,6 +17,14 @@ def test_libugmg_exports_every_declared_symbol():
     assert not missing, missing
 
 
+@pytest.mark.parametrize("header", ["ugmg_np.h", "ugmg_dio.h"])
+def test_libugmg_exports_the_numproc_and_data_file_symbols(header):
+    names = ugmg.declared_symbols(ugmg.HEADER.replace("ugmg.h", header))
+    assert names
+    lib = C.CDLL(ugmg.LIB_PATH)
+    assert [n for n in names if not hasattr(lib, n)] == []
+
+
 def test_problem_library_exports_every_declared_symbol():
     text = open(ugmg.HEADER.replace("ugmg.h", "ugmg_problems.h")).read()
     names = sorted(set(re.findall(r"\b(ugmg_[a-z0-9_]+)\s*\(", text)))

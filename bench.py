@@ -75,8 +75,11 @@ def serving_run(n_req, rate, seed, budget_gb, slots):
             nref += 1
         kappa = problems.KAPPA_ONE if dim == 2 else problems.KAPPA_JUMP
         P = problems.q1(dim, cells, nref, kappa, seed=int(seed * 1000 + i))
-        # base level: the largest level with at most 729 vectors
-        bl = max(l for l in range(P.top + 1) if P.levels[l].n <= 729)
+        # base level: the largest level with at most 300 vectors (3D: 5^3, 2D:
+        # 17^2).  The 9^3 level (BFS bandwidth 259) would cost ~40 ms of host
+        # band LU + inverse per request against ~0.15 ms per cycle for
+        # smoothing it instead.
+        bl = max(l for l in range(P.top + 1) if P.levels[l].n <= 300)
         prm = ugmg.default_params(nu1=2, nu2=2, gamma=1, baselevel=bl, smoother=ugmg.SMOOTH_GS, damp=1.0,
                                   base_abslimit=1e-300, use_graph=1)
         reqs.append((P, prm))
@@ -97,14 +100,15 @@ def serving_run(n_req, rate, seed, budget_gb, slots):
             break
         if a == 0 and k < n_req:
             time.sleep(min(0.001, max(0.0, arrivals[k] - now)))
-    lat, gpu, ok = [], [], 0
+    lat, gpu, ok, rows = [], [], 0, []
     t_first, t_last = None, 0.0
-    for rid in ids:
+    for k, rid in enumerate(ids):
         st, res, (ts, ta, td) = srv.status(rid)
         if st == ugmg.REQ_DONE and res.converged:
             ok += 1
         lat.append(td - ts)
         gpu.append(td - ta)
+        rows.append((td - ts, ta - ts, td - ta, reqs[k][0].levels[-1].n, res.number_of_linear_iterations))
         t_first = ts if t_first is None else min(t_first, ts)
         t_last = max(t_last, td)
     srv.close()
@@ -116,6 +120,8 @@ def serving_run(n_req, rate, seed, budget_gb, slots):
         "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
         "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2),
         "admitted_to_done_p50_ms": round(float(np.percentile(gpu, 50)) * 1e3, 2),
+        "slowest": {"unknowns": int(max(rows)[3]), "submit_to_admit_ms": round(max(rows)[1] * 1e3, 2),
+                    "admit_to_done_ms": round(max(rows)[2] * 1e3, 2), "iterations": int(max(rows)[4])},
         "solves_per_s": round(n_req / (t_last - t_first), 2),
         "note": "latency = completion - submission, including the server's host layout build",
     }

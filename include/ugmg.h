@@ -258,11 +258,12 @@ int ugmg_coarse_inverse(int n, const int *row_ptr, const int *col_idx,
 
 /* ---- serving: continuous batching of independent solves ----------------
  * No counterpart in UG (its ugd daemon is a remote UI, ugd.cc:528-608).
- * Each request is a whole hierarchy + rhs; the server builds its layouts on
- * a worker thread, admits it (FIFO) when a slot and the HBM budget allow,
- * and keeps one ls iteration of every admitted request in flight on that
- * request's own stream.  The caller drives progress with ugmg_server_step;
- * results equal those of a stand-alone ugmg_solve of the same request. */
+ * Each request is a whole hierarchy + rhs; worker threads build the layouts
+ * (taken in submission order), and ugmg_server_step admits the first
+ * prepared request in submission order when a slot and the HBM budget allow
+ * (one waiting for memory holds back those behind it), then keeps one ls
+ * iteration of every admitted request in flight on that request's own
+ * stream.  Results equal those of a stand-alone ugmg_solve of the request. */
 typedef struct {
   int n;
   const int *row_ptr, *col_idx;

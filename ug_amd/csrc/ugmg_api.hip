@@ -190,8 +190,8 @@ int upload_sell(ugmg_ctx *ctx, const HostSell &S, DevSellBuf &D) {
   D.nuniform = 0;
   for (int o : S.sdoff) D.nuniform += o >= 0;
   if ((rc = upload(ctx, &D.v64, S.val.data(), S.val.size()))) return rc;
-  std::vector<float> v32;
-  to_float(S.val, v32);
+  hvec<float> v32;
+  to_float(S.val.data(), S.val.size(), v32);
   if ((rc = upload(ctx, &D.v32, v32.data(), v32.size()))) return rc;
   return 0;
 }
@@ -762,8 +762,8 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
       max_slices = std::max(max_slices, hp.A.nslices);
       if ((rc = upload_sell(ctx, hp.A, D.A))) return rc;
       if ((rc = upload(ctx, &D.diag64, hp.diag.data(), hp.diag.size()))) return rc;
-      std::vector<float> d32;
-      to_float(hp.diag, d32);
+      hvec<float> d32;
+      to_float(hp.diag.data(), hp.diag.size(), d32);
       if ((rc = upload(ctx, &D.diag32, d32.data(), d32.size()))) return rc;
       if ((rc = upload(ctx, &D.vclass, H.vc.data(), H.vc.size()))) return rc;
       if ((rc = upload(ctx, &D.vnclass, H.vnc.data(), H.vnc.size()))) return rc;
@@ -1832,10 +1832,12 @@ int ugmg_time_kernel(ugmg_ctx *ctx, int level, int kind, int reps, double *ms, d
 // ================================================================= serving
 //
 // Continuous batching of independent solves.  Every submitted request gets
-// its own context (its own stream); a worker thread builds the host layouts
-// (prep_host) in submission order while the GPU works on admitted requests;
-// ugmg_server_step() admits prepared requests in FIFO order while slots and
-// the HBM budget allow, uploads them, and keeps one solver iteration of every
+// its own context (its own stream); kSrvWorkers threads build the host
+// layouts (prep_host, taken in submission order) while the GPU works on
+// admitted requests; ugmg_server_step() admits prepared requests in
+// submission order -- a request whose layout is still being built does not
+// hold back the prepared ones behind it -- while slots and the HBM budget
+// allow, uploads them, and keeps one solver iteration of every
 // admitted request in flight, polling each request's norm copy with an
 // event.  Requests therefore overlap on the device (one stream each), join as
 // soon as a slot and memory free up, and run exactly the kernels of a
@@ -1846,6 +1848,7 @@ int ugmg_time_kernel(ugmg_ctx *ctx, int level, int kind, int reps, double *ms, d
 #include <deque>
 #include <memory>
 #include <mutex>
+#include <system_error>
 #include <thread>
 
 namespace {
@@ -1889,6 +1892,9 @@ double est_device_bytes(const ugmg_ctx *ctx, int top, int bl) {
 
 }  // namespace
 
+// host layout builders of a server (prep_host runs OpenMP loops inside)
+constexpr int kSrvWorkers = 4;
+
 struct ugmg_server {
   int device = 0;
   double budget = 0.0, used = 0.0;
@@ -1899,7 +1905,7 @@ struct ugmg_server {
   int ndone = 0;
   std::string err;
   std::chrono::steady_clock::time_point t0;
-  std::thread worker;
+  std::vector<std::thread> workers;
   std::mutex mu;
   std::condition_variable cv;
   std::deque<int> prep_queue;
@@ -2056,7 +2062,18 @@ int ugmg_server_create(int device, double hbm_budget_bytes, int max_slots, ugmg_
   s->budget = hbm_budget_bytes;
   s->max_slots = max_slots;
   s->t0 = std::chrono::steady_clock::now();
-  s->worker = std::thread(server_worker, s);
+  try {
+    for (int k = 0; k < kSrvWorkers; ++k) s->workers.emplace_back(server_worker, s);
+  } catch (const std::system_error &) {
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->stop = true;
+    }
+    s->cv.notify_all();
+    for (auto &w : s->workers) w.join();
+    delete s;
+    return UGMG_E_NOMEM;
+  }
   *out = s;
   return UGMG_OK;
 }
@@ -2068,7 +2085,8 @@ int ugmg_server_destroy(ugmg_server *s) {
     s->stop = true;
   }
   s->cv.notify_all();
-  if (s->worker.joinable()) s->worker.join();
+  for (auto &w : s->workers)
+    if (w.joinable()) w.join();
   for (auto &r : s->reqs)
     if (r && r->ctx) {
       if (r->ctx->stream) hipStreamSynchronize(r->ctx->stream);
@@ -2148,30 +2166,38 @@ int ugmg_server_step(ugmg_server *s, int *nactive, int *nqueued, int *ndone) {
     if (finished) s->active.erase(s->active.begin() + k);
     else ++k;
   }
-  // admission: FIFO, prepared, within slots and HBM budget (a request that
-  // alone exceeds the budget is rejected)
+  // admission: the first prepared request in submission order, within slots
+  // and HBM budget (a request that alone exceeds the budget is rejected; one
+  // that waits for memory holds back those behind it, so large requests are
+  // not starved)
   for (;;) {
     if (s->queue.empty() || (int)s->active.size() >= s->max_slots) break;
-    SrvReq *r = s->reqs[s->queue.front()].get();
+    SrvReq *r = nullptr;
+    size_t qi = 0;
     {
       std::lock_guard<std::mutex> lk(s->mu);
-      if (!r->host_ready) break;
+      for (; qi < s->queue.size(); ++qi)
+        if (s->reqs[s->queue[qi]]->host_ready) {
+          r = s->reqs[s->queue[qi]].get();
+          break;
+        }
     }
+    if (!r) break;
     if (r->prep_rc) {
-      s->queue.pop_front();
+      s->queue.erase(s->queue.begin() + qi);
       r->res.error_code = r->prep_rc;
       srv_finish(s, r, UGMG_REQ_FAILED);
       continue;
     }
     if (r->est_bytes > s->budget) {
-      s->queue.pop_front();
+      s->queue.erase(s->queue.begin() + qi);
       r->err = "request exceeds the HBM budget";
       r->res.error_code = UGMG_E_BUDGET;
       srv_finish(s, r, UGMG_REQ_FAILED);
       continue;
     }
     if (s->used + r->est_bytes > s->budget) break;
-    s->queue.pop_front();
+    s->queue.erase(s->queue.begin() + qi);
     const int rc = srv_admit(s, r);
     if (rc) {
       r->res.error_code = rc;

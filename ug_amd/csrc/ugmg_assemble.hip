@@ -189,7 +189,7 @@ inline unsigned nblk(long long n) { return (unsigned)std::max<long long>(1, (n +
 }  // namespace
 
 int assemble_device(hipStream_t st, const AsmInput &in, double *rhs, std::vector<int> &rp_h,
-                    std::vector<int> &ci_h, std::vector<double> &vals_h, std::string &err) {
+                    hvec<int> &ci_h, hvec<double> &vals_h, std::string &err) {
   const int n = in.n;
   const long long total = (long long)in.nelem * in.nc;
   const int bb = in.b * in.b;

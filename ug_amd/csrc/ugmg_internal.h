@@ -2,7 +2,9 @@
 #pragma once
 
 #include <cstdint>
+#include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "ugmg_common.h"
@@ -13,8 +15,9 @@ namespace ugmg {
 struct HostLevel {
   int n = 0, nnzb = 0;
   int b = 1;   // components per vector; a holds b*b values per entry
-  std::vector<int> rp, ci;
-  std::vector<double> a;
+  std::vector<int> rp;
+  hvec<int> ci;
+  hvec<double> a;
   std::vector<uint32_t> skip;
   std::vector<uint8_t> vc, vnc;
   bool has_p = false;
@@ -30,9 +33,9 @@ struct HostSell {
   int nslices = 0;
   std::vector<int> soff;  // nslices + 1, entry offsets (multiples of 64)
   std::vector<int> rows;  // nslices * 64
-  std::vector<int> col;
+  hvec<int> col;
   // bb planes of soff[nslices] values: component q of entry e at q*nent + e
-  std::vector<double> val;
+  hvec<double> val;
   int bb = 1;
   bool identity_rows = false;
   // uniform-delta slices: every row of the slice has the column offsets
@@ -85,7 +88,7 @@ void build_sell(const HostLevel &L, const std::vector<int> *rows, Part part,
                 const std::vector<int> *color, HostSell &out);
 void build_sell_csr(int n, const int *rp, const int *ci, const double *a, HostSell &out);
 int greedy_color(const HostLevel &L, std::vector<int> &color);
-void to_float(const std::vector<double> &in, std::vector<float> &out);
+void to_float(const double *in, size_t n, hvec<float> &out);
 // L (and its transfer) in device order: row p of out is row perm[p] of L,
 // columns through pos; interpolation columns through cpos (coarse list
 // index -> coarse device position; null: identity)

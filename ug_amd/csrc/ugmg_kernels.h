@@ -141,7 +141,7 @@ struct AsmInput {
 // first, then ascending columns) to the host vectors; rhs (device, n b) or
 // null.  Returns 0 or a negative UGMG_E_* code with err set.
 int assemble_device(hipStream_t st, const AsmInput &in, double *rhs, std::vector<int> &rp_h,
-                    std::vector<int> &ci_h, std::vector<double> &vals_h, std::string &err);
+                    hvec<int> &ci_h, hvec<double> &vals_h, std::string &err);
 
 // Band LU (no pivoting) of the base level and its explicit inverse, on the
 // device: band (n rows, bandwidth bw, UG's EX_MAT layout) is factored in

@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <unordered_map>
 
 namespace ugmg {
 
@@ -44,7 +45,12 @@ static void sell_from_csr(int n, int ncols, const int *rp, const int *ci, const 
   out.sdoff.assign(ns, -1);
   std::vector<int> width(ns, 0);
   std::vector<std::vector<int>> sdel(ns);   // common offsets of uniform slices
-#pragma omp parallel for schedule(static)
+  std::vector<uint64_t> shash(ns, 0);       // FNV-1a of each list
+#pragma omp parallel
+  {
+  std::vector<int> U;   // per-thread scratch: the candidate offset list of a slice
+  U.reserve(64);
+#pragma omp for schedule(static)
   for (int s = 0; s < ns; ++s) {
     int wmax = 0, longest = -1;
     for (int lane = 0; lane < kSlice; ++lane) {
@@ -59,7 +65,7 @@ static void sell_from_csr(int n, int ncols, const int *rp, const int *ci, const 
     }
     width[s] = wmax;
     if (!uniform || wmax == 0 || out.rows[(size_t)s * kSlice] < 0) continue;
-    std::vector<int> U;
+    U.clear();
     for (int m = rp[longest]; m < rp[longest + 1]; ++m)
       if (keep(longest, m)) U.push_back(ci[m] - longest);
     bool ok = true;
@@ -77,22 +83,31 @@ static void sell_from_csr(int n, int ncols, const int *rp, const int *ci, const 
         else ++k;
       }
     }
-    if (ok) sdel[s].swap(U);
+    if (ok) {
+      uint64_t hv = 1469598103934665603ULL;
+      for (int u : U) hv = (hv ^ (uint32_t)u) * 1099511628211ULL;
+      shash[s] = hv ^ U.size();
+      sdel[s].assign(U.begin(), U.end());
+    }
+  }
   }
   for (int s = 0; s < ns; ++s) out.soff[s + 1] = out.soff[s] + width[s] * kSlice;
-  // shared offset lists (most slices of a stencil matrix share one)
+  // shared offset lists (most slices of a stencil matrix share one): equal
+  // lists found by hash, confirmed by comparison
   out.dtab.clear();
   {
-    std::vector<std::pair<std::vector<int>, int>> seen;
+    std::unordered_map<uint64_t, std::vector<int>> seen;   // hash -> dtab offsets
     for (int s = 0; s < ns; ++s) {
       if (sdel[s].empty()) continue;
+      const std::vector<int> &U = sdel[s];
       int off = -1;
-      for (auto &e : seen)
-        if (e.first == sdel[s]) { off = e.second; break; }
+      auto &cands = seen[shash[s]];
+      for (int o : cands)
+        if (std::equal(U.begin(), U.end(), out.dtab.begin() + o)) { off = o; break; }
       if (off < 0) {
         off = (int)out.dtab.size();
-        out.dtab.insert(out.dtab.end(), sdel[s].begin(), sdel[s].end());
-        if (seen.size() < 64) seen.emplace_back(sdel[s], off);
+        out.dtab.insert(out.dtab.end(), U.begin(), U.end());
+        cands.push_back(off);
       }
       out.sdoff[s] = off;
     }
@@ -177,44 +192,48 @@ void build_sell_csr(int n, const int *rp, const int *ci, const double *a, HostSe
   sell_from_csr(n, 0, rp, ci, a, 1, nullptr, [](int, int) { return true; }, false, out);
 }
 
-void to_float(const std::vector<double> &in, std::vector<float> &out) {
-  out.resize(in.size());
-  const long long ne = (long long)in.size();
+void to_float(const double *in, size_t n, hvec<float> &out) {
+  out.resize(n);
+  const long long ne = (long long)n;
 #pragma omp parallel for schedule(static)
   for (long long i = 0; i < ne; ++i) out[i] = (float)in[i];
 }
 
 int greedy_color(const HostLevel &L, std::vector<int> &color) {
+  // First fit in list order over the symmetrised pattern.  When vector i is
+  // coloured only its neighbours w < i carry a colour, so it looks at the
+  // lower entries of its own row and at the rows w < i that list i (the
+  // transposed lower part, gathered here).
   const int n = L.n;
-  // symmetrised adjacency
-  std::vector<int> deg(n + 1, 0);
-  for (int i = 0; i < n; ++i)
-    for (int m = L.rp[i]; m < L.rp[i + 1]; ++m) {
-      int w = L.ci[m];
-      if (w == i) continue;
-      ++deg[i + 1];
-      ++deg[w + 1];
+  std::vector<int> tptr(n + 1, 0);
+#pragma omp parallel for schedule(static)
+  for (int w = 0; w < n; ++w)
+    for (int m = L.rp[w]; m < L.rp[w + 1]; ++m) {
+      const int i = L.ci[m];
+      if (i > w) __atomic_fetch_add(&tptr[i + 1], 1, __ATOMIC_RELAXED);
     }
-  for (int i = 0; i < n; ++i) deg[i + 1] += deg[i];
-  std::vector<int> adj(deg[n]), fill(deg.begin(), deg.end() - 1);
-  for (int i = 0; i < n; ++i)
-    for (int m = L.rp[i]; m < L.rp[i + 1]; ++m) {
-      int w = L.ci[m];
-      if (w == i) continue;
-      adj[fill[i]++] = w;
-      adj[fill[w]++] = i;
+  for (int i = 0; i < n; ++i) tptr[i + 1] += tptr[i];
+  hvec<int> tadj(tptr[n]);
+  std::vector<int> cur(tptr.begin(), tptr.end() - 1);
+#pragma omp parallel for schedule(static)
+  for (int w = 0; w < n; ++w)
+    for (int m = L.rp[w]; m < L.rp[w + 1]; ++m) {
+      const int i = L.ci[m];
+      if (i > w) tadj[__atomic_fetch_add(&cur[i], 1, __ATOMIC_RELAXED)] = w;
     }
   color.assign(n, -1);
   int ncol = 0;
   std::vector<int> seen(64, -1);
+  auto mark = [&](int c, int i) {
+    if (c >= (int)seen.size()) seen.resize(c + 1, -1);
+    seen[c] = i;
+  };
   for (int i = 0; i < n; ++i) {
-    for (int q = deg[i]; q < deg[i + 1]; ++q) {
-      int c = color[adj[q]];
-      if (c >= 0) {
-        if (c >= (int)seen.size()) seen.resize(c + 1, -1);
-        seen[c] = i;
-      }
+    for (int m = L.rp[i]; m < L.rp[i + 1]; ++m) {
+      const int w = L.ci[m];
+      if (w < i) mark(color[w], i);
     }
+    for (int q = tptr[i]; q < tptr[i + 1]; ++q) mark(color[tadj[q]], i);
     int c = 0;
     while (c < (int)seen.size() && seen[c] == i) ++c;
     color[i] = c;
@@ -254,6 +273,7 @@ void permute_level(const HostLevel &L, const std::vector<int> &perm, const std::
     for (int p = 0; p < n; ++p) out.pp[p + 1] = out.pp[p] + (L.pp[perm[p] + 1] - L.pp[perm[p]]);
     out.pi.resize(L.pi.size());
     out.pw.resize(L.pw.size());
+#pragma omp parallel for schedule(static)
     for (int p = 0; p < n; ++p) {
       const int r = perm[p];
       int q = out.pp[p];

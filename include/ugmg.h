@@ -64,6 +64,9 @@ typedef struct {
   int precision;                    /* UGMG_FP64 or UGMG_MIXED                */
   int inner_cycles;                 /* MIXED: FP32 V-cycles per FP64 step     */
   int use_graph;                    /* replay the cycle from a hipGraph       */
+  int ex_float;                     /* ex $f: band LU in FLOAT arithmetic on
+                                       FLOAT entries (EXDecomposeMatrixFLOAT),
+                                       applied in DOUBLE (EXApplyLUFLOAT)     */
 } ugmg_params;
 
 /* LRESULT (np/procs/ls.h:71-77) */

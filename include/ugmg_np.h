@@ -20,7 +20,7 @@
  *   jac, gs   SmootherInit                            iter.cc:763-790
  *             $damp 1   (gs is the multicolour-ordered l_lgs of the GPU)
  *   ex        EXInit                                  iter.cc:8248-8260
- *             $o 1 (BFS band reorder), $damp 1
+ *             $o 1 (BFS band reorder), $f (FLOAT band LU), $damp 1
  *   transfer  the standard transfer (no options read here)
  * GPU-only options of lmgc: $precision fp64|mixed, $inner k (FP32 cycles
  * per FP64 step), $graph 0|1; of ls/bcgs: $nrhs k (bcgs columns).

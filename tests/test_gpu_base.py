@@ -21,11 +21,11 @@ pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
 
-def _context(P, bl, host_factor, reorder=1):
+def _context(P, bl, host_factor, reorder=1, ex_float=0):
     ctx = ugmg.Context(0)
     ctx.set_hierarchy(P)
     ctx.set_params(nu1=1, nu2=1, baselevel=bl, smoother=ugmg.SMOOTH_JAC, damp=0.8, base_abslimit=1e-300,
-                   base_maxiter=1, ex_reorder=reorder)
+                   base_maxiter=1, ex_reorder=reorder, ex_float=ex_float)
     # small base levels are factored on the host unless forced to the device
     os.environ["UGMG_BASE_HOST"] = "1" if host_factor else "0"
     try:
@@ -88,3 +88,45 @@ def test_device_base_zero_pivot_fails_like_the_host(host_factor):
         ctx.base_solve(c, b)
     assert e.value.code == -5
     ctx.close()
+
+
+def _float_band_lu_inverse(A, bw):
+    """EXDecomposeMatrixFLOAT restated in numpy float32 (no pivoting, band
+    only), then the inverse from the FLOAT factors in float64 (what
+    EXApplyLUFLOAT computes on unit vectors, up to summation order)."""
+    import scipy.linalg as sl
+    M = A.astype(np.float32)
+    n = len(M)
+    for i in range(n - 1):
+        e = min(i + bw, n - 1) + 1
+        f = M[i + 1:e, i] / M[i, i]
+        M[i + 1:e, i] = f
+        M[i + 1:e, i + 1:e] = M[i + 1:e, i + 1:e] - np.outer(f, M[i, i + 1:e])
+    F = M.astype(np.float64)
+    L = np.tril(F, -1) + np.eye(n)
+    U = np.triu(F)
+    return sl.solve_triangular(U, sl.solve_triangular(L, np.eye(n), lower=True, unit_diagonal=True))
+
+
+def test_ex_float_mode():
+    """ex $f (EXInit's $f, np/procs/iter.cc:8253; factors by
+    EXDecomposeMatrixFLOAT, applied by EXApplyLUFLOAT)."""
+    P, bl = problems.jump3d(4, seed=14), 3
+    lv = P.levels[bl]
+    n = lv.n
+    dev = _context(P, bl, host_factor=False, ex_float=1)
+    host = _context(P, bl, host_factor=True, ex_float=1)
+    dbl = _context(P, bl, host_factor=False, ex_float=0)
+    got, want_host, inv64 = _inverse_columns(dev, n), _inverse_columns(host, n), _inverse_columns(dbl, n)
+    for c in (dev, host, dbl):
+        c.close()
+    assert np.array_equal(got, want_host)
+    rc, _, order, bw = ugmg.coarse_inverse(lv, reorder=1)
+    assert rc == 0
+    A = lv.dense()[np.ix_(order, order)]
+    ref = np.empty_like(got)
+    ref[np.ix_(order, order)] = _float_band_lu_inverse(A, bw)
+    scale = np.max(np.abs(ref))
+    assert np.max(np.abs(got - ref)) < 1e-10 * scale
+    # FLOAT factors: the inverse differs from the DOUBLE one at float level
+    assert 1e-9 < np.max(np.abs(got - inv64)) / scale < 1e-3

@@ -113,3 +113,21 @@ def test_gpu_options():
     assert env.init("s", "$x x $b b $A A $m 50 $R 10 $I mgc $red 1e-8 $nrhs 4") == NP.EXECUTABLE
     p = env.params("s")
     assert p.precision == ugmg.MIXED and p.inner_cycles == 3 and p.use_graph == 0
+
+
+def test_ex_float_option_reaches_the_parameters():
+    """EXInit's $f (np/procs/iter.cc:8253): FLOAT band LU of the base level."""
+    from ug_amd import numproc as NP
+    for opt, want in (("$o 1", 0), ("$o 1 $f", 1)):
+        env = NP.NumProcEnv(None)
+        x = np.zeros(9)
+        env.vector("sol", x)
+        env.vector("rhs", x.copy())
+        for name, cls, o in [("base", "ex", opt), ("bs", "ls", "$m 1 $red 1e-12 $I base"),
+                             ("sm", "jac", "$damp 0.8"), ("tr", "transfer", ""),
+                             ("mgc", "lmgc", "$S sm sm bs $T tr"),
+                             ("mgs", "ls", "$x sol $b rhs $m 10 $red 1e-10 $I mgc")]:
+            env.create(name, cls)
+            env.init(name, o)
+        assert env.params("mgs").ex_float == want
+        env.close()

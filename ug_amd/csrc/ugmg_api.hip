@@ -682,7 +682,7 @@ int prep_host(ugmg_ctx *ctx, int level, PrepState &ps) {
     // large base level run on the device in prep_device; a small one is
     // factored here, where the server overlaps it with other requests'
     // device work (UGMG_BASE_HOST=1: always here, 0: always on the device).
-    build_base_band(ctx->host[bl], p.ex_reorder, ps.base);
+    build_base_band(ctx->host[bl], p.ex_reorder, ps.base, p.ex_float != 0);
     const char *env_bh = std::getenv("UGMG_BASE_HOST");
     const int base_mode = env_bh ? std::atoi(env_bh) : -1;
     const double nb = ps.base.n, bwb = ps.base.bw;
@@ -724,7 +724,7 @@ int factor_base_device(ugmg_ctx *ctx, HostBase &B) {
     e = hipMemcpyAsync(pos, B.pos.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(flag, &zero, sizeof(int), hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess) {
-    launch_base_factor(ctx->stream, band, n, B.bw, pos, X, ctx->base_inv, flag);
+    launch_base_factor(ctx->stream, band, n, B.bw, pos, X, ctx->base_inv, flag, B.fmode);
     e = hipGetLastError();
   }
   int failed = 0;
@@ -884,6 +884,7 @@ void ugmg_params_default(ugmg_params *p) {
   p->precision = UGMG_FP64;
   p->inner_cycles = 1;
   p->use_graph = 0;
+  p->ex_float = 0;
 }
 
 int ugmg_create(int device, ugmg_ctx **out) {
@@ -1097,7 +1098,8 @@ int ugmg_set_params(ugmg_ctx *ctx, const ugmg_params *p) {
         return fail(ctx, UGMG_E_ARG,
                     "block hierarchies take one damping / base limit for all components");
   const bool relayout = !ctx->params_set || p->smoother != ctx->p.smoother ||
-                        p->baselevel != ctx->p.baselevel || p->ex_reorder != ctx->p.ex_reorder;
+                        p->baselevel != ctx->p.baselevel || p->ex_reorder != ctx->p.ex_reorder ||
+                        p->ex_float != ctx->p.ex_float;
   ctx->p = *p;
   ctx->params_set = true;
   drop_graph(ctx);

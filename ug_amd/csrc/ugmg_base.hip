@@ -1,7 +1,9 @@
 // Base-level set-up on the device: the band LU of EXPreProcess
-// (EXDecomposeMatrixDOUBLE, gm/evm.cc:1400-1418: no pivoting, a zero pivot
-// fails) and the explicit inverse, one column per unit vector pushed through
-// EXApplyLUDOUBLE (gm/evm.cc:1493-1510).
+// (EXDecomposeMatrixDOUBLE, gm/evm.cc:1400-1418, or EXDecomposeMatrixFLOAT,
+// 1355-1373, for ex $f: no pivoting, a zero pivot fails) and the explicit
+// inverse, one column per unit vector pushed through EXApplyLUDOUBLE
+// (gm/evm.cc:1493-1510; EXApplyLUFLOAT, 1448-1466, is the same double
+// arithmetic on the FLOAT factors).
 //
 // Every value is formed by the same operations in the same order as the host
 // loops of build_base, with products and differences rounded separately (no
@@ -30,14 +32,18 @@ __device__ __forceinline__ double &bandref(double *m, int bw, int i, int j) {
 // i - 1 into column i - 1, which no thread of step i reads.  Step n - 1 only
 // stores the last multipliers.  A zero pivot sets *failed and leaves the
 // band as it is (the host loop stops there).
+// R = float: EXDecomposeMatrixFLOAT (gm/evm.cc:1355-1373) on a band of
+// FLOAT values held as doubles: every operation in float arithmetic, the
+// results stored back exactly.
+template <typename R>
 __global__ __launch_bounds__(256) void k_band_lu_step(double *__restrict__ band, int bw, int n,
                                                       int i, int *__restrict__ failed) {
 #pragma clang fp contract(off)
   if (*failed) return;
   const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
   if (i < n - 1) {
-    const double d = bandref(band, bw, i, i);
-    if (d == 0.0) {
+    const R d = (R)bandref(band, bw, i, i);
+    if (d == (R)0) {
       if (idx == 0) *failed = 1;
       return;
     }
@@ -46,9 +52,9 @@ __global__ __launch_bounds__(256) void k_band_lu_step(double *__restrict__ band,
     if (idx < (long long)m * m) {
       const int j = i + 1 + (int)(idx / m);
       const int k = i + 1 + (int)(idx % m);
-      const double f = bandref(band, bw, j, i) / d;
-      const double p = f * bandref(band, bw, i, k);
-      bandref(band, bw, j, k) = bandref(band, bw, j, k) - p;
+      const R f = (R)bandref(band, bw, j, i) / d;
+      const R p = f * (R)bandref(band, bw, i, k);
+      bandref(band, bw, j, k) = (double)((R)bandref(band, bw, j, k) - p);
     }
   }
   if (i > 0) {
@@ -56,7 +62,7 @@ __global__ __launch_bounds__(256) void k_band_lu_step(double *__restrict__ band,
     const int mp = min(q + bw, n - 1) - q;
     if (idx < mp) {
       const int j = q + 1 + (int)idx;
-      bandref(band, bw, j, q) = bandref(band, bw, j, q) / bandref(band, bw, q, q);
+      bandref(band, bw, j, q) = (double)((R)bandref(band, bw, j, q) / (R)bandref(band, bw, q, q));
     }
   }
 }
@@ -154,13 +160,14 @@ __global__ __launch_bounds__(256) void k_band_inverse_out(const double *__restri
 }  // namespace
 
 void launch_base_factor(hipStream_t st, double *band, int n, int bw, const int *pos, double *X,
-                        double *inv, int *failed) {
+                        double *inv, int *failed, bool fmode) {
   if (n <= 0) return;
   for (int i = 0; i < n; ++i) {
     const int m = i < n - 1 ? std::min(i + bw, n - 1) - i : 0;
     const long long work = std::max<long long>({(long long)m * m, (long long)bw, 1LL});
     const unsigned nb = (unsigned)((work + 255) / 256);
-    k_band_lu_step<<<nb, 256, 0, st>>>(band, bw, n, i, failed);
+    if (fmode) k_band_lu_step<float><<<nb, 256, 0, st>>>(band, bw, n, i, failed);
+    else k_band_lu_step<double><<<nb, 256, 0, st>>>(band, bw, n, i, failed);
   }
   k_band_inverse<<<(n + 63) / 64, 64, 0, st>>>(band, bw, n, X, failed);
   k_band_inverse_out<<<dim3((n + 255) / 256, n), 256, 0, st>>>(X, pos, n, inv, failed);

@@ -76,6 +76,7 @@ struct HostBase {
   std::vector<int> order;      // BFS list order
   std::vector<int> pos;        // scalar unknown (list order) -> band position
   std::vector<double> band;    // EX_MAT band, n (2 bw + 1) + 1 (before factoring)
+  bool fmode = false;          // ex $f: FLOAT entries, LU in FLOAT arithmetic
   std::vector<double> inv;     // n x n row-major, list order (vector-major);
                                // empty until factor_base_host (or factored
                                // on the device)
@@ -108,9 +109,9 @@ int choose_gs_window(const HostLevel &L, const std::vector<int> &color, int ncol
 void build_prep(const HostLevel &L, const HostLevel *coarse, const std::vector<int> *color,
                 int ncolors, HostPrep &out);
 // EXPreProcess's order (BFS unless reorder is 0), bandwidth and band
-void build_base_band(const HostLevel &L, int reorder, HostBase &out);
+void build_base_band(const HostLevel &L, int reorder, HostBase &out, bool fmode = false);
 // band LU without pivoting and the inverse on the host (frees the band)
 void factor_base_host(HostBase &out);
-void build_base(const HostLevel &L, int reorder, HostBase &out);
+void build_base(const HostLevel &L, int reorder, HostBase &out, bool fmode = false);
 
 }  // namespace ugmg

@@ -150,7 +150,7 @@ int assemble_device(hipStream_t st, const AsmInput &in, double *rhs, std::vector
 // order: inv[r][u] = entry (pos[r], pos[u]).  *failed (0 on entry) is set
 // on a zero pivot.
 void launch_base_factor(hipStream_t st, double *band, int n, int bw, const int *pos, double *X,
-                        double *inv, int *failed);
+                        double *inv, int *failed, bool fmode = false);
 template <typename T>
 void launch_base_gemv(hipStream_t st, int n, const double *inv, const T *b, T *c, T *xout,
                       int cmode, double damp, double abslimit, double red, const double *part,

@@ -105,7 +105,7 @@ struct NumProc {
   int gamma = 1, nu1 = 1, nu2 = 1, bl = 0, precision = UGMG_FP64, inner = 1, graph = 1;
   // smoothers
   double damp[UGMG_MAXB] = {1, 1, 1};
-  int reorder = 1;
+  int reorder = 1, fmode = 0;
 };
 
 bool is_iter(const std::string &c) { return c == "lmgc" || c == "jac" || c == "gs" || c == "ex"; }
@@ -218,6 +218,7 @@ int init_object(ugmg_np_env *e, NumProc &o) {
   }
   if (o.cls == "ex") {   // EXInit (iter.cc:8248-8260)
     if (read_int(av, "o", &o.reorder)) o.reorder = 1;
+    o.fmode = read_option(av, "f");
     for (int i = 0; i < UGMG_MAXB; ++i) o.damp[i] = 1.0;
     sc_read(av, "damp", o.damp);
     return iter_state(av, o);
@@ -256,6 +257,7 @@ int build_params(ugmg_np_env *e, NumProc &o, ugmg_params &p, bool &use_iter) {
   }
   p.base_maxiter = ba->m;
   p.ex_reorder = ex->reorder;
+  p.ex_float = ex->fmode;
   p.precision = mg->precision;
   p.inner_cycles = mg->inner;
   p.use_graph = mg->graph;

@@ -434,7 +434,7 @@ void build_prep(const HostLevel &L, const HostLevel *coarse, const std::vector<i
 
 #define EXM(m, bw, i, j) ((m)[2 * (size_t)(bw) * (i) + (j)])
 
-void build_base_band(const HostLevel &L, int reorder, HostBase &out) {
+void build_base_band(const HostLevel &L, int reorder, HostBase &out, bool fmode) {
   const int nvec = L.n, B = L.b, bb = B * B;
   std::vector<int> order(nvec);
   for (int i = 0; i < nvec; ++i) order[i] = i;
@@ -489,6 +489,10 @@ void build_base_band(const HostLevel &L, int reorder, HostBase &out) {
       for (int i = 0; i < B; ++i)
         for (int j = 0; j < B; ++j)
           EXM(band.data(), bw, index[v] + i, index[L.ci[m]] + j) = L.a[(size_t)m * bb + i * B + j];
+  // ex $f: the band holds FLOAT entries (EXCopyMatrixFLOAT, iter.cc:8325)
+  out.fmode = fmode;
+  if (fmode)
+    for (double &x : band) x = (double)(float)x;
   out.failed = false;
   out.inv.clear();
 }
@@ -497,7 +501,22 @@ void factor_base_host(HostBase &out) {
   const int n = out.n, bw = out.bw;
   std::vector<double> &band = out.band;
   out.failed = false;
-  for (int i = 0; i < n - 1; ++i) {
+  if (out.fmode) {   // EXDecomposeMatrixFLOAT (gm/evm.cc:1355-1373): FLOAT arithmetic
+    for (int i = 0; i < n - 1; ++i) {
+      const float d = (float)EXM(band.data(), bw, i, i);
+      if (d == 0.0f) { out.failed = true; break; }
+      const int jmax = std::min(i + bw, n - 1);
+      for (int j = i + 1; j <= jmax; ++j) {
+        const float f = (float)EXM(band.data(), bw, j, i) / d;
+        EXM(band.data(), bw, j, i) = f;
+        for (int k = i + 1; k <= jmax; ++k) {
+          const float p = f * (float)EXM(band.data(), bw, i, k);
+          EXM(band.data(), bw, j, k) = (float)EXM(band.data(), bw, j, k) - p;
+        }
+      }
+    }
+  }
+  for (int i = 0; i < (out.fmode ? 0 : n - 1); ++i) {
     double d = EXM(band.data(), bw, i, i);
     if (d == 0.0) { out.failed = true; break; }
     int jmax = std::min(i + bw, n - 1);
@@ -536,8 +555,8 @@ void factor_base_host(HostBase &out) {
   std::vector<double>().swap(out.band);
 }
 
-void build_base(const HostLevel &L, int reorder, HostBase &out) {
-  build_base_band(L, reorder, out);
+void build_base(const HostLevel &L, int reorder, HostBase &out, bool fmode) {
+  build_base_band(L, reorder, out, fmode);
   factor_base_host(out);
 }
 

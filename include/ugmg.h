@@ -119,6 +119,8 @@ int ugmg_set_params(ugmg_ctx *ctx, const ugmg_params *p);
 int ugmg_preprocess(ugmg_ctx *ctx, int level);
 int ugmg_postprocess(ugmg_ctx *ctx);         /* LmgcPostProcess            */
 int ugmg_level_size(const ugmg_ctx *ctx, int level, int *n, int *nnzb);
+/* components per vector of the hierarchy (ugmg_set_nlevels' b), or < 0 */
+int ugmg_block(const ugmg_ctx *ctx);
 int ugmg_num_colors(const ugmg_ctx *ctx, int level);
 /* greedy colour of every vector of a level (host array of n ints) */
 int ugmg_get_colors(ugmg_ctx *ctx, int level, int *colors);

@@ -10,10 +10,10 @@
  * UG's Init functions:
  *   ls        LinearSolverInit + NPLinearSolverInit   ls.cc:771-797, 382-407
  *             $m (required), $I, $red (required), $abslimit (ABS_LIMIT 1e-10),
- *             $x $b $A
+ *             $x $b $A, $display no|red|full (ReadArgvDisplay, default no)
  *   bcgs      BCGSInit                                ls.cc:1750-1800
  *             $m (required), $R (0; < 0 not active), $I, $red, $abslimit,
- *             $weight (1; only unit weights are supported)
+ *             $weight (1; only unit weights are supported), $display
  *   lmgc      LmgcInit                                iter.cc:7613-7656
  *             $S pre post base, $T transfer, $g 1, $n1 1, $n2 1, $b 0
  *             (negative: deepest level with at most -b vectors), $damp 1
@@ -64,6 +64,13 @@ int ugmg_np_create(ugmg_np_env *env, const char *name, const char *cls);
 int ugmg_np_init(ugmg_np_env *env, const char *name, const char *options);
 /* npexecute <name> <flags>: 0 or a negative code */
 int ugmg_np_execute(ugmg_np_env *env, const char *name, const char *flags);
+/* The text of the linear solvers' displays since the last call: the head
+ * line, DoPCR's per-iteration defect and rate lines, the averages and the
+ * "LS  :" / "BCGS:" line, in UG's formats (np/udm/pcr.cc:263-606,
+ * ls.cc:666-744 and 1896-2060; TSOLVE/TIT in seconds of wall clock).  A UG
+ * plugin hands it to UserWrite.  Valid until the next call on env. */
+const char *ugmg_np_output(ugmg_np_env *env);
+
 /* the LRESULT of the last Residuum / Solver of a linear solver (column j) */
 int ugmg_np_result(const ugmg_np_env *env, const char *name, int col, ugmg_lresult *res);
 /* the parameters a linear solver object hands to libugmg (for inspection) */

@@ -68,3 +68,57 @@ def test_bcgs_script_with_two_rhs():
         assert np.max(np.abs(X[j] - want["x"])) < 1e-8 * np.max(np.abs(want["x"]))
     env.close()
     ctx.close()
+
+
+def _ug_display(name, defects, level, nit):
+    """LinearSolver's display text (np/procs/ls.cc:666-744 with DoPCR,
+    np/udm/pcr.cc:505-606, full display, one component), times elided."""
+    head = ("*" * ((50 - len(name)) // 2 - 1) + " " + name + " ").ljust(50, "*")
+    lines = ["", head]
+    lines.append(" %-3d  %c: %-12.7e   %-12.7s" % (0, "u", defects[0], "---"))
+    for k in range(1, len(defects)):
+        lines.append(" %-3d  %c: %-12.7e   %-12.7e" % (k, "u", defects[k], defects[k] / defects[k - 1]))
+    lines.append("")
+    nb = len(defects)
+    lines.append(" %-3d avg:  %c: %-12.7e   %-12.7e   %-12.7e"
+                 % (nb - 1, "u", defects[0], defects[-1], (defects[-1] / defects[0]) ** (1.0 / (nb - 1))))
+    lines.append("")
+    return "\n".join(lines) + "\n", "LS  : L=%2d N=%2d TSOLVE=" % (level, nit)
+
+
+def test_ls_display_full_is_ugs_text(poisson2d):
+    from ug_amd import numproc as NP
+    from ug_amd import ugmg
+    P = poisson2d
+    # the defect sequence of the same solve, through the C ABI's callback
+    ctx = ugmg.Context(0)
+    ctx.set_hierarchy(P)
+    ctx.set_params(nu1=2, nu2=2, baselevel=0, smoother=ugmg.SMOOTH_JAC, damp=0.8, base_abslimit=1e-300,
+                   base_red=1e-12, base_maxiter=1)
+    ctx.preprocess()
+    _, _, res, d = ctx.solve_host(P.top, P.rhs, red=1e-10, maxiter=50)
+    defects = [row[0] for row in d]
+    env = NP.NumProcEnv(ctx)
+    env.vector("sol", np.zeros(P.levels[-1].n))
+    env.vector("rhs", P.rhs)
+    env.create("transfer", "transfer"); env.init("transfer")
+    env.create("jac", "jac"); env.init("jac", "$damp 0.8")
+    env.create("ex", "ex"); env.init("ex")
+    env.create("base", "ls"); env.init("base", "$red 1e-12 $m 1 $I ex $abslimit 1e-300")
+    env.create("mgc", "lmgc"); env.init("mgc", "$S jac jac base $T transfer $n1 2 $n2 2 $b 0")
+    env.create("mgs", "ls")
+    env.init("mgs", "$A MAT $x sol $b rhs $m 50 $red 1e-10 $abslimit 1e-300 $I mgc $display full")
+    env.execute("mgs", "$i $d $r $s")
+    out = env.output()
+    body, last = _ug_display("mgs", defects, P.top, res.number_of_linear_iterations)
+    assert out.startswith(body), out[:400]
+    tail = out[len(body):]
+    assert tail.startswith(last) and tail.endswith("\n") and "TIT=" in tail
+    assert env.output() == ""                        # handed out once
+    env.init("mgs", "$A MAT $x sol $b rhs $m 50 $red 1e-10 $abslimit 1e-300 $I mgc $display red")
+    env.vector("sol", np.zeros(P.levels[-1].n))
+    env.execute("mgs", "$d $r $s")
+    red = env.output()                               # head, averages and the LS line only
+    assert " 1    u:" not in red and " avg:  u:" in red and "LS  : L=" in red
+    env.close()
+    ctx.close()

@@ -1123,6 +1123,8 @@ int ugmg_postprocess(ugmg_ctx *ctx) {
   return UGMG_OK;
 }
 
+int ugmg_block(const ugmg_ctx *ctx) { return ctx && ctx->nlevels > 0 ? ctx->b : UGMG_E_STATE; }
+
 int ugmg_level_size(const ugmg_ctx *ctx, int level, int *n, int *nnzb) {
   if (!ctx || level < 0 || level >= ctx->nlevels || !ctx->level_set[level]) return UGMG_E_ARG;
   if (n) *n = ctx->host[level].n;

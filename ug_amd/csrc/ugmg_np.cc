@@ -9,7 +9,10 @@
 // same defaults and the same returned state.
 #include "../../include/ugmg_np.h"
 
+#include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -99,6 +102,7 @@ struct NumProc {
   std::string I, x, b, A, c, r;
   std::vector<ugmg_lresult> res;
   bool pending_defect = false;
+  int display = 0;   // PCR_NO_DISPLAY 0, PCR_RED_DISPLAY 1, PCR_FULL_DISPLAY 2
   ugmg_params params{};
   // lmgc
   std::string T, pre, post, base;
@@ -119,6 +123,7 @@ struct ugmg_np_env {
   std::map<std::string, NumProc> np;
   std::map<std::string, std::pair<double *, int>> vec;
   std::string err;
+  std::string out, taken;   // PCR display text: pending, and the last one handed out
 };
 
 namespace {
@@ -141,8 +146,22 @@ int iter_state(const Argv &av, NumProc &o) {
   return (o.A.empty() || o.c.empty() || o.r.empty()) ? UGMG_NP_ACTIVE : UGMG_NP_EXECUTABLE;
 }
 
+// ReadArgvDisplay (np/udm/npscan.cc:654-672): "$display no|red|full"
+int read_display(const Argv &av) {
+  for (const auto &a : av) {
+    if (a.compare(0, 7, "display") != 0) continue;
+    char v[64];
+    if (std::sscanf(a.c_str(), "display %63s", v) != 1) continue;
+    if (!std::strcmp(v, "no")) return 0;
+    if (!std::strcmp(v, "red")) return 1;
+    if (!std::strcmp(v, "full")) return 2;
+  }
+  return 0;
+}
+
 // NPLinearSolverInit (ls.cc:382-407)
 int ls_state(ugmg_np_env *e, const Argv &av, NumProc &o) {
+  o.display = read_display(av);
   read_string(av, "A", o.A);
   read_string(av, "x", o.x);
   read_string(av, "b", o.b);
@@ -152,6 +171,110 @@ int ls_state(ugmg_np_env *e, const Argv &av, NumProc &o) {
   if (o.x.empty() || o.b.empty() || o.A.empty() || !e->vec.count(o.x) || !e->vec.count(o.b))
     return UGMG_NP_ACTIVE;
   return UGMG_NP_EXECUTABLE;
+}
+
+// The display of one solve: PreparePCR / DoPCR (np/udm/pcr.cc:263-606) with
+// UG's formats; component names from DEFAULT_NAMES (np/udm/udm.h:77).
+class Pcr {
+ public:
+  Pcr(std::string &out, int mode, int ncomp) : out_(out), mode_(mode), nc_(std::min(ncomp, UGMG_MAXB)) {}
+  // PreparePCR: the head line, the solver's name centred in 50 '*'
+  // (CenterInPattern, low/misc.cc:167-190)
+  void prepare(const std::string &name) {
+    if (!mode_) return;
+    out_ += "\n";
+    out_ += center(name);
+  }
+  // DoPCR with PCR_CRATE (sd = false) or PCR_CRATE_SD
+  void crate(const double *d, bool sd) {
+    const double s = norm(d);
+    if (nb_ == 0) {
+      for (int j = 0; j < nc_; ++j) init_[j] = d[j];
+      init_norm_ = s;
+      if (mode_ == 2) {
+        put(" %-3d  %c: %-12.7e   %-12.7s\n", nb_, kNames[0], d[0], "---");
+        for (int i = 1; i < nc_; ++i) put("      %c: %-12.7e   %-12.7s\n", kNames[i], d[i], "---");
+        if (nc_ > 1 && sd) put("   norm: %-12.7e   %-12.7s\n", s, "---");
+        if (nc_ > 1) out_ += "\n";
+      }
+    } else if (mode_ == 2) {
+      if (old_[0] != 0.0) put(" %-3d  %c: %-12.7e   %-12.7e\n", nb_, kNames[0], d[0], d[0] / old_[0]);
+      else put(" %-3d  %c: %-12.7e   %-12.7s\n", nb_, kNames[0], d[0], "NaN");
+      for (int i = 1; i < nc_; ++i) {
+        if (old_[i] != 0.0) put("      %c: %-12.7e   %-12.7e\n", kNames[i], d[i], d[i] / old_[i]);
+        else put("      %c: %-12.7e   %-12.7s\n", kNames[i], d[i], "NaN");
+      }
+      if (nc_ > 1 && sd) put("   norm: %-12.7e   %-12.7e\n", s, s / old_norm_);
+      if (nc_ > 1) out_ += "\n";
+    }
+    for (int j = 0; j < nc_; ++j) old_[j] = d[j];
+    old_norm_ = s;
+    ++nb_;
+  }
+  // DoPCR with PCR_AVERAGE (sd = false) or PCR_AVERAGE_SD
+  void average(const double *d, bool sd) {
+    if (nb_ < 2 || !mode_) return;
+    const double s = norm(d), e = 1.0 / (nb_ - 1);
+    if (mode_ == 2) out_ += "\n";
+    if (init_[0] != 0.0)
+      put(" %-3d avg:  %c: %-12.7e   %-12.7e   %-12.7e\n", nb_ - 1, kNames[0], init_[0], d[0],
+          std::pow(d[0] / init_[0], e));
+    else
+      put(" %-3d avg:  %c: %-12.7e   %-12.7e   %-12.7s\n", nb_ - 1, kNames[0], init_[0], d[0], "NaN");
+    for (int i = 1; i < nc_; ++i) {
+      if (init_[i] != 0.0)
+        put("           %c: %-12.7e   %-12.7e   %-12.7e\n", kNames[i], init_[i], d[i],
+            std::pow(d[i] / init_[i], e));
+      else
+        put("           %c: %-12.7e   %-12.7e   %-12.7s\n", kNames[i], init_[i], d[i], "NaN");
+    }
+    if (nc_ > 1 && sd)
+      put("        norm: %-12.7e   %-12.7e   %-12.7e\n", init_norm_, s, std::pow(s / init_norm_, e));
+    out_ += "\n";
+  }
+  // the solver's closing line ("LS  : " / "BCGS: "), times in seconds of
+  // wall clock (UG's CSTART/CSTOP read the process clock)
+  void times(const char *who, int level, int nit, double t) {
+    if (!mode_) return;
+    if (nit != 0) put("%s: L=%2d N=%2d TSOLVE=%10.4g TIT=%10.4g\n", who, level, nit, t, t / nit);
+    else put("%s: L=%2d N=%2d TSOLVE=%10.4g\n", who, level, nit, t);
+  }
+
+ private:
+  static constexpr const char *kNames = "uvwzpabcdefghijklmnoPQRSTUVWXYZ123456789";
+  double norm(const double *d) const {
+    double s = 0.0;
+    for (int j = 0; j < nc_; ++j) s += d[j] * d[j];
+    return std::sqrt(s);
+  }
+  static std::string center(const std::string &text) {
+    constexpr int kWidth = 50;   // DISPLAY_WIDTH (np/np.h:98)
+    const std::string t = (int)text.size() > kWidth ? std::string(" text too long ") : text;
+    const int len = (int)t.size(), begin = (kWidth - len) / 2, end = begin + len;
+    std::string s(kWidth, '*');
+    int i = std::max(begin - 1, 0);
+    s[i] = ' ';
+    for (i = begin; i < end; ++i) s[i] = t[i - begin];
+    if (i < kWidth) s[i] = ' ';
+    return s + "\n";
+  }
+  void put(const char *fmt, ...) {
+    char buf[256];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    out_ += buf;
+  }
+  std::string &out_;
+  int mode_, nc_, nb_ = 0;
+  double init_[UGMG_MAXB] = {}, old_[UGMG_MAXB] = {}, init_norm_ = 0.0, old_norm_ = 0.0;
+};
+
+void pcr_defect_cb(void *user, int, int, const double *d) { static_cast<Pcr *>(user)->crate(d, true); }
+
+double seconds_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
 int init_object(ugmg_np_env *e, NumProc &o) {
@@ -390,20 +513,50 @@ int ugmg_np_execute(ugmg_np_env *env, const char *name, const char *flags) {
     o->res[0] = r;
   }
   if (want_s) {
+    const int nb = std::max(1, ugmg_block(env->ctx));
+    const auto t0 = std::chrono::steady_clock::now();
     if (o->cls == "ls") {
+      // LinearSolver's display (ls.cc:666-744): DoPCR(PCR_CRATE_SD) on the
+      // first and every iteration's defect, then PCR_AVERAGE and "LS  :"
+      Pcr pcr(env->out, o->display, nb);
+      pcr.prepare(name);
       if ((rc = ugmg_solve_host(env->ctx, level, x, b, o->abslimit, o->red, o->m, o->pending_defect ? 1 : 0,
-                                &o->res[0], nullptr, nullptr)))
+                                &o->res[0], o->display ? pcr_defect_cb : nullptr, &pcr)))
         return fail(env, rc, ugmg_last_error(env->ctx));
+      pcr.average(o->res[0].last_defect, false);
+      pcr.times("LS  ", level, o->res[0].number_of_linear_iterations, seconds_since(t0));
     } else {
+      // BCGSSolver's display (ls.cc:1896-2060): DoPCR(PCR_CRATE) on the first
+      // defect and every LinearResiduum, then PCR_AVERAGE and "BCGS:", per
+      // right-hand side (scalar hierarchies)
+      const int md = 2 * std::max(o->m, 0) + 1;
+      std::vector<double> dfs(o->display ? (size_t)ncols * md : 0);
       if ((rc = ugmg_bcgs_solve_host(env->ctx, level, ncols, x, b, n, o->abslimit, o->red, o->m,
                                      o->restart, o->I.empty() ? 0 : 1, o->pending_defect ? 1 : 0,
-                                     o->res.data(), nullptr, 0)))
+                                     o->res.data(), o->display ? dfs.data() : nullptr, md)))
         return fail(env, rc, ugmg_last_error(env->ctx));
+      const double t = seconds_since(t0);
+      for (int j = 0; j < ncols && o->display; ++j) {
+        Pcr pcr(env->out, o->display, 1);
+        pcr.prepare(name);
+        const ugmg_lresult &r = o->res[j];
+        const int nd = std::min(md, 1 + r.number_of_linear_iterations);
+        for (int k = 0; k < nd; ++k) pcr.crate(&dfs[(size_t)j * md + k], false);
+        pcr.average(r.last_defect, false);
+        pcr.times("BCGS", level, r.number_of_linear_iterations, t / ncols);
+      }
     }
     o->pending_defect = false;
   }
   if (read_option(av, "p")) ugmg_postprocess(env->ctx);
   return UGMG_OK;
+}
+
+const char *ugmg_np_output(ugmg_np_env *env) {
+  if (!env) return "";
+  env->taken.swap(env->out);
+  env->out.clear();
+  return env->taken.c_str();
 }
 
 int ugmg_np_result(const ugmg_np_env *env, const char *name, int col, ugmg_lresult *res) {

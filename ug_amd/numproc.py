@@ -35,6 +35,7 @@ def _lib():
             "ugmg_np_execute": ([vp, cp, cp], ip),
             "ugmg_np_result": ([vp, cp, ip, C.POINTER(ugmg.LResult)], ip),
             "ugmg_np_params": ([vp, cp, C.POINTER(ugmg.Params)], ip),
+            "ugmg_np_output": ([vp], cp),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -102,6 +103,10 @@ class NumProcEnv:
         r = ugmg.LResult()
         self._chk(self.lib.ugmg_np_result(self.h, name.encode(), col, C.byref(r)))
         return r
+
+    def output(self):
+        """The display text of the solves since the last call (UG's PCR lines)."""
+        return self.lib.ugmg_np_output(self.h).decode()
 
     def params(self, name):
         p = ugmg.Params()

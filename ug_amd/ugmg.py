@@ -87,6 +87,7 @@ def load():
         "ugmg_preprocess": ([vp, ip], ip),
         "ugmg_postprocess": ([vp], ip),
         "ugmg_level_size": ([vp, ip, C.POINTER(ip), C.POINTER(ip)], ip),
+        "ugmg_block": ([vp], ip),
         "ugmg_num_colors": ([vp, ip], ip),
         "ugmg_get_colors": ([vp, ip, vp], ip),
         "ugmg_get_pattern": ([vp, ip, vp, vp, vp], ip),

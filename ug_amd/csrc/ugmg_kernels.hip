@@ -778,6 +778,23 @@ __global__ __launch_bounds__(kBlock) void k_dot_part_b3(int n, const T *__restri
     for (int q = 0; q < B; ++q) part[((size_t)blockIdx.y * B + q) * gridDim.x + blockIdx.x] = s[q];
 }
 
+// lane's share of row . b for the base GEMV: four independent accumulators
+// keep four loads of the row in flight per lane (the wave sums the shares)
+template <typename T>
+__device__ __forceinline__ double base_row_share(const double *__restrict__ rr,
+                                                 const T *__restrict__ bb, int n, int lane) {
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int j = lane;
+  for (; j + 192 < n; j += 256) {
+    a0 += rr[j] * (double)bb[j];
+    a1 += rr[j + 64] * (double)bb[j + 64];
+    a2 += rr[j + 128] * (double)bb[j + 128];
+    a3 += rr[j + 192] * (double)bb[j + 192];
+  }
+  for (; j < n; j += 64) a0 += rr[j] * (double)bb[j];
+  return (a0 + a1) + (a2 + a3);
+}
+
 // ls over ex on the base level with per-component stop tests (sc_cmp:
 // every component below its limit); inv is (3n) x (3n)
 template <typename T>
@@ -793,6 +810,8 @@ __global__ __launch_bounds__(kBlock) void k_base_gemv_b3(int n3, const double *_
   const int lane = threadIdx.x & 63;
   const int cj = blockIdx.y;
   const size_t o = (size_t)cj * vs;
+  // the row product first, while the norms below are reduced
+  double x = row < n3 ? base_row_share<T>(inv + (size_t)row * n3, b + o, n3, lane) : 0.0;
   double cur[B];
 #pragma unroll
   for (int q = 0; q < B; ++q) {
@@ -818,11 +837,9 @@ __global__ __launch_bounds__(kBlock) void k_base_gemv_b3(int n3, const double *_
   }
   const bool conv = below_abs || below_red;
   if (row >= n3) return;
-  double x = 0.0;
-  if (!conv) {
-    const double *__restrict__ rr = inv + (size_t)row * n3;
-    const T *__restrict__ bb = b + o;
-    for (int j = lane; j < n3; j += 64) x += rr[j] * (double)bb[j];
+  if (conv) {
+    x = 0.0;
+  } else {
     for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
     x *= damp;
   }
@@ -1257,6 +1274,9 @@ __global__ __launch_bounds__(kBlock) void k_base_gemv(int n, const double *__res
   const int cj = blockIdx.y;
   const size_t o = (size_t)cj * vs;
   part += (size_t)cj * nparts;
+  // the row product is formed while the stop test's norm is reduced (and
+  // dropped if the test says converged)
+  double x = row < n ? base_row_share<T>(inv + (size_t)row * n, b + o, n, lane) : 0.0;
   double s = 0.0;
   for (int i = lane; i < nparts; i += 64) s += part[i];
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
@@ -1271,11 +1291,9 @@ __global__ __launch_bounds__(kBlock) void k_base_gemv(int n, const double *__res
     conv = fabs(cur) < fabs(abslimit) || fabs(cur) < fabs(target);
   }
   if (row >= n) return;
-  double x = 0.0;
-  if (!conv) {
-    const double *__restrict__ rr = inv + (size_t)row * n;
-    const T *__restrict__ bb = b + o;
-    for (int j = lane; j < n; j += 64) x += rr[j] * (double)bb[j];
+  if (conv) {
+    x = 0.0;
+  } else {
     for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
     x *= damp;
   }
@@ -1325,6 +1343,13 @@ void launch_interp(hipStream_t st, const DSell<T> &P, const uint32_t *fskip, con
                                                                         damp, t, c, vsf, vsc);
                 else k_interp_b3<T, 0, NTV><<<g, kBlock, 0, st>>>(P, P.nslices, fskip, cc, damp,
                                                                   t, c, vsf, vsc));
+    return;
+  }
+  // interpolation rows hold 1, 2, 4 or 8 coarse corners: chunks of 4
+  // entries load fewer clamped repeats than 8 (measured faster)
+  if (P.wmax <= 8) {
+    NT_DISPATCH(nt, if (c) k_interp<4, T, 1, NTV><<<g, kBlock, 0, st>>>(P, P.nslices, fskip, cc, damp, t, c, vsf, vsc);
+                else k_interp<4, T, 0, NTV><<<g, kBlock, 0, st>>>(P, P.nslices, fskip, cc, damp, t, c, vsf, vsc));
     return;
   }
   W_DISPATCH(P.wmax, NT_DISPATCH(nt, if (c) k_interp<MWV, T, 1, NTV><<<g, kBlock, 0, st>>>(

@@ -314,8 +314,10 @@ def main():
     ap.add_argument("--no-bcgs", action="store_true")
     ap.add_argument("--elasticity-nref", type=int, default=6, help="Kuhn-cube refinements (6: 65^3 nodes)")
     ap.add_argument("--no-elasticity", action="store_true")
-    ap.add_argument("--kernels-only", action="store_true",
-                    help="run only the roofline kernels and a copy kernel of known size (for rocprofv3 --pmc)")
+    ap.add_argument("--kernels-only", nargs="?", const="spmv", choices=["spmv", "gs"],
+                    help="only the finest level's kernels (for PMC runs): 'spmv' (10 FP64 SpMV "
+                         "launches) or 'gs' (10 FP64 MC-GS steps: 8 colour sweeps + defect each), "
+                         "then a copy kernel of known size")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -341,16 +343,19 @@ def main():
     ctx64.preprocess()
     prep_s = time.time() - t0
     if args.kernels_only:
-        # 10 FP64 SpMV launches on the finest level, then a copy kernel of
-        # known size (torch elementwise: reads 8*m bytes, writes 8*m bytes)
-        ms, by = ctx64.time_kernel(top, ugmg.K_SPMV64, 10)
+        # 10 FP64 SpMV launches (or MC-GS steps) on the finest level, then a
+        # copy kernel of known size (torch elementwise: reads 8*m bytes,
+        # writes 8*m bytes)
+        kind = ugmg.K_SPMV64 if args.kernels_only == "spmv" else ugmg.K_GS64
+        ms, by = ctx64.time_kernel(top, kind, 10)
         m = 1 << 26
         src = torch.ones(m, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
         for _ in range(5):
             torch.mul(src, 1.0, out=dst)
         torch.cuda.synchronize()
-        print(json.dumps({"spmv_fp64_us": ms * 1e3, "spmv_fp64_algorithmic_bytes": by,
+        print(json.dumps({f"{args.kernels_only}_fp64_us": ms * 1e3,
+                          f"{args.kernels_only}_fp64_algorithmic_bytes": by,
                           "copy_kernel_bytes_read": 8 * m, "copy_kernel_bytes_written": 8 * m}))
         ctx64.close()
         return 0

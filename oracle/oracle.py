@@ -66,6 +66,9 @@ def lib():
         L.orc_ls_lmgc_solve.argtypes = [vp, C.c_int, vp, dp, dp, C.c_int, dp, dp, C.c_int, dp,
                                         C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.orc_lmgc_iter.argtypes = [vp, C.c_int, vp, dp, dp]
+        L.orc_fullrefinelevel.argtypes = [vp, C.c_int]
+        L.orc_ls_lmgc_solve_surface.argtypes = [vp, C.c_int, vp, vp, vp, C.c_int, dp, dp, C.c_int, dp,
+                                                C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.orc_galerkin.argtypes = [vp, vp, dp]
         L.orc_galerkin.restype = None
         L.orc_bcgs_solve.argtypes = [vp, C.c_int, vp, C.c_int, dp, dp, C.c_int, dp, dp, C.c_int, C.c_int, dp,
@@ -248,6 +251,35 @@ def ls_solve(H, top, p, rhs, x0=None, red=1e-10, abslimit=1e-300, maxiter=50, do
         raise RuntimeError(f"oracle solve failed ({rc})")
     k = nit.value
     return dict(x=x, b=b, defects=defects[:(k + 1) * nb].reshape(k + 1, nb), nit=k, converged=bool(conv.value))
+
+
+def fullrefinelevel(H, top):
+    return lib().orc_fullrefinelevel(H.ptr(0), top)
+
+
+def ls_solve_surface(H, top, p, bs, xs=None, red=1e-10, abslimit=1e-300, maxiter=50, do_defect=True):
+    """ls + lmgc on a locally refined hierarchy: bs[l] / xs[l] for levels
+    baselevel..top (the defect recurrence runs in every level's b); returns
+    dict(x=[per level], b=[per level], defects[nit+1, b], nit, converged)."""
+    nb = H.levels[top].b
+    bl = min(p.baselevel, top)
+    xs = [None if l < bl else (np.zeros(H.levels[l].n * nb) if xs is None or xs[l] is None
+                               else np.array(xs[l], dtype=np.float64, copy=True)) for l in range(top + 1)]
+    bs = [None if l < bl else np.array(bs[l], dtype=np.float64, copy=True) for l in range(top + 1)]
+    XP = (C.c_void_p * (top + 1))(*[None if v is None else v.ctypes.data for v in xs])
+    BP = (C.c_void_p * (top + 1))(*[None if v is None else v.ctypes.data for v in bs])
+    ab = np.full(3, abslimit)
+    rd = np.full(3, red)
+    defects = np.zeros((maxiter + 1) * nb)
+    nit, conv = C.c_int(), C.c_int()
+    rc = lib().orc_ls_lmgc_solve_surface(H.ptr(0), top, C.byref(p), XP, BP, int(do_defect),
+                                         ab.ctypes.data, rd.ctypes.data, maxiter, defects.ctypes.data,
+                                         C.byref(nit), C.byref(conv))
+    if rc:
+        raise RuntimeError(f"oracle surface solve failed ({rc})")
+    k = nit.value
+    return dict(x=xs, b=bs, defects=defects[:(k + 1) * nb].reshape(k + 1, nb), nit=k,
+                converged=bool(conv.value))
 
 
 def bcgs_solve(H, top, p, rhs, x0=None, red=1e-8, abslimit=1e-300, maxiter=200, restart=0, use_iter=True,

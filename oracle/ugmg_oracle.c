@@ -20,9 +20,11 @@
  *   LinearSolver/LSUpdate     np/procs/ls.cc:637-749, 869-875
  *   LinearDefect/Residuum     np/procs/ls.cc:562-593
  *   sc_cmp / sc_mul_check     np/udm/npscan.cc:1027-1057, 1151-1161
- * The hierarchy is uniformly refined: FULLREFINELEVEL is the top level, so
- * ON_SURFACE operations touch the top level's NEW_DEFECT (VCLASS >= 2)
- * vectors only (vecloop.ct:22-39).
+ * orc_ls_lmgc_solve takes a uniformly refined hierarchy: FULLREFINELEVEL is
+ * the top level, so ON_SURFACE operations touch the top level's NEW_DEFECT
+ * (VCLASS >= 2) vectors only (vecloop.ct:22-39).  orc_ls_lmgc_solve_surface
+ * takes a locally refined one: the surface then reaches down to
+ * FULLREFINELEVEL (FINE_GRID_DOF vectors of the lower levels).
  */
 #include "ugmg_oracle.h"
 
@@ -673,6 +675,120 @@ int orc_ls_lmgc_solve(const orc_level *levels, int top, const orc_params *p,
     }
   }
   if (!conv) it = i;
+  *nit = it;
+  *converged = conv;
+  mg_free(&mg);
+  return err;
+}
+
+/* ------------------------------------------------ composite surface */
+
+/* FINE_GRID_DOF (SetSurfaceClasses, gm/algebra.cc:2908-2909) */
+static int fine_grid_dof(const orc_level *L, int v) {
+  return L->vclass[v] >= NEWDEF_CLASS && L->vnclass[v] <= 1;
+}
+
+/* FULLREFINELEVEL as SetSurfaceClasses leaves it (algebra.cc:2885-2916):
+ * the lowest level holding a FINE_GRID_DOF vector, the top level if none */
+int orc_fullrefinelevel(const orc_level *levels, int top) {
+  int f = top;
+  for (int l = top; l >= 0; --l)
+    for (int v = 0; v < levels[l].n; ++v)
+      if (fine_grid_dof(&levels[l], v)) { f = l; break; }
+  return f;
+}
+
+/* row v of level l on the surface of levels frl..tl (vecloop.ct:22-39,
+ * matloop.ct:22-50): FINE_GRID_DOF below tl, NEW_DEFECT on tl */
+static int on_surface(const orc_level *L, int l, int tl, int v) {
+  return l < tl ? fine_grid_dof(L, v) : L->vclass[v] >= NEWDEF_CLASS;
+}
+
+/* dmatmul_minus ON_SURFACE (ugblas.cc:3997-4042 in the T_SURFACE_MODE loop
+ * of matmode.ct): level by level from FULLREFINELEVEL, x_l -= A_l y_l */
+static void composite_dmatmul_minus(const orc_level *levels, int frl, int tl,
+                                    double **x, double *const *y) {
+  for (int l = frl; l <= tl; ++l) {
+    const orc_level *L = &levels[l];
+    const int b = L->b, bb = b * b;
+    for (int v = 0; v < L->n; ++v) {
+      if (!on_surface(L, l, tl, v)) continue;
+      double s[ORC_MAXB] = {0.0, 0.0, 0.0};
+      for (int m = L->rp[v]; m < L->rp[v + 1]; ++m) {
+        const double *M = L->a + (size_t)m * bb;
+        const double *w = y[l] + (size_t)L->ci[m] * b;
+        for (int i = 0; i < b; ++i) {
+          double t = 0.0;
+          for (int j = 0; j < b; ++j) t += M[i * b + j] * w[j];
+          s[i] += t;
+        }
+      }
+      for (int i = 0; i < b; ++i) x[l][(size_t)v * b + i] -= s[i];
+    }
+  }
+}
+
+/* dnrm2x ON_SURFACE (ugblas.cc:3156-3176): one running sum per component
+ * over the surface rows, level by level, then sqrt */
+static void composite_nrm2x(const orc_level *levels, int frl, int tl,
+                            double *const *x, double *out) {
+  const int b = levels[tl].b;
+  double a[ORC_MAXB] = {0.0, 0.0, 0.0};
+  for (int l = frl; l <= tl; ++l) {
+    const orc_level *L = &levels[l];
+    for (int v = 0; v < L->n; ++v) {
+      if (!on_surface(L, l, tl, v)) continue;
+      for (int i = 0; i < b; ++i) {
+        double s = x[l][(size_t)v * b + i];
+        a[i] += s * s;
+      }
+    }
+  }
+  for (int i = 0; i < b; ++i) out[i] = sqrt(a[i]);
+}
+
+int orc_ls_lmgc_solve_surface(const orc_level *levels, int top, const orc_params *p,
+                              double **x, double **b, int do_defect,
+                              const double *abslimit, const double *reduction,
+                              int maxiter, double *defects, int *nit, int *converged) {
+  orc_mg mg;
+  const int nb = levels[top].b;
+  const int bl = p->baselevel < top ? p->baselevel : top;
+  const int frl = orc_fullrefinelevel(levels, top);
+  if (frl < bl) return 3;   /* surface rows below the base level */
+  int err = mg_setup(&mg, levels, top, p);
+  if (err) { mg_free(&mg); return err; }
+  /* the defect recurrence runs in the coarse levels' b too */
+  for (int l = bl; l <= top; ++l)
+    memcpy(mg.b[l], b[l], sizeof(double) * (size_t)levels[l].n * nb);
+  if (do_defect) composite_dmatmul_minus(levels, frl, top, mg.b, x); /* LinearDefect */
+  double first[ORC_MAXB], last[ORC_MAXB], defect2reach[ORC_MAXB];
+  composite_nrm2x(levels, frl, top, mg.b, first);                    /* LinearResiduum */
+  for (int i = 0; i < nb; ++i) {
+    defects[i] = first[i];
+    defect2reach[i] = first[i] * reduction[i];
+    if (defect2reach[i] == 0.0) defect2reach[i] = reduction[i];
+  }
+  int conv = sc_cmp(nb, first, abslimit);
+  int it = 0;
+  int i;
+  for (i = 0; i < maxiter; ++i) {
+    if (conv) break;
+    memset(mg.c[top], 0, sizeof(double) * (size_t)levels[top].n * nb); /* dset(c,0) */
+    if (lmgc(&mg, top)) { err = 1; break; }
+    for (int l = bl; l <= top; ++l)                 /* LSUpdate: baselevel..level */
+      dadd((size_t)levels[l].n * nb, x[l], mg.c[l]);
+    composite_nrm2x(levels, frl, top, mg.b, last);
+    for (int k = 0; k < nb; ++k) defects[(size_t)(i + 1) * nb + k] = last[k];
+    if (sc_cmp(nb, last, abslimit) || sc_cmp(nb, last, defect2reach)) {
+      conv = 1;
+      it = i + 1;
+      break;
+    }
+  }
+  if (!conv) it = i;
+  for (int l = bl; l <= top; ++l)
+    memcpy(b[l], mg.b[l], sizeof(double) * (size_t)levels[l].n * nb);
   *nit = it;
   *converged = conv;
   mg_free(&mg);

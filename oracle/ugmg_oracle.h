@@ -112,6 +112,21 @@ int orc_ls_lmgc_solve(const orc_level *levels, int top, const orc_params *p,
                       const double *abslimit, const double *reduction,
                       int maxiter, double *defects, int *nit, int *converged);
 
+/* FULLREFINELEVEL of SetSurfaceClasses (gm/algebra.cc:2885-2916): the
+ * lowest level with a FINE_GRID_DOF vector (VCLASS >= 2, VNCLASS <= 1), the
+ * top level if there is none. */
+int orc_fullrefinelevel(const orc_level *levels, int top);
+/* The same solve on a locally refined hierarchy: x[l], b[l] for every level
+ * l in baselevel..top (entries below may be NULL).  LinearDefect and
+ * LinearResiduum run ON_SURFACE from FULLREFINELEVEL (FINE_GRID_DOF rows on
+ * the lower levels, NEW_DEFECT rows on top; vecloop.ct / matloop.ct), the
+ * coarse levels' b carry the defect recurrence from call to call, and
+ * LSUpdate adds c into x on baselevel..top (ls.cc:869-875).  Returns 3 when
+ * FULLREFINELEVEL lies below the base level. */
+int orc_ls_lmgc_solve_surface(const orc_level *levels, int top, const orc_params *p,
+                              double **x, double **b, int do_defect,
+                              const double *abslimit, const double *reduction,
+                              int maxiter, double *defects, int *nit, int *converged);
 /* One lmgc iteration on `top` (NP_ITER semantics): c overwritten with the
  * correction, b := b - A c.  Work vectors for the coarse levels are
  * allocated internally. */

@@ -324,3 +324,210 @@ def jump3d(nref: int = 7, seed: int = 0x5EED0002) -> Problem:
     (nref=7: 129^3 = 2,146,689 nodes), kappa = 10^U[-2,2] per finest element."""
     n = 2 ** nref + 1
     return q1(3, 1, nref, KAPPA_JUMP, seed, name=f"jump3d_q1_{n}^3")
+
+
+# ------------------------------------------------------- local refinement
+def _propagate(lv: Level, cls: np.ndarray) -> np.ndarray:
+    """PropagateVectorClass(3) then (2) (gm/algebra.cc:4063-4080, 4145-4190):
+    matrix neighbours of class-3 vectors below 3 become 2, then neighbours of
+    class-2 vectors below 2 become 1 (row lists without the diagonal)."""
+    cls = cls.copy()
+    rp, ci = lv.row_ptr, lv.col_idx
+    rows = np.repeat(np.arange(lv.n), np.diff(rp))
+    off = ci != rows
+    for k in (3, 2):
+        src = cls[rows] == k
+        dst = ci[off & src]
+        hit = np.zeros(lv.n, dtype=bool)
+        hit[dst] = True
+        cls[hit & (cls < k)] = k - 1
+    return cls
+
+
+def fine_grid_dof(lv: Level) -> np.ndarray:
+    """FINE_GRID_DOF (gm/algebra.cc:2908-2909): VCLASS >= 2 and VNCLASS <= 1."""
+    return (lv.vclass >= 2) & (lv.vnclass <= 1)
+
+
+def fullrefinelevel(P: Problem) -> int:
+    """The lowest level with a FINE_GRID_DOF vector, the top level if none
+    (SetSurfaceClasses, gm/algebra.cc:2885-2916)."""
+    for l in range(P.top + 1):
+        if fine_grid_dof(P.levels[l]).any():
+            return l
+    return P.top
+
+
+def surface_rhs(P: Problem, bl: int, seed: int) -> list:
+    """Right-hand sides f_l ~ U[-1, 1] on every vector of levels bl..top (0 on
+    Dirichlet components), None below bl."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for l, lv in enumerate(P.levels):
+        if l < bl:
+            out.append(None)
+            continue
+        f = rng.uniform(-1, 1, lv.n * lv.b)
+        for j in range(lv.b):
+            f[j::lv.b][(lv.skip >> j) & 1 == 1] = 0.0
+        out.append(f)
+    return out
+
+
+# 3x3 coupling of the block variant (symmetric positive definite)
+_M3 = np.array([[2.0, 0.5, 0.2], [0.5, 2.0, 0.3], [0.2, 0.3, 2.0]])
+
+
+def _p1_stiffness(xy: np.ndarray, tri: np.ndarray) -> np.ndarray:
+    """P1 stiffness of -Laplace on each triangle: (ne, 3, 3)."""
+    p0, p1, p2 = xy[tri[:, 0]], xy[tri[:, 1]], xy[tri[:, 2]]
+    bx = np.stack([p1[:, 1] - p2[:, 1], p2[:, 1] - p0[:, 1], p0[:, 1] - p1[:, 1]], 1)
+    cx = np.stack([p2[:, 0] - p1[:, 0], p0[:, 0] - p2[:, 0], p1[:, 0] - p0[:, 0]], 1)
+    area = 0.5 * np.abs((p1[:, 0] - p0[:, 0]) * (p2[:, 1] - p0[:, 1]) - (p2[:, 0] - p0[:, 0]) * (p1[:, 1] - p0[:, 1]))
+    return (bx[:, :, None] * bx[:, None, :] + cx[:, :, None] * cx[:, None, :]) / (4.0 * area)[:, None, None]
+
+
+def _assemble_p1(xy: np.ndarray, tri: np.ndarray, ncomp: int):
+    """Assembly over the elements in order (element matrices added as
+    AddVlistMValues does), rows diagonal first then ascending columns, UG's
+    Dirichlet rows (AssembleDirichletBoundary: row zeroed, diagonal block I)
+    on the boundary of the unit square, every component."""
+    n = len(xy)
+    K = _p1_stiffness(xy, tri)
+    r = np.repeat(tri, 3, axis=1).ravel()
+    c = np.tile(tri, (1, 3)).ravel()
+    # unique (row, col) pairs, diagonal first in each row (code 0), then the
+    # columns ascending (code col + 1)
+    key = r.astype(np.int64) * (n + 1) + np.where(r == c, 0, c + 1)
+    uniq, inv = np.unique(key, return_inverse=True)
+    vals = np.zeros(len(uniq))
+    np.add.at(vals, inv, K.ravel())
+    rows = uniq // (n + 1)
+    code = uniq % (n + 1)
+    cols = np.where(code == 0, rows, code - 1)
+    diag = code == 0
+    row_ptr = np.zeros(n + 1, dtype=np.int32)
+    np.add.at(row_ptr, rows + 1, 1)
+    row_ptr = np.cumsum(row_ptr).astype(np.int32)
+    bnd = (np.isclose(xy[:, 0], 0) | np.isclose(xy[:, 0], 1) | np.isclose(xy[:, 1], 0) | np.isclose(xy[:, 1], 1))
+    vals[bnd[rows]] = 0.0
+    vals[bnd[rows] & diag] = 1.0
+    if ncomp == 1:
+        blocks = vals
+    else:
+        blocks = vals[:, None, None] * _M3[None, :ncomp, :ncomp]
+        blocks[bnd[rows]] = 0.0
+        blocks[bnd[rows] & diag] = np.eye(ncomp)
+        blocks = blocks.ravel()
+    skip = np.where(bnd, (1 << ncomp) - 1, 0).astype(np.uint32)
+    return row_ptr, cols.astype(np.int32), blocks.astype(np.float64), skip
+
+
+def _classes(lv: Level, tri: np.ndarray, seed_nodes: np.ndarray) -> np.ndarray:
+    """SeedVectorClasses over the elements whose corners all carry node class
+    3 (MinNodeClass == 3), then PropagateVectorClasses."""
+    full = seed_nodes[tri].all(axis=1)
+    cls = np.zeros(lv.n, dtype=np.uint8)
+    cls[tri[full].ravel()] = 3
+    return _propagate(lv, cls)
+
+
+def p1_local(nuni: int, depth: int, ncomp: int = 1) -> Problem:
+    """2D P1 -Laplace (ncomp = 3: the same stiffness times a fixed 3x3 SPD
+    coupling per entry) on the unit square, coarse grid 2x2 squares split
+    into 8 triangles, refined uniformly nuni times and then `depth` more
+    times only toward the corner at the origin: refinement step k marks the
+    red (regular) triangles inside [0, 2^-k]^2.  Each step refines as
+    red-green refinement does: marked triangles red (4 children), unmarked
+    triangles with one refined edge green (bisected), every other triangle
+    copied (yellow).  Vector classes as UG computes them: NCLASS 3 on the
+    corners of the red and green elements of a level (refine.cc:6760-6790),
+    NNCLASS 3 on the corners of the coarse elements refined red or green
+    (refine.cc:2890-2935), VCLASS/VNCLASS seeded from the elements whose
+    corners all have class 3 and propagated over the matrix pattern
+    (SetSurfaceClasses, gm/algebra.cc:2838-2930); the top level's VNCLASS
+    is 0.  Level l's list: the sons of level l-1's vectors in order, then the
+    edge midpoints (ascending edge); interpolation: son 1, midpoint 1/2 + 1/2.
+    Level 0 holds 9 vectors; the copies keep every level on the whole
+    square (far copies have class 0, off the surface)."""
+    if nuni < 0 or depth < 0:
+        raise ValueError("nuni, depth >= 0")
+    xs = np.linspace(0.0, 1.0, 3)
+    xy = np.array([(x, y) for y in xs for x in xs])
+    tri = []
+    for j in range(2):
+        for i in range(2):
+            a, b, c, d = i + 3 * j, i + 1 + 3 * j, i + 1 + 3 * (j + 1), i + 3 * (j + 1)
+            tri += [(a, b, c), (a, c, d)]
+    tri = np.array(tri, dtype=np.int64)
+    kind = np.zeros(len(tri), dtype=np.int8)          # 0 red (regular), 1 green, 2 copy
+    meshes = [(xy, tri, kind)]
+    marks = []
+    for step in range(nuni + depth):
+        a = 1.0 if step < nuni else 0.5 ** (step - nuni + 1)
+        inside = (xy[tri] <= a + 1e-12).all(axis=(1, 2))
+        marked = (kind == 0) & inside
+        edges_of = np.stack([tri[:, [0, 1]], tri[:, [1, 2]], tri[:, [2, 0]]], 1)   # (ne, 3, 2)
+        ekey = np.sort(edges_of, axis=2)
+        nv = len(xy)
+        while True:   # closure: two or more refined edges -> red as well
+            ref = np.unique((ekey[marked][:, :, 0] * nv + ekey[marked][:, :, 1]).ravel())
+            hit = np.isin(ekey[:, :, 0] * nv + ekey[:, :, 1], ref).sum(axis=1)
+            more = (~marked) & (hit >= 2)
+            if not more.any():
+                break
+            if (kind[more] != 0).any():
+                raise ValueError("closure reaches an irregular element")
+            marked |= more
+        green = (~marked) & (hit == 1)
+        if (kind[green] != 0).any():
+            raise ValueError("green closure of an irregular element")
+        mid = {int(k): nv + i for i, k in enumerate(ref)}
+        xy_new = np.vstack([xy, 0.5 * (xy[ref // nv] + xy[ref % nv])])
+        tri_new, kind_new = [], []
+        for e in range(len(tri)):
+            p, q, r = (int(v) for v in tri[e])
+            if marked[e]:
+                mpq, mqr, mrp = (mid[min(u, w) * nv + max(u, w)] for u, w in ((p, q), (q, r), (r, p)))
+                tri_new += [(p, mpq, mrp), (mpq, q, mqr), (mrp, mqr, r), (mpq, mqr, mrp)]
+                kind_new += [0, 0, 0, 0]
+            elif green[e]:
+                for (u, w, o) in ((p, q, r), (q, r, p), (r, p, q)):
+                    k = min(u, w) * nv + max(u, w)
+                    if k in mid:
+                        m = mid[k]
+                        tri_new += [(u, m, o), (m, w, o)]
+                        kind_new += [1, 1]
+                        break
+            else:
+                tri_new.append((p, q, r))
+                kind_new.append(2)
+        marks.append((marked | green, ref))
+        xy, tri, kind = xy_new, np.array(tri_new, dtype=np.int64), np.array(kind_new, dtype=np.int8)
+        meshes.append((xy, tri, kind))
+    levels = []
+    top = len(meshes) - 1
+    for l, (xy_l, tri_l, kind_l) in enumerate(meshes):
+        rp, ci, vals, skip = _assemble_p1(xy_l, tri_l, ncomp)
+        n = len(xy_l)
+        lv = Level(n=n, b=ncomp, row_ptr=rp, col_idx=ci, vals=vals, skip=skip,
+                   vclass=np.full(n, 3, np.uint8), vnclass=np.zeros(n, np.uint8))
+        if l >= 1:
+            nprev = len(meshes[l - 1][0])
+            ref = marks[l - 1][1]
+            pp = np.concatenate([np.arange(nprev + 1), nprev + 2 * np.arange(1, len(ref) + 1)]).astype(np.int32)
+            pi = np.concatenate([np.arange(nprev), np.stack([ref // nprev, ref % nprev], 1).ravel()]).astype(np.int32)
+            pw = np.concatenate([np.ones(nprev), np.full(2 * len(ref), 0.5)])
+            lv.p_ptr, lv.p_idx, lv.p_w = pp, pi, pw
+            real = np.zeros(n, dtype=bool)
+            real[tri_l[kind_l < 2].ravel()] = True            # NCLASS 3
+            lv.vclass = _classes(lv, tri_l, real)
+        levels.append(lv)
+    for l in range(top):
+        xy_l, tri_l, _ = meshes[l]
+        refined, _ = marks[l]
+        nn3 = np.zeros(levels[l].n, dtype=bool)
+        nn3[tri_l[refined].ravel()] = True                     # NNCLASS 3
+        levels[l].vnclass = _classes(levels[l], tri_l, nn3)
+    rhs = np.zeros(levels[-1].n * ncomp)
+    return Problem(dim=2, levels=levels, rhs=rhs, name=f"p1_local_{nuni}u{depth}l_b{ncomp}")

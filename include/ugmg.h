@@ -166,6 +166,27 @@ int ugmg_solve_host(ugmg_ctx *ctx, int level, double *x, double *b,
                     int maxiter, int do_defect, ugmg_lresult *res,
                     ugmg_defect_cb cb, void *user);
 
+/* ---- NP_LINEAR_SOLVER on a locally refined hierarchy --------------------
+ * FULLREFINELEVEL as SetSurfaceClasses (gm/algebra.cc:2885-2916) leaves it:
+ * the lowest preprocessed level with a FINE_GRID_DOF vector (VCLASS >= 2,
+ * VNCLASS <= 1), the top level when there is none (uniform refinement). */
+int ugmg_fullrefinelevel(const ugmg_ctx *ctx);
+/* LinearSolver (ls.cc:637-749) when the surface spans several levels:
+ * x[l], b[l] for every level l in baselevel..level (n_l * b doubles, list
+ * order; entries below baselevel are not read).  LinearDefect / Residuum
+ * run ON_SURFACE from FULLREFINELEVEL (FINE_GRID_DOF rows below `level`,
+ * NEW_DEFECT rows on it: matloop.ct / vecloop.ct), the coarse levels' b keep
+ * their defect rows from iteration to iteration, and LSUpdate adds c into x
+ * on baselevel..level (ls.cc:869-875).  FP64 (precision UGMG_FP64), one
+ * column; UGMG_E_ARG when FULLREFINELEVEL lies below the base level.  On a
+ * uniformly refined hierarchy this is ugmg_solve.  _host: host arrays. */
+int ugmg_solve_surface(ugmg_ctx *ctx, int level, double *const *x, double *const *b,
+                       const double *abslimit, const double *reduction, int maxiter,
+                       int do_defect, ugmg_lresult *res, ugmg_defect_cb cb, void *user);
+int ugmg_solve_surface_host(ugmg_ctx *ctx, int level, double *const *x, double *const *b,
+                            const double *abslimit, const double *reduction, int maxiter,
+                            int do_defect, ugmg_lresult *res, ugmg_defect_cb cb, void *user);
+
 /* ---- several right-hand sides per launch --------------------------------
  * UG solves one right-hand side per call; these entry points run nrhs
  * independent columns through the same kernels (one launch per operation,

@@ -77,6 +77,10 @@ struct DevLevel {
   int *perm = nullptr;
   double *c = nullptr, *b = nullptr, *t = nullptr;
   float *c32 = nullptr, *b32 = nullptr, *t32 = nullptr;
+  // surface solve (allocated on first use): FINE_GRID_DOF class bytes, the
+  // level's x in device order, its b-component squared sums
+  uint8_t *fgd = nullptr;
+  double *xs = nullptr, *sq = nullptr;
 
   template <typename T> T *cv() const { if constexpr (std::is_same_v<T, double>) return c; else return c32; }
   template <typename T> T *bv() const { if constexpr (std::is_same_v<T, double>) return b; else return b32; }
@@ -98,6 +102,7 @@ struct ugmg_ctx {
   bool params_set = false;
   bool prepared = false;
   int top = -1, bl = 0;
+  int frl = -1;   // FULLREFINELEVEL of the levels set
   std::vector<DevLevel> dev;
   std::vector<void *> allocs;
   double dev_bytes = 0.0;
@@ -860,6 +865,17 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
     return fail(ctx, UGMG_E_NOMEM, "host allocation during preprocess");
   }
   ctx->top = level;
+  // FULLREFINELEVEL (SetSurfaceClasses, gm/algebra.cc:2885-2916): the lowest
+  // level with a FINE_GRID_DOF vector, the top level if there is none
+  ctx->frl = level;
+  for (int l = level; l >= 0; --l) {
+    const HostLevel &H = ctx->host[l];
+    for (size_t v = 0; v < H.vc.size() && v < H.vnc.size(); ++v)
+      if (H.vc[v] >= 2 && H.vnc[v] <= 1) {
+        ctx->frl = l;
+        break;
+      }
+  }
   ctx->prepared = true;
   return UGMG_OK;
 }
@@ -1472,6 +1488,175 @@ int ugmg_solve(ugmg_ctx *ctx, int level, double *x, double *b, const double *abs
   if (!conv) res->number_of_linear_iterations = i;
   res->converged = conv;
   return move_out();
+}
+
+int ugmg_fullrefinelevel(const ugmg_ctx *ctx) {
+  if (!ctx) return UGMG_E_ARG;
+  return ctx->prepared ? ctx->frl : UGMG_E_STATE;
+}
+
+// LinearSolver (ls.cc:637-749) on a locally refined hierarchy.  The surface
+// runs from FULLREFINELEVEL: LinearDefect and LinearResiduum take the
+// FINE_GRID_DOF rows of the levels below `level` and the NEW_DEFECT rows of
+// `level` (the T_SURFACE_MODE loops of matloop.ct / vecloop.ct, whatever
+// their from-level); the coarse levels' b carry the defect recurrence from
+// iteration to iteration (lmgc zeroes only what the restriction covers);
+// LSUpdate adds each level's correction into x on baselevel..level.
+int ugmg_solve_surface(ugmg_ctx *ctx, int level, double *const *x, double *const *b,
+                       const double *abslimit, const double *reduction, int maxiter, int do_defect,
+                       ugmg_lresult *res, ugmg_defect_cb cb, void *user) {
+  int rc = check_ready(ctx, level);
+  if (rc) return rc;
+  if (!x || !b || !abslimit || !reduction || !res || maxiter < 0)
+    return fail(ctx, UGMG_E_ARG, "null argument");
+  std::memset(res, 0, sizeof(*res));
+  // the level's own NEW_DEFECT loop always runs, the FINE_GRID_DOF loop
+  // from FULLREFINELEVEL up to the level below it
+  const int bl = ctx->bl, frl = std::min(ctx->frl, level);
+  if (frl < bl) return fail(ctx, UGMG_E_ARG, "FULLREFINELEVEL lies below the base level");
+  if (ctx->p.precision != UGMG_FP64) return fail(ctx, UGMG_E_ARG, "the surface solve runs in FP64");
+  for (int l = bl; l <= level; ++l)
+    if (!x[l] || !b[l]) return fail(ctx, UGMG_E_ARG, "x[l] and b[l] are needed on baselevel..level");
+  if (ctx->base.failed) {
+    res->error_code = UGMG_E_SINGULAR;
+    return fail(ctx, UGMG_E_SINGULAR, "zero pivot in the band LU of the base level");
+  }
+  hipStream_t st = ctx->stream;
+  for (int l = bl; l <= level; ++l) {
+    DevLevel &d = ctx->dev[l];
+    if (!d.fgd) {
+      if ((rc = dalloc(ctx, &d.fgd, (size_t)d.n))) return rc;
+      launch_fgd_class(st, d.n, d.vclass, d.vnclass, d.fgd);
+    }
+    if (!d.sq && (rc = dalloc(ctx, &d.sq, (size_t)8))) return rc;
+    if (l < level && !d.xs && (rc = dalloc(ctx, &d.xs, d.vs))) return rc;
+  }
+  // device-order working copies: the top level iterates on its c (x) and b
+  // when it is permuted, a coarse level on xs and its own b (the cycle's)
+  std::vector<double *> X(level + 1, nullptr), B(level + 1, nullptr);
+  for (int l = bl; l <= level; ++l) {
+    DevLevel &d = ctx->dev[l];
+    if (l == level && !d.perm) {
+      X[l] = x[l];
+      B[l] = b[l];
+      continue;
+    }
+    X[l] = l == level ? d.c : d.xs;
+    B[l] = d.b;
+    to_dev<double, double>(ctx, l, x[l], d.vs, X[l]);
+    to_dev<double, double>(ctx, l, b[l], d.vs, B[l]);
+  }
+  auto move_out = [&]() -> int {
+    for (int l = bl; l <= level; ++l) {
+      if (X[l] == x[l]) continue;
+      to_user<double, double>(ctx, l, X[l], x[l], ctx->dev[l].vs);
+      to_user<double, double>(ctx, l, B[l], b[l], ctx->dev[l].vs);
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+  };
+  auto cls = [&](int l) { return l < level ? ctx->dev[l].fgd : ctx->dev[l].vclass; };
+  const int nc = ctx->dev[level].nb;
+  if (do_defect)   // LinearDefect ON_SURFACE: b -= A x on the surface rows
+    for (int l = frl; l <= level; ++l) {
+      DevLevel &d = ctx->dev[l];
+      launch_convert<double, double>(st, d.n * d.nb, B[l], d.t, 1, d.vs, d.vs);
+      launch_spmv<double, double, double>(st, d.A.d64(), 0, d.A.nslices, X[l], d.t, nullptr, 0, d.nt, B1(d));
+      launch_copy_rows(st, d.n, d.nb, cls(l), d.t, B[l]);
+    }
+  // dnrm2x ON_SURFACE: per-level squared sums, added in level order
+  auto norm = [&](double *out) -> int {
+    for (int l = frl; l <= level; ++l) {
+      DevLevel &d = ctx->dev[l];
+      launch_dot<double>(st, d.n, B[l], nullptr, cls(l), ctx->part, d.sq, 0, B1(d));
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    double acc[UGMG_MAXB] = {0.0, 0.0, 0.0};
+    for (int l = frl; l <= level; ++l) {
+      double q[UGMG_MAXB];
+      HIPCHK(hipMemcpy(q, ctx->dev[l].sq, sizeof(double) * nc, hipMemcpyDeviceToHost));
+      for (int i = 0; i < nc; ++i) acc[i] += q[i];
+    }
+    for (int i = 0; i < nc; ++i) out[i] = std::sqrt(acc[i]);
+    return 0;
+  };
+  auto sc_cmp = [nc](const double *u, const double *v) {
+    for (int i = 0; i < nc; ++i)
+      if (!sc_cmp1(u[i], v[i])) return false;
+    return true;
+  };
+  double first[UGMG_MAXB] = {0.0, 0.0, 0.0}, target[UGMG_MAXB], last[UGMG_MAXB];
+  if ((rc = norm(first))) return rc;
+  for (int i = 0; i < nc; ++i) {
+    res->first_defect[i] = res->last_defect[i] = first[i];
+    target[i] = first[i] * reduction[i];
+    if (target[i] == 0.0) target[i] = reduction[i];
+  }
+  if (cb) cb(user, 0, nc, first);
+  int conv = sc_cmp(first, abslimit);
+  int i;
+  for (i = 0; i < maxiter; ++i) {
+    if (conv) break;
+    // c accumulates straight into the top level's x; the coarse levels'
+    // corrections stay in their c and are added after the cycle (LSUpdate)
+    lmgc<double>(ctx, level, X[level], B[level], 1);
+    for (int l = bl; l < level; ++l)
+      launch_add<double>(st, ctx->dev[l].n, X[l], ctx->dev[l].c, B1(ctx->dev[l]));
+    if ((rc = norm(last))) {
+      res->error_code = rc;
+      move_out();
+      return rc;
+    }
+    for (int k = 0; k < nc; ++k) res->last_defect[k] = last[k];
+    if (cb) cb(user, i + 1, nc, last);
+    if (sc_cmp(last, abslimit) || sc_cmp(last, target)) {
+      conv = 1;
+      res->number_of_linear_iterations = i + 1;
+      break;
+    }
+  }
+  if (!conv) res->number_of_linear_iterations = i;
+  res->converged = conv;
+  return move_out();
+}
+
+int ugmg_solve_surface_host(ugmg_ctx *ctx, int level, double *const *x, double *const *b,
+                            const double *abslimit, const double *reduction, int maxiter,
+                            int do_defect, ugmg_lresult *res, ugmg_defect_cb cb, void *user) {
+  int rc = check_ready(ctx, level);
+  if (rc) return rc;
+  if (!x || !b) return fail(ctx, UGMG_E_ARG, "null argument");
+  const int bl = ctx->bl;
+  std::vector<double *> dx(level + 1, nullptr), db(level + 1, nullptr);
+  auto release = [&]() {
+    for (double *p : dx) hipFree(p);
+    for (double *p : db) hipFree(p);
+  };
+  rc = UGMG_OK;
+  for (int l = bl; l <= level && !rc; ++l) {
+    if (!x[l] || !b[l]) {
+      rc = fail(ctx, UGMG_E_ARG, "x[l] and b[l] are needed on baselevel..level");
+      break;
+    }
+    const size_t bytes = sizeof(double) * ctx->dev[l].n * ctx->b;
+    if (hipMalloc(&dx[l], bytes) != hipSuccess || hipMalloc(&db[l], bytes) != hipSuccess)
+      rc = fail(ctx, UGMG_E_NOMEM, "hipMalloc");
+    else if (hipMemcpy(dx[l], x[l], bytes, hipMemcpyHostToDevice) != hipSuccess ||
+             hipMemcpy(db[l], b[l], bytes, hipMemcpyHostToDevice) != hipSuccess)
+      rc = fail(ctx, UGMG_E_HIP, "copy in");
+  }
+  if (!rc)
+    rc = ugmg_solve_surface(ctx, level, dx.data(), db.data(), abslimit, reduction, maxiter, do_defect,
+                            res, cb, user);
+  for (int l = bl; l <= level && !rc; ++l) {
+    const size_t bytes = sizeof(double) * ctx->dev[l].n * ctx->b;
+    if (hipMemcpy(x[l], dx[l], bytes, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(b[l], db[l], bytes, hipMemcpyDeviceToHost) != hipSuccess)
+      rc = fail(ctx, UGMG_E_HIP, "copy out");
+  }
+  release();
+  return rc;
 }
 
 int ugmg_solve_host(ugmg_ctx *ctx, int level, double *x, double *b, const double *abslimit,

@@ -176,6 +176,11 @@ template <typename TD, typename TS>
 void launch_convert(hipStream_t st, int n, const TS *s, TD *d, int nrhs, size_t vss, size_t vsd);
 template <typename T>
 void launch_add(hipStream_t st, int n, T *x, const T *y, Batch bt);
+// out[i] = 3 for FINE_GRID_DOF vectors (VCLASS >= 2, VNCLASS <= 1), else 0
+void launch_fgd_class(hipStream_t st, int n, const uint8_t *vc, const uint8_t *vnc, uint8_t *out);
+// dst = src on the n vectors (b components) whose class byte is >= 2
+void launch_copy_rows(hipStream_t st, int n, int b, const uint8_t *cls, const double *src,
+                      double *dst);
 // list order <-> device order of a level: gather d[p] = s[perm[p]],
 // scatter d[perm[p]] = s[p]; column strides vss (source), vsd (destination)
 template <typename TD, typename TS>

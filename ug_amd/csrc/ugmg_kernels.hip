@@ -519,6 +519,24 @@ __global__ __launch_bounds__(kBlock) void k_add(int n, T *__restrict__ x, const 
   if (i < n) x[o + i] += y[o + i];
 }
 
+// class byte of the FINE_GRID_DOF flag (gm/algebra.cc:2908-2909): 3 where
+// VCLASS >= 2 and VNCLASS <= 1, else 0, for the class-filtered reductions
+__global__ __launch_bounds__(kBlock) void k_fgd_class(int n, const uint8_t *__restrict__ vc,
+                                                       const uint8_t *__restrict__ vnc,
+                                                       uint8_t *__restrict__ out) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) out[i] = (vc[i] >= 2 && vnc[i] <= 1) ? 3 : 0;
+}
+
+// dst = src on the rows whose class byte is >= 2 (scalar entry e, row e / b)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_copy_rows(int ne, int b, const uint8_t *__restrict__ cls,
+                                                       const T *__restrict__ src,
+                                                       T *__restrict__ dst) {
+  const int e = blockIdx.x * kBlock + threadIdx.x;
+  if (e < ne && cls[e / b] >= 2) dst[e] = src[e];
+}
+
 // ======================================================== 3x3 blocks
 // The same operations on vectors of B = 3 components with b x b matrix
 // blocks (linear elasticity); diag holds the diagonal block of every row.
@@ -1425,6 +1443,14 @@ void launch_add(hipStream_t st, int n, T *x, const T *y, Batch bt) {
   if (n > 0) k_add<T><<<grid2(nblk(n), bt.nrhs), kBlock, 0, st>>>(n, x, y, bt.vs);
 }
 template void launch_add<double>(hipStream_t, int, double *, const double *, Batch);
+
+void launch_fgd_class(hipStream_t st, int n, const uint8_t *vc, const uint8_t *vnc, uint8_t *out) {
+  if (n > 0) k_fgd_class<<<nblk(n), kBlock, 0, st>>>(n, vc, vnc, out);
+}
+void launch_copy_rows(hipStream_t st, int n, int b, const uint8_t *cls, const double *src,
+                      double *dst) {
+  if (n > 0) k_copy_rows<double><<<nblk(n * b), kBlock, 0, st>>>(n * b, b, cls, src, dst);
+}
 template void launch_add<float>(hipStream_t, int, float *, const float *, Batch);
 
 template <typename TD, typename TS, int SCATTER>

@@ -105,6 +105,9 @@ def load():
         "ugmg_iter_host": ([vp, ip, dp, dp], ip),
         "ugmg_solve": ([vp, ip, dp, dp, dp, dp, ip, ip, C.POINTER(LResult), DEFECT_CB, vp], ip),
         "ugmg_solve_host": ([vp, ip, dp, dp, dp, dp, ip, ip, C.POINTER(LResult), DEFECT_CB, vp], ip),
+        "ugmg_fullrefinelevel": ([vp], ip),
+        "ugmg_solve_surface": ([vp, ip, vp, vp, dp, dp, ip, ip, C.POINTER(LResult), DEFECT_CB, vp], ip),
+        "ugmg_solve_surface_host": ([vp, ip, vp, vp, dp, dp, ip, ip, C.POINTER(LResult), DEFECT_CB, vp], ip),
         "ugmg_time_kernel": ([vp, ip, ip, ip, C.POINTER(C.c_double), C.POINTER(C.c_double)], ip),
         "ugmg_set_batch": ([vp, ip], ip),
         "ugmg_galerkin": ([vp, ip], ip),
@@ -339,6 +342,42 @@ class Context:
         self._chk(self.lib.ugmg_solve_host(self.h, level, x.ctypes.data, b.ctypes.data, C.addressof(ab),
                                            C.addressof(rd), maxiter, int(do_defect), C.byref(res), cb, None))
         return x, b, res, np.array(defects)
+
+    def fullrefinelevel(self):
+        r = self.lib.ugmg_fullrefinelevel(self.h)
+        if r < 0:
+            self._chk(r)
+        return r
+
+    def solve_surface_host(self, level, rhs, x0=None, red=1e-10, abslimit=1e-300, maxiter=50,
+                           do_defect=True):
+        """ugmg_solve_surface_host: rhs[l] (and x0[l]) for levels baselevel..level
+        (None below); returns (x list, b list, LResult, defects array)."""
+        nb = self.nb
+        bl = max(0, min(self.params.baselevel, self.top))
+        xs, bs = [], []
+        for l in range(level + 1):
+            if l < bl:
+                xs.append(None)
+                bs.append(None)
+                continue
+            n = self.problem.levels[l].n * nb
+            xs.append(np.zeros(n) if x0 is None or x0[l] is None else np.array(x0[l], dtype=np.float64, copy=True))
+            bs.append(np.array(rhs[l], dtype=np.float64, copy=True))
+        XP = (C.c_void_p * (level + 1))(*[None if v is None else v.ctypes.data for v in xs])
+        BP = (C.c_void_p * (level + 1))(*[None if v is None else v.ctypes.data for v in bs])
+        res = LResult()
+        defects = []
+
+        def _cb(user, it, ncomp, d):
+            defects.append([d[i] for i in range(ncomp)])
+
+        cb = DEFECT_CB(_cb)
+        ab = (C.c_double * 3)(abslimit, abslimit, abslimit)
+        rd = (C.c_double * 3)(red, red, red)
+        self._chk(self.lib.ugmg_solve_surface_host(self.h, level, XP, BP, C.addressof(ab), C.addressof(rd),
+                                                   maxiter, int(do_defect), C.byref(res), cb, None))
+        return xs, bs, res, np.array(defects)
 
     def assemble_level(self, level, n, elem, emat=None, kref=None, scale=None, evec=None, skip=None,
                        dirichlet=None, vclass=None, vnclass=None, rhs=None):

@@ -171,6 +171,10 @@ int ugmg_solve_host(ugmg_ctx *ctx, int level, double *x, double *b,
  * the lowest preprocessed level with a FINE_GRID_DOF vector (VCLASS >= 2,
  * VNCLASS <= 1), the top level when there is none (uniform refinement). */
 int ugmg_fullrefinelevel(const ugmg_ctx *ctx);
+/* the base level the last ugmg_preprocess settled on (lmgc's $b resolved:
+ * min(baselevel, level), or the deepest level with at most -baselevel
+ * vectors), or < 0 before it */
+int ugmg_base_level(const ugmg_ctx *ctx);
 /* LinearSolver (ls.cc:637-749) when the surface spans several levels:
  * x[l], b[l] for every level l in baselevel..level (n_l * b doubles, list
  * order; entries below baselevel are not read).  LinearDefect / Residuum

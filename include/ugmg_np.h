@@ -58,6 +58,12 @@ int ugmg_np_set_level(ugmg_np_env *env, int level);
  * per column, `ncols` columns at stride n*b */
 int ugmg_np_set_vector(ugmg_np_env *env, const char *name, double *data, int ncols);
 
+/* the named vector's array (n_l*b doubles) on a coarser level l: on a
+ * locally refined hierarchy (ugmg_fullrefinelevel below the current level)
+ * ls works ON_SURFACE on x and b of every level from its base level up
+ * (ugmg_solve_surface), as UG's LinearSolver does on the levels' VECTOR
+ * lists; the finest level's array is the one ugmg_np_set_vector registers */
+int ugmg_np_set_level_vector(ugmg_np_env *env, const char *name, int level, double *data);
 /* npcreate <name> $c <class>: 0 or a negative code */
 int ugmg_np_create(ugmg_np_env *env, const char *name, const char *cls);
 /* npinit <name> <options>: returns the numproc state (UGMG_NP_*) or < 0 */

@@ -122,3 +122,51 @@ def test_ls_display_full_is_ugs_text(poisson2d):
     assert " 1    u:" not in red and " avg:  u:" in red and "LS  : L=" in red
     env.close()
     ctx.close()
+
+
+def test_ls_script_on_a_locally_refined_hierarchy():
+    """The same ls script on a red-green refined hierarchy: npexecute runs
+    LinearSolver ON_SURFACE over the registered levels and gives the
+    oracle's surface solve; bcgs refuses such a hierarchy."""
+    from ug_amd import numproc as NP
+    from ug_amd import problems, ugmg
+    Q = problems.p1_local(5, 2)
+    bl = 1
+    fs = problems.surface_rhs(Q, bl, 4)
+    ctx = ugmg.Context(0)
+    ctx.set_hierarchy(Q)
+    env = NP.NumProcEnv(ctx)
+    x = env.vector("sol", np.zeros(Q.levels[-1].n))
+    b = env.vector("rhs", fs[-1])
+    xs = {l: env.level_vector("sol", l, np.zeros(Q.levels[l].n)) for l in range(bl, Q.top)}
+    bs = {l: env.level_vector("rhs", l, fs[l]) for l in range(bl, Q.top)}
+    env.create("transfer", "transfer"); env.init("transfer")
+    env.create("jac", "jac"); env.init("jac", "$damp 0.8")
+    env.create("ex", "ex"); env.init("ex")
+    env.create("base", "ls"); env.init("base", "$red 1e-12 $m 1 $I ex $abslimit 1e-300")
+    env.create("mgc", "lmgc"); env.init("mgc", f"$S jac jac base $T transfer $n1 2 $n2 2 $b {bl}")
+    env.create("mgs", "ls")
+    assert env.init("mgs", "$A MAT $x sol $b rhs $m 50 $red 1e-10 $abslimit 1e-300 $I mgc $display red") \
+        == NP.EXECUTABLE
+    print("npexecute mgs (locally refined)", flush=True)
+    env.execute("mgs", "$i $d $r $s")
+    r = env.result("mgs")
+    text = env.output()
+    want = O.ls_solve_surface(O.Hierarchy(Q.levels), Q.top, O.params(smoother=O.JAC, damp=0.8, baselevel=bl),
+                              fs, red=1e-10, maxiter=50)
+    assert r.converged and r.number_of_linear_iterations == want["nit"]
+    assert abs(r.last_defect[0] - want["defects"][-1, 0]) < 1e-10 * want["defects"][-1, 0]
+    assert np.max(np.abs(x - want["x"][-1])) < 1e-10 * np.max(np.abs(want["x"][-1]))
+    for l in range(bl, Q.top):
+        assert np.max(np.abs(xs[l] - want["x"][l])) < 1e-10 * np.max(np.abs(want["x"][l]))
+        # b is the defect on the surface rows; the covered rows are the cycle's scratch
+        m = problems.fine_grid_dof(Q.levels[l])
+        assert np.max(np.abs(bs[l][m] - want["b"][l][m]), initial=0.0) < 1e-9
+    assert "LS  : L=" in text
+    env.create("k", "bcgs")
+    assert env.init("k", "$A MAT $x sol $b rhs $m 10 $red 1e-8 $I mgc") == NP.EXECUTABLE
+    with pytest.raises(Exception):
+        env.execute("k", "$s")
+    env.execute("mgs", "$p")
+    env.close()
+    ctx.close()

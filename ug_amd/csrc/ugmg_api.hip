@@ -1490,6 +1490,11 @@ int ugmg_solve(ugmg_ctx *ctx, int level, double *x, double *b, const double *abs
   return move_out();
 }
 
+int ugmg_base_level(const ugmg_ctx *ctx) {
+  if (!ctx) return UGMG_E_ARG;
+  return ctx->prepared ? ctx->bl : UGMG_E_STATE;
+}
+
 int ugmg_fullrefinelevel(const ugmg_ctx *ctx) {
   if (!ctx) return UGMG_E_ARG;
   return ctx->prepared ? ctx->frl : UGMG_E_STATE;

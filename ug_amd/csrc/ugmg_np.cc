@@ -19,6 +19,7 @@
 #include <new>
 #include <string>
 #include <vector>
+#include <vector>
 
 namespace {
 
@@ -122,6 +123,8 @@ struct ugmg_np_env {
   int level = -1;
   std::map<std::string, NumProc> np;
   std::map<std::string, std::pair<double *, int>> vec;
+  // the same vectors on the coarser levels (locally refined hierarchies)
+  std::map<std::string, std::map<int, double *>> lvec;
   std::string err;
   std::string out, taken;   // PCR display text: pending, and the last one handed out
 };
@@ -426,6 +429,12 @@ int ugmg_np_set_vector(ugmg_np_env *env, const char *name, double *data, int nco
   return UGMG_OK;
 }
 
+int ugmg_np_set_level_vector(ugmg_np_env *env, const char *name, int level, double *data) {
+  if (!env || !name || !data || level < 0) return UGMG_E_ARG;
+  env->lvec[name][level] = data;
+  return UGMG_OK;
+}
+
 int ugmg_np_create(ugmg_np_env *env, const char *name, const char *cls) {
   if (!env || !name || !cls) return UGMG_E_ARG;
   const std::string c = cls;
@@ -500,15 +509,39 @@ int ugmg_np_execute(ugmg_np_env *env, const char *name, const char *flags) {
   }
   const int ncols = o->cls == "bcgs" ? o->nrhs : 1;
   o->res.assign(ncols, ugmg_lresult{});
+  // a locally refined hierarchy (FULLREFINELEVEL below the level): LinearSolver
+  // runs ON_SURFACE over x and b of every level from the base level up
+  const int frl = ugmg_fullrefinelevel(env->ctx);
+  const bool surface = frl >= 0 && frl < level;
+  std::vector<double *> XL(level + 1, nullptr), BL(level + 1, nullptr);
+  if (surface) {
+    if (o->cls != "ls") return fail(env, UGMG_E_ARG, "bcgs runs on uniformly refined hierarchies only");
+    const int bl = ugmg_base_level(env->ctx);
+    if (bl < 0) return fail(env, UGMG_E_STATE, "the hierarchy is not preprocessed ($i)");
+    for (int l = bl; l < level; ++l) {
+      auto fx = env->lvec[o->x].find(l), fb = env->lvec[o->b].find(l);
+      if (fx == env->lvec[o->x].end() || fb == env->lvec[o->b].end())
+        return fail(env, UGMG_E_STATE, "locally refined hierarchy: $x and $b need their coarser levels "
+                                       "(ugmg_np_set_level_vector) from the base level up");
+      XL[l] = fx->second;
+      BL[l] = fb->second;
+    }
+    XL[level] = x;
+    BL[level] = b;
+  }
+  auto ls_solve = [&](int maxit, ugmg_lresult *r, ugmg_defect_cb cb, void *user) {
+    const int dd = o->pending_defect ? 1 : 0;
+    return surface ? ugmg_solve_surface_host(env->ctx, level, XL.data(), BL.data(), o->abslimit, o->red,
+                                             maxit, dd, r, cb, user)
+                   : ugmg_solve_host(env->ctx, level, x, b, o->abslimit, o->red, maxit, dd, r, cb, user);
+  };
   if (read_option(av, "d")) o->pending_defect = true;   // formed by the next Residuum/Solver
   const bool want_r = read_option(av, "r") != 0, want_s = read_option(av, "s") != 0;
   if (want_r && !want_s) {
     // LinearResiduum alone: LinearDefect (if pending) and the surface norm,
     // through a zero-iteration solve
     ugmg_lresult r;
-    if ((rc = ugmg_solve_host(env->ctx, level, x, b, o->abslimit, o->red, 0, o->pending_defect ? 1 : 0, &r,
-                              nullptr, nullptr)))
-      return fail(env, rc, ugmg_last_error(env->ctx));
+    if ((rc = ls_solve(0, &r, nullptr, nullptr))) return fail(env, rc, ugmg_last_error(env->ctx));
     o->pending_defect = false;
     o->res[0] = r;
   }
@@ -520,8 +553,7 @@ int ugmg_np_execute(ugmg_np_env *env, const char *name, const char *flags) {
       // first and every iteration's defect, then PCR_AVERAGE and "LS  :"
       Pcr pcr(env->out, o->display, nb);
       pcr.prepare(name);
-      if ((rc = ugmg_solve_host(env->ctx, level, x, b, o->abslimit, o->red, o->m, o->pending_defect ? 1 : 0,
-                                &o->res[0], o->display ? pcr_defect_cb : nullptr, &pcr)))
+      if ((rc = ls_solve(o->m, &o->res[0], o->display ? pcr_defect_cb : nullptr, &pcr)))
         return fail(env, rc, ugmg_last_error(env->ctx));
       pcr.average(o->res[0].last_defect, false);
       pcr.times("LS  ", level, o->res[0].number_of_linear_iterations, seconds_since(t0));

@@ -30,6 +30,7 @@ def _lib():
             "ugmg_np_last_error": ([vp], cp),
             "ugmg_np_set_level": ([vp, ip], ip),
             "ugmg_np_set_vector": ([vp, cp, vp, ip], ip),
+            "ugmg_np_set_level_vector": ([vp, cp, ip, vp], ip),
             "ugmg_np_create": ([vp, cp, cp], ip),
             "ugmg_np_init": ([vp, cp, cp], ip),
             "ugmg_np_execute": ([vp, cp, cp], ip),
@@ -87,6 +88,14 @@ class NumProcEnv:
         ncols = a.shape[0] if a.ndim == 2 else 1
         self._vecs[name] = a
         self._chk(self.lib.ugmg_np_set_vector(self.h, name.encode(), a.ctypes.data, ncols))
+        return a
+
+    def level_vector(self, name, level, array):
+        """Register a copy of `array` as the named vector's coarser level
+        `level` (locally refined hierarchies); returns the copy."""
+        a = np.array(array, dtype=np.float64, copy=True, order="C")
+        self._vecs[(name, level)] = a
+        self._chk(self.lib.ugmg_np_set_level_vector(self.h, name.encode(), level, a.ctypes.data))
         return a
 
     def create(self, name, cls):

@@ -60,7 +60,7 @@ struct DevLevel {
   int n = 0, nnzb = 0;
   DevSellBuf A, L, U;
   int ncolors = 0;
-  std::vector<int> color_slices, color, color_wmax;
+  std::vector<int> color_slices, color, color_wmax, color_wmax_u;
   bool all_active = false;   // every vector has VCLASS >= 3
   double *diag64 = nullptr;
   float *diag32 = nullptr;
@@ -300,8 +300,9 @@ void smooth(ugmg_ctx *ctx, int l, T *t, T *b, T *c, int cmode, int nsteps) {
   for (int i = 1; i < nsteps; ++i)
     for (int k = 0; k < d.ncolors; ++k)
       launch_gs_color_fused<T, T>(st, d.L.d<T>(), d.U.d<T>(), d.color_slices[k],
-                                  d.color_slices[k + 1] - d.color_slices[k], b, d.diag<T>(),
-                                  damp, t, c, i == 1 ? cmode : 1, d.nt, bt);
+                                  d.color_slices[k + 1] - d.color_slices[k], d.color_wmax[k],
+                                  d.color_wmax_u[k], b, d.diag<T>(), damp, t, c,
+                                  i == 1 ? cmode : 1, d.nt, bt);
   launch_gs_defect<T, T>(st, d.U.d<T>(), t, b, d.vclass, damp, c, 1, nullptr, d.nt, bt);
 }
 
@@ -782,8 +783,11 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
         D.all_active = true;
         for (uint8_t v : H.vc) D.all_active &= v >= 3;
         D.color_wmax.resize(D.ncolors);
-        for (int k = 0; k < D.ncolors; ++k)
+        D.color_wmax_u.resize(D.ncolors);
+        for (int k = 0; k < D.ncolors; ++k) {
           D.color_wmax[k] = D.L.max_width(D.color_slices[k], D.color_slices[k + 1]);
+          D.color_wmax_u[k] = D.U.max_width(D.color_slices[k], D.color_slices[k + 1]);
+        }
       }
       if (l > bl) {
         D.p_nnz = H.pp[H.n];

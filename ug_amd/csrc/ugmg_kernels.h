@@ -113,8 +113,8 @@ void launch_gs_color(hipStream_t st, const DSell<TV> &L, int slice0, int nsl, in
 // active); cmode is the correction mode of the previous step
 template <typename TV, typename T>
 void launch_gs_color_fused(hipStream_t st, const DSell<TV> &L, const DSell<TV> &U, int slice0,
-                           int nsl, T *b, const T *diag, T damp, T *v, T *c, int cmode, bool nt,
-                           Batch bt);
+                           int nsl, int wmax_l, int wmax_u, T *b, const T *diag, T damp, T *v,
+                           T *c, int cmode, bool nt, Batch bt);
 template <typename TV, typename T>
 void launch_gs_defect(hipStream_t st, const DSell<TV> &U, const T *v, T *b, const uint8_t *vclass,
                       T damp, T *c, int cmode, T *xout, bool nt, Batch bt);

@@ -108,6 +108,10 @@ struct ugmg_ctx {
   double dev_bytes = 0.0;
   HostBase base;
   double *base_inv = nullptr;
+  float *base_inv32 = nullptr;   // FP32 copy for the FP32 cycle
+  template <typename T> const T *base_inv_t() const {
+    if constexpr (std::is_same_v<T, double>) return base_inv; else return base_inv32;
+  }
   double *part = nullptr;
   int part_cap = 0;
   double *red = nullptr;      // device scalars
@@ -222,6 +226,7 @@ void free_device(ugmg_ctx *ctx) {
   ctx->dev_bytes = 0.0;
   ctx->dev.clear();
   ctx->base_inv = nullptr;
+  ctx->base_inv32 = nullptr;
   ctx->clev64 = nullptr;
   ctx->clev32 = nullptr;
   ctx->fuse_top = -1;
@@ -357,7 +362,7 @@ void base_solve(ugmg_ctx *ctx, T *c, T *b, int cmode, bool have_parts, bool need
   if (m == 0 && cmode == 0) launch_fill<T>(st, d.n, c, (T)0, bt);
   for (int it = 0; it < m; ++it) {
     if (!ctx->base.failed)
-      launch_base_gemv<T>(st, d.n * d.nb, ctx->base_inv, b, c, x, it == 0 ? cmode : 1, p.base_damp[0],
+      launch_base_gemv<T, T>(st, d.n * d.nb, ctx->base_inv_t<T>(), b, c, x, it == 0 ? cmode : 1, p.base_damp[0],
                           p.base_abslimit[0], p.base_red[0], ctx->base_part, np,
                           ctx->base_first, it, bt);
     const bool more = it + 1 < m;
@@ -814,6 +819,16 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
       std::vector<double>().swap(ctx->base.inv);
     } else if ((rc = factor_base_device(ctx, ctx->base))) {
       return rc;
+    }
+    {   // FP32 copy of the inverse, read by the FP32 cycle's base solve (kept
+        // whatever the precision, so that ugmg_set_params may switch it)
+      const int nsc = ctx->dev[bl].n * ctx->dev[bl].nb;
+      if (nsc > 0 && ctx->base_inv) {
+        const size_t ninv = (size_t)nsc * nsc;
+        if ((rc = dalloc(ctx, &ctx->base_inv32, ninv))) return rc;
+        launch_convert<float, double>(ctx->stream, (int)ninv, ctx->base_inv, ctx->base_inv32, 1, ninv, ninv);
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+      }
     }
     // fused coarse levels (opt-in): the base level and every level above it
     // up to fuse_rows vectors (and at most kMaxColors colours)

@@ -151,8 +151,10 @@ int assemble_device(hipStream_t st, const AsmInput &in, double *rhs, std::vector
 // on a zero pivot.
 void launch_base_factor(hipStream_t st, double *band, int n, int bw, const int *pos, double *X,
                         double *inv, int *failed, bool fmode = false);
-template <typename T>
-void launch_base_gemv(hipStream_t st, int n, const double *inv, const T *b, T *c, T *xout,
+// inv: the dense inverse in FP64 (TI = double) or its FP32 copy for the
+// FP32 cycle (TI = float); products are summed in FP64 either way
+template <typename T, typename TI>
+void launch_base_gemv(hipStream_t st, int n, const TI *inv, const T *b, T *c, T *xout,
                       int cmode, double damp, double abslimit, double red, const double *part,
                       int nparts, double *first, int it, Batch bt);
 // columns of the partial sums: one per component of every right-hand side

@@ -798,25 +798,25 @@ __global__ __launch_bounds__(kBlock) void k_dot_part_b3(int n, const T *__restri
 
 // lane's share of row . b for the base GEMV: four independent accumulators
 // keep four loads of the row in flight per lane (the wave sums the shares)
-template <typename T>
-__device__ __forceinline__ double base_row_share(const double *__restrict__ rr,
+template <typename T, typename TI>
+__device__ __forceinline__ double base_row_share(const TI *__restrict__ rr,
                                                  const T *__restrict__ bb, int n, int lane) {
   double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
   int j = lane;
   for (; j + 192 < n; j += 256) {
-    a0 += rr[j] * (double)bb[j];
-    a1 += rr[j + 64] * (double)bb[j + 64];
-    a2 += rr[j + 128] * (double)bb[j + 128];
-    a3 += rr[j + 192] * (double)bb[j + 192];
+    a0 += (double)rr[j] * (double)bb[j];
+    a1 += (double)rr[j + 64] * (double)bb[j + 64];
+    a2 += (double)rr[j + 128] * (double)bb[j + 128];
+    a3 += (double)rr[j + 192] * (double)bb[j + 192];
   }
-  for (; j < n; j += 64) a0 += rr[j] * (double)bb[j];
+  for (; j < n; j += 64) a0 += (double)rr[j] * (double)bb[j];
   return (a0 + a1) + (a2 + a3);
 }
 
 // ls over ex on the base level with per-component stop tests (sc_cmp:
 // every component below its limit); inv is (3n) x (3n)
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_base_gemv_b3(int n3, const double *__restrict__ inv,
+template <typename T, typename TI>
+__global__ __launch_bounds__(kBlock) void k_base_gemv_b3(int n3, const TI *__restrict__ inv,
                                                           const T *__restrict__ b,
                                                           T *__restrict__ c, T *__restrict__ xout,
                                                           int cmode, double damp, double abslimit,
@@ -829,7 +829,7 @@ __global__ __launch_bounds__(kBlock) void k_base_gemv_b3(int n3, const double *_
   const int cj = blockIdx.y;
   const size_t o = (size_t)cj * vs;
   // the row product first, while the norms below are reduced
-  double x = row < n3 ? base_row_share<T>(inv + (size_t)row * n3, b + o, n3, lane) : 0.0;
+  double x = row < n3 ? base_row_share<T, TI>(inv + (size_t)row * n3, b + o, n3, lane) : 0.0;
   double cur[B];
 #pragma unroll
   for (int q = 0; q < B; ++q) {
@@ -1338,8 +1338,8 @@ template int launch_restrict<float>(hipStream_t, const DSell<float> &, const uin
 //   cur = |b| from the block partials of the defect norm;
 //   converged: cur < abslimit (it == 0), cur < abslimit or cur < red*first;
 //   x = converged ? 0 : damp * inv b;  c (+)= x;  xout = x.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_base_gemv(int n, const double *__restrict__ inv,
+template <typename T, typename TI>
+__global__ __launch_bounds__(kBlock) void k_base_gemv(int n, const TI *__restrict__ inv,
                                                        const T *__restrict__ b,
                                                        T *__restrict__ c, T *__restrict__ xout,
                                                        int cmode, double damp, double abslimit,
@@ -1353,7 +1353,7 @@ __global__ __launch_bounds__(kBlock) void k_base_gemv(int n, const double *__res
   part += (size_t)cj * nparts;
   // the row product is formed while the stop test's norm is reduced (and
   // dropped if the test says converged)
-  double x = row < n ? base_row_share<T>(inv + (size_t)row * n, b + o, n, lane) : 0.0;
+  double x = row < n ? base_row_share<T, TI>(inv + (size_t)row * n, b + o, n, lane) : 0.0;
   double s = 0.0;
   for (int i = lane; i < nparts; i += 64) s += part[i];
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
@@ -1381,25 +1381,26 @@ __global__ __launch_bounds__(kBlock) void k_base_gemv(int n, const double *__res
   }
 }
 
-template <typename T>
-void launch_base_gemv(hipStream_t st, int n, const double *inv, const T *b, T *c, T *xout,
+template <typename T, typename TI>
+void launch_base_gemv(hipStream_t st, int n, const TI *inv, const T *b, T *c, T *xout,
                       int cmode, double damp, double abslimit, double red, const double *part,
                       int nparts, double *first, int it, Batch bt) {
   const int g = (n * 64 + kBlock - 1) / kBlock;
   if (bt.b == 3)   // n counts scalar unknowns
-    k_base_gemv_b3<T><<<grid2(g, bt.nrhs), kBlock, 0, st>>>(n, inv, b, c, xout, cmode, damp,
-                                                            abslimit, red, part, nparts, first, it,
-                                                            bt.vs);
+    k_base_gemv_b3<T, TI><<<grid2(g, bt.nrhs), kBlock, 0, st>>>(n, inv, b, c, xout, cmode, damp,
+                                                                abslimit, red, part, nparts, first,
+                                                                it, bt.vs);
   else
-    k_base_gemv<T><<<grid2(g, bt.nrhs), kBlock, 0, st>>>(n, inv, b, c, xout, cmode, damp, abslimit,
-                                                         red, part, nparts, first, it, bt.vs);
+    k_base_gemv<T, TI><<<grid2(g, bt.nrhs), kBlock, 0, st>>>(n, inv, b, c, xout, cmode, damp,
+                                                             abslimit, red, part, nparts, first,
+                                                             it, bt.vs);
 }
-template void launch_base_gemv<double>(hipStream_t, int, const double *, const double *,
-                                       double *, double *, int, double, double, double,
-                                       const double *, int, double *, int, Batch);
-template void launch_base_gemv<float>(hipStream_t, int, const double *, const float *, float *,
-                                      float *, int, double, double, double, const double *, int,
-                                      double *, int, Batch);
+template void launch_base_gemv<double, double>(hipStream_t, int, const double *, const double *,
+                                               double *, double *, int, double, double, double,
+                                               const double *, int, double *, int, Batch);
+template void launch_base_gemv<float, float>(hipStream_t, int, const float *, const float *, float *,
+                                             float *, int, double, double, double, const double *,
+                                             int, double *, int, Batch);
 
 template <typename T>
 int launch_dot_part(hipStream_t st, int n, const T *x, const uint8_t *vclass, double *part,

@@ -108,9 +108,13 @@ struct ugmg_ctx {
   double dev_bytes = 0.0;
   HostBase base;
   double *base_inv = nullptr;
-  float *base_inv32 = nullptr;   // FP32 copy for the FP32 cycle
-  template <typename T> const T *base_inv_t() const {
-    if constexpr (std::is_same_v<T, double>) return base_inv; else return base_inv32;
+  // the inverse's rows that are not unit rows, FP64 and FP32 (the FP32
+  // cycle's base solve), and each row's index among them (-1: unit row)
+  double *base_cinv = nullptr;
+  float *base_cinv32 = nullptr;
+  int *base_rowmap = nullptr;
+  template <typename T> const T *base_cinv_t() const {
+    if constexpr (std::is_same_v<T, double>) return base_cinv; else return base_cinv32;
   }
   double *part = nullptr;
   int part_cap = 0;
@@ -226,7 +230,9 @@ void free_device(ugmg_ctx *ctx) {
   ctx->dev_bytes = 0.0;
   ctx->dev.clear();
   ctx->base_inv = nullptr;
-  ctx->base_inv32 = nullptr;
+  ctx->base_cinv = nullptr;
+  ctx->base_cinv32 = nullptr;
+  ctx->base_rowmap = nullptr;
   ctx->clev64 = nullptr;
   ctx->clev32 = nullptr;
   ctx->fuse_top = -1;
@@ -362,7 +368,8 @@ void base_solve(ugmg_ctx *ctx, T *c, T *b, int cmode, bool have_parts, bool need
   if (m == 0 && cmode == 0) launch_fill<T>(st, d.n, c, (T)0, bt);
   for (int it = 0; it < m; ++it) {
     if (!ctx->base.failed)
-      launch_base_gemv<T, T>(st, d.n * d.nb, ctx->base_inv_t<T>(), b, c, x, it == 0 ? cmode : 1, p.base_damp[0],
+      launch_base_gemv<T, T>(st, d.n * d.nb, ctx->base_cinv_t<T>(), ctx->base_rowmap, b, c, x,
+                             it == 0 ? cmode : 1, p.base_damp[0],
                           p.base_abslimit[0], p.base_red[0], ctx->base_part, np,
                           ctx->base_first, it, bt);
     const bool more = it + 1 < m;
@@ -820,13 +827,43 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
     } else if ((rc = factor_base_device(ctx, ctx->base))) {
       return rc;
     }
-    {   // FP32 copy of the inverse, read by the FP32 cycle's base solve (kept
-        // whatever the precision, so that ugmg_set_params may switch it)
-      const int nsc = ctx->dev[bl].n * ctx->dev[bl].nb;
+    {   // The GEMVs read the inverse without its unit rows.  A Dirichlet row
+        // of the base matrix (its one entry a diagonal 1) is a unit row of
+        // L, of U and so of the inverse, exactly, since the LU has no
+        // pivoting; its product with b is b's entry.  An FP32 copy serves
+        // the FP32 cycle (kept whatever the precision, so that
+        // ugmg_set_params may switch it).
+      const HostLevel &H = ctx->host[bl];
+      const int nb = ctx->dev[bl].nb, nsc = ctx->dev[bl].n * nb;
       if (nsc > 0 && ctx->base_inv) {
-        const size_t ninv = (size_t)nsc * nsc;
-        if ((rc = dalloc(ctx, &ctx->base_inv32, ninv))) return rc;
-        launch_convert<float, double>(ctx->stream, (int)ninv, ctx->base_inv, ctx->base_inv32, 1, ninv, ninv);
+        std::vector<int> rowmap(nsc, -1), act;
+        for (int v = 0; v < H.n; ++v)
+          for (int q = 0; q < nb; ++q) {
+            bool unit = H.rp[v] < H.rp[v + 1] && H.ci[H.rp[v]] == v;
+            for (int m = H.rp[v]; m < H.rp[v + 1] && unit; ++m)
+              for (int j = 0; j < nb; ++j) {
+                const double want = (H.ci[m] == v && j == q) ? 1.0 : 0.0;
+                if (H.a[((size_t)m * nb + q) * nb + j] != want) {
+                  unit = false;
+                  break;
+                }
+              }
+            if (!unit) {
+              rowmap[(size_t)v * nb + q] = (int)act.size();
+              act.push_back(v * nb + q);
+            }
+          }
+        const int nact = (int)act.size();
+        const size_t ne = std::max<size_t>(1, (size_t)nact * nsc);
+        if (act.empty()) act.push_back(0);   // (nothing to copy; keeps the upload non-empty)
+        int *act_d = nullptr;
+        if ((rc = upload(ctx, &ctx->base_rowmap, rowmap.data(), rowmap.size())) ||
+            (rc = upload(ctx, &act_d, act.data(), act.size())) ||
+            (rc = dalloc(ctx, &ctx->base_cinv, ne)) || (rc = dalloc(ctx, &ctx->base_cinv32, ne)))
+          return rc;
+        launch_base_compact(ctx->stream, nact, nsc, act_d, ctx->base_inv, ctx->base_cinv,
+                            ctx->base_cinv32);
+        HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(ctx->stream));
       }
     }

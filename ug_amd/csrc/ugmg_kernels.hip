@@ -813,10 +813,23 @@ __device__ __forceinline__ double base_row_share(const TI *__restrict__ rr,
   return (a0 + a1) + (a2 + a3);
 }
 
+// This lane's share of row `row` of the inverse times b.  The inverse holds
+// only the rows that are not unit rows (rowmap[row] = their index; -1: a unit
+// row, from a Dirichlet row of the matrix, whose product is b[row] itself).
+template <typename T, typename TI>
+__device__ __forceinline__ double base_row_or_unit(const TI *__restrict__ inv,
+                                                   const int *__restrict__ rowmap, int row,
+                                                   const T *__restrict__ bb, int n, int lane) {
+  const int k = rowmap[row];
+  if (k < 0) return lane == 0 ? (double)bb[row] : 0.0;
+  return base_row_share<T, TI>(inv + (size_t)k * n, bb, n, lane);
+}
+
 // ls over ex on the base level with per-component stop tests (sc_cmp:
 // every component below its limit); inv is (3n) x (3n)
 template <typename T, typename TI>
 __global__ __launch_bounds__(kBlock) void k_base_gemv_b3(int n3, const TI *__restrict__ inv,
+                                                          const int *__restrict__ rowmap,
                                                           const T *__restrict__ b,
                                                           T *__restrict__ c, T *__restrict__ xout,
                                                           int cmode, double damp, double abslimit,
@@ -829,7 +842,7 @@ __global__ __launch_bounds__(kBlock) void k_base_gemv_b3(int n3, const TI *__res
   const int cj = blockIdx.y;
   const size_t o = (size_t)cj * vs;
   // the row product first, while the norms below are reduced
-  double x = row < n3 ? base_row_share<T, TI>(inv + (size_t)row * n3, b + o, n3, lane) : 0.0;
+  double x = row < n3 ? base_row_or_unit<T, TI>(inv, rowmap, row, b + o, n3, lane) : 0.0;
   double cur[B];
 #pragma unroll
   for (int q = 0; q < B; ++q) {
@@ -1340,6 +1353,7 @@ template int launch_restrict<float>(hipStream_t, const DSell<float> &, const uin
 //   x = converged ? 0 : damp * inv b;  c (+)= x;  xout = x.
 template <typename T, typename TI>
 __global__ __launch_bounds__(kBlock) void k_base_gemv(int n, const TI *__restrict__ inv,
+                                                       const int *__restrict__ rowmap,
                                                        const T *__restrict__ b,
                                                        T *__restrict__ c, T *__restrict__ xout,
                                                        int cmode, double damp, double abslimit,
@@ -1353,7 +1367,7 @@ __global__ __launch_bounds__(kBlock) void k_base_gemv(int n, const TI *__restric
   part += (size_t)cj * nparts;
   // the row product is formed while the stop test's norm is reduced (and
   // dropped if the test says converged)
-  double x = row < n ? base_row_share<T, TI>(inv + (size_t)row * n, b + o, n, lane) : 0.0;
+  double x = row < n ? base_row_or_unit<T, TI>(inv, rowmap, row, b + o, n, lane) : 0.0;
   double s = 0.0;
   for (int i = lane; i < nparts; i += 64) s += part[i];
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
@@ -1382,25 +1396,44 @@ __global__ __launch_bounds__(kBlock) void k_base_gemv(int n, const TI *__restric
 }
 
 template <typename T, typename TI>
-void launch_base_gemv(hipStream_t st, int n, const TI *inv, const T *b, T *c, T *xout,
-                      int cmode, double damp, double abslimit, double red, const double *part,
-                      int nparts, double *first, int it, Batch bt) {
+void launch_base_gemv(hipStream_t st, int n, const TI *inv, const int *rowmap, const T *b, T *c,
+                      T *xout, int cmode, double damp, double abslimit, double red,
+                      const double *part, int nparts, double *first, int it, Batch bt) {
   const int g = (n * 64 + kBlock - 1) / kBlock;
   if (bt.b == 3)   // n counts scalar unknowns
-    k_base_gemv_b3<T, TI><<<grid2(g, bt.nrhs), kBlock, 0, st>>>(n, inv, b, c, xout, cmode, damp,
-                                                                abslimit, red, part, nparts, first,
-                                                                it, bt.vs);
+    k_base_gemv_b3<T, TI><<<grid2(g, bt.nrhs), kBlock, 0, st>>>(n, inv, rowmap, b, c, xout, cmode,
+                                                                damp, abslimit, red, part, nparts,
+                                                                first, it, bt.vs);
   else
-    k_base_gemv<T, TI><<<grid2(g, bt.nrhs), kBlock, 0, st>>>(n, inv, b, c, xout, cmode, damp,
-                                                             abslimit, red, part, nparts, first,
-                                                             it, bt.vs);
+    k_base_gemv<T, TI><<<grid2(g, bt.nrhs), kBlock, 0, st>>>(n, inv, rowmap, b, c, xout, cmode,
+                                                             damp, abslimit, red, part, nparts,
+                                                             first, it, bt.vs);
 }
-template void launch_base_gemv<double, double>(hipStream_t, int, const double *, const double *,
-                                               double *, double *, int, double, double, double,
-                                               const double *, int, double *, int, Batch);
-template void launch_base_gemv<float, float>(hipStream_t, int, const float *, const float *, float *,
-                                             float *, int, double, double, double, const double *,
-                                             int, double *, int, Batch);
+template void launch_base_gemv<double, double>(hipStream_t, int, const double *, const int *,
+                                               const double *, double *, double *, int, double,
+                                               double, double, const double *, int, double *, int,
+                                               Batch);
+template void launch_base_gemv<float, float>(hipStream_t, int, const float *, const int *,
+                                             const float *, float *, float *, int, double, double,
+                                             double, const double *, int, double *, int, Batch);
+
+// rows act[k] of the n x n inverse -> row k of the FP64 and FP32 copies
+__global__ __launch_bounds__(kBlock) void k_base_compact(int nact, int n, const int *__restrict__ act,
+                                                          const double *__restrict__ inv,
+                                                          double *__restrict__ c64,
+                                                          float *__restrict__ c32) {
+  const size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= (size_t)nact * n) return;
+  const int k = (int)(e / (size_t)n), j = (int)(e % (size_t)n);
+  const double v = inv[(size_t)act[k] * n + j];
+  c64[e] = v;
+  c32[e] = (float)v;
+}
+void launch_base_compact(hipStream_t st, int nact, int n, const int *act, const double *inv,
+                         double *c64, float *c32) {
+  const size_t ne = (size_t)nact * n;
+  if (ne > 0) k_base_compact<<<(unsigned)((ne + kBlock - 1) / kBlock), kBlock, 0, st>>>(nact, n, act, inv, c64, c32);
+}
 
 template <typename T>
 int launch_dot_part(hipStream_t st, int n, const T *x, const uint8_t *vclass, double *part,

@@ -235,6 +235,31 @@ def test_jacobi_solve_3d_defects_match_ug(jump3d_33):
     ctx.close()
 
 
+@pytest.mark.parametrize("bl,maxit", [(1, 1), (2, 2)])
+def test_w_cycle_jacobi_defects_match_ug(jump3d_33, bl, maxit):
+    """lmgc $g 2 (W-cycle): the base level is visited twice per visit of the
+    level above, the second time on the defect the first leaves."""
+    from ug_amd import ugmg
+    P = jump3d_33
+    H = O.Hierarchy(P.levels)
+    want = O.ls_solve(H, P.top, O.params(smoother=O.JAC, damp=0.8, baselevel=bl, gamma=2, base_maxiter=maxit),
+                      P.rhs, red=1e-10, maxiter=100)
+    ctx = ugmg.Context(0)
+    ctx.set_hierarchy(P)
+    ctx.set_params(nu1=2, nu2=2, gamma=2, baselevel=bl, smoother=ugmg.SMOOTH_JAC, damp=0.8,
+                   base_abslimit=1e-300, base_red=1e-12, base_maxiter=maxit, use_graph=1)
+    ctx.preprocess()
+    n = P.levels[-1].n
+    x, b = _full(n), _dev(P.rhs)
+    print(f"w-cycle bl={bl} maxit={maxit}", flush=True)
+    res, defects = ctx.solve(P.top, x, b, red=1e-10, maxiter=100)
+    d = np.array(defects)[:, 0]
+    w = want["defects"][:, 0]
+    assert res.number_of_linear_iterations == want["nit"]
+    assert np.max(np.abs(d - w) / w) < 1e-10
+    ctx.close()
+
+
 def test_mcgs_solution_matches_ug_lexicographic_gs(jump3d_33):
     from ug_amd import ugmg
     P = jump3d_33

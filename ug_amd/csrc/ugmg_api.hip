@@ -113,6 +113,12 @@ struct ugmg_ctx {
   double *base_cinv = nullptr;
   float *base_cinv32 = nullptr;
   int *base_rowmap = nullptr;
+  // column order of the compact inverse: the unknowns whose defect can be
+  // nonzero after the restriction first (base_ncyc of them), then the
+  // Dirichlet unknowns the restriction sets to zero (VECSKIP component of a
+  // unit row with VNCLASS >= 2), whose columns a cycle's base solve skips
+  int *base_colperm = nullptr;
+  int base_ncyc = 0;
   template <typename T> const T *base_cinv_t() const {
     if constexpr (std::is_same_v<T, double>) return base_cinv; else return base_cinv32;
   }
@@ -233,6 +239,8 @@ void free_device(ugmg_ctx *ctx) {
   ctx->base_cinv = nullptr;
   ctx->base_cinv32 = nullptr;
   ctx->base_rowmap = nullptr;
+  ctx->base_colperm = nullptr;
+  ctx->base_ncyc = 0;
   ctx->clev64 = nullptr;
   ctx->clev32 = nullptr;
   ctx->fuse_top = -1;
@@ -355,8 +363,11 @@ void coarse_cycle(ugmg_ctx *ctx, int top, T *c, T *b, int cmode) {
 // dead once the coarse correction is known (the next restriction overwrites
 // every entry when all base vectors have VNCLASS >= 2), so b -= A x is only
 // formed when a later iteration or the caller needs it.
+// restricted: b comes from the restriction inside a cycle (zero on the
+// skipped Dirichlet unknowns, and b -= A x keeps it so)
 template <typename T>
-void base_solve(ugmg_ctx *ctx, T *c, T *b, int cmode, bool have_parts, bool need_b) {
+void base_solve(ugmg_ctx *ctx, T *c, T *b, int cmode, bool have_parts, bool need_b,
+                bool restricted = false) {
   DevLevel &d = ctx->dev[ctx->bl];
   hipStream_t st = ctx->stream;
   const ugmg_params &p = ctx->p;
@@ -368,7 +379,8 @@ void base_solve(ugmg_ctx *ctx, T *c, T *b, int cmode, bool have_parts, bool need
   if (m == 0 && cmode == 0) launch_fill<T>(st, d.n, c, (T)0, bt);
   for (int it = 0; it < m; ++it) {
     if (!ctx->base.failed)
-      launch_base_gemv<T, T>(st, d.n * d.nb, ctx->base_cinv_t<T>(), ctx->base_rowmap, b, c, x,
+      launch_base_gemv<T, T>(st, d.n * d.nb, ctx->base_cinv_t<T>(), ctx->base_rowmap,
+                             ctx->base_colperm, restricted ? ctx->base_ncyc : d.n * d.nb, b, c, x,
                              it == 0 ? cmode : 1, p.base_damp[0],
                           p.base_abslimit[0], p.base_red[0], ctx->base_part, np,
                           ctx->base_first, it, bt);
@@ -409,7 +421,8 @@ void lmgc(ugmg_ctx *ctx, int l, T *c, T *b, int cmode) {
       // the first visit uses the restriction's partial norms; after a
       // base solve b has changed, so later visits recompute them
       if (g == 0) ctx->base_nparts = np;
-      base_solve<T>(ctx, dc.cv<T>(), dc.bv<T>(), g == 0 ? 0 : 1, g == 0, false);
+      // a later visit (gamma > 1) reads the base defect the earlier one leaves
+      base_solve<T>(ctx, dc.cv<T>(), dc.bv<T>(), g == 0 ? 0 : 1, g == 0, g + 1 < p.gamma, true);
     } else {
       lmgc<T>(ctx, l - 1, dc.cv<T>(), dc.bv<T>(), g == 0 ? 0 : 1);
     }
@@ -836,7 +849,7 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
       const HostLevel &H = ctx->host[bl];
       const int nb = ctx->dev[bl].nb, nsc = ctx->dev[bl].n * nb;
       if (nsc > 0 && ctx->base_inv) {
-        std::vector<int> rowmap(nsc, -1), act;
+        std::vector<int> rowmap(nsc, -1), act, keep, drop;
         for (int v = 0; v < H.n; ++v)
           for (int q = 0; q < nb; ++q) {
             bool unit = H.rp[v] < H.rp[v + 1] && H.ci[H.rp[v]] == v;
@@ -848,21 +861,29 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
                   break;
                 }
               }
+            const int u = v * nb + q;
             if (!unit) {
-              rowmap[(size_t)v * nb + q] = (int)act.size();
-              act.push_back(v * nb + q);
+              rowmap[u] = (int)act.size();
+              act.push_back(u);
             }
+            // the restriction writes 0 to this unknown: VECSKIP component of
+            // a vector with VNCLASS >= 2 (StandardRestrictNodeVector)
+            const bool zero = unit && ((H.skip[v] >> q) & 1u) && H.vnc[v] >= 2;
+            (zero ? drop : keep).push_back(u);
           }
         const int nact = (int)act.size();
+        ctx->base_ncyc = (int)keep.size();
+        keep.insert(keep.end(), drop.begin(), drop.end());
         const size_t ne = std::max<size_t>(1, (size_t)nact * nsc);
         if (act.empty()) act.push_back(0);   // (nothing to copy; keeps the upload non-empty)
         int *act_d = nullptr;
         if ((rc = upload(ctx, &ctx->base_rowmap, rowmap.data(), rowmap.size())) ||
+            (rc = upload(ctx, &ctx->base_colperm, keep.data(), keep.size())) ||
             (rc = upload(ctx, &act_d, act.data(), act.size())) ||
             (rc = dalloc(ctx, &ctx->base_cinv, ne)) || (rc = dalloc(ctx, &ctx->base_cinv32, ne)))
           return rc;
-        launch_base_compact(ctx->stream, nact, nsc, act_d, ctx->base_inv, ctx->base_cinv,
-                            ctx->base_cinv32);
+        launch_base_compact(ctx->stream, nact, nsc, act_d, ctx->base_colperm, ctx->base_inv,
+                            ctx->base_cinv, ctx->base_cinv32);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(ctx->stream));
       }

@@ -152,15 +152,18 @@ int assemble_device(hipStream_t st, const AsmInput &in, double *rhs, std::vector
 void launch_base_factor(hipStream_t st, double *band, int n, int bw, const int *pos, double *X,
                         double *inv, int *failed, bool fmode = false);
 // inv: the rows of the dense inverse that are not unit rows (rowmap[r]: the
-// row's index there, -1 for a unit row), in FP64 (TI = double) or the FP32
-// copy for the FP32 cycle (TI = float); products are summed in FP64
+// row's index there, -1 for a unit row), columns in the order cp, in FP64
+// (TI = double) or the FP32 copy for the FP32 cycle (TI = float); the
+// products run over the first ncols columns and are summed in FP64
 template <typename T, typename TI>
-void launch_base_gemv(hipStream_t st, int n, const TI *inv, const int *rowmap, const T *b, T *c,
-                      T *xout, int cmode, double damp, double abslimit, double red,
-                      const double *part, int nparts, double *first, int it, Batch bt);
-// rows act[0..nact) of the n x n inverse into compact FP64 / FP32 copies
-void launch_base_compact(hipStream_t st, int nact, int n, const int *act, const double *inv,
-                         double *c64, float *c32);
+void launch_base_gemv(hipStream_t st, int n, const TI *inv, const int *rowmap, const int *cp,
+                      int ncols, const T *b, T *c, T *xout, int cmode, double damp,
+                      double abslimit, double red, const double *part, int nparts, double *first,
+                      int it, Batch bt);
+// rows act[0..nact), columns in the order cp, of the n x n inverse into
+// compact FP64 / FP32 copies
+void launch_base_compact(hipStream_t st, int nact, int n, const int *act, const int *cp,
+                         const double *inv, double *c64, float *c32);
 // columns of the partial sums: one per component of every right-hand side
 // (column j * b + i), nparts each
 template <typename T>

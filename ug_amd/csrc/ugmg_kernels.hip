@@ -798,31 +798,38 @@ __global__ __launch_bounds__(kBlock) void k_dot_part_b3(int n, const T *__restri
 
 // lane's share of row . b for the base GEMV: four independent accumulators
 // keep four loads of the row in flight per lane (the wave sums the shares)
+// (rr's columns are in the order cp: column j multiplies b[cp[j]]; the
+// first ncols of them are used)
 template <typename T, typename TI>
 __device__ __forceinline__ double base_row_share(const TI *__restrict__ rr,
-                                                 const T *__restrict__ bb, int n, int lane) {
+                                                 const T *__restrict__ bb,
+                                                 const int *__restrict__ cp, int ncols, int lane) {
   double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
   int j = lane;
-  for (; j + 192 < n; j += 256) {
-    a0 += (double)rr[j] * (double)bb[j];
-    a1 += (double)rr[j + 64] * (double)bb[j + 64];
-    a2 += (double)rr[j + 128] * (double)bb[j + 128];
-    a3 += (double)rr[j + 192] * (double)bb[j + 192];
+  for (; j + 192 < ncols; j += 256) {
+    a0 += (double)rr[j] * (double)bb[cp[j]];
+    a1 += (double)rr[j + 64] * (double)bb[cp[j + 64]];
+    a2 += (double)rr[j + 128] * (double)bb[cp[j + 128]];
+    a3 += (double)rr[j + 192] * (double)bb[cp[j + 192]];
   }
-  for (; j < n; j += 64) a0 += (double)rr[j] * (double)bb[j];
+  for (; j < ncols; j += 64) a0 += (double)rr[j] * (double)bb[cp[j]];
   return (a0 + a1) + (a2 + a3);
 }
 
 // This lane's share of row `row` of the inverse times b.  The inverse holds
 // only the rows that are not unit rows (rowmap[row] = their index; -1: a unit
-// row, from a Dirichlet row of the matrix, whose product is b[row] itself).
+// row, from a Dirichlet row of the matrix, whose product is b[row] itself),
+// n columns each, in the order cp (the non-unit unknowns first); the product
+// runs over the first ncols columns (ncols < n when b is known to vanish on
+// the others).
 template <typename T, typename TI>
 __device__ __forceinline__ double base_row_or_unit(const TI *__restrict__ inv,
-                                                   const int *__restrict__ rowmap, int row,
+                                                   const int *__restrict__ rowmap,
+                                                   const int *__restrict__ cp, int ncols, int row,
                                                    const T *__restrict__ bb, int n, int lane) {
   const int k = rowmap[row];
   if (k < 0) return lane == 0 ? (double)bb[row] : 0.0;
-  return base_row_share<T, TI>(inv + (size_t)k * n, bb, n, lane);
+  return base_row_share<T, TI>(inv + (size_t)k * n, bb, cp, ncols, lane);
 }
 
 // ls over ex on the base level with per-component stop tests (sc_cmp:
@@ -830,6 +837,7 @@ __device__ __forceinline__ double base_row_or_unit(const TI *__restrict__ inv,
 template <typename T, typename TI>
 __global__ __launch_bounds__(kBlock) void k_base_gemv_b3(int n3, const TI *__restrict__ inv,
                                                           const int *__restrict__ rowmap,
+                                                          const int *__restrict__ cp, int ncols,
                                                           const T *__restrict__ b,
                                                           T *__restrict__ c, T *__restrict__ xout,
                                                           int cmode, double damp, double abslimit,
@@ -842,7 +850,7 @@ __global__ __launch_bounds__(kBlock) void k_base_gemv_b3(int n3, const TI *__res
   const int cj = blockIdx.y;
   const size_t o = (size_t)cj * vs;
   // the row product first, while the norms below are reduced
-  double x = row < n3 ? base_row_or_unit<T, TI>(inv, rowmap, row, b + o, n3, lane) : 0.0;
+  double x = row < n3 ? base_row_or_unit<T, TI>(inv, rowmap, cp, ncols, row, b + o, n3, lane) : 0.0;
   double cur[B];
 #pragma unroll
   for (int q = 0; q < B; ++q) {
@@ -1354,6 +1362,7 @@ template int launch_restrict<float>(hipStream_t, const DSell<float> &, const uin
 template <typename T, typename TI>
 __global__ __launch_bounds__(kBlock) void k_base_gemv(int n, const TI *__restrict__ inv,
                                                        const int *__restrict__ rowmap,
+                                                       const int *__restrict__ cp, int ncols,
                                                        const T *__restrict__ b,
                                                        T *__restrict__ c, T *__restrict__ xout,
                                                        int cmode, double damp, double abslimit,
@@ -1367,7 +1376,7 @@ __global__ __launch_bounds__(kBlock) void k_base_gemv(int n, const TI *__restric
   part += (size_t)cj * nparts;
   // the row product is formed while the stop test's norm is reduced (and
   // dropped if the test says converged)
-  double x = row < n ? base_row_or_unit<T, TI>(inv, rowmap, row, b + o, n, lane) : 0.0;
+  double x = row < n ? base_row_or_unit<T, TI>(inv, rowmap, cp, ncols, row, b + o, n, lane) : 0.0;
   double s = 0.0;
   for (int i = lane; i < nparts; i += 64) s += part[i];
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
@@ -1396,43 +1405,48 @@ __global__ __launch_bounds__(kBlock) void k_base_gemv(int n, const TI *__restric
 }
 
 template <typename T, typename TI>
-void launch_base_gemv(hipStream_t st, int n, const TI *inv, const int *rowmap, const T *b, T *c,
-                      T *xout, int cmode, double damp, double abslimit, double red,
-                      const double *part, int nparts, double *first, int it, Batch bt) {
+void launch_base_gemv(hipStream_t st, int n, const TI *inv, const int *rowmap, const int *cp,
+                      int ncols, const T *b, T *c, T *xout, int cmode, double damp,
+                      double abslimit, double red, const double *part, int nparts, double *first,
+                      int it, Batch bt) {
   const int g = (n * 64 + kBlock - 1) / kBlock;
   if (bt.b == 3)   // n counts scalar unknowns
-    k_base_gemv_b3<T, TI><<<grid2(g, bt.nrhs), kBlock, 0, st>>>(n, inv, rowmap, b, c, xout, cmode,
-                                                                damp, abslimit, red, part, nparts,
-                                                                first, it, bt.vs);
+    k_base_gemv_b3<T, TI><<<grid2(g, bt.nrhs), kBlock, 0, st>>>(n, inv, rowmap, cp, ncols, b, c,
+                                                                xout, cmode, damp, abslimit, red,
+                                                                part, nparts, first, it, bt.vs);
   else
-    k_base_gemv<T, TI><<<grid2(g, bt.nrhs), kBlock, 0, st>>>(n, inv, rowmap, b, c, xout, cmode,
-                                                             damp, abslimit, red, part, nparts,
-                                                             first, it, bt.vs);
+    k_base_gemv<T, TI><<<grid2(g, bt.nrhs), kBlock, 0, st>>>(n, inv, rowmap, cp, ncols, b, c, xout,
+                                                             cmode, damp, abslimit, red, part,
+                                                             nparts, first, it, bt.vs);
 }
 template void launch_base_gemv<double, double>(hipStream_t, int, const double *, const int *,
-                                               const double *, double *, double *, int, double,
-                                               double, double, const double *, int, double *, int,
-                                               Batch);
+                                               const int *, int, const double *, double *,
+                                               double *, int, double, double, double,
+                                               const double *, int, double *, int, Batch);
 template void launch_base_gemv<float, float>(hipStream_t, int, const float *, const int *,
-                                             const float *, float *, float *, int, double, double,
-                                             double, const double *, int, double *, int, Batch);
+                                             const int *, int, const float *, float *, float *, int,
+                                             double, double, double, const double *, int, double *,
+                                             int, Batch);
 
-// rows act[k] of the n x n inverse -> row k of the FP64 and FP32 copies
+// entry (act[k], cp[j]) of the n x n inverse -> (k, j) of the FP64 and FP32
+// copies (rows act, columns in the order cp)
 __global__ __launch_bounds__(kBlock) void k_base_compact(int nact, int n, const int *__restrict__ act,
+                                                          const int *__restrict__ cp,
                                                           const double *__restrict__ inv,
                                                           double *__restrict__ c64,
                                                           float *__restrict__ c32) {
   const size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x;
   if (e >= (size_t)nact * n) return;
   const int k = (int)(e / (size_t)n), j = (int)(e % (size_t)n);
-  const double v = inv[(size_t)act[k] * n + j];
+  const double v = inv[(size_t)act[k] * n + cp[j]];
   c64[e] = v;
   c32[e] = (float)v;
 }
-void launch_base_compact(hipStream_t st, int nact, int n, const int *act, const double *inv,
-                         double *c64, float *c32) {
+void launch_base_compact(hipStream_t st, int nact, int n, const int *act, const int *cp,
+                         const double *inv, double *c64, float *c32) {
   const size_t ne = (size_t)nact * n;
-  if (ne > 0) k_base_compact<<<(unsigned)((ne + kBlock - 1) / kBlock), kBlock, 0, st>>>(nact, n, act, inv, c64, c32);
+  if (ne > 0)
+    k_base_compact<<<(unsigned)((ne + kBlock - 1) / kBlock), kBlock, 0, st>>>(nact, n, act, cp, inv, c64, c32);
 }
 
 template <typename T>

@@ -636,14 +636,14 @@ __global__ __launch_bounds__(kBlock) void k_jac_b3(int n, const T *__restrict__ 
   for (int q = 0; q < 3; ++q) t[o + q] = x[q] * damp;
 }
 
-template <typename TV, typename T, int NT>
-__global__ __launch_bounds__(kBlock) void k_gs_color_b3(DSell<TV> L, int slice0, int nsl,
+template <typename TV, typename T, int NT, int BS = kBlock>
+__global__ __launch_bounds__(BS) void k_gs_color_b3(DSell<TV> L, int slice0, int nsl,
                                                          const T *__restrict__ b,
                                                          const T *__restrict__ diag,
                                                          const uint8_t *__restrict__ vclass,
                                                          T *__restrict__ v, size_t vs) {
   constexpr int B = 3;
-  SLICE_PROLOGUE(L, slice0, nsl)
+  SLICE_PROLOGUE_BS(L, slice0, nsl, BS)
   const size_t o = (size_t)col_ * vs;
   v += o;
   T sum[B];
@@ -691,15 +691,15 @@ __global__ __launch_bounds__(kBlock) void k_gs_defect_b3(DSell<TV> U, int nsl,
   }
 }
 
-template <typename TV, typename T, int CMODE, int NT>
-__global__ __launch_bounds__(kBlock) void k_gs_color_fused_b3(DSell<TV> L, DSell<TV> U,
+template <typename TV, typename T, int CMODE, int NT, int BS = kBlock>
+__global__ __launch_bounds__(BS) void k_gs_color_fused_b3(DSell<TV> L, DSell<TV> U,
                                                                int slice0, int nsl,
                                                                T *__restrict__ b,
                                                                const T *__restrict__ diag, T damp,
                                                                T *__restrict__ v,
                                                                T *__restrict__ c, size_t vs) {
   constexpr int B = 3;
-  SLICE_PROLOGUE(L, slice0, nsl)
+  SLICE_PROLOGUE_BS(L, slice0, nsl, BS)
   const size_t o = (size_t)col_ * vs;
   v += o;
   const int eu = U.soff[s];
@@ -1224,8 +1224,10 @@ static int gs_chunk(int w) {
 
 // Colour sweeps run in workgroups of one wavefront (one slice): measured on
 // the 129^3 solves against 128/256/512/1024 threads, the FP64 solve 4 % and
-// the mixed one 2 % faster than with 256 (a colour's launch is short, and
-// single-wave groups start and retire independently).
+// the mixed one 2 % faster than with 256, the 65^3 elasticity solves (3x3
+// blocks) 6-8 % (a colour's launch is short, and single-wave groups start
+// and retire independently).  The launches over a whole level (SpMV, GS
+// defect, interpolation) gain nothing from it (measured) and keep 256.
 constexpr int kSweepBlock = 64;
 #define GSB_DISPATCH(NSL, ...)                                                  \
   {                                                                             \
@@ -1239,8 +1241,8 @@ void launch_gs_color(hipStream_t st, const DSell<TV> &L, int slice0, int nsl, in
                      const T *diag, const uint8_t *vclass, T *v, bool nt, Batch bt) {
   if (nsl <= 0) return;
   if (bt.b == 3) {
-    NT_DISPATCH(nt, k_gs_color_b3<TV, T, NTV><<<grid2(nblk_slices(nsl), bt.nrhs), kBlock, 0, st>>>(
-                        L, slice0, nsl, b, diag, vclass, v, bt.vs));
+    GSB_DISPATCH(nsl, NT_DISPATCH(nt, k_gs_color_b3<TV, T, NTV, BSV><<<g, BSV, 0, st>>>(
+                                          L, slice0, nsl, b, diag, vclass, v, bt.vs)));
     return;
   }
   if constexpr (!std::is_same_v<T, double>) {
@@ -1283,12 +1285,11 @@ void launch_gs_color_fused(hipStream_t st, const DSell<TV> &L, const DSell<TV> &
                            int nsl, int wmax_l, int wmax_u, T *b, const T *diag, T damp, T *v,
                            T *c, int cmode, bool nt, Batch bt) {
   if (nsl <= 0) return;
-  const dim3 g = grid2(nblk_slices(nsl), bt.nrhs);
   if (bt.b == 3) {
-    NT_DISPATCH(nt, if (cmode == 0) k_gs_color_fused_b3<TV, T, 0, NTV><<<g, kBlock, 0, st>>>(
+    GSB_DISPATCH(nsl, NT_DISPATCH(nt, if (cmode == 0) k_gs_color_fused_b3<TV, T, 0, NTV, BSV><<<g, BSV, 0, st>>>(
                         L, U, slice0, nsl, b, diag, damp, v, c, bt.vs);
-                else k_gs_color_fused_b3<TV, T, 1, NTV><<<g, kBlock, 0, st>>>(
-                        L, U, slice0, nsl, b, diag, damp, v, c, bt.vs));
+                else k_gs_color_fused_b3<TV, T, 1, NTV, BSV><<<g, BSV, 0, st>>>(
+                        L, U, slice0, nsl, b, diag, damp, v, c, bt.vs)));
     return;
   }
   // one chunk per part sized to the colour's widest row (a colour's upper

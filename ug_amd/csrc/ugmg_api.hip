@@ -820,7 +820,8 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
         if ((rc = upload_sell(ctx, hp.P, D.P))) return rc;
         if ((rc = upload_sell(ctx, hp.PT, D.PT))) return rc;
       }
-      D.nt = (double)hp.A.col.size() * 12.0 > ps.nt_bytes;
+      // matrix bytes: b*b FP64 values and one column index per entry
+      D.nt = (double)hp.A.col.size() * (8.0 * D.nb * D.nb + 4.0) > ps.nt_bytes;
       // columns padded to 64 entries (512-byte aligned FP64 columns)
       D.vs = ((size_t)std::max(D.n, 1) * D.nb + 63) / 64 * 64;
       const size_t nv = D.vs * (size_t)ctx->nrhs_cap;

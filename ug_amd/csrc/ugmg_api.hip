@@ -69,6 +69,8 @@ struct DevLevel {
   int p_nnz = 0, pt_nnz = 0;
   DevSellBuf P, PT;   // interpolation from / restriction to level-1
   bool nt = false;    // stream the matrix with non-temporal loads
+  bool nt32 = false;  // the same for the FP32 copy
+  template <typename T> bool ntv() const { if constexpr (std::is_same_v<T, double>) return nt; else return nt32; }
   int nb = 1;         // components per vector (block size)
   size_t vs = 0;      // column stride of the level vectors (scalars; batched launches)
   // device order (GS levels): perm_h[p] = list index of device position p;
@@ -288,14 +290,14 @@ void smooth_step(ugmg_ctx *ctx, int l, T *t, T *b, T *c, int cmode, T *xout) {
   if (ctx->p.smoother == UGMG_SMOOTH_JAC) {
     launch_jac<T>(st, d.n, b, d.diag<T>(), d.vclass, damp, t, bt);
     const int mode = cmode < 0 ? 0 : (cmode == 0 ? 3 : 2);
-    launch_spmv<T, T, T>(st, d.A.d<T>(), 0, d.A.nslices, t, b, c, mode, d.nt, bt);
+    launch_spmv<T, T, T>(st, d.A.d<T>(), 0, d.A.nslices, t, b, c, mode, d.ntv<T>(), bt);
   } else {
     for (int k = 0; k < d.ncolors; ++k)
       launch_gs_color<T, T>(st, d.L.d<T>(), d.color_slices[k],
                             d.color_slices[k + 1] - d.color_slices[k], d.color_wmax[k], b,
-                            d.diag<T>(), d.vclass, t, d.nt, bt);
+                            d.diag<T>(), d.vclass, t, d.ntv<T>(), bt);
     launch_gs_defect<T, T>(st, d.U.d<T>(), t, b, d.vclass, damp, cmode < 0 ? nullptr : c,
-                           cmode < 0 ? 1 : cmode, xout, d.nt, bt);
+                           cmode < 0 ? 1 : cmode, xout, d.ntv<T>(), bt);
   }
 }
 
@@ -315,14 +317,14 @@ void smooth(ugmg_ctx *ctx, int l, T *t, T *b, T *c, int cmode, int nsteps) {
   for (int k = 0; k < d.ncolors; ++k)
     launch_gs_color<T, T>(st, d.L.d<T>(), d.color_slices[k],
                           d.color_slices[k + 1] - d.color_slices[k], d.color_wmax[k], b,
-                          d.diag<T>(), d.vclass, t, d.nt, bt);
+                          d.diag<T>(), d.vclass, t, d.ntv<T>(), bt);
   for (int i = 1; i < nsteps; ++i)
     for (int k = 0; k < d.ncolors; ++k)
       launch_gs_color_fused<T, T>(st, d.L.d<T>(), d.U.d<T>(), d.color_slices[k],
                                   d.color_slices[k + 1] - d.color_slices[k], d.color_wmax[k],
                                   d.color_wmax_u[k], b, d.diag<T>(), damp, t, c,
-                                  i == 1 ? cmode : 1, d.nt, bt);
-  launch_gs_defect<T, T>(st, d.U.d<T>(), t, b, d.vclass, damp, c, 1, nullptr, d.nt, bt);
+                                  i == 1 ? cmode : 1, d.ntv<T>(), bt);
+  launch_gs_defect<T, T>(st, d.U.d<T>(), t, b, d.vclass, damp, c, 1, nullptr, d.ntv<T>(), bt);
 }
 
 BaseArgs base_args(ugmg_ctx *ctx) {
@@ -386,7 +388,7 @@ void base_solve(ugmg_ctx *ctx, T *c, T *b, int cmode, bool have_parts, bool need
                           ctx->base_first, it, bt);
     const bool more = it + 1 < m;
     if (more || need_b || !ctx->base_b_dead)
-      launch_spmv<T, T, T>(st, d.A.d<T>(), 0, d.A.nslices, x, b, (T *)nullptr, 0, d.nt, bt);
+      launch_spmv<T, T, T>(st, d.A.d<T>(), 0, d.A.nslices, x, b, (T *)nullptr, 0, d.ntv<T>(), bt);
     if (more) np = launch_dot_part<T>(st, d.n, b, d.vclass, ctx->base_part, bt);
   }
 }
@@ -413,7 +415,7 @@ void lmgc(ugmg_ctx *ctx, int l, T *c, T *b, int cmode) {
   // RestrictDefect(level, b, b, Factor_One)
   const bool to_base = l - 1 == ctx->bl && (ctx->fuse_top < ctx->bl || nb > 1);
   const int np = launch_restrict<T>(st, d.PT.d<T>(), dc.skip, dc.vnclass, dc.vclass, b, (T)1,
-                                    dc.bv<T>(), to_base ? ctx->base_part : nullptr, d.nt, nb,
+                                    dc.bv<T>(), to_base ? ctx->base_part : nullptr, d.ntv<T>(), nb,
                                     d.vs, dc.vs, d.nb);
   if (p.gamma <= 0) launch_fill<T>(st, dc.n, dc.cv<T>(), (T)0, batch(ctx, l - 1));
   for (int g = 0; g < p.gamma; ++g) {
@@ -428,9 +430,9 @@ void lmgc(ugmg_ctx *ctx, int l, T *c, T *b, int cmode) {
     }
   }
   // InterpolateCorrection(level, t, c, damp); dadd(c, t); dmatmul_minus(b, A, t)
-  launch_interp<T>(st, d.P.d<T>(), d.skip, dc.cv<T>(), (T)p.lmgc_damp[0], t, c, d.nt, nb, d.vs,
+  launch_interp<T>(st, d.P.d<T>(), d.skip, dc.cv<T>(), (T)p.lmgc_damp[0], t, c, d.ntv<T>(), nb, d.vs,
                    dc.vs, d.nb);
-  launch_spmv<T, T, T>(st, d.A.d<T>(), 0, d.A.nslices, t, b, (T *)nullptr, 0, d.nt,
+  launch_spmv<T, T, T>(st, d.A.d<T>(), 0, d.A.nslices, t, b, (T *)nullptr, 0, d.ntv<T>(),
                        batch(ctx, l));
   smooth<T>(ctx, l, t, b, c, 1, p.nu2);
 }
@@ -820,8 +822,10 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
         if ((rc = upload_sell(ctx, hp.P, D.P))) return rc;
         if ((rc = upload_sell(ctx, hp.PT, D.PT))) return rc;
       }
-      // matrix bytes: b*b FP64 values and one column index per entry
+      // matrix bytes: b*b values (FP64, or FP32 for the FP32 copy) and one
+      // column index per entry
       D.nt = (double)hp.A.col.size() * (8.0 * D.nb * D.nb + 4.0) > ps.nt_bytes;
+      D.nt32 = (double)hp.A.col.size() * (4.0 * D.nb * D.nb + 4.0) > ps.nt_bytes;   // the FP32 copy
       // columns padded to 64 entries (512-byte aligned FP64 columns)
       D.vs = ((size_t)std::max(D.n, 1) * D.nb + 63) / 64 * 64;
       const size_t nv = D.vs * (size_t)ctx->nrhs_cap;

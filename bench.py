@@ -302,7 +302,9 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nref", type=int, default=7, help="refinements of the unit hex (7: 129^3)")
-    ap.add_argument("--inner", type=int, default=3, help="FP32 V-cycles per FP64 refinement step")
+    ap.add_argument("--inner", type=int, default=7,
+                    help="FP32 V-cycles per FP64 refinement step (at most: a step runs fewer when the "
+                         "rate predicts the tolerance met)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--cpu-cycles", type=int, default=2)
     ap.add_argument("--serve-requests", type=int, default=32)

@@ -67,6 +67,10 @@ typedef struct {
   int ex_float;                     /* ex $f: band LU in FLOAT arithmetic on
                                        FLOAT entries (EXDecomposeMatrixFLOAT),
                                        applied in DOUBLE (EXApplyLUFLOAT)     */
+  int inner_adapt;                  /* MIXED: a refinement step runs fewer than
+                                       inner_cycles FP32 cycles when the rate
+                                       of the previous steps predicts that
+                                       fewer reach the stop test (default 1) */
 } ugmg_params;
 
 /* LRESULT (np/procs/ls.h:71-77) */

@@ -23,7 +23,9 @@
  *             $o 1 (BFS band reorder), $f (FLOAT band LU), $damp 1
  *   transfer  the standard transfer (no options read here)
  * GPU-only options of lmgc: $precision fp64|mixed, $inner k (FP32 cycles
- * per FP64 step), $graph 0|1; of ls/bcgs: $nrhs k (bcgs columns).
+ * per FP64 step), $adapt 0|1 (a step runs fewer cycles when the rate
+ * predicts the stop test met; default 1), $graph 0|1; of ls/bcgs: $nrhs k
+ * (bcgs columns).
  *
  * Differences a script sees: the pre- and post-smoother of one lmgc must be
  * the same class with the same $damp (one smoother per hierarchy on the

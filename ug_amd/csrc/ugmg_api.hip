@@ -129,7 +129,9 @@ struct ugmg_ctx {
   double *red = nullptr;      // device scalars
   double *red_host = nullptr; // pinned
   // cached graph of one solver iteration
-  hipGraphExec_t gexec = nullptr;
+  // (mixed: one graph per number of FP32 cycles, index = cycles)
+  static constexpr int kMaxGraphs = 65;
+  hipGraphExec_t gexec[kMaxGraphs] = {};
   int gkind = -1, glevel = -1;
   const void *gx = nullptr, *gb = nullptr;
   // levels bl..fuse_top run inside one fused coarse-cycle kernel
@@ -226,10 +228,11 @@ void drop_bcgs_graphs(ugmg_ctx *ctx) {
 }
 
 void free_device(ugmg_ctx *ctx) {
-  if (ctx->gexec) {
-    hipGraphExecDestroy(ctx->gexec);
-    ctx->gexec = nullptr;
-  }
+  for (hipGraphExec_t &g : ctx->gexec)
+    if (g) {
+      hipGraphExecDestroy(g);
+      g = nullptr;
+    }
   ctx->gkind = -1;
   drop_bcgs_graphs(ctx);
   ctx->bc = ugmg_ctx::Bcgs();
@@ -255,8 +258,11 @@ void free_device(ugmg_ctx *ctx) {
 }
 
 void drop_graph(ugmg_ctx *ctx) {
-  if (ctx->gexec) hipGraphExecDestroy(ctx->gexec);
-  ctx->gexec = nullptr;
+  for (hipGraphExec_t &g : ctx->gexec)
+    if (g) {
+      hipGraphExecDestroy(g);
+      g = nullptr;
+    }
   ctx->gkind = -1;
   drop_bcgs_graphs(ctx);
 }
@@ -456,10 +462,10 @@ int surface_norm(ugmg_ctx *ctx, int level, const double *x, double *out) {
 
 // one iteration of the linear solver, enqueued on the stream; leaves the
 // defect norm in ctx->red[0]
-void enqueue_iteration(ugmg_ctx *ctx, int level, double *x, double *b) {
+void enqueue_iteration(ugmg_ctx *ctx, int level, double *x, double *b, int cycles) {
   DevLevel &d = ctx->dev[level];
   if (ctx->p.precision == UGMG_MIXED) {
-    const int k = std::max(1, ctx->p.inner_cycles);
+    const int k = std::max(1, cycles);
     for (int j = 0; j < k; ++j) lmgc<float>(ctx, level, d.c32, d.b32, j == 0 ? 0 : 1);
     int np = 0;
     launch_ir_update(ctx->stream, d.A.d64(), d.c32, b, x, d.b32, d.vclass, ctx->part, &np, d.nt,
@@ -475,39 +481,44 @@ bool can_capture(ugmg_ctx *ctx) { return ctx->p.use_graph && ctx->stream != null
 
 // enqueue one solver iteration (from the cached graph when enabled) and the
 // asynchronous copy of its defect norm to ctx->red_host[0]
-int launch_iteration(ugmg_ctx *ctx, int level, double *x, double *b) {
-  if (!can_capture(ctx)) {
-    enqueue_iteration(ctx, level, x, b);
+// cycles: FP32 cycles of a mixed step (ignored in FP64)
+int launch_iteration(ugmg_ctx *ctx, int level, double *x, double *b, int cycles) {
+  const bool mixed = ctx->p.precision == UGMG_MIXED;
+  cycles = mixed ? std::max(1, cycles) : 1;
+  if (!can_capture(ctx) || cycles >= ugmg_ctx::kMaxGraphs) {
+    enqueue_iteration(ctx, level, x, b, cycles);
     HIPCHK(hipGetLastError());
   } else {
     const int kind = ctx->p.precision;
-    if (!ctx->gexec || ctx->gkind != kind || ctx->glevel != level || ctx->gx != x ||
-        ctx->gb != b) {
+    if (ctx->gkind != kind || ctx->glevel != level || ctx->gx != x || ctx->gb != b) {
       drop_graph(ctx);
-      hipGraph_t g = nullptr;
-      HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-      enqueue_iteration(ctx, level, x, b);
-      HIPCHK(hipStreamEndCapture(ctx->stream, &g));
-      hipError_t e = hipGraphInstantiate(&ctx->gexec, g, nullptr, nullptr, 0);
-      hipGraphDestroy(g);
-      if (e != hipSuccess) {
-        ctx->gexec = nullptr;
-        return fail(ctx, UGMG_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-      }
       ctx->gkind = kind;
       ctx->glevel = level;
       ctx->gx = x;
       ctx->gb = b;
     }
-    HIPCHK(hipGraphLaunch(ctx->gexec, ctx->stream));
+    hipGraphExec_t &ge = ctx->gexec[cycles];
+    if (!ge) {
+      hipGraph_t g = nullptr;
+      HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      enqueue_iteration(ctx, level, x, b, cycles);
+      HIPCHK(hipStreamEndCapture(ctx->stream, &g));
+      hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+      hipGraphDestroy(g);
+      if (e != hipSuccess) {
+        ge = nullptr;
+        return fail(ctx, UGMG_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+      }
+    }
+    HIPCHK(hipGraphLaunch(ge, ctx->stream));
   }
   HIPCHK(hipMemcpyAsync(ctx->red_host, ctx->red, sizeof(double) * ctx->b, hipMemcpyDeviceToHost,
                         ctx->stream));
   return 0;
 }
 
-int run_iteration(ugmg_ctx *ctx, int level, double *x, double *b) {
-  int rc = launch_iteration(ctx, level, x, b);
+int run_iteration(ugmg_ctx *ctx, int level, double *x, double *b, int cycles) {
+  int rc = launch_iteration(ctx, level, x, b, cycles);
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(ctx->stream));
   return 0;
@@ -983,6 +994,7 @@ void ugmg_params_default(ugmg_params *p) {
   p->inner_cycles = 1;
   p->use_graph = 0;
   p->ex_float = 0;
+  p->inner_adapt = 1;
 }
 
 int ugmg_create(int device, ugmg_ctx **out) {
@@ -1551,10 +1563,36 @@ int ugmg_solve(ugmg_ctx *ctx, int level, double *x, double *b, const double *abs
   int conv = sc_cmp(first, abslimit);
   if (ctx->p.precision == UGMG_MIXED)
     launch_convert<float, double>(ctx->stream, d.n * d.nb, b, d.b32, 1, d.vs, d.vs);
+  // mixed steps: inner_cycles FP32 cycles, fewer when the per-cycle rate of
+  // the previous step (the slowest component's) predicts that fewer bring
+  // every component below 0.9 of the target (or of abslimit)
+  const int kin = std::max(1, ctx->p.inner_cycles);
+  const bool adapt = ctx->p.precision == UGMG_MIXED && ctx->p.inner_adapt && kin > 1;
+  double prev[UGMG_MAXB] = {0.0, 0.0, 0.0};
+  int prev_cycles = 0;
+  for (int k = 0; k < nc; ++k) last[k] = first[k];
+  auto cycles_for = [&](const double *lim) {
+    int need = 1;
+    for (int q = 0; q < nc; ++q) {
+      const double goal = 0.9 * std::fabs(lim[q]);
+      if (last[q] <= goal) continue;
+      if (!(prev[q] > 0.0) || !(last[q] > 0.0) || !(goal > 0.0)) return kin;
+      const double rho = std::pow(last[q] / prev[q], 1.0 / prev_cycles);
+      if (!(rho < 1.0) || !(rho > 0.0)) return kin;
+      const double m = std::ceil(std::log(goal / last[q]) / std::log(rho));
+      if (!(m < kin)) return kin;
+      need = std::max(need, (int)m);
+    }
+    return need;
+  };
   int i;
   for (i = 0; i < maxiter; ++i) {
     if (conv) break;
-    if ((rc = run_iteration(ctx, level, x, b))) {
+    int cycles = kin;
+    if (adapt && prev_cycles > 0) cycles = std::min(cycles_for(target), cycles_for(abslimit));
+    for (int k = 0; k < nc; ++k) prev[k] = last[k];
+    prev_cycles = cycles;
+    if ((rc = run_iteration(ctx, level, x, b, cycles))) {
       res->error_code = rc;
       move_out();
       return rc;
@@ -2311,7 +2349,7 @@ int srv_advance(ugmg_server *s, SrvReq *r) {
     srv_finish(s, r, UGMG_REQ_DONE);
     return 1;
   }
-  int rc = launch_iteration(ctx, r->top, r->x, r->b);
+  int rc = launch_iteration(ctx, r->top, r->x, r->b, ctx->p.inner_cycles);
   if (rc || hipEventRecord(r->ev, ctx->stream) != hipSuccess) {
     r->err = ctx->err;
     r->res.error_code = rc ? rc : UGMG_E_HIP;

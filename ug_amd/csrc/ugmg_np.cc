@@ -107,7 +107,7 @@ struct NumProc {
   ugmg_params params{};
   // lmgc
   std::string T, pre, post, base;
-  int gamma = 1, nu1 = 1, nu2 = 1, bl = 0, precision = UGMG_FP64, inner = 1, graph = 1;
+  int gamma = 1, nu1 = 1, nu2 = 1, bl = 0, precision = UGMG_FP64, inner = 1, graph = 1, adapt = 1;
   // smoothers
   double damp[UGMG_MAXB] = {1, 1, 1};
   int reorder = 1, fmode = 0;
@@ -334,6 +334,7 @@ int init_object(ugmg_np_env *e, NumProc &o) {
       else if (prec != "fp64") return fail(e, UGMG_NP_NOT_ACTIVE, "lmgc: $precision is fp64 or mixed");
     }
     if (read_int(av, "inner", &o.inner)) o.inner = 1;
+    if (read_int(av, "adapt", &o.adapt)) o.adapt = 1;
     if (read_int(av, "graph", &o.graph)) o.graph = 1;
     return iter_state(av, o);
   }
@@ -386,6 +387,7 @@ int build_params(ugmg_np_env *e, NumProc &o, ugmg_params &p, bool &use_iter) {
   p.ex_float = ex->fmode;
   p.precision = mg->precision;
   p.inner_cycles = mg->inner;
+  p.inner_adapt = mg->adapt;
   p.use_graph = mg->graph;
   return 0;
 }

@@ -38,7 +38,8 @@ class Params(C.Structure):
                 ("damp", C.c_double * 3), ("lmgc_damp", C.c_double * 3), ("base_damp", C.c_double * 3),
                 ("base_abslimit", C.c_double * 3), ("base_red", C.c_double * 3),
                 ("base_maxiter", C.c_int), ("ex_reorder", C.c_int), ("precision", C.c_int),
-                ("inner_cycles", C.c_int), ("use_graph", C.c_int), ("ex_float", C.c_int)]
+                ("inner_cycles", C.c_int), ("use_graph", C.c_int), ("ex_float", C.c_int),
+                ("inner_adapt", C.c_int)]
 
 
 class LResult(C.Structure):

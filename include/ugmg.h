@@ -327,7 +327,10 @@ int ugmg_server_result(ugmg_server *s, int id, double *x, int n);
 /* ---- measurement --------------------------------------------------------
  * kind: 0 dmatmul_minus FP64, 1 smatmul_minus FP32, 2 jacobi step FP64,
  *       3 gs step FP64, 4 restrict, 5 interpolate, 6 one cycle (current
- *       precision), 7 gs step FP32.  Runs `reps` back-to-back launches on
+ *       precision), 7 gs step FP32, 8 gs step FP64 as a cycle runs the
+ *       steps after the first (8 colour sweeps with the previous step's
+ *       defect folded in; scalar levels whose rows are all active; the
+ *       bytes of kind 3).  Runs `reps` back-to-back launches on
  *       the context stream between two hipEvents and returns the mean
  *       duration in ms, plus the algorithmic bytes of one launch. */
 int ugmg_time_kernel(ugmg_ctx *ctx, int level, int kind, int reps,

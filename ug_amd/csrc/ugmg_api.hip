@@ -2057,7 +2057,9 @@ int ugmg_time_kernel(ugmg_ctx *ctx, int level, int kind, int reps, double *ms, d
   if (reps <= 0 || !ms || !bytes) return fail(ctx, UGMG_E_ARG, "bad argument");
   DevLevel &d = ctx->dev[level];
   if ((kind == 4 || kind == 5) && level <= ctx->bl) return fail(ctx, UGMG_E_ARG, "transfer needs level > base");
-  if ((kind == 3 || kind == 7) && d.ncolors == 0) return fail(ctx, UGMG_E_STATE, "not a GS level");
+  if ((kind == 3 || kind == 7 || kind == 8) && d.ncolors == 0) return fail(ctx, UGMG_E_STATE, "not a GS level");
+  if (kind == 8 && (!d.all_active || d.nb != 1))
+    return fail(ctx, UGMG_E_STATE, "the fused GS step needs a scalar level whose rows are all active");
   hipStream_t st = ctx->stream;
   // deterministic nonzero vectors
   launch_fill<double>(st, d.n, d.t, 1.0, B1(d));
@@ -2100,6 +2102,14 @@ int ugmg_time_kernel(ugmg_ctx *ctx, int level, int kind, int reps, double *ms, d
         ctx->p.smoother = s;
         break;
       }
+      case 8:   // a GS step as a cycle runs its steps after the first
+        for (int k = 0; k < d.ncolors; ++k)
+          launch_gs_color_fused<double, double>(st, d.L.d64(), d.U.d64(), d.color_slices[k],
+                                                d.color_slices[k + 1] - d.color_slices[k],
+                                                d.color_wmax[k], d.color_wmax_u[k], d.b,
+                                                d.diag<double>(), ctx->p.damp[0], d.t, d.c, 1,
+                                                d.nt, B1(d));
+        break;
       default: break;
     }
   };
@@ -2108,7 +2118,8 @@ int ugmg_time_kernel(ugmg_ctx *ctx, int level, int kind, int reps, double *ms, d
     case 1: by = spmv_bytes(d, 4, 4); break;
     case 2: by = spmv_bytes(d, 8, 8) + n * (8 + 8 + 16); break;
     // colour sweep (lower entries, b, diag, v) + defect (upper entries, b rmw, v, c rmw)
-    case 3: by = (nnz - n) * 12.0 + 8.0 * n * (1 + 1 + 1 + 2 + 1 + 2) + 8.0 * n; break;
+    case 3:
+    case 8: by = (nnz - n) * 12.0 + 8.0 * n * (1 + 1 + 1 + 2 + 1 + 2) + 8.0 * n; break;
     case 7: by = (nnz - n) * 8.0 + 4.0 * n * (1 + 1 + 1 + 2 + 1 + 2) + 8.0 * n; break;
     case 4: {
       DevLevel &dc = ctx->dev[level - 1];
@@ -2122,7 +2133,7 @@ int ugmg_time_kernel(ugmg_ctx *ctx, int level, int kind, int reps, double *ms, d
     }
     default: by = 0.0;
   }
-  if (kind < 0 || kind > 7) return fail(ctx, UGMG_E_ARG, "unknown kernel kind");
+  if (kind < 0 || kind > 8) return fail(ctx, UGMG_E_ARG, "unknown kernel kind");
   once();  // warm
   HIPCHK(hipStreamSynchronize(st));
   hipEvent_t e0, e1;

@@ -23,7 +23,7 @@ ERRORS = {0: "OK", -1: "E_ARG", -2: "E_HIP", -3: "E_STATE", -4: "E_NOMEM", -5: "
           -6: "E_BUDGET", -7: "E_NODEV"}
 
 # time_kernel kinds
-K_SPMV64, K_SPMV32, K_JAC64, K_GS64, K_RESTRICT, K_INTERP, K_CYCLE, K_GS32 = range(8)
+K_SPMV64, K_SPMV32, K_JAC64, K_GS64, K_RESTRICT, K_INTERP, K_CYCLE, K_GS32, K_GSF64 = range(9)
 
 
 class UgmgError(RuntimeError):

@@ -383,8 +383,10 @@ def test_fused_coarse_cycle_matches(poisson2d, monkeypatch):
 @pytest.mark.parametrize("precision,nu", [(0, 2), (1, 3), (0, 1)])
 def test_folded_gs_defects_are_bitwise_the_separate_passes(jump3d_17, monkeypatch, precision, nu):
     """Folding each GS step's defect into the next step's colour sweeps
-    (the default) changes no bit of the cycle against separate defect
-    passes (UGMG_GS_FUSE=0), in FP64 and in the FP32 cycle; and the list
+    (the default), and the post-smoother's colour-0 sweep into the SpMV
+    that applies the coarse-grid correction, change no bit of the cycle
+    against separate passes (UGMG_GS_FUSE=0), in FP64 and in the FP32
+    cycle; and the list
     order (UGMG_GS_WINDOW=1) gives the device order's results to rounding."""
     from ug_amd import ugmg
     P = jump3d_17

@@ -79,6 +79,10 @@ struct DevLevel {
   int *perm = nullptr;
   double *c = nullptr, *b = nullptr, *t = nullptr;
   float *c32 = nullptr, *b32 = nullptr, *t32 = nullptr;
+  // the post-smoother's iterate when its colour-0 sweep runs inside the
+  // SpMV after the interpolation (which still reads t)
+  double *u = nullptr;
+  float *u32 = nullptr;
   // surface solve (allocated on first use): FINE_GRID_DOF class bytes, the
   // level's x in device order, its b-component squared sums
   uint8_t *fgd = nullptr;
@@ -87,6 +91,7 @@ struct DevLevel {
   template <typename T> T *cv() const { if constexpr (std::is_same_v<T, double>) return c; else return c32; }
   template <typename T> T *bv() const { if constexpr (std::is_same_v<T, double>) return b; else return b32; }
   template <typename T> T *tv() const { if constexpr (std::is_same_v<T, double>) return t; else return t32; }
+  template <typename T> T *uv() const { if constexpr (std::is_same_v<T, double>) return u; else return u32; }
   template <typename T> const T *diag() const { if constexpr (std::is_same_v<T, double>) return diag64; else return diag32; }
 };
 
@@ -333,6 +338,36 @@ void smooth(ugmg_ctx *ctx, int l, T *t, T *b, T *c, int cmode, int nsteps) {
   launch_gs_defect<T, T>(st, d.U.d<T>(), t, b, d.vclass, damp, c, 1, nullptr, d.ntv<T>(), bt);
 }
 
+// Post-smoothing after the coarse-grid correction t on a GS level whose
+// rows are all active: b -= A t also sweeps colour 0 (into u, since t is
+// still read), so the first step's sweeps start at colour 1; the steps run
+// as in smooth() with u as the iterate (bit-identical to the separate passes)
+bool fold_colour0(const ugmg_ctx *ctx, int l) {
+  const DevLevel &d = ctx->dev[l];
+  return ctx->p.smoother == UGMG_SMOOTH_GS && d.all_active && d.nb == 1 && ctx->fuse_steps &&
+         ctx->p.nu2 >= 1 && d.ncolors > 0 && d.color_slices[0] == 0;
+}
+
+template <typename T>
+void post_smooth_folded(ugmg_ctx *ctx, int l, const T *t, T *b, T *c, int nsteps) {
+  DevLevel &d = ctx->dev[l];
+  hipStream_t st = ctx->stream;
+  const T damp = (T)ctx->p.damp[0];
+  const Batch bt = batch(ctx, l);
+  T *u = d.uv<T>();
+  launch_spmv_c0<T, T>(st, d.A.d<T>(), t, b, d.diag<T>(), u, d.color_slices[1], d.ntv<T>(), bt);
+  for (int k = 1; k < d.ncolors; ++k)
+    launch_gs_color<T, T>(st, d.L.d<T>(), d.color_slices[k],
+                          d.color_slices[k + 1] - d.color_slices[k], d.color_wmax[k], b,
+                          d.diag<T>(), d.vclass, u, d.ntv<T>(), bt);
+  for (int i = 1; i < nsteps; ++i)
+    for (int k = 0; k < d.ncolors; ++k)
+      launch_gs_color_fused<T, T>(st, d.L.d<T>(), d.U.d<T>(), d.color_slices[k],
+                                  d.color_slices[k + 1] - d.color_slices[k], d.color_wmax[k],
+                                  d.color_wmax_u[k], b, d.diag<T>(), damp, u, c, 1, d.ntv<T>(), bt);
+  launch_gs_defect<T, T>(st, d.U.d<T>(), u, b, d.vclass, damp, c, 1, nullptr, d.ntv<T>(), bt);
+}
+
 BaseArgs base_args(ugmg_ctx *ctx) {
   BaseArgs ba;
   ba.n = ctx->dev[ctx->bl].n;
@@ -438,6 +473,10 @@ void lmgc(ugmg_ctx *ctx, int l, T *c, T *b, int cmode) {
   // InterpolateCorrection(level, t, c, damp); dadd(c, t); dmatmul_minus(b, A, t)
   launch_interp<T>(st, d.P.d<T>(), d.skip, dc.cv<T>(), (T)p.lmgc_damp[0], t, c, d.ntv<T>(), nb, d.vs,
                    dc.vs, d.nb);
+  if (fold_colour0(ctx, l)) {
+    post_smooth_folded<T>(ctx, l, t, b, c, p.nu2);
+    return;
+  }
   launch_spmv<T, T, T>(st, d.A.d<T>(), 0, d.A.nslices, t, b, (T *)nullptr, 0, d.ntv<T>(),
                        batch(ctx, l));
   smooth<T>(ctx, l, t, b, c, 1, p.nu2);
@@ -842,12 +881,13 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
       const size_t nv = D.vs * (size_t)ctx->nrhs_cap;
       if ((rc = dalloc(ctx, &D.c, nv)) || (rc = dalloc(ctx, &D.b, nv)) ||
           (rc = dalloc(ctx, &D.t, nv)) || (rc = dalloc(ctx, &D.c32, nv)) ||
-          (rc = dalloc(ctx, &D.b32, nv)) || (rc = dalloc(ctx, &D.t32, nv)))
+          (rc = dalloc(ctx, &D.b32, nv)) || (rc = dalloc(ctx, &D.t32, nv)) ||
+          (rc = dalloc(ctx, &D.u, nv)) || (rc = dalloc(ctx, &D.u32, nv)))
         return rc;
       // every level vector starts at zero: the zero-valued phantom entries of
       // uniform slices gather from them before a sweep has written them
-      for (double *v : {D.c, D.b, D.t}) HIPCHK(hipMemset(v, 0, sizeof(double) * nv));
-      for (float *v : {D.c32, D.b32, D.t32}) HIPCHK(hipMemset(v, 0, sizeof(float) * nv));
+      for (double *v : {D.c, D.b, D.t, D.u}) HIPCHK(hipMemset(v, 0, sizeof(double) * nv));
+      for (float *v : {D.c32, D.b32, D.t32, D.u32}) HIPCHK(hipMemset(v, 0, sizeof(float) * nv));
     }
     ctx->base = std::move(ps.base);
     if (!ctx->base.inv.empty() || ctx->base.n == 0) {   // factored on the host

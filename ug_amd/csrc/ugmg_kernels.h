@@ -115,6 +115,11 @@ template <typename TV, typename T>
 void launch_gs_color_fused(hipStream_t st, const DSell<TV> &L, const DSell<TV> &U, int slice0,
                            int nsl, int wmax_l, int wmax_u, T *b, const T *diag, T damp, T *v,
                            T *c, int cmode, bool nt, Batch bt);
+// y -= A x on a GS level's colour-grouped rows, plus u = y / diag on the rows
+// of colour 0 (the next sweep's first colour; scalar, every row active)
+template <typename TV, typename T>
+void launch_spmv_c0(hipStream_t st, const DSell<TV> &A, const T *x, T *y, const T *diag, T *u,
+                    int ncs0, bool nt, Batch bt);
 template <typename TV, typename T>
 void launch_gs_defect(hipStream_t st, const DSell<TV> &U, const T *v, T *b, const uint8_t *vclass,
                       T damp, T *c, int cmode, T *xout, bool nt, Batch bt);

@@ -292,6 +292,25 @@ __global__ __launch_bounds__(kBlock) void k_spmv(DSell<TV> A, int slice0, int ns
   }
 }
 
+// y -= A x over a GS level's colour-grouped rows, and for the rows of colour
+// 0 (slices below ncs0) also the next GS sweep's colour 0, u_r = y_r / a_rr
+// (colour 0 has no lower neighbours; every row active): the separate sweep
+// would compute (y_r - 0) / a_rr, the same value.
+template <int MW, typename TV, typename T, int NT>
+__global__ __launch_bounds__(kBlock) void k_spmv_c0(DSell<TV> A, int nsl, const T *__restrict__ x,
+                                                     T *__restrict__ y, const T *__restrict__ diag,
+                                                     T *__restrict__ u, int ncs0, size_t vs) {
+  SLICE_PROLOGUE(A, 0, nsl)
+  const size_t o = (size_t)col_ * vs;
+  x += o;
+  y += o;
+  const T sum = sell_dot<MW, NT, TV, T, T>(A.val + e0 + lane, A.col + e0 + lane, w, x, sdo, A.dtab, rb);
+  if (r < 0) return;
+  const T yn = y[r] - sum;
+  y[r] = yn;
+  if (s < ncs0) u[o + r] = yn / diag[r];
+}
+
 // FP64 defect of a mixed-precision refinement step:
 //   r -= A c32 (FP64 values and sums); x += c32; r32 = (float) r;
 //   per-block partial sums of r^2 over NEW_DEFECT rows.
@@ -1184,6 +1203,19 @@ template void launch_spmv<double, double, double>(hipStream_t, const DSell<doubl
                                                   Batch);
 template void launch_spmv<float, float, float>(hipStream_t, const DSell<float> &, int, int,
                                                const float *, float *, float *, int, bool, Batch);
+
+template <typename TV, typename T>
+void launch_spmv_c0(hipStream_t st, const DSell<TV> &A, const T *x, T *y, const T *diag, T *u,
+                    int ncs0, bool nt, Batch bt) {
+  if (A.nslices <= 0) return;
+  const dim3 g = grid2(nblk_slices(A.nslices), bt.nrhs);
+  W_DISPATCH(A.wmax, NT_DISPATCH(nt, k_spmv_c0<MWV, TV, T, NTV><<<g, kBlock, 0, st>>>(
+                                         A, A.nslices, x, y, diag, u, ncs0, bt.vs)));
+}
+template void launch_spmv_c0<double, double>(hipStream_t, const DSell<double> &, const double *,
+                                             double *, const double *, double *, int, bool, Batch);
+template void launch_spmv_c0<float, float>(hipStream_t, const DSell<float> &, const float *, float *,
+                                           const float *, float *, int, bool, Batch);
 
 void launch_ir_update(hipStream_t st, const DSell<double> &A, const float *c32, double *r,
                       double *x, float *r32, const uint8_t *vclass, double *part, int *nparts,

@@ -65,6 +65,7 @@ struct DevLevel {
   double *diag64 = nullptr;
   float *diag32 = nullptr;
   uint8_t *vclass = nullptr, *vnclass = nullptr;
+  uint8_t *c0 = nullptr;   // GS levels: 1 on the rows of colour 0 (device order)
   uint32_t *skip = nullptr;
   int p_nnz = 0, pt_nnz = 0;
   DevSellBuf P, PT;   // interpolation from / restriction to level-1
@@ -293,7 +294,7 @@ void to_user(ugmg_ctx *ctx, int l, const TS *src, TD *dst, size_t ldd, int nrhs 
 // ------------------------------------------------------------ cycle drivers
 
 template <typename T>
-void smooth_step(ugmg_ctx *ctx, int l, T *t, T *b, T *c, int cmode, T *xout) {
+void smooth_step(ugmg_ctx *ctx, int l, T *t, T *b, T *c, int cmode, T *xout, int k0 = 0) {
   DevLevel &d = ctx->dev[l];
   hipStream_t st = ctx->stream;
   const T damp = (T)ctx->p.damp[0];
@@ -303,7 +304,7 @@ void smooth_step(ugmg_ctx *ctx, int l, T *t, T *b, T *c, int cmode, T *xout) {
     const int mode = cmode < 0 ? 0 : (cmode == 0 ? 3 : 2);
     launch_spmv<T, T, T>(st, d.A.d<T>(), 0, d.A.nslices, t, b, c, mode, d.ntv<T>(), bt);
   } else {
-    for (int k = 0; k < d.ncolors; ++k)
+    for (int k = k0; k < d.ncolors; ++k)
       launch_gs_color<T, T>(st, d.L.d<T>(), d.color_slices[k],
                             d.color_slices[k + 1] - d.color_slices[k], d.color_wmax[k], b,
                             d.diag<T>(), d.vclass, t, d.ntv<T>(), bt);
@@ -314,18 +315,20 @@ void smooth_step(ugmg_ctx *ctx, int l, T *t, T *b, T *c, int cmode, T *xout) {
 
 // nsteps smoother steps (cmode: correction mode of the first).  On a GS
 // level whose rows are all active, the defect of every step but the last is
-// folded into the next step's colour sweeps.
+// folded into the next step's colour sweeps.  k0 = 1: the first step's
+// colour-0 sweep has already written t (by the kernel that produced b).
 template <typename T>
-void smooth(ugmg_ctx *ctx, int l, T *t, T *b, T *c, int cmode, int nsteps) {
+void smooth(ugmg_ctx *ctx, int l, T *t, T *b, T *c, int cmode, int nsteps, int k0 = 0) {
   DevLevel &d = ctx->dev[l];
   if (ctx->p.smoother != UGMG_SMOOTH_GS || !d.all_active || nsteps < 2 || !ctx->fuse_steps) {
-    for (int i = 0; i < nsteps; ++i) smooth_step<T>(ctx, l, t, b, c, i == 0 ? cmode : 1, nullptr);
+    for (int i = 0; i < nsteps; ++i)
+      smooth_step<T>(ctx, l, t, b, c, i == 0 ? cmode : 1, nullptr, i == 0 ? k0 : 0);
     return;
   }
   hipStream_t st = ctx->stream;
   const T damp = (T)ctx->p.damp[0];
   const Batch bt = batch(ctx, l);
-  for (int k = 0; k < d.ncolors; ++k)
+  for (int k = k0; k < d.ncolors; ++k)
     launch_gs_color<T, T>(st, d.L.d<T>(), d.color_slices[k],
                           d.color_slices[k + 1] - d.color_slices[k], d.color_wmax[k], b,
                           d.diag<T>(), d.vclass, t, d.ntv<T>(), bt);
@@ -338,34 +341,26 @@ void smooth(ugmg_ctx *ctx, int l, T *t, T *b, T *c, int cmode, int nsteps) {
   launch_gs_defect<T, T>(st, d.U.d<T>(), t, b, d.vclass, damp, c, 1, nullptr, d.ntv<T>(), bt);
 }
 
-// Post-smoothing after the coarse-grid correction t on a GS level whose
-// rows are all active: b -= A t also sweeps colour 0 (into u, since t is
-// still read), so the first step's sweeps start at colour 1; the steps run
-// as in smooth() with u as the iterate (bit-identical to the separate passes)
-bool fold_colour0(const ugmg_ctx *ctx, int l) {
+// A GS level whose rows are all active (scalar): the colour-0 sweep of a
+// smoother step (no lower neighbours: v_r = b_r / a_rr) can run inside the
+// kernel that produces b, bit-identical to the separate sweep
+bool fold_colour0(const ugmg_ctx *ctx, int l, int nsteps) {
   const DevLevel &d = ctx->dev[l];
   return ctx->p.smoother == UGMG_SMOOTH_GS && d.all_active && d.nb == 1 && ctx->fuse_steps &&
-         ctx->p.nu2 >= 1 && d.ncolors > 0 && d.color_slices[0] == 0;
+         nsteps >= 1 && d.ncolors > 0 && d.c0;
 }
 
+// Post-smoothing after the coarse-grid correction t: b -= A t also sweeps
+// colour 0 (A's rows are grouped by colour, so colour 0 is its first
+// color_slices[1] slices) into u, since t is still read; the steps then run
+// with u as the iterate
 template <typename T>
 void post_smooth_folded(ugmg_ctx *ctx, int l, const T *t, T *b, T *c, int nsteps) {
   DevLevel &d = ctx->dev[l];
-  hipStream_t st = ctx->stream;
-  const T damp = (T)ctx->p.damp[0];
-  const Batch bt = batch(ctx, l);
   T *u = d.uv<T>();
-  launch_spmv_c0<T, T>(st, d.A.d<T>(), t, b, d.diag<T>(), u, d.color_slices[1], d.ntv<T>(), bt);
-  for (int k = 1; k < d.ncolors; ++k)
-    launch_gs_color<T, T>(st, d.L.d<T>(), d.color_slices[k],
-                          d.color_slices[k + 1] - d.color_slices[k], d.color_wmax[k], b,
-                          d.diag<T>(), d.vclass, u, d.ntv<T>(), bt);
-  for (int i = 1; i < nsteps; ++i)
-    for (int k = 0; k < d.ncolors; ++k)
-      launch_gs_color_fused<T, T>(st, d.L.d<T>(), d.U.d<T>(), d.color_slices[k],
-                                  d.color_slices[k + 1] - d.color_slices[k], d.color_wmax[k],
-                                  d.color_wmax_u[k], b, d.diag<T>(), damp, u, c, 1, d.ntv<T>(), bt);
-  launch_gs_defect<T, T>(st, d.U.d<T>(), u, b, d.vclass, damp, c, 1, nullptr, d.ntv<T>(), bt);
+  launch_spmv_c0<T, T>(ctx->stream, d.A.d<T>(), t, b, d.diag<T>(), u, d.color_slices[1],
+                       d.ntv<T>(), batch(ctx, l));
+  smooth<T>(ctx, l, u, b, c, 1, nsteps, 1);
 }
 
 BaseArgs base_args(ugmg_ctx *ctx) {
@@ -436,7 +431,7 @@ void base_solve(ugmg_ctx *ctx, T *c, T *b, int cmode, bool have_parts, bool need
 
 // Lmgc on level l; cmode 0: c := correction, 1: c += correction
 template <typename T>
-void lmgc(ugmg_ctx *ctx, int l, T *c, T *b, int cmode) {
+void lmgc(ugmg_ctx *ctx, int l, T *c, T *b, int cmode, bool c0_done = false) {
   const ugmg_params &p = ctx->p;
   hipStream_t st = ctx->stream;
   if (l <= ctx->fuse_top && ctx->nb == 1) {
@@ -451,13 +446,18 @@ void lmgc(ugmg_ctx *ctx, int l, T *c, T *b, int cmode) {
   DevLevel &dc = ctx->dev[l - 1];
   T *t = d.tv<T>();
   const int nb = ctx->nb;
-  smooth<T>(ctx, l, t, b, c, cmode, p.nu1);
+  smooth<T>(ctx, l, t, b, c, cmode, p.nu1, c0_done ? 1 : 0);
   if (p.nu1 == 0 && cmode == 0) launch_fill<T>(st, d.n, c, (T)0, batch(ctx, l));
-  // RestrictDefect(level, b, b, Factor_One)
+  // RestrictDefect(level, b, b, Factor_One); on a coarse level that runs
+  // lmgc below (not the base solve or the fused coarse cycle) also its
+  // first pre-smoothing sweep's colour 0
   const bool to_base = l - 1 == ctx->bl && (ctx->fuse_top < ctx->bl || nb > 1);
+  const bool pre0 = l - 1 > ctx->bl && l - 1 > ctx->fuse_top && p.gamma >= 1 &&
+                    fold_colour0(ctx, l - 1, p.nu1);
   const int np = launch_restrict<T>(st, d.PT.d<T>(), dc.skip, dc.vnclass, dc.vclass, b, (T)1,
                                     dc.bv<T>(), to_base ? ctx->base_part : nullptr, d.ntv<T>(), nb,
-                                    d.vs, dc.vs, d.nb);
+                                    d.vs, dc.vs, d.nb, pre0 ? dc.c0 : nullptr, dc.diag<T>(),
+                                    pre0 ? dc.tv<T>() : nullptr);
   if (p.gamma <= 0) launch_fill<T>(st, dc.n, dc.cv<T>(), (T)0, batch(ctx, l - 1));
   for (int g = 0; g < p.gamma; ++g) {
     if (to_base) {
@@ -467,13 +467,13 @@ void lmgc(ugmg_ctx *ctx, int l, T *c, T *b, int cmode) {
       // a later visit (gamma > 1) reads the base defect the earlier one leaves
       base_solve<T>(ctx, dc.cv<T>(), dc.bv<T>(), g == 0 ? 0 : 1, g == 0, g + 1 < p.gamma, true);
     } else {
-      lmgc<T>(ctx, l - 1, dc.cv<T>(), dc.bv<T>(), g == 0 ? 0 : 1);
+      lmgc<T>(ctx, l - 1, dc.cv<T>(), dc.bv<T>(), g == 0 ? 0 : 1, g == 0 && pre0);
     }
   }
   // InterpolateCorrection(level, t, c, damp); dadd(c, t); dmatmul_minus(b, A, t)
   launch_interp<T>(st, d.P.d<T>(), d.skip, dc.cv<T>(), (T)p.lmgc_damp[0], t, c, d.ntv<T>(), nb, d.vs,
                    dc.vs, d.nb);
-  if (fold_colour0(ctx, l)) {
+  if (fold_colour0(ctx, l, p.nu2)) {
     post_smooth_folded<T>(ctx, l, t, b, c, p.nu2);
     return;
   }
@@ -857,6 +857,9 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
         D.color = hp.color_ug;
         if ((rc = upload_sell(ctx, hp.L, D.L))) return rc;
         if ((rc = upload_sell(ctx, hp.U, D.U))) return rc;
+        std::vector<uint8_t> c0(H.n);
+        for (int q = 0; q < H.n; ++q) c0[q] = hp.color[q] == 0;
+        if ((rc = upload(ctx, &D.c0, c0.data(), c0.size()))) return rc;
         D.all_active = true;
         for (uint8_t v : H.vc) D.all_active &= v >= 3;
         D.color_wmax.resize(D.ncolors);

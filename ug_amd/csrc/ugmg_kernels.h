@@ -124,11 +124,14 @@ template <typename TV, typename T>
 void launch_gs_defect(hipStream_t st, const DSell<TV> &U, const T *v, T *b, const uint8_t *vclass,
                       T damp, T *c, int cmode, T *xout, bool nt, Batch bt);
 // returns the number of blocks per column (= partial sums per column when
-// part != null; column j's partials start at part + j * return value)
+// part != null; column j's partials start at part + j * return value).
+// With ct (scalar only) also ct_r = bc_r / cdiag_r on the coarse rows whose
+// cc0 flag is set: the coarse GS pre-smoother's colour-0 sweep.
 template <typename T>
 int launch_restrict(hipStream_t st, const DSell<T> &PT, const uint32_t *cskip,
                     const uint8_t *cvnclass, const uint8_t *cvclass, const T *bf, T damp, T *bc,
-                    double *part, bool nt, int nrhs, size_t vsf, size_t vsc, int b);
+                    double *part, bool nt, int nrhs, size_t vsf, size_t vsc, int b,
+                    const uint8_t *cc0 = nullptr, const T *cdiag = nullptr, T *ct = nullptr);
 // Element lists of one level on the device (all pointers device memory):
 // nelem elements of nc corners, elem[e*nc + i] = vector of corner i; element
 // matrices (nc b) x (nc b) row-major over corner-major scalar unknowns,

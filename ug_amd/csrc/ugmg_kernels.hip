@@ -460,7 +460,10 @@ __global__ __launch_bounds__(kBlock) void k_restrict(DSell<T> PT, int nsl,
                                                       const T *__restrict__ bf, T damp,
                                                       T *__restrict__ bc,
                                                       double *__restrict__ part, size_t vsf,
-                                                      size_t vsc) {
+                                                      size_t vsc,
+                                                      const uint8_t *__restrict__ cc0,
+                                                      const T *__restrict__ cdiag,
+                                                      T *__restrict__ ct) {
   __shared__ double red[kBlock / 64];
   SLICE_PROLOGUE_RED(PT, nsl)
   bf += (size_t)col_ * vsf;
@@ -472,6 +475,9 @@ __global__ __launch_bounds__(kBlock) void k_restrict(DSell<T> PT, int nsl,
       const T base = cvnclass[r] >= 2 ? (T)0 : bc[r];
       const T v = (cskip[r] & 1u) ? base : base + damp * sum;
       bc[r] = v;
+      // the coarse pre-smoother's colour-0 sweep: no lower neighbours, so
+      // its value is (b_r - 0) / a_rr
+      if (ct && cc0[r]) ct[(size_t)col_ * vsc + r] = v / cdiag[r];
       if (part && cvclass[r] >= 2) sq = (double)v * (double)v;
     }
   }
@@ -1395,7 +1401,8 @@ template void launch_gs_defect<float, float>(hipStream_t, const DSell<float> &, 
 template <typename T>
 int launch_restrict(hipStream_t st, const DSell<T> &PT, const uint32_t *cskip,
                     const uint8_t *cvnclass, const uint8_t *cvclass, const T *bf, T damp, T *bc,
-                    double *part, bool nt, int nrhs, size_t vsf, size_t vsc, int b) {
+                    double *part, bool nt, int nrhs, size_t vsf, size_t vsc, int b,
+                    const uint8_t *cc0, const T *cdiag, T *ct) {
   if (PT.nslices <= 0) return 0;
   const int g = nblk_slices(PT.nslices);
   if (b == 3) {
@@ -1405,15 +1412,17 @@ int launch_restrict(hipStream_t st, const DSell<T> &PT, const uint32_t *cskip,
   }
   W_DISPATCH(PT.wmax, NT_DISPATCH(nt, k_restrict<MWV, T, NTV><<<grid2(g, nrhs), kBlock, 0, st>>>(
                                           PT, PT.nslices, cskip, cvnclass, cvclass, bf, damp, bc,
-                                          part, vsf, vsc)));
+                                          part, vsf, vsc, cc0, cdiag, ct)));
   return g;
 }
 template int launch_restrict<double>(hipStream_t, const DSell<double> &, const uint32_t *,
                                      const uint8_t *, const uint8_t *, const double *, double,
-                                     double *, double *, bool, int, size_t, size_t, int);
+                                     double *, double *, bool, int, size_t, size_t, int,
+                                     const uint8_t *, const double *, double *);
 template int launch_restrict<float>(hipStream_t, const DSell<float> &, const uint32_t *,
                                     const uint8_t *, const uint8_t *, const float *, float,
-                                    float *, double *, bool, int, size_t, size_t, int);
+                                    float *, double *, bool, int, size_t, size_t, int,
+                                    const uint8_t *, const float *, float *);
 
 // --------------------------------------------------- base solve (GEMV)
 // ls over ex on the base level, one wavefront per row of the dense inverse:

@@ -386,15 +386,19 @@ def test_folded_gs_defects_are_bitwise_the_separate_passes(jump3d_17, monkeypatc
     (the default), and the post-smoother's colour-0 sweep into the SpMV
     that applies the coarse-grid correction, change no bit of the cycle
     against separate passes (UGMG_GS_FUSE=0), in FP64 and in the FP32
-    cycle; and the list
+    cycle; the folded sweep gives the same bits with its two row products on
+    two wavefronts per slice (the default) or on one (UGMG_GS_SPLIT=0); and
+    the list
     order (UGMG_GS_WINDOW=1) gives the device order's results to rounding."""
     from ug_amd import ugmg
     P = jump3d_17
     n = P.levels[-1].n
     b0 = P.rhs.copy()
     out = {}
-    for tag, env in (("fused", {}), ("split", {"UGMG_GS_FUSE": "0"}), ("list", {"UGMG_GS_WINDOW": "1"})):
-        for k in ("UGMG_GS_FUSE", "UGMG_GS_WINDOW"):
+    legs = (("fused", {}), ("split", {"UGMG_GS_FUSE": "0"}), ("onewave", {"UGMG_GS_SPLIT": "0"}),
+            ("list", {"UGMG_GS_WINDOW": "1"}))
+    for tag, env in legs:
+        for k in ("UGMG_GS_FUSE", "UGMG_GS_WINDOW", "UGMG_GS_SPLIT"):
             monkeypatch.delenv(k, raising=False)
         for k, v in env.items():
             monkeypatch.setenv(k, v)
@@ -406,6 +410,7 @@ def test_folded_gs_defects_are_bitwise_the_separate_passes(jump3d_17, monkeypatc
         out[tag] = ctx.iter_host(P.top, b0)
         ctx.close()
     assert np.array_equal(out["fused"][0], out["split"][0]) and np.array_equal(out["fused"][1], out["split"][1])
+    assert np.array_equal(out["fused"][0], out["onewave"][0]) and np.array_equal(out["fused"][1], out["onewave"][1])
     tol = 1e-12 if precision == 0 else 1e-5
     assert _rel(out["fused"][0], out["list"][0]) < tol
 

@@ -153,6 +153,9 @@ struct ugmg_ctx {
   // the work being enqueued
   int nrhs_cap = 1, nb = 1;
   bool fuse_steps = true;   // fold GS defects into the next sweep (UGMG_GS_FUSE=0: off)
+  // folded sweeps with the upper and lower row products on two wavefronts
+  // per slice (UGMG_GS_SPLIT=0: one wavefront per slice)
+  bool gs_split = true;
   // BCGSSolver work (allocated on first use, freed with the hierarchy)
   struct Bcgs {
     int nrhs = 0, maxdef = 0, npart = 0;
@@ -337,7 +340,7 @@ void smooth(ugmg_ctx *ctx, int l, T *t, T *b, T *c, int cmode, int nsteps, int k
       launch_gs_color_fused<T, T>(st, d.L.d<T>(), d.U.d<T>(), d.color_slices[k],
                                   d.color_slices[k + 1] - d.color_slices[k], d.color_wmax[k],
                                   d.color_wmax_u[k], b, d.diag<T>(), damp, t, c,
-                                  i == 1 ? cmode : 1, d.ntv<T>(), bt);
+                                  i == 1 ? cmode : 1, d.ntv<T>(), bt, ctx->gs_split);
   launch_gs_defect<T, T>(st, d.U.d<T>(), t, b, d.vclass, damp, c, 1, nullptr, d.ntv<T>(), bt);
 }
 
@@ -725,6 +728,8 @@ int prep_host(ugmg_ctx *ctx, int level, PrepState &ps) {
   ps.nt_bytes = env_nt ? std::atof(env_nt) : 64.0e6;
   const char *env_gsf = std::getenv("UGMG_GS_FUSE");
   ctx->fuse_steps = !(env_gsf && std::atoi(env_gsf) == 0);
+  const char *env_split = std::getenv("UGMG_GS_SPLIT");
+  ctx->gs_split = !(env_split && std::atoi(env_split) == 0);
   const char *env_fuse = std::getenv("UGMG_FUSE_ROWS");
   ps.fuse_rows = env_fuse ? std::atoi(env_fuse) : 0;
   ps.level = level;
@@ -2151,7 +2156,7 @@ int ugmg_time_kernel(ugmg_ctx *ctx, int level, int kind, int reps, double *ms, d
                                                 d.color_slices[k + 1] - d.color_slices[k],
                                                 d.color_wmax[k], d.color_wmax_u[k], d.b,
                                                 d.diag<double>(), ctx->p.damp[0], d.t, d.c, 1,
-                                                d.nt, B1(d));
+                                                d.nt, B1(d), ctx->gs_split);
         break;
       default: break;
     }

@@ -110,11 +110,12 @@ template <typename TV, typename T>
 void launch_gs_color(hipStream_t st, const DSell<TV> &L, int slice0, int nsl, int wmax, const T *b,
                      const T *diag, const uint8_t *vclass, T *v, bool nt, Batch bt);
 // sweep of one colour with the previous step's defect folded in (all rows
-// active); cmode is the correction mode of the previous step
+// active); cmode is the correction mode of the previous step; split (scalar
+// rows): the upper and lower row products on two wavefronts per slice
 template <typename TV, typename T>
 void launch_gs_color_fused(hipStream_t st, const DSell<TV> &L, const DSell<TV> &U, int slice0,
                            int nsl, int wmax_l, int wmax_u, T *b, const T *diag, T damp, T *v,
-                           T *c, int cmode, bool nt, Batch bt);
+                           T *c, int cmode, bool nt, Batch bt, bool split);
 // y -= A x on a GS level's colour-grouped rows, plus u = y / diag on the rows
 // of colour 0 (the next sweep's first colour; scalar, every row active)
 template <typename TV, typename T>

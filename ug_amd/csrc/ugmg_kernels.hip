@@ -414,6 +414,55 @@ __global__ __launch_bounds__(BS) void k_gs_color_fused(DSell<TV> L, DSell<TV> U,
   v[r] = (bn - sl) / dr;
 }
 
+// The same step with the row's two products on the two wavefronts of one
+// workgroup per slice: wavefront 0 forms U v_s (then b, c and v), wavefront
+// 1 forms L v_{s+1} and passes it through LDS.  Both read v only at other
+// colours, and each product runs in the same order as above, so the results
+// are the same bits with half the dependent load rounds per wavefront.
+template <int CHU, int CHL, typename TV, typename T, int CMODE, int NT>
+__global__ __launch_bounds__(2 * kSlice) void k_gs_color_fused2(DSell<TV> L, DSell<TV> U,
+                                                                 int slice0, int nsl,
+                                                                 T *__restrict__ b,
+                                                                 const T *__restrict__ diag,
+                                                                 T damp, T *__restrict__ v,
+                                                                 T *__restrict__ c, size_t vs) {
+  __shared__ T lsum[kSlice];
+  int sb_, col_;
+  batch_map<false>(sb_, col_);
+  const int s = __builtin_amdgcn_readfirstlane(slice0 + sb_);
+  if (s >= slice0 + nsl) return;   // the whole workgroup
+  const int half = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63;
+  const int r = L.rows[s * kSlice + lane];
+  const int rb = r >= 0 ? r : L.rows[s * kSlice];
+  const size_t o = (size_t)col_ * vs;
+  v += o;
+  T br = 0, dr = 1, vo = 0, cr = 0, part;
+  if (half == 0) {
+    if (r >= 0) {
+      br = b[o + r];
+      dr = diag[r];
+      vo = v[r];
+      if (CMODE == 1) cr = c[o + r];
+    }
+    const int eu = U.soff[s];
+    part = sell_dot<CHU, NT, TV, T, T>(U.val + eu + lane, U.col + eu + lane, (U.soff[s + 1] - eu) >> 6,
+                                       v, U.sdoff[s], U.dtab, rb);
+  } else {
+    const int e0 = L.soff[s];
+    part = sell_dot<CHL, NT, TV, T, T>(L.val + e0 + lane, L.col + e0 + lane, (L.soff[s + 1] - e0) >> 6,
+                                       v, L.sdoff[s], L.dtab, rb);
+    lsum[lane] = part;
+  }
+  __syncthreads();
+  if (half == 1 || r < 0) return;
+  const T bn = br - damp * (br + part);
+  b[o + r] = bn;
+  const T t = damp * vo;
+  c[o + r] = CMODE == 0 ? t : cr + t;
+  v[r] = (bn - lsum[lane]) / dr;
+}
+
 // defect after the sweep, b -= A (damp v), with (D+L) v = b on active rows:
 //   active   b <- b - damp (b + U v)
 //   inactive b <- b - damp (A v)      (v_r = 0; U holds the whole row)
@@ -1310,10 +1359,25 @@ template void launch_gs_color<float, float>(hipStream_t, const DSell<float> &, i
                                             const float *, const float *, const uint8_t *,
                                             float *, bool, Batch);
 
+// Colours of at most this many slices run the folded sweep on two
+// wavefronts per slice (launches there are bound by the chain of dependent
+// loads: -0.2 to -0.7 us each on 33^3 and 65^3 levels); wider ones on one
+// (on 4096+ slices the second wavefront costs more than it hides: +0.1 to
+// +1.8 us per launch at 129^3).
+constexpr int kSplitMaxSlices = 1024;
+
 template <int CHU, int CHL, typename TV, typename T, int NTV>
 void fused_launch(hipStream_t st, int nrhs, const DSell<TV> &L, const DSell<TV> &U, int slice0, int nsl,
-                  T *b, const T *diag, T damp, T *v, T *c, int cmode, size_t vs) {
+                  T *b, const T *diag, T damp, T *v, T *c, int cmode, size_t vs, bool split) {
   const Batch bt{nrhs, vs, 1};
+  if (split && nsl <= kSplitMaxSlices) {
+    const dim3 g = grid2(nsl, nrhs);
+    if (cmode == 0)
+      k_gs_color_fused2<CHU, CHL, TV, T, 0, NTV><<<g, 2 * kSlice, 0, st>>>(L, U, slice0, nsl, b, diag, damp, v, c, vs);
+    else
+      k_gs_color_fused2<CHU, CHL, TV, T, 1, NTV><<<g, 2 * kSlice, 0, st>>>(L, U, slice0, nsl, b, diag, damp, v, c, vs);
+    return;
+  }
   GSB_DISPATCH(nsl, if (cmode == 0)
     k_gs_color_fused<CHU, CHL, TV, T, 0, NTV, BSV><<<g, BSV, 0, st>>>(L, U, slice0, nsl, b, diag, damp, v, c, vs);
   else
@@ -1323,7 +1387,7 @@ void fused_launch(hipStream_t st, int nrhs, const DSell<TV> &L, const DSell<TV> 
 template <typename TV, typename T>
 void launch_gs_color_fused(hipStream_t st, const DSell<TV> &L, const DSell<TV> &U, int slice0,
                            int nsl, int wmax_l, int wmax_u, T *b, const T *diag, T damp, T *v,
-                           T *c, int cmode, bool nt, Batch bt) {
+                           T *c, int cmode, bool nt, Batch bt, bool split) {
   if (nsl <= 0) return;
   if (bt.b == 3) {
     GSB_DISPATCH(nsl, NT_DISPATCH(nt, if (cmode == 0) k_gs_color_fused_b3<TV, T, 0, NTV, BSV><<<g, BSV, 0, st>>>(
@@ -1337,7 +1401,7 @@ void launch_gs_color_fused(hipStream_t st, const DSell<TV> &L, const DSell<TV> &
   // narrow): pairs with both parts wider than 8 fall back to 16 x 16
   if constexpr (!std::is_same_v<T, double>) {
     NT_DISPATCH(nt, (fused_launch<8, 8, TV, T, NTV>(st, bt.nrhs, L, U, slice0, nsl, b, diag, damp, v, c, cmode,
-                                                    bt.vs)));
+                                                    bt.vs, split)));
   } else {
     int cu = gs_chunk<T>(wmax_u), cl = gs_chunk<T>(wmax_l);
     if (cu > 8 && cl > 8) cu = cl = 16;
@@ -1345,7 +1409,7 @@ void launch_gs_color_fused(hipStream_t st, const DSell<TV> &L, const DSell<TV> &
 #define FUSED_CASE(A, B)                                                                      \
   case A * 100 + B:                                                                           \
     NT_DISPATCH(nt, (fused_launch<A, B, TV, T, NTV>(st, bt.nrhs, L, U, slice0, nsl, b, diag, damp, v, c, \
-                                                    cmode, bt.vs)));                           \
+                                                    cmode, bt.vs, split)));                           \
     break;
     switch (code) {
       FUSED_CASE(28, 8)
@@ -1357,7 +1421,7 @@ void launch_gs_color_fused(hipStream_t st, const DSell<TV> &L, const DSell<TV> &
       FUSED_CASE(16, 16)
       default:
         NT_DISPATCH(nt, (fused_launch<8, 8, TV, T, NTV>(st, bt.nrhs, L, U, slice0, nsl, b, diag, damp, v, c, cmode,
-                                                        bt.vs)));
+                                                        bt.vs, split)));
     }
 #undef FUSED_CASE
   }
@@ -1365,11 +1429,11 @@ void launch_gs_color_fused(hipStream_t st, const DSell<TV> &L, const DSell<TV> &
 template void launch_gs_color_fused<double, double>(hipStream_t, const DSell<double> &,
                                                     const DSell<double> &, int, int, int, int,
                                                     double *, const double *, double, double *,
-                                                    double *, int, bool, Batch);
+                                                    double *, int, bool, Batch, bool);
 template void launch_gs_color_fused<float, float>(hipStream_t, const DSell<float> &,
                                                   const DSell<float> &, int, int, int, int, float *,
                                                   const float *, float, float *, float *, int,
-                                                  bool, Batch);
+                                                  bool, Batch, bool);
 
 template <typename TV, typename T>
 void launch_gs_defect(hipStream_t st, const DSell<TV> &U, const T *v, T *b, const uint8_t *vclass,

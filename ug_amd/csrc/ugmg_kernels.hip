@@ -1489,10 +1489,33 @@ template int launch_restrict<float>(hipStream_t, const DSell<float> &, const uin
                                     const uint8_t *, const float *, float *);
 
 // --------------------------------------------------- base solve (GEMV)
-// ls over ex on the base level, one wavefront per row of the dense inverse:
+// A quarter of a row of the inverse times b: columns j0 + 256 m (j0 = 64 q +
+// lane for wavefront q of the row's workgroup), four streams in flight
+template <typename T, typename TI>
+__device__ __forceinline__ double base_row_quarter(const TI *__restrict__ rr,
+                                                   const T *__restrict__ bb,
+                                                   const int *__restrict__ cp, int ncols, int j0) {
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int j = j0;
+  for (; j + 768 < ncols; j += 1024) {
+    a0 += (double)ld<1>(rr + j) * (double)bb[cp[j]];
+    a1 += (double)ld<1>(rr + j + 256) * (double)bb[cp[j + 256]];
+    a2 += (double)ld<1>(rr + j + 512) * (double)bb[cp[j + 512]];
+    a3 += (double)ld<1>(rr + j + 768) * (double)bb[cp[j + 768]];
+  }
+  for (; j < ncols; j += 256) a0 += (double)ld<1>(rr + j) * (double)bb[cp[j]];
+  return (a0 + a1) + (a2 + a3);
+}
+
+
+// ls over ex on a scalar base level:
 //   cur = |b| from the block partials of the defect norm;
 //   converged: cur < abslimit (it == 0), cur < abslimit or cur < red*first;
 //   x = converged ? 0 : damp * inv b;  c (+)= x;  xout = x.
+// One workgroup (4 wavefronts) per row, each wavefront a quarter of the
+// row's columns (one wavefront per row kept too few loads in flight: <= 3.2
+// TB/s on a 3375 x 3375 inverse); wavefront 0 also reduces the stop test's
+// norm, then adds the four shares and writes.
 template <typename T, typename TI>
 __global__ __launch_bounds__(kBlock) void k_base_gemv(int n, const TI *__restrict__ inv,
                                                        const int *__restrict__ rowmap,
@@ -1503,17 +1526,27 @@ __global__ __launch_bounds__(kBlock) void k_base_gemv(int n, const TI *__restric
                                                        double red, const double *__restrict__ part,
                                                        int nparts, double *__restrict__ first,
                                                        int it, size_t vs) {
-  const int row = (blockIdx.x * kBlock + threadIdx.x) >> 6;
+  __shared__ double wsum[kBlock / 64];
+  const int row = blockIdx.x;
+  const int q = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
   const int cj = blockIdx.y;
   const size_t o = (size_t)cj * vs;
   part += (size_t)cj * nparts;
-  // the row product is formed while the stop test's norm is reduced (and
-  // dropped if the test says converged)
-  double x = row < n ? base_row_or_unit<T, TI>(inv, rowmap, cp, ncols, row, b + o, n, lane) : 0.0;
+  const int k = rowmap[row];
+  double x;
+  if (k < 0) x = (q == 0 && lane == 0) ? (double)b[o + row] : 0.0;   // unit row
+  else x = base_row_quarter<T, TI>(inv + (size_t)k * n, b + o, cp, ncols, 64 * q + lane);
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+  if (lane == 0) wsum[q] = x;
+  // the stop test's norm, reduced while the other wavefronts finish
   double s = 0.0;
-  for (int i = lane; i < nparts; i += 64) s += part[i];
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (q == 0) {
+    for (int i = lane; i < nparts; i += 64) s += part[i];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  }
+  __syncthreads();
+  if (q != 0) return;
   const double cur = sqrt(s);
   bool conv;
   if (it == 0) {
@@ -1524,13 +1557,7 @@ __global__ __launch_bounds__(kBlock) void k_base_gemv(int n, const TI *__restric
     if (target == 0.0) target = red;
     conv = fabs(cur) < fabs(abslimit) || fabs(cur) < fabs(target);
   }
-  if (row >= n) return;
-  if (conv) {
-    x = 0.0;
-  } else {
-    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-    x *= damp;
-  }
+  x = conv ? 0.0 : damp * ((wsum[0] + wsum[1]) + (wsum[2] + wsum[3]));
   if (lane == 0) {
     const T xv = (T)x;
     if (!conv || cmode == 0) c[o + row] = cmode ? c[o + row] + xv : xv;
@@ -1543,15 +1570,17 @@ void launch_base_gemv(hipStream_t st, int n, const TI *inv, const int *rowmap, c
                       int ncols, const T *b, T *c, T *xout, int cmode, double damp,
                       double abslimit, double red, const double *part, int nparts, double *first,
                       int it, Batch bt) {
-  const int g = (n * 64 + kBlock - 1) / kBlock;
-  if (bt.b == 3)   // n counts scalar unknowns
+  if (n <= 0) return;
+  if (bt.b == 3) {   // n counts scalar unknowns; one wavefront per row
+    const int g = (n * 64 + kBlock - 1) / kBlock;
     k_base_gemv_b3<T, TI><<<grid2(g, bt.nrhs), kBlock, 0, st>>>(n, inv, rowmap, cp, ncols, b, c,
                                                                 xout, cmode, damp, abslimit, red,
                                                                 part, nparts, first, it, bt.vs);
-  else
-    k_base_gemv<T, TI><<<grid2(g, bt.nrhs), kBlock, 0, st>>>(n, inv, rowmap, cp, ncols, b, c, xout,
+  } else {           // one workgroup per row
+    k_base_gemv<T, TI><<<grid2(n, bt.nrhs), kBlock, 0, st>>>(n, inv, rowmap, cp, ncols, b, c, xout,
                                                              cmode, damp, abslimit, red, part,
                                                              nparts, first, it, bt.vs);
+  }
 }
 template void launch_base_gemv<double, double>(hipStream_t, int, const double *, const int *,
                                                const int *, int, const double *, double *,

@@ -127,6 +127,9 @@ struct ugmg_ctx {
   // unit row with VNCLASS >= 2), whose columns a cycle's base solve skips
   int *base_colperm = nullptr;
   int base_ncyc = 0;
+  // position of each base unknown among the first base_ncyc columns (-1:
+  // none), for the restriction that writes b in column order
+  int *base_cpos = nullptr;
   template <typename T> const T *base_cinv_t() const {
     if constexpr (std::is_same_v<T, double>) return base_cinv; else return base_cinv32;
   }
@@ -255,6 +258,7 @@ void free_device(ugmg_ctx *ctx) {
   ctx->base_rowmap = nullptr;
   ctx->base_colperm = nullptr;
   ctx->base_ncyc = 0;
+  ctx->base_cpos = nullptr;
   ctx->clev64 = nullptr;
   ctx->clev32 = nullptr;
   ctx->fuse_top = -1;
@@ -408,7 +412,7 @@ void coarse_cycle(ugmg_ctx *ctx, int top, T *c, T *b, int cmode) {
 // skipped Dirichlet unknowns, and b -= A x keeps it so)
 template <typename T>
 void base_solve(ugmg_ctx *ctx, T *c, T *b, int cmode, bool have_parts, bool need_b,
-                bool restricted = false) {
+                bool restricted = false, const T *bc = nullptr) {
   DevLevel &d = ctx->dev[ctx->bl];
   hipStream_t st = ctx->stream;
   const ugmg_params &p = ctx->p;
@@ -421,7 +425,8 @@ void base_solve(ugmg_ctx *ctx, T *c, T *b, int cmode, bool have_parts, bool need
   for (int it = 0; it < m; ++it) {
     if (!ctx->base.failed)
       launch_base_gemv<T, T>(st, d.n * d.nb, ctx->base_cinv_t<T>(), ctx->base_rowmap,
-                             ctx->base_colperm, restricted ? ctx->base_ncyc : d.n * d.nb, b, c, x,
+                             ctx->base_colperm, restricted ? ctx->base_ncyc : d.n * d.nb, b,
+                             it == 0 ? bc : nullptr, c, x,
                              it == 0 ? cmode : 1, p.base_damp[0],
                           p.base_abslimit[0], p.base_red[0], ctx->base_part, np,
                           ctx->base_first, it, bt);
@@ -457,10 +462,14 @@ void lmgc(ugmg_ctx *ctx, int l, T *c, T *b, int cmode, bool c0_done = false) {
   const bool to_base = l - 1 == ctx->bl && (ctx->fuse_top < ctx->bl || nb > 1);
   const bool pre0 = l - 1 > ctx->bl && l - 1 > ctx->fuse_top && p.gamma >= 1 &&
                     fold_colour0(ctx, l - 1, p.nu1);
+  // onto the base level (scalar) also b in the inverse's column order, into
+  // the base level's spare vector u, for the first base iteration
+  const bool bcol = to_base && nb == 1 && ctx->base_cpos && !ctx->base.failed;
   const int np = launch_restrict<T>(st, d.PT.d<T>(), dc.skip, dc.vnclass, dc.vclass, b, (T)1,
                                     dc.bv<T>(), to_base ? ctx->base_part : nullptr, d.ntv<T>(), nb,
                                     d.vs, dc.vs, d.nb, pre0 ? dc.c0 : nullptr, dc.diag<T>(),
-                                    pre0 ? dc.tv<T>() : nullptr);
+                                    pre0 ? dc.tv<T>() : nullptr, bcol ? ctx->base_cpos : nullptr,
+                                    dc.uv<T>());
   if (p.gamma <= 0) launch_fill<T>(st, dc.n, dc.cv<T>(), (T)0, batch(ctx, l - 1));
   for (int g = 0; g < p.gamma; ++g) {
     if (to_base) {
@@ -468,7 +477,8 @@ void lmgc(ugmg_ctx *ctx, int l, T *c, T *b, int cmode, bool c0_done = false) {
       // base solve b has changed, so later visits recompute them
       if (g == 0) ctx->base_nparts = np;
       // a later visit (gamma > 1) reads the base defect the earlier one leaves
-      base_solve<T>(ctx, dc.cv<T>(), dc.bv<T>(), g == 0 ? 0 : 1, g == 0, g + 1 < p.gamma, true);
+      base_solve<T>(ctx, dc.cv<T>(), dc.bv<T>(), g == 0 ? 0 : 1, g == 0, g + 1 < p.gamma, true,
+                    g == 0 && bcol ? dc.uv<T>() : nullptr);
     } else {
       lmgc<T>(ctx, l - 1, dc.cv<T>(), dc.bv<T>(), g == 0 ? 0 : 1, g == 0 && pre0);
     }
@@ -937,6 +947,9 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
           }
         const int nact = (int)act.size();
         ctx->base_ncyc = (int)keep.size();
+        std::vector<int> cpos(nsc, -1);
+        for (int j = 0; j < ctx->base_ncyc; ++j) cpos[keep[j]] = j;
+        if ((rc = upload(ctx, &ctx->base_cpos, cpos.data(), cpos.size()))) return rc;
         keep.insert(keep.end(), drop.begin(), drop.end());
         const size_t ne = std::max<size_t>(1, (size_t)nact * nsc);
         if (act.empty()) act.push_back(0);   // (nothing to copy; keeps the upload non-empty)

@@ -127,12 +127,15 @@ void launch_gs_defect(hipStream_t st, const DSell<TV> &U, const T *v, T *b, cons
 // returns the number of blocks per column (= partial sums per column when
 // part != null; column j's partials start at part + j * return value).
 // With ct (scalar only) also ct_r = bc_r / cdiag_r on the coarse rows whose
-// cc0 flag is set: the coarse GS pre-smoother's colour-0 sweep.
+// cc0 flag is set: the coarse GS pre-smoother's colour-0 sweep.  With cpos
+// (scalar only) also cbc[cpos_r] = bc_r where cpos_r >= 0: b in the base
+// inverse's column order.
 template <typename T>
 int launch_restrict(hipStream_t st, const DSell<T> &PT, const uint32_t *cskip,
                     const uint8_t *cvnclass, const uint8_t *cvclass, const T *bf, T damp, T *bc,
                     double *part, bool nt, int nrhs, size_t vsf, size_t vsc, int b,
-                    const uint8_t *cc0 = nullptr, const T *cdiag = nullptr, T *ct = nullptr);
+                    const uint8_t *cc0 = nullptr, const T *cdiag = nullptr, T *ct = nullptr,
+                    const int *cpos = nullptr, T *cbc = nullptr);
 // Element lists of one level on the device (all pointers device memory):
 // nelem elements of nc corners, elem[e*nc + i] = vector of corner i; element
 // matrices (nc b) x (nc b) row-major over corner-major scalar unknowns,
@@ -163,10 +166,11 @@ void launch_base_factor(hipStream_t st, double *band, int n, int bw, const int *
 // inv: the rows of the dense inverse that are not unit rows (rowmap[r]: the
 // row's index there, -1 for a unit row), columns in the order cp, in FP64
 // (TI = double) or the FP32 copy for the FP32 cycle (TI = float); the
-// products run over the first ncols columns and are summed in FP64
+// products run over the first ncols columns and are summed in FP64.  bc
+// (scalar rows; may be null): b already in column order, bc_j = b[cp[j]]
 template <typename T, typename TI>
 void launch_base_gemv(hipStream_t st, int n, const TI *inv, const int *rowmap, const int *cp,
-                      int ncols, const T *b, T *c, T *xout, int cmode, double damp,
+                      int ncols, const T *b, const T *bc, T *c, T *xout, int cmode, double damp,
                       double abslimit, double red, const double *part, int nparts, double *first,
                       int it, Batch bt);
 // rows act[0..nact), columns in the order cp, of the n x n inverse into

@@ -512,7 +512,9 @@ __global__ __launch_bounds__(kBlock) void k_restrict(DSell<T> PT, int nsl,
                                                       size_t vsc,
                                                       const uint8_t *__restrict__ cc0,
                                                       const T *__restrict__ cdiag,
-                                                      T *__restrict__ ct) {
+                                                      T *__restrict__ ct,
+                                                      const int *__restrict__ cpos,
+                                                      T *__restrict__ cbc) {
   __shared__ double red[kBlock / 64];
   SLICE_PROLOGUE_RED(PT, nsl)
   bf += (size_t)col_ * vsf;
@@ -527,6 +529,11 @@ __global__ __launch_bounds__(kBlock) void k_restrict(DSell<T> PT, int nsl,
       // the coarse pre-smoother's colour-0 sweep: no lower neighbours, so
       // its value is (b_r - 0) / a_rr
       if (ct && cc0[r]) ct[(size_t)col_ * vsc + r] = v / cdiag[r];
+      // b in the base inverse's column order
+      if (cpos) {
+        const int pj = cpos[r];
+        if (pj >= 0) cbc[(size_t)col_ * vsc + pj] = v;
+      }
       if (part && cvclass[r] >= 2) sq = (double)v * (double)v;
     }
   }
@@ -1466,7 +1473,7 @@ template <typename T>
 int launch_restrict(hipStream_t st, const DSell<T> &PT, const uint32_t *cskip,
                     const uint8_t *cvnclass, const uint8_t *cvclass, const T *bf, T damp, T *bc,
                     double *part, bool nt, int nrhs, size_t vsf, size_t vsc, int b,
-                    const uint8_t *cc0, const T *cdiag, T *ct) {
+                    const uint8_t *cc0, const T *cdiag, T *ct, const int *cpos, T *cbc) {
   if (PT.nslices <= 0) return 0;
   const int g = nblk_slices(PT.nslices);
   if (b == 3) {
@@ -1476,34 +1483,37 @@ int launch_restrict(hipStream_t st, const DSell<T> &PT, const uint32_t *cskip,
   }
   W_DISPATCH(PT.wmax, NT_DISPATCH(nt, k_restrict<MWV, T, NTV><<<grid2(g, nrhs), kBlock, 0, st>>>(
                                           PT, PT.nslices, cskip, cvnclass, cvclass, bf, damp, bc,
-                                          part, vsf, vsc, cc0, cdiag, ct)));
+                                          part, vsf, vsc, cc0, cdiag, ct, cpos, cbc)));
   return g;
 }
 template int launch_restrict<double>(hipStream_t, const DSell<double> &, const uint32_t *,
                                      const uint8_t *, const uint8_t *, const double *, double,
                                      double *, double *, bool, int, size_t, size_t, int,
-                                     const uint8_t *, const double *, double *);
+                                     const uint8_t *, const double *, double *, const int *,
+                                     double *);
 template int launch_restrict<float>(hipStream_t, const DSell<float> &, const uint32_t *,
                                     const uint8_t *, const uint8_t *, const float *, float,
                                     float *, double *, bool, int, size_t, size_t, int,
-                                    const uint8_t *, const float *, float *);
+                                    const uint8_t *, const float *, float *, const int *, float *);
 
 // --------------------------------------------------- base solve (GEMV)
 // A quarter of a row of the inverse times b: columns j0 + 256 m (j0 = 64 q +
-// lane for wavefront q of the row's workgroup), four streams in flight
-template <typename T, typename TI>
+// lane for wavefront q of the row's workgroup), four streams in flight; b
+// read as bb[cp[j]], or (COL) as bc[j] when it is already in column order
+template <bool COL, typename T, typename TI>
 __device__ __forceinline__ double base_row_quarter(const TI *__restrict__ rr,
                                                    const T *__restrict__ bb,
                                                    const int *__restrict__ cp, int ncols, int j0) {
+  auto bj = [&](int j) { return COL ? (double)bb[j] : (double)bb[cp[j]]; };
   double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
   int j = j0;
   for (; j + 768 < ncols; j += 1024) {
-    a0 += (double)ld<1>(rr + j) * (double)bb[cp[j]];
-    a1 += (double)ld<1>(rr + j + 256) * (double)bb[cp[j + 256]];
-    a2 += (double)ld<1>(rr + j + 512) * (double)bb[cp[j + 512]];
-    a3 += (double)ld<1>(rr + j + 768) * (double)bb[cp[j + 768]];
+    a0 += (double)ld<1>(rr + j) * bj(j);
+    a1 += (double)ld<1>(rr + j + 256) * bj(j + 256);
+    a2 += (double)ld<1>(rr + j + 512) * bj(j + 512);
+    a3 += (double)ld<1>(rr + j + 768) * bj(j + 768);
   }
-  for (; j < ncols; j += 256) a0 += (double)ld<1>(rr + j) * (double)bb[cp[j]];
+  for (; j < ncols; j += 256) a0 += (double)ld<1>(rr + j) * bj(j);
   return (a0 + a1) + (a2 + a3);
 }
 
@@ -1521,6 +1531,7 @@ __global__ __launch_bounds__(kBlock) void k_base_gemv(int n, const TI *__restric
                                                        const int *__restrict__ rowmap,
                                                        const int *__restrict__ cp, int ncols,
                                                        const T *__restrict__ b,
+                                                       const T *__restrict__ bc,
                                                        T *__restrict__ c, T *__restrict__ xout,
                                                        int cmode, double damp, double abslimit,
                                                        double red, const double *__restrict__ part,
@@ -1536,7 +1547,8 @@ __global__ __launch_bounds__(kBlock) void k_base_gemv(int n, const TI *__restric
   const int k = rowmap[row];
   double x;
   if (k < 0) x = (q == 0 && lane == 0) ? (double)b[o + row] : 0.0;   // unit row
-  else x = base_row_quarter<T, TI>(inv + (size_t)k * n, b + o, cp, ncols, 64 * q + lane);
+  else if (bc) x = base_row_quarter<true, T, TI>(inv + (size_t)k * n, bc + o, cp, ncols, 64 * q + lane);
+  else x = base_row_quarter<false, T, TI>(inv + (size_t)k * n, b + o, cp, ncols, 64 * q + lane);
   for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
   if (lane == 0) wsum[q] = x;
   // the stop test's norm, reduced while the other wavefronts finish
@@ -1567,7 +1579,7 @@ __global__ __launch_bounds__(kBlock) void k_base_gemv(int n, const TI *__restric
 
 template <typename T, typename TI>
 void launch_base_gemv(hipStream_t st, int n, const TI *inv, const int *rowmap, const int *cp,
-                      int ncols, const T *b, T *c, T *xout, int cmode, double damp,
+                      int ncols, const T *b, const T *bc, T *c, T *xout, int cmode, double damp,
                       double abslimit, double red, const double *part, int nparts, double *first,
                       int it, Batch bt) {
   if (n <= 0) return;
@@ -1577,17 +1589,17 @@ void launch_base_gemv(hipStream_t st, int n, const TI *inv, const int *rowmap, c
                                                                 xout, cmode, damp, abslimit, red,
                                                                 part, nparts, first, it, bt.vs);
   } else {           // one workgroup per row
-    k_base_gemv<T, TI><<<grid2(n, bt.nrhs), kBlock, 0, st>>>(n, inv, rowmap, cp, ncols, b, c, xout,
+    k_base_gemv<T, TI><<<grid2(n, bt.nrhs), kBlock, 0, st>>>(n, inv, rowmap, cp, ncols, b, bc, c, xout,
                                                              cmode, damp, abslimit, red, part,
                                                              nparts, first, it, bt.vs);
   }
 }
 template void launch_base_gemv<double, double>(hipStream_t, int, const double *, const int *,
-                                               const int *, int, const double *, double *,
+                                               const int *, int, const double *, const double *, double *,
                                                double *, int, double, double, double,
                                                const double *, int, double *, int, Batch);
 template void launch_base_gemv<float, float>(hipStream_t, int, const float *, const int *,
-                                             const int *, int, const float *, float *, float *, int,
+                                             const int *, int, const float *, const float *, float *, float *, int,
                                              double, double, double, const double *, int, double *,
                                              int, Batch);
 

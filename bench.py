@@ -322,8 +322,9 @@ def main():
                          "then a copy kernel of known size")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the sharded form is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: the solver runs on one GPU; --gpus {args.gpus} is not supported "
+              "(levels would be partitioned by coordinate bisection with halo exchanges; "
+              "that sharded form is designed, not built)", file=sys.stderr)
         return 2
 
     import numpy as np

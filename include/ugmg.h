@@ -71,6 +71,10 @@ typedef struct {
                                        inner_cycles FP32 cycles when the rate
                                        of the previous steps predicts that
                                        fewer reach the stop test (default 1) */
+  int base_inv32;                   /* MIXED: the FP32 cycle's base solve reads
+                                       an FP32 copy of the dense coarse inverse
+                                       (default 0: the FP64 inverse, sums in
+                                       FP64)                                   */
 } ugmg_params;
 
 /* LRESULT (np/procs/ls.h:71-77) */
@@ -169,6 +173,13 @@ int ugmg_solve_host(ugmg_ctx *ctx, int level, double *x, double *b,
                     const double *abslimit, const double *reduction,
                     int maxiter, int do_defect, ugmg_lresult *res,
                     ugmg_defect_cb cb, void *user);
+
+/* The last ugmg_solve on this context: V-cycles it ran (FP32 cycles in the
+ * mixed form) and the algorithmic HBM bytes of every kernel it launched, from
+ * the same layout-true byte model as ugmg_time_kernel (every value slot of a
+ * slab, column indices of explicit slices only, each vector once per pass).
+ * No counterpart in UG; for rooflines of whole solves. */
+int ugmg_solve_stats(const ugmg_ctx *ctx, long long *cycles, double *bytes);
 
 /* ---- NP_LINEAR_SOLVER on a locally refined hierarchy --------------------
  * FULLREFINELEVEL as SetSurfaceClasses (gm/algebra.cc:2885-2916) leaves it:
@@ -329,10 +340,13 @@ int ugmg_server_result(ugmg_server *s, int id, double *x, int n);
  *       3 gs step FP64, 4 restrict, 5 interpolate, 6 one cycle (current
  *       precision), 7 gs step FP32, 8 gs step FP64 as a cycle runs the
  *       steps after the first (8 colour sweeps with the previous step's
- *       defect folded in; scalar levels whose rows are all active; the
- *       bytes of kind 3).  Runs `reps` back-to-back launches on
- *       the context stream between two hipEvents and returns the mean
- *       duration in ms, plus the algorithmic bytes of one launch. */
+ *       defect folded in; scalar levels whose rows are all active).  Runs
+ *       `reps` back-to-back launches on the context stream between two
+ *       hipEvents and returns the mean duration in ms, plus the algorithmic
+ *       bytes of one launch as the data are laid out: every value slot of
+ *       the slabs (padding and the zero phantoms of uniform slices
+ *       included), column indices of explicit slices only, the slot-to-row
+ *       maps, and each vector once per pass (twice when read and written). */
 int ugmg_time_kernel(ugmg_ctx *ctx, int level, int kind, int reps,
                      double *ms_per_launch, double *bytes_per_launch);
 

@@ -307,6 +307,35 @@ def test_mixed_precision_reaches_fp64_accuracy(jump3d_33, smoother):
     ctx.close()
 
 
+def test_mixed_coarse_inverse_precision(jump3d_33):
+    """The FP32 cycle's base solve reads the FP64 dense inverse by default
+    (the coarsest-grid solve kept in FP64); its FP32 copy (base_inv32) gives
+    the same outer iterations and FP32 cycle count and the same final
+    accuracy against a 1e-13 FP64 oracle solve."""
+    from ug_amd import ugmg
+    P = jump3d_33
+    H = O.Hierarchy(P.levels)
+    ref = O.ls_solve(H, P.top, O.params(smoother=O.GS, damp=1.0, baselevel=1), P.rhs, red=1e-13, maxiter=400)
+    n = P.levels[-1].n
+    out = {}
+    for inv32 in (0, 1):
+        ctx = _ctx(P, ugmg.SMOOTH_GS, 1, 1.0, precision=ugmg.MIXED, inner_cycles=7, use_graph=1,
+                   base_inv32=inv32)
+        x, b = _full(n), _dev(P.rhs)
+        res, _ = ctx.solve(P.top, x, b, red=1e-10, maxiter=200)
+        cycles, by = ctx.solve_stats()
+        assert res.converged and by > 0
+        xs = _host(x)
+        r = O.dmatmul_minus(H, P.top, P.rhs, xs)
+        out[inv32] = (res.number_of_linear_iterations, cycles, np.linalg.norm(r) / np.linalg.norm(P.rhs),
+                      np.linalg.norm(xs - ref["x"]) / np.linalg.norm(ref["x"]))
+        ctx.close()
+    print(f"FP64 inverse {out[0]}, FP32 copy {out[1]} (outer its, FP32 cycles, true defect, error)")
+    assert out[0][:2] == out[1][:2]
+    for k in (0, 1):
+        assert out[k][2] < 2e-10 and out[k][3] < 1e-8
+
+
 def test_host_entry_points(poisson2d):
     from ug_amd import ugmg
     ctx = _ctx(poisson2d, ugmg.SMOOTH_JAC, 0, 0.8)

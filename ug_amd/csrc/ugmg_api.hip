@@ -37,11 +37,30 @@ struct DevSellBuf {
   int nslices = 0;
   size_t nent = 0;
   int nuniform = 0;
+  size_t nent_explicit = 0;     // entries of slices that read column indices
+  size_t ndtab = 0;             // shared offset-list length
+  int bb = 1;                   // value planes (b*b)
   int *soff = nullptr, *rows = nullptr, *col = nullptr, *sdoff = nullptr, *dtab = nullptr;
   double *v64 = nullptr;
   float *v32 = nullptr;
   int wmax = 0;                 // widest slice
   std::vector<int> width_h;     // per-slice width (host)
+  std::vector<char> expl_h;     // per-slice: explicit column indices (host)
+  // Bytes one pass over slices [s0, s1) reads as laid out: every value slot
+  // (padding lanes and zero phantoms of uniform slices included, sv bytes
+  // per value), column indices only of explicit slices, the slot->row map
+  // and soff/sdoff per slice.  Vectors not included.
+  double bytes_range(int s0, int s1, int sv) const {
+    double e = 0.0, ex = 0.0;
+    for (int s = s0; s < s1; ++s) {
+      const double w = 64.0 * width_h[s];
+      e += w;
+      if (expl_h[s]) ex += w;
+    }
+    return e * bb * sv + 4.0 * ex + (4.0 * 64.0 + 8.0) * (s1 - s0);
+  }
+  // the whole slab, with its shared offset lists
+  double bytes(int sv) const { return bytes_range(0, nslices, sv) + 4.0 * (double)ndtab; }
   int max_width(int s0, int s1) const {
     int m = 0;
     for (int s = s0; s < s1; ++s) m = std::max(m, width_h[s]);
@@ -61,6 +80,7 @@ struct DevLevel {
   DevSellBuf A, L, U;
   int ncolors = 0;
   std::vector<int> color_slices, color, color_wmax, color_wmax_u;
+  std::vector<int> color_rows;   // vectors per colour
   bool all_active = false;   // every vector has VCLASS >= 3
   double *diag64 = nullptr;
   float *diag32 = nullptr;
@@ -127,12 +147,14 @@ struct ugmg_ctx {
   // unit row with VNCLASS >= 2), whose columns a cycle's base solve skips
   int *base_colperm = nullptr;
   int base_ncyc = 0;
+  int base_nact = 0;   // rows of the compact inverse
+  // the last ugmg_solve: V-cycles launched (FP32 cycles in the mixed form)
+  // and the algorithmic bytes of every kernel it launched
+  long long stat_cycles = 0;
+  double stat_bytes = 0.0;
   // position of each base unknown among the first base_ncyc columns (-1:
   // none), for the restriction that writes b in column order
   int *base_cpos = nullptr;
-  template <typename T> const T *base_cinv_t() const {
-    if constexpr (std::is_same_v<T, double>) return base_cinv; else return base_cinv32;
-  }
   double *part = nullptr;
   int part_cap = 0;
   double *red = nullptr;      // device scalars
@@ -223,7 +245,17 @@ int upload_sell(ugmg_ctx *ctx, const HostSell &S, DevSellBuf &D) {
   if ((rc = upload(ctx, &D.sdoff, S.sdoff.data(), S.sdoff.size()))) return rc;
   if ((rc = upload(ctx, &D.dtab, S.dtab.data(), S.dtab.size()))) return rc;
   D.nuniform = 0;
-  for (int o : S.sdoff) D.nuniform += o >= 0;
+  D.nent_explicit = 0;
+  D.expl_h.assign(S.nslices, 0);
+  for (int q = 0; q < S.nslices; ++q) {
+    D.nuniform += S.sdoff[q] >= 0;
+    if (S.sdoff[q] < 0) {
+      D.nent_explicit += (size_t)(S.soff[q + 1] - S.soff[q]);
+      D.expl_h[q] = 1;
+    }
+  }
+  D.ndtab = S.dtab.size();
+  D.bb = S.bb;
   if ((rc = upload(ctx, &D.v64, S.val.data(), S.val.size()))) return rc;
   hvec<float> v32;
   to_float(S.val.data(), S.val.size(), v32);
@@ -258,6 +290,7 @@ void free_device(ugmg_ctx *ctx) {
   ctx->base_rowmap = nullptr;
   ctx->base_colperm = nullptr;
   ctx->base_ncyc = 0;
+  ctx->base_nact = 0;
   ctx->base_cpos = nullptr;
   ctx->clev64 = nullptr;
   ctx->clev32 = nullptr;
@@ -422,14 +455,23 @@ void base_solve(ugmg_ctx *ctx, T *c, T *b, int cmode, bool have_parts, bool need
   T *x = d.tv<T>();
   const int m = std::max(0, p.base_maxiter);
   if (m == 0 && cmode == 0) launch_fill<T>(st, d.n, c, (T)0, bt);
+  // the FP32 cycle reads the FP64 inverse (sums in FP64) unless base_inv32
+  // asks for its FP32 copy
+  const bool inv64 = std::is_same_v<T, double> || !p.base_inv32;
   for (int it = 0; it < m; ++it) {
-    if (!ctx->base.failed)
-      launch_base_gemv<T, T>(st, d.n * d.nb, ctx->base_cinv_t<T>(), ctx->base_rowmap,
-                             ctx->base_colperm, restricted ? ctx->base_ncyc : d.n * d.nb, b,
-                             it == 0 ? bc : nullptr, c, x,
-                             it == 0 ? cmode : 1, p.base_damp[0],
-                          p.base_abslimit[0], p.base_red[0], ctx->base_part, np,
-                          ctx->base_first, it, bt);
+    if (!ctx->base.failed) {
+      const int ncols = restricted ? ctx->base_ncyc : d.n * d.nb;
+      if (inv64)
+        launch_base_gemv<T, double>(st, d.n * d.nb, ctx->base_cinv, ctx->base_rowmap,
+                                    ctx->base_colperm, ncols, b, it == 0 ? bc : nullptr, c, x,
+                                    it == 0 ? cmode : 1, p.base_damp[0], p.base_abslimit[0],
+                                    p.base_red[0], ctx->base_part, np, ctx->base_first, it, bt);
+      else
+        launch_base_gemv<T, float>(st, d.n * d.nb, ctx->base_cinv32, ctx->base_rowmap,
+                                   ctx->base_colperm, ncols, b, it == 0 ? bc : nullptr, c, x,
+                                   it == 0 ? cmode : 1, p.base_damp[0], p.base_abslimit[0],
+                                   p.base_red[0], ctx->base_part, np, ctx->base_first, it, bt);
+    }
     const bool more = it + 1 < m;
     if (more || need_b || !ctx->base_b_dead)
       launch_spmv<T, T, T>(st, d.A.d<T>(), 0, d.A.nslices, x, b, (T *)nullptr, 0, d.ntv<T>(), bt);
@@ -578,8 +620,136 @@ int run_iteration(ugmg_ctx *ctx, int level, double *x, double *b, int cycles) {
 
 inline bool sc_cmp1(double x, double y) { return std::fabs(x) < std::fabs(y); }
 
-double spmv_bytes(const DevLevel &d, int sv, int sx) {
-  return (double)d.nnzb * (sv + 4) + 4.0 * (d.n + 1) + (double)d.n * (sx + 2.0 * sx);
+// ------------------------------------------------------------ byte model
+// Algorithmic bytes of each pass as the data are laid out, the figures the
+// roofline fractions divide by: slab bytes (DevSellBuf::bytes), every vector
+// a pass touches counted once per pass (a gathered vector once, one read
+// and written twice), flags one byte per vector.  sv = bytes per value and
+// vector entry (8 FP64, 4 FP32).
+enum class Op { Spmv, SpmvC, SpmvC0, Jac, Restrict, Interp };
+
+double vbytes(const DevLevel &d, int sv) { return (double)d.n * d.nb * sv; }
+
+double op_bytes(const ugmg_ctx *ctx, int l, Op op, int sv) {
+  const DevLevel &d = ctx->dev[l];
+  const double v = vbytes(d, sv);
+  switch (op) {
+    case Op::Spmv: return d.A.bytes(sv) + 3.0 * v;    // x gathered, y read and written
+    case Op::SpmvC: return d.A.bytes(sv) + 5.0 * v;   // and c += x
+    case Op::SpmvC0: {                                // colour-0 rows: diagonal read, u written
+      const double r0 = d.color_rows.empty() ? 0.0 : d.color_rows[0];
+      return d.A.bytes(sv) + 3.0 * v + r0 * d.nb * sv * (d.nb + 1);
+    }
+    case Op::Jac: return (double)d.n * (d.nb * sv * (2.0 + d.nb) + 1.0);   // b, diagonal block, class, t
+    case Op::Restrict: {                               // b_f gathered, b_c written; skip, VNCLASS, VCLASS
+      const DevLevel &dc = ctx->dev[l - 1];
+      return d.PT.bytes(sv) + v + vbytes(dc, sv) + 6.0 * dc.n;
+    }
+    case Op::Interp: {                                 // c_c gathered, t written, c read and written; skip
+      const DevLevel &dc = ctx->dev[l - 1];
+      return d.P.bytes(sv) + vbytes(dc, sv) + 3.0 * v + 4.0 * d.n;
+    }
+  }
+  return 0.0;
+}
+
+// colour sweeps k0.. of one GS step: L slab of those colours; b, diagonal
+// block and class read and v written per row; v gathered
+double sweep_bytes(const DevLevel &d, int sv, int k0) {
+  if (d.ncolors == 0) return 0.0;
+  double rows = 0.0;
+  for (int k = k0; k < d.ncolors; ++k) rows += d.color_rows[k];
+  return d.L.bytes_range(d.color_slices[k0], d.color_slices[d.ncolors], sv) +
+         rows * (d.nb * sv * (2.0 + d.nb) + 1.0) + vbytes(d, sv);
+}
+// the defect after the sweeps: U slab; b read and written, v, class, c
+// (read and written, or written), v gathered
+double defect_bytes(const DevLevel &d, int sv, bool c_rmw) {
+  return d.U.bytes(sv) + vbytes(d, sv) * (5.0 + (c_rmw ? 1.0 : 0.0)) + d.n;
+}
+// a step after the first with the previous defect folded in: L and U
+// slabs; b read and written, diagonal block, old and new v, c; v gathered
+double fused_bytes(const DevLevel &d, int sv, bool c_rmw) {
+  return d.L.bytes(sv) + d.U.bytes(sv) +
+         (double)d.n * d.nb * sv * (2.0 + d.nb + 3.0 + (c_rmw ? 2.0 : 1.0));
+}
+double gs_step_bytes(const ugmg_ctx *ctx, int l, int sv, bool fused) {
+  const DevLevel &d = ctx->dev[l];
+  return fused ? fused_bytes(d, sv, true) : sweep_bytes(d, sv, 0) + defect_bytes(d, sv, true);
+}
+
+// nsteps smoother steps as smooth() issues them
+double smooth_bytes(const ugmg_ctx *ctx, int l, int sv, int cmode, int nsteps, int k0) {
+  const DevLevel &d = ctx->dev[l];
+  double by = 0.0;
+  if (ctx->p.smoother == UGMG_SMOOTH_JAC) {
+    for (int i = 0; i < nsteps; ++i)
+      by += op_bytes(ctx, l, Op::Jac, sv) + d.A.bytes(sv) + vbytes(d, sv) * ((i == 0 && cmode == 0) ? 4.0 : 5.0);
+    return by;
+  }
+  if (!d.all_active || nsteps < 2 || !ctx->fuse_steps) {
+    for (int i = 0; i < nsteps; ++i)
+      by += sweep_bytes(d, sv, i == 0 ? k0 : 0) + defect_bytes(d, sv, (i == 0 ? cmode : 1) == 1);
+    return by;
+  }
+  by = sweep_bytes(d, sv, k0);
+  for (int i = 1; i < nsteps; ++i) by += fused_bytes(d, sv, (i == 1 ? cmode : 1) == 1);
+  return by + defect_bytes(d, sv, true);
+}
+
+// ls over ex on the base level (m GEMVs over the compact inverse, whose
+// entries are FP64 or the FP32 copy), with the defects between them
+double base_bytes(const ugmg_ctx *ctx, int sv, bool restricted, bool need_b) {
+  const DevLevel &d = ctx->dev[ctx->bl];
+  const int m = std::max(0, ctx->p.base_maxiter);
+  const int nsc = d.n * d.nb;
+  const int si = (sv == 4 && ctx->p.base_inv32) ? 4 : 8;
+  const double ncols = restricted ? ctx->base_ncyc : nsc;
+  double by = 0.0;
+  for (int it = 0; it < m; ++it) {
+    if (!ctx->base.failed)
+      by += (double)ctx->base_nact * ncols * si + ncols * (sv + 4.0) + 4.0 * nsc + 2.0 * nsc * sv;
+    const bool more = it + 1 < m;
+    if (more || need_b || !ctx->base_b_dead) by += op_bytes(ctx, ctx->bl, Op::Spmv, sv);
+    if (more) by += vbytes(d, sv) + d.n;
+  }
+  return by;
+}
+
+// one lmgc cycle as lmgc() issues it (the fused coarse-cycle kernel, an
+// opt-in for tiny hierarchies, is not modelled and counts 0)
+double lmgc_bytes(const ugmg_ctx *ctx, int l, int sv, int cmode, bool c0_done) {
+  const ugmg_params &p = ctx->p;
+  if (l <= ctx->fuse_top) return 0.0;
+  if (l <= ctx->bl) return base_bytes(ctx, sv, false, true) + vbytes(ctx->dev[l], sv) + ctx->dev[l].n;
+  const DevLevel &d = ctx->dev[l], &dc = ctx->dev[l - 1];
+  double by = smooth_bytes(ctx, l, sv, cmode, p.nu1, c0_done ? 1 : 0);
+  if (p.nu1 == 0 && cmode == 0) by += vbytes(d, sv);
+  const bool to_base = l - 1 == ctx->bl && ctx->fuse_top < ctx->bl;
+  const bool pre0 = l - 1 > ctx->bl && l - 1 > ctx->fuse_top && p.gamma >= 1 && fold_colour0(ctx, l - 1, p.nu1);
+  by += op_bytes(ctx, l, Op::Restrict, sv);
+  if (pre0) by += dc.n + (double)dc.color_rows[0] * 2.0 * sv;
+  if (to_base && d.nb == 1 && ctx->base_cpos && !ctx->base.failed) by += 4.0 * dc.n + (double)ctx->base_ncyc * sv;
+  if (p.gamma <= 0) by += vbytes(dc, sv);
+  for (int g = 0; g < p.gamma; ++g)
+    by += to_base ? base_bytes(ctx, sv, true, g + 1 < p.gamma)
+                  : lmgc_bytes(ctx, l - 1, sv, g == 0 ? 0 : 1, g == 0 && pre0);
+  by += op_bytes(ctx, l, Op::Interp, sv);
+  if (fold_colour0(ctx, l, p.nu2)) return by + op_bytes(ctx, l, Op::SpmvC0, sv) + smooth_bytes(ctx, l, sv, 1, p.nu2, 1);
+  return by + op_bytes(ctx, l, Op::Spmv, sv) + smooth_bytes(ctx, l, sv, 1, p.nu2, 0);
+}
+
+double cycle_bytes(const ugmg_ctx *ctx, int l, int sv) { return lmgc_bytes(ctx, l, sv, 0, false); }
+
+// one iteration of the linear solver (enqueue_iteration): the FP64 cycle
+// and the defect norm, or `cycles` FP32 cycles and the FP64 refinement
+// update (r -= A c32, x += c32, r32 = r, norm partials)
+double iteration_bytes(const ugmg_ctx *ctx, int l, int cycles) {
+  const DevLevel &d = ctx->dev[l];
+  if (ctx->p.precision != UGMG_MIXED) return lmgc_bytes(ctx, l, 8, 1, false) + vbytes(d, 8) + d.n;
+  double by = 0.0;
+  for (int j = 0; j < cycles; ++j) by += lmgc_bytes(ctx, l, 4, j == 0 ? 0 : 1, false);
+  return by + d.A.bytes(8) + (double)d.n * d.nb * (4.0 + 4.0 + 16.0 + 16.0 + 4.0) + d.n;
 }
 
 // ------------------------------------------------------------ BCGSSolver
@@ -706,6 +876,37 @@ int launch_bcgs_iteration(ugmg_ctx *ctx, int level, bool use_iter, int restart, 
 // it runs on the device: about a second of host time
 constexpr double kBaseDeviceWork = 2.0e9;
 
+// FULLREFINELEVEL of the levels set up to `level` (SetSurfaceClasses,
+// gm/algebra.cc:2885-2916): the lowest level holding a FINE_GRID_DOF vector
+// (VCLASS >= 2, VNCLASS <= 1); `level` when there is none
+static int host_fullrefinelevel(const ugmg_ctx *ctx, int level) {
+  int frl = level;
+  for (int l = level; l >= 0; --l) {
+    if (!ctx->level_set[l]) continue;
+    const HostLevel &H = ctx->host[l];
+    for (size_t v = 0; v < H.vc.size() && v < H.vnc.size(); ++v)
+      if (H.vc[v] >= 2 && H.vnc[v] <= 1) {
+        frl = l;
+        break;
+      }
+  }
+  return frl;
+}
+
+// lmgc's $b: a level, or (negative) LmgcInit's choice (np/procs/iter.cc:
+// 7643-7648): from FULLREFINELEVEL down, the first level with at most
+// -baselevel vectors (level 0 if none); never above `level`
+static int resolve_baselevel(const ugmg_ctx *ctx, int level) {
+  int bl = ctx->p.baselevel;
+  if (bl < 0) {
+    int i;
+    for (i = host_fullrefinelevel(ctx, level); i > 0; --i)
+      if (ctx->level_set[i] && ctx->host[i].n <= -ctx->p.baselevel) break;
+    bl = i;
+  }
+  return std::min(bl, level);
+}
+
 struct PrepState {
   int level = -1, bl = 0, fuse_rows = 0;
   double nt_bytes = 64.0e6;
@@ -719,14 +920,7 @@ int prep_host(ugmg_ctx *ctx, int level, PrepState &ps) {
   if (level < 0 || level >= ctx->nlevels) return fail(ctx, UGMG_E_ARG, "level out of range");
   if (!ctx->params_set) return fail(ctx, UGMG_E_STATE, "ugmg_set_params first");
   const ugmg_params &p = ctx->p;
-  int bl = p.baselevel;
-  if (bl < 0) {  // LmgcInit: deepest level with at most -baselevel vectors
-    int i;
-    for (i = level; i > 0; --i)
-      if (ctx->level_set[i] && ctx->host[i].n <= -bl) break;
-    bl = i;
-  }
-  bl = std::min(bl, level);
+  const int bl = resolve_baselevel(ctx, level);
   for (int l = bl; l <= level; ++l) {
     if (!ctx->level_set[l]) return fail(ctx, UGMG_E_STATE, "level " + std::to_string(l) + " not set");
     if (l > bl && !ctx->transfer_set[l])
@@ -874,6 +1068,8 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
         if ((rc = upload_sell(ctx, hp.U, D.U))) return rc;
         std::vector<uint8_t> c0(H.n);
         for (int q = 0; q < H.n; ++q) c0[q] = hp.color[q] == 0;
+        D.color_rows.assign(D.ncolors, 0);
+        for (int q = 0; q < H.n; ++q) ++D.color_rows[hp.color[q]];
         if ((rc = upload(ctx, &D.c0, c0.data(), c0.size()))) return rc;
         D.all_active = true;
         for (uint8_t v : H.vc) D.all_active &= v >= 3;
@@ -946,6 +1142,7 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
             (zero ? drop : keep).push_back(u);
           }
         const int nact = (int)act.size();
+        ctx->base_nact = nact;
         ctx->base_ncyc = (int)keep.size();
         std::vector<int> cpos(nsc, -1);
         for (int j = 0; j < ctx->base_ncyc; ++j) cpos[keep[j]] = j;
@@ -1021,15 +1218,7 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
   ctx->top = level;
   // FULLREFINELEVEL (SetSurfaceClasses, gm/algebra.cc:2885-2916): the lowest
   // level with a FINE_GRID_DOF vector, the top level if there is none
-  ctx->frl = level;
-  for (int l = level; l >= 0; --l) {
-    const HostLevel &H = ctx->host[l];
-    for (size_t v = 0; v < H.vc.size() && v < H.vnc.size(); ++v)
-      if (H.vc[v] >= 2 && H.vnc[v] <= 1) {
-        ctx->frl = l;
-        break;
-      }
-  }
+  ctx->frl = host_fullrefinelevel(ctx, level);
   ctx->prepared = true;
   return UGMG_OK;
 }
@@ -1056,6 +1245,7 @@ void ugmg_params_default(ugmg_params *p) {
   p->use_graph = 0;
   p->ex_float = 0;
   p->inner_adapt = 1;
+  p->base_inv32 = 0;
 }
 
 int ugmg_create(int device, ugmg_ctx **out) {
@@ -1401,11 +1591,11 @@ int ugmg_smatmul_minus(ugmg_ctx *ctx, int level, float *x, const float *y) {
   if (rc) return rc;
   DevLevel &d = ctx->dev[level];
   if (!d.perm) {
-    launch_spmv<float, float, float>(ctx->stream, d.A.d32(), 0, d.A.nslices, y, x, nullptr, 0, d.nt, B1(d));
+    launch_spmv<float, float, float>(ctx->stream, d.A.d32(), 0, d.A.nslices, y, x, nullptr, 0, d.nt32, B1(d));
   } else {
     to_dev<float, float>(ctx, level, y, d.vs, d.t32);
     to_dev<float, float>(ctx, level, x, d.vs, d.b32);
-    launch_spmv<float, float, float>(ctx->stream, d.A.d32(), 0, d.A.nslices, d.t32, d.b32, nullptr, 0, d.nt, B1(d));
+    launch_spmv<float, float, float>(ctx->stream, d.A.d32(), 0, d.A.nslices, d.t32, d.b32, nullptr, 0, d.nt32, B1(d));
     to_user<float, float>(ctx, level, d.b32, x, d.vs);
   }
   HIPCHK(hipGetLastError());
@@ -1588,6 +1778,9 @@ int ugmg_solve(ugmg_ctx *ctx, int level, double *x, double *b, const double *abs
     return fail(ctx, UGMG_E_SINGULAR, "zero pivot in the band LU of the base level");
   }
   DevLevel &d = ctx->dev[level];
+  ctx->stat_cycles = 0;
+  ctx->stat_bytes = 0.0;
+  const double vb = vbytes(d, 8);
   // a level in device order iterates on its own c (x) and b, moved in and
   // out once per solve
   double *const xu = x, *const bu = b;
@@ -1596,16 +1789,21 @@ int ugmg_solve(ugmg_ctx *ctx, int level, double *x, double *b, const double *abs
     to_dev<double, double>(ctx, level, bu, d.vs, d.b);
     x = d.c;
     b = d.b;
+    ctx->stat_bytes += 2.0 * (2.0 * vb + 4.0 * d.n);
   }
   auto move_out = [&]() -> int {
     if (!d.perm) return 0;
     to_user<double, double>(ctx, level, x, xu, d.vs);
     to_user<double, double>(ctx, level, b, bu, d.vs);
+    ctx->stat_bytes += 2.0 * (2.0 * vb + 4.0 * d.n);
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return 0;
   };
-  if (do_defect)  // LinearDefect: b -= A x
+  if (do_defect) {  // LinearDefect: b -= A x
     launch_spmv<double, double, double>(ctx->stream, d.A.d64(), 0, d.A.nslices, x, b, nullptr, 0, d.nt, B1(d));
+    ctx->stat_bytes += op_bytes(ctx, level, Op::Spmv, 8);
+  }
+  ctx->stat_bytes += vb + d.n;   // the first defect norm
   const int nc = d.nb;
   // sc_cmp (npscan.cc:1027-1057): every component below its limit
   auto sc_cmp = [nc](const double *x, const double *y) {
@@ -1622,8 +1820,10 @@ int ugmg_solve(ugmg_ctx *ctx, int level, double *x, double *b, const double *abs
   }
   if (cb) cb(user, 0, nc, first);
   int conv = sc_cmp(first, abslimit);
-  if (ctx->p.precision == UGMG_MIXED)
+  if (ctx->p.precision == UGMG_MIXED) {
     launch_convert<float, double>(ctx->stream, d.n * d.nb, b, d.b32, 1, d.vs, d.vs);
+    ctx->stat_bytes += vbytes(d, 12);
+  }
   // mixed steps: inner_cycles FP32 cycles, fewer when the per-cycle rate of
   // the previous step (the slowest component's) predicts that fewer bring
   // every component below 0.9 of the target (or of abslimit)
@@ -1658,6 +1858,9 @@ int ugmg_solve(ugmg_ctx *ctx, int level, double *x, double *b, const double *abs
       move_out();
       return rc;
     }
+    const int ncyc = ctx->p.precision == UGMG_MIXED ? std::max(1, cycles) : 1;
+    ctx->stat_cycles += ncyc;
+    ctx->stat_bytes += iteration_bytes(ctx, level, ncyc);
     for (int k = 0; k < nc; ++k) res->last_defect[k] = last[k] = ctx->red_host[k];
     if (cb) cb(user, i + 1, nc, last);
     if (sc_cmp(last, abslimit) || sc_cmp(last, target)) {
@@ -1669,6 +1872,13 @@ int ugmg_solve(ugmg_ctx *ctx, int level, double *x, double *b, const double *abs
   if (!conv) res->number_of_linear_iterations = i;
   res->converged = conv;
   return move_out();
+}
+
+int ugmg_solve_stats(const ugmg_ctx *ctx, long long *cycles, double *bytes) {
+  if (!ctx) return UGMG_E_ARG;
+  if (cycles) *cycles = ctx->stat_cycles;
+  if (bytes) *bytes = ctx->stat_bytes;
+  return UGMG_OK;
 }
 
 int ugmg_base_level(const ugmg_ctx *ctx) {
@@ -2127,12 +2337,11 @@ int ugmg_time_kernel(ugmg_ctx *ctx, int level, int kind, int reps, double *ms, d
   launch_fill<double>(st, d.n, d.b, 0.5, B1(d));
   launch_fill<float>(st, d.n, d.t32, 1.0f, B1(d));
   launch_fill<float>(st, d.n, d.b32, 0.5f, B1(d));
-  const double n = d.n, nnz = d.nnzb;
   double by = 0.0;
   auto once = [&]() {
     switch (kind) {
       case 0: launch_spmv<double, double, double>(st, d.A.d64(), 0, d.A.nslices, d.t, d.b, nullptr, 0, d.nt, B1(d)); break;
-      case 1: launch_spmv<float, float, float>(st, d.A.d32(), 0, d.A.nslices, d.t32, d.b32, nullptr, 0, d.nt, B1(d)); break;
+      case 1: launch_spmv<float, float, float>(st, d.A.d32(), 0, d.A.nslices, d.t32, d.b32, nullptr, 0, d.nt32, B1(d)); break;
       case 2: smooth_step<double>(ctx, level, d.t, d.b, d.c, 1, nullptr); break;
       case 3: {
         const int s = ctx->p.smoother;
@@ -2175,23 +2384,15 @@ int ugmg_time_kernel(ugmg_ctx *ctx, int level, int kind, int reps, double *ms, d
     }
   };
   switch (kind) {
-    case 0: by = spmv_bytes(d, 8, 8); break;
-    case 1: by = spmv_bytes(d, 4, 4); break;
-    case 2: by = spmv_bytes(d, 8, 8) + n * (8 + 8 + 16); break;
-    // colour sweep (lower entries, b, diag, v) + defect (upper entries, b rmw, v, c rmw)
-    case 3:
-    case 8: by = (nnz - n) * 12.0 + 8.0 * n * (1 + 1 + 1 + 2 + 1 + 2) + 8.0 * n; break;
-    case 7: by = (nnz - n) * 8.0 + 4.0 * n * (1 + 1 + 1 + 2 + 1 + 2) + 8.0 * n; break;
-    case 4: {
-      DevLevel &dc = ctx->dev[level - 1];
-      by = d.pt_nnz * 12.0 + 4.0 * (dc.n + 1) + 8.0 * d.n + 8.0 * dc.n;
-      break;
-    }
-    case 5: {
-      DevLevel &dc = ctx->dev[level - 1];
-      by = d.p_nnz * 12.0 + 4.0 * (d.n + 1) + 8.0 * dc.n + 8.0 * d.n + 16.0 * d.n;
-      break;
-    }
+    case 0: by = op_bytes(ctx, level, Op::Spmv, 8); break;
+    case 1: by = op_bytes(ctx, level, Op::Spmv, 4); break;
+    case 2: by = op_bytes(ctx, level, Op::Jac, 8) + op_bytes(ctx, level, Op::SpmvC, 8); break;
+    case 3: by = gs_step_bytes(ctx, level, 8, false); break;
+    case 7: by = gs_step_bytes(ctx, level, 4, false); break;
+    case 8: by = gs_step_bytes(ctx, level, 8, true); break;
+    case 4: by = op_bytes(ctx, level, Op::Restrict, 8); break;
+    case 5: by = op_bytes(ctx, level, Op::Interp, 8); break;
+    case 6: by = cycle_bytes(ctx, level, ctx->p.precision == UGMG_MIXED ? 4 : 8); break;
     default: by = 0.0;
   }
   if (kind < 0 || kind > 8) return fail(ctx, UGMG_E_ARG, "unknown kernel kind");
@@ -2517,8 +2718,7 @@ int ugmg_server_submit(ugmg_server *s, int nlevels, const ugmg_level_desc *level
   r->red = red;
   r->abslimit = abslimit;
   r->maxiter = maxiter;
-  const int bl = std::min(std::max(q.baselevel, 0), r->top);
-  r->est_bytes = est_device_bytes(ctx, r->top, bl);
+  r->est_bytes = est_device_bytes(ctx, r->top, resolve_baselevel(ctx, r->top));
   r->t_submit = s->now();
   std::lock_guard<std::mutex> lk(s->mu);
   r->id = (int)s->reqs.size();

@@ -1598,6 +1598,10 @@ template void launch_base_gemv<double, double>(hipStream_t, int, const double *,
                                                const int *, int, const double *, const double *, double *,
                                                double *, int, double, double, double,
                                                const double *, int, double *, int, Batch);
+template void launch_base_gemv<float, double>(hipStream_t, int, const double *, const int *,
+                                              const int *, int, const float *, const float *, float *, float *, int,
+                                              double, double, double, const double *, int, double *,
+                                              int, Batch);
 template void launch_base_gemv<float, float>(hipStream_t, int, const float *, const int *,
                                              const int *, int, const float *, const float *, float *, float *, int,
                                              double, double, double, const double *, int, double *,

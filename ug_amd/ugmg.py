@@ -39,7 +39,7 @@ class Params(C.Structure):
                 ("base_abslimit", C.c_double * 3), ("base_red", C.c_double * 3),
                 ("base_maxiter", C.c_int), ("ex_reorder", C.c_int), ("precision", C.c_int),
                 ("inner_cycles", C.c_int), ("use_graph", C.c_int), ("ex_float", C.c_int),
-                ("inner_adapt", C.c_int)]
+                ("inner_adapt", C.c_int), ("base_inv32", C.c_int)]
 
 
 class LResult(C.Structure):
@@ -110,6 +110,8 @@ def load():
         "ugmg_solve_surface": ([vp, ip, vp, vp, dp, dp, ip, ip, C.POINTER(LResult), DEFECT_CB, vp], ip),
         "ugmg_solve_surface_host": ([vp, ip, vp, vp, dp, dp, ip, ip, C.POINTER(LResult), DEFECT_CB, vp], ip),
         "ugmg_time_kernel": ([vp, ip, ip, ip, C.POINTER(C.c_double), C.POINTER(C.c_double)], ip),
+        "ugmg_solve_stats": ([vp, C.POINTER(C.c_longlong), C.POINTER(C.c_double)], ip),
+        "ugmg_base_level": ([vp], ip),
         "ugmg_set_batch": ([vp, ip], ip),
         "ugmg_galerkin": ([vp, ip], ip),
         "ugmg_iter_batch": ([vp, ip, ip, dp, dp, ip], ip),
@@ -344,6 +346,18 @@ class Context:
                                            C.addressof(rd), maxiter, int(do_defect), C.byref(res), cb, None))
         return x, b, res, np.array(defects)
 
+    def solve_stats(self):
+        """(V-cycles, algorithmic bytes) of the last solve."""
+        cyc, by = C.c_longlong(), C.c_double()
+        self._chk(self.lib.ugmg_solve_stats(self.h, C.byref(cyc), C.byref(by)))
+        return cyc.value, by.value
+
+    def base_level(self):
+        r = self.lib.ugmg_base_level(self.h)
+        if r < 0:
+            self._chk(r)
+        return r
+
     def fullrefinelevel(self):
         r = self.lib.ugmg_fullrefinelevel(self.h)
         if r < 0:
@@ -355,7 +369,7 @@ class Context:
         """ugmg_solve_surface_host: rhs[l] (and x0[l]) for levels baselevel..level
         (None below); returns (x list, b list, LResult, defects array)."""
         nb = self.nb
-        bl = max(0, min(self.params.baselevel, self.top))
+        bl = self.base_level()
         xs, bs = [], []
         for l in range(level + 1):
             if l < bl:

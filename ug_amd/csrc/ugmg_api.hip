@@ -45,18 +45,17 @@ struct DevSellBuf {
   float *v32 = nullptr;
   int wmax = 0;                 // widest slice
   std::vector<int> width_h;     // per-slice width (host)
-  std::vector<char> expl_h;     // per-slice: explicit column indices (host)
+  // prefix sums over the slices (host): value slots, and the slots of
+  // explicit slices (those that read column indices)
+  std::vector<double> slot_pre, expl_pre;
   // Bytes one pass over slices [s0, s1) reads as laid out: every value slot
   // (padding lanes and zero phantoms of uniform slices included, sv bytes
   // per value), column indices only of explicit slices, the slot->row map
-  // and soff/sdoff per slice.  Vectors not included.
+  // and soff/sdoff per slice.  Vectors not included.  O(1): the solver adds
+  // these up after every iteration.
   double bytes_range(int s0, int s1, int sv) const {
-    double e = 0.0, ex = 0.0;
-    for (int s = s0; s < s1; ++s) {
-      const double w = 64.0 * width_h[s];
-      e += w;
-      if (expl_h[s]) ex += w;
-    }
+    if (s1 <= s0) return 0.0;
+    const double e = slot_pre[s1] - slot_pre[s0], ex = expl_pre[s1] - expl_pre[s0];
     return e * bb * sv + 4.0 * ex + (4.0 * 64.0 + 8.0) * (s1 - s0);
   }
   // the whole slab, with its shared offset lists
@@ -246,13 +245,14 @@ int upload_sell(ugmg_ctx *ctx, const HostSell &S, DevSellBuf &D) {
   if ((rc = upload(ctx, &D.dtab, S.dtab.data(), S.dtab.size()))) return rc;
   D.nuniform = 0;
   D.nent_explicit = 0;
-  D.expl_h.assign(S.nslices, 0);
+  D.slot_pre.assign(S.nslices + 1, 0.0);
+  D.expl_pre.assign(S.nslices + 1, 0.0);
   for (int q = 0; q < S.nslices; ++q) {
+    const size_t slots = (size_t)(S.soff[q + 1] - S.soff[q]);
     D.nuniform += S.sdoff[q] >= 0;
-    if (S.sdoff[q] < 0) {
-      D.nent_explicit += (size_t)(S.soff[q + 1] - S.soff[q]);
-      D.expl_h[q] = 1;
-    }
+    if (S.sdoff[q] < 0) D.nent_explicit += slots;
+    D.slot_pre[q + 1] = D.slot_pre[q] + (double)slots;
+    D.expl_pre[q + 1] = D.expl_pre[q] + (S.sdoff[q] < 0 ? (double)slots : 0.0);
   }
   D.ndtab = S.dtab.size();
   D.bb = S.bb;

@@ -145,6 +145,17 @@ int ugmg_smatmul_minus(ugmg_ctx *ctx, int level, float *x, const float *y);   /*
 int ugmg_dnrm2x(ugmg_ctx *ctx, int level, const double *x, double *out_host);
 int ugmg_ddot(ugmg_ctx *ctx, int level, const double *x, const double *y,
               double *out_host);
+/* ugmg_dnrm2x that allocates nothing.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory at call time (tight
+ * serving loops, harnesses that forbid hipMalloc).  The partial sums and
+ * the norms live in a static device array of the library (1024 partials per
+ * component, 3 components, plus the norms: 3,136 doubles, 24.5 KB per GPU,
+ * held until the process ends) instead of the context's reduction buffers;
+ * same kernels, same result bit for bit.  UGMG_E_ARG if a level would need
+ * more partial sums than the array holds.  Every call on one GPU shares
+ * that array: calls that overlap on one GPU (other streams, other
+ * contexts, other host threads) race and must be serialised by the caller. */
+int ugmg_dnrm2x_noalloc(ugmg_ctx *ctx, int level, const double *x, double *out_host);
 
 /* ---- smoothers: Smoother (iter.cc:817-842) = Step; x *= damp; b -= A x -- */
 int ugmg_jacobi_step(ugmg_ctx *ctx, int level, double *x, double *b);

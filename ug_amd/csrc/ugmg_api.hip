@@ -20,6 +20,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <type_traits>
@@ -28,6 +29,39 @@
 #include "../../include/ugmg.h"
 #include "ugmg_internal.h"
 #include "ugmg_kernels.h"
+
+// Scratch of ugmg_dnrm2x_noalloc, the entry point that allocates nothing:
+// the reduction's partial sums (at most 1024 blocks per component, the cap
+// of reduce_parts) followed by the final norms.  Every GPU the library runs
+// on holds its own copy until the process ends.  Host code never uses the
+// array's name: its device address is resolved once per device (when the
+// first context on that device is created) and only that address is passed
+// to kernels and copies.
+constexpr int kScratchParts = 1024 * UGMG_MAXB;
+constexpr int kScratchDoubles = kScratchParts + 64;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+
+constexpr int kScratchMaxDevices = 64;
+std::mutex g_scratch_mu;
+bool g_scratch_done[kScratchMaxDevices] = {};
+double *g_scratch_ptr[kScratchMaxDevices] = {};
+hipError_t g_scratch_err[kScratchMaxDevices] = {};
+
+// with `device` current: the device address of this GPU's scratch (once)
+void resolve_scratch(int device) {
+  if (device < 0 || device >= kScratchMaxDevices) return;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_done[device]) return;
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  g_scratch_err[device] = e;
+  g_scratch_ptr[device] = e == hipSuccess ? static_cast<double *>(ptr) : nullptr;
+  g_scratch_done[device] = true;
+}
+
+}  // namespace
 
 using namespace ugmg;
 
@@ -1265,6 +1299,7 @@ int ugmg_create(int device, ugmg_ctx **out) {
   }
   ctx->own_stream = true;
   ugmg_params_default(&ctx->p);
+  resolve_scratch(device);
   *out = ctx;
   return UGMG_OK;
 }
@@ -1609,6 +1644,34 @@ int ugmg_dnrm2x(ugmg_ctx *ctx, int level, const double *x, double *out) {
   DevLevel &d = ctx->dev[level];
   launch_dot<double>(ctx->stream, d.n, x, nullptr, nullptr, ctx->part, ctx->red, 1, B1(d));
   HIPCHK(hipMemcpyAsync(ctx->red_host, ctx->red, sizeof(double) * d.nb, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  for (int i = 0; i < d.nb; ++i) out[i] = ctx->red_host[i];
+  return UGMG_OK;
+}
+
+// ugmg_dnrm2x with the partial sums and the norms in the static scratch
+// instead of the context's reduction buffers: the same kernels over the
+// same partition, so the result is bit-identical; no allocation
+int ugmg_dnrm2x_noalloc(ugmg_ctx *ctx, int level, const double *x, double *out) {
+  int rc = check_ready(ctx, level);
+  if (rc) return rc;
+  if (!x || !out) return fail(ctx, UGMG_E_ARG, "null argument");
+  const int dev = ctx->device;
+  if (dev < 0 || dev >= kScratchMaxDevices || !g_scratch_done[dev])
+    return fail(ctx, UGMG_E_STATE, "no scratch on this device");
+  if (!g_scratch_ptr[dev])
+    return fail(ctx, UGMG_E_HIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                     hipGetErrorString(g_scratch_err[dev]));
+  DevLevel &d = ctx->dev[level];
+  const long long need = (long long)reduce_parts(d.n) * d.nb;
+  if (need > kScratchParts || d.nb > UGMG_MAXB)
+    return fail(ctx, UGMG_E_ARG, "ugmg_dnrm2x_noalloc: " + std::to_string(need) +
+                                     " partial sums exceed the scratch of " +
+                                     std::to_string(kScratchParts));
+  double *part = g_scratch_ptr[dev], *norms = g_scratch_ptr[dev] + kScratchParts;
+  launch_dot<double>(ctx->stream, d.n, x, nullptr, nullptr, part, norms, 1, B1(d));
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(ctx->red_host, norms, sizeof(double) * d.nb, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   for (int i = 0; i < d.nb; ++i) out[i] = ctx->red_host[i];
   return UGMG_OK;

@@ -96,6 +96,7 @@ def load():
         "ugmg_dmatmul": ([vp, ip, dp, dp], ip),
         "ugmg_smatmul_minus": ([vp, ip, dp, dp], ip),
         "ugmg_dnrm2x": ([vp, ip, dp, dp], ip),
+        "ugmg_dnrm2x_noalloc": ([vp, ip, dp, dp], ip),
         "ugmg_ddot": ([vp, ip, dp, dp, dp], ip),
         "ugmg_jacobi_step": ([vp, ip, dp, dp], ip),
         "ugmg_gs_step": ([vp, ip, dp, dp], ip),
@@ -278,6 +279,13 @@ class Context:
         """Per-component norm (a float for scalar hierarchies)."""
         out = (C.c_double * 3)()
         self._chk(self.lib.ugmg_dnrm2x(self.h, level, _ptr(x), C.addressof(out)))
+        b = self._ncomp()
+        return out[0] if b == 1 else np.array(out[:b])
+
+    def dnrm2x_noalloc(self, level, x):
+        """dnrm2x through the entry point that allocates nothing."""
+        out = (C.c_double * 3)()
+        self._chk(self.lib.ugmg_dnrm2x_noalloc(self.h, level, _ptr(x), C.addressof(out)))
         b = self._ncomp()
         return out[0] if b == 1 else np.array(out[:b])
 

@@ -67,6 +67,25 @@ double ugmg_problem_theta(const ugmg_problem *p);
 int ugmg_problem_elasticity(int nref, double E, double nu, uint64_t seed,
                             ugmg_problem **out);
 
+/* 3D P1 linear elasticity (as above: b = 3, E, nu, clamped on x = 0) on a
+ * locally refined tetrahedral hierarchy: the 6 Kuhn tetrahedra of the unit
+ * cube refined red nuni times, then depth times only toward the corner
+ * (0,0,0) (step k marks the regular elements inside [0, 2^-k]^3), closed
+ * green as UG closes tetrahedra without a complete rule set (a centre node
+ * and one son per side triangle), every other element copied.  Vector
+ * classes, list order and interpolation as UG's refinement and
+ * SetSurfaceClasses give them; every level's own matrix.  No load: the
+ * right-hand sides are per level (the surface solve's).  nuni + depth <= 9. */
+int ugmg_problem_elasticity_local(int nuni, int depth, double E, double nu,
+                                  ugmg_problem **out);
+
+/* The tetrahedral mesh of a level of ugmg_problem_elasticity_local: node
+ * coordinates (3 per vector, list order), 4 corners per element in UG's
+ * corner order and the element kind (0 red, 1 green, 2 copy).  -1 for
+ * the other generators. */
+int ugmg_problem_mesh(const ugmg_problem *p, int level, int *nnodes, const double **xyz,
+                      int *nelem, const int **tets, const uint8_t **kind);
+
 void ugmg_problem_free(ugmg_problem *p);
 
 int ugmg_problem_nlevels(const ugmg_problem *p);     /* nref + 1            */

@@ -61,6 +61,11 @@ struct ugmg_problem {
   double theta = 0.0;
   std::vector<Level> lv;
   std::vector<double> rhs;
+  // unstructured tetrahedral meshes (ugmg_problem_elasticity_local only):
+  // node coordinates, corners and kind (0 red, 1 green, 2 copy) per level
+  std::vector<std::vector<double>> xyz;
+  std::vector<std::vector<int>> tets;
+  std::vector<std::vector<uint8_t>> tet_kind;
 };
 
 extern "C" void ugmg_splitmix_uniform(uint64_t seed, int64_t count, double a,
@@ -751,6 +756,352 @@ extern "C" int ugmg_problem_elasticity(int nref, double E, double nu, uint64_t s
     return -4;
   }
   *out = p;
+  return 0;
+}
+
+// ----------------------------------------------- locally refined elasticity
+// P1 linear elasticity (the element matrices above) on a tetrahedral
+// hierarchy refined the way UG refines tetrahedra in its default build,
+// where tetrahedra have no complete rule set (TET_RULESET undefined:
+// gm/rm.h:173, gm/refine.cc:160-166, gm/rm.cc:2764-2776):
+//   * level 0: the unit cube split into the 6 Kuhn tetrahedra;
+//   * refinement step s marks the regular (red) elements whose corners all
+//     lie in [0, a]^3, a = 1 for the first nuni steps, then 1/2, 1/4, ...;
+//   * a marked element is refined red into 8 (TET_RED, gm/rm.cc:536-575):
+//     the four corner tetrahedra and the inner octahedron split along the
+//     diagonal joining the midpoints of edges (0,2) and (1,3), which keeps
+//     every child a Kuhn simplex of half the size, so a fully refined region
+//     is the uniform Kuhn lattice of the next level;
+//   * an unmarked element with refined edges is closed green
+//     (BuildGreenClosure, gm/refine.cc:1850-2005; RefineElementGreen,
+//     4493-5360): a centre node at the mean of its corners (CreateCenterNode,
+//     gm/ugm.cc:1656-1770) and one son per triangle of each side, a side
+//     with one refined edge cut in two, one with three in four, each son's
+//     fourth corner the centre node; green elements are not refined again;
+//   * every other element is copied (yellow).  UG copies only a
+//     neighbourhood of the refined region; here every level keeps copies of
+//     the whole cube (the far copies have class 0 and are off the surface).
+// Vector classes: NCLASS 3 on the corners of the red and green elements of
+// a level (refine.cc:6759-6790), NNCLASS 3 on the corners of the elements
+// refined red or green (refine.cc:2890-2910); VCLASS / VNCLASS 3 on the
+// vectors of elements whose corners all carry node class 3, then 2 on their
+// matrix neighbours and 1 on the neighbours of those (SetSurfaceClasses,
+// gm/algebra.cc:2838-2930; PropagateVectorClasses, 4063-4190); level 0
+// VCLASS 3, top level VNCLASS 0.  Level l+1 lists the sons of level l's
+// vectors in their order, then the edge midpoints (ascending edge), then
+// the centre nodes (element order).  Interpolation (GNs of the father,
+// gm/shapes.cc:748-819): sons 1, midpoints 1/2 to the edge's end points
+// (lower first), centre nodes 1/4 to the father's corners in corner order.
+// Clamped (all components Dirichlet) on x = 0; no load here (the surface
+// solve takes one right-hand side per level).
+namespace {
+
+enum : uint8_t { kTetRed = 0, kTetGreen = 1, kTetCopy = 2 };
+
+struct TetMesh {
+  std::vector<double> X;            // 3 per node
+  std::vector<int> T;               // 4 per element, UG corner order
+  std::vector<uint8_t> kind;        // kTetRed / kTetGreen / kTetCopy
+  int nodes() const { return (int)(X.size() / 3); }
+  int64_t elems() const { return (int64_t)kind.size(); }
+};
+
+// UG's reference tetrahedron (gm/elements.cc:121-135): corners of each
+// edge and of each side
+const int kTetEdge[6][2] = {{0, 1}, {1, 2}, {0, 2}, {0, 3}, {1, 3}, {2, 3}};
+const int kTetSide[4][3] = {{0, 2, 1}, {1, 2, 3}, {0, 3, 2}, {0, 1, 3}};
+
+inline uint64_t edge_key(int a, int b, int n) {
+  return a < b ? (uint64_t)a * (uint64_t)n + (uint64_t)b : (uint64_t)b * (uint64_t)n + (uint64_t)a;
+}
+
+// refined[e] = 1 for elements of c refined red or green (NNCLASS seeds);
+// fine gets its transfer rows (pp/pi/pw) and f its mesh
+int refine_tets(const TetMesh &c, double a, TetMesh &f, std::vector<uint8_t> &refined, Level &fine) {
+  const int n = c.nodes();
+  const int64_t ne = c.elems();
+  std::vector<uint8_t> marked(ne, 0);
+  std::vector<uint64_t> ref;
+  for (int64_t e = 0; e < ne; ++e) {
+    if (c.kind[e] != kTetRed) continue;
+    bool in = true;
+    for (int q = 0; q < 4 && in; ++q)
+      for (int d = 0; d < 3; ++d) in = in && c.X[(size_t)c.T[e * 4 + q] * 3 + d] <= a + 1e-12;
+    if (!in) continue;
+    marked[e] = 1;
+    for (int k = 0; k < 6; ++k) ref.push_back(edge_key(c.T[e * 4 + kTetEdge[k][0]], c.T[e * 4 + kTetEdge[k][1]], n));
+  }
+  std::sort(ref.begin(), ref.end());
+  ref.erase(std::unique(ref.begin(), ref.end()), ref.end());
+  auto mid = [&](int u, int w) -> int {   // midpoint node of a refined edge, -1 if none
+    const uint64_t k = edge_key(u, w, n);
+    auto it = std::lower_bound(ref.begin(), ref.end(), k);
+    return (it != ref.end() && *it == k) ? n + (int)(it - ref.begin()) : -1;
+  };
+  // green closure: unmarked elements with a refined edge
+  std::vector<uint8_t> green(ne, 0);
+  int64_t ngreen = 0;
+  for (int64_t e = 0; e < ne; ++e) {
+    if (marked[e]) continue;
+    const int *t = &c.T[e * 4];
+    int cnt = 0;
+    for (int k = 0; k < 6; ++k) cnt += mid(t[kTetEdge[k][0]], t[kTetEdge[k][1]]) >= 0;
+    if (!cnt) continue;
+    if (c.kind[e] != kTetRed) return -1;   // only regular elements can be closed
+    for (int s = 0; s < 4; ++s) {
+      int m = 0;
+      for (int j = 0; j < 3; ++j) m += mid(t[kTetSide[s][j]], t[kTetSide[s][(j + 1) % 3]]) >= 0;
+      if (m == 2) return -1;               // not produced by refinement toward a corner
+    }
+    green[e] = 1;
+    ++ngreen;
+  }
+  refined.assign(ne, 0);
+  for (int64_t e = 0; e < ne; ++e) refined[e] = marked[e] | green[e];
+  // nodes: sons, edge midpoints, centres
+  const int nm = (int)ref.size();
+  const int64_t nf = (int64_t)n + nm + ngreen;
+  if (nf > 2000000000LL) return -1;
+  f.X.resize((size_t)nf * 3);
+  std::copy(c.X.begin(), c.X.end(), f.X.begin());
+  fine.pp.assign(nf + 1, 0);
+  fine.pi.clear();
+  fine.pw.clear();
+  for (int v = 0; v < n; ++v) {
+    fine.pi.push_back(v);
+    fine.pw.push_back(1.0);
+    fine.pp[v + 1] = (int)fine.pi.size();
+  }
+  for (int k = 0; k < nm; ++k) {
+    const int u = (int)(ref[k] / (uint64_t)n), w = (int)(ref[k] % (uint64_t)n);
+    for (int d = 0; d < 3; ++d) f.X[(size_t)(n + k) * 3 + d] = 0.5 * (c.X[(size_t)u * 3 + d] + c.X[(size_t)w * 3 + d]);
+    fine.pi.push_back(u);
+    fine.pw.push_back(0.5);
+    fine.pi.push_back(w);
+    fine.pw.push_back(0.5);
+    fine.pp[n + k + 1] = (int)fine.pi.size();
+  }
+  // sons in father element order
+  f.T.clear();
+  f.kind.clear();
+  f.T.reserve((size_t)ne * 4 + (size_t)(ne - ngreen) * 28);
+  int next_centre = n + nm;
+  auto son = [&](int p, int q, int r, int s, uint8_t kind) {
+    f.T.insert(f.T.end(), {p, q, r, s});
+    f.kind.push_back(kind);
+  };
+  for (int64_t e = 0; e < ne; ++e) {
+    const int *t = &c.T[e * 4];
+    if (marked[e]) {
+      const int x0 = t[0], x1 = t[1], x2 = t[2], x3 = t[3];
+      const int m01 = mid(x0, x1), m02 = mid(x0, x2), m03 = mid(x0, x3);
+      const int m12 = mid(x1, x2), m13 = mid(x1, x3), m23 = mid(x2, x3);
+      son(x0, m01, m02, m03, kTetRed);
+      son(m01, x1, m12, m13, kTetRed);
+      son(m02, m12, x2, m23, kTetRed);
+      son(m03, m13, m23, x3, kTetRed);
+      son(m01, m02, m03, m13, kTetRed);
+      son(m01, m02, m12, m13, kTetRed);
+      son(m02, m03, m13, m23, kTetRed);
+      son(m02, m12, m13, m23, kTetRed);
+    } else if (green[e]) {
+      const int ctr = next_centre++;
+      for (int d = 0; d < 3; ++d) {
+        double s = 0.0;
+        for (int q = 0; q < 4; ++q) s += c.X[(size_t)t[q] * 3 + d];
+        f.X[(size_t)ctr * 3 + d] = 0.25 * s;
+      }
+      for (int q = 0; q < 4; ++q) {
+        fine.pi.push_back(t[q]);
+        fine.pw.push_back(0.25);
+      }
+      fine.pp[ctr + 1] = (int)fine.pi.size();
+      for (int s = 0; s < 4; ++s) {
+        const int c0 = t[kTetSide[s][0]], c1 = t[kTetSide[s][1]], c2 = t[kTetSide[s][2]];
+        const int cs[3] = {c0, c1, c2};
+        const int m[3] = {mid(c0, c1), mid(c1, c2), mid(c2, c0)};
+        const int k = (m[0] >= 0) + (m[1] >= 0) + (m[2] >= 0);
+        // a side triangle (p, q, r) becomes the son with corners p, r, q and
+        // the centre (the side's corners fill son corners 0, 2, 1)
+        auto tri = [&](int p, int q, int r) { son(p, r, q, ctr, kTetGreen); };
+        if (k == 0) {
+          tri(c0, c1, c2);
+        } else if (k == 1) {
+          int j = 0;
+          while (m[j] < 0) ++j;
+          tri(m[j], cs[(j + 1) % 3], cs[(j + 2) % 3]);
+          tri(m[j], cs[(j + 2) % 3], cs[j]);
+        } else {
+          tri(m[0], c1, m[1]);
+          tri(m[0], m[2], c0);
+          tri(m[1], c2, m[2]);
+          tri(m[0], m[1], m[2]);
+        }
+      }
+    } else {
+      son(t[0], t[1], t[2], t[3], kTetCopy);
+    }
+  }
+  fine.n = (int)nf;
+  return 0;
+}
+
+// level matrix from the element list: pattern = vectors sharing an element
+// (diagonal first, then ascending columns), element blocks added in element
+// order, Dirichlet rows on x = 0
+void assemble_tet_level(const TetMesh &m, double lam, double mu, Level &L) {
+  const int n = m.nodes();
+  const int64_t ne = m.elems();
+  std::vector<uint64_t> keys;
+  keys.reserve((size_t)ne * 16);
+  for (int64_t e = 0; e < ne; ++e)
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j) {
+        const int r = m.T[e * 4 + i], c = m.T[e * 4 + j];
+        keys.push_back((uint64_t)r * ((uint64_t)n + 1) + (r == c ? 0 : (uint64_t)c + 1));
+      }
+  std::sort(keys.begin(), keys.end());
+  keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+  L.n = n;
+  L.rp.assign(n + 1, 0);
+  L.ci.resize(keys.size());
+  for (size_t q = 0; q < keys.size(); ++q) {
+    const int r = (int)(keys[q] / ((uint64_t)n + 1));
+    const uint64_t code = keys[q] % ((uint64_t)n + 1);
+    L.ci[q] = code == 0 ? r : (int)(code - 1);
+    ++L.rp[r + 1];
+  }
+  for (int r = 0; r < n; ++r) L.rp[r + 1] += L.rp[r];
+  L.v.assign(keys.size() * 9, 0.0);
+  auto slot = [&](int r, int c) -> size_t {
+    if (r == c) return (size_t)L.rp[r];
+    const int *b = &L.ci[L.rp[r] + 1], *en = &L.ci[L.rp[r + 1]];
+    return (size_t)(std::lower_bound(b, en, c) - &L.ci[0]);
+  };
+  for (int64_t e = 0; e < ne; ++e) {
+    double X[4][3];
+    for (int q = 0; q < 4; ++q)
+      for (int d = 0; d < 3; ++d) X[q][d] = m.X[(size_t)m.T[e * 4 + q] * 3 + d];
+    double K[4][4][9], M[4][4];
+    p1_elasticity_element(X, lam, mu, K, M);
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j) {
+        double *dst = &L.v[slot(m.T[e * 4 + i], m.T[e * 4 + j]) * 9];
+        for (int q = 0; q < 9; ++q) dst[q] += K[i][j][q];
+      }
+  }
+  L.skip.assign(n, 0u);
+  for (int r = 0; r < n; ++r) {
+    if (m.X[(size_t)r * 3] != 0.0) continue;
+    L.skip[r] = 7u;
+    for (int64_t q = L.rp[r]; q < L.rp[r + 1]; ++q)
+      for (int k = 0; k < 9; ++k) L.v[(size_t)q * 9 + k] = 0.0;
+    for (int a = 0; a < 3; ++a) L.v[(size_t)L.rp[r] * 9 + a * 4] = 1.0;
+  }
+}
+
+// class 3 on the vectors of elements whose corners all carry node class 3,
+// then 2 on matrix neighbours of class 3 and 1 on neighbours of class 2
+std::vector<uint8_t> tet_classes(const TetMesh &m, const std::vector<uint8_t> &node3, const Level &L) {
+  const int n = L.n;
+  std::vector<uint8_t> cls(n, 0);
+  for (int64_t e = 0; e < m.elems(); ++e) {
+    const int *t = &m.T[e * 4];
+    if (node3[t[0]] && node3[t[1]] && node3[t[2]] && node3[t[3]])
+      for (int q = 0; q < 4; ++q) cls[t[q]] = 3;
+  }
+  for (int k = 3; k >= 2; --k) {
+    std::vector<uint8_t> hit(n, 0);
+    for (int r = 0; r < n; ++r) {
+      if (cls[r] != k) continue;
+      for (int q = L.rp[r] + 1; q < L.rp[r + 1]; ++q) hit[L.ci[q]] = 1;
+    }
+    for (int r = 0; r < n; ++r)
+      if (hit[r] && cls[r] < k) cls[r] = (uint8_t)(k - 1);
+  }
+  return cls;
+}
+
+}  // namespace
+
+extern "C" int ugmg_problem_elasticity_local(int nuni, int depth, double E, double nu,
+                                             ugmg_problem **out) {
+  if (!out || nuni < 0 || depth < 0 || nuni + depth > 9 || !(E > 0.0) || !(nu > -1.0 && nu < 0.5))
+    return -1;
+  ugmg_problem *p = new (std::nothrow) ugmg_problem;
+  if (!p) return -4;
+  p->dim = 3;
+  p->b = 3;
+  const double lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)), mu = E / (2.0 * (1.0 + nu));
+  const int L = nuni + depth;
+  try {
+    std::vector<TetMesh> mesh(L + 1);
+    TetMesh &m0 = mesh[0];
+    for (int v = 0; v < 8; ++v) m0.X.insert(m0.X.end(), {(double)(v & 1), (double)((v >> 1) & 1), (double)((v >> 2) & 1)});
+    for (int t = 0; t < 6; ++t) {
+      int id = 0, path[4] = {0, 0, 0, 0};
+      for (int q = 1; q < 4; ++q) path[q] = (id |= 1 << kPerm[t][q - 1]);
+      m0.T.insert(m0.T.end(), {path[0], path[1], path[2], path[3]});
+      m0.kind.push_back(kTetRed);
+    }
+    p->lv.resize(L + 1);
+    std::vector<std::vector<uint8_t>> refined(L);
+    for (int s = 0; s < L; ++s) {
+      const double a = s < nuni ? 1.0 : std::ldexp(1.0, -(s - nuni + 1));
+      if (refine_tets(mesh[s], a, mesh[s + 1], refined[s], p->lv[s + 1])) {
+        delete p;
+        return -1;
+      }
+    }
+    for (int l = 0; l <= L; ++l) {
+      Level &lv = p->lv[l];
+      assemble_tet_level(mesh[l], lam, mu, lv);
+      lv.N = 0;
+      lv.C = 0;
+      if (l == 0) {
+        lv.vc.assign(lv.n, 3);
+      } else {
+        std::vector<uint8_t> nc3(lv.n, 0);
+        for (int64_t e = 0; e < mesh[l].elems(); ++e)
+          if (mesh[l].kind[e] != kTetCopy)
+            for (int q = 0; q < 4; ++q) nc3[mesh[l].T[e * 4 + q]] = 1;
+        lv.vc = tet_classes(mesh[l], nc3, lv);
+      }
+      if (l == L) {
+        lv.vnc.assign(lv.n, 0);
+      } else {
+        std::vector<uint8_t> nn3(lv.n, 0);
+        for (int64_t e = 0; e < mesh[l].elems(); ++e)
+          if (refined[l][e])
+            for (int q = 0; q < 4; ++q) nn3[mesh[l].T[e * 4 + q]] = 1;
+        lv.vnc = tet_classes(mesh[l], nn3, lv);
+      }
+    }
+    p->rhs.assign((size_t)p->lv[L].n * 3, 0.0);
+    p->xyz.resize(L + 1);
+    p->tets.resize(L + 1);
+    p->tet_kind.resize(L + 1);
+    for (int l = 0; l <= L; ++l) {
+      p->xyz[l].swap(mesh[l].X);
+      p->tets[l].swap(mesh[l].T);
+      p->tet_kind[l].swap(mesh[l].kind);
+    }
+  } catch (const std::bad_alloc &) {
+    delete p;
+    return -4;
+  }
+  *out = p;
+  return 0;
+}
+
+extern "C" int ugmg_problem_mesh(const ugmg_problem *p, int level, int *nnodes, const double **xyz,
+                                 int *nelem, const int **tets, const uint8_t **kind) {
+  if (!p || level < 0 || level >= (int)p->xyz.size()) return -1;
+  if (nnodes) *nnodes = (int)(p->xyz[level].size() / 3);
+  if (xyz) *xyz = p->xyz[level].data();
+  if (nelem) *nelem = (int)p->tet_kind[level].size();
+  if (tets) *tets = p->tets[level].data();
+  if (kind) *kind = p->tet_kind[level].data();
   return 0;
 }
 

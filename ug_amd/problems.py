@@ -45,6 +45,11 @@ def _load():
         lib.ugmg_problem_p1_cd.restype = C.c_int
         lib.ugmg_problem_elasticity.argtypes = [C.c_int, C.c_double, C.c_double, C.c_uint64, C.POINTER(P)]
         lib.ugmg_problem_elasticity.restype = C.c_int
+        lib.ugmg_problem_elasticity_local.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.POINTER(P)]
+        lib.ugmg_problem_elasticity_local.restype = C.c_int
+        lib.ugmg_problem_mesh.argtypes = [P, C.c_int, ip, C.POINTER(dp), ip, C.POINTER(ip),
+                                          C.POINTER(C.POINTER(C.c_uint8))]
+        lib.ugmg_problem_mesh.restype = C.c_int
         lib.ugmg_problem_theta.argtypes = [P]
         lib.ugmg_problem_theta.restype = C.c_double
         E = C.POINTER(C.POINTER(_ElementData))
@@ -217,12 +222,22 @@ class Level:
 
 
 @dataclass
+class Mesh:
+    """Tetrahedral mesh of one level: node coordinates (n, 3) in list order,
+    corners (nelem, 4) in UG's corner order, kind 0 red / 1 green / 2 copy."""
+    xyz: np.ndarray
+    tets: np.ndarray
+    kind: np.ndarray
+
+
+@dataclass
 class Problem:
     dim: int
     levels: list = field(default_factory=list)
     rhs: np.ndarray | None = None
     name: str = ""
     theta: float = 0.0
+    meshes: list | None = None
 
     @property
     def top(self) -> int:
@@ -269,7 +284,36 @@ def elasticity(nref: int, E: float = 1.0, nu: float = 0.3, seed: int = 0x5EED000
     return _extract(lib, h, 3, name or f"elasticity_p1_{2 ** nref + 1}^3")
 
 
-def _extract(lib, h, dim, name):
+def _meshes(lib, h, nl):
+    out = []
+    for l in range(nl):
+        nn, ne = C.c_int(), C.c_int()
+        xyz, tets = C.POINTER(C.c_double)(), C.POINTER(C.c_int)()
+        kind = C.POINTER(C.c_uint8)()
+        if lib.ugmg_problem_mesh(h, l, C.byref(nn), C.byref(xyz), C.byref(ne), C.byref(tets), C.byref(kind)):
+            return None
+        out.append(Mesh(xyz=_arr(xyz, 3 * nn.value, np.float64).reshape(-1, 3),
+                        tets=_arr(tets, 4 * ne.value, np.int32).reshape(-1, 4),
+                        kind=_arr(kind, ne.value, np.uint8)))
+    return out
+
+
+def elasticity_local(nuni: int, depth: int, E: float = 1.0, nu: float = 0.3, name: str = "") -> Problem:
+    """3D P1 linear elasticity (b = 3, clamped on x = 0) on the Kuhn-split
+    unit cube refined red nuni times, then `depth` times toward the corner
+    (0,0,0) with UG's green closure of tetrahedra (centre node, one son per
+    side triangle) and UG's vector classes (see include/ugmg_problems.h).
+    P.meshes holds each level's tetrahedra.  Right-hand sides are per level:
+    surface_load()."""
+    lib = _load()
+    h = C.c_void_p()
+    rc = lib.ugmg_problem_elasticity_local(nuni, depth, E, nu, C.byref(h))
+    if rc != 0:
+        raise ValueError(f"ugmg_problem_elasticity_local failed ({rc})")
+    return _extract(lib, h, 3, name or f"elasticity_local_{2 ** nuni + 1}^3_{depth}l", meshes=True)
+
+
+def _extract(lib, h, dim, name, meshes=False):
     try:
         nl = lib.ugmg_problem_nlevels(h)
         b = lib.ugmg_problem_block(h)
@@ -301,9 +345,10 @@ def _extract(lib, h, dim, name):
         rb = C.POINTER(C.c_double)()
         lib.ugmg_problem_rhs(h, C.byref(rb))
         rhs = _arr(rb, levels[-1].n * b, np.float64)
+        ms = _meshes(lib, h, nl) if meshes else None
     finally:
         lib.ugmg_problem_free(h)
-    return Problem(dim=dim, levels=levels, rhs=rhs, name=name)
+    return Problem(dim=dim, levels=levels, rhs=rhs, name=name, meshes=ms)
 
 
 def splitmix_uniform(seed: int, count: int, a: float = 0.0, b: float = 1.0) -> np.ndarray:
@@ -370,6 +415,34 @@ def surface_rhs(P: Problem, bl: int, seed: int) -> list:
         f = rng.uniform(-1, 1, lv.n * lv.b)
         for j in range(lv.b):
             f[j::lv.b][(lv.skip >> j) & 1 == 1] = 0.0
+        out.append(f)
+    return out
+
+
+def surface_mask(P: Problem, l: int) -> np.ndarray:
+    """The surface vectors of level l (ON_SURFACE): FINE_GRID_DOF below the
+    top level, NEW_DEFECT (VCLASS >= 2) on it."""
+    lv = P.levels[l]
+    return fine_grid_dof(lv) if l < P.top else lv.vclass >= 2
+
+
+def surface_load(P: Problem, bl: int, seed: int) -> list:
+    """Right-hand sides of a surface solve: a nodal load ~U[-1,1] per
+    component (SplitMix64, drawn for every vector of levels bl..top in order)
+    kept on the surface vectors from FULLREFINELEVEL up, 0 elsewhere and on
+    Dirichlet components; None below bl."""
+    frl = fullrefinelevel(P)
+    out = []
+    s = seed
+    for l, lv in enumerate(P.levels):
+        if l < bl:
+            out.append(None)
+            continue
+        f = splitmix_uniform(s, lv.n * lv.b, -1.0, 1.0)
+        s += 1
+        keep = surface_mask(P, l) if l >= frl else np.zeros(lv.n, dtype=bool)
+        for j in range(lv.b):
+            f[j::lv.b][~keep | ((lv.skip >> j) & 1 == 1)] = 0.0
         out.append(f)
     return out
 

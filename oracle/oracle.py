@@ -142,6 +142,13 @@ def dnrm2x(H, l, x):
     return out[:H.levels[l].b]
 
 
+def ddot(H, l, x, y):
+    out = np.zeros(3)
+    lib().orc_ddot(H.ptr(l), np.ascontiguousarray(x, np.float64).ctypes.data,
+                   np.ascontiguousarray(y, np.float64).ctypes.data, out.ctypes.data)
+    return out[:H.levels[l].b]
+
+
 def l_jac(H, l, d):
     lv = H.levels[l]
     v = np.zeros(lv.n * lv.b)

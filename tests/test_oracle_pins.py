@@ -291,6 +291,54 @@ def test_ex_band_lu_solves(jump3d_17):
         assert ex.bandwidth == np.max(np.abs(pos[rows] - pos[lv.col_idx]))
 
 
+def _graph_level(rows):
+    """Scalar level whose row v lists the diagonal, then rows[v] in order
+    (values: 4 on the diagonal, -1 off it)."""
+    n = len(rows)
+    rp = np.zeros(n + 1, np.int32)
+    ci, vals = [], []
+    for v, nb in enumerate(rows):
+        ci += [v] + list(nb)
+        vals += [4.0] + [-1.0] * len(nb)
+        rp[v + 1] = len(ci)
+    return problems.Level(n=n, b=1, row_ptr=rp, col_idx=np.array(ci, np.int32), vals=np.array(vals),
+                          skip=np.zeros(n, np.uint32), vclass=np.full(n, 3, np.uint8),
+                          vnclass=np.zeros(n, np.uint8))
+
+
+@pytest.mark.parametrize("rows,want,bw", [
+    # 6-cycle 0-3-5-2-4-1-0: BFS from 0 ends at 2; BFS from 2 in row order
+    ([[3, 1], [0, 4], [5, 4], [0, 5], [1, 2], [3, 2]], [2, 5, 4, 3, 1, 0], 2),
+    # a path 4-1-0-2-3 with row 0 listing 2 before 1: the first BFS ends at 4
+    ([[2, 1], [0, 4], [0, 3], [2], [1]], [4, 1, 0, 2, 3], 1),
+    # the same path with row 0 listing 1 first: it ends at 3
+    ([[1, 2], [0, 4], [0, 3], [2], [1]], [3, 2, 0, 1, 4], 1),
+])
+def test_ex_bfs_order_by_hand(rows, want, bw):
+    """EXPreProcess (iter.cc:8774-8811): a breadth-first search from the first
+    vector, visiting each row's off-diagonal entries in list order; a second
+    one from the last vector the first reached gives the new list order.
+    The expected orders are worked out by hand from that rule."""
+    H = O.Hierarchy([_graph_level(rows)])
+    ex = O.Ex(H, 0, reorder=1)
+    assert ex.rc == 0
+    assert list(ex.order) == want
+    assert ex.bandwidth == bw
+
+
+def test_ddot_per_component():
+    import math
+    rng = np.random.default_rng(4)
+    for b in (1, 3):
+        lv = random_block_csr(40, b, 0.2, seed=b)
+        H = O.Hierarchy([lv])
+        x, y = rng.uniform(-1, 1, 40 * b), rng.uniform(-1, 1, 40 * b)
+        got = O.ddot(H, 0, x, y)
+        want = [math.fsum(x[i::b] * y[i::b]) for i in range(b)]
+        assert np.max(np.abs(got - want)) < 1e-14
+        assert np.allclose(O.ddot(H, 0, x, x), O.dnrm2x(H, 0, x) ** 2, rtol=1e-14)
+
+
 def test_ex_zero_pivot_fails_like_ug():
     lv = random_block_csr(6, 1, 0.3, seed=3)
     lv.vals[lv.row_ptr[0]] = 0.0
@@ -303,10 +351,11 @@ def test_ex_zero_pivot_fails_like_ug():
 
 # ------------------------------------------------------------ lmgc and ls
 
-def mg_error_operator(P, top, bl, kind, damp, nu1, nu2, rank=None):
-    """Textbook E_l = S^nu2 (I - P (I - E_{l-1}) A_{l-1}^{-1} R A_l) S^nu1
+def mg_error_operator(P, top, bl, kind, damp, nu1, nu2, rank=None, gamma=1):
+    """Textbook E_l = S^nu2 (I - P (I - E_{l-1}^gamma) A_{l-1}^{-1} R A_l) S^nu1
     with E_bl = 0 (exact coarse solve), R = masked P^T, P masked by fine
-    Dirichlet rows, S = I - damp M^{-1} A (M = diag or colour-ordered D+L)."""
+    Dirichlet rows, S = I - damp M^{-1} A (M = diag or colour-ordered D+L);
+    gamma coarse cycles per level (1: V-cycle, 2: W-cycle)."""
     def smoother_op(l):
         A = P.levels[l].dense()
         if kind == "jac":
@@ -326,7 +375,8 @@ def mg_error_operator(P, top, bl, kind, damp, nu1, nu2, rank=None):
         Pint = Pm * (P.levels[l].skip == 0)[:, None]
         R = (Pm * (P.levels[l - 1].skip == 0)[None, :]).T
         S = smoother_op(l)
-        CG = np.eye(A.shape[0]) - Pint @ (np.eye(Ac.shape[0]) - E[l - 1]) @ np.linalg.solve(Ac, R @ A)
+        Ec = np.linalg.matrix_power(E[l - 1], gamma)
+        CG = np.eye(A.shape[0]) - Pint @ (np.eye(Ac.shape[0]) - Ec) @ np.linalg.solve(Ac, R @ A)
         E[l] = np.linalg.matrix_power(S, nu2) @ CG @ np.linalg.matrix_power(S, nu1)
     return E[top]
 
@@ -344,6 +394,28 @@ def test_lmgc_cycle_is_textbook_vcycle(kind, damp):
     c, bn = O.lmgc_iter(H, top, p, b)
     want_c = (np.eye(A.shape[0]) - E) @ np.linalg.solve(A, b)
     assert np.max(np.abs(c - want_c)) < 1e-12 * np.max(np.abs(want_c))
+    assert np.max(np.abs(bn - (b - A @ c))) < 1e-13 * np.max(np.abs(b))
+
+
+@pytest.mark.parametrize("kind,damp", [("jac", 0.8), ("mcgs", 1.0)])
+def test_lmgc_w_cycle_is_textbook(kind, damp):
+    """gamma = 2: the coarse correction is two cycles on the coarse defect,
+    E_c^2 in the recursion (5 levels, so the W shape reaches level 1 four
+    times)."""
+    P = problems.q1(2, 2, 4, problems.KAPPA_JUMP, seed=12)     # 33x33, 5 levels
+    H = O.Hierarchy(P.levels)
+    top, bl = P.top, 1
+    E = mg_error_operator(P, top, bl, kind, damp, 1, 2, gamma=2)
+    E1 = mg_error_operator(P, top, bl, kind, damp, 1, 2, gamma=1)
+    A = P.levels[top].dense()
+    p = O.params(smoother=O.JAC if kind == "jac" else O.MCGS, damp=damp, baselevel=bl, nu1=1, nu2=2, gamma=2)
+    b = np.random.default_rng(1).uniform(-1, 1, P.levels[top].n)
+    b[P.levels[top].skip == 1] = 0.0
+    c, bn = O.lmgc_iter(H, top, p, b)
+    want_c = (np.eye(A.shape[0]) - E) @ np.linalg.solve(A, b)
+    v_c = (np.eye(A.shape[0]) - E1) @ np.linalg.solve(A, b)
+    assert np.max(np.abs(c - want_c)) < 1e-12 * np.max(np.abs(want_c))
+    assert np.max(np.abs(c - v_c)) > 1e-6 * np.max(np.abs(v_c))     # not the V-cycle
     assert np.max(np.abs(bn - (b - A @ c))) < 1e-13 * np.max(np.abs(b))
 
 

@@ -110,7 +110,3 @@ def test_refusals():
     with pytest.raises(ugmg.UgmgError):
         ctx.solve_surface_host(Q.top, fs, maxiter=2)      # FULLREFINELEVEL below the base level
     ctx.close()
-    ctx = _ctx(Q, ugmg.SMOOTH_JAC, 0.8, precision=ugmg.MIXED)
-    with pytest.raises(ugmg.UgmgError):
-        ctx.solve_surface_host(Q.top, problems.surface_rhs(Q, BL, 1), maxiter=2)   # FP64 only
-    ctx.close()

@@ -140,7 +140,8 @@ struct DevLevel {
   // surface solve (allocated on first use): FINE_GRID_DOF class bytes, the
   // level's x in device order, its b-component squared sums
   uint8_t *fgd = nullptr;
-  double *xs = nullptr, *sq = nullptr;
+  double *xs = nullptr;
+  float *acc32 = nullptr;   // mixed surface solve: a step's summed FP32 corrections
 
   template <typename T> T *cv() const { if constexpr (std::is_same_v<T, double>) return c; else return c32; }
   template <typename T> T *bv() const { if constexpr (std::is_same_v<T, double>) return b; else return b32; }
@@ -198,6 +199,14 @@ struct ugmg_ctx {
   hipGraphExec_t gexec[kMaxGraphs] = {};
   int gkind = -1, glevel = -1;
   const void *gx = nullptr, *gb = nullptr;
+  // cached graphs of one surface-solve iteration (as gexec), and the
+  // per-level squared surface norms (device, pinned host copy)
+  hipGraphExec_t sgexec[kMaxGraphs] = {};
+  int sgkind = -1, sglevel = -1;
+  const void *sgx = nullptr, *sgb = nullptr;
+  double *surf_sq = nullptr;
+  double *surf_sq_host = nullptr;
+  int surf_sq_host_cap = 0;
   // levels bl..fuse_top run inside one fused coarse-cycle kernel
   int fuse_top = -1;
   CLevel<double> *clev64 = nullptr;
@@ -305,15 +314,12 @@ void drop_bcgs_graphs(ugmg_ctx *ctx) {
   ctx->bc.glevel = -1;
 }
 
+void drop_graph(ugmg_ctx *ctx);
+
 void free_device(ugmg_ctx *ctx) {
-  for (hipGraphExec_t &g : ctx->gexec)
-    if (g) {
-      hipGraphExecDestroy(g);
-      g = nullptr;
-    }
-  ctx->gkind = -1;
-  drop_bcgs_graphs(ctx);
+  drop_graph(ctx);
   ctx->bc = ugmg_ctx::Bcgs();
+  ctx->surf_sq = nullptr;
   for (void *p : ctx->allocs) hipFree(p);
   ctx->allocs.clear();
   ctx->dev_bytes = 0.0;
@@ -344,6 +350,12 @@ void drop_graph(ugmg_ctx *ctx) {
       g = nullptr;
     }
   ctx->gkind = -1;
+  for (hipGraphExec_t &g : ctx->sgexec)
+    if (g) {
+      hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+  ctx->sgkind = -1;
   drop_bcgs_graphs(ctx);
 }
 
@@ -1310,6 +1322,7 @@ int ugmg_destroy(ugmg_ctx *ctx) {
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
   free_device(ctx);
   if (ctx->red_host) hipHostFree(ctx->red_host);
+  if (ctx->surf_sq_host) hipHostFree(ctx->surf_sq_host);
   if (ctx->state_host) hipHostFree(ctx->state_host);
   if (ctx->own_stream && ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -1961,6 +1974,153 @@ int ugmg_fullrefinelevel(const ugmg_ctx *ctx) {
 // their from-level); the coarse levels' b carry the defect recurrence from
 // iteration to iteration (lmgc zeroes only what the restriction covers);
 // LSUpdate adds each level's correction into x on baselevel..level.
+//
+// Mixed form: an iteration runs `cycles` FP32 cycles on the FP32 copies of
+// every level's b, sums each coarse level's FP32 corrections, then per
+// level in FP64: b -= A dx, x += dx, b32 = b (the refinement update of the
+// plain solve, on every level of the recurrence), with the surface norms.
+
+namespace {
+
+struct SurfaceRun {
+  int level = 0, bl = 0, frl = 0;
+  std::vector<double *> X, B;   // device-order x and FP64 b per level
+};
+
+inline const uint8_t *surface_cls(const ugmg_ctx *ctx, const SurfaceRun &s, int l) {
+  return l < s.level ? ctx->dev[l].fgd : ctx->dev[l].vclass;
+}
+
+// one iteration, enqueued on the stream; leaves each surface level's
+// per-component squared sums in surf_sq[l * UGMG_MAXB ..]
+void enqueue_surface_iteration(ugmg_ctx *ctx, const SurfaceRun &s, int cycles) {
+  hipStream_t st = ctx->stream;
+  const int level = s.level;
+  if (ctx->p.precision == UGMG_MIXED) {
+    DevLevel &dt = ctx->dev[level];
+    const int k = std::max(1, cycles);
+    for (int j = 0; j < k; ++j) {
+      lmgc<float>(ctx, level, dt.c32, dt.b32, j == 0 ? 0 : 1);
+      for (int l = s.bl; l < level; ++l) {
+        DevLevel &d = ctx->dev[l];
+        if (j == 0) launch_convert<float, float>(st, d.n * d.nb, d.c32, d.acc32, 1, d.vs, d.vs);
+        else launch_add<float>(st, d.n, d.acc32, d.c32, B1(d));
+      }
+    }
+    for (int l = s.bl; l <= level; ++l) {
+      DevLevel &d = ctx->dev[l];
+      int np = 0;
+      launch_ir_update(st, d.A.d64(), l == level ? d.c32 : d.acc32, s.B[l], s.X[l], d.b32,
+                       l >= s.frl ? surface_cls(ctx, s, l) : d.vclass, ctx->part, &np, d.nt, B1(d));
+      if (l >= s.frl) launch_reduce_final(st, np, ctx->part, ctx->surf_sq + (size_t)l * UGMG_MAXB, 0, d.nb);
+    }
+    return;
+  }
+  // c accumulates straight into the top level's x; the coarse levels'
+  // corrections stay in their c and are added after the cycle (LSUpdate)
+  lmgc<double>(ctx, level, s.X[level], s.B[level], 1);
+  for (int l = s.bl; l < level; ++l) launch_add<double>(st, ctx->dev[l].n, s.X[l], ctx->dev[l].c, B1(ctx->dev[l]));
+  for (int l = s.frl; l <= level; ++l) {
+    DevLevel &d = ctx->dev[l];
+    launch_dot<double>(st, d.n, s.B[l], nullptr, surface_cls(ctx, s, l), ctx->part,
+                       ctx->surf_sq + (size_t)l * UGMG_MAXB, 0, B1(d));
+  }
+}
+
+// squared sums -> per-component norms, summed over the surface levels in
+// level order
+void surface_norms(const ugmg_ctx *ctx, const SurfaceRun &s, const double *sq, double *out) {
+  const int nc = ctx->dev[s.level].nb;
+  double acc[UGMG_MAXB] = {0.0, 0.0, 0.0};
+  for (int l = s.frl; l <= s.level; ++l)
+    for (int i = 0; i < nc; ++i) acc[i] += sq[(size_t)l * UGMG_MAXB + i];
+  for (int i = 0; i < nc; ++i) out[i] = std::sqrt(acc[i]);
+}
+
+// enqueue one iteration (from a cached graph when enabled) and the copy of
+// the squared sums to the pinned surf_sq_host
+int launch_surface_iteration(ugmg_ctx *ctx, const SurfaceRun &s, int cycles) {
+  const bool mixed = ctx->p.precision == UGMG_MIXED;
+  cycles = mixed ? std::max(1, cycles) : 1;
+  if (!can_capture(ctx) || cycles >= ugmg_ctx::kMaxGraphs) {
+    enqueue_surface_iteration(ctx, s, cycles);
+    HIPCHK(hipGetLastError());
+  } else {
+    const int kind = ctx->p.precision;
+    if (ctx->sgkind != kind || ctx->sglevel != s.level || ctx->sgx != s.X[s.level] ||
+        ctx->sgb != s.B[s.level]) {
+      for (hipGraphExec_t &g : ctx->sgexec)
+        if (g) {
+          hipGraphExecDestroy(g);
+          g = nullptr;
+        }
+      ctx->sgkind = kind;
+      ctx->sglevel = s.level;
+      ctx->sgx = s.X[s.level];
+      ctx->sgb = s.B[s.level];
+    }
+    hipGraphExec_t &ge = ctx->sgexec[cycles];
+    if (!ge) {
+      hipGraph_t g = nullptr;
+      HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      enqueue_surface_iteration(ctx, s, cycles);
+      HIPCHK(hipStreamEndCapture(ctx->stream, &g));
+      hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+      hipGraphDestroy(g);
+      if (e != hipSuccess) {
+        ge = nullptr;
+        return fail(ctx, UGMG_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+      }
+    }
+    HIPCHK(hipGraphLaunch(ge, ctx->stream));
+  }
+  HIPCHK(hipMemcpyAsync(ctx->surf_sq_host, ctx->surf_sq, sizeof(double) * UGMG_MAXB * (s.level + 1),
+                        hipMemcpyDeviceToHost, ctx->stream));
+  return 0;
+}
+
+// algorithmic bytes of one surface iteration (the byte model of the plain
+// solve, plus the per-level LSUpdate / refinement updates and norms)
+double surface_iteration_bytes(const ugmg_ctx *ctx, const SurfaceRun &s, int cycles) {
+  double by = 0.0;
+  if (ctx->p.precision == UGMG_MIXED) {
+    for (int j = 0; j < cycles; ++j) {
+      by += lmgc_bytes(ctx, s.level, 4, j == 0 ? 0 : 1, false);
+      for (int l = s.bl; l < s.level; ++l) by += vbytes(ctx->dev[l], 4) * (j == 0 ? 2.0 : 3.0);
+    }
+    for (int l = s.bl; l <= s.level; ++l) {
+      const DevLevel &d = ctx->dev[l];
+      by += d.A.bytes(8) + (double)d.n * d.nb * (4.0 + 4.0 + 16.0 + 16.0 + 4.0) + d.n;
+    }
+    return by;
+  }
+  by = lmgc_bytes(ctx, s.level, 8, 1, false);
+  for (int l = s.bl; l < s.level; ++l) by += 3.0 * vbytes(ctx->dev[l], 8);
+  for (int l = s.frl; l <= s.level; ++l) by += vbytes(ctx->dev[l], 8) + ctx->dev[l].n;
+  return by;
+}
+
+// FP32 cycles of a mixed step: inner_cycles, fewer when the per-cycle rate
+// of the previous step (the slowest component's) predicts that fewer bring
+// every component below 0.9 of `lim`
+int adapt_cycles(int kin, int nc, const double *last, const double *prev, int prev_cycles,
+                 const double *lim) {
+  int need = 1;
+  for (int q = 0; q < nc; ++q) {
+    const double goal = 0.9 * std::fabs(lim[q]);
+    if (last[q] <= goal) continue;
+    if (!(prev[q] > 0.0) || !(last[q] > 0.0) || !(goal > 0.0)) return kin;
+    const double rho = std::pow(last[q] / prev[q], 1.0 / prev_cycles);
+    if (!(rho < 1.0) || !(rho > 0.0)) return kin;
+    const double m = std::ceil(std::log(goal / last[q]) / std::log(rho));
+    if (!(m < kin)) return kin;
+    need = std::max(need, (int)m);
+  }
+  return need;
+}
+
+}  // namespace
+
 int ugmg_solve_surface(ugmg_ctx *ctx, int level, double *const *x, double *const *b,
                        const double *abslimit, const double *reduction, int maxiter, int do_defect,
                        ugmg_lresult *res, ugmg_defect_cb cb, void *user) {
@@ -1971,82 +2131,93 @@ int ugmg_solve_surface(ugmg_ctx *ctx, int level, double *const *x, double *const
   std::memset(res, 0, sizeof(*res));
   // the level's own NEW_DEFECT loop always runs, the FINE_GRID_DOF loop
   // from FULLREFINELEVEL up to the level below it
-  const int bl = ctx->bl, frl = std::min(ctx->frl, level);
-  if (frl < bl) return fail(ctx, UGMG_E_ARG, "FULLREFINELEVEL lies below the base level");
-  if (ctx->p.precision != UGMG_FP64) return fail(ctx, UGMG_E_ARG, "the surface solve runs in FP64");
+  SurfaceRun s;
+  s.level = level;
+  s.bl = ctx->bl;
+  s.frl = std::min(ctx->frl, level);
+  const int bl = s.bl;
+  if (s.frl < bl) return fail(ctx, UGMG_E_ARG, "FULLREFINELEVEL lies below the base level");
   for (int l = bl; l <= level; ++l)
     if (!x[l] || !b[l]) return fail(ctx, UGMG_E_ARG, "x[l] and b[l] are needed on baselevel..level");
   if (ctx->base.failed) {
     res->error_code = UGMG_E_SINGULAR;
     return fail(ctx, UGMG_E_SINGULAR, "zero pivot in the band LU of the base level");
   }
+  const bool mixed = ctx->p.precision == UGMG_MIXED;
   hipStream_t st = ctx->stream;
+  // surface work arrays, allocated on first use and kept with the hierarchy
+  if (!ctx->surf_sq && (rc = dalloc(ctx, &ctx->surf_sq, (size_t)UGMG_MAXB * (ctx->top + 1)))) return rc;
+  if (ctx->surf_sq_host_cap < UGMG_MAXB * (ctx->top + 1)) {
+    if (ctx->surf_sq_host) hipHostFree(ctx->surf_sq_host);
+    ctx->surf_sq_host = nullptr;
+    ctx->surf_sq_host_cap = 0;
+    if (hipHostMalloc((void **)&ctx->surf_sq_host, sizeof(double) * UGMG_MAXB * (ctx->top + 1),
+                      hipHostMallocDefault) != hipSuccess)
+      return fail(ctx, UGMG_E_NOMEM, "hipHostMalloc");
+    ctx->surf_sq_host_cap = UGMG_MAXB * (ctx->top + 1);
+  }
   for (int l = bl; l <= level; ++l) {
     DevLevel &d = ctx->dev[l];
     if (!d.fgd) {
       if ((rc = dalloc(ctx, &d.fgd, (size_t)d.n))) return rc;
       launch_fgd_class(st, d.n, d.vclass, d.vnclass, d.fgd);
     }
-    if (!d.sq && (rc = dalloc(ctx, &d.sq, (size_t)8))) return rc;
     if (l < level && !d.xs && (rc = dalloc(ctx, &d.xs, d.vs))) return rc;
+    if (mixed && l < level && !d.acc32 && (rc = dalloc(ctx, &d.acc32, d.vs))) return rc;
   }
   // device-order working copies: the top level iterates on its c (x) and b
   // when it is permuted, a coarse level on xs and its own b (the cycle's)
-  std::vector<double *> X(level + 1, nullptr), B(level + 1, nullptr);
+  s.X.assign(level + 1, nullptr);
+  s.B.assign(level + 1, nullptr);
   for (int l = bl; l <= level; ++l) {
     DevLevel &d = ctx->dev[l];
     if (l == level && !d.perm) {
-      X[l] = x[l];
-      B[l] = b[l];
+      s.X[l] = x[l];
+      s.B[l] = b[l];
       continue;
     }
-    X[l] = l == level ? d.c : d.xs;
-    B[l] = d.b;
-    to_dev<double, double>(ctx, l, x[l], d.vs, X[l]);
-    to_dev<double, double>(ctx, l, b[l], d.vs, B[l]);
+    s.X[l] = l == level ? d.c : d.xs;
+    s.B[l] = d.b;
+    to_dev<double, double>(ctx, l, x[l], d.vs, s.X[l]);
+    to_dev<double, double>(ctx, l, b[l], d.vs, s.B[l]);
   }
+  ctx->stat_cycles = 0;
+  ctx->stat_bytes = 0.0;
   auto move_out = [&]() -> int {
     for (int l = bl; l <= level; ++l) {
-      if (X[l] == x[l]) continue;
-      to_user<double, double>(ctx, l, X[l], x[l], ctx->dev[l].vs);
-      to_user<double, double>(ctx, l, B[l], b[l], ctx->dev[l].vs);
+      if (s.X[l] == x[l]) continue;
+      to_user<double, double>(ctx, l, s.X[l], x[l], ctx->dev[l].vs);
+      to_user<double, double>(ctx, l, s.B[l], b[l], ctx->dev[l].vs);
     }
     HIPCHK(hipStreamSynchronize(st));
     return 0;
   };
-  auto cls = [&](int l) { return l < level ? ctx->dev[l].fgd : ctx->dev[l].vclass; };
   const int nc = ctx->dev[level].nb;
   if (do_defect)   // LinearDefect ON_SURFACE: b -= A x on the surface rows
-    for (int l = frl; l <= level; ++l) {
+    for (int l = s.frl; l <= level; ++l) {
       DevLevel &d = ctx->dev[l];
-      launch_convert<double, double>(st, d.n * d.nb, B[l], d.t, 1, d.vs, d.vs);
-      launch_spmv<double, double, double>(st, d.A.d64(), 0, d.A.nslices, X[l], d.t, nullptr, 0, d.nt, B1(d));
-      launch_copy_rows(st, d.n, d.nb, cls(l), d.t, B[l]);
+      launch_convert<double, double>(st, d.n * d.nb, s.B[l], d.t, 1, d.vs, d.vs);
+      launch_spmv<double, double, double>(st, d.A.d64(), 0, d.A.nslices, s.X[l], d.t, nullptr, 0, d.nt, B1(d));
+      launch_copy_rows(st, d.n, d.nb, surface_cls(ctx, s, l), d.t, s.B[l]);
+      ctx->stat_bytes += op_bytes(ctx, l, Op::Spmv, 8) + 2.0 * vbytes(d, 8) + d.n;
     }
   // dnrm2x ON_SURFACE: per-level squared sums, added in level order
-  auto norm = [&](double *out) -> int {
-    for (int l = frl; l <= level; ++l) {
-      DevLevel &d = ctx->dev[l];
-      launch_dot<double>(st, d.n, B[l], nullptr, cls(l), ctx->part, d.sq, 0, B1(d));
-    }
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(st));
-    double acc[UGMG_MAXB] = {0.0, 0.0, 0.0};
-    for (int l = frl; l <= level; ++l) {
-      double q[UGMG_MAXB];
-      HIPCHK(hipMemcpy(q, ctx->dev[l].sq, sizeof(double) * nc, hipMemcpyDeviceToHost));
-      for (int i = 0; i < nc; ++i) acc[i] += q[i];
-    }
-    for (int i = 0; i < nc; ++i) out[i] = std::sqrt(acc[i]);
-    return 0;
-  };
+  for (int l = s.frl; l <= level; ++l) {
+    DevLevel &d = ctx->dev[l];
+    launch_dot<double>(st, d.n, s.B[l], nullptr, surface_cls(ctx, s, l), ctx->part,
+                       ctx->surf_sq + (size_t)l * UGMG_MAXB, 0, B1(d));
+    ctx->stat_bytes += vbytes(d, 8) + d.n;
+  }
+  HIPCHK(hipMemcpyAsync(ctx->surf_sq_host, ctx->surf_sq, sizeof(double) * UGMG_MAXB * (level + 1),
+                        hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
   auto sc_cmp = [nc](const double *u, const double *v) {
     for (int i = 0; i < nc; ++i)
       if (!sc_cmp1(u[i], v[i])) return false;
     return true;
   };
   double first[UGMG_MAXB] = {0.0, 0.0, 0.0}, target[UGMG_MAXB], last[UGMG_MAXB];
-  if ((rc = norm(first))) return rc;
+  surface_norms(ctx, s, ctx->surf_sq_host, first);
   for (int i = 0; i < nc; ++i) {
     res->first_defect[i] = res->last_defect[i] = first[i];
     target[i] = first[i] * reduction[i];
@@ -2054,19 +2225,35 @@ int ugmg_solve_surface(ugmg_ctx *ctx, int level, double *const *x, double *const
   }
   if (cb) cb(user, 0, nc, first);
   int conv = sc_cmp(first, abslimit);
+  if (mixed)   // the FP32 copies of every level's b
+    for (int l = bl; l <= level; ++l) {
+      DevLevel &d = ctx->dev[l];
+      launch_convert<float, double>(st, d.n * d.nb, s.B[l], d.b32, 1, d.vs, d.vs);
+      ctx->stat_bytes += vbytes(d, 12);
+    }
+  const int kin = std::max(1, ctx->p.inner_cycles);
+  const bool adapt = mixed && ctx->p.inner_adapt && kin > 1;
+  double prev[UGMG_MAXB] = {0.0, 0.0, 0.0};
+  int prev_cycles = 0;
+  for (int k = 0; k < nc; ++k) last[k] = first[k];
   int i;
   for (i = 0; i < maxiter; ++i) {
     if (conv) break;
-    // c accumulates straight into the top level's x; the coarse levels'
-    // corrections stay in their c and are added after the cycle (LSUpdate)
-    lmgc<double>(ctx, level, X[level], B[level], 1);
-    for (int l = bl; l < level; ++l)
-      launch_add<double>(st, ctx->dev[l].n, X[l], ctx->dev[l].c, B1(ctx->dev[l]));
-    if ((rc = norm(last))) {
+    int cycles = mixed ? kin : 1;
+    if (adapt && prev_cycles > 0)
+      cycles = std::min(adapt_cycles(kin, nc, last, prev, prev_cycles, target),
+                        adapt_cycles(kin, nc, last, prev, prev_cycles, abslimit));
+    for (int k = 0; k < nc; ++k) prev[k] = last[k];
+    prev_cycles = cycles;
+    if ((rc = launch_surface_iteration(ctx, s, cycles)) ||
+        (hipStreamSynchronize(st) != hipSuccess && (rc = fail(ctx, UGMG_E_HIP, "hipStreamSynchronize")))) {
       res->error_code = rc;
       move_out();
       return rc;
     }
+    ctx->stat_cycles += cycles;
+    ctx->stat_bytes += surface_iteration_bytes(ctx, s, cycles);
+    surface_norms(ctx, s, ctx->surf_sq_host, last);
     for (int k = 0; k < nc; ++k) res->last_defect[k] = last[k];
     if (cb) cb(user, i + 1, nc, last);
     if (sc_cmp(last, abslimit) || sc_cmp(last, target)) {

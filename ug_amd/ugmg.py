@@ -372,6 +372,25 @@ class Context:
             self._chk(r)
         return r
 
+    def solve_surface(self, level, xs, bs, red=1e-10, abslimit=1e-300, maxiter=50, do_defect=True,
+                      record=True):
+        """ugmg_solve_surface on device tensors xs[l], bs[l] for levels
+        baselevel..level (None below); returns (LResult, defects list)."""
+        XP = (C.c_void_p * (level + 1))(*[None if v is None else v.data_ptr() for v in xs[:level + 1]])
+        BP = (C.c_void_p * (level + 1))(*[None if v is None else v.data_ptr() for v in bs[:level + 1]])
+        res = LResult()
+        defects = []
+
+        def _cb(user, it, ncomp, d):
+            defects.append([d[i] for i in range(ncomp)])
+
+        cb = DEFECT_CB(_cb) if record else DEFECT_CB()
+        ab = (C.c_double * 3)(abslimit, abslimit, abslimit)
+        rd = (C.c_double * 3)(red, red, red)
+        self._chk(self.lib.ugmg_solve_surface(self.h, level, XP, BP, C.addressof(ab), C.addressof(rd),
+                                              maxiter, int(do_defect), C.byref(res), cb, None))
+        return res, defects
+
     def solve_surface_host(self, level, rhs, x0=None, red=1e-10, abslimit=1e-300, maxiter=50,
                            do_defect=True):
         """ugmg_solve_surface_host: rhs[l] (and x0[l]) for levels baselevel..level

@@ -314,12 +314,20 @@ int ugmg_coarse_inverse(int n, const int *row_ptr, const int *col_idx,
 
 /* ---- serving: continuous batching of independent solves ----------------
  * No counterpart in UG (its ugd daemon is a remote UI, ugd.cc:528-608).
- * Each request is a whole hierarchy + rhs; worker threads build the layouts
- * (taken in submission order), and ugmg_server_step admits the first
- * prepared request in submission order when a slot and the HBM budget allow
- * (one waiting for memory holds back those behind it), then keeps one ls
- * iteration of every admitted request in flight on that request's own
- * stream.  Results equal those of a stand-alone ugmg_solve of the request. */
+ * Each request is a whole hierarchy + rhs (scalar, FP64 solves).  Requests
+ * with the same hierarchy (patterns, transfers, classes, Dirichlet bits)
+ * and parameters share a resident group: one device hierarchy holding up
+ * to max_slots matrices of that pattern, one per column of its vectors,
+ * whose layout worker threads build once.  ugmg_server_step admits queued
+ * requests in submission order into free slots of idle groups (a request
+ * whose group is still being laid out holds back none) while the total of
+ * admitted requests stays within max_slots; a group is set up on the device
+ * when the HBM budget allows, releasing idle groups least recently used
+ * first.  Each step launches the next ls iteration of every idle group as
+ * one batched launch sequence over its occupied columns (a captured graph
+ * per column count), so concurrent solves of one shape share every kernel
+ * launch and groups overlap through their streams.  Results equal those of
+ * a stand-alone ugmg_solve of the request, bit for bit. */
 typedef struct {
   int n;
   const int *row_ptr, *col_idx;
@@ -341,6 +349,11 @@ int ugmg_server_submit(ugmg_server *s, int nlevels, const ugmg_level_desc *level
                        const ugmg_params *p, const double *rhs, double red,
                        double abslimit, int maxiter, int *id);
 int ugmg_server_step(ugmg_server *s, int *nactive, int *nqueued, int *ndone);
+/* resident groups (hierarchies on the device), group iterations launched
+ * (each one batched launch sequence over its slots) and the solve
+ * iterations they carried: columns / launches = solves per launch */
+int ugmg_server_stats(const ugmg_server *s, int *groups, long long *launches,
+                      long long *columns);
 /* times[3]: submit, admission, completion in seconds since creation */
 int ugmg_server_status(ugmg_server *s, int id, int *state, ugmg_lresult *res,
                        double *times);

@@ -476,6 +476,47 @@ def test_server_results_equal_standalone_solves():
     srv.close()
 
 
+def test_server_batches_same_shape_solves_bit_identically():
+    """Six requests of two shapes (the 17^3 jump-coefficient hierarchy with
+    three kappa draws, so three matrices of one pattern, and 65^2 Poisson
+    with three right-hand sides), four slots: each shape runs as one
+    resident group whose iterations are single batched launch sequences
+    over its occupied slots, the last two requests join groups that are
+    already iterating, and every result equals its stand-alone solve bit
+    for bit."""
+    from ug_amd import problems, ugmg
+    probs = [problems.jump3d(4, seed=11), problems.q1(2, 2, 5, problems.KAPPA_ONE, seed=21),
+             problems.jump3d(4, seed=12), problems.q1(2, 2, 5, problems.KAPPA_ONE, seed=22),
+             problems.jump3d(4, seed=13), problems.q1(2, 2, 5, problems.KAPPA_ONE, seed=23)]
+    kw = dict(nu1=2, nu2=2, gamma=1, baselevel=1, smoother=ugmg.SMOOTH_GS, damp=1.0, base_abslimit=1e-300,
+              use_graph=1)
+    alone = []
+    for P in probs:
+        ctx = ugmg.Context(0)
+        ctx.set_hierarchy(P)
+        ctx.set_params(**kw)
+        ctx.preprocess()
+        x, b, res, d = ctx.solve_host(P.top, P.rhs, red=1e-8, maxiter=200)
+        assert res.converged
+        alone.append((x, res.number_of_linear_iterations))
+        ctx.close()
+    assert not np.array_equal(probs[0].levels[-1].vals, probs[2].levels[-1].vals)
+    srv = ugmg.Server(0, budget_bytes=8e9, max_slots=4)
+    ids = [srv.submit(P, ugmg.default_params(**kw), red=1e-8, maxiter=200) for P in probs]
+    srv.drain(len(ids))
+    groups, launches, columns = srv.stats()
+    print(f"groups {groups}, group iterations {launches}, solve iterations {columns}")
+    assert groups == 2
+    assert columns == sum(nit for _, nit in alone)
+    assert columns > launches          # launches carried several solves
+    for P, rid, (xa, nit) in zip(probs, ids, alone):
+        st, res, times = srv.status(rid)
+        assert st == ugmg.REQ_DONE and res.converged
+        assert res.number_of_linear_iterations == nit
+        assert np.array_equal(srv.result(rid, P.levels[-1].n), xa)
+    srv.close()
+
+
 def test_server_budget_rejects_oversized_request():
     from ug_amd import problems, ugmg
     P = problems.jump3d(4, seed=5)

@@ -99,8 +99,12 @@ struct DevSellBuf {
     for (int s = s0; s < s1; ++s) m = std::max(m, width_h[s]);
     return m;
   }
-  DSell<double> d64() const { return {nslices, soff, rows, col, v64, sdoff, dtab, wmax, nent}; }
-  DSell<float> d32() const { return {nslices, soff, rows, col, v32, sdoff, dtab, wmax, nent}; }
+  // members: values of member m at v64/v32 + m * mst; the level's diagonal
+  // blocks of member m at diag + m * dms (0: one matrix)
+  size_t mst = 0, dms = 0;
+  size_t nval = 0;              // values of one member (bb planes of nent)
+  DSell<double> d64() const { return {nslices, soff, rows, col, v64, sdoff, dtab, wmax, nent, mst, dms}; }
+  DSell<float> d32() const { return {nslices, soff, rows, col, v32, sdoff, dtab, wmax, nent, mst, dms}; }
   template <typename T>
   DSell<T> d() const {
     if constexpr (std::is_same_v<T, double>) return d64();
@@ -142,6 +146,11 @@ struct DevLevel {
   uint8_t *fgd = nullptr;
   double *xs = nullptr;
   float *acc32 = nullptr;   // mixed surface solve: a step's summed FP32 corrections
+  // members: for every value slot of the A, L and U slabs and of the
+  // diagonal blocks, the index of its value in the level's list-order CSR
+  // values (-1: padding or phantom); diag member stride
+  int *map_a = nullptr, *map_l = nullptr, *map_u = nullptr, *map_d = nullptr;
+  size_t dms = 0;
 
   template <typename T> T *cv() const { if constexpr (std::is_same_v<T, double>) return c; else return c32; }
   template <typename T> T *bv() const { if constexpr (std::is_same_v<T, double>) return b; else return b32; }
@@ -199,6 +208,14 @@ struct ugmg_ctx {
   hipGraphExec_t gexec[kMaxGraphs] = {};
   int gkind = -1, glevel = -1;
   const void *gx = nullptr, *gb = nullptr;
+  // members: `members` matrices of one pattern, one per column of the
+  // level vectors (batched launches read column j's values, diagonal
+  // blocks and base inverse); set through member_set
+  int members = 1;
+  size_t base_ims = 0;          // values of one member's compact base inverse
+  int *base_act = nullptr;      // rows of the inverse kept in the compact copy
+  double *stage = nullptr;      // staging of one level's member values
+  double *base_stage = nullptr; // staging of a member's full base inverse
   // cached graphs of one surface-solve iteration (as gexec), and the
   // per-level squared surface norms (device, pinned host copy)
   hipGraphExec_t sgexec[kMaxGraphs] = {};
@@ -271,7 +288,7 @@ int upload(ugmg_ctx *ctx, T **out, const T *src, size_t n) {
   return 0;
 }
 
-int upload_sell(ugmg_ctx *ctx, const HostSell &S, DevSellBuf &D) {
+int upload_sell(ugmg_ctx *ctx, const HostSell &S, DevSellBuf &D, int members = 1) {
   D.nslices = S.nslices;
   D.nent = S.col.size();
   D.width_h.resize(S.nslices);
@@ -299,10 +316,26 @@ int upload_sell(ugmg_ctx *ctx, const HostSell &S, DevSellBuf &D) {
   }
   D.ndtab = S.dtab.size();
   D.bb = S.bb;
-  if ((rc = upload(ctx, &D.v64, S.val.data(), S.val.size()))) return rc;
+  D.nval = S.val.size();
   hvec<float> v32;
   to_float(S.val.data(), S.val.size(), v32);
-  if ((rc = upload(ctx, &D.v32, v32.data(), v32.size()))) return rc;
+  const int m = members;
+  if ((rc = dalloc(ctx, &D.v64, D.nval * m)) || (rc = dalloc(ctx, &D.v32, D.nval * m))) return rc;
+  for (int k = 0; k < m && D.nval; ++k) {
+    HIPCHK(hipMemcpy(D.v64 + k * D.nval, S.val.data(), D.nval * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(D.v32 + k * D.nval, v32.data(), D.nval * sizeof(float), hipMemcpyHostToDevice));
+  }
+  D.mst = m > 1 ? D.nval : 0;
+  return 0;
+}
+
+// the same for a per-row array of a level (the diagonal blocks)
+template <typename T>
+int upload_members(ugmg_ctx *ctx, T **out, const T *src, size_t n, int members) {
+  int rc = dalloc(ctx, out, n * members);
+  if (rc) return rc;
+  for (int k = 0; k < members && n; ++k)
+    HIPCHK(hipMemcpy(*out + k * n, src, n * sizeof(T), hipMemcpyHostToDevice));
   return 0;
 }
 
@@ -320,6 +353,10 @@ void free_device(ugmg_ctx *ctx) {
   drop_graph(ctx);
   ctx->bc = ugmg_ctx::Bcgs();
   ctx->surf_sq = nullptr;
+  ctx->base_ims = 0;
+  ctx->base_act = nullptr;
+  ctx->stage = nullptr;
+  ctx->base_stage = nullptr;
   for (void *p : ctx->allocs) hipFree(p);
   ctx->allocs.clear();
   ctx->dev_bytes = 0.0;
@@ -359,7 +396,10 @@ void drop_graph(ugmg_ctx *ctx) {
   drop_bcgs_graphs(ctx);
 }
 
-inline Batch batch(const ugmg_ctx *ctx, int l) { return Batch{ctx->nb, ctx->dev[l].vs, ctx->dev[l].nb}; }
+inline Batch batch(const ugmg_ctx *ctx, int l) {
+  const DevLevel &d = ctx->dev[l];
+  return Batch{ctx->nb, d.vs, d.nb, d.dms, ctx->base_ims};
+}
 inline Batch B1(const DevLevel &d) { return Batch{1, d.vs, d.nb}; }
 
 // the caller's vectors (list order, column stride ld) to the level's device
@@ -557,7 +597,7 @@ void lmgc(ugmg_ctx *ctx, int l, T *c, T *b, int cmode, bool c0_done = false) {
                                     dc.bv<T>(), to_base ? ctx->base_part : nullptr, d.ntv<T>(), nb,
                                     d.vs, dc.vs, d.nb, pre0 ? dc.c0 : nullptr, dc.diag<T>(),
                                     pre0 ? dc.tv<T>() : nullptr, bcol ? ctx->base_cpos : nullptr,
-                                    dc.uv<T>());
+                                    dc.uv<T>(), dc.dms);
   if (p.gamma <= 0) launch_fill<T>(st, dc.n, dc.cv<T>(), (T)0, batch(ctx, l - 1));
   for (int g = 0; g < p.gamma; ++g) {
     if (to_base) {
@@ -959,7 +999,24 @@ struct PrepState {
   std::vector<HostPrep> hp;
   std::vector<HostLevel> plv;   // levels in device order (GS levels)
   HostBase base;
+  // members: per level, the CSR index of every slab value (see DevLevel)
+  std::vector<std::vector<int>> map_a, map_l, map_u, map_d;
 };
+
+// the layout of `hp` built from an index-valued copy of the level (value k
+// + 1 at CSR value k): the builders only move values, so each slab value
+// slot then holds its source index (0 for padding and phantoms)
+void index_maps(const HostPrep &idx, std::vector<int> &ma, std::vector<int> &ml,
+                std::vector<int> &mu, std::vector<int> &md) {
+  auto conv = [](const auto &src, std::vector<int> &dst) {
+    dst.resize(src.size());
+    for (size_t q = 0; q < src.size(); ++q) dst[q] = (int)src[q] - 1;
+  };
+  conv(idx.A.val, ma);
+  conv(idx.L.val, ml);
+  conv(idx.U.val, mu);
+  conv(idx.diag, md);
+}
 
 int prep_host(ugmg_ctx *ctx, int level, PrepState &ps) {
   if (ctx->nlevels <= 0) return fail(ctx, UGMG_E_STATE, "ugmg_set_nlevels first");
@@ -1015,6 +1072,29 @@ int prep_host(ugmg_ctx *ctx, int level, PrepState &ps) {
       for (int q = 0; q < H.n; ++q) colp[q] = col[hp.perm[q]];
       build_prep(plv[l], coarse, &colp, nc, hp);
       hp.color_ug = col;
+    }
+    if (ctx->members > 1) {
+      ps.map_a.assign(level + 1, {});
+      ps.map_l.assign(level + 1, {});
+      ps.map_u.assign(level + 1, {});
+      ps.map_d.assign(level + 1, {});
+      for (int l = bl; l <= level; ++l) {
+        HostLevel I = ctx->host[l];
+        for (size_t q = 0; q < I.a.size(); ++q) I.a[q] = (double)(q + 1);
+        I.has_p = false;
+        HostPrep hi;
+        const HostPrep &hp = ps.hp[l];
+        if (hp.perm.empty()) {
+          build_prep(I, nullptr, nullptr, 0, hi);
+        } else {
+          HostLevel IP;
+          std::vector<int> ipos(I.n);
+          for (int q = 0; q < I.n; ++q) ipos[hp.perm[q]] = q;
+          permute_level(I, hp.perm, ipos, nullptr, IP);
+          build_prep(IP, nullptr, &hp.color, hp.ncolors, hi);
+        }
+        index_maps(hi, ps.map_a[l], ps.map_l[l], ps.map_u[l], ps.map_d[l]);
+      }
     }
     // EXPreProcess: order and band here.  The band LU and the inverse of a
     // large base level run on the device in prep_device; a small one is
@@ -1098,11 +1178,17 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
       D.nb = ctx->b;
       D.nnzb = H.nnzb;
       max_slices = std::max(max_slices, hp.A.nslices);
-      if ((rc = upload_sell(ctx, hp.A, D.A))) return rc;
-      if ((rc = upload(ctx, &D.diag64, hp.diag.data(), hp.diag.size()))) return rc;
+      const int mb = ctx->members;
+      if ((rc = upload_sell(ctx, hp.A, D.A, mb))) return rc;
+      if ((rc = upload_members(ctx, &D.diag64, hp.diag.data(), hp.diag.size(), mb))) return rc;
       hvec<float> d32;
       to_float(hp.diag.data(), hp.diag.size(), d32);
-      if ((rc = upload(ctx, &D.diag32, d32.data(), d32.size()))) return rc;
+      if ((rc = upload_members(ctx, &D.diag32, d32.data(), d32.size(), mb))) return rc;
+      D.dms = mb > 1 ? hp.diag.size() : 0;
+      if (mb > 1 &&
+          ((rc = upload(ctx, &D.map_a, ps.map_a[l].data(), ps.map_a[l].size())) ||
+           (rc = upload(ctx, &D.map_d, ps.map_d[l].data(), ps.map_d[l].size()))))
+        return rc;
       if ((rc = upload(ctx, &D.vclass, H.vc.data(), H.vc.size()))) return rc;
       if ((rc = upload(ctx, &D.vnclass, H.vnc.data(), H.vnc.size()))) return rc;
       if ((rc = upload(ctx, &D.skip, H.skip.data(), H.skip.size()))) return rc;
@@ -1110,8 +1196,12 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
         D.ncolors = hp.ncolors;
         D.color_slices = hp.color_slices;
         D.color = hp.color_ug;
-        if ((rc = upload_sell(ctx, hp.L, D.L))) return rc;
-        if ((rc = upload_sell(ctx, hp.U, D.U))) return rc;
+        if ((rc = upload_sell(ctx, hp.L, D.L, mb))) return rc;
+        if ((rc = upload_sell(ctx, hp.U, D.U, mb))) return rc;
+        if (mb > 1 &&
+            ((rc = upload(ctx, &D.map_l, ps.map_l[l].data(), ps.map_l[l].size())) ||
+             (rc = upload(ctx, &D.map_u, ps.map_u[l].data(), ps.map_u[l].size()))))
+          return rc;
         std::vector<uint8_t> c0(H.n);
         for (int q = 0; q < H.n; ++q) c0[q] = hp.color[q] == 0;
         D.color_rows.assign(D.ncolors, 0);
@@ -1126,6 +1216,7 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
           D.color_wmax_u[k] = D.U.max_width(D.color_slices[k], D.color_slices[k + 1]);
         }
       }
+      D.A.dms = D.L.dms = D.U.dms = D.dms;
       if (l > bl) {
         D.p_nnz = H.pp[H.n];
         D.pt_nnz = (int)hp.pt_idx.size();
@@ -1196,14 +1287,17 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
         keep.insert(keep.end(), drop.begin(), drop.end());
         const size_t ne = std::max<size_t>(1, (size_t)nact * nsc);
         if (act.empty()) act.push_back(0);   // (nothing to copy; keeps the upload non-empty)
-        int *act_d = nullptr;
+        const int mb = ctx->members;
         if ((rc = upload(ctx, &ctx->base_rowmap, rowmap.data(), rowmap.size())) ||
             (rc = upload(ctx, &ctx->base_colperm, keep.data(), keep.size())) ||
-            (rc = upload(ctx, &act_d, act.data(), act.size())) ||
-            (rc = dalloc(ctx, &ctx->base_cinv, ne)) || (rc = dalloc(ctx, &ctx->base_cinv32, ne)))
+            (rc = upload(ctx, &ctx->base_act, act.data(), act.size())) ||
+            (rc = dalloc(ctx, &ctx->base_cinv, ne * mb)) || (rc = dalloc(ctx, &ctx->base_cinv32, ne * mb)))
           return rc;
-        launch_base_compact(ctx->stream, nact, nsc, act_d, ctx->base_colperm, ctx->base_inv,
-                            ctx->base_cinv, ctx->base_cinv32);
+        for (int k = 0; k < mb; ++k)
+          launch_base_compact(ctx->stream, nact, nsc, ctx->base_act, ctx->base_colperm, ctx->base_inv,
+                              ctx->base_cinv + k * ne, ctx->base_cinv32 + k * ne);
+        ctx->base_ims = mb > 1 ? ne : 0;
+        if (mb > 1 && (rc = dalloc(ctx, &ctx->base_stage, (size_t)nsc * nsc))) return rc;
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(ctx->stream));
       }
@@ -1211,7 +1305,12 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
     // fused coarse levels (opt-in): the base level and every level above it
     // up to fuse_rows vectors (and at most kMaxColors colours)
     ctx->fuse_top = -1;
-    if (ps.fuse_rows > 0 && ctx->dev[bl].n <= 2048 && ctx->b == 1) {
+    if (ctx->members > 1) {
+      size_t mx = 1;
+      for (int l = bl; l <= level; ++l) mx = std::max(mx, ctx->host[l].a.size());
+      if ((rc = dalloc(ctx, &ctx->stage, mx))) return rc;
+    }
+    if (ps.fuse_rows > 0 && ctx->dev[bl].n <= 2048 && ctx->b == 1 && ctx->members == 1) {
       ctx->fuse_top = bl;
       for (int l = bl + 1; l <= level; ++l) {
         if (ctx->dev[l].n > ps.fuse_rows || ctx->dev[l].ncolors > kMaxColors) break;
@@ -2668,17 +2767,22 @@ int ugmg_time_kernel(ugmg_ctx *ctx, int level, int kind, int reps, double *ms, d
 
 // ================================================================= serving
 //
-// Continuous batching of independent solves.  Every submitted request gets
-// its own context (its own stream); kSrvWorkers threads build the host
-// layouts (prep_host, taken in submission order) while the GPU works on
-// admitted requests; ugmg_server_step() admits prepared requests in
-// submission order -- a request whose layout is still being built does not
-// hold back the prepared ones behind it -- while slots and the HBM budget
-// allow, uploads them, and keeps one solver iteration of every
-// admitted request in flight, polling each request's norm copy with an
-// event.  Requests therefore overlap on the device (one stream each), join as
-// soon as a slot and memory free up, and run exactly the kernels of a
-// stand-alone ugmg_solve, so every result is the same alone and in the mix.
+// Continuous batching of independent solves over resident, batched
+// hierarchies.  Requests with the same hierarchy (pattern, transfers,
+// classes and Dirichlet bits on every level) and the same solver parameters
+// share a group: one context whose levels hold `cap` members -- one matrix
+// per column of the level vectors, one pattern -- whose host layout is built
+// once (on kSrvWorkers threads) and which stays resident while the HBM
+// budget allows.  A request joins a free slot of its group at an iteration
+// boundary: its matrices are gathered into the slot's value planes (device
+// maps from its CSR values to slab positions), its base level is factored,
+// its rhs and a zero x enter the slot's column.  Every ls iteration of a
+// group is one batched launch sequence over columns 0..w-1 (w = the highest
+// occupied slot + 1), replayed from a graph captured per w, so the
+// same-level kernels of all of a group's solves run as single launches;
+// groups overlap on the device through their streams.  Each column runs
+// exactly the kernels of a stand-alone ugmg_solve of its request on its own
+// values, so every result is the same alone and in the mix.
 
 #include <chrono>
 #include <condition_variable>
@@ -2690,28 +2794,98 @@ int ugmg_time_kernel(ugmg_ctx *ctx, int level, int kind, int reps, double *ms, d
 
 namespace {
 
+constexpr int kSrvWorkers = 4;
+constexpr int kSrvMaxCap = 64;
+
 struct SrvReq {
   int id = 0;
   int state = UGMG_REQ_QUEUED;
   double t_submit = 0.0, t_admit = -1.0, t_done = -1.0;
-  ugmg_ctx *ctx = nullptr;
-  int top = 0;
+  int top = 0, n = 0;
+  int group = -1, slot = -1;
+  std::vector<std::vector<double>> vals;   // per level: list-order CSR values
   std::vector<double> rhs;
   double red = 1e-8, abslimit = 1e-300;
   int maxiter = 100;
-  double est_bytes = 0.0, charged = 0.0;
-  PrepState ps;
-  bool host_ready = false;
-  int prep_rc = 0;
-  double *x = nullptr, *b = nullptr;
-  int phase = 0, it = 0;
+  int it = 0;
   double target = 0.0;
-  bool pending = false;
-  hipEvent_t ev = nullptr;
   ugmg_lresult res{};
   std::vector<double> xout;
   std::string err;
 };
+
+struct SrvGroup {
+  uint64_t key = 0;
+  ugmg_ctx *ctx = nullptr;
+  int top = 0, cap = 1;
+  PrepState ps;
+  bool host_ready = false, dev_ready = false, dead = false;
+  int prep_rc = 0;
+  std::string err;
+  double est_bytes = 0.0, charged = 0.0;
+  std::vector<int> slot;        // request id per slot (-1: free)
+  bool busy = false;            // an iteration is in flight
+  int width = 0;                // its columns
+  hipEvent_t ev = nullptr;
+  double *x = nullptr, *b = nullptr;   // top level, cap columns, device order
+  double *norms = nullptr;             // pinned, cap
+  double last_used = 0.0;
+  int occupied() const {
+    int k = 0;
+    for (int r : slot) k += r >= 0;
+    return k;
+  }
+};
+
+uint64_t fnv(uint64_t h, const void *p, size_t n) {
+  const unsigned char *c = static_cast<const unsigned char *>(p);
+  for (size_t i = 0; i < n; ++i) h = (h ^ c[i]) * 1099511628211ULL;
+  return h;
+}
+
+// everything a group shares: the level patterns, transfers, classes,
+// Dirichlet bits and the solver parameters
+uint64_t group_key(int nlevels, const ugmg_level_desc *lv, const ugmg_params &p) {
+  uint64_t h = fnv(1469598103934665603ULL, &nlevels, sizeof nlevels);
+  h = fnv(h, &p, sizeof p);
+  for (int l = 0; l < nlevels; ++l) {
+    const ugmg_level_desc &L = lv[l];
+    const int n = L.n, nnz = L.row_ptr[L.n];
+    h = fnv(h, &n, sizeof n);
+    h = fnv(h, L.row_ptr, sizeof(int) * (n + 1));
+    h = fnv(h, L.col_idx, sizeof(int) * nnz);
+    h = fnv(h, L.skip, sizeof(uint32_t) * n);
+    h = fnv(h, L.vclass, n);
+    h = fnv(h, L.vnclass, n);
+    if (l > 0) {
+      h = fnv(h, L.p_ptr, sizeof(int) * (n + 1));
+      h = fnv(h, L.p_idx, sizeof(int) * L.p_ptr[n]);
+      h = fnv(h, L.p_w, sizeof(double) * L.p_ptr[n]);
+    }
+  }
+  return h;
+}
+
+// the same data compared in full against a group's host levels
+bool same_shape(const ugmg_ctx *ctx, int nlevels, const ugmg_level_desc *lv) {
+  if (ctx->nlevels != nlevels) return false;
+  for (int l = 0; l < nlevels; ++l) {
+    const HostLevel &H = ctx->host[l];
+    const ugmg_level_desc &L = lv[l];
+    if (H.n != L.n || H.nnzb != L.row_ptr[L.n]) return false;
+    if (!std::equal(H.rp.begin(), H.rp.end(), L.row_ptr) ||
+        !std::equal(H.ci.begin(), H.ci.end(), L.col_idx) ||
+        !std::equal(H.skip.begin(), H.skip.end(), L.skip) ||
+        !std::equal(H.vc.begin(), H.vc.end(), L.vclass) ||
+        !std::equal(H.vnc.begin(), H.vnc.end(), L.vnclass))
+      return false;
+    if (l > 0 && (!H.has_p || !std::equal(H.pp.begin(), H.pp.end(), L.p_ptr) ||
+                  !std::equal(H.pi.begin(), H.pi.end(), L.p_idx) ||
+                  !std::equal(H.pw.begin(), H.pw.end(), L.p_w)))
+      return false;
+  }
+  return true;
+}
 
 double est_device_bytes(const ugmg_ctx *ctx, int top, int bl) {
   double by = 0.0;
@@ -2729,23 +2903,128 @@ double est_device_bytes(const ugmg_ctx *ctx, int top, int bl) {
 
 }  // namespace
 
-// host layout builders of a server (prep_host runs OpenMP loops inside)
-constexpr int kSrvWorkers = 4;
+// ------------------------------------------------------------ members
+// Member m of a context with members: its matrices (per level, list-order
+// CSR values of the context's pattern) into its value planes, diagonal
+// blocks and base inverse (EXPreProcess's band LU of its base level on the
+// host, as for a stand-alone context with a small base level).  The base
+// level's unit rows must be the prototype's.  Synchronous.
+static int member_set(ugmg_ctx *ctx, int m, const std::vector<std::vector<double>> &vals) {
+  if (m < 0 || m >= ctx->members) return fail(ctx, UGMG_E_ARG, "member out of range");
+  hipStream_t st = ctx->stream;
+  const int bl = ctx->bl, top = ctx->top;
+  for (int l = bl; l <= top; ++l) {
+    DevLevel &D = ctx->dev[l];
+    const HostLevel &H = ctx->host[l];
+    if (vals[l].size() != H.a.size()) return fail(ctx, UGMG_E_ARG, "member values of the wrong size");
+    HIPCHK(hipMemcpyAsync(ctx->stage, vals[l].data(), sizeof(double) * H.a.size(), hipMemcpyHostToDevice, st));
+    launch_member_gather(st, D.A.nval, D.map_a, ctx->stage, D.A.v64 + m * D.A.nval, D.A.v32 + m * D.A.nval);
+    launch_member_gather(st, D.dms, D.map_d, ctx->stage, D.diag64 + m * D.dms, D.diag32 + m * D.dms);
+    if (D.ncolors > 0) {
+      launch_member_gather(st, D.L.nval, D.map_l, ctx->stage, D.L.v64 + m * D.L.nval, D.L.v32 + m * D.L.nval);
+      launch_member_gather(st, D.U.nval, D.map_u, ctx->stage, D.U.v64 + m * D.U.nval, D.U.v32 + m * D.U.nval);
+    }
+    // the copy must land before the next level's overwrites the staging
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  // the base level: the unit rows of the inverse are those of the prototype
+  HostLevel Hb = ctx->host[bl];
+  Hb.a.assign(vals[bl].begin(), vals[bl].end());
+  const int nb = ctx->b, nsc = Hb.n * nb;
+  std::vector<int> rowmap(nsc);
+  HIPCHK(hipMemcpy(rowmap.data(), ctx->base_rowmap, sizeof(int) * nsc, hipMemcpyDeviceToHost));
+  for (int v = 0; v < Hb.n; ++v)
+    for (int q = 0; q < nb; ++q) {
+      bool unit = Hb.rp[v] < Hb.rp[v + 1] && Hb.ci[Hb.rp[v]] == v;
+      for (int e = Hb.rp[v]; e < Hb.rp[v + 1] && unit; ++e)
+        for (int j = 0; j < nb; ++j)
+          if (Hb.a[((size_t)e * nb + q) * nb + j] != ((Hb.ci[e] == v && j == q) ? 1.0 : 0.0)) {
+            unit = false;
+            break;
+          }
+      if (unit != (rowmap[v * nb + q] < 0))
+        return fail(ctx, UGMG_E_ARG, "the base level's Dirichlet rows differ from the group's");
+    }
+  HostBase B;
+  try {
+    build_base(Hb, ctx->p.ex_reorder, B, ctx->p.ex_float != 0);
+  } catch (const std::bad_alloc &) {
+    return fail(ctx, UGMG_E_NOMEM, "host allocation (base level)");
+  }
+  if (B.failed) return fail(ctx, UGMG_E_SINGULAR, "zero pivot in the band LU of the base level");
+  if (nsc > 0 && ctx->base_nact > 0) {
+    HIPCHK(hipMemcpyAsync(ctx->base_stage, B.inv.data(), sizeof(double) * (size_t)nsc * nsc,
+                          hipMemcpyHostToDevice, st));
+    launch_base_compact(st, ctx->base_nact, nsc, ctx->base_act, ctx->base_colperm, ctx->base_stage,
+                        ctx->base_cinv + m * ctx->base_ims, ctx->base_cinv32 + m * ctx->base_ims);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(st));
+  return 0;
+}
+
+namespace {
+
+// one ls iteration of a group over columns 0..w-1: the FP64 cycle into x,
+// then every column's defect norm into red[0..w)
+void enqueue_group_iteration(ugmg_ctx *ctx, int top, double *x, double *b, int w) {
+  ctx->nb = w;
+  lmgc<double>(ctx, top, x, b, 1);
+  DevLevel &d = ctx->dev[top];
+  launch_dot<double>(ctx->stream, d.n, b, nullptr, d.vclass, ctx->part, ctx->red, 1,
+                     Batch{w, d.vs, d.nb});
+  ctx->nb = 1;
+}
+
+// from the graph captured for w columns (kept with the group)
+int launch_group_iteration(ugmg_ctx *ctx, int top, double *x, double *b, int w, double *norms_host) {
+  if (!can_capture(ctx) || w >= ugmg_ctx::kMaxGraphs) {
+    enqueue_group_iteration(ctx, top, x, b, w);
+    HIPCHK(hipGetLastError());
+  } else {
+    if (ctx->gkind != -2 || ctx->glevel != top || ctx->gx != x || ctx->gb != b) {
+      drop_graph(ctx);
+      ctx->gkind = -2;
+      ctx->glevel = top;
+      ctx->gx = x;
+      ctx->gb = b;
+    }
+    hipGraphExec_t &ge = ctx->gexec[w];
+    if (!ge) {
+      hipGraph_t g = nullptr;
+      HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      enqueue_group_iteration(ctx, top, x, b, w);
+      HIPCHK(hipStreamEndCapture(ctx->stream, &g));
+      hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+      hipGraphDestroy(g);
+      if (e != hipSuccess) {
+        ge = nullptr;
+        return fail(ctx, UGMG_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+      }
+    }
+    HIPCHK(hipGraphLaunch(ge, ctx->stream));
+  }
+  HIPCHK(hipMemcpyAsync(norms_host, ctx->red, sizeof(double) * w, hipMemcpyDeviceToHost, ctx->stream));
+  return 0;
+}
+
+}  // namespace
 
 struct ugmg_server {
   int device = 0;
   double budget = 0.0, used = 0.0;
   int max_slots = 1;
   std::vector<std::unique_ptr<SrvReq>> reqs;
-  std::deque<int> queue;
-  std::vector<int> active;
-  int ndone = 0;
+  std::deque<int> queue;                       // submitted, not admitted
+  std::vector<std::unique_ptr<SrvGroup>> groups;
+  int nactive = 0, ndone = 0;
+  long long launches = 0, columns = 0;   // group iterations, solve iterations in them
   std::string err;
   std::chrono::steady_clock::time_point t0;
   std::vector<std::thread> workers;
   std::mutex mu;
   std::condition_variable cv;
-  std::deque<int> prep_queue;
+  std::deque<SrvGroup *> prep_queue;
   bool stop = false;
 
   double now() const {
@@ -2757,19 +3036,19 @@ namespace {
 
 void server_worker(ugmg_server *s) {
   for (;;) {
-    SrvReq *r = nullptr;
+    SrvGroup *g = nullptr;
     {
       std::unique_lock<std::mutex> lk(s->mu);
       s->cv.wait(lk, [&] { return s->stop || !s->prep_queue.empty(); });
       if (s->stop) return;
-      r = s->reqs[s->prep_queue.front()].get();
+      g = s->prep_queue.front();
       s->prep_queue.pop_front();
     }
-    const int rc = prep_host(r->ctx, r->top, r->ps);
+    const int rc = prep_host(g->ctx, g->top, g->ps);
     std::lock_guard<std::mutex> lk(s->mu);
-    r->prep_rc = rc;
-    if (rc) r->err = r->ctx->err;
-    r->host_ready = true;
+    g->prep_rc = rc;
+    if (rc) g->err = g->ctx->err;
+    g->host_ready = true;
   }
 }
 
@@ -2778,109 +3057,167 @@ int srv_fail(ugmg_server *s, int code, const std::string &msg) {
   return code;
 }
 
+// request done: its solution (slot column of the group, device order) to
+// the host in list order; the slot is freed and its columns cleared
 void srv_finish(ugmg_server *s, SrvReq *r, int state) {
-  ugmg_ctx *ctx = r->ctx;
-  if (ctx && ctx->stream) hipStreamSynchronize(ctx->stream);
-  if (state == UGMG_REQ_DONE && ctx && r->x) {
-    const DevLevel &d = ctx->dev[r->top];
-    r->xout.resize(d.n);
-    if (hipMemcpy(r->xout.data(), r->x, sizeof(double) * r->xout.size(), hipMemcpyDeviceToHost) !=
-        hipSuccess) {
-      state = UGMG_REQ_FAILED;
-      r->err = "copy of the solution failed";
-    } else if (!d.perm_h.empty()) {   // device order -> list order
-      std::vector<double> xl(d.n);
-      for (int q = 0; q < d.n; ++q) xl[d.perm_h[q]] = r->xout[q];
-      r->xout.swap(xl);
+  if (r->group >= 0 && r->slot >= 0) {
+    SrvGroup *g = s->groups[r->group].get();
+    ugmg_ctx *ctx = g->ctx;
+    const DevLevel &d = ctx->dev[g->top];
+    if (state == UGMG_REQ_DONE) {
+      r->xout.resize(d.n);
+      if (hipMemcpyAsync(r->xout.data(), g->x + (size_t)r->slot * d.vs, sizeof(double) * d.n,
+                         hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+          hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        state = UGMG_REQ_FAILED;
+        r->err = "copy of the solution failed";
+      } else if (!d.perm_h.empty()) {   // device order -> list order
+        std::vector<double> xl(d.n);
+        for (int q = 0; q < d.n; ++q) xl[d.perm_h[q]] = r->xout[q];
+        r->xout.swap(xl);
+      }
     }
+    launch_fill<double>(ctx->stream, d.n, g->x + (size_t)r->slot * d.vs, 0.0, B1(d));
+    launch_fill<double>(ctx->stream, d.n, g->b + (size_t)r->slot * d.vs, 0.0, B1(d));
+    g->slot[r->slot] = -1;
+    g->last_used = s->now();
+    --s->nactive;
+    r->slot = -1;
   }
-  if (r->ev) hipEventDestroy(r->ev);
-  r->ev = nullptr;
-  if (ctx) ugmg_destroy(ctx);
-  r->ctx = nullptr;
-  r->x = r->b = nullptr;
-  r->ps = PrepState();
+  std::vector<std::vector<double>>().swap(r->vals);
   std::vector<double>().swap(r->rhs);
-  s->used -= r->charged;
-  r->charged = 0.0;
   r->state = state;
   r->t_done = s->now();
   ++s->ndone;
 }
 
-// admit one prepared request: uploads, first defect norm in flight
-int srv_admit(ugmg_server *s, SrvReq *r) {
-  ugmg_ctx *ctx = r->ctx;
-  if (hipSetDevice(s->device) != hipSuccess) return UGMG_E_HIP;
-  int rc = prep_device(ctx, r->ps);
-  r->ps = PrepState();
+void group_release(ugmg_server *s, SrvGroup *g) {
+  if (g->ctx) {
+    if (g->ctx->stream) hipStreamSynchronize(g->ctx->stream);
+    ugmg_destroy(g->ctx);
+    g->ctx = nullptr;
+  }
+  if (g->ev) hipEventDestroy(g->ev);
+  g->ev = nullptr;
+  if (g->norms) hipHostFree(g->norms);
+  g->norms = nullptr;
+  s->used -= g->charged;
+  g->charged = 0.0;
+  g->dead = true;
+  g->dev_ready = false;
+}
+
+// device set-up of a group whose host layout is ready
+int group_setup(ugmg_server *s, SrvGroup *g) {
+  ugmg_ctx *ctx = g->ctx;
+  int rc = prep_device(ctx, g->ps);
+  g->ps = PrepState();
+  if (rc) {
+    g->err = ctx->err;
+    return rc;
+  }
+  const DevLevel &d = ctx->dev[g->top];
+  const size_t nv = d.vs * (size_t)g->cap;
+  if ((rc = dalloc(ctx, &g->x, nv)) || (rc = dalloc(ctx, &g->b, nv))) {
+    g->err = ctx->err;
+    return rc;
+  }
+  launch_fill<double>(ctx->stream, d.n, g->x, 0.0, Batch{g->cap, d.vs, d.nb});
+  launch_fill<double>(ctx->stream, d.n, g->b, 0.0, Batch{g->cap, d.vs, d.nb});
+  if (hipEventCreateWithFlags(&g->ev, hipEventDisableTiming) != hipSuccess ||
+      hipHostMalloc((void **)&g->norms, sizeof(double) * g->cap, hipHostMallocDefault) != hipSuccess ||
+      hipStreamSynchronize(ctx->stream) != hipSuccess) {
+    g->err = "event / pinned memory / first fill";
+    return UGMG_E_HIP;
+  }
+  g->charged = ctx->dev_bytes;
+  s->used += g->charged;
+  g->dev_ready = true;
+  g->last_used = s->now();
+  return UGMG_OK;
+}
+
+// admit a request into free slot j of its (idle) group: its matrices, its
+// rhs and a zero x into column j, every level's vectors of that column
+// cleared, the first defect norm and stop test.  Returns 1 when the request
+// is already finished.
+int srv_admit(ugmg_server *s, SrvReq *r, SrvGroup *g, int j) {
+  ugmg_ctx *ctx = g->ctx;
+  int rc = member_set(ctx, j, r->vals);
   if (rc) {
     r->err = ctx->err;
     return rc;
   }
-  DevLevel &d = ctx->dev[r->top];
-  if ((rc = dalloc(ctx, &r->x, d.n)) || (rc = dalloc(ctx, &r->b, d.n))) {
-    r->err = ctx->err;
-    return rc;
+  std::vector<std::vector<double>>().swap(r->vals);
+  hipStream_t st = ctx->stream;
+  for (int l = ctx->bl; l <= g->top; ++l) {
+    DevLevel &D = ctx->dev[l];
+    for (double *v : {D.c, D.b, D.t, D.u}) launch_fill<double>(st, D.n, v + (size_t)j * D.vs, 0.0, B1(D));
   }
-  r->charged = ctx->dev_bytes;
-  s->used += r->charged;
+  DevLevel &d = ctx->dev[g->top];
   if (!d.perm_h.empty()) {   // list order -> device order
     std::vector<double> rp(d.n);
     for (int q = 0; q < d.n; ++q) rp[q] = r->rhs[d.perm_h[q]];
     r->rhs.swap(rp);
   }
-  if (hipMemcpyAsync(r->b, r->rhs.data(), sizeof(double) * d.n, hipMemcpyHostToDevice,
-                     ctx->stream) != hipSuccess ||
-      hipEventCreateWithFlags(&r->ev, hipEventDisableTiming) != hipSuccess)
-    return UGMG_E_HIP;
-  launch_fill<double>(ctx->stream, d.n, r->x, 0.0, B1(d));
-  if (ctx->p.precision == UGMG_MIXED)
-    launch_convert<float, double>(ctx->stream, d.n, r->b, d.b32, 1, d.vs, d.vs);
-  launch_dot<double>(ctx->stream, d.n, r->b, nullptr, d.vclass, ctx->part, ctx->red, 1, B1(d));
-  if (hipMemcpyAsync(ctx->red_host, ctx->red, sizeof(double), hipMemcpyDeviceToHost,
-                     ctx->stream) != hipSuccess ||
-      hipEventRecord(r->ev, ctx->stream) != hipSuccess)
-    return UGMG_E_HIP;
-  r->pending = true;
-  r->phase = 0;
-  r->it = 0;
+  double *bj = g->b + (size_t)j * d.vs, *xj = g->x + (size_t)j * d.vs;
+  HIPCHK(hipMemcpyAsync(bj, r->rhs.data(), sizeof(double) * d.n, hipMemcpyHostToDevice, st));
+  launch_fill<double>(st, d.n, xj, 0.0, B1(d));
+  launch_dot<double>(st, d.n, bj, nullptr, d.vclass, ctx->part, ctx->red, 1, B1(d));
+  HIPCHK(hipMemcpyAsync(ctx->red_host, ctx->red, sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  g->slot[j] = r->id;
+  r->group = (int)(std::find_if(s->groups.begin(), s->groups.end(),
+                                [g](const std::unique_ptr<SrvGroup> &q) { return q.get() == g; }) -
+                   s->groups.begin());
+  r->slot = j;
+  ++s->nactive;
   r->state = UGMG_REQ_ACTIVE;
   r->t_admit = s->now();
-  return UGMG_OK;
-}
-
-// one completed norm: stop test (sc_cmp) and, if not done, the next iteration
-int srv_advance(ugmg_server *s, SrvReq *r) {
-  ugmg_ctx *ctx = r->ctx;
+  g->last_used = r->t_admit;
+  // the first defect: sc_mul_check target, stop on $abslimit
   const double val = ctx->red_host[0];
-  bool conv;
-  if (r->phase == 0) {
-    r->res.first_defect[0] = r->res.last_defect[0] = val;
-    r->target = val * r->red;
-    if (r->target == 0.0) r->target = r->red;
-    conv = sc_cmp1(val, r->abslimit);
-    r->phase = 1;
-  } else {
-    ++r->it;
-    r->res.last_defect[0] = val;
-    conv = sc_cmp1(val, r->abslimit) || sc_cmp1(val, r->target);
-  }
-  if (conv || r->it >= r->maxiter) {
-    r->res.converged = conv ? 1 : 0;
-    r->res.number_of_linear_iterations = r->it;
+  r->res.first_defect[0] = r->res.last_defect[0] = val;
+  r->target = val * r->red;
+  if (r->target == 0.0) r->target = r->red;
+  if (sc_cmp1(val, r->abslimit) || r->maxiter <= 0) {
+    r->res.converged = sc_cmp1(val, r->abslimit) ? 1 : 0;
+    r->res.number_of_linear_iterations = 0;
     srv_finish(s, r, UGMG_REQ_DONE);
     return 1;
   }
-  int rc = launch_iteration(ctx, r->top, r->x, r->b, ctx->p.inner_cycles);
-  if (rc || hipEventRecord(r->ev, ctx->stream) != hipSuccess) {
-    r->err = ctx->err;
-    r->res.error_code = rc ? rc : UGMG_E_HIP;
-    srv_finish(s, r, UGMG_REQ_FAILED);
-    return 1;
-  }
-  r->pending = true;
   return 0;
+}
+
+// a group's completed iteration: every column's stop test
+void srv_progress(ugmg_server *s, SrvGroup *g) {
+  for (int j = 0; j < g->width; ++j) {
+    const int id = g->slot[j];
+    if (id < 0) continue;
+    SrvReq *r = s->reqs[id].get();
+    const double val = g->norms[j];
+    ++r->it;
+    r->res.last_defect[0] = val;
+    const bool conv = sc_cmp1(val, r->abslimit) || sc_cmp1(val, r->target);
+    if (conv || r->it >= r->maxiter) {
+      r->res.converged = conv ? 1 : 0;
+      r->res.number_of_linear_iterations = r->it;
+      srv_finish(s, r, UGMG_REQ_DONE);
+    }
+  }
+}
+
+void group_fail_all(ugmg_server *s, SrvGroup *g, int code, const std::string &msg) {
+  for (int j = 0; j < (int)g->slot.size(); ++j)
+    if (g->slot[j] >= 0) {
+      SrvReq *r = s->reqs[g->slot[j]].get();
+      r->err = msg;
+      r->res.error_code = code;
+      r->slot = -1;
+      g->slot[j] = -1;
+      --s->nactive;
+      srv_finish(s, r, UGMG_REQ_FAILED);
+    }
 }
 
 }  // namespace
@@ -2924,13 +3261,8 @@ int ugmg_server_destroy(ugmg_server *s) {
   s->cv.notify_all();
   for (auto &w : s->workers)
     if (w.joinable()) w.join();
-  for (auto &r : s->reqs)
-    if (r && r->ctx) {
-      if (r->ctx->stream) hipStreamSynchronize(r->ctx->stream);
-      if (r->ev) hipEventDestroy(r->ev);
-      ugmg_destroy(r->ctx);
-      r->ctx = nullptr;
-    }
+  for (auto &g : s->groups)
+    if (g && !g->dead) group_release(s, g.get());
   delete s;
   return UGMG_OK;
 }
@@ -2941,108 +3273,190 @@ int ugmg_server_submit(ugmg_server *s, int nlevels, const ugmg_level_desc *level
                        const ugmg_params *p, const double *rhs, double red, double abslimit,
                        int maxiter, int *id) {
   if (!s || !levels || !p || !rhs || !id || nlevels < 1 || maxiter < 0) return UGMG_E_ARG;
-  auto r = std::make_unique<SrvReq>();
-  ugmg_ctx *ctx = nullptr;
-  int rc = ugmg_create(s->device, &ctx);
-  if (rc) return srv_fail(s, rc, "ugmg_create failed");
-  r->ctx = ctx;
-  auto bail = [&](int code) {
-    s->err = ctx->err;
-    ugmg_destroy(ctx);
-    return code;
-  };
-  if ((rc = ugmg_set_nlevels(ctx, nlevels, 1))) return bail(rc);
+  if (p->precision != UGMG_FP64) return srv_fail(s, UGMG_E_ARG, "the server runs FP64 solves");
   for (int l = 0; l < nlevels; ++l) {
     const ugmg_level_desc &L = levels[l];
-    if ((rc = ugmg_set_level(ctx, l, L.n, L.row_ptr, L.col_idx, L.vals, L.skip, L.vclass, L.vnclass)))
-      return bail(rc);
+    if (L.n < 0 || !L.row_ptr || !L.col_idx || !L.vals || !L.skip || !L.vclass || !L.vnclass ||
+        (l > 0 && (!L.p_ptr || !L.p_idx || !L.p_w)))
+      return srv_fail(s, UGMG_E_ARG, "incomplete level description");
   }
-  for (int l = 1; l < nlevels; ++l)
-    if ((rc = ugmg_set_transfer(ctx, l, levels[l].p_ptr, levels[l].p_idx, levels[l].p_w)))
-      return bail(rc);
-  ugmg_params q = *p;
-  if ((rc = ugmg_set_params(ctx, &q))) return bail(rc);
+  auto r = std::make_unique<SrvReq>();
   r->top = nlevels - 1;
-  const int n = levels[r->top].n;
-  r->rhs.assign(rhs, rhs + n);
+  r->n = levels[r->top].n;
+  r->rhs.assign(rhs, rhs + r->n);
   r->red = red;
   r->abslimit = abslimit;
   r->maxiter = maxiter;
-  r->est_bytes = est_device_bytes(ctx, r->top, resolve_baselevel(ctx, r->top));
+  r->vals.resize(nlevels);
+  for (int l = 0; l < nlevels; ++l)
+    r->vals[l].assign(levels[l].vals, levels[l].vals + levels[l].row_ptr[levels[l].n]);
+  // the request's group: the first live one of its shape, or a new one
+  const uint64_t key = group_key(nlevels, levels, *p);
+  int gi = -1;
+  for (int k = 0; k < (int)s->groups.size() && gi < 0; ++k) {
+    SrvGroup *g = s->groups[k].get();
+    if (!g->dead && g->key == key && same_shape(g->ctx, nlevels, levels)) gi = k;
+  }
+  if (gi < 0) {
+    auto g = std::make_unique<SrvGroup>();
+    g->key = key;
+    g->top = r->top;
+    g->cap = std::min(s->max_slots, kSrvMaxCap);
+    g->slot.assign(g->cap, -1);
+    ugmg_ctx *ctx = nullptr;
+    int rc = ugmg_create(s->device, &ctx);
+    if (rc) return srv_fail(s, rc, "ugmg_create failed");
+    g->ctx = ctx;
+    auto bail = [&](int code) {
+      s->err = ctx->err;
+      ugmg_destroy(ctx);
+      return code;
+    };
+    if ((rc = ugmg_set_nlevels(ctx, nlevels, 1))) return bail(rc);
+    for (int l = 0; l < nlevels; ++l) {
+      const ugmg_level_desc &L = levels[l];
+      if ((rc = ugmg_set_level(ctx, l, L.n, L.row_ptr, L.col_idx, L.vals, L.skip, L.vclass, L.vnclass)))
+        return bail(rc);
+    }
+    for (int l = 1; l < nlevels; ++l)
+      if ((rc = ugmg_set_transfer(ctx, l, levels[l].p_ptr, levels[l].p_idx, levels[l].p_w)))
+        return bail(rc);
+    ugmg_params q = *p;
+    if ((rc = ugmg_set_params(ctx, &q))) return bail(rc);
+    ctx->nrhs_cap = g->cap;
+    ctx->members = g->cap;
+    // every member's matrices, vectors and base inverse; patterns once
+    g->est_bytes = est_device_bytes(ctx, r->top, resolve_baselevel(ctx, r->top)) * g->cap;
+    gi = (int)s->groups.size();
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->prep_queue.push_back(g.get());
+    s->groups.push_back(std::move(g));
+    s->cv.notify_one();
+  }
+  r->group = gi;
   r->t_submit = s->now();
-  std::lock_guard<std::mutex> lk(s->mu);
   r->id = (int)s->reqs.size();
   *id = r->id;
   s->queue.push_back(r->id);
-  s->prep_queue.push_back(r->id);
   s->reqs.push_back(std::move(r));
-  s->cv.notify_one();
   return UGMG_OK;
 }
 
 int ugmg_server_step(ugmg_server *s, int *nactive, int *nqueued, int *ndone) {
   if (!s) return UGMG_E_ARG;
   if (hipSetDevice(s->device) != hipSuccess) return srv_fail(s, UGMG_E_HIP, "hipSetDevice");
-  // progress: every request whose norm copy has landed takes its stop test
-  for (size_t k = 0; k < s->active.size();) {
-    SrvReq *r = s->reqs[s->active[k]].get();
-    bool finished = false;
-    if (r->pending) {
-      hipError_t q = hipEventQuery(r->ev);
-      if (q == hipSuccess) {
-        r->pending = false;
-        finished = srv_advance(s, r) != 0;
-      } else if (q != hipErrorNotReady) {
-        r->err = std::string("hipEventQuery: ") + hipGetErrorString(q);
-        r->res.error_code = UGMG_E_HIP;
+  // 1. completed iterations: every column's stop test
+  for (auto &gp : s->groups) {
+    SrvGroup *g = gp.get();
+    if (g->dead || !g->busy) continue;
+    const hipError_t q = hipEventQuery(g->ev);
+    if (q == hipErrorNotReady) continue;
+    g->busy = false;
+    if (q != hipSuccess) {
+      group_fail_all(s, g, UGMG_E_HIP, std::string("hipEventQuery: ") + hipGetErrorString(q));
+      continue;
+    }
+    srv_progress(s, g);
+  }
+  // 2. device set-up of groups whose layout is ready, within the budget
+  //    (idle resident groups are released, least recently used first, to
+  //    make room); a group that alone exceeds it fails its requests
+  std::vector<SrvGroup *> ready;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (auto &gp : s->groups)
+      if (!gp->dead && gp->host_ready && !gp->dev_ready) ready.push_back(gp.get());
+  }
+  std::vector<char> failed_group(s->groups.size(), 0);
+  for (SrvGroup *g : ready) {
+    const int gi = (int)(std::find_if(s->groups.begin(), s->groups.end(),
+                                      [g](const std::unique_ptr<SrvGroup> &q) { return q.get() == g; }) -
+                         s->groups.begin());
+    int rc = g->prep_rc;
+    if (!rc && g->est_bytes > s->budget) {
+      rc = UGMG_E_BUDGET;
+      g->err = "request exceeds the HBM budget";
+    }
+    while (!rc && s->used + g->est_bytes > s->budget) {
+      SrvGroup *lru = nullptr;
+      for (auto &q : s->groups)
+        if (!q->dead && q->dev_ready && !q->busy && q->occupied() == 0 &&
+            (!lru || q->last_used < lru->last_used))
+          lru = q.get();
+      if (!lru) break;
+      group_release(s, lru);
+    }
+    if (!rc && s->used + g->est_bytes > s->budget) continue;   // waits for memory
+    if (!rc) rc = group_setup(s, g);
+    if (rc) {
+      failed_group[gi] = 1;
+      for (size_t k = 0; k < s->queue.size();) {
+        SrvReq *r = s->reqs[s->queue[k]].get();
+        if (r->group == gi) {
+          s->queue.erase(s->queue.begin() + k);
+          r->err = g->err;
+          r->res.error_code = rc;
+          r->group = -1;
+          srv_finish(s, r, UGMG_REQ_FAILED);
+        } else {
+          ++k;
+        }
+      }
+      group_release(s, g);
+    }
+  }
+  // 3. admission in submission order into free slots of idle groups, while
+  //    slots allow (a request whose group is not ready yet holds back none)
+  for (size_t k = 0; k < s->queue.size() && s->nactive < s->max_slots;) {
+    SrvReq *r = s->reqs[s->queue[k]].get();
+    SrvGroup *g = s->groups[r->group].get();
+    if (g->dead) {   // released while the request waited: a new group
+      s->queue.erase(s->queue.begin() + k);
+      r->err = "the request's group was released";
+      r->res.error_code = UGMG_E_STATE;
+      r->group = -1;
+      srv_finish(s, r, UGMG_REQ_FAILED);
+      continue;
+    }
+    if (!g->dev_ready || g->busy) {
+      ++k;
+      continue;
+    }
+    int j = 0;
+    while (j < g->cap && g->slot[j] >= 0) ++j;
+    if (j == g->cap) {
+      ++k;
+      continue;
+    }
+    s->queue.erase(s->queue.begin() + k);
+    const int rc = srv_admit(s, r, g, j);
+    if (rc < 0) {
+      r->res.error_code = rc;
+      if (r->slot >= 0) srv_finish(s, r, UGMG_REQ_FAILED);
+      else {
+        r->group = -1;
         srv_finish(s, r, UGMG_REQ_FAILED);
-        finished = true;
       }
     }
-    if (finished) s->active.erase(s->active.begin() + k);
-    else ++k;
   }
-  // admission: the first prepared request in submission order, within slots
-  // and HBM budget (a request that alone exceeds the budget is rejected; one
-  // that waits for memory holds back those behind it, so large requests are
-  // not starved)
-  for (;;) {
-    if (s->queue.empty() || (int)s->active.size() >= s->max_slots) break;
-    SrvReq *r = nullptr;
-    size_t qi = 0;
-    {
-      std::lock_guard<std::mutex> lk(s->mu);
-      for (; qi < s->queue.size(); ++qi)
-        if (s->reqs[s->queue[qi]]->host_ready) {
-          r = s->reqs[s->queue[qi]].get();
-          break;
-        }
-    }
-    if (!r) break;
-    if (r->prep_rc) {
-      s->queue.erase(s->queue.begin() + qi);
-      r->res.error_code = r->prep_rc;
-      srv_finish(s, r, UGMG_REQ_FAILED);
+  // 4. the next iteration of every idle group with solves in it
+  for (auto &gp : s->groups) {
+    SrvGroup *g = gp.get();
+    if (g->dead || !g->dev_ready || g->busy || g->occupied() == 0) continue;
+    int w = 0;
+    for (int j = 0; j < g->cap; ++j)
+      if (g->slot[j] >= 0) w = j + 1;
+    const int rc = launch_group_iteration(g->ctx, g->top, g->x, g->b, w, g->norms);
+    if (rc || hipEventRecord(g->ev, g->ctx->stream) != hipSuccess) {
+      group_fail_all(s, g, rc ? rc : UGMG_E_HIP, g->ctx->err);
       continue;
     }
-    if (r->est_bytes > s->budget) {
-      s->queue.erase(s->queue.begin() + qi);
-      r->err = "request exceeds the HBM budget";
-      r->res.error_code = UGMG_E_BUDGET;
-      srv_finish(s, r, UGMG_REQ_FAILED);
-      continue;
-    }
-    if (s->used + r->est_bytes > s->budget) break;
-    s->queue.erase(s->queue.begin() + qi);
-    const int rc = srv_admit(s, r);
-    if (rc) {
-      r->res.error_code = rc;
-      srv_finish(s, r, UGMG_REQ_FAILED);
-      continue;
-    }
-    s->active.push_back(r->id);
+    g->width = w;
+    g->busy = true;
+    ++s->launches;
+    s->columns += g->occupied();
   }
-  if (nactive) *nactive = (int)s->active.size();
+  if (nactive) *nactive = s->nactive;
   if (nqueued) *nqueued = (int)s->queue.size();
   if (ndone) *ndone = s->ndone;
   return UGMG_OK;
@@ -3058,6 +3472,18 @@ int ugmg_server_status(ugmg_server *s, int id, int *state, ugmg_lresult *res, do
     times[1] = r->t_admit;
     times[2] = r->t_done;
   }
+  return UGMG_OK;
+}
+
+int ugmg_server_stats(const ugmg_server *s, int *groups, long long *launches, long long *columns) {
+  if (!s) return UGMG_E_ARG;
+  if (groups) {
+    int k = 0;
+    for (const auto &g : s->groups) k += !g->dead && g->dev_ready;
+    *groups = k;
+  }
+  if (launches) *launches = s->launches;
+  if (columns) *columns = s->columns;
   return UGMG_OK;
 }
 

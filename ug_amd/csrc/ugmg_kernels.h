@@ -27,6 +27,11 @@ struct DSell {
   const int *dtab;    // shared column-offset lists of uniform slices
   int wmax;           // widest slice (entries per row)
   size_t pst;         // entries per value plane (b x b blocks: b*b planes)
+  // members (one matrix per column of a batched launch, same pattern):
+  // column j reads its values at val + j * mst and the level's diagonal
+  // blocks at diag + j * dms; 0 = one matrix for every column
+  size_t mst = 0;
+  size_t dms = 0;
 };
 
 // base-level solver settings: ls($m,$red,$abslimit) over ex($damp)
@@ -79,6 +84,10 @@ struct Batch {
   int nrhs;
   size_t vs;
   int b = 1;
+  // members: column j's diagonal blocks at diag + j * dms and its base
+  // inverse at inv + j * ims (0: shared by every column)
+  size_t dms = 0;
+  size_t ims = 0;
 };
 
 // per right-hand side state of the batched BCGSSolver (np/procs/ls.cc:
@@ -135,7 +144,7 @@ int launch_restrict(hipStream_t st, const DSell<T> &PT, const uint32_t *cskip,
                     const uint8_t *cvnclass, const uint8_t *cvclass, const T *bf, T damp, T *bc,
                     double *part, bool nt, int nrhs, size_t vsf, size_t vsc, int b,
                     const uint8_t *cc0 = nullptr, const T *cdiag = nullptr, T *ct = nullptr,
-                    const int *cpos = nullptr, T *cbc = nullptr);
+                    const int *cpos = nullptr, T *cbc = nullptr, size_t cdms = 0);
 // Element lists of one level on the device (all pointers device memory):
 // nelem elements of nc corners, elem[e*nc + i] = vector of corner i; element
 // matrices (nc b) x (nc b) row-major over corner-major scalar unknowns,
@@ -173,6 +182,9 @@ void launch_base_gemv(hipStream_t st, int n, const TI *inv, const int *rowmap, c
                       int ncols, const T *b, const T *bc, T *c, T *xout, int cmode, double damp,
                       double abslimit, double red, const double *part, int nparts, double *first,
                       int it, Batch bt);
+// members: d64[j] = src[map[j]] (0 where map[j] < 0), d32[j] = (float) d64[j]
+void launch_member_gather(hipStream_t st, size_t n, const int *map, const double *src, double *d64,
+                          float *d32);
 // rows act[0..nact), columns in the order cp, of the n x n inverse into
 // compact FP64 / FP32 copies
 void launch_base_compact(hipStream_t st, int nact, int n, const int *act, const int *cp,

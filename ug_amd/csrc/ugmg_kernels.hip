@@ -260,6 +260,9 @@ __device__ __forceinline__ void batch_map(int &sb, int &col) {
   const int sdo = sok ? (S).sdoff[s] : -1;                                               \
   const int rb = r >= 0 ? r : (sok ? (S).rows[s * kSlice] : 0);
 
+// a slab's values for the workgroup's column (members: one matrix per column)
+#define MV(S) ((S).val + (S).mst * (size_t)col_)
+
 // block sum of one double per thread; thread 0 gets the total
 __device__ __forceinline__ double block_sum(double v, double *red) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
@@ -281,7 +284,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(DSell<TV> A, int slice0, int ns
   const size_t o = (size_t)col_ * vs;
   x += o;
   y += o;
-  const TY sum = sell_dot<MW, NT, TV, TX, TY>(A.val + e0 + lane, A.col + e0 + lane, w, x, sdo, A.dtab, rb);
+  const TY sum = sell_dot<MW, NT, TV, TX, TY>(MV(A) + e0 + lane, A.col + e0 + lane, w, x, sdo, A.dtab, rb);
   if (r < 0) return;
   if (MODE == 1) {
     y[r] = sum;
@@ -304,11 +307,11 @@ __global__ __launch_bounds__(kBlock) void k_spmv_c0(DSell<TV> A, int nsl, const 
   const size_t o = (size_t)col_ * vs;
   x += o;
   y += o;
-  const T sum = sell_dot<MW, NT, TV, T, T>(A.val + e0 + lane, A.col + e0 + lane, w, x, sdo, A.dtab, rb);
+  const T sum = sell_dot<MW, NT, TV, T, T>(MV(A) + e0 + lane, A.col + e0 + lane, w, x, sdo, A.dtab, rb);
   if (r < 0) return;
   const T yn = y[r] - sum;
   y[r] = yn;
-  if (s < ncs0) u[o + r] = yn / diag[r];
+  if (s < ncs0) u[o + r] = yn / diag[A.dms * col_ + r];
 }
 
 // FP64 defect of a mixed-precision refinement step:
@@ -328,7 +331,7 @@ __global__ __launch_bounds__(kBlock) void k_ir_update(DSell<double> A, int nsl,
   double sq = 0.0;
   if (sok) {
     const double sum =
-        sell_dot<MW, NT, double, float, double>(A.val + e0 + lane, A.col + e0 + lane, w, c32 + o, sdo, A.dtab, rb);
+        sell_dot<MW, NT, double, float, double>(MV(A) + e0 + lane, A.col + e0 + lane, w, c32 + o, sdo, A.dtab, rb);
     if (r >= 0) {
       const double nr = rv[o + r] - sum;
       rv[o + r] = nr;
@@ -347,11 +350,11 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_jac(int n, const T *__restrict__ b,
                                                  const T *__restrict__ diag,
                                                  const uint8_t *__restrict__ vclass, T damp,
-                                                 T *__restrict__ t, size_t vs) {
+                                                 T *__restrict__ t, size_t vs, size_t dms) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   const size_t o = (size_t)blockIdx.y * vs;
-  t[o + i] = vclass[i] < 3 ? (T)0 : (b[o + i] / diag[i]) * damp;
+  t[o + i] = vclass[i] < 3 ? (T)0 : (b[o + i] / diag[dms * blockIdx.y + i]) * damp;
 }
 
 // ------------------------------------------------------ multicolour GS
@@ -370,10 +373,10 @@ __global__ __launch_bounds__(BS) void k_gs_color(DSell<TV> L, int slice0, int ns
   int act = 0;
   if (r >= 0) {
     br = b[o + r];
-    dr = diag[r];
+    dr = diag[L.dms * col_ + r];
     act = vclass[r] >= 3;
   }
-  const T sum = sell_dot<MW, NT, TV, T, T>(L.val + e0 + lane, L.col + e0 + lane, w, v, sdo, L.dtab, rb);
+  const T sum = sell_dot<MW, NT, TV, T, T>(MV(L) + e0 + lane, L.col + e0 + lane, w, v, sdo, L.dtab, rb);
   if (r < 0) return;
   v[r] = act ? (br - sum) / dr : (T)0;
 }
@@ -400,12 +403,12 @@ __global__ __launch_bounds__(BS) void k_gs_color_fused(DSell<TV> L, DSell<TV> U,
   T br = 0, dr = 1, vo = 0, cr = 0;
   if (r >= 0) {
     br = b[o + r];
-    dr = diag[r];
+    dr = diag[L.dms * col_ + r];
     vo = v[r];
     if (CMODE == 1) cr = c[o + r];
   }
-  const T su = sell_dot<CHU, NT, TV, T, T>(U.val + eu + lane, U.col + eu + lane, wu, v, sdu, U.dtab, rb);
-  const T sl = sell_dot<CHL, NT, TV, T, T>(L.val + e0 + lane, L.col + e0 + lane, w, v, sdo, L.dtab, rb);
+  const T su = sell_dot<CHU, NT, TV, T, T>(MV(U) + eu + lane, U.col + eu + lane, wu, v, sdu, U.dtab, rb);
+  const T sl = sell_dot<CHL, NT, TV, T, T>(MV(L) + e0 + lane, L.col + e0 + lane, w, v, sdo, L.dtab, rb);
   if (r < 0) return;
   const T bn = br - damp * (br + su);
   b[o + r] = bn;
@@ -441,16 +444,16 @@ __global__ __launch_bounds__(2 * kSlice) void k_gs_color_fused2(DSell<TV> L, DSe
   if (half == 0) {
     if (r >= 0) {
       br = b[o + r];
-      dr = diag[r];
+      dr = diag[L.dms * col_ + r];
       vo = v[r];
       if (CMODE == 1) cr = c[o + r];
     }
     const int eu = U.soff[s];
-    part = sell_dot<CHU, NT, TV, T, T>(U.val + eu + lane, U.col + eu + lane, (U.soff[s + 1] - eu) >> 6,
+    part = sell_dot<CHU, NT, TV, T, T>(MV(U) + eu + lane, U.col + eu + lane, (U.soff[s + 1] - eu) >> 6,
                                        v, U.sdoff[s], U.dtab, rb);
   } else {
     const int e0 = L.soff[s];
-    part = sell_dot<CHL, NT, TV, T, T>(L.val + e0 + lane, L.col + e0 + lane, (L.soff[s + 1] - e0) >> 6,
+    part = sell_dot<CHL, NT, TV, T, T>(MV(L) + e0 + lane, L.col + e0 + lane, (L.soff[s + 1] - e0) >> 6,
                                        v, L.sdoff[s], L.dtab, rb);
     lsum[lane] = part;
   }
@@ -485,7 +488,7 @@ __global__ __launch_bounds__(kBlock) void k_gs_defect(DSell<TV> U, int nsl,
     act = vclass[r] >= 3;
     if (CMODE == 1 && c) cr = c[o + r];
   }
-  const T sum = sell_dot<MW, NT, TV, T, T>(U.val + e0 + lane, U.col + e0 + lane, w, v, sdo, U.dtab, rb);
+  const T sum = sell_dot<MW, NT, TV, T, T>(MV(U) + e0 + lane, U.col + e0 + lane, w, v, sdo, U.dtab, rb);
   if (r < 0) return;
   b[o + r] = act ? br - damp * (br + sum) : br - damp * sum;
   const T t = damp * vr;
@@ -514,21 +517,21 @@ __global__ __launch_bounds__(kBlock) void k_restrict(DSell<T> PT, int nsl,
                                                       const T *__restrict__ cdiag,
                                                       T *__restrict__ ct,
                                                       const int *__restrict__ cpos,
-                                                      T *__restrict__ cbc) {
+                                                      T *__restrict__ cbc, size_t cdms) {
   __shared__ double red[kBlock / 64];
   SLICE_PROLOGUE_RED(PT, nsl)
   bf += (size_t)col_ * vsf;
   bc += (size_t)col_ * vsc;
   double sq = 0.0;
   if (sok) {
-    const T sum = sell_dot<MW, NT, T, T, T>(PT.val + e0 + lane, PT.col + e0 + lane, w, bf, sdo, PT.dtab, rb);
+    const T sum = sell_dot<MW, NT, T, T, T>(MV(PT) + e0 + lane, PT.col + e0 + lane, w, bf, sdo, PT.dtab, rb);
     if (r >= 0) {
       const T base = cvnclass[r] >= 2 ? (T)0 : bc[r];
       const T v = (cskip[r] & 1u) ? base : base + damp * sum;
       bc[r] = v;
       // the coarse pre-smoother's colour-0 sweep: no lower neighbours, so
       // its value is (b_r - 0) / a_rr
-      if (ct && cc0[r]) ct[(size_t)col_ * vsc + r] = v / cdiag[r];
+      if (ct && cc0[r]) ct[(size_t)col_ * vsc + r] = v / cdiag[cdms * col_ + r];
       // b in the base inverse's column order
       if (cpos) {
         const int pj = cpos[r];
@@ -552,7 +555,7 @@ __global__ __launch_bounds__(kBlock) void k_interp(DSell<T> P, int nsl,
   SLICE_PROLOGUE_MAP(P, 0, nsl, true)
   cc += (size_t)col_ * vsc;
   const size_t o = (size_t)col_ * vsf;
-  const T sum = sell_dot<MW, NT, T, T, T>(P.val + e0 + lane, P.col + e0 + lane, w, cc, sdo, P.dtab, rb);
+  const T sum = sell_dot<MW, NT, T, T, T>(MV(P) + e0 + lane, P.col + e0 + lane, w, cc, sdo, P.dtab, rb);
   if (r < 0) return;
   const T tv = (fskip[r] & 1u) ? (T)0 : damp * sum;
   t[o + r] = tv;
@@ -647,7 +650,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_b3(DSell<TV> A, int slice0, int
   x += o;
   y += o;
   TY sum[B];
-  sell_dot_blk<B, NT, TV, TX, TY>(A.val + e0 + lane, A.pst, A.col + e0 + lane, w, x, sdo, A.dtab, rb, sum);
+  sell_dot_blk<B, NT, TV, TX, TY>(MV(A) + e0 + lane, A.pst, A.col + e0 + lane, w, x, sdo, A.dtab, rb, sum);
   if (r < 0) return;
 #pragma unroll
   for (int i = 0; i < B; ++i) {
@@ -677,7 +680,7 @@ __global__ __launch_bounds__(kBlock) void k_ir_update_b3(DSell<double> A, int ns
   double sq[B] = {0.0, 0.0, 0.0};
   if (sok) {
     double sum[B];
-    sell_dot_blk<B, NT, double, float, double>(A.val + e0 + lane, A.pst, A.col + e0 + lane, w,
+    sell_dot_blk<B, NT, double, float, double>(MV(A) + e0 + lane, A.pst, A.col + e0 + lane, w,
                                                c32 + o, sdo, A.dtab, rb, sum);
     if (r >= 0)
 #pragma unroll
@@ -699,10 +702,11 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_jac_b3(int n, const T *__restrict__ b,
                                                     const T *__restrict__ diag,
                                                     const uint8_t *__restrict__ vclass, T damp,
-                                                    T *__restrict__ t, size_t vs) {
+                                                    T *__restrict__ t, size_t vs, size_t dms) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   const size_t o = (size_t)blockIdx.y * vs + 3 * (size_t)i;
+  diag += dms * blockIdx.y;
   if (vclass[i] < 3) {
     t[o] = t[o + 1] = t[o + 2] = (T)0;
     return;
@@ -728,13 +732,13 @@ __global__ __launch_bounds__(BS) void k_gs_color_b3(DSell<TV> L, int slice0, int
   const size_t o = (size_t)col_ * vs;
   v += o;
   T sum[B];
-  sell_dot_blk<B, NT, TV, T, T>(L.val + e0 + lane, L.pst, L.col + e0 + lane, w, v, sdo, L.dtab, rb, sum);
+  sell_dot_blk<B, NT, TV, T, T>(MV(L) + e0 + lane, L.pst, L.col + e0 + lane, w, v, sdo, L.dtab, rb, sum);
   if (r < 0) return;
   T x[B] = {0, 0, 0};
   if (vclass[r] >= 3) {
     T m[9], rr[B];
 #pragma unroll
-    for (int q = 0; q < 9; ++q) m[q] = diag[9 * (size_t)r + q];
+    for (int q = 0; q < 9; ++q) m[q] = diag[L.dms * col_ + 9 * (size_t)r + q];
 #pragma unroll
     for (int q = 0; q < B; ++q) rr[q] = b[o + (size_t)B * r + q] - sum[q];
     solve3<T>(m, rr, x);
@@ -755,7 +759,7 @@ __global__ __launch_bounds__(kBlock) void k_gs_defect_b3(DSell<TV> U, int nsl,
   const size_t o = (size_t)col_ * vs;
   v += o;
   T sum[B];
-  sell_dot_blk<B, NT, TV, T, T>(U.val + e0 + lane, U.pst, U.col + e0 + lane, w, v, sdo, U.dtab, rb, sum);
+  sell_dot_blk<B, NT, TV, T, T>(MV(U) + e0 + lane, U.pst, U.col + e0 + lane, w, v, sdo, U.dtab, rb, sum);
   if (r < 0) return;
   const bool act = vclass[r] >= 3;
 #pragma unroll
@@ -787,12 +791,12 @@ __global__ __launch_bounds__(BS) void k_gs_color_fused_b3(DSell<TV> L, DSell<TV>
   const int wu = (U.soff[s + 1] - eu) >> 6;
   const int sdu = U.sdoff[s];
   T su[B], sl[B];
-  sell_dot_blk<B, NT, TV, T, T>(U.val + eu + lane, U.pst, U.col + eu + lane, wu, v, sdu, U.dtab, rb, su);
-  sell_dot_blk<B, NT, TV, T, T>(L.val + e0 + lane, L.pst, L.col + e0 + lane, w, v, sdo, L.dtab, rb, sl);
+  sell_dot_blk<B, NT, TV, T, T>(MV(U) + eu + lane, U.pst, U.col + eu + lane, wu, v, sdu, U.dtab, rb, su);
+  sell_dot_blk<B, NT, TV, T, T>(MV(L) + e0 + lane, L.pst, L.col + e0 + lane, w, v, sdo, L.dtab, rb, sl);
   if (r < 0) return;
   T m[9], rr[B], x[B];
 #pragma unroll
-  for (int q = 0; q < 9; ++q) m[q] = diag[9 * (size_t)r + q];
+  for (int q = 0; q < 9; ++q) m[q] = diag[L.dms * col_ + 9 * (size_t)r + q];
 #pragma unroll
   for (int q = 0; q < B; ++q) {
     const size_t k = (size_t)B * r + q;
@@ -825,7 +829,7 @@ __global__ __launch_bounds__(kBlock) void k_restrict_b3(DSell<T> PT, int nsl,
   double sq[B] = {0.0, 0.0, 0.0};
   if (sok) {
     T sum[B];
-    sell_dot_diag<B, NT, T, T, T>(PT.val + e0 + lane, PT.col + e0 + lane, w, bf, sdo, PT.dtab, rb, sum);
+    sell_dot_diag<B, NT, T, T, T>(MV(PT) + e0 + lane, PT.col + e0 + lane, w, bf, sdo, PT.dtab, rb, sum);
     if (r >= 0) {
       const bool zero = cvnclass[r] >= 2;
       const uint32_t sk = cskip[r];
@@ -856,7 +860,7 @@ __global__ __launch_bounds__(kBlock) void k_interp_b3(DSell<T> P, int nsl,
   cc += (size_t)col_ * vsc;
   const size_t o = (size_t)col_ * vsf;
   T sum[B];
-  sell_dot_diag<B, NT, T, T, T>(P.val + e0 + lane, P.col + e0 + lane, w, cc, sdo, P.dtab, rb, sum);
+  sell_dot_diag<B, NT, T, T, T>(MV(P) + e0 + lane, P.col + e0 + lane, w, cc, sdo, P.dtab, rb, sum);
   if (r < 0) return;
   const uint32_t sk = fskip[r];
 #pragma unroll
@@ -940,11 +944,12 @@ __global__ __launch_bounds__(kBlock) void k_base_gemv_b3(int n3, const TI *__res
                                                           int cmode, double damp, double abslimit,
                                                           double red, const double *__restrict__ part,
                                                           int nparts, double *__restrict__ first,
-                                                          int it, size_t vs) {
+                                                          int it, size_t vs, size_t ims) {
   constexpr int B = 3;
   const int row = (blockIdx.x * kBlock + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   const int cj = blockIdx.y;
+  inv += ims * cj;
   const size_t o = (size_t)cj * vs;
   // the row product first, while the norms below are reduced
   double x = row < n3 ? base_row_or_unit<T, TI>(inv, rowmap, cp, ncols, row, b + o, n3, lane) : 0.0;
@@ -1298,8 +1303,8 @@ template <typename T>
 void launch_jac(hipStream_t st, int n, const T *b, const T *diag, const uint8_t *vclass, T damp,
                 T *t, Batch bt) {
   if (n <= 0) return;
-  if (bt.b == 3) k_jac_b3<T><<<grid2(nblk(n), bt.nrhs), kBlock, 0, st>>>(n, b, diag, vclass, damp, t, bt.vs);
-  else k_jac<T><<<grid2(nblk(n), bt.nrhs), kBlock, 0, st>>>(n, b, diag, vclass, damp, t, bt.vs);
+  if (bt.b == 3) k_jac_b3<T><<<grid2(nblk(n), bt.nrhs), kBlock, 0, st>>>(n, b, diag, vclass, damp, t, bt.vs, bt.dms);
+  else k_jac<T><<<grid2(nblk(n), bt.nrhs), kBlock, 0, st>>>(n, b, diag, vclass, damp, t, bt.vs, bt.dms);
 }
 template void launch_jac<double>(hipStream_t, int, const double *, const double *,
                                  const uint8_t *, double, double *, Batch);
@@ -1473,7 +1478,8 @@ template <typename T>
 int launch_restrict(hipStream_t st, const DSell<T> &PT, const uint32_t *cskip,
                     const uint8_t *cvnclass, const uint8_t *cvclass, const T *bf, T damp, T *bc,
                     double *part, bool nt, int nrhs, size_t vsf, size_t vsc, int b,
-                    const uint8_t *cc0, const T *cdiag, T *ct, const int *cpos, T *cbc) {
+                    const uint8_t *cc0, const T *cdiag, T *ct, const int *cpos, T *cbc,
+                    size_t cdms) {
   if (PT.nslices <= 0) return 0;
   const int g = nblk_slices(PT.nslices);
   if (b == 3) {
@@ -1483,18 +1489,19 @@ int launch_restrict(hipStream_t st, const DSell<T> &PT, const uint32_t *cskip,
   }
   W_DISPATCH(PT.wmax, NT_DISPATCH(nt, k_restrict<MWV, T, NTV><<<grid2(g, nrhs), kBlock, 0, st>>>(
                                           PT, PT.nslices, cskip, cvnclass, cvclass, bf, damp, bc,
-                                          part, vsf, vsc, cc0, cdiag, ct, cpos, cbc)));
+                                          part, vsf, vsc, cc0, cdiag, ct, cpos, cbc, cdms)));
   return g;
 }
 template int launch_restrict<double>(hipStream_t, const DSell<double> &, const uint32_t *,
                                      const uint8_t *, const uint8_t *, const double *, double,
                                      double *, double *, bool, int, size_t, size_t, int,
                                      const uint8_t *, const double *, double *, const int *,
-                                     double *);
+                                     double *, size_t);
 template int launch_restrict<float>(hipStream_t, const DSell<float> &, const uint32_t *,
                                     const uint8_t *, const uint8_t *, const float *, float,
                                     float *, double *, bool, int, size_t, size_t, int,
-                                    const uint8_t *, const float *, float *, const int *, float *);
+                                    const uint8_t *, const float *, float *, const int *, float *,
+                                    size_t);
 
 // --------------------------------------------------- base solve (GEMV)
 // A quarter of a row of the inverse times b: columns j0 + 256 m (j0 = 64 q +
@@ -1536,12 +1543,13 @@ __global__ __launch_bounds__(kBlock) void k_base_gemv(int n, const TI *__restric
                                                        int cmode, double damp, double abslimit,
                                                        double red, const double *__restrict__ part,
                                                        int nparts, double *__restrict__ first,
-                                                       int it, size_t vs) {
+                                                       int it, size_t vs, size_t ims) {
   __shared__ double wsum[kBlock / 64];
   const int row = blockIdx.x;
   const int q = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
   const int cj = blockIdx.y;
+  inv += ims * cj;
   const size_t o = (size_t)cj * vs;
   part += (size_t)cj * nparts;
   const int k = rowmap[row];
@@ -1587,11 +1595,11 @@ void launch_base_gemv(hipStream_t st, int n, const TI *inv, const int *rowmap, c
     const int g = (n * 64 + kBlock - 1) / kBlock;
     k_base_gemv_b3<T, TI><<<grid2(g, bt.nrhs), kBlock, 0, st>>>(n, inv, rowmap, cp, ncols, b, c,
                                                                 xout, cmode, damp, abslimit, red,
-                                                                part, nparts, first, it, bt.vs);
+                                                                part, nparts, first, it, bt.vs, bt.ims);
   } else {           // one workgroup per row
     k_base_gemv<T, TI><<<grid2(n, bt.nrhs), kBlock, 0, st>>>(n, inv, rowmap, cp, ncols, b, bc, c, xout,
                                                              cmode, damp, abslimit, red, part,
-                                                             nparts, first, it, bt.vs);
+                                                             nparts, first, it, bt.vs, bt.ims);
   }
 }
 template void launch_base_gemv<double, double>(hipStream_t, int, const double *, const int *,
@@ -1606,6 +1614,24 @@ template void launch_base_gemv<float, float>(hipStream_t, int, const float *, co
                                              const int *, int, const float *, const float *, float *, float *, int,
                                              double, double, double, const double *, int, double *,
                                              int, Batch);
+
+// members: d64[j] = src[map[j]] (0 where map[j] < 0) and its FP32 copy
+__global__ __launch_bounds__(kBlock) void k_member_gather(size_t n, const int *__restrict__ map,
+                                                           const double *__restrict__ src,
+                                                           double *__restrict__ d64,
+                                                           float *__restrict__ d32) {
+  const size_t j = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  if (j >= n) return;
+  const int k = map[j];
+  const double v = k >= 0 ? src[k] : 0.0;
+  d64[j] = v;
+  d32[j] = (float)v;
+}
+void launch_member_gather(hipStream_t st, size_t n, const int *map, const double *src, double *d64,
+                          float *d32) {
+  if (n > 0)
+    k_member_gather<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, st>>>(n, map, src, d64, d32);
+}
 
 // entry (act[k], cp[j]) of the n x n inverse -> (k, j) of the FP64 and FP32
 // copies (rows act, columns in the order cp)
@@ -1968,7 +1994,7 @@ __global__ __launch_bounds__(kBlock) void k_bcgs_spmv(DSell<double> A, int nsl,
   double yz = 0.0, y2 = 0.0;
   if (sok) {
     const double sum =
-        sell_dot<MW, NT, double, TQ, double>(A.val + e0 + lane, A.col + e0 + lane, w, q + o, sdo, A.dtab, rb);
+        sell_dot<MW, NT, double, TQ, double>(MV(A) + e0 + lane, A.col + e0 + lane, w, q + o, sdo, A.dtab, rb);
     if (r >= 0) {
       y[o + r] = sum;
       if (vclass[r] >= 2) {

@@ -527,6 +527,7 @@ def _load_server(L):
         "ugmg_server_step": ([vp, C.POINTER(ip), C.POINTER(ip), C.POINTER(ip)], ip),
         "ugmg_server_status": ([vp, ip, C.POINTER(ip), C.POINTER(LResult), vp], ip),
         "ugmg_server_result": ([vp, ip, vp, ip], ip),
+        "ugmg_server_stats": ([vp, C.POINTER(ip), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)], ip),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -601,6 +602,12 @@ class Server:
         x = np.zeros(n)
         self._chk(self.lib.ugmg_server_result(self.h, rid, x.ctypes.data, n))
         return x
+
+    def stats(self):
+        """(resident groups, group iterations launched, solve iterations in them)"""
+        g, la, co = C.c_int(), C.c_longlong(), C.c_longlong()
+        self._chk(self.lib.ugmg_server_stats(self.h, C.byref(g), C.byref(la), C.byref(co)))
+        return g.value, la.value, co.value
 
     def drain(self, total, sleep=0.0):
         import time

@@ -54,77 +54,111 @@ def load_traffic(name):
 
 
 def serving_run(n_req, rate, seed, budget_gb, slots):
-    """Solve-serving stream: Poisson arrivals of 2D Q1 Poisson and 3D Q1
-    jump-coefficient (kappa = 10^U[-2,2]) solves with unknowns log-uniform in
-    [1e3, 1e6] (the largest uniformly refined level not above the drawn
-    size), relative defect 1e-8, FP64 MC-GS V(2,2)."""
+    """Solve-serving stream: 2D Q1 Poisson and 3D Q1 jump-coefficient
+    (kappa = 10^U[-2,2]) solves with unknowns log-uniform in [1e3, 1e6] (the
+    largest uniformly refined level not above the drawn size), relative
+    defect 1e-8, FP64 MC-GS V(2,2).  Two streams on one server: Poisson
+    arrivals at `rate` (latency), then a burst of as many requests submitted
+    at once (arrivals above capacity: saturated throughput), whose shapes
+    find their groups resident."""
     import numpy as np
     from ug_amd import problems, ugmg
 
     rng = np.random.default_rng(seed)
-    reqs = []
-    for i in range(n_req):
-        dim = 2 if rng.random() < 0.5 else 3
-        target = 10.0 ** rng.uniform(3.0, 6.0)
-        cells = 2 if dim == 2 else 1
-        size = lambda r: (cells * 2 ** r + 1) ** dim
-        nref = 1
-        while size(nref + 1) <= target:
-            nref += 1
-        while size(nref) < 1000:
-            nref += 1
-        kappa = problems.KAPPA_ONE if dim == 2 else problems.KAPPA_JUMP
-        P = problems.q1(dim, cells, nref, kappa, seed=int(seed * 1000 + i))
-        # base level: the largest level with at most 300 vectors (3D: 5^3, 2D:
-        # 17^2).  The 9^3 level (BFS bandwidth 259) would cost ~40 ms of host
-        # band LU + inverse per request against ~0.15 ms per cycle for
-        # smoothing it instead.
-        bl = max(l for l in range(P.top + 1) if P.levels[l].n <= 300)
-        prm = ugmg.default_params(nu1=2, nu2=2, gamma=1, baselevel=bl, smoother=ugmg.SMOOTH_GS, damp=1.0,
-                                  base_abslimit=1e-300, use_graph=1)
-        reqs.append((P, prm))
-    gaps = rng.exponential(1.0 / rate, n_req)
-    arrivals = np.cumsum(gaps) - gaps[0]
+
+    def draw(k0):
+        reqs = []
+        for i in range(n_req):
+            dim = 2 if rng.random() < 0.5 else 3
+            target = 10.0 ** rng.uniform(3.0, 6.0)
+            cells = 2 if dim == 2 else 1
+            size = lambda r: (cells * 2 ** r + 1) ** dim
+            nref = 1
+            while size(nref + 1) <= target:
+                nref += 1
+            while size(nref) < 1000:
+                nref += 1
+            kappa = problems.KAPPA_ONE if dim == 2 else problems.KAPPA_JUMP
+            P = problems.q1(dim, cells, nref, kappa, seed=int(seed * 1000 + k0 + i))
+            # base level: the largest level with at most 300 vectors (3D:
+            # 5^3, 2D: 17^2).  The 9^3 level (BFS bandwidth 259) would cost
+            # ~40 ms of host band LU + inverse per request against ~0.15 ms
+            # per cycle for smoothing it instead.
+            bl = max(l for l in range(P.top + 1) if P.levels[l].n <= 300)
+            prm = ugmg.default_params(nu1=2, nu2=2, gamma=1, baselevel=bl, smoother=ugmg.SMOOTH_GS, damp=1.0,
+                                      base_abslimit=1e-300, use_graph=1)
+            reqs.append((P, prm))
+        return reqs
+
+    def run(srv, reqs, arrivals):
+        ids = []
+        t0 = time.perf_counter()
+        k = 0
+        done0 = srv.step()[2]
+        while True:
+            now = time.perf_counter() - t0
+            while k < len(reqs) and arrivals[k] <= now:
+                P, prm = reqs[k]
+                ids.append(srv.submit(P, prm, red=1e-8, maxiter=500))
+                k += 1
+            a, q, d = srv.step()
+            if k == len(reqs) and d - done0 >= len(reqs):
+                break
+            if a == 0 and k < len(reqs):
+                time.sleep(min(0.001, max(0.0, arrivals[k] - now)))
+        lat, gpu, ok, rows = [], [], 0, []
+        t_first, t_last = None, 0.0
+        for i, rid in enumerate(ids):
+            st, res, (ts, ta, td) = srv.status(rid)
+            if st == ugmg.REQ_DONE and res.converged:
+                ok += 1
+            lat.append(td - ts)
+            gpu.append(td - ta)
+            rows.append((td - ts, ta - ts, td - ta, reqs[i][0].levels[-1].n, res.number_of_linear_iterations))
+            t_first = ts if t_first is None else min(t_first, ts)
+            t_last = max(t_last, td)
+        slow = max(rows)
+        return {
+            "requests": len(reqs), "converged": ok,
+            "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
+            "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2),
+            "admitted_to_done_p50_ms": round(float(np.percentile(gpu, 50)) * 1e3, 2),
+            "slowest": {"unknowns": int(slow[3]), "submit_to_admit_ms": round(slow[1] * 1e3, 2),
+                        "admit_to_done_ms": round(slow[2] * 1e3, 2), "iterations": int(slow[4])},
+            "solves_per_s": round(len(reqs) / (t_last - t_first), 2),
+        }
+
     srv = ugmg.Server(0, budget_bytes=budget_gb * 1e9, max_slots=slots)
-    ids = []
-    t0 = time.perf_counter()
-    k = 0
-    while True:
-        now = time.perf_counter() - t0
-        while k < n_req and arrivals[k] <= now:
-            P, prm = reqs[k]
-            ids.append(srv.submit(P, prm, red=1e-8, maxiter=500))
-            k += 1
-        a, q, d = srv.step()
-        if k == n_req and d >= n_req:
-            break
-        if a == 0 and k < n_req:
-            time.sleep(min(0.001, max(0.0, arrivals[k] - now)))
-    lat, gpu, ok, rows = [], [], 0, []
-    t_first, t_last = None, 0.0
-    for k, rid in enumerate(ids):
-        st, res, (ts, ta, td) = srv.status(rid)
-        if st == ugmg.REQ_DONE and res.converged:
-            ok += 1
-        lat.append(td - ts)
-        gpu.append(td - ta)
-        rows.append((td - ts, ta - ts, td - ta, reqs[k][0].levels[-1].n, res.number_of_linear_iterations))
-        t_first = ts if t_first is None else min(t_first, ts)
-        t_last = max(t_last, td)
+    # cold start: a burst on an empty server, every shape's first request
+    # waits for its group's host layout and device set-up
+    cold = draw(0)
+    cold_run = run(srv, cold, np.zeros(n_req))
+    g0, l0, c0 = srv.stats()
+    # Poisson arrivals with the groups resident
+    reqs = draw(n_req)
+    gaps = rng.exponential(1.0 / rate, n_req)
+    lat_run = run(srv, reqs, np.cumsum(gaps) - gaps[0])
+    g1, l1, c1 = srv.stats()
+    # arrivals above capacity: a burst
+    burst = draw(2 * n_req)
+    cap_run = run(srv, burst, np.zeros(n_req))
+    g2, l2, c2 = srv.stats()
     srv.close()
-    sizes = [r[0].levels[-1].n for r in reqs]
-    return {
-        "requests": n_req, "converged": ok, "arrival_rate_per_s": rate, "slots": slots,
-        "hbm_budget_gb": budget_gb,
-        "unknowns_min_median_max": [int(min(sizes)), int(np.median(sizes)), int(max(sizes))],
-        "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
-        "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2),
-        "admitted_to_done_p50_ms": round(float(np.percentile(gpu, 50)) * 1e3, 2),
-        "slowest": {"unknowns": int(max(rows)[3]), "submit_to_admit_ms": round(max(rows)[1] * 1e3, 2),
-                    "admit_to_done_ms": round(max(rows)[2] * 1e3, 2), "iterations": int(max(rows)[4])},
-        "solves_per_s": round(n_req / (t_last - t_first), 2),
-        "note": "latency = completion - submission, including the server's host layout build",
-    }
+    sizes = [r[0].levels[-1].n for r in cold + reqs + burst]
+    cold_run.update({"arrival": f"all {n_req} submitted at once to an empty server",
+                     "resident_groups": g0, "solves_per_launch": round(c0 / max(l0, 1), 2)})
+    lat_run.update({"arrival_rate_per_s": rate, "solves_per_launch": round((c1 - c0) / max(l1 - l0, 1), 2)})
+    cap_run.update({"arrival": f"all {n_req} submitted at once (above capacity), groups resident",
+                    "solves_per_launch": round((c2 - c1) / max(l2 - l1, 1), 2),
+                    "resident_groups": g2})
+    return {"slots": slots, "hbm_budget_gb": budget_gb,
+            "unknowns_min_median_max": [int(min(sizes)), int(np.median(sizes)), int(max(sizes))],
+            "poisson_arrivals": lat_run, "saturated": cap_run, "cold_start": cold_run,
+            "p50_ms": lat_run["p50_ms"], "p99_ms": lat_run["p99_ms"],
+            "capacity_solves_per_s": cap_run["solves_per_s"],
+            "note": "latency = completion - submission; each group iteration is one batched launch sequence over "
+                    "the group's solves (solves_per_launch); cold_start includes each shape's first host layout "
+                    "build and device set-up"}
 
 
 def assemble_run(problems, ugmg, torch, nref, top):

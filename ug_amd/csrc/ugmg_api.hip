@@ -317,25 +317,19 @@ int upload_sell(ugmg_ctx *ctx, const HostSell &S, DevSellBuf &D, int members = 1
   D.ndtab = S.dtab.size();
   D.bb = S.bb;
   D.nval = S.val.size();
+  if (members > 1) {   // the values are the index maps; member_set fills the planes
+    if ((rc = dalloc(ctx, &D.v64, D.nval * members)) || (rc = dalloc(ctx, &D.v32, D.nval * members)))
+      return rc;
+    HIPCHK(hipMemset(D.v64, 0, sizeof(double) * D.nval * members));
+    HIPCHK(hipMemset(D.v32, 0, sizeof(float) * D.nval * members));
+    D.mst = D.nval;
+    return 0;
+  }
+  if ((rc = upload(ctx, &D.v64, S.val.data(), D.nval))) return rc;
   hvec<float> v32;
   to_float(S.val.data(), S.val.size(), v32);
-  const int m = members;
-  if ((rc = dalloc(ctx, &D.v64, D.nval * m)) || (rc = dalloc(ctx, &D.v32, D.nval * m))) return rc;
-  for (int k = 0; k < m && D.nval; ++k) {
-    HIPCHK(hipMemcpy(D.v64 + k * D.nval, S.val.data(), D.nval * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(D.v32 + k * D.nval, v32.data(), D.nval * sizeof(float), hipMemcpyHostToDevice));
-  }
-  D.mst = m > 1 ? D.nval : 0;
-  return 0;
-}
-
-// the same for a per-row array of a level (the diagonal blocks)
-template <typename T>
-int upload_members(ugmg_ctx *ctx, T **out, const T *src, size_t n, int members) {
-  int rc = dalloc(ctx, out, n * members);
-  if (rc) return rc;
-  for (int k = 0; k < members && n; ++k)
-    HIPCHK(hipMemcpy(*out + k * n, src, n * sizeof(T), hipMemcpyHostToDevice));
+  if ((rc = upload(ctx, &D.v32, v32.data(), v32.size()))) return rc;
+  D.mst = 0;
   return 0;
 }
 
@@ -1050,16 +1044,29 @@ int prep_host(ugmg_ctx *ctx, int level, PrepState &ps) {
     std::vector<HostLevel> &plv = ps.plv;          // levels in device order
     plv.assign(level + 1, HostLevel());
     std::vector<std::vector<int>> pos(level + 1);   // list index -> device position
+    // members: the layouts are built from index-valued copies of the levels
+    // (value k + 1 at CSR value k); the builders only move values, so every
+    // slab value slot then holds its source index, and each member's values
+    // are gathered through those maps (member_set)
+    std::vector<HostLevel> idx;
+    if (ctx->members > 1) {
+      idx.resize(level + 1);
+      for (int l = bl; l <= level; ++l) {
+        idx[l] = ctx->host[l];
+        for (size_t q = 0; q < idx[l].a.size(); ++q) idx[l].a[q] = (double)(q + 1);
+      }
+    }
+    auto src = [&](int l) -> const HostLevel & { return idx.empty() ? ctx->host[l] : idx[l]; };
     for (int l = bl; l <= level; ++l) {
       const bool gs = p.smoother == UGMG_SMOOTH_GS && l > bl;
       HostPrep &hp = ps.hp[l];
       const bool cperm = l > bl && !pos[l - 1].empty();
-      const HostLevel *coarse = l > bl ? (cperm ? &plv[l - 1] : &ctx->host[l - 1]) : nullptr;
+      const HostLevel *coarse = l > bl ? (cperm ? &plv[l - 1] : &src(l - 1)) : nullptr;
       if (!gs) {
-        build_prep(ctx->host[l], coarse, nullptr, 0, hp);
+        build_prep(src(l), coarse, nullptr, 0, hp);
         continue;
       }
-      const HostLevel &H = ctx->host[l];
+      const HostLevel &H = src(l);
       std::vector<int> col;
       const int nc = greedy_color(H, col);
       const int win = force_win >= 0 ? force_win : choose_gs_window(H, col, nc);
@@ -1078,23 +1085,8 @@ int prep_host(ugmg_ctx *ctx, int level, PrepState &ps) {
       ps.map_l.assign(level + 1, {});
       ps.map_u.assign(level + 1, {});
       ps.map_d.assign(level + 1, {});
-      for (int l = bl; l <= level; ++l) {
-        HostLevel I = ctx->host[l];
-        for (size_t q = 0; q < I.a.size(); ++q) I.a[q] = (double)(q + 1);
-        I.has_p = false;
-        HostPrep hi;
-        const HostPrep &hp = ps.hp[l];
-        if (hp.perm.empty()) {
-          build_prep(I, nullptr, nullptr, 0, hi);
-        } else {
-          HostLevel IP;
-          std::vector<int> ipos(I.n);
-          for (int q = 0; q < I.n; ++q) ipos[hp.perm[q]] = q;
-          permute_level(I, hp.perm, ipos, nullptr, IP);
-          build_prep(IP, nullptr, &hp.color, hp.ncolors, hi);
-        }
-        index_maps(hi, ps.map_a[l], ps.map_l[l], ps.map_u[l], ps.map_d[l]);
-      }
+      for (int l = bl; l <= level; ++l)
+        index_maps(ps.hp[l], ps.map_a[l], ps.map_l[l], ps.map_u[l], ps.map_d[l]);
     }
     // EXPreProcess: order and band here.  The band LU and the inverse of a
     // large base level run on the device in prep_device; a small one is
@@ -1180,10 +1172,19 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
       max_slices = std::max(max_slices, hp.A.nslices);
       const int mb = ctx->members;
       if ((rc = upload_sell(ctx, hp.A, D.A, mb))) return rc;
-      if ((rc = upload_members(ctx, &D.diag64, hp.diag.data(), hp.diag.size(), mb))) return rc;
-      hvec<float> d32;
-      to_float(hp.diag.data(), hp.diag.size(), d32);
-      if ((rc = upload_members(ctx, &D.diag32, d32.data(), d32.size(), mb))) return rc;
+      if (mb > 1) {   // diagonal blocks 1 until member_set fills a member's
+        if ((rc = dalloc(ctx, &D.diag64, hp.diag.size() * mb)) ||
+            (rc = dalloc(ctx, &D.diag32, hp.diag.size() * mb)))
+          return rc;
+        launch_fill<double>(ctx->stream, (int)(hp.diag.size() * mb), D.diag64, 1.0, Batch{1, 0, 1});
+        launch_fill<float>(ctx->stream, (int)(hp.diag.size() * mb), D.diag32, 1.0f, Batch{1, 0, 1});
+        HIPCHK(hipGetLastError());
+      } else {
+        if ((rc = upload(ctx, &D.diag64, hp.diag.data(), hp.diag.size()))) return rc;
+        hvec<float> d32;
+        to_float(hp.diag.data(), hp.diag.size(), d32);
+        if ((rc = upload(ctx, &D.diag32, d32.data(), d32.size()))) return rc;
+      }
       D.dms = mb > 1 ? hp.diag.size() : 0;
       if (mb > 1 &&
           ((rc = upload(ctx, &D.map_a, ps.map_a[l].data(), ps.map_a[l].size())) ||
@@ -2837,9 +2838,18 @@ struct SrvGroup {
   }
 };
 
+// 64-bit hash of a byte range, eight bytes per step (a key for finding a
+// request's group; equal keys are confirmed by same_shape)
 uint64_t fnv(uint64_t h, const void *p, size_t n) {
   const unsigned char *c = static_cast<const unsigned char *>(p);
-  for (size_t i = 0; i < n; ++i) h = (h ^ c[i]) * 1099511628211ULL;
+  size_t i = 0;
+  for (; i + 8 <= n; i += 8) {
+    uint64_t w;
+    std::memcpy(&w, c + i, 8);
+    h = (h ^ w) * 0x9E3779B97F4A7C15ULL;
+    h ^= h >> 29;
+  }
+  for (; i < n; ++i) h = (h ^ c[i]) * 1099511628211ULL;
   return h;
 }
 

@@ -15,11 +15,17 @@ minutes, through properties that do not depend on size:
   own is bit-identical to the same column inside the batch.
 * 65^3 P1 elasticity (3x3 blocks): FP64 and mixed solves reach 1e-10 per
   component (true defect per component, recomputed on the host).
+* Against the oracle at full size: the FP64 and mixed MC-GS solutions of
+  the 129^3 problem within 1e-8 relative of the oracle's lexicographic-GS
+  solve to the same tolerance (UG's l_lgs order), and two of the 32 FP64
+  BiCGStab columns at 513^2 with the oracle's half-step defects (1e-6
+  relative, or 5e-11 of the first) and x within 1e-8.
 """
 import numpy as np
 import pytest
 import scipy.sparse as sp
 
+from oracle import oracle as O
 from ug_amd import problems, ugmg
 
 pytestmark = pytest.mark.gpu
@@ -156,3 +162,48 @@ def test_65_elasticity_solves():
         r = f - A @ x
         for c in range(3):
             assert np.linalg.norm(r[c::3]) <= 2e-10 * np.linalg.norm(f[c::3]), (prec, c)
+
+
+def test_129_mcgs_solution_matches_oracle_lexicographic_gs(jump129):
+    P = jump129
+    print("129^3 oracle lexicographic GS solve (about half a minute on the host)", flush=True)
+    H = O.Hierarchy(P.levels)
+    ref = O.ls_solve(H, P.top, O.params(smoother=O.GS, damp=1.0, baselevel=4), P.rhs, red=1e-10, maxiter=500)
+    assert ref["converged"]
+    for prec in (ugmg.FP64, ugmg.MIXED):
+        ctx = _ctx(P, prec)
+        x, b, res, d = ctx.solve_host(P.top, P.rhs, red=1e-10, maxiter=200)
+        ctx.close()
+        assert res.converged
+        rel = np.linalg.norm(x - ref["x"]) / np.linalg.norm(ref["x"])
+        print(f"precision {prec}: GPU MC-GS vs oracle lexicographic GS {rel:.2e} "
+              f"({res.number_of_linear_iterations} vs {ref['nit']} iterations)")
+        assert rel <= 1e-8, (prec, rel)
+
+
+def test_513_bcgs_columns_match_oracle():
+    C = problems.p1_cd(8, seed=0x5EED0003)
+    n = C.levels[-1].n
+    bl = max(l for l in range(C.top + 1) if C.levels[l].n <= 729)
+    rng = np.random.default_rng(513)
+    R = rng.uniform(-1, 1, (32, n))
+    R[:, C.levels[-1].skip.astype(bool)] = 0.0
+    ctx = ugmg.Context(0)
+    ctx.set_hierarchy(C)
+    ctx.set_params(nu1=1, nu2=1, gamma=1, baselevel=bl, smoother=ugmg.SMOOTH_GS, damp=1.0,
+                   base_abslimit=1e-300, base_red=1e-12, base_maxiter=1, use_graph=1, precision=ugmg.FP64)
+    ctx.set_batch(32)
+    ctx.preprocess()
+    X, B, results, seqs = ctx.bcgs_host(C.top, R, red=1e-8, maxiter=200)
+    ctx.close()
+    H = O.Hierarchy(C.levels)
+    p = O.params(smoother=O.MCGS, damp=1.0, baselevel=bl, nu1=1, nu2=1)
+    for j in (0, 17):
+        ref = O.bcgs_solve(H, C.top, p, R[j], red=1e-8, maxiter=200)
+        assert ref["converged"] and results[j].converged
+        assert results[j].number_of_linear_iterations == ref["nit"], j
+        orc = ref["defects"][:, 0]
+        gpu = np.asarray(seqs[j])[:len(orc)]
+        np.testing.assert_allclose(gpu, orc, rtol=1e-6, atol=5e-11 * orc[0])
+        rel = np.linalg.norm(X[j] - ref["x"]) / np.linalg.norm(ref["x"])
+        assert rel <= 1e-8, (j, rel)

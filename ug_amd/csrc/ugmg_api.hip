@@ -429,7 +429,8 @@ void smooth_step(ugmg_ctx *ctx, int l, T *t, T *b, T *c, int cmode, T *xout, int
                             d.color_slices[k + 1] - d.color_slices[k], d.color_wmax[k], b,
                             d.diag<T>(), d.vclass, t, d.ntv<T>(), bt);
     launch_gs_defect<T, T>(st, d.U.d<T>(), t, b, d.vclass, damp, cmode < 0 ? nullptr : c,
-                           cmode < 0 ? 1 : cmode, xout, d.ntv<T>(), bt);
+                           cmode < 0 ? 1 : cmode, xout, d.ntv<T>(), bt, d.color_slices.data(),
+                           d.ncolors);
   }
 }
 
@@ -458,7 +459,8 @@ void smooth(ugmg_ctx *ctx, int l, T *t, T *b, T *c, int cmode, int nsteps, int k
                                   d.color_slices[k + 1] - d.color_slices[k], d.color_wmax[k],
                                   d.color_wmax_u[k], b, d.diag<T>(), damp, t, c,
                                   i == 1 ? cmode : 1, d.ntv<T>(), bt, ctx->gs_split);
-  launch_gs_defect<T, T>(st, d.U.d<T>(), t, b, d.vclass, damp, c, 1, nullptr, d.ntv<T>(), bt);
+  launch_gs_defect<T, T>(st, d.U.d<T>(), t, b, d.vclass, damp, c, 1, nullptr, d.ntv<T>(), bt,
+                         d.color_slices.data(), d.ncolors);
 }
 
 // A GS level whose rows are all active (scalar): the colour-0 sweep of a

@@ -197,6 +197,28 @@ __device__ __forceinline__ int xcd_chunked(int bid, int nb) {
   return ((k / kChunk) * 8 + x) * kChunk + k % kChunk;
 }
 
+// Colour-balanced spatial map for a launch over every colour of a
+// colour-grouped slab (the GS defect): XCD x (hardware block h runs on XCD
+// h % 8) takes the x-th eighth of each colour's blocks, so every XCD gets
+// the same share of each colour's work (their rows' widths differ) and its
+// gathers stay in one spatial eighth of the level, whose vector fits its
+// L2.  cb[k]: first block of colour k, cb[n] = all blocks.  -1: no block.
+struct ColourBlocks {
+  int n;
+  int cb[kMaxColors + 1];
+};
+__device__ __forceinline__ int xcd_colour_block(int h, const ColourBlocks &m) {
+  const int x = h & 7;
+  int j = h >> 3;
+  for (int k = 0; k < m.n; ++k) {
+    const int nk = m.cb[k + 1] - m.cb[k];
+    const int lo = (x * nk) >> 3, hi = ((x + 1) * nk) >> 3;
+    if (j < hi - lo) return m.cb[k] + lo + j;
+    j -= hi - lo;
+  }
+  return -1;
+}
+
 // Slice block and column of this workgroup.  Single column: the XCD-aware
 // remap of blockIdx.x (contiguous range per XCD, or chunked round-robin
 // with CHUNKED).  Several columns: the linear workgroup id is remapped over
@@ -476,8 +498,24 @@ __global__ __launch_bounds__(kBlock) void k_gs_defect(DSell<TV> U, int nsl,
                                                        T *__restrict__ b,
                                                        const uint8_t *__restrict__ vclass,
                                                        T damp, T *__restrict__ c,
-                                                       T *__restrict__ xout, size_t vs) {
-  SLICE_PROLOGUE_MAP(U, 0, nsl, true)
+                                                       T *__restrict__ xout, size_t vs,
+                                                       ColourBlocks cbm) {
+  int sb_, col_;
+  if (cbm.n > 0 && gridDim.y == 1) {
+    sb_ = xcd_colour_block(blockIdx.x, cbm);
+    col_ = 0;
+    if (sb_ < 0) return;
+  } else {
+    batch_map<true>(sb_, col_);
+  }
+  const int s = __builtin_amdgcn_readfirstlane(sb_ * (kBlock / kSlice) + (int)(threadIdx.x >> 6));
+  if (s >= nsl) return;
+  const int lane = threadIdx.x & 63;
+  const int r = U.rows[s * kSlice + lane];
+  const int e0 = U.soff[s];
+  const int w = (U.soff[s + 1] - e0) >> 6;
+  const int sdo = U.sdoff[s];
+  const int rb = r >= 0 ? r : U.rows[s * kSlice];
   const size_t o = (size_t)col_ * vs;
   v += o;
   T br = 0, vr = 0, cr = 0;
@@ -1449,9 +1487,27 @@ template void launch_gs_color_fused<float, float>(hipStream_t, const DSell<float
 
 template <typename TV, typename T>
 void launch_gs_defect(hipStream_t st, const DSell<TV> &U, const T *v, T *b, const uint8_t *vclass,
-                      T damp, T *c, int cmode, T *xout, bool nt, Batch bt) {
+                      T damp, T *c, int cmode, T *xout, bool nt, Batch bt, const int *color_slices,
+                      int ncolors) {
   if (U.nslices <= 0) return;
-  const dim3 g = grid2(nblk_slices(U.nslices), bt.nrhs);
+  dim3 g = grid2(nblk_slices(U.nslices), bt.nrhs);
+  ColourBlocks cbm{};
+  if (color_slices && ncolors > 0 && ncolors <= kMaxColors && bt.nrhs <= 1 && bt.b == 1) {
+    const int nb = (int)g.x, per = kBlock / kSlice;
+    cbm.n = ncolors;
+    for (int k = 0; k < ncolors; ++k) cbm.cb[k] = std::min(color_slices[k] / per, nb);
+    cbm.cb[ncolors] = nb;
+    int most = 0;   // the busiest XCD's blocks
+    for (int x = 0; x < 8; ++x) {
+      int tot = 0;
+      for (int k = 0; k < ncolors; ++k) {
+        const int nk = cbm.cb[k + 1] - cbm.cb[k];
+        tot += ((x + 1) * nk >> 3) - (x * nk >> 3);
+      }
+      most = std::max(most, tot);
+    }
+    g.x = 8 * most;
+  }
   if (bt.b == 3) {
     NT_DISPATCH(nt, if (cmode == 0) k_gs_defect_b3<TV, T, 0, NTV><<<g, kBlock, 0, st>>>(
                         U, U.nslices, v, b, vclass, damp, c, xout, bt.vs);
@@ -1463,16 +1519,17 @@ void launch_gs_defect(hipStream_t st, const DSell<TV> &U, const T *v, T *b, cons
   // off-diagonal row down to 0, and a wider chunk mostly loads clamped
   // repeats on the narrow colours (measured slower for 28 and 16)
   NT_DISPATCH(nt, if (cmode == 0) k_gs_defect<8, TV, T, 0, NTV><<<g, kBlock, 0, st>>>(
-                      U, U.nslices, v, b, vclass, damp, c, xout, bt.vs);
+                      U, U.nslices, v, b, vclass, damp, c, xout, bt.vs, cbm);
               else k_gs_defect<8, TV, T, 1, NTV><<<g, kBlock, 0, st>>>(
-                      U, U.nslices, v, b, vclass, damp, c, xout, bt.vs));
+                      U, U.nslices, v, b, vclass, damp, c, xout, bt.vs, cbm));
 }
 template void launch_gs_defect<double, double>(hipStream_t, const DSell<double> &,
                                                const double *, double *, const uint8_t *, double,
-                                               double *, int, double *, bool, Batch);
+                                               double *, int, double *, bool, Batch, const int *,
+                                               int);
 template void launch_gs_defect<float, float>(hipStream_t, const DSell<float> &, const float *,
                                              float *, const uint8_t *, float, float *, int,
-                                             float *, bool, Batch);
+                                             float *, bool, Batch, const int *, int);
 
 template <typename T>
 int launch_restrict(hipStream_t st, const DSell<T> &PT, const uint32_t *cskip,

@@ -103,8 +103,18 @@ struct DevSellBuf {
   // blocks of member m at diag + m * dms (0: one matrix)
   size_t mst = 0, dms = 0;
   size_t nval = 0;              // values of one member (bb planes of nent)
-  DSell<double> d64() const { return {nslices, soff, rows, col, v64, sdoff, dtab, wmax, nent, mst, dms}; }
-  DSell<float> d32() const { return {nslices, soff, rows, col, v32, sdoff, dtab, wmax, nent, mst, dms}; }
+  // colour map of a colour-grouped slab (DSell::ncb, cb)
+  int ncb = 0;
+  int cb[kMaxColors + 1] = {};
+  template <typename TV>
+  DSell<TV> view(const TV *v) const {
+    DSell<TV> d{nslices, soff, rows, col, v, sdoff, dtab, wmax, nent, mst, dms};
+    d.ncb = ncb;
+    for (int k = 0; k <= ncb && k <= kMaxColors; ++k) d.cb[k] = cb[k];
+    return d;
+  }
+  DSell<double> d64() const { return view<double>(v64); }
+  DSell<float> d32() const { return view<float>(v32); }
   template <typename T>
   DSell<T> d() const {
     if constexpr (std::is_same_v<T, double>) return d64();
@@ -429,8 +439,7 @@ void smooth_step(ugmg_ctx *ctx, int l, T *t, T *b, T *c, int cmode, T *xout, int
                             d.color_slices[k + 1] - d.color_slices[k], d.color_wmax[k], b,
                             d.diag<T>(), d.vclass, t, d.ntv<T>(), bt);
     launch_gs_defect<T, T>(st, d.U.d<T>(), t, b, d.vclass, damp, cmode < 0 ? nullptr : c,
-                           cmode < 0 ? 1 : cmode, xout, d.ntv<T>(), bt, d.color_slices.data(),
-                           d.ncolors);
+                           cmode < 0 ? 1 : cmode, xout, d.ntv<T>(), bt);
   }
 }
 
@@ -459,8 +468,7 @@ void smooth(ugmg_ctx *ctx, int l, T *t, T *b, T *c, int cmode, int nsteps, int k
                                   d.color_slices[k + 1] - d.color_slices[k], d.color_wmax[k],
                                   d.color_wmax_u[k], b, d.diag<T>(), damp, t, c,
                                   i == 1 ? cmode : 1, d.ntv<T>(), bt, ctx->gs_split);
-  launch_gs_defect<T, T>(st, d.U.d<T>(), t, b, d.vclass, damp, c, 1, nullptr, d.ntv<T>(), bt,
-                         d.color_slices.data(), d.ncolors);
+  launch_gs_defect<T, T>(st, d.U.d<T>(), t, b, d.vclass, damp, c, 1, nullptr, d.ntv<T>(), bt);
 }
 
 // A GS level whose rows are all active (scalar): the colour-0 sweep of a
@@ -1220,6 +1228,15 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
         }
       }
       D.A.dms = D.L.dms = D.U.dms = D.dms;
+      // scalar GS levels: the colour map of the whole-slab launches over A and U
+      if (hp.ncolors > 0 && hp.ncolors <= kMaxColors && D.nb == 1 && D.A.nslices == D.U.nslices) {
+        const int per = kBlock / kSlice, nblk = (D.A.nslices + per - 1) / per;
+        for (DevSellBuf *S : {&D.A, &D.U}) {
+          S->ncb = hp.ncolors;
+          for (int k = 0; k < hp.ncolors; ++k) S->cb[k] = std::min(hp.color_slices[k] / per, nblk);
+          S->cb[hp.ncolors] = nblk;
+        }
+      }
       if (l > bl) {
         D.p_nnz = H.pp[H.n];
         D.pt_nnz = (int)hp.pt_idx.size();

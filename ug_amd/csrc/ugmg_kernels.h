@@ -32,6 +32,12 @@ struct DSell {
   // blocks at diag + j * dms; 0 = one matrix for every column
   size_t mst = 0;
   size_t dms = 0;
+  // colour-grouped slab of a scalar GS level (A, U): launches over the whole
+  // slab with one column give XCD x the x-th eighth of every colour's blocks
+  // (blocks of kBlock / 64 slices; cb[k] = first block of colour k, cb[ncb]
+  // = all blocks); ncb = 0: the plain XCD remap
+  int ncb = 0;
+  int cb[kMaxColors + 1] = {};
 };
 
 // base-level solver settings: ls($m,$red,$abslimit) over ex($damp)
@@ -130,13 +136,9 @@ void launch_gs_color_fused(hipStream_t st, const DSell<TV> &L, const DSell<TV> &
 template <typename TV, typename T>
 void launch_spmv_c0(hipStream_t st, const DSell<TV> &A, const T *x, T *y, const T *diag, T *u,
                     int ncs0, bool nt, Batch bt);
-// color_slices (host, ncolors + 1 slice offsets of the colour-grouped rows;
-// null: no colour map): single-column launches give XCD x the x-th eighth
-// of every colour's blocks
 template <typename TV, typename T>
 void launch_gs_defect(hipStream_t st, const DSell<TV> &U, const T *v, T *b, const uint8_t *vclass,
-                      T damp, T *c, int cmode, T *xout, bool nt, Batch bt,
-                      const int *color_slices = nullptr, int ncolors = 0);
+                      T damp, T *c, int cmode, T *xout, bool nt, Batch bt);
 // returns the number of blocks per column (= partial sums per column when
 // part != null; column j's partials start at part + j * return value).
 // With ct (scalar only) also ct_r = bc_r / cdiag_r on the coarse rows whose

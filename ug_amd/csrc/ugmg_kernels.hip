@@ -203,17 +203,13 @@ __device__ __forceinline__ int xcd_chunked(int bid, int nb) {
 // the same share of each colour's work (their rows' widths differ) and its
 // gathers stay in one spatial eighth of the level, whose vector fits its
 // L2.  cb[k]: first block of colour k, cb[n] = all blocks.  -1: no block.
-struct ColourBlocks {
-  int n;
-  int cb[kMaxColors + 1];
-};
-__device__ __forceinline__ int xcd_colour_block(int h, const ColourBlocks &m) {
+__device__ __forceinline__ int xcd_colour_block(int h, int n, const int *cb) {
   const int x = h & 7;
   int j = h >> 3;
-  for (int k = 0; k < m.n; ++k) {
-    const int nk = m.cb[k + 1] - m.cb[k];
+  for (int k = 0; k < n; ++k) {
+    const int nk = cb[k + 1] - cb[k];
     const int lo = (x * nk) >> 3, hi = ((x + 1) * nk) >> 3;
-    if (j < hi - lo) return m.cb[k] + lo + j;
+    if (j < hi - lo) return cb[k] + lo + j;
     j -= hi - lo;
   }
   return -1;
@@ -237,13 +233,26 @@ __device__ __forceinline__ void batch_map(int &sb, int &col) {
   col = lr % gridDim.y;
 }
 
+// the slice block of a launch over a slab: the colour map of a whole
+// colour-grouped slab with one column, else batch_map; -1: no block
+template <bool CHUNKED, typename TV>
+__device__ __forceinline__ void slab_map(const DSell<TV> &S, int slice0, int nsl, int &sb, int &col) {
+  if (S.ncb > 0 && gridDim.y == 1 && slice0 == 0 && nsl == S.nslices) {
+    sb = xcd_colour_block(blockIdx.x, S.ncb, S.cb);
+    col = 0;
+    return;
+  }
+  batch_map<CHUNKED>(sb, col);
+}
+
 // slice s (wave-uniform, in scalar registers), lane, row r (-1 = padding),
 // entry base e0, width w, uniform-offset list sdo, index base rb, slice
 // block sb_, column col_
 #define SLICE_PROLOGUE(S, slice0, nsl) SLICE_PROLOGUE_MAP(S, slice0, nsl, false)
 #define SLICE_PROLOGUE_MAP(S, slice0, nsl, CHUNKED)                                      \
   int sb_, col_;                                                                         \
-  batch_map<CHUNKED>(sb_, col_);                                                         \
+  slab_map<CHUNKED>(S, slice0, nsl, sb_, col_);                                          \
+  if (sb_ < 0) return;                                                                   \
   const int s = __builtin_amdgcn_readfirstlane((slice0) + sb_ * (kBlock / kSlice) +      \
                                                (int)(threadIdx.x >> 6));                 \
   if (s >= (slice0) + (nsl)) return;                                                     \
@@ -271,7 +280,8 @@ __device__ __forceinline__ void batch_map(int &sb, int &col) {
 // same, for kernels that end in a block reduction (no early return)
 #define SLICE_PROLOGUE_RED(S, nsl)                                                       \
   int sb_, col_;                                                                         \
-  batch_map(sb_, col_);                                                                  \
+  slab_map<false>(S, 0, nsl, sb_, col_);                                                 \
+  if (sb_ < 0) return;   /* the whole workgroup */                                       \
   const int s = __builtin_amdgcn_readfirstlane(sb_ * (kBlock / kSlice) +                 \
                                                (int)(threadIdx.x >> 6));                 \
   const int lane = threadIdx.x & 63;                                                     \
@@ -498,24 +508,8 @@ __global__ __launch_bounds__(kBlock) void k_gs_defect(DSell<TV> U, int nsl,
                                                        T *__restrict__ b,
                                                        const uint8_t *__restrict__ vclass,
                                                        T damp, T *__restrict__ c,
-                                                       T *__restrict__ xout, size_t vs,
-                                                       ColourBlocks cbm) {
-  int sb_, col_;
-  if (cbm.n > 0 && gridDim.y == 1) {
-    sb_ = xcd_colour_block(blockIdx.x, cbm);
-    col_ = 0;
-    if (sb_ < 0) return;
-  } else {
-    batch_map<true>(sb_, col_);
-  }
-  const int s = __builtin_amdgcn_readfirstlane(sb_ * (kBlock / kSlice) + (int)(threadIdx.x >> 6));
-  if (s >= nsl) return;
-  const int lane = threadIdx.x & 63;
-  const int r = U.rows[s * kSlice + lane];
-  const int e0 = U.soff[s];
-  const int w = (U.soff[s + 1] - e0) >> 6;
-  const int sdo = U.sdoff[s];
-  const int rb = r >= 0 ? r : U.rows[s * kSlice];
+                                                       T *__restrict__ xout, size_t vs) {
+  SLICE_PROLOGUE_MAP(U, 0, nsl, true)
   const size_t o = (size_t)col_ * vs;
   v += o;
   T br = 0, vr = 0, cr = 0;
@@ -1255,6 +1249,24 @@ int reduce_parts(int n);
 static inline int nblk(int n) { return (n + kBlock - 1) / kBlock; }
 static inline int nblk_slices(int nsl) { return (nsl + kBlock / kSlice - 1) / (kBlock / kSlice); }
 static inline dim3 grid2(int gx, int nrhs) { return dim3(gx, nrhs < 1 ? 1 : nrhs); }
+// grid of a launch over slices [slice0, slice0 + nsl) of S: with the colour
+// map (whole colour-grouped slab, one column), 8 x the busiest XCD's blocks
+template <typename TV>
+static dim3 slab_grid(const DSell<TV> &S, int slice0, int nsl, int nrhs) {
+  if (S.ncb > 0 && nrhs <= 1 && slice0 == 0 && nsl == S.nslices) {
+    int most = 0;
+    for (int x = 0; x < 8; ++x) {
+      int tot = 0;
+      for (int k = 0; k < S.ncb; ++k) {
+        const int nk = S.cb[k + 1] - S.cb[k];
+        tot += ((x + 1) * nk >> 3) - (x * nk >> 3);
+      }
+      most = std::max(most, tot);
+    }
+    return dim3(8 * most, 1);
+  }
+  return grid2(nblk_slices(nsl), nrhs);
+}
 
 // widest row of a launch -> chunk MW (one round of loads up to 28 entries)
 #define W_DISPATCH(wmax, ...)     \
@@ -1286,7 +1298,7 @@ template <typename TV, typename TX, typename TY>
 void launch_spmv(hipStream_t st, const DSell<TV> &A, int slice0, int nsl, const TX *x, TY *y,
                  TY *c, int mode, bool nt, Batch bt) {
   if (nsl <= 0) return;
-  const dim3 g = grid2(nblk_slices(nsl), bt.nrhs), blk(kBlock);
+  const dim3 g = slab_grid(A, slice0, nsl, bt.nrhs), blk(kBlock);
   if (bt.b == 3) {
     NT_DISPATCH(nt, switch (mode) {
       case 0: k_spmv_b3<TV, TX, TY, 0, NTV><<<g, blk, 0, st>>>(A, slice0, nsl, x, y, c, bt.vs); break;
@@ -1313,7 +1325,7 @@ template <typename TV, typename T>
 void launch_spmv_c0(hipStream_t st, const DSell<TV> &A, const T *x, T *y, const T *diag, T *u,
                     int ncs0, bool nt, Batch bt) {
   if (A.nslices <= 0) return;
-  const dim3 g = grid2(nblk_slices(A.nslices), bt.nrhs);
+  const dim3 g = slab_grid(A, 0, A.nslices, bt.nrhs);
   W_DISPATCH(A.wmax, NT_DISPATCH(nt, k_spmv_c0<MWV, TV, T, NTV><<<g, kBlock, 0, st>>>(
                                          A, A.nslices, x, y, diag, u, ncs0, bt.vs)));
 }
@@ -1325,7 +1337,7 @@ template void launch_spmv_c0<float, float>(hipStream_t, const DSell<float> &, co
 void launch_ir_update(hipStream_t st, const DSell<double> &A, const float *c32, double *r,
                       double *x, float *r32, const uint8_t *vclass, double *part, int *nparts,
                       bool nt, Batch bt) {
-  const int g = nblk_slices(A.nslices);
+  const int g = nblk_slices(A.nslices);   // partial sums: one per slice block
   *nparts = g;
   if (g == 0) return;
   if (bt.b == 3) {
@@ -1333,7 +1345,7 @@ void launch_ir_update(hipStream_t st, const DSell<double> &A, const float *c32, 
                         A, A.nslices, c32, r, x, r32, vclass, part, bt.vs));
     return;
   }
-  W_DISPATCH(A.wmax, NT_DISPATCH(nt, k_ir_update<MWV, NTV><<<grid2(g, bt.nrhs), kBlock, 0, st>>>(
+  W_DISPATCH(A.wmax, NT_DISPATCH(nt, k_ir_update<MWV, NTV><<<slab_grid(A, 0, A.nslices, bt.nrhs), kBlock, 0, st>>>(
                                          A, A.nslices, c32, r, x, r32, vclass, part, bt.vs)));
 }
 
@@ -1487,27 +1499,9 @@ template void launch_gs_color_fused<float, float>(hipStream_t, const DSell<float
 
 template <typename TV, typename T>
 void launch_gs_defect(hipStream_t st, const DSell<TV> &U, const T *v, T *b, const uint8_t *vclass,
-                      T damp, T *c, int cmode, T *xout, bool nt, Batch bt, const int *color_slices,
-                      int ncolors) {
+                      T damp, T *c, int cmode, T *xout, bool nt, Batch bt) {
   if (U.nslices <= 0) return;
-  dim3 g = grid2(nblk_slices(U.nslices), bt.nrhs);
-  ColourBlocks cbm{};
-  if (color_slices && ncolors > 0 && ncolors <= kMaxColors && bt.nrhs <= 1 && bt.b == 1) {
-    const int nb = (int)g.x, per = kBlock / kSlice;
-    cbm.n = ncolors;
-    for (int k = 0; k < ncolors; ++k) cbm.cb[k] = std::min(color_slices[k] / per, nb);
-    cbm.cb[ncolors] = nb;
-    int most = 0;   // the busiest XCD's blocks
-    for (int x = 0; x < 8; ++x) {
-      int tot = 0;
-      for (int k = 0; k < ncolors; ++k) {
-        const int nk = cbm.cb[k + 1] - cbm.cb[k];
-        tot += ((x + 1) * nk >> 3) - (x * nk >> 3);
-      }
-      most = std::max(most, tot);
-    }
-    g.x = 8 * most;
-  }
+  const dim3 g = slab_grid(U, 0, U.nslices, bt.nrhs);
   if (bt.b == 3) {
     NT_DISPATCH(nt, if (cmode == 0) k_gs_defect_b3<TV, T, 0, NTV><<<g, kBlock, 0, st>>>(
                         U, U.nslices, v, b, vclass, damp, c, xout, bt.vs);
@@ -1519,17 +1513,16 @@ void launch_gs_defect(hipStream_t st, const DSell<TV> &U, const T *v, T *b, cons
   // off-diagonal row down to 0, and a wider chunk mostly loads clamped
   // repeats on the narrow colours (measured slower for 28 and 16)
   NT_DISPATCH(nt, if (cmode == 0) k_gs_defect<8, TV, T, 0, NTV><<<g, kBlock, 0, st>>>(
-                      U, U.nslices, v, b, vclass, damp, c, xout, bt.vs, cbm);
+                      U, U.nslices, v, b, vclass, damp, c, xout, bt.vs);
               else k_gs_defect<8, TV, T, 1, NTV><<<g, kBlock, 0, st>>>(
-                      U, U.nslices, v, b, vclass, damp, c, xout, bt.vs, cbm));
+                      U, U.nslices, v, b, vclass, damp, c, xout, bt.vs));
 }
 template void launch_gs_defect<double, double>(hipStream_t, const DSell<double> &,
                                                const double *, double *, const uint8_t *, double,
-                                               double *, int, double *, bool, Batch, const int *,
-                                               int);
+                                               double *, int, double *, bool, Batch);
 template void launch_gs_defect<float, float>(hipStream_t, const DSell<float> &, const float *,
                                              float *, const uint8_t *, float, float *, int,
-                                             float *, bool, Batch, const int *, int);
+                                             float *, bool, Batch);
 
 template <typename T>
 int launch_restrict(hipStream_t st, const DSell<T> &PT, const uint32_t *cskip,
@@ -2091,8 +2084,10 @@ void launch_bcgs_spmv(hipStream_t st, const DSell<double> &A, const TQ *q, doubl
                       const double *z, const uint8_t *vclass, double *part, int yy, int skip_half, const BcgsState *s, bool nt, Batch bt) {
   if (A.nslices <= 0) return;
   const dim3 g = grid2(nblk_slices(A.nslices), bt.nrhs);
+  DSell<double> Ap = A;
+  Ap.ncb = 0;   // its partial sums are strided by the grid: the plain map
   W_DISPATCH(A.wmax, NT_DISPATCH(nt, k_bcgs_spmv<MWV, TQ, NTV><<<g, kBlock, 0, st>>>(
-                                         A, A.nslices, q, y, z, vclass, part, yy, skip_half, s,
+                                         Ap, A.nslices, q, y, z, vclass, part, yy, skip_half, s,
                                          bt.vs)));
 }
 template void launch_bcgs_spmv<double>(hipStream_t, const DSell<double> &, const double *,

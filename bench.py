@@ -429,6 +429,30 @@ def elasticity_local_run(nuni, depth, steps, warmup, inner, cpu):
     return out
 
 
+def hbm_probe(torch, reps=20):
+    """Achievable HBM bandwidth on this GPU: a device copy of 1 GiB (reads
+    and writes counted) and a sum over 1 GiB (reads), best of `reps`, timed
+    with events; reported beside the 8 TB/s peak the rooflines divide by."""
+    m = 1 << 27
+    src = torch.ones(m, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    out = {}
+    for name, fn, by in (("copy", lambda: dst.copy_(src), 16.0 * m), ("read", lambda: src.sum(), 8.0 * m)):
+        best = None
+        for _ in range(reps):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            fn()
+            b.record()
+            b.synchronize()
+            t = a.elapsed_time(b) * 1e-3
+            best = t if best is None else min(best, t)
+        out[name + "_gbs"] = round(by / best / 1e9, 1)
+    del src, dst
+    torch.cuda.empty_cache()
+    return out
+
+
 def roofline_entry(kernel, ms, by, traffic=None, **extra):
     ach = by / (ms * 1e-3) / 1e9
     e = {"kernel": kernel, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -696,6 +720,7 @@ def main():
         "roofline_kernels": kernels,
         "solve_roofline": solve_roof,
         "roofline_fracs_le_1": bool(all(f <= 1.0 for f in fracs)),
+        "hbm_probe": hbm_probe(torch),
         "setup_s": {"generate": round(gen_s, 2), "preprocess": round(prep_s, 2), "device_assembly": asm},
     }
     ctx64.close()

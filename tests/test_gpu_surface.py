@@ -99,6 +99,23 @@ def test_uniform_hierarchy_equals_the_plain_solve(smoother, damp):
     assert np.array_equal(bs[Q.top], b)
 
 
+def test_negative_baselevel_searches_from_fullrefinelevel():
+    """LmgcInit (np/procs/iter.cc:7643-7648): $b -N takes, from FULLREFINELEVEL
+    down, the first level with at most N vectors (level 0 when none has);
+    the locally refined levels above FULLREFINELEVEL are never chosen."""
+    Q = problems.p1_local(4, 2)
+    frl = problems.fullrefinelevel(Q)
+    sizes = [lv.n for lv in Q.levels]
+    assert frl < Q.top and sizes[frl - 1] < sizes[frl]
+    for N, want in ((10 ** 9, frl), (sizes[frl - 1], frl - 1), (sizes[1], 1), (1, 0)):
+        ctx = ugmg.Context(0)
+        ctx.set_hierarchy(Q)
+        ctx.set_params(baselevel=-N, smoother=ugmg.SMOOTH_JAC, damp=0.8, base_abslimit=1e-300)
+        ctx.preprocess()
+        assert ctx.base_level() == want, (N, ctx.base_level(), want)
+        ctx.close()
+
+
 def test_refusals():
     Q = problems.p1_local(3, 2)
     frl = problems.fullrefinelevel(Q)

@@ -125,6 +125,23 @@ int ugmg_set_params(ugmg_ctx *ctx, const ugmg_params *p);
 /* LmgcPreProcess (iter.cc:7707-7739): device layouts of levels bl..level,
  * colouring (GS), coarse band LU at the base level (EXPreProcess). */
 int ugmg_preprocess(ugmg_ctx *ctx, int level);
+/* Where ugmg_preprocess builds the level layouts (colouring, GS device
+ * order, permuted levels, SELL slabs of A/L/U/P/P^T) -- the part of
+ * LmgcPreProcess's flatten (np/udm/disctools.cc:2061-2128) that runs over
+ * every entry: UGMG_SETUP_DEVICE (default) on the GPU from the level's CSR,
+ * UGMG_SETUP_HOST on the host, then uploaded.  Both give the same layout,
+ * value for value (ugmg_layout_digest).  The base level's band LU is
+ * ordered on the host either way.  Environment UGMG_SETUP=host|device
+ * overrides the choice. */
+enum { UGMG_SETUP_HOST = 0, UGMG_SETUP_DEVICE = 1 };
+int ugmg_set_setup(ugmg_ctx *ctx, int where);
+/* 64-bit FNV-1a digests of every array of a prepared level's device layout
+ * (slabs A, L, U, P, P^T: offsets, slot rows, columns, uniform-slice
+ * offsets, offset lists, FP64 and FP32 values; diagonal blocks FP64/FP32,
+ * classes, VECSKIP, colour-0 flags, device order) and one of its host-side
+ * description (colours, colour slices, widths, byte model); an absent array
+ * digests to 0.  Returns the number of digests (out may hold nout). */
+int ugmg_layout_digest(ugmg_ctx *ctx, int level, unsigned long long *out, int nout);
 int ugmg_postprocess(ugmg_ctx *ctx);         /* LmgcPostProcess            */
 int ugmg_level_size(const ugmg_ctx *ctx, int level, int *n, int *nnzb);
 /* components per vector of the hierarchy (ugmg_set_nlevels' b), or < 0 */
@@ -308,6 +325,12 @@ int ugmg_get_level(ugmg_ctx *ctx, int level, int *n, int *nnzb, int *row_ptr,
  *   LU solves of the unit vectors: inv (n*n, row-major, list order), order
  *   (new list order), bandwidth.  UGMG_E_SINGULAR on a zero pivot. */
 int ugmg_color(int n, const int *row_ptr, const int *col_idx, int *colors);
+/* ugmg_color on the device of ctx (host arrays in and out): the same
+ * colouring, in rounds over the vectors whose lower neighbours are all
+ * coloured; *rounds (may be NULL) receives the number of rounds.  Returns
+ * the colour count or < 0. */
+int ugmg_color_device(ugmg_ctx *ctx, int n, const int *row_ptr, const int *col_idx,
+                      int *colors, int *rounds);
 int ugmg_coarse_inverse(int n, const int *row_ptr, const int *col_idx,
                         const double *vals, int reorder, double *inv,
                         int *order, int *bandwidth);

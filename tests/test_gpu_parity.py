@@ -517,6 +517,36 @@ def test_server_batches_same_shape_solves_bit_identically():
     srv.close()
 
 
+def test_server_single_slot_groups_reuse_the_layout():
+    """max_slots = 1: each group holds one member, which later requests of
+    the same shape replace through the group's index maps (the group's
+    layout is built once); results equal stand-alone solves bit for bit."""
+    from ug_amd import problems, ugmg
+    probs = [problems.jump3d(4, seed=31), problems.jump3d(4, seed=32), problems.jump3d(4, seed=33)]
+    kw = dict(nu1=2, nu2=2, gamma=1, baselevel=1, smoother=ugmg.SMOOTH_GS, damp=1.0, base_abslimit=1e-300,
+              use_graph=1)
+    alone = []
+    for P in probs:
+        ctx = ugmg.Context(0)
+        ctx.set_hierarchy(P)
+        ctx.set_params(**kw)
+        ctx.preprocess()
+        x, b, res, d = ctx.solve_host(P.top, P.rhs, red=1e-8, maxiter=200)
+        assert res.converged
+        alone.append((x, res.number_of_linear_iterations))
+        ctx.close()
+    srv = ugmg.Server(0, budget_bytes=8e9, max_slots=1)
+    ids = [srv.submit(P, ugmg.default_params(**kw), red=1e-8, maxiter=200) for P in probs]
+    srv.drain(len(ids))
+    groups, launches, columns = srv.stats()
+    assert groups == 1 and columns == launches == sum(nit for _, nit in alone)
+    for P, rid, (xa, nit) in zip(probs, ids, alone):
+        st, res, _ = srv.status(rid)
+        assert st == ugmg.REQ_DONE and res.converged and res.number_of_linear_iterations == nit
+        assert np.array_equal(srv.result(rid, P.levels[-1].n), xa)
+    srv.close()
+
+
 def test_server_budget_rejects_oversized_request():
     from ug_amd import problems, ugmg
     P = problems.jump3d(4, seed=5)

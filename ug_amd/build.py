@@ -41,7 +41,8 @@ def build_oracle():
 def build_ugmg():
     os.makedirs(BUILD, exist_ok=True)
     objs = []
-    for src in ("ugmg_kernels.hip", "ugmg_base.hip", "ugmg_assemble.hip", "ugmg_api.hip"):
+    for src in ("ugmg_kernels.hip", "ugmg_base.hip", "ugmg_assemble.hip", "ugmg_devsetup.hip",
+                "ugmg_api.hip"):
         obj = os.path.join(BUILD, src.replace(".hip", ".o"))
         _run([HIPCC, *HIP_FLAGS, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)

@@ -222,6 +222,12 @@ struct ugmg_ctx {
   // level vectors (batched launches read column j's values, diagonal
   // blocks and base inverse); set through member_set
   int members = 1;
+  // a serving group's context: every level carries member planes and the
+  // index maps that fill them (also with one member)
+  bool grouped = false;
+  // level layouts built on the device (ugmg_set_setup; default) or on the
+  // host
+  bool setup_device = true;
   size_t base_ims = 0;          // values of one member's compact base inverse
   int *base_act = nullptr;      // rows of the inverse kept in the compact copy
   double *stage = nullptr;      // staging of one level's member values
@@ -298,36 +304,42 @@ int upload(ugmg_ctx *ctx, T **out, const T *src, size_t n) {
   return 0;
 }
 
-int upload_sell(ugmg_ctx *ctx, const HostSell &S, DevSellBuf &D, int members = 1) {
-  D.nslices = S.nslices;
-  D.nent = S.col.size();
-  D.width_h.resize(S.nslices);
+// host-side slice metadata of a slab from its entry offsets and offset-list
+// starts (per slice)
+void sell_meta(DevSellBuf &D, int nslices, const int *soff, const int *sdoff) {
+  D.nslices = nslices;
+  D.width_h.resize(nslices);
   D.wmax = 0;
-  for (int q = 0; q < S.nslices; ++q) {
-    D.width_h[q] = (S.soff[q + 1] - S.soff[q]) / kSlice;
+  D.nuniform = 0;
+  D.nent_explicit = 0;
+  D.slot_pre.assign(nslices + 1, 0.0);
+  D.expl_pre.assign(nslices + 1, 0.0);
+  for (int q = 0; q < nslices; ++q) {
+    D.width_h[q] = (soff[q + 1] - soff[q]) / kSlice;
     D.wmax = std::max(D.wmax, D.width_h[q]);
+    const size_t slots = (size_t)(soff[q + 1] - soff[q]);
+    D.nuniform += sdoff[q] >= 0;
+    if (sdoff[q] < 0) D.nent_explicit += slots;
+    D.slot_pre[q + 1] = D.slot_pre[q] + (double)slots;
+    D.expl_pre[q + 1] = D.expl_pre[q] + (sdoff[q] < 0 ? (double)slots : 0.0);
   }
+}
+
+// members > 0: `members` value planes filled later through the index maps
+// (the slab values are those maps), else the values themselves
+int upload_sell(ugmg_ctx *ctx, const HostSell &S, DevSellBuf &D, int members = 0) {
+  sell_meta(D, S.nslices, S.soff.data(), S.sdoff.data());
+  D.nent = S.col.size();
   int rc;
   if ((rc = upload(ctx, &D.soff, S.soff.data(), S.soff.size()))) return rc;
   if ((rc = upload(ctx, &D.rows, S.rows.data(), S.rows.size()))) return rc;
   if ((rc = upload(ctx, &D.col, S.col.data(), S.col.size()))) return rc;
   if ((rc = upload(ctx, &D.sdoff, S.sdoff.data(), S.sdoff.size()))) return rc;
   if ((rc = upload(ctx, &D.dtab, S.dtab.data(), S.dtab.size()))) return rc;
-  D.nuniform = 0;
-  D.nent_explicit = 0;
-  D.slot_pre.assign(S.nslices + 1, 0.0);
-  D.expl_pre.assign(S.nslices + 1, 0.0);
-  for (int q = 0; q < S.nslices; ++q) {
-    const size_t slots = (size_t)(S.soff[q + 1] - S.soff[q]);
-    D.nuniform += S.sdoff[q] >= 0;
-    if (S.sdoff[q] < 0) D.nent_explicit += slots;
-    D.slot_pre[q + 1] = D.slot_pre[q] + (double)slots;
-    D.expl_pre[q + 1] = D.expl_pre[q] + (S.sdoff[q] < 0 ? (double)slots : 0.0);
-  }
   D.ndtab = S.dtab.size();
   D.bb = S.bb;
   D.nval = S.val.size();
-  if (members > 1) {   // the values are the index maps; member_set fills the planes
+  if (members > 0) {   // the values are the index maps; member_set fills the planes
     if ((rc = dalloc(ctx, &D.v64, D.nval * members)) || (rc = dalloc(ctx, &D.v32, D.nval * members)))
       return rc;
     HIPCHK(hipMemset(D.v64, 0, sizeof(double) * D.nval * members));
@@ -339,6 +351,28 @@ int upload_sell(ugmg_ctx *ctx, const HostSell &S, DevSellBuf &D, int members = 1
   hvec<float> v32;
   to_float(S.val.data(), S.val.size(), v32);
   if ((rc = upload(ctx, &D.v32, v32.data(), v32.size()))) return rc;
+  D.mst = 0;
+  return 0;
+}
+
+// a slab laid out on the device (build_level_device): its arrays are the
+// context's already; the FP32 copy of the values is converted here
+int adopt_sell(ugmg_ctx *ctx, const DevSellParts &S, DevSellBuf &D) {
+  sell_meta(D, S.nslices, S.soff_h.data(), S.sdoff_h.data());
+  D.nent = S.nent;
+  D.soff = S.soff;
+  D.rows = S.rows;
+  D.col = S.col;
+  D.sdoff = S.sdoff;
+  D.dtab = S.dtab;
+  D.ndtab = S.ndtab;
+  D.bb = S.bb;
+  D.nval = S.nent * S.bb;
+  D.v64 = S.val;
+  int rc;
+  if ((rc = dalloc(ctx, &D.v32, D.nval))) return rc;
+  if (D.nval) launch_convert<float, double>(ctx->stream, (int)D.nval, D.v64, D.v32, 1, 0, 0);
+  HIPCHK(hipGetLastError());
   D.mst = 0;
   return 0;
 }
@@ -1000,6 +1034,11 @@ static int resolve_baselevel(const ugmg_ctx *ctx, int level) {
 struct PrepState {
   int level = -1, bl = 0, fuse_rows = 0;
   double nt_bytes = 64.0e6;
+  // layouts built on the device (build_level_device) in prep_device, or on
+  // the host here (hp, plv)
+  bool device = false;
+  int force_win = -1;
+  std::vector<DevPrep> dp;
   std::vector<HostPrep> hp;
   std::vector<HostLevel> plv;   // levels in device order (GS levels)
   HostBase base;
@@ -1049,6 +1088,15 @@ int prep_host(ugmg_ctx *ctx, int level, PrepState &ps) {
   // 1: list order)
   const char *env_win = std::getenv("UGMG_GS_WINDOW");
   const int force_win = env_win ? std::atoi(env_win) : -1;
+  ps.force_win = force_win;
+  // where the level layouts are built: UGMG_SETUP=host|device overrides the
+  // context's choice; a context with several members keeps the host build
+  // (its layouts carry index maps)
+  ps.device = ctx->setup_device && !ctx->grouped;
+  if (const char *env_setup = std::getenv("UGMG_SETUP")) {
+    if (!std::strcmp(env_setup, "host")) ps.device = false;
+    else if (!std::strcmp(env_setup, "device")) ps.device = !ctx->grouped;
+  }
   try {
     ps.hp.assign(level + 1, HostPrep());
     std::vector<HostLevel> &plv = ps.plv;          // levels in device order
@@ -1059,7 +1107,7 @@ int prep_host(ugmg_ctx *ctx, int level, PrepState &ps) {
     // slab value slot then holds its source index, and each member's values
     // are gathered through those maps (member_set)
     std::vector<HostLevel> idx;
-    if (ctx->members > 1) {
+    if (ctx->grouped) {
       idx.resize(level + 1);
       for (int l = bl; l <= level; ++l) {
         idx[l] = ctx->host[l];
@@ -1067,7 +1115,7 @@ int prep_host(ugmg_ctx *ctx, int level, PrepState &ps) {
       }
     }
     auto src = [&](int l) -> const HostLevel & { return idx.empty() ? ctx->host[l] : idx[l]; };
-    for (int l = bl; l <= level; ++l) {
+    for (int l = bl; l <= level && !ps.device; ++l) {
       const bool gs = p.smoother == UGMG_SMOOTH_GS && l > bl;
       HostPrep &hp = ps.hp[l];
       const bool cperm = l > bl && !pos[l - 1].empty();
@@ -1090,7 +1138,7 @@ int prep_host(ugmg_ctx *ctx, int level, PrepState &ps) {
       build_prep(plv[l], coarse, &colp, nc, hp);
       hp.color_ug = col;
     }
-    if (ctx->members > 1) {
+    if (ctx->grouped) {
       ps.map_a.assign(level + 1, {});
       ps.map_l.assign(level + 1, {});
       ps.map_u.assign(level + 1, {});
@@ -1158,7 +1206,82 @@ int factor_base_device(ugmg_ctx *ctx, HostBase &B) {
   return 0;
 }
 
-// Device half: uploads the layouts built by prep_host.
+// the layouts of levels bl..level built on the device into ps.dp
+int build_layouts_device(ugmg_ctx *ctx, PrepState &ps) {
+  const int level = ps.level, bl = ps.bl;
+  ps.dp.assign(level + 1, DevPrep());
+  DevKeep keep;
+  keep.user = ctx;
+  keep.alloc = [](void *u, void **p, size_t bytes) {
+    return dalloc((ugmg_ctx *)u, (char **)p, bytes);
+  };
+  int rc = 0;
+  for (int l = bl; l <= level && !rc; ++l) {
+    const bool gs = ctx->p.smoother == UGMG_SMOOTH_GS && l > bl;
+    const int coarse_n = l > bl ? ctx->host[l - 1].n : 0;
+    const int *coarse_pos = l > bl ? ps.dp[l - 1].pos : nullptr;
+    std::string err;
+    rc = build_level_device(ctx->stream, ctx->host[l], coarse_n, coarse_pos, gs, ps.force_win, keep,
+                            ps.dp[l], err);
+    if (rc) rc = fail(ctx, rc, "level " + std::to_string(l) + " layout: " + err);
+    if (l > bl && ps.dp[l - 1].pos) {
+      (void)hipFree(ps.dp[l - 1].pos);
+      ps.dp[l - 1].pos = nullptr;
+    }
+  }
+  for (DevPrep &d : ps.dp)
+    if (d.pos) {
+      (void)hipFree(d.pos);
+      d.pos = nullptr;
+    }
+  return rc;
+}
+
+// DevLevel fields of a level laid out on the device
+int adopt_level(ugmg_ctx *ctx, DevPrep &dp, bool has_p, DevLevel &D) {
+  int rc;
+  D.n = dp.n;
+  D.nb = ctx->b;
+  D.nnzb = dp.nnzb;
+  if (dp.perm) {
+    D.perm_h = std::move(dp.perm_h);
+    D.perm = dp.perm;
+  }
+  if ((rc = adopt_sell(ctx, dp.A, D.A))) return rc;
+  const size_t nd = (size_t)dp.n * D.nb * D.nb;
+  D.diag64 = dp.diag;
+  if ((rc = dalloc(ctx, &D.diag32, nd))) return rc;
+  launch_convert<float, double>(ctx->stream, (int)nd, D.diag64, D.diag32, 1, 0, 0);
+  HIPCHK(hipGetLastError());
+  D.dms = 0;
+  D.vclass = dp.vc;
+  D.vnclass = dp.vnc;
+  D.skip = dp.skip;
+  if (dp.ncolors > 0) {
+    D.ncolors = dp.ncolors;
+    D.color_slices = dp.color_slices;
+    D.color = std::move(dp.color_ug_h);
+    if ((rc = adopt_sell(ctx, dp.L, D.L)) || (rc = adopt_sell(ctx, dp.U, D.U))) return rc;
+    D.c0 = dp.c0;
+    D.color_rows = dp.color_rows;
+    D.all_active = dp.all_active;
+    D.color_wmax.resize(D.ncolors);
+    D.color_wmax_u.resize(D.ncolors);
+    for (int k = 0; k < D.ncolors; ++k) {
+      D.color_wmax[k] = D.L.max_width(D.color_slices[k], D.color_slices[k + 1]);
+      D.color_wmax_u[k] = D.U.max_width(D.color_slices[k], D.color_slices[k + 1]);
+    }
+  }
+  if (has_p) {
+    D.p_nnz = dp.p_nnz;
+    D.pt_nnz = dp.pt_nnz;
+    if ((rc = adopt_sell(ctx, dp.P, D.P)) || (rc = adopt_sell(ctx, dp.PT, D.PT))) return rc;
+  }
+  return 0;
+}
+
+// Device half: builds the level layouts on the device (ps.device) or
+// uploads those prep_host built, then the base inverse and the work vectors.
 int prep_device(ugmg_ctx *ctx, PrepState &ps) {
   free_device(ctx);
   const int level = ps.level, bl = ps.bl;
@@ -1167,11 +1290,19 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
     ctx->bl = bl;
     int rc;
     int max_slices = 0;
+    if (ps.device && (rc = build_layouts_device(ctx, ps))) {
+      free_device(ctx);
+      return rc;
+    }
     for (int l = bl; l <= level; ++l) {
+      DevLevel &D = ctx->dev[l];
+      if (ps.device) {
+        if ((rc = adopt_level(ctx, ps.dp[l], l > bl, D))) return rc;
+        max_slices = std::max(max_slices, D.A.nslices);
+      } else {
       HostPrep &hp = ps.hp[l];
       const bool permuted = !hp.perm.empty();
       const HostLevel &H = permuted ? ps.plv[l] : ctx->host[l];
-      DevLevel &D = ctx->dev[l];
       if (permuted) {
         D.perm_h = hp.perm;
         if ((rc = upload(ctx, &D.perm, hp.perm.data(), hp.perm.size()))) return rc;
@@ -1180,9 +1311,10 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
       D.nb = ctx->b;
       D.nnzb = H.nnzb;
       max_slices = std::max(max_slices, hp.A.nslices);
-      const int mb = ctx->members;
+      const bool grp = ctx->grouped;
+      const int mb = grp ? ctx->members : 0;
       if ((rc = upload_sell(ctx, hp.A, D.A, mb))) return rc;
-      if (mb > 1) {   // diagonal blocks 1 until member_set fills a member's
+      if (grp) {   // diagonal blocks 1 until member_set fills a member's
         if ((rc = dalloc(ctx, &D.diag64, hp.diag.size() * mb)) ||
             (rc = dalloc(ctx, &D.diag32, hp.diag.size() * mb)))
           return rc;
@@ -1195,8 +1327,8 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
         to_float(hp.diag.data(), hp.diag.size(), d32);
         if ((rc = upload(ctx, &D.diag32, d32.data(), d32.size()))) return rc;
       }
-      D.dms = mb > 1 ? hp.diag.size() : 0;
-      if (mb > 1 &&
+      D.dms = grp ? hp.diag.size() : 0;
+      if (grp &&
           ((rc = upload(ctx, &D.map_a, ps.map_a[l].data(), ps.map_a[l].size())) ||
            (rc = upload(ctx, &D.map_d, ps.map_d[l].data(), ps.map_d[l].size()))))
         return rc;
@@ -1209,7 +1341,7 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
         D.color = hp.color_ug;
         if ((rc = upload_sell(ctx, hp.L, D.L, mb))) return rc;
         if ((rc = upload_sell(ctx, hp.U, D.U, mb))) return rc;
-        if (mb > 1 &&
+        if (grp &&
             ((rc = upload(ctx, &D.map_l, ps.map_l[l].data(), ps.map_l[l].size())) ||
              (rc = upload(ctx, &D.map_u, ps.map_u[l].data(), ps.map_u[l].size()))))
           return rc;
@@ -1227,26 +1359,27 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
           D.color_wmax_u[k] = D.U.max_width(D.color_slices[k], D.color_slices[k + 1]);
         }
       }
-      D.A.dms = D.L.dms = D.U.dms = D.dms;
-      // scalar GS levels: the colour map of the whole-slab launches over A and U
-      if (hp.ncolors > 0 && hp.ncolors <= kMaxColors && D.nb == 1 && D.A.nslices == D.U.nslices) {
-        const int per = kBlock / kSlice, nblk = (D.A.nslices + per - 1) / per;
-        for (DevSellBuf *S : {&D.A, &D.U}) {
-          S->ncb = hp.ncolors;
-          for (int k = 0; k < hp.ncolors; ++k) S->cb[k] = std::min(hp.color_slices[k] / per, nblk);
-          S->cb[hp.ncolors] = nblk;
-        }
-      }
       if (l > bl) {
         D.p_nnz = H.pp[H.n];
         D.pt_nnz = (int)hp.pt_idx.size();
         if ((rc = upload_sell(ctx, hp.P, D.P))) return rc;
         if ((rc = upload_sell(ctx, hp.PT, D.PT))) return rc;
       }
+      }
+      D.A.dms = D.L.dms = D.U.dms = D.dms;
+      // scalar GS levels: the colour map of the whole-slab launches over A and U
+      if (D.ncolors > 0 && D.ncolors <= kMaxColors && D.nb == 1 && D.A.nslices == D.U.nslices) {
+        const int per = kBlock / kSlice, nblk = (D.A.nslices + per - 1) / per;
+        for (DevSellBuf *S : {&D.A, &D.U}) {
+          S->ncb = D.ncolors;
+          for (int k = 0; k < D.ncolors; ++k) S->cb[k] = std::min(D.color_slices[k] / per, nblk);
+          S->cb[D.ncolors] = nblk;
+        }
+      }
       // matrix bytes: b*b values (FP64, or FP32 for the FP32 copy) and one
       // column index per entry
-      D.nt = (double)hp.A.col.size() * (8.0 * D.nb * D.nb + 4.0) > ps.nt_bytes;
-      D.nt32 = (double)hp.A.col.size() * (4.0 * D.nb * D.nb + 4.0) > ps.nt_bytes;   // the FP32 copy
+      D.nt = (double)D.A.nent * (8.0 * D.nb * D.nb + 4.0) > ps.nt_bytes;
+      D.nt32 = (double)D.A.nent * (4.0 * D.nb * D.nb + 4.0) > ps.nt_bytes;   // the FP32 copy
       // columns padded to 64 entries (512-byte aligned FP64 columns)
       D.vs = ((size_t)std::max(D.n, 1) * D.nb + 63) / 64 * 64;
       const size_t nv = D.vs * (size_t)ctx->nrhs_cap;
@@ -1316,8 +1449,8 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
         for (int k = 0; k < mb; ++k)
           launch_base_compact(ctx->stream, nact, nsc, ctx->base_act, ctx->base_colperm, ctx->base_inv,
                               ctx->base_cinv + k * ne, ctx->base_cinv32 + k * ne);
-        ctx->base_ims = mb > 1 ? ne : 0;
-        if (mb > 1 && (rc = dalloc(ctx, &ctx->base_stage, (size_t)nsc * nsc))) return rc;
+        ctx->base_ims = ctx->grouped ? ne : 0;
+        if (ctx->grouped && (rc = dalloc(ctx, &ctx->base_stage, (size_t)nsc * nsc))) return rc;
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(ctx->stream));
       }
@@ -1325,12 +1458,12 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
     // fused coarse levels (opt-in): the base level and every level above it
     // up to fuse_rows vectors (and at most kMaxColors colours)
     ctx->fuse_top = -1;
-    if (ctx->members > 1) {
+    if (ctx->grouped) {
       size_t mx = 1;
       for (int l = bl; l <= level; ++l) mx = std::max(mx, ctx->host[l].a.size());
       if ((rc = dalloc(ctx, &ctx->stage, mx))) return rc;
     }
-    if (ps.fuse_rows > 0 && ctx->dev[bl].n <= 2048 && ctx->b == 1 && ctx->members == 1) {
+    if (ps.fuse_rows > 0 && ctx->dev[bl].n <= 2048 && ctx->b == 1 && !ctx->grouped) {
       ctx->fuse_top = bl;
       for (int l = bl + 1; l <= level; ++l) {
         if (ctx->dev[l].n > ps.fuse_rows || ctx->dev[l].ncolors > kMaxColors) break;
@@ -2668,6 +2801,124 @@ int ugmg_color(int n, const int *row_ptr, const int *col_idx, int *colors) {
   }
 }
 
+int ugmg_color_device(ugmg_ctx *ctx, int n, const int *row_ptr, const int *col_idx, int *colors,
+                      int *rounds) {
+  if (!ctx || n < 0 || (n > 0 && (!row_ptr || !col_idx || !colors))) return UGMG_E_ARG;
+  if (n == 0) return 0;
+  const int nnz = row_ptr[n];
+  if (row_ptr[0] != 0 || nnz < 0) return fail(ctx, UGMG_E_ARG, "row_ptr");
+  for (int i = 0; i < n; ++i)
+    if (row_ptr[i + 1] < row_ptr[i]) return fail(ctx, UGMG_E_ARG, "row_ptr not ascending");
+  for (int m = 0; m < nnz; ++m)
+    if (col_idx[m] < 0 || col_idx[m] >= n) return fail(ctx, UGMG_E_ARG, "column index out of range");
+  HIPCHK(hipSetDevice(ctx->device));
+  int *rp = nullptr, *ci = nullptr, *col = nullptr;
+  auto release = [&] {
+    (void)hipFree(rp);
+    (void)hipFree(ci);
+    (void)hipFree(col);
+  };
+  hipError_t e = hipMalloc((void **)&rp, sizeof(int) * ((size_t)n + 1));
+  if (e == hipSuccess) e = hipMalloc((void **)&ci, sizeof(int) * std::max<size_t>(nnz, 1));
+  if (e == hipSuccess) e = hipMalloc((void **)&col, sizeof(int) * (size_t)n);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(rp, row_ptr, sizeof(int) * ((size_t)n + 1), hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess && nnz)
+    e = hipMemcpyAsync(ci, col_idx, sizeof(int) * (size_t)nnz, hipMemcpyHostToDevice, ctx->stream);
+  if (e != hipSuccess) {
+    release();
+    return fail(ctx, UGMG_E_HIP, std::string("colouring upload: ") + hipGetErrorString(e));
+  }
+  std::string err;
+  int sw = 0;
+  const int nc = color_device(ctx->stream, n, rp, ci, col, &sw, err);
+  if (nc < 0) {
+    release();
+    return fail(ctx, nc, err);
+  }
+  e = hipMemcpyAsync(colors, col, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  release();
+  if (e != hipSuccess) return fail(ctx, UGMG_E_HIP, std::string("colouring: ") + hipGetErrorString(e));
+  if (rounds) *rounds = sw;
+  return nc;
+}
+
+int ugmg_set_setup(ugmg_ctx *ctx, int where) {
+  if (!ctx || (where != UGMG_SETUP_HOST && where != UGMG_SETUP_DEVICE)) return UGMG_E_ARG;
+  ctx->setup_device = where == UGMG_SETUP_DEVICE;
+  return UGMG_OK;
+}
+
+int ugmg_layout_digest(ugmg_ctx *ctx, int level, unsigned long long *out, int nout) {
+  int rc = check_ready(ctx, level);
+  if (rc) return rc;
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  const DevLevel &d = ctx->dev[level];
+  std::vector<unsigned long long> dg;
+  auto fnv = [](const void *p, size_t bytes) {
+    const unsigned char *c = (const unsigned char *)p;
+    unsigned long long h = 1469598103934665603ULL;
+    for (size_t q = 0; q < bytes; ++q) h = (h ^ c[q]) * 1099511628211ULL;
+    return h;
+  };
+  std::vector<unsigned char> buf;
+  auto dev = [&](const void *p, size_t bytes) -> int {
+    if (!p) {
+      dg.push_back(0);
+      return 0;
+    }
+    buf.resize(bytes);
+    if (bytes) HIPCHK(hipMemcpy(buf.data(), p, bytes, hipMemcpyDeviceToHost));
+    dg.push_back(fnv(buf.data(), bytes));
+    return 0;
+  };
+  const int n = d.n, bb = d.nb * d.nb;
+  const int mb = std::max(ctx->members, 1);
+  for (const DevSellBuf *S : {&d.A, &d.L, &d.U, &d.P, &d.PT}) {
+    const size_t ns = S->soff ? (size_t)S->nslices : 0;
+    if ((rc = dev(S->soff, S->soff ? 4 * (ns + 1) : 0)) || (rc = dev(S->rows, 4 * ns * kSlice)) ||
+        (rc = dev(S->col, 4 * S->nent)) || (rc = dev(S->sdoff, 4 * ns)) ||
+        (rc = dev(S->dtab, 4 * S->ndtab)) || (rc = dev(S->v64, 8 * S->nval * mb)) ||
+        (rc = dev(S->v32, 4 * S->nval * mb)))
+      return rc;
+  }
+  if ((rc = dev(d.diag64, 8 * (size_t)n * bb * mb)) || (rc = dev(d.diag32, 4 * (size_t)n * bb * mb)) ||
+      (rc = dev(d.vclass, n)) || (rc = dev(d.vnclass, n)) || (rc = dev(d.skip, 4 * (size_t)n)) ||
+      (rc = dev(d.c0, n)) || (rc = dev(d.perm, 4 * (size_t)n)))
+    return rc;
+  // host-side description
+  std::vector<long long> h;
+  auto put = [&](const auto &v) {
+    h.push_back((long long)v.size());
+    for (auto x : v) h.push_back((long long)x);
+  };
+  put(d.color);
+  put(d.color_slices);
+  put(d.color_rows);
+  put(d.color_wmax);
+  put(d.color_wmax_u);
+  put(d.perm_h);
+  for (long long x : {(long long)d.n, (long long)d.nnzb, (long long)d.ncolors, (long long)d.all_active,
+                      (long long)d.p_nnz, (long long)d.pt_nnz, (long long)d.nt, (long long)d.nt32})
+    h.push_back(x);
+  for (const DevSellBuf *S : {&d.A, &d.L, &d.U, &d.P, &d.PT}) {
+    for (long long x : {(long long)S->nslices, (long long)S->nent, (long long)S->nuniform,
+                        (long long)S->nent_explicit, (long long)S->ndtab, (long long)S->bb,
+                        (long long)S->wmax, (long long)S->nval, (long long)S->ncb})
+      h.push_back(x);
+    put(S->width_h);
+    for (int k = 0; k <= S->ncb && k <= kMaxColors; ++k) h.push_back(S->cb[k]);
+    dg.push_back(fnv(S->slot_pre.data(), S->slot_pre.size() * sizeof(double)));
+    dg.push_back(fnv(S->expl_pre.data(), S->expl_pre.size() * sizeof(double)));
+  }
+  dg.push_back(fnv(h.data(), h.size() * sizeof(long long)));
+  if (out)
+    for (int q = 0; q < nout && q < (int)dg.size(); ++q) out[q] = dg[q];
+  return (int)dg.size();
+}
+
 int ugmg_coarse_inverse(int n, const int *row_ptr, const int *col_idx, const double *vals,
                         int reorder, double *inv, int *order, int *bandwidth) {
   if (n < 0 || (n > 0 && (!row_ptr || !col_idx || !vals || !inv))) return UGMG_E_ARG;
@@ -3354,6 +3605,7 @@ int ugmg_server_submit(ugmg_server *s, int nlevels, const ugmg_level_desc *level
     if ((rc = ugmg_set_params(ctx, &q))) return bail(rc);
     ctx->nrhs_cap = g->cap;
     ctx->members = g->cap;
+    ctx->grouped = true;
     // every member's matrices, vectors and base inverse; patterns once
     g->est_bytes = est_device_bytes(ctx, r->top, resolve_baselevel(ctx, r->top)) * g->cap;
     gi = (int)s->groups.size();

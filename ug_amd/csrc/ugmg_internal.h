@@ -85,6 +85,54 @@ struct HostBase {
 // filters for build_sell
 enum class Part { All, Lower, Upper };
 
+// ---- the same products built on the device (ugmg_devsetup.hip) ----------
+// A SELL-64 slab in device memory (the arrays of HostSell; values in bb
+// planes of nent) with host copies of its per-slice offsets.
+struct DevSellParts {
+  int nrows = 0, nslices = 0, bb = 1;
+  size_t nent = 0, ndtab = 0;
+  int *soff = nullptr, *rows = nullptr, *col = nullptr, *sdoff = nullptr, *dtab = nullptr;
+  double *val = nullptr;
+  std::vector<int> soff_h, sdoff_h;
+};
+// One level laid out on the device: what build_prep (after greedy_color,
+// choose_gs_window, gs_device_order and permute_level on a GS level) gives
+// on the host, value for value.
+struct DevPrep {
+  int n = 0, nnzb = 0, window = 0, ncolors = 0;
+  std::vector<int> perm_h;            // GS levels: device position -> list index
+  std::vector<int> color_ug_h;        // GS levels: colouring in list order
+  std::vector<int> color_slices;      // ncolors + 1
+  std::vector<int> color_rows;        // vectors per colour
+  int *perm = nullptr;                // device copy of perm_h
+  int *pos = nullptr;                 // list index -> device position (caller frees)
+  DevSellParts A, L, U, P, PT;
+  double *diag = nullptr;             // n b*b
+  uint8_t *vc = nullptr, *vnc = nullptr, *c0 = nullptr;   // device order
+  uint32_t *skip = nullptr;
+  int p_nnz = 0, pt_nnz = 0;
+  bool all_active = false;
+};
+// Final device arrays are allocated through `keep` (which registers them
+// with their owner); returns 0 or a UGMG error code.
+struct DevKeep {
+  void *user = nullptr;
+  int (*alloc)(void *user, void **p, size_t bytes) = nullptr;
+};
+// Lays out level H on the device: coarse_n vectors on level l-1 (0: none),
+// coarse_pos its list index -> device position map (device; null: list
+// order), gs: a multicolour GS level, force_win >= 0: the device-order
+// window instead of the chosen one.  Returns 0 or a UGMG error code (err
+// says why).
+int build_level_device(void *stream, const HostLevel &H, int coarse_n, const int *coarse_pos,
+                       bool gs, int force_win, const DevKeep &keep, DevPrep &out,
+                       std::string &err);
+// greedy_color on the device (the same colouring): colors (device, n ints)
+// receives it, *rounds the frontier rounds it took; returns the colour count
+// or < 0.
+int color_device(void *stream, int n, const int *rp, const int *ci, int *colors, int *rounds,
+                 std::string &err);
+
 void build_sell(const HostLevel &L, const std::vector<int> *rows, Part part,
                 const std::vector<int> *color, HostSell &out);
 void build_sell_csr(int n, const int *rp, const int *ci, const double *a, HostSell &out);

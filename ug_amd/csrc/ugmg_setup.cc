@@ -96,18 +96,22 @@ static void sell_from_csr(int n, int ncols, const int *rp, const int *ci, const 
   // lists found by hash, confirmed by comparison
   out.dtab.clear();
   {
-    std::unordered_map<uint64_t, std::vector<int>> seen;   // hash -> dtab offsets
+    // hash -> (dtab offset, length) of the lists stored so far
+    std::unordered_map<uint64_t, std::vector<std::pair<int, int>>> seen;
     for (int s = 0; s < ns; ++s) {
       if (sdel[s].empty()) continue;
       const std::vector<int> &U = sdel[s];
       int off = -1;
       auto &cands = seen[shash[s]];
-      for (int o : cands)
-        if (std::equal(U.begin(), U.end(), out.dtab.begin() + o)) { off = o; break; }
+      for (const auto &o : cands)
+        if (o.second == (int)U.size() && std::equal(U.begin(), U.end(), out.dtab.begin() + o.first)) {
+          off = o.first;
+          break;
+        }
       if (off < 0) {
         off = (int)out.dtab.size();
         out.dtab.insert(out.dtab.end(), U.begin(), U.end());
-        cands.push_back(off);
+        cands.emplace_back(off, (int)U.size());
       }
       out.sdoff[s] = off;
     }

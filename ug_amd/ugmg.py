@@ -19,6 +19,7 @@ HEADER = os.path.join(os.path.dirname(_HERE), "include", "ugmg.h")
 
 SMOOTH_JAC, SMOOTH_GS = 0, 1
 FP64, MIXED = 0, 1
+SETUP_HOST, SETUP_DEVICE = 0, 1
 ERRORS = {0: "OK", -1: "E_ARG", -2: "E_HIP", -3: "E_STATE", -4: "E_NOMEM", -5: "E_SINGULAR",
           -6: "E_BUDGET", -7: "E_NODEV"}
 
@@ -122,6 +123,9 @@ def load():
         "ugmg_assemble_level": ([vp, ip, C.POINTER(Elements), dp], ip),
         "ugmg_get_level": ([vp, ip, C.POINTER(ip), C.POINTER(ip), vp, vp, vp, vp], ip),
         "ugmg_color": ([ip, vp, vp, vp], ip),
+        "ugmg_color_device": ([vp, ip, vp, vp, vp, C.POINTER(ip)], ip),
+        "ugmg_set_setup": ([vp, ip], ip),
+        "ugmg_layout_digest": ([vp, ip, vp, ip], ip),
         "ugmg_coarse_inverse": ([ip, vp, vp, vp, ip, vp, vp, C.POINTER(ip)], ip),
     }
     for name, (args, res) in sig.items():
@@ -236,6 +240,31 @@ class Context:
     def set_params(self, **kw):
         self.params = default_params(**kw)
         self._chk(self.lib.ugmg_set_params(self.h, C.byref(self.params)))
+
+    def set_setup(self, where):
+        """Where preprocess builds the level layouts: SETUP_DEVICE or SETUP_HOST."""
+        self._chk(self.lib.ugmg_set_setup(self.h, where))
+
+    def layout_digest(self, level):
+        """FNV-1a digests of every array of a prepared level's device layout."""
+        nd = self.lib.ugmg_layout_digest(self.h, level, None, 0)
+        if nd < 0:
+            self._chk(nd)
+        out = np.zeros(nd, np.uint64)
+        self._chk(min(0, self.lib.ugmg_layout_digest(self.h, level, out.ctypes.data, nd)))
+        return out
+
+    def color_device(self, level):
+        """(colour count, colours, sweeps) of a problems.Level by the device colouring."""
+        rp = np.ascontiguousarray(level.row_ptr, np.int32)
+        ci = np.ascontiguousarray(level.col_idx, np.int32)
+        out = np.zeros(level.n, np.int32)
+        sw = C.c_int()
+        nc = self.lib.ugmg_color_device(self.h, level.n, rp.ctypes.data, ci.ctypes.data, out.ctypes.data,
+                                        C.byref(sw))
+        if nc < 0:
+            self._chk(nc)
+        return nc, out, sw.value
 
     def preprocess(self, level=None):
         level = self.nlevels - 1 if level is None else level

@@ -161,12 +161,17 @@ def serving_run(n_req, rate, seed, budget_gb, slots):
                     "build and device set-up"}
 
 
-def assemble_run(problems, ugmg, torch, nref, top):
-    """Device-side assembly of the finest level from its element list
-    (hexes with kappa_e times the reference Q1 matrix, in device memory):
-    wall time of ugmg_assemble_level including the copy of the flattened
-    level to the host, and whether it equals the host generator's level."""
+def assemble_run(problems, ugmg, torch, nref, P, base):
+    """Device-side set-up of the finest level from its element list (hexes
+    with kappa_e times the reference Q1 matrix, in device memory):
+    ugmg_assemble_level alone (flatten to CSR in UG's row layout, with the
+    copy of the level to the host), whether it equals the host generator's
+    level, and the set-up of the whole hierarchy after it: the assembled
+    level's CSR stays on the device for the device layout build (coarse
+    levels from the host generator), against the same hierarchy laid out on
+    the host."""
     import numpy as np
+    top = P.levels[-1]
     el = problems.q1_elements(3, 1, nref, problems.KAPPA_JUMP, seed=0x5EED0002)
     elem = torch.from_numpy(el.elem).cuda()
     kref = torch.from_numpy(el.kref).cuda()
@@ -185,9 +190,29 @@ def assemble_run(problems, ugmg, torch, nref, top):
     same = bool(np.array_equal(rp, top.row_ptr) and np.array_equal(ci, top.col_idx)
                 and np.array_equal(vals.view(np.int64), top.vals.view(np.int64)))
     ctx.close()
-    return {"elements": int(el.nelem), "nnz": int(rp[-1]), "seconds_min": round(min(times), 3),
-            "equals_host_level": same,
-            "note": "element list on the GPU -> CSR in UG row layout, incl. the copy to the host"}
+    out = {"elements": int(el.nelem), "nnz": int(rp[-1]), "assemble_s": round(min(times), 3),
+           "equals_host_level": same}
+    # the hierarchy's set-up: device assembly + device layout, device
+    # layout of the uploaded levels, host layout (each includes the base
+    # level's band LU and inverse)
+    for key, where, assemble in (("assemble_and_device_layout_s", ugmg.SETUP_DEVICE, True),
+                                 ("device_layout_s", ugmg.SETUP_DEVICE, False),
+                                 ("host_layout_s", ugmg.SETUP_HOST, False)):
+        ctx = ugmg.Context(0)
+        ctx.set_setup(where)
+        ctx.set_hierarchy(P)
+        ctx.set_params(**base, precision=ugmg.FP64)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        if assemble:
+            ctx.assemble_level(P.top, el.n, elem, kref=kref, scale=scale, skip=skip)
+        ctx.preprocess()
+        ctx.sync()
+        out[key] = round(time.perf_counter() - t0, 3)
+        ctx.close()
+    out["note"] = ("element list on the GPU -> CSR in UG row layout (incl. its copy to the host); "
+                   "set-up times cover levels 4..7 incl. the 17^3 base level's band LU and inverse")
+    return out
 
 
 def bcgs_run(nref, nrhs, steps, warmup, cpu_sample):
@@ -587,7 +612,7 @@ def main():
         ctx64.close()
         return 0
     print("bench: 129^3 set up", file=sys.stderr, flush=True)
-    asm = assemble_run(problems, ugmg, torch, args.nref, P.levels[-1])
+    asm = assemble_run(problems, ugmg, torch, args.nref, P, base)
     ctxmx = ugmg.Context(0)
     ctxmx.set_hierarchy(P)
     ctxmx.set_params(**base, precision=ugmg.MIXED, inner_cycles=args.inner)

@@ -168,3 +168,32 @@ def test_layout_jump3d_129_full_size():
     P = problems.jump3d(7)
     th, td = _compare_builds(P, ugmg.SMOOTH_GS, 4, solve=False)
     print(f"129^3 GS: host build {th:.3f} s, device build {td:.3f} s")
+
+
+def test_assembled_level_laid_out_from_its_device_csr(jump3d_33):
+    """ugmg_assemble_level keeps the flattened level on the device and the
+    device layout build reads it there (no upload): the layout equals the
+    one built from the generator's host level, byte for byte."""
+    from ug_amd import problems, ugmg
+    P = jump3d_33
+    el = problems.q1_elements(3, 1, 5, problems.KAPPA_JUMP, seed=0x5EED0002)
+    elem = torch.from_numpy(el.elem).cuda()
+    kref = torch.from_numpy(el.kref).cuda()
+    scale = torch.from_numpy(el.scale).cuda()
+    skip = torch.from_numpy(el.skip.view(np.int32)).cuda()
+    torch.cuda.synchronize()
+    ref, _ = _ctx(P, ugmg.SMOOTH_GS, 2, ugmg.SETUP_HOST)
+    ctx = ugmg.Context(0)
+    try:
+        ctx.set_hierarchy(P)
+        ctx.assemble_level(P.top, el.n, elem, kref=kref, scale=scale, skip=skip)
+        ctx.set_params(nu1=2, nu2=2, gamma=1, baselevel=2, smoother=ugmg.SMOOTH_GS, damp=1.0,
+                       base_abslimit=1e-300, base_red=1e-12, base_maxiter=1)
+        ctx.preprocess()
+        for l in range(2, P.top + 1):
+            assert np.array_equal(ctx.layout_digest(l), ref.layout_digest(l)), l
+        ctx.preprocess()   # again: the kept CSR was consumed, the host copy serves
+        assert np.array_equal(ctx.layout_digest(P.top), ref.layout_digest(P.top))
+    finally:
+        ctx.close()
+        ref.close()

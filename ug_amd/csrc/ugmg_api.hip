@@ -221,6 +221,14 @@ struct ugmg_ctx {
   // members: `members` matrices of one pattern, one per column of the
   // level vectors (batched launches read column j's values, diagonal
   // blocks and base inverse); set through member_set
+  // levels flattened on the device by ugmg_assemble_level: their CSR kept
+  // there for the next device layout build (freed by it, or when the level
+  // is set again)
+  struct DevCsr {
+    int *rp = nullptr, *ci = nullptr;
+    double *a = nullptr;
+  };
+  std::vector<DevCsr> dcsr;
   int members = 1;
   // a serving group's context: every level carries member planes and the
   // index maps that fill them (also with one member)
@@ -375,6 +383,15 @@ int adopt_sell(ugmg_ctx *ctx, const DevSellParts &S, DevSellBuf &D) {
   HIPCHK(hipGetLastError());
   D.mst = 0;
   return 0;
+}
+
+void drop_dcsr(ugmg_ctx *ctx, int level) {
+  if (level < 0 || level >= (int)ctx->dcsr.size()) return;
+  auto &d = ctx->dcsr[level];
+  (void)hipFree(d.rp);
+  (void)hipFree(d.ci);
+  (void)hipFree(d.a);
+  d = ugmg_ctx::DevCsr();
 }
 
 void drop_bcgs_graphs(ugmg_ctx *ctx) {
@@ -1221,8 +1238,10 @@ int build_layouts_device(ugmg_ctx *ctx, PrepState &ps) {
     const int coarse_n = l > bl ? ctx->host[l - 1].n : 0;
     const int *coarse_pos = l > bl ? ps.dp[l - 1].pos : nullptr;
     std::string err;
+    const ugmg_ctx::DevCsr dc = l < (int)ctx->dcsr.size() ? ctx->dcsr[l] : ugmg_ctx::DevCsr();
     rc = build_level_device(ctx->stream, ctx->host[l], coarse_n, coarse_pos, gs, ps.force_win, keep,
-                            ps.dp[l], err);
+                            ps.dp[l], err, dc.rp, dc.ci, dc.a);
+    if (!rc) drop_dcsr(ctx, l);   // laid out: the host copy serves later builds
     if (rc) rc = fail(ctx, rc, "level " + std::to_string(l) + " layout: " + err);
     if (l > bl && ps.dp[l - 1].pos) {
       (void)hipFree(ps.dp[l - 1].pos);
@@ -1573,6 +1592,7 @@ int ugmg_destroy(ugmg_ctx *ctx) {
   hipSetDevice(ctx->device);
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
   free_device(ctx);
+  for (int l = 0; l < (int)ctx->dcsr.size(); ++l) drop_dcsr(ctx, l);
   if (ctx->red_host) hipHostFree(ctx->red_host);
   if (ctx->surf_sq_host) hipHostFree(ctx->surf_sq_host);
   if (ctx->state_host) hipHostFree(ctx->state_host);
@@ -1612,6 +1632,8 @@ int ugmg_set_nlevels(ugmg_ctx *ctx, int nlevels, int b) {
   if (b != 1 && b != 3)
     return fail(ctx, UGMG_E_ARG, "block size must be 1 (scalar) or 3 (3x3 blocks)");
   free_device(ctx);
+  for (int l = 0; l < (int)ctx->dcsr.size(); ++l) drop_dcsr(ctx, l);
+  ctx->dcsr.assign(nlevels, ugmg_ctx::DevCsr());
   ctx->nlevels = nlevels;
   ctx->b = b;
   ctx->host.assign(nlevels, HostLevel());
@@ -1627,6 +1649,7 @@ int ugmg_set_level(ugmg_ctx *ctx, int level, int n, const int *row_ptr, const in
   if (level < 0 || level >= ctx->nlevels) return fail(ctx, UGMG_E_ARG, "level out of range");
   if (n < 0 || (n > 0 && (!row_ptr || !col_idx || !vals || !skip || !vclass || !vnclass)))
     return fail(ctx, UGMG_E_ARG, "null array");
+  drop_dcsr(ctx, level);
   try {
     HostLevel &L = ctx->host[level];
     L.n = n;
@@ -1667,10 +1690,13 @@ int ugmg_assemble_level(ugmg_ctx *ctx, int level, const ugmg_elements *el, doubl
   AsmInput in{el->n, ctx->b, el->nelem, el->nc, el->elem, el->emat, el->kref, el->scale, el->evec,
               el->skip, el->dirichlet};
   std::string err;
+  drop_dcsr(ctx, level);
   try {
     HostLevel L;
-    const int rc = assemble_device(ctx->stream, in, rhs, L.rp, L.ci, L.a, err);
+    AsmKeep kept;
+    const int rc = assemble_device(ctx->stream, in, rhs, L.rp, L.ci, L.a, err, &kept);
     if (rc) return fail(ctx, rc, "ugmg_assemble_level: " + err);
+    ctx->dcsr[level] = ugmg_ctx::DevCsr{kept.rp, kept.ci, kept.vals};
     const int n = el->n;
     L.n = n;
     L.b = ctx->b;

@@ -189,7 +189,7 @@ inline unsigned nblk(long long n) { return (unsigned)std::max<long long>(1, (n +
 }  // namespace
 
 int assemble_device(hipStream_t st, const AsmInput &in, double *rhs, std::vector<int> &rp_h,
-                    hvec<int> &ci_h, hvec<double> &vals_h, std::string &err) {
+                    hvec<int> &ci_h, hvec<double> &vals_h, std::string &err, AsmKeep *keep) {
   const int n = in.n;
   const long long total = (long long)in.nelem * in.nc;
   const int bb = in.b * in.b;
@@ -269,6 +269,13 @@ int assemble_device(hipStream_t st, const AsmInput &in, double *rhs, std::vector
                           hipMemcpyDeviceToHost, st)) ||
       (e = hipStreamSynchronize(st)))
     return hip_fail(e, "assembly copy-out");
+  if (keep) {   // the flattened arrays stay on the device for the layout build
+    keep->rp = rp;
+    keep->ci = ci;
+    keep->vals = vals;
+    rp = ci = nullptr;
+    vals = nullptr;
+  }
   release();
   return 0;
 }

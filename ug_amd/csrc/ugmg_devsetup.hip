@@ -963,7 +963,7 @@ int color_device(void *stream, int n, const int *rp, const int *ci, int *colors,
 
 int build_level_device(void *stream, const HostLevel &H, int coarse_n, const int *coarse_pos,
                        bool gs, int force_win, const DevKeep &keep, DevPrep &out,
-                       std::string &err) {
+                       std::string &err, const int *drp, const int *dci, const double *da) {
   Ctx C{(hipStream_t)stream, keep, err};
   Temps T;
   const int n = H.n, bb = H.b * H.b;
@@ -971,13 +971,24 @@ int build_level_device(void *stream, const HostLevel &H, int coarse_n, const int
   out.n = n;
   out.nnzb = H.nnzb;
   // the level as it crosses the ABI (list order)
-  int *rp, *ci;
-  double *a;
+  const int *rp, *ci;
+  const double *a;
   uint32_t *skip;
   uint8_t *vc, *vnc;
-  TRYRC(up(C, T, &rp, H.rp.data(), H.rp.size()));
-  TRYRC(up(C, T, &ci, H.ci.data(), H.ci.size()));
-  TRYRC(up(C, T, &a, H.a.data(), H.a.size()));
+  if (drp && dci && da) {   // resident on the device (assembled there)
+    rp = drp;
+    ci = dci;
+    a = da;
+  } else {
+    int *urp, *uci;
+    double *ua;
+    TRYRC(up(C, T, &urp, H.rp.data(), H.rp.size()));
+    TRYRC(up(C, T, &uci, H.ci.data(), H.ci.size()));
+    TRYRC(up(C, T, &ua, H.a.data(), H.a.size()));
+    rp = urp;
+    ci = uci;
+    a = ua;
+  }
   int *pp = nullptr, *pi = nullptr;
   double *pw = nullptr;
   const bool has_p = coarse_n > 0 && H.has_p;

@@ -124,9 +124,12 @@ struct DevKeep {
 // order), gs: a multicolour GS level, force_win >= 0: the device-order
 // window instead of the chosen one.  Returns 0 or a UGMG error code (err
 // says why).
+// drp/dci/da (optional): the level's CSR already on the device (equal to
+// H's), used instead of an upload.
 int build_level_device(void *stream, const HostLevel &H, int coarse_n, const int *coarse_pos,
                        bool gs, int force_win, const DevKeep &keep, DevPrep &out,
-                       std::string &err);
+                       std::string &err, const int *drp = nullptr, const int *dci = nullptr,
+                       const double *da = nullptr);
 // greedy_color on the device (the same colouring): colors (device, n ints)
 // receives it, *rounds the frontier rounds it took; returns the colour count
 // or < 0.

@@ -167,8 +167,15 @@ struct AsmInput {
 // Assembles the level on the device and copies the CSR (diagonal block
 // first, then ascending columns) to the host vectors; rhs (device, n b) or
 // null.  Returns 0 or a negative UGMG_E_* code with err set.
+// keep (optional): the flattened level's device arrays, handed to the
+// caller (hipFree) instead of freed
+struct AsmKeep {
+  int *rp = nullptr, *ci = nullptr;
+  double *vals = nullptr;
+};
 int assemble_device(hipStream_t st, const AsmInput &in, double *rhs, std::vector<int> &rp_h,
-                    hvec<int> &ci_h, hvec<double> &vals_h, std::string &err);
+                    hvec<int> &ci_h, hvec<double> &vals_h, std::string &err,
+                    AsmKeep *keep = nullptr);
 
 // Band LU (no pivoting) of the base level and its explicit inverse, on the
 // device: band (n rows, bandwidth bw, UG's EX_MAT layout) is factored in

@@ -604,3 +604,67 @@ def p1_local(nuni: int, depth: int, ncomp: int = 1) -> Problem:
         levels[l].vnclass = _classes(levels[l], tri_l, nn3)
     rhs = np.zeros(levels[-1].n * ncomp)
     return Problem(dim=2, levels=levels, rhs=rhs, name=f"p1_local_{nuni}u{depth}l_b{ncomp}")
+
+
+def reordered(P: Problem, seed: int, newest_first: bool = True) -> Problem:
+    """P with each level's vector list in a seeded random order and, with
+    newest_first, every row listing its off-diagonal entries in the reverse
+    of the builder's ascending order (the diagonal stays first).
+
+    UG lists a level's vectors in creation order (gm/dlmgr.t:401-420) and
+    inserts each new connection at the head of both rows' lists
+    (CreateConnection, gm/algebra.cc:1042-1075), so neither the builders'
+    lexicographic vector lists nor their ascending rows are UG's.  Every
+    result the library computes depends on these orders only through the
+    arrays it is given (row sums run in list order, the greedy colouring in
+    vector-list order); this map produces such inputs.  The transfer keeps
+    each fine row's entries and maps both sides; the right-hand side moves
+    with the top level's vectors (rhs_new[q] = rhs[perm[q]])."""
+    rng = np.random.default_rng(seed)
+    perms, invs = [], []
+    for lv in P.levels:
+        pm = rng.permutation(lv.n).astype(np.int64)
+        inv = np.empty_like(pm)
+        inv[pm] = np.arange(lv.n)
+        perms.append(pm)
+        invs.append(inv)
+    out = []
+    for l, lv in enumerate(P.levels):
+        pm, inv, b = perms[l], invs[l], lv.b
+        bb = b * b
+        blocks = lv.vals.reshape(-1, bb)
+        rp = np.zeros(lv.n + 1, np.int32)
+        ci, va = [], []
+        for q in range(lv.n):
+            r = pm[q]
+            lo, hi = lv.row_ptr[r], lv.row_ptr[r + 1]
+            ent = list(range(lo, hi))
+            diag, off = ent[0], ent[1:]
+            if newest_first:
+                off = off[::-1]
+            for m in [diag] + off:
+                ci.append(inv[lv.col_idx[m]])
+                va.append(blocks[m])
+            rp[q + 1] = rp[q] + (hi - lo)
+        new = Level(n=lv.n, b=b, row_ptr=rp, col_idx=np.asarray(ci, np.int32),
+                    vals=np.ascontiguousarray(np.concatenate(va) if va else np.zeros(0)),
+                    skip=lv.skip[pm].copy(), vclass=lv.vclass[pm].copy(), vnclass=lv.vnclass[pm].copy())
+        if l > 0:
+            cinv = invs[l - 1]
+            pp = np.zeros(lv.n + 1, np.int32)
+            pi, pw = [], []
+            for q in range(lv.n):
+                r = pm[q]
+                lo, hi = lv.p_ptr[r], lv.p_ptr[r + 1]
+                pi.extend(cinv[lv.p_idx[lo:hi]])
+                pw.extend(lv.p_w[lo:hi])
+                pp[q + 1] = pp[q] + (hi - lo)
+            new.p_ptr, new.p_idx, new.p_w = pp, np.asarray(pi, np.int32), np.asarray(pw, np.float64)
+        out.append(new)
+    rhs = None
+    if P.rhs is not None:
+        top = P.levels[-1]
+        rhs = np.asarray(P.rhs).reshape(top.n, top.b)[perms[-1]].ravel().copy()
+    Q = Problem(dim=P.dim, levels=out, rhs=rhs, name=P.name + f"_reordered{seed}", theta=P.theta)
+    Q.perms = perms
+    return Q

@@ -110,10 +110,10 @@ int ugmg_sync(ugmg_ctx *ctx);
  * VECSKIP bits, vclass/vnclass = VCLASS/VNCLASS (gm/gm.h:2200-2288).
  * Interpolation rows: coarse corners of the father element with their GNs
  * weights (StandardIntCorNodeVector, transgrid.cc:235-336). */
-/* b: components per vector, 1 or 3 (3x3 blocks: SolveSmallBlock-based
- * Jacobi and GS, per-component Dirichlet bits, norms and stop tests; one
- * damping factor and base limit for all components; bcgs and the server
- * take b = 1) */
+/* b: components per vector, 1, 2 or 3 (b x b blocks: Jacobi and GS with
+ * SolveSmallBlock's n = 2 and n = 3 forms, per-component Dirichlet bits,
+ * norms and stop tests; one damping factor and base limit for all
+ * components; bcgs and the server take b = 1) */
 int ugmg_set_nlevels(ugmg_ctx *ctx, int nlevels, int b);
 int ugmg_set_level(ugmg_ctx *ctx, int level, int n, const int *row_ptr,
                    const int *col_idx, const double *vals,

@@ -1,0 +1,164 @@
+"""GPU parity of the 2x2-block path (b = 2: SolveSmallBlock's n = 2 form,
+np/algebra/block.cc:119-130) against the oracle, on the 2D P1 coupled
+system of problems.p1_local(.., ncomp=2): block SpMV and norms, Jacobi and
+multicolour-GS steps, per-component transfer and the ex base solve, one
+lmgc cycle, ls + lmgc solves with per-component defect norms (Jacobi every
+iteration within 1e-10 relative; MC-GS defects against the oracle's
+colour-ordered restatement within 1e-9 and the final solution within 1e-8
+of lexicographic GS; FP64 and the mixed form), and the ON_SURFACE solve of
+a locally refined 2-component hierarchy.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _dev(a):
+    t = torch.from_numpy(np.ascontiguousarray(a, np.float64)).cuda()
+    torch.cuda.synchronize()
+    return t
+
+
+def _host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def _rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+@pytest.fixture(scope="module")
+def sys2():
+    from ug_amd import problems
+    P = problems.p1_local(6, 0, 2)     # 129^2 nodes, 33,282 unknowns
+    lv = P.levels[-1]
+    rhs = np.random.default_rng(0x5EED0301).uniform(-1, 1, 2 * lv.n)
+    rhs[np.repeat(lv.skip != 0, 2)] = 0.0
+    P.rhs = rhs
+    return P
+
+
+def _ctx(P, smoother, damp, baselevel=2, **kw):
+    from ug_amd import ugmg
+    ctx = ugmg.Context(0)
+    ctx.set_hierarchy(P)
+    ctx.set_params(nu1=2, nu2=2, gamma=1, baselevel=baselevel, smoother=smoother, damp=damp,
+                   base_abslimit=1e-300, base_red=1e-12, base_maxiter=1, **kw)
+    ctx.preprocess()
+    return ctx
+
+
+def test_block2_kernels(sys2):
+    from ug_amd import ugmg
+    P = sys2
+    H = O.Hierarchy(P.levels)
+    cg, cj = _ctx(P, ugmg.SMOOTH_GS, 1.0), _ctx(P, ugmg.SMOOTH_JAC, 0.6)
+    try:
+        for l in range(3, P.top + 1):
+            lv = P.levels[l]
+            n2 = 2 * lv.n
+            rp, ci, va = cg.pattern(l)
+            assert np.array_equal(rp, lv.row_ptr) and np.array_equal(ci, lv.col_idx)
+            assert np.array_equal(va, lv.vals)
+            r = np.random.default_rng(l)
+            x, y = r.uniform(-1, 1, n2), r.uniform(-1, 1, n2)
+            xd = _dev(x)
+            cg.dmatmul_minus(l, xd, _dev(y))
+            assert _rel(_host(xd), O.dmatmul_minus(H, l, x, y)) < 1e-14, l
+            np.testing.assert_allclose(cg.dnrm2x(l, _dev(x)), O.dnrm2x(H, l, x), rtol=1e-13)
+            b = r.uniform(-1, 1, n2)
+            xw, bw = O.smoother(H, l, O.JAC, 0.6, b)
+            xd, bd = _dev(np.zeros(n2)), _dev(b)
+            cj.jacobi_step(l, xd, bd)
+            assert _rel(_host(xd), xw) < 1e-13 and _rel(_host(bd), bw) < 1e-12, l
+            _, col = O.greedy_color(H, l)
+            rank = (col.astype(np.int64) * lv.n + np.arange(lv.n)).astype(np.int32)
+            xw, bw = O.smoother(H, l, O.GS, 1.0, b, rank=rank)
+            xd, bd = _dev(np.zeros(n2)), _dev(b)
+            cg.gs_step(l, xd, bd)
+            assert _rel(_host(xd), xw) < 1e-12 and _rel(_host(bd), bw) < 1e-11, l
+            nc2 = 2 * P.levels[l - 1].n
+            bf = r.uniform(-1, 1, n2)
+            cc = r.uniform(-1, 1, nc2)
+            to_c = _dev(np.full(nc2, 5.0))
+            cg.restrict(l, to_c, _dev(bf))
+            assert _rel(_host(to_c), O.restrict(H, l, bf)) < 1e-14, l
+            to_f = _dev(np.full(n2, 3.0))
+            cg.interpolate(l, to_f, _dev(cc), damp=0.7)
+            assert _rel(_host(to_f), O.interpolate(H, l, cc, damp=0.7)) < 1e-15, l
+        n2 = 2 * P.levels[2].n
+        b = np.random.default_rng(60).uniform(-1, 1, n2)
+        rc, xw, bw = O.Ex(H, 2, reorder=1).smooth(b)
+        assert rc == 0
+        cd, bd = _dev(np.zeros(n2)), _dev(b)
+        cg.base_solve(cd, bd)
+        assert _rel(_host(cd), xw) < 1e-12
+        cw, bw = O.lmgc_iter(H, P.top, O.params(smoother=O.MCGS, damp=1.0, baselevel=2), P.rhs)
+        c, bn = cg.iter_host(P.top, P.rhs)
+        assert _rel(c, cw) < 1e-11 and _rel(bn, bw) < 1e-10
+    finally:
+        cg.close()
+        cj.close()
+
+
+@pytest.mark.parametrize("graph", [0, 1])
+def test_block2_jacobi_solve_defects_per_component(sys2, graph):
+    from ug_amd import ugmg
+    P = sys2
+    want = O.ls_solve(O.Hierarchy(P.levels), P.top, O.params(smoother=O.JAC, damp=0.6, baselevel=2), P.rhs,
+                      red=1e-10, maxiter=200)
+    ctx = _ctx(P, ugmg.SMOOTH_JAC, 0.6, use_graph=graph)
+    try:
+        x, b, res, d = ctx.solve_host(P.top, P.rhs, red=1e-10, maxiter=200)
+    finally:
+        ctx.close()
+    assert res.converged and res.number_of_linear_iterations == want["nit"]
+    assert d.shape[1] == 2
+    assert np.max(np.abs(d - want["defects"]) / want["defects"]) < 1e-10
+    assert _rel(x, want["x"]) < 1e-10
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_block2_gs_solve(sys2, precision):
+    from ug_amd import ugmg
+    P = sys2
+    H = O.Hierarchy(P.levels)
+    lex = O.ls_solve(H, P.top, O.params(smoother=O.GS, damp=1.0, baselevel=2), P.rhs, red=1e-10, maxiter=200)
+    mc = O.ls_solve(H, P.top, O.params(smoother=O.MCGS, damp=1.0, baselevel=2), P.rhs, red=1e-10, maxiter=200)
+    ctx = _ctx(P, ugmg.SMOOTH_GS, 1.0, precision=precision, inner_cycles=2, use_graph=1)
+    try:
+        x, b, res, d = ctx.solve_host(P.top, P.rhs, red=1e-10, maxiter=200)
+    finally:
+        ctx.close()
+    assert res.converged
+    assert np.all(np.array(res.last_defect[:2]) < 1e-10 * np.array(res.first_defect[:2]))
+    assert np.max(np.abs(x - lex["x"])) < 1e-8 * np.max(np.abs(lex["x"]))
+    if precision == 0:
+        assert res.number_of_linear_iterations == mc["nit"]
+        assert np.max(np.abs(d - mc["defects"]) / mc["defects"]) < 1e-9
+
+
+def test_block2_surface_solve():
+    from ug_amd import problems, ugmg
+    P = problems.p1_local(5, 2, 2)
+    bl = 1
+    assert problems.fullrefinelevel(P) < P.top
+    fs = problems.surface_rhs(P, bl, 0x5EED0302)
+    want = O.ls_solve_surface(O.Hierarchy(P.levels), P.top, O.params(baselevel=bl, smoother=O.JAC, damp=0.8),
+                              fs, red=1e-10, maxiter=40)
+    ctx = _ctx(P, ugmg.SMOOTH_JAC, 0.8, baselevel=bl)
+    try:
+        xs, bs, res, d = ctx.solve_surface_host(P.top, fs, red=1e-10, maxiter=40)
+    finally:
+        ctx.close()
+    assert res.number_of_linear_iterations == want["nit"]
+    assert d.shape == want["defects"].shape and d.shape[1] == 2
+    assert np.all(np.abs(d - want["defects"]) <= 1e-10 * want["defects"])
+    for l in range(bl, P.top + 1):
+        assert _rel(xs[l], want["x"][l]) <= 1e-10, l

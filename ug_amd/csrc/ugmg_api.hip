@@ -1629,8 +1629,8 @@ int ugmg_sync(ugmg_ctx *ctx) {
 
 int ugmg_set_nlevels(ugmg_ctx *ctx, int nlevels, int b) {
   if (!ctx || nlevels < 1 || nlevels > 32) return ctx ? fail(ctx, UGMG_E_ARG, "nlevels must be 1..32") : UGMG_E_ARG;
-  if (b != 1 && b != 3)
-    return fail(ctx, UGMG_E_ARG, "block size must be 1 (scalar) or 3 (3x3 blocks)");
+  if (b < 1 || b > UGMG_MAXB)
+    return fail(ctx, UGMG_E_ARG, "block size must be 1 (scalar), 2 or 3 (2x2 / 3x3 blocks)");
   free_device(ctx);
   for (int l = 0; l < (int)ctx->dcsr.size(); ++l) drop_dcsr(ctx, l);
   ctx->dcsr.assign(nlevels, ugmg_ctx::DevCsr());

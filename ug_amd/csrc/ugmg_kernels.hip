@@ -164,6 +164,21 @@ __device__ __forceinline__ void solve3(const T (&m)[9], const T (&r)[3], T (&s)[
   s[0] = (r[0] - m[1] * s[1] - m[2] * s[2]) / m[0];
 }
 
+// SolveSmallBlock (np/algebra/block.cc:119-142) for n = 2 (the inverse
+// determinant times the cofactor products, UG's operation order) and n = 3
+template <int B, typename T>
+__device__ __forceinline__ void solve_small(const T (&m)[B * B], const T (&r)[B], T (&s)[B]) {
+  if constexpr (B == 2) {
+    T det = m[0] * m[3] - m[1] * m[2];
+    det = (T)1 / det;
+    s[0] = (r[0] * m[3] - r[1] * m[1]) * det;
+    s[1] = (r[1] * m[0] - r[0] * m[2]) * det;
+  } else {
+    static_assert(B == 3, "blocks of 2 or 3 components");
+    solve3<T>(m, r, s);
+  }
+}
+
 // block sum of B doubles per thread; thread 0 gets the totals
 template <int B>
 __device__ __forceinline__ void block_sum_b(double (&v)[B], double (*red)[kBlock / 64]) {
@@ -667,16 +682,16 @@ __global__ __launch_bounds__(kBlock) void k_copy_rows(int ne, int b, const uint8
   if (e < ne && cls[e / b] >= 2) dst[e] = src[e];
 }
 
-// ======================================================== 3x3 blocks
-// The same operations on vectors of B = 3 components with b x b matrix
-// blocks (linear elasticity); diag holds the diagonal block of every row.
+// ================================================= 2x2 and 3x3 blocks
+// The same operations on vectors of B = 2 or 3 components with B x B matrix
+// blocks (linear elasticity, coupled systems); diag holds the diagonal
+// block of every row.
 
-template <typename TV, typename TX, typename TY, int MODE, int NT>
-__global__ __launch_bounds__(kBlock) void k_spmv_b3(DSell<TV> A, int slice0, int nsl,
+template <int B, typename TV, typename TX, typename TY, int MODE, int NT>
+__global__ __launch_bounds__(kBlock) void k_spmv_bk(DSell<TV> A, int slice0, int nsl,
                                                      const TX *__restrict__ x,
                                                      TY *__restrict__ y, TY *__restrict__ c,
                                                      size_t vs) {
-  constexpr int B = 3;
   SLICE_PROLOGUE(A, slice0, nsl)
   const size_t o = (size_t)col_ * vs;
   x += o;
@@ -697,19 +712,18 @@ __global__ __launch_bounds__(kBlock) void k_spmv_b3(DSell<TV> A, int slice0, int
   }
 }
 
-template <int NT>
-__global__ __launch_bounds__(kBlock) void k_ir_update_b3(DSell<double> A, int nsl,
+template <int B, int NT>
+__global__ __launch_bounds__(kBlock) void k_ir_update_bk(DSell<double> A, int nsl,
                                                           const float *__restrict__ c32,
                                                           double *__restrict__ rv,
                                                           double *__restrict__ x,
                                                           float *__restrict__ r32,
                                                           const uint8_t *__restrict__ vclass,
                                                           double *__restrict__ part, size_t vs) {
-  constexpr int B = 3;
   __shared__ double red[B][kBlock / 64];
   SLICE_PROLOGUE_RED(A, nsl)
   const size_t o = (size_t)col_ * vs;
-  double sq[B] = {0.0, 0.0, 0.0};
+  double sq[B] = {};
   if (sok) {
     double sum[B];
     sell_dot_blk<B, NT, double, float, double>(MV(A) + e0 + lane, A.pst, A.col + e0 + lane, w,
@@ -730,63 +744,62 @@ __global__ __launch_bounds__(kBlock) void k_ir_update_b3(DSell<double> A, int ns
     for (int i = 0; i < B; ++i) part[((size_t)col_ * B + i) * gridDim.x + sb_] = sq[i];
 }
 
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_jac_b3(int n, const T *__restrict__ b,
+template <int B, typename T>
+__global__ __launch_bounds__(kBlock) void k_jac_bk(int n, const T *__restrict__ b,
                                                     const T *__restrict__ diag,
                                                     const uint8_t *__restrict__ vclass, T damp,
                                                     T *__restrict__ t, size_t vs, size_t dms) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
-  const size_t o = (size_t)blockIdx.y * vs + 3 * (size_t)i;
+  const size_t o = (size_t)blockIdx.y * vs + B * (size_t)i;
   diag += dms * blockIdx.y;
   if (vclass[i] < 3) {
-    t[o] = t[o + 1] = t[o + 2] = (T)0;
+#pragma unroll
+    for (int q = 0; q < B; ++q) t[o + q] = (T)0;
     return;
   }
-  T m[9], r[3], x[3];
+  T m[B * B], r[B], x[B];
 #pragma unroll
-  for (int q = 0; q < 9; ++q) m[q] = diag[9 * (size_t)i + q];
+  for (int q = 0; q < B * B; ++q) m[q] = diag[B * B * (size_t)i + q];
 #pragma unroll
-  for (int q = 0; q < 3; ++q) r[q] = b[o + q];
-  solve3<T>(m, r, x);
+  for (int q = 0; q < B; ++q) r[q] = b[o + q];
+  solve_small<B, T>(m, r, x);
 #pragma unroll
-  for (int q = 0; q < 3; ++q) t[o + q] = x[q] * damp;
+  for (int q = 0; q < B; ++q) t[o + q] = x[q] * damp;
 }
 
-template <typename TV, typename T, int NT, int BS = kBlock>
-__global__ __launch_bounds__(BS) void k_gs_color_b3(DSell<TV> L, int slice0, int nsl,
+template <int B, typename TV, typename T, int NT, int BS = kBlock>
+__global__ __launch_bounds__(BS) void k_gs_color_bk(DSell<TV> L, int slice0, int nsl,
                                                          const T *__restrict__ b,
                                                          const T *__restrict__ diag,
                                                          const uint8_t *__restrict__ vclass,
                                                          T *__restrict__ v, size_t vs) {
-  constexpr int B = 3;
   SLICE_PROLOGUE_BS(L, slice0, nsl, BS)
   const size_t o = (size_t)col_ * vs;
   v += o;
   T sum[B];
   sell_dot_blk<B, NT, TV, T, T>(MV(L) + e0 + lane, L.pst, L.col + e0 + lane, w, v, sdo, L.dtab, rb, sum);
   if (r < 0) return;
-  T x[B] = {0, 0, 0};
+  T x[B] = {};
   if (vclass[r] >= 3) {
-    T m[9], rr[B];
+    T m[B * B], rr[B];
 #pragma unroll
-    for (int q = 0; q < 9; ++q) m[q] = diag[L.dms * col_ + 9 * (size_t)r + q];
+    for (int q = 0; q < B * B; ++q) m[q] = diag[L.dms * col_ + B * B * (size_t)r + q];
 #pragma unroll
     for (int q = 0; q < B; ++q) rr[q] = b[o + (size_t)B * r + q] - sum[q];
-    solve3<T>(m, rr, x);
+    solve_small<B, T>(m, rr, x);
   }
 #pragma unroll
   for (int q = 0; q < B; ++q) v[(size_t)B * r + q] = x[q];
 }
 
-template <typename TV, typename T, int CMODE, int NT>
-__global__ __launch_bounds__(kBlock) void k_gs_defect_b3(DSell<TV> U, int nsl,
+template <int B, typename TV, typename T, int CMODE, int NT>
+__global__ __launch_bounds__(kBlock) void k_gs_defect_bk(DSell<TV> U, int nsl,
                                                           const T *__restrict__ v,
                                                           T *__restrict__ b,
                                                           const uint8_t *__restrict__ vclass,
                                                           T damp, T *__restrict__ c,
                                                           T *__restrict__ xout, size_t vs) {
-  constexpr int B = 3;
   SLICE_PROLOGUE_MAP(U, 0, nsl, true)
   const size_t o = (size_t)col_ * vs;
   v += o;
@@ -808,14 +821,13 @@ __global__ __launch_bounds__(kBlock) void k_gs_defect_b3(DSell<TV> U, int nsl,
   }
 }
 
-template <typename TV, typename T, int CMODE, int NT, int BS = kBlock>
-__global__ __launch_bounds__(BS) void k_gs_color_fused_b3(DSell<TV> L, DSell<TV> U,
+template <int B, typename TV, typename T, int CMODE, int NT, int BS = kBlock>
+__global__ __launch_bounds__(BS) void k_gs_color_fused_bk(DSell<TV> L, DSell<TV> U,
                                                                int slice0, int nsl,
                                                                T *__restrict__ b,
                                                                const T *__restrict__ diag, T damp,
                                                                T *__restrict__ v,
                                                                T *__restrict__ c, size_t vs) {
-  constexpr int B = 3;
   SLICE_PROLOGUE_BS(L, slice0, nsl, BS)
   const size_t o = (size_t)col_ * vs;
   v += o;
@@ -826,9 +838,9 @@ __global__ __launch_bounds__(BS) void k_gs_color_fused_b3(DSell<TV> L, DSell<TV>
   sell_dot_blk<B, NT, TV, T, T>(MV(U) + eu + lane, U.pst, U.col + eu + lane, wu, v, sdu, U.dtab, rb, su);
   sell_dot_blk<B, NT, TV, T, T>(MV(L) + e0 + lane, L.pst, L.col + e0 + lane, w, v, sdo, L.dtab, rb, sl);
   if (r < 0) return;
-  T m[9], rr[B], x[B];
+  T m[B * B], rr[B], x[B];
 #pragma unroll
-  for (int q = 0; q < 9; ++q) m[q] = diag[L.dms * col_ + 9 * (size_t)r + q];
+  for (int q = 0; q < B * B; ++q) m[q] = diag[L.dms * col_ + B * B * (size_t)r + q];
 #pragma unroll
   for (int q = 0; q < B; ++q) {
     const size_t k = (size_t)B * r + q;
@@ -839,13 +851,13 @@ __global__ __launch_bounds__(BS) void k_gs_color_fused_b3(DSell<TV> L, DSell<TV>
     c[o + k] = CMODE == 0 ? t : c[o + k] + t;
     rr[q] = bn - sl[q];
   }
-  solve3<T>(m, rr, x);
+  solve_small<B, T>(m, rr, x);
 #pragma unroll
   for (int q = 0; q < B; ++q) v[(size_t)B * r + q] = x[q];
 }
 
-template <typename T, int NT>
-__global__ __launch_bounds__(kBlock) void k_restrict_b3(DSell<T> PT, int nsl,
+template <int B, typename T, int NT>
+__global__ __launch_bounds__(kBlock) void k_restrict_bk(DSell<T> PT, int nsl,
                                                          const uint32_t *__restrict__ cskip,
                                                          const uint8_t *__restrict__ cvnclass,
                                                          const uint8_t *__restrict__ cvclass,
@@ -853,12 +865,11 @@ __global__ __launch_bounds__(kBlock) void k_restrict_b3(DSell<T> PT, int nsl,
                                                          T *__restrict__ bc,
                                                          double *__restrict__ part, size_t vsf,
                                                          size_t vsc) {
-  constexpr int B = 3;
   __shared__ double red[B][kBlock / 64];
   SLICE_PROLOGUE_RED(PT, nsl)
   bf += (size_t)col_ * vsf;
   bc += (size_t)col_ * vsc;
-  double sq[B] = {0.0, 0.0, 0.0};
+  double sq[B] = {};
   if (sok) {
     T sum[B];
     sell_dot_diag<B, NT, T, T, T>(MV(PT) + e0 + lane, PT.col + e0 + lane, w, bf, sdo, PT.dtab, rb, sum);
@@ -881,13 +892,12 @@ __global__ __launch_bounds__(kBlock) void k_restrict_b3(DSell<T> PT, int nsl,
     for (int i = 0; i < B; ++i) part[((size_t)col_ * B + i) * gridDim.x + sb_] = sq[i];
 }
 
-template <typename T, int CADD, int NT>
-__global__ __launch_bounds__(kBlock) void k_interp_b3(DSell<T> P, int nsl,
+template <int B, typename T, int CADD, int NT>
+__global__ __launch_bounds__(kBlock) void k_interp_bk(DSell<T> P, int nsl,
                                                        const uint32_t *__restrict__ fskip,
                                                        const T *__restrict__ cc, T damp,
                                                        T *__restrict__ t, T *__restrict__ c,
                                                        size_t vsf, size_t vsc) {
-  constexpr int B = 3;
   SLICE_PROLOGUE_MAP(P, 0, nsl, true)
   cc += (size_t)col_ * vsc;
   const size_t o = (size_t)col_ * vsf;
@@ -905,15 +915,14 @@ __global__ __launch_bounds__(kBlock) void k_interp_b3(DSell<T> P, int nsl,
 }
 
 // partial sums per component: part[(column * 3 + i) * gridDim.x + block]
-template <typename T, int DOT>
-__global__ __launch_bounds__(kBlock) void k_dot_part_b3(int n, const T *__restrict__ x,
+template <int B, typename T, int DOT>
+__global__ __launch_bounds__(kBlock) void k_dot_part_bk(int n, const T *__restrict__ x,
                                                          const T *__restrict__ y,
                                                          const uint8_t *__restrict__ vclass,
                                                          double *__restrict__ part, size_t vs) {
-  constexpr int B = 3;
   __shared__ double red[B][kBlock / 64];
   const size_t o = (size_t)blockIdx.y * vs;
-  double s[B] = {0.0, 0.0, 0.0};
+  double s[B] = {};
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     if (vclass && vclass[i] < 2) continue;
 #pragma unroll
@@ -967,8 +976,8 @@ __device__ __forceinline__ double base_row_or_unit(const TI *__restrict__ inv,
 
 // ls over ex on the base level with per-component stop tests (sc_cmp:
 // every component below its limit); inv is (3n) x (3n)
-template <typename T, typename TI>
-__global__ __launch_bounds__(kBlock) void k_base_gemv_b3(int n3, const TI *__restrict__ inv,
+template <int B, typename T, typename TI>
+__global__ __launch_bounds__(kBlock) void k_base_gemv_bk(int n3, const TI *__restrict__ inv,
                                                           const int *__restrict__ rowmap,
                                                           const int *__restrict__ cp, int ncols,
                                                           const T *__restrict__ b,
@@ -977,7 +986,6 @@ __global__ __launch_bounds__(kBlock) void k_base_gemv_b3(int n3, const TI *__res
                                                           double red, const double *__restrict__ part,
                                                           int nparts, double *__restrict__ first,
                                                           int it, size_t vs, size_t ims) {
-  constexpr int B = 3;
   const int row = (blockIdx.x * kBlock + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   const int cj = blockIdx.y;
@@ -1070,6 +1078,8 @@ void launch_galerkin(hipStream_t st, int nc, int bb, const int *ptp, const int *
   if (nc <= 0) return;
   if (bb == 9)
     k_galerkin<9><<<(nc + kBlock - 1) / kBlock, kBlock, 0, st>>>(nc, ptp, pti, ptw, frp, fci, fa, pp, pi, pw, crp, cci, ca);
+  else if (bb == 4)
+    k_galerkin<4><<<(nc + kBlock - 1) / kBlock, kBlock, 0, st>>>(nc, ptp, pti, ptw, frp, fci, fa, pp, pi, pw, crp, cci, ca);
   else
     k_galerkin<1><<<(nc + kBlock - 1) / kBlock, kBlock, 0, st>>>(nc, ptp, pti, ptw, frp, fci, fa, pp, pi, pw, crp, cci, ca);
 }
@@ -1283,6 +1293,18 @@ static dim3 slab_grid(const DSell<TV> &S, int slice0, int nsl, int nrhs) {
     }                             \
   } while (0)
 
+// block vectors: BK = 2 or 3 components
+#define BK_DISPATCH(BB, ...)   \
+  do {                         \
+    if ((BB) == 3) {           \
+      constexpr int BK = 3;    \
+      __VA_ARGS__;             \
+    } else {                   \
+      constexpr int BK = 2;    \
+      __VA_ARGS__;             \
+    }                          \
+  } while (0)
+
 #define NT_DISPATCH(nt, ...) \
   do {                       \
     if (nt) {                \
@@ -1299,13 +1321,13 @@ void launch_spmv(hipStream_t st, const DSell<TV> &A, int slice0, int nsl, const 
                  TY *c, int mode, bool nt, Batch bt) {
   if (nsl <= 0) return;
   const dim3 g = slab_grid(A, slice0, nsl, bt.nrhs), blk(kBlock);
-  if (bt.b == 3) {
-    NT_DISPATCH(nt, switch (mode) {
-      case 0: k_spmv_b3<TV, TX, TY, 0, NTV><<<g, blk, 0, st>>>(A, slice0, nsl, x, y, c, bt.vs); break;
-      case 1: k_spmv_b3<TV, TX, TY, 1, NTV><<<g, blk, 0, st>>>(A, slice0, nsl, x, y, c, bt.vs); break;
-      case 2: k_spmv_b3<TV, TX, TY, 2, NTV><<<g, blk, 0, st>>>(A, slice0, nsl, x, y, c, bt.vs); break;
-      default: k_spmv_b3<TV, TX, TY, 3, NTV><<<g, blk, 0, st>>>(A, slice0, nsl, x, y, c, bt.vs); break;
-    });
+  if (bt.b > 1) {
+    BK_DISPATCH(bt.b, NT_DISPATCH(nt, switch (mode) {
+      case 0: k_spmv_bk<BK, TV, TX, TY, 0, NTV><<<g, blk, 0, st>>>(A, slice0, nsl, x, y, c, bt.vs); break;
+      case 1: k_spmv_bk<BK, TV, TX, TY, 1, NTV><<<g, blk, 0, st>>>(A, slice0, nsl, x, y, c, bt.vs); break;
+      case 2: k_spmv_bk<BK, TV, TX, TY, 2, NTV><<<g, blk, 0, st>>>(A, slice0, nsl, x, y, c, bt.vs); break;
+      default: k_spmv_bk<BK, TV, TX, TY, 3, NTV><<<g, blk, 0, st>>>(A, slice0, nsl, x, y, c, bt.vs); break;
+    }));
     return;
   }
   W_DISPATCH(A.wmax, NT_DISPATCH(nt, switch (mode) {
@@ -1340,9 +1362,9 @@ void launch_ir_update(hipStream_t st, const DSell<double> &A, const float *c32, 
   const int g = nblk_slices(A.nslices);   // partial sums: one per slice block
   *nparts = g;
   if (g == 0) return;
-  if (bt.b == 3) {
-    NT_DISPATCH(nt, k_ir_update_b3<NTV><<<grid2(g, bt.nrhs), kBlock, 0, st>>>(
-                        A, A.nslices, c32, r, x, r32, vclass, part, bt.vs));
+  if (bt.b > 1) {
+    BK_DISPATCH(bt.b, NT_DISPATCH(nt, k_ir_update_bk<BK, NTV><<<grid2(g, bt.nrhs), kBlock, 0, st>>>(
+                                          A, A.nslices, c32, r, x, r32, vclass, part, bt.vs)));
     return;
   }
   W_DISPATCH(A.wmax, NT_DISPATCH(nt, k_ir_update<MWV, NTV><<<slab_grid(A, 0, A.nslices, bt.nrhs), kBlock, 0, st>>>(
@@ -1353,7 +1375,9 @@ template <typename T>
 void launch_jac(hipStream_t st, int n, const T *b, const T *diag, const uint8_t *vclass, T damp,
                 T *t, Batch bt) {
   if (n <= 0) return;
-  if (bt.b == 3) k_jac_b3<T><<<grid2(nblk(n), bt.nrhs), kBlock, 0, st>>>(n, b, diag, vclass, damp, t, bt.vs, bt.dms);
+  if (bt.b > 1)
+    BK_DISPATCH(bt.b, k_jac_bk<BK, T><<<grid2(nblk(n), bt.nrhs), kBlock, 0, st>>>(n, b, diag, vclass, damp, t,
+                                                                                 bt.vs, bt.dms));
   else k_jac<T><<<grid2(nblk(n), bt.nrhs), kBlock, 0, st>>>(n, b, diag, vclass, damp, t, bt.vs, bt.dms);
 }
 template void launch_jac<double>(hipStream_t, int, const double *, const double *,
@@ -1391,9 +1415,9 @@ template <typename TV, typename T>
 void launch_gs_color(hipStream_t st, const DSell<TV> &L, int slice0, int nsl, int wmax, const T *b,
                      const T *diag, const uint8_t *vclass, T *v, bool nt, Batch bt) {
   if (nsl <= 0) return;
-  if (bt.b == 3) {
-    GSB_DISPATCH(nsl, NT_DISPATCH(nt, k_gs_color_b3<TV, T, NTV, BSV><<<g, BSV, 0, st>>>(
-                                          L, slice0, nsl, b, diag, vclass, v, bt.vs)));
+  if (bt.b > 1) {
+    BK_DISPATCH(bt.b, GSB_DISPATCH(nsl, NT_DISPATCH(nt, k_gs_color_bk<BK, TV, T, NTV, BSV><<<g, BSV, 0, st>>>(
+                                                            L, slice0, nsl, b, diag, vclass, v, bt.vs))));
     return;
   }
   if constexpr (!std::is_same_v<T, double>) {
@@ -1451,11 +1475,11 @@ void launch_gs_color_fused(hipStream_t st, const DSell<TV> &L, const DSell<TV> &
                            int nsl, int wmax_l, int wmax_u, T *b, const T *diag, T damp, T *v,
                            T *c, int cmode, bool nt, Batch bt, bool split) {
   if (nsl <= 0) return;
-  if (bt.b == 3) {
-    GSB_DISPATCH(nsl, NT_DISPATCH(nt, if (cmode == 0) k_gs_color_fused_b3<TV, T, 0, NTV, BSV><<<g, BSV, 0, st>>>(
+  if (bt.b > 1) {
+    BK_DISPATCH(bt.b, GSB_DISPATCH(nsl, NT_DISPATCH(nt, if (cmode == 0) k_gs_color_fused_bk<BK, TV, T, 0, NTV, BSV><<<g, BSV, 0, st>>>(
                         L, U, slice0, nsl, b, diag, damp, v, c, bt.vs);
-                else k_gs_color_fused_b3<TV, T, 1, NTV, BSV><<<g, BSV, 0, st>>>(
-                        L, U, slice0, nsl, b, diag, damp, v, c, bt.vs)));
+                else k_gs_color_fused_bk<BK, TV, T, 1, NTV, BSV><<<g, BSV, 0, st>>>(
+                        L, U, slice0, nsl, b, diag, damp, v, c, bt.vs))));
     return;
   }
   // one chunk per part sized to the colour's widest row (a colour's upper
@@ -1502,11 +1526,11 @@ void launch_gs_defect(hipStream_t st, const DSell<TV> &U, const T *v, T *b, cons
                       T damp, T *c, int cmode, T *xout, bool nt, Batch bt) {
   if (U.nslices <= 0) return;
   const dim3 g = slab_grid(U, 0, U.nslices, bt.nrhs);
-  if (bt.b == 3) {
-    NT_DISPATCH(nt, if (cmode == 0) k_gs_defect_b3<TV, T, 0, NTV><<<g, kBlock, 0, st>>>(
+  if (bt.b > 1) {
+    BK_DISPATCH(bt.b, NT_DISPATCH(nt, if (cmode == 0) k_gs_defect_bk<BK, TV, T, 0, NTV><<<g, kBlock, 0, st>>>(
                         U, U.nslices, v, b, vclass, damp, c, xout, bt.vs);
-                else k_gs_defect_b3<TV, T, 1, NTV><<<g, kBlock, 0, st>>>(
-                        U, U.nslices, v, b, vclass, damp, c, xout, bt.vs));
+                else k_gs_defect_bk<BK, TV, T, 1, NTV><<<g, kBlock, 0, st>>>(
+                        U, U.nslices, v, b, vclass, damp, c, xout, bt.vs)));
     return;
   }
   // chunks of 8 entries: the upper widths of the colours run from the whole
@@ -1532,9 +1556,9 @@ int launch_restrict(hipStream_t st, const DSell<T> &PT, const uint32_t *cskip,
                     size_t cdms) {
   if (PT.nslices <= 0) return 0;
   const int g = nblk_slices(PT.nslices);
-  if (b == 3) {
-    NT_DISPATCH(nt, k_restrict_b3<T, NTV><<<grid2(g, nrhs), kBlock, 0, st>>>(
-                        PT, PT.nslices, cskip, cvnclass, cvclass, bf, damp, bc, part, vsf, vsc));
+  if (b > 1) {
+    BK_DISPATCH(b, NT_DISPATCH(nt, k_restrict_bk<BK, T, NTV><<<grid2(g, nrhs), kBlock, 0, st>>>(
+                                       PT, PT.nslices, cskip, cvnclass, cvclass, bf, damp, bc, part, vsf, vsc)));
     return g;
   }
   W_DISPATCH(PT.wmax, NT_DISPATCH(nt, k_restrict<MWV, T, NTV><<<grid2(g, nrhs), kBlock, 0, st>>>(
@@ -1641,11 +1665,11 @@ void launch_base_gemv(hipStream_t st, int n, const TI *inv, const int *rowmap, c
                       double abslimit, double red, const double *part, int nparts, double *first,
                       int it, Batch bt) {
   if (n <= 0) return;
-  if (bt.b == 3) {   // n counts scalar unknowns; one wavefront per row
+  if (bt.b > 1) {   // n counts scalar unknowns; one wavefront per row
     const int g = (n * 64 + kBlock - 1) / kBlock;
-    k_base_gemv_b3<T, TI><<<grid2(g, bt.nrhs), kBlock, 0, st>>>(n, inv, rowmap, cp, ncols, b, c,
-                                                                xout, cmode, damp, abslimit, red,
-                                                                part, nparts, first, it, bt.vs, bt.ims);
+    BK_DISPATCH(bt.b, k_base_gemv_bk<BK, T, TI><<<grid2(g, bt.nrhs), kBlock, 0, st>>>(
+                          n, inv, rowmap, cp, ncols, b, c, xout, cmode, damp, abslimit, red, part, nparts,
+                          first, it, bt.vs, bt.ims));
   } else {           // one workgroup per row
     k_base_gemv<T, TI><<<grid2(n, bt.nrhs), kBlock, 0, st>>>(n, inv, rowmap, cp, ncols, b, bc, c, xout,
                                                              cmode, damp, abslimit, red, part,
@@ -1708,7 +1732,9 @@ template <typename T>
 int launch_dot_part(hipStream_t st, int n, const T *x, const uint8_t *vclass, double *part,
                     Batch bt) {
   const int g = reduce_parts(n);
-  if (bt.b == 3) k_dot_part_b3<T, 0><<<grid2(g, bt.nrhs), kBlock, 0, st>>>(n, x, nullptr, vclass, part, bt.vs);
+  if (bt.b > 1)
+    BK_DISPATCH(bt.b, k_dot_part_bk<BK, T, 0><<<grid2(g, bt.nrhs), kBlock, 0, st>>>(n, x, nullptr, vclass, part,
+                                                                                   bt.vs));
   else k_dot_part<T, 0><<<grid2(g, bt.nrhs), kBlock, 0, st>>>(n, x, nullptr, vclass, part, bt.vs);
   return g;
 }
@@ -1718,11 +1744,11 @@ void launch_interp(hipStream_t st, const DSell<T> &P, const uint32_t *fskip, con
                    T *t, T *c, bool nt, int nrhs, size_t vsf, size_t vsc, int b) {
   if (P.nslices <= 0) return;
   const dim3 g = grid2(nblk_slices(P.nslices), nrhs);
-  if (b == 3) {
-    NT_DISPATCH(nt, if (c) k_interp_b3<T, 1, NTV><<<g, kBlock, 0, st>>>(P, P.nslices, fskip, cc,
-                                                                        damp, t, c, vsf, vsc);
-                else k_interp_b3<T, 0, NTV><<<g, kBlock, 0, st>>>(P, P.nslices, fskip, cc, damp,
-                                                                  t, c, vsf, vsc));
+  if (b > 1) {
+    BK_DISPATCH(b, NT_DISPATCH(nt, if (c) k_interp_bk<BK, T, 1, NTV><<<g, kBlock, 0, st>>>(P, P.nslices, fskip, cc,
+                                                                                       damp, t, c, vsf, vsc);
+                               else k_interp_bk<BK, T, 0, NTV><<<g, kBlock, 0, st>>>(P, P.nslices, fskip, cc,
+                                                                                    damp, t, c, vsf, vsc)));
     return;
   }
   // interpolation rows hold 1, 2, 4 or 8 coarse corners: chunks of 4
@@ -1754,10 +1780,10 @@ void launch_dot(hipStream_t st, int n, const T *x, const T *y, const uint8_t *vc
                 double *part, double *out, int do_sqrt, Batch bt) {
   const int g = reduce_parts(n);
   const dim3 gg = grid2(g, bt.nrhs);
-  if (bt.b == 3) {
-    if (y) k_dot_part_b3<T, 1><<<gg, kBlock, 0, st>>>(n, x, y, vclass, part, bt.vs);
-    else k_dot_part_b3<T, 0><<<gg, kBlock, 0, st>>>(n, x, y, vclass, part, bt.vs);
-    k_reduce_final<<<gg.y * 3, 1024, 0, st>>>(g, part, out, do_sqrt);
+  if (bt.b > 1) {
+    BK_DISPATCH(bt.b, if (y) k_dot_part_bk<BK, T, 1><<<gg, kBlock, 0, st>>>(n, x, y, vclass, part, bt.vs);
+                      else k_dot_part_bk<BK, T, 0><<<gg, kBlock, 0, st>>>(n, x, y, vclass, part, bt.vs));
+    k_reduce_final<<<gg.y * bt.b, 1024, 0, st>>>(g, part, out, do_sqrt);
     return;
   }
   if (y) k_dot_part<T, 1><<<gg, kBlock, 0, st>>>(n, x, y, vclass, part, bt.vs);

@@ -506,8 +506,9 @@ def _classes(lv: Level, tri: np.ndarray, seed_nodes: np.ndarray) -> np.ndarray:
 
 
 def p1_local(nuni: int, depth: int, ncomp: int = 1) -> Problem:
-    """2D P1 -Laplace (ncomp = 3: the same stiffness times a fixed 3x3 SPD
-    coupling per entry) on the unit square, coarse grid 2x2 squares split
+    """2D P1 -Laplace (ncomp = 2 or 3: the same stiffness times the leading
+    ncomp x ncomp part of a fixed 3x3 SPD coupling per entry) on the unit
+    square, coarse grid 2x2 squares split
     into 8 triangles, refined uniformly nuni times and then `depth` more
     times only toward the corner at the origin: refinement step k marks the
     red (regular) triangles inside [0, 2^-k]^2.  Each step refines as

@@ -17,7 +17,7 @@ def test_libugmg_exports_every_declared_symbol():
     assert not missing, missing
 
 
-@pytest.mark.parametrize("header", ["ugmg_np.h", "ugmg_dio.h"])
+@pytest.mark.parametrize("header", ["ugmg_np.h", "ugmg_dio.h", "ugmg_mgio.h"])
 def test_libugmg_exports_the_numproc_and_data_file_symbols(header):
     names = ugmg.declared_symbols(ugmg.HEADER.replace("ugmg.h", header))
     assert names

@@ -55,6 +55,9 @@ def build_ugmg():
     obj = os.path.join(BUILD, "ugmg_dio.o")  # UG data files (host only)
     _run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, "ugmg_dio.cc"), "-o", obj])
     objs.append(obj)
+    obj = os.path.join(BUILD, "ugmg_mgio.o")  # UG multigrid files (host only)
+    _run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, "ugmg_mgio.cc"), "-o", obj])
+    objs.append(obj)
     out = os.path.join(PKG, "libugmg.so")
     tmp = out + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lgomp"])

@@ -237,6 +237,12 @@ class Context:
         self.nb = b
         self.nlevels = nlevels
 
+    def set_transfer(self, level, p_ptr, p_idx, p_w):
+        """ugmg_set_transfer: interpolation rows of `level` from level - 1."""
+        arrs = [np.ascontiguousarray(p_ptr, np.int32), np.ascontiguousarray(p_idx, np.int32),
+                np.ascontiguousarray(p_w, np.float64)]
+        self._chk(self.lib.ugmg_set_transfer(self.h, level, *[a.ctypes.data for a in arrs]))
+
     def set_params(self, **kw):
         self.params = default_params(**kw)
         self._chk(self.lib.ugmg_set_params(self.h, C.byref(self.params)))

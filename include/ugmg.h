@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define UGMG_MAXB 3
+#define UGMG_MAXB 4
 
 enum {
   UGMG_OK = 0,
@@ -110,8 +110,10 @@ int ugmg_sync(ugmg_ctx *ctx);
  * VECSKIP bits, vclass/vnclass = VCLASS/VNCLASS (gm/gm.h:2200-2288).
  * Interpolation rows: coarse corners of the father element with their GNs
  * weights (StandardIntCorNodeVector, transgrid.cc:235-336). */
-/* b: components per vector, 1, 2 or 3 (b x b blocks: Jacobi and GS with
- * SolveSmallBlock's n = 2 and n = 3 forms, per-component Dirichlet bits,
+/* b: components per vector, 1 .. UGMG_MAXB (b x b blocks: Jacobi and GS
+ * with SolveSmallBlock's n = 2 and n = 3 forms and, for n = 4, its Gauss
+ * elimination with row pivoting where |pivot| < 1e-10; per-component
+ * Dirichlet bits,
  * norms and stop tests; one damping factor and base limit for all
  * components; bcgs and the server take b = 1) */
 int ugmg_set_nlevels(ugmg_ctx *ctx, int nlevels, int b);

@@ -18,6 +18,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
 JAC, GS, MCGS = 0, 1, 2
+MAXB = 4        # ORC_MAXB of ugmg_oracle.h
 
 
 class OrcLevel(C.Structure):
@@ -30,8 +31,8 @@ class OrcLevel(C.Structure):
 class OrcParams(C.Structure):
     _fields_ = [("nu1", C.c_int), ("nu2", C.c_int), ("gamma", C.c_int), ("baselevel", C.c_int),
                 ("smoother", C.c_int),
-                ("damp", C.c_double * 3), ("lmgc_damp", C.c_double * 3), ("base_damp", C.c_double * 3),
-                ("base_abslimit", C.c_double * 3), ("base_red", C.c_double * 3),
+                ("damp", C.c_double * MAXB), ("lmgc_damp", C.c_double * MAXB), ("base_damp", C.c_double * MAXB),
+                ("base_abslimit", C.c_double * MAXB), ("base_red", C.c_double * MAXB),
                 ("base_maxiter", C.c_int), ("ex_reorder", C.c_int)]
 
 
@@ -113,7 +114,7 @@ def params(nu1=2, nu2=2, gamma=1, baselevel=0, smoother=JAC, damp=0.8, lmgc_damp
            base_damp=1.0, base_abslimit=1e-300, base_red=1e-12, base_maxiter=1, ex_reorder=1):
     p = OrcParams()
     p.nu1, p.nu2, p.gamma, p.baselevel, p.smoother = nu1, nu2, gamma, baselevel, smoother
-    for i in range(3):
+    for i in range(MAXB):
         p.damp[i] = damp
         p.lmgc_damp[i] = lmgc_damp
         p.base_damp[i] = base_damp
@@ -137,13 +138,13 @@ def dmatmul(H, l, y):
 
 
 def dnrm2x(H, l, x):
-    out = np.zeros(3)
+    out = np.zeros(MAXB)
     lib().orc_dnrm2x(H.ptr(l), np.ascontiguousarray(x, np.float64).ctypes.data, out.ctypes.data)
     return out[:H.levels[l].b]
 
 
 def ddot(H, l, x, y):
-    out = np.zeros(3)
+    out = np.zeros(MAXB)
     lib().orc_ddot(H.ptr(l), np.ascontiguousarray(x, np.float64).ctypes.data,
                    np.ascontiguousarray(y, np.float64).ctypes.data, out.ctypes.data)
     return out[:H.levels[l].b]
@@ -175,7 +176,7 @@ def smoother(H, l, kind, damp, b, rank=None):
     lv = H.levels[l]
     x = np.zeros(lv.n * lv.b)
     b = np.array(b, dtype=np.float64, copy=True)
-    dv = np.full(3, damp, dtype=np.float64) if np.isscalar(damp) else np.asarray(damp, np.float64)
+    dv = np.full(MAXB, damp, dtype=np.float64) if np.isscalar(damp) else np.asarray(damp, np.float64)
     rk = None if rank is None else np.ascontiguousarray(rank, np.int32)
     rc = lib().orc_smoother(H.ptr(l), kind, _p(rk), dv.ctypes.data, x.ctypes.data, b.ctypes.data)
     if rc:
@@ -186,7 +187,7 @@ def smoother(H, l, kind, damp, b, rank=None):
 def restrict(H, l, bf, damp=1.0, to_c=None):
     lc = H.levels[l - 1]
     out = np.zeros(lc.n * lc.b) if to_c is None else np.array(to_c, dtype=np.float64, copy=True)
-    dv = np.full(3, damp, dtype=np.float64)
+    dv = np.full(MAXB, damp, dtype=np.float64)
     lib().orc_restrict(H.ptr(l), H.ptr(l - 1), out.ctypes.data, np.ascontiguousarray(bf, np.float64).ctypes.data,
                        dv.ctypes.data)
     return out
@@ -195,7 +196,7 @@ def restrict(H, l, bf, damp=1.0, to_c=None):
 def interpolate(H, l, cc, damp=1.0):
     lf = H.levels[l]
     out = np.zeros(lf.n * lf.b)
-    dv = np.full(3, damp, dtype=np.float64)
+    dv = np.full(MAXB, damp, dtype=np.float64)
     lib().orc_interpolate(H.ptr(l), H.ptr(l - 1), out.ctypes.data, np.ascontiguousarray(cc, np.float64).ctypes.data,
                           dv.ctypes.data)
     return out
@@ -221,7 +222,7 @@ class Ex:
         lv = self.H.levels[self.l]
         x = np.zeros(lv.n * lv.b)
         b = np.array(b, dtype=np.float64, copy=True)
-        dv = np.full(3, damp, dtype=np.float64)
+        dv = np.full(MAXB, damp, dtype=np.float64)
         rc = lib().orc_ex_smoother(self.h, self.H.ptr(self.l), dv.ctypes.data, x.ctypes.data, b.ctypes.data)
         return rc, x, b
 
@@ -247,8 +248,8 @@ def ls_solve(H, top, p, rhs, x0=None, red=1e-10, abslimit=1e-300, maxiter=50, do
     nb = lv.b
     x = np.zeros(lv.n * nb) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
     b = np.array(rhs, dtype=np.float64, copy=True)
-    ab = np.full(3, abslimit)
-    rd = np.full(3, red)
+    ab = np.full(MAXB, abslimit)
+    rd = np.full(MAXB, red)
     defects = np.zeros((maxiter + 1) * nb)
     nit, conv = C.c_int(), C.c_int()
     rc = lib().orc_ls_lmgc_solve(H.ptr(0), top, C.byref(p), x.ctypes.data, b.ctypes.data, int(do_defect),
@@ -275,8 +276,8 @@ def ls_solve_surface(H, top, p, bs, xs=None, red=1e-10, abslimit=1e-300, maxiter
     bs = [None if l < bl else np.array(bs[l], dtype=np.float64, copy=True) for l in range(top + 1)]
     XP = (C.c_void_p * (top + 1))(*[None if v is None else v.ctypes.data for v in xs])
     BP = (C.c_void_p * (top + 1))(*[None if v is None else v.ctypes.data for v in bs])
-    ab = np.full(3, abslimit)
-    rd = np.full(3, red)
+    ab = np.full(MAXB, abslimit)
+    rd = np.full(MAXB, red)
     defects = np.zeros((maxiter + 1) * nb)
     nit, conv = C.c_int(), C.c_int()
     rc = lib().orc_ls_lmgc_solve_surface(H.ptr(0), top, C.byref(p), XP, BP, int(do_defect),
@@ -296,7 +297,7 @@ def bcgs_solve(H, top, p, rhs, x0=None, red=1e-8, abslimit=1e-300, maxiter=200, 
     nb = lv.b
     x = np.zeros(lv.n * nb) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
     b = np.array(rhs, dtype=np.float64, copy=True)
-    ab, rd = np.full(3, abslimit), np.full(3, red)
+    ab, rd = np.full(MAXB, abslimit), np.full(MAXB, red)
     maxdef = 2 * maxiter + 1
     defects = np.zeros(maxdef * nb)
     nd, nit, conv = C.c_int(), C.c_int(), C.c_int()

@@ -40,7 +40,7 @@
 void orc_dmatmul_minus(const orc_level *L, double *x, const double *y) {
   const int b = L->b, bb = b * b;
   for (int v = 0; v < L->n; ++v) {
-    double s[ORC_MAXB] = {0.0, 0.0, 0.0};
+    double s[ORC_MAXB] = {0.0};
     for (int m = L->rp[v]; m < L->rp[v + 1]; ++m) {
       const double *M = L->a + (size_t)m * bb;
       const double *w = y + (size_t)L->ci[m] * b;
@@ -60,7 +60,7 @@ void orc_dmatmul(const orc_level *L, double *x, const double *y) {
   /* x = 0; x += M y  (matmode.ct:76-78) */
   const int bb = b * b;
   for (int v = 0; v < L->n; ++v) {
-    double s[ORC_MAXB] = {0.0, 0.0, 0.0};
+    double s[ORC_MAXB] = {0.0};
     for (int m = L->rp[v]; m < L->rp[v + 1]; ++m) {
       const double *M = L->a + (size_t)m * bb;
       const double *w = y + (size_t)L->ci[m] * b;
@@ -76,7 +76,7 @@ void orc_dmatmul(const orc_level *L, double *x, const double *y) {
 
 void orc_dnrm2x(const orc_level *L, const double *x, double *out) {
   const int b = L->b;
-  double a[ORC_MAXB] = {0.0, 0.0, 0.0};
+  double a[ORC_MAXB] = {0.0};
   for (int v = 0; v < L->n; ++v)
     for (int i = 0; i < b; ++i) {
       double s = x[(size_t)v * b + i];
@@ -88,7 +88,7 @@ void orc_dnrm2x(const orc_level *L, const double *x, double *out) {
 void orc_ddot(const orc_level *L, const double *x, const double *y,
               double *out) {
   const int b = L->b;
-  double a[ORC_MAXB] = {0.0, 0.0, 0.0};
+  double a[ORC_MAXB] = {0.0};
   for (int v = 0; v < L->n; ++v)
     for (int i = 0; i < b; ++i)
       a[i] += x[(size_t)v * b + i] * y[(size_t)v * b + i];
@@ -98,7 +98,7 @@ void orc_ddot(const orc_level *L, const double *x, const double *y,
 /* surface norm: top level, NEW_DEFECT vectors */
 static void surface_nrm2x(const orc_level *L, const double *x, double *out) {
   const int b = L->b;
-  double a[ORC_MAXB] = {0.0, 0.0, 0.0};
+  double a[ORC_MAXB] = {0.0};
   for (int v = 0; v < L->n; ++v) {
     if (L->vclass[v] < NEWDEF_CLASS) continue;
     for (int i = 0; i < b; ++i) {
@@ -114,7 +114,7 @@ static void surface_dmatmul_minus(const orc_level *L, double *x,
   const int b = L->b, bb = b * b;
   for (int v = 0; v < L->n; ++v) {
     if (L->vclass[v] < NEWDEF_CLASS) continue;
-    double s[ORC_MAXB] = {0.0, 0.0, 0.0};
+    double s[ORC_MAXB] = {0.0};
     for (int m = L->rp[v]; m < L->rp[v + 1]; ++m) {
       const double *M = L->a + (size_t)m * bb;
       const double *w = y + (size_t)L->ci[m] * b;
@@ -146,10 +146,16 @@ static int sc_cmp(int b, const double *x, const double *y) {
 
 /* ---------------------------------------------------------- small blocks */
 
+/* MAX_SINGLE_VEC_COMP (np/udm/udm.h:62): UG refuses blocks of 40 and more */
+#define ORC_MAX_SINGLE_VEC_COMP 40
+/* SMALL_D = DBL_EPSILON * SMALL_FAC (low/architecture.h:27,34) */
+#define ORC_SMALL_D (2.2204460492503131e-16 * 10)
+
 int orc_solve_small_block(int n, const double *mat, const double *rhs_in,
                           double *sol) {
-  double rhs[8], M[64], X[8];
-  if (n >= 8) return 1;
+  double rhs[ORC_MAX_SINGLE_VEC_COMP], M[ORC_MAX_SINGLE_VEC_COMP * ORC_MAX_SINGLE_VEC_COMP],
+      X[ORC_MAX_SINGLE_VEC_COMP];
+  if (n >= ORC_MAX_SINGLE_VEC_COMP) return 1;
   for (int i = 0; i < n; ++i) rhs[i] = rhs_in[i];
   switch (n) {
     case 1:
@@ -184,7 +190,7 @@ int orc_solve_small_block(int n, const double *mat, const double *rhs_in,
           int pivrow = i;
           for (int j = i + 1; j < n; ++j)
             if (fabs(M[j * n + i]) > fabs(piv)) { piv = M[j * n + i]; pivrow = j; }
-          if (fabs(piv) < 2.2204460492503131e-16 * 1e-10) return 1;
+          if (fabs(piv) < ORC_SMALL_D * 1e-10) return 1;
           for (int k = i; k < n; ++k) {
             double f = M[i * n + k]; M[i * n + k] = M[pivrow * n + k]; M[pivrow * n + k] = f;
           }
@@ -263,7 +269,7 @@ int orc_l_lgs(const orc_level *L, const int *rank, double *v, const double *d) {
     } else {
       double s[ORC_MAXB];
       for (int k = 0; k < b; ++k) s[k] = d[(size_t)i * b + k];
-      double acc[ORC_MAXB] = {0.0, 0.0, 0.0};
+      double acc[ORC_MAXB] = {0.0};
       for (int m = L->rp[i] + 1; m < L->rp[i + 1]; ++m) {
         int w = L->ci[m];
         long wi = rank ? rank[w] : w;
@@ -562,7 +568,8 @@ static int lmgc(orc_mg *mg, int level) {
       return 1;
     dadd(len, mg->c[level], t);
   }
-  const double one[ORC_MAXB] = {1.0, 1.0, 1.0}; /* Factor_One */
+  double one[ORC_MAXB]; /* Factor_One */
+  for (int i = 0; i < ORC_MAXB; ++i) one[i] = 1.0;
   orc_restrict(L, &mg->lv[level - 1], mg->b[level - 1], mg->b[level], one);
   memset(mg->c[level - 1], 0, sizeof(double) * (size_t)mg->lv[level - 1].n * L->b);
   for (int g = 0; g < p->gamma; ++g)
@@ -713,7 +720,7 @@ static void composite_dmatmul_minus(const orc_level *levels, int frl, int tl,
     const int b = L->b, bb = b * b;
     for (int v = 0; v < L->n; ++v) {
       if (!on_surface(L, l, tl, v)) continue;
-      double s[ORC_MAXB] = {0.0, 0.0, 0.0};
+      double s[ORC_MAXB] = {0.0};
       for (int m = L->rp[v]; m < L->rp[v + 1]; ++m) {
         const double *M = L->a + (size_t)m * bb;
         const double *w = y[l] + (size_t)L->ci[m] * b;
@@ -733,7 +740,7 @@ static void composite_dmatmul_minus(const orc_level *levels, int frl, int tl,
 static void composite_nrm2x(const orc_level *levels, int frl, int tl,
                             double *const *x, double *out) {
   const int b = levels[tl].b;
-  double a[ORC_MAXB] = {0.0, 0.0, 0.0};
+  double a[ORC_MAXB] = {0.0};
   for (int l = frl; l <= tl; ++l) {
     const orc_level *L = &levels[l];
     for (int v = 0; v < L->n; ++v) {
@@ -800,7 +807,7 @@ int orc_ls_lmgc_solve_surface(const orc_level *levels, int top, const orc_params
 static double ddotw_surface(const orc_level *L, const double *x, const double *y) {
   /* ddotw (ugblas.cc:3023-3044) with unit weights, ON_SURFACE */
   const int b = L->b;
-  double a[ORC_MAXB] = {0.0, 0.0, 0.0};
+  double a[ORC_MAXB] = {0.0};
   for (int v = 0; v < L->n; ++v) {
     if (L->vclass[v] < NEWDEF_CLASS) continue;
     for (int i = 0; i < b; ++i) a[i] += x[(size_t)v * b + i] * y[(size_t)v * b + i];

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define ORC_MAXB 3
+#define ORC_MAXB 4
 
 typedef struct {
   int n;                  /* vectors on the level                         */

@@ -52,6 +52,19 @@ def test_params_default_are_numproc_defaults():
     assert p.damp[0] == 1.0 and p.ex_reorder == 1
 
 
+def test_block_limit_matches_the_headers():
+    hdr = open(ugmg.HEADER).read()
+    assert int(re.search(r"#define UGMG_MAXB (\d+)", hdr).group(1)) == ugmg.MAXB
+    ohdr = open(O.LIB_PATH.replace("liboracle.so", "ugmg_oracle.h")).read()
+    assert int(re.search(r"#define ORC_MAXB (\d+)", ohdr).group(1)) == O.MAXB
+    # every component written by ugmg_params_default, the fields after the arrays in place
+    p = ugmg.default_params()
+    assert list(p.damp) == [1.0] * ugmg.MAXB and list(p.base_red) == [1e-12] * ugmg.MAXB
+    assert (p.base_maxiter, p.ex_reorder, p.precision, p.inner_cycles, p.inner_adapt) == (1, 1, 0, 1, 1)
+    p = ugmg.default_params(damp=[0.5, 0.6])
+    assert list(p.damp) == [0.5] + [0.6] * (ugmg.MAXB - 1)
+
+
 def test_host_colouring_equals_oracle(poisson2d, jump3d_17):
     for P in (poisson2d, jump3d_17):
         H = O.Hierarchy(P.levels)

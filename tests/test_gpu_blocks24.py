@@ -1,12 +1,15 @@
-"""GPU parity of the 2x2-block path (b = 2: SolveSmallBlock's n = 2 form,
-np/algebra/block.cc:119-130) against the oracle, on the 2D P1 coupled
-system of problems.p1_local(.., ncomp=2): block SpMV and norms, Jacobi and
+"""GPU parity of the 2x2- and 4x4-block paths (SolveSmallBlock's n = 2 form
+and its Gauss elimination with the row exchange below |pivot| 1e-10,
+np/algebra/block.cc:119-199) against the oracle, on the 2D P1 coupled
+system of problems.p1_local(.., ncomp=b): block SpMV and norms, Jacobi and
 multicolour-GS steps, per-component transfer and the ex base solve, one
 lmgc cycle, ls + lmgc solves with per-component defect norms (Jacobi every
 iteration within 1e-10 relative; MC-GS defects against the oracle's
 colour-ordered restatement within 1e-9 and the final solution within 1e-8
 of lexicographic GS; FP64 and the mixed form), and the ON_SURFACE solve of
-a locally refined 2-component hierarchy.
+a locally refined hierarchy; 4x4 kernels on blocks that take the row
+exchange; and the singular blocks SolveSmallBlock refuses (n = 2: det 0;
+n = 4: pivot below SMALL_D 1e-10), refused at preprocess.
 """
 import numpy as np
 import pytest
@@ -33,13 +36,18 @@ def _rel(a, b):
     return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
 
 
-@pytest.fixture(scope="module")
-def sys2():
+# (0, 0) of every interior block far below 1e-10: the row exchange
+PIVOTING = np.array([[1e-12, 1.0, 0.2, 0.1], [1.0, 2.0, 0.3, 0.1], [0.2, 0.3, 2.0, 0.25], [0.1, 0.1, 0.25, 2.0]])
+
+
+@pytest.fixture(scope="module", params=[2, 4])
+def sys2(request):
     from ug_amd import problems
-    P = problems.p1_local(6, 0, 2)     # 129^2 nodes, 33,282 unknowns
+    b = request.param
+    P = problems.p1_local(6, 0, b)     # 129^2 nodes, 33,282 (b = 2) / 66,564 (b = 4) unknowns
     lv = P.levels[-1]
-    rhs = np.random.default_rng(0x5EED0301).uniform(-1, 1, 2 * lv.n)
-    rhs[np.repeat(lv.skip != 0, 2)] = 0.0
+    rhs = np.random.default_rng(0x5EED0301).uniform(-1, 1, b * lv.n)
+    rhs[np.repeat(lv.skip != 0, b)] = 0.0
     P.rhs = rhs
     return P
 
@@ -54,15 +62,15 @@ def _ctx(P, smoother, damp, baselevel=2, **kw):
     return ctx
 
 
-def test_block2_kernels(sys2):
+def _kernels(P, cycle=True):
     from ug_amd import ugmg
-    P = sys2
     H = O.Hierarchy(P.levels)
+    B = P.levels[-1].b
     cg, cj = _ctx(P, ugmg.SMOOTH_GS, 1.0), _ctx(P, ugmg.SMOOTH_JAC, 0.6)
     try:
         for l in range(3, P.top + 1):
             lv = P.levels[l]
-            n2 = 2 * lv.n
+            n2 = B * lv.n
             rp, ci, va = cg.pattern(l)
             assert np.array_equal(rp, lv.row_ptr) and np.array_equal(ci, lv.col_idx)
             assert np.array_equal(va, lv.vals)
@@ -83,7 +91,7 @@ def test_block2_kernels(sys2):
             xd, bd = _dev(np.zeros(n2)), _dev(b)
             cg.gs_step(l, xd, bd)
             assert _rel(_host(xd), xw) < 1e-12 and _rel(_host(bd), bw) < 1e-11, l
-            nc2 = 2 * P.levels[l - 1].n
+            nc2 = B * P.levels[l - 1].n
             bf = r.uniform(-1, 1, n2)
             cc = r.uniform(-1, 1, nc2)
             to_c = _dev(np.full(nc2, 5.0))
@@ -92,7 +100,9 @@ def test_block2_kernels(sys2):
             to_f = _dev(np.full(n2, 3.0))
             cg.interpolate(l, to_f, _dev(cc), damp=0.7)
             assert _rel(_host(to_f), O.interpolate(H, l, cc, damp=0.7)) < 1e-15, l
-        n2 = 2 * P.levels[2].n
+        if not cycle:
+            return
+        n2 = B * P.levels[2].n
         b = np.random.default_rng(60).uniform(-1, 1, n2)
         rc, xw, bw = O.Ex(H, 2, reorder=1).smooth(b)
         assert rc == 0
@@ -107,8 +117,50 @@ def test_block2_kernels(sys2):
         cj.close()
 
 
+def test_block_kernels(sys2):
+    _kernels(sys2)
+
+
+def test_block4_kernels_with_row_exchange():
+    from ug_amd import problems
+    P = problems.p1_local(5, 0, 4, coupling=PIVOTING)
+    lv = P.levels[-1]
+    inner = np.flatnonzero((lv.skip == 0) & (lv.vclass >= 3))
+    assert len(inner) and np.all(np.abs(lv.vals[lv.row_ptr[inner] * 16]) < 1e-10)
+    P.rhs = np.random.default_rng(0x5EED0304).uniform(-1, 1, 4 * lv.n)
+    _kernels(P, cycle=False)      # (the base level's band LU has no exchange: its tiny pivots are not compared)
+
+
+@pytest.mark.parametrize("b", [2, 4])
+def test_singular_diagonal_block_refused_at_preprocess(b):
+    from ug_amd import problems, ugmg
+    P = problems.p1_local(4, 0, b)
+    l = P.top - 1
+    lv = P.levels[l]
+    inner = np.flatnonzero((lv.skip == 0) & (lv.vclass >= 3))
+    i = int(inner[len(inner) // 3])
+    blk = lv.vals[lv.row_ptr[i] * b * b:(lv.row_ptr[i] + 1) * b * b].reshape(b, b)
+    if b == 2:
+        blk[1] = 2.0 * blk[0]                          # det 0
+    else:
+        blk[:, 0] = [1e-26, 2e-26, 0.0, 0.0]           # pivot below SMALL_D 1e-10 after the exchange
+    assert O.l_jac(O.Hierarchy(P.levels), l, np.ones(b * lv.n))[0] == 1
+    ctx = ugmg.Context(0)
+    try:
+        ctx.set_hierarchy(P)
+        ctx.set_params(nu1=2, nu2=2, baselevel=1, smoother=ugmg.SMOOTH_JAC, damp=0.6)
+        with pytest.raises(ugmg.UgmgError) as e:
+            ctx.preprocess()
+        assert e.value.code == -5 and f"vector {i} on level {l}" in str(e.value)
+        blk[:] = np.eye(b)                             # a regular block: the same context prepares
+        ctx.set_hierarchy(P)
+        ctx.preprocess()
+    finally:
+        ctx.close()
+
+
 @pytest.mark.parametrize("graph", [0, 1])
-def test_block2_jacobi_solve_defects_per_component(sys2, graph):
+def test_block_jacobi_solve_defects_per_component(sys2, graph):
     from ug_amd import ugmg
     P = sys2
     want = O.ls_solve(O.Hierarchy(P.levels), P.top, O.params(smoother=O.JAC, damp=0.6, baselevel=2), P.rhs,
@@ -119,13 +171,13 @@ def test_block2_jacobi_solve_defects_per_component(sys2, graph):
     finally:
         ctx.close()
     assert res.converged and res.number_of_linear_iterations == want["nit"]
-    assert d.shape[1] == 2
+    assert d.shape[1] == P.levels[-1].b
     assert np.max(np.abs(d - want["defects"]) / want["defects"]) < 1e-10
     assert _rel(x, want["x"]) < 1e-10
 
 
 @pytest.mark.parametrize("precision", [0, 1])
-def test_block2_gs_solve(sys2, precision):
+def test_block_gs_solve(sys2, precision):
     from ug_amd import ugmg
     P = sys2
     H = O.Hierarchy(P.levels)
@@ -137,16 +189,18 @@ def test_block2_gs_solve(sys2, precision):
     finally:
         ctx.close()
     assert res.converged
-    assert np.all(np.array(res.last_defect[:2]) < 1e-10 * np.array(res.first_defect[:2]))
+    b = P.levels[-1].b
+    assert np.all(np.array(res.last_defect[:b]) < 1e-10 * np.array(res.first_defect[:b]))
     assert np.max(np.abs(x - lex["x"])) < 1e-8 * np.max(np.abs(lex["x"]))
     if precision == 0:
         assert res.number_of_linear_iterations == mc["nit"]
         assert np.max(np.abs(d - mc["defects"]) / mc["defects"]) < 1e-9
 
 
-def test_block2_surface_solve():
+@pytest.mark.parametrize("b", [2, 4])
+def test_block_surface_solve(b):
     from ug_amd import problems, ugmg
-    P = problems.p1_local(5, 2, 2)
+    P = problems.p1_local(5, 2, b)
     bl = 1
     assert problems.fullrefinelevel(P) < P.top
     fs = problems.surface_rhs(P, bl, 0x5EED0302)
@@ -158,7 +212,7 @@ def test_block2_surface_solve():
     finally:
         ctx.close()
     assert res.number_of_linear_iterations == want["nit"]
-    assert d.shape == want["defects"].shape and d.shape[1] == 2
+    assert d.shape == want["defects"].shape and d.shape[1] == b
     assert np.all(np.abs(d - want["defects"]) <= 1e-10 * want["defects"])
     for l in range(bl, P.top + 1):
         assert _rel(xs[l], want["x"][l]) <= 1e-10, l

@@ -1,0 +1,108 @@
+"""CPU pins of the oracle on 2x2 and 4x4 blocks (SolveSmallBlock's n = 2
+form and its Gauss elimination for n >= 4, np/algebra/block.cc:119-199):
+the 2D P1 coupled system of problems.p1_local(.., ncomp=b) is the scalar
+stiffness times an SPD b x b coupling, and one lmgc cycle with block Jacobi
+or block MC-GS equals the textbook V-cycle operator built densely with
+kron(P, I_b) transfers masked per Dirichlet component; a block Jacobi step
+whose blocks need the row exchange equals the dense block solves.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from ug_amd import problems
+
+
+def _ops(P, l, kind, damp, H):
+    lv = P.levels[l]
+    b, n = lv.b, lv.n
+    A = lv.dense()
+    if kind == "jac":
+        blk = np.kron(np.eye(n), np.ones((b, b))).astype(bool)
+    else:
+        _, col = O.greedy_color(H, l)
+        rk = col.astype(np.int64) * n + np.arange(n)
+        blk = np.kron(rk[None, :] <= rk[:, None], np.ones((b, b))).astype(bool)
+    M = np.where(blk, A, 0.0)
+    return A, np.eye(b * n) - damp * np.linalg.solve(M, A)
+
+
+def mg_error(P, top, bl, kind, damp, nu1, nu2):
+    H = O.Hierarchy(P.levels)
+    b = P.levels[0].b
+    E = {bl: np.zeros((b * P.levels[bl].n,) * 2)}
+    for l in range(bl + 1, top + 1):
+        f, c = P.levels[l], P.levels[l - 1]
+        A, S = _ops(P, l, kind, damp, H)
+        Ac = c.dense()
+        Pb = np.kron(f.dense_p(c.n), np.eye(b))
+        fmask = ((f.skip[:, None] >> np.arange(b)) & 1).ravel() == 0
+        cmask = ((c.skip[:, None] >> np.arange(b)) & 1).ravel() == 0
+        CG = np.eye(A.shape[0]) - (Pb * fmask[:, None]) @ (np.eye(Ac.shape[0]) - E[l - 1]) @ \
+            np.linalg.solve(Ac, (Pb * cmask[None, :]).T @ A)
+        E[l] = np.linalg.matrix_power(S, nu2) @ CG @ np.linalg.matrix_power(S, nu1)
+    return E[top]
+
+
+COUPLING = {2: np.array([[2.0, 0.5], [0.5, 2.0]]),
+            4: np.array([[2.0, 0.5, 0.2, 0.1], [0.5, 2.0, 0.3, 0.15], [0.2, 0.3, 2.0, 0.25],
+                         [0.1, 0.15, 0.25, 2.0]])}
+# (0, 0) far below 1e-10 times any stiffness diagonal: every interior block
+# takes SolveSmallBlock's row exchange
+PIVOTING = np.array([[1e-12, 1.0, 0.2, 0.1], [1.0, 2.0, 0.3, 0.1], [0.2, 0.3, 2.0, 0.25], [0.1, 0.1, 0.25, 2.0]])
+
+
+@pytest.mark.parametrize("b", [2, 4])
+def test_coupled_system_is_stiffness_times_coupling(b):
+    P = problems.p1_local(2, 0, b)
+    s = problems.p1_local(2, 0, 1)
+    for a, sl in zip(P.levels, s.levels):
+        assert a.b == b
+        K = np.kron(sl.dense(), COUPLING[b])
+        free = np.repeat(sl.skip == 0, b)
+        assert np.max(np.abs(a.dense()[free] - K[free])) < 1e-14
+        assert np.array_equal(a.skip, np.where(sl.skip != 0, (1 << b) - 1, 0))
+
+
+@pytest.mark.parametrize("b", [2, 4])
+@pytest.mark.parametrize("kind,damp", [("jac", 0.6), ("mcgs", 1.0)])
+def test_block_lmgc_cycle_is_textbook_vcycle(b, kind, damp):
+    P = problems.p1_local(3, 0, b)
+    H = O.Hierarchy(P.levels)
+    top, bl = P.top, 1
+    E = mg_error(P, top, bl, kind, damp, 2, 2)
+    A = P.levels[top].dense()
+    n2 = A.shape[0]
+    rhs = np.random.default_rng(4).uniform(-1, 1, n2)
+    rhs[np.repeat(P.levels[top].skip != 0, b)] = 0.0
+    p = O.params(smoother=O.JAC if kind == "jac" else O.MCGS, damp=damp, baselevel=bl)
+    c, bn = O.lmgc_iter(H, top, p, rhs)
+    want = (np.eye(n2) - E) @ np.linalg.solve(A, rhs)
+    assert np.max(np.abs(c - want)) < 1e-11 * np.max(np.abs(want))
+    assert np.max(np.abs(bn - (rhs - A @ c))) < 1e-13 * np.max(np.abs(rhs))
+
+
+def test_block4_jacobi_step_with_row_exchange_is_dense_block_solve():
+    P = problems.p1_local(2, 0, 4, coupling=PIVOTING)
+    H = O.Hierarchy(P.levels)
+    lv = P.levels[-1]
+    A = lv.dense()
+    d = np.random.default_rng(7).uniform(-1, 1, 4 * lv.n)
+    rc, x = O.l_jac(H, P.top, d)
+    assert rc == 0
+    for i in range(lv.n):
+        s = slice(4 * i, 4 * i + 4)
+        want = np.zeros(4) if lv.vclass[i] < 3 else np.linalg.solve(A[s, s], d[s])
+        assert np.max(np.abs(x[s] - want)) < 1e-12 * max(1.0, np.max(np.abs(want)))
+    inner = np.flatnonzero((lv.skip == 0) & (lv.vclass >= 3))
+    assert len(inner) > 0 and np.all(np.abs(A[4 * inner, 4 * inner]) < 1e-10)   # the exchange taken
+
+
+def test_block4_singular_diagonal_block_is_refused():
+    P = problems.p1_local(2, 0, 4, coupling=PIVOTING)
+    lv = P.levels[-1]
+    i = int(np.flatnonzero((lv.skip == 0) & (lv.vclass >= 3))[0])
+    blk = lv.vals[lv.row_ptr[i] * 16:(lv.row_ptr[i] + 1) * 16].reshape(4, 4)
+    blk[:, 0] = [1e-26, 1e-26, 0.0, 0.0]              # column 0 below SMALL_D * 1e-10 = 2.2e-25
+    rc, _ = O.l_jac(O.Hierarchy(P.levels), P.top, np.ones(4 * lv.n))
+    assert rc == 1

@@ -191,6 +191,28 @@ def test_solve_small_block_pivot_and_singular():
     assert O.lib().orc_solve_small_block(2, S.ctypes.data, r.ctypes.data, sol.ctypes.data) == 1
 
 
+def test_solve_small_block_singular_threshold_is_small_d():
+    # column 0 at most 1.5e-25 after the exchange: below SMALL_D * 1e-10 =
+    # DBL_EPSILON * 10 * 1e-10 = 2.2e-25 (refused), above DBL_EPSILON * 1e-10
+    M = np.array([[1e-25, 1.0, 0, 0], [1.5e-25, 1.0, 1.0, 0], [0, 0, 2.0, 0], [0, 0, 0, 3.0]])
+    r, sol = np.ones(4), np.zeros(4)
+    assert O.lib().orc_solve_small_block(4, M.ctypes.data, r.ctypes.data, sol.ctypes.data) == 1
+    M[1, 0] = 3e-25                                   # above the threshold: exchanged and solved
+    assert O.lib().orc_solve_small_block(4, M.ctypes.data, r.ctypes.data, sol.ctypes.data) == 0
+    np.testing.assert_allclose(M @ sol, r, rtol=1e-12)
+
+
+def test_solve_small_block_size_limit():
+    # MAX_SINGLE_VEC_COMP = 40: blocks of 40 components and more are refused
+    rng = np.random.default_rng(39)
+    M = rng.uniform(-1, 1, (39, 39)) + 40 * np.eye(39)
+    r, sol = rng.uniform(-1, 1, 39), np.zeros(40)
+    assert O.lib().orc_solve_small_block(39, M.ctypes.data, r.ctypes.data, sol.ctypes.data) == 0
+    assert np.max(np.abs(sol[:39] - np.linalg.solve(M, r))) < 1e-13
+    M40 = np.eye(40)
+    assert O.lib().orc_solve_small_block(40, M40.ctypes.data, np.ones(40).ctypes.data, sol.ctypes.data) == 1
+
+
 @pytest.mark.parametrize("b", [1, 3])
 def test_l_jac_closed_form(b):
     lv = random_block_csr(30, b, 0.2, seed=10 + b)

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -1412,6 +1413,31 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
       for (double *v : {D.c, D.b, D.t, D.u}) HIPCHK(hipMemset(v, 0, sizeof(double) * nv));
       for (float *v : {D.c32, D.b32, D.t32, D.u32}) HIPCHK(hipMemset(v, 0, sizeof(float) * nv));
     }
+    // the diagonal blocks the smoothers solve on levels bl+1..level, refused
+    // where SolveSmallBlock would refuse them in the first step (l_jac,
+    // l_lgs: NUM_SMALL_DIAG); served groups fill theirs per member later
+    if ((ctx->b == 2 || ctx->b == 4) && !ctx->grouped && level > bl) {
+      int *bad = nullptr;
+      std::vector<int> badh(level - bl, INT_MAX);
+      if ((rc = upload(ctx, &bad, badh.data(), badh.size()))) return rc;
+      for (int l = bl + 1; l <= level; ++l)
+        launch_block_check(ctx->stream, ctx->dev[l].n, ctx->b, ctx->dev[l].diag64, ctx->dev[l].vclass,
+                           bad + (l - bl - 1));
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpyAsync(badh.data(), bad, sizeof(int) * badh.size(), hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+      for (int l = bl + 1; l <= level; ++l) {
+        const int r = badh[l - bl - 1];
+        if (r == INT_MAX) continue;
+        const DevLevel &D = ctx->dev[l];
+        const int v = D.perm_h.empty() ? r : D.perm_h[r];
+        char msg[160];
+        std::snprintf(msg, sizeof msg, "singular %dx%d diagonal block (SolveSmallBlock) of vector %d on level %d",
+                      ctx->b, ctx->b, v, l);
+        free_device(ctx);
+        return fail(ctx, UGMG_E_SINGULAR, msg);
+      }
+    }
     ctx->base = std::move(ps.base);
     if (!ctx->base.inv.empty() || ctx->base.n == 0) {   // factored on the host
       if ((rc = upload(ctx, &ctx->base_inv, ctx->base.inv.data(), ctx->base.inv.size()))) return rc;
@@ -1630,7 +1656,7 @@ int ugmg_sync(ugmg_ctx *ctx) {
 int ugmg_set_nlevels(ugmg_ctx *ctx, int nlevels, int b) {
   if (!ctx || nlevels < 1 || nlevels > 32) return ctx ? fail(ctx, UGMG_E_ARG, "nlevels must be 1..32") : UGMG_E_ARG;
   if (b < 1 || b > UGMG_MAXB)
-    return fail(ctx, UGMG_E_ARG, "block size must be 1 (scalar), 2 or 3 (2x2 / 3x3 blocks)");
+    return fail(ctx, UGMG_E_ARG, "block size must be 1 (scalar) .. 4 (b x b blocks)");
   free_device(ctx);
   for (int l = 0; l < (int)ctx->dcsr.size(); ++l) drop_dcsr(ctx, l);
   ctx->dcsr.assign(nlevels, ugmg_ctx::DevCsr());
@@ -2165,7 +2191,7 @@ int ugmg_solve(ugmg_ctx *ctx, int level, double *x, double *b, const double *abs
       if (!sc_cmp1(x[i], y[i])) return false;
     return true;
   };
-  double first[UGMG_MAXB] = {0.0, 0.0, 0.0}, target[UGMG_MAXB], last[UGMG_MAXB];
+  double first[UGMG_MAXB] = {}, target[UGMG_MAXB], last[UGMG_MAXB];
   if ((rc = surface_norm(ctx, level, b, first))) return rc;
   for (int i = 0; i < nc; ++i) {
     res->first_defect[i] = res->last_defect[i] = first[i];
@@ -2183,7 +2209,7 @@ int ugmg_solve(ugmg_ctx *ctx, int level, double *x, double *b, const double *abs
   // every component below 0.9 of the target (or of abslimit)
   const int kin = std::max(1, ctx->p.inner_cycles);
   const bool adapt = ctx->p.precision == UGMG_MIXED && ctx->p.inner_adapt && kin > 1;
-  double prev[UGMG_MAXB] = {0.0, 0.0, 0.0};
+  double prev[UGMG_MAXB] = {};
   int prev_cycles = 0;
   for (int k = 0; k < nc; ++k) last[k] = first[k];
   auto cycles_for = [&](const double *lim) {
@@ -2309,7 +2335,7 @@ void enqueue_surface_iteration(ugmg_ctx *ctx, const SurfaceRun &s, int cycles) {
 // level order
 void surface_norms(const ugmg_ctx *ctx, const SurfaceRun &s, const double *sq, double *out) {
   const int nc = ctx->dev[s.level].nb;
-  double acc[UGMG_MAXB] = {0.0, 0.0, 0.0};
+  double acc[UGMG_MAXB] = {};
   for (int l = s.frl; l <= s.level; ++l)
     for (int i = 0; i < nc; ++i) acc[i] += sq[(size_t)l * UGMG_MAXB + i];
   for (int i = 0; i < nc; ++i) out[i] = std::sqrt(acc[i]);
@@ -2494,7 +2520,7 @@ int ugmg_solve_surface(ugmg_ctx *ctx, int level, double *const *x, double *const
       if (!sc_cmp1(u[i], v[i])) return false;
     return true;
   };
-  double first[UGMG_MAXB] = {0.0, 0.0, 0.0}, target[UGMG_MAXB], last[UGMG_MAXB];
+  double first[UGMG_MAXB] = {}, target[UGMG_MAXB], last[UGMG_MAXB];
   surface_norms(ctx, s, ctx->surf_sq_host, first);
   for (int i = 0; i < nc; ++i) {
     res->first_defect[i] = res->last_defect[i] = first[i];
@@ -2511,7 +2537,7 @@ int ugmg_solve_surface(ugmg_ctx *ctx, int level, double *const *x, double *const
     }
   const int kin = std::max(1, ctx->p.inner_cycles);
   const bool adapt = mixed && ctx->p.inner_adapt && kin > 1;
-  double prev[UGMG_MAXB] = {0.0, 0.0, 0.0};
+  double prev[UGMG_MAXB] = {};
   int prev_cycles = 0;
   for (int k = 0; k < nc; ++k) last[k] = first[k];
   int i;

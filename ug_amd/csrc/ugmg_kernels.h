@@ -240,6 +240,10 @@ template <typename T>
 void launch_coarse_cycle(hipStream_t st, const CoarseArgs<T> &args, int base_n);
 // coarse values ca (pattern crp/cci, bb values per entry) = P^T A P; pt*:
 // every fine vector that interpolates from coarse row I, in list order
+// *bad = min(*bad, first vector of class >= 3 whose diagonal block
+// SolveSmallBlock refuses); b = 2 and 4 (UG checks no other size)
+void launch_block_check(hipStream_t st, int n, int b, const double *diag, const uint8_t *vclass,
+                        int *bad);
 void launch_galerkin(hipStream_t st, int nc, int bb, const int *ptp, const int *pti,
                      const double *ptw, const int *frp, const int *fci, const double *fa,
                      const int *pp, const int *pi, const double *pw, const int *crp,

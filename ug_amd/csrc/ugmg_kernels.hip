@@ -164,8 +164,14 @@ __device__ __forceinline__ void solve3(const T (&m)[9], const T (&r)[3], T (&s)[
   s[0] = (r[0] - m[1] * s[1] - m[2] * s[2]) / m[0];
 }
 
-// SolveSmallBlock (np/algebra/block.cc:119-142) for n = 2 (the inverse
-// determinant times the cofactor products, UG's operation order) and n = 3
+// SolveSmallBlock (np/algebra/block.cc:119-205) for n = 2 (the inverse
+// determinant times the cofactor products, UG's operation order), n = 3,
+// and n >= 4: Gauss elimination in which row i is exchanged with the row of
+// largest |entry| in column i only where |pivot| < 1e-10 (block.cc:148-186),
+// then back substitution.  Blocks UG would refuse (n = 2: det = 0; n >= 4:
+// |pivot| < SMALL_D 1e-10 after the exchange) are refused at preprocess
+// (k_block_check), so the solve here never meets them.  Fully unrolled: the
+// row exchange selects by compile-time index and the block stays in registers.
 template <int B, typename T>
 __device__ __forceinline__ void solve_small(const T (&m)[B * B], const T (&r)[B], T (&s)[B]) {
   if constexpr (B == 2) {
@@ -173,9 +179,53 @@ __device__ __forceinline__ void solve_small(const T (&m)[B * B], const T (&r)[B]
     det = (T)1 / det;
     s[0] = (r[0] * m[3] - r[1] * m[1]) * det;
     s[1] = (r[1] * m[0] - r[0] * m[2]) * det;
-  } else {
-    static_assert(B == 3, "blocks of 2 or 3 components");
+  } else if constexpr (B == 3) {
     solve3<T>(m, r, s);
+  } else {
+    T M[B * B], rh[B];
+#pragma unroll
+    for (int q = 0; q < B * B; ++q) M[q] = m[q];
+#pragma unroll
+    for (int q = 0; q < B; ++q) rh[q] = r[q];
+#pragma unroll
+    for (int i = 0; i < B - 1; ++i) {
+      T piv = M[i * B + i];
+      if (fabs(piv) < (T)1e-10) {
+        int pr = i;
+#pragma unroll
+        for (int j = i + 1; j < B; ++j)
+          if (fabs(M[j * B + i]) > fabs(piv)) {
+            piv = M[j * B + i];
+            pr = j;
+          }
+#pragma unroll
+        for (int j = i + 1; j < B; ++j)
+          if (j == pr) {
+#pragma unroll
+            for (int k = i; k < B; ++k) {
+              const T f = M[i * B + k];
+              M[i * B + k] = M[j * B + k];
+              M[j * B + k] = f;
+            }
+            const T f = rh[i];
+            rh[i] = rh[j];
+            rh[j] = f;
+          }
+      }
+#pragma unroll
+      for (int j = i + 1; j < B; ++j) {
+        const T f = M[j * B + i] / piv;
+#pragma unroll
+        for (int k = i + 1; k < B; ++k) M[j * B + k] -= M[i * B + k] * f;
+        rh[j] -= rh[i] * f;
+      }
+    }
+#pragma unroll
+    for (int i = B - 1; i >= 0; --i) {
+#pragma unroll
+      for (int j = i + 1; j < B; ++j) rh[i] -= M[i * B + j] * s[j];
+      s[i] = rh[i] / M[i * B + i];
+    }
   }
 }
 
@@ -682,8 +732,8 @@ __global__ __launch_bounds__(kBlock) void k_copy_rows(int ne, int b, const uint8
   if (e < ne && cls[e / b] >= 2) dst[e] = src[e];
 }
 
-// ================================================= 2x2 and 3x3 blocks
-// The same operations on vectors of B = 2 or 3 components with B x B matrix
+// ================================================= 2x2 .. 4x4 blocks
+// The same operations on vectors of B = 2, 3 or 4 components with B x B matrix
 // blocks (linear elasticity, coupled systems); diag holds the diagonal
 // block of every row.
 
@@ -914,6 +964,72 @@ __global__ __launch_bounds__(kBlock) void k_interp_bk(DSell<T> P, int nsl,
   }
 }
 
+// The blocks SolveSmallBlock refuses (np/algebra/block.cc:119-205; l_jac
+// and l_lgs then fail with NUM_SMALL_DIAG): n = 2 with det = 0, n >= 4 where
+// the elimination meets |pivot| < SMALL_D 1e-10 after the row exchange
+// (SMALL_D = DBL_EPSILON * 10, low/architecture.h:27,34).  n = 1 and n = 3
+// divide unchecked in UG and are not checked here.  One thread per vector of
+// class >= 3 (the ones the smoothers solve); *bad = the smallest failing
+// row.  FP64 without contraction: UG's products and differences.
+template <int B>
+__global__ __launch_bounds__(kBlock) void k_block_check(int n, const double *__restrict__ diag,
+                                                         const uint8_t *__restrict__ vclass,
+                                                         int *__restrict__ bad) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n || vclass[i] < 3) return;
+  double M[B * B];
+#pragma unroll
+  for (int q = 0; q < B * B; ++q) M[q] = diag[B * B * (size_t)i + q];
+  bool fails = false;
+  if constexpr (B == 2) {
+    fails = M[0] * M[3] - M[1] * M[2] == 0.0;
+  } else {
+    constexpr double kSmallD = 2.2204460492503131e-16 * 10;
+#pragma unroll
+    for (int c = 0; c < B - 1; ++c) {
+      double piv = M[c * B + c];
+      if (fabs(piv) < 1e-10) {
+        int pr = c;
+#pragma unroll
+        for (int j = c + 1; j < B; ++j)
+          if (fabs(M[j * B + c]) > fabs(piv)) {
+            piv = M[j * B + c];
+            pr = j;
+          }
+        if (fabs(piv) < kSmallD * 1e-10) fails = true;
+#pragma unroll
+        for (int j = c + 1; j < B; ++j)
+          if (j == pr)
+#pragma unroll
+            for (int k = c; k < B; ++k) {
+              const double f = M[c * B + k];
+              M[c * B + k] = M[j * B + k];
+              M[j * B + k] = f;
+            }
+      }
+      if (fails) break;
+#pragma unroll
+      for (int j = c + 1; j < B; ++j) {
+        const double f = M[j * B + c] / piv;
+#pragma unroll
+        for (int k = c + 1; k < B; ++k) M[j * B + k] -= M[c * B + k] * f;
+      }
+    }
+  }
+  if (fails) atomicMin(bad, i);
+}
+
+void launch_block_check(hipStream_t st, int n, int b, const double *diag, const uint8_t *vclass,
+                        int *bad) {
+  if (n <= 0) return;
+  const int g = (n + kBlock - 1) / kBlock;
+  if (b == 2)
+    k_block_check<2><<<g, kBlock, 0, st>>>(n, diag, vclass, bad);
+  else if (b == 4)
+    k_block_check<4><<<g, kBlock, 0, st>>>(n, diag, vclass, bad);
+}
+
 // partial sums per component: part[(column * 3 + i) * gridDim.x + block]
 template <int B, typename T, int DOT>
 __global__ __launch_bounds__(kBlock) void k_dot_part_bk(int n, const T *__restrict__ x,
@@ -1076,7 +1192,9 @@ void launch_galerkin(hipStream_t st, int nc, int bb, const int *ptp, const int *
                      const int *pp, const int *pi, const double *pw, const int *crp,
                      const int *cci, double *ca) {
   if (nc <= 0) return;
-  if (bb == 9)
+  if (bb == 16)
+    k_galerkin<16><<<(nc + kBlock - 1) / kBlock, kBlock, 0, st>>>(nc, ptp, pti, ptw, frp, fci, fa, pp, pi, pw, crp, cci, ca);
+  else if (bb == 9)
     k_galerkin<9><<<(nc + kBlock - 1) / kBlock, kBlock, 0, st>>>(nc, ptp, pti, ptw, frp, fci, fa, pp, pi, pw, crp, cci, ca);
   else if (bb == 4)
     k_galerkin<4><<<(nc + kBlock - 1) / kBlock, kBlock, 0, st>>>(nc, ptp, pti, ptw, frp, fci, fa, pp, pi, pw, crp, cci, ca);
@@ -1293,11 +1411,14 @@ static dim3 slab_grid(const DSell<TV> &S, int slice0, int nsl, int nrhs) {
     }                             \
   } while (0)
 
-// block vectors: BK = 2 or 3 components
+// block vectors: BK = 2, 3 or 4 components
 #define BK_DISPATCH(BB, ...)   \
   do {                         \
     if ((BB) == 3) {           \
       constexpr int BK = 3;    \
+      __VA_ARGS__;             \
+    } else if ((BB) == 4) {    \
+      constexpr int BK = 4;    \
       __VA_ARGS__;             \
     } else {                   \
       constexpr int BK = 2;    \

@@ -98,8 +98,7 @@ struct NumProc {
   Argv av;
   // ls / bcgs
   int m = 0, restart = 0, nrhs = 1;
-  double red[UGMG_MAXB] = {0, 0, 0}, abslimit[UGMG_MAXB] = {kAbsLimit, kAbsLimit, kAbsLimit};
-  double weight[UGMG_MAXB] = {1, 1, 1};
+  double red[UGMG_MAXB], abslimit[UGMG_MAXB], weight[UGMG_MAXB];
   std::string I, x, b, A, c, r;
   std::vector<ugmg_lresult> res;
   bool pending_defect = false;
@@ -109,8 +108,16 @@ struct NumProc {
   std::string T, pre, post, base;
   int gamma = 1, nu1 = 1, nu2 = 1, bl = 0, precision = UGMG_FP64, inner = 1, graph = 1, adapt = 1;
   // smoothers
-  double damp[UGMG_MAXB] = {1, 1, 1};
+  double damp[UGMG_MAXB];
   int reorder = 1, fmode = 0;
+
+  NumProc() {
+    for (int i = 0; i < UGMG_MAXB; ++i) {
+      red[i] = 0.0;
+      abslimit[i] = kAbsLimit;
+      weight[i] = damp[i] = 1.0;
+    }
+  }
 };
 
 bool is_iter(const std::string &c) { return c == "lmgc" || c == "jac" || c == "gs" || c == "ex"; }

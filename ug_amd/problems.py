@@ -447,8 +447,9 @@ def surface_load(P: Problem, bl: int, seed: int) -> list:
     return out
 
 
-# 3x3 coupling of the block variant (symmetric positive definite)
-_M3 = np.array([[2.0, 0.5, 0.2], [0.5, 2.0, 0.3], [0.2, 0.3, 2.0]])
+# coupling of the block variants (symmetric positive definite); ncomp
+# components take its leading ncomp x ncomp part
+_M4 = np.array([[2.0, 0.5, 0.2, 0.1], [0.5, 2.0, 0.3, 0.15], [0.2, 0.3, 2.0, 0.25], [0.1, 0.15, 0.25, 2.0]])
 
 
 def _p1_stiffness(xy: np.ndarray, tri: np.ndarray) -> np.ndarray:
@@ -460,7 +461,7 @@ def _p1_stiffness(xy: np.ndarray, tri: np.ndarray) -> np.ndarray:
     return (bx[:, :, None] * bx[:, None, :] + cx[:, :, None] * cx[:, None, :]) / (4.0 * area)[:, None, None]
 
 
-def _assemble_p1(xy: np.ndarray, tri: np.ndarray, ncomp: int):
+def _assemble_p1(xy: np.ndarray, tri: np.ndarray, ncomp: int, coupling=None):
     """Assembly over the elements in order (element matrices added as
     AddVlistMValues does), rows diagonal first then ascending columns, UG's
     Dirichlet rows (AssembleDirichletBoundary: row zeroed, diagonal block I)
@@ -488,7 +489,8 @@ def _assemble_p1(xy: np.ndarray, tri: np.ndarray, ncomp: int):
     if ncomp == 1:
         blocks = vals
     else:
-        blocks = vals[:, None, None] * _M3[None, :ncomp, :ncomp]
+        C = _M4[:ncomp, :ncomp] if coupling is None else np.asarray(coupling, np.float64)
+        blocks = vals[:, None, None] * C[None]
         blocks[bnd[rows]] = 0.0
         blocks[bnd[rows] & diag] = np.eye(ncomp)
         blocks = blocks.ravel()
@@ -505,9 +507,10 @@ def _classes(lv: Level, tri: np.ndarray, seed_nodes: np.ndarray) -> np.ndarray:
     return _propagate(lv, cls)
 
 
-def p1_local(nuni: int, depth: int, ncomp: int = 1) -> Problem:
-    """2D P1 -Laplace (ncomp = 2 or 3: the same stiffness times the leading
-    ncomp x ncomp part of a fixed 3x3 SPD coupling per entry) on the unit
+def p1_local(nuni: int, depth: int, ncomp: int = 1, coupling=None) -> Problem:
+    """2D P1 -Laplace (ncomp = 2 .. 4: the same stiffness times the leading
+    ncomp x ncomp part of a fixed 4x4 SPD coupling per entry, or times the
+    ncomp x ncomp `coupling` given) on the unit
     square, coarse grid 2x2 squares split
     into 8 triangles, refined uniformly nuni times and then `depth` more
     times only toward the corner at the origin: refinement step k marks the
@@ -582,7 +585,7 @@ def p1_local(nuni: int, depth: int, ncomp: int = 1) -> Problem:
     levels = []
     top = len(meshes) - 1
     for l, (xy_l, tri_l, kind_l) in enumerate(meshes):
-        rp, ci, vals, skip = _assemble_p1(xy_l, tri_l, ncomp)
+        rp, ci, vals, skip = _assemble_p1(xy_l, tri_l, ncomp, coupling)
         n = len(xy_l)
         lv = Level(n=n, b=ncomp, row_ptr=rp, col_idx=ci, vals=vals, skip=skip,
                    vclass=np.full(n, 3, np.uint8), vnclass=np.zeros(n, np.uint8))
@@ -604,7 +607,7 @@ def p1_local(nuni: int, depth: int, ncomp: int = 1) -> Problem:
         nn3[tri_l[refined].ravel()] = True                     # NNCLASS 3
         levels[l].vnclass = _classes(levels[l], tri_l, nn3)
     rhs = np.zeros(levels[-1].n * ncomp)
-    return Problem(dim=2, levels=levels, rhs=rhs, name=f"p1_local_{nuni}u{depth}l_b{ncomp}")
+    return Problem(dim=2, levels=levels, rhs=rhs, name=f"p1_local_{nuni}u{depth}l_b{ncomp}" + ("" if coupling is None else "_c"))
 
 
 def reordered(P: Problem, seed: int, newest_first: bool = True) -> Problem:

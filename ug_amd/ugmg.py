@@ -16,6 +16,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libugmg.so")
 HEADER = os.path.join(os.path.dirname(_HERE), "include", "ugmg.h")
+MAXB = 4        # UGMG_MAXB: components per vector, 1..MAXB
 
 SMOOTH_JAC, SMOOTH_GS = 0, 1
 FP64, MIXED = 0, 1
@@ -36,16 +37,16 @@ class UgmgError(RuntimeError):
 class Params(C.Structure):
     _fields_ = [("nu1", C.c_int), ("nu2", C.c_int), ("gamma", C.c_int), ("baselevel", C.c_int),
                 ("smoother", C.c_int),
-                ("damp", C.c_double * 3), ("lmgc_damp", C.c_double * 3), ("base_damp", C.c_double * 3),
-                ("base_abslimit", C.c_double * 3), ("base_red", C.c_double * 3),
+                ("damp", C.c_double * MAXB), ("lmgc_damp", C.c_double * MAXB), ("base_damp", C.c_double * MAXB),
+                ("base_abslimit", C.c_double * MAXB), ("base_red", C.c_double * MAXB),
                 ("base_maxiter", C.c_int), ("ex_reorder", C.c_int), ("precision", C.c_int),
                 ("inner_cycles", C.c_int), ("use_graph", C.c_int), ("ex_float", C.c_int),
                 ("inner_adapt", C.c_int), ("base_inv32", C.c_int)]
 
 
 class LResult(C.Structure):
-    _fields_ = [("error_code", C.c_int), ("converged", C.c_int), ("first_defect", C.c_double * 3),
-                ("last_defect", C.c_double * 3), ("number_of_linear_iterations", C.c_int)]
+    _fields_ = [("error_code", C.c_int), ("converged", C.c_int), ("first_defect", C.c_double * MAXB),
+                ("last_defect", C.c_double * MAXB), ("number_of_linear_iterations", C.c_int)]
 
 
 DEFECT_CB = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double))
@@ -142,8 +143,8 @@ def default_params(**kw) -> Params:
     for k, v in kw.items():
         if k in ("damp", "lmgc_damp", "base_damp", "base_abslimit", "base_red"):
             arr = getattr(p, k)
-            vals = [v] * 3 if np.isscalar(v) else list(v)
-            for i in range(3):
+            vals = [v] * MAXB if np.isscalar(v) else list(v)
+            for i in range(MAXB):
                 arr[i] = vals[min(i, len(vals) - 1)]
         else:
             setattr(p, k, v)
@@ -312,20 +313,20 @@ class Context:
 
     def dnrm2x(self, level, x):
         """Per-component norm (a float for scalar hierarchies)."""
-        out = (C.c_double * 3)()
+        out = (C.c_double * MAXB)()
         self._chk(self.lib.ugmg_dnrm2x(self.h, level, _ptr(x), C.addressof(out)))
         b = self._ncomp()
         return out[0] if b == 1 else np.array(out[:b])
 
     def dnrm2x_noalloc(self, level, x):
         """dnrm2x through the entry point that allocates nothing."""
-        out = (C.c_double * 3)()
+        out = (C.c_double * MAXB)()
         self._chk(self.lib.ugmg_dnrm2x_noalloc(self.h, level, _ptr(x), C.addressof(out)))
         b = self._ncomp()
         return out[0] if b == 1 else np.array(out[:b])
 
     def ddot(self, level, x, y):
-        out = (C.c_double * 3)()
+        out = (C.c_double * MAXB)()
         self._chk(self.lib.ugmg_ddot(self.h, level, _ptr(x), _ptr(y), C.addressof(out)))
         b = self._ncomp()
         return out[0] if b == 1 else np.array(out[:b])
@@ -337,11 +338,11 @@ class Context:
         self._chk(self.lib.ugmg_gs_step(self.h, level, _ptr(x), _ptr(b)))
 
     def restrict(self, level, to_c, from_f, damp=1.0):
-        d = (C.c_double * 3)(damp, damp, damp)
+        d = (C.c_double * MAXB)(*[damp] * MAXB)
         self._chk(self.lib.ugmg_restrict(self.h, level, _ptr(to_c), _ptr(from_f), C.addressof(d)))
 
     def interpolate(self, level, to_f, from_c, damp=1.0):
-        d = (C.c_double * 3)(damp, damp, damp)
+        d = (C.c_double * MAXB)(*[damp] * MAXB)
         self._chk(self.lib.ugmg_interpolate(self.h, level, _ptr(to_f), _ptr(from_c), C.addressof(d)))
 
     def base_solve(self, c, b):
@@ -366,8 +367,8 @@ class Context:
             defects.append([d[i] for i in range(ncomp)])
 
         cb = DEFECT_CB(_cb) if record else DEFECT_CB()
-        ab = (C.c_double * 3)(abslimit, abslimit, abslimit)
-        rd = (C.c_double * 3)(red, red, red)
+        ab = (C.c_double * MAXB)(*[abslimit] * MAXB)
+        rd = (C.c_double * MAXB)(*[red] * MAXB)
         self._chk(self.lib.ugmg_solve(self.h, level, _ptr(x), _ptr(b), C.addressof(ab), C.addressof(rd),
                                       maxiter, int(do_defect), C.byref(res), cb, None))
         return res, defects
@@ -383,8 +384,8 @@ class Context:
             defects.append([d[i] for i in range(ncomp)])
 
         cb = DEFECT_CB(_cb)
-        ab = (C.c_double * 3)(abslimit, abslimit, abslimit)
-        rd = (C.c_double * 3)(red, red, red)
+        ab = (C.c_double * MAXB)(*[abslimit] * MAXB)
+        rd = (C.c_double * MAXB)(*[red] * MAXB)
         self._chk(self.lib.ugmg_solve_host(self.h, level, x.ctypes.data, b.ctypes.data, C.addressof(ab),
                                            C.addressof(rd), maxiter, int(do_defect), C.byref(res), cb, None))
         return x, b, res, np.array(defects)
@@ -420,8 +421,8 @@ class Context:
             defects.append([d[i] for i in range(ncomp)])
 
         cb = DEFECT_CB(_cb) if record else DEFECT_CB()
-        ab = (C.c_double * 3)(abslimit, abslimit, abslimit)
-        rd = (C.c_double * 3)(red, red, red)
+        ab = (C.c_double * MAXB)(*[abslimit] * MAXB)
+        rd = (C.c_double * MAXB)(*[red] * MAXB)
         self._chk(self.lib.ugmg_solve_surface(self.h, level, XP, BP, C.addressof(ab), C.addressof(rd),
                                               maxiter, int(do_defect), C.byref(res), cb, None))
         return res, defects
@@ -450,8 +451,8 @@ class Context:
             defects.append([d[i] for i in range(ncomp)])
 
         cb = DEFECT_CB(_cb)
-        ab = (C.c_double * 3)(abslimit, abslimit, abslimit)
-        rd = (C.c_double * 3)(red, red, red)
+        ab = (C.c_double * MAXB)(*[abslimit] * MAXB)
+        rd = (C.c_double * MAXB)(*[red] * MAXB)
         self._chk(self.lib.ugmg_solve_surface_host(self.h, level, XP, BP, C.addressof(ab), C.addressof(rd),
                                                    maxiter, int(do_defect), C.byref(res), cb, None))
         return xs, bs, res, np.array(defects)
@@ -507,8 +508,8 @@ class Context:
         md = 2 * maxiter + 1 if maxdef is None else maxdef
         res = (LResult * nrhs)()
         dfs = np.zeros((nrhs, md))
-        ab = (C.c_double * 3)(abslimit, abslimit, abslimit)
-        rd = (C.c_double * 3)(red, red, red)
+        ab = (C.c_double * MAXB)(*[abslimit] * MAXB)
+        rd = (C.c_double * MAXB)(*[red] * MAXB)
         self._chk(self.lib.ugmg_bcgs_solve(self.h, level, nrhs, _ptr(x), _ptr(b), ld, C.addressof(ab),
                                            C.addressof(rd), maxiter, restart, int(use_iter), int(do_defect),
                                            res, dfs.ctypes.data, md))
@@ -524,8 +525,8 @@ class Context:
         md = 2 * maxiter + 1
         res = (LResult * nrhs)()
         dfs = np.zeros((nrhs, md))
-        ab = (C.c_double * 3)(abslimit, abslimit, abslimit)
-        rd = (C.c_double * 3)(red, red, red)
+        ab = (C.c_double * MAXB)(*[abslimit] * MAXB)
+        rd = (C.c_double * MAXB)(*[red] * MAXB)
         self._chk(self.lib.ugmg_bcgs_solve_host(self.h, level, nrhs, x.ctypes.data, b.ctypes.data, n,
                                                 C.addressof(ab), C.addressof(rd), maxiter, restart,
                                                 int(use_iter), int(do_defect), res, dfs.ctypes.data, md))

@@ -25,7 +25,8 @@ class OrcLevel(C.Structure):
     _fields_ = [("n", C.c_int), ("b", C.c_int),
                 ("rp", C.c_void_p), ("ci", C.c_void_p), ("a", C.c_void_p),
                 ("skip", C.c_void_p), ("vclass", C.c_void_p), ("vnclass", C.c_void_p),
-                ("pp", C.c_void_p), ("pi", C.c_void_p), ("pw", C.c_void_p)]
+                ("pp", C.c_void_p), ("pi", C.c_void_p), ("pw", C.c_void_p),
+                ("vtype", C.c_void_p), ("ntypes", C.c_int)]
 
 
 class OrcParams(C.Structure):
@@ -82,11 +83,18 @@ def _p(a):
     return None if a is None else a.ctypes.data
 
 
+def ntypes(levels):
+    """Vector types with components: 1 + the largest VTYPE of any level."""
+    vt = [lv.vtype for lv in levels if getattr(lv, "vtype", None) is not None and len(lv.vtype)]
+    return 1 + max(int(np.max(v)) for v in vt) if vt else 1
+
+
 class Hierarchy:
     """Keeps numpy arrays alive and exposes an OrcLevel array."""
 
     def __init__(self, levels):
         self._keep = []
+        self.ntypes = ntypes(levels)
         arr = (OrcLevel * len(levels))()
         for i, lv in enumerate(levels):
             fields = dict(rp=np.ascontiguousarray(lv.row_ptr, np.int32),
@@ -99,8 +107,10 @@ class Hierarchy:
                 fields.update(pp=np.ascontiguousarray(lv.p_ptr, np.int32),
                               pi=np.ascontiguousarray(lv.p_idx, np.int32),
                               pw=np.ascontiguousarray(lv.p_w, np.float64))
+            if getattr(lv, "vtype", None) is not None:
+                fields.update(vtype=np.ascontiguousarray(lv.vtype, np.uint8))
             self._keep.append(fields)
-            arr[i].n, arr[i].b = lv.n, lv.b
+            arr[i].n, arr[i].b, arr[i].ntypes = lv.n, lv.b, self.ntypes
             for k, v in fields.items():
                 setattr(arr[i], k, v.ctypes.data)
         self.arr = arr
@@ -108,6 +118,10 @@ class Hierarchy:
 
     def ptr(self, l=0):
         return C.addressof(self.arr) + l * C.sizeof(OrcLevel)
+
+    def ncomp(self, l=0):
+        """Entries of a norm or VEC_SCALAR: ntypes * b."""
+        return self.ntypes * self.levels[l].b
 
 
 def params(nu1=2, nu2=2, gamma=1, baselevel=0, smoother=JAC, damp=0.8, lmgc_damp=1.0,
@@ -140,14 +154,14 @@ def dmatmul(H, l, y):
 def dnrm2x(H, l, x):
     out = np.zeros(MAXB)
     lib().orc_dnrm2x(H.ptr(l), np.ascontiguousarray(x, np.float64).ctypes.data, out.ctypes.data)
-    return out[:H.levels[l].b]
+    return out[:H.ncomp(l)]
 
 
 def ddot(H, l, x, y):
     out = np.zeros(MAXB)
     lib().orc_ddot(H.ptr(l), np.ascontiguousarray(x, np.float64).ctypes.data,
                    np.ascontiguousarray(y, np.float64).ctypes.data, out.ctypes.data)
-    return out[:H.levels[l].b]
+    return out[:H.ncomp(l)]
 
 
 def l_jac(H, l, d):
@@ -243,10 +257,10 @@ def lmgc_iter(H, top, p, b):
 
 
 def ls_solve(H, top, p, rhs, x0=None, red=1e-10, abslimit=1e-300, maxiter=50, do_defect=True):
-    """ls + lmgc solve; returns dict(x, b, defects[nit+1, b], nit, converged)."""
+    """ls + lmgc solve; returns dict(x, b, defects[nit+1, ntypes*b], nit, converged)."""
     lv = H.levels[top]
-    nb = lv.b
-    x = np.zeros(lv.n * nb) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
+    nb = H.ncomp(top)
+    x = np.zeros(lv.n * lv.b) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
     b = np.array(rhs, dtype=np.float64, copy=True)
     ab = np.full(MAXB, abslimit)
     rd = np.full(MAXB, red)
@@ -268,10 +282,10 @@ def fullrefinelevel(H, top):
 def ls_solve_surface(H, top, p, bs, xs=None, red=1e-10, abslimit=1e-300, maxiter=50, do_defect=True):
     """ls + lmgc on a locally refined hierarchy: bs[l] / xs[l] for levels
     baselevel..top (the defect recurrence runs in every level's b); returns
-    dict(x=[per level], b=[per level], defects[nit+1, b], nit, converged)."""
-    nb = H.levels[top].b
+    dict(x=[per level], b=[per level], defects[nit+1, ntypes*b], nit, converged)."""
+    nb, vb = H.ncomp(top), H.levels[top].b
     bl = min(p.baselevel, top)
-    xs = [None if l < bl else (np.zeros(H.levels[l].n * nb) if xs is None or xs[l] is None
+    xs = [None if l < bl else (np.zeros(H.levels[l].n * vb) if xs is None or xs[l] is None
                                else np.array(xs[l], dtype=np.float64, copy=True)) for l in range(top + 1)]
     bs = [None if l < bl else np.array(bs[l], dtype=np.float64, copy=True) for l in range(top + 1)]
     XP = (C.c_void_p * (top + 1))(*[None if v is None else v.ctypes.data for v in xs])

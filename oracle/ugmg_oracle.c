@@ -34,6 +34,13 @@
 
 #define ACTIVE_CLASS 3
 #define NEWDEF_CLASS 2
+#define EDGE_TYPE 1
+
+/* VTYPE of vector v and the first VEC_SCALAR entry of its type */
+static int vtype_of(const orc_level *L, int v) { return L->vtype ? L->vtype[v] : 0; }
+static int aoff(const orc_level *L, int v) { return vtype_of(L, v) * L->b; }
+/* VD_NCOMP: entries of a VEC_SCALAR */
+static int ncomp_of(const orc_level *L) { return (L->ntypes > 0 ? L->ntypes : 1) * L->b; }
 
 /* ---------------------------------------------------------------- ugblas */
 
@@ -74,39 +81,41 @@ void orc_dmatmul(const orc_level *L, double *x, const double *y) {
   }
 }
 
+/* dnrm2x (ugblas.cc:3156-3176): a[aoff[VTYPE(v)] + i] += x_i^2, each entry
+ * summed in list order (vecmode.ct visits one type at a time) */
 void orc_dnrm2x(const orc_level *L, const double *x, double *out) {
-  const int b = L->b;
+  const int b = L->b, nc = ncomp_of(L);
   double a[ORC_MAXB] = {0.0};
   for (int v = 0; v < L->n; ++v)
     for (int i = 0; i < b; ++i) {
       double s = x[(size_t)v * b + i];
-      a[i] += s * s;
+      a[aoff(L, v) + i] += s * s;
     }
-  for (int i = 0; i < b; ++i) out[i] = sqrt(a[i]);
+  for (int i = 0; i < nc; ++i) out[i] = sqrt(a[i]);
 }
 
 void orc_ddot(const orc_level *L, const double *x, const double *y,
               double *out) {
-  const int b = L->b;
+  const int b = L->b, nc = ncomp_of(L);
   double a[ORC_MAXB] = {0.0};
   for (int v = 0; v < L->n; ++v)
     for (int i = 0; i < b; ++i)
-      a[i] += x[(size_t)v * b + i] * y[(size_t)v * b + i];
-  for (int i = 0; i < b; ++i) out[i] = a[i];
+      a[aoff(L, v) + i] += x[(size_t)v * b + i] * y[(size_t)v * b + i];
+  for (int i = 0; i < nc; ++i) out[i] = a[i];
 }
 
 /* surface norm: top level, NEW_DEFECT vectors */
 static void surface_nrm2x(const orc_level *L, const double *x, double *out) {
-  const int b = L->b;
+  const int b = L->b, nc = ncomp_of(L);
   double a[ORC_MAXB] = {0.0};
   for (int v = 0; v < L->n; ++v) {
     if (L->vclass[v] < NEWDEF_CLASS) continue;
     for (int i = 0; i < b; ++i) {
       double s = x[(size_t)v * b + i];
-      a[i] += s * s;
+      a[aoff(L, v) + i] += s * s;
     }
   }
-  for (int i = 0; i < b; ++i) out[i] = sqrt(a[i]);
+  for (int i = 0; i < nc; ++i) out[i] = sqrt(a[i]);
 }
 
 static void surface_dmatmul_minus(const orc_level *L, double *x,
@@ -128,9 +137,11 @@ static void surface_dmatmul_minus(const orc_level *L, double *x,
   }
 }
 
-static void dscalx(int n, int b, double *x, const double *a) {
-  for (int v = 0; v < n; ++v)
-    for (int i = 0; i < b; ++i) x[(size_t)v * b + i] *= a[i];
+/* dscalx (ugblas.cc:2510-2528): x_i *= a[aoff[VTYPE(v)] + i] */
+static void dscalx(const orc_level *L, double *x, const double *a) {
+  const int b = L->b;
+  for (int v = 0; v < L->n; ++v)
+    for (int i = 0; i < b; ++i) x[(size_t)v * b + i] *= a[aoff(L, v) + i];
 }
 
 static void dadd(size_t len, double *x, const double *y) {
@@ -323,7 +334,7 @@ int orc_smoother(const orc_level *L, int kind, const int *rank,
                  const double *damp, double *x, double *b) {
   int err = (kind == ORC_JAC) ? orc_l_jac(L, x, b) : orc_l_lgs(L, rank, x, b);
   if (err) return err;
-  dscalx(L->n, L->b, x, damp);
+  dscalx(L, x, damp);
   orc_dmatmul_minus(L, b, x);
   return 0;
 }
@@ -337,6 +348,7 @@ void orc_restrict(const orc_level *F, const orc_level *C, double *to_c,
     if (C->vnclass[v] >= NEWDEF_CLASS)
       for (int i = 0; i < b; ++i) to_c[(size_t)v * b + i] = 0.0;
   for (int v = 0; v < F->n; ++v) {
+    if (vtype_of(F, v) != 0) continue;
     if (F->vclass[v] < NEWDEF_CLASS) continue;
     double s[ORC_MAXB];
     for (int i = 0; i < b; ++i) s[i] = damp[i] * from_f[(size_t)v * b + i];
@@ -348,6 +360,18 @@ void orc_restrict(const orc_level *F, const orc_level *C, double *to_c,
         if (!(vecskip & (1u << j))) to_c[(size_t)vc * b + j] += c * s[j];
     }
   }
+  /* edge vectors (transgrid.cc:188-212): 0.5 damp_j b_j to the father
+   * object of each of the edge's nodes, in the row's order */
+  for (int v = 0; v < F->n; ++v) {
+    if (vtype_of(F, v) != EDGE_TYPE) continue;
+    for (int q = F->pp[v]; q < F->pp[v + 1]; ++q) {
+      int vc = F->pi[q];
+      uint32_t vecskip = C->skip[vc];
+      for (int j = 0; j < b; ++j)
+        if (!(vecskip & (1u << j)))
+          to_c[(size_t)vc * b + j] += F->pw[q] * damp[j] * from_f[(size_t)v * b + j];
+    }
+  }
 }
 
 void orc_interpolate(const orc_level *F, const orc_level *C, double *to_f,
@@ -356,6 +380,7 @@ void orc_interpolate(const orc_level *F, const orc_level *C, double *to_f,
   (void)C;
   for (size_t i = 0; i < (size_t)F->n * b; ++i) to_f[i] = 0.0;
   for (int v = 0; v < F->n; ++v) {
+    if (vtype_of(F, v) != 0) continue;
     uint32_t vecskip = F->skip[v];
     int skip = 1;
     for (int i = 0; i < b; ++i)
@@ -368,6 +393,18 @@ void orc_interpolate(const orc_level *F, const orc_level *C, double *to_f,
         if (c != 0.0)
           to_f[(size_t)v * b + j] += c * damp[j] * from_c[(size_t)F->pi[q] * b + j];
       }
+    }
+  }
+  /* edge vectors (transgrid.cc:317-336): += 0.5 damp_j c_j of each father
+   * object whose VECSKIP bit j is clear */
+  for (int v = 0; v < F->n; ++v) {
+    if (vtype_of(F, v) != EDGE_TYPE) continue;
+    for (int q = F->pp[v]; q < F->pp[v + 1]; ++q) {
+      int vc = F->pi[q];
+      uint32_t vecskip = C->skip[vc];
+      for (int j = 0; j < b; ++j)
+        if (!(vecskip & (1u << j)))
+          to_f[(size_t)v * b + j] += F->pw[q] * damp[j] * from_c[(size_t)vc * b + j];
     }
   }
 }
@@ -508,7 +545,7 @@ int orc_ex_smoother(const orc_ex *ex, const orc_level *L, const double *damp,
     int v = ex->order[k];
     for (int j = 0; j < nb; ++j) x[(size_t)v * nb + j] = Vec[k * nb + j];
   }
-  dscalx(L->n, nb, x, damp);
+  dscalx(L, x, damp);
   orc_dmatmul_minus(L, b, x);
   return 0;
 }
@@ -530,14 +567,14 @@ typedef struct {
 static int base_solver(orc_mg *mg, int level, const double *first) {
   const orc_level *L = &mg->lv[level];
   const orc_params *p = mg->p;
-  const int b = L->b;
+  const int b = L->b, nc = ncomp_of(L);
   double defect2reach[ORC_MAXB], last[ORC_MAXB];
-  for (int i = 0; i < b; ++i) {
+  for (int i = 0; i < nc; ++i) {
     defect2reach[i] = first[i] * p->base_red[i];
     if (defect2reach[i] == 0.0) defect2reach[i] = p->base_red[i];
     last[i] = first[i];
   }
-  int converged = sc_cmp(b, first, p->base_abslimit);
+  int converged = sc_cmp(nc, first, p->base_abslimit);
   for (int i = 0; i < p->base_maxiter; ++i) {
     if (converged) break;
     memset(mg->cbase, 0, sizeof(double) * (size_t)L->n * b);
@@ -545,7 +582,7 @@ static int base_solver(orc_mg *mg, int level, const double *first) {
       return 1;
     dadd((size_t)L->n * b, mg->c[level], mg->cbase); /* LSUpdate */
     surface_nrm2x(L, mg->b[level], last);             /* LinearResiduum */
-    if (sc_cmp(b, last, p->base_abslimit) || sc_cmp(b, last, defect2reach)) {
+    if (sc_cmp(nc, last, p->base_abslimit) || sc_cmp(nc, last, defect2reach)) {
       converged = 1;
       break;
     }
@@ -651,8 +688,8 @@ int orc_ls_lmgc_solve(const orc_level *levels, int top, const orc_params *p,
                       int maxiter, double *defects, int *nit, int *converged) {
   orc_mg mg;
   const orc_level *L = &levels[top];
-  const int nb = L->b;
-  const size_t len = (size_t)L->n * nb;
+  const int nb = ncomp_of(L);                      /* norm entries */
+  const size_t len = (size_t)L->n * L->b;
   int err = mg_setup(&mg, levels, top, p);
   if (err) { mg_free(&mg); return err; }
   if (do_defect) surface_dmatmul_minus(L, b, x); /* LinearDefect */
@@ -735,11 +772,11 @@ static void composite_dmatmul_minus(const orc_level *levels, int frl, int tl,
   }
 }
 
-/* dnrm2x ON_SURFACE (ugblas.cc:3156-3176): one running sum per component
- * over the surface rows, level by level, then sqrt */
+/* dnrm2x ON_SURFACE (ugblas.cc:3156-3176): one running sum per (type,
+ * component) over the surface rows, level by level, then sqrt */
 static void composite_nrm2x(const orc_level *levels, int frl, int tl,
                             double *const *x, double *out) {
-  const int b = levels[tl].b;
+  const int b = levels[tl].b, nc = ncomp_of(&levels[tl]);
   double a[ORC_MAXB] = {0.0};
   for (int l = frl; l <= tl; ++l) {
     const orc_level *L = &levels[l];
@@ -747,11 +784,11 @@ static void composite_nrm2x(const orc_level *levels, int frl, int tl,
       if (!on_surface(L, l, tl, v)) continue;
       for (int i = 0; i < b; ++i) {
         double s = x[l][(size_t)v * b + i];
-        a[i] += s * s;
+        a[aoff(L, v) + i] += s * s;
       }
     }
   }
-  for (int i = 0; i < b; ++i) out[i] = sqrt(a[i]);
+  for (int i = 0; i < nc; ++i) out[i] = sqrt(a[i]);
 }
 
 int orc_ls_lmgc_solve_surface(const orc_level *levels, int top, const orc_params *p,
@@ -759,7 +796,8 @@ int orc_ls_lmgc_solve_surface(const orc_level *levels, int top, const orc_params
                               const double *abslimit, const double *reduction,
                               int maxiter, double *defects, int *nit, int *converged) {
   orc_mg mg;
-  const int nb = levels[top].b;
+  const int vb = levels[top].b;              /* components per vector */
+  const int nb = ncomp_of(&levels[top]);      /* norm entries */
   const int bl = p->baselevel < top ? p->baselevel : top;
   const int frl = orc_fullrefinelevel(levels, top);
   if (frl < bl) return 3;   /* surface rows below the base level */
@@ -767,7 +805,7 @@ int orc_ls_lmgc_solve_surface(const orc_level *levels, int top, const orc_params
   if (err) { mg_free(&mg); return err; }
   /* the defect recurrence runs in the coarse levels' b too */
   for (int l = bl; l <= top; ++l)
-    memcpy(mg.b[l], b[l], sizeof(double) * (size_t)levels[l].n * nb);
+    memcpy(mg.b[l], b[l], sizeof(double) * (size_t)levels[l].n * vb);
   if (do_defect) composite_dmatmul_minus(levels, frl, top, mg.b, x); /* LinearDefect */
   double first[ORC_MAXB], last[ORC_MAXB], defect2reach[ORC_MAXB];
   composite_nrm2x(levels, frl, top, mg.b, first);                    /* LinearResiduum */
@@ -781,10 +819,10 @@ int orc_ls_lmgc_solve_surface(const orc_level *levels, int top, const orc_params
   int i;
   for (i = 0; i < maxiter; ++i) {
     if (conv) break;
-    memset(mg.c[top], 0, sizeof(double) * (size_t)levels[top].n * nb); /* dset(c,0) */
+    memset(mg.c[top], 0, sizeof(double) * (size_t)levels[top].n * vb); /* dset(c,0) */
     if (lmgc(&mg, top)) { err = 1; break; }
     for (int l = bl; l <= top; ++l)                 /* LSUpdate: baselevel..level */
-      dadd((size_t)levels[l].n * nb, x[l], mg.c[l]);
+      dadd((size_t)levels[l].n * vb, x[l], mg.c[l]);
     composite_nrm2x(levels, frl, top, mg.b, last);
     for (int k = 0; k < nb; ++k) defects[(size_t)(i + 1) * nb + k] = last[k];
     if (sc_cmp(nb, last, abslimit) || sc_cmp(nb, last, defect2reach)) {
@@ -795,7 +833,7 @@ int orc_ls_lmgc_solve_surface(const orc_level *levels, int top, const orc_params
   }
   if (!conv) it = i;
   for (int l = bl; l <= top; ++l)
-    memcpy(b[l], mg.b[l], sizeof(double) * (size_t)levels[l].n * nb);
+    memcpy(b[l], mg.b[l], sizeof(double) * (size_t)levels[l].n * vb);
   *nit = it;
   *converged = conv;
   mg_free(&mg);

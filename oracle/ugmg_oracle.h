@@ -30,9 +30,13 @@ extern "C" {
 
 #define ORC_MAXB 4
 
+/* Vector types (VTYPE): 0 node vectors, 1 edge vectors (the two the
+ * standard transfer serves, transgrid.cc:461-503).  Every type carries b
+ * components; norms, dot products and VEC_SCALAR parameters have one entry
+ * per (type, component), type-major (VD_OFFSET, np/udm/udm.h:105). */
 typedef struct {
   int n;                  /* vectors on the level                         */
-  int b;                  /* components per vector (1 or 3)               */
+  int b;                  /* components per vector (1 .. ORC_MAXB)        */
   const int *rp;          /* n+1                                          */
   const int *ci;          /* nnzb; diagonal first in every row            */
   const double *a;        /* nnzb*b*b                                     */
@@ -40,8 +44,12 @@ typedef struct {
   const uint8_t *vclass;  /* VCLASS                                       */
   const uint8_t *vnclass; /* VNCLASS                                      */
   const int *pp;          /* interpolation from level-1 (NULL on level 0) */
-  const int *pi;
-  const double *pw;
+  const int *pi;          /* node rows: father corners with GNs weights;  */
+  const double *pw;       /* edge rows: the father objects of the edge's  */
+                          /* two nodes (NFATHEREDGE / NFATHER), 0.5 each  */
+  const uint8_t *vtype;   /* VTYPE per vector, NULL: all node vectors     */
+  int ntypes;             /* types with components (0 read as 1); the     */
+                          /* norms have ntypes * b entries                */
 } orc_level;
 
 enum { ORC_JAC = 0, ORC_GS = 1, ORC_MCGS = 2 };
@@ -61,7 +69,7 @@ typedef struct {
 /* ugblas (np/algebra/ugblas.cc) on one level, ALL_VECTORS */
 void orc_dmatmul_minus(const orc_level *L, double *x, const double *y);
 void orc_dmatmul(const orc_level *L, double *x, const double *y);
-void orc_dnrm2x(const orc_level *L, const double *x, double *out /* b */);
+void orc_dnrm2x(const orc_level *L, const double *x, double *out /* ntypes * b */);
 void orc_ddot(const orc_level *L, const double *x, const double *y, double *out);
 
 /* SolveSmallBlock (np/algebra/block.cc:104-205); 0 ok, 1 singular */
@@ -84,7 +92,9 @@ int orc_smoother(const orc_level *L, int kind, const int *rank,
                  const double *damp, double *x, double *b);
 
 /* StandardRestrictNodeVector / StandardIntCorNodeVector
- * (np/algebra/transgrid.cc:117-215, 235-336) */
+ * (np/algebra/transgrid.cc:117-215, 235-336): the node loop, then the edge
+ * loop over the edge vectors (no VCLASS test; each father object's VECSKIP
+ * bits mask its term; the fine edge vector's own bits are not read) */
 void orc_restrict(const orc_level *F, const orc_level *C, double *to_c,
                   const double *from_f, const double *damp);
 void orc_interpolate(const orc_level *F, const orc_level *C, double *to_f,

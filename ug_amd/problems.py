@@ -197,6 +197,7 @@ class Level:
     p_ptr: np.ndarray | None = None     # interpolation from level-1
     p_idx: np.ndarray | None = None
     p_w: np.ndarray | None = None
+    vtype: np.ndarray | None = None     # VTYPE per vector (0 node, 1 edge); None: all node
 
     @property
     def nnzb(self) -> int:
@@ -608,6 +609,137 @@ def p1_local(nuni: int, depth: int, ncomp: int = 1, coupling=None) -> Problem:
         levels[l].vnclass = _classes(levels[l], tri_l, nn3)
     rhs = np.zeros(levels[-1].n * ncomp)
     return Problem(dim=2, levels=levels, rhs=rhs, name=f"p1_local_{nuni}u{depth}l_b{ncomp}" + ("" if coupling is None else "_c"))
+
+
+def _p2_stiffness(xy: np.ndarray, tri: np.ndarray) -> np.ndarray:
+    """P2 stiffness of -Laplace on each triangle, local dofs corners 0, 1, 2
+    then the midpoints of edges (0,1), (1,2), (2,0): (ne, 6, 6), by the
+    edge-midpoint rule (exact for the degree-2 integrands)."""
+    p0, p1, p2 = xy[tri[:, 0]], xy[tri[:, 1]], xy[tri[:, 2]]
+    det = (p1[:, 0] - p0[:, 0]) * (p2[:, 1] - p0[:, 1]) - (p2[:, 0] - p0[:, 0]) * (p1[:, 1] - p0[:, 1])
+    area = 0.5 * np.abs(det)
+    # grad lambda_i = (y_j - y_k, x_k - x_j) / det for (i, j, k) cyclic
+    g = np.stack([np.stack([p1[:, 1] - p2[:, 1], p2[:, 0] - p1[:, 0]], 1),
+                  np.stack([p2[:, 1] - p0[:, 1], p0[:, 0] - p2[:, 0]], 1),
+                  np.stack([p0[:, 1] - p1[:, 1], p1[:, 0] - p0[:, 0]], 1)], 1) / det[:, None, None]
+    K = np.zeros((len(tri), 6, 6))
+    for lam in ((0.5, 0.5, 0.0), (0.0, 0.5, 0.5), (0.5, 0.0, 0.5)):
+        gr = [(4 * lam[i] - 1) * g[:, i] for i in range(3)]
+        for (i, j) in ((0, 1), (1, 2), (2, 0)):
+            gr.append(4 * (lam[j] * g[:, i] + lam[i] * g[:, j]))
+        G = np.stack(gr, 1)                                   # (ne, 6, 2)
+        K += (area / 3)[:, None, None] * np.einsum("eak,ebk->eab", G, G)
+    return K
+
+
+def p2_uniform(nref: int, ncomp: int = 1) -> Problem:
+    """2D P2 -Laplace (ncomp > 1: times the leading ncomp x ncomp part of the
+    coupling of p1_local) on the unit square with node vectors (VTYPE 0) and
+    edge vectors (VTYPE 1, one per edge, at its midpoint): the descriptor of
+    two vector types that UG's standard transfer serves (transgrid.cc:
+    461-503).  Coarse grid: 2x2 squares split into 8 triangles, nref uniform
+    red refinements.  Level l's list: its node vectors (the sons of level
+    l-1's nodes in order, then the midnodes of level l-1's edges), then its
+    edge vectors (edges ascending by (min, max) node).  Rows diagonal first
+    then ascending columns; Dirichlet on the boundary nodes and edges
+    (AssembleDirichletBoundary).  Interpolation rows as
+    StandardIntCorNodeVector forms them: a corner's son 1 from its father
+    node, a midnode 1/2 + 1/2 from the father edge's end nodes (GNs); an edge
+    vector 1/2 + 1/2 from the father objects of its two nodes (the father
+    node's vector for a corner's son, the father edge's vector for a
+    midnode), transgrid.cc:317-336.  Uniform refinement: VCLASS 3,
+    VNCLASS 3 below the top level."""
+    xs = np.linspace(0.0, 1.0, 3)
+    xy = np.array([(x, y) for y in xs for x in xs])
+    tri = []
+    for j in range(2):
+        for i in range(2):
+            a, b, c, d = i + 3 * j, i + 1 + 3 * j, i + 1 + 3 * (j + 1), i + 3 * (j + 1)
+            tri += [(a, b, c), (a, c, d)]
+    tri = np.array(tri, dtype=np.int64)
+    C = _M4[:ncomp, :ncomp]
+
+    def edges_of(nv, tri):
+        e = np.sort(np.stack([tri[:, [0, 1]], tri[:, [1, 2]], tri[:, [2, 0]]], 1), axis=2)
+        key = np.unique((e[:, :, 0] * nv + e[:, :, 1]).ravel())
+        return key                                           # ascending (min, max)
+
+    meshes = [(xy, tri)]
+    for _ in range(nref):
+        nv = len(xy)
+        key = edges_of(nv, tri)
+        mid = {int(k): nv + i for i, k in enumerate(key)}
+        xy = np.vstack([xy, 0.5 * (xy[key // nv] + xy[key % nv])])
+        tn = []
+        for p, q, r in tri.tolist():
+            mpq, mqr, mrp = (mid[min(u, w) * nv + max(u, w)] for u, w in ((p, q), (q, r), (r, p)))
+            tn += [(p, mpq, mrp), (mpq, q, mqr), (mrp, mqr, r), (mpq, mqr, mrp)]
+        tri = np.array(tn, dtype=np.int64)
+        meshes.append((xy, tri))
+    levels = []
+    prev = None
+    for l, (xy_l, tri_l) in enumerate(meshes):
+        nv = len(xy_l)
+        key = edges_of(nv, tri_l)
+        ne = len(key)
+        eidx = {int(k): i for i, k in enumerate(key)}
+        n = nv + ne
+        dof = np.zeros((len(tri_l), 6), dtype=np.int64)
+        dof[:, :3] = tri_l
+        for c, (i, j) in enumerate(((0, 1), (1, 2), (2, 0))):
+            k = np.minimum(tri_l[:, i], tri_l[:, j]) * nv + np.maximum(tri_l[:, i], tri_l[:, j])
+            dof[:, 3 + c] = nv + np.array([eidx[int(x)] for x in k])
+        K = _p2_stiffness(xy_l, tri_l)
+        r = np.repeat(dof, 6, axis=1).ravel()
+        c = np.tile(dof, (1, 6)).ravel()
+        code = r.astype(np.int64) * (n + 1) + np.where(r == c, 0, c + 1)
+        uniq, inv = np.unique(code, return_inverse=True)
+        vals = np.zeros(len(uniq))
+        np.add.at(vals, inv, K.ravel())
+        rows = uniq // (n + 1)
+        cc = uniq % (n + 1)
+        cols = np.where(cc == 0, rows, cc - 1)
+        diag = cc == 0
+        rp = np.zeros(n + 1, dtype=np.int32)
+        np.add.at(rp, rows + 1, 1)
+        rp = np.cumsum(rp).astype(np.int32)
+        pos = np.vstack([xy_l, 0.5 * (xy_l[key // nv] + xy_l[key % nv])])
+        bnd = (np.isclose(pos[:, 0], 0) | np.isclose(pos[:, 0], 1) | np.isclose(pos[:, 1], 0) |
+               np.isclose(pos[:, 1], 1))
+        vals[bnd[rows]] = 0.0
+        vals[bnd[rows] & diag] = 1.0
+        if ncomp > 1:
+            blk = vals[:, None, None] * C[None]
+            blk[bnd[rows]] = 0.0
+            blk[bnd[rows] & diag] = np.eye(ncomp)
+            vals = blk.ravel()
+        lv = Level(n=n, b=ncomp, row_ptr=rp, col_idx=cols.astype(np.int32), vals=vals.astype(np.float64),
+                   skip=np.where(bnd, (1 << ncomp) - 1, 0).astype(np.uint32),
+                   vclass=np.full(n, 3, np.uint8), vnclass=np.full(n, 3 if l < nref else 0, np.uint8),
+                   vtype=np.concatenate([np.zeros(nv, np.uint8), np.ones(ne, np.uint8)]))
+        if l >= 1:
+            pnv, pkey, peidx = prev
+            rows_i, rows_w = [], []
+            for v in range(nv):                               # node vectors
+                if v < pnv:
+                    rows_i.append([v]); rows_w.append([1.0])
+                else:
+                    k = int(pkey[v - pnv])
+                    rows_i.append([k // pnv, k % pnv]); rows_w.append([0.5, 0.5])
+
+            def father(u):
+                # a corner's son u < pnv: its father node's vector u; the midnode
+                # pnv + i of coarse edge i: that edge's vector, also pnv + i
+                return u
+            for k in key.tolist():                            # edge vectors
+                rows_i.append([father(k // nv), father(k % nv)]); rows_w.append([0.5, 0.5])
+            lv.p_ptr = np.concatenate([[0], np.cumsum([len(x) for x in rows_i])]).astype(np.int32)
+            lv.p_idx = np.concatenate(rows_i).astype(np.int32)
+            lv.p_w = np.concatenate(rows_w).astype(np.float64)
+        levels.append(lv)
+        prev = (nv, key, eidx)
+    rhs = np.zeros(levels[-1].n * ncomp)
+    return Problem(dim=2, levels=levels, rhs=rhs, name=f"p2_{nref}r_b{ncomp}")
 
 
 def reordered(P: Problem, seed: int, newest_first: bool = True) -> Problem:

@@ -123,6 +123,31 @@ int ugmg_set_level(ugmg_ctx *ctx, int level, int n, const int *row_ptr,
                    const uint8_t *vnclass);
 int ugmg_set_transfer(ugmg_ctx *ctx, int level, const int *p_ptr,
                       const int *p_idx, const double *p_w);
+/* Vector types of a level's vectors (VTYPE, np/udm/udm.h:105-108): 0 node
+ * vectors, 1 edge vectors, each with the b components of the hierarchy;
+ * NULL (or never called): node vectors only.  Call after ugmg_set_level
+ * (which resets the level to node vectors).  With edge vectors present:
+ *  - norms, dot products and the VEC_SCALARs of the stop tests
+ *    (abslimit, reduction, ugmg_lresult's defects, the per-iteration
+ *    callback) take one entry per (type, component), type-major (VD_OFFSET),
+ *    at most UGMG_MAXB of them; the damping factors must agree over them;
+ *  - the transfer rows of an edge vector name the father objects of the
+ *    edge's two nodes (the father node's vector for a corner's son, the
+ *    father edge's vector for a midnode) with weight 1/2 each: the edge
+ *    loops of StandardRestrictNodeVector / StandardIntCorNodeVector
+ *    (transgrid.cc:188-212, 317-336), which read no VCLASS of the edge
+ *    vector in the restriction, mask each father object's term by its own
+ *    VECSKIP bits in the interpolation and not by the edge vector's; a
+ *    father object with only some components Dirichlet is refused (b > 1);
+ *  - ugmg_solve and ugmg_iter run; the surface solve, bcgs and the server
+ *    take node vectors only (UGMG_E_ARG).
+ * UG's standard transfer serves node and edge vectors only (element and
+ * side vectors: "not implemented", transgrid.cc:474-503). */
+int ugmg_set_vtypes(ugmg_ctx *ctx, int level, const uint8_t *vtype);
+/* VD_NCOMP of the prepared hierarchy: the entries of its norms, dot
+ * products and VEC_SCALARs (vector types x b); UGMG_E_STATE before
+ * ugmg_preprocess. */
+int ugmg_norm_entries(const ugmg_ctx *ctx);
 int ugmg_set_params(ugmg_ctx *ctx, const ugmg_params *p);
 /* LmgcPreProcess (iter.cc:7707-7739): device layouts of levels bl..level,
  * colouring (GS), coarse band LU at the base level (EXPreProcess). */

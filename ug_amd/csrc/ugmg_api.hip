@@ -135,6 +135,13 @@ struct DevLevel {
   uint8_t *vclass = nullptr, *vnclass = nullptr;
   uint8_t *c0 = nullptr;   // GS levels: 1 on the rows of colour 0 (device order)
   uint32_t *skip = nullptr;
+  // several vector types: VTYPE per row, and the VECSKIP bits the
+  // interpolation reads (an edge row's own bits are not read: 0); a single
+  // type: vt null, iskip = skip
+  uint8_t *vt = nullptr;
+  uint32_t *iskip = nullptr;
+  int ntypes = 1;
+  int nc() const { return vt ? ntypes * nb : nb; }   // norm entries
   int p_nnz = 0, pt_nnz = 0;
   DevSellBuf P, PT;   // interpolation from / restriction to level-1
   bool nt = false;    // stream the matrix with non-temporal loads
@@ -178,6 +185,9 @@ struct ugmg_ctx {
   bool own_stream = false;
   std::string err;
   int nlevels = 0, b = 1;
+  // vector types of the prepared hierarchy and the entries of its norms
+  // and VEC_SCALARs (ntypes * b)
+  int ntypes = 1, nc = 1;
   std::vector<HostLevel> host;
   std::vector<char> level_set, transfer_set;
   ugmg_params p{};
@@ -454,9 +464,9 @@ void drop_graph(ugmg_ctx *ctx) {
 
 inline Batch batch(const ugmg_ctx *ctx, int l) {
   const DevLevel &d = ctx->dev[l];
-  return Batch{ctx->nb, d.vs, d.nb, d.dms, ctx->base_ims};
+  return Batch{ctx->nb, d.vs, d.nb, d.dms, ctx->base_ims, d.vt, d.ntypes};
 }
-inline Batch B1(const DevLevel &d) { return Batch{1, d.vs, d.nb}; }
+inline Batch B1(const DevLevel &d) { return Batch{1, d.vs, d.nb, 0, 0, d.vt, d.ntypes}; }
 
 // the caller's vectors (list order, column stride ld) to the level's device
 // order and back; a plain (converting) copy on levels kept in list order
@@ -647,10 +657,12 @@ void lmgc(ugmg_ctx *ctx, int l, T *c, T *b, int cmode, bool c0_done = false) {
   const bool pre0 = l - 1 > ctx->bl && l - 1 > ctx->fuse_top && p.gamma >= 1 &&
                     fold_colour0(ctx, l - 1, p.nu1);
   // onto the base level (scalar) also b in the inverse's column order, into
-  // the base level's spare vector u, for the first base iteration
-  const bool bcol = to_base && nb == 1 && ctx->base_cpos && !ctx->base.failed;
+  // the base level's spare vector u, for the first base iteration.  With
+  // several vector types the base solve forms its own per-type norms.
+  const bool typed = dc.vt != nullptr;
+  const bool bcol = to_base && nb == 1 && ctx->base_cpos && !ctx->base.failed && !typed;
   const int np = launch_restrict<T>(st, d.PT.d<T>(), dc.skip, dc.vnclass, dc.vclass, b, (T)1,
-                                    dc.bv<T>(), to_base ? ctx->base_part : nullptr, d.ntv<T>(), nb,
+                                    dc.bv<T>(), to_base && !typed ? ctx->base_part : nullptr, d.ntv<T>(), nb,
                                     d.vs, dc.vs, d.nb, pre0 ? dc.c0 : nullptr, dc.diag<T>(),
                                     pre0 ? dc.tv<T>() : nullptr, bcol ? ctx->base_cpos : nullptr,
                                     dc.uv<T>(), dc.dms);
@@ -661,14 +673,14 @@ void lmgc(ugmg_ctx *ctx, int l, T *c, T *b, int cmode, bool c0_done = false) {
       // base solve b has changed, so later visits recompute them
       if (g == 0) ctx->base_nparts = np;
       // a later visit (gamma > 1) reads the base defect the earlier one leaves
-      base_solve<T>(ctx, dc.cv<T>(), dc.bv<T>(), g == 0 ? 0 : 1, g == 0, g + 1 < p.gamma, true,
+      base_solve<T>(ctx, dc.cv<T>(), dc.bv<T>(), g == 0 ? 0 : 1, g == 0 && !typed, g + 1 < p.gamma, true,
                     g == 0 && bcol ? dc.uv<T>() : nullptr);
     } else {
       lmgc<T>(ctx, l - 1, dc.cv<T>(), dc.bv<T>(), g == 0 ? 0 : 1, g == 0 && pre0);
     }
   }
   // InterpolateCorrection(level, t, c, damp); dadd(c, t); dmatmul_minus(b, A, t)
-  launch_interp<T>(st, d.P.d<T>(), d.skip, dc.cv<T>(), (T)p.lmgc_damp[0], t, c, d.ntv<T>(), nb, d.vs,
+  launch_interp<T>(st, d.P.d<T>(), d.iskip, dc.cv<T>(), (T)p.lmgc_damp[0], t, c, d.ntv<T>(), nb, d.vs,
                    dc.vs, d.nb);
   if (fold_colour0(ctx, l, p.nu2)) {
     post_smooth_folded<T>(ctx, l, t, b, c, p.nu2);
@@ -690,9 +702,9 @@ int check_ready(ugmg_ctx *ctx, int level) {
 int surface_norm(ugmg_ctx *ctx, int level, const double *x, double *out) {
   DevLevel &d = ctx->dev[level];
   launch_dot<double>(ctx->stream, d.n, x, nullptr, d.vclass, ctx->part, ctx->red, 1, B1(d));
-  HIPCHK(hipMemcpyAsync(ctx->red_host, ctx->red, sizeof(double) * d.nb, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(ctx->red_host, ctx->red, sizeof(double) * d.nc(), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
-  for (int i = 0; i < d.nb; ++i) out[i] = ctx->red_host[i];
+  for (int i = 0; i < d.nc(); ++i) out[i] = ctx->red_host[i];
   return 0;
 }
 
@@ -706,7 +718,9 @@ void enqueue_iteration(ugmg_ctx *ctx, int level, double *x, double *b, int cycle
     int np = 0;
     launch_ir_update(ctx->stream, d.A.d64(), d.c32, b, x, d.b32, d.vclass, ctx->part, &np, d.nt,
                      B1(d));
-    launch_reduce_final(ctx->stream, np, ctx->part, ctx->red, 1, d.nb);
+    // several vector types: the norms per (type, component) of the FP64 defect
+    if (d.vt) launch_dot<double>(ctx->stream, d.n, b, nullptr, d.vclass, ctx->part, ctx->red, 1, B1(d));
+    else launch_reduce_final(ctx->stream, np, ctx->part, ctx->red, 1, d.nb);
   } else {
     lmgc<double>(ctx, level, x, b, 1);
     launch_dot<double>(ctx->stream, d.n, b, nullptr, d.vclass, ctx->part, ctx->red, 1, B1(d));
@@ -748,7 +762,7 @@ int launch_iteration(ugmg_ctx *ctx, int level, double *x, double *b, int cycles)
     }
     HIPCHK(hipGraphLaunch(ge, ctx->stream));
   }
-  HIPCHK(hipMemcpyAsync(ctx->red_host, ctx->red, sizeof(double) * ctx->b, hipMemcpyDeviceToHost,
+  HIPCHK(hipMemcpyAsync(ctx->red_host, ctx->red, sizeof(double) * ctx->nc, hipMemcpyDeviceToHost,
                         ctx->stream));
   return 0;
 }
@@ -871,7 +885,8 @@ double lmgc_bytes(const ugmg_ctx *ctx, int l, int sv, int cmode, bool c0_done) {
   const bool pre0 = l - 1 > ctx->bl && l - 1 > ctx->fuse_top && p.gamma >= 1 && fold_colour0(ctx, l - 1, p.nu1);
   by += op_bytes(ctx, l, Op::Restrict, sv);
   if (pre0) by += dc.n + (double)dc.color_rows[0] * 2.0 * sv;
-  if (to_base && d.nb == 1 && ctx->base_cpos && !ctx->base.failed) by += 4.0 * dc.n + (double)ctx->base_ncyc * sv;
+  if (to_base && ctx->nb == 1 && ctx->base_cpos && !ctx->base.failed && !dc.vt)
+    by += 4.0 * dc.n + (double)ctx->base_ncyc * sv;
   if (p.gamma <= 0) by += vbytes(dc, sv);
   for (int g = 0; g < p.gamma; ++g)
     by += to_base ? base_bytes(ctx, sv, true, g + 1 < p.gamma)
@@ -1051,6 +1066,7 @@ static int resolve_baselevel(const ugmg_ctx *ctx, int level) {
 
 struct PrepState {
   int level = -1, bl = 0, fuse_rows = 0;
+  int ntypes = 1;   // vector types of the hierarchy
   double nt_bytes = 64.0e6;
   // layouts built on the device (build_level_device) in prep_device, or on
   // the host here (hp, plv)
@@ -1092,6 +1108,50 @@ int prep_host(ugmg_ctx *ctx, int level, PrepState &ps) {
   }
   if ((long long)ctx->host[bl].n * ctx->b > 8192)
     return fail(ctx, UGMG_E_ARG, "base level has more than 8192 unknowns; choose a coarser baselevel");
+  // vector types (node and edge vectors with b components each): norms,
+  // dot products and VEC_SCALARs take one entry per (type, component)
+  int nt = 1;
+  for (int l = bl; l <= level; ++l)
+    for (uint8_t t : ctx->host[l].vt) nt = std::max(nt, t + 1);
+  ps.ntypes = nt;
+  if (nt > 1) {
+    if (nt * ctx->b > UGMG_MAXB)
+      return fail(ctx, UGMG_E_ARG, "vector types x components exceed UGMG_MAXB norm entries");
+    if (ctx->grouped) return fail(ctx, UGMG_E_ARG, "served contexts take one vector type");
+    for (int i = 1; i < nt * ctx->b; ++i)
+      if (p.damp[i] != p.damp[0] || p.lmgc_damp[i] != p.lmgc_damp[0] || p.base_damp[i] != p.base_damp[0] ||
+          p.base_abslimit[i] != p.base_abslimit[0] || p.base_red[i] != p.base_red[0])
+        return fail(ctx, UGMG_E_ARG, "one damping / base limit for every vector type and component");
+  }
+  try {
+    const uint32_t full = ctx->b >= 32 ? ~0u : (1u << ctx->b) - 1u;
+    for (int l = bl; l <= level; ++l) {
+      HostLevel &F = ctx->host[l];
+      F.pwf.clear();
+      if (nt > 1 && F.vt.empty()) F.vt.assign(F.n, 0);   // node vectors
+      if (l == bl || F.vt.empty() || !F.has_p) continue;
+      // StandardIntCorNodeVector's edge loop adds a father object's value
+      // only where that object's VECSKIP bit is clear (transgrid.cc:317-336):
+      // an object Dirichlet in every component contributes nothing (weight
+      // 0); the restriction skips the same components at the coarse row
+      const HostLevel &Cl = ctx->host[l - 1];
+      std::vector<double> w;
+      for (int v = 0; v < F.n; ++v) {
+        if (F.vt[v] != 1) continue;
+        for (int q = F.pp[v]; q < F.pp[v + 1]; ++q) {
+          const uint32_t m = Cl.skip[F.pi[q]] & full;
+          if (m == 0) continue;
+          if (m != full)
+            return fail(ctx, UGMG_E_ARG, "an edge vector's father object is Dirichlet in some but not all components");
+          if (w.empty()) w = F.pw;
+          w[q] = 0.0;
+        }
+      }
+      F.pwf.swap(w);
+    }
+  } catch (const std::bad_alloc &) {
+    return fail(ctx, UGMG_E_NOMEM, "host allocation during preprocess");
+  }
   const char *env_nt = std::getenv("UGMG_NT_BYTES");
   ps.nt_bytes = env_nt ? std::atof(env_nt) : 64.0e6;
   const char *env_gsf = std::getenv("UGMG_GS_FUSE");
@@ -1277,6 +1337,7 @@ int adopt_level(ugmg_ctx *ctx, DevPrep &dp, bool has_p, DevLevel &D) {
   D.vclass = dp.vc;
   D.vnclass = dp.vnc;
   D.skip = dp.skip;
+  D.vt = dp.vt;
   if (dp.ncolors > 0) {
     D.ncolors = dp.ncolors;
     D.color_slices = dp.color_slices;
@@ -1355,6 +1416,7 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
       if ((rc = upload(ctx, &D.vclass, H.vc.data(), H.vc.size()))) return rc;
       if ((rc = upload(ctx, &D.vnclass, H.vnc.data(), H.vnc.size()))) return rc;
       if ((rc = upload(ctx, &D.skip, H.skip.data(), H.skip.size()))) return rc;
+      if (!H.vt.empty() && (rc = upload(ctx, &D.vt, H.vt.data(), H.vt.size()))) return rc;
       if (hp.ncolors > 0) {
         D.ncolors = hp.ncolors;
         D.color_slices = hp.color_slices;
@@ -1387,6 +1449,16 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
       }
       }
       D.A.dms = D.L.dms = D.U.dms = D.dms;
+      // vector types: the interpolation reads no VECSKIP bits of edge rows
+      D.ntypes = ps.ntypes;
+      if (ps.ntypes > 1 && D.vt) {
+        if ((rc = dalloc(ctx, &D.iskip, std::max(D.n, 1)))) return rc;
+        launch_iskip(ctx->stream, D.n, D.vt, D.skip, D.iskip);
+        HIPCHK(hipGetLastError());
+      } else {
+        D.vt = nullptr;
+        D.iskip = D.skip;
+      }
       // scalar GS levels: the colour map of the whole-slab launches over A and U
       if (D.ncolors > 0 && D.ncolors <= kMaxColors && D.nb == 1 && D.A.nslices == D.U.nslices) {
         const int per = kBlock / kSlice, nblk = (D.A.nslices + per - 1) / per;
@@ -1508,7 +1580,7 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
       for (int l = bl; l <= level; ++l) mx = std::max(mx, ctx->host[l].a.size());
       if ((rc = dalloc(ctx, &ctx->stage, mx))) return rc;
     }
-    if (ps.fuse_rows > 0 && ctx->dev[bl].n <= 2048 && ctx->b == 1 && !ctx->grouped) {
+    if (ps.fuse_rows > 0 && ctx->dev[bl].n <= 2048 && ctx->b == 1 && !ctx->grouped && ps.ntypes == 1) {
       ctx->fuse_top = bl;
       for (int l = bl + 1; l <= level; ++l) {
         if (ctx->dev[l].n > ps.fuse_rows || ctx->dev[l].ncolors > kMaxColors) break;
@@ -1547,7 +1619,8 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
     if ((rc = upload(ctx, &ctx->clev64, c64.data(), c64.size()))) return rc;
     if ((rc = upload(ctx, &ctx->clev32, c32.data(), c32.size()))) return rc;
     ctx->part_cap = std::max(1024, (max_slices + 3) / 4 + 1);
-    const int nc = ctx->nrhs_cap * ctx->b;   // partial-sum columns: rhs x component
+    // partial-sum columns: rhs x norm entry (component, or type x component)
+    const int nc = ctx->nrhs_cap * ctx->b * ps.ntypes;
     if ((rc = dalloc(ctx, &ctx->part, (size_t)ctx->part_cap * nc))) return rc;
     if ((rc = dalloc(ctx, &ctx->red, std::max(8, nc)))) return rc;
     // restriction partials into the base level: one per block of 4 slices
@@ -1559,6 +1632,8 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
     return fail(ctx, UGMG_E_NOMEM, "host allocation during preprocess");
   }
   ctx->top = level;
+  ctx->ntypes = ps.ntypes;
+  ctx->nc = ps.ntypes * ctx->b;
   // FULLREFINELEVEL (SetSurfaceClasses, gm/algebra.cc:2885-2916): the lowest
   // level with a FINE_GRID_DOF vector, the top level if there is none
   ctx->frl = host_fullrefinelevel(ctx, level);
@@ -1695,6 +1770,7 @@ int ugmg_set_level(ugmg_ctx *ctx, int level, int n, const int *row_ptr, const in
     L.skip.assign(skip, skip + n);
     L.vc.assign(vclass, vclass + n);
     L.vnc.assign(vnclass, vnclass + n);
+    L.vt.clear();   // node vectors until ugmg_set_vtypes says otherwise
   } catch (const std::bad_alloc &) {
     return fail(ctx, UGMG_E_NOMEM, "host allocation");
   }
@@ -1797,6 +1873,26 @@ int ugmg_set_transfer(ugmg_ctx *ctx, int level, const int *p_ptr, const int *p_i
   return UGMG_OK;
 }
 
+int ugmg_set_vtypes(ugmg_ctx *ctx, int level, const uint8_t *vtype) {
+  if (!ctx) return UGMG_E_ARG;
+  if (level < 0 || level >= ctx->nlevels) return fail(ctx, UGMG_E_ARG, "level out of range");
+  if (!ctx->level_set[level]) return fail(ctx, UGMG_E_STATE, "set the level before its vector types");
+  HostLevel &L = ctx->host[level];
+  try {
+    if (!vtype) {
+      L.vt.clear();
+    } else {
+      for (int v = 0; v < L.n; ++v)
+        if (vtype[v] > 1) return fail(ctx, UGMG_E_ARG, "vector type must be 0 (node) or 1 (edge)");
+      L.vt.assign(vtype, vtype + L.n);
+    }
+  } catch (const std::bad_alloc &) {
+    return fail(ctx, UGMG_E_NOMEM, "host allocation");
+  }
+  free_device(ctx);
+  return UGMG_OK;
+}
+
 int ugmg_set_params(ugmg_ctx *ctx, const ugmg_params *p) {
   if (!ctx || !p) return UGMG_E_ARG;
   if (p->nu1 < 0 || p->nu2 < 0 || p->gamma < 0 || p->base_maxiter < 0 ||
@@ -1837,6 +1933,11 @@ int ugmg_postprocess(ugmg_ctx *ctx) {
 }
 
 int ugmg_block(const ugmg_ctx *ctx) { return ctx && ctx->nlevels > 0 ? ctx->b : UGMG_E_STATE; }
+
+int ugmg_norm_entries(const ugmg_ctx *ctx) {
+  if (!ctx) return UGMG_E_ARG;
+  return ctx->prepared ? ctx->nc : UGMG_E_STATE;
+}
 
 int ugmg_level_size(const ugmg_ctx *ctx, int level, int *n, int *nnzb) {
   if (!ctx || level < 0 || level >= ctx->nlevels || !ctx->level_set[level]) return UGMG_E_ARG;
@@ -1954,15 +2055,26 @@ int ugmg_smatmul_minus(ugmg_ctx *ctx, int level, float *x, const float *y) {
   return UGMG_OK;
 }
 
+// per-type sums read each row's VTYPE, kept in the level's device order: a
+// list-order argument of such a level moves to that order first (into the
+// level's scratch vector `tmp`)
+const double *typed_order(ugmg_ctx *ctx, int level, const double *x, double *tmp) {
+  const DevLevel &d = ctx->dev[level];
+  if (!d.vt || !d.perm) return x;
+  to_dev<double, double>(ctx, level, x, d.vs, tmp);
+  return tmp;
+}
+
 int ugmg_dnrm2x(ugmg_ctx *ctx, int level, const double *x, double *out) {
   int rc = check_ready(ctx, level);
   if (rc) return rc;
   if (!out) return fail(ctx, UGMG_E_ARG, "null output");
   DevLevel &d = ctx->dev[level];
+  x = typed_order(ctx, level, x, d.t);
   launch_dot<double>(ctx->stream, d.n, x, nullptr, nullptr, ctx->part, ctx->red, 1, B1(d));
-  HIPCHK(hipMemcpyAsync(ctx->red_host, ctx->red, sizeof(double) * d.nb, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(ctx->red_host, ctx->red, sizeof(double) * d.nc(), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
-  for (int i = 0; i < d.nb; ++i) out[i] = ctx->red_host[i];
+  for (int i = 0; i < d.nc(); ++i) out[i] = ctx->red_host[i];
   return UGMG_OK;
 }
 
@@ -1980,17 +2092,18 @@ int ugmg_dnrm2x_noalloc(ugmg_ctx *ctx, int level, const double *x, double *out) 
     return fail(ctx, UGMG_E_HIP, std::string("hipGetSymbolAddress(scratch): ") +
                                      hipGetErrorString(g_scratch_err[dev]));
   DevLevel &d = ctx->dev[level];
-  const long long need = (long long)reduce_parts(d.n) * d.nb;
-  if (need > kScratchParts || d.nb > UGMG_MAXB)
+  const long long need = (long long)reduce_parts(d.n) * d.nc();
+  if (need > kScratchParts || d.nc() > UGMG_MAXB)
     return fail(ctx, UGMG_E_ARG, "ugmg_dnrm2x_noalloc: " + std::to_string(need) +
                                      " partial sums exceed the scratch of " +
                                      std::to_string(kScratchParts));
   double *part = g_scratch_ptr[dev], *norms = g_scratch_ptr[dev] + kScratchParts;
+  x = typed_order(ctx, level, x, d.t);
   launch_dot<double>(ctx->stream, d.n, x, nullptr, nullptr, part, norms, 1, B1(d));
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(ctx->red_host, norms, sizeof(double) * d.nb, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(ctx->red_host, norms, sizeof(double) * d.nc(), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
-  for (int i = 0; i < d.nb; ++i) out[i] = ctx->red_host[i];
+  for (int i = 0; i < d.nc(); ++i) out[i] = ctx->red_host[i];
   return UGMG_OK;
 }
 
@@ -1999,10 +2112,12 @@ int ugmg_ddot(ugmg_ctx *ctx, int level, const double *x, const double *y, double
   if (rc) return rc;
   if (!out || !y) return fail(ctx, UGMG_E_ARG, "null argument");
   DevLevel &d = ctx->dev[level];
+  x = typed_order(ctx, level, x, d.t);
+  y = typed_order(ctx, level, y, d.u);
   launch_dot<double>(ctx->stream, d.n, x, y, nullptr, ctx->part, ctx->red, 0, B1(d));
-  HIPCHK(hipMemcpyAsync(ctx->red_host, ctx->red, sizeof(double) * d.nb, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(ctx->red_host, ctx->red, sizeof(double) * d.nc(), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
-  for (int i = 0; i < d.nb; ++i) out[i] = ctx->red_host[i];
+  for (int i = 0; i < d.nc(); ++i) out[i] = ctx->red_host[i];
   return UGMG_OK;
 }
 
@@ -2076,7 +2191,7 @@ int ugmg_interpolate(ugmg_ctx *ctx, int level, double *to_f, const double *from_
     to_dev<double, double>(ctx, level - 1, from_c, dc.vs, dc.c);
     cd = dc.c;
   }
-  launch_interp<double>(ctx->stream, d.P.d64(), d.skip, cd, damp ? damp[0] : 1.0, fd,
+  launch_interp<double>(ctx->stream, d.P.d64(), d.iskip, cd, damp ? damp[0] : 1.0, fd,
                         nullptr, d.nt, 1, d.vs, dc.vs, d.nb);
   if (d.perm) to_user<double, double>(ctx, level, fd, to_f, d.vs);
   HIPCHK(hipGetLastError());
@@ -2184,7 +2299,7 @@ int ugmg_solve(ugmg_ctx *ctx, int level, double *x, double *b, const double *abs
     ctx->stat_bytes += op_bytes(ctx, level, Op::Spmv, 8);
   }
   ctx->stat_bytes += vb + d.n;   // the first defect norm
-  const int nc = d.nb;
+  const int nc = d.nc();
   // sc_cmp (npscan.cc:1027-1057): every component below its limit
   auto sc_cmp = [nc](const double *x, const double *y) {
     for (int i = 0; i < nc; ++i)
@@ -2432,6 +2547,7 @@ int ugmg_solve_surface(ugmg_ctx *ctx, int level, double *const *x, double *const
   if (rc) return rc;
   if (!x || !b || !abslimit || !reduction || !res || maxiter < 0)
     return fail(ctx, UGMG_E_ARG, "null argument");
+  if (ctx->ntypes > 1) return fail(ctx, UGMG_E_ARG, "the surface solve takes one vector type");
   std::memset(res, 0, sizeof(*res));
   // the level's own NEW_DEFECT loop always runs, the FINE_GRID_DOF loop
   // from FULLREFINELEVEL up to the level below it
@@ -2685,7 +2801,8 @@ int ugmg_bcgs_solve(ugmg_ctx *ctx, int level, int nrhs, double *x, double *b, in
   if (!x || !b || !abslimit || !reduction || !res || maxiter < 0 || restart < 0 ||
       ld < ctx->dev[level].n || (defects && maxdef < 1))
     return fail(ctx, UGMG_E_ARG, "bad argument");
-  if (ctx->b != 1) return fail(ctx, UGMG_E_ARG, "bcgs runs on scalar (b = 1) hierarchies");
+  if (ctx->b != 1 || ctx->ntypes > 1)
+    return fail(ctx, UGMG_E_ARG, "bcgs runs on scalar (b = 1) hierarchies of one vector type");
   std::memset(res, 0, sizeof(*res) * nrhs);
   if (use_iter && ctx->base.failed) {
     for (int j = 0; j < nrhs; ++j) res[j].error_code = UGMG_E_SINGULAR;
@@ -2938,7 +3055,8 @@ int ugmg_layout_digest(ugmg_ctx *ctx, int level, unsigned long long *out, int no
   }
   if ((rc = dev(d.diag64, 8 * (size_t)n * bb * mb)) || (rc = dev(d.diag32, 4 * (size_t)n * bb * mb)) ||
       (rc = dev(d.vclass, n)) || (rc = dev(d.vnclass, n)) || (rc = dev(d.skip, 4 * (size_t)n)) ||
-      (rc = dev(d.c0, n)) || (rc = dev(d.perm, 4 * (size_t)n)))
+      (rc = dev(d.c0, n)) || (rc = dev(d.perm, 4 * (size_t)n)) || (rc = dev(d.vt, n)) ||
+      (rc = dev(d.iskip, 4 * (size_t)n)))
     return rc;
   // host-side description
   std::vector<long long> h;
@@ -3030,7 +3148,7 @@ int ugmg_time_kernel(ugmg_ctx *ctx, int level, int kind, int reps, double *ms, d
       }
       case 5: {
         DevLevel &dc = ctx->dev[level - 1];
-        launch_interp<double>(st, d.P.d64(), d.skip, dc.c, 1.0, d.t, d.c, d.nt, 1, d.vs, dc.vs, d.nb);
+        launch_interp<double>(st, d.P.d64(), d.iskip, dc.c, 1.0, d.t, d.c, d.nt, 1, d.vs, dc.vs, d.nb);
         break;
       }
       case 6:

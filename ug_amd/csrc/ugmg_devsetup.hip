@@ -553,14 +553,24 @@ __global__ void k_sell_fill(SellIn S, int ns, size_t nent, const int *__restrict
 
 // ----------------------------------------------------------- transfer P^T
 
+// the rows P^T takes: node rows of VCLASS >= 2 and every edge row (vt null:
+// node rows only)
 __global__ void k_entry_rows(int n, const int *__restrict__ pp, const uint8_t *__restrict__ vc,
-                             int *__restrict__ erow, int *__restrict__ flag) {
+                             const uint8_t *__restrict__ vt, int *__restrict__ erow,
+                             int *__restrict__ flag) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= n) return;
+  const bool take = vc[v] >= 2 || (vt && vt[v] == 1);
   for (int q = pp[v]; q < pp[v + 1]; ++q) {
     erow[q] = v;
-    flag[q] = vc[v] >= 2;
+    flag[q] = take;
   }
+}
+
+__global__ void k_perm_u8(int n, const int *__restrict__ perm, const uint8_t *__restrict__ src,
+                          uint8_t *__restrict__ dst) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) dst[p] = src[perm[p]];
 }
 
 __global__ void k_pt_keys(int m, const int *__restrict__ sel, const int *__restrict__ pi,
@@ -995,7 +1005,7 @@ int build_level_device(void *stream, const HostLevel &H, int coarse_n, const int
   if (has_p) {
     TRYRC(up(C, T, &pp, H.pp.data(), H.pp.size()));
     TRYRC(up(C, T, &pi, H.pi.data(), H.pi.size()));
-    TRYRC(up(C, T, &pw, H.pw.data(), H.pw.size()));
+    TRYRC(up(C, T, &pw, H.p_weights().data(), H.p_weights().size()));
   }
   // level in device order (GS levels) or as given
   const int *Lrp = rp, *Lci = ci, *Lpp = pp, *Lpi = pi;
@@ -1005,6 +1015,7 @@ int build_level_device(void *stream, const HostLevel &H, int coarse_n, const int
     TRYRC(up_own(C, &out.skip, H.skip.data(), H.skip.size()));
     TRYRC(up_own(C, &out.vc, H.vc.data(), H.vc.size()));
     TRYRC(up_own(C, &out.vnc, H.vnc.data(), H.vnc.size()));
+    if (!H.vt.empty()) TRYRC(up_own(C, &out.vt, H.vt.data(), H.vt.size()));
   } else {
     TRYRC(up(C, T, &skip, H.skip.data(), H.skip.size()));
     TRYRC(up(C, T, &vc, H.vc.data(), H.vc.size()));
@@ -1041,6 +1052,12 @@ int build_level_device(void *stream, const HostLevel &H, int coarse_n, const int
     TRYRC(excl_scan(C, T, len, rp2, n + 1));
     k_perm_rows<<<nblk(n), kT, 0, C.st>>>(n, bb, out.perm, out.pos, rp, ci, a, skip, vc, vnc, rp2, ci2,
                                           a2, out.skip, out.vc, out.vnc);
+    if (!H.vt.empty()) {
+      uint8_t *vt;
+      TRYRC(up(C, T, &vt, H.vt.data(), H.vt.size()));
+      TRYRC(C.own(&out.vt, (size_t)n));
+      k_perm_u8<<<nblk(n), kT, 0, C.st>>>(n, out.perm, vt, out.vt);
+    }
     Lrp = rp2;
     Lci = ci2;
     La = a2;
@@ -1049,7 +1066,7 @@ int build_level_device(void *stream, const HostLevel &H, int coarse_n, const int
       double *pw2;
       TRY(T.get(&pp2, (size_t)n + 1), "hipMalloc (permute)");
       TRY(T.get(&pi2, H.pi.size()), "hipMalloc (permute)");
-      TRY(T.get(&pw2, H.pw.size()), "hipMalloc (permute)");
+      TRY(T.get(&pw2, H.p_weights().size()), "hipMalloc (permute)");
       k_perm_len<<<nblk(n + 1), kT, 0, C.st>>>(n, out.perm, pp, len);
       TRYRC(excl_scan(C, T, len, pp2, n + 1));
       k_perm_p<<<nblk(n), kT, 0, C.st>>>(n, out.perm, pp, pi, pw, coarse_pos, pp2, pi2, pw2);
@@ -1103,8 +1120,8 @@ int build_level_device(void *stream, const HostLevel &H, int coarse_n, const int
     TRYRC(build_sell_dev(C, S, out.A));
   }
   if (has_p) {
-    // P^T rows: the entries of fine vectors with VCLASS >= 2, stably by
-    // coarse vector
+    // P^T rows: the entries of the fine vectors the restriction reads (node
+    // vectors of VCLASS >= 2, edge vectors), stably by coarse vector
     const int pn = H.pp[n];
     out.p_nnz = pn;
     int *erow, *flag, *iota, *sel, *nsel, *cnt, *ptr, *sorted;
@@ -1119,7 +1136,7 @@ int build_level_device(void *stream, const HostLevel &H, int coarse_n, const int
     TRY(T.get(&sorted, (size_t)pn), "hipMalloc (transfer)");
     TRY(T.get(&k0, (size_t)pn), "hipMalloc (transfer)");
     TRY(T.get(&k1, (size_t)pn), "hipMalloc (transfer)");
-    k_entry_rows<<<nblk(n), kT, 0, C.st>>>(n, Lpp, out.vc, erow, flag);
+    k_entry_rows<<<nblk(n), kT, 0, C.st>>>(n, Lpp, out.vc, out.vt, erow, flag);
     k_iota<<<nblk(pn), kT, 0, C.st>>>(pn, iota);
     int m = 0;
     if (pn > 0) {

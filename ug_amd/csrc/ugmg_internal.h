@@ -23,6 +23,15 @@ struct HostLevel {
   bool has_p = false;
   std::vector<int> pp, pi;
   std::vector<double> pw;
+  // vector types (VTYPE: 0 node, 1 edge; empty: all node vectors)
+  std::vector<uint8_t> vt;
+  // the interpolation weights the layouts use when they differ from pw
+  // (edge rows: 0 for a father object whose components are all Dirichlet)
+  std::vector<double> pwf;
+  const std::vector<double> &p_weights() const { return pwf.empty() ? pw : pwf; }
+  // P^T takes node rows of VCLASS >= 2 and every edge row (transgrid.cc:153,
+  // 188-212)
+  bool restricts(int v) const { return vc[v] >= 2 || (!vt.empty() && vt[v] == 1); }
 };
 
 // SELL-64 layout of a set of rows, built on the host.
@@ -109,6 +118,7 @@ struct DevPrep {
   DevSellParts A, L, U, P, PT;
   double *diag = nullptr;             // n b*b
   uint8_t *vc = nullptr, *vnc = nullptr, *c0 = nullptr;   // device order
+  uint8_t *vt = nullptr;                                   // device order (typed levels)
   uint32_t *skip = nullptr;
   int p_nnz = 0, pt_nnz = 0;
   bool all_active = false;

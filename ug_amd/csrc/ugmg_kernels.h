@@ -94,6 +94,11 @@ struct Batch {
   // inverse at inv + j * ims (0: shared by every column)
   size_t dms = 0;
   size_t ims = 0;
+  // vector types (VTYPE per row, nt of them): the norms and dot products
+  // have nt * b entries per column, type-major; null: one type
+  const uint8_t *vt = nullptr;
+  int nt = 1;
+  int nc() const { return vt ? nt * b : b; }
 };
 
 // per right-hand side state of the batched BCGSSolver (np/procs/ls.cc:
@@ -219,6 +224,9 @@ void launch_reduce_final(hipStream_t st, int nparts, const double *part, double 
                          int nrhs);
 template <typename T>
 void launch_fill(hipStream_t st, int n, T *x, T val, Batch bt);
+// the VECSKIP bits the interpolation reads: an edge row's (vt 1) own bits
+// are not read (StandardIntCorNodeVector's edge loop), iskip 0
+void launch_iskip(hipStream_t st, int n, const uint8_t *vt, const uint32_t *skip, uint32_t *iskip);
 template <typename TD, typename TS>
 void launch_convert(hipStream_t st, int n, const TS *s, TD *d, int nrhs, size_t vss, size_t vsd);
 template <typename T>

@@ -21,6 +21,8 @@
 
 #include "ugmg_kernels.h"
 
+#include "../../include/ugmg.h"
+
 namespace ugmg {
 
 __device__ __forceinline__ int xcd_remap(int bid, int nb) {
@@ -1052,6 +1054,34 @@ __global__ __launch_bounds__(kBlock) void k_dot_part_bk(int n, const T *__restri
     for (int q = 0; q < B; ++q) part[((size_t)blockIdx.y * B + q) * gridDim.x + blockIdx.x] = s[q];
 }
 
+// partial sums per (type, component) of vectors with several types (dnrm2x,
+// ddot: a[aoff[VTYPE(v)] + i], ugblas.cc:3104-3176): part[(column * nc + k)
+// * gridDim.x + block], k = vt[row] * b + component, nc <= UGMG_MAXB
+template <typename T, int DOT>
+__global__ __launch_bounds__(kBlock) void k_dot_part_typed(int n, int b, int nc, const T *__restrict__ x,
+                                                            const T *__restrict__ y,
+                                                            const uint8_t *__restrict__ vclass,
+                                                            const uint8_t *__restrict__ vt,
+                                                            double *__restrict__ part, size_t vs) {
+  __shared__ double red[UGMG_MAXB][kBlock / 64];
+  const size_t o = (size_t)blockIdx.y * vs;
+  double s[UGMG_MAXB] = {};
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    if (vclass && vclass[i] < 2) continue;
+    const int k0 = vt[i] * b;
+    for (int q = 0; q < b; ++q) {
+      const double xi = (double)x[o + (size_t)b * i + q];
+      const double v = DOT ? xi * (double)y[o + (size_t)b * i + q] : xi * xi;
+#pragma unroll
+      for (int k = 0; k < UGMG_MAXB; ++k)
+        if (k == k0 + q) s[k] += v;
+    }
+  }
+  block_sum_b<UGMG_MAXB>(s, red);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < nc; ++k) part[((size_t)blockIdx.y * nc + k) * gridDim.x + blockIdx.x] = s[k];
+}
+
 // lane's share of row . b for the base GEMV: four independent accumulators
 // keep four loads of the row in flight per lane (the wave sums the shares)
 // (rr's columns are in the order cp: column j multiplies b[cp[j]]; the
@@ -1091,7 +1121,8 @@ __device__ __forceinline__ double base_row_or_unit(const TI *__restrict__ inv,
 }
 
 // ls over ex on the base level with per-component stop tests (sc_cmp:
-// every component below its limit); inv is (3n) x (3n)
+// every one of the B norm entries below its limit: the components, or the
+// (type, component) pairs of several vector types); inv over scalar unknowns
 template <int B, typename T, typename TI>
 __global__ __launch_bounds__(kBlock) void k_base_gemv_bk(int n3, const TI *__restrict__ inv,
                                                           const int *__restrict__ rowmap,
@@ -1786,9 +1817,11 @@ void launch_base_gemv(hipStream_t st, int n, const TI *inv, const int *rowmap, c
                       double abslimit, double red, const double *part, int nparts, double *first,
                       int it, Batch bt) {
   if (n <= 0) return;
-  if (bt.b > 1) {   // n counts scalar unknowns; one wavefront per row
+  // the stop tests read nc norm entries per column (components, or
+  // (type, component) pairs); n counts scalar unknowns
+  if (bt.nc() > 1) {   // one wavefront per row
     const int g = (n * 64 + kBlock - 1) / kBlock;
-    BK_DISPATCH(bt.b, k_base_gemv_bk<BK, T, TI><<<grid2(g, bt.nrhs), kBlock, 0, st>>>(
+    BK_DISPATCH(bt.nc(), k_base_gemv_bk<BK, T, TI><<<grid2(g, bt.nrhs), kBlock, 0, st>>>(
                           n, inv, rowmap, cp, ncols, b, c, xout, cmode, damp, abslimit, red, part, nparts,
                           first, it, bt.vs, bt.ims));
   } else {           // one workgroup per row
@@ -1853,7 +1886,10 @@ template <typename T>
 int launch_dot_part(hipStream_t st, int n, const T *x, const uint8_t *vclass, double *part,
                     Batch bt) {
   const int g = reduce_parts(n);
-  if (bt.b > 1)
+  if (bt.vt)
+    k_dot_part_typed<T, 0><<<grid2(g, bt.nrhs), kBlock, 0, st>>>(n, bt.b, bt.nc(), x, nullptr, vclass, bt.vt,
+                                                                 part, bt.vs);
+  else if (bt.b > 1)
     BK_DISPATCH(bt.b, k_dot_part_bk<BK, T, 0><<<grid2(g, bt.nrhs), kBlock, 0, st>>>(n, x, nullptr, vclass, part,
                                                                                    bt.vs));
   else k_dot_part<T, 0><<<grid2(g, bt.nrhs), kBlock, 0, st>>>(n, x, nullptr, vclass, part, bt.vs);
@@ -1901,6 +1937,12 @@ void launch_dot(hipStream_t st, int n, const T *x, const T *y, const uint8_t *vc
                 double *part, double *out, int do_sqrt, Batch bt) {
   const int g = reduce_parts(n);
   const dim3 gg = grid2(g, bt.nrhs);
+  if (bt.vt) {
+    if (y) k_dot_part_typed<T, 1><<<gg, kBlock, 0, st>>>(n, bt.b, bt.nc(), x, y, vclass, bt.vt, part, bt.vs);
+    else k_dot_part_typed<T, 0><<<gg, kBlock, 0, st>>>(n, bt.b, bt.nc(), x, y, vclass, bt.vt, part, bt.vs);
+    k_reduce_final<<<gg.y * bt.nc(), 1024, 0, st>>>(g, part, out, do_sqrt);
+    return;
+  }
   if (bt.b > 1) {
     BK_DISPATCH(bt.b, if (y) k_dot_part_bk<BK, T, 1><<<gg, kBlock, 0, st>>>(n, x, y, vclass, part, bt.vs);
                       else k_dot_part_bk<BK, T, 0><<<gg, kBlock, 0, st>>>(n, x, y, vclass, part, bt.vs));
@@ -1923,6 +1965,16 @@ template int launch_dot_part<float>(hipStream_t, int, const float *, const uint8
 void launch_reduce_final(hipStream_t st, int nparts, const double *part, double *out, int do_sqrt,
                          int nrhs) {
   k_reduce_final<<<nrhs < 1 ? 1 : nrhs, 1024, 0, st>>>(nparts, part, out, do_sqrt);
+}
+
+__global__ __launch_bounds__(kBlock) void k_iskip(int n, const uint8_t *__restrict__ vt,
+                                                   const uint32_t *__restrict__ skip,
+                                                   uint32_t *__restrict__ iskip) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) iskip[i] = vt[i] == 1 ? 0u : skip[i];
+}
+void launch_iskip(hipStream_t st, int n, const uint8_t *vt, const uint32_t *skip, uint32_t *iskip) {
+  if (n > 0) k_iskip<<<nblk(n), kBlock, 0, st>>>(n, vt, skip, iskip);
 }
 
 template <typename T>

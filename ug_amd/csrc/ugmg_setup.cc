@@ -259,9 +259,11 @@ void permute_level(const HostLevel &L, const std::vector<int> &perm, const std::
   out.skip.resize(n);
   out.vc.resize(n);
   out.vnc.resize(n);
+  out.vt.resize(L.vt.empty() ? 0 : n);
 #pragma omp parallel for schedule(static)
   for (int p = 0; p < n; ++p) {
     const int r = perm[p];
+    if (!L.vt.empty()) out.vt[p] = L.vt[r];
     int q = out.rp[p];
     for (int m = L.rp[r]; m < L.rp[r + 1]; ++m, ++q) {
       out.ci[q] = pos[L.ci[m]];
@@ -277,6 +279,7 @@ void permute_level(const HostLevel &L, const std::vector<int> &perm, const std::
     for (int p = 0; p < n; ++p) out.pp[p + 1] = out.pp[p] + (L.pp[perm[p] + 1] - L.pp[perm[p]]);
     out.pi.resize(L.pi.size());
     out.pw.resize(L.pw.size());
+    out.pwf.resize(L.pwf.size());
 #pragma omp parallel for schedule(static)
     for (int p = 0; p < n; ++p) {
       const int r = perm[p];
@@ -284,6 +287,7 @@ void permute_level(const HostLevel &L, const std::vector<int> &perm, const std::
       for (int m = L.pp[r]; m < L.pp[r + 1]; ++m, ++q) {
         out.pi[q] = cpos ? (*cpos)[L.pi[m]] : L.pi[m];
         out.pw[q] = L.pw[m];
+        if (!L.pwf.empty()) out.pwf[q] = L.pwf[m];
       }
     }
   }
@@ -415,23 +419,24 @@ void build_prep(const HostLevel &L, const HostLevel *coarse, const std::vector<i
     const int nc = coarse->n;
     out.pt_ptr.assign(nc + 1, 0);
     for (int v = 0; v < L.n; ++v) {
-      if (L.vc[v] < 2) continue;
+      if (!L.restricts(v)) continue;
       for (int q = L.pp[v]; q < L.pp[v + 1]; ++q) ++out.pt_ptr[L.pi[q] + 1];
     }
     for (int i = 0; i < nc; ++i) out.pt_ptr[i + 1] += out.pt_ptr[i];
     out.pt_idx.resize(out.pt_ptr[nc]);
     out.pt_w.resize(out.pt_ptr[nc]);
     std::vector<int> fill(out.pt_ptr.begin(), out.pt_ptr.end() - 1);
+    const std::vector<double> &W = L.p_weights();
     for (int v = 0; v < L.n; ++v) {
-      if (L.vc[v] < 2) continue;
+      if (!L.restricts(v)) continue;
       for (int q = L.pp[v]; q < L.pp[v + 1]; ++q) {
         int c = L.pi[q];
         out.pt_idx[fill[c]] = v;
-        out.pt_w[fill[c]] = L.pw[q];
+        out.pt_w[fill[c]] = W[q];
         ++fill[c];
       }
     }
-    build_sell_csr(L.n, L.pp.data(), L.pi.data(), L.pw.data(), out.P);
+    build_sell_csr(L.n, L.pp.data(), L.pi.data(), W.data(), out.P);
     build_sell_csr(nc, out.pt_ptr.data(), out.pt_idx.data(), out.pt_w.data(), out.PT);
   }
 }

@@ -784,7 +784,8 @@ def reordered(P: Problem, seed: int, newest_first: bool = True) -> Problem:
             rp[q + 1] = rp[q] + (hi - lo)
         new = Level(n=lv.n, b=b, row_ptr=rp, col_idx=np.asarray(ci, np.int32),
                     vals=np.ascontiguousarray(np.concatenate(va) if va else np.zeros(0)),
-                    skip=lv.skip[pm].copy(), vclass=lv.vclass[pm].copy(), vnclass=lv.vnclass[pm].copy())
+                    skip=lv.skip[pm].copy(), vclass=lv.vclass[pm].copy(), vnclass=lv.vnclass[pm].copy(),
+                    vtype=None if lv.vtype is None else lv.vtype[pm].copy())
         if l > 0:
             cinv = invs[l - 1]
             pp = np.zeros(lv.n + 1, np.int32)

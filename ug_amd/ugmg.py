@@ -86,6 +86,8 @@ def load():
         "ugmg_set_nlevels": ([vp, ip, ip], ip),
         "ugmg_set_level": ([vp, ip, ip, vp, vp, vp, vp, vp, vp], ip),
         "ugmg_set_transfer": ([vp, ip, vp, vp, vp], ip),
+        "ugmg_set_vtypes": ([vp, ip, vp], ip),
+        "ugmg_norm_entries": ([vp], ip),
         "ugmg_set_params": ([vp, C.POINTER(Params)], ip),
         "ugmg_preprocess": ([vp, ip], ip),
         "ugmg_postprocess": ([vp], ip),
@@ -230,6 +232,10 @@ class Context:
             arrs = [np.ascontiguousarray(lv.p_ptr, np.int32), np.ascontiguousarray(lv.p_idx, np.int32),
                     np.ascontiguousarray(lv.p_w, np.float64)]
             self._chk(self.lib.ugmg_set_transfer(self.h, l, *[a.ctypes.data for a in arrs]))
+        for l, lv in enumerate(levels):
+            if getattr(lv, "vtype", None) is not None:
+                vt = np.ascontiguousarray(lv.vtype, np.uint8)
+                self._chk(self.lib.ugmg_set_vtypes(self.h, l, vt.ctypes.data))
         self.problem = problem
         self.nlevels = len(levels)
 
@@ -309,7 +315,11 @@ class Context:
         self._chk(self.lib.ugmg_smatmul_minus(self.h, level, _ptr(x), _ptr(y)))
 
     def _ncomp(self):
-        return self.problem.b if self.problem is not None else 1
+        """Entries of a norm: components, times the vector types."""
+        r = self.lib.ugmg_norm_entries(self.h)
+        if r < 0:
+            self._chk(r)
+        return r
 
     def dnrm2x(self, level, x):
         """Per-component norm (a float for scalar hierarchies)."""

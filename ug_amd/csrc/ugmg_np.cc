@@ -555,7 +555,10 @@ int ugmg_np_execute(ugmg_np_env *env, const char *name, const char *flags) {
     o->res[0] = r;
   }
   if (want_s) {
-    const int nb = std::max(1, ugmg_block(env->ctx));
+    // VD_NCOMP: the entries of the norms (components, times the vector
+    // types of a node + edge hierarchy)
+    const int ne = ugmg_norm_entries(env->ctx);
+    const int nb = ne > 0 ? ne : std::max(1, ugmg_block(env->ctx));
     const auto t0 = std::chrono::steady_clock::now();
     if (o->cls == "ls") {
       // LinearSolver's display (ls.cc:666-744): DoPCR(PCR_CRATE_SD) on the

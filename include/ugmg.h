@@ -123,6 +123,21 @@ int ugmg_set_level(ugmg_ctx *ctx, int level, int n, const int *row_ptr,
                    const uint8_t *vnclass);
 int ugmg_set_transfer(ugmg_ctx *ctx, int level, const int *p_ptr,
                       const int *p_idx, const double *p_w);
+/* The interpolation matrix of `level` (UG's transfer $M: RestrictByMatrix /
+ * InterpolateCorrectionByMatrix, transgrid.cc:1160-1380): per fine vector the
+ * same rows as ugmg_set_transfer, each entry a b x b block (b*b doubles,
+ * p_blocks[q*b*b + i*b + j] = coarse component i, fine component j: MVALUE
+ * of the IMAT entry).  Restriction b_c,i += sum_j M_ij b_f,j over fine
+ * vectors of VCLASS >= 2 on the coarse components whose VECSKIP bit is
+ * clear (coarse rows of VNCLASS >= 2 reset first, $damp scaling them after
+ * the sum); interpolation t_f,i = $damp sum_j M_ji c_j on the fine
+ * components whose VECSKIP bit is clear.  Replaces the level's weights;
+ * ugmg_set_transfer returns it to the standard transfer.  Levels with a
+ * matrix are laid out on the host; served groups and ugmg_galerkin take
+ * scalar weights.  b = 1: the scalar kernels (a $damp other than 1 in
+ * ugmg_restrict scales each term). */
+int ugmg_set_transfer_matrix(ugmg_ctx *ctx, int level, const int *p_ptr, const int *p_idx,
+                             const double *p_blocks);
 /* Vector types of a level's vectors (VTYPE, np/udm/udm.h:105-108): 0 node
  * vectors, 1 edge vectors, each with the b components of the hierarchy;
  * NULL (or never called): node vectors only.  Call after ugmg_set_level

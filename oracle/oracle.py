@@ -26,7 +26,7 @@ class OrcLevel(C.Structure):
                 ("rp", C.c_void_p), ("ci", C.c_void_p), ("a", C.c_void_p),
                 ("skip", C.c_void_p), ("vclass", C.c_void_p), ("vnclass", C.c_void_p),
                 ("pp", C.c_void_p), ("pi", C.c_void_p), ("pw", C.c_void_p),
-                ("vtype", C.c_void_p), ("ntypes", C.c_int)]
+                ("vtype", C.c_void_p), ("ntypes", C.c_int), ("pblk", C.c_void_p)]
 
 
 class OrcParams(C.Structure):
@@ -109,6 +109,8 @@ class Hierarchy:
                               pw=np.ascontiguousarray(lv.p_w, np.float64))
             if getattr(lv, "vtype", None) is not None:
                 fields.update(vtype=np.ascontiguousarray(lv.vtype, np.uint8))
+            if getattr(lv, "p_blk", None) is not None:
+                fields.update(pblk=np.ascontiguousarray(lv.p_blk, np.float64))
             self._keep.append(fields)
             arr[i].n, arr[i].b, arr[i].ntypes = lv.n, lv.b, self.ntypes
             for k, v in fields.items():

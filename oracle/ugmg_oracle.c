@@ -341,8 +341,74 @@ int orc_smoother(const orc_level *L, int kind, const int *rank,
 
 /* -------------------------------------------------------------- transfer */
 
+/* CheckDamp (transgrid.cc): 1 iff some entry differs from 1 */
+static int check_damp(int n, const double *damp) {
+  for (int i = 0; i < n; ++i)
+    if (damp[i] != 1.0) return 1;
+  return 0;
+}
+
+/* RestrictByMatrix_General with flag 0 (transgrid.cc:1160-1232; no CRITBIT
+ * is set here): reset the coarse rows of VNCLASS >= 2, add M b_f over the
+ * fine vectors of VCLASS >= 2 on the coarse components whose VECSKIP bit is
+ * clear, then scale the coarse rows of VNCLASS >= 2 by damp */
+static void restrict_by_matrix(const orc_level *F, const orc_level *C, double *to_c,
+                               const double *from_f, const double *damp) {
+  const int b = F->b, bb = b * b;
+  for (int w = 0; w < C->n; ++w)
+    if (C->vnclass[w] >= NEWDEF_CLASS)
+      for (int i = 0; i < b; ++i) to_c[(size_t)w * b + i] = 0.0;
+  for (int v = 0; v < F->n; ++v) {
+    if (F->vclass[v] < NEWDEF_CLASS) continue;
+    const double *vp = from_f + (size_t)v * b;
+    for (int q = F->pp[v]; q < F->pp[v + 1]; ++q) {
+      const int w = F->pi[q];
+      const double *M = F->pblk + (size_t)q * bb;
+      const uint32_t vecskip = C->skip[w];
+      for (int i = 0; i < b; ++i) {
+        if (vecskip & (1u << i)) continue;
+        double sum = 0.0;
+        for (int j = 0; j < b; ++j) sum += M[i * b + j] * vp[j];
+        to_c[(size_t)w * b + i] += sum;
+      }
+    }
+  }
+  if (check_damp(ncomp_of(C), damp))
+    for (int w = 0; w < C->n; ++w)
+      if (C->vnclass[w] >= NEWDEF_CLASS)
+        for (int i = 0; i < b; ++i) to_c[(size_t)w * b + i] *= damp[aoff(C, w) + i];
+}
+
+/* InterpolateCorrectionByMatrix_General with flag 0 (transgrid.cc:1300-
+ * 1380): dset(to, 0); t_v,i += sum_j M_ji c_j on the fine components whose
+ * VECSKIP bit is clear; dscalx(damp) when some damp differs from 1 */
+static void interpolate_by_matrix(const orc_level *F, double *to_f, const double *from_c,
+                                  const double *damp) {
+  const int b = F->b, bb = b * b;
+  for (size_t i = 0; i < (size_t)F->n * b; ++i) to_f[i] = 0.0;
+  for (int v = 0; v < F->n; ++v) {
+    const uint32_t vecskip = F->skip[v];
+    double *vp = to_f + (size_t)v * b;
+    for (int q = F->pp[v]; q < F->pp[v + 1]; ++q) {
+      const double *M = F->pblk + (size_t)q * bb;
+      const double *wp = from_c + (size_t)F->pi[q] * b;
+      for (int i = 0; i < b; ++i) {
+        if (vecskip & (1u << i)) continue;
+        double sum = 0.0;
+        for (int j = 0; j < b; ++j) sum += M[j * b + i] * wp[j];
+        vp[i] += sum;
+      }
+    }
+  }
+  if (check_damp(ncomp_of(F), damp)) dscalx(F, to_f, damp);
+}
+
 void orc_restrict(const orc_level *F, const orc_level *C, double *to_c,
                   const double *from_f, const double *damp) {
+  if (F->pblk) {
+    restrict_by_matrix(F, C, to_c, from_f, damp);
+    return;
+  }
   const int b = F->b;
   for (int v = 0; v < C->n; ++v)
     if (C->vnclass[v] >= NEWDEF_CLASS)
@@ -378,6 +444,10 @@ void orc_interpolate(const orc_level *F, const orc_level *C, double *to_f,
                      const double *from_c, const double *damp) {
   const int b = F->b;
   (void)C;
+  if (F->pblk) {
+    interpolate_by_matrix(F, to_f, from_c, damp);
+    return;
+  }
   for (size_t i = 0; i < (size_t)F->n * b; ++i) to_f[i] = 0.0;
   for (int v = 0; v < F->n; ++v) {
     if (vtype_of(F, v) != 0) continue;

@@ -50,6 +50,9 @@ typedef struct {
   const uint8_t *vtype;   /* VTYPE per vector, NULL: all node vectors     */
   int ntypes;             /* types with components (0 read as 1); the     */
                           /* norms have ntypes * b entries                */
+  const double *pblk;     /* interpolation matrix (transfer $M): a b x b  */
+                          /* block per entry of pp/pi, [i*b+j] = coarse   */
+                          /* component i, fine component j; NULL: pw      */
 } orc_level;
 
 enum { ORC_JAC = 0, ORC_GS = 1, ORC_MCGS = 2 };
@@ -94,7 +97,9 @@ int orc_smoother(const orc_level *L, int kind, const int *rank,
 /* StandardRestrictNodeVector / StandardIntCorNodeVector
  * (np/algebra/transgrid.cc:117-215, 235-336): the node loop, then the edge
  * loop over the edge vectors (no VCLASS test; each father object's VECSKIP
- * bits mask its term; the fine edge vector's own bits are not read) */
+ * bits mask its term; the fine edge vector's own bits are not read).
+ * With F->pblk: RestrictByMatrix / InterpolateCorrectionByMatrix
+ * (transgrid.cc:1160-1232, 1300-1380) over the interpolation matrix */
 void orc_restrict(const orc_level *F, const orc_level *C, double *to_c,
                   const double *from_f, const double *damp);
 void orc_interpolate(const orc_level *F, const orc_level *C, double *to_f,

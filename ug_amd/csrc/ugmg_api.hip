@@ -111,6 +111,7 @@ struct DevSellBuf {
   DSell<TV> view(const TV *v) const {
     DSell<TV> d{nslices, soff, rows, col, v, sdoff, dtab, wmax, nent, mst, dms};
     d.ncb = ncb;
+    d.bb = bb;
     for (int k = 0; k <= ncb && k <= kMaxColors; ++k) d.cb[k] = cb[k];
     return d;
   }
@@ -1067,6 +1068,7 @@ static int resolve_baselevel(const ugmg_ctx *ctx, int level) {
 struct PrepState {
   int level = -1, bl = 0, fuse_rows = 0;
   int ntypes = 1;   // vector types of the hierarchy
+  bool mats = false;   // some level has an interpolation matrix
   double nt_bytes = 64.0e6;
   // layouts built on the device (build_level_device) in prep_device, or on
   // the host here (hp, plv)
@@ -1123,13 +1125,20 @@ int prep_host(ugmg_ctx *ctx, int level, PrepState &ps) {
           p.base_abslimit[i] != p.base_abslimit[0] || p.base_red[i] != p.base_red[0])
         return fail(ctx, UGMG_E_ARG, "one damping / base limit for every vector type and component");
   }
+  // interpolation matrices: laid out on the host (the device build takes
+  // scalar weights); not in served groups
+  bool mats = false;
+  for (int l = bl + 1; l <= level; ++l) mats |= !ctx->host[l].pblk.empty();
+  if (mats && ctx->grouped) return fail(ctx, UGMG_E_ARG, "served contexts take the standard transfer");
+  ps.mats = mats;
   try {
     const uint32_t full = ctx->b >= 32 ? ~0u : (1u << ctx->b) - 1u;
     for (int l = bl; l <= level; ++l) {
       HostLevel &F = ctx->host[l];
       F.pwf.clear();
       if (nt > 1 && F.vt.empty()) F.vt.assign(F.n, 0);   // node vectors
-      if (l == bl || F.vt.empty() || !F.has_p) continue;
+      // the matrix transfer has no edge branch: its entries cover every type
+      if (l == bl || F.vt.empty() || !F.has_p || !F.pblk.empty()) continue;
       // StandardIntCorNodeVector's edge loop adds a father object's value
       // only where that object's VECSKIP bit is clear (transgrid.cc:317-336):
       // an object Dirichlet in every component contributes nothing (weight
@@ -1175,6 +1184,7 @@ int prep_host(ugmg_ctx *ctx, int level, PrepState &ps) {
     if (!std::strcmp(env_setup, "host")) ps.device = false;
     else if (!std::strcmp(env_setup, "device")) ps.device = !ctx->grouped;
   }
+  if (ps.mats) ps.device = false;
   try {
     ps.hp.assign(level + 1, HostPrep());
     std::vector<HostLevel> &plv = ps.plv;          // levels in device order
@@ -1451,12 +1461,13 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
       D.A.dms = D.L.dms = D.U.dms = D.dms;
       // vector types: the interpolation reads no VECSKIP bits of edge rows
       D.ntypes = ps.ntypes;
-      if (ps.ntypes > 1 && D.vt) {
+      const bool mat = l > bl && !ctx->host[l].pblk.empty();
+      if (ps.ntypes > 1 && D.vt && !mat) {
         if ((rc = dalloc(ctx, &D.iskip, std::max(D.n, 1)))) return rc;
         launch_iskip(ctx->stream, D.n, D.vt, D.skip, D.iskip);
         HIPCHK(hipGetLastError());
       } else {
-        D.vt = nullptr;
+        if (ps.ntypes == 1) D.vt = nullptr;
         D.iskip = D.skip;
       }
       // scalar GS levels: the colour map of the whole-slab launches over A and U
@@ -1864,12 +1875,38 @@ int ugmg_set_transfer(ugmg_ctx *ctx, int level, const int *p_ptr, const int *p_i
     for (int q = 0; q < nnz; ++q)
       if (L.pi[q] < 0 || L.pi[q] >= nc) return fail(ctx, UGMG_E_ARG, "coarse index out of range");
     L.pw.assign(p_w, p_w + nnz);
+    L.pblk.clear();
   } catch (const std::bad_alloc &) {
     return fail(ctx, UGMG_E_NOMEM, "host allocation");
   }
   L.has_p = true;
   ctx->transfer_set[level] = 1;
   free_device(ctx);
+  return UGMG_OK;
+}
+
+int ugmg_set_transfer_matrix(ugmg_ctx *ctx, int level, const int *p_ptr, const int *p_idx,
+                             const double *p_blocks) {
+  if (!ctx) return UGMG_E_ARG;
+  if (level >= 1 && level < ctx->nlevels && ctx->host[level].n > 0 && !p_blocks)
+    return fail(ctx, UGMG_E_ARG, "null array");
+  // the pattern as ugmg_set_transfer checks it, with unit weights
+  std::vector<double> ones;
+  try {
+    if (level >= 1 && level < ctx->nlevels && p_ptr && ctx->host[level].n >= 0)
+      ones.assign(std::max(0, p_ptr[ctx->host[level].n]), 1.0);
+  } catch (const std::bad_alloc &) {
+    return fail(ctx, UGMG_E_NOMEM, "host allocation");
+  }
+  int rc = ugmg_set_transfer(ctx, level, p_ptr, p_idx, ones.data());
+  if (rc) return rc;
+  HostLevel &L = ctx->host[level];
+  const size_t bb = (size_t)ctx->b * ctx->b;
+  try {
+    L.pblk.assign(p_blocks, p_blocks + L.pi.size() * bb);
+  } catch (const std::bad_alloc &) {
+    return fail(ctx, UGMG_E_NOMEM, "host allocation");
+  }
   return UGMG_OK;
 }
 
@@ -2897,6 +2934,8 @@ int ugmg_galerkin(ugmg_ctx *ctx, int level) {
   if (!ctx->level_set[level] || !ctx->level_set[level - 1] || !ctx->transfer_set[level])
     return fail(ctx, UGMG_E_STATE, "set both levels and the transfer first");
   HostLevel &F = ctx->host[level], &Cl = ctx->host[level - 1];
+  if (!F.pblk.empty())
+    return fail(ctx, UGMG_E_ARG, "the Galerkin product takes scalar interpolation weights (ugmg_set_transfer)");
   const int bb = ctx->b * ctx->b;
   // P^T over every fine vector (the coarse matrix takes all of them)
   std::vector<int> ptp(Cl.n + 1, 0), pti, fillp;

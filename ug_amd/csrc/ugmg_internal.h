@@ -29,9 +29,14 @@ struct HostLevel {
   // (edge rows: 0 for a father object whose components are all Dirichlet)
   std::vector<double> pwf;
   const std::vector<double> &p_weights() const { return pwf.empty() ? pw : pwf; }
-  // P^T takes node rows of VCLASS >= 2 and every edge row (transgrid.cc:153,
-  // 188-212)
-  bool restricts(int v) const { return vc[v] >= 2 || (!vt.empty() && vt[v] == 1); }
+  // an interpolation matrix (RestrictByMatrix / InterpolateCorrectionByMatrix):
+  // a b x b block per entry of pp/pi, M[i*b+j] = coarse component i, fine
+  // component j (MVALUE of the IMAT entry); empty: the scalar weights pw
+  std::vector<double> pblk;
+  // P^T takes node rows of VCLASS >= 2 and every edge row of the standard
+  // transfer (transgrid.cc:153, 188-212); with a matrix, every row of VCLASS
+  // >= 2 (transgrid.cc:1180)
+  bool restricts(int v) const { return vc[v] >= 2 || (pblk.empty() && !vt.empty() && vt[v] == 1); }
 };
 
 // SELL-64 layout of a set of rows, built on the host.
@@ -148,7 +153,7 @@ int color_device(void *stream, int n, const int *rp, const int *ci, int *colors,
 
 void build_sell(const HostLevel &L, const std::vector<int> *rows, Part part,
                 const std::vector<int> *color, HostSell &out);
-void build_sell_csr(int n, const int *rp, const int *ci, const double *a, HostSell &out);
+void build_sell_csr(int n, const int *rp, const int *ci, const double *a, HostSell &out, int bb = 1);
 int greedy_color(const HostLevel &L, std::vector<int> &color);
 void to_float(const double *in, size_t n, hvec<float> &out);
 // L (and its transfer) in device order: row p of out is row perm[p] of L,

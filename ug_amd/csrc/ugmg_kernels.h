@@ -38,6 +38,9 @@ struct DSell {
   // = all blocks); ncb = 0: the plain XCD remap
   int ncb = 0;
   int cb[kMaxColors + 1] = {};
+  // value planes per entry: 1 (scalar weights), or b*b (a b x b block per
+  // entry: matrices, and the transfer slabs of an interpolation matrix)
+  int bb = 1;
 };
 
 // base-level solver settings: ls($m,$red,$abslimit) over ex($damp)

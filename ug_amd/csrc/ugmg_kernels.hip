@@ -908,7 +908,12 @@ __global__ __launch_bounds__(BS) void k_gs_color_fused_bk(DSell<TV> L, DSell<TV>
   for (int q = 0; q < B; ++q) v[(size_t)B * r + q] = x[q];
 }
 
-template <int B, typename T, int NT>
+// MAT: RestrictByMatrix (transgrid.cc:1160-1232), a b x b block M per entry
+// (coarse component i, fine component j): b_c,i += sum_j M_ij b_f,j on the
+// components whose VECSKIP bit is clear, and $damp scales only the coarse
+// rows of VNCLASS >= 2 after the sum; otherwise the scalar weights of
+// StandardRestrictNodeVector (every term times damp)
+template <int B, typename T, int NT, bool MAT = false>
 __global__ __launch_bounds__(kBlock) void k_restrict_bk(DSell<T> PT, int nsl,
                                                          const uint32_t *__restrict__ cskip,
                                                          const uint8_t *__restrict__ cvnclass,
@@ -924,7 +929,10 @@ __global__ __launch_bounds__(kBlock) void k_restrict_bk(DSell<T> PT, int nsl,
   double sq[B] = {};
   if (sok) {
     T sum[B];
-    sell_dot_diag<B, NT, T, T, T>(MV(PT) + e0 + lane, PT.col + e0 + lane, w, bf, sdo, PT.dtab, rb, sum);
+    if constexpr (MAT)
+      sell_dot_blk<B, NT, T, T, T>(MV(PT) + e0 + lane, PT.pst, PT.col + e0 + lane, w, bf, sdo, PT.dtab, rb, sum);
+    else
+      sell_dot_diag<B, NT, T, T, T>(MV(PT) + e0 + lane, PT.col + e0 + lane, w, bf, sdo, PT.dtab, rb, sum);
     if (r >= 0) {
       const bool zero = cvnclass[r] >= 2;
       const uint32_t sk = cskip[r];
@@ -932,7 +940,9 @@ __global__ __launch_bounds__(kBlock) void k_restrict_bk(DSell<T> PT, int nsl,
       for (int q = 0; q < B; ++q) {
         const size_t k = (size_t)B * r + q;
         const T base = zero ? (T)0 : bc[k];
-        const T v = ((sk >> q) & 1u) ? base : base + damp * sum[q];
+        const T v = ((sk >> q) & 1u) ? base
+                    : MAT           ? (zero ? damp * sum[q] : base + sum[q])
+                                    : base + damp * sum[q];
         bc[k] = v;
         if (part && cvclass[r] >= 2) sq[q] = (double)v * (double)v;
       }
@@ -944,7 +954,10 @@ __global__ __launch_bounds__(kBlock) void k_restrict_bk(DSell<T> PT, int nsl,
     for (int i = 0; i < B; ++i) part[((size_t)col_ * B + i) * gridDim.x + sb_] = sq[i];
 }
 
-template <int B, typename T, int CADD, int NT>
+// MAT: InterpolateCorrectionByMatrix (transgrid.cc:1340-1380), t_v,i =
+// damp sum_j M_ji c_j over the entries (the slab holds M^T per entry) on the
+// components whose VECSKIP bit is clear; otherwise scalar weights
+template <int B, typename T, int CADD, int NT, bool MAT = false>
 __global__ __launch_bounds__(kBlock) void k_interp_bk(DSell<T> P, int nsl,
                                                        const uint32_t *__restrict__ fskip,
                                                        const T *__restrict__ cc, T damp,
@@ -954,7 +967,10 @@ __global__ __launch_bounds__(kBlock) void k_interp_bk(DSell<T> P, int nsl,
   cc += (size_t)col_ * vsc;
   const size_t o = (size_t)col_ * vsf;
   T sum[B];
-  sell_dot_diag<B, NT, T, T, T>(MV(P) + e0 + lane, P.col + e0 + lane, w, cc, sdo, P.dtab, rb, sum);
+  if constexpr (MAT)
+    sell_dot_blk<B, NT, T, T, T>(MV(P) + e0 + lane, P.pst, P.col + e0 + lane, w, cc, sdo, P.dtab, rb, sum);
+  else
+    sell_dot_diag<B, NT, T, T, T>(MV(P) + e0 + lane, P.col + e0 + lane, w, cc, sdo, P.dtab, rb, sum);
   if (r < 0) return;
   const uint32_t sk = fskip[r];
 #pragma unroll
@@ -1708,6 +1724,11 @@ int launch_restrict(hipStream_t st, const DSell<T> &PT, const uint32_t *cskip,
                     size_t cdms) {
   if (PT.nslices <= 0) return 0;
   const int g = nblk_slices(PT.nslices);
+  if (b > 1 && PT.bb > 1) {
+    BK_DISPATCH(b, NT_DISPATCH(nt, k_restrict_bk<BK, T, NTV, true><<<grid2(g, nrhs), kBlock, 0, st>>>(
+                                       PT, PT.nslices, cskip, cvnclass, cvclass, bf, damp, bc, part, vsf, vsc)));
+    return g;
+  }
   if (b > 1) {
     BK_DISPATCH(b, NT_DISPATCH(nt, k_restrict_bk<BK, T, NTV><<<grid2(g, nrhs), kBlock, 0, st>>>(
                                        PT, PT.nslices, cskip, cvnclass, cvclass, bf, damp, bc, part, vsf, vsc)));
@@ -1901,6 +1922,13 @@ void launch_interp(hipStream_t st, const DSell<T> &P, const uint32_t *fskip, con
                    T *t, T *c, bool nt, int nrhs, size_t vsf, size_t vsc, int b) {
   if (P.nslices <= 0) return;
   const dim3 g = grid2(nblk_slices(P.nslices), nrhs);
+  if (b > 1 && P.bb > 1) {
+    BK_DISPATCH(b, NT_DISPATCH(nt, if (c) k_interp_bk<BK, T, 1, NTV, true><<<g, kBlock, 0, st>>>(P, P.nslices, fskip, cc,
+                                                                                       damp, t, c, vsf, vsc);
+                               else k_interp_bk<BK, T, 0, NTV, true><<<g, kBlock, 0, st>>>(P, P.nslices, fskip, cc,
+                                                                                    damp, t, c, vsf, vsc)));
+    return;
+  }
   if (b > 1) {
     BK_DISPATCH(b, NT_DISPATCH(nt, if (c) k_interp_bk<BK, T, 1, NTV><<<g, kBlock, 0, st>>>(P, P.nslices, fskip, cc,
                                                                                        damp, t, c, vsf, vsc);

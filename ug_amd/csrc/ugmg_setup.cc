@@ -192,8 +192,8 @@ void build_sell(const HostLevel &L, const std::vector<int> *rows, Part part,
   sell_from_csr(L.n, L.n, L.rp.data(), L.ci.data(), L.a.data(), L.b * L.b, rows, keep, true, out);
 }
 
-void build_sell_csr(int n, const int *rp, const int *ci, const double *a, HostSell &out) {
-  sell_from_csr(n, 0, rp, ci, a, 1, nullptr, [](int, int) { return true; }, false, out);
+void build_sell_csr(int n, const int *rp, const int *ci, const double *a, HostSell &out, int bb) {
+  sell_from_csr(n, 0, rp, ci, a, bb, nullptr, [](int, int) { return true; }, false, out);
 }
 
 void to_float(const double *in, size_t n, hvec<float> &out) {
@@ -280,6 +280,7 @@ void permute_level(const HostLevel &L, const std::vector<int> &perm, const std::
     out.pi.resize(L.pi.size());
     out.pw.resize(L.pw.size());
     out.pwf.resize(L.pwf.size());
+    out.pblk.resize(L.pblk.size());
 #pragma omp parallel for schedule(static)
     for (int p = 0; p < n; ++p) {
       const int r = perm[p];
@@ -288,6 +289,8 @@ void permute_level(const HostLevel &L, const std::vector<int> &perm, const std::
         out.pi[q] = cpos ? (*cpos)[L.pi[m]] : L.pi[m];
         out.pw[q] = L.pw[m];
         if (!L.pwf.empty()) out.pwf[q] = L.pwf[m];
+        if (!L.pblk.empty())
+          for (int e = 0; e < bb; ++e) out.pblk[(size_t)q * bb + e] = L.pblk[(size_t)m * bb + e];
       }
     }
   }
@@ -427,14 +430,30 @@ void build_prep(const HostLevel &L, const HostLevel *coarse, const std::vector<i
     out.pt_w.resize(out.pt_ptr[nc]);
     std::vector<int> fill(out.pt_ptr.begin(), out.pt_ptr.end() - 1);
     const std::vector<double> &W = L.p_weights();
+    const int b = L.b, bb = b * b;
+    const bool mat = !L.pblk.empty();
+    hvec<double> ptb(mat ? out.pt_idx.size() * bb : 0);
     for (int v = 0; v < L.n; ++v) {
       if (!L.restricts(v)) continue;
       for (int q = L.pp[v]; q < L.pp[v + 1]; ++q) {
         int c = L.pi[q];
         out.pt_idx[fill[c]] = v;
-        out.pt_w[fill[c]] = W[q];
+        out.pt_w[fill[c]] = mat ? 0.0 : W[q];
+        if (mat)   // restriction: M as it is (coarse rows, fine columns)
+          for (int e = 0; e < bb; ++e) ptb[(size_t)fill[c] * bb + e] = L.pblk[(size_t)q * bb + e];
         ++fill[c];
       }
+    }
+    if (mat) {
+      // interpolation: M^T per entry (fine rows, coarse columns)
+      hvec<double> pb(L.pblk.size());
+      const size_t ne = L.pblk.size() / bb;
+      for (size_t q = 0; q < ne; ++q)
+        for (int i = 0; i < b; ++i)
+          for (int j = 0; j < b; ++j) pb[q * bb + i * b + j] = L.pblk[q * bb + j * b + i];
+      build_sell_csr(L.n, L.pp.data(), L.pi.data(), pb.data(), out.P, bb);
+      build_sell_csr(nc, out.pt_ptr.data(), out.pt_idx.data(), ptb.data(), out.PT, bb);
+      return;
     }
     build_sell_csr(L.n, L.pp.data(), L.pi.data(), W.data(), out.P);
     build_sell_csr(nc, out.pt_ptr.data(), out.pt_idx.data(), out.pt_w.data(), out.PT);

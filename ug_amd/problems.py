@@ -8,7 +8,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
-from dataclasses import dataclass, field
+from dataclasses import dataclass, field, replace
 
 import numpy as np
 
@@ -198,6 +198,8 @@ class Level:
     p_idx: np.ndarray | None = None
     p_w: np.ndarray | None = None
     vtype: np.ndarray | None = None     # VTYPE per vector (0 node, 1 edge); None: all node
+    p_blk: np.ndarray | None = None     # interpolation matrix: b x b block per entry of p_idx
+                                        # ([i*b+j] = coarse component i, fine component j)
 
     @property
     def nnzb(self) -> int:
@@ -742,6 +744,23 @@ def p2_uniform(nref: int, ncomp: int = 1) -> Problem:
     return Problem(dim=2, levels=levels, rhs=rhs, name=f"p2_{nref}r_b{ncomp}")
 
 
+def with_imat(P: Problem, coupling) -> Problem:
+    """P with every level's transfer as an interpolation matrix (UG's transfer
+    $M, RestrictByMatrix / InterpolateCorrectionByMatrix): the block of entry q
+    is p_w[q] * coupling (b x b, [i, j] = coarse component i, fine component
+    j); coupling = I gives the standard weights as blocks."""
+    cm = np.asarray(coupling, np.float64)
+    out = []
+    for l, lv in enumerate(P.levels):
+        new = replace(lv)
+        if l > 0:
+            assert cm.shape == (lv.b, lv.b)
+            new.p_blk = (np.asarray(lv.p_w)[:, None, None] * cm[None]).ravel().copy()
+        out.append(new)
+    return Problem(dim=P.dim, levels=out, rhs=None if P.rhs is None else np.array(P.rhs, copy=True),
+                   name=P.name + "_imat", theta=P.theta, meshes=P.meshes)
+
+
 def reordered(P: Problem, seed: int, newest_first: bool = True) -> Problem:
     """P with each level's vector list in a seeded random order and, with
     newest_first, every row listing its off-diagonal entries in the reverse
@@ -789,14 +808,19 @@ def reordered(P: Problem, seed: int, newest_first: bool = True) -> Problem:
         if l > 0:
             cinv = invs[l - 1]
             pp = np.zeros(lv.n + 1, np.int32)
-            pi, pw = [], []
+            pi, pw, pb = [], [], []
+            bb = b * b
             for q in range(lv.n):
                 r = pm[q]
                 lo, hi = lv.p_ptr[r], lv.p_ptr[r + 1]
                 pi.extend(cinv[lv.p_idx[lo:hi]])
                 pw.extend(lv.p_w[lo:hi])
+                if lv.p_blk is not None:
+                    pb.append(lv.p_blk[lo * bb:hi * bb])
                 pp[q + 1] = pp[q] + (hi - lo)
             new.p_ptr, new.p_idx, new.p_w = pp, np.asarray(pi, np.int32), np.asarray(pw, np.float64)
+            if lv.p_blk is not None:
+                new.p_blk = np.concatenate(pb) if pb else np.zeros(0)
         out.append(new)
     rhs = None
     if P.rhs is not None:

@@ -87,6 +87,7 @@ def load():
         "ugmg_set_level": ([vp, ip, ip, vp, vp, vp, vp, vp, vp], ip),
         "ugmg_set_transfer": ([vp, ip, vp, vp, vp], ip),
         "ugmg_set_vtypes": ([vp, ip, vp], ip),
+        "ugmg_set_transfer_matrix": ([vp, ip, vp, vp, vp], ip),
         "ugmg_norm_entries": ([vp], ip),
         "ugmg_set_params": ([vp, C.POINTER(Params)], ip),
         "ugmg_preprocess": ([vp, ip], ip),
@@ -229,6 +230,11 @@ class Context:
             self._chk(self.lib.ugmg_set_level(self.h, l, lv.n, *[a.ctypes.data for a in arrs]))
         for l in range(1, len(levels)):
             lv = levels[l]
+            if getattr(lv, "p_blk", None) is not None:      # an interpolation matrix (transfer $M)
+                arrs = [np.ascontiguousarray(lv.p_ptr, np.int32), np.ascontiguousarray(lv.p_idx, np.int32),
+                        np.ascontiguousarray(lv.p_blk, np.float64)]
+                self._chk(self.lib.ugmg_set_transfer_matrix(self.h, l, *[a.ctypes.data for a in arrs]))
+                continue
             arrs = [np.ascontiguousarray(lv.p_ptr, np.int32), np.ascontiguousarray(lv.p_idx, np.int32),
                     np.ascontiguousarray(lv.p_w, np.float64)]
             self._chk(self.lib.ugmg_set_transfer(self.h, l, *[a.ctypes.data for a in arrs]))

@@ -21,7 +21,9 @@
  *             $damp 1   (gs is the multicolour-ordered l_lgs of the GPU)
  *   ex        EXInit                                  iter.cc:8248-8260
  *             $o 1 (BFS band reorder), $f (FLOAT band LU), $damp 1
- *   transfer  the standard transfer (no options read here)
+ *   transfer  the transfer the hierarchy was given: the standard one
+ *             (ugmg_set_transfer), or $M's interpolation matrices
+ *             (ugmg_set_transfer_matrix); no options read here
  * GPU-only options of lmgc: $precision fp64|mixed, $inner k (FP32 cycles
  * per FP64 step), $adapt 0|1 (a step runs fewer cycles when the rate
  * predicts the stop test met; default 1), $graph 0|1; of ls/bcgs: $nrhs k

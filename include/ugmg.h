@@ -159,6 +159,14 @@ int ugmg_set_transfer_matrix(ugmg_ctx *ctx, int level, const int *p_ptr, const i
  * UG's standard transfer serves node and edge vectors only (element and
  * side vectors: "not implemented", transgrid.cc:474-503). */
 int ugmg_set_vtypes(ugmg_ctx *ctx, int level, const uint8_t *vtype);
+/* VD_IDENT of the solution descriptor (np/udm/udm.h:113): ident[i] for
+ * each of the n norm entries, the leader of its group (ident[ident[i]] ==
+ * ident[i]).  The stop tests of ugmg_solve, ugmg_solve_surface and the
+ * base solve then compare per group the sum of squares of the defects with
+ * that of the limits (sc_cmp, np/udm/npscan.cc:1027-1057); n must equal
+ * ugmg_norm_entries at the solve.  NULL or n <= 0: NO_IDENT (every entry on
+ * its own, the default). */
+int ugmg_set_ident(ugmg_ctx *ctx, const int *ident, int n);
 /* VD_NCOMP of the prepared hierarchy: the entries of its norms, dot
  * products and VEC_SCALARs (vector types x b); UGMG_E_STATE before
  * ugmg_preprocess. */

@@ -34,7 +34,8 @@ class OrcParams(C.Structure):
                 ("smoother", C.c_int),
                 ("damp", C.c_double * MAXB), ("lmgc_damp", C.c_double * MAXB), ("base_damp", C.c_double * MAXB),
                 ("base_abslimit", C.c_double * MAXB), ("base_red", C.c_double * MAXB),
-                ("base_maxiter", C.c_int), ("ex_reorder", C.c_int)]
+                ("base_maxiter", C.c_int), ("ex_reorder", C.c_int),
+                ("has_ident", C.c_int), ("ident", C.c_int * MAXB)]
 
 
 _lib = None
@@ -127,8 +128,13 @@ class Hierarchy:
 
 
 def params(nu1=2, nu2=2, gamma=1, baselevel=0, smoother=JAC, damp=0.8, lmgc_damp=1.0,
-           base_damp=1.0, base_abslimit=1e-300, base_red=1e-12, base_maxiter=1, ex_reorder=1):
+           base_damp=1.0, base_abslimit=1e-300, base_red=1e-12, base_maxiter=1, ex_reorder=1, ident=None):
+    """ident: VD_IDENT per norm entry (sc_cmp's grouping), None: NO_IDENT."""
     p = OrcParams()
+    if ident is not None:
+        p.has_ident = 1
+        for i, v in enumerate(ident):
+            p.ident[i] = v
     p.nu1, p.nu2, p.gamma, p.baselevel, p.smoother = nu1, nu2, gamma, baselevel, smoother
     for i in range(MAXB):
         p.damp[i] = damp

@@ -148,12 +148,29 @@ static void dadd(size_t len, double *x, const double *y) {
   for (size_t i = 0; i < len; ++i) x[i] += y[i];
 }
 
-/* sc_cmp: 1 iff |x_i| < |y_i| for every component */
-static int sc_cmp(int b, const double *x, const double *y) {
+/* sc_cmp (np/udm/npscan.cc:1027-1057): 1 iff |x_i| < |y_i| for every
+ * component; with VD_IDENT (ident != NULL) iff, for every group leader i
+ * (ident[i] == i), the sum of x_j^2 over the group (ident[j] == i) is below
+ * that of y_j^2 */
+static int sc_cmp(int b, const double *x, const double *y, const int *ident) {
+  if (ident) {
+    for (int i = 0; i < b; ++i) {
+      if (ident[i] != i) continue;
+      double sx = 0.0, sy = 0.0;
+      for (int j = 0; j < b; ++j)
+        if (ident[j] == i) {
+          sx += x[j] * x[j];
+          sy += y[j] * y[j];
+        }
+      if (sx >= sy) return 0;
+    }
+    return 1;
+  }
   for (int i = 0; i < b; ++i)
     if (fabs(x[i]) >= fabs(y[i])) return 0;
   return 1;
 }
+#define SC_IDENT(p) ((p)->has_ident ? (p)->ident : NULL)
 
 /* ---------------------------------------------------------- small blocks */
 
@@ -644,7 +661,7 @@ static int base_solver(orc_mg *mg, int level, const double *first) {
     if (defect2reach[i] == 0.0) defect2reach[i] = p->base_red[i];
     last[i] = first[i];
   }
-  int converged = sc_cmp(nc, first, p->base_abslimit);
+  int converged = sc_cmp(nc, first, p->base_abslimit, SC_IDENT(p));
   for (int i = 0; i < p->base_maxiter; ++i) {
     if (converged) break;
     memset(mg->cbase, 0, sizeof(double) * (size_t)L->n * b);
@@ -652,7 +669,7 @@ static int base_solver(orc_mg *mg, int level, const double *first) {
       return 1;
     dadd((size_t)L->n * b, mg->c[level], mg->cbase); /* LSUpdate */
     surface_nrm2x(L, mg->b[level], last);             /* LinearResiduum */
-    if (sc_cmp(nc, last, p->base_abslimit) || sc_cmp(nc, last, defect2reach)) {
+    if (sc_cmp(nc, last, p->base_abslimit, SC_IDENT(p)) || sc_cmp(nc, last, defect2reach, SC_IDENT(p))) {
       converged = 1;
       break;
     }
@@ -770,7 +787,7 @@ int orc_ls_lmgc_solve(const orc_level *levels, int top, const orc_params *p,
     defect2reach[i] = first[i] * reduction[i];     /* sc_mul_check */
     if (defect2reach[i] == 0.0) defect2reach[i] = reduction[i];
   }
-  int conv = sc_cmp(nb, first, abslimit);
+  int conv = sc_cmp(nb, first, abslimit, SC_IDENT(p));
   int it = 0;
   int i;
   for (i = 0; i < maxiter; ++i) {
@@ -782,7 +799,7 @@ int orc_ls_lmgc_solve(const orc_level *levels, int top, const orc_params *p,
     dadd(len, x, mg.c[top]);                        /* LSUpdate */
     surface_nrm2x(L, b, last);
     for (int k = 0; k < nb; ++k) defects[(size_t)(i + 1) * nb + k] = last[k];
-    if (sc_cmp(nb, last, abslimit) || sc_cmp(nb, last, defect2reach)) {
+    if (sc_cmp(nb, last, abslimit, SC_IDENT(p)) || sc_cmp(nb, last, defect2reach, SC_IDENT(p))) {
       conv = 1;
       it = i + 1;
       break;
@@ -884,7 +901,7 @@ int orc_ls_lmgc_solve_surface(const orc_level *levels, int top, const orc_params
     defect2reach[i] = first[i] * reduction[i];
     if (defect2reach[i] == 0.0) defect2reach[i] = reduction[i];
   }
-  int conv = sc_cmp(nb, first, abslimit);
+  int conv = sc_cmp(nb, first, abslimit, SC_IDENT(p));
   int it = 0;
   int i;
   for (i = 0; i < maxiter; ++i) {
@@ -895,7 +912,7 @@ int orc_ls_lmgc_solve_surface(const orc_level *levels, int top, const orc_params
       dadd((size_t)levels[l].n * vb, x[l], mg.c[l]);
     composite_nrm2x(levels, frl, top, mg.b, last);
     for (int k = 0; k < nb; ++k) defects[(size_t)(i + 1) * nb + k] = last[k];
-    if (sc_cmp(nb, last, abslimit) || sc_cmp(nb, last, defect2reach)) {
+    if (sc_cmp(nb, last, abslimit, SC_IDENT(p)) || sc_cmp(nb, last, defect2reach, SC_IDENT(p))) {
       conv = 1;
       it = i + 1;
       break;
@@ -963,7 +980,7 @@ int orc_bcgs_solve(const orc_level *levels, int top, const orc_params *p, int us
     if (nd < maxdef) defects[nd * nb + i] = first[i];
   }
   ++nd;
-  conv = sc_cmp(nb, first, abslimit);
+  conv = sc_cmp(nb, first, abslimit, SC_IDENT(p));
   double alpha = 0.0, rho = 0.0, omega = 0.0, rho_new = 0.0, beta = 0.0, tt = 0.0;
   int need_restart = 1, eq_count = 0;
   /* Iter(q, z): q = lmgc correction of defect z (z left untouched) */
@@ -1005,7 +1022,7 @@ int orc_bcgs_solve(const orc_level *levels, int top, const orc_params *p, int us
     surface_nrm2x(L, sv, last);
     if (nd < maxdef) for (int c = 0; c < nb; ++c) defects[nd * nb + c] = last[c];
     ++nd;
-    if (sc_cmp(nb, last, abslimit) || sc_cmp(nb, last, d2r)) {
+    if (sc_cmp(nb, last, abslimit, SC_IDENT(p)) || sc_cmp(nb, last, d2r, SC_IDENT(p))) {
       memcpy(b, sv, len * sizeof(double));
       conv = 1;
       break;
@@ -1023,7 +1040,7 @@ int orc_bcgs_solve(const orc_level *levels, int top, const orc_params *p, int us
     if (nd < maxdef) for (int c = 0; c < nb; ++c) defects[nd * nb + c] = last[c];
     ++nd;
     ++its;
-    if (sc_cmp(nb, last, abslimit) || sc_cmp(nb, last, d2r)) { conv = 1; break; }
+    if (sc_cmp(nb, last, abslimit, SC_IDENT(p)) || sc_cmp(nb, last, d2r, SC_IDENT(p))) { conv = 1; break; }
     if (sc_eq(nb, last, old, 1e-4)) ++eq_count;
     else eq_count = 0;
     for (int c = 0; c < nb; ++c) old[c] = last[c];

@@ -67,6 +67,8 @@ typedef struct {
   double base_red[ORC_MAXB];     /* base ls $red                           */
   int base_maxiter;              /* base ls $m                             */
   int ex_reorder;                /* ex $o                                  */
+  int has_ident;                 /* VD_NID != NO_IDENT                     */
+  int ident[ORC_MAXB];           /* VD_IDENT per norm entry (sc_cmp)       */
 } orc_params;
 
 /* ugblas (np/algebra/ugblas.cc) on one level, ALL_VECTORS */

@@ -541,3 +541,25 @@ def test_galerkin_matches_exact_coarse_operator_inside():
         rows = np.repeat(np.arange(c.n), np.diff(c.row_ptr))
         inner = (c.skip[rows] == 0) & (c.skip[c.col_idx] == 0)
         assert np.max(np.abs(got[inner] - c.vals[inner])) < 1e-12 * np.max(np.abs(c.vals))
+
+
+@pytest.mark.parametrize("ident", [[0, 0, 0], [0, 1, 1]])
+def test_sc_cmp_groups_by_vd_ident(elasticity_17, ident):
+    """sc_cmp with VD_IDENT (npscan.cc:1027-1057): the ls stops at the first
+    iteration whose defects, summed in squares per group, fall below the
+    first defect's times $red; the defect sequence itself is the ungrouped
+    one."""
+    P = elasticity_17
+    H = O.Hierarchy(P.levels)
+    plain = O.ls_solve(H, P.top, O.params(smoother=O.JAC, damp=0.6, baselevel=1), P.rhs, red=1e-6, maxiter=80)
+    grp = O.ls_solve(H, P.top, O.params(smoother=O.JAC, damp=0.6, baselevel=1, ident=ident), P.rhs, red=1e-6,
+                     maxiter=80)
+    d = grp["defects"]
+    k = grp["nit"]
+    assert np.array_equal(d, plain["defects"][:k + 1]) or np.array_equal(plain["defects"], d[:plain["nit"] + 1])
+
+    def below(x, y):
+        return all(sum(x[j] ** 2 for j in range(3) if ident[j] == i) <
+                   sum(y[j] ** 2 for j in range(3) if ident[j] == i) for i in range(3) if ident[i] == i)
+    target = d[0] * 1e-6
+    assert grp["converged"] and below(d[k], target) and not any(below(d[m], target) for m in range(k))

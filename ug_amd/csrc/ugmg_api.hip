@@ -189,6 +189,10 @@ struct ugmg_ctx {
   // vector types of the prepared hierarchy and the entries of its norms
   // and VEC_SCALARs (ntypes * b)
   int ntypes = 1, nc = 1;
+  // VD_IDENT of the solution descriptor (sc_cmp's grouping), nident entries
+  // (0: NO_IDENT)
+  int nident = 0;
+  int ident[UGMG_MAXB] = {};
   std::vector<HostLevel> host;
   std::vector<char> level_set, transfer_set;
   ugmg_params p{};
@@ -463,9 +467,17 @@ void drop_graph(ugmg_ctx *ctx) {
   drop_bcgs_graphs(ctx);
 }
 
+inline int packed_ident(const ugmg_ctx *ctx) {
+  if (ctx->nident <= 0) return -1;
+  int v = 0;
+  for (int i = 0; i < ctx->nident; ++i) v |= ctx->ident[i] << (4 * i);
+  return v;
+}
 inline Batch batch(const ugmg_ctx *ctx, int l) {
   const DevLevel &d = ctx->dev[l];
-  return Batch{ctx->nb, d.vs, d.nb, d.dms, ctx->base_ims, d.vt, d.ntypes};
+  Batch bt{ctx->nb, d.vs, d.nb, d.dms, ctx->base_ims, d.vt, d.ntypes};
+  bt.idents = packed_ident(ctx);
+  return bt;
 }
 inline Batch B1(const DevLevel &d) { return Batch{1, d.vs, d.nb, 0, 0, d.vt, d.ntypes}; }
 
@@ -776,6 +788,28 @@ int run_iteration(ugmg_ctx *ctx, int level, double *x, double *b, int cycles) {
 }
 
 inline bool sc_cmp1(double x, double y) { return std::fabs(x) < std::fabs(y); }
+
+// sc_cmp (np/udm/npscan.cc:1027-1057) over n entries: |x_i| < |y_i| for
+// every i, or with VD_IDENT, per group leader i (ident[i] == i) the group's
+// sum of x_j^2 below its sum of y_j^2
+bool sc_cmp_vd(const ugmg_ctx *ctx, int n, const double *x, const double *y) {
+  if (ctx->nident == n) {
+    for (int i = 0; i < n; ++i) {
+      if (ctx->ident[i] != i) continue;
+      double sx = 0.0, sy = 0.0;
+      for (int j = 0; j < n; ++j)
+        if (ctx->ident[j] == i) {
+          sx += x[j] * x[j];
+          sy += y[j] * y[j];
+        }
+      if (sx >= sy) return false;
+    }
+    return true;
+  }
+  for (int i = 0; i < n; ++i)
+    if (!sc_cmp1(x[i], y[i])) return false;
+  return true;
+}
 
 // ------------------------------------------------------------ byte model
 // Algorithmic bytes of each pass as the data are laid out, the figures the
@@ -1930,6 +1964,23 @@ int ugmg_set_vtypes(ugmg_ctx *ctx, int level, const uint8_t *vtype) {
   return UGMG_OK;
 }
 
+int ugmg_set_ident(ugmg_ctx *ctx, const int *ident, int n) {
+  if (!ctx) return UGMG_E_ARG;
+  if (!ident || n <= 0) {
+    ctx->nident = 0;
+    drop_graph(ctx);
+    return UGMG_OK;
+  }
+  if (n > UGMG_MAXB || n > 16) return fail(ctx, UGMG_E_ARG, "more VD_IDENT entries than UGMG_MAXB");
+  for (int i = 0; i < n; ++i)
+    if (ident[i] < 0 || ident[i] >= n || ident[ident[i]] != ident[i])
+      return fail(ctx, UGMG_E_ARG, "VD_IDENT must map each entry to a group leader that maps to itself");
+  for (int i = 0; i < n; ++i) ctx->ident[i] = ident[i];
+  ctx->nident = n;
+  drop_graph(ctx);   // the base solve's stop test is captured with the grouping
+  return UGMG_OK;
+}
+
 int ugmg_set_params(ugmg_ctx *ctx, const ugmg_params *p) {
   if (!ctx || !p) return UGMG_E_ARG;
   if (p->nu1 < 0 || p->nu2 < 0 || p->gamma < 0 || p->base_maxiter < 0 ||
@@ -2337,12 +2388,12 @@ int ugmg_solve(ugmg_ctx *ctx, int level, double *x, double *b, const double *abs
   }
   ctx->stat_bytes += vb + d.n;   // the first defect norm
   const int nc = d.nc();
-  // sc_cmp (npscan.cc:1027-1057): every component below its limit
-  auto sc_cmp = [nc](const double *x, const double *y) {
-    for (int i = 0; i < nc; ++i)
-      if (!sc_cmp1(x[i], y[i])) return false;
-    return true;
-  };
+  if (ctx->nident > 0 && ctx->nident != nc)
+    return fail(ctx, UGMG_E_ARG, "ugmg_set_ident gave " + std::to_string(ctx->nident) + " entries, the norms have " +
+                                     std::to_string(nc));
+  // sc_cmp (npscan.cc:1027-1057): every component (or VD_IDENT group) below
+  // its limit
+  auto sc_cmp = [ctx, nc](const double *x, const double *y) { return sc_cmp_vd(ctx, nc, x, y); };
   double first[UGMG_MAXB] = {}, target[UGMG_MAXB], last[UGMG_MAXB];
   if ((rc = surface_norm(ctx, level, b, first))) return rc;
   for (int i = 0; i < nc; ++i) {
@@ -2668,11 +2719,10 @@ int ugmg_solve_surface(ugmg_ctx *ctx, int level, double *const *x, double *const
   HIPCHK(hipMemcpyAsync(ctx->surf_sq_host, ctx->surf_sq, sizeof(double) * UGMG_MAXB * (level + 1),
                         hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
-  auto sc_cmp = [nc](const double *u, const double *v) {
-    for (int i = 0; i < nc; ++i)
-      if (!sc_cmp1(u[i], v[i])) return false;
-    return true;
-  };
+  if (ctx->nident > 0 && ctx->nident != nc)
+    return fail(ctx, UGMG_E_ARG, "ugmg_set_ident gave " + std::to_string(ctx->nident) + " entries, the norms have " +
+                                     std::to_string(nc));
+  auto sc_cmp = [ctx, nc](const double *u, const double *v) { return sc_cmp_vd(ctx, nc, u, v); };
   double first[UGMG_MAXB] = {}, target[UGMG_MAXB], last[UGMG_MAXB];
   surface_norms(ctx, s, ctx->surf_sq_host, first);
   for (int i = 0; i < nc; ++i) {

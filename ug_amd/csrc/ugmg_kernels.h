@@ -102,6 +102,9 @@ struct Batch {
   const uint8_t *vt = nullptr;
   int nt = 1;
   int nc() const { return vt ? nt * b : b; }
+  // the stop tests' VD_IDENT grouping of the norm entries (4 bits per
+  // entry, entry q at bits 4q..4q+3); -1: NO_IDENT
+  int idents = -1;
 };
 
 // per right-hand side state of the batched BCGSSolver (np/procs/ls.cc:

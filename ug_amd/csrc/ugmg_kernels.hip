@@ -1148,7 +1148,7 @@ __global__ __launch_bounds__(kBlock) void k_base_gemv_bk(int n3, const TI *__res
                                                           int cmode, double damp, double abslimit,
                                                           double red, const double *__restrict__ part,
                                                           int nparts, double *__restrict__ first,
-                                                          int it, size_t vs, size_t ims) {
+                                                          int it, size_t vs, size_t ims, int idents) {
   const int row = (blockIdx.x * kBlock + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   const int cj = blockIdx.y;
@@ -1166,19 +1166,35 @@ __global__ __launch_bounds__(kBlock) void k_base_gemv_bk(int n3, const TI *__res
     cur[q] = sqrt(s);
   }
   bool below_abs = true, below_red = true;
-  if (it == 0) {
-    if (row == 0 && lane == 0)
-      for (int q = 0; q < B; ++q) first[(size_t)cj * B + q] = cur[q];
-    for (int q = 0; q < B; ++q) below_abs &= fabs(cur[q]) < fabs(abslimit);
-    below_red = false;
-  } else {
+  double target[B];
+  if (it == 0 && row == 0 && lane == 0)
+    for (int q = 0; q < B; ++q) first[(size_t)cj * B + q] = cur[q];
+  for (int q = 0; q < B; ++q) {
+    target[q] = it == 0 ? 0.0 : first[(size_t)cj * B + q] * red;
+    if (target[q] == 0.0) target[q] = red;
+  }
+  if (idents < 0) {
     for (int q = 0; q < B; ++q) {
-      double target = first[(size_t)cj * B + q] * red;
-      if (target == 0.0) target = red;
       below_abs &= fabs(cur[q]) < fabs(abslimit);
-      below_red &= fabs(cur[q]) < fabs(target);
+      below_red &= fabs(cur[q]) < fabs(target[q]);
+    }
+  } else {
+    // sc_cmp with VD_IDENT (npscan.cc:1033-1048): per group leader i, the
+    // group's sum of squares below the limits' (ident of entry q in bits 4q..)
+    for (int i = 0; i < B; ++i) {
+      if (((idents >> (4 * i)) & 15) != i) continue;
+      double sx = 0.0, sa = 0.0, st = 0.0;
+      for (int j = 0; j < B; ++j)
+        if (((idents >> (4 * j)) & 15) == i) {
+          sx += cur[j] * cur[j];
+          sa += abslimit * abslimit;
+          st += target[j] * target[j];
+        }
+      below_abs &= sx < sa;
+      below_red &= sx < st;
     }
   }
+  if (it == 0) below_red = false;
   const bool conv = below_abs || below_red;
   if (row >= n3) return;
   if (conv) {
@@ -1844,7 +1860,7 @@ void launch_base_gemv(hipStream_t st, int n, const TI *inv, const int *rowmap, c
     const int g = (n * 64 + kBlock - 1) / kBlock;
     BK_DISPATCH(bt.nc(), k_base_gemv_bk<BK, T, TI><<<grid2(g, bt.nrhs), kBlock, 0, st>>>(
                           n, inv, rowmap, cp, ncols, b, c, xout, cmode, damp, abslimit, red, part, nparts,
-                          first, it, bt.vs, bt.ims));
+                          first, it, bt.vs, bt.ims, bt.idents));
   } else {           // one workgroup per row
     k_base_gemv<T, TI><<<grid2(n, bt.nrhs), kBlock, 0, st>>>(n, inv, rowmap, cp, ncols, b, bc, c, xout,
                                                              cmode, damp, abslimit, red, part,

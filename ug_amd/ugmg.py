@@ -89,6 +89,7 @@ def load():
         "ugmg_set_vtypes": ([vp, ip, vp], ip),
         "ugmg_set_transfer_matrix": ([vp, ip, vp, vp, vp], ip),
         "ugmg_norm_entries": ([vp], ip),
+        "ugmg_set_ident": ([vp, vp, ip], ip),
         "ugmg_set_params": ([vp, C.POINTER(Params)], ip),
         "ugmg_preprocess": ([vp, ip], ip),
         "ugmg_postprocess": ([vp], ip),
@@ -259,6 +260,14 @@ class Context:
     def set_params(self, **kw):
         self.params = default_params(**kw)
         self._chk(self.lib.ugmg_set_params(self.h, C.byref(self.params)))
+
+    def set_ident(self, ident=None):
+        """VD_IDENT of the stop tests (a group leader per norm entry); None: NO_IDENT."""
+        if ident is None:
+            self._chk(self.lib.ugmg_set_ident(self.h, None, 0))
+            return
+        a = np.ascontiguousarray(ident, np.int32)
+        self._chk(self.lib.ugmg_set_ident(self.h, a.ctypes.data, len(a)))
 
     def set_setup(self, where):
         """Where preprocess builds the level layouts: SETUP_DEVICE or SETUP_HOST."""

@@ -159,6 +159,16 @@ int ugmg_set_transfer_matrix(ugmg_ctx *ctx, int level, const int *p_ptr, const i
  * UG's standard transfer serves node and edge vectors only (element and
  * side vectors: "not implemented", transgrid.cc:474-503). */
 int ugmg_set_vtypes(ugmg_ctx *ctx, int level, const uint8_t *vtype);
+/* AssembleDirichletBoundary (np/udm/disctools.cc:1837-1875) on a set
+ * level: for every component j with VECSKIP bit j, b[v*b+j] = x[v*b+j]
+ * (host arrays, list order), row j of the vector's diagonal block becomes
+ * the unit row and row j of its off-diagonal blocks zero (columns are not
+ * touched, so the matrix turns nonsymmetric, as in UG).  A change to the
+ * matrix drops the level's layouts: ugmg_preprocess again. */
+int ugmg_assemble_dirichlet(ugmg_ctx *ctx, int level, const double *x, double *b);
+/* ClearDirichletValues (np/udm/disctools.cc:1801-1817): x[v*b+j] = 0 where
+ * VECSKIP bit j is set; x a list-order device vector of a prepared level. */
+int ugmg_clear_dirichlet(ugmg_ctx *ctx, int level, double *x);
 /* VD_IDENT of the solution descriptor (np/udm/udm.h:113): ident[i] for
  * each of the n norm entries, the leader of its group (ident[ident[i]] ==
  * ident[i]).  The stop tests of ugmg_solve, ugmg_solve_surface and the

@@ -1875,6 +1875,53 @@ int ugmg_assemble_level(ugmg_ctx *ctx, int level, const ugmg_elements *el, doubl
   return UGMG_OK;
 }
 
+// AssembleDirichletBoundary (np/udm/disctools.cc:1837-1875) on the level's
+// stored matrix and the caller's host arrays: for every Dirichlet component
+// j of vector v (VECSKIP bit j), b_v,j = x_v,j, row j of the diagonal block
+// the unit row, row j of every off-diagonal block zero (the columns stay)
+int ugmg_assemble_dirichlet(ugmg_ctx *ctx, int level, const double *x, double *b) {
+  if (!ctx) return UGMG_E_ARG;
+  if (level < 0 || level >= ctx->nlevels || !ctx->level_set[level])
+    return fail(ctx, UGMG_E_ARG, "level not set");
+  HostLevel &L = ctx->host[level];
+  if (L.n > 0 && (!x || !b)) return fail(ctx, UGMG_E_ARG, "null array");
+  const int nb = ctx->b, bb = nb * nb;
+  bool changed = false;
+  for (int v = 0; v < L.n; ++v) {
+    const uint32_t sk = L.skip[v];
+    if (!sk) continue;
+    for (int j = 0; j < nb; ++j) {
+      if (!((sk >> j) & 1u)) continue;
+      b[(size_t)v * nb + j] = x[(size_t)v * nb + j];
+      for (int m = L.rp[v]; m < L.rp[v + 1]; ++m) {
+        double *blk = L.a.data() + (size_t)m * bb;
+        for (int i = 0; i < nb; ++i) {
+          const double want = (m == L.rp[v] && i == j) ? 1.0 : 0.0;
+          if (blk[j * nb + i] != want) changed = true;
+          blk[j * nb + i] = want;
+        }
+      }
+    }
+  }
+  if (changed) {   // the layouts (and a device-assembled copy) hold the old rows
+    drop_dcsr(ctx, level);
+    free_device(ctx);
+  }
+  return UGMG_OK;
+}
+
+// ClearDirichletValues (np/udm/disctools.cc:1801-1817): x_v,j = 0 where bit
+// j of VECSKIP is set; x a list-order device vector of a prepared level
+int ugmg_clear_dirichlet(ugmg_ctx *ctx, int level, double *x) {
+  int rc = check_ready(ctx, level);
+  if (rc) return rc;
+  if (!x) return fail(ctx, UGMG_E_ARG, "null array");
+  const DevLevel &d = ctx->dev[level];
+  launch_clear_dirichlet(ctx->stream, d.n, d.nb, d.perm, d.skip, x);
+  HIPCHK(hipGetLastError());
+  return UGMG_OK;
+}
+
 int ugmg_get_level(ugmg_ctx *ctx, int level, int *n, int *nnzb, int *row_ptr, int *col_idx,
                    double *vals, uint32_t *skip) {
   if (!ctx) return UGMG_E_ARG;

@@ -233,6 +233,10 @@ void launch_fill(hipStream_t st, int n, T *x, T val, Batch bt);
 // the VECSKIP bits the interpolation reads: an edge row's (vt 1) own bits
 // are not read (StandardIntCorNodeVector's edge loop), iskip 0
 void launch_iskip(hipStream_t st, int n, const uint8_t *vt, const uint32_t *skip, uint32_t *iskip);
+// ClearDirichletValues on a list-order vector of n vectors of b components:
+// x[v b + j] = 0 where bit j of VECSKIP is set (skip in device order, perm:
+// device position -> list index, null: list order)
+void launch_clear_dirichlet(hipStream_t st, int n, int b, const int *perm, const uint32_t *skip, double *x);
 template <typename TD, typename TS>
 void launch_convert(hipStream_t st, int n, const TS *s, TD *d, int nrhs, size_t vss, size_t vsd);
 template <typename T>

@@ -2021,6 +2021,21 @@ void launch_iskip(hipStream_t st, int n, const uint8_t *vt, const uint32_t *skip
   if (n > 0) k_iskip<<<nblk(n), kBlock, 0, st>>>(n, vt, skip, iskip);
 }
 
+__global__ __launch_bounds__(kBlock) void k_clear_dirichlet(int n, int b, const int *__restrict__ perm,
+                                                             const uint32_t *__restrict__ skip,
+                                                             double *__restrict__ x) {
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= n) return;
+  const uint32_t sk = skip[p];
+  if (!sk) return;
+  const size_t v = perm ? (size_t)perm[p] : (size_t)p;
+  for (int j = 0; j < b; ++j)
+    if ((sk >> j) & 1u) x[v * b + j] = 0.0;
+}
+void launch_clear_dirichlet(hipStream_t st, int n, int b, const int *perm, const uint32_t *skip, double *x) {
+  if (n > 0) k_clear_dirichlet<<<nblk(n), kBlock, 0, st>>>(n, b, perm, skip, x);
+}
+
 template <typename T>
 void launch_fill(hipStream_t st, int n, T *x, T val, Batch bt) {
   n *= bt.b;

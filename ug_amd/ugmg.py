@@ -90,6 +90,8 @@ def load():
         "ugmg_set_transfer_matrix": ([vp, ip, vp, vp, vp], ip),
         "ugmg_norm_entries": ([vp], ip),
         "ugmg_set_ident": ([vp, vp, ip], ip),
+        "ugmg_assemble_dirichlet": ([vp, ip, vp, vp], ip),
+        "ugmg_clear_dirichlet": ([vp, ip, vp], ip),
         "ugmg_set_params": ([vp, C.POINTER(Params)], ip),
         "ugmg_preprocess": ([vp, ip], ip),
         "ugmg_postprocess": ([vp], ip),
@@ -268,6 +270,17 @@ class Context:
             return
         a = np.ascontiguousarray(ident, np.int32)
         self._chk(self.lib.ugmg_set_ident(self.h, a.ctypes.data, len(a)))
+
+    def assemble_dirichlet(self, level, x, b):
+        """AssembleDirichletBoundary on the stored level: b_j = x_j and unit rows on
+        the Dirichlet components (host arrays; b is updated in place)."""
+        x = np.ascontiguousarray(x, np.float64)
+        assert b.dtype == np.float64 and b.flags.c_contiguous
+        self._chk(self.lib.ugmg_assemble_dirichlet(self.h, level, x.ctypes.data, b.ctypes.data))
+
+    def clear_dirichlet(self, level, x):
+        """ClearDirichletValues on a list-order device vector."""
+        self._chk(self.lib.ugmg_clear_dirichlet(self.h, level, _ptr(x)))
 
     def set_setup(self, where):
         """Where preprocess builds the level layouts: SETUP_DEVICE or SETUP_HOST."""

@@ -114,8 +114,11 @@ int ugmg_sync(ugmg_ctx *ctx);
  * with SolveSmallBlock's n = 2 and n = 3 forms and, for n = 4, its Gauss
  * elimination with row pivoting where |pivot| < 1e-10; per-component
  * Dirichlet bits,
- * norms and stop tests; one damping factor and base limit for all
- * components; bcgs and the server take b = 1) */
+ * norms and stop tests; damping factors per component (dscalx: the
+ * smoother's x, lmgc's interpolated correction and ex's x scaled per
+ * component, ugmg_restrict damping each fine term or, over an
+ * interpolation matrix, the reset coarse rows); one base $abslimit / $red
+ * for all components; bcgs and the server take b = 1) */
 int ugmg_set_nlevels(ugmg_ctx *ctx, int nlevels, int b);
 int ugmg_set_level(ugmg_ctx *ctx, int level, int n, const int *row_ptr,
                    const int *col_idx, const double *vals,
@@ -145,7 +148,8 @@ int ugmg_set_transfer_matrix(ugmg_ctx *ctx, int level, const int *p_ptr, const i
  *  - norms, dot products and the VEC_SCALARs of the stop tests
  *    (abslimit, reduction, ugmg_lresult's defects, the per-iteration
  *    callback) take one entry per (type, component), type-major (VD_OFFSET),
- *    at most UGMG_MAXB of them; the damping factors must agree over them;
+ *    at most UGMG_MAXB of them; the damping factors must agree over
+ *    the types and components;
  *  - the transfer rows of an edge vector name the father objects of the
  *    edge's two nodes (the father node's vector for a corner's son, the
  *    father edge's vector for a midnode) with weight 1/2 each: the edge

@@ -24,6 +24,11 @@
  *   transfer  the transfer the hierarchy was given: the standard one
  *             (ugmg_set_transfer), or $M's interpolation matrices
  *             (ugmg_set_transfer_matrix); no options read here
+ * $damp, $red and $abslimit are VEC_SCALARs (sc_read, npscan.cc:1211-1278):
+ * one number for every entry, or per vector type a type letter (n node,
+ * k edge) and the values of its components ("$damp n0.6:0.7:0.8",
+ * "$red n1e-8|k1e-6"); the damping factors of node and edge vectors must
+ * agree per component, and the base ls takes one $red and $abslimit.
  * GPU-only options of lmgc: $precision fp64|mixed, $inner k (FP32 cycles
  * per FP64 step), $adapt 0|1 (a step runs fewer cycles when the rate
  * predicts the stop test met; default 1), $graph 0|1; of ls/bcgs: $nrhs k

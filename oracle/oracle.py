@@ -127,19 +127,28 @@ class Hierarchy:
         return self.ntypes * self.levels[l].b
 
 
+def _dv(v):
+    """A VEC_SCALAR of MAXB entries: one value for all, or one per entry
+    (the last repeated)."""
+    vals = [float(v)] * MAXB if np.isscalar(v) else [float(a) for a in v]
+    return np.array([vals[min(i, len(vals) - 1)] for i in range(MAXB)], np.float64)
+
+
 def params(nu1=2, nu2=2, gamma=1, baselevel=0, smoother=JAC, damp=0.8, lmgc_damp=1.0,
            base_damp=1.0, base_abslimit=1e-300, base_red=1e-12, base_maxiter=1, ex_reorder=1, ident=None):
-    """ident: VD_IDENT per norm entry (sc_cmp's grouping), None: NO_IDENT."""
+    """ident: VD_IDENT per norm entry (sc_cmp's grouping), None: NO_IDENT;
+    damp, lmgc_damp, base_damp: one value or one per component."""
     p = OrcParams()
     if ident is not None:
         p.has_ident = 1
         for i, v in enumerate(ident):
             p.ident[i] = v
     p.nu1, p.nu2, p.gamma, p.baselevel, p.smoother = nu1, nu2, gamma, baselevel, smoother
+    dd, dl, db = _dv(damp), _dv(lmgc_damp), _dv(base_damp)
     for i in range(MAXB):
-        p.damp[i] = damp
-        p.lmgc_damp[i] = lmgc_damp
-        p.base_damp[i] = base_damp
+        p.damp[i] = dd[i]
+        p.lmgc_damp[i] = dl[i]
+        p.base_damp[i] = db[i]
         p.base_abslimit[i] = base_abslimit
         p.base_red[i] = base_red
     p.base_maxiter, p.ex_reorder = base_maxiter, ex_reorder
@@ -198,7 +207,7 @@ def smoother(H, l, kind, damp, b, rank=None):
     lv = H.levels[l]
     x = np.zeros(lv.n * lv.b)
     b = np.array(b, dtype=np.float64, copy=True)
-    dv = np.full(MAXB, damp, dtype=np.float64) if np.isscalar(damp) else np.asarray(damp, np.float64)
+    dv = _dv(damp)
     rk = None if rank is None else np.ascontiguousarray(rank, np.int32)
     rc = lib().orc_smoother(H.ptr(l), kind, _p(rk), dv.ctypes.data, x.ctypes.data, b.ctypes.data)
     if rc:
@@ -209,7 +218,7 @@ def smoother(H, l, kind, damp, b, rank=None):
 def restrict(H, l, bf, damp=1.0, to_c=None):
     lc = H.levels[l - 1]
     out = np.zeros(lc.n * lc.b) if to_c is None else np.array(to_c, dtype=np.float64, copy=True)
-    dv = np.full(MAXB, damp, dtype=np.float64)
+    dv = _dv(damp)
     lib().orc_restrict(H.ptr(l), H.ptr(l - 1), out.ctypes.data, np.ascontiguousarray(bf, np.float64).ctypes.data,
                        dv.ctypes.data)
     return out
@@ -218,7 +227,7 @@ def restrict(H, l, bf, damp=1.0, to_c=None):
 def interpolate(H, l, cc, damp=1.0):
     lf = H.levels[l]
     out = np.zeros(lf.n * lf.b)
-    dv = np.full(MAXB, damp, dtype=np.float64)
+    dv = _dv(damp)
     lib().orc_interpolate(H.ptr(l), H.ptr(l - 1), out.ctypes.data, np.ascontiguousarray(cc, np.float64).ctypes.data,
                           dv.ctypes.data)
     return out
@@ -244,7 +253,7 @@ class Ex:
         lv = self.H.levels[self.l]
         x = np.zeros(lv.n * lv.b)
         b = np.array(b, dtype=np.float64, copy=True)
-        dv = np.full(MAXB, damp, dtype=np.float64)
+        dv = _dv(damp)
         rc = lib().orc_ex_smoother(self.h, self.H.ptr(self.l), dv.ctypes.data, x.ctypes.data, b.ctypes.data)
         return rc, x, b
 

@@ -131,3 +131,61 @@ def test_ex_float_option_reaches_the_parameters():
             env.init(name, o)
         assert env.params("mgs").ex_float == want
         env.close()
+
+
+def test_typed_damp_per_component():
+    """sc_read over ReadVecTypeDOUBLEs (np/udm/npscan.cc:783-855, 1211-1278):
+    "$damp n0.5:0.6 0.7" gives the node components their own factors; one
+    number gives every entry the same; a token without a type letter next
+    to typed ones, or a two-letter type, is refused."""
+    env = NP.NumProcEnv(None)
+    assert script(env, damp="n0.5:0.6 0.7", extra_mg="") == NP.ACTIVE
+    env.create("ex2", "ex")
+    assert env.init("ex2", "$damp n0.9:0.8:0.7") == NP.ACTIVE
+    env.create("base2", "ls")
+    env.init("base2", "$red 1e-12 $m 1 $I ex2")
+    env.create("mg2", "lmgc")
+    env.init("mg2", "$S pre post base2 $T transfer $damp n1.0:0.5:0.25")
+    env.vector("x", np.zeros(3))
+    env.vector("b", np.zeros(3))
+    env.create("s", "ls")
+    assert env.init("s", "$x x $b b $A A $m 5 $I mg2 $red 1e-8") == NP.EXECUTABLE
+    p = env.params("s")
+    assert list(p.damp)[:3] == [0.5, 0.6, 0.7]
+    assert list(p.base_damp)[:3] == [0.9, 0.8, 0.7]
+    assert list(p.lmgc_damp)[:3] == [1.0, 0.5, 0.25]
+    # node and edge values that agree are one factor per component
+    env.create("g", "gs")
+    assert env.init("g", "$damp n0.5:0.6|k0.5:0.6") == NP.ACTIVE
+    env.create("mg3", "lmgc")
+    env.init("mg3", "$S g g base2 $T transfer")
+    env.create("s3", "ls")
+    env.init("s3", "$x x $b b $A A $m 5 $I mg3 $red 1e-8")
+    with pytest.raises(ugmg.UgmgError):      # ex2 gives 3 values, g 2: other counts per type
+        env.params("s3")
+    env.create("ex4", "ex")
+    env.init("ex4", "$damp n0.9:0.8")
+    env.create("base4", "ls")
+    env.init("base4", "$red 1e-12 $m 1 $I ex4")
+    env.create("mg4", "lmgc")
+    env.init("mg4", "$S g g base4 $T transfer")
+    env.create("s4", "ls")
+    env.init("s4", "$x x $b b $A A $m 5 $I mg4 $red 1e-8")
+    p = env.params("s4")
+    assert list(p.damp)[:2] == [0.5, 0.6] and list(p.base_damp)[:2] == [0.9, 0.8]
+    # node and edge factors that differ are refused (one per component)
+    env.create("g5", "gs")
+    env.init("g5", "$damp n0.5|k0.6")
+    env.create("mg5", "lmgc")
+    env.init("mg5", "$S g5 g5 base2 $T transfer")
+    env.create("s5", "ls")
+    env.init("s5", "$x x $b b $A A $m 5 $I mg5 $red 1e-8")
+    with pytest.raises(ugmg.UgmgError):
+        env.params("s5")
+    # malformed forms
+    env.create("bad", "jac")
+    assert env.init("bad", "$damp 0.5|n0.6") == NP.NOT_ACTIVE       # type letter missing
+    assert env.init("bad", "$damp nk0.5") == NP.NOT_ACTIVE          # two-letter type
+    assert env.init("bad", "$damp 0.5 0.6") == NP.NOT_ACTIVE        # several values, no type
+    assert env.init("bad", "$damp n0.5:x") == NP.NOT_ACTIVE         # unreadable value
+    assert env.init("bad", "$damp 0.75") == NP.ACTIVE

@@ -14,6 +14,8 @@ from ug_amd import problems
 
 
 def _ops(P, l, kind, damp, H):
+    """A and the smoother's error operator I - D M^-1 A, D the damping per
+    component (a scalar: the same for all)."""
     lv = P.levels[l]
     b, n = lv.b, lv.n
     A = lv.dense()
@@ -24,13 +26,18 @@ def _ops(P, l, kind, damp, H):
         rk = col.astype(np.int64) * n + np.arange(n)
         blk = np.kron(rk[None, :] <= rk[:, None], np.ones((b, b))).astype(bool)
     M = np.where(blk, A, 0.0)
-    return A, np.eye(b * n) - damp * np.linalg.solve(M, A)
+    D = np.tile(np.broadcast_to(np.asarray(damp, float), (b,)), n)
+    return A, np.eye(b * n) - D[:, None] * np.linalg.solve(M, A)
 
 
-def mg_error(P, top, bl, kind, damp, nu1, nu2):
+def mg_error(P, top, bl, kind, damp, nu1, nu2, lmgc_damp=1.0, base_damp=1.0):
+    """Error operator of one V-cycle: smoothers with $damp, the coarse
+    correction interpolated and scaled by lmgc's $damp (dscalx), the base
+    level solved exactly and scaled by ex's $damp, each per component."""
     H = O.Hierarchy(P.levels)
     b = P.levels[0].b
-    E = {bl: np.zeros((b * P.levels[bl].n,) * 2)}
+    nbl = P.levels[bl].n
+    E = {bl: np.eye(b * nbl) - np.diag(np.tile(np.broadcast_to(np.asarray(base_damp, float), (b,)), nbl))}
     for l in range(bl + 1, top + 1):
         f, c = P.levels[l], P.levels[l - 1]
         A, S = _ops(P, l, kind, damp, H)
@@ -38,7 +45,8 @@ def mg_error(P, top, bl, kind, damp, nu1, nu2):
         Pb = np.kron(f.dense_p(c.n), np.eye(b))
         fmask = ((f.skip[:, None] >> np.arange(b)) & 1).ravel() == 0
         cmask = ((c.skip[:, None] >> np.arange(b)) & 1).ravel() == 0
-        CG = np.eye(A.shape[0]) - (Pb * fmask[:, None]) @ (np.eye(Ac.shape[0]) - E[l - 1]) @ \
+        Dl = np.tile(np.broadcast_to(np.asarray(lmgc_damp, float), (b,)), f.n)
+        CG = np.eye(A.shape[0]) - (Dl * fmask)[:, None] * Pb @ (np.eye(Ac.shape[0]) - E[l - 1]) @ \
             np.linalg.solve(Ac, (Pb * cmask[None, :]).T @ A)
         E[l] = np.linalg.matrix_power(S, nu2) @ CG @ np.linalg.matrix_power(S, nu1)
     return E[top]
@@ -80,6 +88,36 @@ def test_block_lmgc_cycle_is_textbook_vcycle(b, kind, damp):
     want = (np.eye(n2) - E) @ np.linalg.solve(A, rhs)
     assert np.max(np.abs(c - want)) < 1e-11 * np.max(np.abs(want))
     assert np.max(np.abs(bn - (rhs - A @ c))) < 1e-13 * np.max(np.abs(rhs))
+
+
+@pytest.mark.parametrize("b", [2, 4])
+@pytest.mark.parametrize("kind", ["jac", "mcgs"])
+def test_block_lmgc_cycle_with_damping_per_component(b, kind):
+    """Smoother $damp, lmgc $damp and ex $damp with a factor per component
+    (dscalx, np/algebra/ugblas.cc:2481-2515: Smoother np/procs/iter.cc:836,
+    Lmgc's InterpolateCorrection with its $damp iter.cc:7886-7887, EXSmoother
+    iter.cc:9055) against the textbook V-cycle with D = diag(damp) in each
+    place."""
+    P = problems.p1_local(3, 0, b)
+    H = O.Hierarchy(P.levels)
+    top, bl = P.top, 1
+    dm, dl, db = [0.5, 0.7, 0.6, 0.8][:b], [0.9, 0.75, 1.0, 0.85][:b], [0.95, 0.8, 0.9, 0.7][:b]
+    E = mg_error(P, top, bl, kind, dm, 2, 2, lmgc_damp=dl, base_damp=db)
+    A = P.levels[top].dense()
+    n2 = A.shape[0]
+    rhs = np.random.default_rng(5).uniform(-1, 1, n2)
+    rhs[np.repeat(P.levels[top].skip != 0, b)] = 0.0
+    p = O.params(smoother=O.JAC if kind == "jac" else O.MCGS, damp=dm, lmgc_damp=dl, base_damp=db,
+                 baselevel=bl)
+    c, bn = O.lmgc_iter(H, top, p, rhs)
+    want = (np.eye(n2) - E) @ np.linalg.solve(A, rhs)
+    assert np.max(np.abs(c - want)) < 1e-11 * np.max(np.abs(want))
+    assert np.max(np.abs(bn - (rhs - A @ c))) < 1e-13 * np.max(np.abs(rhs))
+    # a scalar factor is the same as that factor for every component
+    q = O.params(smoother=O.JAC if kind == "jac" else O.MCGS, damp=[0.6] * b, baselevel=bl)
+    c1, _ = O.lmgc_iter(H, top, q, rhs)
+    c2, _ = O.lmgc_iter(H, top, O.params(smoother=q.smoother, damp=0.6, baselevel=bl), rhs)
+    assert np.array_equal(c1, c2)
 
 
 def test_block4_jacobi_step_with_row_exchange_is_dense_block_solve():

@@ -498,15 +498,47 @@ void to_user(ugmg_ctx *ctx, int l, const TS *src, TD *dst, size_t ldd, int nrhs 
 
 // ------------------------------------------------------------ cycle drivers
 
+// damping factors that differ between the components of a block
+inline bool per_comp(const ugmg_ctx *ctx, const double *d) {
+  for (int i = 1; i < ctx->b; ++i)
+    if (d[i] != d[0]) return true;
+  return false;
+}
+// ... or between the norm entries ((vector type, component) pairs)
+inline bool per_comp_entries(const ugmg_ctx *ctx, const double *d) {
+  for (int i = 1; i < ctx->nc; ++i)
+    if (d[i] != d[0]) return true;
+  return false;
+}
+
 template <typename T>
 void smooth_step(ugmg_ctx *ctx, int l, T *t, T *b, T *c, int cmode, T *xout, int k0 = 0) {
   DevLevel &d = ctx->dev[l];
   hipStream_t st = ctx->stream;
   const T damp = (T)ctx->p.damp[0];
   const Batch bt = batch(ctx, l);
+  const int mode = cmode < 0 ? 0 : (cmode == 0 ? 3 : 2);
+  if (per_comp(ctx, ctx->p.damp)) {
+    // Smoother (np/procs/iter.cc:817-842, dscalx at 836) with $damp per component: the step's x,
+    // x *= damp per component, then b -= A x (and c (+)= x)
+    if (ctx->p.smoother == UGMG_SMOOTH_JAC) {
+      launch_jac<T>(st, d.n, b, d.diag<T>(), d.vclass, (T)1, t, bt);
+    } else {
+      for (int k = k0; k < d.ncolors; ++k)
+        launch_gs_color<T, T>(st, d.L.d<T>(), d.color_slices[k],
+                              d.color_slices[k + 1] - d.color_slices[k], d.color_wmax[k], b,
+                              d.diag<T>(), d.vclass, t, d.ntv<T>(), bt);
+    }
+    launch_scale_comp<T>(st, d.n, d.nb, ctx->p.damp, t, (T *)nullptr, 0, bt);
+    launch_spmv<T, T, T>(st, d.A.d<T>(), 0, d.A.nslices, t, b, c, mode, d.ntv<T>(), bt);
+    if (xout)   // an error here is left for the caller's hipGetLastError check
+      for (int r = 0; r < bt.nrhs; ++r)
+        (void)(hipMemcpyAsync(xout + r * bt.vs, t + r * bt.vs, sizeof(T) * d.n * d.nb,
+                                hipMemcpyDeviceToDevice, st));
+    return;
+  }
   if (ctx->p.smoother == UGMG_SMOOTH_JAC) {
     launch_jac<T>(st, d.n, b, d.diag<T>(), d.vclass, damp, t, bt);
-    const int mode = cmode < 0 ? 0 : (cmode == 0 ? 3 : 2);
     launch_spmv<T, T, T>(st, d.A.d<T>(), 0, d.A.nslices, t, b, c, mode, d.ntv<T>(), bt);
   } else {
     for (int k = k0; k < d.ncolors; ++k)
@@ -525,7 +557,8 @@ void smooth_step(ugmg_ctx *ctx, int l, T *t, T *b, T *c, int cmode, T *xout, int
 template <typename T>
 void smooth(ugmg_ctx *ctx, int l, T *t, T *b, T *c, int cmode, int nsteps, int k0 = 0) {
   DevLevel &d = ctx->dev[l];
-  if (ctx->p.smoother != UGMG_SMOOTH_GS || !d.all_active || nsteps < 2 || !ctx->fuse_steps) {
+  if (ctx->p.smoother != UGMG_SMOOTH_GS || !d.all_active || nsteps < 2 || !ctx->fuse_steps ||
+      per_comp(ctx, ctx->p.damp)) {
     for (int i = 0; i < nsteps; ++i)
       smooth_step<T>(ctx, l, t, b, c, i == 0 ? cmode : 1, nullptr, i == 0 ? k0 : 0);
     return;
@@ -614,7 +647,11 @@ void base_solve(ugmg_ctx *ctx, T *c, T *b, int cmode, bool have_parts, bool need
   DevLevel &d = ctx->dev[ctx->bl];
   hipStream_t st = ctx->stream;
   const ugmg_params &p = ctx->p;
-  const Batch bt = batch(ctx, ctx->bl);
+  Batch bt = batch(ctx, ctx->bl);
+  if (per_comp(ctx, p.base_damp)) {   // ex $damp per component
+    for (int q = 0; q < d.nb && q < 4; ++q) bt.pdamp.d[q] = p.base_damp[q];
+    bt.pdamp.vb = d.nb;
+  }
   int np = have_parts ? ctx->base_nparts
                       : launch_dot_part<T>(st, d.n, b, d.vclass, ctx->base_part, bt);
   T *x = d.tv<T>();
@@ -693,8 +730,14 @@ void lmgc(ugmg_ctx *ctx, int l, T *c, T *b, int cmode, bool c0_done = false) {
     }
   }
   // InterpolateCorrection(level, t, c, damp); dadd(c, t); dmatmul_minus(b, A, t)
-  launch_interp<T>(st, d.P.d<T>(), d.iskip, dc.cv<T>(), (T)p.lmgc_damp[0], t, c, d.ntv<T>(), nb, d.vs,
-                   dc.vs, d.nb);
+  if (per_comp(ctx, p.lmgc_damp)) {   // damp per component: dscalx after the sum
+    launch_interp<T>(st, d.P.d<T>(), d.iskip, dc.cv<T>(), (T)1, t, (T *)nullptr, d.ntv<T>(), nb, d.vs,
+                     dc.vs, d.nb);
+    launch_scale_comp<T>(st, d.n, d.nb, p.lmgc_damp, t, c, 1, batch(ctx, l));
+  } else {
+    launch_interp<T>(st, d.P.d<T>(), d.iskip, dc.cv<T>(), (T)p.lmgc_damp[0], t, c, d.ntv<T>(), nb, d.vs,
+                     dc.vs, d.nb);
+  }
   if (fold_colour0(ctx, l, p.nu2)) {
     post_smooth_folded<T>(ctx, l, t, b, c, p.nu2);
     return;
@@ -2034,13 +2077,12 @@ int ugmg_set_params(ugmg_ctx *ctx, const ugmg_params *p) {
       (p->smoother != UGMG_SMOOTH_JAC && p->smoother != UGMG_SMOOTH_GS) ||
       (p->precision != UGMG_FP64 && p->precision != UGMG_MIXED) || p->inner_cycles < 0)
     return fail(ctx, UGMG_E_ARG, "invalid parameter");
+  // the damping factors may differ per component ($damp, lmgc $damp, ex
+  // $damp: dscalx per component); the base solve's stop limits may not
   if (ctx->b > 1)
     for (int i = 1; i < ctx->b; ++i)
-      if (p->damp[i] != p->damp[0] || p->lmgc_damp[i] != p->lmgc_damp[0] ||
-          p->base_damp[i] != p->base_damp[0] || p->base_abslimit[i] != p->base_abslimit[0] ||
-          p->base_red[i] != p->base_red[0])
-        return fail(ctx, UGMG_E_ARG,
-                    "block hierarchies take one damping / base limit for all components");
+      if (p->base_abslimit[i] != p->base_abslimit[0] || p->base_red[i] != p->base_red[0])
+        return fail(ctx, UGMG_E_ARG, "block hierarchies take one base $abslimit / $red for all components");
   const bool relayout = !ctx->params_set || p->smoother != ctx->p.smoother ||
                         p->baselevel != ctx->p.baselevel || p->ex_reorder != ctx->p.ex_reorder ||
                         p->ex_float != ctx->p.ex_float;
@@ -2262,7 +2304,12 @@ int ugmg_jacobi_step(ugmg_ctx *ctx, int level, double *x, double *b) {
   DevLevel &d = ctx->dev[level];
   double *xd = d.perm ? d.t : x, *bd = d.perm ? d.b : b;
   if (d.perm) to_dev<double, double>(ctx, level, b, d.vs, bd);
-  launch_jac<double>(ctx->stream, d.n, bd, d.diag64, d.vclass, ctx->p.damp[0], xd, B1(d));
+  if (per_comp(ctx, ctx->p.damp)) {
+    launch_jac<double>(ctx->stream, d.n, bd, d.diag64, d.vclass, 1.0, xd, B1(d));
+    launch_scale_comp<double>(ctx->stream, d.n, d.nb, ctx->p.damp, xd, nullptr, 0, B1(d));
+  } else {
+    launch_jac<double>(ctx->stream, d.n, bd, d.diag64, d.vclass, ctx->p.damp[0], xd, B1(d));
+  }
   launch_spmv<double, double, double>(ctx->stream, d.A.d64(), 0, d.A.nslices, xd, bd, nullptr, 0, d.nt, B1(d));
   if (d.perm) {
     to_user<double, double>(ctx, level, xd, x, d.vs);
@@ -2307,8 +2354,26 @@ int ugmg_restrict(ugmg_ctx *ctx, int level, double *to_c, const double *from_f,
     to_dev<double, double>(ctx, level - 1, to_c, dc.vs, dc.b);
     cd = dc.b;
   }
-  launch_restrict<double>(ctx->stream, d.PT.d64(), dc.skip, dc.vnclass, dc.vclass, fd,
-                          damp ? damp[0] : 1.0, cd, nullptr, d.nt, 1, d.vs, dc.vs, d.nb);
+  if (damp && per_comp_entries(ctx, damp)) {
+    if (ctx->ntypes > 1) return fail(ctx, UGMG_E_ARG, "several vector types take one damping factor");
+    if (d.PT.d64().bb > 1) {
+      // RestrictByMatrix: the coarse rows of VNCLASS >= 2 scaled after the sum
+      launch_restrict<double>(ctx->stream, d.PT.d64(), dc.skip, dc.vnclass, dc.vclass, fd, 1.0, cd,
+                              nullptr, d.nt, 1, d.vs, dc.vs, d.nb);
+      launch_scale_comp<double>(ctx->stream, dc.n, dc.nb, damp, cd, nullptr, 0, B1(dc), dc.vnclass, 2);
+    } else {
+      // the standard restriction damps each fine term: scale b_f first
+      if (fd != d.b) HIPCHK(hipMemcpyAsync(d.t, fd, sizeof(double) * d.n * d.nb, hipMemcpyDeviceToDevice,
+                                           ctx->stream));
+      double *sf = fd != d.b ? d.t : d.b;
+      launch_scale_comp<double>(ctx->stream, d.n, d.nb, damp, sf, nullptr, 0, B1(d));
+      launch_restrict<double>(ctx->stream, d.PT.d64(), dc.skip, dc.vnclass, dc.vclass, sf, 1.0, cd,
+                              nullptr, d.nt, 1, d.vs, dc.vs, d.nb);
+    }
+  } else {
+    launch_restrict<double>(ctx->stream, d.PT.d64(), dc.skip, dc.vnclass, dc.vclass, fd,
+                            damp ? damp[0] : 1.0, cd, nullptr, d.nt, 1, d.vs, dc.vs, d.nb);
+  }
   if (dc.perm) to_user<double, double>(ctx, level - 1, cd, to_c, dc.vs);
   HIPCHK(hipGetLastError());
   return UGMG_OK;
@@ -2326,8 +2391,14 @@ int ugmg_interpolate(ugmg_ctx *ctx, int level, double *to_f, const double *from_
     to_dev<double, double>(ctx, level - 1, from_c, dc.vs, dc.c);
     cd = dc.c;
   }
-  launch_interp<double>(ctx->stream, d.P.d64(), d.iskip, cd, damp ? damp[0] : 1.0, fd,
-                        nullptr, d.nt, 1, d.vs, dc.vs, d.nb);
+  if (damp && per_comp_entries(ctx, damp)) {   // dscalx after the sum
+    if (ctx->ntypes > 1) return fail(ctx, UGMG_E_ARG, "several vector types take one damping factor");
+    launch_interp<double>(ctx->stream, d.P.d64(), d.iskip, cd, 1.0, fd, nullptr, d.nt, 1, d.vs, dc.vs, d.nb);
+    launch_scale_comp<double>(ctx->stream, d.n, d.nb, damp, fd, nullptr, 0, B1(d));
+  } else {
+    launch_interp<double>(ctx->stream, d.P.d64(), d.iskip, cd, damp ? damp[0] : 1.0, fd,
+                          nullptr, d.nt, 1, d.vs, dc.vs, d.nb);
+  }
   if (d.perm) to_user<double, double>(ctx, level, fd, to_f, d.vs);
   HIPCHK(hipGetLastError());
   return UGMG_OK;

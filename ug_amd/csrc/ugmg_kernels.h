@@ -105,7 +105,22 @@ struct Batch {
   // the stop tests' VD_IDENT grouping of the norm entries (4 bits per
   // entry, entry q at bits 4q..4q+3); -1: NO_IDENT
   int idents = -1;
+  // ex $damp per component for the base solve (unknown r takes
+  // pdamp.d[r % pdamp.vb]); pdamp.vb = 0: the one damping factor
+  struct PDamp {
+    double d[4] = {};
+    int vb = 0;
+  } pdamp;
 };
+
+// t[v b + q] *= d[q] on n vectors of b components, then (c non-null) c =
+// t (cmode 0) or c += t (cmode 1): dscalx with a damping factor per
+// component (np/algebra/ugblas.cc:2510-2528) after a step or an
+// interpolation formed with damping 1; cls: only the rows whose class is
+// at least clsmin (the coarse rows a restriction wrote)
+template <typename T>
+void launch_scale_comp(hipStream_t st, int n, int b, const double *d, T *t, T *c, int cmode, Batch bt,
+                       const uint8_t *cls = nullptr, int clsmin = 0);
 
 // per right-hand side state of the batched BCGSSolver (np/procs/ls.cc:
 // 1864-2062); written only by the scalar kernels, read by the vector kernels

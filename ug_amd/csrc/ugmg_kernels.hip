@@ -1148,7 +1148,8 @@ __global__ __launch_bounds__(kBlock) void k_base_gemv_bk(int n3, const TI *__res
                                                           int cmode, double damp, double abslimit,
                                                           double red, const double *__restrict__ part,
                                                           int nparts, double *__restrict__ first,
-                                                          int it, size_t vs, size_t ims, int idents) {
+                                                          int it, size_t vs, size_t ims, int idents,
+                                                          Batch::PDamp pd) {
   const int row = (blockIdx.x * kBlock + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   const int cj = blockIdx.y;
@@ -1201,7 +1202,7 @@ __global__ __launch_bounds__(kBlock) void k_base_gemv_bk(int n3, const TI *__res
     x = 0.0;
   } else {
     for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-    x *= damp;
+    x *= pd.vb ? pd.d[row % pd.vb] : damp;
   }
   if (lane == 0) {
     const T xv = (T)x;
@@ -1860,7 +1861,7 @@ void launch_base_gemv(hipStream_t st, int n, const TI *inv, const int *rowmap, c
     const int g = (n * 64 + kBlock - 1) / kBlock;
     BK_DISPATCH(bt.nc(), k_base_gemv_bk<BK, T, TI><<<grid2(g, bt.nrhs), kBlock, 0, st>>>(
                           n, inv, rowmap, cp, ncols, b, c, xout, cmode, damp, abslimit, red, part, nparts,
-                          first, it, bt.vs, bt.ims, bt.idents));
+                          first, it, bt.vs, bt.ims, bt.idents, bt.pdamp));
   } else {           // one workgroup per row
     k_base_gemv<T, TI><<<grid2(n, bt.nrhs), kBlock, 0, st>>>(n, inv, rowmap, cp, ncols, b, bc, c, xout,
                                                              cmode, damp, abslimit, red, part,
@@ -2020,6 +2021,32 @@ __global__ __launch_bounds__(kBlock) void k_iskip(int n, const uint8_t *__restri
 void launch_iskip(hipStream_t st, int n, const uint8_t *vt, const uint32_t *skip, uint32_t *iskip) {
   if (n > 0) k_iskip<<<nblk(n), kBlock, 0, st>>>(n, vt, skip, iskip);
 }
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_scale_comp(int n, int b, Batch::PDamp pd, T *__restrict__ t,
+                                                        T *__restrict__ c, int cmode, size_t vs,
+                                                        const uint8_t *__restrict__ cls, int clsmin) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n * b) return;
+  if (cls && cls[i / b] < clsmin) return;
+  const size_t k = (size_t)blockIdx.y * vs + i;
+  const T v = t[k] * (T)pd.d[i % b];
+  t[k] = v;
+  if (c) c[k] = cmode == 0 ? v : c[k] + v;
+}
+template <typename T>
+void launch_scale_comp(hipStream_t st, int n, int b, const double *d, T *t, T *c, int cmode, Batch bt,
+                       const uint8_t *cls, int clsmin) {
+  if (n <= 0) return;
+  Batch::PDamp pd;
+  for (int q = 0; q < b && q < 4; ++q) pd.d[q] = d[q];
+  pd.vb = b;
+  k_scale_comp<T><<<grid2(nblk(n * b), bt.nrhs), kBlock, 0, st>>>(n, b, pd, t, c, cmode, bt.vs, cls, clsmin);
+}
+template void launch_scale_comp<double>(hipStream_t, int, int, const double *, double *, double *, int, Batch,
+                                        const uint8_t *, int);
+template void launch_scale_comp<float>(hipStream_t, int, int, const double *, float *, float *, int, Batch,
+                                       const uint8_t *, int);
 
 __global__ __launch_bounds__(kBlock) void k_clear_dirichlet(int n, int b, const int *__restrict__ perm,
                                                              const uint32_t *__restrict__ skip,

@@ -3,8 +3,10 @@
 //   ReadArgvINT / ReadArgvDOUBLE  low/scan.cc:103-160  (first character of
 //                                 the argument, then "%s %d" / "%s %lf")
 //   ReadArgvOption                low/scan.cc:316-333  (flag, or flag k)
-//   sc_read                       np/udm/npscan.cc:1211-1280 (one value for
-//                                 every component; absent -> 2)
+//   sc_read                       np/udm/npscan.cc:1211-1280 over
+//                                 ReadVecTypeDOUBLEs (npscan.cc:783-855): one
+//                                 value for every entry, or values per vector
+//                                 type ("n0.5:0.6|k0.7:0.8"); absent -> 2
 // and each class reads the options its UG Init function reads, with the
 // same defaults and the same returned state.
 #include "../../include/ugmg_np.h"
@@ -12,13 +14,13 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cctype>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <map>
 #include <new>
 #include <string>
-#include <vector>
 #include <vector>
 
 namespace {
@@ -82,13 +84,69 @@ bool read_string(const Argv &av, const char *name, std::string &out) {
   return false;
 }
 
-// sc_read: 0 read, 2 absent, 3 unreadable; one value for every component
-int sc_read(const Argv &av, const char *name, double x[UGMG_MAXB]) {
+// Values a VEC_SCALAR option gave per vector type (node 'n', edge 'k', the
+// default type names of formats.cc:3062-3063); -1: one value for all entries
+struct ScCount {
+  int n[2] = {-1, -1};
+  bool typed() const { return n[0] >= 0; }
+};
+
+// sc_read: 0 read, 2 absent, 3 unreadable, 4 malformed.  The option's value
+// is either one number (every entry) or tokens separated by '|', each a type
+// letter followed by that type's component values separated by blanks or
+// ':'; x receives them type-major (the VD_OFFSET order of the norms)
+int sc_read(const Argv &av, const char *name, double x[UGMG_MAXB], ScCount *cnt = nullptr) {
   std::string v;
   if (!read_string(av, name, v)) return 2;
-  double d;
-  if (std::sscanf(v.c_str(), "%lf", &d) != 1) return 3;
-  for (int i = 0; i < UGMG_MAXB; ++i) x[i] = d;
+  std::vector<double> per[2];
+  bool typed = false, untyped = false;
+  size_t p0 = 0;
+  while (p0 <= v.size()) {
+    size_t p1 = v.find('|', p0);
+    if (p1 == std::string::npos) p1 = v.size();
+    std::string tok = v.substr(p0, p1 - p0);
+    p0 = p1 + 1;
+    const size_t f = tok.find_first_not_of(" \t\n");
+    const char c = f == std::string::npos ? '\0' : tok[f];
+    const int t = c == 'n' ? 0 : c == 'k' ? 1 : -1;
+    if (t < 0) {
+      untyped = true;
+      continue;
+    }
+    typed = true;
+    if (f + 1 < tok.size() && std::isalpha((unsigned char)tok[f + 1])) return 4;   // two-letter type
+    size_t q = f + 1;
+    while (q < tok.size()) {
+      const size_t a = tok.find_first_not_of(" \t:", q);
+      if (a == std::string::npos) break;
+      const size_t z = tok.find_first_of(" \t:", a);
+      const std::string num = tok.substr(a, z == std::string::npos ? std::string::npos : z - a);
+      double d;
+      char extra;
+      if (std::sscanf(num.c_str(), "%lf%c", &d, &extra) != 1) return 3;
+      if (per[t].size() >= (size_t)UGMG_MAXB) return 4;
+      per[t].push_back(d);
+      q = z == std::string::npos ? tok.size() : z;
+    }
+  }
+  if (typed && untyped) return 4;   // type letter missing on one of the tokens
+  if (!typed) {                      // one value for every entry
+    double d;
+    if (std::sscanf(v.c_str(), "%lf", &d) != 1) return 3;
+    const size_t z = v.find_first_of(" \t:");
+    if (z != std::string::npos && v.find_first_not_of(" \t:", z) != std::string::npos) return 4;
+    for (int i = 0; i < UGMG_MAXB; ++i) x[i] = d;
+    if (cnt) *cnt = ScCount{};
+    return 0;
+  }
+  if (per[0].size() + per[1].size() > (size_t)UGMG_MAXB) return 4;
+  int k = 0;
+  for (int t = 0; t < 2; ++t)
+    for (double d : per[t]) x[k++] = d;
+  if (cnt) {
+    cnt->n[0] = (int)per[0].size();
+    cnt->n[1] = (int)per[1].size();
+  }
   return 0;
 }
 
@@ -110,6 +168,7 @@ struct NumProc {
   // smoothers
   double damp[UGMG_MAXB];
   int reorder = 1, fmode = 0;
+  ScCount damp_n, abslimit_n, red_n;   // per-type counts of the typed forms
 
   NumProc() {
     for (int i = 0; i < UGMG_MAXB; ++i) {
@@ -175,9 +234,10 @@ int ls_state(ugmg_np_env *e, const Argv &av, NumProc &o) {
   read_string(av, "A", o.A);
   read_string(av, "x", o.x);
   read_string(av, "b", o.b);
-  if (sc_read(av, "abslimit", o.abslimit))
+  o.abslimit_n = o.red_n = ScCount{};
+  if (sc_read(av, "abslimit", o.abslimit, &o.abslimit_n))
     for (int i = 0; i < UGMG_MAXB; ++i) o.abslimit[i] = kAbsLimit;
-  if (sc_read(av, "red", o.red)) return UGMG_NP_ACTIVE;
+  if (sc_read(av, "red", o.red, &o.red_n)) return UGMG_NP_ACTIVE;
   if (o.x.empty() || o.b.empty() || o.A.empty() || !e->vec.count(o.x) || !e->vec.count(o.b))
     return UGMG_NP_ACTIVE;
   return UGMG_NP_EXECUTABLE;
@@ -332,7 +392,8 @@ int init_object(ugmg_np_env *e, NumProc &o) {
     if (!pr || !po || !is_iter(pr->cls) || !is_iter(po->cls))
       return fail(e, UGMG_NP_NOT_ACTIVE, "lmgc: $S pre/post do not name iterations");
     if (!ba || ba->cls != "ls") return fail(e, UGMG_NP_NOT_ACTIVE, "lmgc: $S base does not name an ls");
-    if (sc_read(av, "damp", o.damp))
+    o.damp_n = ScCount{};
+    if (sc_read(av, "damp", o.damp, &o.damp_n))
       for (int i = 0; i < UGMG_MAXB; ++i) o.damp[i] = 1.0;
     std::string prec;
     o.precision = UGMG_FP64;
@@ -347,18 +408,44 @@ int init_object(ugmg_np_env *e, NumProc &o) {
   }
   if (o.cls == "jac" || o.cls == "gs") {   // SmootherInit (iter.cc:763-790)
     for (int i = 0; i < UGMG_MAXB; ++i) o.damp[i] = 1.0;
-    sc_read(av, "damp", o.damp);
+    o.damp_n = ScCount{};
+    if (sc_read(av, "damp", o.damp, &o.damp_n) > 2) return fail(e, UGMG_NP_NOT_ACTIVE, "jac/gs: bad $damp");
     return iter_state(av, o);
   }
   if (o.cls == "ex") {   // EXInit (iter.cc:8248-8260)
     if (read_int(av, "o", &o.reorder)) o.reorder = 1;
     o.fmode = read_option(av, "f");
     for (int i = 0; i < UGMG_MAXB; ++i) o.damp[i] = 1.0;
-    sc_read(av, "damp", o.damp);
+    o.damp_n = ScCount{};
+    if (sc_read(av, "damp", o.damp, &o.damp_n) > 2) return fail(e, UGMG_NP_NOT_ACTIVE, "ex: bad $damp");
     return iter_state(av, o);
   }
   if (o.cls == "transfer") return UGMG_NP_EXECUTABLE;
   return fail(e, UGMG_E_ARG, "unknown class " + o.cls);
+}
+
+// A typed $damp as the damping per component: each type it names carries
+// the block's b values (sc_read's check against the descriptor,
+// npscan.cc:1258-1277); the device smoothers take one factor per component
+// for every vector type, so node and edge values must agree
+// (all typed ones of a chain the same count, as they describe one vector)
+int resolve_damp(ugmg_np_env *e, const NumProc &o, const char *what, double out[UGMG_MAXB], int &count) {
+  for (int i = 0; i < UGMG_MAXB; ++i) out[i] = o.damp[i];
+  if (!o.damp_n.typed()) return 0;
+  int b = e->ctx ? ugmg_block(e->ctx) : -1;
+  if (b <= 0) b = count;
+  const int nt = o.damp_n.n[0] > 0 && o.damp_n.n[1] > 0 ? 2 : 1;
+  const int per = o.damp_n.n[0] > 0 ? o.damp_n.n[0] : o.damp_n.n[1];
+  if ((nt == 2 && o.damp_n.n[0] != o.damp_n.n[1]) || (b > 0 && per != b))
+    return fail(e, UGMG_E_ARG, std::string(what) + ": $damp gives a count per type other than the block's");
+  for (int j = 0; j < per; ++j) {
+    if (nt == 2 && o.damp[per + j] != o.damp[j])
+      return fail(e, UGMG_E_ARG, std::string(what) + ": node and edge $damp differ (one factor per component)");
+    out[j] = o.damp[j];
+  }
+  for (int j = per; j < UGMG_MAXB; ++j) out[j] = o.damp[per - 1];
+  count = per;
+  return 0;
 }
 
 // the ugmg_params a solver object's $I chain describes
@@ -373,19 +460,28 @@ int build_params(ugmg_np_env *e, NumProc &o, ugmg_params &p, bool &use_iter) {
   if (!pr || !po || !ba) return fail(e, UGMG_E_STATE, "lmgc parts missing");
   if (pr->cls != po->cls || pr->cls == "ex" || pr->cls == "lmgc")
     return fail(e, UGMG_E_ARG, "pre- and post-smoother must be the same jac or gs class");
+  double dpre[UGMG_MAXB], dpost[UGMG_MAXB], dmg[UGMG_MAXB], dex[UGMG_MAXB];
+  int rc, count = -1;
+  if ((rc = resolve_damp(e, *pr, "pre-smoother", dpre, count)) ||
+      (rc = resolve_damp(e, *po, "post-smoother", dpost, count)) ||
+      (rc = resolve_damp(e, *mg, "lmgc", dmg, count)))
+    return rc;
   for (int i = 0; i < UGMG_MAXB; ++i)
-    if (pr->damp[i] != po->damp[i]) return fail(e, UGMG_E_ARG, "pre- and post-smoother $damp differ");
+    if (dpre[i] != dpost[i]) return fail(e, UGMG_E_ARG, "pre- and post-smoother $damp differ");
   NumProc *ex = find(e, ba->I);
   if (!ex || ex->cls != "ex") return fail(e, UGMG_E_ARG, "the base ls must iterate with ex");
+  if ((rc = resolve_damp(e, *ex, "ex", dex, count))) return rc;
+  if (ba->abslimit_n.typed() || ba->red_n.typed())
+    return fail(e, UGMG_E_ARG, "the base ls takes one $abslimit / $red for every entry");
   p.nu1 = mg->nu1;
   p.nu2 = mg->nu2;
   p.gamma = mg->gamma;
   p.baselevel = mg->bl;
   p.smoother = pr->cls == "jac" ? UGMG_SMOOTH_JAC : UGMG_SMOOTH_GS;
   for (int i = 0; i < UGMG_MAXB; ++i) {
-    p.damp[i] = pr->damp[i];
-    p.lmgc_damp[i] = mg->damp[i];
-    p.base_damp[i] = ex->damp[i];
+    p.damp[i] = dpre[i];
+    p.lmgc_damp[i] = dmg[i];
+    p.base_damp[i] = dex[i];
     p.base_abslimit[i] = ba->abslimit[i];
     p.base_red[i] = ba->red[i];
   }
@@ -396,6 +492,20 @@ int build_params(ugmg_np_env *e, NumProc &o, ugmg_params &p, bool &use_iter) {
   p.inner_cycles = mg->inner;
   p.inner_adapt = mg->adapt;
   p.use_graph = mg->graph;
+  return 0;
+}
+
+// typed $abslimit / $red against the prepared hierarchy: b values for each
+// of its vector types and none for the others (npscan.cc:1258-1277)
+int check_limits(ugmg_np_env *e, const NumProc &o) {
+  const int nc = ugmg_norm_entries(e->ctx), b = ugmg_block(e->ctx);
+  if (nc <= 0 || b <= 0) return 0;
+  const int nt = nc / b;
+  for (const ScCount *c : {&o.abslimit_n, &o.red_n})
+    if (c->typed())
+      for (int t = 0; t < 2; ++t)
+        if (c->n[t] != (t < nt ? b : 0))
+          return fail(e, UGMG_E_ARG, "number of $abslimit / $red values per type does not match the vectors");
   return 0;
 }
 
@@ -516,6 +626,7 @@ int ugmg_np_execute(ugmg_np_env *env, const char *name, const char *flags) {
         return fail(env, rc, ugmg_last_error(env->ctx));
     }
   }
+  if ((rc = check_limits(env, *o))) return rc;
   const int ncols = o->cls == "bcgs" ? o->nrhs : 1;
   o->res.assign(ncols, ugmg_lresult{});
   // a locally refined hierarchy (FULLREFINELEVEL below the level): LinearSolver

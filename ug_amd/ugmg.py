@@ -143,6 +143,11 @@ def load():
     return L
 
 
+def _vec_scalar(v):
+    vals = [float(v)] * MAXB if np.isscalar(v) else [float(a) for a in v]
+    return (C.c_double * MAXB)(*[vals[min(i, len(vals) - 1)] for i in range(MAXB)])
+
+
 def default_params(**kw) -> Params:
     p = Params()
     load().ugmg_params_default(C.byref(p))
@@ -376,11 +381,13 @@ class Context:
         self._chk(self.lib.ugmg_gs_step(self.h, level, _ptr(x), _ptr(b)))
 
     def restrict(self, level, to_c, from_f, damp=1.0):
-        d = (C.c_double * MAXB)(*[damp] * MAXB)
+        """damp: one factor, or one per norm entry (VEC_SCALAR)."""
+        d = _vec_scalar(damp)
         self._chk(self.lib.ugmg_restrict(self.h, level, _ptr(to_c), _ptr(from_f), C.addressof(d)))
 
     def interpolate(self, level, to_f, from_c, damp=1.0):
-        d = (C.c_double * MAXB)(*[damp] * MAXB)
+        """damp: one factor, or one per norm entry (VEC_SCALAR)."""
+        d = _vec_scalar(damp)
         self._chk(self.lib.ugmg_interpolate(self.h, level, _ptr(to_f), _ptr(from_c), C.addressof(d)))
 
     def base_solve(self, c, b):

@@ -75,6 +75,12 @@ typedef struct {
                                        an FP32 copy of the dense coarse inverse
                                        (default 0: the FP64 inverse, sums in
                                        FP64)                                   */
+  int post_set;                     /* 0 (default): lmgc's post-smoother is its
+                                       pre-smoother; 1: post_smoother and
+                                       post_damp (lmgc $S pre post base with
+                                       two different smoother objects)        */
+  int post_smoother;                /* UGMG_SMOOTH_JAC / UGMG_SMOOTH_GS       */
+  double post_damp[UGMG_MAXB];      /* post-smoother $damp                    */
 } ugmg_params;
 
 /* LRESULT (np/procs/ls.h:71-77) */

@@ -34,9 +34,10 @@
  * predicts the stop test met; default 1), $graph 0|1; of ls/bcgs: $nrhs k
  * (bcgs columns).
  *
- * Differences a script sees: the pre- and post-smoother of one lmgc must be
- * the same class with the same $damp (one smoother per hierarchy on the
- * GPU); the base solver must be an ls over ex; vectors are host arrays
+ * The pre- and post-smoother of one lmgc may be different jac / gs objects
+ * with their own $damp (ugmg_params.post_set).  Differences a script sees:
+ * the smoothers must be jac or gs; the base solver must be an ls over ex;
+ * vectors are host arrays
  * registered by name.  npinit returns UGMG_NP_NOT_ACTIVE for a
  * configuration the GPU path cannot run, with the reason in
  * ugmg_np_last_error.

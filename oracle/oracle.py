@@ -35,7 +35,8 @@ class OrcParams(C.Structure):
                 ("damp", C.c_double * MAXB), ("lmgc_damp", C.c_double * MAXB), ("base_damp", C.c_double * MAXB),
                 ("base_abslimit", C.c_double * MAXB), ("base_red", C.c_double * MAXB),
                 ("base_maxiter", C.c_int), ("ex_reorder", C.c_int),
-                ("has_ident", C.c_int), ("ident", C.c_int * MAXB)]
+                ("has_ident", C.c_int), ("ident", C.c_int * MAXB),
+                ("post_set", C.c_int), ("post_smoother", C.c_int), ("post_damp", C.c_double * MAXB)]
 
 
 _lib = None
@@ -135,9 +136,12 @@ def _dv(v):
 
 
 def params(nu1=2, nu2=2, gamma=1, baselevel=0, smoother=JAC, damp=0.8, lmgc_damp=1.0,
-           base_damp=1.0, base_abslimit=1e-300, base_red=1e-12, base_maxiter=1, ex_reorder=1, ident=None):
+           base_damp=1.0, base_abslimit=1e-300, base_red=1e-12, base_maxiter=1, ex_reorder=1, ident=None,
+           post=None, post_damp=None):
     """ident: VD_IDENT per norm entry (sc_cmp's grouping), None: NO_IDENT;
-    damp, lmgc_damp, base_damp: one value or one per component."""
+    damp, lmgc_damp, base_damp: one value or one per component; post,
+    post_damp: lmgc's post-smoother when it is another object than the
+    pre-smoother (None: the same)."""
     p = OrcParams()
     if ident is not None:
         p.has_ident = 1
@@ -152,6 +156,12 @@ def params(nu1=2, nu2=2, gamma=1, baselevel=0, smoother=JAC, damp=0.8, lmgc_damp
         p.base_abslimit[i] = base_abslimit
         p.base_red[i] = base_red
     p.base_maxiter, p.ex_reorder = base_maxiter, ex_reorder
+    if post is not None or post_damp is not None:
+        p.post_set = 1
+        p.post_smoother = smoother if post is None else post
+        pd = _dv(damp if post_damp is None else post_damp)
+        for i in range(MAXB):
+            p.post_damp[i] = pd[i]
     return p
 
 

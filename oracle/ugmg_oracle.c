@@ -688,7 +688,8 @@ static int lmgc(orc_mg *mg, int level) {
   }
   double *t = mg->t[level];
   for (int i = 0; i < p->nu1; ++i) {
-    if (orc_smoother(L, p->smoother, mg->rank[level], p->damp, t, mg->b[level]))
+    if (orc_smoother(L, p->smoother, p->smoother == ORC_GS ? NULL : mg->rank[level], p->damp, t,
+                     mg->b[level]))
       return 1;
     dadd(len, mg->c[level], t);
   }
@@ -701,8 +702,11 @@ static int lmgc(orc_mg *mg, int level) {
   orc_interpolate(L, &mg->lv[level - 1], t, mg->c[level - 1], p->lmgc_damp);
   dadd(len, mg->c[level], t);
   orc_dmatmul_minus(L, mg->b[level], t);
+  /* $S pre post base: the post object smooths after the correction */
+  const int post = p->post_set ? p->post_smoother : p->smoother;
+  const double *pdamp = p->post_set ? p->post_damp : p->damp;
   for (int i = 0; i < p->nu2; ++i) {
-    if (orc_smoother(L, p->smoother, mg->rank[level], p->damp, t, mg->b[level]))
+    if (orc_smoother(L, post, post == ORC_GS ? NULL : mg->rank[level], pdamp, t, mg->b[level]))
       return 1;
     dadd(len, mg->c[level], t);
   }
@@ -724,7 +728,7 @@ static int mg_setup(orc_mg *mg, const orc_level *levels, int top,
     mg->c[l] = (double *)calloc(len + 1, sizeof(double));
     mg->b[l] = (double *)calloc(len + 1, sizeof(double));
     mg->t[l] = (double *)calloc(len + 1, sizeof(double));
-    if (p->smoother == ORC_MCGS && l > p->baselevel) {
+    if ((p->smoother == ORC_MCGS || (p->post_set && p->post_smoother == ORC_MCGS)) && l > p->baselevel) {
       int *color = (int *)malloc(sizeof(int) * (size_t)levels[l].n);
       orc_greedy_color(&levels[l], color);
       mg->rank[l] = (int *)malloc(sizeof(int) * (size_t)levels[l].n);

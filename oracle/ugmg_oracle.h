@@ -69,6 +69,9 @@ typedef struct {
   int ex_reorder;                /* ex $o                                  */
   int has_ident;                 /* VD_NID != NO_IDENT                     */
   int ident[ORC_MAXB];           /* VD_IDENT per norm entry (sc_cmp)       */
+  int post_set;                  /* 0: post-smoother = pre-smoother        */
+  int post_smoother;             /* the post-smoother's kind and $damp     */
+  double post_damp[ORC_MAXB];
 } orc_params;
 
 /* ugblas (np/algebra/ugblas.cc) on one level, ALL_VECTORS */

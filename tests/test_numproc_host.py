@@ -89,19 +89,33 @@ def test_error_behaviour():
         env.create("q", "cg")                                         # not a class of this path
 
 
-def test_pre_and_post_must_match():
+def test_pre_and_post_smoother_objects():
+    """$S pre post base (LmgcInit iter.cc:7613-7656): the post object's class
+    and $damp are kept when they differ from the pre object's; an ex or
+    lmgc object as a smoother is refused."""
     env = NP.NumProcEnv(None)
-    script(env)
-    env.create("post2", "gs")
-    env.init("post2", "$damp 0.7")
-    env.create("mg2", "lmgc")
-    env.init("mg2", "$S pre post2 base $T transfer")
+    script(env)                               # pre and post: gs, $damp 1.0
     env.vector("x", np.zeros(2))
     env.vector("b", np.zeros(2))
-    env.create("s", "ls")
-    env.init("s", "$x x $b b $A A $m 5 $I mg2 $red 1e-8")
+    for name, cls, damp, want in (("p1", "gs", "1.0", (0, None)), ("p2", "gs", "0.7", (1, ugmg.SMOOTH_GS)),
+                                  ("p3", "jac", "1.0", (1, ugmg.SMOOTH_JAC))):
+        env.create(name, cls)
+        env.init(name, f"$damp {damp}")
+        env.create("mg" + name, "lmgc")
+        env.init("mg" + name, f"$S pre {name} base $T transfer")
+        env.create("s" + name, "ls")
+        env.init("s" + name, f"$x x $b b $A A $m 5 $I mg{name} $red 1e-8")
+        p = env.params("s" + name)
+        assert p.smoother == ugmg.SMOOTH_GS and p.damp[0] == 1.0
+        assert p.post_set == want[0]
+        if want[0]:
+            assert p.post_smoother == want[1] and p.post_damp[0] == float(damp)
+    env.create("mgx", "lmgc")
+    env.init("mgx", "$S pre ex base $T transfer")
+    env.create("sx", "ls")
+    env.init("sx", "$x x $b b $A A $m 5 $I mgx $red 1e-8")
     with pytest.raises(ugmg.UgmgError):
-        env.params("s")
+        env.params("sx")
 
 
 def test_gpu_options():

@@ -30,10 +30,11 @@ def _ops(P, l, kind, damp, H):
     return A, np.eye(b * n) - D[:, None] * np.linalg.solve(M, A)
 
 
-def mg_error(P, top, bl, kind, damp, nu1, nu2, lmgc_damp=1.0, base_damp=1.0):
+def mg_error(P, top, bl, kind, damp, nu1, nu2, lmgc_damp=1.0, base_damp=1.0, post=None, post_damp=None):
     """Error operator of one V-cycle: smoothers with $damp, the coarse
     correction interpolated and scaled by lmgc's $damp (dscalx), the base
-    level solved exactly and scaled by ex's $damp, each per component."""
+    level solved exactly and scaled by ex's $damp, each per component; post,
+    post_damp: the post-smoother's kind and $damp (None: the pre-smoother's)."""
     H = O.Hierarchy(P.levels)
     b = P.levels[0].b
     nbl = P.levels[bl].n
@@ -41,6 +42,7 @@ def mg_error(P, top, bl, kind, damp, nu1, nu2, lmgc_damp=1.0, base_damp=1.0):
     for l in range(bl + 1, top + 1):
         f, c = P.levels[l], P.levels[l - 1]
         A, S = _ops(P, l, kind, damp, H)
+        _, S2 = _ops(P, l, post or kind, damp if post_damp is None else post_damp, H)
         Ac = c.dense()
         Pb = np.kron(f.dense_p(c.n), np.eye(b))
         fmask = ((f.skip[:, None] >> np.arange(b)) & 1).ravel() == 0
@@ -48,7 +50,7 @@ def mg_error(P, top, bl, kind, damp, nu1, nu2, lmgc_damp=1.0, base_damp=1.0):
         Dl = np.tile(np.broadcast_to(np.asarray(lmgc_damp, float), (b,)), f.n)
         CG = np.eye(A.shape[0]) - (Dl * fmask)[:, None] * Pb @ (np.eye(Ac.shape[0]) - E[l - 1]) @ \
             np.linalg.solve(Ac, (Pb * cmask[None, :]).T @ A)
-        E[l] = np.linalg.matrix_power(S, nu2) @ CG @ np.linalg.matrix_power(S, nu1)
+        E[l] = np.linalg.matrix_power(S2, nu2) @ CG @ np.linalg.matrix_power(S, nu1)
     return E[top]
 
 
@@ -118,6 +120,29 @@ def test_block_lmgc_cycle_with_damping_per_component(b, kind):
     c1, _ = O.lmgc_iter(H, top, q, rhs)
     c2, _ = O.lmgc_iter(H, top, O.params(smoother=q.smoother, damp=0.6, baselevel=bl), rhs)
     assert np.array_equal(c1, c2)
+
+
+@pytest.mark.parametrize("b", [1, 2])
+@pytest.mark.parametrize("pre,post", [("jac", "mcgs"), ("mcgs", "jac"), ("mcgs", "mcgs")])
+def test_lmgc_cycle_with_distinct_pre_and_post_smoothers(b, pre, post):
+    """Lmgc's $S pre post base with two smoother objects (PreSmooth nu1
+    times, np/procs/iter.cc:7812; PostSmooth nu2 times, iter.cc:7922): E = S_post^nu2 CG S_pre^nu1
+    with each smoother's class and $damp."""
+    P = problems.p1_local(3, 0, b)
+    H = O.Hierarchy(P.levels)
+    top, bl = P.top, 1
+    dpre, dpost = [0.6, 0.7][:b], [0.9, 0.8][:b]
+    E = mg_error(P, top, bl, pre, dpre, 2, 1, post=post, post_damp=dpost)
+    A = P.levels[top].dense()
+    n2 = A.shape[0]
+    rhs = np.random.default_rng(6).uniform(-1, 1, n2)
+    rhs[np.repeat(P.levels[top].skip != 0, b)] = 0.0
+    kind = {"jac": O.JAC, "mcgs": O.MCGS}
+    p = O.params(smoother=kind[pre], damp=dpre, nu1=2, nu2=1, baselevel=bl, post=kind[post], post_damp=dpost)
+    c, bn = O.lmgc_iter(H, top, p, rhs)
+    want = (np.eye(n2) - E) @ np.linalg.solve(A, rhs)
+    assert np.max(np.abs(c - want)) < 1e-11 * np.max(np.abs(want))
+    assert np.max(np.abs(bn - (rhs - A @ c))) < 1e-13 * np.max(np.abs(rhs))
 
 
 def test_block4_jacobi_step_with_row_exchange_is_dense_block_solve():

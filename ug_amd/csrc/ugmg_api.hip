@@ -498,6 +498,26 @@ void to_user(ugmg_ctx *ctx, int l, const TS *src, TD *dst, size_t ldd, int nrhs 
 
 // ------------------------------------------------------------ cycle drivers
 
+// lmgc's pre- (post = false) and post-smoother: class and $damp
+inline int sm_kind(const ugmg_ctx *ctx, bool post) {
+  return post && ctx->p.post_set ? ctx->p.post_smoother : ctx->p.smoother;
+}
+inline const double *sm_damp(const ugmg_ctx *ctx, bool post) {
+  return post && ctx->p.post_set ? ctx->p.post_damp : ctx->p.damp;
+}
+// a GS layout (colours, device order) when either smoother is GS
+inline bool uses_gs(const ugmg_params &p) {
+  return p.smoother == UGMG_SMOOTH_GS || (p.post_set && p.post_smoother == UGMG_SMOOTH_GS);
+}
+// pre- and post-smoother differ in class or damping
+inline bool split_smoothers(const ugmg_params &p) {
+  if (!p.post_set) return false;
+  if (p.post_smoother != p.smoother) return true;
+  for (int i = 0; i < UGMG_MAXB; ++i)
+    if (p.post_damp[i] != p.damp[i]) return true;
+  return false;
+}
+
 // damping factors that differ between the components of a block
 inline bool per_comp(const ugmg_ctx *ctx, const double *d) {
   for (int i = 1; i < ctx->b; ++i)
@@ -512,16 +532,19 @@ inline bool per_comp_entries(const ugmg_ctx *ctx, const double *d) {
 }
 
 template <typename T>
-void smooth_step(ugmg_ctx *ctx, int l, T *t, T *b, T *c, int cmode, T *xout, int k0 = 0) {
+void smooth_step(ugmg_ctx *ctx, int l, T *t, T *b, T *c, int cmode, T *xout, int k0 = 0,
+                 bool post = false) {
   DevLevel &d = ctx->dev[l];
   hipStream_t st = ctx->stream;
-  const T damp = (T)ctx->p.damp[0];
+  const int kind = sm_kind(ctx, post);
+  const double *dv = sm_damp(ctx, post);
+  const T damp = (T)dv[0];
   const Batch bt = batch(ctx, l);
   const int mode = cmode < 0 ? 0 : (cmode == 0 ? 3 : 2);
-  if (per_comp(ctx, ctx->p.damp)) {
+  if (per_comp(ctx, dv)) {
     // Smoother (np/procs/iter.cc:817-842, dscalx at 836) with $damp per component: the step's x,
     // x *= damp per component, then b -= A x (and c (+)= x)
-    if (ctx->p.smoother == UGMG_SMOOTH_JAC) {
+    if (kind == UGMG_SMOOTH_JAC) {
       launch_jac<T>(st, d.n, b, d.diag<T>(), d.vclass, (T)1, t, bt);
     } else {
       for (int k = k0; k < d.ncolors; ++k)
@@ -529,7 +552,7 @@ void smooth_step(ugmg_ctx *ctx, int l, T *t, T *b, T *c, int cmode, T *xout, int
                               d.color_slices[k + 1] - d.color_slices[k], d.color_wmax[k], b,
                               d.diag<T>(), d.vclass, t, d.ntv<T>(), bt);
     }
-    launch_scale_comp<T>(st, d.n, d.nb, ctx->p.damp, t, (T *)nullptr, 0, bt);
+    launch_scale_comp<T>(st, d.n, d.nb, dv, t, (T *)nullptr, 0, bt);
     launch_spmv<T, T, T>(st, d.A.d<T>(), 0, d.A.nslices, t, b, c, mode, d.ntv<T>(), bt);
     if (xout)   // an error here is left for the caller's hipGetLastError check
       for (int r = 0; r < bt.nrhs; ++r)
@@ -537,7 +560,7 @@ void smooth_step(ugmg_ctx *ctx, int l, T *t, T *b, T *c, int cmode, T *xout, int
                                 hipMemcpyDeviceToDevice, st));
     return;
   }
-  if (ctx->p.smoother == UGMG_SMOOTH_JAC) {
+  if (kind == UGMG_SMOOTH_JAC) {
     launch_jac<T>(st, d.n, b, d.diag<T>(), d.vclass, damp, t, bt);
     launch_spmv<T, T, T>(st, d.A.d<T>(), 0, d.A.nslices, t, b, c, mode, d.ntv<T>(), bt);
   } else {
@@ -555,16 +578,17 @@ void smooth_step(ugmg_ctx *ctx, int l, T *t, T *b, T *c, int cmode, T *xout, int
 // folded into the next step's colour sweeps.  k0 = 1: the first step's
 // colour-0 sweep has already written t (by the kernel that produced b).
 template <typename T>
-void smooth(ugmg_ctx *ctx, int l, T *t, T *b, T *c, int cmode, int nsteps, int k0 = 0) {
+void smooth(ugmg_ctx *ctx, int l, T *t, T *b, T *c, int cmode, int nsteps, int k0 = 0,
+            bool post = false) {
   DevLevel &d = ctx->dev[l];
-  if (ctx->p.smoother != UGMG_SMOOTH_GS || !d.all_active || nsteps < 2 || !ctx->fuse_steps ||
-      per_comp(ctx, ctx->p.damp)) {
+  if (sm_kind(ctx, post) != UGMG_SMOOTH_GS || !d.all_active || nsteps < 2 || !ctx->fuse_steps ||
+      per_comp(ctx, sm_damp(ctx, post))) {
     for (int i = 0; i < nsteps; ++i)
-      smooth_step<T>(ctx, l, t, b, c, i == 0 ? cmode : 1, nullptr, i == 0 ? k0 : 0);
+      smooth_step<T>(ctx, l, t, b, c, i == 0 ? cmode : 1, nullptr, i == 0 ? k0 : 0, post);
     return;
   }
   hipStream_t st = ctx->stream;
-  const T damp = (T)ctx->p.damp[0];
+  const T damp = (T)sm_damp(ctx, post)[0];
   const Batch bt = batch(ctx, l);
   for (int k = k0; k < d.ncolors; ++k)
     launch_gs_color<T, T>(st, d.L.d<T>(), d.color_slices[k],
@@ -582,9 +606,9 @@ void smooth(ugmg_ctx *ctx, int l, T *t, T *b, T *c, int cmode, int nsteps, int k
 // A GS level whose rows are all active (scalar): the colour-0 sweep of a
 // smoother step (no lower neighbours: v_r = b_r / a_rr) can run inside the
 // kernel that produces b, bit-identical to the separate sweep
-bool fold_colour0(const ugmg_ctx *ctx, int l, int nsteps) {
+bool fold_colour0(const ugmg_ctx *ctx, int l, int nsteps, bool post = false) {
   const DevLevel &d = ctx->dev[l];
-  return ctx->p.smoother == UGMG_SMOOTH_GS && d.all_active && d.nb == 1 && ctx->fuse_steps &&
+  return sm_kind(ctx, post) == UGMG_SMOOTH_GS && d.all_active && d.nb == 1 && ctx->fuse_steps &&
          nsteps >= 1 && d.ncolors > 0 && d.c0;
 }
 
@@ -598,7 +622,7 @@ void post_smooth_folded(ugmg_ctx *ctx, int l, const T *t, T *b, T *c, int nsteps
   T *u = d.uv<T>();
   launch_spmv_c0<T, T>(ctx->stream, d.A.d<T>(), t, b, d.diag<T>(), u, d.color_slices[1],
                        d.ntv<T>(), batch(ctx, l));
-  smooth<T>(ctx, l, u, b, c, 1, nsteps, 1);
+  smooth<T>(ctx, l, u, b, c, 1, nsteps, 1, true);
 }
 
 BaseArgs base_args(ugmg_ctx *ctx) {
@@ -738,13 +762,13 @@ void lmgc(ugmg_ctx *ctx, int l, T *c, T *b, int cmode, bool c0_done = false) {
     launch_interp<T>(st, d.P.d<T>(), d.iskip, dc.cv<T>(), (T)p.lmgc_damp[0], t, c, d.ntv<T>(), nb, d.vs,
                      dc.vs, d.nb);
   }
-  if (fold_colour0(ctx, l, p.nu2)) {
+  if (fold_colour0(ctx, l, p.nu2, true)) {
     post_smooth_folded<T>(ctx, l, t, b, c, p.nu2);
     return;
   }
   launch_spmv<T, T, T>(st, d.A.d<T>(), 0, d.A.nslices, t, b, (T *)nullptr, 0, d.ntv<T>(),
                        batch(ctx, l));
-  smooth<T>(ctx, l, t, b, c, 1, p.nu2);
+  smooth<T>(ctx, l, t, b, c, 1, p.nu2, 0, true);
 }
 
 int check_ready(ugmg_ctx *ctx, int level) {
@@ -913,15 +937,15 @@ double gs_step_bytes(const ugmg_ctx *ctx, int l, int sv, bool fused) {
 }
 
 // nsteps smoother steps as smooth() issues them
-double smooth_bytes(const ugmg_ctx *ctx, int l, int sv, int cmode, int nsteps, int k0) {
+double smooth_bytes(const ugmg_ctx *ctx, int l, int sv, int cmode, int nsteps, int k0, bool post = false) {
   const DevLevel &d = ctx->dev[l];
   double by = 0.0;
-  if (ctx->p.smoother == UGMG_SMOOTH_JAC) {
+  if (sm_kind(ctx, post) == UGMG_SMOOTH_JAC) {
     for (int i = 0; i < nsteps; ++i)
       by += op_bytes(ctx, l, Op::Jac, sv) + d.A.bytes(sv) + vbytes(d, sv) * ((i == 0 && cmode == 0) ? 4.0 : 5.0);
     return by;
   }
-  if (!d.all_active || nsteps < 2 || !ctx->fuse_steps) {
+  if (!d.all_active || nsteps < 2 || !ctx->fuse_steps || per_comp(ctx, sm_damp(ctx, post))) {
     for (int i = 0; i < nsteps; ++i)
       by += sweep_bytes(d, sv, i == 0 ? k0 : 0) + defect_bytes(d, sv, (i == 0 ? cmode : 1) == 1);
     return by;
@@ -970,8 +994,9 @@ double lmgc_bytes(const ugmg_ctx *ctx, int l, int sv, int cmode, bool c0_done) {
     by += to_base ? base_bytes(ctx, sv, true, g + 1 < p.gamma)
                   : lmgc_bytes(ctx, l - 1, sv, g == 0 ? 0 : 1, g == 0 && pre0);
   by += op_bytes(ctx, l, Op::Interp, sv);
-  if (fold_colour0(ctx, l, p.nu2)) return by + op_bytes(ctx, l, Op::SpmvC0, sv) + smooth_bytes(ctx, l, sv, 1, p.nu2, 1);
-  return by + op_bytes(ctx, l, Op::Spmv, sv) + smooth_bytes(ctx, l, sv, 1, p.nu2, 0);
+  if (fold_colour0(ctx, l, p.nu2, true))
+    return by + op_bytes(ctx, l, Op::SpmvC0, sv) + smooth_bytes(ctx, l, sv, 1, p.nu2, 1, true);
+  return by + op_bytes(ctx, l, Op::Spmv, sv) + smooth_bytes(ctx, l, sv, 1, p.nu2, 0, true);
 }
 
 double cycle_bytes(const ugmg_ctx *ctx, int l, int sv) { return lmgc_bytes(ctx, l, sv, 0, false); }
@@ -1281,7 +1306,7 @@ int prep_host(ugmg_ctx *ctx, int level, PrepState &ps) {
     }
     auto src = [&](int l) -> const HostLevel & { return idx.empty() ? ctx->host[l] : idx[l]; };
     for (int l = bl; l <= level && !ps.device; ++l) {
-      const bool gs = p.smoother == UGMG_SMOOTH_GS && l > bl;
+      const bool gs = uses_gs(p) && l > bl;
       HostPrep &hp = ps.hp[l];
       const bool cperm = l > bl && !pos[l - 1].empty();
       const HostLevel *coarse = l > bl ? (cperm ? &plv[l - 1] : &src(l - 1)) : nullptr;
@@ -1382,7 +1407,7 @@ int build_layouts_device(ugmg_ctx *ctx, PrepState &ps) {
   };
   int rc = 0;
   for (int l = bl; l <= level && !rc; ++l) {
-    const bool gs = ctx->p.smoother == UGMG_SMOOTH_GS && l > bl;
+    const bool gs = uses_gs(ctx->p) && l > bl;
     const int coarse_n = l > bl ? ctx->host[l - 1].n : 0;
     const int *coarse_pos = l > bl ? ps.dp[l - 1].pos : nullptr;
     std::string err;
@@ -1668,7 +1693,8 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
       for (int l = bl; l <= level; ++l) mx = std::max(mx, ctx->host[l].a.size());
       if ((rc = dalloc(ctx, &ctx->stage, mx))) return rc;
     }
-    if (ps.fuse_rows > 0 && ctx->dev[bl].n <= 2048 && ctx->b == 1 && !ctx->grouped && ps.ntypes == 1) {
+    if (ps.fuse_rows > 0 && ctx->dev[bl].n <= 2048 && ctx->b == 1 && !ctx->grouped && ps.ntypes == 1 &&
+        !split_smoothers(ctx->p)) {
       ctx->fuse_top = bl;
       for (int l = bl + 1; l <= level; ++l) {
         if (ctx->dev[l].n > ps.fuse_rows || ctx->dev[l].ncolors > kMaxColors) break;
@@ -1752,6 +1778,9 @@ void ugmg_params_default(ugmg_params *p) {
   p->ex_float = 0;
   p->inner_adapt = 1;
   p->base_inv32 = 0;
+  p->post_set = 0;
+  p->post_smoother = UGMG_SMOOTH_JAC;
+  for (int i = 0; i < UGMG_MAXB; ++i) p->post_damp[i] = 1.0;
 }
 
 int ugmg_create(int device, ugmg_ctx **out) {
@@ -2075,6 +2104,7 @@ int ugmg_set_params(ugmg_ctx *ctx, const ugmg_params *p) {
   if (!ctx || !p) return UGMG_E_ARG;
   if (p->nu1 < 0 || p->nu2 < 0 || p->gamma < 0 || p->base_maxiter < 0 ||
       (p->smoother != UGMG_SMOOTH_JAC && p->smoother != UGMG_SMOOTH_GS) ||
+      (p->post_set && p->post_smoother != UGMG_SMOOTH_JAC && p->post_smoother != UGMG_SMOOTH_GS) ||
       (p->precision != UGMG_FP64 && p->precision != UGMG_MIXED) || p->inner_cycles < 0)
     return fail(ctx, UGMG_E_ARG, "invalid parameter");
   // the damping factors may differ per component ($damp, lmgc $damp, ex
@@ -2084,6 +2114,7 @@ int ugmg_set_params(ugmg_ctx *ctx, const ugmg_params *p) {
       if (p->base_abslimit[i] != p->base_abslimit[0] || p->base_red[i] != p->base_red[0])
         return fail(ctx, UGMG_E_ARG, "block hierarchies take one base $abslimit / $red for all components");
   const bool relayout = !ctx->params_set || p->smoother != ctx->p.smoother ||
+                        p->post_set != ctx->p.post_set || p->post_smoother != ctx->p.post_smoother ||
                         p->baselevel != ctx->p.baselevel || p->ex_reorder != ctx->p.ex_reorder ||
                         p->ex_float != ctx->p.ex_float;
   ctx->p = *p;
@@ -3546,7 +3577,7 @@ bool same_shape(const ugmg_ctx *ctx, int nlevels, const ugmg_level_desc *lv) {
 
 double est_device_bytes(const ugmg_ctx *ctx, int top, int bl) {
   double by = 0.0;
-  const bool gs = ctx->p.smoother == UGMG_SMOOTH_GS;
+  const bool gs = uses_gs(ctx->p);
   for (int l = bl; l <= top; ++l) {
     const HostLevel &H = ctx->host[l];
     by += (double)H.nnzb * 16.0 * 1.1;                      // A: col + FP64 + FP32, padding

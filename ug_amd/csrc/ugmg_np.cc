@@ -458,16 +458,15 @@ int build_params(ugmg_np_env *e, NumProc &o, ugmg_params &p, bool &use_iter) {
     return fail(e, UGMG_E_STATE, "$I must be an initialised lmgc");
   NumProc *pr = find(e, mg->pre), *po = find(e, mg->post), *ba = find(e, mg->base);
   if (!pr || !po || !ba) return fail(e, UGMG_E_STATE, "lmgc parts missing");
-  if (pr->cls != po->cls || pr->cls == "ex" || pr->cls == "lmgc")
-    return fail(e, UGMG_E_ARG, "pre- and post-smoother must be the same jac or gs class");
+  auto smoother_cls = [](const std::string &c) { return c == "jac" || c == "gs"; };
+  if (!smoother_cls(pr->cls) || !smoother_cls(po->cls))
+    return fail(e, UGMG_E_ARG, "pre- and post-smoother must be jac or gs objects");
   double dpre[UGMG_MAXB], dpost[UGMG_MAXB], dmg[UGMG_MAXB], dex[UGMG_MAXB];
   int rc, count = -1;
   if ((rc = resolve_damp(e, *pr, "pre-smoother", dpre, count)) ||
       (rc = resolve_damp(e, *po, "post-smoother", dpost, count)) ||
       (rc = resolve_damp(e, *mg, "lmgc", dmg, count)))
     return rc;
-  for (int i = 0; i < UGMG_MAXB; ++i)
-    if (dpre[i] != dpost[i]) return fail(e, UGMG_E_ARG, "pre- and post-smoother $damp differ");
   NumProc *ex = find(e, ba->I);
   if (!ex || ex->cls != "ex") return fail(e, UGMG_E_ARG, "the base ls must iterate with ex");
   if ((rc = resolve_damp(e, *ex, "ex", dex, count))) return rc;
@@ -478,6 +477,14 @@ int build_params(ugmg_np_env *e, NumProc &o, ugmg_params &p, bool &use_iter) {
   p.gamma = mg->gamma;
   p.baselevel = mg->bl;
   p.smoother = pr->cls == "jac" ? UGMG_SMOOTH_JAC : UGMG_SMOOTH_GS;
+  // Lmgc runs $S's pre object nu1 times and its post object nu2 times
+  // (np/procs/iter.cc:7812, 7922): a post object of another class or $damp is kept
+  p.post_smoother = po->cls == "jac" ? UGMG_SMOOTH_JAC : UGMG_SMOOTH_GS;
+  p.post_set = p.post_smoother != p.smoother;
+  for (int i = 0; i < UGMG_MAXB; ++i) {
+    p.post_damp[i] = dpost[i];
+    if (dpost[i] != dpre[i]) p.post_set = 1;
+  }
   for (int i = 0; i < UGMG_MAXB; ++i) {
     p.damp[i] = dpre[i];
     p.lmgc_damp[i] = dmg[i];

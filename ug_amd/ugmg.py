@@ -41,7 +41,8 @@ class Params(C.Structure):
                 ("base_abslimit", C.c_double * MAXB), ("base_red", C.c_double * MAXB),
                 ("base_maxiter", C.c_int), ("ex_reorder", C.c_int), ("precision", C.c_int),
                 ("inner_cycles", C.c_int), ("use_graph", C.c_int), ("ex_float", C.c_int),
-                ("inner_adapt", C.c_int), ("base_inv32", C.c_int)]
+                ("inner_adapt", C.c_int), ("base_inv32", C.c_int),
+                ("post_set", C.c_int), ("post_smoother", C.c_int), ("post_damp", C.c_double * MAXB)]
 
 
 class LResult(C.Structure):
@@ -152,7 +153,7 @@ def default_params(**kw) -> Params:
     p = Params()
     load().ugmg_params_default(C.byref(p))
     for k, v in kw.items():
-        if k in ("damp", "lmgc_damp", "base_damp", "base_abslimit", "base_red"):
+        if k in ("damp", "lmgc_damp", "base_damp", "base_abslimit", "base_red", "post_damp"):
             arr = getattr(p, k)
             vals = [v] * MAXB if np.isscalar(v) else list(v)
             for i in range(MAXB):

@@ -49,6 +49,7 @@ enum {
 
 enum { UGMG_SMOOTH_JAC = 0, UGMG_SMOOTH_GS = 1 };   /* GS = multicolour GS  */
 enum { UGMG_FP64 = 0, UGMG_MIXED = 1 };
+enum { UGMG_BASE_EX = 0, UGMG_BASE_JAC = 1 };   /* the base ls's $I            */
 
 /* lmgc / smoother / base-solver settings, named after the numproc options. */
 typedef struct {
@@ -81,6 +82,11 @@ typedef struct {
                                        two different smoother objects)        */
   int post_smoother;                /* UGMG_SMOOTH_JAC / UGMG_SMOOTH_GS       */
   double post_damp[UGMG_MAXB];      /* post-smoother $damp                    */
+  int base_iter;                    /* the base ls's iteration: UGMG_BASE_EX
+                                       (default: ex, the dense inverse) or
+                                       UGMG_BASE_JAC (jac with base_damp as
+                                       its $damp, base_maxiter steps, the
+                                       base stop test after each)            */
 } ugmg_params;
 
 /* LRESULT (np/procs/ls.h:71-77) */

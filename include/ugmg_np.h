@@ -36,7 +36,8 @@
  *
  * The pre- and post-smoother of one lmgc may be different jac / gs objects
  * with their own $damp (ugmg_params.post_set).  Differences a script sees:
- * the smoothers must be jac or gs; the base solver must be an ls over ex;
+ * the smoothers must be jac or gs; the base solver must be an ls over ex
+ * or jac (ugmg_params.base_iter);
  * vectors are host arrays
  * registered by name.  npinit returns UGMG_NP_NOT_ACTIVE for a
  * configuration the GPU path cannot run, with the reason in

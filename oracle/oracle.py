@@ -36,7 +36,8 @@ class OrcParams(C.Structure):
                 ("base_abslimit", C.c_double * MAXB), ("base_red", C.c_double * MAXB),
                 ("base_maxiter", C.c_int), ("ex_reorder", C.c_int),
                 ("has_ident", C.c_int), ("ident", C.c_int * MAXB),
-                ("post_set", C.c_int), ("post_smoother", C.c_int), ("post_damp", C.c_double * MAXB)]
+                ("post_set", C.c_int), ("post_smoother", C.c_int), ("post_damp", C.c_double * MAXB),
+                ("base_iter", C.c_int)]
 
 
 _lib = None
@@ -137,11 +138,12 @@ def _dv(v):
 
 def params(nu1=2, nu2=2, gamma=1, baselevel=0, smoother=JAC, damp=0.8, lmgc_damp=1.0,
            base_damp=1.0, base_abslimit=1e-300, base_red=1e-12, base_maxiter=1, ex_reorder=1, ident=None,
-           post=None, post_damp=None):
+           post=None, post_damp=None, base_iter=0):
     """ident: VD_IDENT per norm entry (sc_cmp's grouping), None: NO_IDENT;
     damp, lmgc_damp, base_damp: one value or one per component; post,
     post_damp: lmgc's post-smoother when it is another object than the
-    pre-smoother (None: the same)."""
+    pre-smoother (None: the same); base_iter: the base ls's iteration, 0 ex,
+    1 jac with base_damp."""
     p = OrcParams()
     if ident is not None:
         p.has_ident = 1
@@ -155,7 +157,7 @@ def params(nu1=2, nu2=2, gamma=1, baselevel=0, smoother=JAC, damp=0.8, lmgc_damp
         p.base_damp[i] = db[i]
         p.base_abslimit[i] = base_abslimit
         p.base_red[i] = base_red
-    p.base_maxiter, p.ex_reorder = base_maxiter, ex_reorder
+    p.base_maxiter, p.ex_reorder, p.base_iter = base_maxiter, ex_reorder, base_iter
     if post is not None or post_damp is not None:
         p.post_set = 1
         p.post_smoother = smoother if post is None else post

@@ -665,8 +665,11 @@ static int base_solver(orc_mg *mg, int level, const double *first) {
   for (int i = 0; i < p->base_maxiter; ++i) {
     if (converged) break;
     memset(mg->cbase, 0, sizeof(double) * (size_t)L->n * b);
-    if (orc_ex_smoother(mg->ex, L, p->base_damp, mg->cbase, mg->b[level]))
+    if (p->base_iter == 1) { /* ls over jac: Smoother with l_jac */
+      if (orc_smoother(L, ORC_JAC, NULL, p->base_damp, mg->cbase, mg->b[level])) return 1;
+    } else if (orc_ex_smoother(mg->ex, L, p->base_damp, mg->cbase, mg->b[level])) {
       return 1;
+    }
     dadd((size_t)L->n * b, mg->c[level], mg->cbase); /* LSUpdate */
     surface_nrm2x(L, mg->b[level], last);             /* LinearResiduum */
     if (sc_cmp(nc, last, p->base_abslimit, SC_IDENT(p)) || sc_cmp(nc, last, defect2reach, SC_IDENT(p))) {

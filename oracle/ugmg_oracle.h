@@ -72,6 +72,8 @@ typedef struct {
   int post_set;                  /* 0: post-smoother = pre-smoother        */
   int post_smoother;             /* the post-smoother's kind and $damp     */
   double post_damp[ORC_MAXB];
+  int base_iter;                 /* the base ls's $I: 0 ex, 1 jac ($damp =
+                                    base_damp)                             */
 } orc_params;
 
 /* ugblas (np/algebra/ugblas.cc) on one level, ALL_VECTORS */

@@ -203,3 +203,34 @@ def test_typed_damp_per_component():
     assert env.init("bad", "$damp 0.5 0.6") == NP.NOT_ACTIVE        # several values, no type
     assert env.init("bad", "$damp n0.5:x") == NP.NOT_ACTIVE         # unreadable value
     assert env.init("bad", "$damp 0.75") == NP.ACTIVE
+
+
+def test_base_ls_over_jac():
+    """The base ls may iterate with a jac object (LinearSolver over Smoother
+    on the base level): base_iter BASE_JAC, the jac object's $damp as the
+    base damping; a gs object there is refused."""
+    env = NP.NumProcEnv(None)
+    script(env)
+    env.create("bj", "jac")
+    env.init("bj", "$damp 0.65")
+    env.create("bbase", "ls")
+    env.init("bbase", "$red 1e-3 $m 25 $I bj")
+    env.create("mgj", "lmgc")
+    env.init("mgj", "$S pre post bbase $T transfer")
+    env.vector("x", np.zeros(2))
+    env.vector("b", np.zeros(2))
+    env.create("s", "ls")
+    env.init("s", "$x x $b b $A A $m 5 $I mgj $red 1e-8")
+    p = env.params("s")
+    assert p.base_iter == ugmg.BASE_JAC and p.base_damp[0] == 0.65
+    assert p.base_maxiter == 25 and p.base_red[0] == 1e-3
+    env.create("bg", "gs")
+    env.init("bg", "$damp 1.0")
+    env.create("gbase", "ls")
+    env.init("gbase", "$red 1e-3 $m 25 $I bg")
+    env.create("mgg", "lmgc")
+    env.init("mgg", "$S pre post gbase $T transfer")
+    env.create("s2", "ls")
+    env.init("s2", "$x x $b b $A A $m 5 $I mgg $red 1e-8")
+    with pytest.raises(ugmg.UgmgError):
+        env.params("s2")

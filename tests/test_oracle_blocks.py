@@ -30,15 +30,20 @@ def _ops(P, l, kind, damp, H):
     return A, np.eye(b * n) - D[:, None] * np.linalg.solve(M, A)
 
 
-def mg_error(P, top, bl, kind, damp, nu1, nu2, lmgc_damp=1.0, base_damp=1.0, post=None, post_damp=None):
+def mg_error(P, top, bl, kind, damp, nu1, nu2, lmgc_damp=1.0, base_damp=1.0, post=None, post_damp=None,
+             base_jac=0):
     """Error operator of one V-cycle: smoothers with $damp, the coarse
     correction interpolated and scaled by lmgc's $damp (dscalx), the base
     level solved exactly and scaled by ex's $damp, each per component; post,
-    post_damp: the post-smoother's kind and $damp (None: the pre-smoother's)."""
+    post_damp: the post-smoother's kind and $damp (None: the pre-smoother's);
+    base_jac = m > 0: the base ls iterates m Jacobi steps with ex's $damp
+    (no stop test met) instead of the exact solve."""
     H = O.Hierarchy(P.levels)
     b = P.levels[0].b
     nbl = P.levels[bl].n
     E = {bl: np.eye(b * nbl) - np.diag(np.tile(np.broadcast_to(np.asarray(base_damp, float), (b,)), nbl))}
+    if base_jac:
+        E[bl] = np.linalg.matrix_power(_ops(P, bl, "jac", base_damp, H)[1], base_jac)
     for l in range(bl + 1, top + 1):
         f, c = P.levels[l], P.levels[l - 1]
         A, S = _ops(P, l, kind, damp, H)
@@ -143,6 +148,43 @@ def test_lmgc_cycle_with_distinct_pre_and_post_smoothers(b, pre, post):
     want = (np.eye(n2) - E) @ np.linalg.solve(A, rhs)
     assert np.max(np.abs(c - want)) < 1e-11 * np.max(np.abs(want))
     assert np.max(np.abs(bn - (rhs - A @ c))) < 1e-13 * np.max(np.abs(rhs))
+
+
+@pytest.mark.parametrize("b", [1, 2])
+def test_lmgc_cycle_with_jacobi_base_solver(b):
+    """The base ls over jac (LinearSolver np/procs/ls.cc:637-749 with
+    Smoother iter.cc:817-842 on the base level): m steps with no stop test met give
+    E_base = S_jac^m; a base $abslimit met at once leaves the coarse
+    correction zero (E_base = I); a $red met after k steps equals m = k."""
+    P = problems.p1_local(3, 0, b)
+    H = O.Hierarchy(P.levels)
+    top, bl = P.top, 1
+    db = [0.8, 0.7][:b]
+    A = P.levels[top].dense()
+    n2 = A.shape[0]
+    rhs = np.random.default_rng(8).uniform(-1, 1, n2)
+    rhs[np.repeat(P.levels[top].skip != 0, b)] = 0.0
+    for m in (1, 4):
+        E = mg_error(P, top, bl, "jac", 0.6, 2, 2, base_damp=db, base_jac=m)
+        p = O.params(smoother=O.JAC, damp=0.6, base_damp=db, baselevel=bl, base_maxiter=m, base_iter=1)
+        c, _ = O.lmgc_iter(H, top, p, rhs)
+        want = (np.eye(n2) - E) @ np.linalg.solve(A, rhs)
+        assert np.max(np.abs(c - want)) < 1e-11 * np.max(np.abs(want))
+    E = mg_error(P, top, bl, "jac", 0.6, 2, 2, base_jac=0)
+    E_none = mg_error(P, top, bl, "jac", 0.6, 2, 2, base_damp=0.0)    # coarse correction 0
+    c, _ = O.lmgc_iter(H, top, O.params(smoother=O.JAC, damp=0.6, baselevel=bl, base_maxiter=4, base_iter=1,
+                                         base_abslimit=1e300), rhs)
+    want = (np.eye(n2) - E_none) @ np.linalg.solve(A, rhs)
+    assert np.max(np.abs(c - want)) < 1e-11 * np.max(np.abs(want))
+    # the base defect after k Jacobi steps, computed directly: the first k
+    # whose norm (every component) is below 0.5 of the first stops the loop
+    pk = O.params(smoother=O.JAC, damp=0.6, baselevel=bl, base_maxiter=40, base_iter=1, base_red=0.5)
+    c40, _ = O.lmgc_iter(H, top, pk, rhs)
+    outs = [O.lmgc_iter(H, top, O.params(smoother=O.JAC, damp=0.6, baselevel=bl, base_maxiter=k, base_iter=1,
+                                         base_red=1e-300), rhs)[0] for k in range(1, 41)]
+    hits = [k for k in range(1, 41) if np.array_equal(outs[k - 1], c40)]
+    assert hits and hits[0] < 40
+    assert not np.array_equal(outs[hits[0] - 2] if hits[0] > 1 else np.zeros_like(c40), c40)
 
 
 def test_block4_jacobi_step_with_row_exchange_is_dense_block_solve():

@@ -518,6 +518,12 @@ inline bool split_smoothers(const ugmg_params &p) {
   return false;
 }
 
+// the base solve cannot run: ex's band LU met a zero pivot (ls over jac
+// does not use it)
+inline bool base_singular(const ugmg_ctx *ctx) {
+  return ctx->base.failed && ctx->p.base_iter == UGMG_BASE_EX;
+}
+
 // damping factors that differ between the components of a block
 inline bool per_comp(const ugmg_ctx *ctx, const double *d) {
   for (int i = 1; i < ctx->b; ++i)
@@ -681,6 +687,22 @@ void base_solve(ugmg_ctx *ctx, T *c, T *b, int cmode, bool have_parts, bool need
   T *x = d.tv<T>();
   const int m = std::max(0, p.base_maxiter);
   if (m == 0 && cmode == 0) launch_fill<T>(st, d.n, c, (T)0, bt);
+  if (p.base_iter == UGMG_BASE_JAC) {
+    // ls over jac (LinearSolver np/procs/ls.cc:637-749 with Smoother iter.cc:817-842):
+    // x = damp D^-1 b; x = 0 once the defect before it meets the stop test
+    // (every later step is then a no-op, as the loop's break); c (+)= x;
+    // b -= A x; the next defect's partial norms
+    const bool pc = per_comp(ctx, p.base_damp);
+    for (int it = 0; it < m; ++it) {
+      launch_jac<T>(st, d.n, b, d.diag<T>(), d.vclass, (T)(pc ? 1.0 : p.base_damp[0]), x, bt);
+      if (pc) launch_scale_comp<T>(st, d.n, d.nb, p.base_damp, x, (T *)nullptr, 0, bt);
+      launch_base_gate<T>(st, d.n * d.nb, x, c, it == 0 ? cmode : 1, p.base_abslimit[0], p.base_red[0],
+                          ctx->base_part, np, it, ctx->base_first, bt);
+      launch_spmv<T, T, T>(st, d.A.d<T>(), 0, d.A.nslices, x, b, (T *)nullptr, 0, d.ntv<T>(), bt);
+      if (it + 1 < m) np = launch_dot_part<T>(st, d.n, b, d.vclass, ctx->base_part, bt);
+    }
+    return;
+  }
   // the FP32 cycle reads the FP64 inverse (sums in FP64) unless base_inv32
   // asks for its FP32 copy
   const bool inv64 = std::is_same_v<T, double> || !p.base_inv32;
@@ -734,7 +756,8 @@ void lmgc(ugmg_ctx *ctx, int l, T *c, T *b, int cmode, bool c0_done = false) {
   // the base level's spare vector u, for the first base iteration.  With
   // several vector types the base solve forms its own per-type norms.
   const bool typed = dc.vt != nullptr;
-  const bool bcol = to_base && nb == 1 && ctx->base_cpos && !ctx->base.failed && !typed;
+  const bool bcol = to_base && nb == 1 && ctx->base_cpos && !ctx->base.failed && !typed &&
+                    p.base_iter == UGMG_BASE_EX;
   const int np = launch_restrict<T>(st, d.PT.d<T>(), dc.skip, dc.vnclass, dc.vclass, b, (T)1,
                                     dc.bv<T>(), to_base && !typed ? ctx->base_part : nullptr, d.ntv<T>(), nb,
                                     d.vs, dc.vs, d.nb, pre0 ? dc.c0 : nullptr, dc.diag<T>(),
@@ -964,6 +987,12 @@ double base_bytes(const ugmg_ctx *ctx, int sv, bool restricted, bool need_b) {
   const int si = (sv == 4 && ctx->p.base_inv32) ? 4 : 8;
   const double ncols = restricted ? ctx->base_ncyc : nsc;
   double by = 0.0;
+  if (ctx->p.base_iter == UGMG_BASE_JAC) {   // step, gate (x, c), b -= A x, norms
+    for (int it = 0; it < m; ++it)
+      by += op_bytes(ctx, ctx->bl, Op::Jac, sv) + vbytes(d, sv) * 3.0 + op_bytes(ctx, ctx->bl, Op::Spmv, sv) +
+            (it + 1 < m ? vbytes(d, sv) + d.n : 0.0);
+    return by;
+  }
   for (int it = 0; it < m; ++it) {
     if (!ctx->base.failed)
       by += (double)ctx->base_nact * ncols * si + ncols * (sv + 4.0) + 4.0 * nsc + 2.0 * nsc * sv;
@@ -987,7 +1016,8 @@ double lmgc_bytes(const ugmg_ctx *ctx, int l, int sv, int cmode, bool c0_done) {
   const bool pre0 = l - 1 > ctx->bl && l - 1 > ctx->fuse_top && p.gamma >= 1 && fold_colour0(ctx, l - 1, p.nu1);
   by += op_bytes(ctx, l, Op::Restrict, sv);
   if (pre0) by += dc.n + (double)dc.color_rows[0] * 2.0 * sv;
-  if (to_base && ctx->nb == 1 && ctx->base_cpos && !ctx->base.failed && !dc.vt)
+  if (to_base && ctx->nb == 1 && ctx->base_cpos && !ctx->base.failed && !dc.vt &&
+      p.base_iter == UGMG_BASE_EX)
     by += 4.0 * dc.n + (double)ctx->base_ncyc * sv;
   if (p.gamma <= 0) by += vbytes(dc, sv);
   for (int g = 0; g < p.gamma; ++g)
@@ -1694,7 +1724,7 @@ int prep_device(ugmg_ctx *ctx, PrepState &ps) {
       if ((rc = dalloc(ctx, &ctx->stage, mx))) return rc;
     }
     if (ps.fuse_rows > 0 && ctx->dev[bl].n <= 2048 && ctx->b == 1 && !ctx->grouped && ps.ntypes == 1 &&
-        !split_smoothers(ctx->p)) {
+        !split_smoothers(ctx->p) && ctx->p.base_iter == UGMG_BASE_EX) {
       ctx->fuse_top = bl;
       for (int l = bl + 1; l <= level; ++l) {
         if (ctx->dev[l].n > ps.fuse_rows || ctx->dev[l].ncolors > kMaxColors) break;
@@ -2105,6 +2135,7 @@ int ugmg_set_params(ugmg_ctx *ctx, const ugmg_params *p) {
   if (p->nu1 < 0 || p->nu2 < 0 || p->gamma < 0 || p->base_maxiter < 0 ||
       (p->smoother != UGMG_SMOOTH_JAC && p->smoother != UGMG_SMOOTH_GS) ||
       (p->post_set && p->post_smoother != UGMG_SMOOTH_JAC && p->post_smoother != UGMG_SMOOTH_GS) ||
+      (p->base_iter != UGMG_BASE_EX && p->base_iter != UGMG_BASE_JAC) ||
       (p->precision != UGMG_FP64 && p->precision != UGMG_MIXED) || p->inner_cycles < 0)
     return fail(ctx, UGMG_E_ARG, "invalid parameter");
   // the damping factors may differ per component ($damp, lmgc $damp, ex
@@ -2438,7 +2469,7 @@ int ugmg_interpolate(ugmg_ctx *ctx, int level, double *to_f, const double *from_
 int ugmg_base_solve(ugmg_ctx *ctx, double *c, double *b) {
   int rc = check_ready(ctx, ctx ? ctx->bl : 0);
   if (rc) return rc;
-  if (ctx->base.failed) return fail(ctx, UGMG_E_SINGULAR, "zero pivot in the band LU of the base level");
+  if (base_singular(ctx)) return fail(ctx, UGMG_E_SINGULAR, "zero pivot in the band LU of the base level");
   base_solve<double>(ctx, c, b, 0, false, true);
   HIPCHK(hipGetLastError());
   return UGMG_OK;
@@ -2447,7 +2478,7 @@ int ugmg_base_solve(ugmg_ctx *ctx, double *c, double *b) {
 int ugmg_iter(ugmg_ctx *ctx, int level, double *c, double *b) {
   int rc = check_ready(ctx, level);
   if (rc) return rc;
-  if (ctx->base.failed) return fail(ctx, UGMG_E_SINGULAR, "zero pivot in the band LU of the base level");
+  if (base_singular(ctx)) return fail(ctx, UGMG_E_SINGULAR, "zero pivot in the band LU of the base level");
   DevLevel &d = ctx->dev[level];
   if (ctx->p.precision == UGMG_MIXED) {
     to_dev<float, double>(ctx, level, b, d.vs, d.b32);
@@ -2505,7 +2536,7 @@ int ugmg_solve(ugmg_ctx *ctx, int level, double *x, double *b, const double *abs
   if (!x || !b || !abslimit || !reduction || !res || maxiter < 0)
     return fail(ctx, UGMG_E_ARG, "null argument");
   std::memset(res, 0, sizeof(*res));
-  if (ctx->base.failed) {
+  if (base_singular(ctx)) {
     res->error_code = UGMG_E_SINGULAR;
     return fail(ctx, UGMG_E_SINGULAR, "zero pivot in the band LU of the base level");
   }
@@ -2796,7 +2827,7 @@ int ugmg_solve_surface(ugmg_ctx *ctx, int level, double *const *x, double *const
   if (s.frl < bl) return fail(ctx, UGMG_E_ARG, "FULLREFINELEVEL lies below the base level");
   for (int l = bl; l <= level; ++l)
     if (!x[l] || !b[l]) return fail(ctx, UGMG_E_ARG, "x[l] and b[l] are needed on baselevel..level");
-  if (ctx->base.failed) {
+  if (base_singular(ctx)) {
     res->error_code = UGMG_E_SINGULAR;
     return fail(ctx, UGMG_E_SINGULAR, "zero pivot in the band LU of the base level");
   }
@@ -3004,7 +3035,7 @@ int ugmg_iter_batch(ugmg_ctx *ctx, int level, int nrhs, double *c, double *b, in
   if (nrhs < 1 || nrhs > ctx->nrhs_cap)
     return fail(ctx, UGMG_E_ARG, "nrhs exceeds the batch the hierarchy was laid out for (ugmg_set_batch)");
   if (!c || !b || ld < ctx->dev[level].n * ctx->b) return fail(ctx, UGMG_E_ARG, "null vector or ld < n*b");
-  if (ctx->base.failed) return fail(ctx, UGMG_E_SINGULAR, "zero pivot in the band LU of the base level");
+  if (base_singular(ctx)) return fail(ctx, UGMG_E_SINGULAR, "zero pivot in the band LU of the base level");
   DevLevel &d = ctx->dev[level];
   hipStream_t st = ctx->stream;
   ctx->nb = nrhs;

@@ -221,6 +221,12 @@ void launch_base_gemv(hipStream_t st, int n, const TI *inv, const int *rowmap, c
                       int ncols, const T *b, const T *bc, T *c, T *xout, int cmode, double damp,
                       double abslimit, double red, const double *part, int nparts, double *first,
                       int it, Batch bt);
+// ls over jac on the base level: after the Jacobi step's x (n scalar
+// unknowns), the base stop test on the defect's partial norms, x = 0 once
+// it holds, c (+)= x (cmode 0 / 1); first: the first defect's norms
+template <typename T>
+void launch_base_gate(hipStream_t st, int n, T *x, T *c, int cmode, double abslimit, double red,
+                      const double *part, int nparts, int it, double *first, Batch bt);
 // members: d64[j] = src[map[j]] (0 where map[j] < 0), d32[j] = (float) d64[j]
 void launch_member_gather(hipStream_t st, size_t n, const int *map, const double *src, double *d64,
                           float *d32);

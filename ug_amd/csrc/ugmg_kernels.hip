@@ -1868,6 +1868,73 @@ void launch_base_gemv(hipStream_t st, int n, const TI *inv, const int *rowmap, c
                                                              nparts, first, it, bt.vs, bt.ims);
   }
 }
+// ls over jac on the base level, after the Jacobi step formed x: the stop
+// test of k_base_gemv_bk on the current defect's partial norms (nc entries:
+// components, or (type, component) pairs; grouped by VD_IDENT when idents
+// >= 0), then x = 0 once it holds, and c (+)= x.  n scalar unknowns.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_base_gate(int n, int nc, T *__restrict__ x, T *__restrict__ c,
+                                                       int cmode, double abslimit, double red,
+                                                       const double *__restrict__ part, int nparts,
+                                                       double *__restrict__ first, int it, size_t vs,
+                                                       int idents) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int cj = blockIdx.y;
+  double cur[UGMG_MAXB] = {}, target[UGMG_MAXB] = {};
+  for (int q = 0; q < nc; ++q) {
+    const double *p = part + ((size_t)cj * nc + q) * nparts;
+    double s = 0.0;
+    for (int k = lane; k < nparts; k += 64) s += p[k];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    cur[q] = sqrt(s);
+  }
+  if (it == 0 && i == 0)
+    for (int q = 0; q < nc; ++q) first[(size_t)cj * nc + q] = cur[q];
+  for (int q = 0; q < nc; ++q) {
+    target[q] = it == 0 ? 0.0 : first[(size_t)cj * nc + q] * red;
+    if (target[q] == 0.0) target[q] = red;
+  }
+  bool below_abs = true, below_red = true;
+  if (idents < 0) {
+    for (int q = 0; q < nc; ++q) {
+      below_abs &= fabs(cur[q]) < fabs(abslimit);
+      below_red &= fabs(cur[q]) < fabs(target[q]);
+    }
+  } else {   // sc_cmp with VD_IDENT (npscan.cc:1033-1048)
+    for (int g = 0; g < nc; ++g) {
+      if (((idents >> (4 * g)) & 15) != g) continue;
+      double sx = 0.0, sa = 0.0, st = 0.0;
+      for (int j = 0; j < nc; ++j)
+        if (((idents >> (4 * j)) & 15) == g) {
+          sx += cur[j] * cur[j];
+          sa += abslimit * abslimit;
+          st += target[j] * target[j];
+        }
+      below_abs &= sx < sa;
+      below_red &= sx < st;
+    }
+  }
+  if (it == 0) below_red = false;
+  const bool conv = below_abs || below_red;
+  if (i >= n) return;
+  const size_t k = (size_t)cj * vs + i;
+  const T xv = conv ? (T)0 : x[k];
+  x[k] = xv;
+  if (!conv || cmode == 0) c[k] = cmode ? c[k] + xv : xv;
+}
+template <typename T>
+void launch_base_gate(hipStream_t st, int n, T *x, T *c, int cmode, double abslimit, double red,
+                      const double *part, int nparts, int it, double *first, Batch bt) {
+  if (n <= 0) return;
+  k_base_gate<T><<<grid2(nblk(n), bt.nrhs), kBlock, 0, st>>>(n, bt.nc(), x, c, cmode, abslimit, red, part,
+                                                              nparts, first, it, bt.vs, bt.idents);
+}
+template void launch_base_gate<double>(hipStream_t, int, double *, double *, int, double, double,
+                                       const double *, int, int, double *, Batch);
+template void launch_base_gate<float>(hipStream_t, int, float *, float *, int, double, double,
+                                      const double *, int, int, double *, Batch);
+
 template void launch_base_gemv<double, double>(hipStream_t, int, const double *, const int *,
                                                const int *, int, const double *, const double *, double *,
                                                double *, int, double, double, double,

@@ -467,9 +467,13 @@ int build_params(ugmg_np_env *e, NumProc &o, ugmg_params &p, bool &use_iter) {
       (rc = resolve_damp(e, *po, "post-smoother", dpost, count)) ||
       (rc = resolve_damp(e, *mg, "lmgc", dmg, count)))
     return rc;
+  // the base ls iterates with ex (the dense inverse) or jac (LinearSolver
+  // over Smoother on the base level)
   NumProc *ex = find(e, ba->I);
-  if (!ex || ex->cls != "ex") return fail(e, UGMG_E_ARG, "the base ls must iterate with ex");
-  if ((rc = resolve_damp(e, *ex, "ex", dex, count))) return rc;
+  if (!ex || (ex->cls != "ex" && ex->cls != "jac"))
+    return fail(e, UGMG_E_ARG, "the base ls must iterate with ex or jac");
+  if ((rc = resolve_damp(e, *ex, ex->cls.c_str(), dex, count))) return rc;
+  p.base_iter = ex->cls == "jac" ? UGMG_BASE_JAC : UGMG_BASE_EX;
   if (ba->abslimit_n.typed() || ba->red_n.typed())
     return fail(e, UGMG_E_ARG, "the base ls takes one $abslimit / $red for every entry");
   p.nu1 = mg->nu1;

@@ -20,6 +20,7 @@ MAXB = 4        # UGMG_MAXB: components per vector, 1..MAXB
 
 SMOOTH_JAC, SMOOTH_GS = 0, 1
 FP64, MIXED = 0, 1
+BASE_EX, BASE_JAC = 0, 1
 SETUP_HOST, SETUP_DEVICE = 0, 1
 ERRORS = {0: "OK", -1: "E_ARG", -2: "E_HIP", -3: "E_STATE", -4: "E_NOMEM", -5: "E_SINGULAR",
           -6: "E_BUDGET", -7: "E_NODEV"}
@@ -42,7 +43,8 @@ class Params(C.Structure):
                 ("base_maxiter", C.c_int), ("ex_reorder", C.c_int), ("precision", C.c_int),
                 ("inner_cycles", C.c_int), ("use_graph", C.c_int), ("ex_float", C.c_int),
                 ("inner_adapt", C.c_int), ("base_inv32", C.c_int),
-                ("post_set", C.c_int), ("post_smoother", C.c_int), ("post_damp", C.c_double * MAXB)]
+                ("post_set", C.c_int), ("post_smoother", C.c_int), ("post_damp", C.c_double * MAXB),
+                ("base_iter", C.c_int)]
 
 
 class LResult(C.Structure):

@@ -3071,7 +3071,7 @@ int ugmg_bcgs_solve(ugmg_ctx *ctx, int level, int nrhs, double *x, double *b, in
   if (ctx->b != 1 || ctx->ntypes > 1)
     return fail(ctx, UGMG_E_ARG, "bcgs runs on scalar (b = 1) hierarchies of one vector type");
   std::memset(res, 0, sizeof(*res) * nrhs);
-  if (use_iter && ctx->base.failed) {
+  if (use_iter && base_singular(ctx)) {
     for (int j = 0; j < nrhs; ++j) res[j].error_code = UGMG_E_SINGULAR;
     return fail(ctx, UGMG_E_SINGULAR, "zero pivot in the band LU of the base level");
   }

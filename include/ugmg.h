@@ -413,7 +413,9 @@ int ugmg_coarse_inverse(int n, const int *row_ptr, const int *col_idx,
 
 /* ---- serving: continuous batching of independent solves ----------------
  * No counterpart in UG (its ugd daemon is a remote UI, ugd.cc:528-608).
- * Each request is a whole hierarchy + rhs (scalar, FP64 solves).  Requests
+ * Each request is a whole hierarchy + rhs (scalar, FP64 solves, one
+ * smoother class and $damp before and after the coarse correction, the ex
+ * base solve).  Requests
  * with the same hierarchy (patterns, transfers, classes, Dirichlet bits)
  * and parameters share a resident group: one device hierarchy holding up
  * to max_slots matrices of that pattern, one per column of its vectors,

@@ -3993,6 +3993,8 @@ int ugmg_server_submit(ugmg_server *s, int nlevels, const ugmg_level_desc *level
                        int maxiter, int *id) {
   if (!s || !levels || !p || !rhs || !id || nlevels < 1 || maxiter < 0) return UGMG_E_ARG;
   if (p->precision != UGMG_FP64) return srv_fail(s, UGMG_E_ARG, "the server runs FP64 solves");
+  if (p->base_iter != UGMG_BASE_EX || split_smoothers(*p))
+    return srv_fail(s, UGMG_E_ARG, "the server runs one smoother class and the ex base solve");
   for (int l = 0; l < nlevels; ++l) {
     const ugmg_level_desc &L = levels[l];
     if (L.n < 0 || !L.row_ptr || !L.col_idx || !L.vals || !L.skip || !L.vclass || !L.vnclass ||

@@ -187,6 +187,41 @@ def test_lmgc_cycle_with_jacobi_base_solver(b):
     assert not np.array_equal(outs[hits[0] - 2] if hits[0] > 1 else np.zeros_like(c40), c40)
 
 
+@pytest.mark.parametrize("b", [1, 2])
+def test_jacobi_base_solver_stops_where_its_defect_meets_red(b):
+    """The base ls over jac alone (lmgc on the base level): numpy Jacobi
+    steps x_k = damp M^-1 b_k (zero on rows of VCLASS < 3), b_k+1 = b_k - A x_k,
+    stopping at the first k whose per-component defect norms are all below
+    $red times the first; c = the sum of the steps taken."""
+    P = problems.p1_local(3, 0, b)
+    H = O.Hierarchy(P.levels)
+    bl = 1
+    lv = P.levels[bl]
+    A = lv.dense()
+    n2 = A.shape[0]
+    D = np.kron(np.eye(lv.n), np.ones((b, b))).astype(bool)
+    M = np.where(D, A, 0.0)
+    act = np.repeat(lv.vclass >= 3, b)
+    damp = np.tile(np.array([0.8, 0.7][:b]), lv.n)
+    rhs = np.random.default_rng(9).uniform(-1, 1, n2)
+    rhs[np.repeat(lv.skip != 0, b)] = 0.0
+    for red in (0.5, 0.05, 1e-3):
+        r, c = rhs.copy(), np.zeros(n2)
+        first = np.linalg.norm(r.reshape(-1, b), axis=0)
+        for k in range(60):
+            x = np.where(act, damp * np.linalg.solve(M, r), 0.0)
+            c += x
+            r = r - A @ x
+            if np.all(np.linalg.norm(r.reshape(-1, b), axis=0) < red * first):
+                break
+        p = O.params(smoother=O.JAC, damp=0.6, baselevel=bl, base_damp=[0.8, 0.7][:b], base_maxiter=60,
+                     base_iter=1, base_red=red)
+        cw, bw = O.lmgc_iter(H, bl, p, rhs)
+        assert k < 59
+        assert np.max(np.abs(cw - c)) < 1e-12 * np.max(np.abs(c)), red
+        assert np.max(np.abs(bw - r)) < 1e-12 * np.max(np.abs(rhs)), red
+
+
 def test_block4_jacobi_step_with_row_exchange_is_dense_block_solve():
     P = problems.p1_local(2, 0, 4, coupling=PIVOTING)
     H = O.Hierarchy(P.levels)

@@ -291,8 +291,7 @@ def ls_solve(H, top, p, rhs, x0=None, red=1e-10, abslimit=1e-300, maxiter=50, do
     nb = H.ncomp(top)
     x = np.zeros(lv.n * lv.b) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
     b = np.array(rhs, dtype=np.float64, copy=True)
-    ab = np.full(MAXB, abslimit)
-    rd = np.full(MAXB, red)
+    ab, rd = _dv(abslimit), _dv(red)      # one value, or one per norm entry
     defects = np.zeros((maxiter + 1) * nb)
     nit, conv = C.c_int(), C.c_int()
     rc = lib().orc_ls_lmgc_solve(H.ptr(0), top, C.byref(p), x.ctypes.data, b.ctypes.data, int(do_defect),

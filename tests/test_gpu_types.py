@@ -255,3 +255,43 @@ def test_types_through_the_numproc_layer(p2):
     # DoPCR prints one line per norm entry: node ("u") and edge ("v") vectors
     import re
     assert len(re.findall(r"^      v: ", out, re.M)) == want["nit"] + 1
+
+
+def test_types_typed_vec_scalars_through_the_numproc_layer(p2):
+    """UG's typed VEC_SCALAR form (sc_read over ReadVecTypeDOUBLEs,
+    np/udm/npscan.cc:783-855, 1211-1278): "$red n1e-6|k1e-10" gives node and
+    edge norms their own reduction, "$damp n0.6|k0.6" one factor per type;
+    a count per type that does not match the vectors is refused."""
+    from ug_amd import numproc as NP
+    from ug_amd import ugmg
+    P = p2
+    want = O.ls_solve(O.Hierarchy(P.levels), P.top, O.params(smoother=O.JAC, damp=0.6, baselevel=2), P.rhs,
+                      red=[1e-6, 1e-10], maxiter=200)
+    for red, ok in (("n1e-6|k1e-10", True), ("n1e-6", False)):
+        ctx = ugmg.Context(0)
+        ctx.set_hierarchy(P)
+        env = NP.NumProcEnv(ctx)
+        x = env.vector("sol", np.zeros(P.levels[-1].n))
+        env.vector("rhs", P.rhs.copy())
+        env.create("transfer", "transfer"); env.init("transfer")
+        env.create("jac", "jac"); env.init("jac", "$damp n0.6|k0.6")
+        env.create("ex", "ex"); env.init("ex")
+        env.create("base", "ls"); env.init("base", "$red 1e-12 $m 1 $I ex $abslimit 1e-300")
+        env.create("mgc", "lmgc"); env.init("mgc", "$S jac jac base $T transfer $n1 2 $n2 2 $b 2")
+        env.create("mgs", "ls")
+        assert env.init("mgs", f"$A MAT $x sol $b rhs $m 200 $red {red} $abslimit 1e-300 $I mgc") \
+            == NP.EXECUTABLE
+        try:
+            if ok:
+                env.execute("mgs", "$i $d $r $s $p")
+                r = env.result("mgs")
+            else:
+                with pytest.raises(ugmg.UgmgError):
+                    env.execute("mgs", "$i $d $r $s $p")
+        finally:
+            env.close()
+            ctx.close()
+        if ok:
+            assert r.converged and r.number_of_linear_iterations == want["nit"]
+            np.testing.assert_allclose(r.last_defect[:2], want["defects"][-1], rtol=1e-10)
+            assert np.max(np.abs(x - want["x"])) < 1e-10 * np.max(np.abs(want["x"]))

@@ -160,8 +160,8 @@ int ugmg_set_transfer_matrix(ugmg_ctx *ctx, int level, const int *p_ptr, const i
  *  - norms, dot products and the VEC_SCALARs of the stop tests
  *    (abslimit, reduction, ugmg_lresult's defects, the per-iteration
  *    callback) take one entry per (type, component), type-major (VD_OFFSET),
- *    at most UGMG_MAXB of them; the damping factors must agree over
- *    the types and components;
+ *    at most UGMG_MAXB of them; a damping factor per component applies
+ *    alike to both types, one base $abslimit / $red to every entry;
  *  - the transfer rows of an edge vector name the father objects of the
  *    edge's two nodes (the father node's vector for a corner's son, the
  *    father edge's vector for a midnode) with weight 1/2 each: the edge

@@ -295,3 +295,24 @@ def test_types_typed_vec_scalars_through_the_numproc_layer(p2):
             assert r.converged and r.number_of_linear_iterations == want["nit"]
             np.testing.assert_allclose(r.last_defect[:2], want["defects"][-1], rtol=1e-10)
             assert np.max(np.abs(x - want["x"])) < 1e-10 * np.max(np.abs(want["x"]))
+
+
+def test_types_with_blocks_damping_per_component():
+    """$damp per component on node + edge vectors with 2 components: the
+    factor of component j scales entry j of both types (dscalx with the
+    same VEC_SCALAR value for (node, j) and (edge, j)); Jacobi and MC-GS
+    solves against the oracle."""
+    from ug_amd import problems, ugmg
+    P = problems.p2_uniform(5, 2)
+    P.rhs = _rhs(P, 0x5EED0606)
+    H = O.Hierarchy(P.levels)
+    for sm, osm, dm in ((ugmg.SMOOTH_JAC, O.JAC, [0.55, 0.65]), (ugmg.SMOOTH_GS, O.MCGS, [0.9, 1.0])):
+        want = O.ls_solve(H, P.top, O.params(smoother=osm, damp=dm + dm, lmgc_damp=[0.95, 0.85] * 2,
+                                             baselevel=2), P.rhs, red=1e-10, maxiter=200)
+        ctx = _ctx(P, sm, dm, lmgc_damp=[0.95, 0.85])
+        try:
+            x, b, res, d = ctx.solve_host(P.top, P.rhs, red=1e-10, maxiter=200)
+        finally:
+            ctx.close()
+        assert res.converged and res.number_of_linear_iterations == want["nit"], sm
+        assert np.max(np.abs(d - want["defects"]) / want["defects"]) < 1e-9, sm

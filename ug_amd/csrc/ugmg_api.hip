@@ -1252,10 +1252,11 @@ int prep_host(ugmg_ctx *ctx, int level, PrepState &ps) {
     if (nt * ctx->b > UGMG_MAXB)
       return fail(ctx, UGMG_E_ARG, "vector types x components exceed UGMG_MAXB norm entries");
     if (ctx->grouped) return fail(ctx, UGMG_E_ARG, "served contexts take one vector type");
+    // damping factors per component apply alike to every vector type (the
+    // kernels scale entry j of each vector by damp[j]); one base limit
     for (int i = 1; i < nt * ctx->b; ++i)
-      if (p.damp[i] != p.damp[0] || p.lmgc_damp[i] != p.lmgc_damp[0] || p.base_damp[i] != p.base_damp[0] ||
-          p.base_abslimit[i] != p.base_abslimit[0] || p.base_red[i] != p.base_red[0])
-        return fail(ctx, UGMG_E_ARG, "one damping / base limit for every vector type and component");
+      if (p.base_abslimit[i] != p.base_abslimit[0] || p.base_red[i] != p.base_red[0])
+        return fail(ctx, UGMG_E_ARG, "one base $abslimit / $red for every vector type and component");
   }
   // interpolation matrices: laid out on the host (the device build takes
   // scalar weights); not in served groups
